@@ -1,0 +1,190 @@
+/*
+ * pi_hip.h - C ABI of libpi_hip.so, the MI355X (gfx950) implementation of the
+ * PlanktonIndividuals.jl per-timestep hot path (QuotaMode physiology, RK4
+ * particle advection, random-walk diffusion, division/death, particle->grid
+ * consumption, DST3FL tracer advection + diffusion + sinking + forcing).
+ *
+ * The ABI is what a new `HIPArch <: Architecture` (src/Architectures.jl:16)
+ * binds with `ccall`.  Plain C types only: pointers, sizes, POD structs.
+ *
+ * Conventions
+ *  - Arrays are column-major (x fastest), exactly like the reference's Julia
+ *    arrays.  "Padded" fields have shape (Nx+2Hx, Ny+2Hy, Nz+2Hz)
+ *    (src/Biogeochemistry/tracer_fields.jl:10-14).
+ *  - Individuals are per species, per slot, length `capacity`
+ *    (`max_individuals`, src/Model/models.jl:54), slot order = the
+ *    reference's StructArray index order (QuotaMode/plankton_generation.jl:2-13).
+ *  - Host pointers are borrowed for the duration of the call only.  The
+ *    library owns every device buffer.  A handle is not re-entrant; calls on
+ *    one handle are ordered on the handle's own HIP stream.
+ *  - Every function returns a pi_status; pi_last_error() gives the text.
+ */
+#ifndef PI_HIP_H
+#define PI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (errors are never thrown across the ABI) ---------------- */
+typedef enum pi_status {
+    PI_OK = 0,
+    PI_ERR_ARG = 1,        /* bad argument: mirrors the reference's ArgumentError from validators */
+    PI_ERR_CAPACITY = 2,   /* "number of individual exceeds the capacity" (QuotaMode/plankton_update.jl:20-22) */
+    PI_ERR_HIP = 3,        /* a HIP runtime call failed */
+    PI_ERR_UNSUPPORTED = 4,/* feature outside the implemented hot path (e.g. other physiology modes) */
+    PI_ERR_NOMEM = 5
+} pi_status;
+
+/* ---- topology (src/Grids/Grids.jl: Periodic / Bounded) -------------------- */
+enum { PI_PERIODIC = 0, PI_BOUNDED = 1 };
+
+/* ---- tracer ids, in the reference's tracer_names order (Biogeochemistry.jl:20) */
+enum {
+    PI_DIC = 0, PI_NH4, PI_NO3, PI_PO4, PI_DFe, PI_DOC, PI_DON, PI_DOP,
+    PI_PFe_inorg, PI_POC, PI_PON, PI_POP, PI_PFe_bio, PI_Dust,
+    PI_NTRACERS = 14
+};
+
+/* ---- QuotaMode parameters: 40 floats in the reference's `param_names` order
+ *      (QuotaMode/plankton_generation.jl:16-20). ---------------------------- */
+enum {
+    PI_P_Nsuper = 0, PI_P_Cquota, PI_P_mean, PI_P_var, PI_P_Chl2Cint, PI_P_alpha,
+    PI_P_Phi, PI_P_Topt, PI_P_Tmax, PI_P_Ea, PI_P_PCmax, PI_P_VDOCmax, PI_P_VNO3max,
+    PI_P_VNH4max, PI_P_VPO4max, PI_P_KsatDOC, PI_P_KsatNH4, PI_P_KsatNO3, PI_P_KsatPO4,
+    PI_P_Cqmax, PI_P_Nqmax, PI_P_Pqmax, PI_P_Chl2N, PI_P_R_NC, PI_P_R_PC, PI_P_k_mtb,
+    PI_P_respir, PI_P_grz_P, PI_P_dvid_type, PI_P_dvid_P, PI_P_dvid_reg, PI_P_dvid_reg2,
+    PI_P_mort_P, PI_P_mort_reg, PI_P_grazFracC, PI_P_grazFracN, PI_P_grazFracP,
+    PI_P_mortFracC, PI_P_mortFracN, PI_P_mortFracP,
+    PI_NPARAMS = 40
+};
+
+/* ---- biogeochemical parameters (Parameters/param_default.jl:45-71) ------- */
+enum {
+    PI_B_kw = 0, PI_B_kc, PI_B_kDOC, PI_B_Nit, PI_B_kDON, PI_B_kDOP, PI_B_kPOC,
+    PI_B_kPON, PI_B_kPOP, PI_B_kappa_h, PI_B_kappa_v, PI_B_kappa_hP, PI_B_kappa_vP,
+    PI_B_shared_graz, PI_B_w_sink_org, PI_B_w_sink_inorg, PI_B_lambda_POC,
+    PI_B_lambda_min, PI_B_kdiss, PI_B_lambda_dust, PI_B_lambda_Fe, PI_B_ligand,
+    PI_B_DFeFrac,
+    PI_NBGC = 23
+};
+
+/* ---- individual fields (QuotaMode StructArray columns) ------------------- */
+enum {
+    PI_F_x = 0, PI_F_y, PI_F_z, PI_F_Sz, PI_F_Bm, PI_F_Cq, PI_F_Nq, PI_F_Pq,
+    PI_F_Chl, PI_F_gen, PI_F_age, PI_F_graz, PI_F_mort, PI_F_dvid,   /* float32 */
+    PI_F_xi, PI_F_yi, PI_F_zi,                                       /* int64   */
+    PI_F_ac,                                                         /* uint8   */
+    PI_NFIELDS
+};
+
+/* ---- grid-shaped outputs (timestepper + diagnostics) --------------------- */
+enum {
+    PI_G_par = 0,     /* timestepper.par   (padded) */
+    PI_G_par0,        /* timestepper.par₀  (padded) */
+    PI_G_Chl,         /* timestepper.Chl   (padded) accumulated Chl per cell */
+    PI_G_pop,         /* timestepper.pop   (padded) individuals per cell */
+    PI_G_temp,        /* timestepper.temp  (padded) */
+    PI_G_PARF,        /* timestepper.PARF  (Nx,Ny), attenuated to the bottom after a step */
+    PI_G_u, PI_G_v, PI_G_w,   /* timestepper.vel₁ (padded, halos filled) */
+    PI_G_diag_PAR,    /* diags.tracer.PAR (padded, accumulated) */
+    PI_G_plk_DIC      /* timestepper.plk.<tracer> : add the tracer id (padded) */
+};
+enum {                 /* per-species diagnostics (Diagnostics/diagnostics_struct.jl:97-104) */
+    PI_D_num = 0, PI_D_graz, PI_D_mort, PI_D_dvid, PI_NDIAG = 4
+};
+
+typedef struct pi_grid_desc {
+    int32_t Nx, Ny, Nz;          /* interior size */
+    int32_t Hx, Hy, Hz;          /* halo (reference default 2,2,2; >= 2 required by DST3FL) */
+    int32_t topo_x, topo_y, topo_z; /* PI_PERIODIC / PI_BOUNDED; z must be bounded (tracer_sinking.jl) */
+    float dx, dy;                /* RectilinearGrid Δx, Δy (rectilinear_grid.jl:72-73) */
+    const float* dzC;            /* length Nz+2Hz, as built by rectilinear_grid.jl:93-116 */
+    const float* dzF;            /* length Nz+2Hz */
+    const uint8_t* landmask;     /* padded Bool landmask (Grids/utils.jl:92-129); NULL = no land */
+} pi_grid_desc;
+
+typedef struct pi_model_desc {
+    pi_grid_desc grid;
+    int32_t n_species;
+    int64_t capacity;            /* max_individuals per species */
+    const float* params;         /* n_species * PI_NPARAMS floats */
+    const float* bgc;            /* PI_NBGC floats */
+    uint64_t rng_seed;           /* key of the shared uniform stream (pi_math.h) */
+    int32_t n_diag_tracers;      /* tracers accumulated into diags.tracer besides PAR */
+    const int32_t* diag_tracers; /* tracer ids, may be NULL when n_diag_tracers == 0 */
+} pi_model_desc;
+
+/* Host-side view of one species' individuals, all arrays of length capacity.
+ * Pointers may be NULL on upload for fields that are zero. */
+typedef struct pi_individuals_host {
+    float *x, *y, *z, *Sz, *Bm, *Cq, *Nq, *Pq, *Chl, *gen, *age, *graz, *mort, *dvid;
+    int64_t *xi, *yi, *zi;
+    uint8_t *ac;
+} pi_individuals_host;
+
+typedef struct pi_model pi_model;
+
+/* Library / device set-up.  Replaces `isfunctional(arch)` (Architectures.jl:24). */
+pi_status pi_init(int32_t device);
+const char* pi_last_error(void);
+const char* pi_version(void);
+
+/* PlanktonModel(arch, grid; ...) storage (Model/models.jl:51-155, timestepper.jl:23-60).
+ * The model starts at iteration 0 with zero tracers/velocities; upload the
+ * initial state with the calls below. */
+pi_status pi_model_create(const pi_model_desc* desc, pi_model** out);
+pi_status pi_model_destroy(pi_model* m);
+
+/* Initial state (generate_individuals / generate_tracers, Individuals.jl:42-72,
+ * tracer_fields.jl:93-134): full-capacity SoA for species `sp`. */
+pi_status pi_upload_individuals(pi_model* m, int32_t sp, const pi_individuals_host* h);
+/* One padded tracer field (with halos). */
+pi_status pi_upload_tracer(pi_model* m, int32_t tracer, const float* padded);
+
+/* Forcing inputs staged by update! (Simulation/update.jl:18-26):
+ *   velocity interior slices like vel_copy! (Simulation/utils.jl:5-11):
+ *     u: (Nx[+1 if bounded x], Ny, Nz), v: (Nx, Ny[+1], Nz), w: (Nx, Ny, Nz[+1])
+ *   PARF: (Nx, Ny)         temperature interior: (Nx, Ny, Nz)
+ * A call replaces vel₁ / PARF / temp for the following steps. */
+pi_status pi_set_velocity(pi_model* m, const float* u, const float* v, const float* w);
+pi_status pi_set_parf(pi_model* m, const float* parf);
+pi_status pi_set_temperature(pi_model* m, const float* temp);
+
+/* TimeStep!(model, ΔT, diags) (Model/time_step.jl:12-180).  Enqueued on the
+ * model's stream; an error raised on the device (capacity) is reported by this
+ * call or, at the latest, by the next pi_sync / download. */
+pi_status pi_step(pi_model* m, float dt);
+/* Run `n` steps (update! without staging changes). */
+pi_status pi_run(pi_model* m, float dt, int64_t n);
+pi_status pi_sync(pi_model* m);
+
+/* Outputs (Output/write_outputs.jl:159-202 read these). */
+pi_status pi_download_field(pi_model* m, int32_t sp, int32_t field, void* host, int64_t n);
+pi_status pi_download_tracer(pi_model* m, int32_t tracer, float* padded);
+pi_status pi_download_grid(pi_model* m, int32_t which, float* host, int64_t n);
+pi_status pi_download_diag(pi_model* m, int32_t sp, int32_t which, float* padded);
+pi_status pi_download_diag_tracer(pi_model* m, int32_t tracer, float* padded);
+/* Zero every diagnostic accumulator (write_diags_to_jld2, write_outputs.jl:186-201). */
+pi_status pi_reset_diags(pi_model* m);
+/* Choose the tracers accumulated into diags.tracer besides PAR
+ * (PlanktonDiagnostics(model; tracer=...), Diagnostics/diagnostics_struct.jl:55-84);
+ * resets those accumulators. */
+pi_status pi_set_diag_tracers(pi_model* m, int32_t n, const int32_t* ids);
+
+/* Scalars. */
+pi_status pi_get_iteration(pi_model* m, int64_t* iteration);
+pi_status pi_count_active(pi_model* m, int32_t sp, int64_t* n);   /* dot(ac, ac) (models.jl:157-164) */
+
+/* Per-kernel timing of the most recent step (HIP events on the model stream);
+ * names[i] is static text.  Used by the benchmark for the roofline. */
+pi_status pi_enable_timing(pi_model* m, int32_t on);
+pi_status pi_kernel_times(pi_model* m, int32_t* n, const char** names, float* ms, int32_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PI_HIP_H */

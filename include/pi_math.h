@@ -1,0 +1,184 @@
+/*
+ * pi_math.h - numerics every implementation of the hot path must share so that
+ * results are reproducible bit for bit between a CPU and the GPU:
+ *
+ *  1. The shared uniform stream.  The reference draws diffusion and decision
+ *     uniforms with `rand!(rng_type(arch), …)` over the whole capacity
+ *     (Individuals/Advection/particle_diffusion.jl:18-20,
+ *     Plankton/division_death_probability.jl:97-99), and `rng_type(::CPU)`
+ *     reseeds from entropy on every call (src/Architectures.jl:33), so no two
+ *     runs agree.  Here every draw is a pure function of
+ *     (seed, iteration, species, kind, slot): Philox4x32-10 (Salmon, Moraes,
+ *     Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC'11) with
+ *     counter {slot, iteration, species, kind} and key {seed_lo, seed_hi};
+ *     words 0,1,2 of the output are the x/y/z (kind DIFFUSION) or the
+ *     graz/mort/dvid (kind DECISION) draws.  A word u becomes a Float32 in
+ *     [0,1) the way Julia's rand(Float32) does: (u >> 9 | 0x3f800000) - 1.
+ *
+ *  2. Transcendentals.  Float32 exp is defined as exp evaluated in binary64
+ *     and rounded once to binary32.  The binary64 exp is written here from
+ *     its textbook construction (Cody & Waite argument reduction
+ *     x = k ln2 + r with a two-word ln2, Taylor polynomial of degree 13 on
+ *     |r| <= ln2/2, scaling by 2^k) using only IEEE +, -, *, so every
+ *     conforming platform computes identical bits.  `x^4f0` / `x^2f0` follow
+ *     Julia's Float32 `^` with an integer-valued exponent: square (and
+ *     square again) in binary64, round once.
+ *
+ *  3. Julia's IEEE semantics for min/max (NaN-propagating, -0 < +0), isless
+ *     and `Bool * x` (false * x == copysign(0, x)).
+ *
+ * Compile every user of this header without FP contraction
+ * (-ffp-contract=off) and without fast-math.
+ */
+#ifndef PI_MATH_H
+#define PI_MATH_H
+
+#include <stdint.h>
+#include <math.h>
+
+#ifdef __HIPCC__
+#define PI_HD __host__ __device__ __forceinline__
+#else
+#define PI_HD static inline
+#endif
+
+/* ---------------------------------------------------------------- stream -- */
+enum { PI_KIND_DIFFUSION = 0, PI_KIND_DECISION = 1 };
+
+PI_HD void pi_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Julia rand(Float32): 23 random mantissa bits, value in [0, 1). */
+PI_HD float pi_u32_to_unit(uint32_t u)
+{
+    union { uint32_t i; float f; } b;
+    b.i = (u >> 9) | 0x3f800000u;
+    return b.f - 1.0f;
+}
+
+/* Three uniforms of the shared stream. */
+PI_HD void pi_uniform3(uint64_t seed, int64_t iteration, int32_t species, int32_t kind,
+                       int64_t slot, float* u0, float* u1, float* u2)
+{
+    uint32_t ctr[4] = { (uint32_t)slot, (uint32_t)iteration, (uint32_t)species, (uint32_t)kind };
+    uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) };
+    uint32_t o[4];
+    pi_philox4x32_10(ctr, key, o);
+    *u0 = pi_u32_to_unit(o[0]);
+    *u1 = pi_u32_to_unit(o[1]);
+    *u2 = pi_u32_to_unit(o[2]);
+}
+
+/* ------------------------------------------------------- transcendentals -- */
+PI_HD double pi_exp_f64(double x)
+{
+    if (x != x) return x;
+    if (x > 709.782712893384) return INFINITY;
+    if (x < -745.1332191019412) return 0.0;
+    /* k = nearest integer to x / ln2 */
+    const double inv_ln2 = 1.4426950408889634074;
+    const double ln2_hi = 6.93147180369123816490e-01;   /* 32 significant bits: k*ln2_hi exact */
+    const double ln2_lo = 1.90821492927058770002e-10;
+    double kd = rint(x * inv_ln2);
+    double r = (x - kd * ln2_hi) - kd * ln2_lo;
+    /* Taylor series of e^r, Horner form, degree 13 */
+    double p = 1.6059043836821614599e-10;                /* 1/13! */
+    p = p * r + 2.0876756987868098979e-09;               /* 1/12! */
+    p = p * r + 2.5052108385441718775e-08;               /* 1/11! */
+    p = p * r + 2.7557319223985890653e-07;               /* 1/10! */
+    p = p * r + 2.7557319223985890653e-06;               /* 1/9!  */
+    p = p * r + 2.4801587301587301587e-05;               /* 1/8!  */
+    p = p * r + 1.9841269841269841270e-04;               /* 1/7!  */
+    p = p * r + 1.3888888888888888889e-03;               /* 1/6!  */
+    p = p * r + 8.3333333333333333333e-03;               /* 1/5!  */
+    p = p * r + 4.1666666666666666667e-02;               /* 1/4!  */
+    p = p * r + 1.6666666666666666667e-01;               /* 1/3!  */
+    p = p * r + 0.5;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    int k = (int)kd;
+    /* 2^k applied in two halves so that 2^k itself never overflows/underflows */
+    int k1 = k / 2, k2 = k - k1;
+    return ldexp(ldexp(p, k1), k2);
+}
+
+PI_HD float pi_expf(float x) { return (float)pi_exp_f64((double)x); }
+
+/* Julia Float32 `x ^ 2f0` and `x ^ 4f0` (integer-valued exponent branch). */
+PI_HD float pi_pow2f(float x) { double d = (double)x; return (float)(d * d); }
+PI_HD float pi_pow4f(float x) { double d = (double)x; double s = d * d; return (float)(s * s); }
+PI_HD double pi_pow2d(double x) { return x * x; }
+PI_HD double pi_pow4d(double x) { double s = x * x; return s * s; }
+
+/* ------------------------------------------------- Julia IEEE semantics -- */
+PI_HD int pi_signbitf(float x) { union { float f; uint32_t i; } b; b.f = x; return (int)(b.i >> 31); }
+PI_HD int pi_signbitd(double x) { union { double f; uint64_t i; } b; b.f = x; return (int)(b.i >> 63); }
+
+/* Base.max / Base.min for IEEE floats: NaN wins, max(-0,+0) = +0, min = -0. */
+PI_HD float jl_maxf(float x, float y)
+{
+    float d = x - y;
+    if (x != x || y != y) return d;
+    return pi_signbitf(d) ? y : x;
+}
+PI_HD float jl_minf(float x, float y)
+{
+    float d = x - y;
+    if (x != x || y != y) return d;
+    return pi_signbitf(d) ? x : y;
+}
+PI_HD double jl_maxd(double x, double y)
+{
+    double d = x - y;
+    if (x != x || y != y) return d;
+    return pi_signbitd(d) ? y : x;
+}
+PI_HD double jl_mind(double x, double y)
+{
+    double d = x - y;
+    if (x != x || y != y) return d;
+    return pi_signbitd(d) ? x : y;
+}
+/* Base.isless for IEEE floats (total order, NaN last, -0 < +0). */
+PI_HD int jl_islessf(float x, float y)
+{
+    int xn = (x != x), yn = (y != y);
+    return (!xn && (yn || (pi_signbitf(x) && !pi_signbitf(y)))) || (x < y);
+}
+PI_HD int jl_islessd(double x, double y)
+{
+    int xn = (x != x), yn = (y != y);
+    return (!xn && (yn || (pi_signbitd(x) && !pi_signbitd(y)))) || (x < y);
+}
+/* Bool * x  ==  ifelse(b, x, copysign(0, x)) */
+PI_HD float jl_maskf(int b, float x) { return b ? x : copysignf(0.0f, x); }
+PI_HD double jl_maskd(int b, double x) { return b ? x : copysign(0.0, x); }
+/* mod(x, 1) for a float x (Base.mod with a float remainder). */
+PI_HD float jl_mod1f(float x)
+{
+    float r = fmodf(x, 1.0f);
+    if (r == 0.0f) return 0.0f;                 /* copysign(r, 1) */
+    return (r < 0.0f) ? r + 1.0f : r;
+}
+PI_HD double jl_mod1d(double x)
+{
+    double r = fmod(x, 1.0);
+    if (r == 0.0) return 0.0;
+    return (r < 0.0) ? r + 1.0 : r;
+}
+
+#endif /* PI_MATH_H */

@@ -1,0 +1,1068 @@
+/*
+ * pi_oracle.cpp - TEST INFRASTRUCTURE ONLY (the checker, never the product).
+ *
+ * A CPU restatement, kernel by kernel and in the same floating-point operation
+ * order, of PlanktonIndividuals.jl v0.7.5's CPU() time step for QuotaMode
+ * without abiotic particles, as run by `julia -t 1` (KernelAbstractions' CPU
+ * backend with one thread visits every ndrange index in ascending order, so
+ * every `@atomic` accumulation happens in slot order, kernel after kernel).
+ *
+ * Every function cites the reference file:line it restates.  Julia semantics
+ * that matter for bits: Bool*x (jl_mask), NaN-propagating min/max, isless,
+ * unsafe_trunc, Float32 * Int promotions, broadcast unaliasing of halo copies.
+ * Transcendentals and the random stream come from include/pi_math.h (the
+ * stream replaces the reference's entropy-seeded MersenneTwister).
+ *
+ * Deliberate, documented deviations (none changes a Float32 value of an
+ * active individual or of the interior tracer fields):
+ *  - ΔzC(...) in particle_diffusion.jl:4-6 reads dzC[zi+2Hz] which is one
+ *    element past the array when a particle sits exactly at z == Nz; the
+ *    restatement clamps that read to the last element.
+ *  - Individual rate fields (PS, VDOC, ...) are kept internally but only the
+ *    state fields listed in pi_hip.h are downloadable.
+ */
+#include "pi_oracle.h"
+#include "../include/pi_math.h"
+
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+static thread_local std::string g_err;
+static pi_status fail(pi_status s, const std::string& msg) { g_err = msg; return s; }
+
+template <typename FT> struct N_;
+template <> struct N_<float> {
+    static float max(float a, float b) { return jl_maxf(a, b); }
+    static float min(float a, float b) { return jl_minf(a, b); }
+    static int isless(float a, float b) { return jl_islessf(a, b); }
+    static float mask(int b, float x) { return jl_maskf(b, x); }
+    static float mod1(float x) { return jl_mod1f(x); }
+    static float exp(float x) { return pi_expf(x); }
+    static float pow2(float x) { return pi_pow2f(x); }
+    static float pow4(float x) { return pi_pow4f(x); }
+    static float sqrt(float x) { return std::sqrt(x); }
+    static float rem(float x, float y) { return std::fmod(x, y); }
+};
+template <> struct N_<double> {
+    static double max(double a, double b) { return jl_maxd(a, b); }
+    static double min(double a, double b) { return jl_mind(a, b); }
+    static int isless(double a, double b) { return jl_islessd(a, b); }
+    static double mask(int b, double x) { return jl_maskd(b, x); }
+    static double mod1(double x) { return jl_mod1d(x); }
+    static double exp(double x) { return pi_exp_f64(x); }
+    static double pow2(double x) { return pi_pow2d(x); }
+    static double pow4(double x) { return pi_pow4d(x); }
+    static double sqrt(double x) { return std::sqrt(x); }
+    static double rem(double x, double y) { return std::fmod(x, y); }
+};
+
+/* Float32-rounded literals (Julia `0.5f0` etc.), widened in FP64 mode. */
+#define L(v) ((FT)(float)(v))
+
+struct orc_model { virtual ~orc_model() {} int precision = 32; };
+
+template <typename FT>
+struct Oracle : orc_model {
+    using M = N_<FT>;
+    typedef std::vector<FT> V;
+
+    /* ---- grid (Grids/rectilinear_grid.jl:1-25) ---- */
+    int Nx, Ny, Nz, Hx, Hy, Hz, Px, Py, Pz;
+    int64_t ncell;
+    int tx, ty, tz;
+    FT dx, dy;
+    V dzC, dzF;                     /* 1-based: dzC[k-1] */
+    std::vector<uint8_t> lm;        /* padded landmask */
+    int nsp;
+    int64_t cap;
+    std::vector<std::array<FT, PI_NPARAMS>> P;
+    std::vector<int> dvid_type;
+    FT B[PI_NBGC];
+    uint64_t seed;
+    int64_t iteration = 0;
+    FT t = 0;
+    std::vector<int> diag_tr;
+    bool failed = false;
+
+    struct Plank {
+        V x, y, z, Sz, Bm, Cq, Nq, Pq, Chl, gen, age;
+        V PS, VDOC, VNH4, VNO3, VPO4, rho, resp, BS, exu, graz, mort, dvid;
+        std::vector<int64_t> xi, yi, zi, idx;
+        std::vector<uint8_t> ac;
+    };
+    std::vector<Plank> sp;
+
+    V tr[PI_NTRACERS], Gcs[PI_NTRACERS], ttemp[PI_NTRACERS], plk[PI_NTRACERS];
+    V vel0[3], velh[3], vel1[3];
+    V PARF, temp, flux_sink, par, par0, Chl, pop;
+    V diagPAR;
+    std::vector<V> diagTr;
+    std::vector<std::array<V, 5>> diagSp;   /* num, graz, mort, dvid, ptc */
+    /* timestepper scratch (timestepper.jl:40-55) */
+    V rx, ry, rz, vx, vy, vz, u1, v1, w1, u2, v2, w2;
+    V tNH4, tNO3, tPO4, tDOC, tDFe, tpar, tT, tpop, tdpar, idc;
+    std::vector<int64_t> idc_int;
+
+    inline int64_t I(int i, int j, int k) const {
+        return (int64_t)(i - 1) + (int64_t)Px * ((int64_t)(j - 1) + (int64_t)Py * (int64_t)(k - 1));
+    }
+    inline bool LM(int i, int j, int k) const { return lm[I(i, j, k)] != 0; }
+
+    /* ---- metrics (Grids/areas_volumes_spacings.jl:9-22) ---- */
+    inline FT dzC_at(int64_t k) const {   /* g.dzC[k], 1-based; clamp: see header */
+        if (k < 1) k = 1;
+        if (k > Pz) k = Pz;
+        return dzC[k - 1];
+    }
+    inline FT dzF_at(int k) const { return dzF[k - 1]; }
+    inline FT Ax(int k) const { return dy * dzF_at(k); }
+    inline FT Ay(int k) const { return dx * dzF_at(k); }
+    inline FT Az() const { return dx * dy; }
+    inline FT vol(int k) const { return dx * dy * dzF_at(k); }
+
+    /* =============================================================== setup */
+    pi_status init(const pi_model_desc* d) {
+        const pi_grid_desc& g = d->grid;
+        Nx = g.Nx; Ny = g.Ny; Nz = g.Nz; Hx = g.Hx; Hy = g.Hy; Hz = g.Hz;
+        tx = g.topo_x; ty = g.topo_y; tz = g.topo_z;
+        if (Nx < 1 || Ny < 1 || Nz < 1 || Hx < 2 || Hy < 2 || Hz < 2)
+            return fail(PI_ERR_ARG, "grid: N >= 1 and halo >= 2 required");
+        if (tz != PI_BOUNDED) return fail(PI_ERR_ARG, "grid: z topology must be Bounded (tracer_sinking.jl)");
+        Px = Nx + 2 * Hx; Py = Ny + 2 * Hy; Pz = Nz + 2 * Hz;
+        ncell = (int64_t)Px * Py * Pz;
+        dx = (FT)g.dx; dy = (FT)g.dy;
+        dzC.resize(Pz); dzF.resize(Pz);
+        for (int k = 0; k < Pz; ++k) { dzC[k] = (FT)g.dzC[k]; dzF[k] = (FT)g.dzF[k]; }
+        lm.assign(ncell, 0);
+        if (g.landmask) {
+            for (int64_t c = 0; c < ncell; ++c) lm[c] = g.landmask[c] ? 1 : 0;
+        } else {
+            /* landmask_validation with landmask = ones (Grids/utils.jl:92-129) */
+            std::vector<float> tmp(ncell, 0.0f);
+            for (int k = Hz + 1; k <= Hz + Nz; ++k)
+                for (int j = Hy + 1; j <= Hy + Ny; ++j)
+                    for (int i = Hx + 1; i <= Hx + Nx; ++i) tmp[I(i, j, k)] = 1.0f;
+            orc_fill_halo_lm(tmp);
+            for (int64_t c = 0; c < ncell; ++c) lm[c] = tmp[c] != 0.0f;
+        }
+        nsp = d->n_species; cap = d->capacity;
+        if (nsp < 1 || cap < 1) return fail(PI_ERR_ARG, "n_species and capacity must be >= 1");
+        P.resize(nsp); dvid_type.resize(nsp);
+        for (int s = 0; s < nsp; ++s) {
+            for (int q = 0; q < PI_NPARAMS; ++q) P[s][q] = (FT)d->params[s * PI_NPARAMS + q];
+            dvid_type[s] = (int)d->params[s * PI_NPARAMS + PI_P_dvid_type];
+            if (dvid_type[s] < 1 || dvid_type[s] > 3)
+                return fail(PI_ERR_ARG, "Wrong cell division type, must be in 1 to 3");
+        }
+        for (int q = 0; q < PI_NBGC; ++q) B[q] = (FT)d->bgc[q];
+        seed = d->rng_seed;
+        for (int q = 0; q < d->n_diag_tracers; ++q) diag_tr.push_back(d->diag_tracers[q]);
+
+        sp.resize(nsp);
+        for (auto& p : sp) {
+            for (V* f : {&p.x, &p.y, &p.z, &p.Sz, &p.Bm, &p.Cq, &p.Nq, &p.Pq, &p.Chl, &p.gen, &p.age,
+                         &p.PS, &p.VDOC, &p.VNH4, &p.VNO3, &p.VPO4, &p.rho, &p.resp, &p.BS, &p.exu,
+                         &p.graz, &p.mort, &p.dvid})
+                f->assign(cap, (FT)0);
+            p.xi.assign(cap, 0); p.yi.assign(cap, 0); p.zi.assign(cap, 0); p.idx.assign(cap, 0);
+            p.ac.assign(cap, 0);
+        }
+        for (int q = 0; q < PI_NTRACERS; ++q) {
+            tr[q].assign(ncell, 0); Gcs[q].assign(ncell, 0); ttemp[q].assign(ncell, 0); plk[q].assign(ncell, 0);
+        }
+        for (int q = 0; q < 3; ++q) { vel0[q].assign(ncell, 0); velh[q].assign(ncell, 0); vel1[q].assign(ncell, 0); }
+        PARF.assign((int64_t)Nx * Ny, 0);
+        for (V* f : {&temp, &flux_sink, &par, &par0, &Chl, &pop, &diagPAR}) f->assign(ncell, 0);
+        diagTr.assign(diag_tr.size(), V(ncell, 0));
+        diagSp.resize(nsp);
+        for (auto& a : diagSp) for (auto& v : a) v.assign(ncell, 0);
+        for (V* f : {&rx, &ry, &rz, &vx, &vy, &vz, &u1, &v1, &w1, &u2, &v2, &w2, &tNH4, &tNO3, &tPO4,
+                     &tDOC, &tDFe, &tpar, &tT, &tpop, &tdpar, &idc})
+            f->assign(cap, 0);
+        idc_int.assign(cap, 0);
+        return PI_OK;
+    }
+
+    void orc_fill_halo_lm(std::vector<float>& c) {
+        /* Grids/utils.jl:113-127: same copies as fill_halo_tracer for Bool arrays */
+        fill_halo_generic(c.data(), tx, ty);
+    }
+
+    /* ================================================== halo regions ====
+     * Biogeochemistry/HaloRegions/halo_regions.jl:1-35.  Each `@views @.`
+     * statement copies its source first (broadcast unaliasing), hence the
+     * snapshot buffers. */
+    template <typename T>
+    void planes_copy(T* c, int dim, int d0, int s0, int n) {   /* dst[d0..d0+n) <- src[s0..s0+n) */
+        std::vector<T> buf;
+        int64_t np = plane_size(dim);
+        buf.resize((size_t)(np * n));
+        for (int a = 0; a < n; ++a) plane_get(c, dim, s0 + a, buf.data() + a * np);
+        for (int a = 0; a < n; ++a) plane_set(c, dim, d0 + a, buf.data() + a * np);
+    }
+    template <typename T>
+    void planes_bcast(T* c, int dim, int d0, int n, int s) {   /* dst[d0..d0+n) <- src plane s */
+        std::vector<T> buf((size_t)plane_size(dim));
+        plane_get(c, dim, s, buf.data());
+        for (int a = 0; a < n; ++a) plane_set(c, dim, d0 + a, buf.data());
+    }
+    template <typename T>
+    void planes_zero(T* c, int dim, int d0, int n) {
+        std::vector<T> buf((size_t)plane_size(dim), (T)0);
+        for (int a = 0; a < n; ++a) plane_set(c, dim, d0 + a, buf.data());
+    }
+    int64_t plane_size(int dim) const {
+        return dim == 0 ? (int64_t)Py * Pz : dim == 1 ? (int64_t)Px * Pz : (int64_t)Px * Py;
+    }
+    template <typename T>
+    void plane_get(const T* c, int dim, int p, T* out) const {
+        int64_t n = 0;
+        if (dim == 0) { for (int k = 1; k <= Pz; ++k) for (int j = 1; j <= Py; ++j) out[n++] = c[I(p, j, k)]; }
+        else if (dim == 1) { for (int k = 1; k <= Pz; ++k) for (int i = 1; i <= Px; ++i) out[n++] = c[I(i, p, k)]; }
+        else { for (int j = 1; j <= Py; ++j) for (int i = 1; i <= Px; ++i) out[n++] = c[I(i, j, p)]; }
+    }
+    template <typename T>
+    void plane_set(T* c, int dim, int p, const T* in) const {
+        int64_t n = 0;
+        if (dim == 0) { for (int k = 1; k <= Pz; ++k) for (int j = 1; j <= Py; ++j) c[I(p, j, k)] = in[n++]; }
+        else if (dim == 1) { for (int k = 1; k <= Pz; ++k) for (int i = 1; i <= Px; ++i) c[I(i, p, k)] = in[n++]; }
+        else { for (int j = 1; j <= Py; ++j) for (int i = 1; i <= Px; ++i) c[I(i, j, p)] = in[n++]; }
+    }
+    int Nd(int dim) const { return dim == 0 ? Nx : dim == 1 ? Ny : Nz; }
+    int Hd(int dim) const { return dim == 0 ? Hx : dim == 1 ? Hy : Hz; }
+    int Td(int dim) const { return dim == 0 ? tx : dim == 1 ? ty : tz; }
+    /* fill_halo_west!/south!/top! (halo_regions.jl:2-4,10-12) */
+    template <typename T> void fill_lo(T* c, int dim) {
+        int N = Nd(dim), H = Hd(dim);
+        if (Td(dim) == PI_PERIODIC) planes_copy(c, dim, 1, N + 1, H);
+        else planes_bcast(c, dim, 1, H, H + 1);
+    }
+    /* fill_halo_east!/north!/bottom! (halo_regions.jl:6-8,14-16) */
+    template <typename T> void fill_hi(T* c, int dim) {
+        int N = Nd(dim), H = Hd(dim);
+        if (Td(dim) == PI_PERIODIC) planes_copy(c, dim, N + H + 1, 1 + H, H);
+        else planes_bcast(c, dim, N + H + 1, H, N + H);
+    }
+    /* fill_halo_*_vel! (halo_regions.jl:18-20,26-28) */
+    template <typename T> void fill_hi_vel(T* c, int dim) {
+        int N = Nd(dim), H = Hd(dim);
+        if (Td(dim) == PI_PERIODIC) fill_hi(c, dim);
+        else planes_bcast(c, dim, N + H + 2, H - 1, N + H + 1);
+    }
+    /* fill_halo_*_Gc! (halo_regions.jl:22-24,30-32) */
+    template <typename T> void fill_hi_Gc(T* c, int dim) {
+        int N = Nd(dim), H = Hd(dim);
+        if (Td(dim) == PI_PERIODIC) fill_hi(c, dim);
+        else planes_zero(c, dim, N + H + 1, H);
+    }
+    template <typename T> void fill_halo_generic(T* c, int tx_, int ty_) {
+        (void)tx_; (void)ty_;
+        fill_lo(c, 0); fill_hi(c, 0); fill_lo(c, 1); fill_hi(c, 1); fill_lo(c, 2); fill_hi(c, 2);
+    }
+    /* fill_halo_tracer! (halo_regions.jl:37-47) */
+    void fill_halo_tracer(V* f) {
+        for (int q = 0; q < PI_NTRACERS; ++q) fill_halo_generic(f[q].data(), tx, ty);
+    }
+    /* fill_halo_Gcs! (halo_regions.jl:49-56) */
+    void fill_halo_Gcs(V* f) {
+        for (int q = 0; q < PI_NTRACERS; ++q) { fill_hi_Gc(f[q].data(), 0); fill_hi_Gc(f[q].data(), 1); fill_hi_Gc(f[q].data(), 2); }
+    }
+    /* fill_halo_u!/v!/w! (halo_regions.jl:64-99) */
+    void fill_halo_vel(V* v) {
+        FT* u = v[0].data(); FT* vv = v[1].data(); FT* w = v[2].data();
+        fill_hi_vel(u, 0); fill_lo(u, 0); fill_lo(u, 1); fill_hi(u, 1); fill_lo(u, 2); fill_hi(u, 2);
+        fill_hi_vel(vv, 1); fill_lo(vv, 0); fill_hi(vv, 0); fill_lo(vv, 1); fill_lo(vv, 2); fill_hi(vv, 2);
+        fill_hi_vel(w, 2); fill_lo(w, 0); fill_hi(w, 0); fill_lo(w, 1); fill_hi(w, 1); fill_lo(w, 2);
+    }
+
+    /* ================================================== inputs ==========
+     * vel_copy! / copy_interior! (Simulation/utils.jl:5-34) */
+    void set_velocity(const float* u, const float* v, const float* w) {
+        int nxu = Nx + (tx == PI_BOUNDED), nyv = Ny + (ty == PI_BOUNDED), nzw = Nz + (tz == PI_BOUNDED);
+        for (int k = 0; k < Nz; ++k) for (int j = 0; j < Ny; ++j) for (int i = 0; i < nxu; ++i)
+            vel1[0][I(Hx + 1 + i, Hy + 1 + j, Hz + 1 + k)] = (FT)u[i + (int64_t)nxu * (j + (int64_t)Ny * k)];
+        for (int k = 0; k < Nz; ++k) for (int j = 0; j < nyv; ++j) for (int i = 0; i < Nx; ++i)
+            vel1[1][I(Hx + 1 + i, Hy + 1 + j, Hz + 1 + k)] = (FT)v[i + (int64_t)Nx * (j + (int64_t)nyv * k)];
+        for (int k = 0; k < nzw; ++k) for (int j = 0; j < Ny; ++j) for (int i = 0; i < Nx; ++i)
+            vel1[2][I(Hx + 1 + i, Hy + 1 + j, Hz + 1 + k)] = (FT)w[i + (int64_t)Nx * (j + (int64_t)Ny * k)];
+        fill_halo_vel(vel1);
+    }
+    void set_temperature(const float* tt) {
+        for (int k = 0; k < Nz; ++k) for (int j = 0; j < Ny; ++j) for (int i = 0; i < Nx; ++i)
+            temp[I(Hx + 1 + i, Hy + 1 + j, Hz + 1 + k)] = (FT)tt[i + (int64_t)Nx * (j + (int64_t)Ny * k)];
+    }
+
+    /* ================================================== particles =======
+     * Individuals/Advection/interpolation.jl:1-83 */
+    FT itpl(const FT* f, FT x, FT y, FT z, bool ac, int comp) const {
+        /* comp 0: u (x on faces), 1: v (y on faces), 2: w (z on faces) */
+        FT xs = comp == 0 ? x + L(1) : x + L(0.5f);
+        FT ys = comp == 1 ? y + L(1) : y + L(0.5f);
+        FT zs = comp == 2 ? z + L(1) : z + L(0.5f);
+        FT xf = M::mask(ac, xs), yf = M::mask(ac, ys), zf = M::mask(ac, zs);
+        FT xd = M::mod1(xf), yd = M::mod1(yf), zd = M::mod1(zf);
+        int64_t xi = (int64_t)xf + Hx, yi = (int64_t)yf + Hy, zi = (int64_t)zf + Hz;
+        const FT one = L(1);
+        auto D = [&](int64_t k) -> FT { return comp == 0 ? dx : comp == 1 ? dy : dzC_at(k); };
+        auto F = [&](int64_t i, int64_t j, int64_t k) -> FT { return f[I((int)i, (int)j, (int)k)]; };
+        FT s = (one - xd) * (one - yd) * (one - zd) * F(xi, yi, zi) / D(zi);
+        s = s + (one - xd) * (one - yd) * zd * F(xi, yi, zi + 1) / D(zi + 1);
+        s = s + (one - xd) * yd * (one - zd) * F(xi, yi + 1, zi) / D(zi);
+        s = s + (one - xd) * yd * zd * F(xi, yi + 1, zi + 1) / D(zi + 1);
+        s = s + xd * (one - yd) * (one - zd) * F(xi + 1, yi, zi) / D(zi);
+        s = s + xd * (one - yd) * zd * F(xi + 1, yi, zi + 1) / D(zi + 1);
+        s = s + xd * yd * (one - zd) * F(xi + 1, yi + 1, zi) / D(zi);
+        s = s + xd * yd * zd * F(xi + 1, yi + 1, zi + 1) / D(zi + 1);
+        return s;
+    }
+    /* vel_interpolate_kernel! (particle_advection_kernels.jl:26-37) */
+    void vel_interpolate(V& ut, V& vt, V& wt, const V& x, const V& y, const V& z,
+                         const std::vector<uint8_t>& ac, V* vel) {
+        for (int64_t i = 0; i < cap; ++i) {
+            ut[i] = M::mask(ac[i], itpl(vel[0].data(), x[i], y[i], z[i], ac[i], 0));
+            vt[i] = M::mask(ac[i], itpl(vel[1].data(), x[i], y[i], z[i], ac[i], 1));
+            wt[i] = M::mask(ac[i], itpl(vel[2].data(), x[i], y[i], z[i], ac[i], 2));
+        }
+    }
+    /* calc_coord_kernel! (particle_advection_kernels.jl:40-50) */
+    void calc_coord(V& ox, V& oy, V& oz, const V& px, const V& py, const V& pz,
+                    const V& u, const V& v, const V& w, const std::vector<uint8_t>& ac, FT dt, FT wt) {
+        for (int64_t i = 0; i < cap; ++i) {
+            ox[i] = px[i] + M::mask(ac[i], wt * u[i] * dt);
+            oy[i] = py[i] + M::mask(ac[i], wt * v[i] * dt);
+            oz[i] = pz[i] - M::mask(ac[i], wt * w[i] * dt);
+        }
+    }
+    /* particle_boundary_condition (particle_advection_kernels.jl:2-11) */
+    static FT pbc(FT x, int N, int topo) {
+        const FT xl = (FT)0, xr = (FT)N;
+        if (topo == PI_PERIODIC) {
+            if (x > xr) return xl + (x - xr);
+            if (x < xl) return xr - (xl - x);
+            return x;
+        }
+        if (x > xr) return xr - (x - xr) * L(1);
+        if (x < xl) return xl + (xl - x) * L(1);
+        return x;
+    }
+    /* particle_boundaries_kernel! (particle_advection_kernels.jl:13-23) */
+    void boundaries(V& x, V& y, V& z, const std::vector<uint8_t>& ac) {
+        for (int64_t i = 0; i < cap; ++i) {
+            x[i] = M::mask(ac[i], pbc(x[i], Nx, tx));
+            y[i] = M::mask(ac[i], pbc(y[i], Ny, ty));
+            z[i] = M::mask(ac[i], pbc(z[i], Nz, tz));
+        }
+    }
+    /* particle_advection! RK4 (Individuals/Advection/particle_advection.jl:2-56) */
+    void particle_advection(Plank& p, FT dt) {
+        vel_interpolate(u1, v1, w1, p.x, p.y, p.z, p.ac, vel0);
+        u2 = u1; v2 = v1; w2 = w1;
+        calc_coord(vx, vy, vz, p.x, p.y, p.z, u1, v1, w1, p.ac, dt, L(0.5f));
+        boundaries(vx, vy, vz, p.ac);
+        vel_interpolate(u1, v1, w1, vx, vy, vz, p.ac, velh);
+        for (int64_t i = 0; i < cap; ++i) { u2[i] = u2[i] + u1[i] * L(2); v2[i] = v2[i] + v1[i] * L(2); w2[i] = w2[i] + w1[i] * L(2); }
+        calc_coord(vx, vy, vz, p.x, p.y, p.z, u1, v1, w1, p.ac, dt, L(0.5f));
+        boundaries(vx, vy, vz, p.ac);
+        vel_interpolate(u1, v1, w1, vx, vy, vz, p.ac, velh);
+        for (int64_t i = 0; i < cap; ++i) { u2[i] = u2[i] + u1[i] * L(2); v2[i] = v2[i] + v1[i] * L(2); w2[i] = w2[i] + w1[i] * L(2); }
+        calc_coord(vx, vy, vz, p.x, p.y, p.z, u1, v1, w1, p.ac, dt, L(1));
+        boundaries(vx, vy, vz, p.ac);
+        vel_interpolate(u1, v1, w1, vx, vy, vz, p.ac, vel1);
+        for (int64_t i = 0; i < cap; ++i) { u2[i] = u2[i] + u1[i]; v2[i] = v2[i] + v1[i]; w2[i] = w2[i] + w1[i]; }
+        for (int64_t i = 0; i < cap; ++i) { u2[i] = u2[i] / L(6); v2[i] = v2[i] / L(6); w2[i] = w2[i] / L(6); }
+        calc_coord(p.x, p.y, p.z, p.x, p.y, p.z, u2, v2, w2, p.ac, dt, L(1));
+        boundaries(p.x, p.y, p.z, p.ac);
+    }
+    /* particle_diffusion! (Individuals/Advection/particle_diffusion.jl:2-33) */
+    void particle_diffusion(Plank& p, int s, FT dt) {
+        const FT kx = B[PI_B_kappa_hP], ky = B[PI_B_kappa_hP], kz = B[PI_B_kappa_vP];
+        for (int64_t i = 0; i < cap; ++i) {
+            float a, b, c;
+            pi_uniform3(seed, iteration, s, PI_KIND_DIFFUSION, i, &a, &b, &c);
+            rx[i] = (FT)a; ry[i] = (FT)b; rz[i] = (FT)c;
+        }
+        for (int64_t i = 0; i < cap; ++i) {
+            rx[i] = (rx[i] - L(0.5f)) * L(2);
+            ry[i] = (ry[i] - L(0.5f)) * L(2);
+            rz[i] = (rz[i] - L(0.5f)) * L(2);
+        }
+        const FT sx = M::sqrt(kx * dt), sy = M::sqrt(ky * dt), sz = M::sqrt(kz * dt);
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_diffusion_kernel! :2-8, stale xi/yi/zi (+H twice) */
+            rx[i] = rx[i] * sx / dx;
+            ry[i] = ry[i] * sy / dy;
+            rz[i] = rz[i] * sz / dzC_at(p.zi[i] + Hz);
+        }
+        for (int64_t i = 0; i < cap; ++i) {
+            p.x[i] = p.x[i] + M::mask(p.ac[i], rx[i]);
+            p.y[i] = p.y[i] + M::mask(p.ac[i], ry[i]);
+            p.z[i] = p.z[i] + M::mask(p.ac[i], rz[i]);
+        }
+        boundaries(p.x, p.y, p.z, p.ac);
+    }
+    /* find_inds_kernel! (Individuals/utils.jl:2-12) */
+    void find_inds(Plank& p) {
+        for (int64_t i = 0; i < cap; ++i) {
+            p.xi[i] = (int64_t)M::mask(p.ac[i], p.x[i] + L(1)) + Hx;
+            p.yi[i] = (int64_t)M::mask(p.ac[i], p.y[i] + L(1)) + Hy;
+            p.zi[i] = (int64_t)M::mask(p.ac[i], p.z[i] + L(1)) + Hz;
+        }
+    }
+    inline int64_t cell_of(const Plank& p, int64_t i) const { return I((int)p.xi[i], (int)p.yi[i], (int)p.zi[i]); }
+    /* acc_chl_kernel! / acc_counts_kernel! (Individuals/utils.jl:33-52) */
+    void acc_chl(const Plank& p) {
+        for (int64_t i = 0; i < cap; ++i) Chl[cell_of(p, i)] += M::mask(p.ac[i], p.Chl[i]);
+    }
+    void acc_counts(const Plank& p) {
+        for (int64_t i = 0; i < cap; ++i) pop[cell_of(p, i)] += (FT)(p.ac[i] ? 1 : 0);
+    }
+    /* calc_par_kernel! (Individuals/utils.jl:55-68), one call per level (time_step.jl:63-67) */
+    void calc_par(int ki) {
+        const FT kc = B[PI_B_kc], kw = B[PI_B_kw];
+        for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
+            int ii = i + Hx, jj = j + Hy, kk = ki + Hz;
+            FT atten = (Chl[I(ii, jj, kk)] / vol(kk) * kc + kw) * dzF_at(kk);
+            int64_t f = (int64_t)(i - 1) + (int64_t)Nx * (j - 1);
+            par[I(ii, jj, kk)] = PARF[f] * (L(1) - M::exp(-atten)) / atten;
+            PARF[f] = PARF[f] * M::exp(-atten);
+        }
+    }
+    /* find_NPT_kernel! (Individuals/utils.jl:14-30) */
+    void find_NPT(const Plank& p) {
+        const FT z0 = L(0);
+        for (int64_t i = 0; i < cap; ++i) {
+            int64_t c = cell_of(p, i);
+            bool a = p.ac[i];
+            tNH4[i] = M::mask(a, M::max(z0, tr[PI_NH4][c]));
+            tNO3[i] = M::mask(a, M::max(z0, tr[PI_NO3][c]));
+            tPO4[i] = M::mask(a, M::max(z0, tr[PI_PO4][c]));
+            tDOC[i] = M::mask(a, M::max(z0, tr[PI_DOC][c]));
+            tDFe[i] = M::mask(a, M::max(z0, tr[PI_DFe][c]));
+            tpar[i] = M::mask(a, par[c]);
+            tdpar[i] = M::mask(a, par[c] - par0[c]);
+            tT[i] = M::mask(a, temp[c]);
+            tpop[i] = M::mask(a, pop[c]);
+        }
+    }
+
+    /* ---- shape functions (Individuals/utils.jl:109-121) ---- */
+    static FT shape_dec(FT x, FT xmax, FT k, int pw) {
+        FT fx = M::max(L(0), M::min(L(1), L(1) - x / xmax));
+        FT fp = pw == 2 ? M::pow2(fx) : M::pow4(fx);
+        return fp / (k + fp);
+    }
+    static FT shape_inc(FT x, FT xmax, FT k, int pw) { return L(1) - shape_dec(x, xmax, k, pw); }
+
+    /* ---- QuotaMode growth (Plankton/QuotaMode/growth_kernels.jl:1-46) ---- */
+    static FT tempFunc_PS(FT T, const std::array<FT, PI_NPARAMS>& q) {
+        FT x = T - q[PI_P_Topt], xmax = q[PI_P_Tmax] - q[PI_P_Topt];
+        FT regT = shape_dec(x, xmax, L(4.0e-2f), 4);
+        FT k = M::exp(-q[PI_P_Ea] / (L(8.3145f) * (T + L(273.15f)))) * regT;
+        k = M::max(L(0), k);
+        FT OGT = M::exp(-q[PI_P_Ea] / (L(8.3145f) * (q[PI_P_Topt] + L(273.15f))));
+        return M::min(L(1), k / OGT);
+    }
+    static FT tempFunc(FT T, const std::array<FT, PI_NPARAMS>& q) {
+        FT k = M::exp(-q[PI_P_Ea] / (L(8.3145f) * (T + L(273.15f))));
+        k = M::max(L(0), k);
+        FT OGT = M::exp(-q[PI_P_Ea] / (L(8.3145f) * (q[PI_P_Topt] + L(273.15f))));
+        return M::min(L(1), k / OGT);
+    }
+
+    /* ---- losses (Plankton/QuotaMode/consume_loss.jl:17-33, division_death.jl:2-25) ---- */
+    void calc_loss(const Plank& p, const V& loss, FT fC, FT fN, FT fP, FT R_NC, FT R_PC) {
+        for (int64_t i = 0; i < cap; ++i) {
+            int64_t c = cell_of(p, i);
+            bool a = p.ac[i];
+            plk[PI_DOC][c] += M::mask(a, (p.Bm[i] + p.Cq[i]) * fC) * loss[i];
+            plk[PI_DON][c] += M::mask(a, (p.Bm[i] * R_NC + p.Nq[i]) * fN) * loss[i];
+            plk[PI_DOP][c] += M::mask(a, (p.Bm[i] * R_PC + p.Pq[i]) * fP) * loss[i];
+            plk[PI_POC][c] += M::mask(a, (p.Bm[i] + p.Cq[i]) * (L(1) - fC)) * loss[i];
+            plk[PI_PON][c] += M::mask(a, (p.Bm[i] * R_NC + p.Nq[i]) * (L(1) - fN)) * loss[i];
+            plk[PI_POP][c] += M::mask(a, (p.Bm[i] * R_PC + p.Pq[i]) * (L(1) - fP)) * loss[i];
+        }
+    }
+    void inactivate(Plank& p, const V& loss) {   /* Individuals/utils.jl:103-105 */
+        for (int64_t i = 0; i < cap; ++i) p.ac[i] = p.ac[i] && M::isless(loss[i], L(1));
+    }
+    /* diags_proc_kernel! (Diagnostics/diagnostics_kernels.jl:4-12) */
+    void diags_proc(V& d, const V& proc, const Plank& p) {
+        for (int64_t i = 0; i < cap; ++i) d[cell_of(p, i)] += M::mask(p.ac[i], proc[i]);
+    }
+
+    /* ---- plankton_update! (Plankton/QuotaMode/plankton_update.jl:1-34) ---- */
+    pi_status plankton_update(int s, FT dt) {
+        Plank& p = sp[s];
+        const auto& q = P[s];
+        diags_proc(diagSp[s][1], p.graz, p);
+        calc_loss(p, p.graz, q[PI_P_grazFracC], q[PI_P_grazFracN], q[PI_P_grazFracP], q[PI_P_R_NC], q[PI_P_R_PC]);
+        inactivate(p, p.graz);
+        diags_proc(diagSp[s][2], p.mort, p);
+        calc_loss(p, p.mort, q[PI_P_mortFracC], q[PI_P_mortFracN], q[PI_P_mortFracP], q[PI_P_R_NC], q[PI_P_R_PC]);
+        inactivate(p, p.mort);
+        for (int64_t i = 0; i < cap; ++i) p.dvid[i] = M::mask(p.ac[i], p.dvid[i]);
+        diags_proc(diagSp[s][3], p.dvid, p);
+        /* dvidnum = dot(dvid, ac); deactive_ind = findall(!ac) */
+        FT dvidnum = 0;
+        for (int64_t i = 0; i < cap; ++i) dvidnum += M::mask(p.ac[i], p.dvid[i]);
+        std::vector<int64_t> free_ind;
+        for (int64_t i = 0; i < cap; ++i) if (!p.ac[i]) free_ind.push_back(i + 1);
+        if (dvidnum > (FT)free_ind.size()) {
+            char buf[160];
+            snprintf(buf, sizeof buf, "number of individual exceeds the capacity at timestep %g days",
+                     (double)t / 86400.0);
+            failed = true;
+            return fail(PI_ERR_CAPACITY, buf);
+        }
+        divide(p, free_ind);
+        plankton_growth(s, dt);
+        calc_consume(p, dt);
+        /* diags_spcs! is a no-op for the default per-species diagnostics */
+        for (int64_t i = 0; i < cap; ++i) diagSp[s][0][cell_of(p, i)] += (FT)((p.ac[i] && p.ac[i]) ? 1 : 0);
+        return PI_OK;
+    }
+    /* divide! (QuotaMode/division_death.jl:28-77, Individuals/utils.jl:76-86) */
+    void divide(Plank& p, const std::vector<int64_t>& free_ind) {
+        FT acc = 0;
+        for (int64_t i = 0; i < cap; ++i) { acc = acc + p.dvid[i]; idc[i] = acc; }   /* accumulate!(+) */
+        for (int64_t i = 0; i < cap; ++i) idc_int[i] = (int64_t)idc[i];
+        for (int64_t i = 0; i < cap; ++i)                                           /* get_tind_kernel! */
+            if (p.dvid[i] == L(1)) p.idx[i] = free_ind[(size_t)(idc_int[i] - 1)];
+        for (int64_t i = 0; i < cap; ++i) {                                         /* copy_daughter_individuals_kernel! */
+            if (p.dvid[i] == L(1) && p.idx[i] != 0) {
+                int64_t d = p.idx[i] - 1;
+                p.x[d] = p.x[i]; p.y[d] = p.y[i]; p.z[d] = p.z[i];
+                p.Sz[d] = p.Sz[i]; p.Bm[d] = p.Bm[i]; p.Cq[d] = p.Cq[i]; p.Nq[d] = p.Nq[i]; p.Pq[d] = p.Pq[i];
+                p.Chl[d] = p.Chl[i]; p.gen[d] = p.gen[i]; p.ac[d] = p.ac[i]; p.dvid[d] = p.dvid[i];
+            }
+        }
+        for (int64_t i = 0; i < cap; ++i) {                                         /* divide_to_half_kernel! */
+            FT h = (L(2) - p.dvid[i]) / L(2);
+            p.Sz[i] = p.Sz[i] * h; p.Bm[i] = p.Bm[i] * h; p.Cq[i] = p.Cq[i] * h;
+            p.Nq[i] = p.Nq[i] * h; p.Pq[i] = p.Pq[i] * h; p.Chl[i] = p.Chl[i] * h;
+            p.gen[i] = p.gen[i] + p.dvid[i];
+            p.age[i] = p.age[i] * (L(1) - p.dvid[i]);
+        }
+        for (int64_t i = 0; i < cap; ++i) p.idx[i] = 0;
+    }
+    /* plankton_growth! (QuotaMode/plankton_growth.jl:2-24, growth_kernels.jl:48-171) */
+    void plankton_growth(int s, FT dt) {
+        Plank& p = sp[s];
+        const auto& q = P[s];
+        const FT e30 = L(1.0e-30f), z0 = L(0), one = L(1);
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_inorganic_uptake_kernel! */
+            bool a = p.ac[i];
+            FT aI = tpar[i] * q[PI_P_alpha] * q[PI_P_Phi];
+            FT PCm = q[PI_P_PCmax] * tempFunc_PS(tT[i], q);
+            FT PS = PCm * (one - M::exp(-aI / M::max(e30, PCm) * p.Chl[i] / M::max(e30, p.Bm[i]))) * p.Bm[i];
+            p.PS[i] = M::mask(a, PS);
+            FT Bm = p.Bm[i], Cq = p.Cq[i];
+            FT Qn = (p.Nq[i] + Bm * q[PI_P_R_NC]) / M::max(e30, Bm + Cq);
+            FT Qp = (p.Pq[i] + Bm * q[PI_P_R_PC]) / M::max(e30, Bm + Cq);
+            FT regQN = shape_dec(Qn, q[PI_P_Nqmax], L(1.0e-4f), 4);
+            FT regQP = shape_dec(Qp, q[PI_P_Pqmax], L(1.0e-4f), 4);
+            FT T = tT[i];
+            FT VNH4 = M::mask(a, q[PI_P_VNH4max] * regQN * tNH4[i] / M::max(e30, tNH4[i] + q[PI_P_KsatNH4]) * tempFunc(T, q) * Bm);
+            FT VNO3 = M::mask(a, q[PI_P_VNO3max] * regQN * tNO3[i] / M::max(e30, tNO3[i] + q[PI_P_KsatNO3]) * tempFunc(T, q) * Bm);
+            FT VPO4 = M::mask(a, q[PI_P_VPO4max] * regQP * tPO4[i] / M::max(e30, tPO4[i] + q[PI_P_KsatPO4]) * tempFunc(T, q) * Bm);
+            FT pp = M::max(one, tpop[i]);
+            p.VNH4[i] = M::min(VNH4, tNH4[i] / dt / pp);
+            p.VNO3[i] = M::min(VNO3, tNO3[i] / dt / pp);
+            p.VPO4[i] = M::min(VPO4, tPO4[i] / dt / pp);
+        }
+        for (int64_t i = 0; i < cap; ++i) {   /* update_quotas_1_kernel! */
+            p.Cq[i] = p.Cq[i] + p.PS[i] * dt;
+            p.Pq[i] = p.Pq[i] + p.VPO4[i] * dt;
+            p.Nq[i] = p.Nq[i] + (p.VNH4[i] + p.VNO3[i]) * dt;
+        }
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_organic_uptake_kernel! */
+            FT Bm = p.Bm[i], Cq = p.Cq[i];
+            FT Qc = Cq / M::max(e30, Bm + Cq);
+            FT regQ = shape_dec(Qc, q[PI_P_Cqmax], L(1.0e-4f), 4);
+            FT VN = q[PI_P_VDOCmax] * regQ * tDOC[i] / M::max(e30, tDOC[i] + q[PI_P_KsatDOC]) * tempFunc(tT[i], q) * Bm;
+            FT VD = M::min(VN, tDOC[i] / dt / M::max(one, tpop[i]));
+            p.VDOC[i] = M::mask(p.ac[i], VD);
+            p.VDOC[i] = M::mask(M::isless(L(1.0e-2f), p.PS[i] / (p.VDOC[i] + p.PS[i])), p.VDOC[i]);
+        }
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_ρChl_kernel! */
+            FT r = p.PS[i] / M::max(e30, p.Bm[i]) * q[PI_P_Chl2N] /
+                   M::max(e30, tpar[i] * q[PI_P_alpha] * q[PI_P_Phi] * p.Chl[i] / p.Bm[i]);
+            r = M::mask(M::isless(L(1.0e-1f), tpar[i]), r);
+            p.rho[i] = M::mask(p.ac[i], r);
+        }
+        for (int64_t i = 0; i < cap; ++i)     /* calc_respir_kernel! */
+            p.resp[i] = M::mask(p.ac[i], q[PI_P_respir] * p.Bm[i] * tempFunc(tT[i], q));
+        for (int64_t i = 0; i < cap; ++i) {   /* update_quotas_2_kernel! */
+            p.Cq[i] = p.Cq[i] + (p.VDOC[i] - p.resp[i]) * dt;
+            p.Bm[i] = p.Bm[i] - M::max(z0, (z0 - p.Cq[i]));
+            p.Nq[i] = p.Nq[i] + M::max(z0, (z0 - p.Cq[i])) * q[PI_P_R_NC];
+            p.Pq[i] = p.Pq[i] + M::max(z0, (z0 - p.Cq[i])) * q[PI_P_R_PC];
+            p.Cq[i] = p.Cq[i] + M::max(z0, (z0 - p.Cq[i]));
+        }
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_BS_kernel! */
+            FT mn = M::min(M::min(p.Cq[i], p.Nq[i] / q[PI_P_R_NC]), p.Pq[i] / q[PI_P_R_PC]);
+            p.BS[i] = M::mask(p.ac[i], mn * q[PI_P_k_mtb]);
+            FT mn2 = M::min(M::min(p.Cq[i], p.Nq[i] / q[PI_P_R_NC]), p.Pq[i] / q[PI_P_R_PC]);
+            p.exu[i] = M::mask(p.ac[i], M::max(z0, p.Cq[i] - mn2) * q[PI_P_k_mtb]);
+        }
+        for (int64_t i = 0; i < cap; ++i) {   /* update_biomass_kernel! */
+            p.Bm[i] = p.Bm[i] + dt * p.BS[i];
+            p.Cq[i] = p.Cq[i] - dt * (p.BS[i] + p.exu[i]);
+            p.Nq[i] = p.Nq[i] - dt * p.BS[i] * q[PI_P_R_NC];
+            p.Pq[i] = p.Pq[i] - dt * p.BS[i] * q[PI_P_R_PC];
+            p.Chl[i] = p.Chl[i] + dt * p.BS[i] * q[PI_P_R_NC] * p.rho[i];
+            p.age[i] = p.age[i] + M::mask(p.ac[i], dt / L(3600));
+        }
+        for (int64_t i = 0; i < cap; ++i)     /* update_cellsize_kernel! */
+            p.Sz[i] = p.Bm[i] / (q[PI_P_Cquota] * q[PI_P_Nsuper]);
+        /* probabilities (division_death_probability.jl:176-239) */
+        for (int64_t i = 0; i < cap; ++i)     /* calc_graz_quadratic_kernel! */
+            p.graz[i] = M::mask(p.ac[i], tpop[i] * q[PI_P_grz_P]);
+        for (int64_t i = 0; i < cap; ++i)     /* calc_mort_kernel! */
+            p.mort[i] = M::mask(p.ac[i], q[PI_P_mort_P] * shape_dec(p.Sz[i], q[PI_P_mort_reg], L(1.0e-5f), 4));
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_dvid_kernel! with calc_division (:178-183) */
+            FT P_ = q[PI_P_dvid_P], Sz = p.Sz[i], v;
+            FT hour = M::rem(t, L(86400)) / L(3600);
+            if (dvid_type[s] == 1) v = P_ * shape_inc(Sz, q[PI_P_dvid_reg], L(2.0e-3f), 2);
+            else if (dvid_type[s] == 2) v = P_ * shape_inc(hour, q[PI_P_dvid_reg], L(1.0e-5f), 4);
+            else v = P_ * shape_inc(Sz, q[PI_P_dvid_reg], L(1.0e-3f), 4) * shape_inc(hour, q[PI_P_dvid_reg2], L(1.0e-5f), 4);
+            p.dvid[i] = M::mask(p.ac[i], v);
+        }
+        /* get_probability! (division_death_probability.jl:266-278) */
+        for (int64_t i = 0; i < cap; ++i) {
+            float a, b, c;
+            pi_uniform3(seed, iteration, s, PI_KIND_DECISION, i, &a, &b, &c);
+            bool ac = p.ac[i];
+            p.graz[i] = (M::isless((FT)a, p.graz[i] * dt) && ac) ? L(1) : L(0);
+            p.mort[i] = (M::isless((FT)b, p.mort[i] * dt) && ac) ? L(1) : L(0);
+            p.dvid[i] = (M::isless((FT)c, p.dvid[i] * dt) && ac) ? L(1) : L(0);
+        }
+    }
+    /* calc_consume_kernel! (QuotaMode/consume_loss.jl:2-14) */
+    void calc_consume(const Plank& p, FT dt) {
+        for (int64_t i = 0; i < cap; ++i) {
+            int64_t c = cell_of(p, i);
+            bool a = p.ac[i];
+            plk[PI_DIC][c] += M::mask(a, (p.resp[i] - p.PS[i]) * dt);
+            plk[PI_DOC][c] += M::mask(a, (p.exu[i] - p.VDOC[i]) * dt);
+            plk[PI_NH4][c] += M::mask(a, -p.VNH4[i] * dt);
+            plk[PI_NO3][c] += M::mask(a, -p.VNO3[i] * dt);
+            plk[PI_PO4][c] += M::mask(a, -p.VPO4[i] * dt);
+        }
+    }
+
+    /* ================================================== tracers ========= */
+    /* Difference operators (Biogeochemistry/Advection/operaters.jl:1-27) */
+    inline FT dxp(const FT* c, int i, int j, int k) const { return M::mask(LM(i, j, k), M::mask(LM(i + 1, j, k), c[I(i + 1, j, k)] - c[I(i, j, k)])); }
+    inline FT dyp(const FT* c, int i, int j, int k) const { return M::mask(LM(i, j, k), M::mask(LM(i, j + 1, k), c[I(i, j + 1, k)] - c[I(i, j, k)])); }
+    inline FT dzp(const FT* c, int i, int j, int k) const { return M::mask(LM(i, j, k), M::mask(LM(i, j, k + 1), c[I(i, j, k)] - c[I(i, j, k + 1)])); }
+    inline FT dx0(const FT* c, int i, int j, int k) const { return M::mask(LM(i - 1, j, k), M::mask(LM(i, j, k), c[I(i, j, k)] - c[I(i - 1, j, k)])); }
+    inline FT dy0(const FT* c, int i, int j, int k) const { return M::mask(LM(i, j - 1, k), M::mask(LM(i, j, k), c[I(i, j, k)] - c[I(i, j - 1, k)])); }
+    inline FT dz0(const FT* c, int i, int j, int k) const { return M::mask(LM(i, j, k - 1), M::mask(LM(i, j, k), c[I(i, j, k - 1)] - c[I(i, j, k)])); }
+    inline FT dxm(const FT* c, int i, int j, int k) const { return M::mask(LM(i - 2, j, k), M::mask(LM(i - 1, j, k), c[I(i - 1, j, k)] - c[I(i - 2, j, k)])); }
+    inline FT dym(const FT* c, int i, int j, int k) const { return M::mask(LM(i, j - 2, k), M::mask(LM(i, j - 1, k), c[I(i, j - 1, k)] - c[I(i, j - 2, k)])); }
+    inline FT dzm(const FT* c, int i, int j, int k) const { return M::mask(LM(i, j, k - 2), M::mask(LM(i, j, k - 1), c[I(i, j, k - 2)] - c[I(i, j, k - 1)])); }
+
+    /* DST3FL building blocks (Biogeochemistry/Advection/DST3FL.jl:1-73) */
+    static FT d0(FT C) { return (L(2) - C) * (L(1) - C) / L(6); }
+    static FT d1(FT C) { return (L(1) - C * C) / L(6); }
+    static FT theta(FT dnum, FT dden) {
+        const FT tmax = L(1.0e20f);
+        return std::fabs(dden) * tmax <= std::fabs(dnum) ? std::copysign(tmax, dnum * dden) : dnum / dden;
+    }
+    static FT Psi(FT C, FT th) {
+        return M::max(L(0), M::min(M::min(L(1), d0(C) + d1(C) * th), th * (L(1) - C) / (C + L(1.0e-20f))));
+    }
+    FT flux_x(const FT* c, const FT* u, int i, int j, int k, FT dt) const {
+        FT C = std::fabs(u[I(i, j, k)] * dt / dx);
+        FT T = Ax(k) * u[I(i, j, k)];
+        FT d = dx0(c, i, j, k);
+        FT pp = Psi(C, theta(dxm(c, i, j, k), d));
+        FT pm = Psi(C, theta(dxp(c, i, j, k), d));
+        return L(0.5f) * (T + std::fabs(T)) * (c[I(i - 1, j, k)] + pp * d) +
+               L(0.5f) * (T - std::fabs(T)) * (c[I(i, j, k)] - pm * d);
+    }
+    FT flux_y(const FT* c, const FT* v, int i, int j, int k, FT dt) const {
+        FT C = std::fabs(v[I(i, j, k)] * dt / dy);
+        FT T = Ay(k) * v[I(i, j, k)];
+        FT d = dy0(c, i, j, k);
+        FT pp = Psi(C, theta(dym(c, i, j, k), d));
+        FT pm = Psi(C, theta(dyp(c, i, j, k), d));
+        return L(0.5f) * (T + std::fabs(T)) * (c[I(i, j - 1, k)] + pp * d) +
+               L(0.5f) * (T - std::fabs(T)) * (c[I(i, j, k)] - pm * d);
+    }
+    FT flux_z(const FT* c, const FT* w, int i, int j, int k, FT dt) const {
+        FT C = std::fabs(w[I(i, j, k)] * dt / dzC_at(k));
+        FT T = Az() * w[I(i, j, k)];
+        FT d = dz0(c, i, j, k);
+        FT pp = Psi(C, theta(dzm(c, i, j, k), d));
+        FT pm = Psi(C, theta(dzp(c, i, j, k), d));
+        return L(0.5f) * (T + std::fabs(T)) * (c[I(i, j, k)] + pm * d) +
+               L(0.5f) * (T - std::fabs(T)) * (c[I(i, j, k - 1)] - pp * d);
+    }
+    /* tracer_advection! (Biogeochemistry/Advection/multi_dim_adv.jl:111-135) */
+    void tracer_advection(FT dt) {
+        const V* vel = vel1;
+        for (int q = 0; q < PI_NTRACERS; ++q) ttemp[q] = tr[q];
+        for (int dim = 0; dim < 3; ++dim) {
+            const FT* w = vel[dim].data();
+            for (int q = 0; q < PI_NTRACERS; ++q)          /* calc_Gcs{x,y,z}! */
+                for (int k = Hz + 1; k <= Hz + Nz; ++k) for (int j = Hy + 1; j <= Hy + Ny; ++j)
+                    for (int i = Hx + 1; i <= Hx + Nx; ++i)
+                        Gcs[q][I(i, j, k)] = dim == 0 ? flux_x(ttemp[q].data(), w, i, j, k, dt)
+                                           : dim == 1 ? flux_y(ttemp[q].data(), w, i, j, k, dt)
+                                                      : flux_z(ttemp[q].data(), w, i, j, k, dt);
+            fill_halo_Gcs(Gcs);
+            for (int q = 0; q < PI_NTRACERS; ++q) {        /* multi_dim_{x,y,z}! */
+                const FT* G = Gcs[q].data();
+                const FT* c = tr[q].data();
+                for (int k = Hz + 1; k <= Hz + Nz; ++k) for (int j = Hy + 1; j <= Hy + Ny; ++j)
+                    for (int i = Hx + 1; i <= Hx + Nx; ++i) {
+                        FT dG, dT;
+                        if (dim == 0) {
+                            dG = dxp(G, i, j, k);
+                            dT = M::mask(LM(i, j, k), M::mask(LM(i + 1, j, k), Ax(k) * w[I(i + 1, j, k)] - Ax(k) * w[I(i, j, k)]));
+                        } else if (dim == 1) {
+                            dG = dyp(G, i, j, k);
+                            dT = M::mask(LM(i, j, k), M::mask(LM(i, j + 1, k), Ay(k) * w[I(i, j + 1, k)] - Ay(k) * w[I(i, j, k)]));
+                        } else {
+                            dG = dzp(G, i, j, k);
+                            dT = M::mask(LM(i, j, k), M::mask(LM(i, j, k + 1), Az() * w[I(i, j, k)] - Az() * w[I(i, j, k + 1)]));
+                        }
+                        FT& tt = ttemp[q][I(i, j, k)];
+                        tt = tt - dt / vol(k) * (dG - c[I(i, j, k)] * dT);
+                    }
+            }
+            fill_halo_tracer(ttemp);
+        }
+        for (int q = 0; q < PI_NTRACERS; ++q)             /* calc_tracer_tendency! */
+            for (int64_t n = 0; n < ncell; ++n) Gcs[q][n] = ttemp[q][n] - tr[q][n];
+    }
+    /* tracer_diffusion! (Biogeochemistry/Advection/tracer_diffusion.jl:1-30) */
+    void tracer_diffusion(FT dt) {
+        const FT kx = B[PI_B_kappa_h], ky = B[PI_B_kappa_h], kz = B[PI_B_kappa_v];
+        for (int q = 0; q < PI_NTRACERS; ++q) {
+            const FT* c = tr[q].data();
+            auto fx = [&](int i, int j, int k) { return kx * Ax(k) * (dx0(c, i, j, k) / dx); };
+            auto fy = [&](int i, int j, int k) { return ky * Ay(k) * (dy0(c, i, j, k) / dy); };
+            auto fz = [&](int i, int j, int k) { return kz * Az() * (dz0(c, i, j, k) / dzF_at(k)); };
+            for (int k = Hz + 1; k <= Hz + Nz; ++k) for (int j = Hy + 1; j <= Hy + Ny; ++j)
+                for (int i = Hx + 1; i <= Hx + Nx; ++i) {
+                    FT ddx = M::mask(LM(i, j, k), M::mask(LM(i + 1, j, k), fx(i + 1, j, k) - fx(i, j, k)));
+                    FT ddy = M::mask(LM(i, j, k), M::mask(LM(i, j + 1, k), fy(i, j + 1, k) - fy(i, j, k)));
+                    FT ddz = M::mask(LM(i, j, k), M::mask(LM(i, j, k + 1), fz(i, j, k) - fz(i, j, k + 1)));
+                    FT lap = L(1) / vol(k) * (ddx + ddy + ddz);
+                    FT& G = Gcs[q][I(i, j, k)];
+                    G = G + lap * dt;
+                }
+        }
+    }
+    /* tracer_sinking! (Biogeochemistry/Advection/tracer_sinking.jl:1-43) */
+    void tracer_sinking(FT dt) {
+        const int names[3] = {PI_PFe_inorg, PI_Dust, PI_PFe_bio};
+        for (int a = 0; a < 3; ++a) {
+            int q = names[a];
+            FT ws = a < 2 ? B[PI_B_w_sink_inorg] : B[PI_B_w_sink_org];
+            for (int64_t n = 0; n < ncell; ++n) flux_sink[n] = ws * tr[q][n];
+            planes_zero(flux_sink.data(), 2, 1, Hz);            /* fill_halo_top_flux_sink! */
+            planes_zero(flux_sink.data(), 2, Nz + Hz, Hz + 1);  /* fill_halo_bottom_flux_sink!: N+H:N+2H */
+            for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
+                int ii = i + Hx, jj = j + Hy, kk = k + Hz;
+                FT& G = Gcs[q][I(ii, jj, kk)];
+                G = G + (flux_sink[I(ii, jj, kk - 1)] - flux_sink[I(ii, jj, kk)]) / dzC_at(k) * dt;
+            }
+        }
+    }
+    /* tracer_forcing! (Biogeochemistry/tracer_forcings.jl:2-50), over padded arrays */
+    void tracer_forcing(FT dt) {
+        const FT z0 = L(0);
+        for (int q = 0; q < PI_NTRACERS; ++q)
+            for (int64_t n = 0; n < ncell; ++n) ttemp[q][n] = M::max(z0, tr[q][n]);
+        const FT kDOC = B[PI_B_kDOC], kPOC = B[PI_B_kPOC], kDON = B[PI_B_kDON], kPON = B[PI_B_kPON];
+        const FT kDOP = B[PI_B_kDOP], kPOP = B[PI_B_kPOP], Nit = B[PI_B_Nit];
+        const FT lPOC = B[PI_B_lambda_POC], lmin = B[PI_B_lambda_min], ldust = B[PI_B_lambda_dust];
+        const FT DFeFrac = B[PI_B_DFeFrac], kdiss = B[PI_B_kdiss], lig = B[PI_B_ligand], lFe = B[PI_B_lambda_Fe];
+        auto T = [&](int q, int64_t n) { return ttemp[q][n]; };
+        for (int64_t n = 0; n < ncell; ++n) {
+            FT* F[PI_NTRACERS];
+            for (int q = 0; q < PI_NTRACERS; ++q) F[q] = &Gcs[q][n];
+            *F[PI_DIC] = *F[PI_DIC] + T(PI_DOC, n) * kDOC * dt;
+            *F[PI_DOC] = *F[PI_DOC] - T(PI_DOC, n) * kDOC * dt + T(PI_POC, n) * kPOC * dt;
+            *F[PI_POC] = *F[PI_POC] - T(PI_POC, n) * kPOC * dt;
+            *F[PI_NH4] = *F[PI_NH4] + T(PI_DON, n) * kDON * dt - T(PI_NH4, n) * Nit * dt;
+            *F[PI_NO3] = *F[PI_NO3] + T(PI_NH4, n) * Nit * dt;
+            *F[PI_DON] = *F[PI_DON] - T(PI_DON, n) * kDON * dt + T(PI_PON, n) * kPON * dt;
+            *F[PI_PON] = *F[PI_PON] - T(PI_PON, n) * kPON * dt;
+            *F[PI_PO4] = *F[PI_PO4] + T(PI_DOP, n) * kDOP * dt;
+            *F[PI_DOP] = *F[PI_DOP] - T(PI_DOP, n) * kDOP * dt + T(PI_POP, n) * kPOP * dt;
+            *F[PI_POP] = *F[PI_POP] - T(PI_POP, n) * kPOP * dt;
+            FT DFe = T(PI_DFe, n);
+            FT excess = M::max((DFe - lig), z0);
+            *F[PI_DFe] = *F[PI_DFe] + T(PI_PFe_bio, n) * kPOC * dt -
+                         (lPOC * T(PI_POC, n) + lmin + ldust * T(PI_Dust, n)) * DFe * DFeFrac * dt +
+                         T(PI_PFe_inorg, n) * kdiss * dt -
+                         excess * DFe * DFeFrac * lFe * dt;
+            *F[PI_PFe_bio] = *F[PI_PFe_bio] + T(PI_POC, n) * DFe * DFeFrac * lPOC * dt -
+                             kPOC * T(PI_PFe_bio, n) * dt;
+            *F[PI_PFe_inorg] = *F[PI_PFe_inorg] - T(PI_PFe_inorg, n) * kdiss * dt +
+                               (lmin + ldust * T(PI_Dust, n)) * DFe * DFeFrac * dt +
+                               excess * DFe * DFeFrac * lFe * dt;
+        }
+    }
+    /* tracer_update! (Biogeochemistry/tracer_update.jl:22-43) */
+    void tracer_update(FT dt) {
+        tracer_advection(dt);
+        tracer_diffusion(dt);
+        tracer_sinking(dt);
+        for (int q = 0; q < PI_NTRACERS; ++q) std::fill(ttemp[q].begin(), ttemp[q].end(), (FT)0);
+        tracer_forcing(dt);
+        /* apply_bcs! is a no-op without flux boundary conditions (apply_bcs.jl:101-106) */
+        for (int q = 0; q < PI_NTRACERS; ++q)       /* apply_tendency_kernel! (:6-20) */
+            for (int k = Hz + 1; k <= Hz + Nz; ++k) for (int j = Hy + 1; j <= Hy + Ny; ++j)
+                for (int i = Hx + 1; i <= Hx + Nx; ++i) {
+                    int64_t n = I(i, j, k);
+                    tr[q][n] = tr[q][n] + (Gcs[q][n] + plk[q][n] / vol(k));
+                }
+        fill_halo_tracer(tr);
+    }
+
+    /* ================================================== TimeStep! =======
+     * Model/time_step.jl:12-180 */
+    pi_status step(float dt_in) {
+        if (failed) return fail(PI_ERR_CAPACITY, "model is in an error state from an earlier step");
+        const FT dt = (FT)dt_in;
+        iteration += 1;
+        t = (FT)(float)iteration * dt;
+        for (int q = 0; q < 3; ++q)
+            for (int64_t n = 0; n < ncell; ++n) velh[q][n] = (vel0[q][n] + vel1[q][n]) * L(0.5f);
+        for (int q = 0; q < PI_NTRACERS; ++q) std::fill(plk[q].begin(), plk[q].end(), (FT)0);
+        std::fill(Chl.begin(), Chl.end(), (FT)0);
+        if (B[PI_B_shared_graz] == L(1)) {
+            std::fill(pop.begin(), pop.end(), (FT)0);
+            for (int s = 0; s < nsp; ++s) {
+                particle_advection(sp[s], dt);
+                particle_diffusion(sp[s], s, dt);
+                find_inds(sp[s]);
+                acc_chl(sp[s]);
+                acc_counts(sp[s]);
+            }
+            for (int ki = 1; ki <= Nz; ++ki) calc_par(ki);
+            for (int s = 0; s < nsp; ++s) {
+                find_NPT(sp[s]);
+                pi_status st = plankton_update(s, dt);
+                if (st != PI_OK) return st;
+            }
+        } else {
+            for (int s = 0; s < nsp; ++s) {
+                particle_advection(sp[s], dt);
+                particle_diffusion(sp[s], s, dt);
+                find_inds(sp[s]);
+                acc_chl(sp[s]);
+            }
+            for (int ki = 1; ki <= Nz; ++ki) calc_par(ki);
+            for (int s = 0; s < nsp; ++s) {
+                std::fill(pop.begin(), pop.end(), (FT)0);
+                acc_counts(sp[s]);
+                find_NPT(sp[s]);
+                pi_status st = plankton_update(s, dt);
+                if (st != PI_OK) return st;
+            }
+        }
+        tracer_update(dt);
+        for (int64_t n = 0; n < ncell; ++n) diagPAR[n] += par[n];
+        for (size_t a = 0; a < diag_tr.size(); ++a)
+            for (int64_t n = 0; n < ncell; ++n) diagTr[a][n] += tr[diag_tr[a]][n];
+        for (int q = 0; q < 3; ++q) vel0[q] = vel1[q];
+        par0 = par;
+        return PI_OK;
+    }
+};
+
+/* ====================================================== C ABI wrappers == */
+template <typename FT> static Oracle<FT>* as(orc_model* m) { return static_cast<Oracle<FT>*>(m); }
+
+#define DISPATCH(m, expr)                                   \
+    do {                                                    \
+        if (!(m)) return fail(PI_ERR_ARG, "null model");    \
+        if ((m)->precision == 64) { auto* o = as<double>(m); expr; } \
+        else { auto* o = as<float>(m); expr; }              \
+    } while (0)
+
+const char* orc_last_error(void) { return g_err.c_str(); }
+
+pi_status orc_model_create(const pi_model_desc* d, int32_t precision, orc_model** out) {
+    if (!d || !out) return fail(PI_ERR_ARG, "null argument");
+    if (precision == 64) {
+        auto* o = new Oracle<double>();
+        o->precision = 64;
+        pi_status s = o->init(d);
+        if (s != PI_OK) { delete o; return s; }
+        *out = o;
+    } else {
+        auto* o = new Oracle<float>();
+        o->precision = 32;
+        pi_status s = o->init(d);
+        if (s != PI_OK) { delete o; return s; }
+        *out = o;
+    }
+    return PI_OK;
+}
+
+pi_status orc_model_destroy(orc_model* m) { delete m; return PI_OK; }
+
+template <typename FT>
+static pi_status upload_impl(Oracle<FT>* o, int32_t s, const pi_individuals_host* h) {
+    if (s < 0 || s >= o->nsp) return fail(PI_ERR_ARG, "species out of range");
+    auto& p = o->sp[s];
+    auto cp = [&](std::vector<FT>& dst, const float* src) {
+        for (int64_t i = 0; i < o->cap; ++i) dst[i] = src ? (FT)src[i] : (FT)0;
+    };
+    cp(p.x, h->x); cp(p.y, h->y); cp(p.z, h->z); cp(p.Sz, h->Sz); cp(p.Bm, h->Bm); cp(p.Cq, h->Cq);
+    cp(p.Nq, h->Nq); cp(p.Pq, h->Pq); cp(p.Chl, h->Chl); cp(p.gen, h->gen); cp(p.age, h->age);
+    cp(p.graz, h->graz); cp(p.mort, h->mort); cp(p.dvid, h->dvid);
+    for (int64_t i = 0; i < o->cap; ++i) {
+        p.xi[i] = h->xi ? h->xi[i] : 0; p.yi[i] = h->yi ? h->yi[i] : 0; p.zi[i] = h->zi ? h->zi[i] : 0;
+        p.ac[i] = h->ac ? (h->ac[i] != 0) : 0;
+    }
+    return PI_OK;
+}
+pi_status orc_upload_individuals(orc_model* m, int32_t sp, const pi_individuals_host* h) {
+    if (!h) return fail(PI_ERR_ARG, "null host view");
+    DISPATCH(m, return upload_impl(o, sp, h));
+}
+pi_status orc_upload_tracer(orc_model* m, int32_t q, const float* c) {
+    if (q < 0 || q >= PI_NTRACERS || !c) return fail(PI_ERR_ARG, "tracer id");
+    DISPATCH(m, for (int64_t n = 0; n < o->ncell; ++n) o->tr[q][n] = c[n]; return PI_OK);
+}
+pi_status orc_set_velocity(orc_model* m, const float* u, const float* v, const float* w) {
+    if (!u || !v || !w) return fail(PI_ERR_ARG, "null velocity");
+    DISPATCH(m, o->set_velocity(u, v, w); return PI_OK);
+}
+pi_status orc_set_parf(orc_model* m, const float* parf) {
+    if (!parf) return fail(PI_ERR_ARG, "null PARF");
+    DISPATCH(m, for (int64_t n = 0; n < (int64_t)o->Nx * o->Ny; ++n) o->PARF[n] = parf[n]; return PI_OK);
+}
+pi_status orc_set_temperature(orc_model* m, const float* tt) {
+    if (!tt) return fail(PI_ERR_ARG, "null temperature");
+    DISPATCH(m, o->set_temperature(tt); return PI_OK);
+}
+pi_status orc_step(orc_model* m, float dt) { DISPATCH(m, return o->step(dt)); }
+
+template <typename FT, typename OUT>
+static pi_status field_impl(Oracle<FT>* o, int32_t s, int32_t f, OUT* out, int64_t n, bool ints_ok) {
+    if (s < 0 || s >= o->nsp) return fail(PI_ERR_ARG, "species out of range");
+    if (n != o->cap) return fail(PI_ERR_ARG, "n must equal capacity");
+    auto& p = o->sp[s];
+    const std::vector<FT>* src = nullptr;
+    switch (f) {
+        case PI_F_x: src = &p.x; break; case PI_F_y: src = &p.y; break; case PI_F_z: src = &p.z; break;
+        case PI_F_Sz: src = &p.Sz; break; case PI_F_Bm: src = &p.Bm; break; case PI_F_Cq: src = &p.Cq; break;
+        case PI_F_Nq: src = &p.Nq; break; case PI_F_Pq: src = &p.Pq; break; case PI_F_Chl: src = &p.Chl; break;
+        case PI_F_gen: src = &p.gen; break; case PI_F_age: src = &p.age; break;
+        case PI_F_graz: src = &p.graz; break; case PI_F_mort: src = &p.mort; break; case PI_F_dvid: src = &p.dvid; break;
+        default: break;
+    }
+    if (src) { for (int64_t i = 0; i < n; ++i) out[i] = (OUT)(*src)[i]; return PI_OK; }
+    if (!ints_ok) return fail(PI_ERR_ARG, "field is not floating point");
+    if (f == PI_F_xi || f == PI_F_yi || f == PI_F_zi) {
+        const auto& v = f == PI_F_xi ? p.xi : f == PI_F_yi ? p.yi : p.zi;
+        int64_t* o64 = reinterpret_cast<int64_t*>(out);
+        for (int64_t i = 0; i < n; ++i) o64[i] = v[i];
+        return PI_OK;
+    }
+    if (f == PI_F_ac) {
+        uint8_t* o8 = reinterpret_cast<uint8_t*>(out);
+        for (int64_t i = 0; i < n; ++i) o8[i] = p.ac[i];
+        return PI_OK;
+    }
+    return fail(PI_ERR_ARG, "unknown field");
+}
+pi_status orc_download_field(orc_model* m, int32_t sp, int32_t f, void* host, int64_t n) {
+    DISPATCH(m, return field_impl(o, sp, f, reinterpret_cast<float*>(host), n, true));
+}
+pi_status orc_download_field_f64(orc_model* m, int32_t sp, int32_t f, double* host, int64_t n) {
+    DISPATCH(m, return field_impl(o, sp, f, host, n, false));
+}
+pi_status orc_download_tracer(orc_model* m, int32_t q, float* c) {
+    if (q < 0 || q >= PI_NTRACERS) return fail(PI_ERR_ARG, "tracer id");
+    DISPATCH(m, for (int64_t n = 0; n < o->ncell; ++n) c[n] = (float)o->tr[q][n]; return PI_OK);
+}
+pi_status orc_download_tracer_f64(orc_model* m, int32_t q, double* c) {
+    if (q < 0 || q >= PI_NTRACERS) return fail(PI_ERR_ARG, "tracer id");
+    DISPATCH(m, for (int64_t n = 0; n < o->ncell; ++n) c[n] = (double)o->tr[q][n]; return PI_OK);
+}
+template <typename FT>
+static pi_status grid_impl(Oracle<FT>* o, int32_t which, float* h, int64_t n) {
+    const std::vector<FT>* src = nullptr;
+    switch (which) {
+        case PI_G_par: src = &o->par; break; case PI_G_par0: src = &o->par0; break;
+        case PI_G_Chl: src = &o->Chl; break; case PI_G_pop: src = &o->pop; break;
+        case PI_G_temp: src = &o->temp; break; case PI_G_PARF: src = &o->PARF; break;
+        case PI_G_u: src = &o->vel1[0]; break; case PI_G_v: src = &o->vel1[1]; break; case PI_G_w: src = &o->vel1[2]; break;
+        case PI_G_diag_PAR: src = &o->diagPAR; break;
+        default:
+            if (which >= PI_G_plk_DIC && which < PI_G_plk_DIC + PI_NTRACERS) src = &o->plk[which - PI_G_plk_DIC];
+    }
+    if (!src) return fail(PI_ERR_ARG, "unknown grid field");
+    if (n != (int64_t)src->size()) return fail(PI_ERR_ARG, "size mismatch");
+    for (int64_t i = 0; i < n; ++i) h[i] = (float)(*src)[i];
+    return PI_OK;
+}
+pi_status orc_download_grid(orc_model* m, int32_t which, float* h, int64_t n) {
+    DISPATCH(m, return grid_impl(o, which, h, n));
+}
+pi_status orc_download_diag(orc_model* m, int32_t s, int32_t which, float* h) {
+    DISPATCH(m, if (s < 0 || s >= o->nsp || which < 0 || which >= PI_NDIAG) return fail(PI_ERR_ARG, "diag id");
+             for (int64_t n = 0; n < o->ncell; ++n) h[n] = (float)o->diagSp[s][which][n]; return PI_OK);
+}
+pi_status orc_download_diag_tracer(orc_model* m, int32_t q, float* h) {
+    DISPATCH(m, for (size_t a = 0; a < o->diag_tr.size(); ++a) if (o->diag_tr[a] == q) {
+                    for (int64_t n = 0; n < o->ncell; ++n) h[n] = (float)o->diagTr[a][n];
+                    return PI_OK; }
+             return fail(PI_ERR_ARG, "tracer not diagnosed"));
+}
+pi_status orc_reset_diags(orc_model* m) {
+    DISPATCH(m, std::fill(o->diagPAR.begin(), o->diagPAR.end(), 0);
+             for (auto& v : o->diagTr) std::fill(v.begin(), v.end(), 0);
+             for (auto& a : o->diagSp) for (auto& v : a) std::fill(v.begin(), v.end(), 0);
+             return PI_OK);
+}
+pi_status orc_set_diag_tracers(orc_model* m, int32_t n, const int32_t* ids) {
+    for (int32_t a = 0; a < n; ++a)
+        if (ids[a] < 0 || ids[a] >= PI_NTRACERS) return fail(PI_ERR_ARG, "tracer id");
+    DISPATCH(m, o->diag_tr.assign(ids, ids + n);
+             o->diagTr.assign(o->diag_tr.size(), std::vector<decltype(o->t)>(o->ncell, 0)); return PI_OK);
+}
+pi_status orc_get_iteration(orc_model* m, int64_t* it) { DISPATCH(m, *it = o->iteration; return PI_OK); }
+pi_status orc_count_active(orc_model* m, int32_t s, int64_t* n) {
+    DISPATCH(m, if (s < 0 || s >= o->nsp) return fail(PI_ERR_ARG, "species");
+             int64_t c = 0; for (auto a : o->sp[s].ac) c += a; *n = c; return PI_OK);
+}
+
+pi_status orc_fill_halo(int32_t Nx, int32_t Ny, int32_t Nz, int32_t Hx, int32_t Hy, int32_t Hz,
+                        int32_t tx, int32_t ty, int32_t tz, float* c) {
+    Oracle<float> o;
+    o.Nx = Nx; o.Ny = Ny; o.Nz = Nz; o.Hx = Hx; o.Hy = Hy; o.Hz = Hz;
+    o.Px = Nx + 2 * Hx; o.Py = Ny + 2 * Hy; o.Pz = Nz + 2 * Hz;
+    o.tx = tx; o.ty = ty; o.tz = tz;
+    o.fill_halo_generic(c, tx, ty);
+    return PI_OK;
+}
+float orc_expf(float x) { return pi_expf(x); }
+double orc_exp_f64(double x) { return pi_exp_f64(x); }
+void orc_uniform3(uint64_t seed, int64_t it, int32_t s, int32_t kind, int64_t slot, float* o3) {
+    pi_uniform3(seed, it, s, kind, slot, &o3[0], &o3[1], &o3[2]);
+}
+void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { pi_philox4x32_10(ctr, key, out); }
+float orc_tempfunc(const float* params, float T) {
+    std::array<float, PI_NPARAMS> q; for (int i = 0; i < PI_NPARAMS; ++i) q[i] = params[i];
+    return Oracle<float>::tempFunc(T, q);
+}
+float orc_tempfunc_ps(const float* params, float T) {
+    std::array<float, PI_NPARAMS> q; for (int i = 0; i < PI_NPARAMS; ++i) q[i] = params[i];
+    return Oracle<float>::tempFunc_PS(T, q);
+}
+

@@ -1,0 +1,126 @@
+"""ctypes mirror of include/pi_hip.h (descriptor structs, ids, prototypes).
+
+The same prototypes describe any library exposing the ABI under a prefix, so
+one table serves libpi_hip.so ("pi_").
+"""
+import ctypes as C
+import numpy as np
+
+# ---- status codes (pi_status) ------------------------------------------------
+PI_OK, PI_ERR_ARG, PI_ERR_CAPACITY, PI_ERR_HIP, PI_ERR_UNSUPPORTED, PI_ERR_NOMEM = range(6)
+PERIODIC, BOUNDED = 0, 1
+
+TRACER_NAMES = ("DIC", "NH4", "NO3", "PO4", "DFe", "DOC", "DON", "DOP",
+                "PFe_inorg", "POC", "PON", "POP", "PFe_bio", "Dust")
+TRACER_ID = {n: i for i, n in enumerate(TRACER_NAMES)}
+
+# QuotaMode parameter order (QuotaMode/plankton_generation.jl:16-20)
+PARAM_NAMES = ("Nsuper", "Cquota", "mean", "var", "Chl2Cint", "α", "Φ", "Topt", "Tmax", "Ea",
+               "PCmax", "VDOCmax", "VNO3max", "VNH4max", "VPO4max",
+               "KsatDOC", "KsatNH4", "KsatNO3", "KsatPO4", "Cqmax", "Nqmax", "Pqmax",
+               "Chl2N", "R_NC", "R_PC", "k_mtb", "respir", "grz_P", "dvid_type", "dvid_P",
+               "dvid_reg", "dvid_reg2", "mort_P", "mort_reg", "grazFracC", "grazFracN",
+               "grazFracP", "mortFracC", "mortFracN", "mortFracP")
+BGC_NAMES = ("kw", "kc", "kDOC", "Nit", "kDON", "kDOP", "kPOC", "kPON", "kPOP", "κh", "κv",
+             "κhP", "κvP", "shared_graz", "w_sink_org", "w_sink_inorg", "lambda_POC",
+             "lambda_min", "kdiss", "lambda_dust", "lambda_Fe", "ligand", "DFeFrac")
+
+FLOAT_FIELDS = ("x", "y", "z", "Sz", "Bm", "Cq", "Nq", "Pq", "Chl", "gen", "age",
+                "graz", "mort", "dvid")
+INT_FIELDS = ("xi", "yi", "zi")
+FIELD_ID = {n: i for i, n in enumerate(FLOAT_FIELDS + INT_FIELDS + ("ac",))}
+
+GRID_ID = {"par": 0, "par₀": 1, "par0": 1, "Chl": 2, "pop": 3, "temp": 4, "PARF": 5,
+           "u": 6, "v": 7, "w": 8, "diag_PAR": 9}
+PLK_BASE = 10
+DIAG_ID = {"num": 0, "graz": 1, "mort": 2, "dvid": 3}
+
+fp = C.POINTER(C.c_float)
+i64p = C.POINTER(C.c_int64)
+u8p = C.POINTER(C.c_uint8)
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("Nx", C.c_int32), ("Ny", C.c_int32), ("Nz", C.c_int32),
+                ("Hx", C.c_int32), ("Hy", C.c_int32), ("Hz", C.c_int32),
+                ("topo_x", C.c_int32), ("topo_y", C.c_int32), ("topo_z", C.c_int32),
+                ("dx", C.c_float), ("dy", C.c_float),
+                ("dzC", fp), ("dzF", fp), ("landmask", u8p)]
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [("grid", GridDesc), ("n_species", C.c_int32), ("capacity", C.c_int64),
+                ("params", fp), ("bgc", fp), ("rng_seed", C.c_uint64),
+                ("n_diag_tracers", C.c_int32), ("diag_tracers", C.POINTER(C.c_int32))]
+
+
+class IndividualsHost(C.Structure):
+    _fields_ = [(n, fp) for n in FLOAT_FIELDS] + [(n, i64p) for n in INT_FIELDS] + [("ac", u8p)]
+
+
+def ptr(a, ctype):
+    """Pointer into a contiguous numpy array (caller keeps `a` alive)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"] or a.flags["F_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def make_desc(grid, n_species, capacity, params, bgc, seed, diag_tracers=()):
+    """Build a ModelDesc; returns (desc, keepalive) — keep the second alive while desc is used."""
+    dzC = np.ascontiguousarray(grid.dzC, dtype=np.float32)
+    dzF = np.ascontiguousarray(grid.dzF, dtype=np.float32)
+    lm = None if grid.no_land else np.asfortranarray(grid.landmask, dtype=np.uint8)
+    params = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+    bgc = np.ascontiguousarray(bgc, dtype=np.float32)
+    dt = np.ascontiguousarray(np.array(diag_tracers, dtype=np.int32))
+    g = GridDesc(grid.Nx, grid.Ny, grid.Nz, grid.Hx, grid.Hy, grid.Hz,
+                 grid.topology_ids[0], grid.topology_ids[1], grid.topology_ids[2],
+                 np.float32(grid.Δx), np.float32(grid.Δy),
+                 ptr(dzC, C.c_float), ptr(dzF, C.c_float),
+                 None if lm is None else lm.ctypes.data_as(u8p))
+    d = ModelDesc(g, n_species, capacity, ptr(params, C.c_float), ptr(bgc, C.c_float),
+                  C.c_uint64(seed), len(dt), ptr(dt, C.c_int32) if len(dt) else None)
+    return d, (dzC, dzF, lm, params, bgc, dt)
+
+
+def individuals_view(arrays):
+    """IndividualsHost over a dict of full-capacity numpy arrays (keep `arrays` alive)."""
+    h = IndividualsHost()
+    for n in FLOAT_FIELDS:
+        a = arrays.get(n)
+        setattr(h, n, ptr(a, C.c_float) if a is not None else None)
+    for n in INT_FIELDS:
+        a = arrays.get(n)
+        setattr(h, n, ptr(a, C.c_int64) if a is not None else None)
+    a = arrays.get("ac")
+    h.ac = ptr(a, C.c_uint8) if a is not None else None
+    return h
+
+
+def declare(lib, prefix, model_t=C.c_void_p):
+    """Attach argtypes/restype for the ABI functions exported under `prefix`."""
+    M = model_t
+    proto = {
+        "last_error": (C.c_char_p, []),
+        "model_destroy": (C.c_int, [M]),
+        "upload_individuals": (C.c_int, [M, C.c_int32, C.POINTER(IndividualsHost)]),
+        "upload_tracer": (C.c_int, [M, C.c_int32, fp]),
+        "set_velocity": (C.c_int, [M, fp, fp, fp]),
+        "set_parf": (C.c_int, [M, fp]),
+        "set_temperature": (C.c_int, [M, fp]),
+        "step": (C.c_int, [M, C.c_float]),
+        "download_field": (C.c_int, [M, C.c_int32, C.c_int32, C.c_void_p, C.c_int64]),
+        "download_tracer": (C.c_int, [M, C.c_int32, fp]),
+        "download_grid": (C.c_int, [M, C.c_int32, fp, C.c_int64]),
+        "download_diag": (C.c_int, [M, C.c_int32, C.c_int32, fp]),
+        "download_diag_tracer": (C.c_int, [M, C.c_int32, fp]),
+        "reset_diags": (C.c_int, [M]),
+        "get_iteration": (C.c_int, [M, i64p]),
+        "count_active": (C.c_int, [M, C.c_int32, i64p]),
+    }
+    for name, (res, args) in proto.items():
+        f = getattr(lib, prefix + name)
+        f.restype = res
+        f.argtypes = args
+    return lib

@@ -1,0 +1,222 @@
+"""Loader of libpi_hip.so and a thin handle class over its C ABI.
+
+The product has exactly one compute path: the HIP library.  If the library is
+missing, every entry point raises — there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import PI_OK, PI_ERR_CAPACITY, PI_ERR_ARG, PI_ERR_UNSUPPORTED
+
+LIB_NAME = "libpi_hip.so"
+_lib = None
+
+
+class CapacityError(ValueError):
+    """ArgumentError("number of individual exceeds the capacity …") — QuotaMode/plankton_update.jl:20-22."""
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def lib_path():
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+
+def load_library():
+    """Load libpi_hip.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = lib_path()
+    if not os.path.exists(p):
+        raise NativeError(f"{LIB_NAME} is not built ({p}); run __graft_entry__.build() — "
+                          "there is no CPU fallback for the hot path")
+    lib = C.CDLL(p)
+    _abi.declare(lib, "pi_")
+    lib.pi_init.restype = C.c_int
+    lib.pi_init.argtypes = [C.c_int32]
+    lib.pi_version.restype = C.c_char_p
+    lib.pi_model_create.restype = C.c_int
+    lib.pi_model_create.argtypes = [C.POINTER(_abi.ModelDesc), C.POINTER(C.c_void_p)]
+    lib.pi_run.restype = C.c_int
+    lib.pi_run.argtypes = [C.c_void_p, C.c_float, C.c_int64]
+    lib.pi_sync.restype = C.c_int
+    lib.pi_sync.argtypes = [C.c_void_p]
+    lib.pi_set_diag_tracers.restype = C.c_int
+    lib.pi_set_diag_tracers.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
+    lib.pi_enable_timing.restype = C.c_int
+    lib.pi_enable_timing.argtypes = [C.c_void_p, C.c_int32]
+    lib.pi_kernel_times.restype = C.c_int
+    lib.pi_kernel_times.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_char_p),
+                                    C.POINTER(C.c_float), C.c_int32]
+    _lib = lib
+    return lib
+
+
+def check(lib, prefix, status):
+    if status == PI_OK:
+        return
+    msg = getattr(lib, prefix + "last_error")().decode(errors="replace")
+    if status == PI_ERR_CAPACITY:
+        raise CapacityError(msg)
+    if status in (PI_ERR_ARG, PI_ERR_UNSUPPORTED):
+        raise ValueError(msg)
+    raise NativeError(f"status {status}: {msg}")
+
+
+class Handle:
+    """One model instance behind an ABI with prefix `prefix` (shared by the product and
+    by test-side wrappers of other implementations of the same ABI)."""
+
+    def __init__(self, lib, prefix, raw, grid, nsp, cap, diag_tracers=()):
+        self.lib, self.prefix, self.raw = lib, prefix, C.c_void_p(raw)
+        self.grid, self.nsp, self.cap = grid, nsp, cap
+        self.diag_tracers = tuple(diag_tracers)
+
+    def _f(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def _call(self, name, *args):
+        check(self.lib, self.prefix, self._f(name)(self.raw, *args))
+
+    def close(self):
+        if self.raw is not None and self.raw.value:
+            self._f("model_destroy")(self.raw)
+            self.raw = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- uploads
+    def upload_individuals(self, sp, arrays):
+        keep = {}
+        for n in _abi.FLOAT_FIELDS:
+            if arrays.get(n) is not None:
+                keep[n] = np.ascontiguousarray(arrays[n], dtype=np.float32)
+                assert keep[n].shape == (self.cap,)
+        for n in _abi.INT_FIELDS:
+            if arrays.get(n) is not None:
+                keep[n] = np.ascontiguousarray(arrays[n], dtype=np.int64)
+        if arrays.get("ac") is not None:
+            keep["ac"] = np.ascontiguousarray(arrays["ac"], dtype=np.uint8)
+        h = _abi.individuals_view(keep)
+        self._call("upload_individuals", sp, C.byref(h))
+
+    def upload_tracer(self, tid, padded):
+        a = np.asfortranarray(padded, dtype=np.float32)
+        assert a.shape == self.grid.padded_shape
+        self._call("upload_tracer", tid, a.ctypes.data_as(_abi.fp))
+
+    def set_velocity(self, u, v, w):
+        u, v, w = (np.asfortranarray(a, dtype=np.float32) for a in (u, v, w))
+        self._call("set_velocity", u.ctypes.data_as(_abi.fp), v.ctypes.data_as(_abi.fp),
+                   w.ctypes.data_as(_abi.fp))
+
+    def set_parf(self, parf):
+        a = np.asfortranarray(parf, dtype=np.float32)
+        assert a.shape == (self.grid.Nx, self.grid.Ny)
+        self._call("set_parf", a.ctypes.data_as(_abi.fp))
+
+    def set_temperature(self, temp):
+        a = np.asfortranarray(temp, dtype=np.float32)
+        assert a.shape == (self.grid.Nx, self.grid.Ny, self.grid.Nz)
+        self._call("set_temperature", a.ctypes.data_as(_abi.fp))
+
+    def step(self, dt):
+        self._call("step", C.c_float(dt))
+
+    # ---- downloads
+    def field(self, sp, name):
+        fid = _abi.FIELD_ID[name]
+        if name in _abi.INT_FIELDS:
+            out = np.empty(self.cap, np.int64)
+        elif name == "ac":
+            out = np.empty(self.cap, np.uint8)
+        else:
+            out = np.empty(self.cap, np.float32)
+        self._call("download_field", sp, fid, out.ctypes.data_as(C.c_void_p), self.cap)
+        return out.astype(bool) if name == "ac" else out
+
+    def tracer(self, tid):
+        out = np.empty(self.grid.padded_shape, np.float32, order="F")
+        self._call("download_tracer", tid, out.ctypes.data_as(_abi.fp))
+        return out
+
+    def grid_field(self, name):
+        if name == "PARF":
+            out = np.empty((self.grid.Nx, self.grid.Ny), np.float32, order="F")
+            wid = _abi.GRID_ID[name]
+        elif name.startswith("plk_"):
+            out = np.empty(self.grid.padded_shape, np.float32, order="F")
+            wid = _abi.PLK_BASE + _abi.TRACER_ID[name[4:]]
+        else:
+            out = np.empty(self.grid.padded_shape, np.float32, order="F")
+            wid = _abi.GRID_ID[name]
+        self._call("download_grid", wid, out.ctypes.data_as(_abi.fp), out.size)
+        return out
+
+    def diag(self, sp, name):
+        out = np.empty(self.grid.padded_shape, np.float32, order="F")
+        self._call("download_diag", sp, _abi.DIAG_ID[name], out.ctypes.data_as(_abi.fp))
+        return out
+
+    def diag_tracer(self, tid):
+        out = np.empty(self.grid.padded_shape, np.float32, order="F")
+        self._call("download_diag_tracer", tid, out.ctypes.data_as(_abi.fp))
+        return out
+
+    def reset_diags(self):
+        self._call("reset_diags")
+
+    def iteration(self):
+        v = C.c_int64(0)
+        self._call("get_iteration", C.byref(v))
+        return v.value
+
+    def count_active(self, sp):
+        v = C.c_int64(0)
+        self._call("count_active", sp, C.byref(v))
+        return v.value
+
+
+class NativeModel(Handle):
+    """A model living on the GPU (libpi_hip.so)."""
+
+    def __init__(self, grid, nsp, cap, params, bgc, seed, diag_tracers=(), device=0):
+        lib = load_library()
+        check(lib, "pi_", lib.pi_init(device))
+        desc, keep = _abi.make_desc(grid, nsp, cap, params, bgc, seed, diag_tracers)
+        raw = C.c_void_p()
+        check(lib, "pi_", lib.pi_model_create(C.byref(desc), C.byref(raw)))
+        del keep
+        super().__init__(lib, "pi_", raw.value, grid, nsp, cap, diag_tracers)
+
+    def run(self, dt, n):
+        self._call("run", C.c_float(dt), C.c_int64(n))
+
+    def sync(self):
+        self._call("sync")
+
+    def set_diag_tracers(self, ids):
+        a = np.ascontiguousarray(np.array(ids, dtype=np.int32))
+        self._call("set_diag_tracers", len(a), a.ctypes.data_as(C.POINTER(C.c_int32)) if len(a) else None)
+        self.diag_tracers = tuple(ids)
+
+    def enable_timing(self, on=True):
+        self._call("enable_timing", 1 if on else 0)
+
+    def kernel_times(self):
+        cap = 64
+        n = C.c_int32(0)
+        names = (C.c_char_p * cap)()
+        ms = (C.c_float * cap)()
+        self._call("kernel_times", C.byref(n), names, ms, cap)
+        return [(names[i].decode(), ms[i]) for i in range(n.value)]

@@ -1,0 +1,117 @@
+/*
+ * k_args.h - kernel argument blocks and the host-side launchers of each kernel
+ * file (launchers live next to their kernels).
+ */
+#pragma once
+#include "pi_device.h"
+
+struct AdvArgs {
+    DevGrid g;
+    DevState* st;
+    Vel v0, vh, v1;
+    float dt, sx, sy, sz;
+    uint64_t seed;
+    uint32_t* cnt;
+};
+
+struct PhysArgs {
+    DevGrid g;
+    DevState* st;
+    SlotTable tab;            /* per species rows at tab + sp*cap */
+    long long cap;
+    int nsp;
+    const float* params;      /* nsp * PI_NPARAMS */
+    const uint32_t* seg;      /* seg[ncell] = number of live individuals */
+    const uint32_t* pkey;
+    const uint32_t* psrc;
+    const float *temp, *pop;
+    const uint32_t* bmA;
+    const uint32_t* bmD;
+    const uint32_t* G;        /* exclusive scan over [D words | free words] of all species */
+    int nw;
+    Contrib con;
+    float* diag_num;          /* diags: species stride 4*ncell, field num */
+    float dt;
+    uint64_t seed;
+};
+
+struct PlkArgs {
+    DevGrid g;
+    const uint32_t* seg;
+    const uint32_t* pkey;
+    Contrib con;
+    float* plk[PI_NTRACERS];
+    float* diag;              /* [sp][4][ncell] */
+    DevState* st;
+};
+
+struct SweepArgs {
+    DevGrid g;
+    DevState* st;
+    const float* vel;                 /* u (x sweep) or v (y sweep) of vel₁ */
+    float* out[PI_NTRACERS];          /* temp after this sweep */
+    const float* prev[PI_NTRACERS];   /* temp before this sweep; null = current tracers */
+    float dt;
+    int dim;                          /* 0 = x, 1 = y */
+};
+
+struct FillArgs {
+    DevGrid g;
+    DevState* st;
+    float* f[PI_NTRACERS];     /* null = the tracers written by this step */
+    int dim, lines_full, nf;
+};
+
+struct FinalArgs {
+    DevGrid g;
+    DevState* st;
+    const float* w;                   /* vel₁ w */
+    const float* prev[PI_NTRACERS];   /* temp after the y sweep (z halos filled) */
+    const float* plk[PI_NTRACERS];    /* null for tracers without particle exchange */
+    float dt;
+    float kh, kv;                     /* κh, κv (tracer_diffusion!) */
+    float wsink_inorg, wsink_org;
+    float kDOC, kPOC, kDON, kPON, kDOP, kPOP, Nit;
+    float lPOC, lmin, ldust, DFeFrac, kdiss, lig, lFe;
+};
+
+struct ScanIn {
+    const uint32_t* a;
+    uint32_t n;
+    int bits;
+    const uint32_t* D;
+    const uint32_t* A;
+    int nsp, nw;
+    long long cap;
+};
+
+/* k_particles.hip */
+void launch_adv(hipStream_t st, int blocks, const AdvArgs& a);
+void launch_marks(hipStream_t s, int blocks, DevState* st, uint32_t* bmA, uint32_t* bmD, int nw);
+void launch_place(hipStream_t s, int blocks, DevState* st, const uint32_t* seg, uint32_t* cnt, uint32_t* pkey, uint32_t* psrc);
+void launch_cells(hipStream_t s, int blocks, DevState* st, long long ncell, const uint32_t* seg, uint32_t* pkey,
+                  uint32_t* psrc, float* Chl, float* pop, uint32_t* big);
+void launch_bigcells(hipStream_t s, int blocks, DevState* st, const uint32_t* big, const uint32_t* seg,
+                     uint32_t* pkey, uint32_t* psrc, uint32_t* skey, uint32_t* ssrc, float* Chl);
+void launch_par(hipStream_t s, int blocks, DevState* st, const DevGrid& g, const float* Chl, const float* PARF_in,
+                float* PARF_out, float* diag_par, float kc, float kw);
+void launch_phys(hipStream_t st, int blocks, const PhysArgs& a);
+void launch_plk(hipStream_t st, int blocks, const PlkArgs& a);
+/* k_tracers.hip */
+void launch_sweep(hipStream_t s, int blocks, const SweepArgs& a);
+void launch_fill(hipStream_t s, int blocks, const FillArgs& a);
+void launch_final(hipStream_t s, int blocks, const FinalArgs& a);
+/* k_util.hip */
+void launch_scan(hipStream_t s, const ScanIn& in, uint32_t* partials, uint32_t* out);
+ScanIn scan_plain(const uint32_t* a, uint32_t n);
+ScanIn scan_bits(const uint32_t* D, const uint32_t* A, int nsp, int nw, long long cap);
+void launch_capcheck(hipStream_t s, DevState* st, const uint32_t* G, int nsp, int nw);
+void launch_finish(hipStream_t s, DevState* st, const uint32_t* seg, long long ncell, int nsp);
+void launch_zero(hipStream_t s, uint32_t* p, long long n, const DevState* st);
+void launch_velhalf(hipStream_t s, const float* a, const float* b, float* h, long long n, const DevState* st);
+void launch_accum(hipStream_t s, float* d, const DevState* st, int q, long long n);
+void launch_store_to_table(hipStream_t s, DevState* st, const SlotTable& tab, long long cap, long long maxn);
+void launch_rebuild(hipStream_t s, DevState* st, const SlotTable& tab, long long cap, int nsp);
+void launch_table_field(hipStream_t s, const DevState* st, const SlotTable& tab, int sp, long long cap, int field,
+                        const float* params, const DevGrid& g, void* out);
+void launch_count(hipStream_t s, const DevState* st, int sp, unsigned long long* out, long long maxn);

@@ -1,0 +1,638 @@
+/*
+ * k_particles.hip - particle kernels of the hot path on gfx950.
+ *
+ *   k_adv     RK4 advection + random-walk diffusion + cell index, one lane per
+ *             live individual, store order (cell-sorted from the previous step).
+ *             Also counts individuals per cell and marks survivors/dividers in
+ *             per-species slot bitmaps.
+ *   k_place   counting-sort scatter of (key, store index) into cell segments.
+ *   k_cells   per cell: order its segment by (species, slot), pop = count,
+ *             Chl = ordered sum (the reference's atomic order under julia -t 1).
+ *   k_bigcells  same for segments longer than SMALL_SEG (block bitonic sort).
+ *   k_par     Beer-Lambert column scan (PAR), one lane per column.
+ *   k_phys    per individual in cell order: losses, division (slot assignment
+ *             from the bitmaps), QuotaMode growth, decisions, consumption.
+ *   k_plk     per cell: ordered sums of losses/consumption into plk + counts
+ *             for the diagnostics.
+ *
+ * Every floating-point expression keeps the reference's operation order
+ * (cited per function); the build uses -ffp-contract=off and no fast-math.
+ */
+#include "pi_device.h"
+#include "k_args.h"
+
+#define SMALL_SEG 16
+
+/* ---------------------------------------------------------------- advection
+ * u_itpl / v_itpl / w_itpl with ac = true (Individuals/Advection/interpolation.jl:1-83).
+ * Corner indices are clamped into the padded array so that positions outside
+ * the halo (not produced by the reference's boundary conditions) cannot fault. */
+__device__ __forceinline__ float itpl(const DevGrid& g, const float* __restrict__ f,
+                                      float x, float y, float z, int comp)
+{
+    float xs = comp == 0 ? x + 1.0f : x + 0.5f;
+    float ys = comp == 1 ? y + 1.0f : y + 0.5f;
+    float zs = comp == 2 ? z + 1.0f : z + 0.5f;
+    float xd = jl_mod1f(xs), yd = jl_mod1f(ys), zd = jl_mod1f(zs);
+    int xi = (int)xs + g.Hx, yi = (int)ys + g.Hy, zi = (int)zs + g.Hz;
+    xi = min(max(xi, 1), g.Px - 1);
+    yi = min(max(yi, 1), g.Py - 1);
+    zi = min(max(zi, 1), g.Pz - 1);
+    const long long sx = 1, sy = g.Px, sz = (long long)g.Px * g.Py;
+    const long long b = gidx(g, xi, yi, zi);
+    float D0, D1;
+    if (comp == 0) { D0 = g.dx; D1 = g.dx; }
+    else if (comp == 1) { D0 = g.dy; D1 = g.dy; }
+    else { D0 = dzC_at(g, zi); D1 = dzC_at(g, zi + 1); }
+    const float ox = 1.0f - xd, oy = 1.0f - yd, oz = 1.0f - zd;
+    float s = ox * oy * oz * f[b] / D0;
+    s = s + ox * oy * zd * f[b + sz] / D1;
+    s = s + ox * yd * oz * f[b + sy] / D0;
+    s = s + ox * yd * zd * f[b + sy + sz] / D1;
+    s = s + xd * oy * oz * f[b + sx] / D0;
+    s = s + xd * oy * zd * f[b + sx + sz] / D1;
+    s = s + xd * yd * oz * f[b + sx + sy] / D0;
+    s = s + xd * yd * zd * f[b + sx + sy + sz] / D1;
+    return s;
+}
+
+/* particle_boundary_condition (particle_advection_kernels.jl:2-11) */
+__device__ __forceinline__ float pbc(float x, int N, int topo)
+{
+    const float xl = 0.0f, xr = (float)N;
+    if (topo == PI_PERIODIC) {
+        if (x > xr) return xl + (x - xr);
+        if (x < xl) return xr - (xl - x);
+        return x;
+    }
+    if (x > xr) return xr - (x - xr) * 1.0f;
+    if (x < xl) return xl + (xl - x) * 1.0f;
+    return x;
+}
+
+
+__global__ void __launch_bounds__(256) k_adv(AdvArgs a)
+{
+    const DevState* st = a.st;
+    if (st->err) return;
+    const DevGrid& g = a.g;
+    const Store S = st->store[st->scur];
+    const uint32_t n = st->nstore;
+    const long long iteration = st->iteration + 1;
+    const float dt = a.dt;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        uint8_t fl = S.flags[e];
+        if (!(fl & F_AC)) continue;
+        const uint32_t key = S.key[e];
+        const int sp = (int)(key >> PI_SLOT_BITS);
+        const uint32_t slot = key & PI_SLOT_MASK;
+        const float x0 = S.x[e], y0 = S.y[e], z0 = S.z[e];
+        /* ---- RK4, particle_advection.jl:2-56 */
+        float u1 = itpl(g, a.v0.u, x0, y0, z0, 0);
+        float v1 = itpl(g, a.v0.v, x0, y0, z0, 1);
+        float w1 = itpl(g, a.v0.w, x0, y0, z0, 2);
+        float u2 = u1, v2 = v1, w2 = w1;
+        float px = pbc(x0 + 0.5f * u1 * dt, g.Nx, g.tx);
+        float py = pbc(y0 + 0.5f * v1 * dt, g.Ny, g.ty);
+        float pz = pbc(z0 - 0.5f * w1 * dt, g.Nz, g.tz);
+        u1 = itpl(g, a.vh.u, px, py, pz, 0);
+        v1 = itpl(g, a.vh.v, px, py, pz, 1);
+        w1 = itpl(g, a.vh.w, px, py, pz, 2);
+        u2 = u2 + u1 * 2.0f; v2 = v2 + v1 * 2.0f; w2 = w2 + w1 * 2.0f;
+        px = pbc(x0 + 0.5f * u1 * dt, g.Nx, g.tx);
+        py = pbc(y0 + 0.5f * v1 * dt, g.Ny, g.ty);
+        pz = pbc(z0 - 0.5f * w1 * dt, g.Nz, g.tz);
+        u1 = itpl(g, a.vh.u, px, py, pz, 0);
+        v1 = itpl(g, a.vh.v, px, py, pz, 1);
+        w1 = itpl(g, a.vh.w, px, py, pz, 2);
+        u2 = u2 + u1 * 2.0f; v2 = v2 + v1 * 2.0f; w2 = w2 + w1 * 2.0f;
+        px = pbc(x0 + 1.0f * u1 * dt, g.Nx, g.tx);
+        py = pbc(y0 + 1.0f * v1 * dt, g.Ny, g.ty);
+        pz = pbc(z0 - 1.0f * w1 * dt, g.Nz, g.tz);
+        u1 = itpl(g, a.v1.u, px, py, pz, 0);
+        v1 = itpl(g, a.v1.v, px, py, pz, 1);
+        w1 = itpl(g, a.v1.w, px, py, pz, 2);
+        u2 = u2 + u1; v2 = v2 + v1; w2 = w2 + w1;
+        u2 = u2 / 6.0f; v2 = v2 / 6.0f; w2 = w2 / 6.0f;
+        float x = pbc(x0 + 1.0f * u2 * dt, g.Nx, g.tx);
+        float y = pbc(y0 + 1.0f * v2 * dt, g.Ny, g.ty);
+        float z = pbc(z0 - 1.0f * w2 * dt, g.Nz, g.tz);
+        /* ---- random walk, particle_diffusion.jl:2-33 (stale zi from the previous find_inds) */
+        float r0, r1, r2;
+        pi_uniform3(a.seed, iteration, sp, PI_KIND_DIFFUSION, slot, &r0, &r1, &r2);
+        r0 = (r0 - 0.5f) * 2.0f; r1 = (r1 - 0.5f) * 2.0f; r2 = (r2 - 0.5f) * 2.0f;
+        const uint32_t cprev = S.cell[e];
+        const long long zprev = (fl & F_ZERO) ? 0 : (long long)(cprev / (uint32_t)(g.Px * g.Py)) + 1;
+        r0 = r0 * a.sx / g.dx;
+        r1 = r1 * a.sy / g.dy;
+        r2 = r2 * a.sz / dzC_at(g, zprev + g.Hz);
+        x = pbc(x + r0, g.Nx, g.tx);
+        y = pbc(y + r1, g.Ny, g.ty);
+        z = pbc(z + r2, g.Nz, g.tz);
+        /* ---- find_inds (Individuals/utils.jl:2-12) */
+        int xi = (int)(x + 1.0f) + g.Hx, yi = (int)(y + 1.0f) + g.Hy, zi = (int)(z + 1.0f) + g.Hz;
+        xi = min(max(xi, 1), g.Px); yi = min(max(yi, 1), g.Py); zi = min(max(zi, 1), g.Pz);
+        const uint32_t c = (uint32_t)gidx(g, xi, yi, zi);
+        S.x[e] = x; S.y[e] = y; S.z[e] = z;
+        S.cell[e] = c;
+        S.flags[e] = fl & (uint8_t)~F_ZERO;
+        atomicAdd(&a.cnt[c], 1u);
+    }
+}
+
+/* Survivors of this step's losses and the dividers, as per-species slot bitmaps
+ * (plankton_update.jl:5-24: ac after grazing!/mortality!, dvid .*= ac).  They
+ * depend only on the decisions taken at the end of the previous step, so the
+ * capacity check runs before anything is modified. */
+__global__ void __launch_bounds__(256) k_marks(DevState* st, uint32_t* bmA, uint32_t* bmD, int nw)
+{
+    if (st->err) return;
+    const Store S = st->store[st->scur];
+    const uint32_t n = st->nstore;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const uint8_t fl = S.flags[e];
+        if (!(fl & F_AC) || (fl & (F_GRAZ | F_MORT))) continue;
+        const uint32_t key = S.key[e];
+        const uint32_t sp = key >> PI_SLOT_BITS, slot = key & PI_SLOT_MASK;
+        const uint32_t w = sp * (uint32_t)nw + (slot >> 5), bit = 1u << (slot & 31u);
+        atomicOr(&bmA[w], bit);
+        if (fl & F_DVID) atomicOr(&bmD[w], bit);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_place(DevState* st, const uint32_t* seg,
+                                               uint32_t* cnt, uint32_t* pkey, uint32_t* psrc)
+{
+    if (st->err) return;
+    const Store s = st->store[st->scur];
+    const uint32_t n = st->nstore;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        if (!(s.flags[e] & F_AC)) continue;
+        const uint32_t c = s.cell[e];
+        const uint32_t pos = seg[c] + atomicSub(&cnt[c], 1u) - 1u;
+        pkey[pos] = s.key[e];
+        psrc[pos] = e;
+    }
+}
+
+/* acc_chl! / acc_counts! (Individuals/utils.jl:33-52) as an ordered segmented sum. */
+__global__ void __launch_bounds__(256) k_cells(DevState* st, long long ncell, const uint32_t* seg, uint32_t* pkey,
+                                               uint32_t* psrc, float* Chl, float* pop, uint32_t* big)
+{
+    if (st->err) return;
+    const float* __restrict__ chl_store = st->store[st->scur].Chl;
+    uint32_t* nbig = &st->nbig;
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < ncell;
+         c += (long long)gridDim.x * blockDim.x) {
+        const uint32_t s0 = seg[c], n = seg[c + 1] - s0;
+        pop[c] = (float)n;
+        if (n == 0) { Chl[c] = 0.0f; continue; }
+        if (n > SMALL_SEG) { big[atomicAdd(nbig, 1u)] = (uint32_t)c; continue; }
+        /* insertion sort of the segment by key (species-major, then slot) */
+        for (uint32_t a = 1; a < n; ++a) {
+            uint32_t k = pkey[s0 + a], v = psrc[s0 + a];
+            uint32_t b = a;
+            while (b > 0 && pkey[s0 + b - 1] > k) {
+                pkey[s0 + b] = pkey[s0 + b - 1];
+                psrc[s0 + b] = psrc[s0 + b - 1];
+                --b;
+            }
+            pkey[s0 + b] = k;
+            psrc[s0 + b] = v;
+        }
+        float sum = 0.0f;
+        for (uint32_t a = 0; a < n; ++a) sum = sum + chl_store[psrc[s0 + a]];
+        Chl[c] = sum;
+    }
+}
+
+/* Long segments: one block per cell, bitonic sort in a padded scratch region
+ * [2*s0, 2*s0 + pow2(n)) that no other segment overlaps. */
+__global__ void __launch_bounds__(256) k_bigcells(DevState* st, const uint32_t* big, const uint32_t* seg,
+                                                  uint32_t* pkey, uint32_t* psrc, uint32_t* skey, uint32_t* ssrc,
+                                                  float* Chl)
+{
+    if (st->err) return;
+    const float* __restrict__ chl_store = st->store[st->scur].Chl;
+    const uint32_t nb = st->nbig;
+    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const uint32_t c = big[b];
+        const uint32_t s0 = seg[c], n = seg[c + 1] - s0;
+        uint32_t P2 = 1;
+        while (P2 < n) P2 <<= 1;
+        uint32_t* K = skey + 2ull * s0;
+        uint32_t* S = ssrc + 2ull * s0;
+        for (uint32_t i = threadIdx.x; i < P2; i += blockDim.x) {
+            K[i] = i < n ? pkey[s0 + i] : 0xFFFFFFFFu;
+            S[i] = i < n ? psrc[s0 + i] : 0u;
+        }
+        __syncthreads();
+        for (uint32_t k = 2; k <= P2; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = threadIdx.x; i < P2; i += blockDim.x) {
+                    const uint32_t l = i ^ j;
+                    if (l > i) {
+                        const bool up = (i & k) == 0;
+                        const uint32_t ki = K[i], kl = K[l];
+                        if ((ki > kl) == up) {
+                            K[i] = kl; K[l] = ki;
+                            const uint32_t t = S[i]; S[i] = S[l]; S[l] = t;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            pkey[s0 + i] = K[i];
+            psrc[s0 + i] = S[i];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float sum = 0.0f;
+            for (uint32_t a = 0; a < n; ++a) sum = sum + chl_store[S[a]];
+            Chl[c] = sum;
+        }
+        __syncthreads();
+    }
+}
+
+/* calc_par_kernel! looped over levels (Individuals/utils.jl:55-68, time_step.jl:63-67) */
+__global__ void __launch_bounds__(256) k_par(DevState* st, DevGrid g, const float* Chl, const float* PARF_in,
+                                             float* PARF_out, float* diag_par, float kc, float kw)
+{
+    if (st->err) return;
+    float* par = st->par[1 - st->pcur];
+    const int ncol = g.Nx * g.Ny;
+    for (int col = blockIdx.x * blockDim.x + threadIdx.x; col < ncol; col += gridDim.x * blockDim.x) {
+        const int i = col % g.Nx + 1, j = col / g.Nx + 1;
+        const int ii = i + g.Hx, jj = j + g.Hy;
+        float P = PARF_in[col];
+        for (int ki = 1; ki <= g.Nz; ++ki) {
+            const int kk = ki + g.Hz;
+            const long long c = gidx(g, ii, jj, kk);
+            const float vol = g.dx * g.dy * dzF_at(g, kk);
+            const float atten = (Chl[c] / vol * kc + kw) * dzF_at(g, kk);
+            const float ex = pi_expf(-atten);
+            const float p = P * (1.0f - ex) / atten;
+            par[c] = p;
+            diag_par[c] = diag_par[c] + p;
+            P = P * pi_expf(-atten);
+        }
+        PARF_out[col] = P;
+    }
+}
+
+/* ------------------------------------------------------------- physiology */
+struct PState { float x, y, z, Sz, Bm, Cq, Nq, Pq, Chl, gen, age; };
+struct Trs { float NH4, NO3, PO4, DOC, DFe, par, dpar, T, pop; };
+struct Rates { float PS, VDOC, VNH4, VNO3, VPO4, resp, BS, exu, rho; };
+
+/* shape_func_dec / shape_func_inc (Individuals/utils.jl:109-121) */
+__device__ __forceinline__ float shape_dec(float x, float xmax, float k, int pw)
+{
+    float fx = jl_maxf(0.0f, jl_minf(1.0f, 1.0f - x / xmax));
+    float fp = pw == 2 ? pi_pow2f(fx) : pi_pow4f(fx);
+    return fp / (k + fp);
+}
+__device__ __forceinline__ float shape_inc(float x, float xmax, float k, int pw)
+{
+    return 1.0f - shape_dec(x, xmax, k, pw);
+}
+/* tempFunc_PS / tempFunc (QuotaMode/growth_kernels.jl:1-17) */
+__device__ __forceinline__ float tempFunc_PS(float T, const float* q)
+{
+    float x = T - q[PI_P_Topt], xmax = q[PI_P_Tmax] - q[PI_P_Topt];
+    float regT = shape_dec(x, xmax, 4.0e-2f, 4);
+    float k = pi_expf(-q[PI_P_Ea] / (8.3145f * (T + 273.15f))) * regT;
+    k = jl_maxf(0.0f, k);
+    float OGT = pi_expf(-q[PI_P_Ea] / (8.3145f * (q[PI_P_Topt] + 273.15f)));
+    return jl_minf(1.0f, k / OGT);
+}
+__device__ __forceinline__ float tempFunc(float T, const float* q)
+{
+    float k = pi_expf(-q[PI_P_Ea] / (8.3145f * (T + 273.15f)));
+    k = jl_maxf(0.0f, k);
+    float OGT = pi_expf(-q[PI_P_Ea] / (8.3145f * (q[PI_P_Topt] + 273.15f)));
+    return jl_minf(1.0f, k / OGT);
+}
+
+/* plankton_growth! for one individual (QuotaMode/plankton_growth.jl:2-24,
+ * growth_kernels.jl:48-171, division_death_probability.jl:176-278). */
+__device__ void growth(PState& s, const Trs& t, const float* q, bool ac, float dt, float tmodel,
+                       int dvid_type, float U0, float U1, float U2, Rates& r,
+                       bool& dgraz, bool& dmort, bool& ddvid)
+{
+    const float e30 = 1.0e-30f;
+    {   /* calc_inorganic_uptake_kernel! */
+        float aI = t.par * q[PI_P_alpha] * q[PI_P_Phi];
+        float PCm = q[PI_P_PCmax] * tempFunc_PS(t.T, q);
+        float PS = PCm * (1.0f - pi_expf(-aI / jl_maxf(e30, PCm) * s.Chl / jl_maxf(e30, s.Bm))) * s.Bm;
+        r.PS = jl_maskf(ac, PS);
+        float Qn = (s.Nq + s.Bm * q[PI_P_R_NC]) / jl_maxf(e30, s.Bm + s.Cq);
+        float Qp = (s.Pq + s.Bm * q[PI_P_R_PC]) / jl_maxf(e30, s.Bm + s.Cq);
+        float regQN = shape_dec(Qn, q[PI_P_Nqmax], 1.0e-4f, 4);
+        float regQP = shape_dec(Qp, q[PI_P_Pqmax], 1.0e-4f, 4);
+        float tf = tempFunc(t.T, q);
+        float VNH4 = jl_maskf(ac, q[PI_P_VNH4max] * regQN * t.NH4 / jl_maxf(e30, t.NH4 + q[PI_P_KsatNH4]) * tf * s.Bm);
+        float VNO3 = jl_maskf(ac, q[PI_P_VNO3max] * regQN * t.NO3 / jl_maxf(e30, t.NO3 + q[PI_P_KsatNO3]) * tf * s.Bm);
+        float VPO4 = jl_maskf(ac, q[PI_P_VPO4max] * regQP * t.PO4 / jl_maxf(e30, t.PO4 + q[PI_P_KsatPO4]) * tf * s.Bm);
+        float pp = jl_maxf(1.0f, t.pop);
+        r.VNH4 = jl_minf(VNH4, t.NH4 / dt / pp);
+        r.VNO3 = jl_minf(VNO3, t.NO3 / dt / pp);
+        r.VPO4 = jl_minf(VPO4, t.PO4 / dt / pp);
+    }
+    /* update_quotas_1_kernel! */
+    s.Cq = s.Cq + r.PS * dt;
+    s.Pq = s.Pq + r.VPO4 * dt;
+    s.Nq = s.Nq + (r.VNH4 + r.VNO3) * dt;
+    {   /* calc_organic_uptake_kernel! */
+        float Qc = s.Cq / jl_maxf(e30, s.Bm + s.Cq);
+        float regQ = shape_dec(Qc, q[PI_P_Cqmax], 1.0e-4f, 4);
+        float VN = q[PI_P_VDOCmax] * regQ * t.DOC / jl_maxf(e30, t.DOC + q[PI_P_KsatDOC]) * tempFunc(t.T, q) * s.Bm;
+        float VD = jl_minf(VN, t.DOC / dt / jl_maxf(1.0f, t.pop));
+        r.VDOC = jl_maskf(ac, VD);
+        r.VDOC = jl_maskf(jl_islessf(1.0e-2f, r.PS / (r.VDOC + r.PS)), r.VDOC);
+    }
+    {   /* calc_ρChl_kernel! */
+        float rr = r.PS / jl_maxf(e30, s.Bm) * q[PI_P_Chl2N] /
+                   jl_maxf(e30, t.par * q[PI_P_alpha] * q[PI_P_Phi] * s.Chl / s.Bm);
+        rr = jl_maskf(jl_islessf(1.0e-1f, t.par), rr);
+        r.rho = jl_maskf(ac, rr);
+    }
+    /* calc_respir_kernel! */
+    r.resp = jl_maskf(ac, q[PI_P_respir] * s.Bm * tempFunc(t.T, q));
+    /* update_quotas_2_kernel! */
+    s.Cq = s.Cq + (r.VDOC - r.resp) * dt;
+    s.Bm = s.Bm - jl_maxf(0.0f, (0.0f - s.Cq));
+    s.Nq = s.Nq + jl_maxf(0.0f, (0.0f - s.Cq)) * q[PI_P_R_NC];
+    s.Pq = s.Pq + jl_maxf(0.0f, (0.0f - s.Cq)) * q[PI_P_R_PC];
+    s.Cq = s.Cq + jl_maxf(0.0f, (0.0f - s.Cq));
+    {   /* calc_BS_kernel! */
+        float mn = jl_minf(jl_minf(s.Cq, s.Nq / q[PI_P_R_NC]), s.Pq / q[PI_P_R_PC]);
+        r.BS = jl_maskf(ac, mn * q[PI_P_k_mtb]);
+        r.exu = jl_maskf(ac, jl_maxf(0.0f, s.Cq - mn) * q[PI_P_k_mtb]);
+    }
+    /* update_biomass_kernel! */
+    s.Bm = s.Bm + dt * r.BS;
+    s.Cq = s.Cq - dt * (r.BS + r.exu);
+    s.Nq = s.Nq - dt * r.BS * q[PI_P_R_NC];
+    s.Pq = s.Pq - dt * r.BS * q[PI_P_R_PC];
+    s.Chl = s.Chl + dt * r.BS * q[PI_P_R_NC] * r.rho;
+    s.age = s.age + jl_maskf(ac, dt / 3600.0f);
+    /* update_cellsize_kernel! */
+    s.Sz = s.Bm / (q[PI_P_Cquota] * q[PI_P_Nsuper]);
+    /* probabilities + get_probability! */
+    float graz = jl_maskf(ac, t.pop * q[PI_P_grz_P]);
+    float mort = jl_maskf(ac, q[PI_P_mort_P] * shape_dec(s.Sz, q[PI_P_mort_reg], 1.0e-5f, 4));
+    float dv;
+    const float hour = fmodf(tmodel, 86400.0f) / 3600.0f;
+    if (dvid_type == 1) dv = q[PI_P_dvid_P] * shape_inc(s.Sz, q[PI_P_dvid_reg], 2.0e-3f, 2);
+    else if (dvid_type == 2) dv = q[PI_P_dvid_P] * shape_inc(hour, q[PI_P_dvid_reg], 1.0e-5f, 4);
+    else dv = q[PI_P_dvid_P] * shape_inc(s.Sz, q[PI_P_dvid_reg], 1.0e-3f, 4) * shape_inc(hour, q[PI_P_dvid_reg2], 1.0e-5f, 4);
+    float dvid = jl_maskf(ac, dv);
+    dgraz = jl_islessf(U0, graz * dt) && ac;
+    dmort = jl_islessf(U1, mort * dt) && ac;
+    ddvid = jl_islessf(U2, dvid * dt) && ac;
+}
+
+/* k-th set bit (0-based) of a word */
+__device__ __forceinline__ int select_bit(uint32_t w, uint32_t k)
+{
+    for (int b = 0; b < 32; ++b) {
+        if (w & (1u << b)) {
+            if (k == 0) return b;
+            --k;
+        }
+    }
+    return -1;
+}
+
+
+__device__ __forceinline__ uint32_t prefG(const PhysArgs& a, int which, int sp, uint32_t w)
+{
+    const uint32_t base = (uint32_t)which * (uint32_t)a.nsp * (uint32_t)a.nw + (uint32_t)sp * (uint32_t)a.nw;
+    return a.G[base + w] - a.G[base];
+}
+__device__ __forceinline__ uint32_t valid_mask(const PhysArgs& a, uint32_t w)
+{
+    const long long lo = (long long)w * 32;
+    const long long rem = a.cap - lo;
+    return rem >= 32 ? 0xFFFFFFFFu : ((1u << rem) - 1u);
+}
+
+__global__ void __launch_bounds__(256) k_phys(PhysArgs a)
+{
+    const DevState* st = a.st;
+    if (st->err) return;
+    const DevGrid& g = a.g;
+    const uint32_t n = a.seg[g.ncell];
+    const float dt = a.dt;
+    const Store IN = st->store[st->scur], OUT = st->store[1 - st->scur];
+    const long long iteration = st->iteration + 1;
+    const float tmodel = (float)iteration * dt;
+    const float* const* TR = st->tr[st->tcur];
+    const float* PAR = st->par[1 - st->pcur];
+    const float* PAR0 = st->par[st->pcur];
+    const uint32_t hhh = (uint32_t)gidx(g, g.Hx, g.Hy, g.Hz);
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const uint32_t key = a.pkey[e], src = a.psrc[e];
+        const int sp = (int)(key >> PI_SLOT_BITS);
+        const uint32_t slot = key & PI_SLOT_MASK;
+        const float* q = a.params + sp * PI_NPARAMS;
+        const int dvid_type = (int)q[PI_P_dvid_type];
+        const uint8_t fl = IN.flags[src];
+        const uint32_t c = IN.cell[src];
+        PState s;
+        s.x = IN.x[src]; s.y = IN.y[src]; s.z = IN.z[src];
+        s.Sz = IN.Sz[src]; s.Bm = IN.Bm[src]; s.Cq = IN.Cq[src]; s.Nq = IN.Nq[src];
+        s.Pq = IN.Pq[src]; s.Chl = IN.Chl[src]; s.gen = IN.gen[src]; s.age = IN.age[src];
+        /* find_NPT_kernel! (Individuals/utils.jl:14-30), ac = true at gather time */
+        Trs t;
+        t.NH4 = jl_maxf(0.0f, TR[PI_NH4][c]);
+        t.NO3 = jl_maxf(0.0f, TR[PI_NO3][c]);
+        t.PO4 = jl_maxf(0.0f, TR[PI_PO4][c]);
+        t.DOC = jl_maxf(0.0f, TR[PI_DOC][c]);
+        t.DFe = jl_maxf(0.0f, TR[PI_DFe][c]);
+        t.par = PAR[c];
+        t.dpar = PAR[c] - PAR0[c];
+        t.T = a.temp[c];
+        t.pop = a.pop[c];
+        /* grazing! then mortality! (QuotaMode/division_death.jl:2-25, consume_loss.jl:17-33) */
+        bool ac = true;
+        uint8_t tag = 0;
+        float L0 = 0.f, L1 = 0.f, L2 = 0.f, L3 = 0.f, L4 = 0.f, L5 = 0.f;
+        if (fl & (F_GRAZ | F_MORT)) {
+            const bool gz = (fl & F_GRAZ) != 0;
+            const float fC = gz ? q[PI_P_grazFracC] : q[PI_P_mortFracC];
+            const float fN = gz ? q[PI_P_grazFracN] : q[PI_P_mortFracN];
+            const float fP = gz ? q[PI_P_grazFracP] : q[PI_P_mortFracP];
+            L0 = (s.Bm + s.Cq) * fC * 1.0f;
+            L1 = (s.Bm * q[PI_P_R_NC] + s.Nq) * fN * 1.0f;
+            L2 = (s.Bm * q[PI_P_R_PC] + s.Pq) * fP * 1.0f;
+            L3 = (s.Bm + s.Cq) * (1.0f - fC) * 1.0f;
+            L4 = (s.Bm * q[PI_P_R_NC] + s.Nq) * (1.0f - fN) * 1.0f;
+            L5 = (s.Bm * q[PI_P_R_PC] + s.Pq) * (1.0f - fP) * 1.0f;
+            tag = gz ? T_GRAZ : T_MORT;
+            ac = false;
+        }
+        const bool divider = ac && (fl & F_DVID);
+        float U0, U1, U2;
+        if (divider) {
+            /* divide! (QuotaMode/division_death.jl:28-77): the k-th divider in slot order
+             * takes the k-th inactive slot in slot order. */
+            const uint32_t w = slot >> 5;
+            const uint32_t below = (1u << (slot & 31u)) - 1u;
+            const uint32_t r = prefG(a, 0, sp, w) + __popc(a.bmD[(uint32_t)sp * a.nw + w] & below);
+            uint32_t lo = 0, hi = (uint32_t)a.nw;           /* largest w' with prefF(w') <= r */
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (prefG(a, 1, sp, mid) <= r) lo = mid; else hi = mid;
+            }
+            const uint32_t fw = ~a.bmA[(uint32_t)sp * a.nw + lo] & valid_mask(a, lo);
+            const int bit = select_bit(fw, r - prefG(a, 1, sp, lo));
+            if (bit < 0) {   /* cannot happen after the capacity check; never write out of bounds */
+                atomicCAS((uint32_t*)&st->err, 0u, (uint32_t)PI_ERR_HIP);
+                continue;
+            }
+            const uint32_t dslot = lo * 32u + (uint32_t)bit;
+            /* daughter = copy before halving; divide_to_half on both */
+            PState d = s;
+            const float h = (2.0f - 1.0f) / 2.0f;
+            s.Sz = s.Sz * h; s.Bm = s.Bm * h; s.Cq = s.Cq * h; s.Nq = s.Nq * h; s.Pq = s.Pq * h; s.Chl = s.Chl * h;
+            s.gen = s.gen + 1.0f; s.age = s.age * (1.0f - 1.0f);
+            d.Sz = d.Sz * h; d.Bm = d.Bm * h; d.Cq = d.Cq * h; d.Nq = d.Nq * h; d.Pq = d.Pq * h; d.Chl = d.Chl * h;
+            d.gen = d.gen + 1.0f;
+            d.age = a.tab.age[(long long)sp * a.cap + dslot] * (1.0f - 1.0f);
+            /* the daughter's first growth sees zero tracers (gathered while inactive) */
+            Trs tz;
+            tz.NH4 = tz.NO3 = tz.PO4 = tz.DOC = tz.DFe = tz.par = tz.dpar = tz.T = tz.pop = 0.0f;
+            pi_uniform3(a.seed, iteration, sp, PI_KIND_DECISION, dslot, &U0, &U1, &U2);
+            Rates rd;
+            bool gz, mt, dv;
+            growth(d, tz, q, true, dt, tmodel, dvid_type, U0, U1, U2, rd, gz, mt, dv);
+            const uint32_t o = n + st->doff[sp] + r;
+            OUT.key[o] = ((uint32_t)sp << PI_SLOT_BITS) | dslot;
+            OUT.x[o] = d.x; OUT.y[o] = d.y; OUT.z[o] = d.z;
+            OUT.Sz[o] = d.Sz; OUT.Bm[o] = d.Bm; OUT.Cq[o] = d.Cq; OUT.Nq[o] = d.Nq;
+            OUT.Pq[o] = d.Pq; OUT.Chl[o] = d.Chl; OUT.gen[o] = d.gen; OUT.age[o] = d.age;
+            OUT.flags[o] = (uint8_t)(F_AC | (gz ? F_GRAZ : 0) | (mt ? F_MORT : 0) | (dv ? F_DVID : 0));
+            OUT.cell[o] = hhh;      /* its find_inds ran while it was inactive: (Hx,Hy,Hz) */
+            atomicAdd(&a.diag_num[(long long)sp * 4 * g.ncell + hhh], 1.0f);
+        }
+        pi_uniform3(a.seed, iteration, sp, PI_KIND_DECISION, slot, &U0, &U1, &U2);
+        Rates r;
+        bool gz, mt, dv;
+        growth(s, t, q, ac, dt, tmodel, dvid_type, U0, U1, U2, r, gz, mt, dv);
+        if (ac) {
+            tag = T_ALIVE | (divider ? T_DIV : 0);
+            L0 = (r.resp - r.PS) * dt;
+            L1 = (r.exu - r.VDOC) * dt;
+            L2 = -r.VNH4 * dt;
+            L3 = -r.VNO3 * dt;
+            L4 = -r.VPO4 * dt;
+            L5 = 0.0f;
+        } else {
+            /* inactive from now on: record its last state in the slot table */
+            const long long ts = (long long)sp * a.cap + slot;
+            a.tab.x[ts] = s.x; a.tab.y[ts] = s.y; a.tab.z[ts] = s.z;
+            a.tab.Sz[ts] = s.Sz; a.tab.Bm[ts] = s.Bm; a.tab.Cq[ts] = s.Cq; a.tab.Nq[ts] = s.Nq;
+            a.tab.Pq[ts] = s.Pq; a.tab.Chl[ts] = s.Chl; a.tab.gen[ts] = s.gen; a.tab.age[ts] = s.age;
+            a.tab.flags[ts] = 0;
+            a.tab.cell[ts] = c;
+            a.tab.death[ts] = (int32_t)iteration;
+        }
+        OUT.key[e] = key;
+        OUT.x[e] = s.x; OUT.y[e] = s.y; OUT.z[e] = s.z;
+        OUT.Sz[e] = s.Sz; OUT.Bm[e] = s.Bm; OUT.Cq[e] = s.Cq; OUT.Nq[e] = s.Nq;
+        OUT.Pq[e] = s.Pq; OUT.Chl[e] = s.Chl; OUT.gen[e] = s.gen; OUT.age[e] = s.age;
+        OUT.flags[e] = ac ? (uint8_t)(F_AC | (gz ? F_GRAZ : 0) | (mt ? F_MORT : 0) | (dv ? F_DVID : 0)) : (uint8_t)0;
+        OUT.cell[e] = c;
+        a.con.v[0][e] = L0; a.con.v[1][e] = L1; a.con.v[2][e] = L2;
+        a.con.v[3][e] = L3; a.con.v[4][e] = L4; a.con.v[5][e] = L5;
+        a.con.tag[e] = tag;
+    }
+}
+
+/* Ordered per-cell sums: calc_loss! (graz), calc_loss! (mort), calc_consume! per
+ * species in slot order (plankton_update.jl:5-29), and the per-species
+ * diagnostics num/graz/mort/dvid (diagnostics_kernels.jl:4-12). */
+
+__global__ void __launch_bounds__(256) k_plk(PlkArgs a)
+{
+    if (a.st->err) return;
+    const DevGrid& g = a.g;
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < g.ncell;
+         c += (long long)gridDim.x * blockDim.x) {
+        const uint32_t s0 = a.seg[c], s1 = a.seg[c + 1];
+        float DIC = 0.f, NH4 = 0.f, NO3 = 0.f, PO4 = 0.f, DOC = 0.f, DON = 0.f, DOP = 0.f;
+        float POC = 0.f, PON = 0.f, POP = 0.f;
+        uint32_t e = s0;
+        while (e < s1) {
+            const uint32_t sp = a.pkey[e] >> PI_SLOT_BITS;
+            uint32_t e1 = e + 1;
+            while (e1 < s1 && (a.pkey[e1] >> PI_SLOT_BITS) == sp) ++e1;
+            uint32_t ng = 0, nm = 0, nd = 0, na = 0;
+            for (uint32_t x = e; x < e1; ++x) {
+                if (a.con.tag[x] & T_GRAZ) {
+                    DOC = DOC + a.con.v[0][x]; DON = DON + a.con.v[1][x]; DOP = DOP + a.con.v[2][x];
+                    POC = POC + a.con.v[3][x]; PON = PON + a.con.v[4][x]; POP = POP + a.con.v[5][x];
+                    ++ng;
+                }
+            }
+            for (uint32_t x = e; x < e1; ++x) {
+                if (a.con.tag[x] & T_MORT) {
+                    DOC = DOC + a.con.v[0][x]; DON = DON + a.con.v[1][x]; DOP = DOP + a.con.v[2][x];
+                    POC = POC + a.con.v[3][x]; PON = PON + a.con.v[4][x]; POP = POP + a.con.v[5][x];
+                    ++nm;
+                }
+            }
+            for (uint32_t x = e; x < e1; ++x) {
+                const uint8_t t = a.con.tag[x];
+                if (t & T_ALIVE) {
+                    DIC = DIC + a.con.v[0][x]; DOC = DOC + a.con.v[1][x]; NH4 = NH4 + a.con.v[2][x];
+                    NO3 = NO3 + a.con.v[3][x]; PO4 = PO4 + a.con.v[4][x];
+                    ++na;
+                    if (t & T_DIV) ++nd;
+                }
+            }
+            float* d = a.diag + (long long)sp * 4 * g.ncell;
+            if (na) d[0 * g.ncell + c] = d[0 * g.ncell + c] + (float)na;
+            if (ng) d[1 * g.ncell + c] = d[1 * g.ncell + c] + (float)ng;
+            if (nm) d[2 * g.ncell + c] = d[2 * g.ncell + c] + (float)nm;
+            if (nd) d[3 * g.ncell + c] = d[3 * g.ncell + c] + (float)nd;
+            e = e1;
+        }
+        a.plk[PI_DIC][c] = DIC; a.plk[PI_NH4][c] = NH4; a.plk[PI_NO3][c] = NO3; a.plk[PI_PO4][c] = PO4;
+        a.plk[PI_DOC][c] = DOC; a.plk[PI_DON][c] = DON; a.plk[PI_DOP][c] = DOP;
+        a.plk[PI_POC][c] = POC; a.plk[PI_PON][c] = PON; a.plk[PI_POP][c] = POP;
+    }
+}
+
+/* ------------------------------------------------------- host-side launchers */
+void launch_adv(hipStream_t st, int blocks, const AdvArgs& a) { hipLaunchKernelGGL(k_adv, dim3(blocks), dim3(256), 0, st, a); }
+void launch_marks(hipStream_t s, int blocks, DevState* st, uint32_t* bmA, uint32_t* bmD, int nw)
+{
+    hipLaunchKernelGGL(k_marks, dim3(blocks), dim3(256), 0, s, st, bmA, bmD, nw);
+}
+void launch_place(hipStream_t s, int blocks, DevState* st, const uint32_t* seg, uint32_t* cnt, uint32_t* pkey, uint32_t* psrc)
+{
+    hipLaunchKernelGGL(k_place, dim3(blocks), dim3(256), 0, s, st, seg, cnt, pkey, psrc);
+}
+void launch_cells(hipStream_t s, int blocks, DevState* st, long long ncell, const uint32_t* seg, uint32_t* pkey,
+                  uint32_t* psrc, float* Chl, float* pop, uint32_t* big)
+{
+    hipLaunchKernelGGL(k_cells, dim3(blocks), dim3(256), 0, s, st, ncell, seg, pkey, psrc, Chl, pop, big);
+}
+void launch_bigcells(hipStream_t s, int blocks, DevState* st, const uint32_t* big, const uint32_t* seg,
+                     uint32_t* pkey, uint32_t* psrc, uint32_t* skey, uint32_t* ssrc, float* Chl)
+{
+    hipLaunchKernelGGL(k_bigcells, dim3(blocks), dim3(256), 0, s, st, big, seg, pkey, psrc, skey, ssrc, Chl);
+}
+void launch_par(hipStream_t s, int blocks, DevState* st, const DevGrid& g, const float* Chl, const float* PARF_in,
+                float* PARF_out, float* diag_par, float kc, float kw)
+{
+    hipLaunchKernelGGL(k_par, dim3(blocks), dim3(256), 0, s, st, g, Chl, PARF_in, PARF_out, diag_par, kc, kw);
+}
+void launch_phys(hipStream_t st, int blocks, const PhysArgs& a) { hipLaunchKernelGGL(k_phys, dim3(blocks), dim3(256), 0, st, a); }
+void launch_plk(hipStream_t st, int blocks, const PlkArgs& a) { hipLaunchKernelGGL(k_plk, dim3(blocks), dim3(256), 0, st, a); }

@@ -1,0 +1,298 @@
+/*
+ * k_util.hip - device-wide exclusive scan, capacity check, end-of-step flip,
+ * and the slot-table <-> store conversions used by uploads and downloads.
+ */
+#include "pi_device.h"
+#include "k_args.h"
+
+#define SCAN_THREADS 256
+#define SCAN_ITEMS 8
+#define SCAN_TILE (SCAN_THREADS * SCAN_ITEMS)
+
+/* Input of a scan: either a plain u32 array, or the division bitmaps laid out as
+ * [popcount(D words) of all species | popcount(free words) of all species]. */
+
+__device__ __forceinline__ uint32_t scan_load(const ScanIn& s, uint32_t i)
+{
+    if (i >= s.n) return 0;
+    if (!s.bits) return s.a[i];
+    const uint32_t half = (uint32_t)s.nsp * (uint32_t)s.nw;
+    if (i < half) return __popc(s.D[i]);
+    const uint32_t j = i - half, w = j % (uint32_t)s.nw;
+    const long long rem = s.cap - (long long)w * 32;
+    const uint32_t valid = rem >= 32 ? 0xFFFFFFFFu : ((1u << rem) - 1u);
+    return __popc(~s.A[j] & valid);
+}
+
+/* exclusive scan of one value per thread across a 256-thread block */
+__device__ __forceinline__ uint32_t block_exscan(uint32_t v, uint32_t* total)
+{
+    __shared__ uint32_t wsum[SCAN_THREADS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (int w = 0; w < SCAN_THREADS / 64; ++w) {
+        if (w < wid) off += wsum[w];
+        tot += wsum[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + x - v;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_partials(ScanIn s, uint32_t* partials)
+{
+    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    uint32_t sum = 0;
+    for (int k = 0; k < SCAN_ITEMS; ++k) sum += scan_load(s, base + k);
+    uint32_t tot;
+    block_exscan(sum, &tot);
+    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+/* single block: exclusive scan of the per-tile partials, total -> out[n] */
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_top(uint32_t* partials, uint32_t nb, uint32_t* out, uint32_t n)
+{
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += SCAN_TILE) {
+        const uint32_t i0 = base + threadIdx.x * SCAN_ITEMS;
+        uint32_t v[SCAN_ITEMS], sum = 0;
+        for (int k = 0; k < SCAN_ITEMS; ++k) { v[k] = (i0 + k < nb) ? partials[i0 + k] : 0; sum += v[k]; }
+        uint32_t tot;
+        uint32_t pre = block_exscan(sum, &tot) + carry;
+        for (int k = 0; k < SCAN_ITEMS; ++k) {
+            if (i0 + k < nb) partials[i0 + k] = pre;
+            pre += v[k];
+        }
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_final(ScanIn s, const uint32_t* partials, uint32_t* out)
+{
+    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS], sum = 0;
+    for (int k = 0; k < SCAN_ITEMS; ++k) { v[k] = scan_load(s, base + k); sum += v[k]; }
+    uint32_t tot;
+    uint32_t pre = block_exscan(sum, &tot) + partials[blockIdx.x];
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        if (base + k < s.n) out[base + k] = pre;
+        pre += v[k];
+    }
+}
+
+/* totals, daughter offsets and the capacity check (plankton_update.jl:18-22:
+ * dot(dvid, ac) > length(findall(!ac)) throws). */
+__global__ void k_capcheck(DevState* st, const uint32_t* G, int nsp, int nw)
+{
+    if (threadIdx.x != 0 || st->err) return;
+    uint32_t off = 0;
+    for (int s = 0; s < nsp; ++s) {
+        const uint32_t d = G[(uint32_t)(s + 1) * nw] - G[(uint32_t)s * nw];
+        const uint32_t f = G[(uint32_t)(nsp + s + 1) * nw] - G[(uint32_t)(nsp + s) * nw];
+        st->dtot[s] = d;
+        st->ftot[s] = f;
+        st->doff[s] = off;
+        off += d;
+        if (d > f && !st->err) { st->err = PI_ERR_CAPACITY; st->err_sp = (uint32_t)s; }
+    }
+    st->nbig = 0;
+}
+
+/* end of TimeStep!: vel₀ <- vel₁, par₀ <- par, new tracers and store become current */
+__global__ void k_finish(DevState* st, const uint32_t* seg, long long ncell, int nsp)
+{
+    if (threadIdx.x != 0 || st->err) return;
+    uint32_t born = 0;
+    for (int s = 0; s < nsp; ++s) born += st->dtot[s];
+    st->nstore = seg[ncell] + born;
+    st->scur ^= 1;
+    st->tcur ^= 1;
+    st->pcur ^= 1;
+    st->iteration += 1;
+}
+
+__global__ void k_zero_u32(uint32_t* p, long long n, const DevState* st)
+{
+    if (st && st->err) return;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+/* vel½ = (vel₀ + vel₁) * 0.5f0 (time_step.jl:17-19) */
+__global__ void k_velhalf(const float* a, const float* b, float* h, long long n, const DevState* st)
+{
+    if (st->err) return;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        h[i] = (a[i] + b[i]) * 0.5f;
+}
+
+/* diags.tracer[key] .+= tracers[key].data over padded arrays (time_step.jl:168-172) */
+__global__ void k_accum(float* d, const DevState* st, int q, long long n)
+{
+    if (st->err) return;
+    const float* src = st->tr[st->tcur ^ 1][q];   /* the tracers written by this step */
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        d[i] = d[i] + src[i];
+}
+
+/* ---------------------------------------------------- store <-> slot table */
+/* live store entries -> their slot-table rows (marks them active there) */
+__global__ void k_store_to_table(DevState* st, SlotTable tab, long long cap)
+{
+    const Store S = st->store[st->scur];
+    const uint32_t n = st->nstore;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const uint8_t fl = S.flags[e];
+        if (!(fl & F_AC)) continue;
+        const uint32_t key = S.key[e];
+        const long long r = (long long)(key >> PI_SLOT_BITS) * cap + (key & PI_SLOT_MASK);
+        tab.x[r] = S.x[e]; tab.y[r] = S.y[e]; tab.z[r] = S.z[e];
+        tab.Sz[r] = S.Sz[e]; tab.Bm[r] = S.Bm[e]; tab.Cq[r] = S.Cq[e]; tab.Nq[r] = S.Nq[e];
+        tab.Pq[r] = S.Pq[e]; tab.Chl[r] = S.Chl[e]; tab.gen[r] = S.gen[e]; tab.age[r] = S.age[e];
+        tab.flags[r] = fl;
+        tab.cell[r] = S.cell[e];
+        tab.death[r] = (int32_t)st->iteration;
+    }
+}
+
+/* slot-table rows marked active -> a fresh store (order is irrelevant: the
+ * next step sorts by cell) */
+__global__ void k_rebuild(DevState* st, SlotTable tab, long long cap, int nsp)
+{
+    const Store S = st->store[st->scur];
+    const long long total = cap * nsp;
+    for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < total; r += (long long)gridDim.x * blockDim.x) {
+        const uint8_t fl = tab.flags[r];
+        if (!(fl & F_AC)) continue;
+        const uint32_t e = atomicAdd(&st->nstore, 1u);
+        S.key[e] = ((uint32_t)(r / cap) << PI_SLOT_BITS) | (uint32_t)(r % cap);
+        S.x[e] = tab.x[r]; S.y[e] = tab.y[r]; S.z[e] = tab.z[r];
+        S.Sz[e] = tab.Sz[r]; S.Bm[e] = tab.Bm[r]; S.Cq[e] = tab.Cq[r]; S.Nq[e] = tab.Nq[r];
+        S.Pq[e] = tab.Pq[r]; S.Chl[e] = tab.Chl[r]; S.gen[e] = tab.gen[r]; S.age[e] = tab.age[r];
+        S.flags[e] = fl;
+        S.cell[e] = tab.cell[r];
+    }
+}
+
+/* One column of `model.individuals.phytos.spN.data` in the reference layout.
+ * Rows of inactive slots written before the current iteration have gone
+ * through at least one step while inactive, whose fixed point is: x = y = z = 0
+ * (boundaries * ac), xi = yi = zi = H (find_inds), Sz = Bm / (Cquota * Nsuper)
+ * (update_cellsize! runs on every slot) and zero decisions; every other field
+ * is unchanged by a step with ac = false. */
+__global__ void k_table_field(const DevState* st, SlotTable tab, int sp, long long cap, int field,
+                              const float* params, DevGrid g, void* out)
+{
+    const long long it = st->iteration;
+    const float* q = params + sp * PI_NPARAMS;
+    for (long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x; s < cap; s += (long long)gridDim.x * blockDim.x) {
+        const long long r = (long long)sp * cap + s;
+        const uint8_t fl = tab.flags[r];
+        const bool act = (fl & F_AC) != 0;
+        const bool settled = !act && (long long)tab.death[r] < it;
+        float v = 0.0f;
+        switch (field) {
+            case PI_F_x: v = settled ? 0.0f : tab.x[r]; break;
+            case PI_F_y: v = settled ? 0.0f : tab.y[r]; break;
+            case PI_F_z: v = settled ? 0.0f : tab.z[r]; break;
+            case PI_F_Sz: v = settled ? tab.Bm[r] / (q[PI_P_Cquota] * q[PI_P_Nsuper]) : tab.Sz[r]; break;
+            case PI_F_Bm: v = tab.Bm[r]; break;
+            case PI_F_Cq: v = tab.Cq[r]; break;
+            case PI_F_Nq: v = tab.Nq[r]; break;
+            case PI_F_Pq: v = tab.Pq[r]; break;
+            case PI_F_Chl: v = tab.Chl[r]; break;
+            case PI_F_gen: v = tab.gen[r]; break;
+            case PI_F_age: v = tab.age[r]; break;
+            case PI_F_graz: v = (!settled && (fl & F_GRAZ)) ? 1.0f : 0.0f; break;
+            case PI_F_mort: v = (!settled && (fl & F_MORT)) ? 1.0f : 0.0f; break;
+            case PI_F_dvid: v = (!settled && (fl & F_DVID)) ? 1.0f : 0.0f; break;
+            default: break;
+        }
+        if (field <= PI_F_dvid) { ((float*)out)[s] = v; continue; }
+        if (field == PI_F_ac) { ((uint8_t*)out)[s] = act ? 1 : 0; continue; }
+        long long xi, yi, zi;
+        if (settled) { xi = g.Hx; yi = g.Hy; zi = g.Hz; }
+        else if (fl & F_ZERO) { xi = yi = zi = 0; }
+        else {
+            const uint32_t c = tab.cell[r];
+            xi = c % (uint32_t)g.Px + 1;
+            yi = (c / (uint32_t)g.Px) % (uint32_t)g.Py + 1;
+            zi = c / ((uint32_t)g.Px * (uint32_t)g.Py) + 1;
+        }
+        ((long long*)out)[s] = field == PI_F_xi ? xi : field == PI_F_yi ? yi : zi;
+    }
+}
+
+__global__ void k_count(const DevState* st, int sp, unsigned long long* out)
+{
+    const Store S = st->store[st->scur];
+    const uint32_t n = st->nstore;
+    unsigned long long c = 0;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+        if ((S.flags[e] & F_AC) && (S.key[e] >> PI_SLOT_BITS) == (uint32_t)sp) ++c;
+    if (c) atomicAdd(out, c);
+}
+
+/* ------------------------------------------------------------- launchers */
+void launch_scan(hipStream_t s, const ScanIn& in, uint32_t* partials, uint32_t* out)
+{
+    const uint32_t nb = (in.n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb == 0) return;   /* callers never scan empty arrays */
+    hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(SCAN_THREADS), 0, s, in, partials);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, s, partials, nb, out, in.n);
+    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(SCAN_THREADS), 0, s, in, partials, out);
+}
+ScanIn scan_plain(const uint32_t* a, uint32_t n)
+{
+    ScanIn s{}; s.a = a; s.n = n; s.bits = 0; return s;
+}
+ScanIn scan_bits(const uint32_t* D, const uint32_t* A, int nsp, int nw, long long cap)
+{
+    ScanIn s{}; s.bits = 1; s.D = D; s.A = A; s.nsp = nsp; s.nw = nw; s.cap = cap; s.n = 2u * nsp * nw; return s;
+}
+void launch_capcheck(hipStream_t s, DevState* st, const uint32_t* G, int nsp, int nw)
+{
+    hipLaunchKernelGGL(k_capcheck, dim3(1), dim3(64), 0, s, st, G, nsp, nw);
+}
+void launch_finish(hipStream_t s, DevState* st, const uint32_t* seg, long long ncell, int nsp)
+{
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, s, st, seg, ncell, nsp);
+}
+void launch_zero(hipStream_t s, uint32_t* p, long long n, const DevState* st)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, p, n, st);
+}
+void launch_velhalf(hipStream_t s, const float* a, const float* b, float* h, long long n, const DevState* st)
+{
+    hipLaunchKernelGGL(k_velhalf, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, a, b, h, n, st);
+}
+void launch_accum(hipStream_t s, float* d, const DevState* st, int q, long long n)
+{
+    hipLaunchKernelGGL(k_accum, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, d, st, q, n);
+}
+void launch_store_to_table(hipStream_t s, DevState* st, const SlotTable& tab, long long cap, long long maxn)
+{
+    hipLaunchKernelGGL(k_store_to_table, dim3(grid_for(maxn, 256, 8192)), dim3(256), 0, s, st, tab, cap);
+}
+void launch_rebuild(hipStream_t s, DevState* st, const SlotTable& tab, long long cap, int nsp)
+{
+    hipLaunchKernelGGL(k_rebuild, dim3(grid_for(cap * nsp, 256, 8192)), dim3(256), 0, s, st, tab, cap, nsp);
+}
+void launch_table_field(hipStream_t s, const DevState* st, const SlotTable& tab, int sp, long long cap, int field,
+                        const float* params, const DevGrid& g, void* out)
+{
+    hipLaunchKernelGGL(k_table_field, dim3(grid_for(cap, 256, 8192)), dim3(256), 0, s, st, tab, sp, cap, field, params, g, out);
+}
+void launch_count(hipStream_t s, const DevState* st, int sp, unsigned long long* out, long long maxn)
+{
+    hipLaunchKernelGGL(k_count, dim3(grid_for(maxn, 256, 2048)), dim3(256), 0, s, st, sp, out);
+}

@@ -1,0 +1,105 @@
+/*
+ * pi_device.h - device-side data layout and shared helpers of libpi_hip.so.
+ *
+ * HBM layout (one model):
+ *   - grid fields: padded (Nx+2Hx, Ny+2Hy, Nz+2Hz) float32, x fastest;
+ *     14 tracers double-buffered, sweep scratch 2 x 14, vel0/vel1/velhalf,
+ *     par/par0/Chl/pop/temp/plk/diagnostics.
+ *   - particle STORE: only live individuals, all species, SoA, kept sorted by
+ *     (cell, species, slot) from one step to the next.  Key = species << 27 |
+ *     slot.  Neighbouring lanes therefore touch neighbouring cells: velocity and
+ *     tracer gathers hit L1/L2, and every per-cell sum is a contiguous segment.
+ *   - SLOT TABLE: the reference's by-slot layout (per species, length capacity),
+ *     written on upload and when an individual dies; downloads merge it with
+ *     the store.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/pi_hip.h"
+#include "../../include/pi_math.h"
+
+#define PI_SLOT_BITS 27
+#define PI_SLOT_MASK ((1u << PI_SLOT_BITS) - 1u)
+
+/* flags byte of a store entry / slot-table row */
+#define F_AC    1u   /* active */
+#define F_GRAZ  2u   /* grazing decision of the previous step */
+#define F_MORT  4u   /* mortality decision */
+#define F_DVID  8u   /* division decision */
+#define F_ZERO  16u  /* stale (xi,yi,zi) = (0,0,0): initial state of the reference */
+
+/* contrib tags */
+#define T_ALIVE 1u
+#define T_GRAZ  2u
+#define T_MORT  4u
+#define T_DIV   8u
+
+struct DevGrid {
+    int Nx, Ny, Nz, Hx, Hy, Hz, Px, Py, Pz;
+    int tx, ty, tz;
+    int has_land;
+    float dx, dy;
+    long long ncell;
+    const float* dzC;   /* Pz, 0-based (reference index k -> dzC[k-1]) */
+    const float* dzF;
+    const uint8_t* lm;  /* padded landmask */
+};
+
+struct Store {
+    uint32_t* key;
+    float *x, *y, *z, *Sz, *Bm, *Cq, *Nq, *Pq, *Chl, *gen, *age;
+    uint8_t* flags;
+    uint32_t* cell;
+};
+
+struct SlotTable {               /* per species, stride cap */
+    float *x, *y, *z, *Sz, *Bm, *Cq, *Nq, *Pq, *Chl, *gen, *age;
+    uint8_t* flags;
+    uint32_t* cell;
+    int32_t* death;              /* iteration at which the row was last written */
+};
+
+struct Contrib {
+    float* v[6];
+    uint8_t* tag;
+};
+
+struct Vel { const float *u, *v, *w; };
+
+/* 1-based padded (i,j,k) -> linear index */
+__device__ __forceinline__ long long gidx(const DevGrid& g, int i, int j, int k) {
+    return (long long)(i - 1) + (long long)g.Px * ((long long)(j - 1) + (long long)g.Py * (long long)(k - 1));
+}
+__device__ __forceinline__ bool LM(const DevGrid& g, long long c) { return g.has_land ? g.lm[c] != 0 : true; }
+__device__ __forceinline__ float dzC_at(const DevGrid& g, long long k) {   /* clamp as documented */
+    if (k < 1) k = 1;
+    if (k > g.Pz) k = g.Pz;
+    return g.dzC[k - 1];
+}
+__device__ __forceinline__ float dzF_at(const DevGrid& g, int k) { return g.dzF[k - 1]; }
+
+/* launch helpers */
+static inline int grid_for(long long n, int block, int maxblocks) {
+    long long b = (n + block - 1) / block;
+    if (b > maxblocks) b = maxblocks;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+#define PI_MAXSP 31
+
+/* Everything a step flips or counts lives on the device, so a step is a fixed
+ * sequence of launches with fixed arguments (capturable in a graph) and an
+ * error raised early in a step leaves every buffer untouched. */
+struct DevState {
+    Store store[2];                    /* live individuals: [scur] current, [1-scur] next */
+    float* tr[2][PI_NTRACERS];         /* tracers: [tcur] current */
+    float* par[2];                     /* [pcur] = par of the previous step (par₀) */
+    int32_t scur, tcur, pcur, vcur;
+    long long iteration;               /* completed steps */
+    uint32_t nstore;                   /* entries in store[scur] */
+    uint32_t err, err_sp;
+    uint32_t nbig;
+    uint32_t dtot[PI_MAXSP], ftot[PI_MAXSP], doff[PI_MAXSP];
+};

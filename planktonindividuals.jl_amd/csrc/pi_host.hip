@@ -1,0 +1,724 @@
+/*
+ * pi_host.hip - the C ABI of libpi_hip.so (include/pi_hip.h): model storage in
+ * HBM, input staging, the per-step launch sequence and the reference-layout
+ * downloads.
+ *
+ * One step (Model/time_step.jl:12-180) is this fixed launch sequence on the
+ * model's stream, with no host synchronisation:
+ *   [vel½]  marks -> bitmap scan -> capacity check
+ *   adv -> cell-count scan -> place -> cells (+bigcells) -> par -> phys -> plk
+ *   sweep x -> fill y -> sweep y -> fill z -> final -> fill x,y,z -> [diag tracers]
+ *   finish (flip current buffers, count the store, iteration += 1)
+ */
+#include "pi_device.h"
+
+#include <cstdio>
+#include <cstring>
+#include <cmath>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include <cstddef>
+
+#include "k_args.h"
+
+static thread_local std::string g_err;
+static pi_status fail(pi_status s, const std::string& m) { g_err = m; return s; }
+#define HIPCHECK(x)                                                                      \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess)                                                            \
+            return fail(PI_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+struct KTimer { const char* name; hipEvent_t a, b; };
+
+struct pi_model {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    DevGrid g{};
+    int nsp = 0;
+    long long cap = 0, ctot = 0, ncell = 0, nint = 0;
+    int nw = 0;
+    std::vector<float> h_params, bgc;
+    float* d_params = nullptr;
+    uint64_t seed = 0;
+    DevState* d_state = nullptr;
+    DevState h_state{};
+    SlotTable tab{};
+    float* T1[PI_NTRACERS]{};
+    float* T2[PI_NTRACERS]{};
+    float* vel[3][3]{};        /* buffer set b, component c */
+    int v0 = 0, v1 = 0;        /* current buffer sets of vel₀ and vel₁ */
+    bool vel_same = true;
+    float *Chl = nullptr, *pop = nullptr, *temp = nullptr, *PARF_in = nullptr, *PARF_out = nullptr;
+    float* plk[PI_NTRACERS]{};
+    float* diagPAR = nullptr;
+    float* diag = nullptr;     /* [sp][4][ncell] */
+    std::vector<int> diag_ids;
+    std::vector<float*> diag_tr;
+    uint32_t *cnt = nullptr, *seg = nullptr, *partials = nullptr, *pkey = nullptr, *psrc = nullptr;
+    uint32_t *skey = nullptr, *ssrc = nullptr, *big = nullptr, *bm = nullptr, *G = nullptr;
+    Contrib con{};
+    unsigned long long* d_count = nullptr;
+    bool dirty = false;
+    long long host_iter = 0;
+    float last_dt = 0.0f;
+    std::vector<void*> allocs;
+    bool timing = false;
+    std::vector<KTimer> timers;
+    size_t tcount = 0;
+};
+
+static hipError_t dalloc(pi_model* m, void** p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes ? bytes : 4);
+    if (e == hipSuccess) {
+        m->allocs.push_back(*p);
+        e = hipMemset(*p, 0, bytes ? bytes : 4);
+    }
+    return e;
+}
+#define DALLOC(ptr, bytes) HIPCHECK(dalloc(m, (void**)&(ptr), (size_t)(bytes)))
+
+/* ------------------------------------------------------------------ timing */
+static void tbegin(pi_model* m, const char* name)
+{
+    if (!m->timing) return;
+    if (m->tcount == m->timers.size()) {
+        KTimer t{name, nullptr, nullptr};
+        (void)hipEventCreate(&t.a);
+        (void)hipEventCreate(&t.b);
+        m->timers.push_back(t);
+    }
+    m->timers[m->tcount].name = name;
+    (void)hipEventRecord(m->timers[m->tcount].a, m->stream);
+}
+static void tend(pi_model* m)
+{
+    if (!m->timing) return;
+    (void)hipEventRecord(m->timers[m->tcount].b, m->stream);
+    m->tcount++;
+}
+
+/* ------------------------------------------------------------------ helpers */
+static int blocks_for(long long n) { return grid_for(n, 256, 4096); }
+
+static pi_status sync_state(pi_model* m)
+{
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    HIPCHECK(hipMemcpy(&m->h_state, m->d_state, sizeof(DevState), hipMemcpyDeviceToHost));
+    if (m->h_state.err == PI_ERR_HIP) return fail(PI_ERR_HIP, "device-side consistency check failed in division");
+    if (m->h_state.err == PI_ERR_CAPACITY) {
+        char buf[200];
+        snprintf(buf, sizeof buf,
+                 "number of individual exceeds the capacity at timestep %g days (species %u)",
+                 (double)(m->h_state.iteration + 1) * (double)m->last_dt / 86400.0, m->h_state.err_sp + 1);
+        return fail(PI_ERR_CAPACITY, buf);
+    }
+    return PI_OK;
+}
+
+static pi_status rebuild_if_dirty(pi_model* m)
+{
+    if (!m->dirty) return PI_OK;
+    uint32_t zero = 0;
+    HIPCHECK(hipMemcpyAsync((char*)m->d_state + offsetof(DevState, nstore), &zero, sizeof zero,
+                            hipMemcpyHostToDevice, m->stream));
+    launch_rebuild(m->stream, m->d_state, m->tab, m->cap, m->nsp);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    m->dirty = false;
+    return PI_OK;
+}
+
+/* host-side fill_halo_* of an input velocity (halo_regions.jl:64-99) */
+static void host_fill(const pi_model* m, std::vector<float>& c, int dim, int mode)
+{
+    /* mode 0: lo, 1: hi, 2: hi_vel */
+    const DevGrid& g = m->g;
+    const int N = dim == 0 ? g.Nx : dim == 1 ? g.Ny : g.Nz;
+    const int H = dim == 0 ? g.Hx : dim == 1 ? g.Hy : g.Hz;
+    const int topo = dim == 0 ? g.tx : dim == 1 ? g.ty : g.tz;
+    const long long stride = dim == 0 ? 1 : dim == 1 ? g.Px : (long long)g.Px * g.Py;
+    const int P1 = dim == 0 ? g.Py : g.Px, P2 = dim == 2 ? g.Py : g.Pz;
+    for (int o2 = 1; o2 <= P2; ++o2)
+        for (int o1 = 1; o1 <= P1; ++o1) {
+            long long base;
+            if (dim == 0) base = (long long)(o1 - 1) * g.Px + (long long)(o2 - 1) * g.Px * g.Py;
+            else if (dim == 1) base = (long long)(o1 - 1) + (long long)(o2 - 1) * g.Px * g.Py;
+            else base = (long long)(o1 - 1) + (long long)(o2 - 1) * g.Px;
+            auto at = [&](int p) -> float& { return c[base + (long long)(p - 1) * stride]; };
+            float tmp[16];
+            if (mode == 0) {
+                if (topo == PI_PERIODIC) { for (int h = 0; h < H; ++h) tmp[h] = at(N + 1 + h); for (int h = 0; h < H; ++h) at(1 + h) = tmp[h]; }
+                else { float v = at(H + 1); for (int h = 0; h < H; ++h) at(1 + h) = v; }
+            } else if (mode == 1 || topo == PI_PERIODIC) {
+                if (topo == PI_PERIODIC) { for (int h = 0; h < H; ++h) tmp[h] = at(H + 1 + h); for (int h = 0; h < H; ++h) at(N + H + 1 + h) = tmp[h]; }
+                else { float v = at(N + H); for (int h = 0; h < H; ++h) at(N + H + 1 + h) = v; }
+            } else {   /* hi_vel, bounded: c[N+H+2:N+2H] = c[N+H+1] */
+                float v = at(N + H + 1);
+                for (int h = 1; h < H; ++h) at(N + H + 1 + h) = v;
+            }
+        }
+}
+
+static pi_status create_impl(const pi_model_desc* d, pi_model* m)
+{
+    const pi_grid_desc& gd = d->grid;
+    HIPCHECK(hipGetDevice(&m->dev));
+    HIPCHECK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    DevGrid& g = m->g;
+    g.Nx = gd.Nx; g.Ny = gd.Ny; g.Nz = gd.Nz; g.Hx = gd.Hx; g.Hy = gd.Hy; g.Hz = gd.Hz;
+    g.Px = gd.Nx + 2 * gd.Hx; g.Py = gd.Ny + 2 * gd.Hy; g.Pz = gd.Nz + 2 * gd.Hz;
+    g.tx = gd.topo_x; g.ty = gd.topo_y; g.tz = gd.topo_z;
+    g.dx = gd.dx; g.dy = gd.dy;
+    g.ncell = (long long)g.Px * g.Py * g.Pz;
+    if (g.ncell >= (1ll << 32) - 2) return fail(PI_ERR_ARG, "grid too large for 32-bit cell ids");
+    m->ncell = g.ncell;
+    m->nint = (long long)g.Nx * g.Ny * g.Nz;
+    m->nsp = d->n_species;
+    m->cap = d->capacity;
+    m->ctot = m->cap * m->nsp;
+    if (m->ctot >= (1ll << 32)) return fail(PI_ERR_ARG, "total capacity too large");
+    m->nw = (int)((m->cap + 31) / 32);
+    m->seed = d->rng_seed;
+    m->h_params.assign(d->params, d->params + m->nsp * PI_NPARAMS);
+    m->bgc.assign(d->bgc, d->bgc + PI_NBGC);
+    float *dzC, *dzF;
+    DALLOC(dzC, g.Pz * 4); DALLOC(dzF, g.Pz * 4);
+    HIPCHECK(hipMemcpy(dzC, gd.dzC, g.Pz * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(dzF, gd.dzF, g.Pz * 4, hipMemcpyHostToDevice));
+    g.dzC = dzC; g.dzF = dzF;
+    g.has_land = 0; g.lm = nullptr;
+    if (gd.landmask) {
+        bool any0 = false;
+        for (long long c = 0; c < g.ncell; ++c) if (!gd.landmask[c]) { any0 = true; break; }
+        if (any0) {
+            uint8_t* lm;
+            DALLOC(lm, g.ncell);
+            HIPCHECK(hipMemcpy(lm, gd.landmask, g.ncell, hipMemcpyHostToDevice));
+            g.lm = lm; g.has_land = 1;
+        }
+    }
+    DALLOC(m->d_params, m->h_params.size() * 4);
+    HIPCHECK(hipMemcpy(m->d_params, m->h_params.data(), m->h_params.size() * 4, hipMemcpyHostToDevice));
+    /* store x2 */
+    DevState& hs = m->h_state;
+    memset(&hs, 0, sizeof hs);
+    for (int b = 0; b < 2; ++b) {
+        Store& S = hs.store[b];
+        DALLOC(S.key, m->ctot * 4);
+        for (float** f : {&S.x, &S.y, &S.z, &S.Sz, &S.Bm, &S.Cq, &S.Nq, &S.Pq, &S.Chl, &S.gen, &S.age})
+            DALLOC(*f, m->ctot * 4);
+        DALLOC(S.flags, m->ctot);
+        DALLOC(S.cell, m->ctot * 4);
+    }
+    /* slot table */
+    SlotTable& T = m->tab;
+    for (float** f : {&T.x, &T.y, &T.z, &T.Sz, &T.Bm, &T.Cq, &T.Nq, &T.Pq, &T.Chl, &T.gen, &T.age})
+        DALLOC(*f, m->ctot * 4);
+    DALLOC(T.flags, m->ctot);
+    DALLOC(T.cell, m->ctot * 4);
+    DALLOC(T.death, m->ctot * 4);
+    /* grid fields */
+    const long long nc = g.ncell;
+    for (int b = 0; b < 2; ++b) for (int q = 0; q < PI_NTRACERS; ++q) DALLOC(hs.tr[b][q], nc * 4);
+    for (int q = 0; q < PI_NTRACERS; ++q) { DALLOC(m->T1[q], nc * 4); DALLOC(m->T2[q], nc * 4); }
+    for (int b = 0; b < 3; ++b) for (int c = 0; c < 3; ++c) DALLOC(m->vel[b][c], nc * 4);
+    m->v0 = 0; m->v1 = 0; m->vel_same = true;
+    DALLOC(hs.par[0], nc * 4); DALLOC(hs.par[1], nc * 4);
+    DALLOC(m->Chl, nc * 4); DALLOC(m->pop, nc * 4); DALLOC(m->temp, nc * 4);
+    DALLOC(m->PARF_in, (long long)g.Nx * g.Ny * 4); DALLOC(m->PARF_out, (long long)g.Nx * g.Ny * 4);
+    const int plk_ids[10] = {PI_DIC, PI_NH4, PI_NO3, PI_PO4, PI_DOC, PI_DON, PI_DOP, PI_POC, PI_PON, PI_POP};
+    for (int q : plk_ids) DALLOC(m->plk[q], nc * 4);
+    DALLOC(m->diagPAR, nc * 4);
+    DALLOC(m->diag, (long long)m->nsp * 4 * nc * 4);
+    /* sort / scan scratch */
+    DALLOC(m->cnt, nc * 4);
+    DALLOC(m->seg, (nc + 1) * 4);
+    const long long scan_n = std::max<long long>(nc, 2ll * m->nsp * m->nw);
+    DALLOC(m->partials, ((scan_n + 2047) / 2048 + 1) * 4);
+    DALLOC(m->pkey, m->ctot * 4); DALLOC(m->psrc, m->ctot * 4);
+    DALLOC(m->skey, 2 * m->ctot * 4); DALLOC(m->ssrc, 2 * m->ctot * 4);
+    DALLOC(m->big, nc * 4);
+    DALLOC(m->bm, 2ll * m->nsp * m->nw * 4);
+    DALLOC(m->G, (2ll * m->nsp * m->nw + 1) * 4);
+    for (int q = 0; q < 6; ++q) DALLOC(m->con.v[q], m->ctot * 4);
+    DALLOC(m->con.tag, m->ctot);
+    DALLOC(m->d_count, 8);
+    DALLOC(m->d_state, sizeof(DevState));
+    hs.scur = hs.tcur = hs.pcur = 0;
+    hs.iteration = 0;
+    hs.nstore = 0;
+    HIPCHECK(hipMemcpy(m->d_state, &hs, sizeof hs, hipMemcpyHostToDevice));
+    for (int q = 0; q < d->n_diag_tracers; ++q) {
+        if (d->diag_tracers[q] < 0 || d->diag_tracers[q] >= PI_NTRACERS) return fail(PI_ERR_ARG, "diag tracer id");
+        float* p; DALLOC(p, nc * 4);
+        m->diag_ids.push_back(d->diag_tracers[q]); m->diag_tr.push_back(p);
+    }
+    return PI_OK;
+}
+
+/* ======================================================================= ABI */
+extern "C" {
+
+const char* pi_last_error(void) { return g_err.c_str(); }
+const char* pi_version(void) { return "libpi_hip 0.1 (gfx950)"; }
+
+pi_status pi_init(int32_t device)
+{
+    int n = 0;
+    HIPCHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(PI_ERR_ARG, "no such HIP device");
+    HIPCHECK(hipSetDevice(device));
+    return PI_OK;
+}
+
+pi_status pi_model_create(const pi_model_desc* d, pi_model** out)
+{
+    if (!d || !out) return fail(PI_ERR_ARG, "null argument");
+    const pi_grid_desc& gd = d->grid;
+    if (gd.Nx < 1 || gd.Ny < 1 || gd.Nz < 1 || gd.Hx < 2 || gd.Hy < 2 || gd.Hz < 2 || gd.Hx > 8 || gd.Hy > 8 || gd.Hz > 8)
+        return fail(PI_ERR_ARG, "grid: N >= 1 and 2 <= halo <= 8 required");
+    if (gd.topo_z != PI_BOUNDED) return fail(PI_ERR_ARG, "grid: z topology must be Bounded");
+    if ((gd.topo_x == PI_PERIODIC && gd.Nx < gd.Hx) || (gd.topo_y == PI_PERIODIC && gd.Ny < gd.Hy))
+        return fail(PI_ERR_UNSUPPORTED, "periodic directions with N < halo are not supported on the GPU path");
+    if (d->n_species < 1 || d->n_species > PI_MAXSP) return fail(PI_ERR_ARG, "n_species out of range");
+    if (d->capacity < 1 || d->capacity > (long long)PI_SLOT_MASK) return fail(PI_ERR_ARG, "capacity out of range");
+    if (d->bgc[PI_B_shared_graz] != 1.0f)
+        return fail(PI_ERR_UNSUPPORTED, "only shared_graz = 1 is implemented on the GPU path");
+    for (int s = 0; s < d->n_species; ++s) {
+        float t = d->params[s * PI_NPARAMS + PI_P_dvid_type];
+        if (!(t == 1.0f || t == 2.0f || t == 3.0f)) return fail(PI_ERR_ARG, "Wrong cell division type, must be in 1 to 3");
+    }
+    pi_model* m = new pi_model();
+    pi_status st = create_impl(d, m);
+    if (st != PI_OK) {
+        for (void* p : m->allocs) (void)hipFree(p);
+        if (m->stream) (void)hipStreamDestroy(m->stream);
+        delete m;
+        return st;
+    }
+    *out = m;
+    return PI_OK;
+}
+
+pi_status pi_model_destroy(pi_model* m)
+{
+    if (!m) return PI_OK;
+    (void)hipStreamSynchronize(m->stream);
+    for (auto& t : m->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+    for (void* p : m->allocs) (void)hipFree(p);
+    (void)hipStreamDestroy(m->stream);
+    delete m;
+    return PI_OK;
+}
+
+pi_status pi_set_diag_tracers(pi_model* m, int32_t n, const int32_t* ids)
+{
+    if (!m) return fail(PI_ERR_ARG, "null model");
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    for (int32_t a = 0; a < n; ++a) if (ids[a] < 0 || ids[a] >= PI_NTRACERS) return fail(PI_ERR_ARG, "tracer id");
+    /* reuse or allocate accumulators */
+    std::vector<float*> keep = m->diag_tr;
+    m->diag_tr.clear(); m->diag_ids.clear();
+    for (int32_t a = 0; a < n; ++a) {
+        float* p;
+        if (!keep.empty()) { p = keep.back(); keep.pop_back(); HIPCHECK(hipMemset(p, 0, m->ncell * 4)); }
+        else DALLOC(p, m->ncell * 4);
+        m->diag_ids.push_back(ids[a]); m->diag_tr.push_back(p);
+    }
+    return PI_OK;
+}
+
+pi_status pi_upload_individuals(pi_model* m, int32_t sp, const pi_individuals_host* h)
+{
+    if (!m || !h) return fail(PI_ERR_ARG, "null argument");
+    if (sp < 0 || sp >= m->nsp) return fail(PI_ERR_ARG, "species out of range");
+    pi_status s = rebuild_if_dirty(m);
+    if (s != PI_OK) return s;
+    s = sync_state(m);
+    if (s != PI_OK) return s;
+    /* keep the other species' live state: flush the store into the table */
+    launch_store_to_table(m->stream, m->d_state, m->tab, m->cap, 2 * m->ctot);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    const long long cap = m->cap, off = (long long)sp * cap;
+    const DevGrid& g = m->g;
+    std::vector<uint8_t> flags(cap);
+    std::vector<uint32_t> cell(cap);
+    std::vector<int32_t> death(cap, (int32_t)m->h_state.iteration);
+    for (long long i = 0; i < cap; ++i) {
+        uint8_t f = (h->ac && h->ac[i]) ? F_AC : 0;
+        const float* dec[3] = {h->graz, h->mort, h->dvid};
+        const uint8_t bits[3] = {F_GRAZ, F_MORT, F_DVID};
+        for (int k = 0; k < 3; ++k) {
+            float v = dec[k] ? dec[k][i] : 0.0f;
+            if (v != 0.0f && v != 1.0f) return fail(PI_ERR_ARG, "graz/mort/dvid decisions must be 0 or 1");
+            if (v == 1.0f) f |= bits[k];
+        }
+        long long xi = h->xi ? h->xi[i] : 0, yi = h->yi ? h->yi[i] : 0, zi = h->zi ? h->zi[i] : 0;
+        if (xi == 0 && yi == 0 && zi == 0) { f |= F_ZERO; cell[i] = 0; }
+        else {
+            if (xi < 1 || xi > g.Px || yi < 1 || yi > g.Py || zi < 1 || zi > g.Pz)
+                return fail(PI_ERR_ARG, "xi/yi/zi outside the padded grid");
+            cell[i] = (uint32_t)((xi - 1) + (long long)g.Px * ((yi - 1) + (long long)g.Py * (zi - 1)));
+        }
+        flags[i] = f;
+    }
+    const SlotTable& T = m->tab;
+    const float* src[11] = {h->x, h->y, h->z, h->Sz, h->Bm, h->Cq, h->Nq, h->Pq, h->Chl, h->gen, h->age};
+    float* dst[11] = {T.x, T.y, T.z, T.Sz, T.Bm, T.Cq, T.Nq, T.Pq, T.Chl, T.gen, T.age};
+    for (int k = 0; k < 11; ++k) {
+        if (src[k]) HIPCHECK(hipMemcpy(dst[k] + off, src[k], cap * 4, hipMemcpyHostToDevice));
+        else HIPCHECK(hipMemset(dst[k] + off, 0, cap * 4));
+    }
+    HIPCHECK(hipMemcpy(T.flags + off, flags.data(), cap, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(T.cell + off, cell.data(), cap * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(T.death + off, death.data(), cap * 4, hipMemcpyHostToDevice));
+    m->dirty = true;
+    return PI_OK;
+}
+
+pi_status pi_upload_tracer(pi_model* m, int32_t q, const float* c)
+{
+    if (!m || !c || q < 0 || q >= PI_NTRACERS) return fail(PI_ERR_ARG, "bad tracer upload");
+    pi_status s = sync_state(m);
+    if (s != PI_OK) return s;
+    HIPCHECK(hipMemcpy(m->h_state.tr[m->h_state.tcur][q], c, m->ncell * 4, hipMemcpyHostToDevice));
+    return PI_OK;
+}
+
+pi_status pi_set_velocity(pi_model* m, const float* u, const float* v, const float* w)
+{
+    if (!m || !u || !v || !w) return fail(PI_ERR_ARG, "null velocity");
+    const DevGrid& g = m->g;
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    int b = 0;
+    while (b == m->v0 || (b == m->v1 && m->v1 != m->v0)) ++b;   /* a set used by neither vel₀ nor vel₁ */
+    if (b > 2) b = (m->v0 + 1) % 3;
+    const int nx[3] = {g.Nx + (g.tx == PI_BOUNDED), g.Nx, g.Nx};
+    const int ny[3] = {g.Ny, g.Ny + (g.ty == PI_BOUNDED), g.Ny};
+    const int nz[3] = {g.Nz, g.Nz, g.Nz + (g.tz == PI_BOUNDED)};
+    const float* in[3] = {u, v, w};
+    std::vector<float> pad(m->ncell);
+    for (int c = 0; c < 3; ++c) {
+        std::fill(pad.begin(), pad.end(), 0.0f);
+        HIPCHECK(hipMemcpy(pad.data(), m->vel[m->v1][c], m->ncell * 4, hipMemcpyDeviceToHost)); /* previous halos */
+        for (int k = 0; k < nz[c]; ++k) for (int j = 0; j < ny[c]; ++j) for (int i = 0; i < nx[c]; ++i)
+            pad[(long long)(g.Hx + i) + (long long)g.Px * ((g.Hy + j) + (long long)g.Py * (g.Hz + k))] =
+                in[c][i + (long long)nx[c] * (j + (long long)ny[c] * k)];
+        /* fill_halo_u!/v!/w! order */
+        if (c == 0) { host_fill(m, pad, 0, 2); host_fill(m, pad, 0, 0); host_fill(m, pad, 1, 0); host_fill(m, pad, 1, 1); host_fill(m, pad, 2, 0); host_fill(m, pad, 2, 1); }
+        if (c == 1) { host_fill(m, pad, 1, 2); host_fill(m, pad, 0, 0); host_fill(m, pad, 0, 1); host_fill(m, pad, 1, 0); host_fill(m, pad, 2, 0); host_fill(m, pad, 2, 1); }
+        if (c == 2) { host_fill(m, pad, 2, 2); host_fill(m, pad, 0, 0); host_fill(m, pad, 0, 1); host_fill(m, pad, 1, 0); host_fill(m, pad, 1, 1); host_fill(m, pad, 2, 0); }
+        HIPCHECK(hipMemcpy(m->vel[b][c], pad.data(), m->ncell * 4, hipMemcpyHostToDevice));
+    }
+    m->v1 = b;
+    m->vel_same = false;
+    return PI_OK;
+}
+
+pi_status pi_set_parf(pi_model* m, const float* parf)
+{
+    if (!m || !parf) return fail(PI_ERR_ARG, "null PARF");
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    HIPCHECK(hipMemcpy(m->PARF_in, parf, (size_t)m->g.Nx * m->g.Ny * 4, hipMemcpyHostToDevice));
+    return PI_OK;
+}
+
+pi_status pi_set_temperature(pi_model* m, const float* t)
+{
+    if (!m || !t) return fail(PI_ERR_ARG, "null temperature");
+    const DevGrid& g = m->g;
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    std::vector<float> pad(m->ncell, 0.0f);
+    for (int k = 0; k < g.Nz; ++k) for (int j = 0; j < g.Ny; ++j) for (int i = 0; i < g.Nx; ++i)
+        pad[(long long)(g.Hx + i) + (long long)g.Px * ((g.Hy + j) + (long long)g.Py * (g.Hz + k))] =
+            t[i + (long long)g.Nx * (j + (long long)g.Ny * k)];
+    HIPCHECK(hipMemcpy(m->temp, pad.data(), m->ncell * 4, hipMemcpyHostToDevice));
+    return PI_OK;
+}
+
+/* ------------------------------------------------------------------ step */
+static pi_status enqueue_step(pi_model* m, float dt)
+{
+    const DevGrid& g = m->g;
+    hipStream_t s = m->stream;
+    DevState* st = m->d_state;
+    const int pb = blocks_for(2 * m->ctot);
+    const int cb = blocks_for(m->ncell);
+    const int ib = blocks_for(m->nint);
+    m->tcount = 0;
+    /* vel½ (time_step.jl:17-19); equal to vel₀ bit for bit when vel₀ == vel₁ */
+    int vh = m->v0;
+    if (!m->vel_same) {
+        vh = 3 - m->v0 - m->v1;
+        if (vh < 0 || vh > 2 || vh == m->v0 || vh == m->v1) {
+            for (vh = 0; vh < 3 && (vh == m->v0 || vh == m->v1); ++vh) {}
+        }
+        tbegin(m, "velhalf");
+        for (int c = 0; c < 3; ++c) launch_velhalf(s, m->vel[m->v0][c], m->vel[m->v1][c], m->vel[vh][c], m->ncell, st);
+        tend(m);
+    }
+    const Vel V0{m->vel[m->v0][0], m->vel[m->v0][1], m->vel[m->v0][2]};
+    const Vel VH{m->vel[vh][0], m->vel[vh][1], m->vel[vh][2]};
+    const Vel V1{m->vel[m->v1][0], m->vel[m->v1][1], m->vel[m->v1][2]};
+    /* division marks + capacity check */
+    tbegin(m, "marks+scan");
+    launch_zero(s, m->bm, 2ll * m->nsp * m->nw, st);
+    launch_marks(s, pb, st, m->bm, m->bm + (size_t)m->nsp * m->nw, m->nw);
+    launch_scan(s, scan_bits(m->bm + (size_t)m->nsp * m->nw, m->bm, m->nsp, m->nw, m->cap), m->partials, m->G);
+    launch_capcheck(s, st, m->G, m->nsp, m->nw);
+    tend(m);
+    /* advection + diffusion + cells */
+    AdvArgs aa{};
+    aa.g = g; aa.st = st; aa.v0 = V0; aa.vh = VH; aa.v1 = V1; aa.dt = dt;
+    aa.sx = std::sqrt(m->bgc[PI_B_kappa_hP] * dt);
+    aa.sy = std::sqrt(m->bgc[PI_B_kappa_hP] * dt);
+    aa.sz = std::sqrt(m->bgc[PI_B_kappa_vP] * dt);
+    aa.seed = m->seed; aa.cnt = m->cnt;
+    tbegin(m, "adv");
+    launch_adv(s, pb, aa);
+    tend(m);
+    tbegin(m, "bin");
+    launch_scan(s, scan_plain(m->cnt, (uint32_t)m->ncell), m->partials, m->seg);
+    launch_place(s, pb, st, m->seg, m->cnt, m->pkey, m->psrc);
+    launch_cells(s, cb, st, m->ncell, m->seg, m->pkey, m->psrc, m->Chl, m->pop, m->big);
+    launch_bigcells(s, 256, st, m->big, m->seg, m->pkey, m->psrc, m->skey, m->ssrc, m->Chl);
+    tend(m);
+    tbegin(m, "par");
+    launch_par(s, blocks_for((long long)g.Nx * g.Ny), st, g, m->Chl, m->PARF_in, m->PARF_out, m->diagPAR,
+               m->bgc[PI_B_kc], m->bgc[PI_B_kw]);
+    tend(m);
+    /* physiology */
+    PhysArgs pa{};
+    pa.g = g; pa.st = st; pa.tab = m->tab; pa.cap = m->cap; pa.nsp = m->nsp; pa.params = m->d_params;
+    pa.seg = m->seg; pa.pkey = m->pkey; pa.psrc = m->psrc; pa.temp = m->temp; pa.pop = m->pop;
+    pa.bmA = m->bm; pa.bmD = m->bm + (size_t)m->nsp * m->nw; pa.G = m->G; pa.nw = m->nw;
+    pa.con = m->con; pa.diag_num = m->diag; pa.dt = dt; pa.seed = m->seed;
+    tbegin(m, "phys");
+    launch_phys(s, pb, pa);
+    tend(m);
+    PlkArgs ka{};
+    ka.g = g; ka.seg = m->seg; ka.pkey = m->pkey; ka.con = m->con; ka.diag = m->diag; ka.st = st;
+    for (int q = 0; q < PI_NTRACERS; ++q) ka.plk[q] = m->plk[q] ? m->plk[q] : m->plk[PI_DIC];
+    tbegin(m, "plk");
+    launch_plk(s, cb, ka);
+    tend(m);
+    /* tracers */
+    SweepArgs sa{};
+    sa.g = g; sa.st = st; sa.dt = dt;
+    sa.vel = V1.u; sa.dim = 0;
+    for (int q = 0; q < PI_NTRACERS; ++q) { sa.out[q] = m->T1[q]; sa.prev[q] = nullptr; }
+    tbegin(m, "sweep_x");
+    launch_sweep(s, ib, sa);
+    tend(m);
+    FillArgs fa{};
+    fa.g = g; fa.st = st; fa.nf = PI_NTRACERS; fa.dim = 1; fa.lines_full = 0;
+    for (int q = 0; q < PI_NTRACERS; ++q) fa.f[q] = m->T1[q];
+    tbegin(m, "fill");
+    launch_fill(s, blocks_for((long long)g.Nx * g.Nz * PI_NTRACERS), fa);
+    tend(m);
+    sa.vel = V1.v; sa.dim = 1;
+    for (int q = 0; q < PI_NTRACERS; ++q) { sa.out[q] = m->T2[q]; sa.prev[q] = m->T1[q]; }
+    tbegin(m, "sweep_y");
+    launch_sweep(s, ib, sa);
+    tend(m);
+    fa.dim = 2;
+    for (int q = 0; q < PI_NTRACERS; ++q) fa.f[q] = m->T2[q];
+    tbegin(m, "fill");
+    launch_fill(s, blocks_for((long long)g.Nx * g.Ny * PI_NTRACERS), fa);
+    tend(m);
+    FinalArgs fn{};
+    fn.g = g; fn.st = st; fn.w = V1.w; fn.dt = dt;
+    for (int q = 0; q < PI_NTRACERS; ++q) { fn.prev[q] = m->T2[q]; fn.plk[q] = m->plk[q]; }
+    fn.kh = m->bgc[PI_B_kappa_h]; fn.kv = m->bgc[PI_B_kappa_v];
+    fn.wsink_inorg = m->bgc[PI_B_w_sink_inorg]; fn.wsink_org = m->bgc[PI_B_w_sink_org];
+    fn.kDOC = m->bgc[PI_B_kDOC]; fn.kPOC = m->bgc[PI_B_kPOC]; fn.kDON = m->bgc[PI_B_kDON];
+    fn.kPON = m->bgc[PI_B_kPON]; fn.kDOP = m->bgc[PI_B_kDOP]; fn.kPOP = m->bgc[PI_B_kPOP]; fn.Nit = m->bgc[PI_B_Nit];
+    fn.lPOC = m->bgc[PI_B_lambda_POC]; fn.lmin = m->bgc[PI_B_lambda_min]; fn.ldust = m->bgc[PI_B_lambda_dust];
+    fn.DFeFrac = m->bgc[PI_B_DFeFrac]; fn.kdiss = m->bgc[PI_B_kdiss]; fn.lig = m->bgc[PI_B_ligand];
+    fn.lFe = m->bgc[PI_B_lambda_Fe];
+    tbegin(m, "final");
+    launch_final(s, ib, fn);
+    tend(m);
+    /* fill_halo_tracer! of the new tracers: x, then y, then z over whole planes */
+    tbegin(m, "fill");
+    for (int q = 0; q < PI_NTRACERS; ++q) fa.f[q] = nullptr;
+    fa.lines_full = 1;
+    fa.dim = 0; launch_fill(s, blocks_for((long long)g.Py * g.Pz * PI_NTRACERS), fa);
+    fa.dim = 1; launch_fill(s, blocks_for((long long)g.Px * g.Pz * PI_NTRACERS), fa);
+    fa.dim = 2; launch_fill(s, blocks_for((long long)g.Px * g.Py * PI_NTRACERS), fa);
+    tend(m);
+    for (size_t a = 0; a < m->diag_ids.size(); ++a) launch_accum(s, m->diag_tr[a], st, m->diag_ids[a], m->ncell);
+    launch_finish(s, st, m->seg, m->ncell, m->nsp);
+    HIPCHECK(hipGetLastError());
+    /* vel₀ <- vel₁ */
+    m->v0 = m->v1;
+    m->vel_same = true;
+    m->host_iter += 1;
+    m->last_dt = dt;
+    return PI_OK;
+}
+
+pi_status pi_step(pi_model* m, float dt)
+{
+    if (!m) return fail(PI_ERR_ARG, "null model");
+    pi_status s = rebuild_if_dirty(m);
+    if (s != PI_OK) return s;
+    if (m->h_state.err) return fail(PI_ERR_CAPACITY, "model is in an error state from an earlier step");
+    return enqueue_step(m, dt);
+}
+
+pi_status pi_run(pi_model* m, float dt, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        pi_status s = pi_step(m, dt);
+        if (s != PI_OK) return s;
+    }
+    return PI_OK;
+}
+
+pi_status pi_sync(pi_model* m)
+{
+    if (!m) return fail(PI_ERR_ARG, "null model");
+    return sync_state(m);
+}
+
+/* ------------------------------------------------------------- downloads */
+pi_status pi_download_field(pi_model* m, int32_t sp, int32_t field, void* host, int64_t n)
+{
+    if (!m || !host) return fail(PI_ERR_ARG, "null argument");
+    if (sp < 0 || sp >= m->nsp || field < 0 || field >= PI_NFIELDS) return fail(PI_ERR_ARG, "field id");
+    if (n != m->cap) return fail(PI_ERR_ARG, "n must equal capacity");
+    pi_status s = rebuild_if_dirty(m);
+    if (s != PI_OK) return s;
+    s = sync_state(m);
+    if (s != PI_OK && s != PI_ERR_CAPACITY) return s;
+    launch_store_to_table(m->stream, m->d_state, m->tab, m->cap, 2 * m->ctot);
+    const size_t es = (field >= PI_F_xi && field <= PI_F_zi) ? 8 : field == PI_F_ac ? 1 : 4;
+    void* tmp = (void*)m->skey;   /* scratch, 8*ctot bytes >= 8*cap */
+    launch_table_field(m->stream, m->d_state, m->tab, sp, m->cap, field, m->d_params, m->g, tmp);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(host, tmp, es * m->cap, hipMemcpyDeviceToHost, m->stream));
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    return s;
+}
+
+pi_status pi_download_tracer(pi_model* m, int32_t q, float* c)
+{
+    if (!m || !c || q < 0 || q >= PI_NTRACERS) return fail(PI_ERR_ARG, "bad tracer id");
+    pi_status s = sync_state(m);
+    if (s != PI_OK && s != PI_ERR_CAPACITY) return s;
+    HIPCHECK(hipMemcpy(c, m->h_state.tr[m->h_state.tcur][q], m->ncell * 4, hipMemcpyDeviceToHost));
+    return s;
+}
+
+pi_status pi_download_grid(pi_model* m, int32_t which, float* h, int64_t n)
+{
+    if (!m || !h) return fail(PI_ERR_ARG, "null argument");
+    pi_status s = sync_state(m);
+    if (s != PI_OK && s != PI_ERR_CAPACITY) return s;
+    const float* src = nullptr;
+    long long cnt = m->ncell;
+    switch (which) {
+        case PI_G_par: case PI_G_par0: src = m->h_state.par[m->h_state.pcur]; break;
+        case PI_G_Chl: src = m->Chl; break;
+        case PI_G_pop: src = m->pop; break;
+        case PI_G_temp: src = m->temp; break;
+        case PI_G_PARF: src = m->PARF_out; cnt = (long long)m->g.Nx * m->g.Ny; break;
+        case PI_G_u: src = m->vel[m->v1][0]; break;
+        case PI_G_v: src = m->vel[m->v1][1]; break;
+        case PI_G_w: src = m->vel[m->v1][2]; break;
+        case PI_G_diag_PAR: src = m->diagPAR; break;
+        default:
+            if (which >= PI_G_plk_DIC && which < PI_G_plk_DIC + PI_NTRACERS) {
+                src = m->plk[which - PI_G_plk_DIC];
+                if (!src) {
+                    if (n != cnt) return fail(PI_ERR_ARG, "size mismatch");
+                    memset(h, 0, cnt * 4);
+                    return s;
+                }
+            }
+    }
+    if (!src) return fail(PI_ERR_ARG, "unknown grid field");
+    if (n != cnt) return fail(PI_ERR_ARG, "size mismatch");
+    HIPCHECK(hipMemcpy(h, src, cnt * 4, hipMemcpyDeviceToHost));
+    return s;
+}
+
+pi_status pi_download_diag(pi_model* m, int32_t sp, int32_t which, float* h)
+{
+    if (!m || !h || sp < 0 || sp >= m->nsp || which < 0 || which >= PI_NDIAG) return fail(PI_ERR_ARG, "diag id");
+    pi_status s = sync_state(m);
+    if (s != PI_OK && s != PI_ERR_CAPACITY) return s;
+    HIPCHECK(hipMemcpy(h, m->diag + ((long long)sp * 4 + which) * m->ncell, m->ncell * 4, hipMemcpyDeviceToHost));
+    return s;
+}
+
+pi_status pi_download_diag_tracer(pi_model* m, int32_t q, float* h)
+{
+    if (!m || !h) return fail(PI_ERR_ARG, "null argument");
+    pi_status s = sync_state(m);
+    if (s != PI_OK && s != PI_ERR_CAPACITY) return s;
+    for (size_t a = 0; a < m->diag_ids.size(); ++a)
+        if (m->diag_ids[a] == q) { HIPCHECK(hipMemcpy(h, m->diag_tr[a], m->ncell * 4, hipMemcpyDeviceToHost)); return s; }
+    return fail(PI_ERR_ARG, "tracer not diagnosed");
+}
+
+pi_status pi_reset_diags(pi_model* m)
+{
+    if (!m) return fail(PI_ERR_ARG, "null model");
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    HIPCHECK(hipMemset(m->diagPAR, 0, m->ncell * 4));
+    HIPCHECK(hipMemset(m->diag, 0, (size_t)m->nsp * 4 * m->ncell * 4));
+    for (float* p : m->diag_tr) HIPCHECK(hipMemset(p, 0, m->ncell * 4));
+    return PI_OK;
+}
+
+pi_status pi_get_iteration(pi_model* m, int64_t* it)
+{
+    if (!m || !it) return fail(PI_ERR_ARG, "null argument");
+    *it = m->host_iter;
+    return PI_OK;
+}
+
+pi_status pi_count_active(pi_model* m, int32_t sp, int64_t* n)
+{
+    if (!m || !n || sp < 0 || sp >= m->nsp) return fail(PI_ERR_ARG, "bad argument");
+    pi_status s = rebuild_if_dirty(m);
+    if (s != PI_OK) return s;
+    HIPCHECK(hipMemsetAsync(m->d_count, 0, 8, m->stream));
+    launch_count(m->stream, m->d_state, sp, m->d_count, 2 * m->ctot);
+    unsigned long long c = 0;
+    HIPCHECK(hipMemcpyAsync(&c, m->d_count, 8, hipMemcpyDeviceToHost, m->stream));
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    *n = (int64_t)c;
+    return PI_OK;
+}
+
+pi_status pi_enable_timing(pi_model* m, int32_t on)
+{
+    if (!m) return fail(PI_ERR_ARG, "null model");
+    m->timing = on != 0;
+    return PI_OK;
+}
+
+pi_status pi_kernel_times(pi_model* m, int32_t* n, const char** names, float* ms, int32_t cap)
+{
+    if (!m || !n) return fail(PI_ERR_ARG, "null argument");
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    int32_t k = 0;
+    for (size_t i = 0; i < m->tcount && k < cap; ++i, ++k) {
+        float t = 0.0f;
+        HIPCHECK(hipEventElapsedTime(&t, m->timers[i].a, m->timers[i].b));
+        names[k] = m->timers[i].name;
+        ms[k] = t;
+    }
+    *n = k;
+    return PI_OK;
+}
+
+}  /* extern "C" */

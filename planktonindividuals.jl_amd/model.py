@@ -1,0 +1,274 @@
+"""PlanktonModel on the MI355X architecture (reference: src/Model/models.jl,
+src/Architectures.jl, src/Individuals/Individuals.jl,
+src/Individuals/Plankton/QuotaMode/plankton_generation.jl)."""
+import numpy as np
+
+from . import _abi, params as P, stream
+from .fields import default_tracer_init, generate_tracers
+from ._native import NativeModel
+
+
+# --------------------------------------------------------------- architectures
+class Architecture:
+    """abstract type Architecture (src/Architectures.jl:16)."""
+
+
+class HIPArch(Architecture):
+    """Run the per-timestep hot path on one AMD Instinct MI355X (gfx950) through
+    libpi_hip.so.  The MI355X counterpart of `GPU()` (src/Architectures.jl:24)."""
+
+    def __init__(self, device=0):
+        self.device = device
+
+    def __repr__(self):
+        return f"HIPArch({self.device})"
+
+
+GPU = HIPArch
+
+
+def isfunctional(arch):
+    """isfunctional(arch) (src/Architectures.jl:34): the HIP library loads and a device opens."""
+    if not isinstance(arch, HIPArch):
+        return False
+    from ._native import load_library
+    lib = load_library()
+    return lib.pi_init(arch.device) == 0
+
+
+class phyto_setup:
+    """phyto_setup(params, N, Nsp) (src/PlanktonIndividuals.jl:112-116)."""
+
+    def __init__(self, params=None, N=(2 ** 10,), Nsp=1):
+        self.params = params
+        self.N = list(N)
+        self.Nsp = Nsp
+
+
+# ------------------------------------------------------ initial individuals
+def _mask(ac, v):
+    """Bool * x  ==  ifelse(b, x, copysign(0, x))."""
+    return np.where(ac, v, np.copysign(np.float32(0), v)).astype(np.float32)
+
+
+def initialize_plankton(p, N, cap, grid, seed, sp):
+    """initialize_plankton! (QuotaMode/plankton_generation.jl:35-72) with the draws of
+    `randn!`/`rand!` taken from the shared stream (the reference's generator is
+    entropy-seeded).  `p` is the species' 40-float parameter row."""
+    f32 = np.float32
+    q = {n: p[i] for i, n in enumerate(_abi.PARAM_NAMES)}
+    slots = np.arange(cap, dtype=np.int64)
+    ux, uy, uz = stream.uniform3(seed, 0, sp, stream.KIND_INIT_POS, slots)
+    uc, un, up = stream.uniform3(seed, 0, sp, stream.KIND_INIT_QUOTA, slots)
+    a, b, _ = stream.uniform3(seed, 0, sp, stream.KIND_INIT_SIZE, slots)
+    # randn via Box-Muller on (0,1] x [0,1)
+    a64 = 1.0 - a.astype(np.float64)
+    Sz = (np.sqrt(-2.0 * np.log(a64)) * np.cos(2.0 * np.pi * b.astype(np.float64))).astype(np.float32)
+    ac = np.zeros(cap, dtype=bool)
+    ac[:N] = True
+    gen = np.zeros(cap, f32)
+    gen[:N] = f32(1.0)
+    age = np.zeros(cap, f32)
+    x = _mask(ac, ux * f32(grid.Nx))
+    y = _mask(ac, uy * f32(grid.Ny))
+    z = _mask(ac, uz * f32(grid.Nz))
+    Sz = _mask(ac, np.maximum(f32(1.0), Sz * q["var"] + q["mean"]))
+    Bm = (q["Cquota"] * Sz) * q["Nsuper"]
+    Cq = (uc * q["Cqmax"]) * Bm
+    Nq = un * (q["Nqmax"] * (Bm + Cq) - Bm * q["R_NC"])
+    Pq = up * (q["Pqmax"] * (Bm + Cq) - Bm * q["R_PC"])
+    Chl = Bm * q["Chl2Cint"]
+    # mask_individuals! over the first N slots (Individuals/utils.jl:89-100)
+    Hx, Hy, Hz = grid.Hx, grid.Hy, grid.Hz
+    n = slice(0, N)
+    xi = np.trunc(_mask(ac[n], x[n] + f32(1))).astype(np.int64) + Hx
+    yi = np.trunc(_mask(ac[n], y[n] + f32(1))).astype(np.int64) + Hy
+    zi = np.trunc(_mask(ac[n], z[n] + f32(1))).astype(np.int64) + Hz
+    ac[n] = grid.landmask[xi - 1, yi - 1, zi - 1] & ac[n]
+    zeros = np.zeros(cap, f32)
+    return dict(x=x, y=y, z=z, Sz=Sz, Bm=Bm.astype(f32), Cq=Cq.astype(f32), Nq=Nq.astype(f32),
+                Pq=Pq.astype(f32), Chl=Chl.astype(f32), gen=gen, age=age, graz=zeros.copy(),
+                mort=zeros.copy(), dvid=zeros.copy(), xi=np.zeros(cap, np.int64),
+                yi=np.zeros(cap, np.int64), zi=np.zeros(cap, np.int64), ac=ac)
+
+
+# ------------------------------------------------------------------- views
+class _SpeciesData:
+    """`model.individuals.phytos.spN.data`: attribute access downloads one column."""
+
+    def __init__(self, model, sp):
+        self._model, self._sp = model, sp
+
+    def __getattr__(self, name):
+        if name.startswith("_"):
+            raise AttributeError(name)
+        if name == "idx":
+            return np.zeros(self._model.max_individuals, np.int64)
+        return self._model._native.field(self._sp, name)
+
+
+class _Species:
+    def __init__(self, model, sp):
+        self.data = _SpeciesData(model, sp)
+        self.p = dict(zip(_abi.PARAM_NAMES, model.species_params[sp]))
+
+
+class _Phytos(dict):
+    def __getattr__(self, k):
+        try:
+            return self[k]
+        except KeyError:
+            raise AttributeError(k)
+
+
+class _Individuals:
+    def __init__(self, model):
+        self.phytos = _Phytos({f"sp{s + 1}": _Species(model, s) for s in range(model.n_species)})
+        self.abiotics = {}
+
+
+class _Field:
+    def __init__(self, model, tid):
+        self._m, self._t = model, tid
+
+    @property
+    def data(self):
+        return self._m._native.tracer(self._t)
+
+
+class _Tracers(dict):
+    def __getattr__(self, k):
+        try:
+            return self[k]
+        except KeyError:
+            raise AttributeError(k)
+
+
+class _Timestepper:
+    """Read access to the scratch fields of `model.timestepper` (Model/timestepper.jl:1-21)."""
+
+    def __init__(self, model):
+        self._m = model
+
+    def __getattr__(self, name):
+        if name.startswith("_"):
+            raise AttributeError(name)
+        if name == "plk":
+            return _Tracers({n: type("F", (), {"data": self._m._native.grid_field("plk_" + n)})()
+                             for n in _abi.TRACER_NAMES})
+        key = {"par₀": "par0"}.get(name, name)
+        return self._m._native.grid_field(key)
+
+
+# ------------------------------------------------------------------- model
+class ModelSpec:
+    """Everything `PlanktonModel(arch, grid; ...)` decides on the host before any
+    device work: validated parameters (models.jl:65-97) and the initial state
+    (generate_individuals, Individuals.jl:42-72; generate_tracers,
+    tracer_fields.jl:93-134).  Pure numpy; any implementation of the ABI can be
+    loaded from it."""
+
+    def __init__(self, grid, *, FT=np.float32, mode=None, max_individuals=8 * 1024,
+                 bgc_params=None, tracer_initial=None, phyto=None, abiotic=None, t=0.0,
+                 max_candidates=25, seed=0):
+        mode = mode if mode is not None else P.QuotaMode()
+        if not isinstance(mode, P.QuotaMode):
+            raise NotImplementedError(f"{mode}: only QuotaMode is implemented on the MI355X hot path")
+        if abiotic is not None:
+            raise NotImplementedError("abiotic particles are outside the MI355X hot path")
+        if FT is not np.float32:
+            raise ValueError("FT must be Float32")
+        if bgc_params is None:
+            bgc = P.update_bgc_params(P.bgc_params_default(FT), FT)
+        elif isinstance(bgc_params, dict):
+            bgc = P.update_bgc_params(bgc_params, FT)
+        else:
+            raise ValueError("Phytoplankton parameters must be either Nothing or Dict!")
+        if phyto is None:
+            phyto = phyto_setup(None, [2 ** 10], 1)
+        if not isinstance(phyto, phyto_setup):
+            raise ValueError("PlanktonModel:`phyto` must be either Nothing or `phyto_setup`!")
+        if max(phyto.N) > max_individuals:
+            raise AssertionError("maximum(phyto.N) ≤ max_individuals")
+        if len(phyto.N) != phyto.Nsp:
+            raise ValueError(f"PlanktonModel: `phyto`: The length of `N` must be {phyto.Nsp}, the same as `Nsp`, "
+                             "each species has its own initial condition")
+        if phyto.params is None:
+            pp = P.update_phyt_params(P.phyt_params_default(phyto.Nsp, mode), FT, N=phyto.Nsp, mode=mode)
+        elif isinstance(phyto.params, dict):
+            pp = P.update_phyt_params(phyto.params, FT, N=phyto.Nsp, mode=mode)
+        else:
+            raise ValueError("Phytoplankton parameters must be either Nothing or Dict!")
+        self.grid, self.mode, self.FT = grid, mode, FT
+        self.max_individuals = int(max_individuals)
+        self.max_candidates = max_candidates
+        self.bgc_params = bgc
+        self.phyt_params = pp
+        self.n_species = phyto.Nsp
+        self.N = list(phyto.N)
+        self.species_params = P.species_param_matrix(pp, phyto.Nsp)
+        self.bgc_vector = P.bgc_vector(bgc)
+        self.seed = int(seed)
+        self.t = np.float32(t)
+        self.initial_individuals = [initialize_plankton(self.species_params[s], int(phyto.N[s]),
+                                                        self.max_individuals, grid, self.seed, s)
+                                    for s in range(self.n_species)]
+        self.initial_tracers = generate_tracers(grid, tracer_initial if tracer_initial is not None
+                                                else default_tracer_init(), self.seed)
+
+
+class PlanktonModel:
+    """PlanktonModel(arch, grid; FT=Float32, mode=QuotaMode(), max_individuals=8*1024,
+    bgc_params=nothing, tracer_initial=default_tracer_init(), phyto=nothing,
+    abiotic=nothing, t=0.0f0, max_candidates=25) — Model/models.jl:51-155.
+
+    Extra keyword: `seed`, the key of the shared random stream (initial state and
+    every per-step draw).  `spec=` builds from an existing ModelSpec instead.
+    """
+
+    def __init__(self, arch, grid=None, *, spec=None, **kw):
+        if not isinstance(arch, HIPArch):
+            raise ValueError(f"{arch}: this package implements the HIPArch architecture only")
+        if spec is None:
+            spec = ModelSpec(grid, **kw)
+        self.spec = spec
+        self.arch, self.grid, self.mode, self.FT = arch, spec.grid, spec.mode, spec.FT
+        self.max_individuals = spec.max_individuals
+        self.max_candidates = spec.max_candidates
+        self.bgc_params = spec.bgc_params
+        self.phyt_params = spec.phyt_params
+        self.n_species = spec.n_species
+        self.species_params = spec.species_params
+        self.t = spec.t
+        self._native = NativeModel(self.grid, self.n_species, self.max_individuals, self.species_params,
+                                   spec.bgc_vector, spec.seed, (), device=arch.device)
+        for s in range(self.n_species):
+            self._native.upload_individuals(s, spec.initial_individuals[s])
+        for tid, c in enumerate(spec.initial_tracers):
+            self._native.upload_tracer(tid, c)
+        self.individuals = _Individuals(self)
+        self.tracers = _Tracers({n: _Field(self, i) for i, n in enumerate(_abi.TRACER_NAMES)})
+        self.timestepper = _Timestepper(self)
+
+    @property
+    def iteration(self):
+        return self._native.iteration()
+
+    def calc_active_individuals(self):
+        return [self._native.count_active(s) for s in range(self.n_species)]
+
+    def __repr__(self):
+        N = self.calc_active_individuals()
+        return ("PlanktonModel:\n"
+                f"├── floating point data type: Float32\n"
+                f"├── grid: {self.grid!r}\n"
+                f"├── {self.mode} is selected for phytoplankton physiology\n"
+                f"├── phytoplankton: {self.n_species} species with {N} individuals for each species\n"
+                "├── No abiotic particles available\n"
+                f"└── maximum number of individuals: {self.max_individuals} per species")
+
+
+def TimeStep(model, ΔT, diags=None):
+    """TimeStep!(model, ΔT, diags) — Model/time_step.jl:12-180, one device step."""
+    model._native.step(np.float32(ΔT))
+    model.t = np.float32(np.float32(model.iteration) * np.float32(ΔT))

@@ -1,0 +1,113 @@
+"""Parameter sets (reference: src/Parameters/param_default.jl, param_update.jl).
+
+Values are the reference's Float64 literals; conversion to Float32 happens where
+the reference converts (`Dict{String, FT}` for bgc parameters, `FT.(tmp)` when a
+species' NamedTuple is built, QuotaMode/plankton_generation.jl:22-31).
+"""
+import numpy as np
+
+from ._abi import PARAM_NAMES, BGC_NAMES
+
+
+class QuotaMode:
+    """QuotaMode <: AbstractMode (src/PlanktonIndividuals.jl:73)."""
+
+    def __repr__(self):
+        return "QuotaMode()"
+
+
+class CarbonMode:
+    def __repr__(self):
+        return "CarbonMode()"
+
+
+class MacroMolecularMode:
+    def __repr__(self):
+        return "MacroMolecularMode()"
+
+
+class IronEnergyMode:
+    def __repr__(self):
+        return "IronEnergyMode()"
+
+
+def bgc_params_default(FT=np.float32):
+    """bgc_params_default(FT) — param_default.jl:45-71."""
+    d = {
+        "kw": 0.046, "kc": 0.04,
+        "kDOC": 1 / 30 / 86400, "Nit": 1 / 30 / 86400, "kDON": 1 / 30 / 86400,
+        "kDOP": 1 / 30 / 86400, "kPOC": 1 / 30 / 86400, "kPON": 1 / 30 / 86400,
+        "kPOP": 1 / 30 / 86400,
+        "κh": 0.0e-6, "κv": 0.0e-6, "κhP": 0.0e-6, "κvP": 0.0e-6,
+        "shared_graz": 1.0,
+        "w_sink_org": 2.31e-8, "w_sink_inorg": 2.31e-5,
+        "lambda_POC": 5.8e-8, "lambda_min": 3.5e-10, "kdiss": 4.4e-7,
+        "lambda_dust": 1.7e-3, "lambda_Fe": 1.7, "ligand": 6.0e-4, "DFeFrac": 0.01,
+    }
+    return {k: FT(v) for k, v in d.items()}
+
+
+def phyt_params_default(N=1, mode=None):
+    """phyt_params_default(N, ::QuotaMode) — param_default.jl:234-284 (Float64 vectors)."""
+    if mode is not None and not isinstance(mode, QuotaMode):
+        raise NotImplementedError(f"{mode} physiology is not part of the MI355X hot path")
+    params = {
+        "Nsuper": [1], "Cquota": [1.8e-11], "mean": [1.2], "var": [0.3], "Chl2Cint": [0.10],
+        "α": [2.0e-2], "Φ": [4.0e-5], "Topt": [27.0], "Tmax": [30.0], "Ea": [5.3e4],
+        "PCmax": [4.2e-5], "VDOCmax": [0.0], "VNH4max": [6.9e-6], "VNO3max": [6.9e-6],
+        "VPO4max": [1.2e-6], "KsatNH4": [0.005], "KsatNO3": [0.010], "KsatPO4": [0.003],
+        "KsatDOC": [0.0], "Nqmax": [0.25], "Pqmax": [0.02], "Cqmax": [0.4], "k_mtb": [3.5e-5],
+        "respir": [1.2e-6], "Chl2N": [3.0], "R_NC": [16 / 106], "R_PC": [1 / 106],
+        "grz_P": [0.0], "dvid_P": [1e-4], "dvid_type": [1], "dvid_reg": [2.5],
+        "dvid_reg2": [12.0], "mort_P": [5e-5], "mort_reg": [0.5],
+        "grazFracC": [0.7], "grazFracN": [0.7], "grazFracP": [0.7],
+        "mortFracC": [0.5], "mortFracN": [0.5], "mortFracP": [0.5],
+    }
+    if N == 1:
+        return params
+    return generate_n_species_params(N, params)
+
+
+def generate_n_species_params(N, params):
+    """param_default.jl:345-352: replicate species 1's values N times."""
+    return {k: [v[0]] * N for k, v in params.items()}
+
+
+def update_bgc_params(tmp, FT=np.float32):
+    """update_bgc_params(tmp::Dict, FT) — param_update.jl:10-22."""
+    parameters = bgc_params_default(FT)
+    for key, val in tmp.items():
+        if key not in parameters:
+            raise ValueError(f"PARAM: bgc parameter not found {key}")
+        parameters[key] = FT(val)
+    return parameters
+
+
+def update_phyt_params(tmp, FT=np.float32, *, N=1, mode=None):
+    """update_phyt_params(tmp::Dict, FT; N, mode) — param_update.jl:34-46."""
+    parameters = phyt_params_default(N, mode if mode is not None else QuotaMode())
+    for key, val in tmp.items():
+        if key not in parameters:
+            raise ValueError(f"PARAM: phyt parameter not found {key}")
+        v = list(np.atleast_1d(np.asarray(val, dtype=np.float64)))
+        parameters[key] = [FT(x) for x in v]
+    return parameters
+
+
+def species_param_matrix(phyt_params, nsp):
+    """(nsp, 40) float32 in param_names order: `p = NamedTuple{param_names}(FT.(tmp))`."""
+    out = np.zeros((nsp, len(PARAM_NAMES)), dtype=np.float32)
+    for s in range(nsp):
+        for q, name in enumerate(PARAM_NAMES):
+            if name not in phyt_params:
+                raise ValueError(f"PARAM: parameter not found {name}")
+            vals = phyt_params[name]
+            out[s, q] = np.float32(np.float64(vals[s]))
+    dt = out[:, PARAM_NAMES.index("dvid_type")]
+    if np.any((dt < 1) | (dt > 3)):
+        raise ValueError("Wrong cell division type, must be in 1 to 3")
+    return out
+
+
+def bgc_vector(bgc):
+    return np.array([bgc[k] for k in BGC_NAMES], dtype=np.float32)

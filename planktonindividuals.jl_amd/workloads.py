@@ -1,0 +1,108 @@
+"""Synthetic workloads used by the benchmark and the parity tests.
+
+  cellular_flow   2-D steady cellular flow from a streamfunction on cell
+                  corners, u = -δψ/Δy, v = δψ/Δx (discretely divergence-free;
+                  the pattern of the reference's examples/helper_functions.jl:38-59).
+  fourier_flow    3-D field from a sum of random Fourier modes with a k^-5/3
+                  energy spectrum: a rotational part (streamfunction on
+                  corners) plus a divergent part whose vertical structure has
+                  zero column mean, w from continuity (zero at the top and
+                  bottom faces), rescaled to a target rms speed.
+"""
+import numpy as np
+
+from .grids import RectilinearGrid, Periodic, Bounded
+from .model import ModelSpec, phyto_setup
+
+
+def cellular_flow(grid, U=0.1, L=16e3):
+    """ψ = (U L/π) sin(πx/L) sin(πy/L) at corners; returns u (Nx,Ny,Nz), v, w (Nx,Ny,Nz+1)."""
+    Nx, Ny, Nz = grid.Nx, grid.Ny, grid.Nz
+    dx, dy = float(grid.Δx), float(grid.Δy)
+    xc = np.arange(Nx + 1) * dx
+    yc = np.arange(Ny + 1) * dy
+    psi = (U * L / np.pi) * np.sin(np.pi * xc[:, None] / L) * np.sin(np.pi * yc[None, :] / L)
+    u2 = -(psi[:Nx, 1:] - psi[:Nx, :-1]) / dy
+    v2 = (psi[1:, :Ny] - psi[:-1, :Ny]) / dx
+    u = np.repeat(u2[:, :, None], Nz, axis=2).astype(np.float32)
+    v = np.repeat(v2[:, :, None], Nz, axis=2).astype(np.float32)
+    w = np.zeros((Nx, Ny, Nz + 1), np.float32)
+    return np.asfortranarray(u), np.asfortranarray(v), np.asfortranarray(w)
+
+
+def fourier_flow(grid, nmodes=64, rms=0.05, seed=0, kmax=None):
+    """Periodic-in-x/y, bounded-in-z synthetic velocity (see module doc)."""
+    rng = np.random.default_rng(seed)
+    Nx, Ny, Nz = grid.Nx, grid.Ny, grid.Nz
+    dx, dy = float(grid.Δx), float(grid.Δy)
+    dz = grid.dzF[grid.Hz:grid.Hz + Nz].astype(np.float64)
+    Lx, Ly = Nx * dx, Ny * dy
+    kmax = kmax or max(2, min(Nx, Ny) // 4)
+    xf = np.arange(Nx + 1) * dx             # x faces / corners
+    yf = np.arange(Ny + 1) * dy
+    xc = (np.arange(Nx) + 0.5) * dx
+    yc = (np.arange(Ny) + 0.5) * dy
+    kk = (np.arange(Nz) + 0.5) / Nz           # normalised depth of cell centres
+    psi = np.zeros((Nx + 1, Ny + 1, Nz))
+    chi = np.zeros((Nx, Ny, Nz))
+    for m in range(nmodes):
+        while True:
+            nx_, ny_ = rng.integers(-kmax, kmax + 1, size=2)
+            if nx_ or ny_:
+                break
+        kx, ky = 2 * np.pi * nx_ / Lx, 2 * np.pi * ny_ / Ly
+        kmag = np.hypot(kx, ky)
+        amp = np.sqrt(np.hypot(nx_, ny_) ** (-5.0 / 3.0)) / kmag
+        ph = rng.uniform(0, 2 * np.pi)
+        nz_ = rng.integers(0, 3)
+        vert = np.cos(np.pi * nz_ * kk)
+        psi += amp * np.cos(kx * xf[:, None, None] + ky * yf[None, :, None] + ph) * vert[None, None, :]
+        vdiv = np.cos(2 * np.pi * (1 + (m % 2)) * kk)         # zero column mean
+        ph2 = rng.uniform(0, 2 * np.pi)
+        chi += 0.3 * amp * np.cos(kx * xc[:, None, None] + ky * yc[None, :, None] + ph2) * vdiv[None, None, :]
+    # C-grid: u on x faces (i = 0..Nx-1, periodic), v on y faces
+    u = -(psi[:Nx, 1:, :] - psi[:Nx, :-1, :]) / dy
+    v = (psi[1:, :Ny, :] - psi[:-1, :Ny, :]) / dx
+    u += (chi - np.roll(chi, 1, axis=0)) / dx
+    v += (chi - np.roll(chi, 1, axis=1)) / dy
+    s = rms / np.sqrt(np.mean(u ** 2 + v ** 2))
+    u *= s
+    v *= s
+    div = (np.roll(u, -1, axis=0) - u) / dx + (np.roll(v, -1, axis=1) - v) / dy
+    # w positive upward on z faces k = 0 (top) .. Nz (bottom); bottom = 0, integrate upward
+    w = np.zeros((Nx, Ny, Nz + 1))
+    for k in range(Nz - 1, -1, -1):
+        w[:, :, k] = w[:, :, k + 1] - dz[k] * div[:, :, k]
+    w[:, :, 0] = 0.0
+    f = lambda a: np.asfortranarray(a.astype(np.float32))
+    return f(u), f(v), f(w)
+
+
+# ------------------------------------------------------------- configurations
+def config_2d(N=32768, n=128, cap=None, seed=1, overrides=None, bgc=None):
+    """Plain 2-D horizontal run: 128x128x1 periodic, Δ = 1 km, one species."""
+    grid = RectilinearGrid(size=(n, n, 1), x=(0, n * 1000), y=(0, n * 1000), z=(0, -10),
+                           topology=(Periodic, Periodic, Bounded))
+    params = dict(overrides or {})
+    spec = ModelSpec(grid, max_individuals=cap or 2 * N, phyto=phyto_setup(params or None, [N], 1),
+                     bgc_params=bgc, seed=seed)
+    return spec, cellular_flow(grid, U=0.1, L=16e3 * n / 128)
+
+
+# Overrides that make grazing, death and division happen within a short run
+# (the defaults give no death above Sz = 0.5 and no grazing).
+ACTIVE_PHYSIOLOGY = {"mort_reg": [1.0], "grz_P": [1.0e-6], "dvid_reg": [1.8]}
+
+
+def config_3d(n=256, nz=64, N=4194304, nsp=2, cap=None, seed=2, nmodes=64, overrides=None,
+              dz=10.0, kappa=(1e-2, 1e-5)):
+    """Plain 3-D run: n x n x nz, periodic x/y, bounded z, nsp species, Fourier flow."""
+    grid = RectilinearGrid(size=(n, n, nz), x=(0, n * 1000), y=(0, n * 1000), z=(0, -nz * dz),
+                           topology=(Periodic, Periodic, Bounded))
+    ov = dict(ACTIVE_PHYSIOLOGY)
+    ov.update(overrides or {})
+    params = {k: v * nsp if len(v) == 1 else v for k, v in ov.items()}
+    bgc = {"κh": kappa[0], "κv": kappa[1], "κhP": kappa[0], "κvP": kappa[1]}
+    spec = ModelSpec(grid, max_individuals=cap or 2 * N, phyto=phyto_setup(params, [N] * nsp, nsp),
+                     bgc_params=bgc, seed=seed)
+    return spec, fourier_flow(grid, nmodes=nmodes, rms=0.05, seed=seed)

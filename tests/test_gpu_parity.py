@@ -1,0 +1,104 @@
+"""GPU parity: libpi_hip.so against the oracle (FP32 reference order) on the same
+ModelSpec and inputs.  Bar: every individual field, every tracer, par/Chl/pop and
+the per-species diagnostics identical (up to the sign of zero / NaN payload)."""
+import numpy as np
+import pytest
+
+from parity import compare_models
+
+pytestmark = pytest.mark.gpu
+
+
+def _drive(pi, orc, spec, uvw, dt, n, parf_dt=600.0, seed=3):
+    g = spec.grid
+    rng = np.random.default_rng(seed)
+    nt = int(np.ceil(n * dt / parf_dt)) + 1
+    PARF = (400.0 + 300.0 * rng.random((g.Nx, g.Ny, nt))).astype(np.float32)
+    temp = pi.default_temperature(g, dt, n).astype(np.float32)
+    u, v, w = uvw
+    gm = pi.PlanktonModel(pi.HIPArch(), spec=spec)
+    sim = pi.PlanktonSimulation(gm, ΔT=dt, iterations=n, PARF=PARF, temp=temp, ΔT_PAR=parf_dt,
+                                vels=dict(u=u[..., None], v=v[..., None], w=w[..., None]), ΔT_vel=dt * 1e6)
+    pi.update(sim)
+    om = orc.OracleModel(spec)
+    idx = lambda t, s: int(np.floor(np.float32(t) / np.float32(s)))
+    om.update(dt, n, vel=(u, v, w), parf_fn=lambda t: PARF[:, :, idx(t, parf_dt)],
+              temp_fn=lambda t: temp[:, :, :, idx(t, 3600.0)])
+    return gm, om
+
+
+def test_parity_2d_division_death(pi, orc):
+    g = pi.RectilinearGrid(size=(32, 32, 1), x=(0, 32e3), y=(0, 32e3), z=(0, -10))
+    spec = pi.ModelSpec(g, max_individuals=6000, seed=11,
+                        phyto=pi.phyto_setup({"dvid_P": [2e-3], "mort_reg": [1.0], "mort_P": [1e-3],
+                                              "grz_P": [2e-4], "dvid_reg": [1.5]}, [3000], 1),
+                        bgc_params={"κhP": 1.0, "κvP": 1e-4, "κh": 50.0, "κv": 1e-4})
+    uvw = pi.workloads.cellular_flow(g, U=2.0, L=8e3)
+    gm, om = _drive(pi, orc, spec, uvw, 60.0, 24)
+    n_gpu = gm._native.count_active(0)
+    assert n_gpu == om.count_active(0)
+    bad = compare_models(gm._native, om, 1)
+    assert not bad, "\n".join(bad[:20])
+
+
+def test_parity_3d_two_species(pi, orc):
+    spec, uvw = pi.workloads.config_3d(n=16, nz=8, N=1500, nsp=2, cap=4000, seed=5, nmodes=12,
+                                       overrides={"dvid_P": [1e-3], "mort_P": [5e-4]}, kappa=(1.0, 1e-4))
+    u, v, w = uvw
+    gm, om = _drive(pi, orc, spec, (u * 20, v * 20, w * 20), 60.0, 16)
+    bad = compare_models(gm._native, om, 2)
+    assert not bad, "\n".join(bad[:20])
+
+
+def test_parity_bounded_box(pi, orc):
+    g = pi.RectilinearGrid(size=(12, 10, 4), x=(0, 12e3), y=(0, 10e3), z=[0.0, -2.0, -5.0, -9.0, -14.0],
+                           topology=(pi.Bounded, pi.Bounded, pi.Bounded))
+    spec = pi.ModelSpec(g, max_individuals=1200, seed=21,
+                        phyto=pi.phyto_setup({"dvid_P": [2e-3], "mort_reg": [1.0]}, [600], 1),
+                        bgc_params={"κhP": 5.0, "κvP": 1e-3, "κh": 20.0, "κv": 1e-3})
+    rng = np.random.default_rng(4)
+    u = (rng.standard_normal((13, 10, 4)) * 3).astype(np.float32)
+    v = (rng.standard_normal((12, 11, 4)) * 3).astype(np.float32)
+    w = (rng.standard_normal((12, 10, 5)) * 1e-3).astype(np.float32)
+    gm, om = _drive(pi, orc, spec, (u, v, w), 60.0, 12)
+    bad = compare_models(gm._native, om, 1)
+    assert not bad, "\n".join(bad[:20])
+
+
+def test_parity_crowded_cells(pi, orc):
+    """Hundreds of individuals per cell: the long-segment sort path."""
+    g = pi.RectilinearGrid(size=(4, 4, 2), x=(0, 4e3), y=(0, 4e3), z=(0, -20))
+    spec = pi.ModelSpec(g, max_individuals=12000, seed=8,
+                        phyto=pi.phyto_setup({"dvid_P": [1e-3], "grz_P": [1e-6]}, [8000], 1),
+                        bgc_params={"κhP": 10.0, "κvP": 1e-3})
+    uvw = pi.workloads.cellular_flow(g, U=1.0, L=2e3)
+    gm, om = _drive(pi, orc, spec, uvw, 60.0, 6)
+    bad = compare_models(gm._native, om, 1)
+    assert not bad, "\n".join(bad[:20])
+
+
+def test_capacity_error_matches(pi, orc):
+    g = pi.RectilinearGrid(size=(8, 8, 1), x=(0, 8e3), y=(0, 8e3), z=(0, -10))
+    spec = pi.ModelSpec(g, max_individuals=120, seed=2,
+                        phyto=pi.phyto_setup({"dvid_P": [0.05], "dvid_reg": [1.0e-3]}, [100], 1))
+    gm = pi.PlanktonModel(pi.HIPArch(), spec=spec)
+    om = orc.OracleModel(spec)
+    parf = np.full((8, 8), 500.0, np.float32)
+    temp = np.full((8, 8, 1), 25.0, np.float32)
+    gm._native.set_parf(parf); gm._native.set_temperature(temp)
+    steps_gpu = steps_ref = None
+    for i in range(1, 30):
+        om.set_parf(parf); om.set_temperature(temp)
+        try:
+            om.step(60.0)
+        except pi.CapacityError:
+            steps_ref = i
+            break
+    for i in range(1, 30):
+        gm._native.step(60.0)
+        try:
+            gm._native.sync()
+        except pi.CapacityError:
+            steps_gpu = i
+            break
+    assert steps_ref is not None and steps_gpu == steps_ref
