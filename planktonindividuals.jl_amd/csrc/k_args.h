@@ -12,6 +12,9 @@ struct AdvArgs {
     float dt, sx, sy, sz;
     uint64_t seed;
     uint32_t* cnt;
+    int32_t* dstamp;          /* per slot: iteration in which its occupant dies */
+    uint32_t* dcell;          /* per slot: that occupant's cell in this step */
+    long long cap;
 };
 
 struct PhysArgs {
@@ -33,6 +36,10 @@ struct PhysArgs {
     float* diag_num;          /* diags: species stride 4*ncell, field num */
     float dt;
     uint64_t seed;
+    const int32_t* dstamp;
+    const uint32_t* dcell;
+    int32_t* rstamp;          /* per slot: iteration in which a daughter was born there */
+    float* rcons;             /* per slot x 5: that daughter's consumption */
 };
 
 struct PlkArgs {
@@ -43,6 +50,9 @@ struct PlkArgs {
     float* plk[PI_NTRACERS];
     float* diag;              /* [sp][4][ncell] */
     DevState* st;
+    const int32_t* rstamp;
+    const float* rcons;
+    long long cap;
 };
 
 struct SweepArgs {

@@ -137,6 +137,11 @@ __global__ void __launch_bounds__(256) k_adv(AdvArgs a)
         S.cell[e] = c;
         S.flags[e] = fl & (uint8_t)~F_ZERO;
         atomicAdd(&a.cnt[c], 1u);
+        if (fl & (F_GRAZ | F_MORT)) {       /* dies this step: remember where, by slot */
+            const long long r = (long long)sp * a.cap + slot;
+            a.dstamp[r] = (int32_t)iteration;
+            a.dcell[r] = c;
+        }
     }
 }
 
@@ -503,22 +508,48 @@ __global__ void __launch_bounds__(256) k_phys(PhysArgs a)
             s.gen = s.gen + 1.0f; s.age = s.age * (1.0f - 1.0f);
             d.Sz = d.Sz * h; d.Bm = d.Bm * h; d.Cq = d.Cq * h; d.Nq = d.Nq * h; d.Pq = d.Pq * h; d.Chl = d.Chl * h;
             d.gen = d.gen + 1.0f;
-            d.age = a.tab.age[(long long)sp * a.cap + dslot] * (1.0f - 1.0f);
-            /* the daughter's first growth sees zero tracers (gathered while inactive) */
+            const long long dr = (long long)sp * a.cap + dslot;
+            d.age = a.tab.age[dr] * (1.0f - 1.0f);
+            /* The daughter's slot kept what find_inds!/find_NPT! wrote into it this
+             * step: if its previous occupant was still active then (it dies in this
+             * step), the tracers gathered at that occupant's cell and that cell;
+             * otherwise zeros and the halo cell (Hx,Hy,Hz). */
+            const bool reborn = a.dstamp[dr] == (int32_t)iteration;
+            const uint32_t dc = reborn ? a.dcell[dr] : hhh;
             Trs tz;
-            tz.NH4 = tz.NO3 = tz.PO4 = tz.DOC = tz.DFe = tz.par = tz.dpar = tz.T = tz.pop = 0.0f;
+            if (reborn) {
+                tz.NH4 = jl_maxf(0.0f, TR[PI_NH4][dc]);
+                tz.NO3 = jl_maxf(0.0f, TR[PI_NO3][dc]);
+                tz.PO4 = jl_maxf(0.0f, TR[PI_PO4][dc]);
+                tz.DOC = jl_maxf(0.0f, TR[PI_DOC][dc]);
+                tz.DFe = jl_maxf(0.0f, TR[PI_DFe][dc]);
+                tz.par = PAR[dc];
+                tz.dpar = PAR[dc] - PAR0[dc];
+                tz.T = a.temp[dc];
+                tz.pop = a.pop[dc];
+            } else {
+                tz.NH4 = tz.NO3 = tz.PO4 = tz.DOC = tz.DFe = tz.par = tz.dpar = tz.T = tz.pop = 0.0f;
+            }
             pi_uniform3(a.seed, iteration, sp, PI_KIND_DECISION, dslot, &U0, &U1, &U2);
             Rates rd;
             bool gz, mt, dv;
             growth(d, tz, q, true, dt, tmodel, dvid_type, U0, U1, U2, rd, gz, mt, dv);
+            if (reborn) {   /* its calc_consume! lands in that cell, at the slot's place */
+                a.rcons[dr * 5 + 0] = (rd.resp - rd.PS) * dt;
+                a.rcons[dr * 5 + 1] = (rd.exu - rd.VDOC) * dt;
+                a.rcons[dr * 5 + 2] = -rd.VNH4 * dt;
+                a.rcons[dr * 5 + 3] = -rd.VNO3 * dt;
+                a.rcons[dr * 5 + 4] = -rd.VPO4 * dt;
+                a.rstamp[dr] = (int32_t)iteration;
+            }
             const uint32_t o = n + st->doff[sp] + r;
             OUT.key[o] = ((uint32_t)sp << PI_SLOT_BITS) | dslot;
             OUT.x[o] = d.x; OUT.y[o] = d.y; OUT.z[o] = d.z;
             OUT.Sz[o] = d.Sz; OUT.Bm[o] = d.Bm; OUT.Cq[o] = d.Cq; OUT.Nq[o] = d.Nq;
             OUT.Pq[o] = d.Pq; OUT.Chl[o] = d.Chl; OUT.gen[o] = d.gen; OUT.age[o] = d.age;
             OUT.flags[o] = (uint8_t)(F_AC | (gz ? F_GRAZ : 0) | (mt ? F_MORT : 0) | (dv ? F_DVID : 0));
-            OUT.cell[o] = hhh;      /* its find_inds ran while it was inactive: (Hx,Hy,Hz) */
-            atomicAdd(&a.diag_num[(long long)sp * 4 * g.ncell + hhh], 1.0f);
+            OUT.cell[o] = dc;
+            if (!reborn) atomicAdd(&a.diag_num[(long long)sp * 4 * g.ncell + hhh], 1.0f);
         }
         pi_uniform3(a.seed, iteration, sp, PI_KIND_DECISION, slot, &U0, &U1, &U2);
         Rates r;
@@ -561,6 +592,7 @@ __global__ void __launch_bounds__(256) k_phys(PhysArgs a)
 __global__ void __launch_bounds__(256) k_plk(PlkArgs a)
 {
     if (a.st->err) return;
+    const int32_t it = (int32_t)(a.st->iteration + 1);
     const DevGrid& g = a.g;
     for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < g.ncell;
          c += (long long)gridDim.x * blockDim.x) {
@@ -594,6 +626,15 @@ __global__ void __launch_bounds__(256) k_plk(PlkArgs a)
                     NO3 = NO3 + a.con.v[3][x]; PO4 = PO4 + a.con.v[4][x];
                     ++na;
                     if (t & T_DIV) ++nd;
+                } else {
+                    /* died here this step; a daughter born into its slot consumes here */
+                    const long long r = (long long)sp * a.cap + (a.pkey[x] & PI_SLOT_MASK);
+                    if (a.rstamp[r] == it) {
+                        const float* rc = a.rcons + r * 5;
+                        DIC = DIC + rc[0]; DOC = DOC + rc[1]; NH4 = NH4 + rc[2];
+                        NO3 = NO3 + rc[3]; PO4 = PO4 + rc[4];
+                        ++na;
+                    }
                 }
             }
             float* d = a.diag + (long long)sp * 4 * g.ncell;

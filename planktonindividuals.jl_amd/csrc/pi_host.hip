@@ -61,6 +61,9 @@ struct pi_model {
     uint32_t *skey = nullptr, *ssrc = nullptr, *big = nullptr, *bm = nullptr, *G = nullptr;
     Contrib con{};
     unsigned long long* d_count = nullptr;
+    int32_t *dstamp = nullptr, *rstamp = nullptr;
+    uint32_t* dcell = nullptr;
+    float* rcons = nullptr;
     bool dirty = false;
     long long host_iter = 0;
     float last_dt = 0.0f;
@@ -206,13 +209,16 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     /* store x2 */
     DevState& hs = m->h_state;
     memset(&hs, 0, sizeof hs);
+    /* a step's output holds the live individuals of its input (those dying
+     * this step stay as inactive entries) plus the daughters: <= 2 x capacity */
+    const long long scap = 2 * m->ctot;
     for (int b = 0; b < 2; ++b) {
         Store& S = hs.store[b];
-        DALLOC(S.key, m->ctot * 4);
+        DALLOC(S.key, scap * 4);
         for (float** f : {&S.x, &S.y, &S.z, &S.Sz, &S.Bm, &S.Cq, &S.Nq, &S.Pq, &S.Chl, &S.gen, &S.age})
-            DALLOC(*f, m->ctot * 4);
-        DALLOC(S.flags, m->ctot);
-        DALLOC(S.cell, m->ctot * 4);
+            DALLOC(*f, scap * 4);
+        DALLOC(S.flags, scap);
+        DALLOC(S.cell, scap * 4);
     }
     /* slot table */
     SlotTable& T = m->tab;
@@ -247,6 +253,13 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     for (int q = 0; q < 6; ++q) DALLOC(m->con.v[q], m->ctot * 4);
     DALLOC(m->con.tag, m->ctot);
     DALLOC(m->d_count, 8);
+    DALLOC(m->dstamp, m->ctot * 4); DALLOC(m->rstamp, m->ctot * 4);
+    DALLOC(m->dcell, m->ctot * 4); DALLOC(m->rcons, m->ctot * 5 * 4);
+    {   /* stamps start below any iteration */
+        std::vector<int32_t> neg(m->ctot, -1);
+        HIPCHECK(hipMemcpy(m->dstamp, neg.data(), m->ctot * 4, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(m->rstamp, neg.data(), m->ctot * 4, hipMemcpyHostToDevice));
+    }
     DALLOC(m->d_state, sizeof(DevState));
     hs.scur = hs.tcur = hs.pcur = 0;
     hs.iteration = 0;
@@ -479,6 +492,7 @@ static pi_status enqueue_step(pi_model* m, float dt)
     aa.sy = std::sqrt(m->bgc[PI_B_kappa_hP] * dt);
     aa.sz = std::sqrt(m->bgc[PI_B_kappa_vP] * dt);
     aa.seed = m->seed; aa.cnt = m->cnt;
+    aa.dstamp = m->dstamp; aa.dcell = m->dcell; aa.cap = m->cap;
     tbegin(m, "adv");
     launch_adv(s, pb, aa);
     tend(m);
@@ -498,11 +512,13 @@ static pi_status enqueue_step(pi_model* m, float dt)
     pa.seg = m->seg; pa.pkey = m->pkey; pa.psrc = m->psrc; pa.temp = m->temp; pa.pop = m->pop;
     pa.bmA = m->bm; pa.bmD = m->bm + (size_t)m->nsp * m->nw; pa.G = m->G; pa.nw = m->nw;
     pa.con = m->con; pa.diag_num = m->diag; pa.dt = dt; pa.seed = m->seed;
+    pa.dstamp = m->dstamp; pa.dcell = m->dcell; pa.rstamp = m->rstamp; pa.rcons = m->rcons;
     tbegin(m, "phys");
     launch_phys(s, pb, pa);
     tend(m);
     PlkArgs ka{};
     ka.g = g; ka.seg = m->seg; ka.pkey = m->pkey; ka.con = m->con; ka.diag = m->diag; ka.st = st;
+    ka.rstamp = m->rstamp; ka.rcons = m->rcons; ka.cap = m->cap;
     for (int q = 0; q < PI_NTRACERS; ++q) ka.plk[q] = m->plk[q] ? m->plk[q] : m->plk[PI_DIC];
     tbegin(m, "plk");
     launch_plk(s, cb, ka);
