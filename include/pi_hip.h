@@ -179,10 +179,12 @@ pi_status pi_set_diag_tracers(pi_model* m, int32_t n, const int32_t* ids);
 pi_status pi_get_iteration(pi_model* m, int64_t* iteration);
 pi_status pi_count_active(pi_model* m, int32_t sp, int64_t* n);   /* dot(ac, ac) (models.jl:157-164) */
 
-/* Per-kernel timing of the most recent step (HIP events on the model stream);
- * names[i] is static text.  Used by the benchmark for the roofline. */
+/* Per-kernel timing with HIP events recorded on the model's stream around each
+ * launch group; pi_kernel_times returns, per group name (static text), the total
+ * milliseconds and the number of launches since timing was enabled or last
+ * read.  No counterpart in the reference (its benchmarks time whole steps). */
 pi_status pi_enable_timing(pi_model* m, int32_t on);
-pi_status pi_kernel_times(pi_model* m, int32_t* n, const char** names, float* ms, int32_t cap);
+pi_status pi_kernel_times(pi_model* m, int32_t* n, const char** names, float* ms, int32_t* counts, int32_t cap);
 
 #ifdef __cplusplus
 }

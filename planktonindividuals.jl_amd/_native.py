@@ -53,7 +53,7 @@ def load_library():
     lib.pi_enable_timing.argtypes = [C.c_void_p, C.c_int32]
     lib.pi_kernel_times.restype = C.c_int
     lib.pi_kernel_times.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_char_p),
-                                    C.POINTER(C.c_float), C.c_int32]
+                                    C.POINTER(C.c_float), C.POINTER(C.c_int32), C.c_int32]
     _lib = lib
     return lib
 
@@ -214,9 +214,11 @@ class NativeModel(Handle):
         self._call("enable_timing", 1 if on else 0)
 
     def kernel_times(self):
+        """{group name: (total ms, launches)} since timing was enabled or last read."""
         cap = 64
         n = C.c_int32(0)
         names = (C.c_char_p * cap)()
         ms = (C.c_float * cap)()
-        self._call("kernel_times", C.byref(n), names, ms, cap)
-        return [(names[i].decode(), ms[i]) for i in range(n.value)]
+        cnt = (C.c_int32 * cap)()
+        self._call("kernel_times", C.byref(n), names, ms, cnt, cap)
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(n.value)}

@@ -463,7 +463,6 @@ static pi_status enqueue_step(pi_model* m, float dt)
     const int pb = blocks_for(2 * m->ctot);
     const int cb = blocks_for(m->ncell);
     const int ib = blocks_for(m->nint);
-    m->tcount = 0;
     /* vel½ (time_step.jl:17-19); equal to vel₀ bit for bit when vel₀ == vel₁ */
     int vh = m->v0;
     if (!m->vel_same) {
@@ -718,22 +717,33 @@ pi_status pi_count_active(pi_model* m, int32_t sp, int64_t* n)
 pi_status pi_enable_timing(pi_model* m, int32_t on)
 {
     if (!m) return fail(PI_ERR_ARG, "null model");
+    HIPCHECK(hipStreamSynchronize(m->stream));
     m->timing = on != 0;
+    m->tcount = 0;
     return PI_OK;
 }
 
-pi_status pi_kernel_times(pi_model* m, int32_t* n, const char** names, float* ms, int32_t cap)
+/* Per launch group recorded since timing was enabled (or last read): total
+ * milliseconds and number of launches, aggregated by name; resets the record. */
+pi_status pi_kernel_times(pi_model* m, int32_t* n, const char** names, float* ms, int32_t* counts, int32_t cap)
 {
     if (!m || !n) return fail(PI_ERR_ARG, "null argument");
     HIPCHECK(hipStreamSynchronize(m->stream));
     int32_t k = 0;
-    for (size_t i = 0; i < m->tcount && k < cap; ++i, ++k) {
+    for (size_t i = 0; i < m->tcount; ++i) {
         float t = 0.0f;
         HIPCHECK(hipEventElapsedTime(&t, m->timers[i].a, m->timers[i].b));
-        names[k] = m->timers[i].name;
-        ms[k] = t;
+        int32_t j = 0;
+        while (j < k && strcmp(names[j], m->timers[i].name) != 0) ++j;
+        if (j == k) {
+            if (k == cap) continue;
+            names[k] = m->timers[i].name; ms[k] = 0.0f; counts[k] = 0; ++k;
+        }
+        ms[j] += t;
+        counts[j] += 1;
     }
     *n = k;
+    m->tcount = 0;
     return PI_OK;
 }
 

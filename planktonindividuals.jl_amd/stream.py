@@ -51,8 +51,21 @@ def to_unit(u):
 
 
 def uniform3(seed, iteration, species, kind, slots):
-    """The three Float32 uniforms of the shared stream for each slot in `slots`."""
+    """The three Float32 uniforms of the shared stream for each slot in `slots`.
+    Large requests are split over threads (numpy releases the GIL in ufuncs)."""
     slots = np.asarray(slots, dtype=np.uint64)
+    if slots.size >= (1 << 21):
+        import os
+        from concurrent.futures import ThreadPoolExecutor
+        nth = max(1, min(16, os.cpu_count() or 1))
+        parts = np.array_split(slots, nth * 4)
+        with ThreadPoolExecutor(nth) as ex:
+            res = list(ex.map(lambda p: _uniform3(seed, iteration, species, kind, p), parts))
+        return tuple(np.concatenate([r[i] for r in res]) for i in range(3))
+    return _uniform3(seed, iteration, species, kind, slots)
+
+
+def _uniform3(seed, iteration, species, kind, slots):
     k0 = np.uint32(seed & 0xFFFFFFFF)
     k1 = np.uint32((seed >> 32) & 0xFFFFFFFF)
     o0, o1, o2, _ = philox4x32_10((slots & _MASK).astype(np.uint32), np.uint32(iteration & 0xFFFFFFFF),

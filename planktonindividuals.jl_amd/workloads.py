@@ -56,10 +56,10 @@ def fourier_flow(grid, nmodes=64, rms=0.05, seed=0, kmax=None):
         ph = rng.uniform(0, 2 * np.pi)
         nz_ = rng.integers(0, 3)
         vert = np.cos(np.pi * nz_ * kk)
-        psi += amp * np.cos(kx * xf[:, None, None] + ky * yf[None, :, None] + ph) * vert[None, None, :]
+        psi += (amp * np.cos(kx * xf[:, None] + ky * yf[None, :] + ph))[:, :, None] * vert[None, None, :]
         vdiv = np.cos(2 * np.pi * (1 + (m % 2)) * kk)         # zero column mean
         ph2 = rng.uniform(0, 2 * np.pi)
-        chi += 0.3 * amp * np.cos(kx * xc[:, None, None] + ky * yc[None, :, None] + ph2) * vdiv[None, None, :]
+        chi += (0.3 * amp * np.cos(kx * xc[:, None] + ky * yc[None, :] + ph2))[:, :, None] * vdiv[None, None, :]
     # C-grid: u on x faces (i = 0..Nx-1, periodic), v on y faces
     u = -(psi[:Nx, 1:, :] - psi[:Nx, :-1, :]) / dy
     v = (psi[1:, :Ny, :] - psi[:-1, :Ny, :]) / dx
