@@ -168,17 +168,17 @@ PI_HD int jl_islessd(double x, double y)
 PI_HD float jl_maskf(int b, float x) { return b ? x : copysignf(0.0f, x); }
 PI_HD double jl_maskd(int b, double x) { return b ? x : copysign(0.0, x); }
 /* mod(x, 1) for a float x (Base.mod with a float remainder). */
+/* mod(x, 1): rem(x, 1) = x - trunc(x) exactly (the fractional part of a float
+ * is a float), then the sign correction of mod */
 PI_HD float jl_mod1f(float x)
 {
-    float r = fmodf(x, 1.0f);
-    if (r == 0.0f) return 0.0f;                 /* copysign(r, 1) */
-    return (r < 0.0f) ? r + 1.0f : r;
+    const float r = x - truncf(x);
+    return r == 0.0f ? 0.0f : (r < 0.0f ? r + 1.0f : r);   /* zero: copysign(r, 1) */
 }
 PI_HD double jl_mod1d(double x)
 {
-    double r = fmod(x, 1.0);
-    if (r == 0.0) return 0.0;
-    return (r < 0.0) ? r + 1.0 : r;
+    const double r = x - trunc(x);
+    return r == 0.0 ? 0.0 : (r < 0.0 ? r + 1.0 : r);
 }
 
 #endif /* PI_MATH_H */

@@ -55,12 +55,13 @@ struct PlkArgs {
     long long cap;
 };
 
+/* Tracer sets are slabs of PI_NTRACERS padded fields: field q at slab + q·ncell. */
 struct SweepArgs {
     DevGrid g;
     DevState* st;
     const float* vel;                 /* u (x sweep) or v (y sweep) of vel₁ */
-    float* out[PI_NTRACERS];          /* temp after this sweep */
-    const float* prev[PI_NTRACERS];   /* temp before this sweep; null = current tracers */
+    float* out;                       /* slab: temp after this sweep */
+    const float* prev;                /* slab: temp before this sweep (y sweep) */
     float dt;
     int dim;                          /* 0 = x, 1 = y */
 };
@@ -68,7 +69,7 @@ struct SweepArgs {
 struct FillArgs {
     DevGrid g;
     DevState* st;
-    float* f[PI_NTRACERS];     /* null = the tracers written by this step */
+    float* f;                  /* slab; null = the tracers written by this step */
     int dim, lines_full, nf;
 };
 
@@ -76,8 +77,8 @@ struct FinalArgs {
     DevGrid g;
     DevState* st;
     const float* w;                   /* vel₁ w */
-    const float* prev[PI_NTRACERS];   /* temp after the y sweep (z halos filled) */
-    const float* plk[PI_NTRACERS];    /* null for tracers without particle exchange */
+    const float* prev;                /* slab: temp after the y sweep (z halos filled) */
+    const float* plk;                 /* slab of plk fields (read for the tracers with particle exchange) */
     float dt;
     float kh, kv;                     /* κh, κv (tracer_diffusion!) */
     float wsink_inorg, wsink_org;

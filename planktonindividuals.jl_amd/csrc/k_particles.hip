@@ -40,19 +40,20 @@ __device__ __forceinline__ float itpl(const DevGrid& g, const float* __restrict_
     zi = min(max(zi, 1), g.Pz - 1);
     const long long sx = 1, sy = g.Px, sz = (long long)g.Px * g.Py;
     const long long b = gidx(g, xi, yi, zi);
-    float D0, D1;
-    if (comp == 0) { D0 = g.dx; D1 = g.dx; }
-    else if (comp == 1) { D0 = g.dy; D1 = g.dy; }
-    else { D0 = dzC_at(g, zi); D1 = dzC_at(g, zi + 1); }
+    float D0, D1;             /* divisors and their reciprocals */
+    double R0, R1;
+    if (comp == 0) { D0 = g.dx; D1 = g.dx; R0 = g.ydx; R1 = g.ydx; }
+    else if (comp == 1) { D0 = g.dy; D1 = g.dy; R0 = g.ydy; R1 = g.ydy; }
+    else { D0 = dzC_at(g, zi); D1 = dzC_at(g, zi + 1); R0 = g.ydzC[zi - 1]; R1 = g.ydzC[zi]; }
     const float ox = 1.0f - xd, oy = 1.0f - yd, oz = 1.0f - zd;
-    float s = ox * oy * oz * f[b] / D0;
-    s = s + ox * oy * zd * f[b + sz] / D1;
-    s = s + ox * yd * oz * f[b + sy] / D0;
-    s = s + ox * yd * zd * f[b + sy + sz] / D1;
-    s = s + xd * oy * oz * f[b + sx] / D0;
-    s = s + xd * oy * zd * f[b + sx + sz] / D1;
-    s = s + xd * yd * oz * f[b + sx + sy] / D0;
-    s = s + xd * yd * zd * f[b + sx + sy + sz] / D1;
+    float s = divc(ox * oy * oz * f[b], D0, R0);
+    s = s + divc(ox * oy * zd * f[b + sz], D1, R1);
+    s = s + divc(ox * yd * oz * f[b + sy], D0, R0);
+    s = s + divc(ox * yd * zd * f[b + sy + sz], D1, R1);
+    s = s + divc(xd * oy * oz * f[b + sx], D0, R0);
+    s = s + divc(xd * oy * zd * f[b + sx + sz], D1, R1);
+    s = s + divc(xd * yd * oz * f[b + sx + sy], D0, R0);
+    s = s + divc(xd * yd * zd * f[b + sx + sy + sz], D1, R1);
     return s;
 }
 
@@ -113,7 +114,7 @@ __global__ void __launch_bounds__(256) k_adv(AdvArgs a)
         v1 = itpl(g, a.v1.v, px, py, pz, 1);
         w1 = itpl(g, a.v1.w, px, py, pz, 2);
         u2 = u2 + u1; v2 = v2 + v1; w2 = w2 + w1;
-        u2 = u2 / 6.0f; v2 = v2 / 6.0f; w2 = w2 / 6.0f;
+        u2 = divc(u2, 6.0f, g.y6); v2 = divc(v2, 6.0f, g.y6); w2 = divc(w2, 6.0f, g.y6);
         float x = pbc(x0 + 1.0f * u2 * dt, g.Nx, g.tx);
         float y = pbc(y0 + 1.0f * v2 * dt, g.Ny, g.ty);
         float z = pbc(z0 - 1.0f * w2 * dt, g.Nz, g.tz);
@@ -123,9 +124,9 @@ __global__ void __launch_bounds__(256) k_adv(AdvArgs a)
         r0 = (r0 - 0.5f) * 2.0f; r1 = (r1 - 0.5f) * 2.0f; r2 = (r2 - 0.5f) * 2.0f;
         const uint32_t cprev = S.cell[e];
         const long long zprev = (fl & F_ZERO) ? 0 : (long long)(cprev / (uint32_t)(g.Px * g.Py)) + 1;
-        r0 = r0 * a.sx / g.dx;
-        r1 = r1 * a.sy / g.dy;
-        r2 = r2 * a.sz / dzC_at(g, zprev + g.Hz);
+        r0 = divc(r0 * a.sx, g.dx, g.ydx);
+        r1 = divc(r1 * a.sy, g.dy, g.ydy);
+        r2 = div_dzC(g, r2 * a.sz, zprev + g.Hz);
         x = pbc(x + r0, g.Nx, g.tx);
         y = pbc(y + r1, g.Ny, g.ty);
         z = pbc(z + r2, g.Nz, g.tz);
@@ -296,7 +297,7 @@ struct Rates { float PS, VDOC, VNH4, VNO3, VPO4, resp, BS, exu, rho; };
 /* shape_func_dec / shape_func_inc (Individuals/utils.jl:109-121) */
 __device__ __forceinline__ float shape_dec(float x, float xmax, float k, int pw)
 {
-    float fx = jl_maxf(0.0f, jl_minf(1.0f, 1.0f - x / xmax));
+    float fx = dmaxf(0.0f, dminf(1.0f, 1.0f - x / xmax));
     float fp = pw == 2 ? pi_pow2f(fx) : pi_pow4f(fx);
     return fp / (k + fp);
 }
@@ -310,16 +311,16 @@ __device__ __forceinline__ float tempFunc_PS(float T, const float* q)
     float x = T - q[PI_P_Topt], xmax = q[PI_P_Tmax] - q[PI_P_Topt];
     float regT = shape_dec(x, xmax, 4.0e-2f, 4);
     float k = pi_expf(-q[PI_P_Ea] / (8.3145f * (T + 273.15f))) * regT;
-    k = jl_maxf(0.0f, k);
+    k = dmaxf(0.0f, k);
     float OGT = pi_expf(-q[PI_P_Ea] / (8.3145f * (q[PI_P_Topt] + 273.15f)));
-    return jl_minf(1.0f, k / OGT);
+    return dminf(1.0f, k / OGT);
 }
 __device__ __forceinline__ float tempFunc(float T, const float* q)
 {
     float k = pi_expf(-q[PI_P_Ea] / (8.3145f * (T + 273.15f)));
-    k = jl_maxf(0.0f, k);
+    k = dmaxf(0.0f, k);
     float OGT = pi_expf(-q[PI_P_Ea] / (8.3145f * (q[PI_P_Topt] + 273.15f)));
-    return jl_minf(1.0f, k / OGT);
+    return dminf(1.0f, k / OGT);
 }
 
 /* plankton_growth! for one individual (QuotaMode/plankton_growth.jl:2-24,
@@ -332,36 +333,36 @@ __device__ void growth(PState& s, const Trs& t, const float* q, bool ac, float d
     {   /* calc_inorganic_uptake_kernel! */
         float aI = t.par * q[PI_P_alpha] * q[PI_P_Phi];
         float PCm = q[PI_P_PCmax] * tempFunc_PS(t.T, q);
-        float PS = PCm * (1.0f - pi_expf(-aI / jl_maxf(e30, PCm) * s.Chl / jl_maxf(e30, s.Bm))) * s.Bm;
+        float PS = PCm * (1.0f - pi_expf(-aI / dmaxf(e30, PCm) * s.Chl / dmaxf(e30, s.Bm))) * s.Bm;
         r.PS = jl_maskf(ac, PS);
-        float Qn = (s.Nq + s.Bm * q[PI_P_R_NC]) / jl_maxf(e30, s.Bm + s.Cq);
-        float Qp = (s.Pq + s.Bm * q[PI_P_R_PC]) / jl_maxf(e30, s.Bm + s.Cq);
+        float Qn = (s.Nq + s.Bm * q[PI_P_R_NC]) / dmaxf(e30, s.Bm + s.Cq);
+        float Qp = (s.Pq + s.Bm * q[PI_P_R_PC]) / dmaxf(e30, s.Bm + s.Cq);
         float regQN = shape_dec(Qn, q[PI_P_Nqmax], 1.0e-4f, 4);
         float regQP = shape_dec(Qp, q[PI_P_Pqmax], 1.0e-4f, 4);
         float tf = tempFunc(t.T, q);
-        float VNH4 = jl_maskf(ac, q[PI_P_VNH4max] * regQN * t.NH4 / jl_maxf(e30, t.NH4 + q[PI_P_KsatNH4]) * tf * s.Bm);
-        float VNO3 = jl_maskf(ac, q[PI_P_VNO3max] * regQN * t.NO3 / jl_maxf(e30, t.NO3 + q[PI_P_KsatNO3]) * tf * s.Bm);
-        float VPO4 = jl_maskf(ac, q[PI_P_VPO4max] * regQP * t.PO4 / jl_maxf(e30, t.PO4 + q[PI_P_KsatPO4]) * tf * s.Bm);
-        float pp = jl_maxf(1.0f, t.pop);
-        r.VNH4 = jl_minf(VNH4, t.NH4 / dt / pp);
-        r.VNO3 = jl_minf(VNO3, t.NO3 / dt / pp);
-        r.VPO4 = jl_minf(VPO4, t.PO4 / dt / pp);
+        float VNH4 = jl_maskf(ac, q[PI_P_VNH4max] * regQN * t.NH4 / dmaxf(e30, t.NH4 + q[PI_P_KsatNH4]) * tf * s.Bm);
+        float VNO3 = jl_maskf(ac, q[PI_P_VNO3max] * regQN * t.NO3 / dmaxf(e30, t.NO3 + q[PI_P_KsatNO3]) * tf * s.Bm);
+        float VPO4 = jl_maskf(ac, q[PI_P_VPO4max] * regQP * t.PO4 / dmaxf(e30, t.PO4 + q[PI_P_KsatPO4]) * tf * s.Bm);
+        float pp = dmaxf(1.0f, t.pop);
+        r.VNH4 = dminf(VNH4, t.NH4 / dt / pp);
+        r.VNO3 = dminf(VNO3, t.NO3 / dt / pp);
+        r.VPO4 = dminf(VPO4, t.PO4 / dt / pp);
     }
     /* update_quotas_1_kernel! */
     s.Cq = s.Cq + r.PS * dt;
     s.Pq = s.Pq + r.VPO4 * dt;
     s.Nq = s.Nq + (r.VNH4 + r.VNO3) * dt;
     {   /* calc_organic_uptake_kernel! */
-        float Qc = s.Cq / jl_maxf(e30, s.Bm + s.Cq);
+        float Qc = s.Cq / dmaxf(e30, s.Bm + s.Cq);
         float regQ = shape_dec(Qc, q[PI_P_Cqmax], 1.0e-4f, 4);
-        float VN = q[PI_P_VDOCmax] * regQ * t.DOC / jl_maxf(e30, t.DOC + q[PI_P_KsatDOC]) * tempFunc(t.T, q) * s.Bm;
-        float VD = jl_minf(VN, t.DOC / dt / jl_maxf(1.0f, t.pop));
+        float VN = q[PI_P_VDOCmax] * regQ * t.DOC / dmaxf(e30, t.DOC + q[PI_P_KsatDOC]) * tempFunc(t.T, q) * s.Bm;
+        float VD = dminf(VN, t.DOC / dt / dmaxf(1.0f, t.pop));
         r.VDOC = jl_maskf(ac, VD);
         r.VDOC = jl_maskf(jl_islessf(1.0e-2f, r.PS / (r.VDOC + r.PS)), r.VDOC);
     }
     {   /* calc_ρChl_kernel! */
-        float rr = r.PS / jl_maxf(e30, s.Bm) * q[PI_P_Chl2N] /
-                   jl_maxf(e30, t.par * q[PI_P_alpha] * q[PI_P_Phi] * s.Chl / s.Bm);
+        float rr = r.PS / dmaxf(e30, s.Bm) * q[PI_P_Chl2N] /
+                   dmaxf(e30, t.par * q[PI_P_alpha] * q[PI_P_Phi] * s.Chl / s.Bm);
         rr = jl_maskf(jl_islessf(1.0e-1f, t.par), rr);
         r.rho = jl_maskf(ac, rr);
     }
@@ -369,14 +370,14 @@ __device__ void growth(PState& s, const Trs& t, const float* q, bool ac, float d
     r.resp = jl_maskf(ac, q[PI_P_respir] * s.Bm * tempFunc(t.T, q));
     /* update_quotas_2_kernel! */
     s.Cq = s.Cq + (r.VDOC - r.resp) * dt;
-    s.Bm = s.Bm - jl_maxf(0.0f, (0.0f - s.Cq));
-    s.Nq = s.Nq + jl_maxf(0.0f, (0.0f - s.Cq)) * q[PI_P_R_NC];
-    s.Pq = s.Pq + jl_maxf(0.0f, (0.0f - s.Cq)) * q[PI_P_R_PC];
-    s.Cq = s.Cq + jl_maxf(0.0f, (0.0f - s.Cq));
+    s.Bm = s.Bm - dmaxf(0.0f, (0.0f - s.Cq));
+    s.Nq = s.Nq + dmaxf(0.0f, (0.0f - s.Cq)) * q[PI_P_R_NC];
+    s.Pq = s.Pq + dmaxf(0.0f, (0.0f - s.Cq)) * q[PI_P_R_PC];
+    s.Cq = s.Cq + dmaxf(0.0f, (0.0f - s.Cq));
     {   /* calc_BS_kernel! */
-        float mn = jl_minf(jl_minf(s.Cq, s.Nq / q[PI_P_R_NC]), s.Pq / q[PI_P_R_PC]);
+        float mn = dminf(dminf(s.Cq, s.Nq / q[PI_P_R_NC]), s.Pq / q[PI_P_R_PC]);
         r.BS = jl_maskf(ac, mn * q[PI_P_k_mtb]);
-        r.exu = jl_maskf(ac, jl_maxf(0.0f, s.Cq - mn) * q[PI_P_k_mtb]);
+        r.exu = jl_maskf(ac, dmaxf(0.0f, s.Cq - mn) * q[PI_P_k_mtb]);
     }
     /* update_biomass_kernel! */
     s.Bm = s.Bm + dt * r.BS;
@@ -454,11 +455,11 @@ __global__ void __launch_bounds__(256) k_phys(PhysArgs a)
         s.Pq = IN.Pq[src]; s.Chl = IN.Chl[src]; s.gen = IN.gen[src]; s.age = IN.age[src];
         /* find_NPT_kernel! (Individuals/utils.jl:14-30), ac = true at gather time */
         Trs t;
-        t.NH4 = jl_maxf(0.0f, TR[PI_NH4][c]);
-        t.NO3 = jl_maxf(0.0f, TR[PI_NO3][c]);
-        t.PO4 = jl_maxf(0.0f, TR[PI_PO4][c]);
-        t.DOC = jl_maxf(0.0f, TR[PI_DOC][c]);
-        t.DFe = jl_maxf(0.0f, TR[PI_DFe][c]);
+        t.NH4 = dmaxf(0.0f, TR[PI_NH4][c]);
+        t.NO3 = dmaxf(0.0f, TR[PI_NO3][c]);
+        t.PO4 = dmaxf(0.0f, TR[PI_PO4][c]);
+        t.DOC = dmaxf(0.0f, TR[PI_DOC][c]);
+        t.DFe = dmaxf(0.0f, TR[PI_DFe][c]);
         t.par = PAR[c];
         t.dpar = PAR[c] - PAR0[c];
         t.T = a.temp[c];
@@ -518,11 +519,11 @@ __global__ void __launch_bounds__(256) k_phys(PhysArgs a)
             const uint32_t dc = reborn ? a.dcell[dr] : hhh;
             Trs tz;
             if (reborn) {
-                tz.NH4 = jl_maxf(0.0f, TR[PI_NH4][dc]);
-                tz.NO3 = jl_maxf(0.0f, TR[PI_NO3][dc]);
-                tz.PO4 = jl_maxf(0.0f, TR[PI_PO4][dc]);
-                tz.DOC = jl_maxf(0.0f, TR[PI_DOC][dc]);
-                tz.DFe = jl_maxf(0.0f, TR[PI_DFe][dc]);
+                tz.NH4 = dmaxf(0.0f, TR[PI_NH4][dc]);
+                tz.NO3 = dmaxf(0.0f, TR[PI_NO3][dc]);
+                tz.PO4 = dmaxf(0.0f, TR[PI_PO4][dc]);
+                tz.DOC = dmaxf(0.0f, TR[PI_DOC][dc]);
+                tz.DFe = dmaxf(0.0f, TR[PI_DFe][dc]);
                 tz.par = PAR[dc];
                 tz.dpar = PAR[dc] - PAR0[dc];
                 tz.T = a.temp[dc];

@@ -1,103 +1,257 @@
 /*
  * k_tracers.hip - tracer_update! (Biogeochemistry/tracer_update.jl:22-43) on gfx950.
  *
- *   k_sweep_x / k_sweep_y  one DST3FL dimension of tracer_advection!
- *                          (Advection/multi_dim_adv.jl:111-135) for all 14
- *                          tracers per lane; face quantities (CFL, d0, d1,
- *                          transport) are computed once and shared by the 14.
- *   k_fill_line            halo fill of one dimension along lines, as the
- *                          reference's fill_halo_* (HaloRegions/halo_regions.jl)
- *                          with broadcast unaliasing (each line is owned by one
- *                          lane, which reads its sources before writing).
- *   k_final                z sweep + Gcs = temp - c + diffusion + sinking +
- *                          forcing + apply_tendency, writing the next tracers.
+ *   k_sweep_x   DST3FL x sweep (calc_Gcsˣ! + multi_dim_x!, Advection/multi_dim_adv.jl:125-190)
+ *               for all 14 tracers.  Lanes own FACES: a row of Nx cells has
+ *               Nx + 1 faces, a wave takes 64 consecutive faces and updates the
+ *               63 cells whose west face it holds (the east face comes from the
+ *               next lane by a shuffle), so every face flux is computed once.
+ *   k_sweep_y   y sweep; each lane walks a run of cells along y, computing each
+ *               face flux once and carrying it (and the stencil) to the next cell.
+ *   k_fill_line halo fill of one dimension along lines, as fill_halo_*
+ *               (HaloRegions/halo_regions.jl) with broadcast unaliasing (each
+ *               line is owned by one lane, which reads its sources first).
+ *   k_final     z sweep + Gcs = temp - c + diffusion + sinking + forcing +
+ *               apply_tendency, lanes walking runs along z (advective and
+ *               diffusive z fluxes carried), writing the next tracers.
  *
- * The face flux at the last interior face is computed from the halo values for
- * Periodic directions (identical to the reference's Gc halo copy of the first
- * face) and is zero for Bounded ones (fill_halo_east_Gc!/..., halo_regions.jl:22-24).
+ * Exactness: every value is the reference's expression in its operation order.
+ *  - The face flux ½(T+|T|)(…) + ½(T−|T|)(…) is evaluated for the sign of T
+ *    only.  For 0 < T < 2^126, ½(T+|T|) = T exactly and ½(T−|T|) = +0.  With the
+ *    four stencil values below 2^100 in magnitude and the Courant number below
+ *    2^60, every Ψ is a number in [0, 1] (no inf − inf, no 0·inf), so the
+ *    second factor is finite and its product a signed zero: the sum equals the
+ *    first product whenever that is non-zero.  Symmetrically for T < 0.  Every
+ *    other case (T = ±0, a zero product, large or non-finite values) takes the
+ *    full formula.
+ *  - Divisions by fixed divisors (Δx, Δy, dz, the volume, 6, and C + 1e-20
+ *    shared by the 14 tracers of a face) use divc (pi_device.h).
+ *  - The face flux at the last interior face is computed from the halo values
+ *    in Periodic directions (identical to the reference's Gc halo copy of the
+ *    first face) and is zero in Bounded ones (fill_halo_east_Gc!, halo_regions.jl:22-24).
  */
 #include "pi_device.h"
 #include "k_args.h"
 
-__device__ __forceinline__ float lmask(const DevGrid& g, long long outer, long long inner, float x)
+
+/* Tracer buffers are slabs of 14 padded fields (field q at slab + q·ncell) and
+ * are addressed as a uniform base plus a 32-bit byte offset (ncell·4 < 2^32 is
+ * checked when the model is created): the loads take the SGPR-base form and
+ * no 64-bit address is kept per tracer. */
+__device__ __forceinline__ const float* fld(const float* slab, const DevGrid& g, int q) { return slab + (long long)q * g.ncell; }
+__device__ __forceinline__ float* fld(float* slab, const DevGrid& g, int q) { return slab + (long long)q * g.ncell; }
+__device__ __forceinline__ float ld(const float* base, uint32_t cell) { return *(const float*)((const char*)base + cell * 4u); }
+__device__ __forceinline__ void st(float* base, uint32_t cell, float v) { *(float*)((char*)base + cell * 4u) = v; }
+
+/* Every kernel here is instantiated twice: LAND = false (no land cell, the
+ * masks are identities and each face is straight-line code) and LAND = true. */
+template <bool LAND>
+__device__ __forceinline__ float lmask(const DevGrid& g, uint32_t outer, uint32_t inner, float x)
 {
-    if (!g.has_land) return x;
+    if (!LAND) return x;
     return jl_maskf(g.lm[outer] != 0, jl_maskf(g.lm[inner] != 0, x));
 }
 
-/* DST3FL.jl:23-57 */
+/* face quantities of DST3FL.jl:27-38, shared by the 14 tracers */
+struct Face {
+    float C, d0, d1, T, omc, Ce;   /* Courant number, δ⁰, δ¹, transport, 1 − C, C + 1e-20 */
+    double yCe;                    /* RN(1 / (C + 1e-20)) */
+    bool fast;                     /* T ≠ ±0, |T| < 2^126, C < 2^60 */
+};
+
+__device__ __forceinline__ Face face(const DevGrid& g, float u, float dt, float D, double yD, float A)
+{
+    Face f;
+    f.C = fabsf(divc(u * dt, D, yD));
+    f.d0 = divc((2.0f - f.C) * (1.0f - f.C), 6.0f, g.y6);
+    f.d1 = divc(1.0f - f.C * f.C, 6.0f, g.y6);
+    f.T = A * u;
+    f.omc = 1.0f - f.C;
+    f.Ce = f.C + 1.0e-20f;
+    f.yCe = 1.0 / (double)f.Ce;
+    f.fast = f.C < 0x1p60f && ((f.T > 0.0f && f.T < 0x1p126f) || (f.T < 0.0f && f.T > -0x1p126f));
+    return f;
+}
+
+/* θ and Ψ (DST3FL.jl:23-57) */
 __device__ __forceinline__ float theta(float dn, float dd)
 {
     const float tmax = 1.0e20f;
-    return fabsf(dd) * tmax <= fabsf(dn) ? copysignf(tmax, dn * dd) : dn / dd;
+    const float q = dn / dd;                       /* both arms evaluated: no branch */
+    const float cap = copysignf(tmax, dn * dd);
+    return fabsf(dd) * tmax <= fabsf(dn) ? cap : q;
 }
-__device__ __forceinline__ float psi(float C, float d0, float d1, float th)
+__device__ __forceinline__ float psi(const Face& F, float th)
 {
-    return jl_maxf(0.0f, jl_minf(jl_minf(1.0f, d0 + d1 * th), th * (1.0f - C) / (C + 1.0e-20f)));
-}
-
-/* adv_flux_{x,y} at face f (the face between f-1 and f) along a line with stride
- * `st` (1 for x, Px for y); c is the field being swept. DST3FL.jl:60-91 */
-__device__ __forceinline__ float flux_xy(const DevGrid& g, const float* __restrict__ c, long long f, long long st,
-                                         float C, float d0, float d1, float T)
-{
-    const float cm2 = c[f - 2 * st], cm1 = c[f - st], c0 = c[f], cp1 = c[f + st];
-    const float dd0 = lmask(g, f - st, f, c0 - cm1);
-    const float ddm = lmask(g, f - 2 * st, f - st, cm1 - cm2);
-    const float ddp = lmask(g, f, f + st, cp1 - c0);
-    const float pp = psi(C, d0, d1, theta(ddm, dd0));
-    const float pm = psi(C, d0, d1, theta(ddp, dd0));
-    return 0.5f * (T + fabsf(T)) * (cm1 + pp * dd0) + 0.5f * (T - fabsf(T)) * (c0 - pm * dd0);
-}
-/* adv_flux_z at face k (between k-1 above and k below), stride sz. DST3FL.jl:93-95 */
-__device__ __forceinline__ float flux_z(const DevGrid& g, const float* __restrict__ c, long long f, long long sz,
-                                        float C, float d0, float d1, float T)
-{
-    const float cm2 = c[f - 2 * sz], cm1 = c[f - sz], c0 = c[f], cp1 = c[f + sz];
-    const float dd0 = lmask(g, f - sz, f, cm1 - c0);
-    const float ddm = lmask(g, f - 2 * sz, f - sz, cm2 - cm1);
-    const float ddp = lmask(g, f, f + sz, c0 - cp1);
-    const float pp = psi(C, d0, d1, theta(ddm, dd0));   /* Ψz⁺ */
-    const float pm = psi(C, d0, d1, theta(ddp, dd0));   /* Ψz⁻ */
-    return 0.5f * (T + fabsf(T)) * (c0 + pm * dd0) + 0.5f * (T - fabsf(T)) * (cm1 - pp * dd0);
+    return dmaxf(0.0f, dminf(dminf(1.0f, F.d0 + F.d1 * th), divc(th * F.omc, F.Ce, F.yCe)));
 }
 
+__device__ __forceinline__ bool small(float x) { return fabsf(x) < 0x1p100f; }   /* false for NaN */
 
-/* calc_Gcsˣ!/multi_dim_x! and calc_Gcsʸ!/multi_dim_y! for one interior cell */
-__global__ void __launch_bounds__(256) k_sweep(SweepArgs a)
+/* The flux of one tracer at one face is
+ *   ½(T+|T|)(bp + Ψ(θ(dnp, dd0))·dd0) + ½(T−|T|)(bm − Ψ(θ(dnm, dd0))·dd0)
+ *   x/y (DST3FL.jl:85-91): bp = c₋₁, dnp = δ₋₁, bm = c₀,  dnm = δ₊₁
+ *   z   (DST3FL.jl:93-95): bp = c₀,  dnp = δ₊₁, bm = c₋₁, dnm = δ₋₁ */
+struct Sten { float dd0, bp, dnp, bm, dnm; bool ok; };
+
+template <bool LAND, bool Z>
+__device__ __forceinline__ Sten stencil(const DevGrid& g, const float* p, uint32_t f, uint32_t s)
 {
-    const DevState* st = a.st;
-    if (st->err) return;
+    const float cm2 = ld(p, f - 2 * s), cm1 = ld(p, f - s), c0 = ld(p, f), cp1 = ld(p, f + s);
+    Sten r;
+    if (Z) {
+        r.dd0 = lmask<LAND>(g, f - s, f, cm1 - c0);
+        r.dnm = lmask<LAND>(g, f - 2 * s, f - s, cm2 - cm1);
+        r.dnp = lmask<LAND>(g, f, f + s, c0 - cp1);
+        r.bp = c0; r.bm = cm1;
+    } else {
+        r.dd0 = lmask<LAND>(g, f - s, f, c0 - cm1);
+        r.dnp = lmask<LAND>(g, f - 2 * s, f - s, cm1 - cm2);
+        r.dnm = lmask<LAND>(g, f, f + s, cp1 - c0);
+        r.bp = cm1; r.bm = c0;
+    }
+    r.ok = small(cm2) && small(cm1) && small(c0) && small(cp1);
+    return r;
+}
+/* the term of the sign of T only; bad = the full formula is needed */
+__device__ __forceinline__ float flux_fast(const Face& F, const Sten& s, bool& bad)
+{
+    const bool pos = F.T > 0.0f;
+    const float p = psi(F, theta(pos ? s.dnp : s.dnm, s.dd0));
+    const float pd = p * s.dd0;
+    const float r = F.T * (pos ? s.bp + pd : s.bm - pd);
+    bad = !(s.ok && F.fast && r != 0.0f);
+    return r;
+}
+__device__ __forceinline__ float flux_full(const Face& F, const Sten& s)
+{
+    const float T = F.T;
+    const float pp = psi(F, theta(s.dnp, s.dd0)), pm = psi(F, theta(s.dnm, s.dd0));
+    return 0.5f * (T + fabsf(T)) * (s.bp + pp * s.dd0) + 0.5f * (T - fabsf(T)) * (s.bm - pm * s.dd0);
+}
+/* the fluxes of tracers Q0..Q1-1 of slab at face f (between f − s and f);
+ * zero = Bounded end face.  The full formula is evaluated only where needed,
+ * after the fast terms of all these tracers, so the face is one block of
+ * straight-line code. */
+template <bool LAND, bool Z, int Q0, int Q1>
+__device__ __forceinline__ void face_fluxes(const DevGrid& g, const Face& F, bool zero, uint32_t f, uint32_t s,
+                                            const float* slab, float* out)
+{
+    uint32_t need = 0;
+#pragma unroll
+    for (int q = Q0; q < Q1; ++q) {
+        const Sten t = stencil<LAND, Z>(g, fld(slab, g, q), f, s);
+        bool bad;
+        const float r = flux_fast(F, t, bad);
+        out[q - Q0] = zero ? 0.0f : r;
+        need |= (uint32_t)(bad && !zero) << q;
+    }
+    if (__builtin_expect(need != 0, 0)) {
+#pragma unroll
+        for (int q = Q0; q < Q1; ++q)
+            if ((need >> q) & 1u) out[q - Q0] = flux_full(F, stencil<LAND, Z>(g, fld(slab, g, q), f, s));
+    }
+}
+
+/* ----------------------------------------------------------------- x sweep */
+template <bool LAND, int Q0, int Q1>
+__device__ __forceinline__ void sweep_x_half(const SweepArgs& a, const DevGrid& g, const Face& Fc, bool zero, bool upd,
+                                             uint32_t c, float coef, float dT, const float* C0)
+{
+    float fl[Q1 - Q0];
+    face_fluxes<LAND, false, Q0, Q1>(g, Fc, zero, c, 1, C0, fl);
+#pragma unroll
+    for (int q = Q0; q < Q1; ++q) {
+        const float f1 = __shfl_down(fl[q - Q0], 1, 64);
+        const float c0 = ld(fld(C0, g, q), c);
+        if (upd) st(fld(a.out, g, q), c, c0 - coef * (lmask<LAND>(g, c, c + 1, f1 - fl[q - Q0]) - c0 * dT));
+    }
+}
+
+template <bool LAND>
+__global__ void __launch_bounds__(256) k_sweep_x(SweepArgs a)
+{
+    const DevState* S = a.st;
+    if (S->err) return;
     const DevGrid& g = a.g;
-    const long long nint = (long long)g.Nx * g.Ny * g.Nz;
-    const float* const* C0 = st->tr[st->tcur];
-    const long long stride = a.dim == 0 ? 1 : g.Px;
+    const long long nrow = (long long)g.Ny * g.Nz;
+    const long long nface = nrow * (g.Nx + 1);
+    const long long gt = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const long long F = (gt >> 6) * 63 + lane;
+    const bool valid = F < nface;
+    const long long Fv = valid ? F : nface - 1;
+    const long long row = Fv / (g.Nx + 1);
+    const int fi = (int)(Fv - row * (g.Nx + 1));          /* 0..Nx: west face of interior cell fi */
+    const int i = fi + 1 + g.Hx;
+    const int j = (int)(row % g.Ny) + 1 + g.Hy;
+    const int k = (int)(row / g.Ny) + 1 + g.Hz;
+    const uint32_t c = (uint32_t)gidx(g, i, j, k);
+    const float* C0 = S->tr[S->tcur][0];
     const float dt = a.dt;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nint; t += (long long)gridDim.x * blockDim.x) {
+    const float A = g.dy * dzF_at(g, k);
+    const Face Fc = face(g, ld(a.vel, c), dt, g.dx, g.ydx, A);
+    const bool zero = fi == g.Nx && g.tx == PI_BOUNDED;
+    const bool upd = valid && lane < 63 && fi < g.Nx;
+    const float T1 = __shfl_down(Fc.T, 1, 64);
+    const float dT = lmask<LAND>(g, c, c + 1, T1 - Fc.T);
+    const float coef = divc(dt, g.vol[k - 1], g.yvol[k - 1]);
+    sweep_x_half<LAND, 0, 7>(a, g, Fc, zero, upd, c, coef, dT, C0);
+    sweep_x_half<LAND, 7, 14>(a, g, Fc, zero, upd, c, coef, dT, C0);
+}
+
+/* ----------------------------------------------------------------- y sweep */
+#define RUN_Y 16
+
+template <bool LAND, int Q0, int Q1>
+__device__ __forceinline__ void sweep_y_half(const SweepArgs& a, const DevGrid& g, const Face& Fb, const Face& Fa,
+                                             bool zero, uint32_t c, uint32_t sy, float coef,
+                                             const float* C0, float* fl)
+{
+    float f[Q1 - Q0];
+    face_fluxes<LAND, false, Q0, Q1>(g, Fb, zero, c, sy, a.prev, f);
+    const uint32_t cs = c - sy;    /* the cell south of face c */
+    const float dT = lmask<LAND>(g, cs, c, Fb.T - Fa.T);
+#pragma unroll
+    for (int q = Q0; q < Q1; ++q) {
+        st(fld(a.out, g, q), cs,
+           ld(fld(a.prev, g, q), cs) - coef * (lmask<LAND>(g, cs, c, f[q - Q0] - fl[q]) - ld(fld(C0, g, q), cs) * dT));
+        fl[q] = f[q - Q0];
+    }
+}
+
+template <bool LAND>
+__global__ void __launch_bounds__(256) k_sweep_y(SweepArgs a)
+{
+    const DevState* S = a.st;
+    if (S->err) return;
+    const DevGrid& g = a.g;
+    const int nrun = (g.Ny + RUN_Y - 1) / RUN_Y;
+    const long long nthr = (long long)g.Nx * nrun * g.Nz;
+    const uint32_t sy = (uint32_t)g.Px;
+    const float* C0 = S->tr[S->tcur][0];
+    const float dt = a.dt;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nthr; t += (long long)gridDim.x * blockDim.x) {
         const int i = (int)(t % g.Nx) + 1 + g.Hx;
-        const int j = (int)((t / g.Nx) % g.Ny) + 1 + g.Hy;
-        const int k = (int)(t / ((long long)g.Nx * g.Ny)) + 1 + g.Hz;
-        const long long c = gidx(g, i, j, k);
-        const int pos = a.dim == 0 ? i : j;
-        const int N = a.dim == 0 ? g.Nx : g.Ny, H = a.dim == 0 ? g.Hx : g.Hy;
-        const int topo = a.dim == 0 ? g.tx : g.ty;
-        const float D = a.dim == 0 ? g.dx : g.dy;
-        const float A = (a.dim == 0 ? g.dy : g.dx) * dzF_at(g, k);            /* Ax / Ay */
-        const float vol = g.dx * g.dy * dzF_at(g, k);
-        const float u0 = a.vel[c], u1 = a.vel[c + stride];
-        const float Cf0 = fabsf(u0 * dt / D), Cf1 = fabsf(u1 * dt / D);
-        const float T0 = A * u0, T1 = A * u1;
-        const float a0 = (2.0f - Cf0) * (1.0f - Cf0) / 6.0f, b0 = (1.0f - Cf0 * Cf0) / 6.0f;
-        const float a1 = (2.0f - Cf1) * (1.0f - Cf1) / 6.0f, b1 = (1.0f - Cf1 * Cf1) / 6.0f;
-        const bool last_bounded = (topo == PI_BOUNDED) && (pos == N + H);
-        const float dT = lmask(g, c, c + stride, T1 - T0);
-        const float coef = dt / vol;
-        for (int q = 0; q < PI_NTRACERS; ++q) {
-            const float* src = a.prev[q] ? a.prev[q] : C0[q];
-            const float F0 = flux_xy(g, src, c, stride, Cf0, a0, b0, T0);
-            const float F1 = last_bounded ? 0.0f : flux_xy(g, src, c + stride, stride, Cf1, a1, b1, T1);
-            const float dG = lmask(g, c, c + stride, F1 - F0);
-            a.out[q][c] = src[c] - coef * (dG - C0[q][c] * dT);
+        const int r = (int)((t / g.Nx) % nrun);
+        const int k = (int)(t / ((long long)g.Nx * nrun)) + 1 + g.Hz;
+        const int j0 = g.Hy + 1 + r * RUN_Y;
+        const int j1 = min(j0 + RUN_Y, g.Ny + g.Hy + 1);     /* cells j0..j1-1, faces j0..j1 */
+        const float A = g.dx * dzF_at(g, k);
+        const float coef = divc(dt, g.vol[k - 1], g.yvol[k - 1]);
+        uint32_t c = (uint32_t)gidx(g, i, j0, k);            /* cell north of the current face */
+        float fl[PI_NTRACERS];                               /* fluxes of the previous face */
+        Face Fa = face(g, ld(a.vel, c), dt, g.dy, g.ydy, A);
+        face_fluxes<LAND, false, 0, 7>(g, Fa, false, c, sy, a.prev, fl);
+        face_fluxes<LAND, false, 7, 14>(g, Fa, false, c, sy, a.prev, fl + 7);
+#pragma unroll 1
+        for (int jj = j0 + 1; jj <= j1; ++jj) {
+            c += sy;
+            const Face Fb = face(g, ld(a.vel, c), dt, g.dy, g.ydy, A);
+            const bool zero = jj == g.Ny + g.Hy + 1 && g.ty == PI_BOUNDED;
+            sweep_y_half<LAND, 0, 7>(a, g, Fb, Fa, zero, c, sy, coef, C0, fl);
+            sweep_y_half<LAND, 7, 14>(a, g, Fb, Fa, zero, c, sy, coef, C0, fl);
+            Fa = Fb;
         }
     }
 }
@@ -105,22 +259,21 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs a)
 /* fill_halo_{west,east} / {south,north} / {top,bottom} of one dimension, on the
  * lines given by (o1, o2) ranges.  lines_full = 1 covers every padded line (the
  * reference fills whole planes); 0 covers interior lines only. */
-
 __global__ void __launch_bounds__(256) k_fill_line(FillArgs a)
 {
-    const DevState* st = a.st;
-    if (st->err) return;
+    const DevState* S = a.st;
+    if (S->err) return;
     const DevGrid& g = a.g;
     const int N = a.dim == 0 ? g.Nx : a.dim == 1 ? g.Ny : g.Nz;
     const int H = a.dim == 0 ? g.Hx : a.dim == 1 ? g.Hy : g.Hz;
     const int topo = a.dim == 0 ? g.tx : a.dim == 1 ? g.ty : g.tz;
     const long long stride = a.dim == 0 ? 1 : a.dim == 1 ? g.Px : (long long)g.Px * g.Py;
-    /* the two other dimensions */
     const int P1 = a.dim == 0 ? g.Py : g.Px, P2 = a.dim == 2 ? g.Py : g.Pz;
     const int N1 = a.dim == 0 ? g.Ny : g.Nx, N2 = a.dim == 2 ? g.Ny : g.Nz;
     const int H1 = a.dim == 0 ? g.Hy : g.Hx, H2 = a.dim == 2 ? g.Hy : g.Hz;
     const long long n1 = a.lines_full ? P1 : N1, n2 = a.lines_full ? P2 : N2;
     const long long nlines = n1 * n2 * a.nf;
+    float* slab = a.f ? a.f : S->tr[S->tcur ^ 1][0];
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nlines; t += (long long)gridDim.x * blockDim.x) {
         const int q = (int)(t / (n1 * n2));
         const long long l = t % (n1 * n2);
@@ -130,7 +283,7 @@ __global__ void __launch_bounds__(256) k_fill_line(FillArgs a)
         if (a.dim == 0) base = gidx(g, 1, o1, o2);
         else if (a.dim == 1) base = gidx(g, o1, 1, o2);
         else base = gidx(g, o1, o2, 1);
-        float* c = a.f[q] ? a.f[q] : st->tr[st->tcur ^ 1][q];
+        float* c = fld(slab, g, q);
         #define AT(p) c[base + (long long)((p) - 1) * stride]
         if (topo == PI_PERIODIC) {
             float tmp[8];
@@ -148,109 +301,195 @@ __global__ void __launch_bounds__(256) k_fill_line(FillArgs a)
     }
 }
 
+/* ---------------------------------------------- z sweep + the rest of the update */
+#define RUN_Z 16
 
+/* the tracers that receive particle exchange (plk), tracer_update.jl:22-43 */
+__device__ __forceinline__ constexpr bool has_plk(int q)
+{
+    return q == PI_DIC || q == PI_NH4 || q == PI_NO3 || q == PI_PO4 || q == PI_DOC || q == PI_DON ||
+           q == PI_DOP || q == PI_POC || q == PI_PON || q == PI_POP;
+}
+
+struct ZCell {      /* cell cc (level k) above face c */
+    uint32_t cc, c;
+    int k;
+    float dT, coef, khAx, khAy, inv_vol, dzf, vol;
+    double ydzf, yvol;
+};
+
+/* Gcs, diffusion, sinking, forcing and apply_tendency of tracers Q0..Q1-1 at
+ * one cell; cu (C0 at the cell), cs (C0 above it, sinking tracers), fz and dz
+ * (advective and diffusive fluxes of its top face) are carried along z. */
+template <bool LAND, int Q0, int Q1>
+__device__ __forceinline__ void final_half(const FinalArgs& a, const DevGrid& g, const Face& Fb, bool zero,
+                                           const ZCell& z, const float* tt, float* cu, float* fz, float* dz, float* cs)
+{
+    const float* C0 = a.st->tr[a.st->tcur][0];
+    float* OUT = a.st->tr[a.st->tcur ^ 1][0];
+    const uint32_t sz = (uint32_t)g.Px * (uint32_t)g.Py, sy = (uint32_t)g.Px;
+    const uint32_t cc = z.cc, c = z.c;
+    const int k = z.k;
+    const float dt = a.dt;
+    const float kvAz = a.kv * (g.dx * g.dy);
+    float f[Q1 - Q0];
+    face_fluxes<LAND, true, Q0, Q1>(g, Fb, zero, c, sz, a.prev, f);
+#pragma unroll
+    for (int q = Q0; q < Q1; ++q) {
+        const float* co = fld(C0, g, q);
+        const float cq = cu[q];
+        const float cn = ld(co, c);
+        /* κv·Az·(δz c / dzF) at face c (Advection/tracer_diffusion.jl:1-30) */
+        const float d = kvAz * divc(lmask<LAND>(g, cc, c, cq - cn), z.dzf, z.ydzf);
+        const float temp3 = ld(fld(a.prev, g, q), cc) - z.coef * (lmask<LAND>(g, cc, c, fz[q] - f[q - Q0]) - cq * z.dT);
+        /* Gcs = tracer_temp - tracer (calc_tracer_tendency!) */
+        float G = temp3 - cq;
+        const float fxa = z.khAx * divc(lmask<LAND>(g, cc - 1, cc, cq - ld(co, cc - 1)), g.dx, g.ydx);
+        const float fxb = z.khAx * divc(lmask<LAND>(g, cc, cc + 1, ld(co, cc + 1) - cq), g.dx, g.ydx);
+        const float fya = z.khAy * divc(lmask<LAND>(g, cc - sy, cc, cq - ld(co, cc - sy)), g.dy, g.ydy);
+        const float fyb = z.khAy * divc(lmask<LAND>(g, cc, cc + sy, ld(co, cc + sy) - cq), g.dy, g.ydy);
+        const float ddx = lmask<LAND>(g, cc, cc + 1, fxb - fxa);
+        const float ddy = lmask<LAND>(g, cc, cc + sy, fyb - fya);
+        const float ddz = lmask<LAND>(g, cc, c, dz[q] - d);
+        const float lap = z.inv_vol * (ddx + ddy + ddz);
+        G = G + lap * dt;
+        /* tracer_sinking! (Advection/tracer_sinking.jl:1-43) */
+        if (q == PI_PFe_inorg || q == PI_Dust || q == PI_PFe_bio) {
+            const int si = q == PI_PFe_inorg ? 0 : q == PI_Dust ? 1 : 2;
+            const float ws = q == PI_PFe_bio ? a.wsink_org : a.wsink_inorg;
+            const float f0 = (k - 1 <= g.Hz) ? 0.0f : ws * cs[si];
+            const float f1 = (k >= g.Nz + g.Hz) ? 0.0f : ws * cq;
+            G = G + div_dzC(g, f0 - f1, k - g.Hz) * dt;
+            cs[si] = cq;
+        }
+        /* tracer_forcing! (tracer_forcings.jl:2-50) */
+        switch (q) {
+            case PI_DIC: G = G + tt[PI_DOC] * a.kDOC * dt; break;
+            case PI_DOC: G = G - tt[PI_DOC] * a.kDOC * dt + tt[PI_POC] * a.kPOC * dt; break;
+            case PI_POC: G = G - tt[PI_POC] * a.kPOC * dt; break;
+            case PI_NH4: G = G + tt[PI_DON] * a.kDON * dt - tt[PI_NH4] * a.Nit * dt; break;
+            case PI_NO3: G = G + tt[PI_NH4] * a.Nit * dt; break;
+            case PI_DON: G = G - tt[PI_DON] * a.kDON * dt + tt[PI_PON] * a.kPON * dt; break;
+            case PI_PON: G = G - tt[PI_PON] * a.kPON * dt; break;
+            case PI_PO4: G = G + tt[PI_DOP] * a.kDOP * dt; break;
+            case PI_DOP: G = G - tt[PI_DOP] * a.kDOP * dt + tt[PI_POP] * a.kPOP * dt; break;
+            case PI_POP: G = G - tt[PI_POP] * a.kPOP * dt; break;
+            case PI_DFe: {
+                const float DFe = tt[PI_DFe];
+                const float ex = dmaxf((DFe - a.lig), 0.0f);
+                G = G + tt[PI_PFe_bio] * a.kPOC * dt -
+                    (a.lPOC * tt[PI_POC] + a.lmin + a.ldust * tt[PI_Dust]) * DFe * a.DFeFrac * dt +
+                    tt[PI_PFe_inorg] * a.kdiss * dt -
+                    ex * DFe * a.DFeFrac * a.lFe * dt;
+                break;
+            }
+            case PI_PFe_bio:
+                G = G + tt[PI_POC] * tt[PI_DFe] * a.DFeFrac * a.lPOC * dt - a.kPOC * tt[PI_PFe_bio] * dt;
+                break;
+            case PI_PFe_inorg: {
+                const float DFe = tt[PI_DFe];
+                const float ex = dmaxf((DFe - a.lig), 0.0f);
+                G = G - tt[PI_PFe_inorg] * a.kdiss * dt +
+                    (a.lmin + a.ldust * tt[PI_Dust]) * DFe * a.DFeFrac * dt +
+                    ex * DFe * a.DFeFrac * a.lFe * dt;
+                break;
+            }
+            default: break;
+        }
+        /* apply_tendency_kernel! (tracer_update.jl:6-20); the tracers without
+         * particle exchange add a zero plk */
+        const float pk = has_plk(q) ? ld(fld(a.plk, g, q), cc) : 0.0f;
+        st(fld(OUT, g, q), cc, cq + (G + divc(pk, z.vol, z.yvol)));
+        fz[q] = f[q - Q0]; dz[q] = d; cu[q] = cn;
+    }
+}
+
+template <bool LAND>
 __global__ void __launch_bounds__(256) k_final(FinalArgs a)
 {
-    const DevState* st = a.st;
-    if (st->err) return;
+    const DevState* S = a.st;
+    if (S->err) return;
     const DevGrid& g = a.g;
-    const long long nint = (long long)g.Nx * g.Ny * g.Nz;
-    const float* const* C0 = st->tr[st->tcur];
-    float* const* OUT = st->tr[st->tcur ^ 1];
-    const long long sz = (long long)g.Px * g.Py, sy = g.Px;
+    const int nrun = (g.Nz + RUN_Z - 1) / RUN_Z;
+    const long long nthr = (long long)g.Nx * g.Ny * nrun;
+    const float* C0 = S->tr[S->tcur][0];
+    const uint32_t sz = (uint32_t)g.Px * (uint32_t)g.Py;
     const float dt = a.dt;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nint; t += (long long)gridDim.x * blockDim.x) {
+    const float Az = g.dx * g.dy;
+    const float kvAz = a.kv * Az;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nthr; t += (long long)gridDim.x * blockDim.x) {
         const int i = (int)(t % g.Nx) + 1 + g.Hx;
         const int j = (int)((t / g.Nx) % g.Ny) + 1 + g.Hy;
-        const int k = (int)(t / ((long long)g.Nx * g.Ny)) + 1 + g.Hz;
-        const long long c = gidx(g, i, j, k);
-        const float vol = g.dx * g.dy * dzF_at(g, k);
-        /* z sweep faces k and k+1 */
-        const float Az = g.dx * g.dy;
-        const float w0 = a.w[c], w1 = a.w[c + sz];
-        const float Cf0 = fabsf(w0 * dt / dzC_at(g, k)), Cf1 = fabsf(w1 * dt / dzC_at(g, k + 1));
-        const float T0 = Az * w0, T1 = Az * w1;
-        const float a0 = (2.0f - Cf0) * (1.0f - Cf0) / 6.0f, b0 = (1.0f - Cf0 * Cf0) / 6.0f;
-        const float a1 = (2.0f - Cf1) * (1.0f - Cf1) / 6.0f, b1 = (1.0f - Cf1 * Cf1) / 6.0f;
-        const bool last_bounded = (g.tz == PI_BOUNDED) && (k == g.Nz + g.Hz);
-        const float dT = lmask(g, c, c + sz, T0 - T1);
-        const float coef = dt / vol;
-        /* tracer_temp = max(0, c) for the forcing (tracer_forcings.jl:3-5) */
-        float tt[PI_NTRACERS];
-        for (int q = 0; q < PI_NTRACERS; ++q) tt[q] = jl_maxf(0.0f, C0[q][c]);
-        const float Ax = g.dy * dzF_at(g, k), Ay = g.dx * dzF_at(g, k);
-        for (int q = 0; q < PI_NTRACERS; ++q) {
-            const float* src = a.prev[q];
-            const float* co = C0[q];
-            const float F0 = flux_z(g, src, c, sz, Cf0, a0, b0, T0);
-            const float F1 = last_bounded ? 0.0f : flux_z(g, src, c + sz, sz, Cf1, a1, b1, T1);
-            const float dG = lmask(g, c, c + sz, F0 - F1);
-            const float temp3 = src[c] - coef * (dG - co[c] * dT);
-            /* Gcs = tracer_temp - tracer (calc_tracer_tendency!) */
-            float G = temp3 - co[c];
-            /* tracer_diffusion! (Advection/tracer_diffusion.jl:1-30) on the original c */
-            {
-                const float fxa = a.kh * Ax * (lmask(g, c - 1, c, co[c] - co[c - 1]) / g.dx);
-                const float fxb = a.kh * Ax * (lmask(g, c, c + 1, co[c + 1] - co[c]) / g.dx);
-                const float fya = a.kh * Ay * (lmask(g, c - sy, c, co[c] - co[c - sy]) / g.dy);
-                const float fyb = a.kh * Ay * (lmask(g, c, c + sy, co[c + sy] - co[c]) / g.dy);
-                const float fza = a.kv * Az * (lmask(g, c - sz, c, co[c - sz] - co[c]) / dzF_at(g, k));
-                const float fzb = a.kv * Az * (lmask(g, c, c + sz, co[c] - co[c + sz]) / dzF_at(g, k + 1));
-                const float ddx = lmask(g, c, c + 1, fxb - fxa);
-                const float ddy = lmask(g, c, c + sy, fyb - fya);
-                const float ddz = lmask(g, c, c + sz, fza - fzb);
-                const float lap = 1.0f / vol * (ddx + ddy + ddz);
-                G = G + lap * dt;
+        const int r = (int)(t / ((long long)g.Nx * g.Ny));
+        const int k0 = g.Hz + 1 + r * RUN_Z;
+        const int k1 = min(k0 + RUN_Z, g.Nz + g.Hz + 1);     /* cells k0..k1-1, faces k0..k1 */
+        uint32_t c = (uint32_t)gidx(g, i, j, k0);            /* cell below the current face */
+        float cu[PI_NTRACERS], cs[3], fz[PI_NTRACERS], dz[PI_NTRACERS];
+        Face Fa = face(g, ld(a.w, c), dt, dzC_at(g, k0), g.ydzC[k0 - 1], Az);
+        face_fluxes<LAND, true, 0, 7>(g, Fa, false, c, sz, a.prev, fz);
+        face_fluxes<LAND, true, 7, 14>(g, Fa, false, c, sz, a.prev, fz + 7);
+        {
+            const float dzf = dzF_at(g, k0);
+            const double ydzf = g.ydzF[k0 - 1];
+#pragma unroll
+            for (int q = 0; q < PI_NTRACERS; ++q) {
+                const float* co = fld(C0, g, q);
+                const float above = ld(co, c - sz), here = ld(co, c);
+                dz[q] = kvAz * divc(lmask<LAND>(g, c - sz, c, above - here), dzf, ydzf);
+                cu[q] = here;
             }
-            /* tracer_sinking! (Advection/tracer_sinking.jl:1-43) */
-            if (q == PI_PFe_inorg || q == PI_Dust || q == PI_PFe_bio) {
-                const float ws = q == PI_PFe_bio ? a.wsink_org : a.wsink_inorg;
-                const float fa = (k - 1 <= g.Hz) ? 0.0f : ws * co[c - sz];
-                const float fb = (k >= g.Nz + g.Hz) ? 0.0f : ws * co[c];
-                G = G + (fa - fb) / dzC_at(g, k - g.Hz) * dt;
-            }
-            /* tracer_forcing! (tracer_forcings.jl:2-50) */
-            switch (q) {
-                case PI_DIC: G = G + tt[PI_DOC] * a.kDOC * dt; break;
-                case PI_DOC: G = G - tt[PI_DOC] * a.kDOC * dt + tt[PI_POC] * a.kPOC * dt; break;
-                case PI_POC: G = G - tt[PI_POC] * a.kPOC * dt; break;
-                case PI_NH4: G = G + tt[PI_DON] * a.kDON * dt - tt[PI_NH4] * a.Nit * dt; break;
-                case PI_NO3: G = G + tt[PI_NH4] * a.Nit * dt; break;
-                case PI_DON: G = G - tt[PI_DON] * a.kDON * dt + tt[PI_PON] * a.kPON * dt; break;
-                case PI_PON: G = G - tt[PI_PON] * a.kPON * dt; break;
-                case PI_PO4: G = G + tt[PI_DOP] * a.kDOP * dt; break;
-                case PI_DOP: G = G - tt[PI_DOP] * a.kDOP * dt + tt[PI_POP] * a.kPOP * dt; break;
-                case PI_POP: G = G - tt[PI_POP] * a.kPOP * dt; break;
-                case PI_DFe: {
-                    const float DFe = tt[PI_DFe];
-                    const float ex = jl_maxf((DFe - a.lig), 0.0f);
-                    G = G + tt[PI_PFe_bio] * a.kPOC * dt -
-                        (a.lPOC * tt[PI_POC] + a.lmin + a.ldust * tt[PI_Dust]) * DFe * a.DFeFrac * dt +
-                        tt[PI_PFe_inorg] * a.kdiss * dt -
-                        ex * DFe * a.DFeFrac * a.lFe * dt;
-                    break;
-                }
-                case PI_PFe_bio:
-                    G = G + tt[PI_POC] * tt[PI_DFe] * a.DFeFrac * a.lPOC * dt - a.kPOC * tt[PI_PFe_bio] * dt;
-                    break;
-                case PI_PFe_inorg: {
-                    const float DFe = tt[PI_DFe];
-                    const float ex = jl_maxf((DFe - a.lig), 0.0f);
-                    G = G - tt[PI_PFe_inorg] * a.kdiss * dt +
-                        (a.lmin + a.ldust * tt[PI_Dust]) * DFe * a.DFeFrac * dt +
-                        ex * DFe * a.DFeFrac * a.lFe * dt;
-                    break;
-                }
-                default: break;
-            }
-            /* apply_tendency_kernel! (tracer_update.jl:6-20) */
-            const float pk = a.plk[q] ? a.plk[q][c] : 0.0f;
-            OUT[q][c] = co[c] + (G + pk / vol);
+            cs[0] = ld(fld(C0, g, PI_PFe_inorg), c - sz);
+            cs[1] = ld(fld(C0, g, PI_Dust), c - sz);
+            cs[2] = ld(fld(C0, g, PI_PFe_bio), c - sz);
+        }
+#pragma unroll 1
+        for (int kk = k0 + 1; kk <= k1; ++kk) {
+            c += sz;
+            ZCell z;
+            z.c = c; z.cc = c - sz; z.k = kk - 1;
+            const Face Fb = face(g, ld(a.w, c), dt, dzC_at(g, kk), g.ydzC[kk - 1], Az);
+            const bool zero = kk == g.Nz + g.Hz + 1 && g.tz == PI_BOUNDED;
+            z.dT = lmask<LAND>(g, z.cc, c, Fa.T - Fb.T);
+            z.vol = g.vol[z.k - 1];
+            z.yvol = g.yvol[z.k - 1];
+            z.coef = divc(dt, z.vol, z.yvol);
+            z.khAx = a.kh * (g.dy * dzF_at(g, z.k));
+            z.khAy = a.kh * (g.dx * dzF_at(g, z.k));
+            z.inv_vol = divc(1.0f, z.vol, z.yvol);
+            z.dzf = dzF_at(g, kk);
+            z.ydzf = g.ydzF[kk - 1];
+            float tt[PI_NTRACERS];    /* tracer_temp = max(0, c) (tracer_forcings.jl:3-5) */
+#pragma unroll
+            for (int q = 0; q < PI_NTRACERS; ++q) tt[q] = dmaxf(0.0f, cu[q]);
+            final_half<LAND, 0, 7>(a, g, Fb, zero, z, tt, cu, fz, dz, cs);
+            final_half<LAND, 7, 14>(a, g, Fb, zero, z, tt, cu, fz, dz, cs);
+            Fa = Fb;
         }
     }
 }
 
 /* ------------------------------------------------------------- launchers */
-void launch_sweep(hipStream_t s, int blocks, const SweepArgs& a) { hipLaunchKernelGGL(k_sweep, dim3(blocks), dim3(256), 0, s, a); }
+void launch_sweep(hipStream_t s, int blocks, const SweepArgs& a)
+{
+    const bool land = a.g.has_land != 0;
+    if (a.dim == 0) {
+        const long long nface = (long long)a.g.Ny * a.g.Nz * (a.g.Nx + 1);
+        const long long nwave = (nface + 62) / 63;
+        const dim3 grid((unsigned)((nwave + 3) / 4));
+        if (land) hipLaunchKernelGGL(k_sweep_x<true>, grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(k_sweep_x<false>, grid, dim3(256), 0, s, a);
+    } else {
+        const long long n = (long long)a.g.Nx * ((a.g.Ny + RUN_Y - 1) / RUN_Y) * a.g.Nz;
+        const dim3 grid(grid_for(n, 256, 8192));
+        if (land) hipLaunchKernelGGL(k_sweep_y<true>, grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(k_sweep_y<false>, grid, dim3(256), 0, s, a);
+    }
+}
 void launch_fill(hipStream_t s, int blocks, const FillArgs& a) { hipLaunchKernelGGL(k_fill_line, dim3(blocks), dim3(256), 0, s, a); }
-void launch_final(hipStream_t s, int blocks, const FinalArgs& a) { hipLaunchKernelGGL(k_final, dim3(blocks), dim3(256), 0, s, a); }
+void launch_final(hipStream_t s, int blocks, const FinalArgs& a)
+{
+    const long long n = (long long)a.g.Nx * a.g.Ny * ((a.g.Nz + RUN_Z - 1) / RUN_Z);
+    const dim3 grid(grid_for(n, 256, 8192));
+    if (a.g.has_land) hipLaunchKernelGGL(k_final<true>, grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_final<false>, grid, dim3(256), 0, s, a);
+}

@@ -44,7 +44,31 @@ struct DevGrid {
     const float* dzC;   /* Pz, 0-based (reference index k -> dzC[k-1]) */
     const float* dzF;
     const uint8_t* lm;  /* padded landmask */
+    /* reciprocals of the fixed divisors, RN_double(1/b), for divc */
+    double ydx, ydy, y6;
+    const double* ydzC;  /* Pz */
+    const double* ydzF;  /* Pz */
+    const double* yvol;  /* Pz */
+    const float* vol;    /* Pz: (Δx·Δy)·dzF[k], the cell volume as the reference forms it */
 };
+
+/* a / b correctly rounded, without branches, given yd = RN_double(1/b).
+ * In binary64, q0 = RN(a·yd) is within an ulp of a/b and yd within half an
+ * ulp of 1/b, so one Markstein step (exact residual r = a − b·q0 by FMA, then
+ * RN(q0 + r·yd)) gives RN_double(a/b); no step can underflow or overflow in
+ * binary64 for float operands.  Rounding RN_double(a/b) to float is RN_float(a/b):
+ * 53 >= 2·24 + 2 (double rounding is innocuous for division), and an exact
+ * quotient that is a float rounding boundary is representable in binary64.
+ * When r is zero or NaN (exact quotient, a or b zero or infinite) q0 is already
+ * the IEEE result, signed zeros and infinities included. */
+__device__ __forceinline__ float divc(float a, float b, double yd)
+{
+    const double ad = (double)a, bd = (double)b;
+    const double q0 = ad * yd;
+    const double r = __builtin_fma(-bd, q0, ad);
+    const double q = __builtin_fma(r, yd, q0);
+    return (float)((r < 0.0 || r > 0.0) ? q : q0);
+}
 
 struct Store {
     uint32_t* key;
@@ -78,6 +102,18 @@ __device__ __forceinline__ float dzC_at(const DevGrid& g, long long k) {   /* cl
     return g.dzC[k - 1];
 }
 __device__ __forceinline__ float dzF_at(const DevGrid& g, int k) { return g.dzF[k - 1]; }
+/* Julia's max / min of Float32 (NaN if either operand is NaN, -0 < +0) are the
+ * IEEE 754-2019 maximum / minimum: one instruction each on gfx950.  For NaN
+ * operands the NaN may carry another payload than Julia's x - y. */
+__device__ __forceinline__ float dmaxf(float x, float y) { return __builtin_elementwise_maximum(x, y); }
+__device__ __forceinline__ float dminf(float x, float y) { return __builtin_elementwise_minimum(x, y); }
+
+/* a / dzC_at(g, k) (same clamp) */
+__device__ __forceinline__ float div_dzC(const DevGrid& g, float a, long long k) {
+    if (k < 1) k = 1;
+    if (k > g.Pz) k = g.Pz;
+    return divc(a, g.dzC[k - 1], g.ydzC[k - 1]);
+}
 
 /* launch helpers */
 static inline int grid_for(long long n, int block, int maxblocks) {

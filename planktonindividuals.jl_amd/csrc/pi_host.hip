@@ -52,7 +52,8 @@ struct pi_model {
     int v0 = 0, v1 = 0;        /* current buffer sets of vel₀ and vel₁ */
     bool vel_same = true;
     float *Chl = nullptr, *pop = nullptr, *temp = nullptr, *PARF_in = nullptr, *PARF_out = nullptr;
-    float* plk[PI_NTRACERS]{};
+    float* plk[PI_NTRACERS]{};  /* null for the tracers without particle exchange */
+    float* plk_slab = nullptr;
     float* diagPAR = nullptr;
     float* diag = nullptr;     /* [sp][4][ncell] */
     std::vector<int> diag_ids;
@@ -177,7 +178,7 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     g.tx = gd.topo_x; g.ty = gd.topo_y; g.tz = gd.topo_z;
     g.dx = gd.dx; g.dy = gd.dy;
     g.ncell = (long long)g.Px * g.Py * g.Pz;
-    if (g.ncell >= (1ll << 32) - 2) return fail(PI_ERR_ARG, "grid too large for 32-bit cell ids");
+    if (g.ncell >= (1ll << 30)) return fail(PI_ERR_ARG, "grid too large: the padded grid must have fewer than 2^30 cells");
     m->ncell = g.ncell;
     m->nint = (long long)g.Nx * g.Ny * g.Nz;
     m->nsp = d->n_species;
@@ -193,6 +194,26 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     HIPCHECK(hipMemcpy(dzC, gd.dzC, g.Pz * 4, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(dzF, gd.dzF, g.Pz * 4, hipMemcpyHostToDevice));
     g.dzC = dzC; g.dzF = dzF;
+    {   /* RN_double(1/b) of the fixed divisors Δx, Δy, 6, dzC[k], dzF[k] and of the
+         * cell volume (Δx·Δy)·dzF[k] (divc in pi_device.h) */
+        const int Pz = g.Pz;
+        g.ydx = 1.0 / (double)g.dx; g.ydy = 1.0 / (double)g.dy; g.y6 = 1.0 / 6.0;
+        std::vector<double> t(3 * (size_t)Pz);
+        std::vector<float> vol(Pz);
+        for (int k = 0; k < Pz; ++k) {
+            vol[k] = (g.dx * g.dy) * gd.dzF[k];
+            t[k] = 1.0 / (double)gd.dzC[k];
+            t[Pz + k] = 1.0 / (double)gd.dzF[k];
+            t[2 * Pz + k] = 1.0 / (double)vol[k];
+        }
+        double* tabs;
+        float* dvol;
+        DALLOC(tabs, 3ll * Pz * 8);
+        DALLOC(dvol, 4ll * Pz);
+        HIPCHECK(hipMemcpy(tabs, t.data(), 3ll * Pz * 8, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dvol, vol.data(), 4ll * Pz, hipMemcpyHostToDevice));
+        g.ydzC = tabs; g.ydzF = tabs + Pz; g.yvol = tabs + 2 * Pz; g.vol = dvol;
+    }
     g.has_land = 0; g.lm = nullptr;
     if (gd.landmask) {
         bool any0 = false;
@@ -229,15 +250,21 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     DALLOC(T.death, m->ctot * 4);
     /* grid fields */
     const long long nc = g.ncell;
-    for (int b = 0; b < 2; ++b) for (int q = 0; q < PI_NTRACERS; ++q) DALLOC(hs.tr[b][q], nc * 4);
-    for (int q = 0; q < PI_NTRACERS; ++q) { DALLOC(m->T1[q], nc * 4); DALLOC(m->T2[q], nc * 4); }
+    /* tracer sets as slabs of 14 fields (field q at slab + q·ncell) */
+    for (int b = 0; b < 2; ++b) {
+        DALLOC(hs.tr[b][0], PI_NTRACERS * nc * 4);
+        for (int q = 1; q < PI_NTRACERS; ++q) hs.tr[b][q] = hs.tr[b][0] + q * nc;
+    }
+    DALLOC(m->T1[0], PI_NTRACERS * nc * 4); DALLOC(m->T2[0], PI_NTRACERS * nc * 4);
+    for (int q = 1; q < PI_NTRACERS; ++q) { m->T1[q] = m->T1[0] + q * nc; m->T2[q] = m->T2[0] + q * nc; }
     for (int b = 0; b < 3; ++b) for (int c = 0; c < 3; ++c) DALLOC(m->vel[b][c], nc * 4);
     m->v0 = 0; m->v1 = 0; m->vel_same = true;
     DALLOC(hs.par[0], nc * 4); DALLOC(hs.par[1], nc * 4);
     DALLOC(m->Chl, nc * 4); DALLOC(m->pop, nc * 4); DALLOC(m->temp, nc * 4);
     DALLOC(m->PARF_in, (long long)g.Nx * g.Ny * 4); DALLOC(m->PARF_out, (long long)g.Nx * g.Ny * 4);
     const int plk_ids[10] = {PI_DIC, PI_NH4, PI_NO3, PI_PO4, PI_DOC, PI_DON, PI_DOP, PI_POC, PI_PON, PI_POP};
-    for (int q : plk_ids) DALLOC(m->plk[q], nc * 4);
+    DALLOC(m->plk_slab, PI_NTRACERS * nc * 4);
+    for (int q : plk_ids) m->plk[q] = m->plk_slab + q * nc;
     DALLOC(m->diagPAR, nc * 4);
     DALLOC(m->diag, (long long)m->nsp * 4 * nc * 4);
     /* sort / scan scratch */
@@ -526,29 +553,29 @@ static pi_status enqueue_step(pi_model* m, float dt)
     SweepArgs sa{};
     sa.g = g; sa.st = st; sa.dt = dt;
     sa.vel = V1.u; sa.dim = 0;
-    for (int q = 0; q < PI_NTRACERS; ++q) { sa.out[q] = m->T1[q]; sa.prev[q] = nullptr; }
+    sa.out = m->T1[0]; sa.prev = nullptr;
     tbegin(m, "sweep_x");
     launch_sweep(s, ib, sa);
     tend(m);
     FillArgs fa{};
     fa.g = g; fa.st = st; fa.nf = PI_NTRACERS; fa.dim = 1; fa.lines_full = 0;
-    for (int q = 0; q < PI_NTRACERS; ++q) fa.f[q] = m->T1[q];
+    fa.f = m->T1[0];
     tbegin(m, "fill");
     launch_fill(s, blocks_for((long long)g.Nx * g.Nz * PI_NTRACERS), fa);
     tend(m);
     sa.vel = V1.v; sa.dim = 1;
-    for (int q = 0; q < PI_NTRACERS; ++q) { sa.out[q] = m->T2[q]; sa.prev[q] = m->T1[q]; }
+    sa.out = m->T2[0]; sa.prev = m->T1[0];
     tbegin(m, "sweep_y");
     launch_sweep(s, ib, sa);
     tend(m);
     fa.dim = 2;
-    for (int q = 0; q < PI_NTRACERS; ++q) fa.f[q] = m->T2[q];
+    fa.f = m->T2[0];
     tbegin(m, "fill");
     launch_fill(s, blocks_for((long long)g.Nx * g.Ny * PI_NTRACERS), fa);
     tend(m);
     FinalArgs fn{};
     fn.g = g; fn.st = st; fn.w = V1.w; fn.dt = dt;
-    for (int q = 0; q < PI_NTRACERS; ++q) { fn.prev[q] = m->T2[q]; fn.plk[q] = m->plk[q]; }
+    fn.prev = m->T2[0]; fn.plk = m->plk_slab;
     fn.kh = m->bgc[PI_B_kappa_h]; fn.kv = m->bgc[PI_B_kappa_v];
     fn.wsink_inorg = m->bgc[PI_B_w_sink_inorg]; fn.wsink_org = m->bgc[PI_B_w_sink_org];
     fn.kDOC = m->bgc[PI_B_kDOC]; fn.kPOC = m->bgc[PI_B_kPOC]; fn.kDON = m->bgc[PI_B_kDON];
@@ -561,7 +588,7 @@ static pi_status enqueue_step(pi_model* m, float dt)
     tend(m);
     /* fill_halo_tracer! of the new tracers: x, then y, then z over whole planes */
     tbegin(m, "fill");
-    for (int q = 0; q < PI_NTRACERS; ++q) fa.f[q] = nullptr;
+    fa.f = nullptr;
     fa.lines_full = 1;
     fa.dim = 0; launch_fill(s, blocks_for((long long)g.Py * g.Pz * PI_NTRACERS), fa);
     fa.dim = 1; launch_fill(s, blocks_for((long long)g.Px * g.Pz * PI_NTRACERS), fa);
