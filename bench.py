@@ -101,6 +101,9 @@ def main():
     ap.add_argument("--individuals", type=int, default=4194304, help="per species at start")
     ap.add_argument("--species", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="for rocprofv3 --pmc runs: a plain device copy of 256 MiB (k_probe_copy, known traffic, to "
+                         "scale the counters), then --warmup + --steps steps without timing; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -123,6 +126,14 @@ def main():
     nat.set_temperature(temp)
     log(f"[bench] model resident on the GPU after {time.perf_counter() - t0:.1f} s")
 
+    if args.pmc_probe:
+        lib = pi._native.load_library()
+        pi._native.check(lib, "pi_", lib.pi_probe_copy(1 << 28))      # k_probe_copy: 256 MiB read + 256 MiB written
+        log("[bench] copy probe: 268435456 bytes read, 268435456 bytes written")
+        nat.run(dt, args.warmup + args.steps)
+        nat.sync()
+        log(f"[bench] {args.warmup + args.steps} steps done")
+        return
     nat.run(dt, args.warmup)
     nat.sync()
     live0 = sum(nat.count_active(s) for s in range(args.species))

@@ -186,6 +186,13 @@ pi_status pi_count_active(pi_model* m, int32_t sp, int64_t* n);   /* dot(ac, ac)
 pi_status pi_enable_timing(pi_model* m, int32_t on);
 pi_status pi_kernel_times(pi_model* m, int32_t* n, const char** names, float* ms, int32_t* counts, int32_t cap);
 
+/* Measurement aid, no counterpart in the reference: copies `bytes` (a multiple
+ * of 16) between two fresh device buffers with the kernel k_probe_copy on the
+ * current device and waits for it.  Hardware counters of that dispatch scale
+ * the HBM traffic counters of the other kernels (bytes read = bytes written =
+ * `bytes`). */
+pi_status pi_probe_copy(int64_t bytes);
+
 #ifdef __cplusplus
 }
 #endif

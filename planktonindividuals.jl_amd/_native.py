@@ -54,6 +54,8 @@ def load_library():
     lib.pi_kernel_times.restype = C.c_int
     lib.pi_kernel_times.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_char_p),
                                     C.POINTER(C.c_float), C.POINTER(C.c_int32), C.c_int32]
+    lib.pi_probe_copy.restype = C.c_int
+    lib.pi_probe_copy.argtypes = [C.c_int64]
     _lib = lib
     return lib
 

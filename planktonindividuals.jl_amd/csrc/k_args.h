@@ -11,7 +11,10 @@ struct AdvArgs {
     Vel v0, vh, v1;
     float dt, sx, sy, sz;
     uint64_t seed;
-    uint32_t* cnt;
+    uint32_t* newc;           /* per store entry: its cell in this step */
+    uint32_t* stayf;          /* per sorted entry: 1 = keeps its cell */
+    uint32_t* small;          /* queue of the other live entries */
+    uint32_t* cnts;           /* per cell: queued arrivals */
     int32_t* dstamp;          /* per slot: iteration in which its occupant dies */
     uint32_t* dcell;          /* per slot: that occupant's cell in this step */
     long long cap;
@@ -32,6 +35,8 @@ struct PhysArgs {
     const uint32_t* bmD;
     const uint32_t* G;        /* exclusive scan over [D words | free words] of all species */
     int nw;
+    const uint32_t* newc;     /* per store entry: its cell in this step */
+    uint8_t* sflag;           /* per slot: flags for the next step's division marks */
     Contrib con;
     float* diag_num;          /* diags: species stride 4*ncell, field num */
     float dt;
@@ -88,7 +93,8 @@ struct FinalArgs {
 
 struct ScanIn {
     const uint32_t* a;
-    uint32_t n;
+    uint32_t n;               /* length (upper bound when dn is set) */
+    const uint32_t* dn;       /* optional device-side length <= n; out[*dn] = total */
     int bits;
     const uint32_t* D;
     const uint32_t* A;
@@ -98,12 +104,20 @@ struct ScanIn {
 
 /* k_particles.hip */
 void launch_adv(hipStream_t st, int blocks, const AdvArgs& a);
-void launch_marks(hipStream_t s, int blocks, DevState* st, uint32_t* bmA, uint32_t* bmD, int nw);
-void launch_place(hipStream_t s, int blocks, DevState* st, const uint32_t* seg, uint32_t* cnt, uint32_t* pkey, uint32_t* psrc);
-void launch_cells(hipStream_t s, int blocks, DevState* st, long long ncell, const uint32_t* seg, uint32_t* pkey,
-                  uint32_t* psrc, float* Chl, float* pop, uint32_t* big);
+void launch_marks(hipStream_t s, const DevState* st, const uint8_t* sflag, uint32_t* bmA, uint32_t* bmD,
+                  int nsp, int nw, long long cap);
+void launch_cellcount(hipStream_t s, const DevState* st, long long ncell, const uint32_t* segp, const uint32_t* P,
+                      const uint32_t* cnts, uint32_t* cnt);
+void launch_place_stay(hipStream_t s, int blocks, const DevState* st, const uint32_t* seg, const uint32_t* segp,
+                       const uint32_t* P, const uint32_t* stayf, const uint32_t* newc, uint32_t* pkey, uint32_t* psrc);
+void launch_place_small(hipStream_t s, int blocks, DevState* st, const uint32_t* seg, const uint32_t* small,
+                        const uint32_t* newc, uint32_t* cnts, uint32_t* pkey, uint32_t* psrc, uint32_t* mix);
+void launch_mixsort(hipStream_t s, int blocks, DevState* st, const uint32_t* seg, const uint32_t* mix,
+                    uint32_t* pkey, uint32_t* psrc, uint32_t* big);
 void launch_bigcells(hipStream_t s, int blocks, DevState* st, const uint32_t* big, const uint32_t* seg,
-                     uint32_t* pkey, uint32_t* psrc, uint32_t* skey, uint32_t* ssrc, float* Chl);
+                     uint32_t* pkey, uint32_t* psrc, uint32_t* skey, uint32_t* ssrc);
+void launch_cellsum(hipStream_t s, const DevState* st, long long ncell, const uint32_t* seg, const uint32_t* psrc,
+                    float* Chl, float* pop);
 void launch_par(hipStream_t s, int blocks, DevState* st, const DevGrid& g, const float* Chl, const float* PARF_in,
                 float* PARF_out, float* diag_par, float kc, float kw);
 void launch_phys(hipStream_t st, int blocks, const PhysArgs& a);
@@ -122,7 +136,9 @@ void launch_zero(hipStream_t s, uint32_t* p, long long n, const DevState* st);
 void launch_velhalf(hipStream_t s, const float* a, const float* b, float* h, long long n, const DevState* st);
 void launch_accum(hipStream_t s, float* d, const DevState* st, int q, long long n);
 void launch_store_to_table(hipStream_t s, DevState* st, const SlotTable& tab, long long cap, long long maxn);
-void launch_rebuild(hipStream_t s, DevState* st, const SlotTable& tab, long long cap, int nsp);
+void launch_rebuild(hipStream_t s, DevState* st, const SlotTable& tab, long long cap, int nsp, uint8_t* sflag);
+void launch_copy_u32(hipStream_t s, uint32_t* dst, const uint32_t* src, long long n, const DevState* st);
+void launch_probe_copy(hipStream_t s, void* dst, const void* src, long long n16);
 void launch_table_field(hipStream_t s, const DevState* st, const SlotTable& tab, int sp, long long cap, int field,
                         const float* params, const DevGrid& g, void* out);
 void launch_count(hipStream_t s, const DevState* st, int sp, unsigned long long* out, long long maxn);

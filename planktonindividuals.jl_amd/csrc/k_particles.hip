@@ -1,19 +1,24 @@
 /*
  * k_particles.hip - particle kernels of the hot path on gfx950.
  *
- *   k_adv     RK4 advection + random-walk diffusion + cell index, one lane per
- *             live individual, store order (cell-sorted from the previous step).
- *             Also counts individuals per cell and marks survivors/dividers in
- *             per-species slot bitmaps.
- *   k_place   counting-sort scatter of (key, store index) into cell segments.
- *   k_cells   per cell: order its segment by (species, slot), pop = count,
- *             Chl = ordered sum (the reference's atomic order under julia -t 1).
- *   k_bigcells  same for segments longer than SMALL_SEG (block bitonic sort).
- *   k_par     Beer-Lambert column scan (PAR), one lane per column.
- *   k_phys    per individual in cell order: losses, division (slot assignment
- *             from the bitmaps), QuotaMode growth, decisions, consumption.
- *   k_plk     per cell: ordered sums of losses/consumption into plk + counts
- *             for the diagnostics.
+ *   k_adv       RK4 advection + random-walk diffusion + cell index, one lane per
+ *               live individual, store order (cell-sorted from the previous
+ *               step).  An individual that keeps its cell is a STAYER; the
+ *               others (movers, and daughters appended last step) are queued.
+ *   k_cellcount per cell: stayers (from the scan of stayer flags over the
+ *               previous segment) + queued arrivals.
+ *   k_place_stay  stayers keep their relative order: scattered without atomics.
+ *   k_place_small queued individuals go after the stayers of their cell; such
+ *               cells are listed and re-sorted (k_mixsort, k_bigcells).
+ *   k_cellsum   per cell: pop = count, Chl = ordered sum (the reference's
+ *               atomic order under julia -t 1).
+ *   k_marks     survivors/dividers as per-species slot bitmaps, from per-slot
+ *               flag bytes written by k_phys (no atomics).
+ *   k_par       Beer-Lambert column scan (PAR), one lane per column.
+ *   k_phys      per individual in cell order: losses, division (slot assignment
+ *               from the bitmaps), QuotaMode growth, decisions, consumption.
+ *   k_plk       per cell: ordered sums of losses/consumption into plk + counts
+ *               for the diagnostics.
  *
  * Every floating-point expression keeps the reference's operation order
  * (cited per function); the build uses -ffp-contract=off and no fast-math.
@@ -72,129 +77,199 @@ __device__ __forceinline__ float pbc(float x, int N, int topo)
 }
 
 
+/* one slot of a device queue per lane with pred, one atomic per wave */
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
+{
+    const unsigned long long m = __ballot(pred);
+    const int lane = (int)(threadIdx.x & 63);
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader < 0 ? 0 : leader, 64);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
 __global__ void __launch_bounds__(256) k_adv(AdvArgs a)
 {
-    const DevState* st = a.st;
+    DevState* st = a.st;
     if (st->err) return;
     const DevGrid& g = a.g;
     const Store S = st->store[st->scur];
     const uint32_t n = st->nstore;
+    const uint32_t nA = st->nA;
     const long long iteration = st->iteration + 1;
     const float dt = a.dt;
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        uint8_t fl = S.flags[e];
-        if (!(fl & F_AC)) continue;
-        const uint32_t key = S.key[e];
-        const int sp = (int)(key >> PI_SLOT_BITS);
-        const uint32_t slot = key & PI_SLOT_MASK;
-        const float x0 = S.x[e], y0 = S.y[e], z0 = S.z[e];
-        /* ---- RK4, particle_advection.jl:2-56 */
-        float u1 = itpl(g, a.v0.u, x0, y0, z0, 0);
-        float v1 = itpl(g, a.v0.v, x0, y0, z0, 1);
-        float w1 = itpl(g, a.v0.w, x0, y0, z0, 2);
-        float u2 = u1, v2 = v1, w2 = w1;
-        float px = pbc(x0 + 0.5f * u1 * dt, g.Nx, g.tx);
-        float py = pbc(y0 + 0.5f * v1 * dt, g.Ny, g.ty);
-        float pz = pbc(z0 - 0.5f * w1 * dt, g.Nz, g.tz);
-        u1 = itpl(g, a.vh.u, px, py, pz, 0);
-        v1 = itpl(g, a.vh.v, px, py, pz, 1);
-        w1 = itpl(g, a.vh.w, px, py, pz, 2);
-        u2 = u2 + u1 * 2.0f; v2 = v2 + v1 * 2.0f; w2 = w2 + w1 * 2.0f;
-        px = pbc(x0 + 0.5f * u1 * dt, g.Nx, g.tx);
-        py = pbc(y0 + 0.5f * v1 * dt, g.Ny, g.ty);
-        pz = pbc(z0 - 0.5f * w1 * dt, g.Nz, g.tz);
-        u1 = itpl(g, a.vh.u, px, py, pz, 0);
-        v1 = itpl(g, a.vh.v, px, py, pz, 1);
-        w1 = itpl(g, a.vh.w, px, py, pz, 2);
-        u2 = u2 + u1 * 2.0f; v2 = v2 + v1 * 2.0f; w2 = w2 + w1 * 2.0f;
-        px = pbc(x0 + 1.0f * u1 * dt, g.Nx, g.tx);
-        py = pbc(y0 + 1.0f * v1 * dt, g.Ny, g.ty);
-        pz = pbc(z0 - 1.0f * w1 * dt, g.Nz, g.tz);
-        u1 = itpl(g, a.v1.u, px, py, pz, 0);
-        v1 = itpl(g, a.v1.v, px, py, pz, 1);
-        w1 = itpl(g, a.v1.w, px, py, pz, 2);
-        u2 = u2 + u1; v2 = v2 + v1; w2 = w2 + w1;
-        u2 = divc(u2, 6.0f, g.y6); v2 = divc(v2, 6.0f, g.y6); w2 = divc(w2, 6.0f, g.y6);
-        float x = pbc(x0 + 1.0f * u2 * dt, g.Nx, g.tx);
-        float y = pbc(y0 + 1.0f * v2 * dt, g.Ny, g.ty);
-        float z = pbc(z0 - 1.0f * w2 * dt, g.Nz, g.tz);
-        /* ---- random walk, particle_diffusion.jl:2-33 (stale zi from the previous find_inds) */
-        float r0, r1, r2;
-        pi_uniform3(a.seed, iteration, sp, PI_KIND_DIFFUSION, slot, &r0, &r1, &r2);
-        r0 = (r0 - 0.5f) * 2.0f; r1 = (r1 - 0.5f) * 2.0f; r2 = (r2 - 0.5f) * 2.0f;
-        const uint32_t cprev = S.cell[e];
-        const long long zprev = (fl & F_ZERO) ? 0 : (long long)(cprev / (uint32_t)(g.Px * g.Py)) + 1;
-        r0 = divc(r0 * a.sx, g.dx, g.ydx);
-        r1 = divc(r1 * a.sy, g.dy, g.ydy);
-        r2 = div_dzC(g, r2 * a.sz, zprev + g.Hz);
-        x = pbc(x + r0, g.Nx, g.tx);
-        y = pbc(y + r1, g.Ny, g.ty);
-        z = pbc(z + r2, g.Nz, g.tz);
-        /* ---- find_inds (Individuals/utils.jl:2-12) */
-        int xi = (int)(x + 1.0f) + g.Hx, yi = (int)(y + 1.0f) + g.Hy, zi = (int)(z + 1.0f) + g.Hz;
-        xi = min(max(xi, 1), g.Px); yi = min(max(yi, 1), g.Py); zi = min(max(zi, 1), g.Pz);
-        const uint32_t c = (uint32_t)gidx(g, xi, yi, zi);
-        S.x[e] = x; S.y[e] = y; S.z[e] = z;
-        S.cell[e] = c;
-        S.flags[e] = fl & (uint8_t)~F_ZERO;
-        atomicAdd(&a.cnt[c], 1u);
-        if (fl & (F_GRAZ | F_MORT)) {       /* dies this step: remember where, by slot */
-            const long long r = (long long)sp * a.cap + slot;
-            a.dstamp[r] = (int32_t)iteration;
-            a.dcell[r] = c;
+    const uint32_t nloop = (n + 63u) & ~63u;          /* whole waves stay in the loop (wave_append) */
+    for (uint32_t e = xcd_block() * blockDim.x + threadIdx.x; e < nloop; e += gridDim.x * blockDim.x) {
+        const uint8_t fl = e < n ? S.flags[e] : (uint8_t)0;
+        bool queue = false;
+        uint32_t c = 0;
+        if (fl & F_AC) {
+            const uint32_t key = S.key[e];
+            const int sp = (int)(key >> PI_SLOT_BITS);
+            const uint32_t slot = key & PI_SLOT_MASK;
+            const float x0 = S.x[e], y0 = S.y[e], z0 = S.z[e];
+            /* ---- RK4, particle_advection.jl:2-56 */
+            float u1 = itpl(g, a.v0.u, x0, y0, z0, 0);
+            float v1 = itpl(g, a.v0.v, x0, y0, z0, 1);
+            float w1 = itpl(g, a.v0.w, x0, y0, z0, 2);
+            float u2 = u1, v2 = v1, w2 = w1;
+            float px = pbc(x0 + 0.5f * u1 * dt, g.Nx, g.tx);
+            float py = pbc(y0 + 0.5f * v1 * dt, g.Ny, g.ty);
+            float pz = pbc(z0 - 0.5f * w1 * dt, g.Nz, g.tz);
+            u1 = itpl(g, a.vh.u, px, py, pz, 0);
+            v1 = itpl(g, a.vh.v, px, py, pz, 1);
+            w1 = itpl(g, a.vh.w, px, py, pz, 2);
+            u2 = u2 + u1 * 2.0f; v2 = v2 + v1 * 2.0f; w2 = w2 + w1 * 2.0f;
+            px = pbc(x0 + 0.5f * u1 * dt, g.Nx, g.tx);
+            py = pbc(y0 + 0.5f * v1 * dt, g.Ny, g.ty);
+            pz = pbc(z0 - 0.5f * w1 * dt, g.Nz, g.tz);
+            u1 = itpl(g, a.vh.u, px, py, pz, 0);
+            v1 = itpl(g, a.vh.v, px, py, pz, 1);
+            w1 = itpl(g, a.vh.w, px, py, pz, 2);
+            u2 = u2 + u1 * 2.0f; v2 = v2 + v1 * 2.0f; w2 = w2 + w1 * 2.0f;
+            px = pbc(x0 + 1.0f * u1 * dt, g.Nx, g.tx);
+            py = pbc(y0 + 1.0f * v1 * dt, g.Ny, g.ty);
+            pz = pbc(z0 - 1.0f * w1 * dt, g.Nz, g.tz);
+            u1 = itpl(g, a.v1.u, px, py, pz, 0);
+            v1 = itpl(g, a.v1.v, px, py, pz, 1);
+            w1 = itpl(g, a.v1.w, px, py, pz, 2);
+            u2 = u2 + u1; v2 = v2 + v1; w2 = w2 + w1;
+            u2 = divc(u2, 6.0f, g.y6); v2 = divc(v2, 6.0f, g.y6); w2 = divc(w2, 6.0f, g.y6);
+            float x = pbc(x0 + 1.0f * u2 * dt, g.Nx, g.tx);
+            float y = pbc(y0 + 1.0f * v2 * dt, g.Ny, g.ty);
+            float z = pbc(z0 - 1.0f * w2 * dt, g.Nz, g.tz);
+            /* ---- random walk, particle_diffusion.jl:2-33 (stale zi from the previous find_inds) */
+            float r0, r1, r2;
+            pi_uniform3(a.seed, iteration, sp, PI_KIND_DIFFUSION, slot, &r0, &r1, &r2);
+            r0 = (r0 - 0.5f) * 2.0f; r1 = (r1 - 0.5f) * 2.0f; r2 = (r2 - 0.5f) * 2.0f;
+            const uint32_t cprev = S.cell[e];
+            const long long zprev = (fl & F_ZERO) ? 0 : (long long)(cprev / (uint32_t)(g.Px * g.Py)) + 1;
+            r0 = divc(r0 * a.sx, g.dx, g.ydx);
+            r1 = divc(r1 * a.sy, g.dy, g.ydy);
+            r2 = div_dzC(g, r2 * a.sz, zprev + g.Hz);
+            x = pbc(x + r0, g.Nx, g.tx);
+            y = pbc(y + r1, g.Ny, g.ty);
+            z = pbc(z + r2, g.Nz, g.tz);
+            /* ---- find_inds (Individuals/utils.jl:2-12) */
+            int xi = (int)(x + 1.0f) + g.Hx, yi = (int)(y + 1.0f) + g.Hy, zi = (int)(z + 1.0f) + g.Hz;
+            xi = min(max(xi, 1), g.Px); yi = min(max(yi, 1), g.Py); zi = min(max(zi, 1), g.Pz);
+            c = (uint32_t)gidx(g, xi, yi, zi);
+            S.x[e] = x; S.y[e] = y; S.z[e] = z;
+            S.flags[e] = fl & (uint8_t)~F_ZERO;
+            a.newc[e] = c;                      /* S.cell keeps the cell of the previous step */
+            const bool stay = e < nA && c == cprev;
+            if (e < nA) a.stayf[e] = stay ? 1u : 0u;
+            queue = !stay;
+            if (fl & (F_GRAZ | F_MORT)) {       /* dies this step: remember where, by slot */
+                const long long r = (long long)sp * a.cap + slot;
+                a.dstamp[r] = (int32_t)iteration;
+                a.dcell[r] = c;
+            }
+        } else if (e < nA) {
+            a.stayf[e] = 0u;
+        }
+        const uint32_t qi = wave_append(&st->nsmall, queue);
+        if (queue) {
+            a.small[qi] = e;
+            atomicAdd(&a.cnts[c], 1u);
         }
     }
 }
 
 /* Survivors of this step's losses and the dividers, as per-species slot bitmaps
  * (plankton_update.jl:5-24: ac after grazing!/mortality!, dvid .*= ac).  They
- * depend only on the decisions taken at the end of the previous step, so the
- * capacity check runs before anything is modified. */
-__global__ void __launch_bounds__(256) k_marks(DevState* st, uint32_t* bmA, uint32_t* bmD, int nw)
+ * depend only on the decisions taken at the end of the previous step, kept as
+ * one flag byte per slot, so the capacity check runs before anything is
+ * modified.  One lane per 32-slot word. */
+__global__ void __launch_bounds__(256) k_marks(const DevState* st, const uint8_t* sflag, uint32_t* bmA, uint32_t* bmD,
+                                               int nsp, int nw, long long cap)
 {
     if (st->err) return;
-    const Store S = st->store[st->scur];
-    const uint32_t n = st->nstore;
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        const uint8_t fl = S.flags[e];
-        if (!(fl & F_AC) || (fl & (F_GRAZ | F_MORT))) continue;
-        const uint32_t key = S.key[e];
-        const uint32_t sp = key >> PI_SLOT_BITS, slot = key & PI_SLOT_MASK;
-        const uint32_t w = sp * (uint32_t)nw + (slot >> 5), bit = 1u << (slot & 31u);
-        atomicOr(&bmA[w], bit);
-        if (fl & F_DVID) atomicOr(&bmD[w], bit);
+    const long long nwords = (long long)nsp * nw;
+    for (long long w = blockIdx.x * (long long)blockDim.x + threadIdx.x; w < nwords; w += (long long)gridDim.x * blockDim.x) {
+        const long long sp = w / nw, wd = w - sp * nw;
+        const long long s0 = wd * 32, s1 = s0 + 32 < cap ? s0 + 32 : cap;
+        const uint8_t* f = sflag + sp * cap;
+        uint32_t A = 0, D = 0;
+        for (long long sl = s0; sl < s1; ++sl) {
+            const uint8_t b = f[sl];
+            const bool alive = (b & F_AC) && !(b & (F_GRAZ | F_MORT));
+            A |= (uint32_t)alive << (sl - s0);
+            D |= (uint32_t)(alive && (b & F_DVID)) << (sl - s0);
+        }
+        bmA[w] = A;
+        bmD[w] = D;
     }
 }
 
-__global__ void __launch_bounds__(256) k_place(DevState* st, const uint32_t* seg,
-                                               uint32_t* cnt, uint32_t* pkey, uint32_t* psrc)
+/* per cell: stayers (their count from the stayer prefix P over the cell's
+ * previous segment segp) + queued arrivals */
+__global__ void __launch_bounds__(256) k_cellcount(const DevState* st, long long ncell, const uint32_t* segp,
+                                                   const uint32_t* P, const uint32_t* cnts, uint32_t* cnt)
 {
     if (st->err) return;
-    const Store s = st->store[st->scur];
-    const uint32_t n = st->nstore;
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        if (!(s.flags[e] & F_AC)) continue;
-        const uint32_t c = s.cell[e];
-        const uint32_t pos = seg[c] + atomicSub(&cnt[c], 1u) - 1u;
-        pkey[pos] = s.key[e];
+    const bool have = st->nA > 0;
+    for (long long c = xcd_block() * (long long)blockDim.x + threadIdx.x; c < ncell; c += (long long)gridDim.x * blockDim.x)
+        cnt[c] = (have ? P[segp[c + 1]] - P[segp[c]] : 0u) + cnts[c];
+}
+
+/* stayers: the stayers of a cell keep their (species, slot) order; position =
+ * start of the cell + rank among the cell's stayers */
+__global__ void __launch_bounds__(256) k_place_stay(const DevState* st, const uint32_t* seg, const uint32_t* segp,
+                                                    const uint32_t* P, const uint32_t* stayf, const uint32_t* newc,
+                                                    uint32_t* pkey, uint32_t* psrc)
+{
+    if (st->err) return;
+    const Store S = st->store[st->scur];
+    const uint32_t nA = st->nA;
+    for (uint32_t e = xcd_block() * blockDim.x + threadIdx.x; e < nA; e += gridDim.x * blockDim.x) {
+        if (!stayf[e]) continue;
+        const uint32_t c = newc[e];
+        const uint32_t pos = seg[c] + (P[e] - P[segp[c]]);
+        pkey[pos] = S.key[e];
         psrc[pos] = e;
     }
 }
 
-/* acc_chl! / acc_counts! (Individuals/utils.jl:33-52) as an ordered segmented sum. */
-__global__ void __launch_bounds__(256) k_cells(DevState* st, long long ncell, const uint32_t* seg, uint32_t* pkey,
-                                               uint32_t* psrc, float* Chl, float* pop, uint32_t* big)
+/* queued individuals: after the stayers of their cell, in any order; the
+ * cell is listed for re-sorting once (by the last one placed) */
+__global__ void __launch_bounds__(256) k_place_small(DevState* st, const uint32_t* seg, const uint32_t* small,
+                                                     const uint32_t* newc, uint32_t* cnts, uint32_t* pkey, uint32_t* psrc,
+                                                     uint32_t* mix)
 {
     if (st->err) return;
-    const float* __restrict__ chl_store = st->store[st->scur].Chl;
-    uint32_t* nbig = &st->nbig;
-    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < ncell;
-         c += (long long)gridDim.x * blockDim.x) {
+    const Store S = st->store[st->scur];
+    const uint32_t ns = st->nsmall;
+    const uint32_t nloop = (ns + 63u) & ~63u;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nloop; t += gridDim.x * blockDim.x) {
+        bool last = false;
+        uint32_t c = 0;
+        if (t < ns) {
+            const uint32_t e = small[t];
+            c = newc[e];
+            const uint32_t r = atomicSub(&cnts[c], 1u);       /* cnts returns to zero */
+            const uint32_t pos = seg[c + 1] - r;
+            pkey[pos] = S.key[e];
+            psrc[pos] = e;
+            last = r == 1u;
+        }
+        const uint32_t mi = wave_append(&st->nmix, last);
+        if (last) mix[mi] = c;
+    }
+}
+
+/* re-sort the listed cells by key (species-major, then slot): insertion sort
+ * for short segments, the rest go to k_bigcells */
+__global__ void __launch_bounds__(256) k_mixsort(DevState* st, const uint32_t* seg, const uint32_t* mix,
+                                                 uint32_t* pkey, uint32_t* psrc, uint32_t* big)
+{
+    if (st->err) return;
+    const uint32_t nm = st->nmix;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < nm; m += gridDim.x * blockDim.x) {
+        const uint32_t c = mix[m];
         const uint32_t s0 = seg[c], n = seg[c + 1] - s0;
-        pop[c] = (float)n;
-        if (n == 0) { Chl[c] = 0.0f; continue; }
-        if (n > SMALL_SEG) { big[atomicAdd(nbig, 1u)] = (uint32_t)c; continue; }
-        /* insertion sort of the segment by key (species-major, then slot) */
+        if (n > SMALL_SEG) { big[atomicAdd(&st->nbig, 1u)] = c; continue; }
         for (uint32_t a = 1; a < n; ++a) {
             uint32_t k = pkey[s0 + a], v = psrc[s0 + a];
             uint32_t b = a;
@@ -206,8 +281,20 @@ __global__ void __launch_bounds__(256) k_cells(DevState* st, long long ncell, co
             pkey[s0 + b] = k;
             psrc[s0 + b] = v;
         }
+    }
+}
+
+/* acc_chl! / acc_counts! (Individuals/utils.jl:33-52) as an ordered segmented sum */
+__global__ void __launch_bounds__(256) k_cellsum(const DevState* st, long long ncell, const uint32_t* seg,
+                                                 const uint32_t* psrc, float* Chl, float* pop)
+{
+    if (st->err) return;
+    const float* __restrict__ chl_store = st->store[st->scur].Chl;
+    for (long long c = xcd_block() * (long long)blockDim.x + threadIdx.x; c < ncell; c += (long long)gridDim.x * blockDim.x) {
+        const uint32_t s0 = seg[c], n = seg[c + 1] - s0;
         float sum = 0.0f;
         for (uint32_t a = 0; a < n; ++a) sum = sum + chl_store[psrc[s0 + a]];
+        pop[c] = (float)n;
         Chl[c] = sum;
     }
 }
@@ -215,11 +302,9 @@ __global__ void __launch_bounds__(256) k_cells(DevState* st, long long ncell, co
 /* Long segments: one block per cell, bitonic sort in a padded scratch region
  * [2*s0, 2*s0 + pow2(n)) that no other segment overlaps. */
 __global__ void __launch_bounds__(256) k_bigcells(DevState* st, const uint32_t* big, const uint32_t* seg,
-                                                  uint32_t* pkey, uint32_t* psrc, uint32_t* skey, uint32_t* ssrc,
-                                                  float* Chl)
+                                                  uint32_t* pkey, uint32_t* psrc, uint32_t* skey, uint32_t* ssrc)
 {
     if (st->err) return;
-    const float* __restrict__ chl_store = st->store[st->scur].Chl;
     const uint32_t nb = st->nbig;
     for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
         const uint32_t c = big[b];
@@ -252,12 +337,6 @@ __global__ void __launch_bounds__(256) k_bigcells(DevState* st, const uint32_t* 
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
             pkey[s0 + i] = K[i];
             psrc[s0 + i] = S[i];
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            float sum = 0.0f;
-            for (uint32_t a = 0; a < n; ++a) sum = sum + chl_store[S[a]];
-            Chl[c] = sum;
         }
         __syncthreads();
     }
@@ -305,41 +384,44 @@ __device__ __forceinline__ float shape_inc(float x, float xmax, float k, int pw)
 {
     return 1.0f - shape_dec(x, xmax, k, pw);
 }
-/* tempFunc_PS / tempFunc (QuotaMode/growth_kernels.jl:1-17) */
-__device__ __forceinline__ float tempFunc_PS(float T, const float* q)
+/* tempFunc_PS / tempFunc (QuotaMode/growth_kernels.jl:1-17).  Both are pure
+ * in (T, species) and share exp(-Ea / (R (T + 273.15))); OGT depends on the
+ * species parameters only.  They are evaluated once per individual (TempF) and
+ * reused by every call site of plankton_growth!, which is bit-identical. */
+struct TempF { float ps, tf; };
+__device__ __forceinline__ float ogt_of(const float* q)
 {
-    float x = T - q[PI_P_Topt], xmax = q[PI_P_Tmax] - q[PI_P_Topt];
-    float regT = shape_dec(x, xmax, 4.0e-2f, 4);
-    float k = pi_expf(-q[PI_P_Ea] / (8.3145f * (T + 273.15f))) * regT;
-    k = dmaxf(0.0f, k);
-    float OGT = pi_expf(-q[PI_P_Ea] / (8.3145f * (q[PI_P_Topt] + 273.15f)));
-    return dminf(1.0f, k / OGT);
+    return pi_expf(-q[PI_P_Ea] / (8.3145f * (q[PI_P_Topt] + 273.15f)));
 }
-__device__ __forceinline__ float tempFunc(float T, const float* q)
+__device__ __forceinline__ TempF temp_funcs(float T, const float* q, float OGT)
 {
-    float k = pi_expf(-q[PI_P_Ea] / (8.3145f * (T + 273.15f)));
-    k = dmaxf(0.0f, k);
-    float OGT = pi_expf(-q[PI_P_Ea] / (8.3145f * (q[PI_P_Topt] + 273.15f)));
-    return dminf(1.0f, k / OGT);
+    const float kx = pi_expf(-q[PI_P_Ea] / (8.3145f * (T + 273.15f)));
+    TempF r;
+    const float x = T - q[PI_P_Topt], xmax = q[PI_P_Tmax] - q[PI_P_Topt];
+    const float regT = shape_dec(x, xmax, 4.0e-2f, 4);
+    r.ps = dminf(1.0f, dmaxf(0.0f, kx * regT) / OGT);
+    r.tf = dminf(1.0f, dmaxf(0.0f, kx) / OGT);
+    return r;
 }
 
 /* plankton_growth! for one individual (QuotaMode/plankton_growth.jl:2-24,
  * growth_kernels.jl:48-171, division_death_probability.jl:176-278). */
-__device__ void growth(PState& s, const Trs& t, const float* q, bool ac, float dt, float tmodel,
+__device__ void growth(PState& s, const Trs& t, const float* q, float OGT, bool ac, float dt, float tmodel,
                        int dvid_type, float U0, float U1, float U2, Rates& r,
                        bool& dgraz, bool& dmort, bool& ddvid)
 {
     const float e30 = 1.0e-30f;
+    const TempF T = temp_funcs(t.T, q, OGT);
     {   /* calc_inorganic_uptake_kernel! */
         float aI = t.par * q[PI_P_alpha] * q[PI_P_Phi];
-        float PCm = q[PI_P_PCmax] * tempFunc_PS(t.T, q);
+        float PCm = q[PI_P_PCmax] * T.ps;
         float PS = PCm * (1.0f - pi_expf(-aI / dmaxf(e30, PCm) * s.Chl / dmaxf(e30, s.Bm))) * s.Bm;
         r.PS = jl_maskf(ac, PS);
         float Qn = (s.Nq + s.Bm * q[PI_P_R_NC]) / dmaxf(e30, s.Bm + s.Cq);
         float Qp = (s.Pq + s.Bm * q[PI_P_R_PC]) / dmaxf(e30, s.Bm + s.Cq);
         float regQN = shape_dec(Qn, q[PI_P_Nqmax], 1.0e-4f, 4);
         float regQP = shape_dec(Qp, q[PI_P_Pqmax], 1.0e-4f, 4);
-        float tf = tempFunc(t.T, q);
+        float tf = T.tf;
         float VNH4 = jl_maskf(ac, q[PI_P_VNH4max] * regQN * t.NH4 / dmaxf(e30, t.NH4 + q[PI_P_KsatNH4]) * tf * s.Bm);
         float VNO3 = jl_maskf(ac, q[PI_P_VNO3max] * regQN * t.NO3 / dmaxf(e30, t.NO3 + q[PI_P_KsatNO3]) * tf * s.Bm);
         float VPO4 = jl_maskf(ac, q[PI_P_VPO4max] * regQP * t.PO4 / dmaxf(e30, t.PO4 + q[PI_P_KsatPO4]) * tf * s.Bm);
@@ -355,7 +437,7 @@ __device__ void growth(PState& s, const Trs& t, const float* q, bool ac, float d
     {   /* calc_organic_uptake_kernel! */
         float Qc = s.Cq / dmaxf(e30, s.Bm + s.Cq);
         float regQ = shape_dec(Qc, q[PI_P_Cqmax], 1.0e-4f, 4);
-        float VN = q[PI_P_VDOCmax] * regQ * t.DOC / dmaxf(e30, t.DOC + q[PI_P_KsatDOC]) * tempFunc(t.T, q) * s.Bm;
+        float VN = q[PI_P_VDOCmax] * regQ * t.DOC / dmaxf(e30, t.DOC + q[PI_P_KsatDOC]) * T.tf * s.Bm;
         float VD = dminf(VN, t.DOC / dt / dmaxf(1.0f, t.pop));
         r.VDOC = jl_maskf(ac, VD);
         r.VDOC = jl_maskf(jl_islessf(1.0e-2f, r.PS / (r.VDOC + r.PS)), r.VDOC);
@@ -367,7 +449,7 @@ __device__ void growth(PState& s, const Trs& t, const float* q, bool ac, float d
         r.rho = jl_maskf(ac, rr);
     }
     /* calc_respir_kernel! */
-    r.resp = jl_maskf(ac, q[PI_P_respir] * s.Bm * tempFunc(t.T, q));
+    r.resp = jl_maskf(ac, q[PI_P_respir] * s.Bm * T.tf);
     /* update_quotas_2_kernel! */
     s.Cq = s.Cq + (r.VDOC - r.resp) * dt;
     s.Bm = s.Bm - dmaxf(0.0f, (0.0f - s.Cq));
@@ -441,14 +523,15 @@ __global__ void __launch_bounds__(256) k_phys(PhysArgs a)
     const float* PAR = st->par[1 - st->pcur];
     const float* PAR0 = st->par[st->pcur];
     const uint32_t hhh = (uint32_t)gidx(g, g.Hx, g.Hy, g.Hz);
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    for (uint32_t e = xcd_block() * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
         const uint32_t key = a.pkey[e], src = a.psrc[e];
         const int sp = (int)(key >> PI_SLOT_BITS);
         const uint32_t slot = key & PI_SLOT_MASK;
         const float* q = a.params + sp * PI_NPARAMS;
         const int dvid_type = (int)q[PI_P_dvid_type];
+        const float OGT = ogt_of(q);
         const uint8_t fl = IN.flags[src];
-        const uint32_t c = IN.cell[src];
+        const uint32_t c = a.newc[src];
         PState s;
         s.x = IN.x[src]; s.y = IN.y[src]; s.z = IN.z[src];
         s.Sz = IN.Sz[src]; s.Bm = IN.Bm[src]; s.Cq = IN.Cq[src]; s.Nq = IN.Nq[src];
@@ -534,7 +617,7 @@ __global__ void __launch_bounds__(256) k_phys(PhysArgs a)
             pi_uniform3(a.seed, iteration, sp, PI_KIND_DECISION, dslot, &U0, &U1, &U2);
             Rates rd;
             bool gz, mt, dv;
-            growth(d, tz, q, true, dt, tmodel, dvid_type, U0, U1, U2, rd, gz, mt, dv);
+            growth(d, tz, q, OGT, true, dt, tmodel, dvid_type, U0, U1, U2, rd, gz, mt, dv);
             if (reborn) {   /* its calc_consume! lands in that cell, at the slot's place */
                 a.rcons[dr * 5 + 0] = (rd.resp - rd.PS) * dt;
                 a.rcons[dr * 5 + 1] = (rd.exu - rd.VDOC) * dt;
@@ -548,14 +631,16 @@ __global__ void __launch_bounds__(256) k_phys(PhysArgs a)
             OUT.x[o] = d.x; OUT.y[o] = d.y; OUT.z[o] = d.z;
             OUT.Sz[o] = d.Sz; OUT.Bm[o] = d.Bm; OUT.Cq[o] = d.Cq; OUT.Nq[o] = d.Nq;
             OUT.Pq[o] = d.Pq; OUT.Chl[o] = d.Chl; OUT.gen[o] = d.gen; OUT.age[o] = d.age;
-            OUT.flags[o] = (uint8_t)(F_AC | (gz ? F_GRAZ : 0) | (mt ? F_MORT : 0) | (dv ? F_DVID : 0));
+            const uint8_t dfl = (uint8_t)(F_AC | (gz ? F_GRAZ : 0) | (mt ? F_MORT : 0) | (dv ? F_DVID : 0));
+            OUT.flags[o] = dfl;
             OUT.cell[o] = dc;
+            a.sflag[dr] = dfl;
             if (!reborn) atomicAdd(&a.diag_num[(long long)sp * 4 * g.ncell + hhh], 1.0f);
         }
         pi_uniform3(a.seed, iteration, sp, PI_KIND_DECISION, slot, &U0, &U1, &U2);
         Rates r;
         bool gz, mt, dv;
-        growth(s, t, q, ac, dt, tmodel, dvid_type, U0, U1, U2, r, gz, mt, dv);
+        growth(s, t, q, OGT, ac, dt, tmodel, dvid_type, U0, U1, U2, r, gz, mt, dv);
         if (ac) {
             tag = T_ALIVE | (divider ? T_DIV : 0);
             L0 = (r.resp - r.PS) * dt;
@@ -578,8 +663,12 @@ __global__ void __launch_bounds__(256) k_phys(PhysArgs a)
         OUT.x[e] = s.x; OUT.y[e] = s.y; OUT.z[e] = s.z;
         OUT.Sz[e] = s.Sz; OUT.Bm[e] = s.Bm; OUT.Cq[e] = s.Cq; OUT.Nq[e] = s.Nq;
         OUT.Pq[e] = s.Pq; OUT.Chl[e] = s.Chl; OUT.gen[e] = s.gen; OUT.age[e] = s.age;
-        OUT.flags[e] = ac ? (uint8_t)(F_AC | (gz ? F_GRAZ : 0) | (mt ? F_MORT : 0) | (dv ? F_DVID : 0)) : (uint8_t)0;
+        const uint8_t ofl = ac ? (uint8_t)(F_AC | (gz ? F_GRAZ : 0) | (mt ? F_MORT : 0) | (dv ? F_DVID : 0)) : (uint8_t)0;
+        OUT.flags[e] = ofl;
         OUT.cell[e] = c;
+        /* a dying individual leaves its byte as it was (GRAZ or MORT set: not
+         * alive for the marks); a daughter may be reborn into that slot */
+        if (ac) a.sflag[(long long)sp * a.cap + slot] = ofl;
         a.con.v[0][e] = L0; a.con.v[1][e] = L1; a.con.v[2][e] = L2;
         a.con.v[3][e] = L3; a.con.v[4][e] = L4; a.con.v[5][e] = L5;
         a.con.tag[e] = tag;
@@ -595,7 +684,7 @@ __global__ void __launch_bounds__(256) k_plk(PlkArgs a)
     if (a.st->err) return;
     const int32_t it = (int32_t)(a.st->iteration + 1);
     const DevGrid& g = a.g;
-    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < g.ncell;
+    for (long long c = xcd_block() * (long long)blockDim.x + threadIdx.x; c < g.ncell;
          c += (long long)gridDim.x * blockDim.x) {
         const uint32_t s0 = a.seg[c], s1 = a.seg[c + 1];
         float DIC = 0.f, NH4 = 0.f, NO3 = 0.f, PO4 = 0.f, DOC = 0.f, DON = 0.f, DOP = 0.f;
@@ -653,23 +742,41 @@ __global__ void __launch_bounds__(256) k_plk(PlkArgs a)
 
 /* ------------------------------------------------------- host-side launchers */
 void launch_adv(hipStream_t st, int blocks, const AdvArgs& a) { hipLaunchKernelGGL(k_adv, dim3(blocks), dim3(256), 0, st, a); }
-void launch_marks(hipStream_t s, int blocks, DevState* st, uint32_t* bmA, uint32_t* bmD, int nw)
+void launch_marks(hipStream_t s, const DevState* st, const uint8_t* sflag, uint32_t* bmA, uint32_t* bmD,
+                  int nsp, int nw, long long cap)
 {
-    hipLaunchKernelGGL(k_marks, dim3(blocks), dim3(256), 0, s, st, bmA, bmD, nw);
+    hipLaunchKernelGGL(k_marks, dim3(grid_for((long long)nsp * nw, 256, 8192)), dim3(256), 0, s, st, sflag, bmA, bmD,
+                       nsp, nw, cap);
 }
-void launch_place(hipStream_t s, int blocks, DevState* st, const uint32_t* seg, uint32_t* cnt, uint32_t* pkey, uint32_t* psrc)
+void launch_cellcount(hipStream_t s, const DevState* st, long long ncell, const uint32_t* segp, const uint32_t* P,
+                      const uint32_t* cnts, uint32_t* cnt)
 {
-    hipLaunchKernelGGL(k_place, dim3(blocks), dim3(256), 0, s, st, seg, cnt, pkey, psrc);
+    hipLaunchKernelGGL(k_cellcount, dim3(grid_for(ncell, 256, 8192)), dim3(256), 0, s, st, ncell, segp, P, cnts, cnt);
 }
-void launch_cells(hipStream_t s, int blocks, DevState* st, long long ncell, const uint32_t* seg, uint32_t* pkey,
-                  uint32_t* psrc, float* Chl, float* pop, uint32_t* big)
+void launch_place_stay(hipStream_t s, int blocks, const DevState* st, const uint32_t* seg, const uint32_t* segp,
+                       const uint32_t* P, const uint32_t* stayf, const uint32_t* newc, uint32_t* pkey, uint32_t* psrc)
 {
-    hipLaunchKernelGGL(k_cells, dim3(blocks), dim3(256), 0, s, st, ncell, seg, pkey, psrc, Chl, pop, big);
+    hipLaunchKernelGGL(k_place_stay, dim3(blocks), dim3(256), 0, s, st, seg, segp, P, stayf, newc, pkey, psrc);
+}
+void launch_place_small(hipStream_t s, int blocks, DevState* st, const uint32_t* seg, const uint32_t* small,
+                        const uint32_t* newc, uint32_t* cnts, uint32_t* pkey, uint32_t* psrc, uint32_t* mix)
+{
+    hipLaunchKernelGGL(k_place_small, dim3(blocks), dim3(256), 0, s, st, seg, small, newc, cnts, pkey, psrc, mix);
+}
+void launch_mixsort(hipStream_t s, int blocks, DevState* st, const uint32_t* seg, const uint32_t* mix,
+                    uint32_t* pkey, uint32_t* psrc, uint32_t* big)
+{
+    hipLaunchKernelGGL(k_mixsort, dim3(blocks), dim3(256), 0, s, st, seg, mix, pkey, psrc, big);
 }
 void launch_bigcells(hipStream_t s, int blocks, DevState* st, const uint32_t* big, const uint32_t* seg,
-                     uint32_t* pkey, uint32_t* psrc, uint32_t* skey, uint32_t* ssrc, float* Chl)
+                     uint32_t* pkey, uint32_t* psrc, uint32_t* skey, uint32_t* ssrc)
 {
-    hipLaunchKernelGGL(k_bigcells, dim3(blocks), dim3(256), 0, s, st, big, seg, pkey, psrc, skey, ssrc, Chl);
+    hipLaunchKernelGGL(k_bigcells, dim3(blocks), dim3(256), 0, s, st, big, seg, pkey, psrc, skey, ssrc);
+}
+void launch_cellsum(hipStream_t s, const DevState* st, long long ncell, const uint32_t* seg, const uint32_t* psrc,
+                    float* Chl, float* pop)
+{
+    hipLaunchKernelGGL(k_cellsum, dim3(grid_for(ncell, 256, 8192)), dim3(256), 0, s, st, ncell, seg, psrc, Chl, pop);
 }
 void launch_par(hipStream_t s, int blocks, DevState* st, const DevGrid& g, const float* Chl, const float* PARF_in,
                 float* PARF_out, float* diag_par, float kc, float kw)

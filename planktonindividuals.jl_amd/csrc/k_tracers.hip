@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(256) k_sweep_x(SweepArgs a)
     const DevGrid& g = a.g;
     const long long nrow = (long long)g.Ny * g.Nz;
     const long long nface = nrow * (g.Nx + 1);
-    const long long gt = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const long long gt = xcd_block() * (long long)blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const long long F = (gt >> 6) * 63 + lane;
     const bool valid = F < nface;
@@ -231,7 +231,7 @@ __global__ void __launch_bounds__(256) k_sweep_y(SweepArgs a)
     const uint32_t sy = (uint32_t)g.Px;
     const float* C0 = S->tr[S->tcur][0];
     const float dt = a.dt;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nthr; t += (long long)gridDim.x * blockDim.x) {
+    for (long long t = xcd_block() * (long long)blockDim.x + threadIdx.x; t < nthr; t += (long long)gridDim.x * blockDim.x) {
         const int i = (int)(t % g.Nx) + 1 + g.Hx;
         const int r = (int)((t / g.Nx) % nrun);
         const int k = (int)(t / ((long long)g.Nx * nrun)) + 1 + g.Hz;
@@ -417,7 +417,7 @@ __global__ void __launch_bounds__(256) k_final(FinalArgs a)
     const float dt = a.dt;
     const float Az = g.dx * g.dy;
     const float kvAz = a.kv * Az;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nthr; t += (long long)gridDim.x * blockDim.x) {
+    for (long long t = xcd_block() * (long long)blockDim.x + threadIdx.x; t < nthr; t += (long long)gridDim.x * blockDim.x) {
         const int i = (int)(t % g.Nx) + 1 + g.Hx;
         const int j = (int)((t / g.Nx) % g.Ny) + 1 + g.Hy;
         const int r = (int)(t / ((long long)g.Nx * g.Ny));

@@ -12,9 +12,11 @@
 /* Input of a scan: either a plain u32 array, or the division bitmaps laid out as
  * [popcount(D words) of all species | popcount(free words) of all species]. */
 
-__device__ __forceinline__ uint32_t scan_load(const ScanIn& s, uint32_t i)
+__device__ __forceinline__ uint32_t scan_len(const ScanIn& s) { return s.dn ? min(*s.dn, s.n) : s.n; }
+
+__device__ __forceinline__ uint32_t scan_load(const ScanIn& s, uint32_t i, uint32_t n)
 {
-    if (i >= s.n) return 0;
+    if (i >= n) return 0;
     if (!s.bits) return s.a[i];
     const uint32_t half = (uint32_t)s.nsp * (uint32_t)s.nw;
     if (i < half) return __popc(s.D[i]);
@@ -48,9 +50,14 @@ __device__ __forceinline__ uint32_t block_exscan(uint32_t v, uint32_t* total)
 
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_partials(ScanIn s, uint32_t* partials)
 {
+    const uint32_t n = scan_len(s);
+    if (blockIdx.x * SCAN_TILE >= n) {               /* beyond a device-side length */
+        if (threadIdx.x == 0) partials[blockIdx.x] = 0;
+        return;
+    }
     const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t sum = 0;
-    for (int k = 0; k < SCAN_ITEMS; ++k) sum += scan_load(s, base + k);
+    for (int k = 0; k < SCAN_ITEMS; ++k) sum += scan_load(s, base + k, n);
     uint32_t tot;
     block_exscan(sum, &tot);
     if (threadIdx.x == 0) partials[blockIdx.x] = tot;
@@ -78,13 +85,15 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_top(uint32_t* partials, u
 
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_final(ScanIn s, const uint32_t* partials, uint32_t* out)
 {
+    const uint32_t n = scan_len(s);
+    if (blockIdx.x * SCAN_TILE > n) return;
     const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS], sum = 0;
-    for (int k = 0; k < SCAN_ITEMS; ++k) { v[k] = scan_load(s, base + k); sum += v[k]; }
+    for (int k = 0; k < SCAN_ITEMS; ++k) { v[k] = scan_load(s, base + k, n); sum += v[k]; }
     uint32_t tot;
     uint32_t pre = block_exscan(sum, &tot) + partials[blockIdx.x];
     for (int k = 0; k < SCAN_ITEMS; ++k) {
-        if (base + k < s.n) out[base + k] = pre;
+        if (base + k <= n && base + k < s.n) out[base + k] = pre;   /* out[n] = total when n < s.n */
         pre += v[k];
     }
 }
@@ -105,6 +114,8 @@ __global__ void k_capcheck(DevState* st, const uint32_t* G, int nsp, int nw)
         if (d > f && !st->err) { st->err = PI_ERR_CAPACITY; st->err_sp = (uint32_t)s; }
     }
     st->nbig = 0;
+    st->nsmall = 0;
+    st->nmix = 0;
 }
 
 /* end of TimeStep!: vel₀ <- vel₁, par₀ <- par, new tracers and store become current */
@@ -113,11 +124,26 @@ __global__ void k_finish(DevState* st, const uint32_t* seg, long long ncell, int
     if (threadIdx.x != 0 || st->err) return;
     uint32_t born = 0;
     for (int s = 0; s < nsp; ++s) born += st->dtot[s];
+    st->nA = seg[ncell];                      /* the sorted part of the next store */
     st->nstore = seg[ncell] + born;
     st->scur ^= 1;
     st->tcur ^= 1;
     st->pcur ^= 1;
     st->iteration += 1;
+}
+
+__global__ void k_copy_u32(uint32_t* dst, const uint32_t* src, long long n, const DevState* st)
+{
+    if (st->err) return;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+/* a plain streaming copy (16 bytes per lane per iteration) */
+__global__ void __launch_bounds__(256) k_probe_copy(uint4* dst, const uint4* src, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 __global__ void k_zero_u32(uint32_t* p, long long n, const DevState* st)
@@ -166,12 +192,13 @@ __global__ void k_store_to_table(DevState* st, SlotTable tab, long long cap)
 
 /* slot-table rows marked active -> a fresh store (order is irrelevant: the
  * next step sorts by cell) */
-__global__ void k_rebuild(DevState* st, SlotTable tab, long long cap, int nsp)
+__global__ void k_rebuild(DevState* st, SlotTable tab, long long cap, int nsp, uint8_t* sflag)
 {
     const Store S = st->store[st->scur];
     const long long total = cap * nsp;
     for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < total; r += (long long)gridDim.x * blockDim.x) {
         const uint8_t fl = tab.flags[r];
+        sflag[r] = (fl & F_AC) ? (uint8_t)(fl & (F_AC | F_GRAZ | F_MORT | F_DVID)) : (uint8_t)0;
         if (!(fl & F_AC)) continue;
         const uint32_t e = atomicAdd(&st->nstore, 1u);
         S.key[e] = ((uint32_t)(r / cap) << PI_SLOT_BITS) | (uint32_t)(r % cap);
@@ -283,9 +310,17 @@ void launch_store_to_table(hipStream_t s, DevState* st, const SlotTable& tab, lo
 {
     hipLaunchKernelGGL(k_store_to_table, dim3(grid_for(maxn, 256, 8192)), dim3(256), 0, s, st, tab, cap);
 }
-void launch_rebuild(hipStream_t s, DevState* st, const SlotTable& tab, long long cap, int nsp)
+void launch_rebuild(hipStream_t s, DevState* st, const SlotTable& tab, long long cap, int nsp, uint8_t* sflag)
 {
-    hipLaunchKernelGGL(k_rebuild, dim3(grid_for(cap * nsp, 256, 8192)), dim3(256), 0, s, st, tab, cap, nsp);
+    hipLaunchKernelGGL(k_rebuild, dim3(grid_for(cap * nsp, 256, 8192)), dim3(256), 0, s, st, tab, cap, nsp, sflag);
+}
+void launch_probe_copy(hipStream_t s, void* dst, const void* src, long long n16)
+{
+    hipLaunchKernelGGL(k_probe_copy, dim3(grid_for(n16, 256, 8192)), dim3(256), 0, s, (uint4*)dst, (const uint4*)src, n16);
+}
+void launch_copy_u32(hipStream_t s, uint32_t* dst, const uint32_t* src, long long n, const DevState* st)
+{
+    hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, dst, src, n, st);
 }
 void launch_table_field(hipStream_t s, const DevState* st, const SlotTable& tab, int sp, long long cap, int field,
                         const float* params, const DevGrid& g, void* out)

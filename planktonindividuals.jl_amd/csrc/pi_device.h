@@ -115,6 +115,18 @@ __device__ __forceinline__ float div_dzC(const DevGrid& g, float a, long long k)
     return divc(a, g.dzC[k - 1], g.ydzC[k - 1]);
 }
 
+/* MI355X dispatches the workgroups of a launch to its 8 XCDs round-robin, and
+ * each XCD has its own L2.  xcd_block() renumbers the workgroups so that each
+ * XCD runs a contiguous range of logical blocks: neighbouring tiles (which
+ * share cache lines: the cells of consecutive individuals, the rows j±1 of a
+ * tracer stencil) then meet in the same L2.  A bijection for any grid size. */
+__device__ __forceinline__ uint32_t xcd_block()
+{
+    const uint32_t b = blockIdx.x, nb = gridDim.x;
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, i = b >> 3;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 /* launch helpers */
 static inline int grid_for(long long n, int block, int maxblocks) {
     long long b = (n + block - 1) / block;
@@ -135,6 +147,8 @@ struct DevState {
     int32_t scur, tcur, pcur, vcur;
     long long iteration;               /* completed steps */
     uint32_t nstore;                   /* entries in store[scur] */
+    uint32_t nA;                       /* of which the first nA are sorted by (cell, key) as laid out by the previous seg */
+    uint32_t nsmall, nmix;             /* queued entries / re-sorted cells of this step */
     uint32_t err, err_sp;
     uint32_t nbig;
     uint32_t dtot[PI_MAXSP], ftot[PI_MAXSP], doff[PI_MAXSP];

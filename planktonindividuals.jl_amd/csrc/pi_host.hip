@@ -65,6 +65,12 @@ struct pi_model {
     int32_t *dstamp = nullptr, *rstamp = nullptr;
     uint32_t* dcell = nullptr;
     float* rcons = nullptr;
+    /* binning: new cell per store entry, stayer flags and their prefix, the
+     * previous step's segments, queued entries, per-cell queued counts,
+     * re-sorted cells; per-slot flags for the division marks */
+    uint32_t *newc = nullptr, *stayf = nullptr, *P = nullptr, *segp = nullptr, *small = nullptr;
+    uint32_t *cnts = nullptr, *mix = nullptr;
+    uint8_t* sflag = nullptr;
     bool dirty = false;
     long long host_iter = 0;
     float last_dt = 0.0f;
@@ -126,10 +132,11 @@ static pi_status sync_state(pi_model* m)
 static pi_status rebuild_if_dirty(pi_model* m)
 {
     if (!m->dirty) return PI_OK;
-    uint32_t zero = 0;
-    HIPCHECK(hipMemcpyAsync((char*)m->d_state + offsetof(DevState, nstore), &zero, sizeof zero,
-                            hipMemcpyHostToDevice, m->stream));
-    launch_rebuild(m->stream, m->d_state, m->tab, m->cap, m->nsp);
+    static const uint32_t zero[2] = {0, 0};
+    static_assert(offsetof(DevState, nA) == offsetof(DevState, nstore) + 4, "nstore, nA adjacent");
+    HIPCHECK(hipMemcpyAsync((char*)m->d_state + offsetof(DevState, nstore), zero, sizeof zero,
+                            hipMemcpyHostToDevice, m->stream));   /* nstore = nA = 0: all queued */
+    launch_rebuild(m->stream, m->d_state, m->tab, m->cap, m->nsp, m->sflag);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(m->stream));
     m->dirty = false;
@@ -270,7 +277,7 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     /* sort / scan scratch */
     DALLOC(m->cnt, nc * 4);
     DALLOC(m->seg, (nc + 1) * 4);
-    const long long scan_n = std::max<long long>(nc, 2ll * m->nsp * m->nw);
+    const long long scan_n = std::max<long long>(std::max<long long>(nc, 2ll * m->nsp * m->nw), 2 * m->ctot);
     DALLOC(m->partials, ((scan_n + 2047) / 2048 + 1) * 4);
     DALLOC(m->pkey, m->ctot * 4); DALLOC(m->psrc, m->ctot * 4);
     DALLOC(m->skey, 2 * m->ctot * 4); DALLOC(m->ssrc, 2 * m->ctot * 4);
@@ -280,6 +287,12 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     for (int q = 0; q < 6; ++q) DALLOC(m->con.v[q], m->ctot * 4);
     DALLOC(m->con.tag, m->ctot);
     DALLOC(m->d_count, 8);
+    DALLOC(m->newc, 2 * m->ctot * 4); DALLOC(m->stayf, 2 * m->ctot * 4); DALLOC(m->P, (2 * m->ctot + 1) * 4);
+    DALLOC(m->segp, (nc + 1) * 4); DALLOC(m->small, 2 * m->ctot * 4);
+    DALLOC(m->cnts, nc * 4); DALLOC(m->mix, nc * 4);
+    DALLOC(m->sflag, m->ctot);
+    HIPCHECK(hipMemset(m->cnts, 0, nc * 4));
+    HIPCHECK(hipMemset(m->sflag, 0, m->ctot));
     DALLOC(m->dstamp, m->ctot * 4); DALLOC(m->rstamp, m->ctot * 4);
     DALLOC(m->dcell, m->ctot * 4); DALLOC(m->rcons, m->ctot * 5 * 4);
     {   /* stamps start below any iteration */
@@ -506,8 +519,7 @@ static pi_status enqueue_step(pi_model* m, float dt)
     const Vel V1{m->vel[m->v1][0], m->vel[m->v1][1], m->vel[m->v1][2]};
     /* division marks + capacity check */
     tbegin(m, "marks+scan");
-    launch_zero(s, m->bm, 2ll * m->nsp * m->nw, st);
-    launch_marks(s, pb, st, m->bm, m->bm + (size_t)m->nsp * m->nw, m->nw);
+    launch_marks(s, st, m->sflag, m->bm, m->bm + (size_t)m->nsp * m->nw, m->nsp, m->nw, m->cap);
     launch_scan(s, scan_bits(m->bm + (size_t)m->nsp * m->nw, m->bm, m->nsp, m->nw, m->cap), m->partials, m->G);
     launch_capcheck(s, st, m->G, m->nsp, m->nw);
     tend(m);
@@ -517,16 +529,25 @@ static pi_status enqueue_step(pi_model* m, float dt)
     aa.sx = std::sqrt(m->bgc[PI_B_kappa_hP] * dt);
     aa.sy = std::sqrt(m->bgc[PI_B_kappa_hP] * dt);
     aa.sz = std::sqrt(m->bgc[PI_B_kappa_vP] * dt);
-    aa.seed = m->seed; aa.cnt = m->cnt;
+    aa.seed = m->seed;
+    aa.newc = m->newc; aa.stayf = m->stayf; aa.small = m->small; aa.cnts = m->cnts;
     aa.dstamp = m->dstamp; aa.dcell = m->dcell; aa.cap = m->cap;
     tbegin(m, "adv");
     launch_adv(s, pb, aa);
     tend(m);
     tbegin(m, "bin");
+    {   /* stayer prefix over the sorted part (device-side length nA) */
+        ScanIn sp = scan_plain(m->stayf, (uint32_t)(2 * m->ctot));
+        sp.dn = (const uint32_t*)((const char*)st + offsetof(DevState, nA));
+        launch_scan(s, sp, m->partials, m->P);
+    }
+    launch_cellcount(s, st, m->ncell, m->segp, m->P, m->cnts, m->cnt);
     launch_scan(s, scan_plain(m->cnt, (uint32_t)m->ncell), m->partials, m->seg);
-    launch_place(s, pb, st, m->seg, m->cnt, m->pkey, m->psrc);
-    launch_cells(s, cb, st, m->ncell, m->seg, m->pkey, m->psrc, m->Chl, m->pop, m->big);
-    launch_bigcells(s, 256, st, m->big, m->seg, m->pkey, m->psrc, m->skey, m->ssrc, m->Chl);
+    launch_place_stay(s, pb, st, m->seg, m->segp, m->P, m->stayf, m->newc, m->pkey, m->psrc);
+    launch_place_small(s, pb, st, m->seg, m->small, m->newc, m->cnts, m->pkey, m->psrc, m->mix);
+    launch_mixsort(s, cb, st, m->seg, m->mix, m->pkey, m->psrc, m->big);
+    launch_bigcells(s, 256, st, m->big, m->seg, m->pkey, m->psrc, m->skey, m->ssrc);
+    launch_cellsum(s, st, m->ncell, m->seg, m->psrc, m->Chl, m->pop);
     tend(m);
     tbegin(m, "par");
     launch_par(s, blocks_for((long long)g.Nx * g.Ny), st, g, m->Chl, m->PARF_in, m->PARF_out, m->diagPAR,
@@ -537,6 +558,7 @@ static pi_status enqueue_step(pi_model* m, float dt)
     pa.g = g; pa.st = st; pa.tab = m->tab; pa.cap = m->cap; pa.nsp = m->nsp; pa.params = m->d_params;
     pa.seg = m->seg; pa.pkey = m->pkey; pa.psrc = m->psrc; pa.temp = m->temp; pa.pop = m->pop;
     pa.bmA = m->bm; pa.bmD = m->bm + (size_t)m->nsp * m->nw; pa.G = m->G; pa.nw = m->nw;
+    pa.newc = m->newc; pa.sflag = m->sflag;
     pa.con = m->con; pa.diag_num = m->diag; pa.dt = dt; pa.seed = m->seed;
     pa.dstamp = m->dstamp; pa.dcell = m->dcell; pa.rstamp = m->rstamp; pa.rcons = m->rcons;
     tbegin(m, "phys");
@@ -595,6 +617,7 @@ static pi_status enqueue_step(pi_model* m, float dt)
     fa.dim = 2; launch_fill(s, blocks_for((long long)g.Px * g.Py * PI_NTRACERS), fa);
     tend(m);
     for (size_t a = 0; a < m->diag_ids.size(); ++a) launch_accum(s, m->diag_tr[a], st, m->diag_ids[a], m->ncell);
+    launch_copy_u32(s, m->segp, m->seg, m->ncell + 1, st);     /* the next step's previous segments */
     launch_finish(s, st, m->seg, m->ncell, m->nsp);
     HIPCHECK(hipGetLastError());
     /* vel₀ <- vel₁ */
@@ -771,6 +794,24 @@ pi_status pi_kernel_times(pi_model* m, int32_t* n, const char** names, float* ms
     }
     *n = k;
     m->tcount = 0;
+    return PI_OK;
+}
+
+pi_status pi_probe_copy(int64_t bytes)
+{
+    if (bytes <= 0 || bytes % 16) return fail(PI_ERR_ARG, "probe size must be a positive multiple of 16");
+    void *a = nullptr, *b = nullptr;
+    HIPCHECK(hipMalloc(&a, bytes));
+    hipError_t e = hipMalloc(&b, bytes);
+    if (e != hipSuccess) { (void)hipFree(a); return fail(PI_ERR_NOMEM, "probe allocation"); }
+    e = hipMemset(a, 0, bytes);
+    if (e == hipSuccess) {
+        launch_probe_copy(nullptr, b, a, bytes / 16);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(a); (void)hipFree(b);
+    if (e != hipSuccess) return fail(PI_ERR_HIP, hipGetErrorString(e));
     return PI_OK;
 }
 
