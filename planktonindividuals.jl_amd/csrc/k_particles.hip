@@ -89,91 +89,126 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
     return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
 }
 
+/* RK4 advection (particle_advection.jl:2-56) and the random walk
+ * (particle_diffusion.jl:2-33) of one individual, up to its new position. */
+__device__ __forceinline__ void move(const AdvArgs& a, const DevGrid& g, float x0, float y0, float z0,
+                                     float r0, float r1, float r2, long long zprev, float& x, float& y, float& z)
+{
+    const float dt = a.dt;
+    /* ---- RK4, particle_advection.jl:2-56 */
+    float u1 = itpl(g, a.v0.u, x0, y0, z0, 0);
+    float v1 = itpl(g, a.v0.v, x0, y0, z0, 1);
+    float w1 = itpl(g, a.v0.w, x0, y0, z0, 2);
+    float u2 = u1, v2 = v1, w2 = w1;
+    float px = pbc(x0 + 0.5f * u1 * dt, g.Nx, g.tx);
+    float py = pbc(y0 + 0.5f * v1 * dt, g.Ny, g.ty);
+    float pz = pbc(z0 - 0.5f * w1 * dt, g.Nz, g.tz);
+    u1 = itpl(g, a.vh.u, px, py, pz, 0);
+    v1 = itpl(g, a.vh.v, px, py, pz, 1);
+    w1 = itpl(g, a.vh.w, px, py, pz, 2);
+    u2 = u2 + u1 * 2.0f; v2 = v2 + v1 * 2.0f; w2 = w2 + w1 * 2.0f;
+    px = pbc(x0 + 0.5f * u1 * dt, g.Nx, g.tx);
+    py = pbc(y0 + 0.5f * v1 * dt, g.Ny, g.ty);
+    pz = pbc(z0 - 0.5f * w1 * dt, g.Nz, g.tz);
+    u1 = itpl(g, a.vh.u, px, py, pz, 0);
+    v1 = itpl(g, a.vh.v, px, py, pz, 1);
+    w1 = itpl(g, a.vh.w, px, py, pz, 2);
+    u2 = u2 + u1 * 2.0f; v2 = v2 + v1 * 2.0f; w2 = w2 + w1 * 2.0f;
+    px = pbc(x0 + 1.0f * u1 * dt, g.Nx, g.tx);
+    py = pbc(y0 + 1.0f * v1 * dt, g.Ny, g.ty);
+    pz = pbc(z0 - 1.0f * w1 * dt, g.Nz, g.tz);
+    u1 = itpl(g, a.v1.u, px, py, pz, 0);
+    v1 = itpl(g, a.v1.v, px, py, pz, 1);
+    w1 = itpl(g, a.v1.w, px, py, pz, 2);
+    u2 = u2 + u1; v2 = v2 + v1; w2 = w2 + w1;
+    u2 = divc(u2, 6.0f, g.y6); v2 = divc(v2, 6.0f, g.y6); w2 = divc(w2, 6.0f, g.y6);
+    x = pbc(x0 + 1.0f * u2 * dt, g.Nx, g.tx);
+    y = pbc(y0 + 1.0f * v2 * dt, g.Ny, g.ty);
+    z = pbc(z0 - 1.0f * w2 * dt, g.Nz, g.tz);
+    /* ---- random walk, particle_diffusion.jl:2-33 (stale zi from the previous find_inds) */
+    r0 = (r0 - 0.5f) * 2.0f; r1 = (r1 - 0.5f) * 2.0f; r2 = (r2 - 0.5f) * 2.0f;
+    r0 = divc(r0 * a.sx, g.dx, g.ydx);
+    r1 = divc(r1 * a.sy, g.dy, g.ydy);
+    {
+        long long kz = zprev + g.Hz;
+        kz = kz < 1 ? 1 : kz > g.Pz ? g.Pz : kz;
+        r2 = divc(r2 * a.sz, g.dzC[kz - 1], g.ydzC[kz - 1]);
+    }
+    x = pbc(x + r0, g.Nx, g.tx);
+    y = pbc(y + r1, g.Ny, g.ty);
+    z = pbc(z + r2, g.Nz, g.tz);
+}
+
+/* One store entry after its move: cell, stayer flag or queue slot, death
+ * bookkeeping.  Every lane of the wave calls it (wave_append); `live` = the
+ * entry is active and was moved. */
+__device__ __forceinline__ void place_entry(const AdvArgs& a, DevState* st, const Store& S, uint32_t e, uint32_t nA,
+                                            int32_t iteration, bool live, uint8_t fl, float x, float y, float z)
+{
+    const DevGrid& g = a.g;
+    bool queue = false;
+    uint32_t c = 0;
+    if (live) {
+        const uint32_t key = S.key[e];
+        const int sp = (int)(key >> PI_SLOT_BITS);
+        const uint32_t slot = key & PI_SLOT_MASK;
+        /* ---- find_inds (Individuals/utils.jl:2-12) */
+        int xi = (int)(x + 1.0f) + g.Hx, yi = (int)(y + 1.0f) + g.Hy, zi = (int)(z + 1.0f) + g.Hz;
+        xi = min(max(xi, 1), g.Px); yi = min(max(yi, 1), g.Py); zi = min(max(zi, 1), g.Pz);
+        c = (uint32_t)gidx(g, xi, yi, zi);
+        const uint32_t cprev = S.cell[e];
+        S.x[e] = x; S.y[e] = y; S.z[e] = z;
+        S.flags[e] = fl & (uint8_t)~F_ZERO;
+        a.newc[e] = c;                      /* S.cell keeps the cell of the previous step */
+        const bool stay = e < nA && c == cprev;
+        if (e < nA) a.stayf[e] = stay ? 1u : 0u;
+        queue = !stay;
+        if (fl & (F_GRAZ | F_MORT)) {       /* dies this step: remember where, by slot */
+            const long long r = (long long)sp * a.cap + slot;
+            a.dstamp[r] = iteration;
+            a.dcell[r] = c;
+        }
+    }
+    const uint32_t qi = wave_append(&st->nsmall, queue);
+    if (queue) {
+        a.small[qi] = e;
+        atomicAdd(&a.cnts[c], 1u);
+    }
+}
+
+/* the move of entry e from its stored state in step `iteration`.  (Values of
+ * the device state are read before the loops that call these: a load that may
+ * alias the loop's stores would be a vector load waiting for every outstanding
+ * store.) */
+__device__ __forceinline__ void move_entry(const AdvArgs& a, const Store& S, uint32_t e, long long iteration, uint8_t fl,
+                                           float& x, float& y, float& z)
+{
+    const DevGrid& g = a.g;
+    const uint32_t key = S.key[e];
+    const int sp = (int)(key >> PI_SLOT_BITS);
+    const uint32_t slot = key & PI_SLOT_MASK;
+    float r0, r1, r2;
+    pi_uniform3(a.seed, iteration, sp, PI_KIND_DIFFUSION, slot, &r0, &r1, &r2);
+    const uint32_t cprev = S.cell[e];
+    const long long zprev = (fl & F_ZERO) ? 0 : (long long)(cprev / (uint32_t)(g.Px * g.Py)) + 1;
+    move(a, g, S.x[e], S.y[e], S.z[e], r0, r1, r2, zprev, x, y, z);
+}
+
 __global__ void __launch_bounds__(256) k_adv(AdvArgs a)
 {
     DevState* st = a.st;
     if (st->err) return;
-    const DevGrid& g = a.g;
     const Store S = st->store[st->scur];
     const uint32_t n = st->nstore;
     const uint32_t nA = st->nA;
     const long long iteration = st->iteration + 1;
-    const float dt = a.dt;
     const uint32_t nloop = (n + 63u) & ~63u;          /* whole waves stay in the loop (wave_append) */
     for (uint32_t e = xcd_block() * blockDim.x + threadIdx.x; e < nloop; e += gridDim.x * blockDim.x) {
         const uint8_t fl = e < n ? S.flags[e] : (uint8_t)0;
-        bool queue = false;
-        uint32_t c = 0;
-        if (fl & F_AC) {
-            const uint32_t key = S.key[e];
-            const int sp = (int)(key >> PI_SLOT_BITS);
-            const uint32_t slot = key & PI_SLOT_MASK;
-            const float x0 = S.x[e], y0 = S.y[e], z0 = S.z[e];
-            /* ---- RK4, particle_advection.jl:2-56 */
-            float u1 = itpl(g, a.v0.u, x0, y0, z0, 0);
-            float v1 = itpl(g, a.v0.v, x0, y0, z0, 1);
-            float w1 = itpl(g, a.v0.w, x0, y0, z0, 2);
-            float u2 = u1, v2 = v1, w2 = w1;
-            float px = pbc(x0 + 0.5f * u1 * dt, g.Nx, g.tx);
-            float py = pbc(y0 + 0.5f * v1 * dt, g.Ny, g.ty);
-            float pz = pbc(z0 - 0.5f * w1 * dt, g.Nz, g.tz);
-            u1 = itpl(g, a.vh.u, px, py, pz, 0);
-            v1 = itpl(g, a.vh.v, px, py, pz, 1);
-            w1 = itpl(g, a.vh.w, px, py, pz, 2);
-            u2 = u2 + u1 * 2.0f; v2 = v2 + v1 * 2.0f; w2 = w2 + w1 * 2.0f;
-            px = pbc(x0 + 0.5f * u1 * dt, g.Nx, g.tx);
-            py = pbc(y0 + 0.5f * v1 * dt, g.Ny, g.ty);
-            pz = pbc(z0 - 0.5f * w1 * dt, g.Nz, g.tz);
-            u1 = itpl(g, a.vh.u, px, py, pz, 0);
-            v1 = itpl(g, a.vh.v, px, py, pz, 1);
-            w1 = itpl(g, a.vh.w, px, py, pz, 2);
-            u2 = u2 + u1 * 2.0f; v2 = v2 + v1 * 2.0f; w2 = w2 + w1 * 2.0f;
-            px = pbc(x0 + 1.0f * u1 * dt, g.Nx, g.tx);
-            py = pbc(y0 + 1.0f * v1 * dt, g.Ny, g.ty);
-            pz = pbc(z0 - 1.0f * w1 * dt, g.Nz, g.tz);
-            u1 = itpl(g, a.v1.u, px, py, pz, 0);
-            v1 = itpl(g, a.v1.v, px, py, pz, 1);
-            w1 = itpl(g, a.v1.w, px, py, pz, 2);
-            u2 = u2 + u1; v2 = v2 + v1; w2 = w2 + w1;
-            u2 = divc(u2, 6.0f, g.y6); v2 = divc(v2, 6.0f, g.y6); w2 = divc(w2, 6.0f, g.y6);
-            float x = pbc(x0 + 1.0f * u2 * dt, g.Nx, g.tx);
-            float y = pbc(y0 + 1.0f * v2 * dt, g.Ny, g.ty);
-            float z = pbc(z0 - 1.0f * w2 * dt, g.Nz, g.tz);
-            /* ---- random walk, particle_diffusion.jl:2-33 (stale zi from the previous find_inds) */
-            float r0, r1, r2;
-            pi_uniform3(a.seed, iteration, sp, PI_KIND_DIFFUSION, slot, &r0, &r1, &r2);
-            r0 = (r0 - 0.5f) * 2.0f; r1 = (r1 - 0.5f) * 2.0f; r2 = (r2 - 0.5f) * 2.0f;
-            const uint32_t cprev = S.cell[e];
-            const long long zprev = (fl & F_ZERO) ? 0 : (long long)(cprev / (uint32_t)(g.Px * g.Py)) + 1;
-            r0 = divc(r0 * a.sx, g.dx, g.ydx);
-            r1 = divc(r1 * a.sy, g.dy, g.ydy);
-            r2 = div_dzC(g, r2 * a.sz, zprev + g.Hz);
-            x = pbc(x + r0, g.Nx, g.tx);
-            y = pbc(y + r1, g.Ny, g.ty);
-            z = pbc(z + r2, g.Nz, g.tz);
-            /* ---- find_inds (Individuals/utils.jl:2-12) */
-            int xi = (int)(x + 1.0f) + g.Hx, yi = (int)(y + 1.0f) + g.Hy, zi = (int)(z + 1.0f) + g.Hz;
-            xi = min(max(xi, 1), g.Px); yi = min(max(yi, 1), g.Py); zi = min(max(zi, 1), g.Pz);
-            c = (uint32_t)gidx(g, xi, yi, zi);
-            S.x[e] = x; S.y[e] = y; S.z[e] = z;
-            S.flags[e] = fl & (uint8_t)~F_ZERO;
-            a.newc[e] = c;                      /* S.cell keeps the cell of the previous step */
-            const bool stay = e < nA && c == cprev;
-            if (e < nA) a.stayf[e] = stay ? 1u : 0u;
-            queue = !stay;
-            if (fl & (F_GRAZ | F_MORT)) {       /* dies this step: remember where, by slot */
-                const long long r = (long long)sp * a.cap + slot;
-                a.dstamp[r] = (int32_t)iteration;
-                a.dcell[r] = c;
-            }
-        } else if (e < nA) {
-            a.stayf[e] = 0u;
-        }
-        const uint32_t qi = wave_append(&st->nsmall, queue);
-        if (queue) {
-            a.small[qi] = e;
-            atomicAdd(&a.cnts[c], 1u);
-        }
+        float x = 0.0f, y = 0.0f, z = 0.0f;
+        if (fl & F_AC) move_entry(a, S, e, iteration, fl, x, y, z);
+        place_entry(a, st, S, e, nA, (int32_t)iteration, (fl & F_AC) != 0, fl, x, y, z);
+        if (!(fl & F_AC) && e < nA) a.stayf[e] = 0u;
     }
 }
 
@@ -519,9 +554,16 @@ __global__ void __launch_bounds__(256) k_phys(PhysArgs a)
     const Store IN = st->store[st->scur], OUT = st->store[1 - st->scur];
     const long long iteration = st->iteration + 1;
     const float tmodel = (float)iteration * dt;
-    const float* const* TR = st->tr[st->tcur];
+    /* device-state values are read before the loop (see move_entry) */
+    const float* TRs = st->tr[st->tcur][0];
+    const float* TR_NH4 = TRs + (long long)PI_NH4 * g.ncell, *TR_NO3 = TRs + (long long)PI_NO3 * g.ncell;
+    const float* TR_PO4 = TRs + (long long)PI_PO4 * g.ncell, *TR_DOC = TRs + (long long)PI_DOC * g.ncell;
+    const float* TR_DFe = TRs + (long long)PI_DFe * g.ncell;
     const float* PAR = st->par[1 - st->pcur];
     const float* PAR0 = st->par[st->pcur];
+    __shared__ uint32_t s_doff[PI_MAXSP];
+    for (int i = threadIdx.x; i < a.nsp; i += blockDim.x) s_doff[i] = st->doff[i];
+    __syncthreads();
     const uint32_t hhh = (uint32_t)gidx(g, g.Hx, g.Hy, g.Hz);
     for (uint32_t e = xcd_block() * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
         const uint32_t key = a.pkey[e], src = a.psrc[e];
@@ -538,11 +580,11 @@ __global__ void __launch_bounds__(256) k_phys(PhysArgs a)
         s.Pq = IN.Pq[src]; s.Chl = IN.Chl[src]; s.gen = IN.gen[src]; s.age = IN.age[src];
         /* find_NPT_kernel! (Individuals/utils.jl:14-30), ac = true at gather time */
         Trs t;
-        t.NH4 = dmaxf(0.0f, TR[PI_NH4][c]);
-        t.NO3 = dmaxf(0.0f, TR[PI_NO3][c]);
-        t.PO4 = dmaxf(0.0f, TR[PI_PO4][c]);
-        t.DOC = dmaxf(0.0f, TR[PI_DOC][c]);
-        t.DFe = dmaxf(0.0f, TR[PI_DFe][c]);
+        t.NH4 = dmaxf(0.0f, TR_NH4[c]);
+        t.NO3 = dmaxf(0.0f, TR_NO3[c]);
+        t.PO4 = dmaxf(0.0f, TR_PO4[c]);
+        t.DOC = dmaxf(0.0f, TR_DOC[c]);
+        t.DFe = dmaxf(0.0f, TR_DFe[c]);
         t.par = PAR[c];
         t.dpar = PAR[c] - PAR0[c];
         t.T = a.temp[c];
@@ -602,11 +644,11 @@ __global__ void __launch_bounds__(256) k_phys(PhysArgs a)
             const uint32_t dc = reborn ? a.dcell[dr] : hhh;
             Trs tz;
             if (reborn) {
-                tz.NH4 = dmaxf(0.0f, TR[PI_NH4][dc]);
-                tz.NO3 = dmaxf(0.0f, TR[PI_NO3][dc]);
-                tz.PO4 = dmaxf(0.0f, TR[PI_PO4][dc]);
-                tz.DOC = dmaxf(0.0f, TR[PI_DOC][dc]);
-                tz.DFe = dmaxf(0.0f, TR[PI_DFe][dc]);
+                tz.NH4 = dmaxf(0.0f, TR_NH4[dc]);
+                tz.NO3 = dmaxf(0.0f, TR_NO3[dc]);
+                tz.PO4 = dmaxf(0.0f, TR_PO4[dc]);
+                tz.DOC = dmaxf(0.0f, TR_DOC[dc]);
+                tz.DFe = dmaxf(0.0f, TR_DFe[dc]);
                 tz.par = PAR[dc];
                 tz.dpar = PAR[dc] - PAR0[dc];
                 tz.T = a.temp[dc];
@@ -626,7 +668,7 @@ __global__ void __launch_bounds__(256) k_phys(PhysArgs a)
                 a.rcons[dr * 5 + 4] = -rd.VPO4 * dt;
                 a.rstamp[dr] = (int32_t)iteration;
             }
-            const uint32_t o = n + st->doff[sp] + r;
+            const uint32_t o = n + s_doff[sp] + r;
             OUT.key[o] = ((uint32_t)sp << PI_SLOT_BITS) | dslot;
             OUT.x[o] = d.x; OUT.y[o] = d.y; OUT.z[o] = d.z;
             OUT.Sz[o] = d.Sz; OUT.Bm[o] = d.Bm; OUT.Cq[o] = d.Cq; OUT.Nq[o] = d.Nq;

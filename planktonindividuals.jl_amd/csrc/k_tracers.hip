@@ -81,9 +81,10 @@ __device__ __forceinline__ float theta(float dn, float dd)
     const float cap = copysignf(tmax, dn * dd);
     return fabsf(dd) * tmax <= fabsf(dn) ? cap : q;
 }
-__device__ __forceinline__ float psi(const Face& F, float th)
+template <bool EXACT>
+__device__ __forceinline__ float psi(const Face& F, float th, bool& sub)
 {
-    return dmaxf(0.0f, dminf(dminf(1.0f, F.d0 + F.d1 * th), divc(th * F.omc, F.Ce, F.yCe)));
+    return dmaxf(0.0f, dminf(dminf(1.0f, F.d0 + F.d1 * th), divq<EXACT>(th * F.omc, F.Ce, F.yCe, sub)));
 }
 
 __device__ __forceinline__ bool small(float x) { return fabsf(x) < 0x1p100f; }   /* false for NaN */
@@ -113,20 +114,23 @@ __device__ __forceinline__ Sten stencil(const DevGrid& g, const float* p, uint32
     r.ok = small(cm2) && small(cm1) && small(c0) && small(cp1);
     return r;
 }
-/* the term of the sign of T only; bad = the full formula is needed */
+/* the term of the sign of T only, with divq; bad = the full formula (with
+ * exact division) is needed */
 __device__ __forceinline__ float flux_fast(const Face& F, const Sten& s, bool& bad)
 {
     const bool pos = F.T > 0.0f;
-    const float p = psi(F, theta(pos ? s.dnp : s.dnm, s.dd0));
+    bool sub = false;
+    const float p = psi<false>(F, theta(pos ? s.dnp : s.dnm, s.dd0), sub);
     const float pd = p * s.dd0;
     const float r = F.T * (pos ? s.bp + pd : s.bm - pd);
-    bad = !(s.ok && F.fast && r != 0.0f);
+    bad = !(s.ok && F.fast && r != 0.0f) || sub;
     return r;
 }
 __device__ __forceinline__ float flux_full(const Face& F, const Sten& s)
 {
     const float T = F.T;
-    const float pp = psi(F, theta(s.dnp, s.dd0)), pm = psi(F, theta(s.dnm, s.dd0));
+    bool sub = false;
+    const float pp = psi<true>(F, theta(s.dnp, s.dd0), sub), pm = psi<true>(F, theta(s.dnm, s.dd0), sub);
     return 0.5f * (T + fabsf(T)) * (s.bp + pp * s.dd0) + 0.5f * (T - fabsf(T)) * (s.bm - pm * s.dd0);
 }
 /* the fluxes of tracers Q0..Q1-1 of slab at face f (between f − s and f);
@@ -318,89 +322,116 @@ struct ZCell {      /* cell cc (level k) above face c */
     double ydzf, yvol;
 };
 
-/* Gcs, diffusion, sinking, forcing and apply_tendency of tracers Q0..Q1-1 at
- * one cell; cu (C0 at the cell), cs (C0 above it, sinking tracers), fz and dz
- * (advective and diffusive fluxes of its top face) are carried along z. */
-template <bool LAND, int Q0, int Q1>
-__device__ __forceinline__ void final_half(const FinalArgs& a, const DevGrid& g, const Face& Fb, bool zero,
-                                           const ZCell& z, const float* tt, float* cu, float* fz, float* dz, float* cs)
+/* Gcs, diffusion, sinking, forcing and apply_tendency (and the diffusive flux
+ * of the face below) of tracer q at the cell above face z.c, from its operands:
+ * cq, cn = c at the cell and below it, xm/xp/ym/yp = its horizontal neighbours,
+ * pv = the z-sweep input at the cell, fzq/fq = advective fluxes of its top and
+ * bottom faces, dzq = diffusive flux of its top face, csq = c above the cell
+ * (sinking tracers), pk = plk. */
+template <bool LAND>
+__device__ __forceinline__ void cell_update(const FinalArgs& a, const DevGrid& g, const ZCell& z, int q, const float* tt,
+                                            float cq, float cn, float xm, float xp, float ym, float yp, float pv,
+                                            float fzq, float fq, float dzq, float csq, float pk,
+                                            float& out, float& d)
 {
-    const float* C0 = a.st->tr[a.st->tcur][0];
-    float* OUT = a.st->tr[a.st->tcur ^ 1][0];
-    const uint32_t sz = (uint32_t)g.Px * (uint32_t)g.Py, sy = (uint32_t)g.Px;
-    const uint32_t cc = z.cc, c = z.c;
+    const uint32_t cc = z.cc, c = z.c, sy = (uint32_t)g.Px;
     const int k = z.k;
     const float dt = a.dt;
     const float kvAz = a.kv * (g.dx * g.dy);
-    float f[Q1 - Q0];
+    /* κv·Az·(δz c / dzF) at face c (Advection/tracer_diffusion.jl:1-30) */
+    d = kvAz * divc(lmask<LAND>(g, cc, c, cq - cn), z.dzf, z.ydzf);
+    const float temp3 = pv - z.coef * (lmask<LAND>(g, cc, c, fzq - fq) - cq * z.dT);
+    /* Gcs = tracer_temp - tracer (calc_tracer_tendency!) */
+    float G = temp3 - cq;
+    const float fxa = z.khAx * divc(lmask<LAND>(g, cc - 1, cc, cq - xm), g.dx, g.ydx);
+    const float fxb = z.khAx * divc(lmask<LAND>(g, cc, cc + 1, xp - cq), g.dx, g.ydx);
+    const float fya = z.khAy * divc(lmask<LAND>(g, cc - sy, cc, cq - ym), g.dy, g.ydy);
+    const float fyb = z.khAy * divc(lmask<LAND>(g, cc, cc + sy, yp - cq), g.dy, g.ydy);
+    const float ddx = lmask<LAND>(g, cc, cc + 1, fxb - fxa);
+    const float ddy = lmask<LAND>(g, cc, cc + sy, fyb - fya);
+    const float ddz = lmask<LAND>(g, cc, c, dzq - d);
+    const float lap = z.inv_vol * (ddx + ddy + ddz);
+    G = G + lap * dt;
+    /* tracer_sinking! (Advection/tracer_sinking.jl:1-43) */
+    if (q == PI_PFe_inorg || q == PI_Dust || q == PI_PFe_bio) {
+        const float ws = q == PI_PFe_bio ? a.wsink_org : a.wsink_inorg;
+        const float f0 = (k - 1 <= g.Hz) ? 0.0f : ws * csq;
+        const float f1 = (k >= g.Nz + g.Hz) ? 0.0f : ws * cq;
+        G = G + div_dzC(g, f0 - f1, k - g.Hz) * dt;
+    }
+    /* tracer_forcing! (tracer_forcings.jl:2-50) */
+    switch (q) {
+        case PI_DIC: G = G + tt[PI_DOC] * a.kDOC * dt; break;
+        case PI_DOC: G = G - tt[PI_DOC] * a.kDOC * dt + tt[PI_POC] * a.kPOC * dt; break;
+        case PI_POC: G = G - tt[PI_POC] * a.kPOC * dt; break;
+        case PI_NH4: G = G + tt[PI_DON] * a.kDON * dt - tt[PI_NH4] * a.Nit * dt; break;
+        case PI_NO3: G = G + tt[PI_NH4] * a.Nit * dt; break;
+        case PI_DON: G = G - tt[PI_DON] * a.kDON * dt + tt[PI_PON] * a.kPON * dt; break;
+        case PI_PON: G = G - tt[PI_PON] * a.kPON * dt; break;
+        case PI_PO4: G = G + tt[PI_DOP] * a.kDOP * dt; break;
+        case PI_DOP: G = G - tt[PI_DOP] * a.kDOP * dt + tt[PI_POP] * a.kPOP * dt; break;
+        case PI_POP: G = G - tt[PI_POP] * a.kPOP * dt; break;
+        case PI_DFe: {
+            const float DFe = tt[PI_DFe];
+            const float ex = dmaxf((DFe - a.lig), 0.0f);
+            G = G + tt[PI_PFe_bio] * a.kPOC * dt -
+                (a.lPOC * tt[PI_POC] + a.lmin + a.ldust * tt[PI_Dust]) * DFe * a.DFeFrac * dt +
+                tt[PI_PFe_inorg] * a.kdiss * dt -
+                ex * DFe * a.DFeFrac * a.lFe * dt;
+            break;
+        }
+        case PI_PFe_bio:
+            G = G + tt[PI_POC] * tt[PI_DFe] * a.DFeFrac * a.lPOC * dt - a.kPOC * tt[PI_PFe_bio] * dt;
+            break;
+        case PI_PFe_inorg: {
+            const float DFe = tt[PI_DFe];
+            const float ex = dmaxf((DFe - a.lig), 0.0f);
+            G = G - tt[PI_PFe_inorg] * a.kdiss * dt +
+                (a.lmin + a.ldust * tt[PI_Dust]) * DFe * a.DFeFrac * dt +
+                ex * DFe * a.DFeFrac * a.lFe * dt;
+            break;
+        }
+        default: break;
+    }
+    /* apply_tendency_kernel! (tracer_update.jl:6-20); the tracers without
+     * particle exchange add a zero plk */
+    out = cq + (G + divc(pk, z.vol, z.yvol));
+}
+
+__device__ __forceinline__ int sink_index(int q) { return q == PI_PFe_inorg ? 0 : q == PI_Dust ? 1 : q == PI_PFe_bio ? 2 : -1; }
+
+/* tracers Q0..Q1-1 at one cell: the fluxes of the face below it, the updates,
+ * the stores and the carried state (cu, cs, fz, dz) */
+template <bool LAND, int Q0, int Q1>
+__device__ __forceinline__ void final_half(const FinalArgs& a, const DevGrid& g, const float* C0, float* OUT,
+                                           const Face& Fb, bool zero, const ZCell& z, const float* tt, float* cu,
+                                           float* fz, float* dz, float* cs)
+{
+    constexpr int NQ = Q1 - Q0;
+    const uint32_t sz = (uint32_t)g.Px * (uint32_t)g.Py, sy = (uint32_t)g.Px;
+    const uint32_t cc = z.cc, c = z.c;
+    float f[NQ];
     face_fluxes<LAND, true, Q0, Q1>(g, Fb, zero, c, sz, a.prev, f);
+    float outv[NQ], dv[NQ], cnv[NQ];
 #pragma unroll
     for (int q = Q0; q < Q1; ++q) {
+        const int i = q - Q0;
         const float* co = fld(C0, g, q);
-        const float cq = cu[q];
-        const float cn = ld(co, c);
-        /* κv·Az·(δz c / dzF) at face c (Advection/tracer_diffusion.jl:1-30) */
-        const float d = kvAz * divc(lmask<LAND>(g, cc, c, cq - cn), z.dzf, z.ydzf);
-        const float temp3 = ld(fld(a.prev, g, q), cc) - z.coef * (lmask<LAND>(g, cc, c, fz[q] - f[q - Q0]) - cq * z.dT);
-        /* Gcs = tracer_temp - tracer (calc_tracer_tendency!) */
-        float G = temp3 - cq;
-        const float fxa = z.khAx * divc(lmask<LAND>(g, cc - 1, cc, cq - ld(co, cc - 1)), g.dx, g.ydx);
-        const float fxb = z.khAx * divc(lmask<LAND>(g, cc, cc + 1, ld(co, cc + 1) - cq), g.dx, g.ydx);
-        const float fya = z.khAy * divc(lmask<LAND>(g, cc - sy, cc, cq - ld(co, cc - sy)), g.dy, g.ydy);
-        const float fyb = z.khAy * divc(lmask<LAND>(g, cc, cc + sy, ld(co, cc + sy) - cq), g.dy, g.ydy);
-        const float ddx = lmask<LAND>(g, cc, cc + 1, fxb - fxa);
-        const float ddy = lmask<LAND>(g, cc, cc + sy, fyb - fya);
-        const float ddz = lmask<LAND>(g, cc, c, dz[q] - d);
-        const float lap = z.inv_vol * (ddx + ddy + ddz);
-        G = G + lap * dt;
-        /* tracer_sinking! (Advection/tracer_sinking.jl:1-43) */
-        if (q == PI_PFe_inorg || q == PI_Dust || q == PI_PFe_bio) {
-            const int si = q == PI_PFe_inorg ? 0 : q == PI_Dust ? 1 : 2;
-            const float ws = q == PI_PFe_bio ? a.wsink_org : a.wsink_inorg;
-            const float f0 = (k - 1 <= g.Hz) ? 0.0f : ws * cs[si];
-            const float f1 = (k >= g.Nz + g.Hz) ? 0.0f : ws * cq;
-            G = G + div_dzC(g, f0 - f1, k - g.Hz) * dt;
-            cs[si] = cq;
-        }
-        /* tracer_forcing! (tracer_forcings.jl:2-50) */
-        switch (q) {
-            case PI_DIC: G = G + tt[PI_DOC] * a.kDOC * dt; break;
-            case PI_DOC: G = G - tt[PI_DOC] * a.kDOC * dt + tt[PI_POC] * a.kPOC * dt; break;
-            case PI_POC: G = G - tt[PI_POC] * a.kPOC * dt; break;
-            case PI_NH4: G = G + tt[PI_DON] * a.kDON * dt - tt[PI_NH4] * a.Nit * dt; break;
-            case PI_NO3: G = G + tt[PI_NH4] * a.Nit * dt; break;
-            case PI_DON: G = G - tt[PI_DON] * a.kDON * dt + tt[PI_PON] * a.kPON * dt; break;
-            case PI_PON: G = G - tt[PI_PON] * a.kPON * dt; break;
-            case PI_PO4: G = G + tt[PI_DOP] * a.kDOP * dt; break;
-            case PI_DOP: G = G - tt[PI_DOP] * a.kDOP * dt + tt[PI_POP] * a.kPOP * dt; break;
-            case PI_POP: G = G - tt[PI_POP] * a.kPOP * dt; break;
-            case PI_DFe: {
-                const float DFe = tt[PI_DFe];
-                const float ex = dmaxf((DFe - a.lig), 0.0f);
-                G = G + tt[PI_PFe_bio] * a.kPOC * dt -
-                    (a.lPOC * tt[PI_POC] + a.lmin + a.ldust * tt[PI_Dust]) * DFe * a.DFeFrac * dt +
-                    tt[PI_PFe_inorg] * a.kdiss * dt -
-                    ex * DFe * a.DFeFrac * a.lFe * dt;
-                break;
-            }
-            case PI_PFe_bio:
-                G = G + tt[PI_POC] * tt[PI_DFe] * a.DFeFrac * a.lPOC * dt - a.kPOC * tt[PI_PFe_bio] * dt;
-                break;
-            case PI_PFe_inorg: {
-                const float DFe = tt[PI_DFe];
-                const float ex = dmaxf((DFe - a.lig), 0.0f);
-                G = G - tt[PI_PFe_inorg] * a.kdiss * dt +
-                    (a.lmin + a.ldust * tt[PI_Dust]) * DFe * a.DFeFrac * dt +
-                    ex * DFe * a.DFeFrac * a.lFe * dt;
-                break;
-            }
-            default: break;
-        }
-        /* apply_tendency_kernel! (tracer_update.jl:6-20); the tracers without
-         * particle exchange add a zero plk */
+        cnv[i] = ld(co, c);
         const float pk = has_plk(q) ? ld(fld(a.plk, g, q), cc) : 0.0f;
-        st(fld(OUT, g, q), cc, cq + (G + divc(pk, z.vol, z.yvol)));
-        fz[q] = f[q - Q0]; dz[q] = d; cu[q] = cn;
+        const int si = sink_index(q);
+        cell_update<LAND>(a, g, z, q, tt, cu[q], cnv[i], ld(co, cc - 1), ld(co, cc + 1), ld(co, cc - sy),
+                          ld(co, cc + sy), ld(fld(a.prev, g, q), cc), fz[q], f[i], dz[q], si >= 0 ? cs[si] : 0.0f,
+                          pk, outv[i], dv[i]);
+    }
+#pragma unroll
+    for (int q = Q0; q < Q1; ++q) {
+        const int i = q - Q0;
+        st(fld(OUT, g, q), cc, outv[i]);
+        fz[q] = f[i]; dz[q] = dv[i];
+        const int si = sink_index(q);
+        if (si >= 0) cs[si] = cu[q];
+        cu[q] = cnv[i];
     }
 }
 
@@ -413,6 +444,7 @@ __global__ void __launch_bounds__(256) k_final(FinalArgs a)
     const int nrun = (g.Nz + RUN_Z - 1) / RUN_Z;
     const long long nthr = (long long)g.Nx * g.Ny * nrun;
     const float* C0 = S->tr[S->tcur][0];
+    float* OUT = S->tr[S->tcur ^ 1][0];
     const uint32_t sz = (uint32_t)g.Px * (uint32_t)g.Py;
     const float dt = a.dt;
     const float Az = g.dx * g.dy;
@@ -461,8 +493,8 @@ __global__ void __launch_bounds__(256) k_final(FinalArgs a)
             float tt[PI_NTRACERS];    /* tracer_temp = max(0, c) (tracer_forcings.jl:3-5) */
 #pragma unroll
             for (int q = 0; q < PI_NTRACERS; ++q) tt[q] = dmaxf(0.0f, cu[q]);
-            final_half<LAND, 0, 7>(a, g, Fb, zero, z, tt, cu, fz, dz, cs);
-            final_half<LAND, 7, 14>(a, g, Fb, zero, z, tt, cu, fz, dz, cs);
+            final_half<LAND, 0, 7>(a, g, C0, OUT, Fb, zero, z, tt, cu, fz, dz, cs);
+            final_half<LAND, 7, 14>(a, g, C0, OUT, Fb, zero, z, tt, cu, fz, dz, cs);
             Fa = Fb;
         }
     }

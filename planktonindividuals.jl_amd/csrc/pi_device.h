@@ -102,6 +102,23 @@ __device__ __forceinline__ float dzC_at(const DevGrid& g, long long k) {   /* cl
     return g.dzC[k - 1];
 }
 __device__ __forceinline__ float dzF_at(const DevGrid& g, int k) { return g.dzF[k - 1]; }
+/* a / b in 3 operations for the hot paths, given yd = RN_double(1/b): the
+ * product rounded once to double then to float, which is RN(a/b) except when
+ * a/b is exactly a rounding boundary of the subnormal range (an odd multiple
+ * of 2^-150; in the normal range the rounded product is always right, see
+ * divc).  A wrong rounding there always yields a subnormal float (the two
+ * candidates of an odd multiple M·2^-150, M >= 3, are subnormal; for M = 1 the
+ * wrong candidate is 2^-149), so `sub` is raised for every subnormal result and
+ * the caller recomputes its work item with divc. */
+template <bool EXACT>
+__device__ __forceinline__ float divq(float a, float b, double yd, bool& sub)
+{
+    if (EXACT) return divc(a, b, yd);
+    const float f = (float)((double)a * yd);
+    sub |= __builtin_isfpclass(f, 0x0090);     /* negative or positive subnormal */
+    return f;
+}
+
 /* Julia's max / min of Float32 (NaN if either operand is NaN, -0 < +0) are the
  * IEEE 754-2019 maximum / minimum: one instruction each on gfx950.  For NaN
  * operands the NaN may carry another payload than Julia's x - y. */

@@ -102,3 +102,25 @@ def test_capacity_error_matches(pi, orc):
             steps_gpu = i
             break
     assert steps_ref is not None and steps_gpu == steps_ref
+
+
+def test_parity_subnormal_quotients(pi, orc):
+    """Velocities and tracer values so small that quotients of the interpolation,
+    the random walk, the tracer diffusion and plk/volume fall in the subnormal
+    range on half of the domain: the fast division flags them and the exact
+    recomputation must give the oracle's values."""
+    g = pi.RectilinearGrid(size=(16, 16, 4), x=(0, 16e3), y=(0, 16e3), z=(0, -40))
+    init = pi.fields.default_tracer_init()
+    init["initial_condition"].update(NH4=1.0e-36, DOC=3.0e-35, PO4=2.0e-37, DON=1.0e-38)
+    spec = pi.ModelSpec(g, max_individuals=3000, seed=17, tracer_initial=init,
+                        phyto=pi.phyto_setup({"dvid_P": [1e-3], "mort_reg": [1.0]}, [1500], 1),
+                        bgc_params={"κhP": 1.0e-2, "κvP": 1.0e-5, "κh": 1.0, "κv": 1e-4})
+    u, v, w = pi.workloads.cellular_flow(g, U=1.0, L=4e3)
+    tiny = np.float32(1e-36)
+    u = u * np.where(np.arange(u.shape[0])[:, None] < u.shape[0] // 2, tiny, np.float32(1))[..., None].astype(np.float32)
+    v = (v * tiny).astype(np.float32)
+    w = np.zeros((16, 16, 5), np.float32)
+    w[:8, :, 1:4] = np.float32(3e-37)
+    gm, om = _drive(pi, orc, spec, (u.astype(np.float32), v, w), 60.0, 8)
+    bad = compare_models(gm._native, om, 1)
+    assert not bad, "\n".join(bad[:20])
