@@ -142,7 +142,7 @@ __device__ __forceinline__ void move(const AdvArgs& a, const DevGrid& g, float x
 /* One store entry after its move: cell, stayer flag or queue slot, death
  * bookkeeping.  Every lane of the wave calls it (wave_append); `live` = the
  * entry is active and was moved. */
-__device__ __forceinline__ void place_entry(const AdvArgs& a, DevState* st, const Store& S, uint32_t e, uint32_t nA,
+__device__ __forceinline__ void place_entry(const AdvArgs& a, DevState* st, const GStore& S, uint32_t e, uint32_t nA,
                                             int32_t iteration, bool live, uint8_t fl, float x, float y, float z)
 {
     const DevGrid& g = a.g;
@@ -180,7 +180,7 @@ __device__ __forceinline__ void place_entry(const AdvArgs& a, DevState* st, cons
  * the device state are read before the loops that call these: a load that may
  * alias the loop's stores would be a vector load waiting for every outstanding
  * store.) */
-__device__ __forceinline__ void move_entry(const AdvArgs& a, const Store& S, uint32_t e, long long iteration, uint8_t fl,
+__device__ __forceinline__ void move_entry(const AdvArgs& a, const GStore& S, uint32_t e, long long iteration, uint8_t fl,
                                            float& x, float& y, float& z)
 {
     const DevGrid& g = a.g;
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(256) k_adv(AdvArgs a)
 {
     DevState* st = a.st;
     if (st->err) return;
-    const Store S = st->store[st->scur];
+    const GStore S = gstore(st->store[st->scur]);
     const uint32_t n = st->nstore;
     const uint32_t nA = st->nA;
     const long long iteration = st->iteration + 1;
@@ -256,7 +256,7 @@ __global__ void __launch_bounds__(256) k_place_stay(const DevState* st, const ui
                                                     uint32_t* pkey, uint32_t* psrc)
 {
     if (st->err) return;
-    const Store S = st->store[st->scur];
+    const GStore S = gstore(st->store[st->scur]);
     const uint32_t nA = st->nA;
     for (uint32_t e = xcd_block() * blockDim.x + threadIdx.x; e < nA; e += gridDim.x * blockDim.x) {
         if (!stayf[e]) continue;
@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(256) k_place_small(DevState* st, const uint32_
                                                      uint32_t* mix)
 {
     if (st->err) return;
-    const Store S = st->store[st->scur];
+    const GStore S = gstore(st->store[st->scur]);
     const uint32_t ns = st->nsmall;
     const uint32_t nloop = (ns + 63u) & ~63u;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nloop; t += gridDim.x * blockDim.x) {
@@ -324,7 +324,7 @@ __global__ void __launch_bounds__(256) k_cellsum(const DevState* st, long long n
                                                  const uint32_t* psrc, float* Chl, float* pop)
 {
     if (st->err) return;
-    const float* __restrict__ chl_store = st->store[st->scur].Chl;
+    const PI_GLOBAL float* chl_store = gp(st->store[st->scur].Chl);
     for (long long c = xcd_block() * (long long)blockDim.x + threadIdx.x; c < ncell; c += (long long)gridDim.x * blockDim.x) {
         const uint32_t s0 = seg[c], n = seg[c + 1] - s0;
         float sum = 0.0f;
@@ -382,7 +382,7 @@ __global__ void __launch_bounds__(256) k_par(DevState* st, DevGrid g, const floa
                                              float* PARF_out, float* diag_par, float kc, float kw)
 {
     if (st->err) return;
-    float* par = st->par[1 - st->pcur];
+    PI_GLOBAL float* par = gp(st->par[1 - st->pcur]);
     const int ncol = g.Nx * g.Ny;
     for (int col = blockIdx.x * blockDim.x + threadIdx.x; col < ncol; col += gridDim.x * blockDim.x) {
         const int i = col % g.Nx + 1, j = col / g.Nx + 1;
@@ -551,16 +551,16 @@ __global__ void __launch_bounds__(256) k_phys(PhysArgs a)
     const DevGrid& g = a.g;
     const uint32_t n = a.seg[g.ncell];
     const float dt = a.dt;
-    const Store IN = st->store[st->scur], OUT = st->store[1 - st->scur];
+    const GStore IN = gstore(st->store[st->scur]), OUT = gstore(st->store[1 - st->scur]);
     const long long iteration = st->iteration + 1;
     const float tmodel = (float)iteration * dt;
     /* device-state values are read before the loop (see move_entry) */
-    const float* TRs = st->tr[st->tcur][0];
-    const float* TR_NH4 = TRs + (long long)PI_NH4 * g.ncell, *TR_NO3 = TRs + (long long)PI_NO3 * g.ncell;
-    const float* TR_PO4 = TRs + (long long)PI_PO4 * g.ncell, *TR_DOC = TRs + (long long)PI_DOC * g.ncell;
-    const float* TR_DFe = TRs + (long long)PI_DFe * g.ncell;
-    const float* PAR = st->par[1 - st->pcur];
-    const float* PAR0 = st->par[st->pcur];
+    const PI_GLOBAL float* TRs = gp(st->tr[st->tcur][0]);
+    const PI_GLOBAL float* TR_NH4 = TRs + (long long)PI_NH4 * g.ncell, *TR_NO3 = TRs + (long long)PI_NO3 * g.ncell;
+    const PI_GLOBAL float* TR_PO4 = TRs + (long long)PI_PO4 * g.ncell, *TR_DOC = TRs + (long long)PI_DOC * g.ncell;
+    const PI_GLOBAL float* TR_DFe = TRs + (long long)PI_DFe * g.ncell;
+    const PI_GLOBAL float* PAR = gp(st->par[1 - st->pcur]);
+    const PI_GLOBAL float* PAR0 = gp(st->par[st->pcur]);
     __shared__ uint32_t s_doff[PI_MAXSP];
     for (int i = threadIdx.x; i < a.nsp; i += blockDim.x) s_doff[i] = st->doff[i];
     __syncthreads();
@@ -721,6 +721,34 @@ __global__ void __launch_bounds__(256) k_phys(PhysArgs a)
  * species in slot order (plankton_update.jl:5-29), and the per-species
  * diagnostics num/graz/mort/dvid (diagnostics_kernels.jl:4-12). */
 
+/* the contributions of one entry x of a cell to the ordered sums, per pass:
+ * 0 = grazing loss, 1 = mortality loss, 2 = consumption (alive, or a daughter
+ * reborn into the slot of this dying entry) */
+struct PlkSums {
+    float DIC, NH4, NO3, PO4, DOC, DON, DOP, POC, PON, POP;
+    uint32_t ng, nm, nd, na;
+};
+__device__ __forceinline__ void plk_loss(PlkSums& S, const float* v)
+{
+    S.DOC = S.DOC + v[0]; S.DON = S.DON + v[1]; S.DOP = S.DOP + v[2];
+    S.POC = S.POC + v[3]; S.PON = S.PON + v[4]; S.POP = S.POP + v[5];
+}
+__device__ __forceinline__ void plk_cons(PlkSums& S, float c0, float c1, float c2, float c3, float c4)
+{
+    S.DIC = S.DIC + c0; S.DOC = S.DOC + c1; S.NH4 = S.NH4 + c2; S.NO3 = S.NO3 + c3; S.PO4 = S.PO4 + c4;
+}
+__device__ __forceinline__ void plk_diag(const PlkArgs& a, long long c, uint32_t sp, PlkSums& S)
+{
+    float* d = a.diag + (long long)sp * 4 * a.g.ncell;
+    if (S.na) d[0 * a.g.ncell + c] = d[0 * a.g.ncell + c] + (float)S.na;
+    if (S.ng) d[1 * a.g.ncell + c] = d[1 * a.g.ncell + c] + (float)S.ng;
+    if (S.nm) d[2 * a.g.ncell + c] = d[2 * a.g.ncell + c] + (float)S.nm;
+    if (S.nd) d[3 * a.g.ncell + c] = d[3 * a.g.ncell + c] + (float)S.nd;
+    S.ng = S.nm = S.nd = S.na = 0;
+}
+
+#define PLK_FAST 6   /* segments up to this length are loaded into registers at once */
+
 __global__ void __launch_bounds__(256) k_plk(PlkArgs a)
 {
     if (a.st->err) return;
@@ -728,57 +756,98 @@ __global__ void __launch_bounds__(256) k_plk(PlkArgs a)
     const DevGrid& g = a.g;
     for (long long c = xcd_block() * (long long)blockDim.x + threadIdx.x; c < g.ncell;
          c += (long long)gridDim.x * blockDim.x) {
-        const uint32_t s0 = a.seg[c], s1 = a.seg[c + 1];
-        float DIC = 0.f, NH4 = 0.f, NO3 = 0.f, PO4 = 0.f, DOC = 0.f, DON = 0.f, DOP = 0.f;
-        float POC = 0.f, PON = 0.f, POP = 0.f;
-        uint32_t e = s0;
-        while (e < s1) {
-            const uint32_t sp = a.pkey[e] >> PI_SLOT_BITS;
-            uint32_t e1 = e + 1;
-            while (e1 < s1 && (a.pkey[e1] >> PI_SLOT_BITS) == sp) ++e1;
-            uint32_t ng = 0, nm = 0, nd = 0, na = 0;
-            for (uint32_t x = e; x < e1; ++x) {
-                if (a.con.tag[x] & T_GRAZ) {
-                    DOC = DOC + a.con.v[0][x]; DON = DON + a.con.v[1][x]; DOP = DOP + a.con.v[2][x];
-                    POC = POC + a.con.v[3][x]; PON = PON + a.con.v[4][x]; POP = POP + a.con.v[5][x];
-                    ++ng;
-                }
+        const uint32_t s0 = a.seg[c], s1 = a.seg[c + 1], n = s1 - s0;
+        PlkSums S{};
+        if (n <= PLK_FAST) {
+            /* one batch of independent loads, then the three ordered passes per
+             * species run on registers (predicated, in slot order) */
+            uint32_t key[PLK_FAST];
+            uint8_t tag[PLK_FAST];
+            float v[PLK_FAST][6];
+#pragma unroll
+            for (int x = 0; x < PLK_FAST; ++x) {
+                const bool in = (uint32_t)x < n;
+                key[x] = in ? a.pkey[s0 + x] : 0xFFFFFFFFu;
+                tag[x] = in ? a.con.tag[s0 + x] : (uint8_t)0;
+#pragma unroll
+                for (int j = 0; j < 6; ++j) v[x][j] = in ? a.con.v[j][s0 + x] : 0.0f;
             }
-            for (uint32_t x = e; x < e1; ++x) {
-                if (a.con.tag[x] & T_MORT) {
-                    DOC = DOC + a.con.v[0][x]; DON = DON + a.con.v[1][x]; DOP = DOP + a.con.v[2][x];
-                    POC = POC + a.con.v[3][x]; PON = PON + a.con.v[4][x]; POP = POP + a.con.v[5][x];
-                    ++nm;
-                }
-            }
-            for (uint32_t x = e; x < e1; ++x) {
-                const uint8_t t = a.con.tag[x];
-                if (t & T_ALIVE) {
-                    DIC = DIC + a.con.v[0][x]; DOC = DOC + a.con.v[1][x]; NH4 = NH4 + a.con.v[2][x];
-                    NO3 = NO3 + a.con.v[3][x]; PO4 = PO4 + a.con.v[4][x];
-                    ++na;
-                    if (t & T_DIV) ++nd;
-                } else {
-                    /* died here this step; a daughter born into its slot consumes here */
-                    const long long r = (long long)sp * a.cap + (a.pkey[x] & PI_SLOT_MASK);
-                    if (a.rstamp[r] == it) {
-                        const float* rc = a.rcons + r * 5;
-                        DIC = DIC + rc[0]; DOC = DOC + rc[1]; NH4 = NH4 + rc[2];
-                        NO3 = NO3 + rc[3]; PO4 = PO4 + rc[4];
-                        ++na;
+            uint32_t e = 0;
+            while (e < n) {
+                uint32_t sp = 0;
+#pragma unroll
+                for (int x = 0; x < PLK_FAST; ++x) if ((uint32_t)x == e) sp = key[x] >> PI_SLOT_BITS;
+                uint32_t e1 = n;
+#pragma unroll
+                for (int x = PLK_FAST - 1; x >= 0; --x)
+                    if ((uint32_t)x > e && (uint32_t)x < n && (key[x] >> PI_SLOT_BITS) != sp) e1 = (uint32_t)x;
+#pragma unroll
+                for (int x = 0; x < PLK_FAST; ++x)
+                    if ((uint32_t)x >= e && (uint32_t)x < e1 && (tag[x] & T_GRAZ)) { plk_loss(S, v[x]); ++S.ng; }
+#pragma unroll
+                for (int x = 0; x < PLK_FAST; ++x)
+                    if ((uint32_t)x >= e && (uint32_t)x < e1 && (tag[x] & T_MORT)) { plk_loss(S, v[x]); ++S.nm; }
+#pragma unroll
+                for (int x = 0; x < PLK_FAST; ++x) {
+                    if (!((uint32_t)x >= e && (uint32_t)x < e1)) continue;
+                    if (tag[x] & T_ALIVE) {
+                        plk_cons(S, v[x][0], v[x][1], v[x][2], v[x][3], v[x][4]);
+                        ++S.na;
+                        if (tag[x] & T_DIV) ++S.nd;
+                    } else {
+                        /* died here this step; a daughter born into its slot consumes here */
+                        const long long r = (long long)sp * a.cap + (key[x] & PI_SLOT_MASK);
+                        if (a.rstamp[r] == it) {
+                            const float* rc = a.rcons + r * 5;
+                            plk_cons(S, rc[0], rc[1], rc[2], rc[3], rc[4]);
+                            ++S.na;
+                        }
                     }
                 }
+                plk_diag(a, c, sp, S);
+                e = e1;
             }
-            float* d = a.diag + (long long)sp * 4 * g.ncell;
-            if (na) d[0 * g.ncell + c] = d[0 * g.ncell + c] + (float)na;
-            if (ng) d[1 * g.ncell + c] = d[1 * g.ncell + c] + (float)ng;
-            if (nm) d[2 * g.ncell + c] = d[2 * g.ncell + c] + (float)nm;
-            if (nd) d[3 * g.ncell + c] = d[3 * g.ncell + c] + (float)nd;
-            e = e1;
+        } else {
+            uint32_t e = s0;
+            while (e < s1) {
+                const uint32_t sp = a.pkey[e] >> PI_SLOT_BITS;
+                uint32_t e1 = e + 1;
+                while (e1 < s1 && (a.pkey[e1] >> PI_SLOT_BITS) == sp) ++e1;
+                float v[6];
+                for (uint32_t x = e; x < e1; ++x) {
+                    if (a.con.tag[x] & T_GRAZ) {
+                        for (int j = 0; j < 6; ++j) v[j] = a.con.v[j][x];
+                        plk_loss(S, v); ++S.ng;
+                    }
+                }
+                for (uint32_t x = e; x < e1; ++x) {
+                    if (a.con.tag[x] & T_MORT) {
+                        for (int j = 0; j < 6; ++j) v[j] = a.con.v[j][x];
+                        plk_loss(S, v); ++S.nm;
+                    }
+                }
+                for (uint32_t x = e; x < e1; ++x) {
+                    const uint8_t t = a.con.tag[x];
+                    if (t & T_ALIVE) {
+                        plk_cons(S, a.con.v[0][x], a.con.v[1][x], a.con.v[2][x], a.con.v[3][x], a.con.v[4][x]);
+                        ++S.na;
+                        if (t & T_DIV) ++S.nd;
+                    } else {
+                        const long long r = (long long)sp * a.cap + (a.pkey[x] & PI_SLOT_MASK);
+                        if (a.rstamp[r] == it) {
+                            const float* rc = a.rcons + r * 5;
+                            plk_cons(S, rc[0], rc[1], rc[2], rc[3], rc[4]);
+                            ++S.na;
+                        }
+                    }
+                }
+                plk_diag(a, c, sp, S);
+                e = e1;
+            }
         }
-        a.plk[PI_DIC][c] = DIC; a.plk[PI_NH4][c] = NH4; a.plk[PI_NO3][c] = NO3; a.plk[PI_PO4][c] = PO4;
-        a.plk[PI_DOC][c] = DOC; a.plk[PI_DON][c] = DON; a.plk[PI_DOP][c] = DOP;
-        a.plk[PI_POC][c] = POC; a.plk[PI_PON][c] = PON; a.plk[PI_POP][c] = POP;
+        a.plk[PI_DIC][c] = S.DIC; a.plk[PI_NH4][c] = S.NH4; a.plk[PI_NO3][c] = S.NO3; a.plk[PI_PO4][c] = S.PO4;
+        a.plk[PI_DOC][c] = S.DOC; a.plk[PI_DON][c] = S.DON; a.plk[PI_DOP][c] = S.DOP;
+        a.plk[PI_POC][c] = S.POC; a.plk[PI_PON][c] = S.PON; a.plk[PI_POP][c] = S.POP;
     }
 }
 

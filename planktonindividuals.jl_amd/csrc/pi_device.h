@@ -77,6 +77,28 @@ struct Store {
     uint32_t* cell;
 };
 
+/* Pointers read from device memory (the DevState) are generic pointers, and
+ * accesses through them compile to FLAT instructions (64-bit addresses, both
+ * wait counters).  Kernels access buffers through global-address-space views
+ * instead: SGPR-base addressing with 32-bit offsets, vector-memory counter only. */
+#define PI_GLOBAL __attribute__((address_space(1)))
+template <typename T> __device__ __forceinline__ PI_GLOBAL T* gp(T* p) { return (PI_GLOBAL T*)p; }
+
+struct GStore {
+    PI_GLOBAL uint32_t* key;
+    PI_GLOBAL float *x, *y, *z, *Sz, *Bm, *Cq, *Nq, *Pq, *Chl, *gen, *age;
+    PI_GLOBAL uint8_t* flags;
+    PI_GLOBAL uint32_t* cell;
+};
+__device__ __forceinline__ GStore gstore(const Store& s)
+{
+    GStore g;
+    g.key = gp(s.key); g.x = gp(s.x); g.y = gp(s.y); g.z = gp(s.z); g.Sz = gp(s.Sz); g.Bm = gp(s.Bm);
+    g.Cq = gp(s.Cq); g.Nq = gp(s.Nq); g.Pq = gp(s.Pq); g.Chl = gp(s.Chl); g.gen = gp(s.gen); g.age = gp(s.age);
+    g.flags = gp(s.flags); g.cell = gp(s.cell);
+    return g;
+}
+
 struct SlotTable {               /* per species, stride cap */
     float *x, *y, *z, *Sz, *Bm, *Cq, *Nq, *Pq, *Chl, *gen, *age;
     uint8_t* flags;
