@@ -48,26 +48,109 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(name, n_live, g):
-    """Compulsory HBM bytes of one launch (each logical array counted once per pass
-    its semantics need; float32 = 4 B, flag = 1 B, key/cell/index = 4 B)."""
-    ncell = g.ncells_padded
-    nint = g.Nx * g.Ny * g.Nz
-    if name == "adv":       # R key,x,y,z,flags,cell  W x,y,z,cell,flags  + velocity (vel₀ = vel₁: 3 fields)
-        return 38 * n_live + 12 * ncell
-    if name == "phys":      # R key,src + record 53 B, W record 53 B + contrib 25 B, + 9 gathered fields/cell
-        return 139 * n_live + 36 * nint
-    if name == "plk":       # R seg, key, contrib 25 B/ind; W 10 plk fields per cell
-        return 29 * n_live + 44 * ncell
-    if name == "sweep_x":   # R 14 tracers + u, W 14 temps
-        return (56 + 4 + 56) * nint
-    if name == "sweep_y":   # R 14 temps + 14 tracers + v, W 14 temps
-        return (56 + 56 + 4 + 56) * nint
-    if name == "final":     # R 14 temps + 14 tracers + 10 plk + w, W 14 tracers
-        return (56 + 56 + 40 + 4 + 56) * nint
-    if name == "par":       # R Chl, W par, RMW diag PAR
-        return 16 * nint
-    return None
+# The logical kernels of the step with their compulsory HBM bytes per step
+# (each logical array counted once per pass its semantics need; float32 = 4 B,
+# flag = 1 B, cell id = 4 B, counter-based uniforms = 0 B), per individual
+# (N = live individuals), per interior cell and per capacity slot, and the
+# launch groups (pi_kernel_times names) and rocprof kernels that implement them.
+LOGICAL = {
+    "K_adv":    dict(groups=["adv"], per_ind=29, per_cell=24, per_slot=0,
+                     kernels=["k_adv"]),
+    "K_phys":   dict(groups=["phys"], per_ind=76, per_cell=36, per_slot=0,
+                     kernels=["k_phys"]),
+    "K_bin":    dict(groups=["bin", "plk"], per_ind=4, per_cell=48, per_slot=0,
+                     kernels=["k_cellcount", "k_place_stay", "k_place_small", "k_mixsort", "k_bigcells",
+                              "k_cellsum", "k_plk", "k_scan_partials", "k_scan_top", "k_scan_final"]),
+    "K_div":    dict(groups=["marks+scan"], per_ind=0, per_cell=0, per_slot=2,
+                     kernels=["k_marks", "k_capcheck"]),
+    "K_dst3fl": dict(groups=["sweep_x", "fill", "sweep_y", "final"], per_ind=0, per_cell=164, per_slot=0,
+                     kernels=["k_sweep_x", "k_sweep_y", "k_fill_line", "k_final"]),
+    "K_par":    dict(groups=["par"], per_ind=0, per_cell=8, per_slot=0, kernels=["k_par"]),
+}
+
+
+def logical_bytes(name, n_live, n_cells, n_slots):
+    L = LOGICAL[name]
+    return L["per_ind"] * n_live + L["per_cell"] * n_cells + L["per_slot"] * n_slots
+
+
+def step_bytes(n_live, n_cells, n_slots):
+    """B_step = 109 N + 2 N_cap + 280 N_cells: the sum of the logical kernels."""
+    return sum(logical_bytes(k, n_live, n_cells, n_slots) for k in LOGICAL)
+
+
+def rocprof_base(name):
+    n = name.split("(")[0].replace("void ", "").strip()
+    return n.split("<")[0]
+
+
+def measured_traffic(logical):
+    """HBM bytes per step of a logical kernel from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy probe), or None."""
+    path = os.path.join(REPO, "profiles", "traffic_3d_256x256x64_2sp.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    fs, ws = t["scale"]["fetch"], t["scale"]["write"]
+    tot, seen = 0.0, False
+    for k, v in t["kernels"].items():
+        if rocprof_base(k) in LOGICAL[logical]["kernels"]:
+            tot += (v["fetch_kib_per_step"] * fs + v["write_kib_per_step"] * ws) * 1024.0
+            seen = True
+    return tot if seen else None
+
+
+def rocpd_summary(db, title, cmd):
+    """Markdown table of a rocprofv3 kernel trace (rocpd SQLite output)."""
+    import sqlite3
+    c = sqlite3.connect(db)
+    rows = c.execute("select name, count(*), sum(duration), avg(duration), min(duration), max(duration) "
+                     "from kernels group by name order by sum(duration) desc").fetchall()
+    tot = sum(r[2] for r in rows)
+    out = [f"# {title}", "", "Command (from the repository root, one MI355X, ROCm 7.2):", "", "```", cmd, "```", "",
+           "Kernel trace statistics (durations in microseconds):", "",
+           "| kernel | calls | total us | avg us | min us | max us | % |", "|---|---|---|---|---|---|---|"]
+    for n, k, sm, a, mn, mx in rows:
+        out.append(f"| {n.split('(')[0]} | {k} | {sm / 1e3:.1f} | {a / 1e3:.1f} | {mn / 1e3:.1f} | {mx / 1e3:.1f} | "
+                   f"{100 * sm / tot:.1f} |")
+    return "\n".join(out)
+
+
+def pmc_summary(fetch_db, write_db, steps):
+    """Per-kernel FETCH_SIZE / WRITE_SIZE (KiB) per step of a --pmc-probe run:
+    dispatches of the last `steps` steps (the first step after an upload sorts
+    from slot order and is not typical), and the scale factors of the copy probe
+    (k_probe_copy reads and writes 262144 KiB)."""
+    import collections
+    import sqlite3
+
+    def load(db, cname):
+        c = sqlite3.connect(db)
+        per = collections.defaultdict(float)
+        for d, k, n, v in c.execute("select dispatch_id, kernel_name, counter_name, value from counters_collection"):
+            if n.startswith(cname):
+                per[(d, k.split("(")[0])] += v
+        byk = collections.defaultdict(list)
+        for (d, k), v in sorted(per.items()):
+            byk[k].append(v)
+        return byk
+
+    f, w = load(fetch_db, "FETCH_SIZE"), load(write_db, "WRITE_SIZE")
+    probe = [k for k in f if "k_probe_copy" in k][0]
+    scale = {"fetch": 262144.0 / f[probe][0], "write": 262144.0 / w[probe][0]}
+    fin = [k for k in f if rocprof_base(k) == "k_finish"]
+    nsteps_total = len(f[fin[0]]) if fin else steps
+    kernels = {}
+    for k in f:
+        if "probe" in k:
+            continue
+        fv, wv = f[k], w.get(k, [0.0] * len(f[k]))
+        per_step = max(1, len(fv) // max(1, nsteps_total))
+        last = slice(len(fv) - per_step * steps, len(fv))
+        kernels[k] = {"fetch_kib_per_step": sum(fv[last]) / steps, "write_kib_per_step": sum(wv[last]) / steps,
+                      "dispatches_per_step": per_step}
+    return {"scale": scale, "steps_averaged": steps, "kernels": kernels}
 
 
 def cpu_baseline(pi, seconds_hint=20.0):
@@ -104,7 +187,19 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: a plain device copy of 256 MiB (k_probe_copy, known traffic, to "
                          "scale the counters), then --warmup + --steps steps without timing; prints no JSON")
+    ap.add_argument("--rocpd-summary", nargs=3, metavar=("DB", "TITLE", "CMD"),
+                    help="print the markdown kernel table of a rocprofv3 kernel-trace database and exit")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DB", "WRITE_DB", "OUT_JSON"),
+                    help="write the per-step FETCH/WRITE summary of two --pmc-probe runs and exit")
     args = ap.parse_args()
+    if args.rocpd_summary:
+        print(rocpd_summary(*args.rocpd_summary))
+        return
+    if args.pmc_summary:
+        res = pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.steps)
+        with open(args.pmc_summary[2], "w") as f:
+            json.dump(res, f, indent=1)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)
@@ -137,14 +232,18 @@ def main():
     nat.run(dt, args.warmup)
     nat.sync()
     live0 = sum(nat.count_active(s) for s in range(args.species))
-    nat.enable_timing(True)
     t1 = time.perf_counter()
-    nat.run(dt, args.steps)
+    nat.run(dt, args.steps)                # steady steps replay the captured step graph
     nat.sync()
     t2 = time.perf_counter()
+    live1 = sum(nat.count_active(s) for s in range(args.species))
+    # per-kernel breakdown: HIP events around each launch group, eager launches
+    nkt = max(5, min(args.steps, 20))
+    nat.enable_timing(True)
+    nat.run(dt, nkt)
+    nat.sync()
     kt = nat.kernel_times()
     nat.enable_timing(False)
-    live1 = sum(nat.count_active(s) for s in range(args.species))
     wall = t2 - t1
     ms_step = 1e3 * wall / args.steps
     live = 0.5 * (live0 + live1)
@@ -152,19 +251,24 @@ def main():
     cell_steps = g.Nx * g.Ny * g.Nz * args.steps / wall
     log(f"[bench] {args.steps} steps in {wall:.3f} s ({ms_step:.3f} ms/step), live {live0} -> {live1}")
 
-    kernels = {}
-    for name, (ms, cnt) in kt.items():
-        b = algorithmic_bytes(name, live, g)
-        avg = ms / max(cnt, 1)
-        kernels[name] = {"ms_per_launch": avg, "launches": cnt,
-                         "share": ms / max(1e-9, sum(x[0] for x in kt.values())),
-                         "gbps_algorithmic": (b / (avg * 1e-3) / 1e9) if b else None}
-    dom = max((k for k in kernels if kernels[k]["gbps_algorithmic"] is not None),
-              key=lambda k: kernels[k]["ms_per_launch"] * kernels[k]["launches"])
-    ach = kernels[dom]["gbps_algorithmic"] * 1e9
+    n_cells = g.Nx * g.Ny * g.Nz
+    n_slots = args.species * 2 * args.individuals
+    kernels = {name: {"ms_per_launch": ms / max(cnt, 1), "launches": cnt} for name, (ms, cnt) in kt.items()}
+    logical = {}
+    for lk, L in LOGICAL.items():
+        ms = sum(kt[gname][0] for gname in L["groups"] if gname in kt) / nkt     # per step
+        b = logical_bytes(lk, live, n_cells, n_slots)
+        logical[lk] = {"ms_per_step": ms, "bytes_per_step": b, "gbps_algorithmic": b / (ms * 1e-3) / 1e9 if ms else None,
+                       "launch_groups": L["groups"]}
+    dom = max(logical, key=lambda k: logical[k]["ms_per_step"])
+    ach = logical[dom]["gbps_algorithmic"] * 1e9
+    traffic = measured_traffic(dom)
     roof = {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": ach / HBM_PEAK, "traffic": None,
-            "bytes_per_launch": algorithmic_bytes(dom, live, g)}
+            "frac": ach / HBM_PEAK, "traffic": traffic, "bytes_per_launch": logical[dom]["bytes_per_step"],
+            "launch_ms": logical[dom]["ms_per_step"]}
+    bstep = step_bytes(live, n_cells, n_slots)
+    step_roof = {"bytes_per_step": bstep, "achieved": bstep / (ms_step * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
+                 "unit": "GB/s", "frac": bstep / (ms_step * 1e-3) / HBM_PEAK}
     base = None
     if not args.no_cpu:
         del model, nat
@@ -192,7 +296,9 @@ def main():
         "tracer_cell_steps_per_s": cell_steps,
         "live_individuals": [live0, live1],
         "roofline": roof,
+        "step_roofline": step_roof,
         "cpu_baseline": base,
+        "logical_kernels": logical,
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)

@@ -77,6 +77,11 @@ struct pi_model {
     std::vector<void*> allocs;
     bool timing = false;
     std::vector<KTimer> timers;
+    /* one steady step (vel₀ = vel₁, no timing) captured as a HIP graph, replayed
+     * by pi_run; dropped when a launch argument changes (velocity buffers,
+     * diagnostic tracers, Δt) */
+    hipGraphExec_t gexec = nullptr;
+    float gdt = 0.0f;
     size_t tcount = 0;
 };
 
@@ -357,10 +362,17 @@ pi_status pi_model_create(const pi_model_desc* d, pi_model** out)
     return PI_OK;
 }
 
+static void drop_graph(pi_model* m)
+{
+    if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
+    m->gexec = nullptr;
+}
+
 pi_status pi_model_destroy(pi_model* m)
 {
     if (!m) return PI_OK;
     (void)hipStreamSynchronize(m->stream);
+    drop_graph(m);
     for (auto& t : m->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (void* p : m->allocs) (void)hipFree(p);
     (void)hipStreamDestroy(m->stream);
@@ -374,6 +386,7 @@ pi_status pi_set_diag_tracers(pi_model* m, int32_t n, const int32_t* ids)
     HIPCHECK(hipStreamSynchronize(m->stream));
     for (int32_t a = 0; a < n; ++a) if (ids[a] < 0 || ids[a] >= PI_NTRACERS) return fail(PI_ERR_ARG, "tracer id");
     /* reuse or allocate accumulators */
+    drop_graph(m);
     std::vector<float*> keep = m->diag_tr;
     m->diag_tr.clear(); m->diag_ids.clear();
     for (int32_t a = 0; a < n; ++a) {
@@ -470,6 +483,7 @@ pi_status pi_set_velocity(pi_model* m, const float* u, const float* v, const flo
     }
     m->v1 = b;
     m->vel_same = false;
+    drop_graph(m);
     return PI_OK;
 }
 
@@ -637,11 +651,40 @@ pi_status pi_step(pi_model* m, float dt)
     return enqueue_step(m, dt);
 }
 
+/* n steps.  Steady steps (vel₀ = vel₁, timing off) replay one captured step
+ * graph: the launch sequence and its arguments are identical from step to step
+ * because everything a step flips or counts lives in the DevState. */
 pi_status pi_run(pi_model* m, float dt, int64_t n)
 {
+    if (!m) return fail(PI_ERR_ARG, "null model");
+    pi_status s = rebuild_if_dirty(m);
+    if (s != PI_OK) return s;
+    if (m->h_state.err) return fail(PI_ERR_CAPACITY, "model is in an error state from an earlier step");
     for (int64_t i = 0; i < n; ++i) {
-        pi_status s = pi_step(m, dt);
-        if (s != PI_OK) return s;
+        if (m->timing || !m->vel_same) {
+            s = enqueue_step(m, dt);
+            if (s != PI_OK) return s;
+            continue;
+        }
+        if (!m->gexec || m->gdt != dt) {
+            drop_graph(m);
+            hipGraph_t graph = nullptr;
+            HIPCHECK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+            s = enqueue_step(m, dt);          /* records the launches and advances the host counters */
+            hipGraph_t g2 = nullptr;
+            const hipError_t e = hipStreamEndCapture(m->stream, &g2);
+            graph = g2;
+            if (s != PI_OK) { if (graph) (void)hipGraphDestroy(graph); return s; }
+            if (e != hipSuccess) { if (graph) (void)hipGraphDestroy(graph); return fail(PI_ERR_HIP, hipGetErrorString(e)); }
+            const hipError_t e2 = hipGraphInstantiate(&m->gexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e2 != hipSuccess) { m->gexec = nullptr; return fail(PI_ERR_HIP, hipGetErrorString(e2)); }
+            m->gdt = dt;
+        } else {
+            m->host_iter += 1;                /* what enqueue_step does on the host for a steady step */
+            m->last_dt = dt;
+        }
+        HIPCHECK(hipGraphLaunch(m->gexec, m->stream));
     }
     return PI_OK;
 }

@@ -124,3 +124,34 @@ def test_parity_subnormal_quotients(pi, orc):
     gm, om = _drive(pi, orc, spec, (u.astype(np.float32), v, w), 60.0, 8)
     bad = compare_models(gm._native, om, 1)
     assert not bad, "\n".join(bad[:20])
+
+
+def test_graph_replay_matches_eager(pi):
+    """pi_run replays one captured step graph for steady steps; its result must be
+    the eager step-by-step result (pinned to the oracle by the other tests) bit
+    for bit, also across a velocity change (graph dropped and captured again) and
+    a change of Δt."""
+    spec, (u, v, w) = pi.workloads.config_3d(n=16, nz=8, N=1500, nsp=2, cap=4000, seed=5, nmodes=12,
+                                             overrides={"dvid_P": [1e-3], "mort_P": [5e-4]}, kappa=(1.0, 1e-4))
+    g = spec.grid
+    A = pi.PlanktonModel(pi.HIPArch(), spec=spec)._native
+    B = pi.PlanktonModel(pi.HIPArch(), spec=spec)._native
+    parf = np.full((g.Nx, g.Ny), 600.0, np.float32)
+    temp = np.full((g.Nx, g.Ny, g.Nz), 22.0, np.float32)
+    for h in (A, B):
+        h.set_velocity(u * 20, v * 20, w * 20); h.set_parf(parf); h.set_temperature(temp)
+    A.run(60.0, 7)
+    for _ in range(7):
+        B.step(60.0)
+    for h in (A, B):
+        h.set_velocity(u * 30, v * 30, w * 30)
+    A.run(60.0, 4)
+    A.run(30.0, 3)
+    for _ in range(4):
+        B.step(60.0)
+    for _ in range(3):
+        B.step(30.0)
+    A.sync(); B.sync()
+    assert A.iteration() == B.iteration() == 14
+    bad = compare_models(A, B, 2)
+    assert not bad, "\n".join(bad[:20])
