@@ -143,9 +143,9 @@ def pmc_summary(fetch_db, write_db, steps):
     nsteps_total = len(f[fin[0]]) if fin else steps
     kernels = {}
     for k in f:
-        if "probe" in k:
-            continue
         fv, wv = f[k], w.get(k, [0.0] * len(f[k]))
+        if "probe" in k or len(fv) % max(1, nsteps_total):      # the probe; launches that are not per step
+            continue
         per_step = max(1, len(fv) // max(1, nsteps_total))
         last = slice(len(fv) - per_step * steps, len(fv))
         kernels[k] = {"fetch_kib_per_step": sum(fv[last]) / steps, "write_kib_per_step": sum(wv[last]) / steps,
