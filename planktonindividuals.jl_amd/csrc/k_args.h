@@ -106,8 +106,8 @@ struct ScanIn {
 void launch_adv(hipStream_t st, int blocks, const AdvArgs& a);
 void launch_marks(hipStream_t s, const DevState* st, const uint8_t* sflag, uint32_t* bmA, uint32_t* bmD,
                   int nsp, int nw, long long cap);
-void launch_cellcount(hipStream_t s, const DevState* st, long long ncell, const uint32_t* segp, const uint32_t* P,
-                      const uint32_t* cnts, uint32_t* cnt);
+void launch_cellcount(hipStream_t s, DevState* st, long long ncell, const uint32_t* segp, const uint32_t* P,
+                      const uint32_t* cnts, uint32_t* cnt, uint32_t* lng);
 void launch_place_stay(hipStream_t s, int blocks, const DevState* st, const uint32_t* seg, const uint32_t* segp,
                        const uint32_t* P, const uint32_t* stayf, const uint32_t* newc, uint32_t* pkey, uint32_t* psrc);
 void launch_place_small(hipStream_t s, int blocks, DevState* st, const uint32_t* seg, const uint32_t* small,
@@ -115,16 +115,19 @@ void launch_place_small(hipStream_t s, int blocks, DevState* st, const uint32_t*
 void launch_mixsort(hipStream_t s, int blocks, DevState* st, const uint32_t* seg, const uint32_t* mix,
                     uint32_t* pkey, uint32_t* psrc, uint32_t* big);
 void launch_bigcells(hipStream_t s, int blocks, DevState* st, const uint32_t* big, const uint32_t* seg,
-                     uint32_t* pkey, uint32_t* psrc, uint32_t* skey, uint32_t* ssrc);
+                     const uint32_t* segp, const uint32_t* P, uint32_t* pkey, uint32_t* psrc, uint32_t* skey,
+                     uint32_t* ssrc);
 void launch_cellsum(hipStream_t s, const DevState* st, long long ncell, const uint32_t* seg, const uint32_t* psrc,
-                    float* Chl, float* pop);
+                    float* Chl, float* pop, const uint32_t* lng, int long_blocks);
 void launch_par(hipStream_t s, int blocks, DevState* st, const DevGrid& g, const float* Chl, const float* PARF_in,
                 float* PARF_out, float* diag_par, float kc, float kw);
 void launch_phys(hipStream_t st, int blocks, const PhysArgs& a);
-void launch_plk(hipStream_t st, int blocks, const PlkArgs& a);
+void launch_plk(hipStream_t st, int blocks, const PlkArgs& a, const uint32_t* lng, uint32_t* lg, uint32_t* lm,
+                int long_blocks);
 /* k_tracers.hip */
 void launch_sweep(hipStream_t s, int blocks, const SweepArgs& a);
 void launch_fill(hipStream_t s, int blocks, const FillArgs& a);
+void launch_halo_copy(hipStream_t s, const FillArgs& a);
 void launch_final(hipStream_t s, int blocks, const FinalArgs& a);
 /* k_util.hip */
 void launch_scan(hipStream_t s, const ScanIn& in, uint32_t* partials, uint32_t* out);

@@ -12,6 +12,10 @@
  *               cells are listed and re-sorted (k_mixsort, k_bigcells).
  *   k_cellsum   per cell: pop = count, Chl = ordered sum (the reference's
  *               atomic order under julia -t 1).
+ *   k_*_long    cells with more than LONG_SEG entries (k_cellcount lists
+ *               them): one block per cell, the entries staged through LDS in
+ *               chunks and the ordered sums run by one wave, so the chain of
+ *               additions runs at the add latency instead of a load each.
  *   k_marks     survivors/dividers as per-species slot bitmaps, from per-slot
  *               flag bytes written by k_phys (no atomics).
  *   k_par       Beer-Lambert column scan (PAR), one lane per column.
@@ -27,6 +31,10 @@
 #include "k_args.h"
 
 #define SMALL_SEG 16
+#define LONG_SEG 64       /* longer cells take the one-block-per-cell kernels */
+#define LCH 1024          /* entries per staged chunk of those kernels */
+#define LPAD (LCH + 4)    /* row stride of the staged values (rows start in different LDS banks) */
+#define MAXARR 2048       /* arrivals of a re-sorted cell sorted in LDS */
 
 /* ---------------------------------------------------------------- advection
  * u_itpl / v_itpl / w_itpl with ac = true (Individuals/Advection/interpolation.jl:1-83).
@@ -240,13 +248,23 @@ __global__ void __launch_bounds__(256) k_marks(const DevState* st, const uint8_t
 
 /* per cell: stayers (their count from the stayer prefix P over the cell's
  * previous segment segp) + queued arrivals */
-__global__ void __launch_bounds__(256) k_cellcount(const DevState* st, long long ncell, const uint32_t* segp,
-                                                   const uint32_t* P, const uint32_t* cnts, uint32_t* cnt)
+__global__ void __launch_bounds__(256) k_cellcount(DevState* st, long long ncell, const uint32_t* segp,
+                                                   const uint32_t* P, const uint32_t* cnts, uint32_t* cnt,
+                                                   uint32_t* lng)
 {
     if (st->err) return;
     const bool have = st->nA > 0;
-    for (long long c = xcd_block() * (long long)blockDim.x + threadIdx.x; c < ncell; c += (long long)gridDim.x * blockDim.x)
-        cnt[c] = (have ? P[segp[c + 1]] - P[segp[c]] : 0u) + cnts[c];
+    const long long nloop = (ncell + 63) & ~63ll;     /* whole waves: wave_append */
+    for (long long c = xcd_block() * (long long)blockDim.x + threadIdx.x; c < nloop; c += (long long)gridDim.x * blockDim.x) {
+        bool isl = false;
+        if (c < ncell) {
+            const uint32_t n = (have ? P[segp[c + 1]] - P[segp[c]] : 0u) + cnts[c];
+            cnt[c] = n;
+            isl = n > LONG_SEG;
+        }
+        const uint32_t li = wave_append(&st->nlong, isl);
+        if (isl) lng[li] = (uint32_t)c;
+    }
 }
 
 /* stayers: the stayers of a cell keep their (species, slot) order; position =
@@ -327,6 +345,7 @@ __global__ void __launch_bounds__(256) k_cellsum(const DevState* st, long long n
     const PI_GLOBAL float* chl_store = gp(st->store[st->scur].Chl);
     for (long long c = xcd_block() * (long long)blockDim.x + threadIdx.x; c < ncell; c += (long long)gridDim.x * blockDim.x) {
         const uint32_t s0 = seg[c], n = seg[c + 1] - s0;
+        if (n > LONG_SEG) continue;                  /* k_cellsum_long */
         float sum = 0.0f;
         for (uint32_t a = 0; a < n; ++a) sum = sum + chl_store[psrc[s0 + a]];
         pop[c] = (float)n;
@@ -334,16 +353,187 @@ __global__ void __launch_bounds__(256) k_cellsum(const DevState* st, long long n
     }
 }
 
-/* Long segments: one block per cell, bitonic sort in a padded scratch region
- * [2*s0, 2*s0 + pow2(n)) that no other segment overlaps. */
+/* ordered sum of m staged values (row, 16-byte aligned) onto s: blocks of 32
+ * values, the LDS reads of the next block issued before the 32 additions of
+ * the current one */
+__device__ __forceinline__ float add32(float s, const float4 (&v)[8])
+{
+#pragma unroll
+    for (int q = 0; q < 8; ++q) { s = s + v[q].x; s = s + v[q].y; s = s + v[q].z; s = s + v[q].w; }
+    return s;
+}
+__device__ __forceinline__ float chain_sum(float s, const float* row, uint32_t m)
+{
+    const float4* r4 = (const float4*)row;
+    const uint32_t nb = m >> 5;
+    if (nb) {
+        float4 cur[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) cur[q] = r4[q];
+        for (uint32_t b = 1; b < nb; ++b) {
+            float4 nxt[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) nxt[q] = r4[b * 8 + q];
+            s = add32(s, cur);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) cur[q] = nxt[q];
+        }
+        s = add32(s, cur);
+    }
+    for (uint32_t i = nb << 5; i < m; ++i) s = s + row[i];
+    return s;
+}
+
+/* gathers the Chl values of entries [from, min(from + LCH, end)) into dst, U
+ * per thread (thread t0 + u·step), every load of a thread issued at once */
+template <int U>
+__device__ __forceinline__ void stage_chl(const PI_GLOBAL float* chl, const uint32_t* psrc, uint32_t from, uint32_t end,
+                                          uint32_t t0, uint32_t step, float* dst)
+{
+    uint32_t src[U];
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t i = t0 + u * step;
+        src[u] = i < LCH && from + i < end ? psrc[from + i] : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = src[u] != 0xFFFFFFFFu ? chl[src[u]] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (src[u] != 0xFFFFFFFFu) dst[t0 + u * step] = v[u];
+}
+
+/* k_cellsum for the listed long cells: the Chl values of a chunk are gathered
+ * into LDS by waves 1-3 while wave 0 adds up the previous chunk */
+__global__ void __launch_bounds__(256) k_cellsum_long(const DevState* st, const uint32_t* lng, const uint32_t* seg,
+                                                      const uint32_t* psrc, float* Chl, float* pop)
+{
+    if (st->err) return;
+    __shared__ __align__(16) float buf[2][LCH];
+    const PI_GLOBAL float* chl_store = gp(st->store[st->scur].Chl);
+    const uint32_t nl = st->nlong;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t b = blockIdx.x; b < nl; b += gridDim.x) {
+        const uint32_t c = lng[b];
+        const uint32_t s0 = seg[c], s1 = seg[c + 1];
+        float sum = 0.0f;
+        stage_chl<4>(chl_store, psrc, s0, s1, t, 256, buf[0]);
+        __syncthreads();
+        uint32_t k = 0;
+        for (uint32_t base = s0; base < s1; base += LCH, ++k) {
+            const uint32_t nb = base + LCH;
+            if (t >= 64) {
+                if (nb < s1) stage_chl<6>(chl_store, psrc, nb, s1, t - 64, 192, buf[(k + 1) & 1]);
+            } else if (t == 0) {
+                sum = chain_sum(sum, buf[k & 1], min((uint32_t)LCH, s1 - base));
+            }
+            __syncthreads();
+        }
+        if (t == 0) { pop[c] = (float)(s1 - s0); Chl[c] = sum; }
+    }
+}
+
+/* Long re-sorted segments, one block per cell.  The cell holds its stayers,
+ * already in key order, then its arrivals in any order.  With at most MAXARR
+ * arrivals: they are sorted in LDS (bitonic) and both runs are merged, each
+ * entry's place = its rank in its own run + the number of keys of the other
+ * run below it (keys are unique), written to the scratch region
+ * [2*s0, 2*s0 + n) that no other segment overlaps and copied back.  With
+ * more, the whole segment is sorted in the scratch region [2*s0, 2*s0 + pow2(n)). */
+/* the number of keys below key in the sorted k[0..A2), A2 a power of two:
+ * a fixed number of steps, no branch */
+__device__ __forceinline__ uint32_t rank_below(const uint32_t* k, uint32_t A2, uint32_t key)
+{
+    uint32_t p = 0;
+    for (uint32_t h = A2; h > 0; h >>= 1)
+        if (p + h <= A2 && k[p + h - 1] < key) p += h;
+    return p;
+}
+__device__ __forceinline__ uint32_t count_below(const uint32_t* k, uint32_t n, uint32_t key)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (k[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
 __global__ void __launch_bounds__(256) k_bigcells(DevState* st, const uint32_t* big, const uint32_t* seg,
+                                                  const uint32_t* segp, const uint32_t* P,
                                                   uint32_t* pkey, uint32_t* psrc, uint32_t* skey, uint32_t* ssrc)
 {
     if (st->err) return;
+    __shared__ uint32_t AK[MAXARR], AS[MAXARR];
     const uint32_t nb = st->nbig;
+    const bool have = st->nA > 0;
     for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
         const uint32_t c = big[b];
         const uint32_t s0 = seg[c], n = seg[c + 1] - s0;
+        const uint32_t nst = have ? P[segp[c + 1]] - P[segp[c]] : 0u;   /* stayers, as k_cellcount */
+        const uint32_t m = n - nst;
+        if (m <= MAXARR) {
+            uint32_t A2 = 1;
+            while (A2 < m) A2 <<= 1;
+            for (uint32_t i = threadIdx.x; i < A2; i += blockDim.x) {
+                AK[i] = i < m ? pkey[s0 + nst + i] : 0xFFFFFFFFu;
+                AS[i] = i < m ? psrc[s0 + nst + i] : 0u;
+            }
+            __syncthreads();
+            for (uint32_t k = 2; k <= A2; k <<= 1) {
+                for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                    for (uint32_t i = threadIdx.x; i < A2; i += blockDim.x) {
+                        const uint32_t l = i ^ j;
+                        if (l > i) {
+                            const bool up = (i & k) == 0;
+                            const uint32_t ki = AK[i], kl = AK[l];
+                            if ((ki > kl) == up) {
+                                AK[i] = kl; AK[l] = ki;
+                                const uint32_t t = AS[i]; AS[i] = AS[l]; AS[l] = t;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            uint32_t* K = skey + 2ull * s0;
+            uint32_t* S = ssrc + 2ull * s0;
+            for (uint32_t base = 0; base < nst; base += 4 * 256) {
+                uint32_t key[4], src[4], pos[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t i = base + u * 256 + threadIdx.x;
+                    key[u] = i < nst ? pkey[s0 + i] : 0u;
+                    src[u] = i < nst ? psrc[s0 + i] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) pos[u] = base + u * 256 + threadIdx.x + rank_below(AK, A2, key[u]);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (base + u * 256 + threadIdx.x < nst) { K[pos[u]] = key[u]; S[pos[u]] = src[u]; }
+                }
+            }
+            for (uint32_t base = 0; base < m; base += 4 * 256) {
+                uint32_t pos[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t q = base + u * 256 + threadIdx.x;
+                    pos[u] = q < m ? q + count_below(pkey + s0, nst, AK[q]) : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t q = base + u * 256 + threadIdx.x;
+                    if (q < m) { K[pos[u]] = AK[q]; S[pos[u]] = AS[q]; }
+                }
+            }
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+                pkey[s0 + i] = K[i];
+                psrc[s0 + i] = S[i];
+            }
+            __syncthreads();
+            continue;
+        }
         uint32_t P2 = 1;
         while (P2 < n) P2 <<= 1;
         uint32_t* K = skey + 2ull * s0;
@@ -757,6 +947,7 @@ __global__ void __launch_bounds__(256) k_plk(PlkArgs a)
     for (long long c = xcd_block() * (long long)blockDim.x + threadIdx.x; c < g.ncell;
          c += (long long)gridDim.x * blockDim.x) {
         const uint32_t s0 = a.seg[c], s1 = a.seg[c + 1], n = s1 - s0;
+        if (n > LONG_SEG) continue;                  /* k_plk_long */
         PlkSums S{};
         if (n <= PLK_FAST) {
             /* one batch of independent loads, then the three ordered passes per
@@ -851,6 +1042,201 @@ __global__ void __launch_bounds__(256) k_plk(PlkArgs a)
     }
 }
 
+/* exclusive prefix over the 256 threads of a block of v; *tot = the block total */
+__device__ __forceinline__ uint32_t block_excl(uint32_t v, uint32_t* ws, uint32_t& tot)
+{
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) ws[w] = x;
+    __syncthreads();
+    uint32_t off = 0;
+    for (uint32_t i = 0; i < w; ++i) off += ws[i];
+    tot = ws[0] + ws[1] + ws[2] + ws[3];
+    __syncthreads();
+    return off + x - v;
+}
+
+/* The ordered sums of k_plk for the listed long cells, one block per cell.
+ * The sums live in lanes 0-9 of wave 0, one tracer per lane in the order
+ * DIC NH4 NO3 PO4 DOC DON DOP POC PON POP.  Per species range (slot order):
+ * the grazed and the dead entries are listed in order (block prefix sums),
+ * then the three passes of the reference run as chains over values staged
+ * in LDS, waves 1-3 staging the next chunk while wave 0 adds up the current
+ * one.  An entry that contributes nothing to a tracer is staged as +0.0: a
+ * sum that starts at +0.0 never becomes −0.0, so adding +0.0 leaves it
+ * unchanged, as skipping the entry does. */
+template <int U>
+__device__ __forceinline__ void stage_loss(const PlkArgs& a, const uint32_t* L, uint32_t from, uint32_t nL,
+                                           uint32_t t0, uint32_t step, float (*dst)[LPAD])
+{
+    uint32_t x[U];
+    float v[U][6];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t i = t0 + u * step;
+        x[u] = i < LCH && from + i < nL ? L[from + i] : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int r = 0; r < 6; ++r) v[u][r] = x[u] != 0xFFFFFFFFu ? a.con.v[r][x[u]] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (x[u] != 0xFFFFFFFFu) {
+#pragma unroll
+            for (int r = 0; r < 6; ++r) dst[r][t0 + u * step] = v[u][r];
+        }
+}
+/* returns (consuming entries) | (dividing entries) << 16 of the staged ones */
+template <int U>
+__device__ __forceinline__ uint32_t stage_cons(const PlkArgs& a, uint32_t sp, int32_t it, uint32_t from, uint32_t e1,
+                                               uint32_t t0, uint32_t step, float (*dst)[LPAD])
+{
+    uint32_t na = 0, nd = 0;
+    bool ok[U];
+    uint8_t tg[U];
+    float v[U][5];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t i = t0 + u * step;
+        ok[u] = i < LCH && from + i < e1;
+        tg[u] = ok[u] ? a.con.tag[from + i] : (uint8_t)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int r = 0; r < 5; ++r) v[u][r] = ok[u] ? a.con.v[r][from + t0 + u * step] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!ok[u]) continue;
+        const uint32_t x = from + t0 + u * step;
+        if (tg[u] & T_ALIVE) {
+            ++na;
+            nd += (tg[u] & T_DIV) ? 1u : 0u;
+        } else {
+            /* died here this step; a daughter born into its slot consumes here */
+            const long long r = (long long)sp * a.cap + (a.pkey[x] & PI_SLOT_MASK);
+            const bool rb = a.rstamp[r] == it;
+#pragma unroll
+            for (int q = 0; q < 5; ++q) v[u][q] = rb ? a.rcons[r * 5 + q] : 0.0f;
+            na += rb ? 1u : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < 5; ++q) dst[q][t0 + u * step] = v[u][q];
+    }
+    return na | (nd << 16);
+}
+
+__global__ void __launch_bounds__(256) k_plk_long(PlkArgs a, const uint32_t* lng, uint32_t* lg, uint32_t* lm)
+{
+    if (a.st->err) return;
+    __shared__ __align__(16) float sv[2][6][LPAD];
+    __shared__ uint32_t ws[4];
+    __shared__ uint32_t ncnt[2];
+    const int32_t it = (int32_t)(a.st->iteration + 1);
+    const uint32_t nl = a.st->nlong;
+    const uint32_t t = threadIdx.x;
+    const long long ncell = a.g.ncell;
+    /* lane -> staged row: the loss passes (lanes 4-9: DOC DON DOP POC PON POP
+     * <- v0..v5) and the consumption pass (DIC DOC NH4 NO3 PO4 <- c0..c4) */
+    const int lrow = t >= 4 && t < 10 ? (int)t - 4 : -1;
+    const int crow = t == 0 ? 0 : t == 1 ? 2 : t == 2 ? 3 : t == 3 ? 4 : t == 4 ? 1 : -1;
+    for (uint32_t b = blockIdx.x; b < nl; b += gridDim.x) {
+        const long long c = lng[b];
+        const uint32_t s0 = a.seg[c], s1 = a.seg[c + 1];
+        float S = 0.0f;
+        uint32_t e = s0;
+        while (e < s1) {
+            const uint32_t sp = a.pkey[e] >> PI_SLOT_BITS;
+            const uint32_t kend = (sp + 1u) << PI_SLOT_BITS;
+            uint32_t lo = e + 1, hi = s1;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (a.pkey[mid] < kend) lo = mid + 1; else hi = mid;
+            }
+            const uint32_t e1 = lo;
+            if (t == 0) { ncnt[0] = 0; ncnt[1] = 0; }
+            /* grazed / dead entries of the range, in order */
+            uint32_t nG = 0, nM = 0;
+            for (uint32_t base = e; base < e1; base += 1024) {
+                uint32_t fg = 0, fm = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t x = base + 4 * t + j;
+                    const uint8_t tg = x < e1 ? a.con.tag[x] : (uint8_t)0;
+                    fg |= (uint32_t)((tg & T_GRAZ) != 0) << j;
+                    fm |= (uint32_t)((tg & T_MORT) != 0) << j;
+                }
+                uint32_t tot;
+                const uint32_t pre = block_excl((uint32_t)__popc(fg) | ((uint32_t)__popc(fm) << 16), ws, tot);
+                uint32_t og = nG + (pre & 0xFFFFu), om = nM + (pre >> 16);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t x = base + 4 * t + j;
+                    if ((fg >> j) & 1u) lg[e + og++] = x;
+                    if ((fm >> j) & 1u) lm[e + om++] = x;
+                }
+                nG += tot & 0xFFFFu;
+                nM += tot >> 16;
+            }
+            __syncthreads();
+            /* calc_loss! (grazing), then calc_loss! (mortality) */
+            for (int pass = 0; pass < 2; ++pass) {
+                const uint32_t* L = (pass == 0 ? lg : lm) + e;
+                const uint32_t nL = pass == 0 ? nG : nM;
+                if (nL == 0) continue;
+                stage_loss<4>(a, L, 0, nL, t, 256, sv[0]);
+                __syncthreads();
+                uint32_t k = 0;
+                for (uint32_t base = 0; base < nL; base += LCH, ++k) {
+                    if (t >= 64) { if (base + LCH < nL) stage_loss<6>(a, L, base + LCH, nL, t - 64, 192, sv[(k + 1) & 1]); }
+                    else if (lrow >= 0) S = chain_sum(S, sv[k & 1][lrow], min((uint32_t)LCH, nL - base));
+                    __syncthreads();
+                }
+            }
+            /* calc_consume! over every entry of the range */
+            uint32_t na = 0, nd = 0;
+            uint32_t pk = stage_cons<4>(a, sp, it, e, e1, t, 256, sv[0]);
+            na += pk & 0xFFFFu; nd += pk >> 16;
+            __syncthreads();
+            uint32_t k = 0;
+            for (uint32_t base = e; base < e1; base += LCH, ++k) {
+                if (t >= 64) {
+                    if (base + LCH < e1) {
+                        pk = stage_cons<6>(a, sp, it, base + LCH, e1, t - 64, 192, sv[(k + 1) & 1]);
+                        na += pk & 0xFFFFu; nd += pk >> 16;
+                    }
+                } else if (crow >= 0) {
+                    S = chain_sum(S, sv[k & 1][crow], min((uint32_t)LCH, e1 - base));
+                }
+                __syncthreads();
+            }
+            if (na) atomicAdd(&ncnt[0], na);
+            if (nd) atomicAdd(&ncnt[1], nd);
+            __syncthreads();
+            if (t == 0) {   /* plk_diag */
+                float* d = a.diag + (long long)sp * 4 * ncell;
+                const uint32_t NA = ncnt[0], ND = ncnt[1];
+                if (NA) d[0 * ncell + c] = d[0 * ncell + c] + (float)NA;
+                if (nG) d[1 * ncell + c] = d[1 * ncell + c] + (float)nG;
+                if (nM) d[2 * ncell + c] = d[2 * ncell + c] + (float)nM;
+                if (ND) d[3 * ncell + c] = d[3 * ncell + c] + (float)ND;
+            }
+            __syncthreads();
+            e = e1;
+        }
+        constexpr int QOF[10] = {PI_DIC, PI_NH4, PI_NO3, PI_PO4, PI_DOC, PI_DON, PI_DOP, PI_POC, PI_PON, PI_POP};
+#pragma unroll
+        for (int l = 0; l < 10; ++l)
+            if (t == (uint32_t)l) a.plk[QOF[l]][c] = S;
+    }
+}
+
 /* ------------------------------------------------------- host-side launchers */
 void launch_adv(hipStream_t st, int blocks, const AdvArgs& a) { hipLaunchKernelGGL(k_adv, dim3(blocks), dim3(256), 0, st, a); }
 void launch_marks(hipStream_t s, const DevState* st, const uint8_t* sflag, uint32_t* bmA, uint32_t* bmD,
@@ -859,10 +1245,10 @@ void launch_marks(hipStream_t s, const DevState* st, const uint8_t* sflag, uint3
     hipLaunchKernelGGL(k_marks, dim3(grid_for((long long)nsp * nw, 256, 8192)), dim3(256), 0, s, st, sflag, bmA, bmD,
                        nsp, nw, cap);
 }
-void launch_cellcount(hipStream_t s, const DevState* st, long long ncell, const uint32_t* segp, const uint32_t* P,
-                      const uint32_t* cnts, uint32_t* cnt)
+void launch_cellcount(hipStream_t s, DevState* st, long long ncell, const uint32_t* segp, const uint32_t* P,
+                      const uint32_t* cnts, uint32_t* cnt, uint32_t* lng)
 {
-    hipLaunchKernelGGL(k_cellcount, dim3(grid_for(ncell, 256, 8192)), dim3(256), 0, s, st, ncell, segp, P, cnts, cnt);
+    hipLaunchKernelGGL(k_cellcount, dim3(grid_for(ncell, 256, 8192)), dim3(256), 0, s, st, ncell, segp, P, cnts, cnt, lng);
 }
 void launch_place_stay(hipStream_t s, int blocks, const DevState* st, const uint32_t* seg, const uint32_t* segp,
                        const uint32_t* P, const uint32_t* stayf, const uint32_t* newc, uint32_t* pkey, uint32_t* psrc)
@@ -880,14 +1266,16 @@ void launch_mixsort(hipStream_t s, int blocks, DevState* st, const uint32_t* seg
     hipLaunchKernelGGL(k_mixsort, dim3(blocks), dim3(256), 0, s, st, seg, mix, pkey, psrc, big);
 }
 void launch_bigcells(hipStream_t s, int blocks, DevState* st, const uint32_t* big, const uint32_t* seg,
-                     uint32_t* pkey, uint32_t* psrc, uint32_t* skey, uint32_t* ssrc)
+                     const uint32_t* segp, const uint32_t* P, uint32_t* pkey, uint32_t* psrc, uint32_t* skey,
+                     uint32_t* ssrc)
 {
-    hipLaunchKernelGGL(k_bigcells, dim3(blocks), dim3(256), 0, s, st, big, seg, pkey, psrc, skey, ssrc);
+    hipLaunchKernelGGL(k_bigcells, dim3(blocks), dim3(256), 0, s, st, big, seg, segp, P, pkey, psrc, skey, ssrc);
 }
 void launch_cellsum(hipStream_t s, const DevState* st, long long ncell, const uint32_t* seg, const uint32_t* psrc,
-                    float* Chl, float* pop)
+                    float* Chl, float* pop, const uint32_t* lng, int long_blocks)
 {
     hipLaunchKernelGGL(k_cellsum, dim3(grid_for(ncell, 256, 8192)), dim3(256), 0, s, st, ncell, seg, psrc, Chl, pop);
+    hipLaunchKernelGGL(k_cellsum_long, dim3(long_blocks), dim3(256), 0, s, st, lng, seg, psrc, Chl, pop);
 }
 void launch_par(hipStream_t s, int blocks, DevState* st, const DevGrid& g, const float* Chl, const float* PARF_in,
                 float* PARF_out, float* diag_par, float kc, float kw)
@@ -895,4 +1283,9 @@ void launch_par(hipStream_t s, int blocks, DevState* st, const DevGrid& g, const
     hipLaunchKernelGGL(k_par, dim3(blocks), dim3(256), 0, s, st, g, Chl, PARF_in, PARF_out, diag_par, kc, kw);
 }
 void launch_phys(hipStream_t st, int blocks, const PhysArgs& a) { hipLaunchKernelGGL(k_phys, dim3(blocks), dim3(256), 0, st, a); }
-void launch_plk(hipStream_t st, int blocks, const PlkArgs& a) { hipLaunchKernelGGL(k_plk, dim3(blocks), dim3(256), 0, st, a); }
+void launch_plk(hipStream_t st, int blocks, const PlkArgs& a, const uint32_t* lng, uint32_t* lg, uint32_t* lm,
+                int long_blocks)
+{
+    hipLaunchKernelGGL(k_plk, dim3(blocks), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_plk_long, dim3(long_blocks), dim3(256), 0, st, a, lng, lg, lm);
+}

@@ -160,10 +160,10 @@ __device__ __forceinline__ void face_fluxes(const DevGrid& g, const Face& F, boo
 /* ----------------------------------------------------------------- x sweep */
 template <bool LAND, int Q0, int Q1>
 __device__ __forceinline__ void sweep_x_half(const SweepArgs& a, const DevGrid& g, const Face& Fc, bool zero, bool upd,
-                                             uint32_t c, float coef, float dT, const float* C0)
+                                             uint32_t c, uint32_t cf, float coef, float dT, const float* C0)
 {
     float fl[Q1 - Q0];
-    face_fluxes<LAND, false, Q0, Q1>(g, Fc, zero, c, 1, C0, fl);
+    face_fluxes<LAND, false, Q0, Q1>(g, Fc, zero, cf, 1, C0, fl);
 #pragma unroll
     for (int q = Q0; q < Q1; ++q) {
         const float f1 = __shfl_down(fl[q - Q0], 1, 64);
@@ -191,17 +191,22 @@ __global__ void __launch_bounds__(256) k_sweep_x(SweepArgs a)
     const int j = (int)(row % g.Ny) + 1 + g.Hy;
     const int k = (int)(row / g.Ny) + 1 + g.Hz;
     const uint32_t c = (uint32_t)gidx(g, i, j, k);
+    /* Periodic with Nx < Hx: the last face takes the first face's flux, as the
+     * reference's Gc halo copy does; its halo stencil would read halo planes
+     * that the aliased fill leaves holding earlier values. */
+    const bool wrap = g.tx == PI_PERIODIC && g.Nx < g.Hx && fi == g.Nx;
+    const uint32_t cf = wrap ? c - (uint32_t)g.Nx : c;
     const float* C0 = S->tr[S->tcur][0];
     const float dt = a.dt;
     const float A = g.dy * dzF_at(g, k);
-    const Face Fc = face(g, ld(a.vel, c), dt, g.dx, g.ydx, A);
+    const Face Fc = face(g, ld(a.vel, cf), dt, g.dx, g.ydx, A);
     const bool zero = fi == g.Nx && g.tx == PI_BOUNDED;
     const bool upd = valid && lane < 63 && fi < g.Nx;
     const float T1 = __shfl_down(Fc.T, 1, 64);
     const float dT = lmask<LAND>(g, c, c + 1, T1 - Fc.T);
     const float coef = divc(dt, g.vol[k - 1], g.yvol[k - 1]);
-    sweep_x_half<LAND, 0, 7>(a, g, Fc, zero, upd, c, coef, dT, C0);
-    sweep_x_half<LAND, 7, 14>(a, g, Fc, zero, upd, c, coef, dT, C0);
+    sweep_x_half<LAND, 0, 7>(a, g, Fc, zero, upd, c, cf, coef, dT, C0);
+    sweep_x_half<LAND, 7, 14>(a, g, Fc, zero, upd, c, cf, coef, dT, C0);
 }
 
 /* ----------------------------------------------------------------- y sweep */
@@ -209,11 +214,11 @@ __global__ void __launch_bounds__(256) k_sweep_x(SweepArgs a)
 
 template <bool LAND, int Q0, int Q1>
 __device__ __forceinline__ void sweep_y_half(const SweepArgs& a, const DevGrid& g, const Face& Fb, const Face& Fa,
-                                             bool zero, uint32_t c, uint32_t sy, float coef,
+                                             bool zero, uint32_t c, uint32_t cf, uint32_t sy, float coef,
                                              const float* C0, float* fl)
 {
     float f[Q1 - Q0];
-    face_fluxes<LAND, false, Q0, Q1>(g, Fb, zero, c, sy, a.prev, f);
+    face_fluxes<LAND, false, Q0, Q1>(g, Fb, zero, cf, sy, a.prev, f);
     const uint32_t cs = c - sy;    /* the cell south of face c */
     const float dT = lmask<LAND>(g, cs, c, Fb.T - Fa.T);
 #pragma unroll
@@ -235,6 +240,8 @@ __global__ void __launch_bounds__(256) k_sweep_y(SweepArgs a)
     const uint32_t sy = (uint32_t)g.Px;
     const float* C0 = S->tr[S->tcur][0];
     const float dt = a.dt;
+    const bool wrapy = g.ty == PI_PERIODIC && g.Ny < g.Hy;    /* the last face copies the first, as in k_sweep_x */
+    const uint32_t back = (uint32_t)g.Ny * sy;
     for (long long t = xcd_block() * (long long)blockDim.x + threadIdx.x; t < nthr; t += (long long)gridDim.x * blockDim.x) {
         const int i = (int)(t % g.Nx) + 1 + g.Hx;
         const int r = (int)((t / g.Nx) % nrun);
@@ -251,10 +258,11 @@ __global__ void __launch_bounds__(256) k_sweep_y(SweepArgs a)
 #pragma unroll 1
         for (int jj = j0 + 1; jj <= j1; ++jj) {
             c += sy;
-            const Face Fb = face(g, ld(a.vel, c), dt, g.dy, g.ydy, A);
+            const uint32_t cf = wrapy && jj == g.Ny + g.Hy + 1 ? c - back : c;
+            const Face Fb = face(g, ld(a.vel, cf), dt, g.dy, g.ydy, A);
             const bool zero = jj == g.Ny + g.Hy + 1 && g.ty == PI_BOUNDED;
-            sweep_y_half<LAND, 0, 7>(a, g, Fb, Fa, zero, c, sy, coef, C0, fl);
-            sweep_y_half<LAND, 7, 14>(a, g, Fb, Fa, zero, c, sy, coef, C0, fl);
+            sweep_y_half<LAND, 0, 7>(a, g, Fb, Fa, zero, c, cf, sy, coef, C0, fl);
+            sweep_y_half<LAND, 7, 14>(a, g, Fb, Fa, zero, c, cf, sy, coef, C0, fl);
             Fa = Fb;
         }
     }
@@ -302,6 +310,28 @@ __global__ void __launch_bounds__(256) k_fill_line(FillArgs a)
             for (int h = 0; h < H; ++h) AT(N + H + 1 + h) = hi;
         }
         #undef AT
+    }
+}
+
+/* Copies every halo cell of the current tracers into slab f (null = the tracers
+ * written by this step).  Used only where a Periodic direction has N < H: there
+ * the reference's halo fill reads halo planes of the array it fills, which hold
+ * the values of its previous fill (tracer_temp is a copy of the tracers, and
+ * apply_tendency! writes the interior only), so the slab to be filled starts
+ * from the current tracers' halos. */
+__global__ void __launch_bounds__(256) k_halo_copy(FillArgs a)
+{
+    const DevState* S = a.st;
+    if (S->err) return;
+    const DevGrid& g = a.g;
+    const float* src = S->tr[S->tcur][0];
+    float* dst = a.f ? a.f : S->tr[S->tcur ^ 1][0];
+    const long long n = (long long)g.ncell * a.nf;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const long long cell = t % g.ncell;
+        const int i = (int)(cell % g.Px), j = (int)((cell / g.Px) % g.Py), k = (int)(cell / ((long long)g.Px * g.Py));
+        const bool inside = i >= g.Hx && i < g.Hx + g.Nx && j >= g.Hy && j < g.Hy + g.Ny && k >= g.Hz && k < g.Hz + g.Nz;
+        if (!inside) dst[t] = src[t];
     }
 }
 
@@ -518,6 +548,10 @@ void launch_sweep(hipStream_t s, int blocks, const SweepArgs& a)
     }
 }
 void launch_fill(hipStream_t s, int blocks, const FillArgs& a) { hipLaunchKernelGGL(k_fill_line, dim3(blocks), dim3(256), 0, s, a); }
+void launch_halo_copy(hipStream_t s, const FillArgs& a)
+{
+    hipLaunchKernelGGL(k_halo_copy, dim3(grid_for((long long)a.g.ncell * a.nf, 256, 4096)), dim3(256), 0, s, a);
+}
 void launch_final(hipStream_t s, int blocks, const FinalArgs& a)
 {
     const long long n = (long long)a.g.Nx * a.g.Ny * ((a.g.Nz + RUN_Z - 1) / RUN_Z);

@@ -114,6 +114,7 @@ __global__ void k_capcheck(DevState* st, const uint32_t* G, int nsp, int nw)
         if (d > f && !st->err) { st->err = PI_ERR_CAPACITY; st->err_sp = (uint32_t)s; }
     }
     st->nbig = 0;
+    st->nlong = 0;
     st->nsmall = 0;
     st->nmix = 0;
 }

@@ -190,5 +190,6 @@ struct DevState {
     uint32_t nsmall, nmix;             /* queued entries / re-sorted cells of this step */
     uint32_t err, err_sp;
     uint32_t nbig;
+    uint32_t nlong;                    /* cells longer than LONG_SEG this step */
     uint32_t dtot[PI_MAXSP], ftot[PI_MAXSP], doff[PI_MAXSP];
 };

@@ -59,7 +59,8 @@ struct pi_model {
     std::vector<int> diag_ids;
     std::vector<float*> diag_tr;
     uint32_t *cnt = nullptr, *seg = nullptr, *partials = nullptr, *pkey = nullptr, *psrc = nullptr;
-    uint32_t *skey = nullptr, *ssrc = nullptr, *big = nullptr, *bm = nullptr, *G = nullptr;
+    uint32_t *skey = nullptr, *ssrc = nullptr, *big = nullptr, *lng = nullptr, *bm = nullptr, *G = nullptr;
+    int long_blocks = 1;       /* blocks of the one-block-per-long-cell kernels */
     Contrib con{};
     unsigned long long* d_count = nullptr;
     int32_t *dstamp = nullptr, *rstamp = nullptr;
@@ -287,6 +288,8 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     DALLOC(m->pkey, m->ctot * 4); DALLOC(m->psrc, m->ctot * 4);
     DALLOC(m->skey, 2 * m->ctot * 4); DALLOC(m->ssrc, 2 * m->ctot * 4);
     DALLOC(m->big, nc * 4);
+    DALLOC(m->lng, nc * 4);
+    m->long_blocks = (int)std::min<long long>((long long)g.Nx * g.Ny * g.Nz, 512);
     DALLOC(m->bm, 2ll * m->nsp * m->nw * 4);
     DALLOC(m->G, (2ll * m->nsp * m->nw + 1) * 4);
     for (int q = 0; q < 6; ++q) DALLOC(m->con.v[q], m->ctot * 4);
@@ -340,8 +343,6 @@ pi_status pi_model_create(const pi_model_desc* d, pi_model** out)
     if (gd.Nx < 1 || gd.Ny < 1 || gd.Nz < 1 || gd.Hx < 2 || gd.Hy < 2 || gd.Hz < 2 || gd.Hx > 8 || gd.Hy > 8 || gd.Hz > 8)
         return fail(PI_ERR_ARG, "grid: N >= 1 and 2 <= halo <= 8 required");
     if (gd.topo_z != PI_BOUNDED) return fail(PI_ERR_ARG, "grid: z topology must be Bounded");
-    if ((gd.topo_x == PI_PERIODIC && gd.Nx < gd.Hx) || (gd.topo_y == PI_PERIODIC && gd.Ny < gd.Hy))
-        return fail(PI_ERR_UNSUPPORTED, "periodic directions with N < halo are not supported on the GPU path");
     if (d->n_species < 1 || d->n_species > PI_MAXSP) return fail(PI_ERR_ARG, "n_species out of range");
     if (d->capacity < 1 || d->capacity > (long long)PI_SLOT_MASK) return fail(PI_ERR_ARG, "capacity out of range");
     if (d->bgc[PI_B_shared_graz] != 1.0f)
@@ -555,13 +556,13 @@ static pi_status enqueue_step(pi_model* m, float dt)
         sp.dn = (const uint32_t*)((const char*)st + offsetof(DevState, nA));
         launch_scan(s, sp, m->partials, m->P);
     }
-    launch_cellcount(s, st, m->ncell, m->segp, m->P, m->cnts, m->cnt);
+    launch_cellcount(s, st, m->ncell, m->segp, m->P, m->cnts, m->cnt, m->lng);
     launch_scan(s, scan_plain(m->cnt, (uint32_t)m->ncell), m->partials, m->seg);
     launch_place_stay(s, pb, st, m->seg, m->segp, m->P, m->stayf, m->newc, m->pkey, m->psrc);
     launch_place_small(s, pb, st, m->seg, m->small, m->newc, m->cnts, m->pkey, m->psrc, m->mix);
     launch_mixsort(s, cb, st, m->seg, m->mix, m->pkey, m->psrc, m->big);
-    launch_bigcells(s, 256, st, m->big, m->seg, m->pkey, m->psrc, m->skey, m->ssrc);
-    launch_cellsum(s, st, m->ncell, m->seg, m->psrc, m->Chl, m->pop);
+    launch_bigcells(s, 256, st, m->big, m->seg, m->segp, m->P, m->pkey, m->psrc, m->skey, m->ssrc);
+    launch_cellsum(s, st, m->ncell, m->seg, m->psrc, m->Chl, m->pop, m->lng, m->long_blocks);
     tend(m);
     tbegin(m, "par");
     launch_par(s, blocks_for((long long)g.Nx * g.Ny), st, g, m->Chl, m->PARF_in, m->PARF_out, m->diagPAR,
@@ -583,7 +584,7 @@ static pi_status enqueue_step(pi_model* m, float dt)
     ka.rstamp = m->rstamp; ka.rcons = m->rcons; ka.cap = m->cap;
     for (int q = 0; q < PI_NTRACERS; ++q) ka.plk[q] = m->plk[q] ? m->plk[q] : m->plk[PI_DIC];
     tbegin(m, "plk");
-    launch_plk(s, cb, ka);
+    launch_plk(s, cb, ka, m->lng, m->skey, m->ssrc, m->long_blocks);
     tend(m);
     /* tracers */
     SweepArgs sa{};
@@ -596,7 +597,10 @@ static pi_status enqueue_step(pi_model* m, float dt)
     FillArgs fa{};
     fa.g = g; fa.st = st; fa.nf = PI_NTRACERS; fa.dim = 1; fa.lines_full = 0;
     fa.f = m->T1[0];
+    /* Periodic with N < H: the fills read halo planes of the slab they fill */
+    const bool alias = (g.tx == PI_PERIODIC && g.Nx < g.Hx) || (g.ty == PI_PERIODIC && g.Ny < g.Hy);
     tbegin(m, "fill");
+    if (alias) launch_halo_copy(s, fa);
     launch_fill(s, blocks_for((long long)g.Nx * g.Nz * PI_NTRACERS), fa);
     tend(m);
     sa.vel = V1.v; sa.dim = 1;
@@ -626,6 +630,7 @@ static pi_status enqueue_step(pi_model* m, float dt)
     tbegin(m, "fill");
     fa.f = nullptr;
     fa.lines_full = 1;
+    if (alias) launch_halo_copy(s, fa);
     fa.dim = 0; launch_fill(s, blocks_for((long long)g.Py * g.Pz * PI_NTRACERS), fa);
     fa.dim = 1; launch_fill(s, blocks_for((long long)g.Px * g.Pz * PI_NTRACERS), fa);
     fa.dim = 2; launch_fill(s, blocks_for((long long)g.Px * g.Py * PI_NTRACERS), fa);

@@ -126,6 +126,57 @@ def test_parity_subnormal_quotients(pi, orc):
     assert not bad, "\n".join(bad[:20])
 
 
+def test_parity_0d_box(pi, orc):
+    """The reference's 0-D box (test/example_0D_test.jl:3): one cell, Periodic x
+    and y with N = 1 < H = 2, so every halo fill aliases."""
+    g = pi.RectilinearGrid(size=(1, 1, 1), x=(0, 32), y=(0, 32), z=(0, -32))
+    spec = pi.ModelSpec(g, max_individuals=4096, seed=8,
+                        phyto=pi.phyto_setup({"dvid_P": [2e-3], "mort_reg": [1.0], "mort_P": [1e-3]}, [1024], 1))
+    z = np.zeros((1, 1, 1), np.float32)
+    gm, om = _drive(pi, orc, spec, (z, z, np.zeros((1, 1, 2), np.float32)), 60.0, 10)
+    bad = compare_models(gm._native, om, 1)
+    assert not bad, "\n".join(bad[:20])
+
+
+@pytest.mark.parametrize("size,n0,steps,grz", [((1, 1, 1), (3000, 2600), 6, 3e-8), ((1, 1, 24), (9000, 7000), 5, 2.5e-7)])
+def test_parity_long_cells(pi, orc, size, n0, steps, grz):
+    """Cells holding thousands of individuals (a 0-D box, a 1-D column), two
+    species, frequent grazing, mortality and division: the one-block-per-cell
+    paths (ordered sums over several LDS chunks, lists of grazed and dead
+    entries, re-sorting by merging the arrivals into the stayers, and the full
+    sort when the arrivals exceed the LDS sort) against the oracle."""
+    g = pi.RectilinearGrid(size=size, x=(0, 32), y=(0, 32), z=(0, -32 * size[2]))
+    spec = pi.ModelSpec(g, max_individuals=3 * max(n0), seed=31,
+                        phyto=pi.phyto_setup({"dvid_P": [5e-3, 4e-3], "mort_reg": [1.0, 1.0], "mort_P": [2e-3, 3e-3],
+                                              "grz_P": [grz, grz / 2], "dvid_reg": [1.5, 1.5]}, list(n0), 2),
+                        bgc_params={"κhP": 1.0, "κvP": 1e-2, "κh": 10.0, "κv": 1e-3})
+    nz = size[2]
+    z = np.zeros((1, 1, nz), np.float32)
+    gm, om = _drive(pi, orc, spec, (z, z, np.zeros((1, 1, nz + 1), np.float32)), 60.0, steps)
+    for s_ in range(2):
+        assert gm._native.count_active(s_) == om.count_active(s_)
+    bad = compare_models(gm._native, om, 2)
+    assert not bad, "\n".join(bad[:20])
+
+
+def test_parity_aliased_halos_with_flow(pi, orc):
+    """A column with Nx = 1 < Hx = 2 and Ny = 2 < Hy = 3, both Periodic, under
+    non-zero u, v, w: the last face of each aliased direction takes the first
+    face's flux and the halo fills read the halos they fill, as in the reference
+    (halo_regions.jl:1-35, multi_dim_adv.jl:111-135)."""
+    g = pi.RectilinearGrid(size=(1, 2, 12), x=(0, 1e3), y=(0, 2e3), z=(0, -60), halo=(2, 3, 2))
+    spec = pi.ModelSpec(g, max_individuals=3000, seed=23,
+                        phyto=pi.phyto_setup({"dvid_P": [2e-3], "mort_reg": [1.0]}, [1200], 1),
+                        bgc_params={"κhP": 2.0, "κvP": 1e-3, "κh": 10.0, "κv": 1e-3})
+    rng = np.random.default_rng(9)
+    u = (rng.standard_normal((1, 2, 12)) * 2).astype(np.float32)
+    v = (rng.standard_normal((1, 2, 12)) * 2).astype(np.float32)
+    w = (rng.standard_normal((1, 2, 13)) * 1e-2).astype(np.float32)
+    gm, om = _drive(pi, orc, spec, (u, v, w), 60.0, 12)
+    bad = compare_models(gm._native, om, 1)
+    assert not bad, "\n".join(bad[:20])
+
+
 def test_graph_replay_matches_eager(pi):
     """pi_run replays one captured step graph for steady steps; its result must be
     the eager step-by-step result (pinned to the oracle by the other tests) bit
