@@ -186,6 +186,32 @@ pi_status pi_count_active(pi_model* m, int32_t sp, int64_t* n);   /* dot(ac, ac)
 pi_status pi_enable_timing(pi_model* m, int32_t on);
 pi_status pi_kernel_times(pi_model* m, int32_t* n, const char** names, float* ms, int32_t* counts, int32_t cap);
 
+/* Device memory.  pi_footprint: the bytes a model with this descriptor would
+ * allocate (a dry run of pi_model_create that touches no device);
+ * pi_model_footprint: the bytes a model holds.  No counterpart in the
+ * reference (Julia allocates per array); used to admit ensemble members by
+ * footprint. */
+pi_status pi_footprint(const pi_model_desc* desc, int64_t* bytes);
+pi_status pi_model_footprint(const pi_model* m, int64_t* bytes);
+
+/* Batches: many models stepped by the same launches.  No counterpart in the
+ * reference, which steps one PlanktonModel per update!/TimeStep! call
+ * (Simulation/update.jl:28, Model/time_step.jl:12); a batch steps each member
+ * exactly as pi_run would, all members together: at every kernel of the step
+ * one launch covers the blocks of all members.  A member keeps its buffers and
+ * its API (staging, downloads) while in a batch; its calls then go to the
+ * batch's stream.  pi_batch_run(b, dt, n) = n steps of every member (members
+ * in an error state are skipped); its errors are reported per member by
+ * pi_sync / the downloads.  Members may be added and removed between runs. */
+typedef struct pi_batch pi_batch;
+pi_status pi_batch_create(pi_batch** out);
+pi_status pi_batch_add(pi_batch* b, pi_model* m);
+pi_status pi_batch_remove(pi_batch* b, pi_model* m);
+pi_status pi_batch_run(pi_batch* b, float dt, int64_t n);
+pi_status pi_batch_sync(pi_batch* b);
+pi_status pi_batch_size(const pi_batch* b, int32_t* n);
+pi_status pi_batch_destroy(pi_batch* b);
+
 /* Measurement aid, no counterpart in the reference: copies `bytes` (a multiple
  * of 16) between two fresh device buffers with the kernel k_probe_copy on the
  * current device and waits for it.  Hardware counters of that dispatch scale

@@ -17,6 +17,7 @@ from .model import (Architecture, HIPArch, GPU, isfunctional, phyto_setup, Plank
 from .simulation import (PlanktonSimulation, PlanktonDiagnostics, update, default_PARF,
                          default_temperature)
 from ._native import CapacityError, NativeError, load_library, lib_path
+from .ensemble import Ensemble
 from . import workloads
 
 update_ = update          # `update!`
@@ -26,4 +27,4 @@ __all__ = ["RectilinearGrid", "Periodic", "Bounded", "QuotaMode", "PlanktonModel
            "phyto_setup", "PlanktonSimulation", "PlanktonDiagnostics", "update", "TimeStep",
            "bgc_params_default", "phyt_params_default", "update_bgc_params", "update_phyt_params",
            "default_tracer_init", "default_PARF", "default_temperature", "CapacityError",
-           "TRACER_NAMES", "isfunctional"]
+           "TRACER_NAMES", "isfunctional", "Ensemble"]

@@ -56,6 +56,17 @@ def load_library():
                                     C.POINTER(C.c_float), C.POINTER(C.c_int32), C.c_int32]
     lib.pi_probe_copy.restype = C.c_int
     lib.pi_probe_copy.argtypes = [C.c_int64]
+    lib.pi_footprint.restype = C.c_int
+    lib.pi_footprint.argtypes = [C.POINTER(_abi.ModelDesc), C.POINTER(C.c_int64)]
+    lib.pi_model_footprint.restype = C.c_int
+    lib.pi_model_footprint.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    for f, a in (("create", [C.POINTER(C.c_void_p)]), ("add", [C.c_void_p, C.c_void_p]),
+                 ("remove", [C.c_void_p, C.c_void_p]), ("run", [C.c_void_p, C.c_float, C.c_int64]),
+                 ("sync", [C.c_void_p]), ("size", [C.c_void_p, C.POINTER(C.c_int32)]),
+                 ("destroy", [C.c_void_p])):
+        fn = getattr(lib, "pi_batch_" + f)
+        fn.restype = C.c_int
+        fn.argtypes = a
     _lib = lib
     return lib
 
@@ -189,6 +200,17 @@ class Handle:
         return v.value
 
 
+def footprint(grid, nsp, cap, params, bgc, seed=0, diag_tracers=()):
+    """Device bytes a model of this shape allocates (pi_footprint: a dry run of
+    pi_model_create, no device needed)."""
+    lib = load_library()
+    desc, keep = _abi.make_desc(grid, nsp, cap, params, bgc, seed, diag_tracers)
+    v = C.c_int64(0)
+    check(lib, "pi_", lib.pi_footprint(C.byref(desc), C.byref(v)))
+    del keep
+    return v.value
+
+
 class NativeModel(Handle):
     """A model living on the GPU (libpi_hip.so)."""
 
@@ -200,6 +222,17 @@ class NativeModel(Handle):
         check(lib, "pi_", lib.pi_model_create(C.byref(desc), C.byref(raw)))
         del keep
         super().__init__(lib, "pi_", raw.value, grid, nsp, cap, diag_tracers)
+        self.batch = None
+
+    def footprint(self):
+        v = C.c_int64(0)
+        self._call("model_footprint", C.byref(v))
+        return v.value
+
+    def close(self):
+        if self.batch is not None:
+            self.batch.remove(self)
+        super().close()
 
     def run(self, dt, n):
         self._call("run", C.c_float(dt), C.c_int64(n))
@@ -224,3 +257,49 @@ class NativeModel(Handle):
         cnt = (C.c_int32 * cap)()
         self._call("kernel_times", C.byref(n), names, ms, cnt, cap)
         return {names[i].decode(): (ms[i], cnt[i]) for i in range(n.value)}
+
+
+class NativeBatch:
+    """Models stepped together by batched launches (pi_batch_*): one launch per
+    kernel of the step covers every member."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        check(self.lib, "pi_", self.lib.pi_init(device))
+        raw = C.c_void_p()
+        check(self.lib, "pi_", self.lib.pi_batch_create(C.byref(raw)))
+        self.raw = raw
+        self.members = []
+
+    def add(self, model):
+        check(self.lib, "pi_", self.lib.pi_batch_add(self.raw, model.raw))
+        model.batch = self
+        self.members.append(model)
+
+    def remove(self, model):
+        check(self.lib, "pi_", self.lib.pi_batch_remove(self.raw, model.raw))
+        model.batch = None
+        self.members.remove(model)
+
+    def run(self, dt, n=1):
+        check(self.lib, "pi_", self.lib.pi_batch_run(self.raw, C.c_float(dt), C.c_int64(n)))
+
+    def sync(self):
+        check(self.lib, "pi_", self.lib.pi_batch_sync(self.raw))
+
+    def __len__(self):
+        return len(self.members)
+
+    def close(self):
+        if self.raw is not None and self.raw.value:
+            for m in list(self.members):
+                m.batch = None
+            self.members = []
+            self.lib.pi_batch_destroy(self.raw)
+            self.raw = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

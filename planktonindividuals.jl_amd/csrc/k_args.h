@@ -58,6 +58,8 @@ struct PlkArgs {
     const int32_t* rstamp;
     const float* rcons;
     long long cap;
+    const uint32_t* lng;      /* the long cells of this step (k_plk_long) */
+    uint32_t *lg, *lm;        /* scratch: ordered lists of grazed / dead entries */
 };
 
 /* Tracer sets are slabs of PI_NTRACERS padded fields: field q at slab + q·ncell. */
@@ -102,46 +104,92 @@ struct ScanIn {
     long long cap;
 };
 
-/* k_particles.hip */
-void launch_adv(hipStream_t st, int blocks, const AdvArgs& a);
-void launch_marks(hipStream_t s, const DevState* st, const uint8_t* sflag, uint32_t* bmA, uint32_t* bmD,
-                  int nsp, int nw, long long cap);
-void launch_cellcount(hipStream_t s, DevState* st, long long ncell, const uint32_t* segp, const uint32_t* P,
-                      const uint32_t* cnts, uint32_t* cnt, uint32_t* lng);
-void launch_place_stay(hipStream_t s, int blocks, const DevState* st, const uint32_t* seg, const uint32_t* segp,
-                       const uint32_t* P, const uint32_t* stayf, const uint32_t* newc, uint32_t* pkey, uint32_t* psrc);
-void launch_place_small(hipStream_t s, int blocks, DevState* st, const uint32_t* seg, const uint32_t* small,
-                        const uint32_t* newc, uint32_t* cnts, uint32_t* pkey, uint32_t* psrc, uint32_t* mix);
-void launch_mixsort(hipStream_t s, int blocks, DevState* st, const uint32_t* seg, const uint32_t* mix,
-                    uint32_t* pkey, uint32_t* psrc, uint32_t* big);
-void launch_bigcells(hipStream_t s, int blocks, DevState* st, const uint32_t* big, const uint32_t* seg,
-                     const uint32_t* segp, const uint32_t* P, uint32_t* pkey, uint32_t* psrc, uint32_t* skey,
-                     uint32_t* ssrc);
-void launch_cellsum(hipStream_t s, const DevState* st, long long ncell, const uint32_t* seg, const uint32_t* psrc,
-                    float* Chl, float* pop, const uint32_t* lng, int long_blocks);
-void launch_par(hipStream_t s, int blocks, DevState* st, const DevGrid& g, const float* Chl, const float* PARF_in,
-                float* PARF_out, float* diag_par, float kc, float kw);
-void launch_phys(hipStream_t st, int blocks, const PhysArgs& a);
-void launch_plk(hipStream_t st, int blocks, const PlkArgs& a, const uint32_t* lng, uint32_t* lg, uint32_t* lm,
-                int long_blocks);
-/* k_tracers.hip */
-void launch_sweep(hipStream_t s, int blocks, const SweepArgs& a);
-void launch_fill(hipStream_t s, int blocks, const FillArgs& a);
-void launch_halo_copy(hipStream_t s, const FillArgs& a);
-void launch_final(hipStream_t s, int blocks, const FinalArgs& a);
-/* k_util.hip */
-void launch_scan(hipStream_t s, const ScanIn& in, uint32_t* partials, uint32_t* out);
-ScanIn scan_plain(const uint32_t* a, uint32_t n);
-ScanIn scan_bits(const uint32_t* D, const uint32_t* A, int nsp, int nw, long long cap);
-void launch_capcheck(hipStream_t s, DevState* st, const uint32_t* G, int nsp, int nw);
-void launch_finish(hipStream_t s, DevState* st, const uint32_t* seg, long long ncell, int nsp);
+struct MarksArgs {
+    const DevState* st;
+    const uint8_t* sflag;
+    uint32_t *bmA, *bmD;
+    int nsp, nw;
+    long long cap;
+};
+
+/* binning: cell counts, placement, re-sorting, per-cell Chl and pop */
+struct BinArgs {
+    DevState* st;
+    long long ncell;
+    const uint32_t* seg;      /* this step's segments */
+    const uint32_t* segp;     /* the previous step's segments */
+    const uint32_t* P;        /* stayer prefix over the previous order */
+    const uint32_t* stayf;
+    const uint32_t* newc;
+    const uint32_t* small;
+    uint32_t* cnts;
+    uint32_t* cnt;
+    uint32_t* lng;
+    uint32_t *pkey, *psrc;
+    uint32_t* mix;
+    uint32_t* big;
+    uint32_t *skey, *ssrc;
+    float *Chl, *pop;
+};
+
+struct ParArgs {
+    DevGrid g;
+    DevState* st;
+    const float* Chl;
+    const float* PARF_in;
+    float *PARF_out, *diag_par;
+    float kc, kw;
+};
+
+struct ScanArgs {
+    ScanIn in;
+    uint32_t* partials;
+    uint32_t* out;
+    uint32_t nb;              /* tiles */
+};
+
+struct CapArgs { DevState* st; const uint32_t* G; int nsp, nw; };
+struct FinishArgs { DevState* st; const uint32_t* seg; long long ncell; int nsp; };
+struct CopyArgs { uint32_t* dst; const uint32_t* src; long long n; const DevState* st; };
+struct VelhalfArgs { const float* a[3]; const float* b[3]; float* h[3]; long long n; const DevState* st; };
+struct AccumArgs { float* d[PI_NTRACERS]; int q[PI_NTRACERS]; int nq; long long n; const DevState* st; };
+
+/* The kernels of one step.  Each has a single-model entry point (arguments by
+ * value) and a batched one (a table of arguments, one per model, and a table
+ * of (model, block, block count) per block). */
+enum KId {
+    K_VELHALF, K_MARKS, K_SCAN_P, K_SCAN_T, K_SCAN_F, K_CAPCHECK, K_ADV, K_CELLCOUNT, K_PLACE_STAY,
+    K_PLACE_SMALL, K_MIXSORT, K_BIGCELLS, K_CELLSUM, K_CELLSUM_LONG, K_PAR, K_PHYS, K_PLK, K_PLK_LONG,
+    K_SWEEPX0, K_SWEEPX1, K_SWEEPY0, K_SWEEPY1, K_HALOCOPY, K_FILL, K_FINAL0, K_FINAL1, K_ACCUM, K_COPY,
+    K_FINISH, K_NKID
+};
+struct KInfo {
+    const void* one;          /* __global__ void NAME(ARGS) */
+    const void* multi;        /* __global__ void NAME_m(const ARGS*, const uint4*) */
+    uint32_t argsz;
+    uint32_t threads;
+    bool stride;              /* the body loops over its work with a block stride (any block count works) */
+    const char* name;
+};
+#define PI_REG(K, ID, NAME, ARGS, THREADS, STRIDE) \
+    K[ID] = KInfo{(const void*)&NAME, (const void*)&NAME##_m, (uint32_t)sizeof(ARGS), THREADS, STRIDE, #NAME}
+void reg_particles(KInfo* k);
+void reg_tracers(KInfo* k);
+void reg_util(KInfo* k);
+
+/* grid sizes of the kernels whose work is one item per thread */
+long long sweep_x_blocks(const DevGrid& g);
+long long sweep_y_items(const DevGrid& g);
+long long final_items(const DevGrid& g);
+uint32_t scan_tiles(uint32_t n);
+
+/* k_util.hip: kernels outside the step (uploads, downloads, measurement) */
 void launch_zero(hipStream_t s, uint32_t* p, long long n, const DevState* st);
-void launch_velhalf(hipStream_t s, const float* a, const float* b, float* h, long long n, const DevState* st);
-void launch_accum(hipStream_t s, float* d, const DevState* st, int q, long long n);
 void launch_store_to_table(hipStream_t s, DevState* st, const SlotTable& tab, long long cap, long long maxn);
 void launch_rebuild(hipStream_t s, DevState* st, const SlotTable& tab, long long cap, int nsp, uint8_t* sflag);
-void launch_copy_u32(hipStream_t s, uint32_t* dst, const uint32_t* src, long long n, const DevState* st);
 void launch_probe_copy(hipStream_t s, void* dst, const void* src, long long n16);
 void launch_table_field(hipStream_t s, const DevState* st, const SlotTable& tab, int sp, long long cap, int field,
                         const float* params, const DevGrid& g, void* out);
 void launch_count(hipStream_t s, const DevState* st, int sp, unsigned long long* out, long long maxn);
+ScanIn scan_plain(const uint32_t* a, uint32_t n);
+ScanIn scan_bits(const uint32_t* D, const uint32_t* A, int nsp, int nw, long long cap);

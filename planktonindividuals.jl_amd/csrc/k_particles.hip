@@ -202,7 +202,7 @@ __device__ __forceinline__ void move_entry(const AdvArgs& a, const GStore& S, ui
     move(a, g, S.x[e], S.y[e], S.z[e], r0, r1, r2, zprev, x, y, z);
 }
 
-__global__ void __launch_bounds__(256) k_adv(AdvArgs a)
+__device__ __forceinline__ void k_adv_body(const AdvArgs& a, Blk K)
 {
     DevState* st = a.st;
     if (st->err) return;
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(256) k_adv(AdvArgs a)
     const uint32_t nA = st->nA;
     const long long iteration = st->iteration + 1;
     const uint32_t nloop = (n + 63u) & ~63u;          /* whole waves stay in the loop (wave_append) */
-    for (uint32_t e = xcd_block() * blockDim.x + threadIdx.x; e < nloop; e += gridDim.x * blockDim.x) {
+    for (uint32_t e = K.b * blockDim.x + threadIdx.x; e < nloop; e += K.n * blockDim.x) {
         const uint8_t fl = e < n ? S.flags[e] : (uint8_t)0;
         float x = 0.0f, y = 0.0f, z = 0.0f;
         if (fl & F_AC) move_entry(a, S, e, iteration, fl, x, y, z);
@@ -219,18 +219,19 @@ __global__ void __launch_bounds__(256) k_adv(AdvArgs a)
         if (!(fl & F_AC) && e < nA) a.stayf[e] = 0u;
     }
 }
+PI_KERNEL_XCD(k_adv, AdvArgs, 256)
 
 /* Survivors of this step's losses and the dividers, as per-species slot bitmaps
  * (plankton_update.jl:5-24: ac after grazing!/mortality!, dvid .*= ac).  They
  * depend only on the decisions taken at the end of the previous step, kept as
  * one flag byte per slot, so the capacity check runs before anything is
  * modified.  One lane per 32-slot word. */
-__global__ void __launch_bounds__(256) k_marks(const DevState* st, const uint8_t* sflag, uint32_t* bmA, uint32_t* bmD,
+__device__ __forceinline__ void k_marks_impl(Blk K, const DevState* st, const uint8_t* sflag, uint32_t* bmA, uint32_t* bmD,
                                                int nsp, int nw, long long cap)
 {
     if (st->err) return;
     const long long nwords = (long long)nsp * nw;
-    for (long long w = blockIdx.x * (long long)blockDim.x + threadIdx.x; w < nwords; w += (long long)gridDim.x * blockDim.x) {
+    for (long long w = K.b * (long long)blockDim.x + threadIdx.x; w < nwords; w += (long long)K.n * blockDim.x) {
         const long long sp = w / nw, wd = w - sp * nw;
         const long long s0 = wd * 32, s1 = s0 + 32 < cap ? s0 + 32 : cap;
         const uint8_t* f = sflag + sp * cap;
@@ -245,17 +246,19 @@ __global__ void __launch_bounds__(256) k_marks(const DevState* st, const uint8_t
         bmD[w] = D;
     }
 }
+__device__ __forceinline__ void k_marks_body(const MarksArgs& x, Blk K) { k_marks_impl(K, x.st, x.sflag, x.bmA, x.bmD, x.nsp, x.nw, x.cap); }
+PI_KERNEL(k_marks, MarksArgs, 256)
 
 /* per cell: stayers (their count from the stayer prefix P over the cell's
  * previous segment segp) + queued arrivals */
-__global__ void __launch_bounds__(256) k_cellcount(DevState* st, long long ncell, const uint32_t* segp,
+__device__ __forceinline__ void k_cellcount_impl(Blk K, DevState* st, long long ncell, const uint32_t* segp,
                                                    const uint32_t* P, const uint32_t* cnts, uint32_t* cnt,
                                                    uint32_t* lng)
 {
     if (st->err) return;
     const bool have = st->nA > 0;
     const long long nloop = (ncell + 63) & ~63ll;     /* whole waves: wave_append */
-    for (long long c = xcd_block() * (long long)blockDim.x + threadIdx.x; c < nloop; c += (long long)gridDim.x * blockDim.x) {
+    for (long long c = K.b * (long long)blockDim.x + threadIdx.x; c < nloop; c += (long long)K.n * blockDim.x) {
         bool isl = false;
         if (c < ncell) {
             const uint32_t n = (have ? P[segp[c + 1]] - P[segp[c]] : 0u) + cnts[c];
@@ -266,17 +269,19 @@ __global__ void __launch_bounds__(256) k_cellcount(DevState* st, long long ncell
         if (isl) lng[li] = (uint32_t)c;
     }
 }
+__device__ __forceinline__ void k_cellcount_body(const BinArgs& x, Blk K) { k_cellcount_impl(K, x.st, x.ncell, x.segp, x.P, x.cnts, x.cnt, x.lng); }
+PI_KERNEL_XCD(k_cellcount, BinArgs, 256)
 
 /* stayers: the stayers of a cell keep their (species, slot) order; position =
  * start of the cell + rank among the cell's stayers */
-__global__ void __launch_bounds__(256) k_place_stay(const DevState* st, const uint32_t* seg, const uint32_t* segp,
+__device__ __forceinline__ void k_place_stay_impl(Blk K, const DevState* st, const uint32_t* seg, const uint32_t* segp,
                                                     const uint32_t* P, const uint32_t* stayf, const uint32_t* newc,
                                                     uint32_t* pkey, uint32_t* psrc)
 {
     if (st->err) return;
     const GStore S = gstore(st->store[st->scur]);
     const uint32_t nA = st->nA;
-    for (uint32_t e = xcd_block() * blockDim.x + threadIdx.x; e < nA; e += gridDim.x * blockDim.x) {
+    for (uint32_t e = K.b * blockDim.x + threadIdx.x; e < nA; e += K.n * blockDim.x) {
         if (!stayf[e]) continue;
         const uint32_t c = newc[e];
         const uint32_t pos = seg[c] + (P[e] - P[segp[c]]);
@@ -284,10 +289,12 @@ __global__ void __launch_bounds__(256) k_place_stay(const DevState* st, const ui
         psrc[pos] = e;
     }
 }
+__device__ __forceinline__ void k_place_stay_body(const BinArgs& x, Blk K) { k_place_stay_impl(K, x.st, x.seg, x.segp, x.P, x.stayf, x.newc, x.pkey, x.psrc); }
+PI_KERNEL_XCD(k_place_stay, BinArgs, 256)
 
 /* queued individuals: after the stayers of their cell, in any order; the
  * cell is listed for re-sorting once (by the last one placed) */
-__global__ void __launch_bounds__(256) k_place_small(DevState* st, const uint32_t* seg, const uint32_t* small,
+__device__ __forceinline__ void k_place_small_impl(Blk K, DevState* st, const uint32_t* seg, const uint32_t* small,
                                                      const uint32_t* newc, uint32_t* cnts, uint32_t* pkey, uint32_t* psrc,
                                                      uint32_t* mix)
 {
@@ -295,7 +302,7 @@ __global__ void __launch_bounds__(256) k_place_small(DevState* st, const uint32_
     const GStore S = gstore(st->store[st->scur]);
     const uint32_t ns = st->nsmall;
     const uint32_t nloop = (ns + 63u) & ~63u;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nloop; t += gridDim.x * blockDim.x) {
+    for (uint32_t t = K.b * blockDim.x + threadIdx.x; t < nloop; t += K.n * blockDim.x) {
         bool last = false;
         uint32_t c = 0;
         if (t < ns) {
@@ -311,15 +318,17 @@ __global__ void __launch_bounds__(256) k_place_small(DevState* st, const uint32_
         if (last) mix[mi] = c;
     }
 }
+__device__ __forceinline__ void k_place_small_body(const BinArgs& x, Blk K) { k_place_small_impl(K, x.st, x.seg, x.small, x.newc, x.cnts, x.pkey, x.psrc, x.mix); }
+PI_KERNEL(k_place_small, BinArgs, 256)
 
 /* re-sort the listed cells by key (species-major, then slot): insertion sort
  * for short segments, the rest go to k_bigcells */
-__global__ void __launch_bounds__(256) k_mixsort(DevState* st, const uint32_t* seg, const uint32_t* mix,
+__device__ __forceinline__ void k_mixsort_impl(Blk K, DevState* st, const uint32_t* seg, const uint32_t* mix,
                                                  uint32_t* pkey, uint32_t* psrc, uint32_t* big)
 {
     if (st->err) return;
     const uint32_t nm = st->nmix;
-    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < nm; m += gridDim.x * blockDim.x) {
+    for (uint32_t m = K.b * blockDim.x + threadIdx.x; m < nm; m += K.n * blockDim.x) {
         const uint32_t c = mix[m];
         const uint32_t s0 = seg[c], n = seg[c + 1] - s0;
         if (n > SMALL_SEG) { big[atomicAdd(&st->nbig, 1u)] = c; continue; }
@@ -336,14 +345,16 @@ __global__ void __launch_bounds__(256) k_mixsort(DevState* st, const uint32_t* s
         }
     }
 }
+__device__ __forceinline__ void k_mixsort_body(const BinArgs& x, Blk K) { k_mixsort_impl(K, x.st, x.seg, x.mix, x.pkey, x.psrc, x.big); }
+PI_KERNEL(k_mixsort, BinArgs, 256)
 
 /* acc_chl! / acc_counts! (Individuals/utils.jl:33-52) as an ordered segmented sum */
-__global__ void __launch_bounds__(256) k_cellsum(const DevState* st, long long ncell, const uint32_t* seg,
+__device__ __forceinline__ void k_cellsum_impl(Blk K, const DevState* st, long long ncell, const uint32_t* seg,
                                                  const uint32_t* psrc, float* Chl, float* pop)
 {
     if (st->err) return;
     const PI_GLOBAL float* chl_store = gp(st->store[st->scur].Chl);
-    for (long long c = xcd_block() * (long long)blockDim.x + threadIdx.x; c < ncell; c += (long long)gridDim.x * blockDim.x) {
+    for (long long c = K.b * (long long)blockDim.x + threadIdx.x; c < ncell; c += (long long)K.n * blockDim.x) {
         const uint32_t s0 = seg[c], n = seg[c + 1] - s0;
         if (n > LONG_SEG) continue;                  /* k_cellsum_long */
         float sum = 0.0f;
@@ -352,6 +363,8 @@ __global__ void __launch_bounds__(256) k_cellsum(const DevState* st, long long n
         Chl[c] = sum;
     }
 }
+__device__ __forceinline__ void k_cellsum_body(const BinArgs& x, Blk K) { k_cellsum_impl(K, x.st, x.ncell, x.seg, x.psrc, x.Chl, x.pop); }
+PI_KERNEL_XCD(k_cellsum, BinArgs, 256)
 
 /* ordered sum of m staged values (row, 16-byte aligned) onto s: blocks of 32
  * values, the LDS reads of the next block issued before the 32 additions of
@@ -406,7 +419,7 @@ __device__ __forceinline__ void stage_chl(const PI_GLOBAL float* chl, const uint
 
 /* k_cellsum for the listed long cells: the Chl values of a chunk are gathered
  * into LDS by waves 1-3 while wave 0 adds up the previous chunk */
-__global__ void __launch_bounds__(256) k_cellsum_long(const DevState* st, const uint32_t* lng, const uint32_t* seg,
+__device__ __forceinline__ void k_cellsum_long_impl(Blk K, const DevState* st, const uint32_t* lng, const uint32_t* seg,
                                                       const uint32_t* psrc, float* Chl, float* pop)
 {
     if (st->err) return;
@@ -414,7 +427,7 @@ __global__ void __launch_bounds__(256) k_cellsum_long(const DevState* st, const 
     const PI_GLOBAL float* chl_store = gp(st->store[st->scur].Chl);
     const uint32_t nl = st->nlong;
     const uint32_t t = threadIdx.x;
-    for (uint32_t b = blockIdx.x; b < nl; b += gridDim.x) {
+    for (uint32_t b = K.b; b < nl; b += K.n) {
         const uint32_t c = lng[b];
         const uint32_t s0 = seg[c], s1 = seg[c + 1];
         float sum = 0.0f;
@@ -433,6 +446,8 @@ __global__ void __launch_bounds__(256) k_cellsum_long(const DevState* st, const 
         if (t == 0) { pop[c] = (float)(s1 - s0); Chl[c] = sum; }
     }
 }
+__device__ __forceinline__ void k_cellsum_long_body(const BinArgs& x, Blk K) { k_cellsum_long_impl(K, x.st, x.lng, x.seg, x.psrc, x.Chl, x.pop); }
+PI_KERNEL(k_cellsum_long, BinArgs, 256)
 
 /* Long re-sorted segments, one block per cell.  The cell holds its stayers,
  * already in key order, then its arrivals in any order.  With at most MAXARR
@@ -459,7 +474,7 @@ __device__ __forceinline__ uint32_t count_below(const uint32_t* k, uint32_t n, u
     }
     return lo;
 }
-__global__ void __launch_bounds__(256) k_bigcells(DevState* st, const uint32_t* big, const uint32_t* seg,
+__device__ __forceinline__ void k_bigcells_impl(Blk K, DevState* st, const uint32_t* big, const uint32_t* seg,
                                                   const uint32_t* segp, const uint32_t* P,
                                                   uint32_t* pkey, uint32_t* psrc, uint32_t* skey, uint32_t* ssrc)
 {
@@ -467,7 +482,7 @@ __global__ void __launch_bounds__(256) k_bigcells(DevState* st, const uint32_t* 
     __shared__ uint32_t AK[MAXARR], AS[MAXARR];
     const uint32_t nb = st->nbig;
     const bool have = st->nA > 0;
-    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    for (uint32_t b = K.b; b < nb; b += K.n) {
         const uint32_t c = big[b];
         const uint32_t s0 = seg[c], n = seg[c + 1] - s0;
         const uint32_t nst = have ? P[segp[c + 1]] - P[segp[c]] : 0u;   /* stayers, as k_cellcount */
@@ -566,15 +581,17 @@ __global__ void __launch_bounds__(256) k_bigcells(DevState* st, const uint32_t* 
         __syncthreads();
     }
 }
+__device__ __forceinline__ void k_bigcells_body(const BinArgs& x, Blk K) { k_bigcells_impl(K, x.st, x.big, x.seg, x.segp, x.P, x.pkey, x.psrc, x.skey, x.ssrc); }
+PI_KERNEL(k_bigcells, BinArgs, 256)
 
 /* calc_par_kernel! looped over levels (Individuals/utils.jl:55-68, time_step.jl:63-67) */
-__global__ void __launch_bounds__(256) k_par(DevState* st, DevGrid g, const float* Chl, const float* PARF_in,
+__device__ __forceinline__ void k_par_impl(Blk K, DevState* st, DevGrid g, const float* Chl, const float* PARF_in,
                                              float* PARF_out, float* diag_par, float kc, float kw)
 {
     if (st->err) return;
     PI_GLOBAL float* par = gp(st->par[1 - st->pcur]);
     const int ncol = g.Nx * g.Ny;
-    for (int col = blockIdx.x * blockDim.x + threadIdx.x; col < ncol; col += gridDim.x * blockDim.x) {
+    for (int col = K.b * blockDim.x + threadIdx.x; col < ncol; col += K.n * blockDim.x) {
         const int i = col % g.Nx + 1, j = col / g.Nx + 1;
         const int ii = i + g.Hx, jj = j + g.Hy;
         float P = PARF_in[col];
@@ -592,6 +609,8 @@ __global__ void __launch_bounds__(256) k_par(DevState* st, DevGrid g, const floa
         PARF_out[col] = P;
     }
 }
+__device__ __forceinline__ void k_par_body(const ParArgs& x, Blk K) { k_par_impl(K, x.st, x.g, x.Chl, x.PARF_in, x.PARF_out, x.diag_par, x.kc, x.kw); }
+PI_KERNEL(k_par, ParArgs, 256)
 
 /* ------------------------------------------------------------- physiology */
 struct PState { float x, y, z, Sz, Bm, Cq, Nq, Pq, Chl, gen, age; };
@@ -734,7 +753,7 @@ __device__ __forceinline__ uint32_t valid_mask(const PhysArgs& a, uint32_t w)
     return rem >= 32 ? 0xFFFFFFFFu : ((1u << rem) - 1u);
 }
 
-__global__ void __launch_bounds__(256) k_phys(PhysArgs a)
+__device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
 {
     const DevState* st = a.st;
     if (st->err) return;
@@ -755,7 +774,7 @@ __global__ void __launch_bounds__(256) k_phys(PhysArgs a)
     for (int i = threadIdx.x; i < a.nsp; i += blockDim.x) s_doff[i] = st->doff[i];
     __syncthreads();
     const uint32_t hhh = (uint32_t)gidx(g, g.Hx, g.Hy, g.Hz);
-    for (uint32_t e = xcd_block() * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    for (uint32_t e = K.b * blockDim.x + threadIdx.x; e < n; e += K.n * blockDim.x) {
         const uint32_t key = a.pkey[e], src = a.psrc[e];
         const int sp = (int)(key >> PI_SLOT_BITS);
         const uint32_t slot = key & PI_SLOT_MASK;
@@ -906,6 +925,7 @@ __global__ void __launch_bounds__(256) k_phys(PhysArgs a)
         a.con.tag[e] = tag;
     }
 }
+PI_KERNEL_XCD(k_phys, PhysArgs, 256)
 
 /* Ordered per-cell sums: calc_loss! (graz), calc_loss! (mort), calc_consume! per
  * species in slot order (plankton_update.jl:5-29), and the per-species
@@ -939,13 +959,13 @@ __device__ __forceinline__ void plk_diag(const PlkArgs& a, long long c, uint32_t
 
 #define PLK_FAST 6   /* segments up to this length are loaded into registers at once */
 
-__global__ void __launch_bounds__(256) k_plk(PlkArgs a)
+__device__ __forceinline__ void k_plk_body(const PlkArgs& a, Blk K)
 {
     if (a.st->err) return;
     const int32_t it = (int32_t)(a.st->iteration + 1);
     const DevGrid& g = a.g;
-    for (long long c = xcd_block() * (long long)blockDim.x + threadIdx.x; c < g.ncell;
-         c += (long long)gridDim.x * blockDim.x) {
+    for (long long c = K.b * (long long)blockDim.x + threadIdx.x; c < g.ncell;
+         c += (long long)K.n * blockDim.x) {
         const uint32_t s0 = a.seg[c], s1 = a.seg[c + 1], n = s1 - s0;
         if (n > LONG_SEG) continue;                  /* k_plk_long */
         PlkSums S{};
@@ -1041,6 +1061,7 @@ __global__ void __launch_bounds__(256) k_plk(PlkArgs a)
         a.plk[PI_POC][c] = S.POC; a.plk[PI_PON][c] = S.PON; a.plk[PI_POP][c] = S.POP;
     }
 }
+PI_KERNEL_XCD(k_plk, PlkArgs, 256)
 
 /* exclusive prefix over the 256 threads of a block of v; *tot = the block total */
 __device__ __forceinline__ uint32_t block_excl(uint32_t v, uint32_t* ws, uint32_t& tot)
@@ -1132,7 +1153,7 @@ __device__ __forceinline__ uint32_t stage_cons(const PlkArgs& a, uint32_t sp, in
     return na | (nd << 16);
 }
 
-__global__ void __launch_bounds__(256) k_plk_long(PlkArgs a, const uint32_t* lng, uint32_t* lg, uint32_t* lm)
+__device__ __forceinline__ void k_plk_long_body(const PlkArgs& a, Blk K)
 {
     if (a.st->err) return;
     __shared__ __align__(16) float sv[2][6][LPAD];
@@ -1146,8 +1167,8 @@ __global__ void __launch_bounds__(256) k_plk_long(PlkArgs a, const uint32_t* lng
      * <- v0..v5) and the consumption pass (DIC DOC NH4 NO3 PO4 <- c0..c4) */
     const int lrow = t >= 4 && t < 10 ? (int)t - 4 : -1;
     const int crow = t == 0 ? 0 : t == 1 ? 2 : t == 2 ? 3 : t == 3 ? 4 : t == 4 ? 1 : -1;
-    for (uint32_t b = blockIdx.x; b < nl; b += gridDim.x) {
-        const long long c = lng[b];
+    for (uint32_t b = K.b; b < nl; b += K.n) {
+        const long long c = a.lng[b];
         const uint32_t s0 = a.seg[c], s1 = a.seg[c + 1];
         float S = 0.0f;
         uint32_t e = s0;
@@ -1178,8 +1199,8 @@ __global__ void __launch_bounds__(256) k_plk_long(PlkArgs a, const uint32_t* lng
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const uint32_t x = base + 4 * t + j;
-                    if ((fg >> j) & 1u) lg[e + og++] = x;
-                    if ((fm >> j) & 1u) lm[e + om++] = x;
+                    if ((fg >> j) & 1u) a.lg[e + og++] = x;
+                    if ((fm >> j) & 1u) a.lm[e + om++] = x;
                 }
                 nG += tot & 0xFFFFu;
                 nM += tot >> 16;
@@ -1187,7 +1208,7 @@ __global__ void __launch_bounds__(256) k_plk_long(PlkArgs a, const uint32_t* lng
             __syncthreads();
             /* calc_loss! (grazing), then calc_loss! (mortality) */
             for (int pass = 0; pass < 2; ++pass) {
-                const uint32_t* L = (pass == 0 ? lg : lm) + e;
+                const uint32_t* L = (pass == 0 ? a.lg : a.lm) + e;
                 const uint32_t nL = pass == 0 ? nG : nM;
                 if (nL == 0) continue;
                 stage_loss<4>(a, L, 0, nL, t, 256, sv[0]);
@@ -1236,56 +1257,22 @@ __global__ void __launch_bounds__(256) k_plk_long(PlkArgs a, const uint32_t* lng
             if (t == (uint32_t)l) a.plk[QOF[l]][c] = S;
     }
 }
+PI_KERNEL(k_plk_long, PlkArgs, 256)
 
-/* ------------------------------------------------------- host-side launchers */
-void launch_adv(hipStream_t st, int blocks, const AdvArgs& a) { hipLaunchKernelGGL(k_adv, dim3(blocks), dim3(256), 0, st, a); }
-void launch_marks(hipStream_t s, const DevState* st, const uint8_t* sflag, uint32_t* bmA, uint32_t* bmD,
-                  int nsp, int nw, long long cap)
+/* ------------------------------------------------------------- registry */
+void reg_particles(KInfo* k)
 {
-    hipLaunchKernelGGL(k_marks, dim3(grid_for((long long)nsp * nw, 256, 8192)), dim3(256), 0, s, st, sflag, bmA, bmD,
-                       nsp, nw, cap);
-}
-void launch_cellcount(hipStream_t s, DevState* st, long long ncell, const uint32_t* segp, const uint32_t* P,
-                      const uint32_t* cnts, uint32_t* cnt, uint32_t* lng)
-{
-    hipLaunchKernelGGL(k_cellcount, dim3(grid_for(ncell, 256, 8192)), dim3(256), 0, s, st, ncell, segp, P, cnts, cnt, lng);
-}
-void launch_place_stay(hipStream_t s, int blocks, const DevState* st, const uint32_t* seg, const uint32_t* segp,
-                       const uint32_t* P, const uint32_t* stayf, const uint32_t* newc, uint32_t* pkey, uint32_t* psrc)
-{
-    hipLaunchKernelGGL(k_place_stay, dim3(blocks), dim3(256), 0, s, st, seg, segp, P, stayf, newc, pkey, psrc);
-}
-void launch_place_small(hipStream_t s, int blocks, DevState* st, const uint32_t* seg, const uint32_t* small,
-                        const uint32_t* newc, uint32_t* cnts, uint32_t* pkey, uint32_t* psrc, uint32_t* mix)
-{
-    hipLaunchKernelGGL(k_place_small, dim3(blocks), dim3(256), 0, s, st, seg, small, newc, cnts, pkey, psrc, mix);
-}
-void launch_mixsort(hipStream_t s, int blocks, DevState* st, const uint32_t* seg, const uint32_t* mix,
-                    uint32_t* pkey, uint32_t* psrc, uint32_t* big)
-{
-    hipLaunchKernelGGL(k_mixsort, dim3(blocks), dim3(256), 0, s, st, seg, mix, pkey, psrc, big);
-}
-void launch_bigcells(hipStream_t s, int blocks, DevState* st, const uint32_t* big, const uint32_t* seg,
-                     const uint32_t* segp, const uint32_t* P, uint32_t* pkey, uint32_t* psrc, uint32_t* skey,
-                     uint32_t* ssrc)
-{
-    hipLaunchKernelGGL(k_bigcells, dim3(blocks), dim3(256), 0, s, st, big, seg, segp, P, pkey, psrc, skey, ssrc);
-}
-void launch_cellsum(hipStream_t s, const DevState* st, long long ncell, const uint32_t* seg, const uint32_t* psrc,
-                    float* Chl, float* pop, const uint32_t* lng, int long_blocks)
-{
-    hipLaunchKernelGGL(k_cellsum, dim3(grid_for(ncell, 256, 8192)), dim3(256), 0, s, st, ncell, seg, psrc, Chl, pop);
-    hipLaunchKernelGGL(k_cellsum_long, dim3(long_blocks), dim3(256), 0, s, st, lng, seg, psrc, Chl, pop);
-}
-void launch_par(hipStream_t s, int blocks, DevState* st, const DevGrid& g, const float* Chl, const float* PARF_in,
-                float* PARF_out, float* diag_par, float kc, float kw)
-{
-    hipLaunchKernelGGL(k_par, dim3(blocks), dim3(256), 0, s, st, g, Chl, PARF_in, PARF_out, diag_par, kc, kw);
-}
-void launch_phys(hipStream_t st, int blocks, const PhysArgs& a) { hipLaunchKernelGGL(k_phys, dim3(blocks), dim3(256), 0, st, a); }
-void launch_plk(hipStream_t st, int blocks, const PlkArgs& a, const uint32_t* lng, uint32_t* lg, uint32_t* lm,
-                int long_blocks)
-{
-    hipLaunchKernelGGL(k_plk, dim3(blocks), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(k_plk_long, dim3(long_blocks), dim3(256), 0, st, a, lng, lg, lm);
+    PI_REG(k, K_ADV, k_adv, AdvArgs, 256, true);
+    PI_REG(k, K_MARKS, k_marks, MarksArgs, 256, true);
+    PI_REG(k, K_CELLCOUNT, k_cellcount, BinArgs, 256, true);
+    PI_REG(k, K_PLACE_STAY, k_place_stay, BinArgs, 256, true);
+    PI_REG(k, K_PLACE_SMALL, k_place_small, BinArgs, 256, true);
+    PI_REG(k, K_MIXSORT, k_mixsort, BinArgs, 256, true);
+    PI_REG(k, K_BIGCELLS, k_bigcells, BinArgs, 256, true);
+    PI_REG(k, K_CELLSUM, k_cellsum, BinArgs, 256, true);
+    PI_REG(k, K_CELLSUM_LONG, k_cellsum_long, BinArgs, 256, true);
+    PI_REG(k, K_PAR, k_par, ParArgs, 256, true);
+    PI_REG(k, K_PHYS, k_phys, PhysArgs, 256, true);
+    PI_REG(k, K_PLK, k_plk, PlkArgs, 256, true);
+    PI_REG(k, K_PLK_LONG, k_plk_long, PlkArgs, 256, true);
 }

@@ -173,14 +173,14 @@ __device__ __forceinline__ void sweep_x_half(const SweepArgs& a, const DevGrid& 
 }
 
 template <bool LAND>
-__global__ void __launch_bounds__(256) k_sweep_x(SweepArgs a)
+__device__ __forceinline__ void k_sweep_x_body(const SweepArgs& a, Blk K)
 {
     const DevState* S = a.st;
     if (S->err) return;
     const DevGrid& g = a.g;
     const long long nrow = (long long)g.Ny * g.Nz;
     const long long nface = nrow * (g.Nx + 1);
-    const long long gt = xcd_block() * (long long)blockDim.x + threadIdx.x;
+    const long long gt = K.b * (long long)blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const long long F = (gt >> 6) * 63 + lane;
     const bool valid = F < nface;
@@ -208,6 +208,8 @@ __global__ void __launch_bounds__(256) k_sweep_x(SweepArgs a)
     sweep_x_half<LAND, 0, 7>(a, g, Fc, zero, upd, c, cf, coef, dT, C0);
     sweep_x_half<LAND, 7, 14>(a, g, Fc, zero, upd, c, cf, coef, dT, C0);
 }
+PI_KERNEL2(k_sweep_x0, k_sweep_x_body<false>, SweepArgs, 256)
+PI_KERNEL2(k_sweep_x1, k_sweep_x_body<true>, SweepArgs, 256)
 
 /* ----------------------------------------------------------------- y sweep */
 #define RUN_Y 16
@@ -230,7 +232,7 @@ __device__ __forceinline__ void sweep_y_half(const SweepArgs& a, const DevGrid& 
 }
 
 template <bool LAND>
-__global__ void __launch_bounds__(256) k_sweep_y(SweepArgs a)
+__device__ __forceinline__ void k_sweep_y_body(const SweepArgs& a, Blk K)
 {
     const DevState* S = a.st;
     if (S->err) return;
@@ -242,7 +244,7 @@ __global__ void __launch_bounds__(256) k_sweep_y(SweepArgs a)
     const float dt = a.dt;
     const bool wrapy = g.ty == PI_PERIODIC && g.Ny < g.Hy;    /* the last face copies the first, as in k_sweep_x */
     const uint32_t back = (uint32_t)g.Ny * sy;
-    for (long long t = xcd_block() * (long long)blockDim.x + threadIdx.x; t < nthr; t += (long long)gridDim.x * blockDim.x) {
+    for (long long t = K.b * (long long)blockDim.x + threadIdx.x; t < nthr; t += (long long)K.n * blockDim.x) {
         const int i = (int)(t % g.Nx) + 1 + g.Hx;
         const int r = (int)((t / g.Nx) % nrun);
         const int k = (int)(t / ((long long)g.Nx * nrun)) + 1 + g.Hz;
@@ -267,11 +269,13 @@ __global__ void __launch_bounds__(256) k_sweep_y(SweepArgs a)
         }
     }
 }
+PI_KERNEL2(k_sweep_y0, k_sweep_y_body<false>, SweepArgs, 256)
+PI_KERNEL2(k_sweep_y1, k_sweep_y_body<true>, SweepArgs, 256)
 
 /* fill_halo_{west,east} / {south,north} / {top,bottom} of one dimension, on the
  * lines given by (o1, o2) ranges.  lines_full = 1 covers every padded line (the
  * reference fills whole planes); 0 covers interior lines only. */
-__global__ void __launch_bounds__(256) k_fill_line(FillArgs a)
+__device__ __forceinline__ void k_fill_line_body(const FillArgs& a, Blk K)
 {
     const DevState* S = a.st;
     if (S->err) return;
@@ -286,7 +290,7 @@ __global__ void __launch_bounds__(256) k_fill_line(FillArgs a)
     const long long n1 = a.lines_full ? P1 : N1, n2 = a.lines_full ? P2 : N2;
     const long long nlines = n1 * n2 * a.nf;
     float* slab = a.f ? a.f : S->tr[S->tcur ^ 1][0];
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nlines; t += (long long)gridDim.x * blockDim.x) {
+    for (long long t = K.b * (long long)blockDim.x + threadIdx.x; t < nlines; t += (long long)K.n * blockDim.x) {
         const int q = (int)(t / (n1 * n2));
         const long long l = t % (n1 * n2);
         const int o1 = (int)(l % n1) + 1 + (a.lines_full ? 0 : H1);
@@ -312,6 +316,7 @@ __global__ void __launch_bounds__(256) k_fill_line(FillArgs a)
         #undef AT
     }
 }
+PI_KERNEL(k_fill_line, FillArgs, 256)
 
 /* Copies every halo cell of the current tracers into slab f (null = the tracers
  * written by this step).  Used only where a Periodic direction has N < H: there
@@ -319,7 +324,7 @@ __global__ void __launch_bounds__(256) k_fill_line(FillArgs a)
  * the values of its previous fill (tracer_temp is a copy of the tracers, and
  * apply_tendency! writes the interior only), so the slab to be filled starts
  * from the current tracers' halos. */
-__global__ void __launch_bounds__(256) k_halo_copy(FillArgs a)
+__device__ __forceinline__ void k_halo_copy_body(const FillArgs& a, Blk K)
 {
     const DevState* S = a.st;
     if (S->err) return;
@@ -327,13 +332,14 @@ __global__ void __launch_bounds__(256) k_halo_copy(FillArgs a)
     const float* src = S->tr[S->tcur][0];
     float* dst = a.f ? a.f : S->tr[S->tcur ^ 1][0];
     const long long n = (long long)g.ncell * a.nf;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    for (long long t = K.b * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)K.n * blockDim.x) {
         const long long cell = t % g.ncell;
         const int i = (int)(cell % g.Px), j = (int)((cell / g.Px) % g.Py), k = (int)(cell / ((long long)g.Px * g.Py));
         const bool inside = i >= g.Hx && i < g.Hx + g.Nx && j >= g.Hy && j < g.Hy + g.Ny && k >= g.Hz && k < g.Hz + g.Nz;
         if (!inside) dst[t] = src[t];
     }
 }
+PI_KERNEL(k_halo_copy, FillArgs, 256)
 
 /* ---------------------------------------------- z sweep + the rest of the update */
 #define RUN_Z 16
@@ -466,7 +472,7 @@ __device__ __forceinline__ void final_half(const FinalArgs& a, const DevGrid& g,
 }
 
 template <bool LAND>
-__global__ void __launch_bounds__(256) k_final(FinalArgs a)
+__device__ __forceinline__ void k_final_body(const FinalArgs& a, Blk K)
 {
     const DevState* S = a.st;
     if (S->err) return;
@@ -479,7 +485,7 @@ __global__ void __launch_bounds__(256) k_final(FinalArgs a)
     const float dt = a.dt;
     const float Az = g.dx * g.dy;
     const float kvAz = a.kv * Az;
-    for (long long t = xcd_block() * (long long)blockDim.x + threadIdx.x; t < nthr; t += (long long)gridDim.x * blockDim.x) {
+    for (long long t = K.b * (long long)blockDim.x + threadIdx.x; t < nthr; t += (long long)K.n * blockDim.x) {
         const int i = (int)(t % g.Nx) + 1 + g.Hx;
         const int j = (int)((t / g.Nx) % g.Ny) + 1 + g.Hy;
         const int r = (int)(t / ((long long)g.Nx * g.Ny));
@@ -529,33 +535,27 @@ __global__ void __launch_bounds__(256) k_final(FinalArgs a)
         }
     }
 }
+PI_KERNEL2(k_final0, k_final_body<false>, FinalArgs, 256)
+PI_KERNEL2(k_final1, k_final_body<true>, FinalArgs, 256)
 
-/* ------------------------------------------------------------- launchers */
-void launch_sweep(hipStream_t s, int blocks, const SweepArgs& a)
+/* ------------------------------------------------------------- registry */
+long long sweep_x_blocks(const DevGrid& g)
 {
-    const bool land = a.g.has_land != 0;
-    if (a.dim == 0) {
-        const long long nface = (long long)a.g.Ny * a.g.Nz * (a.g.Nx + 1);
-        const long long nwave = (nface + 62) / 63;
-        const dim3 grid((unsigned)((nwave + 3) / 4));
-        if (land) hipLaunchKernelGGL(k_sweep_x<true>, grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL(k_sweep_x<false>, grid, dim3(256), 0, s, a);
-    } else {
-        const long long n = (long long)a.g.Nx * ((a.g.Ny + RUN_Y - 1) / RUN_Y) * a.g.Nz;
-        const dim3 grid(grid_for(n, 256, 8192));
-        if (land) hipLaunchKernelGGL(k_sweep_y<true>, grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL(k_sweep_y<false>, grid, dim3(256), 0, s, a);
-    }
+    const long long nface = (long long)g.Ny * g.Nz * (g.Nx + 1);
+    const long long nwave = (nface + 62) / 63;      /* 63 faces' updates per wave */
+    return (nwave + 3) / 4;
 }
-void launch_fill(hipStream_t s, int blocks, const FillArgs& a) { hipLaunchKernelGGL(k_fill_line, dim3(blocks), dim3(256), 0, s, a); }
-void launch_halo_copy(hipStream_t s, const FillArgs& a)
+long long sweep_y_items(const DevGrid& g) { return (long long)g.Nx * ((g.Ny + RUN_Y - 1) / RUN_Y) * g.Nz; }
+long long final_items(const DevGrid& g) { return (long long)g.Nx * g.Ny * ((g.Nz + RUN_Z - 1) / RUN_Z); }
+
+void reg_tracers(KInfo* k)
 {
-    hipLaunchKernelGGL(k_halo_copy, dim3(grid_for((long long)a.g.ncell * a.nf, 256, 4096)), dim3(256), 0, s, a);
-}
-void launch_final(hipStream_t s, int blocks, const FinalArgs& a)
-{
-    const long long n = (long long)a.g.Nx * a.g.Ny * ((a.g.Nz + RUN_Z - 1) / RUN_Z);
-    const dim3 grid(grid_for(n, 256, 8192));
-    if (a.g.has_land) hipLaunchKernelGGL(k_final<true>, grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(k_final<false>, grid, dim3(256), 0, s, a);
+    PI_REG(k, K_SWEEPX0, k_sweep_x0, SweepArgs, 256, false);
+    PI_REG(k, K_SWEEPX1, k_sweep_x1, SweepArgs, 256, false);
+    PI_REG(k, K_SWEEPY0, k_sweep_y0, SweepArgs, 256, true);
+    PI_REG(k, K_SWEEPY1, k_sweep_y1, SweepArgs, 256, true);
+    PI_REG(k, K_FILL, k_fill_line, FillArgs, 256, true);
+    PI_REG(k, K_HALOCOPY, k_halo_copy, FillArgs, 256, true);
+    PI_REG(k, K_FINAL0, k_final0, FinalArgs, 256, true);
+    PI_REG(k, K_FINAL1, k_final1, FinalArgs, 256, true);
 }

@@ -48,23 +48,25 @@ __device__ __forceinline__ uint32_t block_exscan(uint32_t v, uint32_t* total)
     return off + x - v;
 }
 
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_partials(ScanIn s, uint32_t* partials)
+__device__ __forceinline__ void k_scan_partials_impl(Blk K, ScanIn s, uint32_t* partials)
 {
     const uint32_t n = scan_len(s);
-    if (blockIdx.x * SCAN_TILE >= n) {               /* beyond a device-side length */
-        if (threadIdx.x == 0) partials[blockIdx.x] = 0;
+    if (K.b * SCAN_TILE >= n) {               /* beyond a device-side length */
+        if (threadIdx.x == 0) partials[K.b] = 0;
         return;
     }
-    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    const uint32_t base = K.b * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t sum = 0;
     for (int k = 0; k < SCAN_ITEMS; ++k) sum += scan_load(s, base + k, n);
     uint32_t tot;
     block_exscan(sum, &tot);
-    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+    if (threadIdx.x == 0) partials[K.b] = tot;
 }
+__device__ __forceinline__ void k_scan_partials_body(const ScanArgs& x, Blk K) { k_scan_partials_impl(K, x.in, x.partials); }
+PI_KERNEL(k_scan_partials, ScanArgs, 256)
 
 /* single block: exclusive scan of the per-tile partials, total -> out[n] */
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_top(uint32_t* partials, uint32_t nb, uint32_t* out, uint32_t n)
+__device__ __forceinline__ void k_scan_top_impl(Blk K, uint32_t* partials, uint32_t nb, uint32_t* out, uint32_t n)
 {
     uint32_t carry = 0;
     for (uint32_t base = 0; base < nb; base += SCAN_TILE) {
@@ -82,25 +84,29 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_top(uint32_t* partials, u
     }
     if (threadIdx.x == 0) out[n] = carry;
 }
+__device__ __forceinline__ void k_scan_top_body(const ScanArgs& x, Blk K) { k_scan_top_impl(K, x.partials, x.nb, x.out, x.in.n); }
+PI_KERNEL(k_scan_top, ScanArgs, 256)
 
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_final(ScanIn s, const uint32_t* partials, uint32_t* out)
+__device__ __forceinline__ void k_scan_final_impl(Blk K, ScanIn s, const uint32_t* partials, uint32_t* out)
 {
     const uint32_t n = scan_len(s);
-    if (blockIdx.x * SCAN_TILE > n) return;
-    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    if (K.b * SCAN_TILE > n) return;
+    const uint32_t base = K.b * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS], sum = 0;
     for (int k = 0; k < SCAN_ITEMS; ++k) { v[k] = scan_load(s, base + k, n); sum += v[k]; }
     uint32_t tot;
-    uint32_t pre = block_exscan(sum, &tot) + partials[blockIdx.x];
+    uint32_t pre = block_exscan(sum, &tot) + partials[K.b];
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         if (base + k <= n && base + k < s.n) out[base + k] = pre;   /* out[n] = total when n < s.n */
         pre += v[k];
     }
 }
+__device__ __forceinline__ void k_scan_final_body(const ScanArgs& x, Blk K) { k_scan_final_impl(K, x.in, x.partials, x.out); }
+PI_KERNEL(k_scan_final, ScanArgs, 256)
 
 /* totals, daughter offsets and the capacity check (plankton_update.jl:18-22:
  * dot(dvid, ac) > length(findall(!ac)) throws). */
-__global__ void k_capcheck(DevState* st, const uint32_t* G, int nsp, int nw)
+__device__ __forceinline__ void k_capcheck_impl(Blk K, DevState* st, const uint32_t* G, int nsp, int nw)
 {
     if (threadIdx.x != 0 || st->err) return;
     uint32_t off = 0;
@@ -118,9 +124,11 @@ __global__ void k_capcheck(DevState* st, const uint32_t* G, int nsp, int nw)
     st->nsmall = 0;
     st->nmix = 0;
 }
+__device__ __forceinline__ void k_capcheck_body(const CapArgs& x, Blk K) { k_capcheck_impl(K, x.st, x.G, x.nsp, x.nw); }
+PI_KERNEL(k_capcheck, CapArgs, 64)
 
 /* end of TimeStep!: vel₀ <- vel₁, par₀ <- par, new tracers and store become current */
-__global__ void k_finish(DevState* st, const uint32_t* seg, long long ncell, int nsp)
+__device__ __forceinline__ void k_finish_impl(Blk K, DevState* st, const uint32_t* seg, long long ncell, int nsp)
 {
     if (threadIdx.x != 0 || st->err) return;
     uint32_t born = 0;
@@ -132,13 +140,17 @@ __global__ void k_finish(DevState* st, const uint32_t* seg, long long ncell, int
     st->pcur ^= 1;
     st->iteration += 1;
 }
+__device__ __forceinline__ void k_finish_body(const FinishArgs& x, Blk K) { k_finish_impl(K, x.st, x.seg, x.ncell, x.nsp); }
+PI_KERNEL(k_finish, FinishArgs, 64)
 
-__global__ void k_copy_u32(uint32_t* dst, const uint32_t* src, long long n, const DevState* st)
+__device__ __forceinline__ void k_copy_u32_impl(Blk K, uint32_t* dst, const uint32_t* src, long long n, const DevState* st)
 {
     if (st->err) return;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    for (long long i = K.b * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)K.n * blockDim.x)
         dst[i] = src[i];
 }
+__device__ __forceinline__ void k_copy_u32_body(const CopyArgs& x, Blk K) { k_copy_u32_impl(K, x.dst, x.src, x.n, x.st); }
+PI_KERNEL(k_copy_u32, CopyArgs, 256)
 
 /* a plain streaming copy (16 bytes per lane per iteration) */
 __global__ void __launch_bounds__(256) k_probe_copy(uint4* dst, const uint4* src, long long n)
@@ -154,22 +166,33 @@ __global__ void k_zero_u32(uint32_t* p, long long n, const DevState* st)
         p[i] = 0u;
 }
 
-/* vel½ = (vel₀ + vel₁) * 0.5f0 (time_step.jl:17-19) */
-__global__ void k_velhalf(const float* a, const float* b, float* h, long long n, const DevState* st)
+/* vel½ = (vel₀ + vel₁) * 0.5f0 (time_step.jl:17-19), u, v and w */
+__device__ __forceinline__ void k_velhalf_body(const VelhalfArgs& x, Blk K)
 {
-    if (st->err) return;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        h[i] = (a[i] + b[i]) * 0.5f;
+    if (x.st->err) return;
+    for (int c = 0; c < 3; ++c) {
+        const float* a = x.a[c];
+        const float* b = x.b[c];
+        float* h = x.h[c];
+        for (long long i = K.b * (long long)blockDim.x + threadIdx.x; i < x.n; i += (long long)K.n * blockDim.x)
+            h[i] = (a[i] + b[i]) * 0.5f;
+    }
 }
+PI_KERNEL(k_velhalf, VelhalfArgs, 256)
 
 /* diags.tracer[key] .+= tracers[key].data over padded arrays (time_step.jl:168-172) */
-__global__ void k_accum(float* d, const DevState* st, int q, long long n)
+__device__ __forceinline__ void k_accum_body(const AccumArgs& x, Blk K)
 {
+    const DevState* st = x.st;
     if (st->err) return;
-    const float* src = st->tr[st->tcur ^ 1][q];   /* the tracers written by this step */
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        d[i] = d[i] + src[i];
+    for (int j = 0; j < x.nq; ++j) {
+        const float* src = st->tr[st->tcur ^ 1][x.q[j]];   /* the tracers written by this step */
+        float* d = x.d[j];
+        for (long long i = K.b * (long long)blockDim.x + threadIdx.x; i < x.n; i += (long long)K.n * blockDim.x)
+            d[i] = d[i] + src[i];
+    }
 }
+PI_KERNEL(k_accum, AccumArgs, 256)
 
 /* ---------------------------------------------------- store <-> slot table */
 /* live store entries -> their slot-table rows (marks them active there) */
@@ -270,15 +293,22 @@ __global__ void k_count(const DevState* st, int sp, unsigned long long* out)
     if (c) atomicAdd(out, c);
 }
 
-/* ------------------------------------------------------------- launchers */
-void launch_scan(hipStream_t s, const ScanIn& in, uint32_t* partials, uint32_t* out)
+/* ------------------------------------------------------------- registry */
+uint32_t scan_tiles(uint32_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
+void reg_util(KInfo* k)
 {
-    const uint32_t nb = (in.n + SCAN_TILE - 1) / SCAN_TILE;
-    if (nb == 0) return;   /* callers never scan empty arrays */
-    hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(SCAN_THREADS), 0, s, in, partials);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, s, partials, nb, out, in.n);
-    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(SCAN_THREADS), 0, s, in, partials, out);
+    PI_REG(k, K_VELHALF, k_velhalf, VelhalfArgs, 256, true);
+    PI_REG(k, K_SCAN_P, k_scan_partials, ScanArgs, SCAN_THREADS, false);
+    PI_REG(k, K_SCAN_T, k_scan_top, ScanArgs, SCAN_THREADS, false);
+    PI_REG(k, K_SCAN_F, k_scan_final, ScanArgs, SCAN_THREADS, false);
+    PI_REG(k, K_CAPCHECK, k_capcheck, CapArgs, 64, false);
+    PI_REG(k, K_ACCUM, k_accum, AccumArgs, 256, true);
+    PI_REG(k, K_COPY, k_copy_u32, CopyArgs, 256, true);
+    PI_REG(k, K_FINISH, k_finish, FinishArgs, 64, false);
 }
+
+/* ------------------------------------------------------------- launchers */
 ScanIn scan_plain(const uint32_t* a, uint32_t n)
 {
     ScanIn s{}; s.a = a; s.n = n; s.bits = 0; return s;
@@ -287,25 +317,9 @@ ScanIn scan_bits(const uint32_t* D, const uint32_t* A, int nsp, int nw, long lon
 {
     ScanIn s{}; s.bits = 1; s.D = D; s.A = A; s.nsp = nsp; s.nw = nw; s.cap = cap; s.n = 2u * nsp * nw; return s;
 }
-void launch_capcheck(hipStream_t s, DevState* st, const uint32_t* G, int nsp, int nw)
-{
-    hipLaunchKernelGGL(k_capcheck, dim3(1), dim3(64), 0, s, st, G, nsp, nw);
-}
-void launch_finish(hipStream_t s, DevState* st, const uint32_t* seg, long long ncell, int nsp)
-{
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, s, st, seg, ncell, nsp);
-}
 void launch_zero(hipStream_t s, uint32_t* p, long long n, const DevState* st)
 {
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, p, n, st);
-}
-void launch_velhalf(hipStream_t s, const float* a, const float* b, float* h, long long n, const DevState* st)
-{
-    hipLaunchKernelGGL(k_velhalf, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, a, b, h, n, st);
-}
-void launch_accum(hipStream_t s, float* d, const DevState* st, int q, long long n)
-{
-    hipLaunchKernelGGL(k_accum, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, d, st, q, n);
 }
 void launch_store_to_table(hipStream_t s, DevState* st, const SlotTable& tab, long long cap, long long maxn)
 {
@@ -318,10 +332,6 @@ void launch_rebuild(hipStream_t s, DevState* st, const SlotTable& tab, long long
 void launch_probe_copy(hipStream_t s, void* dst, const void* src, long long n16)
 {
     hipLaunchKernelGGL(k_probe_copy, dim3(grid_for(n16, 256, 8192)), dim3(256), 0, s, (uint4*)dst, (const uint4*)src, n16);
-}
-void launch_copy_u32(hipStream_t s, uint32_t* dst, const uint32_t* src, long long n, const DevState* st)
-{
-    hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, dst, src, n, st);
 }
 void launch_table_field(hipStream_t s, const DevState* st, const SlotTable& tab, int sp, long long cap, int field,
                         const float* params, const DevGrid& g, void* out)

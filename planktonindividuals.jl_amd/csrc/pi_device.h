@@ -167,6 +167,28 @@ __device__ __forceinline__ uint32_t xcd_block()
 }
 
 /* launch helpers */
+/* The block a kernel body runs as: its index among the blocks working on one
+ * model, and their count.  A single-model launch passes the XCD-renumbered
+ * block index and the grid size; a batched launch looks its block up in a
+ * table of (model, index, count), so one launch steps many models. */
+struct Blk { uint32_t b, n; };
+
+/* The two entry points of a kernel body BODY(const ARGS&, Blk): NAME for one
+ * model (arguments by value; BIDX = blockIdx.x, or xcd_block() for kernels
+ * whose neighbouring blocks share cache lines) and NAME##_m for a batch
+ * (argument table + block table). */
+#define PI_KERNEL3(NAME, BODY, ARGS, THREADS, BIDX)                                                  \
+    __global__ void __launch_bounds__(THREADS) NAME(ARGS a) { BODY(a, Blk{BIDX, gridDim.x}); }      \
+    __global__ void __launch_bounds__(THREADS) NAME##_m(const ARGS* __restrict__ A,                  \
+                                                        const uint4* __restrict__ bt)                 \
+    {                                                                                                 \
+        const uint4 e = bt[blockIdx.x];                                                               \
+        BODY(A[e.x], Blk{e.y, e.z});                                                                  \
+    }
+#define PI_KERNEL2(NAME, BODY, ARGS, THREADS) PI_KERNEL3(NAME, BODY, ARGS, THREADS, xcd_block())
+#define PI_KERNEL(NAME, ARGS, THREADS) PI_KERNEL3(NAME, NAME##_body, ARGS, THREADS, blockIdx.x)
+#define PI_KERNEL_XCD(NAME, ARGS, THREADS) PI_KERNEL3(NAME, NAME##_body, ARGS, THREADS, xcd_block())
+
 static inline int grid_for(long long n, int block, int maxblocks) {
     long long b = (n + block - 1) / block;
     if (b > maxblocks) b = maxblocks;

@@ -31,6 +31,29 @@ static pi_status fail(pi_status s, const std::string& m) { g_err = m; return s; 
             return fail(PI_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));     \
     } while (0)
 
+/* One step of one model as a plan: the fixed sequence of kernel launches, each
+ * with its argument block and block count (0 = skipped for this model).  Every
+ * model's plan has the same sequence of kernels, so the plans of many models
+ * launch position by position as batched kernels (pi_batch_*). */
+struct Rec { uint16_t kid; uint32_t nblk; uint32_t off; const char* grp; };
+struct Plan {
+    std::vector<Rec> recs;
+    std::vector<uint8_t> args;
+    template <class A> void add(KId k, const char* grp, long long nblk, const A& a)
+    {
+        Rec r;
+        r.kid = (uint16_t)k;
+        r.nblk = (uint32_t)std::max(0ll, nblk);
+        r.off = (uint32_t)args.size();
+        r.grp = grp;
+        args.resize(args.size() + ((sizeof(A) + 15) & ~size_t(15)), 0);
+        std::memcpy(args.data() + r.off, &a, sizeof(A));
+        recs.push_back(r);
+    }
+    void clear() { recs.clear(); args.clear(); }
+};
+template <class A> static A zeroed() { A a; std::memset((void*)&a, 0, sizeof(A)); return a; }
+
 struct KTimer { const char* name; hipEvent_t a, b; };
 
 struct pi_model {
@@ -61,6 +84,11 @@ struct pi_model {
     uint32_t *cnt = nullptr, *seg = nullptr, *partials = nullptr, *pkey = nullptr, *psrc = nullptr;
     uint32_t *skey = nullptr, *ssrc = nullptr, *big = nullptr, *lng = nullptr, *bm = nullptr, *G = nullptr;
     int long_blocks = 1;       /* blocks of the one-block-per-long-cell kernels */
+    Plan plan;                 /* the launches of the last step built */
+    bool dry = false;          /* create_impl only sums the allocation sizes */
+    long long bytes = 0;       /* device bytes allocated (256-byte granules) */
+    struct pi_batch* batch = nullptr;   /* the batch this model steps in, if any */
+    hipStream_t own_stream = nullptr;   /* its stream while it is in a batch */
     Contrib con{};
     unsigned long long* d_count = nullptr;
     int32_t *dstamp = nullptr, *rstamp = nullptr;
@@ -88,6 +116,12 @@ struct pi_model {
 
 static hipError_t dalloc(pi_model* m, void** p, size_t bytes)
 {
+    const size_t rb = ((bytes ? bytes : 4) + 255) & ~size_t(255);
+    m->bytes += (long long)rb;
+    if (m->dry) {                  /* footprint only: a placeholder address */
+        *p = (void*)(uintptr_t)0x100;
+        return hipSuccess;
+    }
     hipError_t e = hipMalloc(p, bytes ? bytes : 4);
     if (e == hipSuccess) {
         m->allocs.push_back(*p);
@@ -96,6 +130,8 @@ static hipError_t dalloc(pi_model* m, void** p, size_t bytes)
     return e;
 }
 #define DALLOC(ptr, bytes) HIPCHECK(dalloc(m, (void**)&(ptr), (size_t)(bytes)))
+/* a device call of create_impl, skipped when only the footprint is computed */
+#define DEVCALL(x) do { if (!m->dry) HIPCHECK(x); } while (0)
 
 /* ------------------------------------------------------------------ timing */
 static void tbegin(pi_model* m, const char* name)
@@ -183,8 +219,10 @@ static void host_fill(const pi_model* m, std::vector<float>& c, int dim, int mod
 static pi_status create_impl(const pi_model_desc* d, pi_model* m)
 {
     const pi_grid_desc& gd = d->grid;
-    HIPCHECK(hipGetDevice(&m->dev));
-    HIPCHECK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    if (!m->dry) {
+        HIPCHECK(hipGetDevice(&m->dev));
+        HIPCHECK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    }
     DevGrid& g = m->g;
     g.Nx = gd.Nx; g.Ny = gd.Ny; g.Nz = gd.Nz; g.Hx = gd.Hx; g.Hy = gd.Hy; g.Hz = gd.Hz;
     g.Px = gd.Nx + 2 * gd.Hx; g.Py = gd.Ny + 2 * gd.Hy; g.Pz = gd.Nz + 2 * gd.Hz;
@@ -204,8 +242,8 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     m->bgc.assign(d->bgc, d->bgc + PI_NBGC);
     float *dzC, *dzF;
     DALLOC(dzC, g.Pz * 4); DALLOC(dzF, g.Pz * 4);
-    HIPCHECK(hipMemcpy(dzC, gd.dzC, g.Pz * 4, hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(dzF, gd.dzF, g.Pz * 4, hipMemcpyHostToDevice));
+    DEVCALL(hipMemcpy(dzC, gd.dzC, g.Pz * 4, hipMemcpyHostToDevice));
+    DEVCALL(hipMemcpy(dzF, gd.dzF, g.Pz * 4, hipMemcpyHostToDevice));
     g.dzC = dzC; g.dzF = dzF;
     {   /* RN_double(1/b) of the fixed divisors Δx, Δy, 6, dzC[k], dzF[k] and of the
          * cell volume (Δx·Δy)·dzF[k] (divc in pi_device.h) */
@@ -223,8 +261,8 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
         float* dvol;
         DALLOC(tabs, 3ll * Pz * 8);
         DALLOC(dvol, 4ll * Pz);
-        HIPCHECK(hipMemcpy(tabs, t.data(), 3ll * Pz * 8, hipMemcpyHostToDevice));
-        HIPCHECK(hipMemcpy(dvol, vol.data(), 4ll * Pz, hipMemcpyHostToDevice));
+        DEVCALL(hipMemcpy(tabs, t.data(), 3ll * Pz * 8, hipMemcpyHostToDevice));
+        DEVCALL(hipMemcpy(dvol, vol.data(), 4ll * Pz, hipMemcpyHostToDevice));
         g.ydzC = tabs; g.ydzF = tabs + Pz; g.yvol = tabs + 2 * Pz; g.vol = dvol;
     }
     g.has_land = 0; g.lm = nullptr;
@@ -234,12 +272,12 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
         if (any0) {
             uint8_t* lm;
             DALLOC(lm, g.ncell);
-            HIPCHECK(hipMemcpy(lm, gd.landmask, g.ncell, hipMemcpyHostToDevice));
+            DEVCALL(hipMemcpy(lm, gd.landmask, g.ncell, hipMemcpyHostToDevice));
             g.lm = lm; g.has_land = 1;
         }
     }
     DALLOC(m->d_params, m->h_params.size() * 4);
-    HIPCHECK(hipMemcpy(m->d_params, m->h_params.data(), m->h_params.size() * 4, hipMemcpyHostToDevice));
+    DEVCALL(hipMemcpy(m->d_params, m->h_params.data(), m->h_params.size() * 4, hipMemcpyHostToDevice));
     /* store x2 */
     DevState& hs = m->h_state;
     memset(&hs, 0, sizeof hs);
@@ -299,20 +337,20 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     DALLOC(m->segp, (nc + 1) * 4); DALLOC(m->small, 2 * m->ctot * 4);
     DALLOC(m->cnts, nc * 4); DALLOC(m->mix, nc * 4);
     DALLOC(m->sflag, m->ctot);
-    HIPCHECK(hipMemset(m->cnts, 0, nc * 4));
-    HIPCHECK(hipMemset(m->sflag, 0, m->ctot));
+    DEVCALL(hipMemset(m->cnts, 0, nc * 4));
+    DEVCALL(hipMemset(m->sflag, 0, m->ctot));
     DALLOC(m->dstamp, m->ctot * 4); DALLOC(m->rstamp, m->ctot * 4);
     DALLOC(m->dcell, m->ctot * 4); DALLOC(m->rcons, m->ctot * 5 * 4);
-    {   /* stamps start below any iteration */
+    if (!m->dry) {   /* stamps start below any iteration */
         std::vector<int32_t> neg(m->ctot, -1);
-        HIPCHECK(hipMemcpy(m->dstamp, neg.data(), m->ctot * 4, hipMemcpyHostToDevice));
-        HIPCHECK(hipMemcpy(m->rstamp, neg.data(), m->ctot * 4, hipMemcpyHostToDevice));
+        DEVCALL(hipMemcpy(m->dstamp, neg.data(), m->ctot * 4, hipMemcpyHostToDevice));
+        DEVCALL(hipMemcpy(m->rstamp, neg.data(), m->ctot * 4, hipMemcpyHostToDevice));
     }
     DALLOC(m->d_state, sizeof(DevState));
     hs.scur = hs.tcur = hs.pcur = 0;
     hs.iteration = 0;
     hs.nstore = 0;
-    HIPCHECK(hipMemcpy(m->d_state, &hs, sizeof hs, hipMemcpyHostToDevice));
+    DEVCALL(hipMemcpy(m->d_state, &hs, sizeof hs, hipMemcpyHostToDevice));
     for (int q = 0; q < d->n_diag_tracers; ++q) {
         if (d->diag_tracers[q] < 0 || d->diag_tracers[q] >= PI_NTRACERS) return fail(PI_ERR_ARG, "diag tracer id");
         float* p; DALLOC(p, nc * 4);
@@ -336,9 +374,8 @@ pi_status pi_init(int32_t device)
     return PI_OK;
 }
 
-pi_status pi_model_create(const pi_model_desc* d, pi_model** out)
+static pi_status check_desc(const pi_model_desc* d)
 {
-    if (!d || !out) return fail(PI_ERR_ARG, "null argument");
     const pi_grid_desc& gd = d->grid;
     if (gd.Nx < 1 || gd.Ny < 1 || gd.Nz < 1 || gd.Hx < 2 || gd.Hy < 2 || gd.Hz < 2 || gd.Hx > 8 || gd.Hy > 8 || gd.Hz > 8)
         return fail(PI_ERR_ARG, "grid: N >= 1 and 2 <= halo <= 8 required");
@@ -351,6 +388,34 @@ pi_status pi_model_create(const pi_model_desc* d, pi_model** out)
         float t = d->params[s * PI_NPARAMS + PI_P_dvid_type];
         if (!(t == 1.0f || t == 2.0f || t == 3.0f)) return fail(PI_ERR_ARG, "Wrong cell division type, must be in 1 to 3");
     }
+    return PI_OK;
+}
+
+pi_status pi_footprint(const pi_model_desc* d, int64_t* bytes)
+{
+    if (!d || !bytes) return fail(PI_ERR_ARG, "null argument");
+    pi_status st = check_desc(d);
+    if (st != PI_OK) return st;
+    pi_model m;
+    m.dry = true;
+    st = create_impl(d, &m);
+    if (st != PI_OK) return st;
+    *bytes = m.bytes;
+    return PI_OK;
+}
+
+pi_status pi_model_footprint(const pi_model* m, int64_t* bytes)
+{
+    if (!m || !bytes) return fail(PI_ERR_ARG, "null argument");
+    *bytes = m->bytes;
+    return PI_OK;
+}
+
+pi_status pi_model_create(const pi_model_desc* d, pi_model** out)
+{
+    if (!d || !out) return fail(PI_ERR_ARG, "null argument");
+    pi_status cs = check_desc(d);
+    if (cs != PI_OK) return cs;
     pi_model* m = new pi_model();
     pi_status st = create_impl(d, m);
     if (st != PI_OK) {
@@ -372,6 +437,7 @@ static void drop_graph(pi_model* m)
 pi_status pi_model_destroy(pi_model* m)
 {
     if (!m) return PI_OK;
+    if (m->batch) (void)pi_batch_remove(m->batch, m);
     (void)hipStreamSynchronize(m->stream);
     drop_graph(m);
     for (auto& t : m->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
@@ -510,141 +576,205 @@ pi_status pi_set_temperature(pi_model* m, const float* t)
 }
 
 /* ------------------------------------------------------------------ step */
-static pi_status enqueue_step(pi_model* m, float dt)
+static KInfo g_kinfo[K_NKID];
+static const KInfo* kinfo()
+{
+    static bool done = false;
+    if (!done) { reg_particles(g_kinfo); reg_tracers(g_kinfo); reg_util(g_kinfo); done = true; }
+    return g_kinfo;
+}
+
+static void plan_scan(Plan& P, const char* grp, const ScanIn& in, uint32_t* partials, uint32_t* out)
+{
+    ScanArgs a = zeroed<ScanArgs>();
+    a.in = in; a.partials = partials; a.out = out; a.nb = scan_tiles(in.n);
+    P.add(K_SCAN_P, grp, a.nb, a);
+    P.add(K_SCAN_T, grp, 1, a);
+    P.add(K_SCAN_F, grp, a.nb, a);
+}
+
+/* Builds the plan of the model's next step and advances the host-side
+ * bookkeeping of that step (vel₀ <- vel₁, iteration count). */
+static void build_step(pi_model* m, float dt, Plan& P)
 {
     const DevGrid& g = m->g;
-    hipStream_t s = m->stream;
     DevState* st = m->d_state;
-    const int pb = blocks_for(2 * m->ctot);
-    const int cb = blocks_for(m->ncell);
-    const int ib = blocks_for(m->nint);
+    const long long pb = blocks_for(2 * m->ctot);
+    const long long cb = blocks_for(m->ncell);
+    P.clear();
     /* vel½ (time_step.jl:17-19); equal to vel₀ bit for bit when vel₀ == vel₁ */
     int vh = m->v0;
-    if (!m->vel_same) {
-        vh = 3 - m->v0 - m->v1;
-        if (vh < 0 || vh > 2 || vh == m->v0 || vh == m->v1) {
-            for (vh = 0; vh < 3 && (vh == m->v0 || vh == m->v1); ++vh) {}
+    {
+        VelhalfArgs a = zeroed<VelhalfArgs>();
+        if (!m->vel_same) {
+            vh = 3 - m->v0 - m->v1;
+            if (vh < 0 || vh > 2 || vh == m->v0 || vh == m->v1) {
+                for (vh = 0; vh < 3 && (vh == m->v0 || vh == m->v1); ++vh) {}
+            }
+            for (int c = 0; c < 3; ++c) { a.a[c] = m->vel[m->v0][c]; a.b[c] = m->vel[m->v1][c]; a.h[c] = m->vel[vh][c]; }
+            a.n = m->ncell; a.st = st;
         }
-        tbegin(m, "velhalf");
-        for (int c = 0; c < 3; ++c) launch_velhalf(s, m->vel[m->v0][c], m->vel[m->v1][c], m->vel[vh][c], m->ncell, st);
-        tend(m);
+        P.add(K_VELHALF, "velhalf", m->vel_same ? 0 : grid_for(m->ncell, 256, 4096), a);
     }
     const Vel V0{m->vel[m->v0][0], m->vel[m->v0][1], m->vel[m->v0][2]};
     const Vel VH{m->vel[vh][0], m->vel[vh][1], m->vel[vh][2]};
     const Vel V1{m->vel[m->v1][0], m->vel[m->v1][1], m->vel[m->v1][2]};
     /* division marks + capacity check */
-    tbegin(m, "marks+scan");
-    launch_marks(s, st, m->sflag, m->bm, m->bm + (size_t)m->nsp * m->nw, m->nsp, m->nw, m->cap);
-    launch_scan(s, scan_bits(m->bm + (size_t)m->nsp * m->nw, m->bm, m->nsp, m->nw, m->cap), m->partials, m->G);
-    launch_capcheck(s, st, m->G, m->nsp, m->nw);
-    tend(m);
-    /* advection + diffusion + cells */
-    AdvArgs aa{};
-    aa.g = g; aa.st = st; aa.v0 = V0; aa.vh = VH; aa.v1 = V1; aa.dt = dt;
-    aa.sx = std::sqrt(m->bgc[PI_B_kappa_hP] * dt);
-    aa.sy = std::sqrt(m->bgc[PI_B_kappa_hP] * dt);
-    aa.sz = std::sqrt(m->bgc[PI_B_kappa_vP] * dt);
-    aa.seed = m->seed;
-    aa.newc = m->newc; aa.stayf = m->stayf; aa.small = m->small; aa.cnts = m->cnts;
-    aa.dstamp = m->dstamp; aa.dcell = m->dcell; aa.cap = m->cap;
-    tbegin(m, "adv");
-    launch_adv(s, pb, aa);
-    tend(m);
-    tbegin(m, "bin");
-    {   /* stayer prefix over the sorted part (device-side length nA) */
-        ScanIn sp = scan_plain(m->stayf, (uint32_t)(2 * m->ctot));
-        sp.dn = (const uint32_t*)((const char*)st + offsetof(DevState, nA));
-        launch_scan(s, sp, m->partials, m->P);
+    {
+        MarksArgs a = zeroed<MarksArgs>();
+        a.st = st; a.sflag = m->sflag; a.bmA = m->bm; a.bmD = m->bm + (size_t)m->nsp * m->nw;
+        a.nsp = m->nsp; a.nw = m->nw; a.cap = m->cap;
+        P.add(K_MARKS, "marks+scan", grid_for((long long)m->nsp * m->nw, 256, 8192), a);
+        plan_scan(P, "marks+scan", scan_bits(m->bm + (size_t)m->nsp * m->nw, m->bm, m->nsp, m->nw, m->cap), m->partials, m->G);
+        CapArgs c = zeroed<CapArgs>();
+        c.st = st; c.G = m->G; c.nsp = m->nsp; c.nw = m->nw;
+        P.add(K_CAPCHECK, "marks+scan", 1, c);
     }
-    launch_cellcount(s, st, m->ncell, m->segp, m->P, m->cnts, m->cnt, m->lng);
-    launch_scan(s, scan_plain(m->cnt, (uint32_t)m->ncell), m->partials, m->seg);
-    launch_place_stay(s, pb, st, m->seg, m->segp, m->P, m->stayf, m->newc, m->pkey, m->psrc);
-    launch_place_small(s, pb, st, m->seg, m->small, m->newc, m->cnts, m->pkey, m->psrc, m->mix);
-    launch_mixsort(s, cb, st, m->seg, m->mix, m->pkey, m->psrc, m->big);
-    launch_bigcells(s, 256, st, m->big, m->seg, m->segp, m->P, m->pkey, m->psrc, m->skey, m->ssrc);
-    launch_cellsum(s, st, m->ncell, m->seg, m->psrc, m->Chl, m->pop, m->lng, m->long_blocks);
-    tend(m);
-    tbegin(m, "par");
-    launch_par(s, blocks_for((long long)g.Nx * g.Ny), st, g, m->Chl, m->PARF_in, m->PARF_out, m->diagPAR,
-               m->bgc[PI_B_kc], m->bgc[PI_B_kw]);
-    tend(m);
+    /* advection + diffusion + cells */
+    {
+        AdvArgs aa = zeroed<AdvArgs>();
+        aa.g = g; aa.st = st; aa.v0 = V0; aa.vh = VH; aa.v1 = V1; aa.dt = dt;
+        aa.sx = std::sqrt(m->bgc[PI_B_kappa_hP] * dt);
+        aa.sy = std::sqrt(m->bgc[PI_B_kappa_hP] * dt);
+        aa.sz = std::sqrt(m->bgc[PI_B_kappa_vP] * dt);
+        aa.seed = m->seed;
+        aa.newc = m->newc; aa.stayf = m->stayf; aa.small = m->small; aa.cnts = m->cnts;
+        aa.dstamp = m->dstamp; aa.dcell = m->dcell; aa.cap = m->cap;
+        P.add(K_ADV, "adv", pb, aa);
+    }
+    {
+        ScanIn sp = scan_plain(m->stayf, (uint32_t)(2 * m->ctot));      /* stayer prefix over the sorted part */
+        sp.dn = (const uint32_t*)((const char*)st + offsetof(DevState, nA));   /* device-side length nA */
+        plan_scan(P, "bin", sp, m->partials, m->P);
+        BinArgs b = zeroed<BinArgs>();
+        b.st = st; b.ncell = m->ncell; b.seg = m->seg; b.segp = m->segp; b.P = m->P; b.stayf = m->stayf;
+        b.newc = m->newc; b.small = m->small; b.cnts = m->cnts; b.cnt = m->cnt; b.lng = m->lng;
+        b.pkey = m->pkey; b.psrc = m->psrc; b.mix = m->mix; b.big = m->big; b.skey = m->skey; b.ssrc = m->ssrc;
+        b.Chl = m->Chl; b.pop = m->pop;
+        P.add(K_CELLCOUNT, "bin", grid_for(m->ncell, 256, 8192), b);
+        plan_scan(P, "bin", scan_plain(m->cnt, (uint32_t)m->ncell), m->partials, m->seg);
+        P.add(K_PLACE_STAY, "bin", pb, b);
+        P.add(K_PLACE_SMALL, "bin", pb, b);
+        P.add(K_MIXSORT, "bin", cb, b);
+        P.add(K_BIGCELLS, "bin", 256, b);
+        P.add(K_CELLSUM, "bin", grid_for(m->ncell, 256, 8192), b);
+        P.add(K_CELLSUM_LONG, "bin", m->long_blocks, b);
+    }
+    {
+        ParArgs a = zeroed<ParArgs>();
+        a.g = g; a.st = st; a.Chl = m->Chl; a.PARF_in = m->PARF_in; a.PARF_out = m->PARF_out;
+        a.diag_par = m->diagPAR; a.kc = m->bgc[PI_B_kc]; a.kw = m->bgc[PI_B_kw];
+        P.add(K_PAR, "par", blocks_for((long long)g.Nx * g.Ny), a);
+    }
     /* physiology */
-    PhysArgs pa{};
-    pa.g = g; pa.st = st; pa.tab = m->tab; pa.cap = m->cap; pa.nsp = m->nsp; pa.params = m->d_params;
-    pa.seg = m->seg; pa.pkey = m->pkey; pa.psrc = m->psrc; pa.temp = m->temp; pa.pop = m->pop;
-    pa.bmA = m->bm; pa.bmD = m->bm + (size_t)m->nsp * m->nw; pa.G = m->G; pa.nw = m->nw;
-    pa.newc = m->newc; pa.sflag = m->sflag;
-    pa.con = m->con; pa.diag_num = m->diag; pa.dt = dt; pa.seed = m->seed;
-    pa.dstamp = m->dstamp; pa.dcell = m->dcell; pa.rstamp = m->rstamp; pa.rcons = m->rcons;
-    tbegin(m, "phys");
-    launch_phys(s, pb, pa);
-    tend(m);
-    PlkArgs ka{};
-    ka.g = g; ka.seg = m->seg; ka.pkey = m->pkey; ka.con = m->con; ka.diag = m->diag; ka.st = st;
-    ka.rstamp = m->rstamp; ka.rcons = m->rcons; ka.cap = m->cap;
-    for (int q = 0; q < PI_NTRACERS; ++q) ka.plk[q] = m->plk[q] ? m->plk[q] : m->plk[PI_DIC];
-    tbegin(m, "plk");
-    launch_plk(s, cb, ka, m->lng, m->skey, m->ssrc, m->long_blocks);
-    tend(m);
+    {
+        PhysArgs pa = zeroed<PhysArgs>();
+        pa.g = g; pa.st = st; pa.tab = m->tab; pa.cap = m->cap; pa.nsp = m->nsp; pa.params = m->d_params;
+        pa.seg = m->seg; pa.pkey = m->pkey; pa.psrc = m->psrc; pa.temp = m->temp; pa.pop = m->pop;
+        pa.bmA = m->bm; pa.bmD = m->bm + (size_t)m->nsp * m->nw; pa.G = m->G; pa.nw = m->nw;
+        pa.newc = m->newc; pa.sflag = m->sflag;
+        pa.con = m->con; pa.diag_num = m->diag; pa.dt = dt; pa.seed = m->seed;
+        pa.dstamp = m->dstamp; pa.dcell = m->dcell; pa.rstamp = m->rstamp; pa.rcons = m->rcons;
+        P.add(K_PHYS, "phys", pb, pa);
+    }
+    {
+        PlkArgs ka = zeroed<PlkArgs>();
+        ka.g = g; ka.seg = m->seg; ka.pkey = m->pkey; ka.con = m->con; ka.diag = m->diag; ka.st = st;
+        ka.rstamp = m->rstamp; ka.rcons = m->rcons; ka.cap = m->cap;
+        for (int q = 0; q < PI_NTRACERS; ++q) ka.plk[q] = m->plk[q] ? m->plk[q] : m->plk[PI_DIC];
+        ka.lng = m->lng; ka.lg = m->skey; ka.lm = m->ssrc;
+        P.add(K_PLK, "plk", cb, ka);
+        P.add(K_PLK_LONG, "plk", m->long_blocks, ka);
+    }
     /* tracers */
-    SweepArgs sa{};
-    sa.g = g; sa.st = st; sa.dt = dt;
-    sa.vel = V1.u; sa.dim = 0;
-    sa.out = m->T1[0]; sa.prev = nullptr;
-    tbegin(m, "sweep_x");
-    launch_sweep(s, ib, sa);
-    tend(m);
-    FillArgs fa{};
-    fa.g = g; fa.st = st; fa.nf = PI_NTRACERS; fa.dim = 1; fa.lines_full = 0;
-    fa.f = m->T1[0];
+    const bool land = g.has_land != 0;
     /* Periodic with N < H: the fills read halo planes of the slab they fill */
     const bool alias = (g.tx == PI_PERIODIC && g.Nx < g.Hx) || (g.ty == PI_PERIODIC && g.Ny < g.Hy);
-    tbegin(m, "fill");
-    if (alias) launch_halo_copy(s, fa);
-    launch_fill(s, blocks_for((long long)g.Nx * g.Nz * PI_NTRACERS), fa);
-    tend(m);
-    sa.vel = V1.v; sa.dim = 1;
-    sa.out = m->T2[0]; sa.prev = m->T1[0];
-    tbegin(m, "sweep_y");
-    launch_sweep(s, ib, sa);
-    tend(m);
-    fa.dim = 2;
-    fa.f = m->T2[0];
-    tbegin(m, "fill");
-    launch_fill(s, blocks_for((long long)g.Nx * g.Ny * PI_NTRACERS), fa);
-    tend(m);
-    FinalArgs fn{};
-    fn.g = g; fn.st = st; fn.w = V1.w; fn.dt = dt;
-    fn.prev = m->T2[0]; fn.plk = m->plk_slab;
-    fn.kh = m->bgc[PI_B_kappa_h]; fn.kv = m->bgc[PI_B_kappa_v];
-    fn.wsink_inorg = m->bgc[PI_B_w_sink_inorg]; fn.wsink_org = m->bgc[PI_B_w_sink_org];
-    fn.kDOC = m->bgc[PI_B_kDOC]; fn.kPOC = m->bgc[PI_B_kPOC]; fn.kDON = m->bgc[PI_B_kDON];
-    fn.kPON = m->bgc[PI_B_kPON]; fn.kDOP = m->bgc[PI_B_kDOP]; fn.kPOP = m->bgc[PI_B_kPOP]; fn.Nit = m->bgc[PI_B_Nit];
-    fn.lPOC = m->bgc[PI_B_lambda_POC]; fn.lmin = m->bgc[PI_B_lambda_min]; fn.ldust = m->bgc[PI_B_lambda_dust];
-    fn.DFeFrac = m->bgc[PI_B_DFeFrac]; fn.kdiss = m->bgc[PI_B_kdiss]; fn.lig = m->bgc[PI_B_ligand];
-    fn.lFe = m->bgc[PI_B_lambda_Fe];
-    tbegin(m, "final");
-    launch_final(s, ib, fn);
-    tend(m);
-    /* fill_halo_tracer! of the new tracers: x, then y, then z over whole planes */
-    tbegin(m, "fill");
-    fa.f = nullptr;
-    fa.lines_full = 1;
-    if (alias) launch_halo_copy(s, fa);
-    fa.dim = 0; launch_fill(s, blocks_for((long long)g.Py * g.Pz * PI_NTRACERS), fa);
-    fa.dim = 1; launch_fill(s, blocks_for((long long)g.Px * g.Pz * PI_NTRACERS), fa);
-    fa.dim = 2; launch_fill(s, blocks_for((long long)g.Px * g.Py * PI_NTRACERS), fa);
-    tend(m);
-    for (size_t a = 0; a < m->diag_ids.size(); ++a) launch_accum(s, m->diag_tr[a], st, m->diag_ids[a], m->ncell);
-    launch_copy_u32(s, m->segp, m->seg, m->ncell + 1, st);     /* the next step's previous segments */
-    launch_finish(s, st, m->seg, m->ncell, m->nsp);
-    HIPCHECK(hipGetLastError());
+    {
+        SweepArgs sa = zeroed<SweepArgs>();
+        sa.g = g; sa.st = st; sa.dt = dt;
+        sa.vel = V1.u; sa.dim = 0;
+        sa.out = m->T1[0]; sa.prev = nullptr;
+        const long long xb = sweep_x_blocks(g);
+        P.add(K_SWEEPX0, "sweep_x", land ? 0 : xb, sa);
+        P.add(K_SWEEPX1, "sweep_x", land ? xb : 0, sa);
+        FillArgs fa = zeroed<FillArgs>();
+        fa.g = g; fa.st = st; fa.nf = PI_NTRACERS; fa.dim = 1; fa.lines_full = 0;
+        fa.f = m->T1[0];
+        P.add(K_HALOCOPY, "fill", alias ? grid_for((long long)g.ncell * PI_NTRACERS, 256, 4096) : 0, fa);
+        P.add(K_FILL, "fill", blocks_for((long long)g.Nx * g.Nz * PI_NTRACERS), fa);
+        sa.vel = V1.v; sa.dim = 1;
+        sa.out = m->T2[0]; sa.prev = m->T1[0];
+        const long long yb = grid_for(sweep_y_items(g), 256, 8192);
+        P.add(K_SWEEPY0, "sweep_y", land ? 0 : yb, sa);
+        P.add(K_SWEEPY1, "sweep_y", land ? yb : 0, sa);
+        fa.dim = 2;
+        fa.f = m->T2[0];
+        P.add(K_FILL, "fill", blocks_for((long long)g.Nx * g.Ny * PI_NTRACERS), fa);
+        FinalArgs fn = zeroed<FinalArgs>();
+        fn.g = g; fn.st = st; fn.w = V1.w; fn.dt = dt;
+        fn.prev = m->T2[0]; fn.plk = m->plk_slab;
+        fn.kh = m->bgc[PI_B_kappa_h]; fn.kv = m->bgc[PI_B_kappa_v];
+        fn.wsink_inorg = m->bgc[PI_B_w_sink_inorg]; fn.wsink_org = m->bgc[PI_B_w_sink_org];
+        fn.kDOC = m->bgc[PI_B_kDOC]; fn.kPOC = m->bgc[PI_B_kPOC]; fn.kDON = m->bgc[PI_B_kDON];
+        fn.kPON = m->bgc[PI_B_kPON]; fn.kDOP = m->bgc[PI_B_kDOP]; fn.kPOP = m->bgc[PI_B_kPOP]; fn.Nit = m->bgc[PI_B_Nit];
+        fn.lPOC = m->bgc[PI_B_lambda_POC]; fn.lmin = m->bgc[PI_B_lambda_min]; fn.ldust = m->bgc[PI_B_lambda_dust];
+        fn.DFeFrac = m->bgc[PI_B_DFeFrac]; fn.kdiss = m->bgc[PI_B_kdiss]; fn.lig = m->bgc[PI_B_ligand];
+        fn.lFe = m->bgc[PI_B_lambda_Fe];
+        const long long fb = grid_for(final_items(g), 256, 8192);
+        P.add(K_FINAL0, "final", land ? 0 : fb, fn);
+        P.add(K_FINAL1, "final", land ? fb : 0, fn);
+        /* fill_halo_tracer! of the new tracers: x, then y, then z over whole planes */
+        fa.f = nullptr;
+        fa.lines_full = 1;
+        P.add(K_HALOCOPY, "fill", alias ? grid_for((long long)g.ncell * PI_NTRACERS, 256, 4096) : 0, fa);
+        fa.dim = 0; P.add(K_FILL, "fill", blocks_for((long long)g.Py * g.Pz * PI_NTRACERS), fa);
+        fa.dim = 1; P.add(K_FILL, "fill", blocks_for((long long)g.Px * g.Pz * PI_NTRACERS), fa);
+        fa.dim = 2; P.add(K_FILL, "fill", blocks_for((long long)g.Px * g.Py * PI_NTRACERS), fa);
+    }
+    {
+        AccumArgs a = zeroed<AccumArgs>();
+        a.nq = (int)m->diag_ids.size(); a.n = m->ncell; a.st = st;
+        for (int j = 0; j < a.nq; ++j) { a.d[j] = m->diag_tr[j]; a.q[j] = m->diag_ids[j]; }
+        P.add(K_ACCUM, "diag", a.nq ? grid_for(m->ncell, 256, 4096) : 0, a);
+        CopyArgs c = zeroed<CopyArgs>();
+        c.dst = m->segp; c.src = m->seg; c.n = m->ncell + 1; c.st = st;   /* the next step's previous segments */
+        P.add(K_COPY, "finish", grid_for(m->ncell + 1, 256, 4096), c);
+        FinishArgs f = zeroed<FinishArgs>();
+        f.st = st; f.seg = m->seg; f.ncell = m->ncell; f.nsp = m->nsp;
+        P.add(K_FINISH, "finish", 1, f);
+    }
     /* vel₀ <- vel₁ */
     m->v0 = m->v1;
     m->vel_same = true;
     m->host_iter += 1;
     m->last_dt = dt;
+}
+
+/* the plan's launches on the model's stream (with per-group timers when on) */
+static pi_status launch_plan(pi_model* m, const Plan& P)
+{
+    const KInfo* K = kinfo();
+    const char* grp = nullptr;
+    for (const Rec& r : P.recs) {
+        if (r.nblk == 0) continue;
+        if (m->timing && grp != r.grp) {
+            if (grp) tend(m);
+            tbegin(m, r.grp);
+            grp = r.grp;
+        }
+        void* argp = (void*)(P.args.data() + r.off);
+        HIPCHECK(hipLaunchKernel(K[r.kid].one, dim3(r.nblk), dim3(K[r.kid].threads), &argp, 0, m->stream));
+    }
+    if (m->timing && grp) tend(m);
+    HIPCHECK(hipGetLastError());
     return PI_OK;
+}
+
+static pi_status enqueue_step(pi_model* m, float dt)
+{
+    build_step(m, dt, m->plan);
+    return launch_plan(m, m->plan);
 }
 
 pi_status pi_step(pi_model* m, float dt)
@@ -860,6 +990,249 @@ pi_status pi_probe_copy(int64_t bytes)
     if (e == hipSuccess) e = hipDeviceSynchronize();
     (void)hipFree(a); (void)hipFree(b);
     if (e != hipSuccess) return fail(PI_ERR_HIP, hipGetErrorString(e));
+    return PI_OK;
+}
+
+
+/* ================================================================ batches
+ * Many models stepped together: at every position of the (common) launch
+ * sequence one batched kernel runs the blocks of all members, each block
+ * finding its model, its index and its block count in a table.  Members keep
+ * their own buffers and device state, so results are the same as stepping
+ * each alone; a member's device-side error stops only its own work. */
+} /* extern "C" */
+
+struct BSlot {
+    uint16_t kid = 0;
+    uint32_t nblk = 0;            /* blocks of the batched launch */
+    void* dargs = nullptr;        /* member m's arguments at m * argsz */
+    uint4* dbt = nullptr;         /* per block: (member, block, member's blocks, 0) */
+    size_t args_cap = 0, bt_cap = 0;
+};
+
+struct pi_batch {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    std::vector<pi_model*> mem;
+    std::vector<Plan> plans;      /* per member: the plan the tables were built from */
+    std::vector<Plan> next;
+    std::vector<BSlot> slots;
+    hipGraphExec_t gexec = nullptr;
+    bool tables = false;
+    long long budget = 16384;     /* blocks per batched launch of a block-stride kernel */
+};
+
+static void batch_drop_graph(pi_batch* b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+}
+
+static bool same_plan(const Plan& a, const Plan& b)
+{
+    if (a.recs.size() != b.recs.size() || a.args.size() != b.args.size()) return false;
+    for (size_t i = 0; i < a.recs.size(); ++i)
+        if (a.recs[i].kid != b.recs[i].kid || a.recs[i].nblk != b.recs[i].nblk || a.recs[i].off != b.recs[i].off)
+            return false;
+    return std::memcmp(a.args.data(), b.args.data(), a.args.size()) == 0;
+}
+
+/* argument and block tables of every position from the members' plans */
+static pi_status batch_tables(pi_batch* b)
+{
+    const KInfo* K = kinfo();
+    const size_t nm = b->mem.size();
+    size_t npos = 0;
+    for (const Plan& P : b->next) if (!P.recs.empty()) { npos = P.recs.size(); break; }
+    for (const Plan& P : b->next)
+        if (!P.recs.empty() && P.recs.size() != npos) return fail(PI_ERR_ARG, "batch: members with different step plans");
+    HIPCHECK(hipStreamSynchronize(b->stream));        /* the previous tables may still be read */
+    if (b->slots.size() < npos) b->slots.resize(npos);
+    std::vector<uint8_t> ha;
+    std::vector<uint4> hb;
+    std::vector<uint32_t> nb(nm);
+    for (size_t p = 0; p < npos; ++p) {
+        BSlot& S = b->slots[p];
+        int kid = -1;
+        for (const Plan& P : b->next) if (!P.recs.empty()) { kid = P.recs[p].kid; break; }
+        for (const Plan& P : b->next)
+            if (!P.recs.empty() && P.recs[p].kid != kid) return fail(PI_ERR_ARG, "batch: members with different step plans");
+        const KInfo& ki = K[kid];
+        S.kid = (uint16_t)kid;
+        long long total = 0;
+        for (size_t m = 0; m < nm; ++m) {
+            nb[m] = b->next[m].recs.empty() ? 0u : b->next[m].recs[p].nblk;
+            total += nb[m];
+        }
+        if (ki.stride && total > b->budget) {      /* share the budget in proportion, >= 1 block each */
+            long long t2 = 0;
+            for (size_t m = 0; m < nm; ++m) {
+                if (!nb[m]) continue;
+                nb[m] = (uint32_t)std::max(1ll, (long long)nb[m] * b->budget / total);
+                t2 += nb[m];
+            }
+            total = t2;
+        }
+        S.nblk = (uint32_t)total;
+        if (!total) continue;
+        ha.assign(nm * ki.argsz, 0);
+        hb.clear();
+        hb.reserve((size_t)total);
+        for (size_t m = 0; m < nm; ++m) {
+            if (!nb[m]) continue;
+            const Plan& P = b->next[m];
+            std::memcpy(ha.data() + m * ki.argsz, P.args.data() + P.recs[p].off, ki.argsz);
+            for (uint32_t k = 0; k < nb[m]; ++k) hb.push_back(make_uint4((uint32_t)m, k, nb[m], 0u));
+        }
+        if (S.args_cap < ha.size()) {
+            if (S.dargs) (void)hipFree(S.dargs);
+            S.dargs = nullptr;
+            HIPCHECK(hipMalloc(&S.dargs, ha.size()));
+            S.args_cap = ha.size();
+        }
+        if (S.bt_cap < hb.size()) {
+            if (S.dbt) (void)hipFree(S.dbt);
+            S.dbt = nullptr;
+            HIPCHECK(hipMalloc((void**)&S.dbt, hb.size() * sizeof(uint4)));
+            S.bt_cap = hb.size();
+        }
+        HIPCHECK(hipMemcpyAsync(S.dargs, ha.data(), ha.size(), hipMemcpyHostToDevice, b->stream));
+        HIPCHECK(hipMemcpyAsync(S.dbt, hb.data(), hb.size() * sizeof(uint4), hipMemcpyHostToDevice, b->stream));
+        HIPCHECK(hipStreamSynchronize(b->stream));    /* ha / hb are reused */
+    }
+    for (size_t p = npos; p < b->slots.size(); ++p) b->slots[p].nblk = 0;
+    b->plans.swap(b->next);
+    b->tables = true;
+    return PI_OK;
+}
+
+static pi_status batch_launch(pi_batch* b)
+{
+    const KInfo* K = kinfo();
+    for (BSlot& S : b->slots) {
+        if (!S.nblk) continue;
+        void* args[2] = {(void*)&S.dargs, (void*)&S.dbt};
+        HIPCHECK(hipLaunchKernel(K[S.kid].multi, dim3(S.nblk), dim3(K[S.kid].threads), args, 0, b->stream));
+    }
+    HIPCHECK(hipGetLastError());
+    return PI_OK;
+}
+
+extern "C" {
+
+pi_status pi_batch_create(pi_batch** out)
+{
+    if (!out) return fail(PI_ERR_ARG, "null argument");
+    pi_batch* b = new pi_batch();
+    hipError_t e = hipGetDevice(&b->dev);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete b; return fail(PI_ERR_HIP, hipGetErrorString(e)); }
+    *out = b;
+    return PI_OK;
+}
+
+pi_status pi_batch_add(pi_batch* b, pi_model* m)
+{
+    if (!b || !m) return fail(PI_ERR_ARG, "null argument");
+    if (m->batch) return fail(PI_ERR_ARG, "model is already in a batch");
+    if (m->dev != b->dev) return fail(PI_ERR_ARG, "model and batch are on different devices");
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    drop_graph(m);
+    m->own_stream = m->stream;
+    m->stream = b->stream;
+    m->batch = b;
+    b->mem.push_back(m);
+    b->plans.emplace_back();
+    b->tables = false;
+    batch_drop_graph(b);
+    return PI_OK;
+}
+
+pi_status pi_batch_remove(pi_batch* b, pi_model* m)
+{
+    if (!b || !m) return fail(PI_ERR_ARG, "null argument");
+    auto it = std::find(b->mem.begin(), b->mem.end(), m);
+    if (it == b->mem.end()) return fail(PI_ERR_ARG, "model is not in this batch");
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    const size_t k = (size_t)(it - b->mem.begin());
+    b->mem.erase(it);
+    b->plans.erase(b->plans.begin() + (long)k);
+    drop_graph(m);
+    m->stream = m->own_stream;
+    m->own_stream = nullptr;
+    m->batch = nullptr;
+    b->tables = false;
+    batch_drop_graph(b);
+    return PI_OK;
+}
+
+/* n steps of every member: each member's next step is planned (staging changes
+ * made since the last step included), the tables are rebuilt when a plan
+ * changed, and a steady step replays one captured graph of the batched
+ * launches. */
+pi_status pi_batch_run(pi_batch* b, float dt, int64_t n)
+{
+    if (!b) return fail(PI_ERR_ARG, "null batch");
+    const size_t nm = b->mem.size();
+    if (nm == 0) return PI_OK;
+    b->next.resize(nm);
+    for (int64_t i = 0; i < n; ++i) {
+        bool same = b->tables;
+        for (size_t m = 0; m < nm; ++m) {
+            pi_model* md = b->mem[m];
+            pi_status s = rebuild_if_dirty(md);
+            if (s != PI_OK) return s;
+            if (md->h_state.err) { b->next[m].clear(); }
+            else build_step(md, dt, b->next[m]);
+            if (same && !same_plan(b->next[m], b->plans[m])) same = false;
+        }
+        if (!same) {
+            batch_drop_graph(b);
+            pi_status s = batch_tables(b);
+            if (s != PI_OK) return s;
+            s = batch_launch(b);                   /* a changed step runs without a graph */
+            if (s != PI_OK) return s;
+            continue;
+        }
+        if (!b->gexec) {
+            hipGraph_t graph = nullptr;
+            HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            pi_status s = batch_launch(b);
+            const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+            if (s != PI_OK) { if (graph) (void)hipGraphDestroy(graph); return s; }
+            if (e != hipSuccess) { if (graph) (void)hipGraphDestroy(graph); return fail(PI_ERR_HIP, hipGetErrorString(e)); }
+            const hipError_t e2 = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e2 != hipSuccess) { b->gexec = nullptr; return fail(PI_ERR_HIP, hipGetErrorString(e2)); }
+        }
+        HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
+    }
+    return PI_OK;
+}
+
+pi_status pi_batch_sync(pi_batch* b)
+{
+    if (!b) return fail(PI_ERR_ARG, "null batch");
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    return PI_OK;
+}
+
+pi_status pi_batch_size(const pi_batch* b, int32_t* n)
+{
+    if (!b || !n) return fail(PI_ERR_ARG, "null argument");
+    *n = (int32_t)b->mem.size();
+    return PI_OK;
+}
+
+pi_status pi_batch_destroy(pi_batch* b)
+{
+    if (!b) return PI_OK;
+    (void)hipStreamSynchronize(b->stream);
+    while (!b->mem.empty()) (void)pi_batch_remove(b, b->mem.back());
+    batch_drop_graph(b);
+    for (BSlot& S : b->slots) { if (S.dargs) (void)hipFree(S.dargs); if (S.dbt) (void)hipFree(S.dbt); }
+    (void)hipStreamDestroy(b->stream);
+    delete b;
     return PI_OK;
 }
 

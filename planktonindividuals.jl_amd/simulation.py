@@ -173,29 +173,46 @@ def update(sim, time_offset=None, check_every=1):
     reference copies every step; the device result is the same).  `check_every`
     controls how often the device error state is checked (1 = after every step,
     like the reference's synchronous exception)."""
-    off = dict(vels=False, PARF=False, temp=False)
-    if time_offset:
-        off.update(time_offset)
+    off = time_offsets(time_offset)
     m = sim.model
     nat = m._native
     for i in range(1, sim.iterations + 1):
-        base = (i - 1) * float(sim.ΔT)
-        if sim.vels is not None:
-            tv = _index(np.float32(base) if off["vels"] else m.t, sim.ΔT_vel)
-            if sim._staged["vel"] != tv:
-                nat.set_velocity(sim.vels[0][..., tv], sim.vels[1][..., tv], sim.vels[2][..., tv])
-                sim._staged["vel"] = tv
-        tp = _index(np.float32(base) if off["PARF"] else m.t, sim.ΔT_PAR)
-        if sim._staged["par"] != tp:
-            nat.set_parf(sim.PARF[:, :, tp])
-            sim._staged["par"] = tp
-        tt = _index(np.float32(base) if off["temp"] else m.t, sim.ΔT_temp)
-        if sim._staged["temp"] != tt:
-            nat.set_temperature(sim.temp[:, :, :, tt])
-            sim._staged["temp"] = tt
+        stage_inputs(sim, i, off)
         nat.step(np.float32(sim.ΔT))
-        # model.t = model.iteration * ΔT (time_step.jl:15)
-        m.t = np.float32(np.float32(nat.iteration()) * sim.ΔT)
+        advance_clock(sim)
         if check_every and (i % check_every == 0):
             nat.sync()
     nat.sync()
+
+
+def time_offsets(time_offset=None):
+    off = dict(vels=False, PARF=False, temp=False)
+    if time_offset:
+        off.update(time_offset)
+    return off
+
+
+def stage_inputs(sim, i, off):
+    """The input staging of iteration i (1-based) of update! (update.jl:18-26)."""
+    m = sim.model
+    nat = m._native
+    base = (i - 1) * float(sim.ΔT)
+    if sim.vels is not None:
+        tv = _index(np.float32(base) if off["vels"] else m.t, sim.ΔT_vel)
+        if sim._staged["vel"] != tv:
+            nat.set_velocity(sim.vels[0][..., tv], sim.vels[1][..., tv], sim.vels[2][..., tv])
+            sim._staged["vel"] = tv
+    tp = _index(np.float32(base) if off["PARF"] else m.t, sim.ΔT_PAR)
+    if sim._staged["par"] != tp:
+        nat.set_parf(sim.PARF[:, :, tp])
+        sim._staged["par"] = tp
+    tt = _index(np.float32(base) if off["temp"] else m.t, sim.ΔT_temp)
+    if sim._staged["temp"] != tt:
+        nat.set_temperature(sim.temp[:, :, :, tt])
+        sim._staged["temp"] = tt
+
+
+def advance_clock(sim):
+    """model.t = model.iteration * ΔT (time_step.jl:15)."""
+    m = sim.model
+    m.t = np.float32(np.float32(m._native.iteration()) * sim.ΔT)

@@ -1,0 +1,134 @@
+"""Batched stepping (pi_batch_*): every member of a batch must end bit-identical
+to the same model stepped alone (which the parity tests pin to the oracle), for
+a mix of 0-D, 1-D, 2-D and 3-D members, with land, diagnostics, a velocity change
+mid-run, a member leaving the batch, and a member that overflows its capacity
+without disturbing the others."""
+import numpy as np
+import pytest
+
+from parity import compare_models
+
+pytestmark = pytest.mark.gpu
+
+
+def _members(pi):
+    out = []
+    g = pi.RectilinearGrid(size=(1, 1, 1), x=(0, 32), y=(0, 32), z=(0, -32))
+    out.append((pi.ModelSpec(g, max_individuals=4000, seed=41,
+                             phyto=pi.phyto_setup({"dvid_P": [3e-3], "mort_P": [2e-3], "grz_P": [2e-8]}, [1500], 1)),
+                None))
+    g = pi.RectilinearGrid(size=(1, 1, 12), x=(0, 32), y=(0, 32), z=(0, -120))
+    rng = np.random.default_rng(1)
+    w = np.zeros((1, 1, 13), np.float32)
+    w[0, 0, 1:12] = (rng.standard_normal(11) * 1e-3).astype(np.float32)
+    out.append((pi.ModelSpec(g, max_individuals=3000, seed=42,
+                             phyto=pi.phyto_setup({"dvid_P": [2e-3, 2e-3]}, [800, 600], 2),
+                             bgc_params={"κvP": 1e-3, "κv": 1e-3}),
+                (np.zeros((1, 1, 12), np.float32), np.zeros((1, 1, 12), np.float32), w)))
+    g = pi.RectilinearGrid(size=(20, 20, 1), x=(0, 20e3), y=(0, 20e3), z=(0, -10))
+    out.append((pi.ModelSpec(g, max_individuals=6000, seed=43,
+                             phyto=pi.phyto_setup({"dvid_P": [2e-3], "mort_P": [1e-3]}, [2500], 1),
+                             bgc_params={"κhP": 1.0, "κh": 20.0}),
+                pi.workloads.cellular_flow(g, U=2.0, L=5e3)))
+    lm = np.ones((12, 10, 4), dtype=bool)
+    lm[3:6, 2:5, 1:3] = False
+    g = pi.RectilinearGrid(size=(12, 10, 4), x=(0, 12e3), y=(0, 10e3), z=(0, -40),
+                           topology=(pi.Bounded, pi.Bounded, pi.Bounded), landmask=lm)
+    rng = np.random.default_rng(4)
+    out.append((pi.ModelSpec(g, max_individuals=2500, seed=44,
+                             phyto=pi.phyto_setup({"dvid_P": [2e-3]}, [1000], 1),
+                             bgc_params={"κhP": 5.0, "κvP": 1e-3, "κh": 20.0, "κv": 1e-3}),
+                ((rng.standard_normal((13, 10, 4)) * 2).astype(np.float32),
+                 (rng.standard_normal((12, 11, 4)) * 2).astype(np.float32),
+                 (rng.standard_normal((12, 10, 5)) * 1e-3).astype(np.float32))))
+    spec, (u, v, w) = pi.workloads.config_3d(n=16, nz=8, N=1500, nsp=2, cap=4000, seed=45, nmodes=12,
+                                             overrides={"dvid_P": [1e-3], "mort_P": [5e-4]}, kappa=(1.0, 1e-4))
+    out.append((spec, (u * 20, v * 20, w * 20)))
+    g = pi.RectilinearGrid(size=(8, 8, 1), x=(0, 8e3), y=(0, 8e3), z=(0, -10))
+    out.append((pi.ModelSpec(g, max_individuals=120, seed=2,
+                             phyto=pi.phyto_setup({"dvid_P": [0.05], "dvid_reg": [1.0e-3]}, [100], 1)), None))
+    return out
+
+
+def _inputs(pi, h, spec, uvw, scale=1.0):
+    g = spec.grid
+    if uvw is not None:
+        h.set_velocity(*(np.asarray(a, np.float32) * np.float32(scale) for a in uvw))
+    h.set_parf(np.full((g.Nx, g.Ny), 550.0, np.float32))
+    h.set_temperature(np.full((g.Nx, g.Ny, g.Nz), 21.0, np.float32))
+
+
+def test_batch_matches_alone(pi):
+    members = _members(pi)
+    A = [pi.PlanktonModel(pi.HIPArch(), spec=s)._native for s, _ in members]
+    B = [pi.PlanktonModel(pi.HIPArch(), spec=s)._native for s, _ in members]
+    DIAG = [pi.TRACER_NAMES.index("NO3"), pi.TRACER_NAMES.index("DOC")]
+    for (s, uvw), a, b in zip(members, A, B):
+        _inputs(pi, a, s, uvw)
+        _inputs(pi, b, s, uvw)
+    A[2].set_diag_tracers(DIAG)
+    B[2].set_diag_tracers(DIAG)
+    # alone: 5 steps, a velocity change on members 2 and 3, 4 more steps
+    for a in A:
+        a.run(60.0, 5)
+    for k in (2, 3):
+        _inputs(pi, A[k], members[k][0], members[k][1], scale=1.5)
+    for a in A:
+        a.run(60.0, 4)
+    # batched: all members for 5 steps; member 0 then leaves and runs alone
+    batch = pi._native.NativeBatch()
+    for b in B:
+        batch.add(b)
+    batch.run(60.0, 5)
+    batch.remove(B[0])
+    for k in (2, 3):
+        _inputs(pi, B[k], members[k][0], members[k][1], scale=1.5)
+    batch.run(60.0, 4)
+    B[0].run(60.0, 4)
+    batch.sync()
+    for a in A[:-1]:
+        a.sync()
+    for b in B[:-1]:
+        b.sync()
+    # the member that overflows its capacity fails alone and in the batch
+    with pytest.raises(pi.CapacityError):
+        A[-1].sync()
+    with pytest.raises(pi.CapacityError):
+        B[-1].sync()
+    for k, ((s, _), a, b) in enumerate(zip(members[:-1], A[:-1], B[:-1])):
+        assert a.iteration() == b.iteration() == 9
+        bad = compare_models(a, b, s.n_species, grid_fields=("par", "Chl", "pop"))
+        assert not bad, f"member {k}: " + "\n".join(bad[:20])
+    for q in DIAG:
+        assert np.array_equal(A[2].diag_tracer(q), B[2].diag_tracer(q))
+    batch.close()
+
+
+def test_ensemble_update_matches_update(pi):
+    """Ensemble.update (batched update! of several simulations with different
+    lengths and forcing) against update on each simulation alone."""
+    def sims():
+        out = []
+        for k, (spec, uvw) in enumerate(_members(pi)[:5]):
+            g = spec.grid
+            n = 6 + 2 * k
+            m = pi.PlanktonModel(pi.HIPArch(), spec=spec)
+            temp = pi.default_temperature(g, 60.0, n).astype(np.float32)
+            kw = {}
+            if uvw is not None:
+                kw = dict(vels=dict(u=uvw[0][..., None], v=uvw[1][..., None], w=uvw[2][..., None]), ΔT_vel=60.0 * 1e6)
+            out.append(pi.PlanktonSimulation(m, ΔT=60.0, iterations=n, temp=temp, **kw))
+        return out
+    alone = sims()
+    for s in alone:
+        pi.update(s)
+    together = sims()
+    ens = pi.Ensemble(together)
+    ens.update()
+    assert not ens.failed
+    for k, (a, b) in enumerate(zip(alone, together)):
+        na, nb = a.model._native, b.model._native
+        assert na.iteration() == nb.iteration() == a.iterations
+        bad = compare_models(na, nb, a.model.n_species)
+        assert not bad, f"member {k}: " + "\n".join(bad[:20])
+    ens.close()

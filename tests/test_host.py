@@ -37,7 +37,9 @@ def test_oracle_exports_mirror_abi(orc):
     for s in declared_symbols():
         name = "orc_" + s[3:]
         if s in ("pi_init", "pi_version", "pi_model_create", "pi_run", "pi_sync", "pi_enable_timing",
-                 "pi_kernel_times", "pi_probe_copy"):
+                 "pi_kernel_times", "pi_probe_copy", "pi_footprint", "pi_model_footprint"):
+            continue
+        if s.startswith("pi_batch_"):       # batches of GPU models: no CPU counterpart
             continue
         assert hasattr(lib, name), name
 
@@ -137,3 +139,23 @@ def test_product_refuses_to_run_without_library(pi, monkeypatch, tmp_path):
     monkeypatch.setattr(pi._native, "lib_path", lambda: str(tmp_path / "libpi_hip.so"))
     with pytest.raises(pi.NativeError, match="no CPU fallback"):
         pi._native.load_library()
+
+
+def test_footprint_is_a_dry_run(pi):
+    """pi_footprint sums the allocations of pi_model_create without a device:
+    positive, growing with capacity and grid, and linear in the capacity up to
+    the 256-byte rounding of each allocation."""
+    if not os.path.exists(pi.lib_path()):
+        pytest.skip("libpi_hip.so not built (run __graft_entry__.build())")
+    from importlib import import_module
+    nat = import_module(pi.__name__ + "._native")
+    P = import_module(pi.__name__ + ".params")
+    g = pi.RectilinearGrid(size=(16, 16, 4), x=(0, 16e3), y=(0, 16e3), z=(0, -40))
+    pp = P.update_phyt_params(P.phyt_params_default(1, pi.QuotaMode()), np.float32, N=1, mode=pi.QuotaMode())
+    prm = P.species_param_matrix(pp, 1)
+    bgc = P.bgc_vector(pi.bgc_params_default())
+    f = [nat.footprint(g, 1, cap, prm, bgc) for cap in (10000, 20000, 40000)]
+    assert 0 < f[0] < f[1] < f[2]
+    assert abs((f[2] - f[1]) - 2 * (f[1] - f[0])) < 64 * 256
+    g2 = pi.RectilinearGrid(size=(32, 32, 4), x=(0, 32e3), y=(0, 32e3), z=(0, -40))
+    assert nat.footprint(g2, 1, 10000, prm, bgc) > f[0]
