@@ -1017,6 +1017,7 @@ struct pi_batch {
     std::vector<Plan> plans;      /* per member: the plan the tables were built from */
     std::vector<Plan> next;
     std::vector<BSlot> slots;
+    std::vector<size_t> order;    /* members by decreasing capacity per cell */
     hipGraphExec_t gexec = nullptr;
     bool tables = false;
     long long budget = 16384;     /* blocks per batched launch of a block-stride kernel */
@@ -1048,6 +1049,13 @@ static pi_status batch_tables(pi_batch* b)
         if (!P.recs.empty() && P.recs.size() != npos) return fail(PI_ERR_ARG, "batch: members with different step plans");
     HIPCHECK(hipStreamSynchronize(b->stream));        /* the previous tables may still be read */
     if (b->slots.size() < npos) b->slots.resize(npos);
+    b->order.resize(nm);
+    for (size_t m = 0; m < nm; ++m) b->order[m] = m;
+    std::stable_sort(b->order.begin(), b->order.end(), [&](size_t x, size_t y) {
+        const pi_model* a = b->mem[x];
+        const pi_model* c = b->mem[y];
+        return (double)a->ctot / (double)a->nint > (double)c->ctot / (double)c->nint;
+    });
     std::vector<uint8_t> ha;
     std::vector<uint4> hb;
     std::vector<uint32_t> nb(nm);
@@ -1082,6 +1090,14 @@ static pi_status batch_tables(pi_batch* b)
             if (!nb[m]) continue;
             const Plan& P = b->next[m];
             std::memcpy(ha.data() + m * ki.argsz, P.args.data() + P.recs[p].off, ki.argsz);
+        }
+        /* Blocks are dispatched roughly in table order.  The per-cell kernels of
+         * crowded cells run one ordered chain per cell, so the members with the
+         * most individuals per cell go first: their chains set the length of
+         * the launch. */
+        const bool chains = kid == K_CELLSUM_LONG || kid == K_PLK_LONG || kid == K_BIGCELLS;
+        for (size_t r = 0; r < nm; ++r) {
+            const size_t m = chains ? b->order[r] : r;
             for (uint32_t k = 0; k < nb[m]; ++k) hb.push_back(make_uint4((uint32_t)m, k, nb[m], 0u));
         }
         if (S.args_cap < ha.size()) {
