@@ -142,6 +142,13 @@ pi_status pi_model_destroy(pi_model* m);
 /* Initial state (generate_individuals / generate_tracers, Individuals.jl:42-72,
  * tracer_fields.jl:93-134): full-capacity SoA for species `sp`. */
 pi_status pi_upload_individuals(pi_model* m, int32_t sp, const pi_individuals_host* h);
+/* The same initial state generated on the device (no host arrays): every
+ * species' slots as initialize_plankton! (QuotaMode/plankton_generation.jl:35-72)
+ * with n_individuals[sp] active, and the tracers as generate_tracers with the
+ * (initial_condition, rand_noise) source (tracer_fields.jl:93-134): value
+ * c0 * (lower + k * 1f-4), k < n drawn per padded cell, then fill_halo_tracer!. */
+pi_status pi_generate_state(pi_model* m, const int64_t* n_individuals, const float* tracer_c0,
+                            const float* tracer_lower, const int32_t* tracer_n);
 /* One padded tracer field (with halos). */
 pi_status pi_upload_tracer(pi_model* m, int32_t tracer, const float* padded);
 

@@ -44,6 +44,9 @@
 
 /* ---------------------------------------------------------------- stream -- */
 enum { PI_KIND_DIFFUSION = 0, PI_KIND_DECISION = 1 };
+/* initial-condition sub-streams (iteration field 0): positions, quotas, the two
+ * uniforms of the size normal, and per tracer (slot = padded cell index) */
+enum { PI_KIND_INIT_POS = 16, PI_KIND_INIT_QUOTA = 17, PI_KIND_INIT_SIZE = 18, PI_KIND_TRACER = 32 };
 
 PI_HD void pi_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4])
 {
@@ -117,6 +120,76 @@ PI_HD double pi_exp_f64(double x)
 }
 
 PI_HD float pi_expf(float x) { return (float)pi_exp_f64((double)x); }
+
+/* Natural logarithm of a finite x > 0 for the initial-state draws: x = m 2^k
+ * with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m-1)/(m+1), odd
+ * series to s^21 (|s| < 0.172: truncation below 1e-17).  Plain binary64
+ * operations in a fixed order, so host and device (and the numpy mirror in
+ * stream.py) give the same bits; within 2 ulp of the true value. */
+PI_HD double pi_log_f64(double x)
+{
+    int k;
+    double m = frexp(x, &k);
+    if (m < 0.70710678118654752440) { m = m + m; k = k - 1; }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double p = 1.0 / 21.0;
+    p = p * z + 1.0 / 19.0;
+    p = p * z + 1.0 / 17.0;
+    p = p * z + 1.0 / 15.0;
+    p = p * z + 1.0 / 13.0;
+    p = p * z + 1.0 / 11.0;
+    p = p * z + 1.0 / 9.0;
+    p = p * z + 1.0 / 7.0;
+    p = p * z + 1.0 / 5.0;
+    p = p * z + 1.0 / 3.0;
+    p = p * z + 1.0;
+    const double kd = (double)k;
+    const double ln2_hi = 6.93147180369123816490e-01;   /* k * ln2_hi exact */
+    const double ln2_lo = 1.90821492927058770002e-10;
+    return kd * ln2_hi + (kd * ln2_lo + (s + s) * p);
+}
+
+/* cos(2 pi b) for b in [0, 1): quadrant q = rint(4b), r = 4b - q exact,
+ * x = r pi/2 in [-pi/4, pi/4], Taylor series of cos and sin to degree 16/17. */
+PI_HD double pi_cos2pi_f64(double b)
+{
+    const double t = 4.0 * b;
+    const double q = rint(t);
+    const double r = t - q;
+    const double x = r * 1.57079632679489661923;
+    const double z = x * x;
+    double c = 1.0 / 20922789888000.0;
+    c = c * z - 1.0 / 87178291200.0;
+    c = c * z + 1.0 / 479001600.0;
+    c = c * z - 1.0 / 3628800.0;
+    c = c * z + 1.0 / 40320.0;
+    c = c * z - 1.0 / 720.0;
+    c = c * z + 1.0 / 24.0;
+    c = c * z - 0.5;
+    c = c * z + 1.0;
+    double sn = 1.0 / 355687428096000.0;
+    sn = sn * z - 1.0 / 1307674368000.0;
+    sn = sn * z + 1.0 / 6227020800.0;
+    sn = sn * z - 1.0 / 39916800.0;
+    sn = sn * z + 1.0 / 362880.0;
+    sn = sn * z - 1.0 / 5040.0;
+    sn = sn * z + 1.0 / 120.0;
+    sn = sn * z - 1.0 / 6.0;
+    sn = sn * z + 1.0;
+    sn = x * sn;
+    const int iq = ((int)q) & 3;
+    return iq == 0 ? c : iq == 1 ? -sn : iq == 2 ? -c : sn;
+}
+
+/* The Box-Muller normal of the initial cell size from two stream uniforms
+ * a, b in [0, 1): sqrt(-2 log(1 - a)) cos(2 pi b), rounded to Float32. */
+PI_HD float pi_init_normal(float a, float b)
+{
+    const double a64 = 1.0 - (double)a;
+    return (float)(sqrt(-2.0 * pi_log_f64(a64)) * pi_cos2pi_f64((double)b));
+}
 
 /* Julia Float32 `x ^ 2f0` and `x ^ 4f0` (integer-valued exponent branch). */
 PI_HD float pi_pow2f(float x) { double d = (double)x; return (float)(d * d); }

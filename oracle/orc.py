@@ -55,6 +55,14 @@ def load():
         lib.orc_expf.argtypes = [C.c_float]
         lib.orc_exp_f64.restype = C.c_double
         lib.orc_exp_f64.argtypes = [C.c_double]
+        dp = C.POINTER(C.c_double)
+        fp = C.POINTER(C.c_float)
+        lib.orc_log_f64_v.restype = None
+        lib.orc_log_f64_v.argtypes = [dp, dp, C.c_int64]
+        lib.orc_cos2pi_f64_v.restype = None
+        lib.orc_cos2pi_f64_v.argtypes = [dp, dp, C.c_int64]
+        lib.orc_init_normal_v.restype = None
+        lib.orc_init_normal_v.argtypes = [fp, fp, fp, C.c_int64]
         lib.orc_uniform3.restype = None
         lib.orc_uniform3.argtypes = [C.c_uint64, C.c_int64, C.c_int32, C.c_int32, C.c_int64, C.POINTER(C.c_float)]
         lib.orc_philox.restype = None

@@ -1053,6 +1053,9 @@ pi_status orc_fill_halo(int32_t Nx, int32_t Ny, int32_t Nz, int32_t Hx, int32_t 
 }
 float orc_expf(float x) { return pi_expf(x); }
 double orc_exp_f64(double x) { return pi_exp_f64(x); }
+void orc_log_f64_v(const double* x, double* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = pi_log_f64(x[i]); }
+void orc_cos2pi_f64_v(const double* x, double* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = pi_cos2pi_f64(x[i]); }
+void orc_init_normal_v(const float* a, const float* b, float* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = pi_init_normal(a[i], b[i]); }
 void orc_uniform3(uint64_t seed, int64_t it, int32_t s, int32_t kind, int64_t slot, float* o3) {
     pi_uniform3(seed, it, s, kind, slot, &o3[0], &o3[1], &o3[2]);
 }

@@ -47,6 +47,9 @@ pi_status orc_fill_halo(int32_t Nx, int32_t Ny, int32_t Nz, int32_t Hx, int32_t 
                         int32_t tx, int32_t ty, int32_t tz, float* c);
 float orc_expf(float x);
 double orc_exp_f64(double x);
+void orc_log_f64_v(const double* x, double* y, int64_t n);
+void orc_cos2pi_f64_v(const double* x, double* y, int64_t n);
+void orc_init_normal_v(const float* a, const float* b, float* y, int64_t n);
 void orc_uniform3(uint64_t seed, int64_t iteration, int32_t sp, int32_t kind, int64_t slot, float* out3);
 void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 float orc_tempfunc(const float* params, float T);

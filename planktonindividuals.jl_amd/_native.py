@@ -56,6 +56,9 @@ def load_library():
                                     C.POINTER(C.c_float), C.POINTER(C.c_int32), C.c_int32]
     lib.pi_probe_copy.restype = C.c_int
     lib.pi_probe_copy.argtypes = [C.c_int64]
+    lib.pi_generate_state.restype = C.c_int
+    lib.pi_generate_state.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_float),
+                                      C.POINTER(C.c_float), C.POINTER(C.c_int32)]
     lib.pi_footprint.restype = C.c_int
     lib.pi_footprint.argtypes = [C.POINTER(_abi.ModelDesc), C.POINTER(C.c_int64)]
     lib.pi_model_footprint.restype = C.c_int
@@ -223,6 +226,16 @@ class NativeModel(Handle):
         del keep
         super().__init__(lib, "pi_", raw.value, grid, nsp, cap, diag_tracers)
         self.batch = None
+
+    def generate_state(self, N, c0, lower, n):
+        """pi_generate_state: the initial state drawn on the device."""
+        Na = np.ascontiguousarray(np.array(N, dtype=np.int64))
+        assert len(Na) == self.nsp
+        c0 = np.ascontiguousarray(c0, dtype=np.float32)
+        lower = np.ascontiguousarray(lower, dtype=np.float32)
+        n = np.ascontiguousarray(n, dtype=np.int32)
+        self._call("generate_state", Na.ctypes.data_as(C.POINTER(C.c_int64)), c0.ctypes.data_as(_abi.fp),
+                   lower.ctypes.data_as(_abi.fp), n.ctypes.data_as(C.POINTER(C.c_int32)))
 
     def footprint(self):
         v = C.c_int64(0)

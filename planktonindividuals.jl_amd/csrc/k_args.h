@@ -183,6 +183,31 @@ long long sweep_y_items(const DevGrid& g);
 long long final_items(const DevGrid& g);
 uint32_t scan_tiles(uint32_t n);
 
+/* initial state generated on the device (initialize_plankton!,
+ * QuotaMode/plankton_generation.jl:35-72, and generate_tracers,
+ * tracer_fields.jl:93-134, with the shared-stream draws) */
+struct InitArgs {
+    DevGrid g;
+    const DevState* st;
+    SlotTable tab;
+    const float* params;
+    long long cap;
+    int nsp;
+    long long N[PI_MAXSP];
+    uint64_t seed;
+};
+struct TracerInitArgs {
+    DevGrid g;
+    float* slab;
+    float c0[PI_NTRACERS];
+    float lower[PI_NTRACERS];
+    int32_t n[PI_NTRACERS];
+    float step;
+    uint64_t seed;
+};
+void launch_init_individuals(hipStream_t s, const InitArgs& a);
+void launch_init_tracers(hipStream_t s, const TracerInitArgs& a);
+
 /* k_util.hip: kernels outside the step (uploads, downloads, measurement) */
 void launch_zero(hipStream_t s, uint32_t* p, long long n, const DevState* st);
 void launch_store_to_table(hipStream_t s, DevState* st, const SlotTable& tab, long long cap, long long maxn);

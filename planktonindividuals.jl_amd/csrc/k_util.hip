@@ -293,6 +293,79 @@ __global__ void k_count(const DevState* st, int sp, unsigned long long* out)
     if (c) atomicAdd(out, c);
 }
 
+/* -------------------------------------------------- initial state on device
+ * initialize_plankton! (QuotaMode/plankton_generation.jl:35-72) for every slot
+ * of every species, written into the slot table like an upload of the host
+ * arrays (ModelSpec in model.py computes the same values on the host with the
+ * same stream and the same binary64 log/cos). */
+__global__ void __launch_bounds__(256) k_init_individuals(InitArgs a)
+{
+    const DevGrid& g = a.g;
+    const long long total = a.cap * a.nsp;
+    const int32_t death = (int32_t)a.st->iteration;
+    for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < total; r += (long long)gridDim.x * blockDim.x) {
+        const int sp = (int)(r / a.cap);
+        const long long i = r - (long long)sp * a.cap;
+        const float* q = a.params + sp * PI_NPARAMS;
+        float ux, uy, uz, uc, un, up, ua, ub, u3;
+        pi_uniform3(a.seed, 0, sp, PI_KIND_INIT_POS, (uint64_t)i, &ux, &uy, &uz);
+        pi_uniform3(a.seed, 0, sp, PI_KIND_INIT_QUOTA, (uint64_t)i, &uc, &un, &up);
+        pi_uniform3(a.seed, 0, sp, PI_KIND_INIT_SIZE, (uint64_t)i, &ua, &ub, &u3);
+        const float Sz0 = pi_init_normal(ua, ub);
+        bool ac = i < a.N[sp];
+        const float x = jl_maskf(ac, ux * (float)g.Nx);
+        const float y = jl_maskf(ac, uy * (float)g.Ny);
+        const float z = jl_maskf(ac, uz * (float)g.Nz);
+        const float Sz = jl_maskf(ac, jl_maxf(1.0f, Sz0 * q[PI_P_var] + q[PI_P_mean]));
+        const float Bm = (q[PI_P_Cquota] * Sz) * q[PI_P_Nsuper];
+        const float Cq = (uc * q[PI_P_Cqmax]) * Bm;
+        const float Nq = un * (q[PI_P_Nqmax] * (Bm + Cq) - Bm * q[PI_P_R_NC]);
+        const float Pq = up * (q[PI_P_Pqmax] * (Bm + Cq) - Bm * q[PI_P_R_PC]);
+        const float Chl = Bm * q[PI_P_Chl2Cint];
+        if (ac && g.has_land) {   /* mask_individuals! (Individuals/utils.jl:89-100) */
+            const int xi = (int)truncf(x + 1.0f) + g.Hx, yi = (int)truncf(y + 1.0f) + g.Hy, zi = (int)truncf(z + 1.0f) + g.Hz;
+            ac = g.lm[gidx(g, xi, yi, zi)] != 0;
+        }
+        a.tab.x[r] = x; a.tab.y[r] = y; a.tab.z[r] = z;
+        a.tab.Sz[r] = Sz; a.tab.Bm[r] = Bm; a.tab.Cq[r] = Cq; a.tab.Nq[r] = Nq; a.tab.Pq[r] = Pq;
+        a.tab.Chl[r] = Chl;
+        a.tab.gen[r] = i < a.N[sp] ? 1.0f : 0.0f;
+        a.tab.age[r] = 0.0f;
+        a.tab.flags[r] = (uint8_t)((ac ? F_AC : 0) | F_ZERO);
+        a.tab.cell[r] = 0u;
+        a.tab.death[r] = death;
+    }
+}
+
+/* generate_tracers with the values source: c0 · vals[pick] per padded cell,
+ * vals = lower : step : upper, pick from the tracer's sub-stream; then the
+ * landmask (Bool mask keeps the sign of zero) */
+__global__ void __launch_bounds__(256) k_init_tracers(TracerInitArgs a)
+{
+    const DevGrid& g = a.g;
+    const long long n = (long long)g.ncell * PI_NTRACERS;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const int q = (int)(t / g.ncell);
+        const long long cell = t - (long long)q * g.ncell;
+        float u, u1, u2;
+        pi_uniform3(a.seed, 0, 0, PI_KIND_TRACER + q, (uint64_t)cell, &u, &u1, &u2);
+        long long pick = (long long)((double)u * (double)a.n[q]);
+        if (pick > a.n[q] - 1) pick = a.n[q] - 1;
+        const float v = (float)((double)a.lower[q] + (double)pick * (double)a.step);
+        const float c = a.c0[q] * v;
+        a.slab[t] = (g.has_land && !g.lm[cell]) ? copysignf(0.0f, c) : c;
+    }
+}
+
+void launch_init_individuals(hipStream_t s, const InitArgs& a)
+{
+    hipLaunchKernelGGL(k_init_individuals, dim3(grid_for(a.cap * a.nsp, 256, 8192)), dim3(256), 0, s, a);
+}
+void launch_init_tracers(hipStream_t s, const TracerInitArgs& a)
+{
+    hipLaunchKernelGGL(k_init_tracers, dim3(grid_for((long long)a.g.ncell * PI_NTRACERS, 256, 8192)), dim3(256), 0, s, a);
+}
+
 /* ------------------------------------------------------------- registry */
 uint32_t scan_tiles(uint32_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 

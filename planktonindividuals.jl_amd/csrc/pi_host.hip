@@ -54,6 +54,14 @@ struct Plan {
 };
 template <class A> static A zeroed() { A a; std::memset((void*)&a, 0, sizeof(A)); return a; }
 
+static KInfo g_kinfo[K_NKID];
+static const KInfo* kinfo()
+{
+    static bool done = false;
+    if (!done) { reg_particles(g_kinfo); reg_tracers(g_kinfo); reg_util(g_kinfo); done = true; }
+    return g_kinfo;
+}
+
 struct KTimer { const char* name; hipEvent_t a, b; };
 
 struct pi_model {
@@ -514,6 +522,48 @@ pi_status pi_upload_individuals(pi_model* m, int32_t sp, const pi_individuals_ho
     return PI_OK;
 }
 
+/* The initial state generated on the device: every species' slots as
+ * initialize_plankton! (QuotaMode/plankton_generation.jl:35-72) and the 14
+ * tracers as generate_tracers with the (c0, rand_noise) source
+ * (tracer_fields.jl:93-134) followed by fill_halo_tracer!, all drawn from the
+ * shared stream: the values the host generator (ModelSpec) uploads. */
+pi_status pi_generate_state(pi_model* m, const int64_t* n_individuals, const float* tracer_c0,
+                            const float* tracer_lower, const int32_t* tracer_n)
+{
+    if (!m || !n_individuals || !tracer_c0 || !tracer_lower || !tracer_n) return fail(PI_ERR_ARG, "null argument");
+    for (int s = 0; s < m->nsp; ++s)
+        if (n_individuals[s] < 0 || n_individuals[s] > m->cap) return fail(PI_ERR_ARG, "individuals exceed the capacity");
+    for (int q = 0; q < PI_NTRACERS; ++q)
+        if (tracer_n[q] < 1 || !(tracer_c0[q] >= 0.0f)) return fail(PI_ERR_ARG, "tracer initial condition");
+    pi_status st = rebuild_if_dirty(m);
+    if (st != PI_OK) return st;
+    st = sync_state(m);
+    if (st != PI_OK) return st;
+    InitArgs a{};
+    a.g = m->g; a.st = m->d_state; a.tab = m->tab; a.params = m->d_params; a.cap = m->cap; a.nsp = m->nsp;
+    for (int s = 0; s < m->nsp; ++s) a.N[s] = n_individuals[s];
+    a.seed = m->seed;
+    launch_init_individuals(m->stream, a);
+    TracerInitArgs t{};
+    t.g = m->g; t.slab = m->h_state.tr[m->h_state.tcur][0]; t.seed = m->seed; t.step = 1.0e-4f;
+    for (int q = 0; q < PI_NTRACERS; ++q) { t.c0[q] = tracer_c0[q]; t.lower[q] = tracer_lower[q]; t.n[q] = tracer_n[q]; }
+    launch_init_tracers(m->stream, t);
+    const KInfo* K = kinfo();
+    const DevGrid& g = m->g;
+    FillArgs fa = zeroed<FillArgs>();
+    fa.g = g; fa.st = m->d_state; fa.f = t.slab; fa.nf = PI_NTRACERS; fa.lines_full = 1;
+    const long long lines[3] = {(long long)g.Py * g.Pz, (long long)g.Px * g.Pz, (long long)g.Px * g.Py};
+    for (int d = 0; d < 3; ++d) {
+        fa.dim = d;
+        void* argp = &fa;
+        HIPCHECK(hipLaunchKernel(K[K_FILL].one, dim3(blocks_for(lines[d] * PI_NTRACERS)), dim3(256), &argp, 0, m->stream));
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    m->dirty = true;
+    return PI_OK;
+}
+
 pi_status pi_upload_tracer(pi_model* m, int32_t q, const float* c)
 {
     if (!m || !c || q < 0 || q >= PI_NTRACERS) return fail(PI_ERR_ARG, "bad tracer upload");
@@ -576,13 +626,6 @@ pi_status pi_set_temperature(pi_model* m, const float* t)
 }
 
 /* ------------------------------------------------------------------ step */
-static KInfo g_kinfo[K_NKID];
-static const KInfo* kinfo()
-{
-    static bool done = false;
-    if (!done) { reg_particles(g_kinfo); reg_tracers(g_kinfo); reg_util(g_kinfo); done = true; }
-    return g_kinfo;
-}
 
 static void plan_scan(Plan& P, const char* grp, const ScanIn& in, uint32_t* partials, uint32_t* out)
 {

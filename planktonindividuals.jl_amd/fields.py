@@ -44,6 +44,29 @@ def _range_values(lower, upper, step=np.float32(1.0e-4)):
     return (np.float64(lower) + np.arange(n, dtype=np.float64) * np.float64(step)).astype(np.float32)
 
 
+def tracer_ranges(source):
+    """(c0, lower, n) per tracer of the values source: tracer q starts at
+    c0[q] * (lower[q] + k * 1f-4) with k < n[q] drawn per padded cell (the ranges
+    generate_tracers draws from; pi_generate_state draws the same on the device)."""
+    if not (isinstance(source, dict) and "initial_condition" in source):
+        raise ValueError("generating the state on the device needs the (initial_condition, rand_noise) source")
+    c0 = np.zeros(len(TRACER_NAMES), np.float32)
+    lower = np.zeros(len(TRACER_NAMES), np.float32)
+    n = np.zeros(len(TRACER_NAMES), np.int32)
+    for q, name in enumerate(TRACER_NAMES):
+        ic = source["initial_condition"]
+        if name not in ic:
+            raise ValueError(f"TRAC_INIT: tracer not found {name}")
+        if ic[name] < 0.0:
+            raise ValueError("TRAC_INIT:  The initial condition should be none-negetive.")
+        eps = source["rand_noise"][name]
+        lo, up = np.float32(1.0 - eps), np.float32(1.0 + eps)
+        c0[q] = np.float32(ic[name])
+        lower[q] = lo
+        n[q] = len(_range_values(lo, up))
+    return c0, lower, n
+
+
 def generate_tracers(grid, source, seed):
     """generate_tracers(arch, grid, source, FT) — tracer_fields.jl:93-134.
 

@@ -4,7 +4,7 @@ src/Individuals/Plankton/QuotaMode/plankton_generation.jl)."""
 import numpy as np
 
 from . import _abi, params as P, stream
-from .fields import default_tracer_init, generate_tracers
+from .fields import default_tracer_init, generate_tracers, tracer_ranges
 from ._native import NativeModel
 
 
@@ -61,9 +61,9 @@ def initialize_plankton(p, N, cap, grid, seed, sp):
     ux, uy, uz = stream.uniform3(seed, 0, sp, stream.KIND_INIT_POS, slots)
     uc, un, up = stream.uniform3(seed, 0, sp, stream.KIND_INIT_QUOTA, slots)
     a, b, _ = stream.uniform3(seed, 0, sp, stream.KIND_INIT_SIZE, slots)
-    # randn via Box-Muller on (0,1] x [0,1)
-    a64 = 1.0 - a.astype(np.float64)
-    Sz = (np.sqrt(-2.0 * np.log(a64)) * np.cos(2.0 * np.pi * b.astype(np.float64))).astype(np.float32)
+    # randn via Box-Muller on (0,1] x [0,1), with the binary64 log and cos of
+    # include/pi_math.h (the device generates the same values)
+    Sz = stream.init_normal(a, b)
     ac = np.zeros(cap, dtype=bool)
     ac[:N] = True
     gen = np.zeros(cap, f32)
@@ -170,7 +170,7 @@ class ModelSpec:
 
     def __init__(self, grid, *, FT=np.float32, mode=None, max_individuals=8 * 1024,
                  bgc_params=None, tracer_initial=None, phyto=None, abiotic=None, t=0.0,
-                 max_candidates=25, seed=0):
+                 max_candidates=25, seed=0, host_state=True):
         mode = mode if mode is not None else P.QuotaMode()
         if not isinstance(mode, P.QuotaMode):
             raise NotImplementedError(f"{mode}: only QuotaMode is implemented on the MI355X hot path")
@@ -210,11 +210,19 @@ class ModelSpec:
         self.bgc_vector = P.bgc_vector(bgc)
         self.seed = int(seed)
         self.t = np.float32(t)
-        self.initial_individuals = [initialize_plankton(self.species_params[s], int(phyto.N[s]),
-                                                        self.max_individuals, grid, self.seed, s)
-                                    for s in range(self.n_species)]
-        self.initial_tracers = generate_tracers(grid, tracer_initial if tracer_initial is not None
-                                                else default_tracer_init(), self.seed)
+        source = tracer_initial if tracer_initial is not None else default_tracer_init()
+        # host_state=False: the device draws the same initial state itself
+        # (pi_generate_state); only the tracer value ranges are kept
+        self.host_state = bool(host_state)
+        self.initial_individuals = self.initial_tracers = None
+        self.tracer_ranges = None
+        if self.host_state:
+            self.initial_individuals = [initialize_plankton(self.species_params[s], int(phyto.N[s]),
+                                                            self.max_individuals, grid, self.seed, s)
+                                        for s in range(self.n_species)]
+            self.initial_tracers = generate_tracers(grid, source, self.seed)
+        else:
+            self.tracer_ranges = tracer_ranges(source)
 
 
 class PlanktonModel:
@@ -242,10 +250,13 @@ class PlanktonModel:
         self.t = spec.t
         self._native = NativeModel(self.grid, self.n_species, self.max_individuals, self.species_params,
                                    spec.bgc_vector, spec.seed, (), device=arch.device)
-        for s in range(self.n_species):
-            self._native.upload_individuals(s, spec.initial_individuals[s])
-        for tid, c in enumerate(spec.initial_tracers):
-            self._native.upload_tracer(tid, c)
+        if spec.host_state:
+            for s in range(self.n_species):
+                self._native.upload_individuals(s, spec.initial_individuals[s])
+            for tid, c in enumerate(spec.initial_tracers):
+                self._native.upload_tracer(tid, c)
+        else:
+            self._native.generate_state(spec.N, *spec.tracer_ranges)
         self.individuals = _Individuals(self)
         self.tracers = _Tracers({n: _Field(self, i) for i, n in enumerate(_abi.TRACER_NAMES)})
         self.timestepper = _Timestepper(self)

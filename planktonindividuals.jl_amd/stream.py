@@ -71,3 +71,54 @@ def _uniform3(seed, iteration, species, kind, slots):
     o0, o1, o2, _ = philox4x32_10((slots & _MASK).astype(np.uint32), np.uint32(iteration & 0xFFFFFFFF),
                                   np.uint32(species), np.uint32(kind), k0, k1)
     return to_unit(o0), to_unit(o1), to_unit(o2)
+
+
+# ------------------------------------------------- initial-state transcendentals
+# Element-wise mirrors of pi_log_f64 / pi_cos2pi_f64 / pi_init_normal
+# (include/pi_math.h): the same binary64 operations in the same order, so the
+# host-generated initial state equals the one generated on the device.
+
+def log_f64(x):
+    x = np.asarray(x, dtype=np.float64)
+    m, k = np.frexp(x)
+    lo = m < 0.70710678118654752440
+    m = np.where(lo, m + m, m)
+    k = np.where(lo, k - 1, k)
+    f = m - 1.0
+    s = f / (2.0 + f)
+    z = s * s
+    p = np.full_like(s, 1.0 / 21.0)
+    for d in (19.0, 17.0, 15.0, 13.0, 11.0, 9.0, 7.0, 5.0, 3.0):
+        p = p * z + 1.0 / d
+    p = p * z + 1.0
+    kd = k.astype(np.float64)
+    return kd * 6.93147180369123816490e-01 + (kd * 1.90821492927058770002e-10 + (s + s) * p)
+
+
+def cos2pi_f64(b):
+    b = np.asarray(b, dtype=np.float64)
+    t = 4.0 * b
+    q = np.rint(t)
+    r = t - q
+    x = r * 1.57079632679489661923
+    z = x * x
+    c = np.full_like(z, 1.0 / 20922789888000.0)
+    for v in (-1.0 / 87178291200.0, 1.0 / 479001600.0, -1.0 / 3628800.0, 1.0 / 40320.0, -1.0 / 720.0,
+              1.0 / 24.0):
+        c = c * z - (-v) if v < 0 else c * z + v
+    c = c * z - 0.5
+    c = c * z + 1.0
+    sn = np.full_like(z, 1.0 / 355687428096000.0)
+    for v in (-1.0 / 1307674368000.0, 1.0 / 6227020800.0, -1.0 / 39916800.0, 1.0 / 362880.0, -1.0 / 5040.0,
+              1.0 / 120.0, -1.0 / 6.0):
+        sn = sn * z - (-v) if v < 0 else sn * z + v
+    sn = sn * z + 1.0
+    sn = x * sn
+    iq = q.astype(np.int64) & 3
+    return np.where(iq == 0, c, np.where(iq == 1, -sn, np.where(iq == 2, -c, sn)))
+
+
+def init_normal(a, b):
+    """sqrt(-2 log(1 - a)) cos(2 pi b) in binary64, rounded to Float32."""
+    a64 = 1.0 - np.asarray(a, dtype=np.float32).astype(np.float64)
+    return (np.sqrt(-2.0 * log_f64(a64)) * cos2pi_f64(np.asarray(b, dtype=np.float32).astype(np.float64))).astype(np.float32)

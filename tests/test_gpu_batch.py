@@ -132,3 +132,28 @@ def test_ensemble_update_matches_update(pi):
         bad = compare_models(na, nb, a.model.n_species)
         assert not bad, f"member {k}: " + "\n".join(bad[:20])
     ens.close()
+
+
+@pytest.mark.parametrize("land", [False, True])
+def test_device_generated_state_matches_host(pi, land):
+    """pi_generate_state (the initial state drawn on the device) against the
+    host-generated arrays of ModelSpec uploaded: identical before and after
+    stepping (two species, N < capacity, periodic and bounded grids, land)."""
+    if land:
+        lm = np.ones((10, 9, 3), dtype=bool)
+        lm[2:5, 3:6, 1:3] = False
+        g = pi.RectilinearGrid(size=(10, 9, 3), x=(0, 10e3), y=(0, 9e3), z=(0, -30),
+                               topology=(pi.Bounded, pi.Periodic, pi.Bounded), landmask=lm)
+    else:
+        g = pi.RectilinearGrid(size=(14, 12, 5), x=(0, 14e3), y=(0, 12e3), z=(0, -50))
+    kw = dict(max_individuals=3000, seed=77, phyto=pi.phyto_setup({"dvid_P": [2e-3, 1e-3]}, [1800, 900], 2))
+    A = pi.PlanktonModel(pi.HIPArch(), spec=pi.ModelSpec(g, **kw))._native
+    B = pi.PlanktonModel(pi.HIPArch(), spec=pi.ModelSpec(g, host_state=False, **kw))._native
+    bad = compare_models(A, B, 2, diags=False)
+    assert not bad, "\n".join(bad[:20])
+    for h in (A, B):
+        _inputs(pi, h, pi.ModelSpec(g, **dict(kw, phyto=pi.phyto_setup(None, [1, 1], 2))), None)
+        h.run(60.0, 3)
+        h.sync()
+    bad = compare_models(A, B, 2)
+    assert not bad, "\n".join(bad[:20])

@@ -141,6 +141,38 @@ def test_exp_f64_accuracy(orc):
         assert abs(e - r) <= 2 * math.ulp(r) + 1e-320, (x, e, r)
 
 
+def test_init_transcendentals_host_mirror_and_accuracy(pi, orc):
+    """pi_log_f64 / pi_cos2pi_f64 / pi_init_normal (include/pi_math.h, compiled
+    by the oracle's gcc) against their numpy mirrors: bit-identical; and against
+    libm: log within 2 ulp, cos(2 pi b) within 2e-15 absolute (the libm
+    reference itself carries the rounding of 2 pi b); the normal has
+    mean 0 and variance 1."""
+    import ctypes as C
+    from importlib import import_module
+    stream = import_module(pi.__name__ + ".stream")
+    lib = orc.load()
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(0, 1, 50000), np.exp(rng.uniform(-700, 700, 50000)),
+                        1.0 - np.arange(1, 2000) * 2.0 ** -24, [1.0, 0.5, 2.0, 5e-324, 1e308]])
+    y = np.empty_like(x)
+    lib.orc_log_f64_v(x.ctypes.data_as(C.POINTER(C.c_double)), y.ctypes.data_as(C.POINTER(C.c_double)), len(x))
+    assert np.array_equal(y, stream.log_f64(x))
+    ref = np.log(x)
+    assert np.all(np.abs(y - ref) <= 2 * np.spacing(np.abs(ref)) + 1e-300)
+    b = np.concatenate([rng.uniform(0, 1, 100000), np.arange(0, 4096) / 4096.0]).astype(np.float32).astype(np.float64)
+    c = np.empty_like(b)
+    lib.orc_cos2pi_f64_v(b.ctypes.data_as(C.POINTER(C.c_double)), c.ctypes.data_as(C.POINTER(C.c_double)), len(b))
+    assert np.array_equal(c, stream.cos2pi_f64(b))
+    assert np.max(np.abs(c - np.cos(2.0 * np.pi * b))) < 2e-15
+    ua = rng.random(200000, dtype=np.float32)
+    ub = rng.random(200000, dtype=np.float32)
+    z = np.empty(200000, np.float32)
+    fp = C.POINTER(C.c_float)
+    lib.orc_init_normal_v(ua.ctypes.data_as(fp), ub.ctypes.data_as(fp), z.ctypes.data_as(fp), len(z))
+    assert np.array_equal(z, stream.init_normal(ua, ub))
+    assert abs(float(z.mean())) < 0.01 and abs(float(z.var()) - 1.0) < 0.01
+
+
 def test_expf_correctly_rounded_sample(orc):
     lib = orc.load()
     rng = np.random.default_rng(1)
