@@ -215,6 +215,10 @@ pi_status pi_batch_create(pi_batch** out);
 pi_status pi_batch_add(pi_batch* b, pi_model* m);
 pi_status pi_batch_remove(pi_batch* b, pi_model* m);
 pi_status pi_batch_run(pi_batch* b, float dt, int64_t n);
+/* Steady steps replay a captured graph unless this is off: turn it off when
+ * other threads make HIP calls (model creation, downloads) while the batch
+ * runs, since a capture forbids them. */
+pi_status pi_batch_graphs(pi_batch* b, int32_t on);
 pi_status pi_batch_sync(pi_batch* b);
 pi_status pi_batch_size(const pi_batch* b, int32_t* n);
 pi_status pi_batch_destroy(pi_batch* b);

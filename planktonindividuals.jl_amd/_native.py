@@ -66,6 +66,7 @@ def load_library():
     for f, a in (("create", [C.POINTER(C.c_void_p)]), ("add", [C.c_void_p, C.c_void_p]),
                  ("remove", [C.c_void_p, C.c_void_p]), ("run", [C.c_void_p, C.c_float, C.c_int64]),
                  ("sync", [C.c_void_p]), ("size", [C.c_void_p, C.POINTER(C.c_int32)]),
+                 ("graphs", [C.c_void_p, C.c_int32]),
                  ("destroy", [C.c_void_p])):
         fn = getattr(lib, "pi_batch_" + f)
         fn.restype = C.c_int
@@ -299,6 +300,9 @@ class NativeBatch:
 
     def sync(self):
         check(self.lib, "pi_", self.lib.pi_batch_sync(self.raw))
+
+    def graphs(self, on):
+        check(self.lib, "pi_", self.lib.pi_batch_graphs(self.raw, 1 if on else 0))
 
     def __len__(self):
         return len(self.members)

@@ -133,7 +133,8 @@ static hipError_t dalloc(pi_model* m, void** p, size_t bytes)
     hipError_t e = hipMalloc(p, bytes ? bytes : 4);
     if (e == hipSuccess) {
         m->allocs.push_back(*p);
-        e = hipMemset(*p, 0, bytes ? bytes : 4);
+        e = hipMemsetAsync(*p, 0, bytes ? bytes : 4, m->stream);   /* on the model's stream: no null-stream ordering */
+        if (e == hipSuccess) e = hipStreamSynchronize(m->stream);  /* before any copy into it */
     }
     return e;
 }
@@ -345,8 +346,6 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     DALLOC(m->segp, (nc + 1) * 4); DALLOC(m->small, 2 * m->ctot * 4);
     DALLOC(m->cnts, nc * 4); DALLOC(m->mix, nc * 4);
     DALLOC(m->sflag, m->ctot);
-    DEVCALL(hipMemset(m->cnts, 0, nc * 4));
-    DEVCALL(hipMemset(m->sflag, 0, m->ctot));
     DALLOC(m->dstamp, m->ctot * 4); DALLOC(m->rstamp, m->ctot * 4);
     DALLOC(m->dcell, m->ctot * 4); DALLOC(m->rcons, m->ctot * 5 * 4);
     if (!m->dry) {   /* stamps start below any iteration */
@@ -1063,6 +1062,7 @@ struct pi_batch {
     std::vector<size_t> order;    /* members by decreasing capacity per cell */
     hipGraphExec_t gexec = nullptr;
     bool tables = false;
+    bool graphs = true;           /* replay captured steps (off when other threads use the device) */
     long long budget = 16384;     /* blocks per batched launch of a block-stride kernel */
 };
 
@@ -1249,7 +1249,9 @@ pi_status pi_batch_run(pi_batch* b, float dt, int64_t n)
             batch_drop_graph(b);
             pi_status s = batch_tables(b);
             if (s != PI_OK) return s;
-            s = batch_launch(b);                   /* a changed step runs without a graph */
+        }
+        if (!same || !b->graphs) {                 /* a changed step runs without a graph */
+            pi_status s = batch_launch(b);
             if (s != PI_OK) return s;
             continue;
         }
@@ -1266,6 +1268,14 @@ pi_status pi_batch_run(pi_batch* b, float dt, int64_t n)
         }
         HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
     }
+    return PI_OK;
+}
+
+pi_status pi_batch_graphs(pi_batch* b, int32_t on)
+{
+    if (!b) return fail(PI_ERR_ARG, "null batch");
+    b->graphs = on != 0;
+    if (!b->graphs) batch_drop_graph(b);
     return PI_OK;
 }
 

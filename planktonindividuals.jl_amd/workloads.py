@@ -106,3 +106,51 @@ def config_3d(n=256, nz=64, N=4194304, nsp=2, cap=None, seed=2, nmodes=64, overr
     spec = ModelSpec(grid, max_individuals=cap or 2 * N, phyto=phyto_setup(params, [N] * nsp, nsp),
                      bgc_params=bgc, seed=seed)
     return spec, fourier_flow(grid, nmodes=nmodes, rms=0.05, seed=seed)
+
+
+# ------------------------------------------------------------- ensembles
+# The ensemble workloads: members drawn from one seed, each a 0-D box (1x1x1,
+# 1k-64k individuals), a 1-D column (1x1x64, 8k-256k) or a 2-D slab
+# (64-256 squared, 16k-1M), one species, with VNH4max, VNO3max, VPO4max and
+# dvid_P scaled by independent U[0.8, 1.2] factors.
+PERTURBED = ("VNH4max", "VNO3max", "VPO4max", "dvid_P")
+
+
+def ensemble_table(n=256, seed=4, lengths=None):
+    """Member table: dicts with kind (0 = box, 1 = column, 2 = slab), size,
+    N, parameter overrides, member seed and (when `lengths` = (lo, hi)) a
+    number of steps drawn uniformly from [lo, hi]."""
+    from .params import phyt_params_default, QuotaMode
+    rng = np.random.default_rng(seed)
+    base = phyt_params_default(1, QuotaMode())
+    out = []
+    for i in range(n):
+        kind = int(rng.integers(0, 3))
+        if kind == 0:
+            size, N = (1, 1, 1), int(rng.integers(1000, 64001))
+        elif kind == 1:
+            size, N = (1, 1, 64), int(rng.integers(8000, 256001))
+        else:
+            m = int(rng.integers(64, 257))
+            size, N = (m, m, 1), int(rng.integers(16000, 1000001))
+        f = rng.uniform(0.8, 1.2, len(PERTURBED))
+        over = {k: [float(np.float32(np.float32(base[k][0]) * np.float32(fk)))] for k, fk in zip(PERTURBED, f)}
+        steps = int(rng.integers(lengths[0], lengths[1] + 1)) if lengths else None
+        out.append(dict(id=i, kind=kind, size=size, N=N, params=over, seed=int(seed) * 100003 + i, steps=steps))
+    return out
+
+
+def ensemble_member(entry, host_state=False):
+    """(ModelSpec, velocities or None) of one member: 1 km boxes and columns
+    with 10 m levels and no flow; slabs of 1 km cells under the cellular flow
+    (U = 0.1 m/s, L = 16 km per 128 cells); capacity 2 N."""
+    size, N = entry["size"], entry["N"]
+    if size[0] == 1:
+        grid = RectilinearGrid(size=size, x=(0, 1e3), y=(0, 1e3), z=(0, -10.0 * size[2]))
+        vel = None
+    else:
+        grid = RectilinearGrid(size=size, x=(0, size[0] * 1e3), y=(0, size[1] * 1e3), z=(0, -10.0))
+        vel = cellular_flow(grid, U=0.1, L=16e3 * size[0] / 128)
+    spec = ModelSpec(grid, max_individuals=2 * N, phyto=phyto_setup(entry["params"], [N], 1),
+                     seed=entry["seed"], host_state=host_state)
+    return spec, vel

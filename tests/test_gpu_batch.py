@@ -157,3 +157,49 @@ def test_device_generated_state_matches_host(pi, land):
         h.sync()
     bad = compare_models(A, B, 2)
     assert not bad, "\n".join(bad[:20])
+
+
+def test_served_members_match_alone(pi):
+    """The serving loop (members admitted by footprint while others run, joining
+    and leaving the running batch) against each member run alone through
+    update: identical final state.  The budget admits only part of the stream
+    at a time, so members also wait in the queue."""
+    from importlib import import_module
+    sv = import_module(pi.__name__ + ".serving")
+    table = [dict(id=i, kind=k, size=sz, N=n, params={"dvid_P": [2e-3 * (1 + 0.1 * i)]}, seed=900 + i, steps=st)
+             for i, (k, sz, n, st) in enumerate([(0, (1, 1, 1), 1500, 30), (1, (1, 1, 16), 4000, 45),
+                                                   (2, (24, 24, 1), 3000, 20), (0, (1, 1, 1), 800, 50),
+                                                   (2, (16, 16, 1), 2000, 35), (1, (1, 1, 8), 2500, 25)])]
+    states = {}
+
+    def grab(req, sim):
+        nat = sim.model._native
+        states[req.rid] = ({f: nat.field(0, f) for f in ("x", "y", "z", "Bm", "Cq", "Nq", "Pq", "Chl", "ac")},
+                           [nat.tracer(q) for q in range(14)])
+
+    reqs = [sv.Request(e["id"], (lambda e=e: pi.workloads.ensemble_member(e)), e["steps"]) for e in table]
+    one = sv.member_footprint(pi.workloads.ensemble_member(table[1])[0])
+    srv = sv.EnsembleServer(budget=3 * one, workers=2, on_finish=grab)
+    try:
+        for r in reqs[:3]:
+            srv.submit(r)
+        for _ in range(10):
+            srv.step()
+        for r in reqs[3:]:
+            srv.submit(r)
+        srv.drain()
+    finally:
+        srv.close()
+    assert not [r.error for r in srv.finished if r.error], [r.error for r in srv.finished]
+    assert sorted(states) == list(range(len(table)))
+    for e in table:
+        spec, vel = pi.workloads.ensemble_member(e)
+        sim = sv.simulation_for(spec, vel, e["steps"])
+        pi.update(sim)
+        nat = sim.model._native
+        f, t = states[e["id"]]
+        for k, v in f.items():
+            assert np.array_equal(nat.field(0, k), v), (e["id"], k)
+        for q in range(14):
+            assert np.array_equal(nat.tracer(q), t[q]), (e["id"], q)
+        nat.close()
