@@ -153,10 +153,11 @@ def pmc_summary(fetch_db, write_db, steps):
     return {"scale": scale, "steps_averaged": steps, "kernels": kernels}
 
 
-def cpu_baseline(pi, seconds_hint=20.0):
+def cpu_baseline(pi, seconds_hint=12.0):
     """The oracle (FP32 reference order, one thread) on a bounded sample of the same
     workload: the 128x128x64 quarter of the domain at the same density
-    (2 x 1,048,576 individuals), one step."""
+    (2 x 1,048,576 individuals); one untimed step (every individual sorted from
+    scratch), then as many timed steps as fit in about `seconds_hint` s."""
     sys.path.insert(0, REPO)
     from oracle import orc
     spec, uvw = pi.workloads.config_3d(n=128, nz=64, N=1048576, nsp=2, cap=2097152, seed=2)
@@ -164,14 +165,146 @@ def cpu_baseline(pi, seconds_hint=20.0):
     om = orc.OracleModel(spec)
     parf = np.full((g.Nx, g.Ny), 1823.7921666666664, np.float32)
     temp = pi.default_temperature(g, 60.0, 1)[..., 12].astype(np.float32)
-    n0 = om.count_active(0) + om.count_active(1)
     t0 = time.perf_counter()
     om.update(60.0, 1, vel=uvw, parf=parf, temp=temp)
+    first = time.perf_counter() - t0
+    k = max(1, min(8, int(round(seconds_hint / max(first, 1e-3)))))
+    n0 = om.count_active(0) + om.count_active(1)
+    t0 = time.perf_counter()
+    om.update(60.0, k, vel=uvw, parf=parf, temp=temp)
     dt = time.perf_counter() - t0
-    return {"value": n0 / dt, "unit": "individual-steps/s", "cores": 1, "kind": "port",
-            "sample": "oracle (CPU restatement of the reference CPU() step, FP32, 1 thread): one step of the "
-                      "128x128x64 quarter domain with 2 x 1,048,576 individuals (same density and forcing)",
-            "seconds": dt, "tracer_cell_steps_per_s": g.Nx * g.Ny * g.Nz / dt}
+    n1 = om.count_active(0) + om.count_active(1)
+    return {"value": 0.5 * (n0 + n1) * k / dt, "unit": "individual-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (CPU restatement of the reference CPU() step, FP32, 1 thread): {k} steps of the "
+                      "128x128x64 quarter domain with 2 x 1,048,576 individuals (same density and forcing), "
+                      "after one untimed step",
+            "seconds": dt, "tracer_cell_steps_per_s": g.Nx * g.Ny * g.Nz * k / dt}
+
+
+def ensemble_leg(pi, members=256, steps=20, seq_steps=4, seed=4):
+    """Batched ensemble: `members` drawn from the ensemble table, stepped
+    one-by-one through the plain path (pi_run + sync per member) and then all
+    together (pi_batch_run); ms per ensemble step of each and the ratio."""
+    table = pi.workloads.ensemble_table(members, seed=seed)
+    t0 = time.perf_counter()
+    nats = []
+    for e in table:
+        spec, vel = pi.workloads.ensemble_member(e)
+        g = spec.grid
+        nat = pi.PlanktonModel(pi.HIPArch(0), spec=spec)._native
+        if vel is not None:
+            nat.set_velocity(*vel)
+        nat.set_parf(np.full((g.Nx, g.Ny), 1823.7921666666664, np.float32))
+        nat.set_temperature(pi.default_temperature(g, 60.0, 1)[..., 12].astype(np.float32))
+        nats.append(nat)
+    log(f"[bench] ensemble: {members} members created in {time.perf_counter() - t0:.1f} s")
+    for nat in nats:
+        nat.run(60.0, 2)
+        nat.sync()
+    t0 = time.perf_counter()
+    for nat in nats:
+        nat.run(60.0, seq_steps)
+        nat.sync()
+    t_seq = (time.perf_counter() - t0) / seq_steps
+    batch = pi._native.NativeBatch()
+    for nat in nats:
+        batch.add(nat)
+    batch.run(60.0, 2)
+    batch.sync()
+    live0 = sum(nat.count_active(0) for nat in nats)
+    t0 = time.perf_counter()
+    batch.run(60.0, steps)
+    batch.sync()
+    t_b = (time.perf_counter() - t0) / steps
+    live1 = sum(nat.count_active(0) for nat in nats)
+    live = 0.5 * (live0 + live1)
+    batch.close()
+    for nat in nats:
+        nat.close()
+    kinds = [sum(1 for e in table if e["kind"] == k) for k in range(3)]
+    log(f"[bench] ensemble: one-by-one {t_seq * 1e3:.2f} ms, batched {t_b * 1e3:.2f} ms per ensemble step")
+    return {"workload": f"{members} members (0-D boxes 1x1x1 with 1k-64k individuals, 1-D columns 1x1x64 with "
+                        f"8k-256k, 2-D slabs 64-256 squared with 16k-1M; VNH4max, VNO3max, VPO4max, dvid_P x "
+                        f"U[0.8,1.2]), dt 60 s, seed {seed}",
+            "members": members, "members_by_kind": kinds, "live_individuals": live,
+            "batched_ms_per_step": t_b * 1e3, "one_by_one_ms_per_step": t_seq * 1e3,
+            "speedup_vs_one_by_one": t_seq / t_b,
+            "batched_individual_steps_per_s": live / t_b, "one_by_one_individual_steps_per_s": live / t_seq,
+            "batched_steps": steps, "one_by_one_steps": seq_steps}
+
+
+def serving_leg(pi, rate=50.0, window=6.0, cap_s=150.0, budget_gib=200, seed=5, n_check=3):
+    """Served ensemble stream: requests from the ensemble table (lengths
+    100-2000 steps) arrive as a Poisson process at `rate`/s for `window` s,
+    are admitted by device footprint within `budget_gib` and stepped in one
+    running batch; per-step and end-to-end latency, throughput, and the
+    `n_check` smallest finished members re-run alone and compared bit for bit."""
+    from importlib import import_module
+    sv = import_module(pi.__name__ + ".serving")
+    table = pi.workloads.ensemble_table(4096, seed=seed, lengths=(100, 2000))
+    rng = np.random.default_rng(seed + 6)
+    arr = np.cumsum(rng.exponential(1.0 / rate, 4096))
+    arr = arr[arr < window]
+    make = lambda e: (lambda: pi.workloads.ensemble_member(e, cap_factor=4))
+    check = sorted(range(len(arr)), key=lambda i: table[i]["N"] * table[i]["steps"])[:n_check]
+    states = {}
+
+    def grab(req, sim):
+        if req.rid in check:
+            nat = sim.model._native
+            states[req.rid] = ([nat.field(0, f) for f in ("x", "Bm", "Cq", "Nq", "Pq", "Chl", "ac")],
+                               [nat.tracer(q) for q in range(14)])
+
+    srv = sv.EnsembleServer(budget=budget_gib * 2 ** 30, workers=8, on_finish=grab)
+    t0 = time.perf_counter()
+    k = 0
+    last = t0
+    try:
+        while True:
+            now = time.perf_counter() - t0
+            while k < len(arr) and arr[k] <= now:
+                srv.submit(sv.Request(k, make(table[k]), table[k]["steps"], arrival=t0 + arr[k]))
+                k += 1
+            busy = srv.step()
+            if not busy and k >= len(arr):
+                break
+            if not busy:
+                time.sleep(0.001)
+            if time.perf_counter() - last > 10:
+                last = time.perf_counter()
+                log(f"[bench] serving t={last - t0:.0f}s arrived={k} active={len(srv.ens.sims)} "
+                    f"finished={len(srv.finished)} used={srv.used / 2 ** 30:.1f} GiB")
+            if time.perf_counter() - t0 > cap_s:
+                break
+        srv.drain(deadline=t0 + cap_s)
+    finally:
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        failed = [r for r in srv.finished if r.error]
+        srv.close()
+    same = 0
+    for i in check:
+        if i not in states:
+            continue
+        spec, vel = pi.workloads.ensemble_member(table[i], cap_factor=4)
+        sim = sv.simulation_for(spec, vel, table[i]["steps"])
+        pi.update(sim)
+        nat = sim.model._native
+        f, t = states[i]
+        ok = all(np.array_equal(nat.field(0, n), a) for n, a in zip(("x", "Bm", "Cq", "Nq", "Pq", "Chl", "ac"), f))
+        ok = ok and all(np.array_equal(nat.tracer(q), t[q]) for q in range(14))
+        same += int(ok)
+        nat.close()
+    log(f"[bench] serving: {k} requests, {st['finished']} finished in {wall:.1f} s, step p50 {st['p50_step_ms']:.2f} ms "
+        f"p99 {st['p99_step_ms']:.2f} ms; {same}/{len(states)} re-run alone identical")
+    return {"workload": f"Poisson arrivals {rate:g}/s for {window:g} s from the ensemble member table (seed {seed}), "
+                        f"lengths U[100,2000] steps, capacity 4 N, dt 60 s, admission by device footprint within "
+                        f"{budget_gib} GiB",
+            "requests": k, "finished": st["finished"], "failed": len(failed), "wall_s": wall,
+            "p50_step_ms": st["p50_step_ms"], "p99_step_ms": st["p99_step_ms"], "mean_step_ms": st["mean_step_ms"],
+            "steps": st["steps"], "p50_end_to_end_s": st["p50_e2e_s"], "p99_end_to_end_s": st["p99_e2e_s"],
+            "member_steps": st["member_steps"], "individual_steps_per_s": st["individual_steps"] / wall,
+            "checked_alone": len(states), "identical_alone": same}
 
 
 def main():
@@ -184,6 +317,11 @@ def main():
     ap.add_argument("--individuals", type=int, default=4194304, help="per species at start")
     ap.add_argument("--species", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-ensemble leg")
+    ap.add_argument("--ens-members", type=int, default=256)
+    ap.add_argument("--serve-window", type=float, default=6.0, help="seconds of arrivals")
+    ap.add_argument("--serve-rate", type=float, default=50.0, help="arrivals per second")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: a plain device copy of 256 MiB (k_probe_copy, known traffic, to "
                          "scale the counters), then --warmup + --steps steps without timing; prints no JSON")
@@ -269,9 +407,15 @@ def main():
     bstep = step_bytes(live, n_cells, n_slots)
     step_roof = {"bytes_per_step": bstep, "achieved": bstep / (ms_step * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
                  "unit": "GB/s", "frac": bstep / (ms_step * 1e-3) / HBM_PEAK}
+    del model, nat
+    ens = None
+    if not args.no_ensemble:
+        ens = ensemble_leg(pi, members=args.ens_members)
+    serve = None
+    if not args.no_serve:
+        serve = serving_leg(pi, rate=args.serve_rate, window=args.serve_window)
     base = None
     if not args.no_cpu:
-        del model, nat
         log("[bench] CPU baseline (oracle) ...")
         base = cpu_baseline(pi)
         log(f"[bench] CPU baseline {base['value']:.4g} individual-steps/s in {base['seconds']:.1f} s")
@@ -298,6 +442,8 @@ def main():
         "roofline": roof,
         "step_roofline": step_roof,
         "cpu_baseline": base,
+        "ensemble": ens,
+        "serving": serve,
         "logical_kernels": logical,
         "kernels": kernels,
     }

@@ -140,10 +140,11 @@ def ensemble_table(n=256, seed=4, lengths=None):
     return out
 
 
-def ensemble_member(entry, host_state=False):
+def ensemble_member(entry, host_state=False, cap_factor=2):
     """(ModelSpec, velocities or None) of one member: 1 km boxes and columns
     with 10 m levels and no flow; slabs of 1 km cells under the cellular flow
-    (U = 0.1 m/s, L = 16 km per 128 cells); capacity 2 N."""
+    (U = 0.1 m/s, L = 16 km per 128 cells); capacity cap_factor x N (a
+    favourable population doubles in about 1.1 days of 60 s steps)."""
     size, N = entry["size"], entry["N"]
     if size[0] == 1:
         grid = RectilinearGrid(size=size, x=(0, 1e3), y=(0, 1e3), z=(0, -10.0 * size[2]))
@@ -151,6 +152,6 @@ def ensemble_member(entry, host_state=False):
     else:
         grid = RectilinearGrid(size=size, x=(0, size[0] * 1e3), y=(0, size[1] * 1e3), z=(0, -10.0))
         vel = cellular_flow(grid, U=0.1, L=16e3 * size[0] / 128)
-    spec = ModelSpec(grid, max_individuals=2 * N, phyto=phyto_setup(entry["params"], [N], 1),
+    spec = ModelSpec(grid, max_individuals=cap_factor * N, phyto=phyto_setup(entry["params"], [N], 1),
                      seed=entry["seed"], host_state=host_state)
     return spec, vel
