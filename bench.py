@@ -181,7 +181,7 @@ def cpu_baseline(pi, seconds_hint=12.0):
             "seconds": dt, "tracer_cell_steps_per_s": g.Nx * g.Ny * g.Nz * k / dt}
 
 
-def ensemble_leg(pi, members=256, steps=20, seq_steps=4, seed=4):
+def ensemble_leg(pi, members=256, steps=20, seq_steps=4, seed=4, batched_only=False):
     """Batched ensemble: `members` drawn from the ensemble table, stepped
     one-by-one through the plain path (pi_run + sync per member) and then all
     together (pi_batch_run); ms per ensemble step of each and the ratio."""
@@ -198,19 +198,23 @@ def ensemble_leg(pi, members=256, steps=20, seq_steps=4, seed=4):
         nat.set_temperature(pi.default_temperature(g, 60.0, 1)[..., 12].astype(np.float32))
         nats.append(nat)
     log(f"[bench] ensemble: {members} members created in {time.perf_counter() - t0:.1f} s")
-    for nat in nats:
-        nat.run(60.0, 2)
-        nat.sync()
-    t0 = time.perf_counter()
-    for nat in nats:
-        nat.run(60.0, seq_steps)
-        nat.sync()
-    t_seq = (time.perf_counter() - t0) / seq_steps
+    t_seq = float("nan")
+    if not batched_only:
+        for nat in nats:
+            nat.run(60.0, 2)
+            nat.sync()
+        t0 = time.perf_counter()
+        for nat in nats:
+            nat.run(60.0, seq_steps)
+            nat.sync()
+        t_seq = (time.perf_counter() - t0) / seq_steps
     batch = pi._native.NativeBatch()
     for nat in nats:
         batch.add(nat)
     batch.run(60.0, 2)
     batch.sync()
+    tuned = batch.autotune(60.0)
+    log("[bench] ensemble: block budget " + ", ".join(f"{b}: {ms:.2f} ms" for b, ms in tuned.items()))
     live0 = sum(nat.count_active(0) for nat in nats)
     t0 = time.perf_counter()
     batch.run(60.0, steps)
@@ -230,7 +234,9 @@ def ensemble_leg(pi, members=256, steps=20, seq_steps=4, seed=4):
             "batched_ms_per_step": t_b * 1e3, "one_by_one_ms_per_step": t_seq * 1e3,
             "speedup_vs_one_by_one": t_seq / t_b,
             "batched_individual_steps_per_s": live / t_b, "one_by_one_individual_steps_per_s": live / t_seq,
-            "batched_steps": steps, "one_by_one_steps": seq_steps}
+            "batched_steps": steps, "one_by_one_steps": seq_steps,
+            "autotune_ms_per_step_by_block_budget": {str(b): ms for b, ms in tuned.items()},
+            "block_budget": min(tuned, key=tuned.get)}
 
 
 def serving_leg(pi, rate=50.0, window=6.0, cap_s=150.0, budget_gib=200, seed=5, n_check=3):
@@ -317,9 +323,13 @@ def main():
     ap.add_argument("--individuals", type=int, default=4194304, help="per species at start")
     ap.add_argument("--species", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain 3-D run (for profiling the other legs); prints their results only")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-ensemble leg")
     ap.add_argument("--ens-members", type=int, default=256)
+    ap.add_argument("--ens-batched-only", action="store_true",
+                    help="ensemble leg without the one-by-one pass (profiling the batched launches)")
     ap.add_argument("--serve-window", type=float, default=6.0, help="seconds of arrivals")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="arrivals per second")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -344,6 +354,12 @@ def main():
         sys.exit(2)
 
     pi = load_pkg()
+    if args.no_plain:
+        out = {"ensemble": None if args.no_ensemble else ensemble_leg(pi, members=args.ens_members,
+                                                                      batched_only=args.ens_batched_only),
+               "serving": None if args.no_serve else serving_leg(pi, rate=args.serve_rate, window=args.serve_window)}
+        print(json.dumps(out), flush=True)
+        return
     t0 = time.perf_counter()
     spec, (u, v, w) = pi.workloads.config_3d(n=args.n, nz=args.nz, N=args.individuals, nsp=args.species,
                                              cap=2 * args.individuals, seed=2)

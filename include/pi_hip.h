@@ -215,6 +215,10 @@ pi_status pi_batch_create(pi_batch** out);
 pi_status pi_batch_add(pi_batch* b, pi_model* m);
 pi_status pi_batch_remove(pi_batch* b, pi_model* m);
 pi_status pi_batch_run(pi_batch* b, float dt, int64_t n);
+/* Blocks per batched launch of a block-stride kernel, shared among the members
+ * in proportion to their own grids (default 16384).  Any value gives the same
+ * results; Ensemble.autotune picks it by timing steps. */
+pi_status pi_batch_set_budget(pi_batch* b, int64_t blocks);
 /* Steady steps replay a captured graph unless this is off: turn it off when
  * other threads make HIP calls (model creation, downloads) while the batch
  * runs, since a capture forbids them. */

@@ -66,7 +66,7 @@ def load_library():
     for f, a in (("create", [C.POINTER(C.c_void_p)]), ("add", [C.c_void_p, C.c_void_p]),
                  ("remove", [C.c_void_p, C.c_void_p]), ("run", [C.c_void_p, C.c_float, C.c_int64]),
                  ("sync", [C.c_void_p]), ("size", [C.c_void_p, C.POINTER(C.c_int32)]),
-                 ("graphs", [C.c_void_p, C.c_int32]),
+                 ("graphs", [C.c_void_p, C.c_int32]), ("set_budget", [C.c_void_p, C.c_int64]),
                  ("destroy", [C.c_void_p])):
         fn = getattr(lib, "pi_batch_" + f)
         fn.restype = C.c_int
@@ -300,6 +300,26 @@ class NativeBatch:
 
     def sync(self):
         check(self.lib, "pi_", self.lib.pi_batch_sync(self.raw))
+
+    def set_budget(self, blocks):
+        check(self.lib, "pi_", self.lib.pi_batch_set_budget(self.raw, C.c_int64(int(blocks))))
+
+    def autotune(self, dt, candidates=(4096, 8192, 16384, 32768, 65536), steps=3):
+        """Block budget of the batched launches chosen by timing `steps` steps
+        under each candidate (the steps are real steps of every member; any
+        budget gives the same results).  Returns {budget: ms per step}."""
+        import time
+        res = {}
+        for b in candidates:
+            self.set_budget(b)
+            self.run(dt, 1)
+            self.sync()
+            t0 = time.perf_counter()
+            self.run(dt, steps)
+            self.sync()
+            res[b] = (time.perf_counter() - t0) * 1e3 / steps
+        self.set_budget(min(res, key=res.get))
+        return res
 
     def graphs(self, on):
         check(self.lib, "pi_", self.lib.pi_batch_graphs(self.raw, 1 if on else 0))

@@ -828,6 +828,16 @@ pi_status pi_step(pi_model* m, float dt)
     return enqueue_step(m, dt);
 }
 
+/* Graph replay can be switched off for the whole process with the environment
+ * variable PI_NO_GRAPHS=1 (every step then launches eagerly; results are the
+ * same), e.g. under tools that intercept graph capture. */
+static bool graphs_allowed()
+{
+    static int v = -1;
+    if (v < 0) { const char* e = getenv("PI_NO_GRAPHS"); v = (e && e[0] && e[0] != '0') ? 0 : 1; }
+    return v == 1;
+}
+
 /* n steps.  Steady steps (vel₀ = vel₁, timing off) replay one captured step
  * graph: the launch sequence and its arguments are identical from step to step
  * because everything a step flips or counts lives in the DevState. */
@@ -838,7 +848,7 @@ pi_status pi_run(pi_model* m, float dt, int64_t n)
     if (s != PI_OK) return s;
     if (m->h_state.err) return fail(PI_ERR_CAPACITY, "model is in an error state from an earlier step");
     for (int64_t i = 0; i < n; ++i) {
-        if (m->timing || !m->vel_same) {
+        if (m->timing || !m->vel_same || !graphs_allowed()) {
             s = enqueue_step(m, dt);
             if (s != PI_OK) return s;
             continue;
@@ -1250,7 +1260,7 @@ pi_status pi_batch_run(pi_batch* b, float dt, int64_t n)
             pi_status s = batch_tables(b);
             if (s != PI_OK) return s;
         }
-        if (!same || !b->graphs) {                 /* a changed step runs without a graph */
+        if (!same || !b->graphs || !graphs_allowed()) {   /* a changed step runs without a graph */
             pi_status s = batch_launch(b);
             if (s != PI_OK) return s;
             continue;
@@ -1267,6 +1277,18 @@ pi_status pi_batch_run(pi_batch* b, float dt, int64_t n)
             if (e2 != hipSuccess) { b->gexec = nullptr; return fail(PI_ERR_HIP, hipGetErrorString(e2)); }
         }
         HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
+    }
+    return PI_OK;
+}
+
+pi_status pi_batch_set_budget(pi_batch* b, int64_t blocks)
+{
+    if (!b) return fail(PI_ERR_ARG, "null batch");
+    if (blocks < 256 || blocks > (1ll << 22)) return fail(PI_ERR_ARG, "budget out of range [256, 2^22]");
+    if (b->budget != blocks) {
+        b->budget = blocks;
+        b->tables = false;             /* rebuilt at the next step */
+        batch_drop_graph(b);
     }
     return PI_OK;
 }

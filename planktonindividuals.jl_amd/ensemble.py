@@ -63,6 +63,26 @@ class Ensemble:
             if self.check_every and self.done[self.sims[0]] % self.check_every == 0:
                 self.check()
 
+    def autotune(self, candidates=(4096, 8192, 16384, 32768, 65536), steps=3):
+        """Picks the block budget of the batched launches by timing `steps`
+        real steps of the ensemble under each candidate (the steps count: the
+        members advance; every candidate gives the same results).  Returns
+        {budget: ms per step} and keeps the fastest."""
+        import time
+        res = {}
+        for b in candidates:
+            self.batch.set_budget(b)
+            self.step(1)                     # tables rebuilt for the new budget
+            self.batch.sync()
+            t0 = time.perf_counter()
+            self.step(steps)
+            self.batch.sync()
+            res[b] = (time.perf_counter() - t0) * 1e3 / steps
+        best = min(res, key=res.get)
+        self.batch.set_budget(best)
+        self.budget = best
+        return res
+
     def check(self):
         """Synchronises and moves members in a device error state to `failed`."""
         self.batch.sync()

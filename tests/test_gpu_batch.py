@@ -203,3 +203,27 @@ def test_served_members_match_alone(pi):
         for q in range(14):
             assert np.array_equal(nat.tracer(q), t[q]), (e["id"], q)
         nat.close()
+
+
+def test_block_budget_does_not_change_results(pi):
+    """The batch's block budget only redistributes blocks: a tiny budget (one
+    block per member for most kernels) and the default give identical states."""
+    members = _members(pi)[:5]
+    sets = []
+    for budget in (256, None):
+        hs = [pi.PlanktonModel(pi.HIPArch(), spec=s)._native for s, _ in members]
+        for (s, uvw), h in zip(members, hs):
+            _inputs(pi, h, s, uvw)
+        b = pi._native.NativeBatch()
+        if budget:
+            b.set_budget(budget)
+        for h in hs:
+            b.add(h)
+        b.run(60.0, 6)
+        b.sync()
+        sets.append((b, hs))
+    for k, ((s, _), x, y) in enumerate(zip(members, sets[0][1], sets[1][1])):
+        bad = compare_models(x, y, s.n_species)
+        assert not bad, f"member {k}: " + "\n".join(bad[:20])
+    for b, _ in sets:
+        b.close()
