@@ -18,6 +18,7 @@ struct AdvArgs {
     int32_t* dstamp;          /* per slot: iteration in which its occupant dies */
     uint32_t* dcell;          /* per slot: that occupant's cell in this step */
     long long cap;
+    long long cap_total;      /* Σ capacity over species (the queue buffer holds 2 x this) */
 };
 
 struct PhysArgs {
@@ -71,6 +72,16 @@ struct SweepArgs {
     const float* prev;                /* slab: temp before this sweep (y sweep) */
     float dt;
     int dim;                          /* 0 = x, 1 = y */
+};
+
+/* fused x + y sweeps: reads the current tracers, writes temp after the y sweep */
+struct XYArgs {
+    DevGrid g;
+    DevState* st;
+    const float* u;                   /* vel₁ u */
+    const float* v;                   /* vel₁ v */
+    float* out;                       /* slab: temp after the y sweep (z halos of interior lines filled) */
+    float dt;
 };
 
 struct FillArgs {
@@ -158,9 +169,9 @@ struct AccumArgs { float* d[PI_NTRACERS]; int q[PI_NTRACERS]; int nq; long long 
  * value) and a batched one (a table of arguments, one per model, and a table
  * of (model, block, block count) per block). */
 enum KId {
-    K_VELHALF, K_MARKS, K_SCAN_P, K_SCAN_T, K_SCAN_F, K_CAPCHECK, K_ADV, K_CELLCOUNT, K_PLACE_STAY,
+    K_VELHALF, K_MARKS, K_SCAN_P, K_SCAN_T, K_SCAN_F, K_CAPCHECK, K_ADV, K_ADV_FIX, K_CELLCOUNT, K_PLACE_STAY,
     K_PLACE_SMALL, K_MIXSORT, K_BIGCELLS, K_CELLSUM, K_CELLSUM_LONG, K_PAR, K_PHYS, K_PLK, K_PLK_LONG,
-    K_SWEEPX0, K_SWEEPX1, K_SWEEPY0, K_SWEEPY1, K_HALOCOPY, K_FILL, K_FINAL0, K_FINAL1, K_ACCUM, K_COPY,
+    K_SWEEPX0, K_SWEEPX1, K_XY0, K_XY1, K_SWEEPY0, K_SWEEPY1, K_HALOCOPY, K_FILL, K_FINAL0, K_FINAL1, K_ACCUM, K_COPY,
     K_FINISH, K_NKID
 };
 struct KInfo {
@@ -181,6 +192,8 @@ void reg_util(KInfo* k);
 long long sweep_x_blocks(const DevGrid& g);
 long long sweep_y_items(const DevGrid& g);
 long long final_items(const DevGrid& g);
+bool xy_supported(const DevGrid& g);   /* k_xy handles this grid */
+long long xy_blocks(const DevGrid& g);
 uint32_t scan_tiles(uint32_t n);
 
 /* initial state generated on the device (initialize_plankton!,

@@ -538,6 +538,260 @@ __device__ __forceinline__ void k_final_body(const FinalArgs& a, Blk K)
 PI_KERNEL2(k_final0, k_final_body<false>, FinalArgs, 256)
 PI_KERNEL2(k_final1, k_final_body<true>, FinalArgs, 256)
 
+/* ------------------------------------------------- fused x + y sweeps (k_xy)
+ * calc_Gcsˣ! + multi_dim_x! + fill_halo_tracer! + calc_Gcsʸ! + multi_dim_y!
+ * (Advection/multi_dim_adv.jl:111-135) in one pass, then the z halos of the
+ * result on interior lines (the halo fill the z sweep reads), for grids whose
+ * Periodic directions have N >= H, H = 2 and Nx <= XY_T.
+ *
+ * A group of roundup64(Nx) threads owns one x-row (thread = cell i) and walks
+ * a run of RUN_XY rows of one level in y.  Each row step computes the x sweep
+ * of one row: every thread the flux of its west face; the east face comes
+ * from the next thread through LDS (one barrier per row, double-buffered); the
+ * east face of the last cell is the first face's flux when x is Periodic
+ * (the reference's Gc halo copy) and 0 when Bounded.  The x-swept values
+ * (temp after the x sweep) of the last rows stay in registers with their
+ * masked y differences, so each y face flux and each difference is computed
+ * once, and the y update of the row two steps back is written out.
+ * Rows outside the interior are the reference's halo fill of the x-swept
+ * field: the x sweep of the row they copy (periodic image or Bounded edge),
+ * recomputed bit for bit.  Tracers are processed in two passes of 7. */
+#define RUN_XY 16
+#define XY_T 256
+#define PI_WAIT_VM0 0x0F70        /* s_waitcnt vmcnt(0): vector memory (incl. loads into LDS) done */
+
+__device__ __forceinline__ int map_row(int j, int N, int topo)
+{
+    if (topo == PI_PERIODIC) return j < 0 ? j + N : j >= N ? j - N : j;
+    return j < 0 ? 0 : j >= N ? N - 1 : j;
+}
+
+/* fast flux from the stencil values bp, bm and the masked differences dd0
+ * (at the face), dnp, dnm (the faces before and after, in the x/y sense of
+ * stencil()); bad = the full formula is needed */
+__device__ __forceinline__ float flux_fast_s(const Face& F, float bp, float bm, float dd0, float dnp, float dnm, bool& bad)
+{
+    const bool pos = F.T > 0.0f;
+    bool sub = false;
+    const float p = psi<false>(F, theta(pos ? dnp : dnm, dd0), sub);
+    const float pd = p * dd0;
+    const float r = F.T * (pos ? bp + pd : bm - pd);
+    /* finite stencil values and differences below 2^126: every Ψ is in [0, 1]
+     * and the omitted term is finite (NaN fails the comparison) */
+    const float m = dmaxf(dmaxf(dmaxf(fabsf(bp), fabsf(bm)), fabsf(dd0)), dmaxf(fabsf(dnp), fabsf(dnm)));
+    bad = !(m < 0x1p126f && F.fast && r != 0.0f) || sub;
+    return r;
+}
+__device__ __forceinline__ float flux_full_s(const Face& F, float bp, float bm, float dd0, float dnp, float dnm)
+{
+    const float T = F.T;
+    bool sub = false;
+    const float pp = psi<true>(F, theta(dnp, dd0), sub), pm = psi<true>(F, theta(dnm, dd0), sub);
+    return 0.5f * (T + fabsf(T)) * (bp + pp * dd0) + 0.5f * (T - fabsf(T)) * (bm - pm * dd0);
+}
+
+struct XYShared {
+    float c[2][7][XY_T];      /* tracer rows of the x sweep (double-buffered) */
+    float F[2][7][XY_T];      /* their west-face fluxes */
+};
+
+/* One pass (tracers Q0..Q1-1) over a run of rows of one level.  Row step n
+ * (n = 0 .. RUN_XY+3) is the x sweep of row jr = j0 - 2 + n.  Tracer rows are
+ * loaded straight into LDS (global_load_lds: no registers) one row step ahead,
+ * each cell once: the x stencil reads its neighbours from LDS.  In the interval
+ * between two barriers a thread finishes row n-1 (east face from LDS, the y
+ * faces and the update of row n-3), starts the load of row n+1 and computes
+ * the fluxes of row n; one barrier per row step.  x halo values are the
+ * reference's filled halos of the row: periodic images, or the edge value when
+ * Bounded. */
+template <bool LAND, int Q0, int Q1>
+__device__ __forceinline__ void xy_run(const XYArgs& a, const DevGrid& g, const PI_GLOBAL float* C0, PI_GLOBAL float* OUT,
+                                       XYShared& sh, int t, int gbase, int i, bool xin, bool act, int j0, int k)
+{
+    constexpr int NQ = Q1 - Q0;
+    constexpr int NS = RUN_XY + 4;
+    const uint32_t sy = (uint32_t)g.Px, nc = (uint32_t)g.ncell;
+    const PI_GLOBAL float* U = gp(a.u);
+    const PI_GLOBAL float* V = gp(a.v);
+    const float dt = a.dt;
+    const int kk = k + g.Hz + 1;                                  /* 1-based padded level */
+    const float Ax = g.dy * dzF_at(g, kk), Ay = g.dx * dzF_at(g, kk);
+    const float coef = divc(dt, g.vol[kk - 1], g.yvol[kk - 1]);
+    const uint32_t zoff = sy * (uint32_t)g.Py * (uint32_t)(k + g.Hz);
+    const uint32_t xoff = (uint32_t)(i + g.Hx);
+    auto cellof = [&](int j) -> uint32_t { return xoff + sy * (uint32_t)(j + g.Hy) + zoff; };
+    const bool last = i == g.Nx - 1;
+    const bool perx = g.tx == PI_PERIODIC;
+    /* x stencil neighbours within the row (LDS slots of this group) */
+    const int im2 = gbase + (perx ? (i >= 2 ? i - 2 : i - 2 + g.Nx) : max(i - 2, 0));
+    const int im1 = gbase + (perx ? (i >= 1 ? i - 1 : i - 1 + g.Nx) : max(i - 1, 0));
+    const int ip1 = gbase + (perx ? (i + 1 < g.Nx ? i + 1 : i + 1 - g.Nx) : min(i + 1, g.Nx - 1));
+    const int tn = last ? gbase : min(t + 1, XY_T - 1);          /* slot of the east face's flux */
+    float A[NQ], B[NQ], dA[NQ], dB[NQ], Fp[NQ];
+    const int wbase = __builtin_amdgcn_readfirstlane(t) & ~63;    /* this wave's LDS slots (uniform) */
+    /* row jr of tracers Q0.. into LDS buffer b: each lane loads its own cell */
+    auto stage = [&](int b, int jr) {
+        const uint32_t c = cellof(map_row(jr, g.Ny, g.ty));
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            __builtin_amdgcn_global_load_lds((PI_GLOBAL void*)((const PI_GLOBAL char*)(C0 + (uint32_t)(Q0 + q) * nc) + c * 4u),
+                                             (__attribute__((address_space(3))) void*)&sh.c[b][q][wbase], 4, 0, 0);
+    };
+    float Tp = 0.0f, Tx = 0.0f, dTx = 0.0f;
+    uint32_t cxp = 0;                                             /* cell of the previous row step */
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) { A[q] = B[q] = dA[q] = dB[q] = Fp[q] = 0.0f; }
+    stage(0, j0 - 2);
+    __builtin_amdgcn_s_waitcnt(PI_WAIT_VM0);
+    __syncthreads();
+#pragma unroll 1
+    for (int n = 0; n <= NS; ++n) {
+        /* ---- finish row step s = n - 1: x update, then y */
+        if (n >= 1) {
+            const int s = n - 1;
+            const int pp = s & 1;
+            const int jr = j0 - 2 + s;
+            float N[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const float Fo = sh.F[pp][q][t];
+                const float Fn = (last && !perx) ? 0.0f : sh.F[pp][q][tn];
+                const float co = sh.c[pp][q][t];
+                N[q] = co - coef * (lmask<LAND>(g, cxp, cxp + 1, Fn - Fo) - co * dTx);
+            }
+            /* difference at face jr (between rows jr-1 and jr), with the
+             * landmask of the rows themselves (halo rows included) */
+            const uint32_t cl = cellof(max(-2, min(jr, g.Ny + 1))), cl1 = cellof(max(-2, min(jr - 1, g.Ny + 1)));
+            float dN[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) dN[q] = lmask<LAND>(g, cl1, cl, N[q] - B[q]);
+            if (s >= 3) {
+                /* face f = jr - 1 (south face of row f): cm2 = row f-2 ... cp1 = row f+1 */
+                const int f = jr - 1;
+                const uint32_t cf = cellof(min(f, g.Ny));
+                const Face Fy = face(g, ldg(V, cf), dt, g.dy, g.ydy, Ay);
+                const bool zero = f == g.Ny && g.ty == PI_BOUNDED;
+                float G[NQ];
+                uint32_t needy = 0;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    bool bad;
+                    G[q] = flux_fast_s(Fy, A[q], B[q], dB[q], dA[q], dN[q], bad);
+                    needy |= (uint32_t)(bad && !zero) << q;
+                }
+                if (__builtin_expect(needy != 0, 0)) {
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q)
+                        if ((needy >> q) & 1u) G[q] = flux_full_s(Fy, A[q], B[q], dB[q], dA[q], dN[q]);
+                }
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) G[q] = zero ? 0.0f : G[q];
+                const int jo = f - 1;                             /* row updated by faces f-1 and f */
+                if (s >= 4 && act && xin && jo < min(j0 + RUN_XY, g.Ny)) {
+                    const uint32_t co = cellof(jo), cn = cellof(f);
+                    const float dTy = lmask<LAND>(g, co, cn, Fy.T - Tp);
+                    /* z halo images of this cell on its (interior) line: Bounded z copies the edge level */
+                    const uint32_t sz = sy * (uint32_t)g.Py;
+                    const bool top = k == 0, bot = k == g.Nz - 1;
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        const PI_GLOBAL float* c = C0 + (uint32_t)(Q0 + q) * nc;
+                        PI_GLOBAL float* o = OUT + (uint32_t)(Q0 + q) * nc;
+                        const float v = A[q] - coef * (lmask<LAND>(g, co, cn, G[q] - Fp[q]) - ldg(c, co) * dTy);
+                        stg(o, co, v);
+                        if (top) { stg(o, co - sz, v); stg(o, co - 2 * sz, v); }
+                        if (bot) { stg(o, co + sz, v); stg(o, co + 2 * sz, v); }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) Fp[q] = G[q];
+                Tp = Fy.T;
+            }
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) { A[q] = B[q]; B[q] = N[q]; dA[q] = dB[q]; dB[q] = dN[q]; }
+        }
+        if (n == NS) break;
+        const int p = n & 1;
+        /* ---- load row n + 1 into the buffer whose row n - 1 was read above */
+        if (n + 1 < NS) stage(p ^ 1, j0 - 1 + n);
+        /* ---- x fluxes of row step n, from the staged row */
+        const uint32_t cx = cellof(map_row(j0 - 2 + n, g.Ny, g.ty));
+        const Face Fx = face(g, ldg(U, cx), dt, g.dx, g.ydx, Ax);
+        uint32_t need = 0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const float cm2 = sh.c[p][q][im2], cm1 = sh.c[p][q][im1], c0 = sh.c[p][q][t], cp1 = sh.c[p][q][ip1];
+            const float dd0 = lmask<LAND>(g, cx - 1, cx, c0 - cm1);
+            const float dnp = lmask<LAND>(g, cx - 2, cx - 1, cm1 - cm2);
+            const float dnm = lmask<LAND>(g, cx, cx + 1, cp1 - c0);
+            bool bad;
+            const float F = flux_fast_s(Fx, cm1, c0, dd0, dnp, dnm, bad);
+            need |= (uint32_t)bad << q;
+            sh.F[p][q][t] = F;
+        }
+        if (__builtin_expect(need != 0, 0)) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                if ((need >> q) & 1u) {
+                    const float cm2 = sh.c[p][q][im2], cm1 = sh.c[p][q][im1], c0 = sh.c[p][q][t], cp1 = sh.c[p][q][ip1];
+                    sh.F[p][q][t] = flux_full_s(Fx, cm1, c0, lmask<LAND>(g, cx - 1, cx, c0 - cm1),
+                                                lmask<LAND>(g, cx - 2, cx - 1, cm1 - cm2),
+                                                lmask<LAND>(g, cx, cx + 1, cp1 - c0));
+                }
+            }
+        }
+        Tx = Fx.T;
+        dTx = lmask<LAND>(g, cx, cx + 1, Ax * ldg(U, cx + 1) - Tx);   /* transport difference across the cell */
+        cxp = cx;
+        __builtin_amdgcn_s_waitcnt(PI_WAIT_VM0);                  /* row n + 1 is in LDS */
+        __syncthreads();
+    }
+}
+
+template <bool LAND>
+__device__ __forceinline__ void k_xy_body(const XYArgs& a, Blk K)
+{
+    __shared__ XYShared sh;
+    const DevState* S = a.st;
+    if (S->err) return;
+    const DevGrid& g = a.g;
+    const int nxr = (g.Nx + 63) & ~63;
+    const int G = XY_T / nxr;                                     /* x-rows per block */
+    const int t = (int)threadIdx.x;
+    const int grp = t / nxr;
+    const int gbase = grp * nxr;
+    const int x = t - gbase;
+    const bool xin = grp < G && x < g.Nx;
+    const int i = min(x, g.Nx - 1);
+    const int nrun = (g.Ny + RUN_XY - 1) / RUN_XY;
+    const uint32_t nitem = (uint32_t)nrun * (uint32_t)g.Nz;
+    const PI_GLOBAL float* C0 = gp((const float*)S->tr[S->tcur][0]);
+    PI_GLOBAL float* OUT = gp(a.out);
+    for (uint32_t base = K.b * (uint32_t)G; base < nitem; base += K.n * (uint32_t)G) {
+        const uint32_t item = base + (uint32_t)min(grp, G - 1);
+        const bool act = grp < G && item < nitem;
+        const uint32_t it = act ? item : base;
+        const int r = (int)(it % (uint32_t)nrun), k = (int)(it / (uint32_t)nrun);
+        xy_run<LAND, 0, 7>(a, g, C0, OUT, sh, t, gbase, i, xin, act, r * RUN_XY, k);
+        xy_run<LAND, 7, 14>(a, g, C0, OUT, sh, t, gbase, i, xin, act, r * RUN_XY, k);
+    }
+}
+PI_KERNEL2(k_xy0, k_xy_body<false>, XYArgs, XY_T)
+PI_KERNEL2(k_xy1, k_xy_body<true>, XYArgs, XY_T)
+
+bool xy_supported(const DevGrid& g)
+{
+    return g.Hx == 2 && g.Hy == 2 && g.Hz == 2 && g.Nx <= XY_T && g.tz == PI_BOUNDED &&
+           !(g.tx == PI_PERIODIC && g.Nx < g.Hx) && !(g.ty == PI_PERIODIC && g.Ny < g.Hy);
+}
+long long xy_blocks(const DevGrid& g)
+{
+    const int nxr = (g.Nx + 63) & ~63;
+    const long long G = XY_T / nxr;
+    const long long nitem = (long long)((g.Ny + RUN_XY - 1) / RUN_XY) * g.Nz;
+    return (nitem + G - 1) / G;
+}
+
 /* ------------------------------------------------------------- registry */
 long long sweep_x_blocks(const DevGrid& g)
 {
@@ -558,4 +812,6 @@ void reg_tracers(KInfo* k)
     PI_REG(k, K_HALOCOPY, k_halo_copy, FillArgs, 256, true);
     PI_REG(k, K_FINAL0, k_final0, FinalArgs, 256, true);
     PI_REG(k, K_FINAL1, k_final1, FinalArgs, 256, true);
+    PI_REG(k, K_XY0, k_xy0, XYArgs, XY_T, true);
+    PI_REG(k, K_XY1, k_xy1, XYArgs, XY_T, true);
 }

@@ -84,6 +84,18 @@ struct Store {
 #define PI_GLOBAL __attribute__((address_space(1)))
 template <typename T> __device__ __forceinline__ PI_GLOBAL T* gp(T* p) { return (PI_GLOBAL T*)p; }
 
+/* Loads and stores through global-address-space pointers: a uniform (SGPR) base plus a
+ * 32-bit byte offset, so a load needs no 64-bit address arithmetic and loads at
+ * fixed distances share one offset register (immediate offsets) */
+__device__ __forceinline__ float ldg(const PI_GLOBAL float* base, uint32_t cell)
+{
+    return *(const PI_GLOBAL float*)((const PI_GLOBAL char*)base + cell * 4u);
+}
+__device__ __forceinline__ void stg(PI_GLOBAL float* base, uint32_t cell, float v)
+{
+    *(PI_GLOBAL float*)((PI_GLOBAL char*)base + cell * 4u) = v;
+}
+
 struct GStore {
     PI_GLOBAL uint32_t* key;
     PI_GLOBAL float *x, *y, *z, *Sz, *Bm, *Cq, *Nq, *Pq, *Chl, *gen, *age;
@@ -210,6 +222,7 @@ struct DevState {
     uint32_t nstore;                   /* entries in store[scur] */
     uint32_t nA;                       /* of which the first nA are sorted by (cell, key) as laid out by the previous seg */
     uint32_t nsmall, nmix;             /* queued entries / re-sorted cells of this step */
+    uint32_t nredo;                    /* entries moved again with exact divisions (k_adv_fix) */
     uint32_t err, err_sp;
     uint32_t nbig;
     uint32_t nlong;                    /* cells longer than LONG_SEG this step */

@@ -681,8 +681,9 @@ static void build_step(pi_model* m, float dt, Plan& P)
         aa.sz = std::sqrt(m->bgc[PI_B_kappa_vP] * dt);
         aa.seed = m->seed;
         aa.newc = m->newc; aa.stayf = m->stayf; aa.small = m->small; aa.cnts = m->cnts;
-        aa.dstamp = m->dstamp; aa.dcell = m->dcell; aa.cap = m->cap;
+        aa.dstamp = m->dstamp; aa.dcell = m->dcell; aa.cap = m->cap; aa.cap_total = m->ctot;
         P.add(K_ADV, "adv", pb, aa);
+        P.add(K_ADV_FIX, "adv", 64, aa);
     }
     {
         ScanIn sp = scan_plain(m->stayf, (uint32_t)(2 * m->ctot));      /* stayer prefix over the sorted part */
@@ -732,27 +733,35 @@ static void build_step(pi_model* m, float dt, Plan& P)
     const bool land = g.has_land != 0;
     /* Periodic with N < H: the fills read halo planes of the slab they fill */
     const bool alias = (g.tx == PI_PERIODIC && g.Nx < g.Hx) || (g.ty == PI_PERIODIC && g.Ny < g.Hy);
+    /* the fused x + y sweeps where the grid allows them (k_xy), else the
+     * separate sweeps with their halo fills */
+    const bool fxy = xy_supported(g);
     {
+        XYArgs xa = zeroed<XYArgs>();
+        xa.g = g; xa.st = st; xa.dt = dt; xa.u = V1.u; xa.v = V1.v; xa.out = m->T2[0];
+        const long long xyb = fxy ? xy_blocks(g) : 0;
+        P.add(K_XY0, "sweep_xy", land ? 0 : xyb, xa);
+        P.add(K_XY1, "sweep_xy", land ? xyb : 0, xa);
         SweepArgs sa = zeroed<SweepArgs>();
         sa.g = g; sa.st = st; sa.dt = dt;
         sa.vel = V1.u; sa.dim = 0;
         sa.out = m->T1[0]; sa.prev = nullptr;
-        const long long xb = sweep_x_blocks(g);
+        const long long xb = fxy ? 0 : sweep_x_blocks(g);
         P.add(K_SWEEPX0, "sweep_x", land ? 0 : xb, sa);
         P.add(K_SWEEPX1, "sweep_x", land ? xb : 0, sa);
         FillArgs fa = zeroed<FillArgs>();
         fa.g = g; fa.st = st; fa.nf = PI_NTRACERS; fa.dim = 1; fa.lines_full = 0;
         fa.f = m->T1[0];
         P.add(K_HALOCOPY, "fill", alias ? grid_for((long long)g.ncell * PI_NTRACERS, 256, 4096) : 0, fa);
-        P.add(K_FILL, "fill", blocks_for((long long)g.Nx * g.Nz * PI_NTRACERS), fa);
+        P.add(K_FILL, "fill", fxy ? 0 : blocks_for((long long)g.Nx * g.Nz * PI_NTRACERS), fa);
         sa.vel = V1.v; sa.dim = 1;
         sa.out = m->T2[0]; sa.prev = m->T1[0];
-        const long long yb = grid_for(sweep_y_items(g), 256, 8192);
+        const long long yb = fxy ? 0 : grid_for(sweep_y_items(g), 256, 8192);
         P.add(K_SWEEPY0, "sweep_y", land ? 0 : yb, sa);
         P.add(K_SWEEPY1, "sweep_y", land ? yb : 0, sa);
         fa.dim = 2;
         fa.f = m->T2[0];
-        P.add(K_FILL, "fill", blocks_for((long long)g.Nx * g.Ny * PI_NTRACERS), fa);
+        P.add(K_FILL, "fill", fxy ? 0 : blocks_for((long long)g.Nx * g.Ny * PI_NTRACERS), fa);
         FinalArgs fn = zeroed<FinalArgs>();
         fn.g = g; fn.st = st; fn.w = V1.w; fn.dt = dt;
         fn.prev = m->T2[0]; fn.plk = m->plk_slab;
