@@ -40,11 +40,10 @@
  * u_itpl / v_itpl / w_itpl with ac = true (Individuals/Advection/interpolation.jl:1-83).
  * Corner indices are clamped into the padded array so that positions outside
  * the halo (not produced by the reference's boundary conditions) cannot fault.
- * EXACT = false divides with divq and raises `sub` when a quotient may be
- * inexact (subnormal); such individuals are moved again with EXACT = true. */
-template <bool EXACT>
+ * The corners (x, x+1) are read in pairs, four 8-byte loads per call: the
+ * kernel is bound by the issue of its gathers. */
 __device__ __forceinline__ float itpl(const DevGrid& g, const float* __restrict__ f,
-                                      float x, float y, float z, int comp, bool& sub)
+                                      float x, float y, float z, int comp)
 {
     float xs = comp == 0 ? x + 1.0f : x + 0.5f;
     float ys = comp == 1 ? y + 1.0f : y + 0.5f;
@@ -62,17 +61,19 @@ __device__ __forceinline__ float itpl(const DevGrid& g, const float* __restrict_
     else if (comp == 1) { D0 = g.dy; D1 = g.dy; R0 = g.ydy; R1 = g.ydy; }
     else { D0 = dzC_at(g, zi); D1 = dzC_at(g, zi + 1); R0 = g.ydzC[zi - 1]; R1 = g.ydzC[zi]; }
     const PI_GLOBAL float* F = gp(f);
-    const float f000 = ldg(F, b), f001 = ldg(F, b + sz), f010 = ldg(F, b + sy), f011 = ldg(F, b + sy + sz);
-    const float f100 = ldg(F, b + sx), f101 = ldg(F, b + sx + sz), f110 = ldg(F, b + sx + sy), f111 = ldg(F, b + sx + sy + sz);
+    const pi_f2u p00 = ldg2(F, b), p01 = ldg2(F, b + sz), p10 = ldg2(F, b + sy), p11 = ldg2(F, b + sy + sz);
+    const float f000 = p00.x, f001 = p01.x, f010 = p10.x, f011 = p11.x;
+    const float f100 = p00.y, f101 = p01.y, f110 = p10.y, f111 = p11.y;
+    (void)sx;
     const float ox = 1.0f - xd, oy = 1.0f - yd, oz = 1.0f - zd;
-    float s = divq<EXACT>(ox * oy * oz * f000, D0, R0, sub);
-    s = s + divq<EXACT>(ox * oy * zd * f001, D1, R1, sub);
-    s = s + divq<EXACT>(ox * yd * oz * f010, D0, R0, sub);
-    s = s + divq<EXACT>(ox * yd * zd * f011, D1, R1, sub);
-    s = s + divq<EXACT>(xd * oy * oz * f100, D0, R0, sub);
-    s = s + divq<EXACT>(xd * oy * zd * f101, D1, R1, sub);
-    s = s + divq<EXACT>(xd * yd * oz * f110, D0, R0, sub);
-    s = s + divq<EXACT>(xd * yd * zd * f111, D1, R1, sub);
+    float s = divc(ox * oy * oz * f000, D0, R0);
+    s = s + divc(ox * oy * zd * f001, D1, R1);
+    s = s + divc(ox * yd * oz * f010, D0, R0);
+    s = s + divc(ox * yd * zd * f011, D1, R1);
+    s = s + divc(xd * oy * oz * f100, D0, R0);
+    s = s + divc(xd * oy * zd * f101, D1, R1);
+    s = s + divc(xd * yd * oz * f110, D0, R0);
+    s = s + divc(xd * yd * zd * f111, D1, R1);
     return s;
 }
 
@@ -107,50 +108,48 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
 
 /* RK4 advection (particle_advection.jl:2-56) and the random walk
  * (particle_diffusion.jl:2-33) of one individual, up to its new position. */
-template <bool EXACT>
 __device__ __forceinline__ void move(const AdvArgs& a, const DevGrid& g, float x0, float y0, float z0,
-                                     float r0, float r1, float r2, long long zprev, float& x, float& y, float& z,
-                                     bool& sub)
+                                     float r0, float r1, float r2, long long zprev, float& x, float& y, float& z)
 {
     const float dt = a.dt;
     /* ---- RK4, particle_advection.jl:2-56 */
-    float u1 = itpl<EXACT>(g, a.v0.u, x0, y0, z0, 0, sub);
-    float v1 = itpl<EXACT>(g, a.v0.v, x0, y0, z0, 1, sub);
-    float w1 = itpl<EXACT>(g, a.v0.w, x0, y0, z0, 2, sub);
+    float u1 = itpl(g, a.v0.u, x0, y0, z0, 0);
+    float v1 = itpl(g, a.v0.v, x0, y0, z0, 1);
+    float w1 = itpl(g, a.v0.w, x0, y0, z0, 2);
     float u2 = u1, v2 = v1, w2 = w1;
     float px = pbc(x0 + 0.5f * u1 * dt, g.Nx, g.tx);
     float py = pbc(y0 + 0.5f * v1 * dt, g.Ny, g.ty);
     float pz = pbc(z0 - 0.5f * w1 * dt, g.Nz, g.tz);
-    u1 = itpl<EXACT>(g, a.vh.u, px, py, pz, 0, sub);
-    v1 = itpl<EXACT>(g, a.vh.v, px, py, pz, 1, sub);
-    w1 = itpl<EXACT>(g, a.vh.w, px, py, pz, 2, sub);
+    u1 = itpl(g, a.vh.u, px, py, pz, 0);
+    v1 = itpl(g, a.vh.v, px, py, pz, 1);
+    w1 = itpl(g, a.vh.w, px, py, pz, 2);
     u2 = u2 + u1 * 2.0f; v2 = v2 + v1 * 2.0f; w2 = w2 + w1 * 2.0f;
     px = pbc(x0 + 0.5f * u1 * dt, g.Nx, g.tx);
     py = pbc(y0 + 0.5f * v1 * dt, g.Ny, g.ty);
     pz = pbc(z0 - 0.5f * w1 * dt, g.Nz, g.tz);
-    u1 = itpl<EXACT>(g, a.vh.u, px, py, pz, 0, sub);
-    v1 = itpl<EXACT>(g, a.vh.v, px, py, pz, 1, sub);
-    w1 = itpl<EXACT>(g, a.vh.w, px, py, pz, 2, sub);
+    u1 = itpl(g, a.vh.u, px, py, pz, 0);
+    v1 = itpl(g, a.vh.v, px, py, pz, 1);
+    w1 = itpl(g, a.vh.w, px, py, pz, 2);
     u2 = u2 + u1 * 2.0f; v2 = v2 + v1 * 2.0f; w2 = w2 + w1 * 2.0f;
     px = pbc(x0 + 1.0f * u1 * dt, g.Nx, g.tx);
     py = pbc(y0 + 1.0f * v1 * dt, g.Ny, g.ty);
     pz = pbc(z0 - 1.0f * w1 * dt, g.Nz, g.tz);
-    u1 = itpl<EXACT>(g, a.v1.u, px, py, pz, 0, sub);
-    v1 = itpl<EXACT>(g, a.v1.v, px, py, pz, 1, sub);
-    w1 = itpl<EXACT>(g, a.v1.w, px, py, pz, 2, sub);
+    u1 = itpl(g, a.v1.u, px, py, pz, 0);
+    v1 = itpl(g, a.v1.v, px, py, pz, 1);
+    w1 = itpl(g, a.v1.w, px, py, pz, 2);
     u2 = u2 + u1; v2 = v2 + v1; w2 = w2 + w1;
-    u2 = divq<EXACT>(u2, 6.0f, g.y6, sub); v2 = divq<EXACT>(v2, 6.0f, g.y6, sub); w2 = divq<EXACT>(w2, 6.0f, g.y6, sub);
+    u2 = divc(u2, 6.0f, g.y6); v2 = divc(v2, 6.0f, g.y6); w2 = divc(w2, 6.0f, g.y6);
     x = pbc(x0 + 1.0f * u2 * dt, g.Nx, g.tx);
     y = pbc(y0 + 1.0f * v2 * dt, g.Ny, g.ty);
     z = pbc(z0 - 1.0f * w2 * dt, g.Nz, g.tz);
     /* ---- random walk, particle_diffusion.jl:2-33 (stale zi from the previous find_inds) */
     r0 = (r0 - 0.5f) * 2.0f; r1 = (r1 - 0.5f) * 2.0f; r2 = (r2 - 0.5f) * 2.0f;
-    r0 = divq<EXACT>(r0 * a.sx, g.dx, g.ydx, sub);
-    r1 = divq<EXACT>(r1 * a.sy, g.dy, g.ydy, sub);
+    r0 = divc(r0 * a.sx, g.dx, g.ydx);
+    r1 = divc(r1 * a.sy, g.dy, g.ydy);
     {
         long long kz = zprev + g.Hz;
         kz = kz < 1 ? 1 : kz > g.Pz ? g.Pz : kz;
-        r2 = divq<EXACT>(r2 * a.sz, g.dzC[kz - 1], g.ydzC[kz - 1], sub);
+        r2 = divc(r2 * a.sz, g.dzC[kz - 1], g.ydzC[kz - 1]);
     }
     x = pbc(x + r0, g.Nx, g.tx);
     y = pbc(y + r1, g.Ny, g.ty);
@@ -198,9 +197,8 @@ __device__ __forceinline__ void place_entry(const AdvArgs& a, DevState* st, cons
  * the device state are read before the loops that call these: a load that may
  * alias the loop's stores would be a vector load waiting for every outstanding
  * store.) */
-template <bool EXACT>
 __device__ __forceinline__ void move_entry(const AdvArgs& a, const GStore& S, uint32_t e, long long iteration, uint8_t fl,
-                                           float& x, float& y, float& z, bool& sub)
+                                           float& x, float& y, float& z)
 {
     const DevGrid& g = a.g;
     const uint32_t key = S.key[e];
@@ -210,12 +208,9 @@ __device__ __forceinline__ void move_entry(const AdvArgs& a, const GStore& S, ui
     pi_uniform3(a.seed, iteration, sp, PI_KIND_DIFFUSION, slot, &r0, &r1, &r2);
     const uint32_t cprev = S.cell[e];
     const long long zprev = (fl & F_ZERO) ? 0 : (long long)(cprev / (uint32_t)(g.Px * g.Py)) + 1;
-    move<EXACT>(a, g, S.x[e], S.y[e], S.z[e], r0, r1, r2, zprev, x, y, z, sub);
+    move(a, g, S.x[e], S.y[e], S.z[e], r0, r1, r2, zprev, x, y, z);
 }
 
-/* Individuals whose fast move had a possibly inexact quotient are listed from
- * the top of the queue buffer down (queued entries fill it from the bottom; the
- * two never exceed its 2 x capacity entries) and moved and placed by k_adv_fix. */
 __device__ __forceinline__ void k_adv_body(const AdvArgs& a, Blk K)
 {
     DevState* st = a.st;
@@ -224,44 +219,16 @@ __device__ __forceinline__ void k_adv_body(const AdvArgs& a, Blk K)
     const uint32_t n = st->nstore;
     const uint32_t nA = st->nA;
     const long long iteration = st->iteration + 1;
-    const uint32_t top = (uint32_t)(2 * a.cap_total) - 1u;
     const uint32_t nloop = (n + 63u) & ~63u;          /* whole waves stay in the loop (wave_append) */
     for (uint32_t e = K.b * blockDim.x + threadIdx.x; e < nloop; e += K.n * blockDim.x) {
         const uint8_t fl = e < n ? S.flags[e] : (uint8_t)0;
         float x = 0.0f, y = 0.0f, z = 0.0f;
-        bool sub = false;
-        if (fl & F_AC) move_entry<false>(a, S, e, iteration, fl, x, y, z, sub);
-        const uint32_t ri = wave_append(&st->nredo, sub);
-        if (sub) a.small[top - ri] = e;
-        place_entry(a, st, S, e, nA, (int32_t)iteration, (fl & F_AC) != 0 && !sub, fl, x, y, z);
+        if (fl & F_AC) move_entry(a, S, e, iteration, fl, x, y, z);
+        place_entry(a, st, S, e, nA, (int32_t)iteration, (fl & F_AC) != 0, fl, x, y, z);
         if (!(fl & F_AC) && e < nA) a.stayf[e] = 0u;
     }
 }
 PI_KERNEL_XCD(k_adv, AdvArgs, 256)
-
-/* the listed individuals, moved with exact divisions and placed */
-__device__ __forceinline__ void k_adv_fix_body(const AdvArgs& a, Blk K)
-{
-    DevState* st = a.st;
-    if (st->err) return;
-    const uint32_t nr = st->nredo;
-    if (nr == 0) return;
-    const GStore S = gstore(st->store[st->scur]);
-    const uint32_t nA = st->nA;
-    const long long iteration = st->iteration + 1;
-    const uint32_t top = (uint32_t)(2 * a.cap_total) - 1u;
-    const uint32_t nloop = (nr + 63u) & ~63u;
-    for (uint32_t t = K.b * blockDim.x + threadIdx.x; t < nloop; t += K.n * blockDim.x) {
-        const bool live = t < nr;
-        const uint32_t e = live ? a.small[top - t] : 0u;
-        const uint8_t fl = live ? S.flags[e] : (uint8_t)0;
-        float x = 0.0f, y = 0.0f, z = 0.0f;
-        bool sub = false;
-        if (live) move_entry<true>(a, S, e, iteration, fl, x, y, z, sub);
-        place_entry(a, st, S, e, nA, (int32_t)iteration, live, fl, x, y, z);
-    }
-}
-PI_KERNEL(k_adv_fix, AdvArgs, 256)
 
 /* Survivors of this step's losses and the dividers, as per-species slot bitmaps
  * (plankton_update.jl:5-24: ac after grazing!/mortality!, dvid .*= ac).  They
@@ -1305,7 +1272,6 @@ PI_KERNEL(k_plk_long, PlkArgs, 256)
 void reg_particles(KInfo* k)
 {
     PI_REG(k, K_ADV, k_adv, AdvArgs, 256, true);
-    PI_REG(k, K_ADV_FIX, k_adv_fix, AdvArgs, 256, true);
     PI_REG(k, K_MARKS, k_marks, MarksArgs, 256, true);
     PI_REG(k, K_CELLCOUNT, k_cellcount, BinArgs, 256, true);
     PI_REG(k, K_PLACE_STAY, k_place_stay, BinArgs, 256, true);

@@ -123,7 +123,6 @@ __device__ __forceinline__ void k_capcheck_impl(Blk K, DevState* st, const uint3
     st->nlong = 0;
     st->nsmall = 0;
     st->nmix = 0;
-    st->nredo = 0;
 }
 __device__ __forceinline__ void k_capcheck_body(const CapArgs& x, Blk K) { k_capcheck_impl(K, x.st, x.G, x.nsp, x.nw); }
 PI_KERNEL(k_capcheck, CapArgs, 64)

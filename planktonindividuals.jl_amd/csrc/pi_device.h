@@ -96,6 +96,13 @@ __device__ __forceinline__ void stg(PI_GLOBAL float* base, uint32_t cell, float 
     *(PI_GLOBAL float*)((PI_GLOBAL char*)base + cell * 4u) = v;
 }
 
+/* two consecutive floats (cells c and c+1) in one 8-byte load (4-byte aligned) */
+typedef float pi_f2u __attribute__((ext_vector_type(2), aligned(4)));
+__device__ __forceinline__ pi_f2u ldg2(const PI_GLOBAL float* base, uint32_t cell)
+{
+    return *(const PI_GLOBAL pi_f2u*)((const PI_GLOBAL char*)base + cell * 4u);
+}
+
 struct GStore {
     PI_GLOBAL uint32_t* key;
     PI_GLOBAL float *x, *y, *z, *Sz, *Bm, *Cq, *Nq, *Pq, *Chl, *gen, *age;
@@ -222,7 +229,6 @@ struct DevState {
     uint32_t nstore;                   /* entries in store[scur] */
     uint32_t nA;                       /* of which the first nA are sorted by (cell, key) as laid out by the previous seg */
     uint32_t nsmall, nmix;             /* queued entries / re-sorted cells of this step */
-    uint32_t nredo;                    /* entries moved again with exact divisions (k_adv_fix) */
     uint32_t err, err_sp;
     uint32_t nbig;
     uint32_t nlong;                    /* cells longer than LONG_SEG this step */

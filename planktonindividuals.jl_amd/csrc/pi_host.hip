@@ -681,9 +681,8 @@ static void build_step(pi_model* m, float dt, Plan& P)
         aa.sz = std::sqrt(m->bgc[PI_B_kappa_vP] * dt);
         aa.seed = m->seed;
         aa.newc = m->newc; aa.stayf = m->stayf; aa.small = m->small; aa.cnts = m->cnts;
-        aa.dstamp = m->dstamp; aa.dcell = m->dcell; aa.cap = m->cap; aa.cap_total = m->ctot;
+        aa.dstamp = m->dstamp; aa.dcell = m->dcell; aa.cap = m->cap;
         P.add(K_ADV, "adv", pb, aa);
-        P.add(K_ADV_FIX, "adv", 64, aa);
     }
     {
         ScanIn sp = scan_plain(m->stayf, (uint32_t)(2 * m->ctot));      /* stayer prefix over the sorted part */
