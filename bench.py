@@ -63,10 +63,13 @@ LOGICAL = {
                               "k_cellsum", "k_plk", "k_scan_partials", "k_scan_top", "k_scan_final"]),
     "K_div":    dict(groups=["marks+scan"], per_ind=0, per_cell=0, per_slot=2,
                      kernels=["k_marks", "k_capcheck"]),
-    "K_dst3fl": dict(groups=["sweep_x", "fill", "sweep_y", "final"], per_ind=0, per_cell=164, per_slot=0,
-                     kernels=["k_sweep_x", "k_sweep_y", "k_fill_line", "k_final"]),
+    "K_dst3fl": dict(groups=["sweep_xy", "sweep_x", "fill", "sweep_y", "final"], per_ind=0, per_cell=164,
+                     per_slot=0, kernels=["k_xy", "k_sweep_x", "k_sweep_y", "k_fill_line", "k_final"]),
     "K_par":    dict(groups=["par"], per_ind=0, per_cell=8, per_slot=0, kernels=["k_par"]),
 }
+
+
+ALL_KERNELS = {k for L in LOGICAL.values() for k in L["kernels"]}
 
 
 def logical_bytes(name, n_live, n_cells, n_slots):
@@ -80,8 +83,27 @@ def step_bytes(n_live, n_cells, n_slots):
 
 
 def rocprof_base(name):
+    """Kernel name without arguments, template arguments and the landmask
+    variant digit (k_final0 / k_final<false> -> k_final)."""
     n = name.split("(")[0].replace("void ", "").strip()
-    return n.split("<")[0]
+    n = n.split("<")[0]
+    return n[:-1] if n[-1:] in ("0", "1") and n[:-1] in ALL_KERNELS else n
+
+
+VALU_PEAK = 256 * 2.4e9   # MI355X: 256 CUs x 2.4 GHz, one wave64 VALU instruction per CU per clock (non-packed)
+
+
+def measured_valu(logical):
+    """VALU wave-instructions per step of a logical kernel (SQ_INSTS_VALU from the
+    committed counter summary), or None."""
+    path = os.path.join(REPO, "profiles", "traffic_3d_256x256x64_2sp.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    tot = [v["valu_insts_per_step"] for k, v in t["kernels"].items()
+           if rocprof_base(k) in LOGICAL[logical]["kernels"] and "valu_insts_per_step" in v]
+    return sum(tot) if tot else None
 
 
 def measured_traffic(logical):
@@ -117,7 +139,7 @@ def rocpd_summary(db, title, cmd):
     return "\n".join(out)
 
 
-def pmc_summary(fetch_db, write_db, steps):
+def pmc_summary(fetch_db, write_db, steps, valu_db=None):
     """Per-kernel FETCH_SIZE / WRITE_SIZE (KiB) per step of a --pmc-probe run:
     dispatches of the last `steps` steps (the first step after an upload sorts
     from slot order and is not typical), and the scale factors of the copy probe
@@ -137,6 +159,7 @@ def pmc_summary(fetch_db, write_db, steps):
         return byk
 
     f, w = load(fetch_db, "FETCH_SIZE"), load(write_db, "WRITE_SIZE")
+    v = load(valu_db, "SQ_INSTS_VALU") if valu_db else {}
     probe = [k for k in f if "k_probe_copy" in k][0]
     scale = {"fetch": 262144.0 / f[probe][0], "write": 262144.0 / w[probe][0]}
     fin = [k for k in f if rocprof_base(k) == "k_finish"]
@@ -150,6 +173,8 @@ def pmc_summary(fetch_db, write_db, steps):
         last = slice(len(fv) - per_step * steps, len(fv))
         kernels[k] = {"fetch_kib_per_step": sum(fv[last]) / steps, "write_kib_per_step": sum(wv[last]) / steps,
                       "dispatches_per_step": per_step}
+        if k in v and len(v[k]) == len(fv):
+            kernels[k]["valu_insts_per_step"] = sum(v[k][last]) / steps
     return {"scale": scale, "steps_averaged": steps, "kernels": kernels}
 
 
@@ -337,14 +362,16 @@ def main():
                          "scale the counters), then --warmup + --steps steps without timing; prints no JSON")
     ap.add_argument("--rocpd-summary", nargs=3, metavar=("DB", "TITLE", "CMD"),
                     help="print the markdown kernel table of a rocprofv3 kernel-trace database and exit")
-    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DB", "WRITE_DB", "OUT_JSON"),
-                    help="write the per-step FETCH/WRITE summary of two --pmc-probe runs and exit")
+    ap.add_argument("--pmc-summary", nargs="+", metavar="DB",
+                    help="FETCH_DB WRITE_DB OUT_JSON [VALU_DB]: write the per-step FETCH/WRITE (and SQ_INSTS_VALU) "
+                         "summary of --pmc-probe runs and exit")
     args = ap.parse_args()
     if args.rocpd_summary:
         print(rocpd_summary(*args.rocpd_summary))
         return
     if args.pmc_summary:
-        res = pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.steps)
+        res = pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.steps,
+                          args.pmc_summary[3] if len(args.pmc_summary) > 3 else None)
         with open(args.pmc_summary[2], "w") as f:
             json.dump(res, f, indent=1)
         return
@@ -420,6 +447,13 @@ def main():
     roof = {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": ach / HBM_PEAK, "traffic": traffic, "bytes_per_launch": logical[dom]["bytes_per_step"],
             "launch_ms": logical[dom]["ms_per_step"]}
+    valu = measured_valu(dom)
+    issue = None
+    if valu:
+        ms = logical[dom]["ms_per_step"]
+        issue = {"bound": "valu-issue", "achieved": valu / (ms * 1e-3), "peak": VALU_PEAK,
+                 "unit": "wave64 VALU instructions/s", "frac": valu / (ms * 1e-3) / VALU_PEAK,
+                 "valu_insts_per_step": valu}
     bstep = step_bytes(live, n_cells, n_slots)
     step_roof = {"bytes_per_step": bstep, "achieved": bstep / (ms_step * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
                  "unit": "GB/s", "frac": bstep / (ms_step * 1e-3) / HBM_PEAK}
@@ -456,6 +490,7 @@ def main():
         "tracer_cell_steps_per_s": cell_steps,
         "live_individuals": [live0, live1],
         "roofline": roof,
+        "issue_roofline": issue,
         "step_roofline": step_roof,
         "cpu_baseline": base,
         "ensemble": ens,

@@ -779,19 +779,33 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
     const PI_GLOBAL float* TR_DFe = TRs + (long long)PI_DFe * g.ncell;
     const PI_GLOBAL float* PAR = gp(st->par[1 - st->pcur]);
     const PI_GLOBAL float* PAR0 = gp(st->par[st->pcur]);
+    /* per-species tables in LDS: the parameters (read by every individual, with
+     * per-lane species) and OGT, which depends on the parameters only */
     __shared__ uint32_t s_doff[PI_MAXSP];
+    __shared__ float s_par[PI_MAXSP * PI_NPARAMS];
+    __shared__ float s_ogt[PI_MAXSP];
     for (int i = threadIdx.x; i < a.nsp; i += blockDim.x) s_doff[i] = st->doff[i];
+    for (int i = threadIdx.x; i < a.nsp * PI_NPARAMS; i += blockDim.x) s_par[i] = a.params[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < a.nsp; i += blockDim.x) s_ogt[i] = ogt_of(s_par + i * PI_NPARAMS);
     __syncthreads();
     const uint32_t hhh = (uint32_t)gidx(g, g.Hx, g.Hy, g.Hz);
-    for (uint32_t e = K.b * blockDim.x + threadIdx.x; e < n; e += K.n * blockDim.x) {
-        const uint32_t key = a.pkey[e], src = a.psrc[e];
+    const uint32_t stride = K.n * blockDim.x;
+    /* the entry, its source and its cell are loaded one iteration ahead, so the
+     * state and tracer gathers of an entry are issued together at the top of
+     * its iteration (one memory latency per entry instead of three) */
+    uint32_t e = K.b * blockDim.x + threadIdx.x;
+    uint32_t key_n = 0, src_n = 0, c_n = 0;
+    if (e < n) { key_n = a.pkey[e]; src_n = a.psrc[e]; c_n = a.newc[src_n]; }
+    for (; e < n; e += stride) {
+        const uint32_t key = key_n, src = src_n, c = c_n;
+        if (e + stride < n) { key_n = a.pkey[e + stride]; src_n = a.psrc[e + stride]; c_n = a.newc[src_n]; }
         const int sp = (int)(key >> PI_SLOT_BITS);
         const uint32_t slot = key & PI_SLOT_MASK;
-        const float* q = a.params + sp * PI_NPARAMS;
+        const float* q = s_par + sp * PI_NPARAMS;
         const int dvid_type = (int)q[PI_P_dvid_type];
-        const float OGT = ogt_of(q);
+        const float OGT = s_ogt[sp];
         const uint8_t fl = IN.flags[src];
-        const uint32_t c = a.newc[src];
         PState s;
         s.x = IN.x[src]; s.y = IN.y[src]; s.z = IN.z[src];
         s.Sz = IN.Sz[src]; s.Bm = IN.Bm[src]; s.Cq = IN.Cq[src]; s.Nq = IN.Nq[src];
@@ -973,9 +987,14 @@ __device__ __forceinline__ void k_plk_body(const PlkArgs& a, Blk K)
     if (a.st->err) return;
     const int32_t it = (int32_t)(a.st->iteration + 1);
     const DevGrid& g = a.g;
-    for (long long c = K.b * (long long)blockDim.x + threadIdx.x; c < g.ncell;
-         c += (long long)K.n * blockDim.x) {
-        const uint32_t s0 = a.seg[c], s1 = a.seg[c + 1], n = s1 - s0;
+    const long long stride = (long long)K.n * blockDim.x;
+    long long c = K.b * (long long)blockDim.x + threadIdx.x;
+    /* the segment bounds are loaded one cell ahead */
+    uint32_t s0n = 0, s1n = 0;
+    if (c < g.ncell) { s0n = a.seg[c]; s1n = a.seg[c + 1]; }
+    for (; c < g.ncell; c += stride) {
+        const uint32_t s0 = s0n, s1 = s1n, n = s1 - s0;
+        if (c + stride < g.ncell) { s0n = a.seg[c + stride]; s1n = a.seg[c + stride + 1]; }
         if (n > LONG_SEG) continue;                  /* k_plk_long */
         PlkSums S{};
         if (n <= PLK_FAST) {
