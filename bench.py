@@ -140,41 +140,48 @@ def rocpd_summary(db, title, cmd):
 
 
 def pmc_summary(fetch_db, write_db, steps, valu_db=None):
-    """Per-kernel FETCH_SIZE / WRITE_SIZE (KiB) per step of a --pmc-probe run:
-    dispatches of the last `steps` steps (the first step after an upload sorts
-    from slot order and is not typical), and the scale factors of the copy probe
-    (k_probe_copy reads and writes 262144 KiB)."""
+    """Per-kernel FETCH_SIZE / WRITE_SIZE (KiB) (and SQ_INSTS_VALU) per step of
+    --pmc-probe runs: the dispatches of the last `steps` steps, a step being the
+    dispatches up to and including a k_finish (the first step after an upload
+    sorts from slot order and is not typical; model creation precedes the
+    steps), and the scale factors of the copy probe (k_probe_copy reads and
+    writes 262144 KiB)."""
     import collections
     import sqlite3
 
     def load(db, cname):
         c = sqlite3.connect(db)
         per = collections.defaultdict(float)
+        names = {}
         for d, k, n, v in c.execute("select dispatch_id, kernel_name, counter_name, value from counters_collection"):
             if n.startswith(cname):
-                per[(d, k.split("(")[0])] += v
-        byk = collections.defaultdict(list)
-        for (d, k), v in sorted(per.items()):
-            byk[k].append(v)
-        return byk
+                per[d] += v
+                names[d] = k.split("(")[0]
+        return per, names
 
-    f, w = load(fetch_db, "FETCH_SIZE"), load(write_db, "WRITE_SIZE")
-    v = load(valu_db, "SQ_INSTS_VALU") if valu_db else {}
-    probe = [k for k in f if "k_probe_copy" in k][0]
-    scale = {"fetch": 262144.0 / f[probe][0], "write": 262144.0 / w[probe][0]}
-    fin = [k for k in f if rocprof_base(k) == "k_finish"]
-    nsteps_total = len(f[fin[0]]) if fin else steps
+    def per_step(db, cname):
+        per, names = load(db, cname)
+        ids = sorted(per)
+        fin = [d for d in ids if rocprof_base(names[d]) == "k_finish"]
+        lo = fin[-steps - 1] if len(fin) > steps else -1
+        out = collections.defaultdict(lambda: [0.0, 0])
+        for d in ids:
+            if lo < d <= fin[-1]:
+                out[names[d]][0] += per[d] / steps
+                out[names[d]][1] += 1
+        probe = [d for d in ids if "k_probe_copy" in names[d]]
+        return out, (per[probe[0]] if probe else None)
+
+    f, fp = per_step(fetch_db, "FETCH_SIZE")
+    w, wp = per_step(write_db, "WRITE_SIZE")
+    v, _ = per_step(valu_db, "SQ_INSTS_VALU") if valu_db else ({}, None)
+    scale = {"fetch": 262144.0 / fp, "write": 262144.0 / wp}
     kernels = {}
-    for k in f:
-        fv, wv = f[k], w.get(k, [0.0] * len(f[k]))
-        if "probe" in k or len(fv) % max(1, nsteps_total):      # the probe; launches that are not per step
-            continue
-        per_step = max(1, len(fv) // max(1, nsteps_total))
-        last = slice(len(fv) - per_step * steps, len(fv))
-        kernels[k] = {"fetch_kib_per_step": sum(fv[last]) / steps, "write_kib_per_step": sum(wv[last]) / steps,
-                      "dispatches_per_step": per_step}
-        if k in v and len(v[k]) == len(fv):
-            kernels[k]["valu_insts_per_step"] = sum(v[k][last]) / steps
+    for k, (fv, cnt) in f.items():
+        kernels[k] = {"fetch_kib_per_step": fv, "write_kib_per_step": w[k][0] if k in w else 0.0,
+                      "dispatches_per_step": cnt // steps}
+        if k in v:
+            kernels[k]["valu_insts_per_step"] = v[k][0]
     return {"scale": scale, "steps_averaged": steps, "kernels": kernels}
 
 

@@ -206,3 +206,34 @@ def test_graph_replay_matches_eager(pi):
     assert A.iteration() == B.iteration() == 14
     bad = compare_models(A, B, 2)
     assert not bad, "\n".join(bad[:20])
+
+
+def test_parity_velocity_cancellation(pi, orc):
+    """A new velocity that nearly cancels the previous one: vel½ = (vel₀ + vel₁)/2
+    holds tiny non-zero values (2^-54 m/s) although neither set does, so the
+    advection must take its exact divisions in that step (the statistics of vel½
+    come from the kernel that forms it).  Results identical to the oracle."""
+    g = pi.RectilinearGrid(size=(16, 16, 4), x=(0, 16e3), y=(0, 16e3), z=(0, -40))
+    spec = pi.ModelSpec(g, max_individuals=3000, seed=23,
+                        phyto=pi.phyto_setup({"dvid_P": [1e-3], "mort_reg": [1.0]}, [1500], 1),
+                        bgc_params={"κhP": 1.0, "κvP": 1e-4, "κh": 10.0, "κv": 1e-4})
+    u, v, w = pi.workloads.cellular_flow(g, U=2.0, L=4e3)
+    u0, u1 = u.copy(), u.copy()
+    u0[:9] = np.float32(2.0 ** -30)
+    u1[:9] = np.float32(-(2.0 ** -30) + 2.0 ** -53)
+    assert np.all((u0[:9] + u1[:9]) * np.float32(0.5) == np.float32(2.0 ** -54))
+    parf = np.full((16, 16), 700.0, np.float32)
+    temp = np.full((16, 16, 4), 22.0, np.float32)
+    gm = pi.PlanktonModel(pi.HIPArch(0), spec=spec)
+    om = orc.OracleModel(spec)
+    for uu in (u0, u1):
+        for h in (gm._native, om):
+            h.set_velocity(uu, v, w)
+        for _ in range(3):
+            for h in (gm._native, om):
+                h.set_parf(parf)
+                h.set_temperature(temp)
+                h.step(60.0)
+    gm._native.sync()
+    bad = compare_models(gm._native, om, 1)
+    assert not bad, "\n".join(bad[:20])
