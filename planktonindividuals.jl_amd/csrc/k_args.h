@@ -18,8 +18,6 @@ struct AdvArgs {
     int32_t* dstamp;          /* per slot: iteration in which its occupant dies */
     uint32_t* dcell;          /* per slot: that occupant's cell in this step */
     long long cap;
-    int vi[3];                /* velocity sets of vel₀, vel½, vel₁ */
-    float lim[3];             /* N·(smallest spacing)/4 per direction, m */
 };
 
 struct PhysArgs {
@@ -163,7 +161,7 @@ struct ScanArgs {
 struct CapArgs { DevState* st; const uint32_t* G; int nsp, nw; };
 struct FinishArgs { DevState* st; const uint32_t* seg; long long ncell; int nsp; };
 struct CopyArgs { uint32_t* dst; const uint32_t* src; long long n; const DevState* st; };
-struct VelhalfArgs { const float* a[3]; const float* b[3]; float* h[3]; long long n; DevState* st; int hset; float thr; };
+struct VelhalfArgs { const float* a[3]; const float* b[3]; float* h[3]; long long n; const DevState* st; };
 struct AccumArgs { float* d[PI_NTRACERS]; int q[PI_NTRACERS]; int nq; long long n; const DevState* st; };
 
 /* The kernels of one step.  Each has a single-model entry point (arguments by

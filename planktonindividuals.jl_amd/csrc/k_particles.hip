@@ -40,10 +40,8 @@
  * u_itpl / v_itpl / w_itpl with ac = true (Individuals/Advection/interpolation.jl:1-83).
  * Corner indices are clamped into the padded array so that positions outside
  * the halo (not produced by the reference's boundary conditions) cannot fault.
- * The corners (x, x+1) are read in pairs, four 8-byte loads per call.
- * EXACT = false divides by the binary64 product (divn), which k_adv uses only
- * when no quotient can be subnormal. */
-template <bool EXACT>
+ * The corners (x, x+1) are read in pairs, four 8-byte loads per call: the
+ * kernel is bound by the issue of its gathers. */
 __device__ __forceinline__ float itpl(const DevGrid& g, const float* __restrict__ f,
                                       float x, float y, float z, int comp)
 {
@@ -68,14 +66,14 @@ __device__ __forceinline__ float itpl(const DevGrid& g, const float* __restrict_
     const float f100 = p00.y, f101 = p01.y, f110 = p10.y, f111 = p11.y;
     (void)sx;
     const float ox = 1.0f - xd, oy = 1.0f - yd, oz = 1.0f - zd;
-    float s = (EXACT ? divc(ox * oy * oz * f000, D0, R0) : divn(ox * oy * oz * f000, R0));
-    s = s + (EXACT ? divc(ox * oy * zd * f001, D1, R1) : divn(ox * oy * zd * f001, R1));
-    s = s + (EXACT ? divc(ox * yd * oz * f010, D0, R0) : divn(ox * yd * oz * f010, R0));
-    s = s + (EXACT ? divc(ox * yd * zd * f011, D1, R1) : divn(ox * yd * zd * f011, R1));
-    s = s + (EXACT ? divc(xd * oy * oz * f100, D0, R0) : divn(xd * oy * oz * f100, R0));
-    s = s + (EXACT ? divc(xd * oy * zd * f101, D1, R1) : divn(xd * oy * zd * f101, R1));
-    s = s + (EXACT ? divc(xd * yd * oz * f110, D0, R0) : divn(xd * yd * oz * f110, R0));
-    s = s + (EXACT ? divc(xd * yd * zd * f111, D1, R1) : divn(xd * yd * zd * f111, R1));
+    float s = divc(ox * oy * oz * f000, D0, R0);
+    s = s + divc(ox * oy * zd * f001, D1, R1);
+    s = s + divc(ox * yd * oz * f010, D0, R0);
+    s = s + divc(ox * yd * zd * f011, D1, R1);
+    s = s + divc(xd * oy * oz * f100, D0, R0);
+    s = s + divc(xd * oy * zd * f101, D1, R1);
+    s = s + divc(xd * yd * oz * f110, D0, R0);
+    s = s + divc(xd * yd * zd * f111, D1, R1);
     return s;
 }
 
@@ -110,36 +108,35 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
 
 /* RK4 advection (particle_advection.jl:2-56) and the random walk
  * (particle_diffusion.jl:2-33) of one individual, up to its new position. */
-template <bool EXACT>
 __device__ __forceinline__ void move(const AdvArgs& a, const DevGrid& g, float x0, float y0, float z0,
                                      float r0, float r1, float r2, long long zprev, float& x, float& y, float& z)
 {
     const float dt = a.dt;
     /* ---- RK4, particle_advection.jl:2-56 */
-    float u1 = itpl<EXACT>(g, a.v0.u, x0, y0, z0, 0);
-    float v1 = itpl<EXACT>(g, a.v0.v, x0, y0, z0, 1);
-    float w1 = itpl<EXACT>(g, a.v0.w, x0, y0, z0, 2);
+    float u1 = itpl(g, a.v0.u, x0, y0, z0, 0);
+    float v1 = itpl(g, a.v0.v, x0, y0, z0, 1);
+    float w1 = itpl(g, a.v0.w, x0, y0, z0, 2);
     float u2 = u1, v2 = v1, w2 = w1;
     float px = pbc(x0 + 0.5f * u1 * dt, g.Nx, g.tx);
     float py = pbc(y0 + 0.5f * v1 * dt, g.Ny, g.ty);
     float pz = pbc(z0 - 0.5f * w1 * dt, g.Nz, g.tz);
-    u1 = itpl<EXACT>(g, a.vh.u, px, py, pz, 0);
-    v1 = itpl<EXACT>(g, a.vh.v, px, py, pz, 1);
-    w1 = itpl<EXACT>(g, a.vh.w, px, py, pz, 2);
+    u1 = itpl(g, a.vh.u, px, py, pz, 0);
+    v1 = itpl(g, a.vh.v, px, py, pz, 1);
+    w1 = itpl(g, a.vh.w, px, py, pz, 2);
     u2 = u2 + u1 * 2.0f; v2 = v2 + v1 * 2.0f; w2 = w2 + w1 * 2.0f;
     px = pbc(x0 + 0.5f * u1 * dt, g.Nx, g.tx);
     py = pbc(y0 + 0.5f * v1 * dt, g.Ny, g.ty);
     pz = pbc(z0 - 0.5f * w1 * dt, g.Nz, g.tz);
-    u1 = itpl<EXACT>(g, a.vh.u, px, py, pz, 0);
-    v1 = itpl<EXACT>(g, a.vh.v, px, py, pz, 1);
-    w1 = itpl<EXACT>(g, a.vh.w, px, py, pz, 2);
+    u1 = itpl(g, a.vh.u, px, py, pz, 0);
+    v1 = itpl(g, a.vh.v, px, py, pz, 1);
+    w1 = itpl(g, a.vh.w, px, py, pz, 2);
     u2 = u2 + u1 * 2.0f; v2 = v2 + v1 * 2.0f; w2 = w2 + w1 * 2.0f;
     px = pbc(x0 + 1.0f * u1 * dt, g.Nx, g.tx);
     py = pbc(y0 + 1.0f * v1 * dt, g.Ny, g.ty);
     pz = pbc(z0 - 1.0f * w1 * dt, g.Nz, g.tz);
-    u1 = itpl<EXACT>(g, a.v1.u, px, py, pz, 0);
-    v1 = itpl<EXACT>(g, a.v1.v, px, py, pz, 1);
-    w1 = itpl<EXACT>(g, a.v1.w, px, py, pz, 2);
+    u1 = itpl(g, a.v1.u, px, py, pz, 0);
+    v1 = itpl(g, a.v1.v, px, py, pz, 1);
+    w1 = itpl(g, a.v1.w, px, py, pz, 2);
     u2 = u2 + u1; v2 = v2 + v1; w2 = w2 + w1;
     u2 = divc(u2, 6.0f, g.y6); v2 = divc(v2, 6.0f, g.y6); w2 = divc(w2, 6.0f, g.y6);
     x = pbc(x0 + 1.0f * u2 * dt, g.Nx, g.tx);
@@ -201,7 +198,7 @@ __device__ __forceinline__ void place_entry(const AdvArgs& a, DevState* st, cons
  * alias the loop's stores would be a vector load waiting for every outstanding
  * store.) */
 __device__ __forceinline__ void move_entry(const AdvArgs& a, const GStore& S, uint32_t e, long long iteration, uint8_t fl,
-                                           bool fast, float& x, float& y, float& z)
+                                           float& x, float& y, float& z)
 {
     const DevGrid& g = a.g;
     const uint32_t key = S.key[e];
@@ -211,13 +208,7 @@ __device__ __forceinline__ void move_entry(const AdvArgs& a, const GStore& S, ui
     pi_uniform3(a.seed, iteration, sp, PI_KIND_DIFFUSION, slot, &r0, &r1, &r2);
     const uint32_t cprev = S.cell[e];
     const long long zprev = (fl & F_ZERO) ? 0 : (long long)(cprev / (uint32_t)(g.Px * g.Py)) + 1;
-    const float x0 = S.x[e], y0 = S.y[e], z0 = S.z[e];
-    /* the fast form also needs the start inside [0, N]: every interpolation
-     * then sees x + 0.5 >= 0.5, so its non-zero weights are >= 2^-24 */
-    if (fast && x0 >= 0.0f && x0 <= (float)g.Nx && y0 >= 0.0f && y0 <= (float)g.Ny && z0 >= 0.0f && z0 <= (float)g.Nz)
-        move<false>(a, g, x0, y0, z0, r0, r1, r2, zprev, x, y, z);
-    else
-        move<true>(a, g, x0, y0, z0, r0, r1, r2, zprev, x, y, z);
+    move(a, g, S.x[e], S.y[e], S.z[e], r0, r1, r2, zprev, x, y, z);
 }
 
 __device__ __forceinline__ void k_adv_body(const AdvArgs& a, Blk K)
@@ -228,26 +219,11 @@ __device__ __forceinline__ void k_adv_body(const AdvArgs& a, Blk K)
     const uint32_t n = st->nstore;
     const uint32_t nA = st->nA;
     const long long iteration = st->iteration + 1;
-    /* The interpolation quotients (w·v)/Δ are divided by the binary64 product
-     * (divn, exact when the quotient is zero or normal) if none can be
-     * subnormal: every velocity value of the three sets is 0 or at least the
-     * model's threshold 2^-50·(largest spacing) (a non-zero weight product is
-     * >= 2^-72, so |w·v|/Δ >= 2^-122), and no stage can move an individual by
-     * more than N/4 cells, so positions stay in [0, N] (checked at the start
-     * per individual). */
-    bool fast = (st->vunsafe[a.vi[0]] | st->vunsafe[a.vi[1]] | st->vunsafe[a.vi[2]]) == 0u;
-    {
-        const float sr[3] = {a.sx, a.sy, a.sz};
-        for (int c = 0; c < 3; ++c) {
-            const float m = fmaxf(fmaxf(st->vmax[a.vi[0]][c], st->vmax[a.vi[1]][c]), st->vmax[a.vi[2]][c]);
-            fast = fast && m * a.dt + sr[c] <= a.lim[c];
-        }
-    }
     const uint32_t nloop = (n + 63u) & ~63u;          /* whole waves stay in the loop (wave_append) */
     for (uint32_t e = K.b * blockDim.x + threadIdx.x; e < nloop; e += K.n * blockDim.x) {
         const uint8_t fl = e < n ? S.flags[e] : (uint8_t)0;
         float x = 0.0f, y = 0.0f, z = 0.0f;
-        if (fl & F_AC) move_entry(a, S, e, iteration, fl, fast, x, y, z);
+        if (fl & F_AC) move_entry(a, S, e, iteration, fl, x, y, z);
         place_entry(a, st, S, e, nA, (int32_t)iteration, (fl & F_AC) != 0, fl, x, y, z);
         if (!(fl & F_AC) && e < nA) a.stayf[e] = 0u;
     }

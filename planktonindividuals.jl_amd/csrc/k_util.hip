@@ -170,26 +170,13 @@ __global__ void k_zero_u32(uint32_t* p, long long n, const DevState* st)
 __device__ __forceinline__ void k_velhalf_body(const VelhalfArgs& x, Blk K)
 {
     if (x.st->err) return;
-    /* and the statistics k_adv uses to choose its division form for this set
-     * (the counters were cleared when the velocity was set) */
-    uint32_t bad = 0;
     for (int c = 0; c < 3; ++c) {
         const float* a = x.a[c];
         const float* b = x.b[c];
         float* h = x.h[c];
-        float mx = 0.0f;
-        for (long long i = K.b * (long long)blockDim.x + threadIdx.x; i < x.n; i += (long long)K.n * blockDim.x) {
-            const float v = (a[i] + b[i]) * 0.5f;
-            h[i] = v;
-            const float av = fabsf(v);
-            bad += (av > 0.0f && av < x.thr) || !(av <= 3.0e38f) ? 1u : 0u;
-            mx = fmaxf(mx, av);
-        }
-        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-        if ((threadIdx.x & 63) == 0 && mx > 0.0f) atomicMax((uint32_t*)&x.st->vmax[x.hset][c], __float_as_uint(mx));
+        for (long long i = K.b * (long long)blockDim.x + threadIdx.x; i < x.n; i += (long long)K.n * blockDim.x)
+            h[i] = (a[i] + b[i]) * 0.5f;
     }
-    for (int o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o, 64);
-    if ((threadIdx.x & 63) == 0 && bad) atomicAdd(&x.st->vunsafe[x.hset], bad);
 }
 PI_KERNEL(k_velhalf, VelhalfArgs, 256)
 

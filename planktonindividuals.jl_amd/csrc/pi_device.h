@@ -160,10 +160,6 @@ __device__ __forceinline__ float divq(float a, float b, double yd, bool& sub)
     return f;
 }
 
-/* a / b from yd = RN_double(1/b) when a / b is known to be zero or in the normal
- * range: RN_float(RN_double(a·yd)) = RN_float(a/b) there (see divq) */
-__device__ __forceinline__ float divn(float a, double yd) { return (float)((double)a * yd); }
-
 /* Julia's max / min of Float32 (NaN if either operand is NaN, -0 < +0) are the
  * IEEE 754-2019 maximum / minimum: one instruction each on gfx950.  For NaN
  * operands the NaN may carry another payload than Julia's x - y. */
@@ -233,9 +229,6 @@ struct DevState {
     uint32_t nstore;                   /* entries in store[scur] */
     uint32_t nA;                       /* of which the first nA are sorted by (cell, key) as laid out by the previous seg */
     uint32_t nsmall, nmix;             /* queued entries / re-sorted cells of this step */
-    /* per velocity set: values with 0 < |v| < the model's threshold (see k_adv), and max |u|, |v|, |w| */
-    uint32_t vunsafe[3];
-    float vmax[3][3];
     uint32_t err, err_sp;
     uint32_t nbig;
     uint32_t nlong;                    /* cells longer than LONG_SEG this step */

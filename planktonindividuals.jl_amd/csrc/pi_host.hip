@@ -76,8 +76,6 @@ struct pi_model {
     uint64_t seed = 0;
     DevState* d_state = nullptr;
     DevState h_state{};
-    float vthr = 0.0f;        /* 2^-50 x the largest spacing: smaller non-zero velocities need exact divisions in k_adv */
-    float vlim[3]{};          /* N x the smallest spacing / 4 per direction (k_adv's bound on one stage's move) */
     SlotTable tab{};
     float* T1[PI_NTRACERS]{};
     float* T2[PI_NTRACERS]{};
@@ -275,14 +273,6 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
         DEVCALL(hipMemcpy(tabs, t.data(), 3ll * Pz * 8, hipMemcpyHostToDevice));
         DEVCALL(hipMemcpy(dvol, vol.data(), 4ll * Pz, hipMemcpyHostToDevice));
         g.ydzC = tabs; g.ydzF = tabs + Pz; g.yvol = tabs + 2 * Pz; g.vol = dvol;
-        float dmax = std::max(g.dx, g.dy), dzmin = gd.dzC[g.Hz];
-        for (int k = 0; k < Pz; ++k) dmax = std::max(dmax, gd.dzC[k]);
-        for (int k = g.Hz; k < g.Hz + g.Nz; ++k) dzmin = std::min(dzmin, gd.dzC[k]);
-        for (int k = 0; k < Pz; ++k) dzmin = std::min(dzmin, gd.dzC[k]);
-        m->vthr = std::ldexp(dmax, -50);
-        m->vlim[0] = 0.25f * (float)g.Nx * g.dx;
-        m->vlim[1] = 0.25f * (float)g.Ny * g.dy;
-        m->vlim[2] = 0.25f * (float)g.Nz * dzmin;
     }
     g.has_land = 0; g.lm = nullptr;
     if (gd.landmask) {
@@ -595,8 +585,6 @@ pi_status pi_set_velocity(pi_model* m, const float* u, const float* v, const flo
     const int nz[3] = {g.Nz, g.Nz, g.Nz + (g.tz == PI_BOUNDED)};
     const float* in[3] = {u, v, w};
     std::vector<float> pad(m->ncell);
-    uint32_t unsafe = 0;
-    float vmax[3] = {0.0f, 0.0f, 0.0f};
     for (int c = 0; c < 3; ++c) {
         std::fill(pad.begin(), pad.end(), 0.0f);
         HIPCHECK(hipMemcpy(pad.data(), m->vel[m->v1][c], m->ncell * 4, hipMemcpyDeviceToHost)); /* previous halos */
@@ -608,24 +596,6 @@ pi_status pi_set_velocity(pi_model* m, const float* u, const float* v, const flo
         if (c == 1) { host_fill(m, pad, 1, 2); host_fill(m, pad, 0, 0); host_fill(m, pad, 0, 1); host_fill(m, pad, 1, 0); host_fill(m, pad, 2, 0); host_fill(m, pad, 2, 1); }
         if (c == 2) { host_fill(m, pad, 2, 2); host_fill(m, pad, 0, 0); host_fill(m, pad, 0, 1); host_fill(m, pad, 1, 0); host_fill(m, pad, 1, 1); host_fill(m, pad, 2, 0); }
         HIPCHECK(hipMemcpy(m->vel[b][c], pad.data(), m->ncell * 4, hipMemcpyHostToDevice));
-        /* the statistics k_adv reads (padded arrays: halos included) */
-        float mx = 0.0f;
-        for (float x : pad) {
-            const float ax = std::fabs(x);
-            unsafe += (ax > 0.0f && ax < m->vthr) || !(ax <= 3.0e38f) ? 1u : 0u;
-            mx = std::max(mx, ax);
-        }
-        vmax[c] = mx;
-    }
-    {
-        /* this set's statistics; the set vel½ will use starts from zero (k_velhalf adds to it) */
-        const int vh = m->v0 != b ? 3 - m->v0 - b : (b + 1) % 3;
-        const uint32_t zero = 0;
-        const float fz[3] = {0.0f, 0.0f, 0.0f};
-        HIPCHECK(hipMemcpy(&m->d_state->vunsafe[b], &unsafe, 4, hipMemcpyHostToDevice));
-        HIPCHECK(hipMemcpy(&m->d_state->vmax[b][0], vmax, 12, hipMemcpyHostToDevice));
-        HIPCHECK(hipMemcpy(&m->d_state->vunsafe[vh], &zero, 4, hipMemcpyHostToDevice));
-        HIPCHECK(hipMemcpy(&m->d_state->vmax[vh][0], fz, 12, hipMemcpyHostToDevice));
     }
     m->v1 = b;
     m->vel_same = false;
@@ -684,7 +654,7 @@ static void build_step(pi_model* m, float dt, Plan& P)
                 for (vh = 0; vh < 3 && (vh == m->v0 || vh == m->v1); ++vh) {}
             }
             for (int c = 0; c < 3; ++c) { a.a[c] = m->vel[m->v0][c]; a.b[c] = m->vel[m->v1][c]; a.h[c] = m->vel[vh][c]; }
-            a.n = m->ncell; a.st = st; a.hset = vh; a.thr = m->vthr;
+            a.n = m->ncell; a.st = st;
         }
         P.add(K_VELHALF, "velhalf", m->vel_same ? 0 : grid_for(m->ncell, 256, 4096), a);
     }
@@ -712,8 +682,6 @@ static void build_step(pi_model* m, float dt, Plan& P)
         aa.seed = m->seed;
         aa.newc = m->newc; aa.stayf = m->stayf; aa.small = m->small; aa.cnts = m->cnts;
         aa.dstamp = m->dstamp; aa.dcell = m->dcell; aa.cap = m->cap;
-        aa.vi[0] = m->v0; aa.vi[1] = vh; aa.vi[2] = m->v1;
-        for (int c = 0; c < 3; ++c) aa.lim[c] = m->vlim[c];
         P.add(K_ADV, "adv", pb, aa);
     }
     {
