@@ -778,9 +778,12 @@ static void build_step(pi_model* m, float dt, Plan& P)
         fa.f = nullptr;
         fa.lines_full = 1;
         P.add(K_HALOCOPY, "fill", alias ? grid_for((long long)g.ncell * PI_NTRACERS, 256, 4096) : 0, fa);
-        fa.dim = 0; P.add(K_FILL, "fill", blocks_for((long long)g.Py * g.Pz * PI_NTRACERS), fa);
-        fa.dim = 1; P.add(K_FILL, "fill", blocks_for((long long)g.Px * g.Pz * PI_NTRACERS), fa);
-        fa.dim = 2; P.add(K_FILL, "fill", blocks_for((long long)g.Px * g.Py * PI_NTRACERS), fa);
+        fa.dim = 0; P.add(K_FILL, "fill", fxy ? 0 : blocks_for((long long)g.Py * g.Pz * PI_NTRACERS), fa);
+        fa.dim = 1; P.add(K_FILL, "fill", fxy ? 0 : blocks_for((long long)g.Px * g.Pz * PI_NTRACERS), fa);
+        fa.dim = 2; P.add(K_FILL, "fill", fxy ? 0 : blocks_for((long long)g.Px * g.Py * PI_NTRACERS), fa);
+        /* the same as one launch over the halo cells where halo 2 makes it a map (the grids k_xy takes) */
+        const long long nh = 4ll * g.Px * g.Py + 4ll * g.Nz * g.Px + 4ll * g.Nz * g.Ny;
+        P.add(K_HALOFILL, "fill", fxy ? grid_for(nh * PI_NTRACERS, 256, 8192) : 0, fa);
     }
     {
         AccumArgs a = zeroed<AccumArgs>();

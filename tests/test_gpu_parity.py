@@ -210,9 +210,9 @@ def test_graph_replay_matches_eager(pi):
 
 def test_parity_velocity_cancellation(pi, orc):
     """A new velocity that nearly cancels the previous one: vel½ = (vel₀ + vel₁)/2
-    holds tiny non-zero values (2^-54 m/s) although neither set does, so the
-    advection must take its exact divisions in that step (the statistics of vel½
-    come from the kernel that forms it).  Results identical to the oracle."""
+    holds tiny non-zero values (2^-54 m/s) in the step after the change, whose
+    interpolation quotients reach the subnormal range.  Results identical to the
+    oracle."""
     g = pi.RectilinearGrid(size=(16, 16, 4), x=(0, 16e3), y=(0, 16e3), z=(0, -40))
     spec = pi.ModelSpec(g, max_individuals=3000, seed=23,
                         phyto=pi.phyto_setup({"dvid_P": [1e-3], "mort_reg": [1.0]}, [1500], 1),
