@@ -604,16 +604,29 @@ __device__ __forceinline__ void k_par_impl(Blk K, DevState* st, DevGrid g, const
         const int i = col % g.Nx + 1, j = col / g.Nx + 1;
         const int ii = i + g.Hx, jj = j + g.Hy;
         float P = PARF_in[col];
-        for (int ki = 1; ki <= g.Nz; ++ki) {
-            const int kk = ki + g.Hz;
-            const long long c = gidx(g, ii, jj, kk);
-            const float vol = g.dx * g.dy * dzF_at(g, kk);
-            const float atten = (Chl[c] / vol * kc + kw) * dzF_at(g, kk);
-            const float ex = pi_expf(-atten);
-            const float p = P * (1.0f - ex) / atten;
-            par[c] = p;
-            diag_par[c] = diag_par[c] + p;
-            P = P * pi_expf(-atten);
+        /* levels in batches of PB: the loads and exps of a batch are
+         * independent; only the multiplications by P run in sequence */
+        constexpr int PB = 8;
+        for (int k0 = 1; k0 <= g.Nz; k0 += PB) {
+            float at[PB], ex[PB], dg[PB];
+#pragma unroll
+            for (int b = 0; b < PB; ++b) {
+                const int kk = min(k0 + b, g.Nz) + g.Hz;
+                const long long c = gidx(g, ii, jj, kk);
+                const float vol = g.dx * g.dy * dzF_at(g, kk);
+                at[b] = (Chl[c] / vol * kc + kw) * dzF_at(g, kk);
+                ex[b] = pi_expf(-at[b]);
+                dg[b] = diag_par[c];
+            }
+#pragma unroll
+            for (int b = 0; b < PB; ++b) {
+                if (k0 + b > g.Nz) break;
+                const long long c = gidx(g, ii, jj, k0 + b + g.Hz);
+                const float p = P * (1.0f - ex[b]) / at[b];
+                par[c] = p;
+                diag_par[c] = dg[b] + p;
+                P = P * ex[b];
+            }
         }
         PARF_out[col] = P;
     }

@@ -26,6 +26,18 @@ __device__ __forceinline__ uint32_t scan_load(const ScanIn& s, uint32_t i, uint3
     return __popc(~s.A[j] & valid);
 }
 
+/* the SCAN_ITEMS consecutive inputs of a thread: two 16-byte loads when they
+ * are whole (tile bases are multiples of 8 elements) */
+__device__ __forceinline__ void scan_load8(const ScanIn& s, uint32_t base, uint32_t n, uint32_t* v)
+{
+    if (!s.bits && base + SCAN_ITEMS <= n) {
+        const uint4 a = *(const uint4*)(s.a + base), b = *(const uint4*)(s.a + base + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+        for (int k = 0; k < SCAN_ITEMS; ++k) v[k] = scan_load(s, base + k, n);
+    }
+}
+
 /* exclusive scan of one value per thread across a 256-thread block */
 __device__ __forceinline__ uint32_t block_exscan(uint32_t v, uint32_t* total)
 {
@@ -56,8 +68,9 @@ __device__ __forceinline__ void k_scan_partials_impl(Blk K, ScanIn s, uint32_t* 
         return;
     }
     const uint32_t base = K.b * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
-    uint32_t sum = 0;
-    for (int k = 0; k < SCAN_ITEMS; ++k) sum += scan_load(s, base + k, n);
+    uint32_t v[SCAN_ITEMS], sum = 0;
+    scan_load8(s, base, n, v);
+    for (int k = 0; k < SCAN_ITEMS; ++k) sum += v[k];
     uint32_t tot;
     block_exscan(sum, &tot);
     if (threadIdx.x == 0) partials[K.b] = tot;
@@ -93,12 +106,18 @@ __device__ __forceinline__ void k_scan_final_impl(Blk K, ScanIn s, const uint32_
     if (K.b * SCAN_TILE > n) return;
     const uint32_t base = K.b * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS], sum = 0;
-    for (int k = 0; k < SCAN_ITEMS; ++k) { v[k] = scan_load(s, base + k, n); sum += v[k]; }
+    scan_load8(s, base, n, v);
+    for (int k = 0; k < SCAN_ITEMS; ++k) sum += v[k];
     uint32_t tot;
     uint32_t pre = block_exscan(sum, &tot) + partials[K.b];
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
-        if (base + k <= n && base + k < s.n) out[base + k] = pre;   /* out[n] = total when n < s.n */
-        pre += v[k];
+    uint32_t o[SCAN_ITEMS];
+    for (int k = 0; k < SCAN_ITEMS; ++k) { o[k] = pre; pre += v[k]; }
+    if (base + SCAN_ITEMS <= n && base + SCAN_ITEMS <= s.n) {        /* outputs are 16-byte aligned */
+        *(uint4*)(out + base) = make_uint4(o[0], o[1], o[2], o[3]);
+        *(uint4*)(out + base + 4) = make_uint4(o[4], o[5], o[6], o[7]);
+    } else {
+        for (int k = 0; k < SCAN_ITEMS; ++k)
+            if (base + k <= n && base + k < s.n) out[base + k] = o[k];   /* out[n] = total when n < s.n */
     }
 }
 __device__ __forceinline__ void k_scan_final_body(const ScanArgs& x, Blk K) { k_scan_final_impl(K, x.in, x.partials, x.out); }
