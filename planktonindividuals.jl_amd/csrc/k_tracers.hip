@@ -96,9 +96,9 @@ __device__ __forceinline__ bool small(float x) { return fabsf(x) < 0x1p100f; }  
 struct Sten { float dd0, bp, dnp, bm, dnm; bool ok; };
 
 template <bool LAND, bool Z>
-__device__ __forceinline__ Sten stencil(const DevGrid& g, const float* p, uint32_t f, uint32_t s)
+__device__ __forceinline__ Sten stencil_of(const DevGrid& g, uint32_t f, uint32_t s, float cm2, float cm1, float c0,
+                                           float cp1)
 {
-    const float cm2 = ld(p, f - 2 * s), cm1 = ld(p, f - s), c0 = ld(p, f), cp1 = ld(p, f + s);
     Sten r;
     if (Z) {
         r.dd0 = lmask<LAND>(g, f - s, f, cm1 - c0);
@@ -113,6 +113,12 @@ __device__ __forceinline__ Sten stencil(const DevGrid& g, const float* p, uint32
     }
     r.ok = small(cm2) && small(cm1) && small(c0) && small(cp1);
     return r;
+}
+/* the stencil of face f of field p: the values at f − 2s, f − s, f, f + s */
+template <bool LAND, bool Z>
+__device__ __forceinline__ Sten stencil(const DevGrid& g, const float* p, uint32_t f, uint32_t s)
+{
+    return stencil_of<LAND, Z>(g, f, s, ld(p, f - 2 * s), ld(p, f - s), ld(p, f), ld(p, f + s));
 }
 /* the term of the sign of T only, with divq; bad = the full formula (with
  * exact division) is needed */
@@ -492,38 +498,143 @@ __device__ __forceinline__ void cell_update(const FinalArgs& a, const DevGrid& g
 
 __device__ __forceinline__ int sink_index(int q) { return q == PI_PFe_inorg ? 0 : q == PI_Dust ? 1 : q == PI_PFe_bio ? 2 : -1; }
 
-/* tracers Q0..Q1-1 at one cell: the fluxes of the face below it, the updates,
- * the stores and the carried state (cu, cs, fz, dz) */
-template <bool LAND, int Q0, int Q1>
-__device__ __forceinline__ void final_half(const FinalArgs& a, const DevGrid& g, const float* C0, float* OUT,
-                                           const Face& Fb, bool zero, const ZCell& z, const float* tt, float* cu,
-                                           float* fz, float* dz, float* cs)
+/* k_final runs two threads per column run, one per tracer group: carbon and
+ * nitrogen (whose forcings read only tracers of the group) and the rest (whose
+ * forcings also read POC, tracer_forcings.jl:2-50).  fgrp(G, i) = the i-th
+ * tracer of group G. */
+#define FG_N 7
+__device__ __forceinline__ constexpr int fgrp(int G, int i)
 {
-    constexpr int NQ = Q1 - Q0;
+    return G == 0 ? (i == 0 ? PI_DIC : i == 1 ? PI_NH4 : i == 2 ? PI_NO3 : i == 3 ? PI_DOC : i == 4 ? PI_DON
+                                                                          : i == 5 ? PI_POC : PI_PON)
+                  : (i == 0 ? PI_PO4 : i == 1 ? PI_DFe : i == 2 ? PI_DOP : i == 3 ? PI_PFe_inorg : i == 4 ? PI_POP
+                                                                          : i == 5 ? PI_PFe_bio : PI_Dust);
+}
+
+/* face_fluxes (vertical) for the tracers of group G at face f from the values
+ * v[i][0..3] of tracer fgrp(G, i) at f − 2s, f − s, f, f + s; out[i] = its flux */
+template <bool LAND, int G>
+__device__ __forceinline__ void face_fluxes_grp(const DevGrid& g, const Face& F, bool zero, uint32_t f, uint32_t s,
+                                                const float (*v)[4], float* out)
+{
+    uint32_t need = 0;
+#pragma unroll
+    for (int i = 0; i < FG_N; ++i) {
+        const Sten t = stencil_of<LAND, true>(g, f, s, v[i][0], v[i][1], v[i][2], v[i][3]);
+        bool bad;
+        const float r = flux_fast(F, t, bad);
+        out[i] = zero ? 0.0f : r;
+        need |= (uint32_t)(bad && !zero) << i;
+    }
+    if (__builtin_expect(need != 0, 0)) {
+#pragma unroll
+        for (int i = 0; i < FG_N; ++i)
+            if ((need >> i) & 1u) out[i] = flux_full(F, stencil_of<LAND, true>(g, f, s, v[i][0], v[i][1], v[i][2], v[i][3]));
+    }
+}
+
+/* the tracers of group G at one cell: the fluxes of the face below it, the
+ * updates, the stores and the carried state (cu, cs, fz, dz; indexed by the
+ * position in the group, cs by sink_index) */
+template <bool LAND, int G>
+__device__ __forceinline__ void final_grp(const FinalArgs& a, const DevGrid& g, const float* C0, float* OUT,
+                                          const Face& Fb, bool zero, const ZCell& z, const float* tt, float* cu,
+                                          float* fz, float* dz, float* cs, float (*pw)[4])
+{
     const uint32_t sz = (uint32_t)g.Px * (uint32_t)g.Py, sy = (uint32_t)g.Px;
     const uint32_t cc = z.cc, c = z.c;
-    float f[NQ];
-    face_fluxes<LAND, true, Q0, Q1>(g, Fb, zero, c, sz, a.prev, f);
-    float outv[NQ], dv[NQ], cnv[NQ];
+    float f[FG_N];
+    face_fluxes_grp<LAND, G>(g, Fb, zero, c, sz, pw, f);
+    float outv[FG_N], dv[FG_N], cnv[FG_N];
 #pragma unroll
-    for (int q = Q0; q < Q1; ++q) {
-        const int i = q - Q0;
+    for (int i = 0; i < FG_N; ++i) {
+        const int q = fgrp(G, i);
         const float* co = fld(C0, g, q);
         cnv[i] = ld(co, c);
         const float pk = has_plk(q) ? ld(fld(a.plk, g, q), cc) : 0.0f;
         const int si = sink_index(q);
-        cell_update<LAND>(a, g, z, q, tt, cu[q], cnv[i], ld(co, cc - 1), ld(co, cc + 1), ld(co, cc - sy),
-                          ld(co, cc + sy), ld(fld(a.prev, g, q), cc), fz[q], f[i], dz[q], si >= 0 ? cs[si] : 0.0f,
+        /* pw[i][1] = the z-sweep input at cc */
+        cell_update<LAND>(a, g, z, q, tt, cu[i], cnv[i], ld(co, cc - 1), ld(co, cc + 1), ld(co, cc - sy),
+                          ld(co, cc + sy), pw[i][1], fz[i], f[i], dz[i], si >= 0 ? cs[si] : 0.0f,
                           pk, outv[i], dv[i]);
     }
 #pragma unroll
-    for (int q = Q0; q < Q1; ++q) {
-        const int i = q - Q0;
+    for (int i = 0; i < FG_N; ++i) {
+        const int q = fgrp(G, i);
         st(fld(OUT, g, q), cc, outv[i]);
-        fz[q] = f[i]; dz[q] = dv[i];
+        fz[i] = f[i]; dz[i] = dv[i];
         const int si = sink_index(q);
-        if (si >= 0) cs[si] = cu[q];
-        cu[q] = cnv[i];
+        if (si >= 0) cs[si] = cu[i];
+        cu[i] = cnv[i];
+    }
+}
+
+/* one column run (cells k0..k1-1 of column i, j) for the tracers of group G */
+template <bool LAND, int G>
+__device__ __forceinline__ void final_run(const FinalArgs& a, const DevGrid& g, const float* C0, float* OUT, int i,
+                                          int j, int r)
+{
+    const uint32_t sz = (uint32_t)g.Px * (uint32_t)g.Py;
+    const float dt = a.dt;
+    const float Az = g.dx * g.dy;
+    const float kvAz = a.kv * Az;
+    const int k0 = g.Hz + 1 + r * RUN_Z;
+    const int k1 = min(k0 + RUN_Z, g.Nz + g.Hz + 1);     /* cells k0..k1-1, faces k0..k1 */
+    uint32_t c = (uint32_t)gidx(g, i, j, k0);            /* cell below the current face */
+    float cu[FG_N], cs[3] = {0.0f, 0.0f, 0.0f}, fz[FG_N], dz[FG_N];
+    /* z-sweep input around the current face c: pw[i] = prev at c − 2sz, c − sz,
+     * c, c + sz (slid up one level per cell, one new load per tracer) */
+    float pw[FG_N][4];
+#pragma unroll
+    for (int x = 0; x < FG_N; ++x) {
+        const float* p = fld(a.prev, g, fgrp(G, x));
+        pw[x][0] = ld(p, c - 2 * sz); pw[x][1] = ld(p, c - sz); pw[x][2] = ld(p, c); pw[x][3] = ld(p, c + sz);
+    }
+    Face Fa = face(g, ld(a.w, c), dt, dzC_at(g, k0), g.ydzC[k0 - 1], Az);
+    face_fluxes_grp<LAND, G>(g, Fa, false, c, sz, pw, fz);
+    {
+        const float dzf = dzF_at(g, k0);
+        const double ydzf = g.ydzF[k0 - 1];
+#pragma unroll
+        for (int x = 0; x < FG_N; ++x) {
+            const float* co = fld(C0, g, fgrp(G, x));
+            const float above = ld(co, c - sz), here = ld(co, c);
+            dz[x] = kvAz * divc(lmask<LAND>(g, c - sz, c, above - here), dzf, ydzf);
+            cu[x] = here;
+        }
+        if (G == 1) {
+            cs[0] = ld(fld(C0, g, PI_PFe_inorg), c - sz);
+            cs[1] = ld(fld(C0, g, PI_Dust), c - sz);
+            cs[2] = ld(fld(C0, g, PI_PFe_bio), c - sz);
+        }
+    }
+#pragma unroll 1
+    for (int kk = k0 + 1; kk <= k1; ++kk) {
+        c += sz;
+#pragma unroll
+        for (int x = 0; x < FG_N; ++x) {
+            pw[x][0] = pw[x][1]; pw[x][1] = pw[x][2]; pw[x][2] = pw[x][3];
+            pw[x][3] = ld(fld(a.prev, g, fgrp(G, x)), c + sz);
+        }
+        ZCell z;
+        z.c = c; z.cc = c - sz; z.k = kk - 1;
+        const Face Fb = face(g, ld(a.w, c), dt, dzC_at(g, kk), g.ydzC[kk - 1], Az);
+        const bool zero = kk == g.Nz + g.Hz + 1 && g.tz == PI_BOUNDED;
+        z.dT = lmask<LAND>(g, z.cc, c, Fa.T - Fb.T);
+        z.vol = g.vol[z.k - 1];
+        z.yvol = g.yvol[z.k - 1];
+        z.coef = divc(dt, z.vol, z.yvol);
+        z.khAx = a.kh * (g.dy * dzF_at(g, z.k));
+        z.khAy = a.kh * (g.dx * dzF_at(g, z.k));
+        z.inv_vol = divc(1.0f, z.vol, z.yvol);
+        z.dzf = dzF_at(g, kk);
+        z.ydzf = g.ydzF[kk - 1];
+        float tt[PI_NTRACERS] = {};    /* tracer_temp = max(0, c) (tracer_forcings.jl:3-5) */
+#pragma unroll
+        for (int x = 0; x < FG_N; ++x) tt[fgrp(G, x)] = dmaxf(0.0f, cu[x]);
+        if (G == 1) tt[PI_POC] = dmaxf(0.0f, ld(fld(C0, g, PI_POC), z.cc));
+        final_grp<LAND, G>(a, g, C0, OUT, Fb, zero, z, tt, cu, fz, dz, cs, pw);
+        Fa = Fb;
     }
 }
 
@@ -534,61 +645,18 @@ __device__ __forceinline__ void k_final_body(const FinalArgs& a, Blk K)
     if (S->err) return;
     const DevGrid& g = a.g;
     const int nrun = (g.Nz + RUN_Z - 1) / RUN_Z;
-    const long long nthr = (long long)g.Nx * g.Ny * nrun;
+    const long long nthr = (long long)g.Nx * g.Ny * nrun * 2;
     const float* C0 = S->tr[S->tcur][0];
     float* OUT = S->tr[S->tcur ^ 1][0];
-    const uint32_t sz = (uint32_t)g.Px * (uint32_t)g.Py;
-    const float dt = a.dt;
-    const float Az = g.dx * g.dy;
-    const float kvAz = a.kv * Az;
     for (long long t = K.b * (long long)blockDim.x + threadIdx.x; t < nthr; t += (long long)K.n * blockDim.x) {
+        /* t = i + Nx·(group + 2·(j + Ny·run)): the two groups of a row are neighbouring blocks */
         const int i = (int)(t % g.Nx) + 1 + g.Hx;
-        const int j = (int)((t / g.Nx) % g.Ny) + 1 + g.Hy;
-        const int r = (int)(t / ((long long)g.Nx * g.Ny));
-        const int k0 = g.Hz + 1 + r * RUN_Z;
-        const int k1 = min(k0 + RUN_Z, g.Nz + g.Hz + 1);     /* cells k0..k1-1, faces k0..k1 */
-        uint32_t c = (uint32_t)gidx(g, i, j, k0);            /* cell below the current face */
-        float cu[PI_NTRACERS], cs[3], fz[PI_NTRACERS], dz[PI_NTRACERS];
-        Face Fa = face(g, ld(a.w, c), dt, dzC_at(g, k0), g.ydzC[k0 - 1], Az);
-        face_fluxes<LAND, true, 0, 7>(g, Fa, false, c, sz, a.prev, fz);
-        face_fluxes<LAND, true, 7, 14>(g, Fa, false, c, sz, a.prev, fz + 7);
-        {
-            const float dzf = dzF_at(g, k0);
-            const double ydzf = g.ydzF[k0 - 1];
-#pragma unroll
-            for (int q = 0; q < PI_NTRACERS; ++q) {
-                const float* co = fld(C0, g, q);
-                const float above = ld(co, c - sz), here = ld(co, c);
-                dz[q] = kvAz * divc(lmask<LAND>(g, c - sz, c, above - here), dzf, ydzf);
-                cu[q] = here;
-            }
-            cs[0] = ld(fld(C0, g, PI_PFe_inorg), c - sz);
-            cs[1] = ld(fld(C0, g, PI_Dust), c - sz);
-            cs[2] = ld(fld(C0, g, PI_PFe_bio), c - sz);
-        }
-#pragma unroll 1
-        for (int kk = k0 + 1; kk <= k1; ++kk) {
-            c += sz;
-            ZCell z;
-            z.c = c; z.cc = c - sz; z.k = kk - 1;
-            const Face Fb = face(g, ld(a.w, c), dt, dzC_at(g, kk), g.ydzC[kk - 1], Az);
-            const bool zero = kk == g.Nz + g.Hz + 1 && g.tz == PI_BOUNDED;
-            z.dT = lmask<LAND>(g, z.cc, c, Fa.T - Fb.T);
-            z.vol = g.vol[z.k - 1];
-            z.yvol = g.yvol[z.k - 1];
-            z.coef = divc(dt, z.vol, z.yvol);
-            z.khAx = a.kh * (g.dy * dzF_at(g, z.k));
-            z.khAy = a.kh * (g.dx * dzF_at(g, z.k));
-            z.inv_vol = divc(1.0f, z.vol, z.yvol);
-            z.dzf = dzF_at(g, kk);
-            z.ydzf = g.ydzF[kk - 1];
-            float tt[PI_NTRACERS];    /* tracer_temp = max(0, c) (tracer_forcings.jl:3-5) */
-#pragma unroll
-            for (int q = 0; q < PI_NTRACERS; ++q) tt[q] = dmaxf(0.0f, cu[q]);
-            final_half<LAND, 0, 7>(a, g, C0, OUT, Fb, zero, z, tt, cu, fz, dz, cs);
-            final_half<LAND, 7, 14>(a, g, C0, OUT, Fb, zero, z, tt, cu, fz, dz, cs);
-            Fa = Fb;
-        }
+        const long long u = t / g.Nx;
+        const int G = (int)(u & 1);
+        const int j = (int)((u >> 1) % g.Ny) + 1 + g.Hy;
+        const int r = (int)((u >> 1) / g.Ny);
+        if (G == 0) final_run<LAND, 0>(a, g, C0, OUT, i, j, r);
+        else final_run<LAND, 1>(a, g, C0, OUT, i, j, r);
     }
 }
 PI_KERNEL2(k_final0, k_final_body<false>, FinalArgs, 256)
@@ -856,7 +924,7 @@ long long sweep_x_blocks(const DevGrid& g)
     return (nwave + 3) / 4;
 }
 long long sweep_y_items(const DevGrid& g) { return (long long)g.Nx * ((g.Ny + RUN_Y - 1) / RUN_Y) * g.Nz; }
-long long final_items(const DevGrid& g) { return (long long)g.Nx * g.Ny * ((g.Nz + RUN_Z - 1) / RUN_Z); }
+long long final_items(const DevGrid& g) { return (long long)g.Nx * g.Ny * ((g.Nz + RUN_Z - 1) / RUN_Z) * 2; }
 
 void reg_tracers(KInfo* k)
 {
