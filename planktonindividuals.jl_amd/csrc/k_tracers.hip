@@ -404,7 +404,7 @@ __device__ __forceinline__ void k_halo_fill_body(const FillArgs& a, Blk K)
 PI_KERNEL(k_halo_fill, FillArgs, 256)
 
 /* ---------------------------------------------- z sweep + the rest of the update */
-#define RUN_Z 16
+#define RUN_Z 8
 
 /* the tracers that receive particle exchange (plk), tracer_update.jl:22-43 */
 __device__ __forceinline__ constexpr bool has_plk(int q)
