@@ -44,7 +44,9 @@
  * kernel is bound by the texture data path returning its 48 gathers per
  * individual (TD busy ~93% of the kernel on MI355X): fewer VALU instructions
  * (a 5-instruction exact quotient) and a lane order with fewer cache lines
- * per gather were both measured to leave its time unchanged. */
+ * per gather were both measured to leave its time unchanged, and staging each
+ * block's velocity neighbourhood in LDS (27 rows per velocity set) relieved
+ * the texture path but made the kernel VALU-bound and 20% slower. */
 __device__ __forceinline__ float itpl(const DevGrid& g, const float* __restrict__ f,
                                       float x, float y, float z, int comp)
 {

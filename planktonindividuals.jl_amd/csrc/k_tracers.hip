@@ -682,7 +682,6 @@ PI_KERNEL2(k_final1, k_final_body<true>, FinalArgs, 256)
  * recomputed bit for bit.  Tracers are processed in two passes of 7. */
 #define RUN_XY 16
 #define XY_T 256
-#define PI_WAIT_VM0 0x0F70        /* s_waitcnt vmcnt(0): vector memory (incl. loads into LDS) done */
 
 __device__ __forceinline__ int map_row(int j, int N, int topo)
 {

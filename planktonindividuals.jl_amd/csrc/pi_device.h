@@ -83,6 +83,7 @@ struct Store {
  * instead: SGPR-base addressing with 32-bit offsets, vector-memory counter only. */
 #define PI_GLOBAL __attribute__((address_space(1)))
 template <typename T> __device__ __forceinline__ PI_GLOBAL T* gp(T* p) { return (PI_GLOBAL T*)p; }
+#define PI_WAIT_VM0 0x0F70        /* s_waitcnt vmcnt(0): vector memory (incl. loads into LDS) done */
 
 /* Loads and stores through global-address-space pointers: a uniform (SGPR) base plus a
  * 32-bit byte offset, so a load needs no 64-bit address arithmetic and loads at

@@ -684,7 +684,9 @@ static void build_step(pi_model* m, float dt, Plan& P)
         aa.seed = m->seed;
         aa.newc = m->newc; aa.stayf = m->stayf; aa.small = m->small; aa.cnts = m->cnts;
         aa.dstamp = m->dstamp; aa.dcell = m->dcell; aa.cap = m->cap;
-        P.add(K_ADV, "adv", pb, aa);
+        /* up to 16384 blocks: with 8.4M individuals (two entries per thread)
+         * ~5% faster than 4096 blocks; 32768 is no faster, 65536 much slower */
+        P.add(K_ADV, "adv", grid_for(2 * m->ctot, 256, 16384), aa);
     }
     {
         ScanIn sp = scan_plain(m->stayf, (uint32_t)(2 * m->ctot));      /* stayer prefix over the sorted part */
