@@ -60,7 +60,7 @@ LOGICAL = {
                      kernels=["k_phys"]),
     "K_bin":    dict(groups=["bin", "plk"], per_ind=4, per_cell=48, per_slot=0,
                      kernels=["k_cellcount", "k_place_stay", "k_place_small", "k_mixsort", "k_bigcells",
-                              "k_cellsum", "k_plk", "k_scan_partials", "k_scan_top", "k_scan_final"]),
+                              "k_cellsum", "k_plk", "k_scan_partials", "k_scan_final"]),
     "K_div":    dict(groups=["marks+scan"], per_ind=0, per_cell=0, per_slot=2,
                      kernels=["k_marks", "k_capcheck"]),
     "K_dst3fl": dict(groups=["sweep_xy", "sweep_x", "fill", "sweep_y", "final"], per_ind=0, per_cell=164,

@@ -78,38 +78,35 @@ __device__ __forceinline__ void k_scan_partials_impl(Blk K, ScanIn s, uint32_t* 
 __device__ __forceinline__ void k_scan_partials_body(const ScanArgs& x, Blk K) { k_scan_partials_impl(K, x.in, x.partials); }
 PI_KERNEL(k_scan_partials, ScanArgs, 256)
 
-/* single block: exclusive scan of the per-tile partials, total -> out[n] */
-__device__ __forceinline__ void k_scan_top_impl(Blk K, uint32_t* partials, uint32_t nb, uint32_t* out, uint32_t n)
+/* Tile b's exclusive prefix (the sum of the partials of tiles 0..b-1, a block
+ * reduction over at most a few thousand values), then its outputs; the last
+ * tile also writes the total to out[n].  One kernel where a single-block scan
+ * of the partials would be a launch of its own. */
+__device__ __forceinline__ void k_scan_final_impl(Blk K, ScanIn s, const uint32_t* partials, uint32_t* out, uint32_t nb)
 {
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < nb; base += SCAN_TILE) {
-        const uint32_t i0 = base + threadIdx.x * SCAN_ITEMS;
-        uint32_t v[SCAN_ITEMS], sum = 0;
-        for (int k = 0; k < SCAN_ITEMS; ++k) { v[k] = (i0 + k < nb) ? partials[i0 + k] : 0; sum += v[k]; }
-        uint32_t tot;
-        uint32_t pre = block_exscan(sum, &tot) + carry;
-        for (int k = 0; k < SCAN_ITEMS; ++k) {
-            if (i0 + k < nb) partials[i0 + k] = pre;
-            pre += v[k];
+    __shared__ uint32_t s_pre;
+    uint32_t acc = 0;
+    for (uint32_t i = threadIdx.x * 4u; i < K.b; i += blockDim.x * 4u) {
+        if (i + 4u <= K.b) {
+            const uint4 p = *(const uint4*)(partials + i);                  /* i is a multiple of 4 */
+            acc += p.x + p.y + p.z + p.w;
+        } else {
+            for (uint32_t k = i; k < K.b; ++k) acc += partials[k];
         }
-        carry += tot;
-        __syncthreads();
     }
-    if (threadIdx.x == 0) out[n] = carry;
-}
-__device__ __forceinline__ void k_scan_top_body(const ScanArgs& x, Blk K) { k_scan_top_impl(K, x.partials, x.nb, x.out, x.in.n); }
-PI_KERNEL(k_scan_top, ScanArgs, 256)
-
-__device__ __forceinline__ void k_scan_final_impl(Blk K, ScanIn s, const uint32_t* partials, uint32_t* out)
-{
+    uint32_t atot;
+    block_exscan(acc, &atot);
+    if (threadIdx.x == 0) s_pre = atot;
+    __syncthreads();
     const uint32_t n = scan_len(s);
-    if (K.b * SCAN_TILE > n) return;
     const uint32_t base = K.b * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS], sum = 0;
     scan_load8(s, base, n, v);
     for (int k = 0; k < SCAN_ITEMS; ++k) sum += v[k];
     uint32_t tot;
-    uint32_t pre = block_exscan(sum, &tot) + partials[K.b];
+    uint32_t pre = block_exscan(sum, &tot) + s_pre;
+    if (K.b == nb - 1 && threadIdx.x == 0) out[s.n] = s_pre + tot;     /* values past n are 0 */
+    if (K.b * SCAN_TILE > n) return;
     uint32_t o[SCAN_ITEMS];
     for (int k = 0; k < SCAN_ITEMS; ++k) { o[k] = pre; pre += v[k]; }
     if (base + SCAN_ITEMS <= n && base + SCAN_ITEMS <= s.n) {        /* outputs are 16-byte aligned */
@@ -120,7 +117,7 @@ __device__ __forceinline__ void k_scan_final_impl(Blk K, ScanIn s, const uint32_
             if (base + k <= n && base + k < s.n) out[base + k] = o[k];   /* out[n] = total when n < s.n */
     }
 }
-__device__ __forceinline__ void k_scan_final_body(const ScanArgs& x, Blk K) { k_scan_final_impl(K, x.in, x.partials, x.out); }
+__device__ __forceinline__ void k_scan_final_body(const ScanArgs& x, Blk K) { k_scan_final_impl(K, x.in, x.partials, x.out, x.nb); }
 PI_KERNEL(k_scan_final, ScanArgs, 256)
 
 /* totals, daughter offsets and the capacity check (plankton_update.jl:18-22:
@@ -392,7 +389,6 @@ void reg_util(KInfo* k)
 {
     PI_REG(k, K_VELHALF, k_velhalf, VelhalfArgs, 256, true);
     PI_REG(k, K_SCAN_P, k_scan_partials, ScanArgs, SCAN_THREADS, false);
-    PI_REG(k, K_SCAN_T, k_scan_top, ScanArgs, SCAN_THREADS, false);
     PI_REG(k, K_SCAN_F, k_scan_final, ScanArgs, SCAN_THREADS, false);
     PI_REG(k, K_CAPCHECK, k_capcheck, CapArgs, 64, false);
     PI_REG(k, K_ACCUM, k_accum, AccumArgs, 256, true);
