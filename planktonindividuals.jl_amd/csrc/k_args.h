@@ -173,6 +173,8 @@ enum KId {
     K_SWEEPX0, K_SWEEPX1, K_XY0, K_XY1, K_SWEEPY0, K_SWEEPY1, K_HALOCOPY, K_FILL, K_HALOFILL, K_FINAL0, K_FINAL1, K_ACCUM, K_COPY,
     K_FINISH, K_NKID
 };
+/* plan markers (not kernels): fork to / join from the side stream */
+enum { K_FORK = K_NKID, K_JOIN = K_NKID + 1 };
 struct KInfo {
     const void* one;          /* __global__ void NAME(ARGS) */
     const void* multi;        /* __global__ void NAME_m(const ARGS*, const uint4*) */
