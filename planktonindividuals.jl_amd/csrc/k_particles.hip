@@ -250,11 +250,24 @@ __device__ __forceinline__ void k_marks_impl(Blk K, const DevState* st, const ui
         const long long s0 = wd * 32, s1 = s0 + 32 < cap ? s0 + 32 : cap;
         const uint8_t* f = sflag + sp * cap;
         uint32_t A = 0, D = 0;
-        for (long long sl = s0; sl < s1; ++sl) {
-            const uint8_t b = f[sl];
-            const bool alive = (b & F_AC) && !(b & (F_GRAZ | F_MORT));
-            A |= (uint32_t)alive << (sl - s0);
-            D |= (uint32_t)(alive && (b & F_DVID)) << (sl - s0);
+        if (s1 - s0 == 32 && ((uintptr_t)(f + s0) & 15u) == 0) {
+            /* the word's 32 flag bytes as two 16-byte loads */
+            const uint4 lo = *(const uint4*)(f + s0), hi = *(const uint4*)(f + s0 + 16);
+            const uint32_t q[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+            for (int k = 0; k < 32; ++k) {
+                const uint32_t b = (q[k >> 2] >> ((k & 3) * 8)) & 0xFFu;
+                const bool alive = (b & F_AC) && !(b & (F_GRAZ | F_MORT));
+                A |= (uint32_t)alive << k;
+                D |= (uint32_t)(alive && (b & F_DVID)) << k;
+            }
+        } else {
+            for (long long sl = s0; sl < s1; ++sl) {
+                const uint8_t b = f[sl];
+                const bool alive = (b & F_AC) && !(b & (F_GRAZ | F_MORT));
+                A |= (uint32_t)alive << (sl - s0);
+                D |= (uint32_t)(alive && (b & F_DVID)) << (sl - s0);
+            }
         }
         bmA[w] = A;
         bmD[w] = D;
