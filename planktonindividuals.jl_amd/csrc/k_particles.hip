@@ -612,41 +612,78 @@ __device__ __forceinline__ void k_bigcells_body(const BinArgs& x, Blk K) { k_big
 PI_KERNEL(k_bigcells, BinArgs, 256)
 
 /* calc_par_kernel! looped over levels (Individuals/utils.jl:55-68, time_step.jl:63-67) */
+#define PAR_TC 64        /* columns per tile */
+#define PAR_NZ 64        /* levels a tile holds in LDS (deeper grids take the column loop) */
 __device__ __forceinline__ void k_par_impl(Blk K, DevState* st, DevGrid g, const float* Chl, const float* PARF_in,
                                              float* PARF_out, float* diag_par, float kc, float kw)
 {
     if (st->err) return;
     PI_GLOBAL float* par = gp(st->par[1 - st->pcur]);
     const int ncol = g.Nx * g.Ny;
-    for (int col = K.b * blockDim.x + threadIdx.x; col < ncol; col += K.n * blockDim.x) {
-        const int i = col % g.Nx + 1, j = col / g.Nx + 1;
-        const int ii = i + g.Hx, jj = j + g.Hy;
-        float P = PARF_in[col];
-        /* levels in batches of PB: the loads and exps of a batch are
-         * independent; only the multiplications by P run in sequence */
-        constexpr int PB = 8;
-        for (int k0 = 1; k0 <= g.Nz; k0 += PB) {
-            float at[PB], ex[PB], dg[PB];
-#pragma unroll
-            for (int b = 0; b < PB; ++b) {
-                const int kk = min(k0 + b, g.Nz) + g.Hz;
+    if (g.Nz > PAR_NZ) {
+        for (int col = K.b * blockDim.x + threadIdx.x; col < ncol; col += K.n * blockDim.x) {
+            const int ii = col % g.Nx + 1 + g.Hx, jj = col / g.Nx + 1 + g.Hy;
+            float P = PARF_in[col];
+            for (int k = 1; k <= g.Nz; ++k) {
+                const int kk = k + g.Hz;
                 const long long c = gidx(g, ii, jj, kk);
                 const float vol = g.dx * g.dy * dzF_at(g, kk);
-                at[b] = (Chl[c] / vol * kc + kw) * dzF_at(g, kk);
-                ex[b] = pi_expf(-at[b]);
-                dg[b] = diag_par[c];
-            }
-#pragma unroll
-            for (int b = 0; b < PB; ++b) {
-                if (k0 + b > g.Nz) break;
-                const long long c = gidx(g, ii, jj, k0 + b + g.Hz);
-                const float p = P * (1.0f - ex[b]) / at[b];
+                const float at = (Chl[c] / vol * kc + kw) * dzF_at(g, kk);
+                const float ex = pi_expf(-at);
+                const float p = P * (1.0f - ex) / at;
                 par[c] = p;
-                diag_par[c] = dg[b] + p;
-                P = P * ex[b];
+                diag_par[c] = diag_par[c] + p;
+                P = P * ex;
             }
+            PARF_out[col] = P;
         }
-        PARF_out[col] = P;
+        return;
+    }
+    /* Tiles of PAR_TC columns: the attenuations and their exponentials of every
+     * (column, level) in parallel, then one lane per column runs the product
+     * P·(1 − ex), P ← P·ex down the column (the only sequential part), then the
+     * quotients by the attenuation and the stores in parallel.  Same operations
+     * in the same order as the column loop above. */
+    __shared__ float s_at[PAR_NZ * PAR_TC], s_q[PAR_NZ * PAR_TC];
+    const int nitem = g.Nz * PAR_TC;
+    constexpr int NR = PAR_NZ * PAR_TC / 256;          /* items per thread (blockDim.x = 256) */
+    for (int col0 = K.b * PAR_TC; col0 < ncol; col0 += K.n * PAR_TC) {
+        float dgv[NR];                                 /* diag_par of the items, loaded with the attenuations */
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int it = (int)threadIdx.x + r * 256;
+            const int col = col0 + (it & (PAR_TC - 1)), kk = it / PAR_TC + 1 + g.Hz;
+            dgv[r] = 0.0f;
+            if (it >= nitem || col >= ncol) continue;
+            const long long c = gidx(g, col % g.Nx + 1 + g.Hx, col / g.Nx + 1 + g.Hy, kk);
+            const float vol = g.dx * g.dy * dzF_at(g, kk);
+            const float at = (Chl[c] / vol * kc + kw) * dzF_at(g, kk);
+            dgv[r] = diag_par[c];
+            s_at[it] = at;
+            s_q[it] = pi_expf(-at);
+        }
+        __syncthreads();
+        if (threadIdx.x < PAR_TC && col0 + (int)threadIdx.x < ncol) {
+            float P = PARF_in[col0 + threadIdx.x];
+            for (int k = 0; k < g.Nz; ++k) {
+                const float ex = s_q[k * PAR_TC + threadIdx.x];
+                s_q[k * PAR_TC + threadIdx.x] = P * (1.0f - ex);
+                P = P * ex;
+            }
+            PARF_out[col0 + threadIdx.x] = P;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int it = (int)threadIdx.x + r * 256;
+            const int col = col0 + (it & (PAR_TC - 1)), kk = it / PAR_TC + 1 + g.Hz;
+            if (it >= nitem || col >= ncol) continue;
+            const long long c = gidx(g, col % g.Nx + 1 + g.Hx, col / g.Nx + 1 + g.Hy, kk);
+            const float p = s_q[it] / s_at[it];
+            par[c] = p;
+            diag_par[c] = dgv[r] + p;
+        }
+        __syncthreads();
     }
 }
 __device__ __forceinline__ void k_par_body(const ParArgs& x, Blk K) { k_par_impl(K, x.st, x.g, x.Chl, x.PARF_in, x.PARF_out, x.diag_par, x.kc, x.kw); }

@@ -776,7 +776,8 @@ static void build_step(pi_model* m, float dt, Plan& P)
         ParArgs a = zeroed<ParArgs>();
         a.g = g; a.st = st; a.Chl = m->Chl; a.PARF_in = m->PARF_in; a.PARF_out = m->PARF_out;
         a.diag_par = m->diagPAR; a.kc = m->bgc[PI_B_kc]; a.kw = m->bgc[PI_B_kw];
-        P.add(K_PAR, "par", blocks_for((long long)g.Nx * g.Ny), a);
+        /* one block per 64 columns (up to 64 levels; deeper grids: one lane per column) */
+        P.add(K_PAR, "par", g.Nz <= 64 ? grid_for((long long)g.Nx * g.Ny, 64, 8192) : blocks_for((long long)g.Nx * g.Ny), a);
     }
     /* physiology */
     {
