@@ -6,7 +6,7 @@
 #include "k_args.h"
 
 #define SCAN_THREADS 256
-#define SCAN_ITEMS 8
+#define SCAN_ITEMS 16
 #define SCAN_TILE (SCAN_THREADS * SCAN_ITEMS)
 
 /* Input of a scan: either a plain u32 array, or the division bitmaps laid out as
@@ -26,13 +26,16 @@ __device__ __forceinline__ uint32_t scan_load(const ScanIn& s, uint32_t i, uint3
     return __popc(~s.A[j] & valid);
 }
 
-/* the SCAN_ITEMS consecutive inputs of a thread: two 16-byte loads when they
- * are whole (tile bases are multiples of 8 elements) */
-__device__ __forceinline__ void scan_load8(const ScanIn& s, uint32_t base, uint32_t n, uint32_t* v)
+/* the SCAN_ITEMS consecutive inputs of a thread: 16-byte loads when they are
+ * whole (thread bases are multiples of SCAN_ITEMS, a multiple of 4) */
+__device__ __forceinline__ void scan_loadv(const ScanIn& s, uint32_t base, uint32_t n, uint32_t* v)
 {
     if (!s.bits && base + SCAN_ITEMS <= n) {
-        const uint4 a = *(const uint4*)(s.a + base), b = *(const uint4*)(s.a + base + 4);
-        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; k += 4) {
+            const uint4 a = *(const uint4*)(s.a + base + k);
+            v[k] = a.x; v[k + 1] = a.y; v[k + 2] = a.z; v[k + 3] = a.w;
+        }
     } else {
         for (int k = 0; k < SCAN_ITEMS; ++k) v[k] = scan_load(s, base + k, n);
     }
@@ -69,7 +72,7 @@ __device__ __forceinline__ void k_scan_partials_impl(Blk K, ScanIn s, uint32_t* 
     }
     const uint32_t base = K.b * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS], sum = 0;
-    scan_load8(s, base, n, v);
+    scan_loadv(s, base, n, v);
     for (int k = 0; k < SCAN_ITEMS; ++k) sum += v[k];
     uint32_t tot;
     block_exscan(sum, &tot);
@@ -101,7 +104,7 @@ __device__ __forceinline__ void k_scan_final_impl(Blk K, ScanIn s, const uint32_
     const uint32_t n = scan_len(s);
     const uint32_t base = K.b * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS], sum = 0;
-    scan_load8(s, base, n, v);
+    scan_loadv(s, base, n, v);
     for (int k = 0; k < SCAN_ITEMS; ++k) sum += v[k];
     uint32_t tot;
     uint32_t pre = block_exscan(sum, &tot) + s_pre;
@@ -110,8 +113,8 @@ __device__ __forceinline__ void k_scan_final_impl(Blk K, ScanIn s, const uint32_
     uint32_t o[SCAN_ITEMS];
     for (int k = 0; k < SCAN_ITEMS; ++k) { o[k] = pre; pre += v[k]; }
     if (base + SCAN_ITEMS <= n && base + SCAN_ITEMS <= s.n) {        /* outputs are 16-byte aligned */
-        *(uint4*)(out + base) = make_uint4(o[0], o[1], o[2], o[3]);
-        *(uint4*)(out + base + 4) = make_uint4(o[4], o[5], o[6], o[7]);
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; k += 4) *(uint4*)(out + base + k) = make_uint4(o[k], o[k + 1], o[k + 2], o[k + 3]);
     } else {
         for (int k = 0; k < SCAN_ITEMS; ++k)
             if (base + k <= n && base + k < s.n) out[base + k] = o[k];   /* out[n] = total when n < s.n */
