@@ -404,7 +404,7 @@ __device__ __forceinline__ void k_halo_fill_body(const FillArgs& a, Blk K)
 PI_KERNEL(k_halo_fill, FillArgs, 256)
 
 /* ---------------------------------------------- z sweep + the rest of the update */
-#define RUN_Z 8
+#define RUN_Z 16
 
 /* the tracers that receive particle exchange (plk), tracer_update.jl:22-43 */
 __device__ __forceinline__ constexpr bool has_plk(int q)
@@ -498,17 +498,20 @@ __device__ __forceinline__ void cell_update(const FinalArgs& a, const DevGrid& g
 
 __device__ __forceinline__ int sink_index(int q) { return q == PI_PFe_inorg ? 0 : q == PI_Dust ? 1 : q == PI_PFe_bio ? 2 : -1; }
 
-/* k_final runs two threads per column run, one per tracer group: carbon and
- * nitrogen (whose forcings read only tracers of the group) and the rest (whose
- * forcings also read POC, tracer_forcings.jl:2-50).  fgrp(G, i) = the i-th
- * tracer of group G. */
-#define FG_N 7
+/* k_final runs four threads per column run, one per tracer group.  The
+ * forcings (tracer_forcings.jl:2-50) couple tracers only within these groups:
+ * DIC DOC POC; NH4 NO3 DON PON; PO4 DOP POP; and DFe PFe_inorg PFe_bio Dust,
+ * whose forcings also read POC.  fgrp(G, i) = the i-th tracer of group G,
+ * fg_n(G) its size. */
+#define FG_MAX 4
+#define FG_NGRP 4
+__device__ __forceinline__ constexpr int fg_n(int G) { return G == 0 || G == 2 ? 3 : 4; }
 __device__ __forceinline__ constexpr int fgrp(int G, int i)
 {
-    return G == 0 ? (i == 0 ? PI_DIC : i == 1 ? PI_NH4 : i == 2 ? PI_NO3 : i == 3 ? PI_DOC : i == 4 ? PI_DON
-                                                                          : i == 5 ? PI_POC : PI_PON)
-                  : (i == 0 ? PI_PO4 : i == 1 ? PI_DFe : i == 2 ? PI_DOP : i == 3 ? PI_PFe_inorg : i == 4 ? PI_POP
-                                                                          : i == 5 ? PI_PFe_bio : PI_Dust);
+    return G == 0 ? (i == 0 ? PI_DIC : i == 1 ? PI_DOC : PI_POC)
+         : G == 1 ? (i == 0 ? PI_NH4 : i == 1 ? PI_NO3 : i == 2 ? PI_DON : PI_PON)
+         : G == 2 ? (i == 0 ? PI_PO4 : i == 1 ? PI_DOP : PI_POP)
+                  : (i == 0 ? PI_DFe : i == 1 ? PI_PFe_inorg : i == 2 ? PI_PFe_bio : PI_Dust);
 }
 
 /* face_fluxes (vertical) for the tracers of group G at face f from the values
@@ -519,7 +522,7 @@ __device__ __forceinline__ void face_fluxes_grp(const DevGrid& g, const Face& F,
 {
     uint32_t need = 0;
 #pragma unroll
-    for (int i = 0; i < FG_N; ++i) {
+    for (int i = 0; i < fg_n(G); ++i) {
         const Sten t = stencil_of<LAND, true>(g, f, s, v[i][0], v[i][1], v[i][2], v[i][3]);
         bool bad;
         const float r = flux_fast(F, t, bad);
@@ -528,7 +531,7 @@ __device__ __forceinline__ void face_fluxes_grp(const DevGrid& g, const Face& F,
     }
     if (__builtin_expect(need != 0, 0)) {
 #pragma unroll
-        for (int i = 0; i < FG_N; ++i)
+        for (int i = 0; i < fg_n(G); ++i)
             if ((need >> i) & 1u) out[i] = flux_full(F, stencil_of<LAND, true>(g, f, s, v[i][0], v[i][1], v[i][2], v[i][3]));
     }
 }
@@ -543,11 +546,11 @@ __device__ __forceinline__ void final_grp(const FinalArgs& a, const DevGrid& g, 
 {
     const uint32_t sz = (uint32_t)g.Px * (uint32_t)g.Py, sy = (uint32_t)g.Px;
     const uint32_t cc = z.cc, c = z.c;
-    float f[FG_N];
+    float f[FG_MAX];
     face_fluxes_grp<LAND, G>(g, Fb, zero, c, sz, pw, f);
-    float outv[FG_N], dv[FG_N], cnv[FG_N];
+    float outv[FG_MAX], dv[FG_MAX], cnv[FG_MAX];
 #pragma unroll
-    for (int i = 0; i < FG_N; ++i) {
+    for (int i = 0; i < fg_n(G); ++i) {
         const int q = fgrp(G, i);
         const float* co = fld(C0, g, q);
         cnv[i] = ld(co, c);
@@ -559,7 +562,7 @@ __device__ __forceinline__ void final_grp(const FinalArgs& a, const DevGrid& g, 
                           pk, outv[i], dv[i]);
     }
 #pragma unroll
-    for (int i = 0; i < FG_N; ++i) {
+    for (int i = 0; i < fg_n(G); ++i) {
         const int q = fgrp(G, i);
         st(fld(OUT, g, q), cc, outv[i]);
         fz[i] = f[i]; dz[i] = dv[i];
@@ -581,12 +584,12 @@ __device__ __forceinline__ void final_run(const FinalArgs& a, const DevGrid& g, 
     const int k0 = g.Hz + 1 + r * RUN_Z;
     const int k1 = min(k0 + RUN_Z, g.Nz + g.Hz + 1);     /* cells k0..k1-1, faces k0..k1 */
     uint32_t c = (uint32_t)gidx(g, i, j, k0);            /* cell below the current face */
-    float cu[FG_N], cs[3] = {0.0f, 0.0f, 0.0f}, fz[FG_N], dz[FG_N];
+    float cu[FG_MAX], cs[3] = {0.0f, 0.0f, 0.0f}, fz[FG_MAX], dz[FG_MAX];
     /* z-sweep input around the current face c: pw[i] = prev at c − 2sz, c − sz,
      * c, c + sz (slid up one level per cell, one new load per tracer) */
-    float pw[FG_N][4];
+    float pw[FG_MAX][4];
 #pragma unroll
-    for (int x = 0; x < FG_N; ++x) {
+    for (int x = 0; x < fg_n(G); ++x) {
         const float* p = fld(a.prev, g, fgrp(G, x));
         pw[x][0] = ld(p, c - 2 * sz); pw[x][1] = ld(p, c - sz); pw[x][2] = ld(p, c); pw[x][3] = ld(p, c + sz);
     }
@@ -596,13 +599,13 @@ __device__ __forceinline__ void final_run(const FinalArgs& a, const DevGrid& g, 
         const float dzf = dzF_at(g, k0);
         const double ydzf = g.ydzF[k0 - 1];
 #pragma unroll
-        for (int x = 0; x < FG_N; ++x) {
+        for (int x = 0; x < fg_n(G); ++x) {
             const float* co = fld(C0, g, fgrp(G, x));
             const float above = ld(co, c - sz), here = ld(co, c);
             dz[x] = kvAz * divc(lmask<LAND>(g, c - sz, c, above - here), dzf, ydzf);
             cu[x] = here;
         }
-        if (G == 1) {
+        if (G == 3) {
             cs[0] = ld(fld(C0, g, PI_PFe_inorg), c - sz);
             cs[1] = ld(fld(C0, g, PI_Dust), c - sz);
             cs[2] = ld(fld(C0, g, PI_PFe_bio), c - sz);
@@ -612,7 +615,7 @@ __device__ __forceinline__ void final_run(const FinalArgs& a, const DevGrid& g, 
     for (int kk = k0 + 1; kk <= k1; ++kk) {
         c += sz;
 #pragma unroll
-        for (int x = 0; x < FG_N; ++x) {
+        for (int x = 0; x < fg_n(G); ++x) {
             pw[x][0] = pw[x][1]; pw[x][1] = pw[x][2]; pw[x][2] = pw[x][3];
             pw[x][3] = ld(fld(a.prev, g, fgrp(G, x)), c + sz);
         }
@@ -631,8 +634,8 @@ __device__ __forceinline__ void final_run(const FinalArgs& a, const DevGrid& g, 
         z.ydzf = g.ydzF[kk - 1];
         float tt[PI_NTRACERS] = {};    /* tracer_temp = max(0, c) (tracer_forcings.jl:3-5) */
 #pragma unroll
-        for (int x = 0; x < FG_N; ++x) tt[fgrp(G, x)] = dmaxf(0.0f, cu[x]);
-        if (G == 1) tt[PI_POC] = dmaxf(0.0f, ld(fld(C0, g, PI_POC), z.cc));
+        for (int x = 0; x < fg_n(G); ++x) tt[fgrp(G, x)] = dmaxf(0.0f, cu[x]);
+        if (G == 3) tt[PI_POC] = dmaxf(0.0f, ld(fld(C0, g, PI_POC), z.cc));
         final_grp<LAND, G>(a, g, C0, OUT, Fb, zero, z, tt, cu, fz, dz, cs, pw);
         Fa = Fb;
     }
@@ -645,18 +648,22 @@ __device__ __forceinline__ void k_final_body(const FinalArgs& a, Blk K)
     if (S->err) return;
     const DevGrid& g = a.g;
     const int nrun = (g.Nz + RUN_Z - 1) / RUN_Z;
-    const long long nthr = (long long)g.Nx * g.Ny * nrun * 2;
+    const long long nthr = (long long)g.Nx * g.Ny * nrun * FG_NGRP;
     const float* C0 = S->tr[S->tcur][0];
     float* OUT = S->tr[S->tcur ^ 1][0];
     for (long long t = K.b * (long long)blockDim.x + threadIdx.x; t < nthr; t += (long long)K.n * blockDim.x) {
-        /* t = i + Nx·(group + 2·(j + Ny·run)): the two groups of a row are neighbouring blocks */
+        /* t = i + Nx·(group + 4·(j + Ny·run)): the groups of a row are neighbouring blocks */
         const int i = (int)(t % g.Nx) + 1 + g.Hx;
         const long long u = t / g.Nx;
-        const int G = (int)(u & 1);
-        const int j = (int)((u >> 1) % g.Ny) + 1 + g.Hy;
-        const int r = (int)((u >> 1) / g.Ny);
-        if (G == 0) final_run<LAND, 0>(a, g, C0, OUT, i, j, r);
-        else final_run<LAND, 1>(a, g, C0, OUT, i, j, r);
+        const int G = (int)(u & 3);
+        const int j = (int)((u >> 2) % g.Ny) + 1 + g.Hy;
+        const int r = (int)((u >> 2) / g.Ny);
+        switch (G) {
+            case 0: final_run<LAND, 0>(a, g, C0, OUT, i, j, r); break;
+            case 1: final_run<LAND, 1>(a, g, C0, OUT, i, j, r); break;
+            case 2: final_run<LAND, 2>(a, g, C0, OUT, i, j, r); break;
+            default: final_run<LAND, 3>(a, g, C0, OUT, i, j, r); break;
+        }
     }
 }
 PI_KERNEL2(k_final0, k_final_body<false>, FinalArgs, 256)
@@ -923,7 +930,7 @@ long long sweep_x_blocks(const DevGrid& g)
     return (nwave + 3) / 4;
 }
 long long sweep_y_items(const DevGrid& g) { return (long long)g.Nx * ((g.Ny + RUN_Y - 1) / RUN_Y) * g.Nz; }
-long long final_items(const DevGrid& g) { return (long long)g.Nx * g.Ny * ((g.Nz + RUN_Z - 1) / RUN_Z) * 2; }
+long long final_items(const DevGrid& g) { return (long long)g.Nx * g.Ny * ((g.Nz + RUN_Z - 1) / RUN_Z) * FG_NGRP; }
 
 void reg_tracers(KInfo* k)
 {
