@@ -101,6 +101,7 @@ struct FinalArgs {
     float wsink_inorg, wsink_org;
     float kDOC, kPOC, kDON, kPON, kDOP, kPOP, Nit;
     float lPOC, lmin, ldust, DFeFrac, kdiss, lig, lFe;
+    int fill_halo;                    /* also write the halo cells of the new tracers (halo 2, no aliasing) */
 };
 
 struct ScanIn {

@@ -498,6 +498,20 @@ __device__ __forceinline__ void cell_update(const FinalArgs& a, const DevGrid& g
 
 __device__ __forceinline__ int sink_index(int q) { return q == PI_PFe_inorg ? 0 : q == PI_Dust ? 1 : q == PI_PFe_bio ? 2 : -1; }
 
+/* The halo cells whose fill_halo_tracer! value is a copy of interior
+ * coordinate s (0-based padded) in one direction, as a mask over the
+ * candidates k = 0: s itself, 1, 2: halo cells 0, 1, 3, 4: halo cells N+2,
+ * N+3 (halo 2): the inverse of halo_src. */
+__device__ __forceinline__ int halo_cand(int k, int s, int H, int N) { return k == 0 ? s : k < 3 ? k - 1 : N + H + (k - 3); }
+__device__ __forceinline__ uint32_t halo_images(int s, int H, int N, int topo)
+{
+    uint32_t m = 1u;
+#pragma unroll
+    for (int k = 1; k < 5; ++k) m |= (uint32_t)(halo_src(halo_cand(k, s, H, N), H, N, topo) == s) << k;
+    return m;
+}
+struct HaloImg { uint32_t mx, my; int sx, sy; };   /* masks and the column (0-based padded) */
+
 /* k_final runs four threads per column run, one per tracer group.  The
  * forcings (tracer_forcings.jl:2-50) couple tracers only within these groups:
  * DIC DOC POC; NH4 NO3 DON PON; PO4 DOP POP; and DFe PFe_inorg PFe_bio Dust,
@@ -542,7 +556,7 @@ __device__ __forceinline__ void face_fluxes_grp(const DevGrid& g, const Face& F,
 template <bool LAND, int G>
 __device__ __forceinline__ void final_grp(const FinalArgs& a, const DevGrid& g, const float* C0, float* OUT,
                                           const Face& Fb, bool zero, const ZCell& z, const float* tt, float* cu,
-                                          float* fz, float* dz, float* cs, float (*pw)[4])
+                                          float* fz, float* dz, float* cs, float (*pw)[4], const HaloImg& hi)
 {
     const uint32_t sz = (uint32_t)g.Px * (uint32_t)g.Py, sy = (uint32_t)g.Px;
     const uint32_t cc = z.cc, c = z.c;
@@ -561,6 +575,12 @@ __device__ __forceinline__ void final_grp(const FinalArgs& a, const DevGrid& g, 
                           ld(co, cc + sy), pw[i][1], fz[i], f[i], dz[i], si >= 0 ? cs[si] : 0.0f,
                           pk, outv[i], dv[i]);
     }
+    /* the halo images of this cell (fill_halo_tracer! of the new tracers): a
+     * lane at the domain's edges writes its value to the halo cells that copy it */
+    const int sx = hi.sx, sy0 = hi.sy, sz0 = z.k - 1;
+    const bool zedge = sz0 < g.Hz + 2 || sz0 >= g.Hz + g.Nz - 2;          /* the same for the whole wave */
+    const uint32_t mz = (a.fill_halo && zedge) ? halo_images(sz0, g.Hz, g.Nz, g.tz) : 1u;
+    const bool img = (hi.mx | hi.my | mz) != 1u;
 #pragma unroll
     for (int i = 0; i < fg_n(G); ++i) {
         const int q = fgrp(G, i);
@@ -569,6 +589,21 @@ __device__ __forceinline__ void final_grp(const FinalArgs& a, const DevGrid& g, 
         const int si = sink_index(q);
         if (si >= 0) cs[si] = cu[i];
         cu[i] = cnv[i];
+    }
+    if (img) {
+        for (uint32_t m3 = mz; m3; m3 &= m3 - 1u) {
+            const int c3 = __ffs(m3) - 1, kz = halo_cand(c3, sz0, g.Hz, g.Nz);
+            for (uint32_t m2 = hi.my; m2; m2 &= m2 - 1u) {
+                const int b3 = __ffs(m2) - 1, ky = halo_cand(b3, sy0, g.Hy, g.Ny);
+                for (uint32_t m1 = hi.mx; m1; m1 &= m1 - 1u) {
+                    const int a3 = __ffs(m1) - 1, kx = halo_cand(a3, sx, g.Hx, g.Nx);
+                    if (!(a3 | b3 | c3)) continue;
+                    const uint32_t h = (uint32_t)(kx + g.Px * (ky + g.Py * kz));
+#pragma unroll
+                    for (int i = 0; i < fg_n(G); ++i) st(fld(OUT, g, fgrp(G, i)), h, outv[i]);
+                }
+            }
+        }
     }
 }
 
@@ -585,6 +620,10 @@ __device__ __forceinline__ void final_run(const FinalArgs& a, const DevGrid& g, 
     const int k1 = min(k0 + RUN_Z, g.Nz + g.Hz + 1);     /* cells k0..k1-1, faces k0..k1 */
     uint32_t c = (uint32_t)gidx(g, i, j, k0);            /* cell below the current face */
     float cu[FG_MAX], cs[3] = {0.0f, 0.0f, 0.0f}, fz[FG_MAX], dz[FG_MAX];
+    HaloImg hi;
+    hi.mx = a.fill_halo ? halo_images(i - 1, g.Hx, g.Nx, g.tx) : 1u;
+    hi.my = a.fill_halo ? halo_images(j - 1, g.Hy, g.Ny, g.ty) : 1u;
+    hi.sx = i - 1; hi.sy = j - 1;
     /* z-sweep input around the current face c: pw[i] = prev at c − 2sz, c − sz,
      * c, c + sz (slid up one level per cell, one new load per tracer) */
     float pw[FG_MAX][4];
@@ -636,7 +675,7 @@ __device__ __forceinline__ void final_run(const FinalArgs& a, const DevGrid& g, 
 #pragma unroll
         for (int x = 0; x < fg_n(G); ++x) tt[fgrp(G, x)] = dmaxf(0.0f, cu[x]);
         if (G == 3) tt[PI_POC] = dmaxf(0.0f, ld(fld(C0, g, PI_POC), z.cc));
-        final_grp<LAND, G>(a, g, C0, OUT, Fb, zero, z, tt, cu, fz, dz, cs, pw);
+        final_grp<LAND, G>(a, g, C0, OUT, Fb, zero, z, tt, cu, fz, dz, cs, pw, hi);
         Fa = Fb;
     }
 }

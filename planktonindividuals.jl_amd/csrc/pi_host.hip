@@ -816,6 +816,7 @@ static void build_step(pi_model* m, float dt, Plan& P)
         fn.lPOC = m->bgc[PI_B_lambda_POC]; fn.lmin = m->bgc[PI_B_lambda_min]; fn.ldust = m->bgc[PI_B_lambda_dust];
         fn.DFeFrac = m->bgc[PI_B_DFeFrac]; fn.kdiss = m->bgc[PI_B_kdiss]; fn.lig = m->bgc[PI_B_ligand];
         fn.lFe = m->bgc[PI_B_lambda_Fe];
+        fn.fill_halo = fxy ? 1 : 0;       /* k_final writes the halo images itself */
         const long long fb = grid_for(final_items(g), 256, 8192);
         P.add(K_FINAL0, "final", land ? 0 : fb, fn);
         P.add(K_FINAL1, "final", land ? fb : 0, fn);
@@ -828,7 +829,8 @@ static void build_step(pi_model* m, float dt, Plan& P)
         fa.dim = 2; P.add(K_FILL, "fill", fxy ? 0 : blocks_for((long long)g.Px * g.Py * PI_NTRACERS), fa);
         /* the same as one launch over the halo cells where halo 2 makes it a map (the grids k_xy takes) */
         const long long nh = 4ll * g.Px * g.Py + 4ll * g.Nz * g.Px + 4ll * g.Nz * g.Ny;
-        P.add(K_HALOFILL, "fill", fxy ? grid_for(nh * PI_NTRACERS, 256, 8192) : 0, fa);
+        (void)nh;   /* on these grids k_final writes the halo images (fill_halo); k_halo_fill stays for tests of the map */
+        P.add(K_HALOFILL, "fill", 0, fa);
     }
     {
         AccumArgs a = zeroed<AccumArgs>();
