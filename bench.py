@@ -64,7 +64,7 @@ LOGICAL = {
     "K_div":    dict(groups=["marks+scan"], per_ind=0, per_cell=0, per_slot=2,
                      kernels=["k_marks", "k_capcheck"]),
     "K_dst3fl": dict(groups=["sweep_xy", "sweep_x", "fill", "sweep_y", "final"], per_ind=0, per_cell=164,
-                     per_slot=0, kernels=["k_xy", "k_sweep_x", "k_sweep_y", "k_fill_line", "k_halo_fill", "k_final"]),
+                     per_slot=0, kernels=["k_xy", "k_sweep_x", "k_sweep_y", "k_fill_line", "k_final"]),
     "K_par":    dict(groups=["par"], per_ind=0, per_cell=8, per_slot=0, kernels=["k_par"]),
 }
 

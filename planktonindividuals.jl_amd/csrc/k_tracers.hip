@@ -351,10 +351,9 @@ PI_KERNEL(k_halo_copy, FillArgs, 256)
  * then y, then z fills over whole planes) for halo 2 and Periodic N >= 2: the
  * composition leaves in every halo cell the value of the interior cell whose
  * index is, per direction, the periodic image (Periodic) or the nearest edge
- * cell (Bounded).  One thread per halo cell, consecutive threads along x: the
- * z-halo planes, then the y-halo rows of the interior levels, then the x-halo
- * cells of the interior rows.  Halo cells are only written and interior cells
- * only read. */
+ * cell (Bounded).  halo_src is that source index per direction (0-based padded
+ * coordinates); k_final writes each interior value to its halo images (the
+ * inverse map, halo_images). */
 __device__ __forceinline__ int halo_src(int p, int H, int N, int topo)
 {
     int x = p - H;                                                /* 0-based interior coordinate, may be outside */
@@ -362,46 +361,6 @@ __device__ __forceinline__ int halo_src(int p, int H, int N, int topo)
     else x = x < 0 ? 0 : x >= N ? N - 1 : x;
     return x + H;
 }
-__device__ __forceinline__ void k_halo_fill_body(const FillArgs& a, Blk K)
-{
-    const DevState* S = a.st;
-    if (S->err) return;
-    const DevGrid& g = a.g;
-    float* slab = a.f ? a.f : S->tr[S->tcur ^ 1][0];
-    const uint32_t Px = (uint32_t)g.Px, Py = (uint32_t)g.Py, Pz = (uint32_t)g.Pz;
-    const uint32_t nA = 4u * Px * Py, nB = (uint32_t)g.Nz * 4u * Px, nC = (uint32_t)g.Nz * (uint32_t)g.Ny * 4u;
-    const uint32_t n = nA + nB + nC;
-    const unsigned long long tot = (unsigned long long)n * (uint32_t)a.nf;
-    for (unsigned long long t = K.b * (unsigned long long)blockDim.x + threadIdx.x; t < tot;
-         t += (unsigned long long)K.n * blockDim.x) {
-        const uint32_t q = (uint32_t)(t / n);
-        uint32_t r = (uint32_t)(t - (unsigned long long)q * n);
-        uint32_t i, j, k;                                         /* 0-based padded halo cell */
-        if (r < nA) {
-            const uint32_t pl = r / (Px * Py), rem = r - pl * (Px * Py);
-            k = pl < 2 ? pl : Pz - 4 + pl;
-            j = rem / Px; i = rem - j * Px;
-        } else if (r < nA + nB) {
-            r -= nA;
-            const uint32_t row = r / Px;
-            k = row / 4 + (uint32_t)g.Hz;
-            const uint32_t jj = row % 4;
-            j = jj < 2 ? jj : Py - 4 + jj;
-            i = r - row * Px;
-        } else {
-            r -= nA + nB;
-            const uint32_t c = r / 4, ii = r % 4;
-            k = c / (uint32_t)g.Ny + (uint32_t)g.Hz;
-            j = c % (uint32_t)g.Ny + (uint32_t)g.Hy;
-            i = ii < 2 ? ii : Px - 4 + ii;
-        }
-        const uint32_t si = (uint32_t)halo_src((int)i, g.Hx, g.Nx, g.tx), sj = (uint32_t)halo_src((int)j, g.Hy, g.Ny, g.ty);
-        const uint32_t sk = (uint32_t)halo_src((int)k, g.Hz, g.Nz, g.tz);
-        PI_GLOBAL float* f = gp(slab) + q * (uint32_t)g.ncell;
-        stg(f, i + Px * (j + Py * k), ldg(f, si + Px * (sj + Py * sk)));
-    }
-}
-PI_KERNEL(k_halo_fill, FillArgs, 256)
 
 /* ---------------------------------------------- z sweep + the rest of the update */
 #define RUN_Z 16
@@ -978,7 +937,6 @@ void reg_tracers(KInfo* k)
     PI_REG(k, K_SWEEPY0, k_sweep_y0, SweepArgs, 256, true);
     PI_REG(k, K_SWEEPY1, k_sweep_y1, SweepArgs, 256, true);
     PI_REG(k, K_FILL, k_fill_line, FillArgs, 256, true);
-    PI_REG(k, K_HALOFILL, k_halo_fill, FillArgs, 256, true);
     PI_REG(k, K_HALOCOPY, k_halo_copy, FillArgs, 256, true);
     PI_REG(k, K_FINAL0, k_final0, FinalArgs, 256, true);
     PI_REG(k, K_FINAL1, k_final1, FinalArgs, 256, true);

@@ -827,10 +827,7 @@ static void build_step(pi_model* m, float dt, Plan& P)
         fa.dim = 0; P.add(K_FILL, "fill", fxy ? 0 : blocks_for((long long)g.Py * g.Pz * PI_NTRACERS), fa);
         fa.dim = 1; P.add(K_FILL, "fill", fxy ? 0 : blocks_for((long long)g.Px * g.Pz * PI_NTRACERS), fa);
         fa.dim = 2; P.add(K_FILL, "fill", fxy ? 0 : blocks_for((long long)g.Px * g.Py * PI_NTRACERS), fa);
-        /* the same as one launch over the halo cells where halo 2 makes it a map (the grids k_xy takes) */
-        const long long nh = 4ll * g.Px * g.Py + 4ll * g.Nz * g.Px + 4ll * g.Nz * g.Ny;
-        (void)nh;   /* on these grids k_final writes the halo images (fill_halo); k_halo_fill stays for tests of the map */
-        P.add(K_HALOFILL, "fill", 0, fa);
+        /* on the grids k_xy takes (halo 2, no aliased periodic direction) k_final writes the halo images */
     }
     {
         AccumArgs a = zeroed<AccumArgs>();
