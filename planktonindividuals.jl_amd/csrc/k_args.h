@@ -59,6 +59,7 @@ struct PlkArgs {
     const float* rcons;
     long long cap;
     const uint32_t* lng;      /* the long cells of this step (k_plk_long) */
+    uint32_t* segp;           /* receives this step's segments (the next step's previous ones) */
     uint32_t *lg, *lm;        /* scratch: ordered lists of grazed / dead entries */
 };
 
@@ -161,7 +162,6 @@ struct ScanArgs {
 
 struct CapArgs { DevState* st; const uint32_t* G; int nsp, nw; };
 struct FinishArgs { DevState* st; const uint32_t* seg; long long ncell; int nsp; };
-struct CopyArgs { uint32_t* dst; const uint32_t* src; long long n; const DevState* st; };
 struct VelhalfArgs { const float* a[3]; const float* b[3]; float* h[3]; long long n; const DevState* st; };
 struct AccumArgs { float* d[PI_NTRACERS]; int q[PI_NTRACERS]; int nq; long long n; const DevState* st; };
 
@@ -171,7 +171,7 @@ struct AccumArgs { float* d[PI_NTRACERS]; int q[PI_NTRACERS]; int nq; long long 
 enum KId {
     K_VELHALF, K_MARKS, K_SCAN_P, K_SCAN_F, K_CAPCHECK, K_ADV, K_CELLCOUNT, K_PLACE_STAY,
     K_PLACE_SMALL, K_MIXSORT, K_BIGCELLS, K_CELLSUM, K_CELLSUM_LONG, K_PAR, K_PHYS, K_PLK, K_PLK_LONG,
-    K_SWEEPX0, K_SWEEPX1, K_XY0, K_XY1, K_SWEEPY0, K_SWEEPY1, K_HALOCOPY, K_FILL, K_FINAL0, K_FINAL1, K_ACCUM, K_COPY,
+    K_SWEEPX0, K_SWEEPX1, K_XY0, K_XY1, K_SWEEPY0, K_SWEEPY1, K_HALOCOPY, K_FILL, K_FINAL0, K_FINAL1, K_ACCUM,
     K_FINISH, K_NKID
 };
 /* plan markers (not kernels): fork to / join from the side stream */

@@ -1063,6 +1063,8 @@ __device__ __forceinline__ void k_plk_body(const PlkArgs& a, Blk K)
     for (; c < g.ncell; c += stride) {
         const uint32_t s0 = s0n, s1 = s1n, n = s1 - s0;
         if (c + stride < g.ncell) { s0n = a.seg[c + stride]; s1n = a.seg[c + stride + 1]; }
+        a.segp[c] = s0;                              /* the next step's previous segments */
+        if (c == g.ncell - 1) a.segp[g.ncell] = s1;
         if (n > LONG_SEG) continue;                  /* k_plk_long */
         PlkSums S{};
         if (n <= PLK_FAST) {

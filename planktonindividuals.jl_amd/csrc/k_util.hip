@@ -162,14 +162,6 @@ __device__ __forceinline__ void k_finish_impl(Blk K, DevState* st, const uint32_
 __device__ __forceinline__ void k_finish_body(const FinishArgs& x, Blk K) { k_finish_impl(K, x.st, x.seg, x.ncell, x.nsp); }
 PI_KERNEL(k_finish, FinishArgs, 64)
 
-__device__ __forceinline__ void k_copy_u32_impl(Blk K, uint32_t* dst, const uint32_t* src, long long n, const DevState* st)
-{
-    if (st->err) return;
-    for (long long i = K.b * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)K.n * blockDim.x)
-        dst[i] = src[i];
-}
-__device__ __forceinline__ void k_copy_u32_body(const CopyArgs& x, Blk K) { k_copy_u32_impl(K, x.dst, x.src, x.n, x.st); }
-PI_KERNEL(k_copy_u32, CopyArgs, 256)
 
 /* a plain streaming copy (16 bytes per lane per iteration) */
 __global__ void __launch_bounds__(256) k_probe_copy(uint4* dst, const uint4* src, long long n)
@@ -395,7 +387,6 @@ void reg_util(KInfo* k)
     PI_REG(k, K_SCAN_F, k_scan_final, ScanArgs, SCAN_THREADS, false);
     PI_REG(k, K_CAPCHECK, k_capcheck, CapArgs, 64, false);
     PI_REG(k, K_ACCUM, k_accum, AccumArgs, 256, true);
-    PI_REG(k, K_COPY, k_copy_u32, CopyArgs, 256, true);
     PI_REG(k, K_FINISH, k_finish, FinishArgs, 64, false);
 }
 

@@ -796,6 +796,7 @@ static void build_step(pi_model* m, float dt, Plan& P)
         ka.rstamp = m->rstamp; ka.rcons = m->rcons; ka.cap = m->cap;
         for (int q = 0; q < PI_NTRACERS; ++q) ka.plk[q] = m->plk[q] ? m->plk[q] : m->plk[PI_DIC];
         ka.lng = m->lng; ka.lg = m->skey; ka.lm = m->ssrc;
+        ka.segp = m->segp;                 /* k_plk copies seg -> segp (the binning above has read segp) */
         /* one resident round: five 256-thread blocks per CU fit k_plk's registers,
          * and a partial last round of blocks measured ~10% slower */
         P.add(K_PLK, "plk", std::min<long long>(cb, 5ll * m->ncu), ka);
@@ -834,9 +835,6 @@ static void build_step(pi_model* m, float dt, Plan& P)
         a.nq = (int)m->diag_ids.size(); a.n = m->ncell; a.st = st;
         for (int j = 0; j < a.nq; ++j) { a.d[j] = m->diag_tr[j]; a.q[j] = m->diag_ids[j]; }
         P.add(K_ACCUM, "diag", a.nq ? grid_for(m->ncell, 256, 4096) : 0, a);
-        CopyArgs c = zeroed<CopyArgs>();
-        c.dst = m->segp; c.src = m->seg; c.n = m->ncell + 1; c.st = st;   /* the next step's previous segments */
-        P.add(K_COPY, "finish", grid_for(m->ncell + 1, 256, 4096), c);
         FinishArgs f = zeroed<FinishArgs>();
         f.st = st; f.seg = m->seg; f.ncell = m->ncell; f.nsp = m->nsp;
         P.add(K_FINISH, "finish", 1, f);
