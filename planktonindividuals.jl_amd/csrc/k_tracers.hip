@@ -11,9 +11,13 @@
  *   k_fill_line halo fill of one dimension along lines, as fill_halo_*
  *               (HaloRegions/halo_regions.jl) with broadcast unaliasing (each
  *               line is owned by one lane, which reads its sources first).
+ *   k_xy        fused x and y sweeps (grids with halo 2, Nx <= 256, no aliased
+ *               periodic direction), rows staged in LDS, one level per block row.
  *   k_final     z sweep + Gcs = temp - c + diffusion + sinking + forcing +
- *               apply_tendency, lanes walking runs along z (advective and
- *               diffusive z fluxes carried), writing the next tracers.
+ *               apply_tendency, four lanes per column run (tracer groups closed
+ *               under the forcings) walking along z with the advective and
+ *               diffusive z fluxes and the z stencil carried, writing the next
+ *               tracers and, on k_xy's grids, their halo images.
  *
  * Exactness: every value is the reference's expression in its operation order.
  *  - The face flux ½(T+|T|)(…) + ½(T−|T|)(…) is evaluated for the sign of T
