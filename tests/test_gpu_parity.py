@@ -286,3 +286,25 @@ def test_parity_zero_tracers(pi, orc):
     assert not bad, "\n".join(bad[:20])
     strict = [q for q in range(14) if not _bits_same(gm._native.tracer(q), om.tracer(q))]
     assert not strict, f"sign of zero differs in tracers {strict}"
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(128, 6, 4), (256, 3, 2), (320, 5, 3)])
+def test_parity_wide_rows(pi, orc, nx, ny, nz):
+    """Rows of two and four waves (the fused x/y sweep with rows of whole
+    waves, up to its 256-cell limit, and the z walk's lanes of one row per wave)
+    and rows wider than that (the separate sweeps and line fills), periodic in x
+    and y, a random flow of both signs: identical to the oracle."""
+    g = pi.RectilinearGrid(size=(nx, ny, nz), x=(0, nx * 1e3), y=(0, ny * 1e3), z=(0, -10.0 * nz))
+    ncell = nx * ny * nz
+    spec = pi.ModelSpec(g, max_individuals=4 * ncell + 1000, seed=31,
+                        phyto=pi.phyto_setup({"dvid_P": [1e-3], "mort_reg": [1.0]}, [2 * ncell], 1),
+                        bgc_params={"κhP": 1.0, "κvP": 1e-4, "κh": 10.0, "κv": 1e-4})
+    rng = np.random.default_rng(nx)
+    u = (rng.standard_normal((nx, ny, nz)) * 2).astype(np.float32)
+    v = (rng.standard_normal((nx, ny, nz)) * 2).astype(np.float32)
+    w = (rng.standard_normal((nx, ny, nz + 1)) * 1e-3).astype(np.float32)
+    w[:, :, 0] = 0.0
+    w[:, :, nz] = 0.0
+    gm, om = _drive(pi, orc, spec, (u, v, w), 60.0, 6)
+    bad = compare_models(gm._native, om, 1)
+    assert not bad, "\n".join(bad[:20])
