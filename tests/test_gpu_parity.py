@@ -288,13 +288,20 @@ def test_parity_zero_tracers(pi, orc):
     assert not strict, f"sign of zero differs in tracers {strict}"
 
 
-@pytest.mark.parametrize("nx,ny,nz", [(128, 6, 4), (256, 3, 2), (320, 5, 3)])
-def test_parity_wide_rows(pi, orc, nx, ny, nz):
+@pytest.mark.parametrize("nx,ny,nz,land", [(128, 6, 4, False), (256, 3, 2, False), (320, 5, 3, False),
+                                            (128, 5, 3, True), (320, 4, 2, True)])
+def test_parity_wide_rows(pi, orc, nx, ny, nz, land):
     """Rows of two and four waves (the fused x/y sweep with rows of whole
     waves, up to its 256-cell limit, and the z walk's lanes of one row per wave)
     and rows wider than that (the separate sweeps and line fills), periodic in x
-    and y, a random flow of both signs: identical to the oracle."""
-    g = pi.RectilinearGrid(size=(nx, ny, nz), x=(0, nx * 1e3), y=(0, ny * 1e3), z=(0, -10.0 * nz))
+    and y, a random flow of both signs, with and without land: identical to the
+    oracle."""
+    lm = None
+    if land:
+        lm = np.ones((nx, ny, nz), dtype=bool)
+        lm[nx // 3: nx // 2, 1:3, :] = False
+        lm[nx - 5:, 0, nz - 1] = False
+    g = pi.RectilinearGrid(size=(nx, ny, nz), x=(0, nx * 1e3), y=(0, ny * 1e3), z=(0, -10.0 * nz), landmask=lm)
     ncell = nx * ny * nz
     spec = pi.ModelSpec(g, max_individuals=4 * ncell + 1000, seed=31,
                         phyto=pi.phyto_setup({"dvid_P": [1e-3], "mort_reg": [1.0]}, [2 * ncell], 1),
