@@ -315,3 +315,23 @@ def test_parity_wide_rows(pi, orc, nx, ny, nz, land):
     gm, om = _drive(pi, orc, spec, (u, v, w), 60.0, 6)
     bad = compare_models(gm._native, om, 1)
     assert not bad, "\n".join(bad[:20])
+
+
+def test_parity_deep_columns(pi, orc):
+    """70 levels: the PAR column scan for columns deeper than its 64-level
+    tiles, and z runs that do not divide the column: identical to the oracle."""
+    nx, ny, nz = 4, 3, 70
+    g = pi.RectilinearGrid(size=(nx, ny, nz), x=(0, nx * 1e3), y=(0, ny * 1e3), z=(0, -2.0 * nz))
+    ncell = nx * ny * nz
+    spec = pi.ModelSpec(g, max_individuals=4 * ncell + 1000, seed=37,
+                        phyto=pi.phyto_setup({"dvid_P": [1e-3], "mort_reg": [1.0]}, [2 * ncell], 1),
+                        bgc_params={"κhP": 1.0, "κvP": 1e-4, "κh": 10.0, "κv": 1e-4})
+    rng = np.random.default_rng(70)
+    u = (rng.standard_normal((nx, ny, nz)) * 2).astype(np.float32)
+    v = (rng.standard_normal((nx, ny, nz)) * 2).astype(np.float32)
+    w = (rng.standard_normal((nx, ny, nz + 1)) * 1e-3).astype(np.float32)
+    w[:, :, 0] = 0.0
+    w[:, :, nz] = 0.0
+    gm, om = _drive(pi, orc, spec, (u, v, w), 60.0, 6)
+    bad = compare_models(gm._native, om, 1)
+    assert not bad, "\n".join(bad[:20])
