@@ -13,7 +13,8 @@ struct AdvArgs {
     uint64_t seed;
     uint32_t* newc;           /* per store entry: its cell in this step */
     uint32_t* stayf;          /* per sorted entry: 1 = keeps its cell */
-    uint32_t* small;          /* queue of the other live entries */
+    uint32_t* small;          /* queue of the other live entries: PI_NQ shards of qcap entries */
+    uint32_t qcap;
     uint32_t* cnts;           /* per cell: queued arrivals */
     int32_t* dstamp;          /* per slot: iteration in which its occupant dies */
     uint32_t* dcell;          /* per slot: that occupant's cell in this step */
@@ -134,6 +135,7 @@ struct BinArgs {
     const uint32_t* stayf;
     const uint32_t* newc;
     const uint32_t* small;
+    uint32_t qcap;
     uint32_t* cnts;
     uint32_t* cnt;
     uint32_t* lng;

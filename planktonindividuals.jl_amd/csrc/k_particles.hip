@@ -191,9 +191,10 @@ __device__ __forceinline__ void place_entry(const AdvArgs& a, DevState* st, cons
             a.dcell[r] = c;
         }
     }
-    const uint32_t qi = wave_append(&st->nsmall, queue);
+    const uint32_t sq = (e >> 8) & (PI_NQ - 1);       /* the same for every lane of a wave */
+    const uint32_t qi = wave_append(&st->nsmall[sq], queue);
     if (queue) {
-        a.small[qi] = e;
+        a.small[sq * a.qcap + qi] = e;
         atomicAdd(&a.cnts[c], 1u);
     }
 }
@@ -322,18 +323,37 @@ PI_KERNEL_XCD(k_place_stay, BinArgs, 256)
 /* queued individuals: after the stayers of their cell, in any order; the
  * cell is listed for re-sorting once (by the last one placed) */
 __device__ __forceinline__ void k_place_small_impl(Blk K, DevState* st, const uint32_t* seg, const uint32_t* small,
-                                                     const uint32_t* newc, uint32_t* cnts, uint32_t* pkey, uint32_t* psrc,
-                                                     uint32_t* mix)
+                                                     uint32_t qcap, const uint32_t* newc, uint32_t* cnts, uint32_t* pkey,
+                                                     uint32_t* psrc, uint32_t* mix)
 {
     if (st->err) return;
     const GStore S = gstore(st->store[st->scur]);
-    const uint32_t ns = st->nsmall;
+    /* queue position t -> shard q (s_pre[q] <= t < s_pre[q + 1]) and its entry */
+    __shared__ uint32_t s_pre[PI_NQ + 1];
+    if (threadIdx.x < PI_NQ) {
+        const uint32_t lane = threadIdx.x;
+        uint32_t x = st->nsmall[lane];
+#pragma unroll
+        for (int o = 1; o < PI_NQ; o <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)x, o, 64);
+            if (lane >= (uint32_t)o) x += y;
+        }
+        s_pre[lane + 1] = x;
+        if (lane == 0) s_pre[0] = 0;
+    }
+    __syncthreads();
+    const uint32_t ns = s_pre[PI_NQ];
     const uint32_t nloop = (ns + 63u) & ~63u;
     for (uint32_t t = K.b * blockDim.x + threadIdx.x; t < nloop; t += K.n * blockDim.x) {
         bool last = false;
         uint32_t c = 0;
         if (t < ns) {
-            const uint32_t e = small[t];
+            uint32_t lo = 0, hi = PI_NQ;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_pre[mid] <= t) lo = mid; else hi = mid;
+            }
+            const uint32_t e = small[lo * qcap + (t - s_pre[lo])];
             c = newc[e];
             const uint32_t r = atomicSub(&cnts[c], 1u);       /* cnts returns to zero */
             const uint32_t pos = seg[c + 1] - r;
@@ -345,7 +365,7 @@ __device__ __forceinline__ void k_place_small_impl(Blk K, DevState* st, const ui
         if (last) mix[mi] = c;
     }
 }
-__device__ __forceinline__ void k_place_small_body(const BinArgs& x, Blk K) { k_place_small_impl(K, x.st, x.seg, x.small, x.newc, x.cnts, x.pkey, x.psrc, x.mix); }
+__device__ __forceinline__ void k_place_small_body(const BinArgs& x, Blk K) { k_place_small_impl(K, x.st, x.seg, x.small, x.qcap, x.newc, x.cnts, x.pkey, x.psrc, x.mix); }
 PI_KERNEL(k_place_small, BinArgs, 256)
 
 /* re-sort the listed cells by key (species-major, then slot): insertion sort

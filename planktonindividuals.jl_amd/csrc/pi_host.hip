@@ -103,6 +103,7 @@ struct pi_model {
     uint32_t *skey = nullptr, *ssrc = nullptr, *big = nullptr, *lng = nullptr, *bm = nullptr, *G = nullptr;
     int long_blocks = 1;       /* blocks of the one-block-per-long-cell kernels */
     int ncu = 256;             /* compute units of the device */
+    uint32_t qcap = 0;         /* entries per queue shard */
     Plan plan;                 /* the launches of the last step built */
     bool dry = false;          /* create_impl only sums the allocation sizes */
     long long bytes = 0;       /* device bytes allocated (256-byte granules) */
@@ -358,7 +359,10 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     DALLOC(m->con.tag, m->ctot);
     DALLOC(m->d_count, 8);
     DALLOC(m->newc, 2 * m->ctot * 4); DALLOC(m->stayf, 2 * m->ctot * 4); DALLOC(m->P, (2 * m->ctot + 1) * 4);
-    DALLOC(m->segp, (nc + 1) * 4); DALLOC(m->small, 2 * m->ctot * 4);
+    DALLOC(m->segp, (nc + 1) * 4);
+    /* queue shards: shard q holds the entries e < 2·ctot with (e / 256) mod PI_NQ = q */
+    m->qcap = (uint32_t)(((2 * m->ctot + 255) / 256 + PI_NQ - 1) / PI_NQ * 256);
+    DALLOC(m->small, (long long)m->qcap * PI_NQ * 4);
     DALLOC(m->cnts, nc * 4); DALLOC(m->mix, nc * 4);
     DALLOC(m->sflag, m->ctot);
     DALLOC(m->dstamp, m->ctot * 4); DALLOC(m->rstamp, m->ctot * 4);
@@ -747,7 +751,7 @@ static void build_step(pi_model* m, float dt, Plan& P)
         aa.sy = std::sqrt(m->bgc[PI_B_kappa_hP] * dt);
         aa.sz = std::sqrt(m->bgc[PI_B_kappa_vP] * dt);
         aa.seed = m->seed;
-        aa.newc = m->newc; aa.stayf = m->stayf; aa.small = m->small; aa.cnts = m->cnts;
+        aa.newc = m->newc; aa.stayf = m->stayf; aa.small = m->small; aa.qcap = m->qcap; aa.cnts = m->cnts;
         aa.dstamp = m->dstamp; aa.dcell = m->dcell; aa.cap = m->cap;
         /* up to 16384 blocks: with 8.4M individuals (two entries per thread)
          * ~5% faster than 4096 blocks; 32768 is no faster, 65536 much slower */
@@ -759,7 +763,7 @@ static void build_step(pi_model* m, float dt, Plan& P)
         plan_scan(P, "bin", sp, m->partials, m->P);
         BinArgs b = zeroed<BinArgs>();
         b.st = st; b.ncell = m->ncell; b.seg = m->seg; b.segp = m->segp; b.P = m->P; b.stayf = m->stayf;
-        b.newc = m->newc; b.small = m->small; b.cnts = m->cnts; b.cnt = m->cnt; b.lng = m->lng;
+        b.newc = m->newc; b.small = m->small; b.qcap = m->qcap; b.cnts = m->cnts; b.cnt = m->cnt; b.lng = m->lng;
         b.pkey = m->pkey; b.psrc = m->psrc; b.mix = m->mix; b.big = m->big; b.skey = m->skey; b.ssrc = m->ssrc;
         b.Chl = m->Chl; b.pop = m->pop;
         P.add(K_CELLCOUNT, "bin", grid_for(m->ncell, 256, 8192), b);
