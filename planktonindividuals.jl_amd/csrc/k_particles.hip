@@ -320,6 +320,25 @@ __device__ __forceinline__ void k_place_stay_impl(Blk K, const DevState* st, con
 __device__ __forceinline__ void k_place_stay_body(const BinArgs& x, Blk K) { k_place_stay_impl(K, x.st, x.seg, x.segp, x.P, x.stayf, x.newc, x.pkey, x.psrc); }
 PI_KERNEL_XCD(k_place_stay, BinArgs, 256)
 
+/* exclusive prefix over the 256 threads of a block of v; *tot = the block total */
+__device__ __forceinline__ uint32_t block_excl(uint32_t v, uint32_t* ws, uint32_t& tot)
+{
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) ws[w] = x;
+    __syncthreads();
+    uint32_t off = 0;
+    for (uint32_t i = 0; i < w; ++i) off += ws[i];
+    tot = ws[0] + ws[1] + ws[2] + ws[3];
+    __syncthreads();
+    return off + x - v;
+}
+
 /* queued individuals: after the stayers of their cell, in any order; the
  * cell is listed for re-sorting once (by the last one placed) */
 __device__ __forceinline__ void k_place_small_impl(Blk K, DevState* st, const uint32_t* seg, const uint32_t* small,
@@ -329,17 +348,13 @@ __device__ __forceinline__ void k_place_small_impl(Blk K, DevState* st, const ui
     if (st->err) return;
     const GStore S = gstore(st->store[st->scur]);
     /* queue position t -> shard q (s_pre[q] <= t < s_pre[q + 1]) and its entry */
-    __shared__ uint32_t s_pre[PI_NQ + 1];
-    if (threadIdx.x < PI_NQ) {
-        const uint32_t lane = threadIdx.x;
-        uint32_t x = st->nsmall[lane];
-#pragma unroll
-        for (int o = 1; o < PI_NQ; o <<= 1) {
-            const uint32_t y = (uint32_t)__shfl_up((int)x, o, 64);
-            if (lane >= (uint32_t)o) x += y;
-        }
-        s_pre[lane + 1] = x;
-        if (lane == 0) s_pre[0] = 0;
+    __shared__ uint32_t s_pre[PI_NQ + 1], s_ws[4];
+    {
+        static_assert(PI_NQ <= 256, "one shard count per thread of the 256-thread block");
+        uint32_t tot;
+        const uint32_t pre = block_excl(threadIdx.x < PI_NQ ? st->nsmall[threadIdx.x] : 0u, s_ws, tot);
+        if (threadIdx.x < PI_NQ) s_pre[threadIdx.x] = pre;
+        if (threadIdx.x == 0) s_pre[PI_NQ] = tot;
     }
     __syncthreads();
     const uint32_t ns = s_pre[PI_NQ];
@@ -1181,24 +1196,6 @@ __device__ __forceinline__ void k_plk_body(const PlkArgs& a, Blk K)
 }
 PI_KERNEL_XCD(k_plk, PlkArgs, 256)
 
-/* exclusive prefix over the 256 threads of a block of v; *tot = the block total */
-__device__ __forceinline__ uint32_t block_excl(uint32_t v, uint32_t* ws, uint32_t& tot)
-{
-    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= (uint32_t)o) x += y;
-    }
-    if (lane == 63) ws[w] = x;
-    __syncthreads();
-    uint32_t off = 0;
-    for (uint32_t i = 0; i < w; ++i) off += ws[i];
-    tot = ws[0] + ws[1] + ws[2] + ws[3];
-    __syncthreads();
-    return off + x - v;
-}
 
 /* The ordered sums of k_plk for the listed long cells, one block per cell.
  * The sums live in lanes 0-9 of wave 0, one tracer per lane in the order
