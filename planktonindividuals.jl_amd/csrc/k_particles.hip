@@ -40,13 +40,11 @@
  * u_itpl / v_itpl / w_itpl with ac = true (Individuals/Advection/interpolation.jl:1-83).
  * Corner indices are clamped into the padded array so that positions outside
  * the halo (not produced by the reference's boundary conditions) cannot fault.
- * The corners (x, x+1) are read in pairs, four 8-byte loads per call.  The
- * kernel is bound by the texture data path returning its 48 gathers per
- * individual (TD busy ~93% of the kernel on MI355X): fewer VALU instructions
- * (a 5-instruction exact quotient) and a lane order with fewer cache lines
- * per gather were both measured to leave its time unchanged, and staging each
- * block's velocity neighbourhood in LDS (27 rows per velocity set) relieved
- * the texture path but made the kernel VALU-bound and 20% slower. */
+ * The corners (x, x+1) are read in pairs, four 8-byte loads per call.  Measured
+ * on MI355X and not kept: a 5-instruction exact quotient (13% fewer VALU
+ * instructions: slower), re-dealing lanes by corner rows (fewer cache lines per
+ * gather: no change), staging each block's velocity neighbourhood in LDS (the
+ * texture path relieved, 20% slower). */
 __device__ __forceinline__ float itpl(const DevGrid& g, const float* __restrict__ f,
                                       float x, float y, float z, int comp)
 {
