@@ -337,6 +337,27 @@ __device__ __forceinline__ uint32_t block_excl(uint32_t v, uint32_t* ws, uint32_
     return off + x - v;
 }
 
+/* exclusive prefix of the PI_NQ shard counts into pre[0..PI_NQ] (pre[PI_NQ] =
+ * total), by every thread of the block */
+__device__ __forceinline__ void shard_prefix(const uint32_t* cnt, uint32_t* pre, uint32_t* ws)
+{
+    static_assert(PI_NQ <= 256, "one shard count per thread of the 256-thread block");
+    uint32_t tot;
+    const uint32_t p = block_excl(threadIdx.x < PI_NQ ? cnt[threadIdx.x] : 0u, ws, tot);
+    if (threadIdx.x < PI_NQ) pre[threadIdx.x] = p;
+    if (threadIdx.x == 0) pre[PI_NQ] = tot;
+    __syncthreads();
+}
+/* the shard q of list position t: pre[q] <= t < pre[q + 1] */
+__device__ __forceinline__ uint32_t shard_of(const uint32_t* pre, uint32_t t)
+{
+    uint32_t lo = 0, hi = PI_NQ;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
 /* queued individuals: after the stayers of their cell, in any order; the
  * cell is listed for re-sorting once (by the last one placed) */
 __device__ __forceinline__ void k_place_small_impl(Blk K, DevState* st, const uint32_t* seg, const uint32_t* small,
@@ -347,25 +368,14 @@ __device__ __forceinline__ void k_place_small_impl(Blk K, DevState* st, const ui
     const GStore S = gstore(st->store[st->scur]);
     /* queue position t -> shard q (s_pre[q] <= t < s_pre[q + 1]) and its entry */
     __shared__ uint32_t s_pre[PI_NQ + 1], s_ws[4];
-    {
-        static_assert(PI_NQ <= 256, "one shard count per thread of the 256-thread block");
-        uint32_t tot;
-        const uint32_t pre = block_excl(threadIdx.x < PI_NQ ? st->nsmall[threadIdx.x] : 0u, s_ws, tot);
-        if (threadIdx.x < PI_NQ) s_pre[threadIdx.x] = pre;
-        if (threadIdx.x == 0) s_pre[PI_NQ] = tot;
-    }
-    __syncthreads();
+    shard_prefix(st->nsmall, s_pre, s_ws);
     const uint32_t ns = s_pre[PI_NQ];
     const uint32_t nloop = (ns + 63u) & ~63u;
     for (uint32_t t = K.b * blockDim.x + threadIdx.x; t < nloop; t += K.n * blockDim.x) {
         bool last = false;
         uint32_t c = 0;
         if (t < ns) {
-            uint32_t lo = 0, hi = PI_NQ;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_pre[mid] <= t) lo = mid; else hi = mid;
-            }
+            const uint32_t lo = shard_of(s_pre, t);
             const uint32_t e = small[lo * qcap + (t - s_pre[lo])];
             c = newc[e];
             const uint32_t r = atomicSub(&cnts[c], 1u);       /* cnts returns to zero */
@@ -374,8 +384,9 @@ __device__ __forceinline__ void k_place_small_impl(Blk K, DevState* st, const ui
             psrc[pos] = e;
             last = r == 1u;
         }
-        const uint32_t mi = wave_append(&st->nmix, last);
-        if (last) mix[mi] = c;
+        const uint32_t sq = (t >> 8) & (PI_NQ - 1);    /* the same for every lane of a wave */
+        const uint32_t mi = wave_append(&st->nmix[sq], last);
+        if (last) mix[sq * qcap + mi] = c;
     }
 }
 __device__ __forceinline__ void k_place_small_body(const BinArgs& x, Blk K) { k_place_small_impl(K, x.st, x.seg, x.small, x.qcap, x.newc, x.cnts, x.pkey, x.psrc, x.mix); }
@@ -384,12 +395,15 @@ PI_KERNEL(k_place_small, BinArgs, 256)
 /* re-sort the listed cells by key (species-major, then slot): insertion sort
  * for short segments, the rest go to k_bigcells */
 __device__ __forceinline__ void k_mixsort_impl(Blk K, DevState* st, const uint32_t* seg, const uint32_t* mix,
-                                                 uint32_t* pkey, uint32_t* psrc, uint32_t* big)
+                                                 uint32_t qcap, uint32_t* pkey, uint32_t* psrc, uint32_t* big)
 {
     if (st->err) return;
-    const uint32_t nm = st->nmix;
+    __shared__ uint32_t s_pre[PI_NQ + 1], s_ws[4];
+    shard_prefix(st->nmix, s_pre, s_ws);
+    const uint32_t nm = s_pre[PI_NQ];
     for (uint32_t m = K.b * blockDim.x + threadIdx.x; m < nm; m += K.n * blockDim.x) {
-        const uint32_t c = mix[m];
+        const uint32_t q = shard_of(s_pre, m);
+        const uint32_t c = mix[q * qcap + (m - s_pre[q])];
         const uint32_t s0 = seg[c], n = seg[c + 1] - s0;
         if (n > SMALL_SEG) { big[atomicAdd(&st->nbig, 1u)] = c; continue; }
         for (uint32_t a = 1; a < n; ++a) {
@@ -405,7 +419,7 @@ __device__ __forceinline__ void k_mixsort_impl(Blk K, DevState* st, const uint32
         }
     }
 }
-__device__ __forceinline__ void k_mixsort_body(const BinArgs& x, Blk K) { k_mixsort_impl(K, x.st, x.seg, x.mix, x.pkey, x.psrc, x.big); }
+__device__ __forceinline__ void k_mixsort_body(const BinArgs& x, Blk K) { k_mixsort_impl(K, x.st, x.seg, x.mix, x.qcap, x.pkey, x.psrc, x.big); }
 PI_KERNEL(k_mixsort, BinArgs, 256)
 
 /* acc_chl! / acc_counts! (Individuals/utils.jl:33-52) as an ordered segmented sum */

@@ -140,8 +140,7 @@ __device__ __forceinline__ void k_capcheck_impl(Blk K, DevState* st, const uint3
     }
     st->nbig = 0;
     st->nlong = 0;
-    for (int q = 0; q < PI_NQ; ++q) st->nsmall[q] = 0;
-    st->nmix = 0;
+    for (int q = 0; q < PI_NQ; ++q) { st->nsmall[q] = 0; st->nmix[q] = 0; }
 }
 __device__ __forceinline__ void k_capcheck_body(const CapArgs& x, Blk K) { k_capcheck_impl(K, x.st, x.G, x.nsp, x.nw); }
 PI_KERNEL(k_capcheck, CapArgs, 64)

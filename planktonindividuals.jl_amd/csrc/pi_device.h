@@ -234,7 +234,7 @@ struct DevState {
     uint32_t nstore;                   /* entries in store[scur] */
     uint32_t nA;                       /* of which the first nA are sorted by (cell, key) as laid out by the previous seg */
     uint32_t nsmall[PI_NQ];            /* queued entries of this step, per queue shard */
-    uint32_t nmix;                     /* re-sorted cells of this step */
+    uint32_t nmix[PI_NQ];              /* re-sorted cells of this step, per shard (of the queue position) */
     uint32_t err, err_sp;
     uint32_t nbig;
     uint32_t nlong;                    /* cells longer than LONG_SEG this step */

@@ -363,7 +363,8 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     /* queue shards: shard q holds the entries e < 2·ctot with (e / 256) mod PI_NQ = q */
     m->qcap = (uint32_t)(((2 * m->ctot + 255) / 256 + PI_NQ - 1) / PI_NQ * 256);
     DALLOC(m->small, (long long)m->qcap * PI_NQ * 4);
-    DALLOC(m->cnts, nc * 4); DALLOC(m->mix, nc * 4);
+    DALLOC(m->cnts, nc * 4);
+    DALLOC(m->mix, (long long)m->qcap * PI_NQ * 4);   /* shards as the queue's (by queue position) */
     DALLOC(m->sflag, m->ctot);
     DALLOC(m->dstamp, m->ctot * 4); DALLOC(m->rstamp, m->ctot * 4);
     DALLOC(m->dcell, m->ctot * 4); DALLOC(m->rcons, m->ctot * 5 * 4);
