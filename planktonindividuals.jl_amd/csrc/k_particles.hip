@@ -906,15 +906,21 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
     __syncthreads();
     const uint32_t hhh = (uint32_t)gidx(g, g.Hx, g.Hy, g.Hz);
     const uint32_t stride = K.n * blockDim.x;
-    /* the entry, its source and its cell are loaded one iteration ahead, so the
-     * state and tracer gathers of an entry are issued together at the top of
-     * its iteration (one memory latency per entry instead of three) */
+    /* the entry and its source are loaded two iterations ahead and its cell
+     * (newc of the source) one ahead, so the state and tracer gathers of an
+     * entry are issued together at the top of its iteration and no load waits
+     * on another within an iteration (one memory latency per entry instead of
+     * three). Staging the next entry's state in LDS instead gains nothing: the
+     * wait for it also waits for the previous entry's stores. */
     uint32_t e = K.b * blockDim.x + threadIdx.x;
-    uint32_t key_n = 0, src_n = 0, c_n = 0;
+    uint32_t key_n = 0, src_n = 0, c_n = 0, key_nn = 0, src_nn = 0;
     if (e < n) { key_n = a.pkey[e]; src_n = a.psrc[e]; c_n = a.newc[src_n]; }
+    if (e < n && n - e > stride) { key_nn = a.pkey[e + stride]; src_nn = a.psrc[e + stride]; }
     for (; e < n; e += stride) {
         const uint32_t key = key_n, src = src_n, c = c_n;
-        if (e + stride < n) { key_n = a.pkey[e + stride]; src_n = a.psrc[e + stride]; c_n = a.newc[src_n]; }
+        key_n = key_nn; src_n = src_nn;
+        if (n - e > stride) c_n = a.newc[src_n];
+        if (n - e > 2 * stride) { key_nn = a.pkey[e + 2 * stride]; src_nn = a.psrc[e + 2 * stride]; }
         const int sp = (int)(key >> PI_SLOT_BITS);
         const uint32_t slot = key & PI_SLOT_MASK;
         const float* q = s_par + sp * PI_NPARAMS;
