@@ -227,3 +227,34 @@ def test_block_budget_does_not_change_results(pi):
         assert not bad, f"member {k}: " + "\n".join(bad[:20])
     for b, _ in sets:
         b.close()
+
+
+@pytest.mark.timeout(300)
+def test_full_size_batched_matches_alone(pi):
+    """The throughput workload at its full size: 256 x 256 x 64, periodic x/y,
+    2 species x 4,194,304 individuals, capacity 2 x that. Stepped alone (every
+    kernel on its full grid) and as a one-member batch on a block budget of 512
+    (a few blocks per kernel, so each thread walks many entries, cells and
+    columns), the states after 3 steps are identical. The oracle would take
+    minutes at this size; this pins the strided paths of every kernel (the
+    look-ahead loads across iterations, the shard queues near their capacity)
+    at full scale, against the small-grid paths the oracle checks."""
+    spec, uvw = pi.workloads.config_3d(n=256, nz=64, N=4194304, nsp=2, cap=2 * 4194304, seed=2)
+    a = pi.PlanktonModel(pi.HIPArch(), spec=spec)._native
+    b = pi.PlanktonModel(pi.HIPArch(), spec=spec)._native
+    _inputs(pi, a, spec, uvw)
+    _inputs(pi, b, spec, uvw)
+    a.run(60.0, 3)
+    a.sync()
+    batch = pi._native.NativeBatch()
+    batch.set_budget(512)
+    batch.add(b)
+    batch.run(60.0, 3)
+    batch.sync()
+    n = [a.count_active(s) for s in range(2)]
+    assert all(k > 4_000_000 for k in n), n
+    bad = compare_models(a, b, 2)
+    assert not bad, "\n".join(bad[:20])
+    batch.close()
+    a.close()
+    b.close()
