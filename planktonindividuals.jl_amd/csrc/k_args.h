@@ -32,6 +32,7 @@ struct PhysArgs {
     const uint32_t* pkey;
     const uint32_t* psrc;
     const float *temp, *pop;
+    int sp_pop;               /* 1: pop per species (shared_graz != 1), counted in the cell's segment */
     const uint32_t* bmA;
     const uint32_t* bmD;
     const uint32_t* G;        /* exclusive scan over [D words | free words] of all species */
@@ -144,6 +145,7 @@ struct BinArgs {
     uint32_t* big;
     uint32_t *skey, *ssrc;
     float *Chl, *pop;
+    int pop_sp;               /* -1: pop counts every species (shared_graz = 1); else only this species */
 };
 
 struct ParArgs {

@@ -422,9 +422,31 @@ __device__ __forceinline__ void k_mixsort_impl(Blk K, DevState* st, const uint32
 __device__ __forceinline__ void k_mixsort_body(const BinArgs& x, Blk K) { k_mixsort_impl(K, x.st, x.seg, x.mix, x.qcap, x.pkey, x.psrc, x.big); }
 PI_KERNEL(k_mixsort, BinArgs, 256)
 
-/* acc_chl! / acc_counts! (Individuals/utils.jl:33-52) as an ordered segmented sum */
+/* the first position in the key-sorted range [lo, hi) whose key is >= k */
+__device__ __forceinline__ uint32_t key_lower_bound(const uint32_t* pkey, uint32_t lo, uint32_t hi, uint32_t k)
+{
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pkey[mid] < k) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+/* individuals of species sp in a cell's segment [s0, s1) (sorted by key,
+ * species-major): what acc_counts! leaves in pop for that species alone
+ * when shared_graz != 1 (time_step.jl:98-102: pop zeroed per species) */
+__device__ __forceinline__ uint32_t species_count(const uint32_t* pkey, uint32_t s0, uint32_t s1, uint32_t sp)
+{
+    const uint32_t a = key_lower_bound(pkey, s0, s1, sp << PI_SLOT_BITS);
+    const uint32_t b = key_lower_bound(pkey, a, s1, (sp + 1u) << PI_SLOT_BITS);
+    return b - a;
+}
+
+/* acc_chl! / acc_counts! (Individuals/utils.jl:33-52) as an ordered segmented
+ * sum.  pop_sp >= 0 (shared_graz != 1): pop holds the counts of that species
+ * only, as the reference's pop after its last species (time_step.jl:98-102). */
 __device__ __forceinline__ void k_cellsum_impl(Blk K, const DevState* st, long long ncell, const uint32_t* seg,
-                                                 const uint32_t* psrc, float* Chl, float* pop)
+                                                 const uint32_t* psrc, const uint32_t* pkey, int pop_sp,
+                                                 float* Chl, float* pop)
 {
     if (st->err) return;
     const PI_GLOBAL float* chl_store = gp(st->store[st->scur].Chl);
@@ -433,11 +455,11 @@ __device__ __forceinline__ void k_cellsum_impl(Blk K, const DevState* st, long l
         if (n > LONG_SEG) continue;                  /* k_cellsum_long */
         float sum = 0.0f;
         for (uint32_t a = 0; a < n; ++a) sum = sum + chl_store[psrc[s0 + a]];
-        pop[c] = (float)n;
+        pop[c] = (float)(pop_sp < 0 ? n : species_count(pkey, s0, s0 + n, (uint32_t)pop_sp));
         Chl[c] = sum;
     }
 }
-__device__ __forceinline__ void k_cellsum_body(const BinArgs& x, Blk K) { k_cellsum_impl(K, x.st, x.ncell, x.seg, x.psrc, x.Chl, x.pop); }
+__device__ __forceinline__ void k_cellsum_body(const BinArgs& x, Blk K) { k_cellsum_impl(K, x.st, x.ncell, x.seg, x.psrc, x.pkey, x.pop_sp, x.Chl, x.pop); }
 PI_KERNEL_XCD(k_cellsum, BinArgs, 256)
 
 /* ordered sum of m staged values (row, 16-byte aligned) onto s: blocks of 32
@@ -494,7 +516,8 @@ __device__ __forceinline__ void stage_chl(const PI_GLOBAL float* chl, const uint
 /* k_cellsum for the listed long cells: the Chl values of a chunk are gathered
  * into LDS by waves 1-3 while wave 0 adds up the previous chunk */
 __device__ __forceinline__ void k_cellsum_long_impl(Blk K, const DevState* st, const uint32_t* lng, const uint32_t* seg,
-                                                      const uint32_t* psrc, float* Chl, float* pop)
+                                                      const uint32_t* psrc, const uint32_t* pkey, int pop_sp,
+                                                      float* Chl, float* pop)
 {
     if (st->err) return;
     __shared__ __align__(16) float buf[2][LCH];
@@ -517,10 +540,13 @@ __device__ __forceinline__ void k_cellsum_long_impl(Blk K, const DevState* st, c
             }
             __syncthreads();
         }
-        if (t == 0) { pop[c] = (float)(s1 - s0); Chl[c] = sum; }
+        if (t == 0) {
+            pop[c] = (float)(pop_sp < 0 ? s1 - s0 : species_count(pkey, s0, s1, (uint32_t)pop_sp));
+            Chl[c] = sum;
+        }
     }
 }
-__device__ __forceinline__ void k_cellsum_long_body(const BinArgs& x, Blk K) { k_cellsum_long_impl(K, x.st, x.lng, x.seg, x.psrc, x.Chl, x.pop); }
+__device__ __forceinline__ void k_cellsum_long_body(const BinArgs& x, Blk K) { k_cellsum_long_impl(K, x.st, x.lng, x.seg, x.psrc, x.pkey, x.pop_sp, x.Chl, x.pop); }
 PI_KERNEL(k_cellsum_long, BinArgs, 256)
 
 /* Long re-sorted segments, one block per cell.  The cell holds its stayers,
@@ -941,7 +967,8 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
         t.par = PAR[c];
         t.dpar = PAR[c] - PAR0[c];
         t.T = a.temp[c];
-        t.pop = a.pop[c];
+        /* shared_graz != 1: pop of this species only (time_step.jl:98-107) */
+        t.pop = a.sp_pop ? (float)species_count(a.pkey, a.seg[c], a.seg[c + 1], (uint32_t)sp) : a.pop[c];
         /* grazing! then mortality! (QuotaMode/division_death.jl:2-25, consume_loss.jl:17-33) */
         bool ac = true;
         uint8_t tag = 0;
@@ -1005,7 +1032,7 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
                 tz.par = PAR[dc];
                 tz.dpar = PAR[dc] - PAR0[dc];
                 tz.T = a.temp[dc];
-                tz.pop = a.pop[dc];
+                tz.pop = a.sp_pop ? (float)species_count(a.pkey, a.seg[dc], a.seg[dc + 1], (uint32_t)sp) : a.pop[dc];
             } else {
                 tz.NH4 = tz.NO3 = tz.PO4 = tz.DOC = tz.DFe = tz.par = tz.dpar = tz.T = tz.pop = 0.0f;
             }

@@ -409,8 +409,6 @@ static pi_status check_desc(const pi_model_desc* d)
     if (gd.topo_z != PI_BOUNDED) return fail(PI_ERR_ARG, "grid: z topology must be Bounded");
     if (d->n_species < 1 || d->n_species > PI_MAXSP) return fail(PI_ERR_ARG, "n_species out of range");
     if (d->capacity < 1 || d->capacity > (long long)PI_SLOT_MASK) return fail(PI_ERR_ARG, "capacity out of range");
-    if (d->bgc[PI_B_shared_graz] != 1.0f)
-        return fail(PI_ERR_UNSUPPORTED, "only shared_graz = 1 is implemented on the GPU path");
     for (int s = 0; s < d->n_species; ++s) {
         float t = d->params[s * PI_NPARAMS + PI_P_dvid_type];
         if (!(t == 1.0f || t == 2.0f || t == 3.0f)) return fail(PI_ERR_ARG, "Wrong cell division type, must be in 1 to 3");
@@ -767,6 +765,7 @@ static void build_step(pi_model* m, float dt, Plan& P)
         b.newc = m->newc; b.small = m->small; b.qcap = m->qcap; b.cnts = m->cnts; b.cnt = m->cnt; b.lng = m->lng;
         b.pkey = m->pkey; b.psrc = m->psrc; b.mix = m->mix; b.big = m->big; b.skey = m->skey; b.ssrc = m->ssrc;
         b.Chl = m->Chl; b.pop = m->pop;
+        b.pop_sp = m->bgc[PI_B_shared_graz] == 1.0f ? -1 : m->nsp - 1;
         P.add(K_CELLCOUNT, "bin", grid_for(m->ncell, 256, 8192), b);
         plan_scan(P, "bin", scan_plain(m->cnt, (uint32_t)m->ncell), m->partials, m->seg);
         P.add(K_PLACE_STAY, "bin", pb, b);
@@ -789,6 +788,7 @@ static void build_step(pi_model* m, float dt, Plan& P)
         PhysArgs pa = zeroed<PhysArgs>();
         pa.g = g; pa.st = st; pa.tab = m->tab; pa.cap = m->cap; pa.nsp = m->nsp; pa.params = m->d_params;
         pa.seg = m->seg; pa.pkey = m->pkey; pa.psrc = m->psrc; pa.temp = m->temp; pa.pop = m->pop;
+        pa.sp_pop = m->bgc[PI_B_shared_graz] == 1.0f ? 0 : 1;   /* time_step.jl:21 vs :80 */
         pa.bmA = m->bm; pa.bmD = m->bm + (size_t)m->nsp * m->nw; pa.G = m->G; pa.nw = m->nw;
         pa.newc = m->newc; pa.sflag = m->sflag;
         pa.con = m->con; pa.diag_num = m->diag; pa.dt = dt; pa.seed = m->seed;

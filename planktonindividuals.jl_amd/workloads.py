@@ -43,8 +43,11 @@ def fourier_flow(grid, nmodes=64, rms=0.05, seed=0, kmax=None):
     xc = (np.arange(Nx) + 0.5) * dx
     yc = (np.arange(Ny) + 0.5) * dy
     kk = (np.arange(Nz) + 0.5) / Nz           # normalised depth of cell centres
-    psi = np.zeros((Nx + 1, Ny + 1, Nz))
-    chi = np.zeros((Nx, Ny, Nz))
+    # every mode's vertical structure is one of three cosines (streamfunction)
+    # or two (divergent part): the horizontal patterns are summed per vertical
+    # structure and expanded in depth once
+    psi2 = np.zeros((3, Nx + 1, Ny + 1))
+    chi2 = np.zeros((2, Nx, Ny))
     for m in range(nmodes):
         while True:
             nx_, ny_ = rng.integers(-kmax, kmax + 1, size=2)
@@ -54,12 +57,16 @@ def fourier_flow(grid, nmodes=64, rms=0.05, seed=0, kmax=None):
         kmag = np.hypot(kx, ky)
         amp = np.sqrt(np.hypot(nx_, ny_) ** (-5.0 / 3.0)) / kmag
         ph = rng.uniform(0, 2 * np.pi)
-        nz_ = rng.integers(0, 3)
-        vert = np.cos(np.pi * nz_ * kk)
-        psi += (amp * np.cos(kx * xf[:, None] + ky * yf[None, :] + ph))[:, :, None] * vert[None, None, :]
-        vdiv = np.cos(2 * np.pi * (1 + (m % 2)) * kk)         # zero column mean
+        nz_ = int(rng.integers(0, 3))
+        psi2[nz_] += amp * np.cos(kx * xf[:, None] + ky * yf[None, :] + ph)
         ph2 = rng.uniform(0, 2 * np.pi)
-        chi += (0.3 * amp * np.cos(kx * xc[:, None] + ky * yc[None, :] + ph2))[:, :, None] * vdiv[None, None, :]
+        chi2[m % 2] += 0.3 * amp * np.cos(kx * xc[:, None] + ky * yc[None, :] + ph2)
+    psi = np.zeros((Nx + 1, Ny + 1, Nz))
+    for a in range(3):
+        psi += psi2[a][:, :, None] * np.cos(np.pi * a * kk)[None, None, :]
+    chi = np.zeros((Nx, Ny, Nz))
+    for a in range(2):
+        chi += chi2[a][:, :, None] * np.cos(2 * np.pi * (1 + a) * kk)[None, None, :]   # zero column mean
     # C-grid: u on x faces (i = 0..Nx-1, periodic), v on y faces
     u = -(psi[:Nx, 1:, :] - psi[:Nx, :-1, :]) / dy
     v = (psi[1:, :Ny, :] - psi[:-1, :Ny, :]) / dx
@@ -79,13 +86,25 @@ def fourier_flow(grid, nmodes=64, rms=0.05, seed=0, kmax=None):
 
 
 # ------------------------------------------------------------- configurations
-def config_2d(N=32768, n=128, cap=None, seed=1, overrides=None, bgc=None):
-    """Plain 2-D horizontal run: 128x128x1 periodic, Δ = 1 km, one species."""
+def noisy_tracer_init(eps=0.05):
+    """The default uniform tracer values (tracer_fields.jl:41-45) with noise of
+    relative amplitude `eps` on every tracer, in the reference's own noise form:
+    c0 * rand(1-eps : 1f-4 : 1+eps) per cell (generate_tracers,
+    tracer_fields.jl:93-134).  The workloads use eps = 0.05."""
+    from .fields import default_tracer_init
+    init = default_tracer_init()
+    init["rand_noise"] = {k: float(eps) for k in init["rand_noise"]}
+    return init
+
+
+def config_2d(N=32768, n=128, cap=None, seed=1, overrides=None, bgc=None, host_state=True):
+    """Plain 2-D horizontal run: 128x128x1 periodic, Δ = 1 km, one species,
+    tracers uniform plus 5 % noise."""
     grid = RectilinearGrid(size=(n, n, 1), x=(0, n * 1000), y=(0, n * 1000), z=(0, -10),
                            topology=(Periodic, Periodic, Bounded))
     params = dict(overrides or {})
     spec = ModelSpec(grid, max_individuals=cap or 2 * N, phyto=phyto_setup(params or None, [N], 1),
-                     bgc_params=bgc, seed=seed)
+                     bgc_params=bgc, seed=seed, tracer_initial=noisy_tracer_init(), host_state=host_state)
     return spec, cellular_flow(grid, U=0.1, L=16e3 * n / 128)
 
 
@@ -95,8 +114,9 @@ ACTIVE_PHYSIOLOGY = {"mort_reg": [1.0], "grz_P": [1.0e-6], "dvid_reg": [1.8]}
 
 
 def config_3d(n=256, nz=64, N=4194304, nsp=2, cap=None, seed=2, nmodes=64, overrides=None,
-              dz=10.0, kappa=(1e-2, 1e-5)):
-    """Plain 3-D run: n x n x nz, periodic x/y, bounded z, nsp species, Fourier flow."""
+              dz=10.0, kappa=(1e-2, 1e-5), host_state=True):
+    """Plain 3-D run: n x n x nz, periodic x/y, bounded z, nsp species, Fourier
+    flow, tracers uniform plus 5 % noise."""
     grid = RectilinearGrid(size=(n, n, nz), x=(0, n * 1000), y=(0, n * 1000), z=(0, -nz * dz),
                            topology=(Periodic, Periodic, Bounded))
     ov = dict(ACTIVE_PHYSIOLOGY)
@@ -104,7 +124,7 @@ def config_3d(n=256, nz=64, N=4194304, nsp=2, cap=None, seed=2, nmodes=64, overr
     params = {k: v * nsp if len(v) == 1 else v for k, v in ov.items()}
     bgc = {"κh": kappa[0], "κv": kappa[1], "κhP": kappa[0], "κvP": kappa[1]}
     spec = ModelSpec(grid, max_individuals=cap or 2 * N, phyto=phyto_setup(params, [N] * nsp, nsp),
-                     bgc_params=bgc, seed=seed)
+                     bgc_params=bgc, seed=seed, tracer_initial=noisy_tracer_init(), host_state=host_state)
     return spec, fourier_flow(grid, nmodes=nmodes, rms=0.05, seed=seed)
 
 

@@ -27,9 +27,18 @@ def first_diff(a, b):
     return f"{len(bad)} differ; first at {i}: {a[i]!r} vs {b[i]!r}"
 
 
-def compare_models(gpu, ref, nsp, tracer_ids=range(14), grid_fields=("par", "Chl", "pop"), diags=True):
-    """Returns a list of mismatch descriptions (empty = identical)."""
+def compare_models(gpu, ref, nsp, tracer_ids=range(14), grid_fields=("par", "Chl", "pop", "diag_PAR", "PARF"),
+                   diags=True, diag_tracers=()):
+    """Returns a list of mismatch descriptions (empty = identical).  Besides the
+    individuals and tracers: par, Chl, pop, the accumulated PAR diagnostic
+    (diags.tracer.PAR, time_step.jl:167), PARF after the column scan, the
+    per-species diagnostics, and the tracer diagnostics of `diag_tracers`
+    (time_step.jl:168-172)."""
     bad = []
+    for q in diag_tracers:
+        a, b = gpu.diag_tracer(q), ref.diag_tracer(q)
+        if not same(a, b):
+            bad.append(f"diag tracer {q}: {first_diff(a, b)}")
     for s in range(nsp):
         for f in FLOAT_FIELDS + INT_FIELDS:
             a, b = gpu.field(s, f), ref.field(s, f)

@@ -104,6 +104,32 @@ def test_batch_matches_alone(pi):
     batch.close()
 
 
+def test_batched_members_match_oracle(pi, orc):
+    """Members stepped in one batch against the oracle directly: a 0-D box, a
+    1-D column with two species and vertical flow, a 2-D slab under the
+    cellular flow, a bounded 3-D box with land and a 3-D Fourier-flow grid with
+    two species, 8 steps in one batch; every member identical to its own oracle
+    run."""
+    members = _members(pi)[:5]
+    hs = [pi.PlanktonModel(pi.HIPArch(), spec=s)._native for s, _ in members]
+    for (s, uvw), h in zip(members, hs):
+        _inputs(pi, h, s, uvw)
+    batch = pi._native.NativeBatch()
+    for h in hs:
+        batch.add(h)
+    batch.run(60.0, 8)
+    batch.sync()
+    for k, ((s, uvw), h) in enumerate(zip(members, hs)):
+        g = s.grid
+        om = orc.OracleModel(s)
+        om.update(60.0, 8, vel=uvw, parf=np.full((g.Nx, g.Ny), 550.0, np.float32),
+                  temp=np.full((g.Nx, g.Ny, g.Nz), 21.0, np.float32))
+        assert h.iteration() == 8
+        bad = compare_models(h, om, s.n_species)
+        assert not bad, f"member {k}: " + "\n".join(bad[:20])
+    batch.close()
+
+
 def test_ensemble_update_matches_update(pi):
     """Ensemble.update (batched update! of several simulations with different
     lengths and forcing) against update on each simulation alone."""
@@ -229,32 +255,100 @@ def test_block_budget_does_not_change_results(pi):
         b.close()
 
 
+def _alone_vs_budget(pi, spec, uvw, dt, steps, budget=512):
+    """The model stepped alone (every kernel on its own full grid) and as a
+    one-member batch on a budget of `budget` blocks per launch (each thread then
+    walks many entries, cells and columns): returns both handles after `steps`
+    steps.  The caller closes them."""
+    a = pi.PlanktonModel(pi.HIPArch(), spec=spec)._native
+    b = None
+    try:
+        b = pi.PlanktonModel(pi.HIPArch(), spec=spec)._native
+        _inputs(pi, a, spec, uvw)
+        _inputs(pi, b, spec, uvw)
+        a.run(dt, steps)
+        a.sync()
+        batch = pi._native.NativeBatch()
+        try:
+            batch.set_budget(budget)
+            batch.add(b)
+            batch.run(dt, steps)
+            batch.sync()
+        finally:
+            batch.close()
+    except BaseException:
+        a.close()
+        if b is not None:
+            b.close()
+        raise
+    return a, b
+
+
 @pytest.mark.timeout(300)
 def test_full_size_batched_matches_alone(pi):
     """The throughput workload at its full size: 256 x 256 x 64, periodic x/y,
-    2 species x 4,194,304 individuals, capacity 2 x that. Stepped alone (every
-    kernel on its full grid) and as a one-member batch on a block budget of 512
-    (a few blocks per kernel, so each thread walks many entries, cells and
-    columns), the states after 3 steps are identical. The oracle would take
-    minutes at this size; this pins the strided paths of every kernel (the
-    look-ahead loads across iterations, the shard queues near their capacity)
-    at full scale, against the small-grid paths the oracle checks."""
-    spec, uvw = pi.workloads.config_3d(n=256, nz=64, N=4194304, nsp=2, cap=2 * 4194304, seed=2)
-    a = pi.PlanktonModel(pi.HIPArch(), spec=spec)._native
-    b = pi.PlanktonModel(pi.HIPArch(), spec=spec)._native
-    _inputs(pi, a, spec, uvw)
-    _inputs(pi, b, spec, uvw)
-    a.run(60.0, 3)
-    a.sync()
-    batch = pi._native.NativeBatch()
-    batch.set_budget(512)
-    batch.add(b)
-    batch.run(60.0, 3)
-    batch.sync()
-    n = [a.count_active(s) for s in range(2)]
-    assert all(k > 4_000_000 for k in n), n
-    bad = compare_models(a, b, 2)
-    assert not bad, "\n".join(bad[:20])
-    batch.close()
-    a.close()
-    b.close()
+    2 species x 4,194,304 individuals, capacity 2 x that.  Stepped alone and as
+    a one-member batch on a block budget of 512, the states after 3 steps are
+    identical.  Both runs use the same kernels, so this shows that the launch
+    shape (grid-stride loops, look-ahead loads across iterations) does not change
+    results at full scale; correctness itself is pinned by the oracle tests at
+    smaller sizes."""
+    spec, uvw = pi.workloads.config_3d(n=256, nz=64, N=4194304, nsp=2, cap=2 * 4194304, seed=2, host_state=False)
+    a, b = _alone_vs_budget(pi, spec, uvw, 60.0, 3)
+    try:
+        n = [a.count_active(s) for s in range(2)]
+        assert all(k > 4_000_000 for k in n), n
+        bad = compare_models(a, b, 2)
+        assert not bad, "\n".join(bad[:20])
+    finally:
+        a.close()
+        b.close()
+
+
+def _p_budget(pi, h, spec):
+    """Total phosphorus (mol): PO4 + DOP + POP over the interior cells times
+    their volume, plus Σ (Pq + Bm·R_PC) over the active individuals
+    (the quantity test/example_2D_test.jl:19-47 checks)."""
+    g = spec.grid
+    vol = g.Δx * g.Δy * g.dzF[g.Hz:g.Hz + g.Nz].astype(np.float64)
+    tot = 0.0
+    for name in ("PO4", "DOP", "POP"):
+        c = g.interior(h.tracer(pi._abi.TRACER_ID[name])).astype(np.float64)
+        tot += float((c.sum(axis=(0, 1)) * vol).sum())
+    R_PC = pi._abi.PARAM_NAMES.index("R_PC")
+    for s in range(spec.n_species):
+        ac = h.field(s, "ac")
+        tot += float(h.field(s, "Pq")[ac].astype(np.float64).sum())
+        tot += float(h.field(s, "Bm")[ac].astype(np.float64).sum()) * float(spec.species_params[s][R_PC])
+    return tot
+
+
+@pytest.mark.timeout(600)
+def test_large_instance_invariants(pi):
+    """The large single instance at its full size: 512 x 512 x 128, 4 species x
+    8,388,608 individuals (capacity 2 x that), a 128-mode Fourier flow, dt = 30 s,
+    state drawn on the device.  After 3 steps: per species, live individuals =
+    initial - grazed - dead + divided (the per-species diagnostics, summed over
+    the grid); total phosphorus conserved to 1e-6 relative; and the run alone
+    equals the run as a one-member batch on 512 blocks."""
+    N = 8388608
+    spec, uvw = pi.workloads.config_3d(n=512, nz=128, N=N, nsp=4, cap=2 * N, seed=3, nmodes=128, host_state=False)
+    a, b = _alone_vs_budget(pi, spec, uvw, 30.0, 3)
+    try:
+        # the initial budget, from a fresh copy of the same device-drawn state
+        c = pi.PlanktonModel(pi.HIPArch(), spec=spec)._native
+        try:
+            P0 = _p_budget(pi, c, spec)
+        finally:
+            c.close()
+        for s in range(4):
+            d = {k: int(a.diag(s, k).astype(np.float64).sum()) for k in ("graz", "mort", "dvid")}
+            assert d["dvid"] > 0 and d["mort"] > 0, d
+            assert a.count_active(s) == N - d["graz"] - d["mort"] + d["dvid"], (s, d, a.count_active(s))
+        P3 = _p_budget(pi, a, spec)
+        assert abs(P3 - P0) <= 1e-6 * P0, (P0, P3)
+        bad = compare_models(a, b, 4)
+        assert not bad, "\n".join(bad[:20])
+    finally:
+        a.close()
+        b.close()

@@ -9,18 +9,23 @@ from parity import compare_models
 pytestmark = pytest.mark.gpu
 
 
-def _drive(pi, orc, spec, uvw, dt, n, parf_dt=600.0, seed=3):
+def _drive(pi, orc, spec, uvw, dt, n, parf_dt=600.0, seed=3, diag_tracers=(), PARF=None):
+    """The same run through update! on the GPU and through the oracle's literal
+    update! (inputs re-staged every step).  `diag_tracers`: tracer names also
+    accumulated into diags.tracer (PlanktonDiagnostics(model; tracer=...))."""
     g = spec.grid
     rng = np.random.default_rng(seed)
     nt = int(np.ceil(n * dt / parf_dt)) + 1
-    PARF = (400.0 + 300.0 * rng.random((g.Nx, g.Ny, nt))).astype(np.float32)
+    if PARF is None:
+        PARF = (400.0 + 300.0 * rng.random((g.Nx, g.Ny, nt))).astype(np.float32)
     temp = pi.default_temperature(g, dt, n).astype(np.float32)
     u, v, w = uvw
     gm = pi.PlanktonModel(pi.HIPArch(), spec=spec)
-    sim = pi.PlanktonSimulation(gm, ΔT=dt, iterations=n, PARF=PARF, temp=temp, ΔT_PAR=parf_dt,
+    diags = pi.PlanktonDiagnostics(gm, tracer=("PAR",) + tuple(diag_tracers))
+    sim = pi.PlanktonSimulation(gm, ΔT=dt, iterations=n, PARF=PARF, temp=temp, ΔT_PAR=parf_dt, diags=diags,
                                 vels=dict(u=u[..., None], v=v[..., None], w=w[..., None]), ΔT_vel=dt * 1e6)
     pi.update(sim)
-    om = orc.OracleModel(spec)
+    om = orc.OracleModel(spec, diag_tracers=[pi._abi.TRACER_ID[t] for t in diag_tracers])
     idx = lambda t, s: int(np.floor(np.float32(t) / np.float32(s)))
     om.update(dt, n, vel=(u, v, w), parf_fn=lambda t: PARF[:, :, idx(t, parf_dt)],
               temp_fn=lambda t: temp[:, :, :, idx(t, 3600.0)])
@@ -334,4 +339,83 @@ def test_parity_deep_columns(pi, orc):
     w[:, :, nz] = 0.0
     gm, om = _drive(pi, orc, spec, (u, v, w), 60.0, 6)
     bad = compare_models(gm._native, om, 1)
+    assert not bad, "\n".join(bad[:20])
+
+
+def test_parity_species_specific_grazing(pi, orc):
+    """shared_graz != 1 (time_step.jl:80-118): pop is counted per species, so each
+    species' grazing probability and uptake caps see only its own individuals
+    per cell, and the pop field keeps the last species' counts.  Three species
+    with different grazing rates on a 3-D grid, tracer diagnostics on."""
+    g = pi.RectilinearGrid(size=(12, 10, 6), x=(0, 12e3), y=(0, 10e3), z=(0, -60))
+    spec = pi.ModelSpec(g, max_individuals=5000, seed=41,
+                        phyto=pi.phyto_setup({"dvid_P": [2e-3, 1e-3, 3e-3], "mort_reg": [1.0, 1.0, 1.0],
+                                              "grz_P": [4e-4, 1e-4, 8e-4], "dvid_reg": [1.5, 1.6, 1.4]},
+                                             [2500, 1500, 2000], 3),
+                        bgc_params={"κhP": 1.0, "κvP": 1e-4, "κh": 10.0, "κv": 1e-4, "shared_graz": 0.0})
+    u, v, w = pi.workloads.cellular_flow(g, U=2.0, L=6e3)
+    dtr = ("NH4", "DOC", "PFe_bio")
+    gm, om = _drive(pi, orc, spec, (u, v, w), 60.0, 14, diag_tracers=dtr)
+    for s in range(3):
+        assert gm._native.count_active(s) == om.count_active(s)
+    assert sum(float(om.diag(s, "graz").sum()) for s in range(3)) > 0
+    bad = compare_models(gm._native, om, 3, diag_tracers=[pi._abi.TRACER_ID[t] for t in dtr])
+    assert not bad, "\n".join(bad[:20])
+
+
+@pytest.mark.parametrize("dvid_type", [2, 3])
+def test_parity_division_timer(pi, orc, dvid_type):
+    """Timer (type 2) and Sizer_Timer (type 3) division
+    (division_death_probability.jl:9-12): the probability depends on the hour of
+    the model day t % 86400 / 3600; 5-minute steps cross the 1-hour threshold
+    of dvid_reg (type 2) or dvid_reg2 (type 3) halfway through the run."""
+    g = pi.RectilinearGrid(size=(16, 16, 2), x=(0, 16e3), y=(0, 16e3), z=(0, -20))
+    over = {"dvid_type": [dvid_type], "dvid_P": [1e-4 if dvid_type == 2 else 5e-4], "mort_reg": [1.0]}
+    if dvid_type == 2:
+        over["dvid_reg"] = [1.0]
+    else:
+        over.update(dvid_reg=[1.6], dvid_reg2=[1.0])
+    spec = pi.ModelSpec(g, max_individuals=6000, seed=43, phyto=pi.phyto_setup(over, [2500], 1),
+                        bgc_params={"κhP": 1.0, "κvP": 1e-4, "κh": 10.0, "κv": 1e-4})
+    uvw = pi.workloads.cellular_flow(g, U=1.0, L=4e3)
+    gm, om = _drive(pi, orc, spec, uvw, 300.0, 20, parf_dt=1200.0)
+    assert om.diag(0, "dvid").sum() > 0
+    assert gm._native.count_active(0) == om.count_active(0)
+    bad = compare_models(gm._native, om, 1)
+    assert not bad, "\n".join(bad[:20])
+
+
+def test_parity_four_species_3d(pi, orc):
+    """Four species as in the large single-instance workload (512x512x128,
+    4 species, dt = 30 s), at 16x16x12 with a 24-mode Fourier flow: the
+    species loops of time_step.jl:44-79, species-major ordered sums, and the
+    per-species division slots."""
+    spec, (u, v, w) = pi.workloads.config_3d(n=16, nz=12, N=900, nsp=4, cap=2400, seed=47, nmodes=24,
+                                             overrides={"dvid_P": [1e-3, 2e-3, 1.5e-3, 5e-4],
+                                                        "mort_P": [5e-4, 1e-3, 2e-4, 8e-4]},
+                                             kappa=(1.0, 1e-4))
+    gm, om = _drive(pi, orc, spec, (u * 20, v * 20, w * 20), 30.0, 16, diag_tracers=("DIC", "Dust"))
+    for s in range(4):
+        assert gm._native.count_active(s) == om.count_active(s)
+    bad = compare_models(gm._native, om, 4, diag_tracers=[pi._abi.TRACER_ID["DIC"], pi._abi.TRACER_ID["Dust"]])
+    assert not bad, "\n".join(bad[:20])
+
+
+def test_parity_plain_2d_workload(pi, orc):
+    """The plain 2-D run at its full size: 128x128x1 periodic, Δ = 1 km, one
+    species of 32,768 individuals, the cellular flow U = 0.1 m/s, L = 16 km,
+    dt = 60 s, division and death on, NO3/NH4/PO4/DOC uniform plus 5 % noise.
+    The reference's diel surface PAR and temperature series
+    (param_default.jl:5-38) are staged by update! every hour; the run covers
+    four hours from 05:00 (PAR switching on at dawn, five PAR and temperature
+    slices).  240 steps, identical to the oracle."""
+    spec, (u, v, w) = pi.workloads.config_2d(overrides={"mort_reg": [1.0], "grz_P": [1.0e-6], "dvid_reg": [1.8]},
+                                             bgc={"κhP": 1.0, "κvP": 0.0, "κh": 10.0, "κv": 0.0})
+    g = spec.grid
+    n, dt = 240, 60.0
+    PARF = np.roll(pi.default_PARF(g, dt, 1440), -5, axis=2).astype(np.float32)
+    gm, om = _drive(pi, orc, spec, (u, v, w), dt, n, parf_dt=3600.0, PARF=PARF, diag_tracers=("NH4",))
+    assert gm._native.count_active(0) == om.count_active(0)
+    assert om.diag(0, "dvid").sum() > 0 and om.diag(0, "mort").sum() > 0
+    bad = compare_models(gm._native, om, 1, diag_tracers=[pi._abi.TRACER_ID["NH4"]])
     assert not bad, "\n".join(bad[:20])
