@@ -219,6 +219,15 @@ pi_status pi_batch_run(pi_batch* b, float dt, int64_t n);
  * in proportion to their own grids (default 16384).  Any value gives the same
  * results; Ensemble.autotune picks it by timing steps. */
 pi_status pi_batch_set_budget(pi_batch* b, int64_t blocks);
+/* pi_batch_run that allocates nothing: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under stream capture,
+ * serving loops that forbid hipMalloc).  Same steps and results as
+ * pi_batch_run, every step launched eagerly (no graph), with the batched
+ * launches' argument and block tables in one static device array of 32 MiB
+ * per GPU instead of device allocations.  A batch whose tables exceed it is
+ * refused with PI_ERR_NOMEM before anything runs.  Calls on one GPU share
+ * that array: calls that overlap in time on one GPU race. */
+pi_status pi_batch_run_noalloc(pi_batch* b, float dt, int64_t n);
 /* Steady steps replay a captured graph unless this is off: turn it off when
  * other threads make HIP calls (model creation, downloads) while the batch
  * runs, since a capture forbids them. */

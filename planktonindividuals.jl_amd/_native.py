@@ -65,6 +65,7 @@ def load_library():
     lib.pi_model_footprint.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
     for f, a in (("create", [C.POINTER(C.c_void_p)]), ("add", [C.c_void_p, C.c_void_p]),
                  ("remove", [C.c_void_p, C.c_void_p]), ("run", [C.c_void_p, C.c_float, C.c_int64]),
+                 ("run_noalloc", [C.c_void_p, C.c_float, C.c_int64]),
                  ("sync", [C.c_void_p]), ("size", [C.c_void_p, C.POINTER(C.c_int32)]),
                  ("graphs", [C.c_void_p, C.c_int32]), ("set_budget", [C.c_void_p, C.c_int64]),
                  ("destroy", [C.c_void_p])):
@@ -297,6 +298,11 @@ class NativeBatch:
 
     def run(self, dt, n=1):
         check(self.lib, "pi_", self.lib.pi_batch_run(self.raw, C.c_float(dt), C.c_int64(n)))
+
+    def run_noalloc(self, dt, n=1):
+        """pi_batch_run_noalloc: the same steps with the launch tables in the
+        library's static device scratch (no device allocation, no graphs)."""
+        check(self.lib, "pi_", self.lib.pi_batch_run_noalloc(self.raw, C.c_float(dt), C.c_int64(n)))
 
     def sync(self):
         check(self.lib, "pi_", self.lib.pi_batch_sync(self.raw))

@@ -19,11 +19,13 @@
 #include <vector>
 #include <algorithm>
 #include <cstddef>
+#include <mutex>
 
 #include "k_args.h"
 
 static thread_local std::string g_err;
 static pi_status fail(pi_status s, const std::string& m) { g_err = m; return s; }
+static pi_status scratch_base(int dev, char** out);
 #define HIPCHECK(x)                                                                      \
     do {                                                                                 \
         hipError_t e_ = (x);                                                             \
@@ -398,7 +400,8 @@ pi_status pi_init(int32_t device)
     HIPCHECK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(PI_ERR_ARG, "no such HIP device");
     HIPCHECK(hipSetDevice(device));
-    return PI_OK;
+    char* base = nullptr;               /* the static scratch of pi_batch_run_noalloc on this device */
+    return scratch_base(device, &base);
 }
 
 static pi_status check_desc(const pi_model_desc* d)
@@ -1129,7 +1132,35 @@ struct BSlot {
     void* dargs = nullptr;        /* member m's arguments at m * argsz */
     uint4* dbt = nullptr;         /* per block: (member, block, member's blocks, 0) */
     size_t args_cap = 0, bt_cap = 0;
+    void* largs = nullptr;        /* the tables the launches read: dargs / dbt, or a place in `scratch` */
+    uint4* lbt = nullptr;
 };
+
+/* The device array that pi_batch_run_noalloc keeps its launch tables in (no
+ * counterpart in the reference).  One copy per GPU, held from the moment the
+ * library's code object is loaded on that GPU until the process ends.  Host
+ * code never passes the array's name to HIP: its address on a device comes
+ * from hipGetSymbolAddress, once per device (scratch_base). */
+#define PI_SCRATCH_BYTES (32u << 20)
+#define PI_SCRATCH_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[PI_SCRATCH_BYTES];
+static char* g_scratch_base[PI_SCRATCH_DEVICES];
+static std::mutex g_scratch_mu;
+
+/* the address of `scratch` on device dev (made current by the caller) */
+static pi_status scratch_base(int dev, char** out)
+{
+    if (dev < 0 || dev >= PI_SCRATCH_DEVICES) return fail(PI_ERR_ARG, "device index out of range for the static scratch");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_base[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return fail(PI_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_base[dev] = (char*)p;
+    }
+    *out = g_scratch_base[dev];
+    return PI_OK;
+}
 
 struct pi_batch {
     int dev = 0;
@@ -1162,8 +1193,10 @@ static bool same_plan(const Plan& a, const Plan& b)
     return std::memcmp(a.args.data(), b.args.data(), a.args.size()) == 0;
 }
 
-/* argument and block tables of every position from the members' plans */
-static pi_status batch_tables(pi_batch* b)
+/* argument and block tables of every position from the members' plans; with
+ * sbase set, all of them are packed into [sbase, sbase + scap) (256-byte
+ * aligned pieces, one copy) instead of the batch's own allocations */
+static pi_status batch_tables(pi_batch* b, char* sbase = nullptr, size_t scap = 0)
 {
     const KInfo* K = kinfo();
     const size_t nm = b->mem.size();
@@ -1183,6 +1216,13 @@ static pi_status batch_tables(pi_batch* b)
     std::vector<uint8_t> ha;
     std::vector<uint4> hb;
     std::vector<uint32_t> nb(nm);
+    std::vector<uint8_t> stage;                       /* sbase mode: every table, packed */
+    auto put = [&](const void* src, size_t n) -> size_t {
+        const size_t off = (stage.size() + 255) & ~size_t(255);
+        stage.resize(off + n);
+        std::memcpy(stage.data() + off, src, n);
+        return off;
+    };
     for (size_t p = 0; p < npos; ++p) {
         BSlot& S = b->slots[p];
         int kid = -1;
@@ -1227,6 +1267,13 @@ static pi_status batch_tables(pi_batch* b)
             const size_t m = chains ? b->order[r] : r;
             for (uint32_t k = 0; k < nb[m]; ++k) hb.push_back(make_uint4((uint32_t)m, k, nb[m], 0u));
         }
+        if (sbase) {
+            const size_t oa = put(ha.data(), ha.size());
+            const size_t ob = put(hb.data(), hb.size() * sizeof(uint4));
+            S.largs = sbase + oa;
+            S.lbt = (uint4*)(sbase + ob);
+            continue;
+        }
         if (S.args_cap < ha.size()) {
             if (S.dargs) (void)hipFree(S.dargs);
             S.dargs = nullptr;
@@ -1242,6 +1289,18 @@ static pi_status batch_tables(pi_batch* b)
         HIPCHECK(hipMemcpyAsync(S.dargs, ha.data(), ha.size(), hipMemcpyHostToDevice, b->stream));
         HIPCHECK(hipMemcpyAsync(S.dbt, hb.data(), hb.size() * sizeof(uint4), hipMemcpyHostToDevice, b->stream));
         HIPCHECK(hipStreamSynchronize(b->stream));    /* ha / hb are reused */
+        S.largs = S.dargs;
+        S.lbt = S.dbt;
+    }
+    if (sbase) {
+        if (stage.size() > scap) {
+            char buf[200];
+            snprintf(buf, sizeof buf, "batch: the launch tables need %zu bytes, more than the %zu bytes of the static scratch",
+                     stage.size(), scap);
+            return fail(PI_ERR_NOMEM, buf);
+        }
+        HIPCHECK(hipMemcpyAsync(sbase, stage.data(), stage.size(), hipMemcpyHostToDevice, b->stream));
+        HIPCHECK(hipStreamSynchronize(b->stream));
     }
     for (size_t p = npos; p < b->slots.size(); ++p) { b->slots[p].nblk = 0; b->slots[p].kid = 0; }
     b->plans.swap(b->next);
@@ -1261,7 +1320,7 @@ static pi_status batch_launch(pi_batch* b)
             continue;
         }
         if (!S.nblk) continue;
-        void* args[2] = {(void*)&S.dargs, (void*)&S.dbt};
+        void* args[2] = {(void*)&S.largs, (void*)&S.lbt};
         HIPCHECK(hipLaunchKernel(K[S.kid].multi, dim3(S.nblk), dim3(K[S.kid].threads), args, 0,
                                  S.side ? b->side : b->stream));
     }
@@ -1370,6 +1429,50 @@ pi_status pi_batch_run(pi_batch* b, float dt, int64_t n)
         }
         HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
     }
+    return PI_OK;
+}
+
+/* pi_batch_run with its launch tables in the static device array `scratch`
+ * instead of device allocations: no hipMalloc / hipFree / graph instantiation
+ * (every step is launched eagerly).  The tables are written at every call
+ * and whenever a member's plan changes; calls of this function on one GPU
+ * share that GPU's array, so calls that overlap in time on one GPU race
+ * (serialise them).  Tables larger than PI_SCRATCH_BYTES are refused with
+ * PI_ERR_NOMEM before anything is launched. */
+pi_status pi_batch_run_noalloc(pi_batch* b, float dt, int64_t n)
+{
+    if (!b) return fail(PI_ERR_ARG, "null batch");
+    const size_t nm = b->mem.size();
+    if (nm == 0 || n <= 0) return PI_OK;
+    int cur = 0;
+    HIPCHECK(hipGetDevice(&cur));
+    if (cur != b->dev) HIPCHECK(hipSetDevice(b->dev));
+    char* base = nullptr;
+    pi_status st = scratch_base(b->dev, &base);
+    if (cur != b->dev) (void)hipSetDevice(cur);
+    if (st != PI_OK) return st;
+    batch_drop_graph(b);
+    b->next.resize(nm);
+    bool fresh = true;                /* this call has not written the scratch tables yet */
+    for (int64_t i = 0; i < n; ++i) {
+        bool same = !fresh;
+        for (size_t m = 0; m < nm; ++m) {
+            pi_model* md = b->mem[m];
+            pi_status s = rebuild_if_dirty(md);
+            if (s != PI_OK) return s;
+            if (md->h_state.err) { b->next[m].clear(); }
+            else build_step(md, dt, b->next[m]);
+            if (same && !same_plan(b->next[m], b->plans[m])) same = false;
+        }
+        if (!same) {
+            pi_status s = batch_tables(b, base, PI_SCRATCH_BYTES);
+            if (s != PI_OK) { b->tables = false; return s; }
+            fresh = false;
+        }
+        pi_status s = batch_launch(b);
+        if (s != PI_OK) { b->tables = false; return s; }
+    }
+    b->tables = false;                /* pi_batch_run rebuilds its own tables */
     return PI_OK;
 }
 
