@@ -1,0 +1,103 @@
+"""TEST INFRASTRUCTURE: checks that the oracle's pin tests can fail.
+
+Each entry below plants one plausible mistake in a copy of oracle/pi_oracle.cpp
+(a wrong index, sign, exponent, dropped term or halo plane), builds the copy
+into a temporary library, and runs the CPU pin tests against it
+(PI_ORACLE_LIB).  Every mistake must make at least one pin fail.  The
+committed source is never modified.
+
+    python oracle/mutation_check.py            # all mistakes
+    python oracle/mutation_check.py "z sign"   # one of them
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "pi_oracle.cpp")
+
+MISTAKES = [
+    ("sinking dzC offset", "/ dzC_at(k) * dt;", "/ dzC_at(k + Hz) * dt;"),
+    ("bottom flux_sink not zeroed at the last level", "planes_zero(c, 2, Nz + Hz, Hz + 1); }",
+     "planes_zero(c, 2, Nz + Hz + 1, Hz); }"),
+    ("z sign", "oz[i] = pz[i] - M::mask(ac[i], wt * w[i] * dt);", "oz[i] = pz[i] + M::mask(ac[i], wt * w[i] * dt);"),
+    ("w face offset", "FT zs = comp == 2 ? z + L(1) : z + L(0.5f);", "FT zs = z + L(0.5f);"),
+    ("PAR with dzC for dzF", "FT atten = (Chl[I(ii, jj, kk)] / vol(kk) * kc + kw) * dzF_at(kk);",
+     "FT atten = (Chl[I(ii, jj, kk)] / vol(kk) * kc + kw) * dzC_at(kk);"),
+    ("nitrification source", "*F[PI_NO3] = *F[PI_NO3] + T(PI_NH4, n) * Nit * dt;",
+     "*F[PI_NO3] = *F[PI_NO3] + T(PI_NO3, n) * Nit * dt;"),
+    ("dust scavenging dropped in PFe_inorg", "(lmin + ldust * T(PI_Dust, n)) * DFe * DFeFrac * dt +\n                               excess",
+     "(lmin) * DFe * DFeFrac * dt +\n                               excess"),
+    ("respiration sign", "p.Cq[i] = p.Cq[i] + (p.VDOC[i] - p.resp[i]) * dt;",
+     "p.Cq[i] = p.Cq[i] + (p.VDOC[i] + p.resp[i]) * dt;"),
+    ("DIC consumption sign", "plk[PI_DIC][c] += M::mask(a, (p.resp[i] - p.PS[i]) * dt);",
+     "plk[PI_DIC][c] += M::mask(a, (p.PS[i] - p.resp[i]) * dt);"),
+    ("exudation dropped", "plk[PI_DOC][c] += M::mask(a, (p.exu[i] - p.VDOC[i]) * dt);",
+     "plk[PI_DOC][c] += M::mask(a, (- p.VDOC[i]) * dt);"),
+    ("mortality with pow 2", "shape_dec(p.Sz[i], q[PI_P_mort_reg], L(1.0e-5f), 4)",
+     "shape_dec(p.Sz[i], q[PI_P_mort_reg], L(1.0e-5f), 2)"),
+    ("Sizer with pow 4", "shape_inc(Sz, q[PI_P_dvid_reg], L(2.0e-3f), 2)", "shape_inc(Sz, q[PI_P_dvid_reg], L(2.0e-3f), 4)"),
+    ("Timer hour in minutes", "FT hour = M::rem(t, L(86400)) / L(3600);", "FT hour = M::rem(t, L(86400)) / L(60);"),
+    ("all species' pop when shared_graz != 1",
+     "                std::fill(pop.begin(), pop.end(), (FT)0);\n                acc_counts(sp[s]);",
+     "                std::fill(pop.begin(), pop.end(), (FT)0);\n                for (int s2 = 0; s2 < nsp; ++s2) acc_counts(sp[s2]);"),
+    ("bounded velocity halo like a tracer halo", "else planes_bcast(c, dim, N + H + 2, H - 1, N + H + 1);",
+     "else planes_bcast(c, dim, N + H + 1, H, N + H);"),
+    ("grazing loss without Cq", "plk[PI_POC][c] += M::mask(a, (p.Bm[i] + p.Cq[i]) * (L(1) - fC)) * loss[i];",
+     "plk[PI_POC][c] += M::mask(a, (p.Bm[i]) * (L(1) - fC)) * loss[i];"),
+    ("z diffusion sign", "FT ddz = M::mask(LM(i, j, k), M::mask(LM(i, j, k + 1), fz(i, j, k) - fz(i, j, k + 1)));",
+     "FT ddz = M::mask(LM(i, j, k), M::mask(LM(i, j, k + 1), fz(i, j, k + 1) - fz(i, j, k)));"),
+    ("z flux from the wrong side", "return L(0.5f) * (T + std::fabs(T)) * (c[I(i, j, k)] + pm * d) +",
+     "return L(0.5f) * (T + std::fabs(T)) * (c[I(i, j, k - 1)] + pm * d) +"),
+    ("x flux from the wrong side", "return L(0.5f) * (T + std::fabs(T)) * (c[I(i - 1, j, k)] + pp * d) +",
+     "return L(0.5f) * (T + std::fabs(T)) * (c[I(i, j, k)] + pp * d) +"),
+    ("division: daughter takes the first free slot", "p.idx[i] = free_ind[(size_t)(idc_int[i] - 1)];",
+     "p.idx[i] = free_ind[0];"),
+    ("PS without the Chl:C ratio", "M::exp(-aI / M::max(e30, PCm) * p.Chl[i] / M::max(e30, p.Bm[i]))",
+     "M::exp(-aI / M::max(e30, PCm) / M::max(e30, p.Bm[i]))"),
+    ("NO3 uptake with the NH4 half-saturation", "tNO3[i] / M::max(e30, tNO3[i] + q[PI_P_KsatNO3])",
+     "tNO3[i] / M::max(e30, tNO3[i] + q[PI_P_KsatNH4])"),
+    ("rhoChl light gate at 1", "r = M::mask(M::isless(L(1.0e-1f), tpar[i]), r);", "r = M::mask(M::isless(L(1.0f), tpar[i]), r);"),
+    ("DOC uptake gate at 10 %", "M::isless(L(1.0e-2f), p.PS[i] / (p.VDOC[i] + p.PS[i]))",
+     "M::isless(L(1.0e-1f), p.PS[i] / (p.VDOC[i] + p.PS[i]))"),
+    ("tempFunc_PS shape constant", "FT regT = shape_dec(x, xmax, L(4.0e-2f), 4);", "FT regT = shape_dec(x, xmax, L(4.0e-3f), 4);"),
+    ("gather NO3 from NH4", "tNO3[i] = M::mask(a, M::max(z0, tr[PI_NO3][c]));", "tNO3[i] = M::mask(a, M::max(z0, tr[PI_NH4][c]));"),
+    ("uptake cap without pop", "FT pp = M::max(one, tpop[i]);", "FT pp = one;"),
+    ("Chl synthesis without R_NC", "p.Chl[i] = p.Chl[i] + dt * p.BS[i] * q[PI_P_R_NC] * p.rho[i];",
+     "p.Chl[i] = p.Chl[i] + dt * p.BS[i] * p.rho[i];"),
+    ("find_inds without the +1", "p.xi[i] = (int64_t)M::mask(p.ac[i], p.x[i] + L(1)) + Hx;",
+     "p.xi[i] = (int64_t)M::mask(p.ac[i], p.x[i]) + Hx;"),
+]
+
+
+def main(only):
+    orig = open(SRC).read()
+    caught, missed = [], []
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, a, b in MISTAKES:
+            if only and name not in only:
+                continue
+            if orig.count(a) != 1:
+                raise SystemExit(f"{name}: the text to change is not unique in pi_oracle.cpp")
+            d = os.path.join(tmp, "oracle")
+            os.makedirs(d, exist_ok=True)
+            src = os.path.join(d, "pi_oracle.cpp")
+            open(src, "w").write(orig.replace(a, b))
+            lib = os.path.join(tmp, "liboracle_mut.so")
+            subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+                            "-fno-finite-math-only", "-fPIC", "-shared", "-iquote", HERE, "-o", lib, src], check=True)
+            env = dict(os.environ, PI_ORACLE_LIB=lib)
+            r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-m", "not gpu", "-p", "no:cacheprovider",
+                                "tests/test_oracle_pins.py", "tests/test_oracle_pins_physics.py"],
+                               cwd=REPO, env=env, capture_output=True, text=True)
+            first = next((l for l in r.stdout.splitlines() if l.startswith("FAILED")), "")
+            (caught if r.returncode != 0 else missed).append(name)
+            print(f"{'caught' if r.returncode else 'MISSED'}: {name}  {first}", flush=True)
+    print(f"{len(caught)} caught, {len(missed)} missed", flush=True)
+    return 1 if missed else 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

@@ -13,7 +13,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
-LIB = os.path.join(HERE, "liboracle_pi.so")
+# PI_ORACLE_LIB: another build of the restatement (oracle/mutation_check.py
+# points the pin tests at copies with planted mistakes)
+LIB = os.environ.get("PI_ORACLE_LIB") or os.path.join(HERE, "liboracle_pi.so")
 
 
 def _pkg():
@@ -51,6 +53,8 @@ def load():
         lib.orc_download_tracer_f64.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
         lib.orc_fill_halo.restype = C.c_int
         lib.orc_fill_halo.argtypes = [C.c_int32] * 9 + [C.POINTER(C.c_float)]
+        lib.orc_fill_halo_op.restype = C.c_int
+        lib.orc_fill_halo_op.argtypes = [C.c_int32] * 9 + [C.POINTER(C.c_float)]
         lib.orc_expf.restype = C.c_float
         lib.orc_expf.argtypes = [C.c_float]
         lib.orc_exp_f64.restype = C.c_double

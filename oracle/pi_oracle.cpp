@@ -259,6 +259,10 @@ struct Oracle : orc_model {
         if (Td(dim) == PI_PERIODIC) fill_hi(c, dim);
         else planes_zero(c, dim, N + H + 1, H);
     }
+    /* fill_halo_top_flux_sink! / fill_halo_bottom_flux_sink! (halo_regions.jl:34-35):
+     * zero planes 1:H and N+H:N+2H (the last interior level included) */
+    template <typename T> void fill_top_flux_sink(T* c) { planes_zero(c, 2, 1, Hz); }
+    template <typename T> void fill_bottom_flux_sink(T* c) { planes_zero(c, 2, Nz + Hz, Hz + 1); }
     template <typename T> void fill_halo_generic(T* c, int tx_, int ty_) {
         (void)tx_; (void)ty_;
         fill_lo(c, 0); fill_hi(c, 0); fill_lo(c, 1); fill_hi(c, 1); fill_lo(c, 2); fill_hi(c, 2);
@@ -767,8 +771,8 @@ struct Oracle : orc_model {
             int q = names[a];
             FT ws = a < 2 ? B[PI_B_w_sink_inorg] : B[PI_B_w_sink_org];
             for (int64_t n = 0; n < ncell; ++n) flux_sink[n] = ws * tr[q][n];
-            planes_zero(flux_sink.data(), 2, 1, Hz);            /* fill_halo_top_flux_sink! */
-            planes_zero(flux_sink.data(), 2, Nz + Hz, Hz + 1);  /* fill_halo_bottom_flux_sink!: N+H:N+2H */
+            fill_top_flux_sink(flux_sink.data());
+            fill_bottom_flux_sink(flux_sink.data());
             for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
                 int ii = i + Hx, jj = j + Hy, kk = k + Hz;
                 FT& G = Gcs[q][I(ii, jj, kk)];
@@ -1049,6 +1053,24 @@ pi_status orc_fill_halo(int32_t Nx, int32_t Ny, int32_t Nz, int32_t Hx, int32_t 
     o.Px = Nx + 2 * Hx; o.Py = Ny + 2 * Hy; o.Pz = Nz + 2 * Hz;
     o.tx = tx; o.ty = ty; o.tz = tz;
     o.fill_halo_generic(c, tx, ty);
+    return PI_OK;
+}
+pi_status orc_fill_halo_op(int32_t op, int32_t dim, int32_t topo, int32_t Nx, int32_t Ny, int32_t Nz,
+                           int32_t Hx, int32_t Hy, int32_t Hz, float* c)
+{
+    if (dim < 0 || dim > 2 || op < 0 || op > 5) return fail(PI_ERR_ARG, "halo op");
+    Oracle<float> o;
+    o.Nx = Nx; o.Ny = Ny; o.Nz = Nz; o.Hx = Hx; o.Hy = Hy; o.Hz = Hz;
+    o.Px = Nx + 2 * Hx; o.Py = Ny + 2 * Hy; o.Pz = Nz + 2 * Hz;
+    o.tx = o.ty = o.tz = topo;
+    switch (op) {
+        case 0: o.fill_lo(c, dim); break;
+        case 1: o.fill_hi(c, dim); break;
+        case 2: o.fill_hi_vel(c, dim); break;
+        case 3: o.fill_hi_Gc(c, dim); break;
+        case 4: o.fill_top_flux_sink(c); break;
+        default: o.fill_bottom_flux_sink(c); break;
+    }
     return PI_OK;
 }
 float orc_expf(float x) { return pi_expf(x); }

@@ -45,6 +45,11 @@ pi_status orc_count_active(orc_model* m, int32_t sp, int64_t* n);
 /* Building blocks exposed for the oracle's own pin tests. */
 pi_status orc_fill_halo(int32_t Nx, int32_t Ny, int32_t Nz, int32_t Hx, int32_t Hy, int32_t Hz,
                         int32_t tx, int32_t ty, int32_t tz, float* c);
+/* one halo statement of halo_regions.jl on a padded float field: op 0 = fill_halo_west!/
+ * south!/top! (dim 0/1/2), 1 = east!/north!/bottom!, 2 = *_vel!, 3 = *_Gc!,
+ * 4 = fill_halo_top_flux_sink!, 5 = fill_halo_bottom_flux_sink!; topo = PI_PERIODIC/PI_BOUNDED */
+pi_status orc_fill_halo_op(int32_t op, int32_t dim, int32_t topo, int32_t Nx, int32_t Ny, int32_t Nz,
+                           int32_t Hx, int32_t Hy, int32_t Hz, float* c);
 float orc_expf(float x);
 double orc_exp_f64(double x);
 void orc_log_f64_v(const double* x, double* y, int64_t n);
