@@ -123,15 +123,30 @@ def measured_traffic(logical):
     return tot if seen else None
 
 
-def rocpd_summary(db, title, cmd):
-    """Markdown table of a rocprofv3 kernel trace (rocpd SQLite output)."""
+def rocpd_summary(db, title, cmd, skip_steps=0):
+    """Markdown table of a rocprofv3 kernel trace (rocpd SQLite output).  With
+    skip_steps > 0 only the dispatches that start after the end of the
+    skip_steps-th step (a step ends with its k_finish) are counted: model
+    creation, the first step after an upload (which sorts every individual from
+    slot order) and the warm-up are left out, so the averages are those of
+    steady steps."""
     import sqlite3
     c = sqlite3.connect(db)
+    cols = [r[1] for r in c.execute("pragma table_info(kernels)")]
+    t0c = next(k for k in ("start", "start_ts", "begin") if k in cols)
+    t1c = next(k for k in ("end", "end_ts", "stop") if k in cols)
+    where = ""
+    if skip_steps > 0:
+        fin = [r[0] for r in c.execute(f'select "{t1c}" from kernels where name like "%k_finish%" order by "{t0c}"')]
+        if len(fin) > skip_steps:
+            where = f'where "{t0c}" > {fin[skip_steps - 1]}'
     rows = c.execute("select name, count(*), sum(duration), avg(duration), min(duration), max(duration) "
-                     "from kernels group by name order by sum(duration) desc").fetchall()
+                     f"from kernels {where} group by name order by sum(duration) desc").fetchall()
     tot = sum(r[2] for r in rows)
+    steady = (f"Steady steps only: dispatches after the end of step {skip_steps} (model creation, the first step "
+              "after the upload, which sorts every individual, and the warm-up are not counted).") if where else ""
     out = [f"# {title}", "", "Command (from the repository root, one MI355X, ROCm 7.2):", "", "```", cmd, "```", "",
-           "Kernel trace statistics (durations in microseconds):", "",
+           steady, "", "Kernel trace statistics (durations in microseconds):", "",
            "| kernel | calls | total us | avg us | min us | max us | % |", "|---|---|---|---|---|---|---|"]
     for n, k, sm, a, mn, mx in rows:
         out.append(f"| {n.split('(')[0]} | {k} | {sm / 1e3:.1f} | {a / 1e3:.1f} | {mn / 1e3:.1f} | {mx / 1e3:.1f} | "
@@ -367,14 +382,16 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: a plain device copy of 256 MiB (k_probe_copy, known traffic, to "
                          "scale the counters), then --warmup + --steps steps without timing; prints no JSON")
-    ap.add_argument("--rocpd-summary", nargs=3, metavar=("DB", "TITLE", "CMD"),
-                    help="print the markdown kernel table of a rocprofv3 kernel-trace database and exit")
+    ap.add_argument("--rocpd-summary", nargs="+", metavar="ARG",
+                    help="DB TITLE CMD [SKIP_STEPS]: print the markdown kernel table of a rocprofv3 kernel-trace "
+                         "database (steady steps only with SKIP_STEPS) and exit")
     ap.add_argument("--pmc-summary", nargs="+", metavar="DB",
                     help="FETCH_DB WRITE_DB OUT_JSON [VALU_DB]: write the per-step FETCH/WRITE (and SQ_INSTS_VALU) "
                          "summary of --pmc-probe runs and exit")
     args = ap.parse_args()
     if args.rocpd_summary:
-        print(rocpd_summary(*args.rocpd_summary))
+        a = args.rocpd_summary
+        print(rocpd_summary(a[0], a[1], a[2], int(a[3]) if len(a) > 3 else 0))
         return
     if args.pmc_summary:
         res = pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.steps,

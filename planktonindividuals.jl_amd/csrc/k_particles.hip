@@ -1118,13 +1118,16 @@ __device__ __forceinline__ void plk_cons(PlkSums& S, float c0, float c1, float c
 {
     S.DIC = S.DIC + c0; S.DOC = S.DOC + c1; S.NH4 = S.NH4 + c2; S.NO3 = S.NO3 + c3; S.PO4 = S.PO4 + c4;
 }
+/* the per-species diagnostics of a cell as atomic additions without return:
+ * no load for a later store to wait on, and the counts are integers, so the
+ * sums are exact in any order */
 __device__ __forceinline__ void plk_diag(const PlkArgs& a, long long c, uint32_t sp, PlkSums& S)
 {
     float* d = a.diag + (long long)sp * 4 * a.g.ncell;
-    if (S.na) d[0 * a.g.ncell + c] = d[0 * a.g.ncell + c] + (float)S.na;
-    if (S.ng) d[1 * a.g.ncell + c] = d[1 * a.g.ncell + c] + (float)S.ng;
-    if (S.nm) d[2 * a.g.ncell + c] = d[2 * a.g.ncell + c] + (float)S.nm;
-    if (S.nd) d[3 * a.g.ncell + c] = d[3 * a.g.ncell + c] + (float)S.nd;
+    if (S.na) atomicAdd(&d[0 * a.g.ncell + c], (float)S.na);
+    if (S.ng) atomicAdd(&d[1 * a.g.ncell + c], (float)S.ng);
+    if (S.nm) atomicAdd(&d[2 * a.g.ncell + c], (float)S.nm);
+    if (S.nd) atomicAdd(&d[3 * a.g.ncell + c], (float)S.nd);
     S.ng = S.nm = S.nd = S.na = 0;
 }
 

@@ -805,9 +805,9 @@ static void build_step(pi_model* m, float dt, Plan& P)
         for (int q = 0; q < PI_NTRACERS; ++q) ka.plk[q] = m->plk[q] ? m->plk[q] : m->plk[PI_DIC];
         ka.lng = m->lng; ka.lg = m->skey; ka.lm = m->ssrc;
         ka.segp = m->segp;                 /* k_plk copies seg -> segp (the binning above has read segp) */
-        /* one resident round: five 256-thread blocks per CU fit k_plk's registers,
-         * and a partial last round of blocks measured ~10% slower */
-        P.add(K_PLK, "plk", std::min<long long>(cb, 5ll * m->ncu), ka);
+        /* one cell per thread: with a loop over cells each iteration's loads
+         * would wait for the previous iteration's stores (one counter for both) */
+        P.add(K_PLK, "plk", grid_for(m->ncell, 256, 1 << 22), ka);
         P.add(K_PLK_LONG, "plk", m->long_blocks, ka);
     }
     P.mark(K_JOIN);
