@@ -200,32 +200,58 @@ def pmc_summary(fetch_db, write_db, steps, valu_db=None):
     return {"scale": scale, "steps_averaged": steps, "kernels": kernels}
 
 
-def cpu_baseline(pi, seconds_hint=12.0):
-    """The oracle (FP32 reference order, one thread) on a bounded sample of the same
+def cpu_threads():
+    """Host threads for the all-cores CPU baseline: the CPU share of this process
+    (affinity), capped by OMP_NUM_THREADS when it is set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_baseline(pi, seconds_hint=10.0):
+    """The oracle (FP32 reference order) on a bounded sample of the same
     workload: the 128x128x64 quarter of the domain at the same density
-    (2 x 1,048,576 individuals); one untimed step (every individual sorted from
-    scratch), then as many timed steps as fit in about `seconds_hint` s."""
+    (2 x 1,048,576 individuals), one untimed step (every individual sorted from
+    scratch), then as many timed steps as fit in about `seconds_hint` s; first
+    on one thread, then on every host core of this process (the element-wise
+    loops on OpenMP threads; the ordered per-cell sums stay serial, so the
+    results are the same bits)."""
     sys.path.insert(0, REPO)
     from oracle import orc
+    lib = orc.load()
     spec, uvw = pi.workloads.config_3d(n=128, nz=64, N=1048576, nsp=2, cap=2097152, seed=2)
     g = spec.grid
-    om = orc.OracleModel(spec)
     parf = np.full((g.Nx, g.Ny), 1823.7921666666664, np.float32)
     temp = pi.default_temperature(g, 60.0, 1)[..., 12].astype(np.float32)
-    t0 = time.perf_counter()
-    om.update(60.0, 1, vel=uvw, parf=parf, temp=temp)
-    first = time.perf_counter() - t0
-    k = max(1, min(8, int(round(seconds_hint / max(first, 1e-3)))))
-    n0 = om.count_active(0) + om.count_active(1)
-    t0 = time.perf_counter()
-    om.update(60.0, k, vel=uvw, parf=parf, temp=temp)
-    dt = time.perf_counter() - t0
-    n1 = om.count_active(0) + om.count_active(1)
-    return {"value": 0.5 * (n0 + n1) * k / dt, "unit": "individual-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (CPU restatement of the reference CPU() step, FP32, 1 thread): {k} steps of the "
-                      "128x128x64 quarter domain with 2 x 1,048,576 individuals (same density and forcing), "
-                      "after one untimed step",
-            "seconds": dt, "tracer_cell_steps_per_s": g.Nx * g.Ny * g.Nz * k / dt}
+    res = {}
+    for nt in (1, cpu_threads()):
+        lib.orc_set_threads(nt)
+        om = orc.OracleModel(spec)
+        t0 = time.perf_counter()
+        om.update(60.0, 1, vel=uvw, parf=parf, temp=temp)
+        first = time.perf_counter() - t0
+        k = max(1, min(8, int(round(seconds_hint / max(first, 1e-3)))))
+        n0 = om.count_active(0) + om.count_active(1)
+        t0 = time.perf_counter()
+        om.update(60.0, k, vel=uvw, parf=parf, temp=temp)
+        dt = time.perf_counter() - t0
+        n1 = om.count_active(0) + om.count_active(1)
+        res[nt] = {"value": 0.5 * (n0 + n1) * k / dt, "steps": k, "seconds": dt,
+                   "tracer_cell_steps_per_s": g.Nx * g.Ny * g.Nz * k / dt}
+        del om
+        log(f"[bench] CPU baseline, {nt} thread(s): {res[nt]['value']:.4g} individual-steps/s ({k} steps, {dt:.1f} s)")
+    lib.orc_set_threads(1)
+    nt = max(res)
+    one = res[1]
+    return {"value": res[nt]["value"], "unit": "individual-steps/s", "cores": nt, "kind": "port",
+            "sample": f"oracle (CPU restatement of the reference CPU() step, FP32, {nt} OpenMP threads): "
+                      f"{res[nt]['steps']} steps of the 128x128x64 quarter domain with 2 x 1,048,576 individuals "
+                      "(same density and forcing), after one untimed step",
+            "seconds": res[nt]["seconds"], "tracer_cell_steps_per_s": res[nt]["tracer_cell_steps_per_s"],
+            "single_thread": {"value": one["value"], "cores": 1, "steps": one["steps"], "seconds": one["seconds"],
+                              "tracer_cell_steps_per_s": one["tracer_cell_steps_per_s"]}}
 
 
 def ensemble_leg(pi, members=256, steps=20, seq_steps=4, seed=4, batched_only=False):
@@ -262,13 +288,12 @@ def ensemble_leg(pi, members=256, steps=20, seq_steps=4, seed=4, batched_only=Fa
     batch.sync()
     tuned = batch.autotune(60.0)
     log("[bench] ensemble: block budget " + ", ".join(f"{b}: {ms:.2f} ms" for b, ms in tuned.items()))
-    live0 = sum(nat.count_active(0) for nat in nats)
+    c0 = sum(nat.individual_steps() for nat in nats)
     t0 = time.perf_counter()
     batch.run(60.0, steps)
     batch.sync()
     t_b = (time.perf_counter() - t0) / steps
-    live1 = sum(nat.count_active(0) for nat in nats)
-    live = 0.5 * (live0 + live1)
+    live = (sum(nat.individual_steps() for nat in nats) - c0) / steps   # individuals stepped per ensemble step
     batch.close()
     for nat in nats:
         nat.close()
@@ -286,12 +311,35 @@ def ensemble_leg(pi, members=256, steps=20, seq_steps=4, seed=4, batched_only=Fa
             "block_budget": min(tuned, key=tuned.get)}
 
 
-def serving_leg(pi, rate=50.0, window=6.0, cap_s=150.0, budget_gib=200, seed=5, n_check=3):
+def _oracle_member(pi, entry, steps, dt=60.0):
+    """A served member re-run by the oracle with update!'s staging (the
+    reference's diel PAR and temperature slices, the steady velocity)."""
+    sys.path.insert(0, REPO)
+    from oracle import orc
+    spec, vel = pi.workloads.ensemble_member(entry, host_state=True, cap_factor=4)
+    g = spec.grid
+    PARF = pi.default_PARF(g, dt, steps).astype(np.float32)
+    temp = pi.default_temperature(g, dt, steps).astype(np.float32)
+    idx = lambda t, s: int(np.floor(np.float32(t) / np.float32(s)))
+    om = orc.OracleModel(spec)
+    om.update(dt, steps, vel=vel, parf_fn=lambda t: PARF[:, :, idx(t, 3600.0)],
+              temp_fn=lambda t: temp[:, :, :, idx(t, 3600.0)])
+    return om
+
+
+SERVE_FIELDS = ("x", "y", "z", "Sz", "Bm", "Cq", "Nq", "Pq", "Chl", "gen", "age", "ac")
+
+
+def serving_leg(pi, rate=50.0, window=6.0, cap_s=150.0, budget_gib=200, seed=5, n_oracle=4, n_alone=8):
     """Served ensemble stream: requests from the ensemble table (lengths
     100-2000 steps) arrive as a Poisson process at `rate`/s for `window` s,
     are admitted by device footprint within `budget_gib` and stepped in one
-    running batch; per-step and end-to-end latency, throughput, and the
-    `n_check` smallest finished members re-run alone and compared bit for bit."""
+    running batch; per-step and end-to-end latency and throughput (live
+    individuals counted on the device at every step).  Validation of finished
+    members: the `n_oracle` smallest (individuals x steps) against the oracle
+    (the CPU restatement of the reference step) and `n_alone` more, spread over
+    the sizes and including the largest, against the same member run alone on
+    the GPU; bit for bit."""
     from importlib import import_module
     sv = import_module(pi.__name__ + ".serving")
     table = pi.workloads.ensemble_table(4096, seed=seed, lengths=(100, 2000))
@@ -299,14 +347,21 @@ def serving_leg(pi, rate=50.0, window=6.0, cap_s=150.0, budget_gib=200, seed=5, 
     arr = np.cumsum(rng.exponential(1.0 / rate, 4096))
     arr = arr[arr < window]
     make = lambda e: (lambda: pi.workloads.ensemble_member(e, cap_factor=4))
-    check = sorted(range(len(arr)), key=lambda i: table[i]["N"] * table[i]["steps"])[:n_check]
+    order = sorted(range(len(arr)), key=lambda i: table[i]["N"] * table[i]["steps"])
+    check_o = order[:n_oracle]
+    rest = order[n_oracle:]
+    by_n = sorted(rest, key=lambda i: table[i]["N"])
+    pick = {by_n[-1]}
+    for q in np.linspace(0, len(by_n) - 1, n_alone - 1):
+        pick.add(by_n[int(q)])
+    check_a = sorted(pick)[:n_alone]
+    check = set(check_o) | set(check_a)
     states = {}
 
     def grab(req, sim):
         if req.rid in check:
             nat = sim.model._native
-            states[req.rid] = ([nat.field(0, f) for f in ("x", "Bm", "Cq", "Nq", "Pq", "Chl", "ac")],
-                               [nat.tracer(q) for q in range(14)])
+            states[req.rid] = ([nat.field(0, f) for f in SERVE_FIELDS], [nat.tracer(q) for q in range(14)])
 
     srv = sv.EnsembleServer(budget=budget_gib * 2 ** 30, workers=8, on_finish=grab)
     t0 = time.perf_counter()
@@ -335,29 +390,42 @@ def serving_leg(pi, rate=50.0, window=6.0, cap_s=150.0, budget_gib=200, seed=5, 
         st = srv.stats()
         failed = [r for r in srv.finished if r.error]
         srv.close()
-    same = 0
-    for i in check:
-        if i not in states:
-            continue
-        spec, vel = pi.workloads.ensemble_member(table[i], cap_factor=4)
-        sim = sv.simulation_for(spec, vel, table[i]["steps"])
-        pi.update(sim)
-        nat = sim.model._native
-        f, t = states[i]
-        ok = all(np.array_equal(nat.field(0, n), a) for n, a in zip(("x", "Bm", "Cq", "Nq", "Pq", "Chl", "ac"), f))
-        ok = ok and all(np.array_equal(nat.tracer(q), t[q]) for q in range(14))
-        same += int(ok)
-        nat.close()
+
+    def same(f, t, ref):
+        ok = all(np.array_equal(ref.field(0, n), a) for n, a in zip(SERVE_FIELDS, f))
+        return ok and all(np.array_equal(ref.tracer(q), t[q]) for q in range(14))
+
+    ok_o = ok_a = 0
+    for i in check_o:
+        if i in states:
+            om = _oracle_member(pi, table[i], table[i]["steps"])
+            ok_o += int(same(*states[i], om))
+            del om
+    for i in check_a:
+        if i in states:
+            spec, vel = pi.workloads.ensemble_member(table[i], cap_factor=4)
+            sim = sv.simulation_for(spec, vel, table[i]["steps"])
+            pi.update(sim)
+            ok_a += int(same(*states[i], sim.model._native))
+            sim.model._native.close()
+    n_o = sum(1 for i in check_o if i in states)
+    n_a = sum(1 for i in check_a if i in states)
     log(f"[bench] serving: {k} requests, {st['finished']} finished in {wall:.1f} s, step p50 {st['p50_step_ms']:.2f} ms "
-        f"p99 {st['p99_step_ms']:.2f} ms; {same}/{len(states)} re-run alone identical")
+        f"p99 {st['p99_step_ms']:.2f} ms; {ok_o}/{n_o} identical to the oracle, {ok_a}/{n_a} identical alone")
     return {"workload": f"Poisson arrivals {rate:g}/s for {window:g} s from the ensemble member table (seed {seed}), "
                         f"lengths U[100,2000] steps, capacity 4 N, dt 60 s, admission by device footprint within "
                         f"{budget_gib} GiB",
             "requests": k, "finished": st["finished"], "failed": len(failed), "wall_s": wall,
             "p50_step_ms": st["p50_step_ms"], "p99_step_ms": st["p99_step_ms"], "mean_step_ms": st["mean_step_ms"],
             "steps": st["steps"], "p50_end_to_end_s": st["p50_e2e_s"], "p99_end_to_end_s": st["p99_e2e_s"],
-            "member_steps": st["member_steps"], "individual_steps_per_s": st["individual_steps"] / wall,
-            "checked_alone": len(states), "identical_alone": same}
+            "member_steps": st["member_steps"], "individual_steps": st["individual_steps"],
+            "individual_steps_per_s": st["individual_steps"] / wall,
+            "validated": {"against_oracle": n_o, "identical_to_oracle": ok_o,
+                          "oracle_members": [dict(N=table[i]["N"], steps=table[i]["steps"], size=table[i]["size"])
+                                             for i in check_o],
+                          "against_alone": n_a, "identical_alone": ok_a,
+                          "alone_members": [dict(N=table[i]["N"], steps=table[i]["steps"], size=table[i]["size"])
+                                            for i in check_a]}}
 
 
 def main():
@@ -370,6 +438,11 @@ def main():
     ap.add_argument("--individuals", type=int, default=4194304, help="per species at start")
     ap.add_argument("--species", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-large", action="store_true", help="skip the large single-instance leg")
+    ap.add_argument("--workload", choices=("plain", "large"), default="plain",
+                    help="large: only the large single instance (for profiling; --steps/--warmup apply to it)")
+    ap.add_argument("--full-steps", type=int, default=500,
+                    help="after the timed and per-kernel steps, run the plain workload to this many steps")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain 3-D run (for profiling the other legs); prints their results only")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
@@ -411,6 +484,13 @@ def main():
                "serving": None if args.no_serve else serving_leg(pi, rate=args.serve_rate, window=args.serve_window)}
         print(json.dumps(out), flush=True)
         return
+    if args.workload == "large":
+        if args.pmc_probe:
+            large_leg(pi, steps=args.steps, warmup=args.warmup, probe=True)
+            return
+        print(json.dumps({"large_instance": large_leg(pi, steps=args.steps, warmup=args.warmup)}), flush=True)
+        return
+
     t0 = time.perf_counter()
     spec, (u, v, w) = pi.workloads.config_3d(n=args.n, nz=args.nz, N=args.individuals, nsp=args.species,
                                              cap=2 * args.individuals, seed=2)
@@ -427,9 +507,7 @@ def main():
     log(f"[bench] model resident on the GPU after {time.perf_counter() - t0:.1f} s")
 
     if args.pmc_probe:
-        lib = pi._native.load_library()
-        pi._native.check(lib, "pi_", lib.pi_probe_copy(1 << 28))      # k_probe_copy: 256 MiB read + 256 MiB written
-        log("[bench] copy probe: 268435456 bytes read, 268435456 bytes written")
+        probe_copy(pi)
         nat.run(dt, args.warmup + args.steps)
         nat.sync()
         log(f"[bench] {args.warmup + args.steps} steps done")
@@ -437,11 +515,19 @@ def main():
     nat.run(dt, args.warmup)
     nat.sync()
     live0 = sum(nat.count_active(s) for s in range(args.species))
+    c0 = nat.individual_steps()
     t1 = time.perf_counter()
     nat.run(dt, args.steps)                # steady steps replay the captured step graph
     nat.sync()
     t2 = time.perf_counter()
+    c1 = nat.individual_steps()
     live1 = sum(nat.count_active(s) for s in range(args.species))
+    wall = t2 - t1
+    ms_step = 1e3 * wall / args.steps
+    live = (c1 - c0) / args.steps          # individuals stepped per step, counted on the device
+    ind_steps = (c1 - c0) / wall
+    cell_steps = g.Nx * g.Ny * g.Nz * args.steps / wall
+    log(f"[bench] {args.steps} steps in {wall:.3f} s ({ms_step:.3f} ms/step), live {live0} -> {live1}")
     # per-kernel breakdown: HIP events around each launch group, eager launches
     nkt = max(5, min(args.steps, 20))
     nat.enable_timing(True)
@@ -449,22 +535,27 @@ def main():
     nat.sync()
     kt = nat.kernel_times()
     nat.enable_timing(False)
-    wall = t2 - t1
-    ms_step = 1e3 * wall / args.steps
-    live = 0.5 * (live0 + live1)
-    ind_steps = live * args.steps / wall
-    cell_steps = g.Nx * g.Ny * g.Nz * args.steps / wall
-    log(f"[bench] {args.steps} steps in {wall:.3f} s ({ms_step:.3f} ms/step), live {live0} -> {live1}")
+    # the rest of the workload's 500 steps (the population keeps evolving)
+    done = args.warmup + args.steps + nkt
+    full = None
+    if args.full_steps > done:
+        c2 = nat.individual_steps()
+        t3 = time.perf_counter()
+        nat.run(dt, args.full_steps - done)
+        nat.sync()
+        t4 = time.perf_counter()
+        c3 = nat.individual_steps()
+        nrest = args.full_steps - done
+        full = {"steps": args.full_steps, "timed_steps": nrest, "ms_per_step": 1e3 * (t4 - t3) / nrest,
+                "individual_steps_per_s": (c3 - c2) / (t4 - t3),
+                "live_at_end": sum(nat.count_active(s) for s in range(args.species))}
+        log(f"[bench] steps {done + 1}-{args.full_steps}: {full['ms_per_step']:.3f} ms/step, "
+            f"live at the end {full['live_at_end']}")
 
     n_cells = g.Nx * g.Ny * g.Nz
     n_slots = args.species * 2 * args.individuals
     kernels = {name: {"ms_per_launch": ms / max(cnt, 1), "launches": cnt} for name, (ms, cnt) in kt.items()}
-    logical = {}
-    for lk, L in LOGICAL.items():
-        ms = sum(kt[gname][0] for gname in L["groups"] if gname in kt) / nkt     # per step
-        b = logical_bytes(lk, live, n_cells, n_slots)
-        logical[lk] = {"ms_per_step": ms, "bytes_per_step": b, "gbps_algorithmic": b / (ms * 1e-3) / 1e9 if ms else None,
-                       "launch_groups": L["groups"]}
+    logical = logical_table(kt, nkt, live, n_cells, n_slots)
     dom = max(logical, key=lambda k: logical[k]["ms_per_step"])
     ach = logical[dom]["gbps_algorithmic"] * 1e9
     traffic = measured_traffic(dom)
@@ -482,6 +573,9 @@ def main():
     step_roof = {"bytes_per_step": bstep, "achieved": bstep / (ms_step * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
                  "unit": "GB/s", "frac": bstep / (ms_step * 1e-3) / HBM_PEAK}
     del model, nat
+    large = None
+    if not args.no_large:
+        large = large_leg(pi)
     ens = None
     if not args.no_ensemble:
         ens = ensemble_leg(pi, members=args.ens_members)
@@ -492,7 +586,6 @@ def main():
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle) ...")
         base = cpu_baseline(pi)
-        log(f"[bench] CPU baseline {base['value']:.4g} individual-steps/s in {base['seconds']:.1f} s")
 
     out = {
         "metric": "individual-steps/s & tracer cell-steps/s (1 GPU); ensemble p50/p99 step latency",
@@ -506,7 +599,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: seeded Fourier velocity (64 modes, k^-5/3, rms 0.05 m/s); initial state from the shared stream",
+        "data": "synthetic: seeded Fourier velocity (64 modes, k^-5/3, rms 0.05 m/s); tracers uniform plus 5 % noise; "
+                "initial state from the shared stream",
         "config": {"workload": f"plain 3-D run {args.n}x{args.n}x{args.nz}, {args.species} species x "
                                f"{args.individuals} individuals (capacity {2 * args.individuals}), dt 60 s",
                    "grid": [args.n, args.n, args.nz], "species": args.species,
@@ -516,13 +610,90 @@ def main():
         "roofline": roof,
         "issue_roofline": issue,
         "step_roofline": step_roof,
+        "full_run": full,
         "cpu_baseline": base,
+        "large_instance": large,
         "ensemble": ens,
         "serving": serve,
         "logical_kernels": logical,
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)
+
+
+def probe_copy(pi):
+    lib = pi._native.load_library()
+    pi._native.check(lib, "pi_", lib.pi_probe_copy(1 << 28))      # k_probe_copy: 256 MiB read + 256 MiB written
+    log("[bench] copy probe: 268435456 bytes read, 268435456 bytes written")
+
+
+def logical_table(kt, nkt, live, n_cells, n_slots):
+    """Per logical kernel: ms per step (HIP events around its launch groups),
+    algorithmic bytes per step and the algorithmic GB/s."""
+    logical = {}
+    for lk, L in LOGICAL.items():
+        ms = sum(kt[gname][0] for gname in L["groups"] if gname in kt) / nkt     # per step
+        b = logical_bytes(lk, live, n_cells, n_slots)
+        logical[lk] = {"ms_per_step": ms, "bytes_per_step": b, "gbps_algorithmic": b / (ms * 1e-3) / 1e9 if ms else None,
+                       "launch_groups": L["groups"]}
+    return logical
+
+
+def large_leg(pi, steps=200, warmup=2, probe=False):
+    """The large single instance: 512 x 512 x 128 (periodic x/y, bounded z),
+    4 species x 8,388,608 individuals (capacity 2 x that), a 128-mode Fourier
+    flow, dt = 30 s, 200 steps; initial state drawn on the device.  Steps per
+    second, individual- and cell-steps/s, and per logical kernel the time (HIP
+    events, eager launches) and algorithmic GB/s."""
+    N, nsp = 8388608, 4
+    t0 = time.perf_counter()
+    spec, (u, v, w) = pi.workloads.config_3d(n=512, nz=128, N=N, nsp=nsp, cap=2 * N, seed=3, nmodes=128,
+                                             host_state=False)
+    g = spec.grid
+    model = pi.PlanktonModel(pi.HIPArch(0), spec=spec)
+    nat = model._native
+    dt = 30.0
+    nat.set_velocity(u, v, w)
+    nat.set_parf(np.full((g.Nx, g.Ny), 1823.7921666666664, np.float32))
+    nat.set_temperature(pi.default_temperature(g, dt, 1)[..., 12].astype(np.float32))
+    log(f"[bench] large instance resident on the GPU after {time.perf_counter() - t0:.1f} s")
+    if probe:
+        probe_copy(pi)
+        nat.run(dt, warmup + steps)
+        nat.sync()
+        log(f"[bench] large instance: {warmup + steps} steps done")
+        return None
+    nat.run(dt, warmup)
+    nat.sync()
+    c0 = nat.individual_steps()
+    t1 = time.perf_counter()
+    nat.run(dt, steps)
+    nat.sync()
+    wall = time.perf_counter() - t1
+    c1 = nat.individual_steps()
+    nkt = 5
+    nat.enable_timing(True)
+    nat.run(dt, nkt)
+    nat.sync()
+    kt = nat.kernel_times()
+    nat.enable_timing(False)
+    live = (c1 - c0) / steps
+    n_cells = g.Nx * g.Ny * g.Nz
+    logical = logical_table(kt, nkt, live, n_cells, nsp * 2 * N)
+    bstep = step_bytes(live, n_cells, nsp * 2 * N)
+    ms = 1e3 * wall / steps
+    out = {"workload": "large single instance 512x512x128, 4 species x 8388608 individuals (capacity 16777216), "
+                       "128-mode Fourier flow, dt 30 s",
+           "steps": steps, "ms_per_step": ms, "individual_steps_per_s": (c1 - c0) / wall,
+           "tracer_cell_steps_per_s": n_cells * steps / wall, "live_individuals_per_step": live,
+           "step_roofline": {"bytes_per_step": bstep, "achieved": bstep / (ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
+                             "unit": "GB/s", "frac": bstep / (ms * 1e-3) / HBM_PEAK},
+           "logical_kernels": logical,
+           "kernels": {name: {"ms_per_launch": t / max(cnt, 1), "launches": cnt} for name, (t, cnt) in kt.items()}}
+    log(f"[bench] large instance: {steps} steps in {wall:.3f} s ({ms:.3f} ms/step), "
+        f"{out['individual_steps_per_s']:.4g} individual-steps/s")
+    del model, nat
+    return out
 
 
 if __name__ == "__main__":

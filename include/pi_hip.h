@@ -185,6 +185,11 @@ pi_status pi_set_diag_tracers(pi_model* m, int32_t n, const int32_t* ids);
 /* Scalars. */
 pi_status pi_get_iteration(pi_model* m, int64_t* iteration);
 pi_status pi_count_active(pi_model* m, int32_t sp, int64_t* n);   /* dot(ac, ac) (models.jl:157-164) */
+/* Σ over the steps taken so far of the individuals each step moved and grew
+ * (all species, alive at the step's gather): the numerator of
+ * individual-steps/s.  No counterpart in the reference (its benchmarks time
+ * whole steps).  Synchronises the model's stream. */
+pi_status pi_individual_steps(pi_model* m, int64_t* n);
 
 /* Per-kernel timing with HIP events recorded on the model's stream around each
  * launch group; pi_kernel_times returns, per group name (static text), the total

@@ -3,7 +3,7 @@
     python oracle/build.py        -> oracle/liboracle_pi.so
 
 g++ only, no contraction, no fast-math, so FP32 arithmetic is plain IEEE binary32
-in source order.
+in source order; OpenMP for the element-wise loops (orc_set_threads, default 1).
 """
 import os
 import subprocess
@@ -16,7 +16,7 @@ OUT = os.path.join(HERE, "liboracle_pi.so")
 def build(verbose=False):
     src = os.path.join(HERE, "pi_oracle.cpp")
     cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fno-finite-math-only",
-           "-Wall", "-fPIC", "-shared", "-o", OUT + ".tmp", src]
+           "-fopenmp", "-Wall", "-fPIC", "-shared", "-o", OUT + ".tmp", src]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -87,7 +87,7 @@ def main(only):
             open(src, "w").write(orig.replace(a, b))
             lib = os.path.join(tmp, "liboracle_mut.so")
             subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
-                            "-fno-finite-math-only", "-fPIC", "-shared", "-iquote", HERE, "-o", lib, src], check=True)
+                            "-fno-finite-math-only", "-fopenmp", "-fPIC", "-shared", "-iquote", HERE, "-o", lib, src], check=True)
             env = dict(os.environ, PI_ORACLE_LIB=lib)
             r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-m", "not gpu", "-p", "no:cacheprovider",
                                 "tests/test_oracle_pins.py", "tests/test_oracle_pins_physics.py"],

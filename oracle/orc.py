@@ -53,6 +53,8 @@ def load():
         lib.orc_download_tracer_f64.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
         lib.orc_fill_halo.restype = C.c_int
         lib.orc_fill_halo.argtypes = [C.c_int32] * 9 + [C.POINTER(C.c_float)]
+        lib.orc_set_threads.restype = C.c_int32
+        lib.orc_set_threads.argtypes = [C.c_int32]
         lib.orc_fill_halo_op.restype = C.c_int
         lib.orc_fill_halo_op.argtypes = [C.c_int32] * 9 + [C.POINTER(C.c_float)]
         lib.orc_expf.restype = C.c_float

@@ -34,6 +34,14 @@
 static thread_local std::string g_err;
 static pi_status fail(pi_status s, const std::string& msg) { g_err = msg; return s; }
 
+/* Threads of the element-wise loops (orc_set_threads; 1 by default).  Only
+ * loops whose iterations write disjoint elements run in parallel; every
+ * accumulation (the per-cell @atomic sums in slot order, the FP32 cumsum and
+ * dot of the division step) stays serial, so the results are bit-identical for
+ * any thread count. */
+static int g_threads = 1;
+#define OMP_FOR _Pragma("omp parallel for schedule(static) num_threads(g_threads) if(g_threads > 1)")
+
 template <typename FT> struct N_;
 template <> struct N_<float> {
     static float max(float a, float b) { return jl_maxf(a, b); }
@@ -84,6 +92,7 @@ struct Oracle : orc_model {
     FT B[PI_NBGC];
     uint64_t seed;
     int64_t iteration = 0;
+    int64_t ind_steps = 0;          /* Σ over steps of the individuals stepped (alive at the gather) */
     FT t = 0;
     std::vector<int> diag_tr;
     bool failed = false;
@@ -326,6 +335,7 @@ struct Oracle : orc_model {
     /* vel_interpolate_kernel! (particle_advection_kernels.jl:26-37) */
     void vel_interpolate(V& ut, V& vt, V& wt, const V& x, const V& y, const V& z,
                          const std::vector<uint8_t>& ac, V* vel) {
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {
             ut[i] = M::mask(ac[i], itpl(vel[0].data(), x[i], y[i], z[i], ac[i], 0));
             vt[i] = M::mask(ac[i], itpl(vel[1].data(), x[i], y[i], z[i], ac[i], 1));
@@ -335,6 +345,7 @@ struct Oracle : orc_model {
     /* calc_coord_kernel! (particle_advection_kernels.jl:40-50) */
     void calc_coord(V& ox, V& oy, V& oz, const V& px, const V& py, const V& pz,
                     const V& u, const V& v, const V& w, const std::vector<uint8_t>& ac, FT dt, FT wt) {
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {
             ox[i] = px[i] + M::mask(ac[i], wt * u[i] * dt);
             oy[i] = py[i] + M::mask(ac[i], wt * v[i] * dt);
@@ -355,6 +366,7 @@ struct Oracle : orc_model {
     }
     /* particle_boundaries_kernel! (particle_advection_kernels.jl:13-23) */
     void boundaries(V& x, V& y, V& z, const std::vector<uint8_t>& ac) {
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {
             x[i] = M::mask(ac[i], pbc(x[i], Nx, tx));
             y[i] = M::mask(ac[i], pbc(y[i], Ny, ty));
@@ -368,15 +380,19 @@ struct Oracle : orc_model {
         calc_coord(vx, vy, vz, p.x, p.y, p.z, u1, v1, w1, p.ac, dt, L(0.5f));
         boundaries(vx, vy, vz, p.ac);
         vel_interpolate(u1, v1, w1, vx, vy, vz, p.ac, velh);
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) { u2[i] = u2[i] + u1[i] * L(2); v2[i] = v2[i] + v1[i] * L(2); w2[i] = w2[i] + w1[i] * L(2); }
         calc_coord(vx, vy, vz, p.x, p.y, p.z, u1, v1, w1, p.ac, dt, L(0.5f));
         boundaries(vx, vy, vz, p.ac);
         vel_interpolate(u1, v1, w1, vx, vy, vz, p.ac, velh);
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) { u2[i] = u2[i] + u1[i] * L(2); v2[i] = v2[i] + v1[i] * L(2); w2[i] = w2[i] + w1[i] * L(2); }
         calc_coord(vx, vy, vz, p.x, p.y, p.z, u1, v1, w1, p.ac, dt, L(1));
         boundaries(vx, vy, vz, p.ac);
         vel_interpolate(u1, v1, w1, vx, vy, vz, p.ac, vel1);
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) { u2[i] = u2[i] + u1[i]; v2[i] = v2[i] + v1[i]; w2[i] = w2[i] + w1[i]; }
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) { u2[i] = u2[i] / L(6); v2[i] = v2[i] / L(6); w2[i] = w2[i] / L(6); }
         calc_coord(p.x, p.y, p.z, p.x, p.y, p.z, u2, v2, w2, p.ac, dt, L(1));
         boundaries(p.x, p.y, p.z, p.ac);
@@ -384,22 +400,26 @@ struct Oracle : orc_model {
     /* particle_diffusion! (Individuals/Advection/particle_diffusion.jl:2-33) */
     void particle_diffusion(Plank& p, int s, FT dt) {
         const FT kx = B[PI_B_kappa_hP], ky = B[PI_B_kappa_hP], kz = B[PI_B_kappa_vP];
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {
             float a, b, c;
             pi_uniform3(seed, iteration, s, PI_KIND_DIFFUSION, i, &a, &b, &c);
             rx[i] = (FT)a; ry[i] = (FT)b; rz[i] = (FT)c;
         }
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {
             rx[i] = (rx[i] - L(0.5f)) * L(2);
             ry[i] = (ry[i] - L(0.5f)) * L(2);
             rz[i] = (rz[i] - L(0.5f)) * L(2);
         }
         const FT sx = M::sqrt(kx * dt), sy = M::sqrt(ky * dt), sz = M::sqrt(kz * dt);
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {   /* calc_diffusion_kernel! :2-8, stale xi/yi/zi (+H twice) */
             rx[i] = rx[i] * sx / dx;
             ry[i] = ry[i] * sy / dy;
             rz[i] = rz[i] * sz / dzC_at(p.zi[i] + Hz);
         }
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {
             p.x[i] = p.x[i] + M::mask(p.ac[i], rx[i]);
             p.y[i] = p.y[i] + M::mask(p.ac[i], ry[i]);
@@ -409,6 +429,7 @@ struct Oracle : orc_model {
     }
     /* find_inds_kernel! (Individuals/utils.jl:2-12) */
     void find_inds(Plank& p) {
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {
             p.xi[i] = (int64_t)M::mask(p.ac[i], p.x[i] + L(1)) + Hx;
             p.yi[i] = (int64_t)M::mask(p.ac[i], p.y[i] + L(1)) + Hy;
@@ -426,6 +447,7 @@ struct Oracle : orc_model {
     /* calc_par_kernel! (Individuals/utils.jl:55-68), one call per level (time_step.jl:63-67) */
     void calc_par(int ki) {
         const FT kc = B[PI_B_kc], kw = B[PI_B_kw];
+        OMP_FOR
         for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
             int ii = i + Hx, jj = j + Hy, kk = ki + Hz;
             FT atten = (Chl[I(ii, jj, kk)] / vol(kk) * kc + kw) * dzF_at(kk);
@@ -437,6 +459,7 @@ struct Oracle : orc_model {
     /* find_NPT_kernel! (Individuals/utils.jl:14-30) */
     void find_NPT(const Plank& p) {
         const FT z0 = L(0);
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {
             int64_t c = cell_of(p, i);
             bool a = p.ac[i];
@@ -490,6 +513,7 @@ struct Oracle : orc_model {
         }
     }
     void inactivate(Plank& p, const V& loss) {   /* Individuals/utils.jl:103-105 */
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) p.ac[i] = p.ac[i] && M::isless(loss[i], L(1));
     }
     /* diags_proc_kernel! (Diagnostics/diagnostics_kernels.jl:4-12) */
@@ -507,6 +531,7 @@ struct Oracle : orc_model {
         diags_proc(diagSp[s][2], p.mort, p);
         calc_loss(p, p.mort, q[PI_P_mortFracC], q[PI_P_mortFracN], q[PI_P_mortFracP], q[PI_P_R_NC], q[PI_P_R_PC]);
         inactivate(p, p.mort);
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) p.dvid[i] = M::mask(p.ac[i], p.dvid[i]);
         diags_proc(diagSp[s][3], p.dvid, p);
         /* dvidnum = dot(dvid, ac); deactive_ind = findall(!ac) */
@@ -532,9 +557,12 @@ struct Oracle : orc_model {
     void divide(Plank& p, const std::vector<int64_t>& free_ind) {
         FT acc = 0;
         for (int64_t i = 0; i < cap; ++i) { acc = acc + p.dvid[i]; idc[i] = acc; }   /* accumulate!(+) */
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) idc_int[i] = (int64_t)idc[i];
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i)                                           /* get_tind_kernel! */
             if (p.dvid[i] == L(1)) p.idx[i] = free_ind[(size_t)(idc_int[i] - 1)];
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {                                         /* copy_daughter_individuals_kernel! */
             if (p.dvid[i] == L(1) && p.idx[i] != 0) {
                 int64_t d = p.idx[i] - 1;
@@ -543,6 +571,7 @@ struct Oracle : orc_model {
                 p.Chl[d] = p.Chl[i]; p.gen[d] = p.gen[i]; p.ac[d] = p.ac[i]; p.dvid[d] = p.dvid[i];
             }
         }
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {                                         /* divide_to_half_kernel! */
             FT h = (L(2) - p.dvid[i]) / L(2);
             p.Sz[i] = p.Sz[i] * h; p.Bm[i] = p.Bm[i] * h; p.Cq[i] = p.Cq[i] * h;
@@ -550,6 +579,7 @@ struct Oracle : orc_model {
             p.gen[i] = p.gen[i] + p.dvid[i];
             p.age[i] = p.age[i] * (L(1) - p.dvid[i]);
         }
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) p.idx[i] = 0;
     }
     /* plankton_growth! (QuotaMode/plankton_growth.jl:2-24, growth_kernels.jl:48-171) */
@@ -557,6 +587,7 @@ struct Oracle : orc_model {
         Plank& p = sp[s];
         const auto& q = P[s];
         const FT e30 = L(1.0e-30f), z0 = L(0), one = L(1);
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {   /* calc_inorganic_uptake_kernel! */
             bool a = p.ac[i];
             FT aI = tpar[i] * q[PI_P_alpha] * q[PI_P_Phi];
@@ -577,11 +608,13 @@ struct Oracle : orc_model {
             p.VNO3[i] = M::min(VNO3, tNO3[i] / dt / pp);
             p.VPO4[i] = M::min(VPO4, tPO4[i] / dt / pp);
         }
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {   /* update_quotas_1_kernel! */
             p.Cq[i] = p.Cq[i] + p.PS[i] * dt;
             p.Pq[i] = p.Pq[i] + p.VPO4[i] * dt;
             p.Nq[i] = p.Nq[i] + (p.VNH4[i] + p.VNO3[i]) * dt;
         }
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {   /* calc_organic_uptake_kernel! */
             FT Bm = p.Bm[i], Cq = p.Cq[i];
             FT Qc = Cq / M::max(e30, Bm + Cq);
@@ -591,14 +624,17 @@ struct Oracle : orc_model {
             p.VDOC[i] = M::mask(p.ac[i], VD);
             p.VDOC[i] = M::mask(M::isless(L(1.0e-2f), p.PS[i] / (p.VDOC[i] + p.PS[i])), p.VDOC[i]);
         }
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {   /* calc_ρChl_kernel! */
             FT r = p.PS[i] / M::max(e30, p.Bm[i]) * q[PI_P_Chl2N] /
                    M::max(e30, tpar[i] * q[PI_P_alpha] * q[PI_P_Phi] * p.Chl[i] / p.Bm[i]);
             r = M::mask(M::isless(L(1.0e-1f), tpar[i]), r);
             p.rho[i] = M::mask(p.ac[i], r);
         }
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i)     /* calc_respir_kernel! */
             p.resp[i] = M::mask(p.ac[i], q[PI_P_respir] * p.Bm[i] * tempFunc(tT[i], q));
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {   /* update_quotas_2_kernel! */
             p.Cq[i] = p.Cq[i] + (p.VDOC[i] - p.resp[i]) * dt;
             p.Bm[i] = p.Bm[i] - M::max(z0, (z0 - p.Cq[i]));
@@ -606,12 +642,14 @@ struct Oracle : orc_model {
             p.Pq[i] = p.Pq[i] + M::max(z0, (z0 - p.Cq[i])) * q[PI_P_R_PC];
             p.Cq[i] = p.Cq[i] + M::max(z0, (z0 - p.Cq[i]));
         }
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {   /* calc_BS_kernel! */
             FT mn = M::min(M::min(p.Cq[i], p.Nq[i] / q[PI_P_R_NC]), p.Pq[i] / q[PI_P_R_PC]);
             p.BS[i] = M::mask(p.ac[i], mn * q[PI_P_k_mtb]);
             FT mn2 = M::min(M::min(p.Cq[i], p.Nq[i] / q[PI_P_R_NC]), p.Pq[i] / q[PI_P_R_PC]);
             p.exu[i] = M::mask(p.ac[i], M::max(z0, p.Cq[i] - mn2) * q[PI_P_k_mtb]);
         }
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {   /* update_biomass_kernel! */
             p.Bm[i] = p.Bm[i] + dt * p.BS[i];
             p.Cq[i] = p.Cq[i] - dt * (p.BS[i] + p.exu[i]);
@@ -620,13 +658,17 @@ struct Oracle : orc_model {
             p.Chl[i] = p.Chl[i] + dt * p.BS[i] * q[PI_P_R_NC] * p.rho[i];
             p.age[i] = p.age[i] + M::mask(p.ac[i], dt / L(3600));
         }
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i)     /* update_cellsize_kernel! */
             p.Sz[i] = p.Bm[i] / (q[PI_P_Cquota] * q[PI_P_Nsuper]);
         /* probabilities (division_death_probability.jl:176-239) */
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i)     /* calc_graz_quadratic_kernel! */
             p.graz[i] = M::mask(p.ac[i], tpop[i] * q[PI_P_grz_P]);
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i)     /* calc_mort_kernel! */
             p.mort[i] = M::mask(p.ac[i], q[PI_P_mort_P] * shape_dec(p.Sz[i], q[PI_P_mort_reg], L(1.0e-5f), 4));
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {   /* calc_dvid_kernel! with calc_division (:178-183) */
             FT P_ = q[PI_P_dvid_P], Sz = p.Sz[i], v;
             FT hour = M::rem(t, L(86400)) / L(3600);
@@ -636,6 +678,7 @@ struct Oracle : orc_model {
             p.dvid[i] = M::mask(p.ac[i], v);
         }
         /* get_probability! (division_death_probability.jl:266-278) */
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {
             float a, b, c;
             pi_uniform3(seed, iteration, s, PI_KIND_DECISION, i, &a, &b, &c);
@@ -647,6 +690,7 @@ struct Oracle : orc_model {
     }
     /* calc_consume_kernel! (QuotaMode/consume_loss.jl:2-14) */
     void calc_consume(const Plank& p, FT dt) {
+        OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {
             int64_t c = cell_of(p, i);
             bool a = p.ac[i];
@@ -714,6 +758,7 @@ struct Oracle : orc_model {
         for (int dim = 0; dim < 3; ++dim) {
             const FT* w = vel[dim].data();
             for (int q = 0; q < PI_NTRACERS; ++q)          /* calc_Gcs{x,y,z}! */
+                OMP_FOR
                 for (int k = Hz + 1; k <= Hz + Nz; ++k) for (int j = Hy + 1; j <= Hy + Ny; ++j)
                     for (int i = Hx + 1; i <= Hx + Nx; ++i)
                         Gcs[q][I(i, j, k)] = dim == 0 ? flux_x(ttemp[q].data(), w, i, j, k, dt)
@@ -723,6 +768,7 @@ struct Oracle : orc_model {
             for (int q = 0; q < PI_NTRACERS; ++q) {        /* multi_dim_{x,y,z}! */
                 const FT* G = Gcs[q].data();
                 const FT* c = tr[q].data();
+                OMP_FOR
                 for (int k = Hz + 1; k <= Hz + Nz; ++k) for (int j = Hy + 1; j <= Hy + Ny; ++j)
                     for (int i = Hx + 1; i <= Hx + Nx; ++i) {
                         FT dG, dT;
@@ -743,6 +789,7 @@ struct Oracle : orc_model {
             fill_halo_tracer(ttemp);
         }
         for (int q = 0; q < PI_NTRACERS; ++q)             /* calc_tracer_tendency! */
+            OMP_FOR
             for (int64_t n = 0; n < ncell; ++n) Gcs[q][n] = ttemp[q][n] - tr[q][n];
     }
     /* tracer_diffusion! (Biogeochemistry/Advection/tracer_diffusion.jl:1-30) */
@@ -753,6 +800,7 @@ struct Oracle : orc_model {
             auto fx = [&](int i, int j, int k) { return kx * Ax(k) * (dx0(c, i, j, k) / dx); };
             auto fy = [&](int i, int j, int k) { return ky * Ay(k) * (dy0(c, i, j, k) / dy); };
             auto fz = [&](int i, int j, int k) { return kz * Az() * (dz0(c, i, j, k) / dzF_at(k)); };
+            OMP_FOR
             for (int k = Hz + 1; k <= Hz + Nz; ++k) for (int j = Hy + 1; j <= Hy + Ny; ++j)
                 for (int i = Hx + 1; i <= Hx + Nx; ++i) {
                     FT ddx = M::mask(LM(i, j, k), M::mask(LM(i + 1, j, k), fx(i + 1, j, k) - fx(i, j, k)));
@@ -770,9 +818,11 @@ struct Oracle : orc_model {
         for (int a = 0; a < 3; ++a) {
             int q = names[a];
             FT ws = a < 2 ? B[PI_B_w_sink_inorg] : B[PI_B_w_sink_org];
+            OMP_FOR
             for (int64_t n = 0; n < ncell; ++n) flux_sink[n] = ws * tr[q][n];
             fill_top_flux_sink(flux_sink.data());
             fill_bottom_flux_sink(flux_sink.data());
+            OMP_FOR
             for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
                 int ii = i + Hx, jj = j + Hy, kk = k + Hz;
                 FT& G = Gcs[q][I(ii, jj, kk)];
@@ -784,12 +834,14 @@ struct Oracle : orc_model {
     void tracer_forcing(FT dt) {
         const FT z0 = L(0);
         for (int q = 0; q < PI_NTRACERS; ++q)
+            OMP_FOR
             for (int64_t n = 0; n < ncell; ++n) ttemp[q][n] = M::max(z0, tr[q][n]);
         const FT kDOC = B[PI_B_kDOC], kPOC = B[PI_B_kPOC], kDON = B[PI_B_kDON], kPON = B[PI_B_kPON];
         const FT kDOP = B[PI_B_kDOP], kPOP = B[PI_B_kPOP], Nit = B[PI_B_Nit];
         const FT lPOC = B[PI_B_lambda_POC], lmin = B[PI_B_lambda_min], ldust = B[PI_B_lambda_dust];
         const FT DFeFrac = B[PI_B_DFeFrac], kdiss = B[PI_B_kdiss], lig = B[PI_B_ligand], lFe = B[PI_B_lambda_Fe];
         auto T = [&](int q, int64_t n) { return ttemp[q][n]; };
+        OMP_FOR
         for (int64_t n = 0; n < ncell; ++n) {
             FT* F[PI_NTRACERS];
             for (int q = 0; q < PI_NTRACERS; ++q) F[q] = &Gcs[q][n];
@@ -825,6 +877,7 @@ struct Oracle : orc_model {
         tracer_forcing(dt);
         /* apply_bcs! is a no-op without flux boundary conditions (apply_bcs.jl:101-106) */
         for (int q = 0; q < PI_NTRACERS; ++q)       /* apply_tendency_kernel! (:6-20) */
+            OMP_FOR
             for (int k = Hz + 1; k <= Hz + Nz; ++k) for (int j = Hy + 1; j <= Hy + Ny; ++j)
                 for (int i = Hx + 1; i <= Hx + Nx; ++i) {
                     int64_t n = I(i, j, k);
@@ -838,9 +891,11 @@ struct Oracle : orc_model {
     pi_status step(float dt_in) {
         if (failed) return fail(PI_ERR_CAPACITY, "model is in an error state from an earlier step");
         const FT dt = (FT)dt_in;
+        for (const auto& p : sp) for (int64_t i = 0; i < cap; ++i) ind_steps += p.ac[i] ? 1 : 0;
         iteration += 1;
         t = (FT)(float)iteration * dt;
         for (int q = 0; q < 3; ++q)
+            OMP_FOR
             for (int64_t n = 0; n < ncell; ++n) velh[q][n] = (vel0[q][n] + vel1[q][n]) * L(0.5f);
         for (int q = 0; q < PI_NTRACERS; ++q) std::fill(plk[q].begin(), plk[q].end(), (FT)0);
         std::fill(Chl.begin(), Chl.end(), (FT)0);
@@ -876,8 +931,10 @@ struct Oracle : orc_model {
             }
         }
         tracer_update(dt);
+        OMP_FOR
         for (int64_t n = 0; n < ncell; ++n) diagPAR[n] += par[n];
         for (size_t a = 0; a < diag_tr.size(); ++a)
+            OMP_FOR
             for (int64_t n = 0; n < ncell; ++n) diagTr[a][n] += tr[diag_tr[a]][n];
         for (int q = 0; q < 3; ++q) vel0[q] = vel1[q];
         par0 = par;
@@ -1041,6 +1098,7 @@ pi_status orc_set_diag_tracers(orc_model* m, int32_t n, const int32_t* ids) {
              o->diagTr.assign(o->diag_tr.size(), std::vector<decltype(o->t)>(o->ncell, 0)); return PI_OK);
 }
 pi_status orc_get_iteration(orc_model* m, int64_t* it) { DISPATCH(m, *it = o->iteration; return PI_OK); }
+pi_status orc_individual_steps(orc_model* m, int64_t* n) { DISPATCH(m, *n = o->ind_steps; return PI_OK); }
 pi_status orc_count_active(orc_model* m, int32_t s, int64_t* n) {
     DISPATCH(m, if (s < 0 || s >= o->nsp) return fail(PI_ERR_ARG, "species");
              int64_t c = 0; for (auto a : o->sp[s].ac) c += a; *n = c; return PI_OK);
@@ -1072,6 +1130,11 @@ pi_status orc_fill_halo_op(int32_t op, int32_t dim, int32_t topo, int32_t Nx, in
         default: o.fill_bottom_flux_sink(c); break;
     }
     return PI_OK;
+}
+int32_t orc_set_threads(int32_t n)
+{
+    g_threads = n < 1 ? 1 : n;
+    return g_threads;
 }
 float orc_expf(float x) { return pi_expf(x); }
 double orc_exp_f64(double x) { return pi_exp_f64(x); }

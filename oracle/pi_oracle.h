@@ -41,6 +41,7 @@ pi_status orc_reset_diags(orc_model* m);
 pi_status orc_set_diag_tracers(orc_model* m, int32_t n, const int32_t* ids);
 pi_status orc_get_iteration(orc_model* m, int64_t* iteration);
 pi_status orc_count_active(orc_model* m, int32_t sp, int64_t* n);
+pi_status orc_individual_steps(orc_model* m, int64_t* n);
 
 /* Building blocks exposed for the oracle's own pin tests. */
 pi_status orc_fill_halo(int32_t Nx, int32_t Ny, int32_t Nz, int32_t Hx, int32_t Hy, int32_t Hz,
@@ -50,6 +51,8 @@ pi_status orc_fill_halo(int32_t Nx, int32_t Ny, int32_t Nz, int32_t Hx, int32_t 
  * 4 = fill_halo_top_flux_sink!, 5 = fill_halo_bottom_flux_sink!; topo = PI_PERIODIC/PI_BOUNDED */
 pi_status orc_fill_halo_op(int32_t op, int32_t dim, int32_t topo, int32_t Nx, int32_t Ny, int32_t Nz,
                            int32_t Hx, int32_t Hy, int32_t Hz, float* c);
+/* threads of the element-wise loops (OpenMP; default 1); returns the value set */
+int32_t orc_set_threads(int32_t n);
 float orc_expf(float x);
 double orc_exp_f64(double x);
 void orc_log_f64_v(const double* x, double* y, int64_t n);

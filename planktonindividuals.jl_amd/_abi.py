@@ -118,6 +118,7 @@ def declare(lib, prefix, model_t=C.c_void_p):
         "reset_diags": (C.c_int, [M]),
         "get_iteration": (C.c_int, [M, i64p]),
         "count_active": (C.c_int, [M, C.c_int32, i64p]),
+        "individual_steps": (C.c_int, [M, i64p]),
     }
     for name, (res, args) in proto.items():
         f = getattr(lib, prefix + name)

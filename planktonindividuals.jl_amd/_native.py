@@ -204,6 +204,12 @@ class Handle:
         self._call("count_active", sp, C.byref(v))
         return v.value
 
+    def individual_steps(self):
+        """Σ over the steps taken of the individuals stepped (pi_individual_steps)."""
+        v = C.c_int64(0)
+        self._call("individual_steps", C.byref(v))
+        return v.value
+
 
 def footprint(grid, nsp, cap, params, bgc, seed=0, diag_tracers=()):
     """Device bytes a model of this shape allocates (pi_footprint: a dry run of

@@ -152,6 +152,7 @@ __device__ __forceinline__ void k_finish_impl(Blk K, DevState* st, const uint32_
     uint32_t born = 0;
     for (int s = 0; s < nsp; ++s) born += st->dtot[s];
     st->nA = seg[ncell];                      /* the sorted part of the next store */
+    st->ind_steps += seg[ncell];              /* individuals stepped (alive at this step's gather) */
     st->nstore = seg[ncell] + born;
     st->scur ^= 1;
     st->tcur ^= 1;

@@ -231,6 +231,7 @@ struct DevState {
     float* par[2];                     /* [pcur] = par of the previous step (par₀) */
     int32_t scur, tcur, pcur, vcur;
     long long iteration;               /* completed steps */
+    unsigned long long ind_steps;      /* Σ over completed steps of the individuals stepped */
     uint32_t nstore;                   /* entries in store[scur] */
     uint32_t nA;                       /* of which the first nA are sorted by (cell, key) as laid out by the previous seg */
     uint32_t nsmall[PI_NQ];            /* queued entries of this step, per queue shard */

@@ -1051,6 +1051,15 @@ pi_status pi_get_iteration(pi_model* m, int64_t* it)
     return PI_OK;
 }
 
+pi_status pi_individual_steps(pi_model* m, int64_t* n)
+{
+    if (!m || !n) return fail(PI_ERR_ARG, "null argument");
+    pi_status s = sync_state(m);
+    if (s != PI_OK && s != PI_ERR_CAPACITY) return s;
+    *n = (int64_t)m->h_state.ind_steps;
+    return PI_OK;
+}
+
 pi_status pi_count_active(pi_model* m, int32_t sp, int64_t* n)
 {
     if (!m || !n || sp < 0 || sp >= m->nsp) return fail(PI_ERR_ARG, "bad argument");

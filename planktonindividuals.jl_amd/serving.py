@@ -70,7 +70,7 @@ class EnsembleServer:
         self.finished = []
         self.step_ms = []                 # wall time of each batched step
         self.member_steps = 0             # Σ members per step
-        self.individual_steps = 0.0       # Σ live individuals per step (members' counts at admission)
+        self.individual_steps = 0         # Σ over members and steps of the live individuals stepped
         self.on_finish = on_finish
         self._lock = threading.Lock()
         self._n = {}
@@ -143,7 +143,6 @@ class EnsembleServer:
         self.ens.batch.sync()
         self.step_ms.append((time.perf_counter() - t0) * 1e3)
         self.member_steps += len(self.ens.sims)
-        self.individual_steps += sum(self._n[s] for s in self.ens.sims)
         for sim in list(self.ens.sims):
             if self.ens.done[sim] >= sim.iterations:
                 req = self.active.pop(sim)
@@ -154,6 +153,8 @@ class EnsembleServer:
                 except _native.CapacityError as e:
                     req.error = str(e)
                 req.result = [sim.model._native.count_active(s) for s in range(sim.model.n_species)]
+                # live individuals of every step it took (counted on the device, pi_individual_steps)
+                self.individual_steps += sim.model._native.individual_steps()
                 self.finished.append(req)
                 del self._n[sim]
                 self.releasing.append(self.pool.submit(self._release, sim, req))
@@ -168,14 +169,17 @@ class EnsembleServer:
         self.releasing = []
 
     def stats(self):
+        """Step and end-to-end latency percentiles; individual_steps counts every
+        step taken so far, members still running included."""
         st = np.array(self.step_ms) if self.step_ms else np.zeros(1)
+        running = sum(sim.model._native.individual_steps() for sim in self.ens.sims)
         done = [r for r in self.finished if r.error is None and r.finished is not None]
         e2e = np.array([r.finished - r.arrival for r in done]) if done else np.zeros(1)
         return dict(steps=len(self.step_ms), p50_step_ms=float(np.percentile(st, 50)),
                     p99_step_ms=float(np.percentile(st, 99)), mean_step_ms=float(st.mean()),
                     finished=len(done), p50_e2e_s=float(np.percentile(e2e, 50)),
                     p99_e2e_s=float(np.percentile(e2e, 99)), member_steps=self.member_steps,
-                    individual_steps=self.individual_steps)
+                    individual_steps=self.individual_steps + running)
 
     def close(self):
         for f in self.releasing:

@@ -35,6 +35,8 @@ def compare_models(gpu, ref, nsp, tracer_ids=range(14), grid_fields=("par", "Chl
     per-species diagnostics, and the tracer diagnostics of `diag_tracers`
     (time_step.jl:168-172)."""
     bad = []
+    if gpu.individual_steps() != ref.individual_steps():
+        bad.append(f"individual steps: {gpu.individual_steps()} vs {ref.individual_steps()}")
     for q in diag_tracers:
         a, b = gpu.diag_tracer(q), ref.diag_tracer(q)
         if not same(a, b):

@@ -432,3 +432,26 @@ def test_fp64_oracle_close_to_fp32(pi, orc):
         x32 = a.field(0, f)[ac].astype(np.float64)
         x64 = b.field_f64(0, f)[ac]
         assert np.allclose(x32, x64, rtol=1e-5, atol=1e-12), f
+
+
+def test_oracle_threads_bit_identical(pi, orc):
+    """The oracle's element-wise loops run on OpenMP threads (the CPU baseline
+    over all host cores); every accumulation stays serial, so 1 and 4 threads
+    give the same bits."""
+    from parity import compare_models
+    lib = orc.load()
+    spec, (u, v, w) = pi.workloads.config_3d(n=12, nz=6, N=900, nsp=2, cap=2400, seed=7, nmodes=10,
+                                             overrides={"dvid_P": [2e-3], "mort_P": [1e-3], "grz_P": [2e-5]},
+                                             kappa=(1.0, 1e-4))
+    g = spec.grid
+    runs = []
+    for nt in (1, 4):
+        assert lib.orc_set_threads(nt) == nt
+        om = orc.OracleModel(spec, diag_tracers=[3])
+        om.update(60.0, 6, vel=(u * 20, v * 20, w * 20), parf=np.full((g.Nx, g.Ny), 800.0, np.float32),
+                  temp=np.full((g.Nx, g.Ny, g.Nz), 24.0, np.float32))
+        runs.append(om)
+    lib.orc_set_threads(1)
+    assert runs[0].diag(0, "dvid").sum() > 0
+    bad = compare_models(runs[1], runs[0], 2, diag_tracers=[3])
+    assert not bad, "\n".join(bad[:10])
