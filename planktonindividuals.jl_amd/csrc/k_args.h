@@ -176,7 +176,7 @@ enum KId {
     K_VELHALF, K_MARKS, K_SCAN_P, K_SCAN_F, K_CAPCHECK, K_ADV, K_CELLCOUNT, K_PLACE_STAY,
     K_PLACE_SMALL, K_MIXSORT, K_BIGCELLS, K_CELLSUM, K_CELLSUM_LONG, K_PAR, K_PHYS, K_PLK, K_PLK_LONG,
     K_SWEEPX0, K_SWEEPX1, K_XY0, K_XY1, K_SWEEPY0, K_SWEEPY1, K_HALOCOPY, K_FILL, K_FINAL0, K_FINAL1, K_ACCUM,
-    K_FINISH, K_NKID
+    K_FINISH, K_XYW0, K_XYW1, K_NKID
 };
 /* plan markers (not kernels): fork to / join from the side stream */
 enum { K_FORK = K_NKID, K_JOIN = K_NKID + 1 };
@@ -199,6 +199,7 @@ long long sweep_x_blocks(const DevGrid& g);
 long long sweep_y_items(const DevGrid& g);
 long long final_items(const DevGrid& g);
 bool xy_supported(const DevGrid& g);   /* k_xy handles this grid */
+bool xy_wide(const DevGrid& g);        /* with the wide blocks (k_xyw) */
 long long xy_blocks(const DevGrid& g);
 uint32_t scan_tiles(uint32_t n);
 

@@ -675,7 +675,8 @@ PI_KERNEL2(k_final1, k_final_body<true>, FinalArgs, 256)
  * calc_Gcsˣ! + multi_dim_x! + fill_halo_tracer! + calc_Gcsʸ! + multi_dim_y!
  * (Advection/multi_dim_adv.jl:111-135) in one pass, then the z halos of the
  * result on interior lines (the halo fill the z sweep reads), for grids whose
- * Periodic directions have N >= H, H = 2 and Nx <= XY_T.
+ * Periodic directions have N >= H, H = 2 and Nx <= XY_W (blocks of XY_T
+ * threads up to Nx = XY_T, of XY_W threads above).
  *
  * A group of roundup64(Nx) threads owns one x-row (thread = cell i) and walks
  * a run of RUN_XY rows of one level in y.  Each row step computes the x sweep
@@ -691,6 +692,7 @@ PI_KERNEL2(k_final1, k_final_body<true>, FinalArgs, 256)
  * recomputed bit for bit.  Tracers are processed in two passes of 7. */
 #define RUN_XY 16
 #define XY_T 256
+#define XY_W 512
 
 __device__ __forceinline__ int map_row(int j, int N, int topo)
 {
@@ -722,9 +724,10 @@ __device__ __forceinline__ float flux_full_s(const Face& F, float bp, float bm, 
     return 0.5f * (T + fabsf(T)) * (bp + pp * dd0) + 0.5f * (T - fabsf(T)) * (bm - pm * dd0);
 }
 
+template <int T>
 struct XYShared {
-    float c[2][7][XY_T];      /* tracer rows of the x sweep (double-buffered) */
-    float F[2][7][XY_T];      /* their west-face fluxes */
+    float c[2][7][T];         /* tracer rows of the x sweep (double-buffered) */
+    float F[2][7][T];         /* their west-face fluxes */
 };
 
 /* One pass (tracers Q0..Q1-1) over a run of rows of one level.  Row step n
@@ -736,9 +739,9 @@ struct XYShared {
  * the fluxes of row n; one barrier per row step.  x halo values are the
  * reference's filled halos of the row: periodic images, or the edge value when
  * Bounded. */
-template <bool LAND, int Q0, int Q1>
+template <bool LAND, int Q0, int Q1, int TB>
 __device__ __forceinline__ void xy_run(const XYArgs& a, const DevGrid& g, const PI_GLOBAL float* C0, PI_GLOBAL float* OUT,
-                                       XYShared& sh, int t, int gbase, int i, bool xin, bool act, int j0, int k)
+                                       XYShared<TB>& sh, int t, int gbase, int i, bool xin, bool act, int j0, int k)
 {
     constexpr int NQ = Q1 - Q0;
     constexpr int NS = RUN_XY + 4;
@@ -758,7 +761,7 @@ __device__ __forceinline__ void xy_run(const XYArgs& a, const DevGrid& g, const 
     const int im2 = gbase + (perx ? (i >= 2 ? i - 2 : i - 2 + g.Nx) : max(i - 2, 0));
     const int im1 = gbase + (perx ? (i >= 1 ? i - 1 : i - 1 + g.Nx) : max(i - 1, 0));
     const int ip1 = gbase + (perx ? (i + 1 < g.Nx ? i + 1 : i + 1 - g.Nx) : min(i + 1, g.Nx - 1));
-    const int tn = last ? gbase : min(t + 1, XY_T - 1);          /* slot of the east face's flux */
+    const int tn = last ? gbase : min(t + 1, TB - 1);            /* slot of the east face's flux */
     float A[NQ], B[NQ], dA[NQ], dB[NQ], Fp[NQ];
     const int wbase = __builtin_amdgcn_readfirstlane(t) & ~63;    /* this wave's LDS slots (uniform) */
     /* row jr of tracers Q0.. into LDS buffer b: each lane loads its own cell */
@@ -880,15 +883,15 @@ __device__ __forceinline__ void xy_run(const XYArgs& a, const DevGrid& g, const 
     }
 }
 
-template <bool LAND>
+template <bool LAND, int TB>
 __device__ __forceinline__ void k_xy_body(const XYArgs& a, Blk K)
 {
-    __shared__ XYShared sh;
+    __shared__ XYShared<TB> sh;
     const DevState* S = a.st;
     if (S->err) return;
     const DevGrid& g = a.g;
     const int nxr = (g.Nx + 63) & ~63;
-    const int G = XY_T / nxr;                                     /* x-rows per block */
+    const int G = TB / nxr;                                       /* x-rows per block */
     const int t = (int)threadIdx.x;
     const int grp = t / nxr;
     const int gbase = grp * nxr;
@@ -908,18 +911,25 @@ __device__ __forceinline__ void k_xy_body(const XYArgs& a, Blk K)
         xy_run<LAND, 7, 14>(a, g, C0, OUT, sh, t, gbase, i, xin, act, r * RUN_XY, k);
     }
 }
-PI_KERNEL2(k_xy0, k_xy_body<false>, XYArgs, XY_T)
-PI_KERNEL2(k_xy1, k_xy_body<true>, XYArgs, XY_T)
+__device__ __forceinline__ void k_xy0_body(const XYArgs& a, Blk K) { k_xy_body<false, XY_T>(a, K); }
+__device__ __forceinline__ void k_xy1_body(const XYArgs& a, Blk K) { k_xy_body<true, XY_T>(a, K); }
+__device__ __forceinline__ void k_xyw0_body(const XYArgs& a, Blk K) { k_xy_body<false, XY_W>(a, K); }
+__device__ __forceinline__ void k_xyw1_body(const XYArgs& a, Blk K) { k_xy_body<true, XY_W>(a, K); }
+PI_KERNEL_XCD(k_xy0, XYArgs, XY_T)
+PI_KERNEL_XCD(k_xy1, XYArgs, XY_T)
+PI_KERNEL_XCD(k_xyw0, XYArgs, XY_W)
+PI_KERNEL_XCD(k_xyw1, XYArgs, XY_W)
 
 bool xy_supported(const DevGrid& g)
 {
-    return g.Hx == 2 && g.Hy == 2 && g.Hz == 2 && g.Nx <= XY_T && g.tz == PI_BOUNDED &&
+    return g.Hx == 2 && g.Hy == 2 && g.Hz == 2 && g.Nx <= XY_W && g.tz == PI_BOUNDED &&
            !(g.tx == PI_PERIODIC && g.Nx < g.Hx) && !(g.ty == PI_PERIODIC && g.Ny < g.Hy);
 }
+bool xy_wide(const DevGrid& g) { return g.Nx > XY_T; }
 long long xy_blocks(const DevGrid& g)
 {
     const int nxr = (g.Nx + 63) & ~63;
-    const long long G = XY_T / nxr;
+    const long long G = (xy_wide(g) ? XY_W : XY_T) / nxr;
     const long long nitem = (long long)((g.Ny + RUN_XY - 1) / RUN_XY) * g.Nz;
     return (nitem + G - 1) / G;
 }
@@ -946,4 +956,6 @@ void reg_tracers(KInfo* k)
     PI_REG(k, K_FINAL1, k_final1, FinalArgs, 256, true);
     PI_REG(k, K_XY0, k_xy0, XYArgs, XY_T, true);
     PI_REG(k, K_XY1, k_xy1, XYArgs, XY_T, true);
+    PI_REG(k, K_XYW0, k_xyw0, XYArgs, XY_W, true);
+    PI_REG(k, K_XYW1, k_xyw1, XYArgs, XY_W, true);
 }

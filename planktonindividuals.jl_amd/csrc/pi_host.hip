@@ -718,8 +718,11 @@ static void build_step(pi_model* m, float dt, Plan& P)
             XYArgs xa = zeroed<XYArgs>();
             xa.g = g; xa.st = st; xa.dt = dt; xa.u = V1.u; xa.v = V1.v; xa.out = m->T2[0];
             const long long xyb = fxy ? xy_blocks(g) : 0;
-            P.add(K_XY0, "sweep_xy", land ? 0 : xyb, xa);
-            P.add(K_XY1, "sweep_xy", land ? xyb : 0, xa);
+            const bool wide = xy_wide(g);
+            P.add(K_XY0, "sweep_xy", land || wide ? 0 : xyb, xa);
+            P.add(K_XY1, "sweep_xy", land && !wide ? xyb : 0, xa);
+            P.add(K_XYW0, "sweep_xy", !land && wide ? xyb : 0, xa);
+            P.add(K_XYW1, "sweep_xy", land && wide ? xyb : 0, xa);
             SweepArgs sa = zeroed<SweepArgs>();
             sa.g = g; sa.st = st; sa.dt = dt;
             sa.vel = V1.u; sa.dim = 0;

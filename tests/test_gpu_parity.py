@@ -294,13 +294,14 @@ def test_parity_zero_tracers(pi, orc):
 
 
 @pytest.mark.parametrize("nx,ny,nz,land", [(128, 6, 4, False), (256, 3, 2, False), (320, 5, 3, False),
-                                            (128, 5, 3, True), (320, 4, 2, True)])
+                                            (128, 5, 3, True), (320, 4, 2, True), (512, 3, 2, False),
+                                            (544, 3, 2, True)])
 def test_parity_wide_rows(pi, orc, nx, ny, nz, land):
-    """Rows of two and four waves (the fused x/y sweep with rows of whole
-    waves, up to its 256-cell limit, and the z walk's lanes of one row per wave)
-    and rows wider than that (the separate sweeps and line fills), periodic in x
-    and y, a random flow of both signs, with and without land: identical to the
-    oracle."""
+    """Rows of two and four waves (the fused x/y sweep with 256-thread blocks,
+    rows of whole waves, and the z walk's lanes of one row per wave), rows of
+    257-512 cells (the fused sweep with 512-thread blocks), and rows wider than
+    that (the separate sweeps and line fills), periodic in x and y, a random
+    flow of both signs, with and without land: identical to the oracle."""
     lm = None
     if land:
         lm = np.ones((nx, ny, nz), dtype=bool)
