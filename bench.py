@@ -254,7 +254,7 @@ def cpu_baseline(pi, seconds_hint=10.0):
                               "tracer_cell_steps_per_s": one["tracer_cell_steps_per_s"]}}
 
 
-def ensemble_leg(pi, members=256, steps=20, seq_steps=4, seed=4, batched_only=False):
+def ensemble_leg(pi, members=256, steps=20, seq_steps=4, seed=4, batched_only=False, tune=True):
     """Batched ensemble: `members` drawn from the ensemble table, stepped
     one-by-one through the plain path (pi_run + sync per member) and then all
     together (pi_batch_run); ms per ensemble step of each and the ratio."""
@@ -286,14 +286,19 @@ def ensemble_leg(pi, members=256, steps=20, seq_steps=4, seed=4, batched_only=Fa
         batch.add(nat)
     batch.run(60.0, 2)
     batch.sync()
-    tuned = batch.autotune(60.0)
-    log("[bench] ensemble: block budget " + ", ".join(f"{b}: {ms:.2f} ms" for b, ms in tuned.items()))
+    t_tune = time.perf_counter()
+    tuned = batch.autotune(60.0, steps=3, reps=2) if tune else None
+    t_tune = time.perf_counter() - t_tune
+    if tuned:
+        log(f"[bench] ensemble: budget {tuned['budget']}, shapes per class {tuned['classes']} "
+            f"(searched in {t_tune:.1f} s)")
     c0 = sum(nat.individual_steps() for nat in nats)
     t0 = time.perf_counter()
     batch.run(60.0, steps)
     batch.sync()
     t_b = (time.perf_counter() - t0) / steps
     live = (sum(nat.individual_steps() for nat in nats) - c0) / steps   # individuals stepped per ensemble step
+    batch_budget = batch.budget
     batch.close()
     for nat in nats:
         nat.close()
@@ -307,8 +312,9 @@ def ensemble_leg(pi, members=256, steps=20, seq_steps=4, seed=4, batched_only=Fa
             "speedup_vs_one_by_one": t_seq / t_b,
             "batched_individual_steps_per_s": live / t_b, "one_by_one_individual_steps_per_s": live / t_seq,
             "batched_steps": steps, "one_by_one_steps": seq_steps,
-            "autotune_ms_per_step_by_block_budget": {str(b): ms for b, ms in tuned.items()},
-            "block_budget": min(tuned, key=tuned.get)}
+            "block_budget": batch_budget,
+            "launch_shapes_by_class": None if tuned is None else tuned["classes"],
+            "autotune": None if tuned is None else {"trials_ms_per_step": tuned["trials"], "seconds": t_tune}}
 
 
 def _oracle_member(pi, entry, steps, dt=60.0):
@@ -455,6 +461,10 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: a plain device copy of 256 MiB (k_probe_copy, known traffic, to "
                          "scale the counters), then --warmup + --steps steps without timing; prints no JSON")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shapes (no autotune search before the timed steps)")
+    ap.add_argument("--shapes", default=None,
+                    help='launch shapes as JSON, e.g. \'{"xy_run": 32}\', set after any search (profiling runs)')
     ap.add_argument("--rocpd-summary", nargs="+", metavar="ARG",
                     help="DB TITLE CMD [SKIP_STEPS]: print the markdown kernel table of a rocprofv3 kernel-trace "
                          "database (steady steps only with SKIP_STEPS) and exit")
@@ -480,15 +490,18 @@ def main():
     pi = load_pkg()
     if args.no_plain:
         out = {"ensemble": None if args.no_ensemble else ensemble_leg(pi, members=args.ens_members,
-                                                                      batched_only=args.ens_batched_only),
+                                                                      batched_only=args.ens_batched_only,
+                                                                      tune=not args.no_tune),
                "serving": None if args.no_serve else serving_leg(pi, rate=args.serve_rate, window=args.serve_window)}
         print(json.dumps(out), flush=True)
         return
     if args.workload == "large":
         if args.pmc_probe:
-            large_leg(pi, steps=args.steps, warmup=args.warmup, probe=True)
+            large_leg(pi, steps=args.steps, warmup=args.warmup, probe=True, shapes=parse_shapes(args.shapes))
             return
-        print(json.dumps({"large_instance": large_leg(pi, steps=args.steps, warmup=args.warmup)}), flush=True)
+        print(json.dumps({"large_instance": large_leg(pi, steps=args.steps, warmup=args.warmup,
+                                                      tune=not args.no_tune, shapes=parse_shapes(args.shapes))}),
+              flush=True)
         return
 
     t0 = time.perf_counter()
@@ -496,14 +509,21 @@ def main():
                                              cap=2 * args.individuals, seed=2)
     g = spec.grid
     log(f"[bench] workload built in {time.perf_counter() - t0:.1f} s")
-    model = pi.PlanktonModel(pi.HIPArch(0), spec=spec)
-    nat = model._native
     dt = 60.0
     parf = np.full((g.Nx, g.Ny), 1823.7921666666664, np.float32)      # default_PARF, 11:00-12:00
     temp = pi.default_temperature(g, dt, 1)[..., 12].astype(np.float32)
+    tuned = None
+    if not args.no_tune and not args.pmc_probe:
+        tuned = tune_clone(pi, spec, (u, v, w), parf, temp, dt)
+    model = pi.PlanktonModel(pi.HIPArch(0), spec=spec)
+    nat = model._native
     nat.set_velocity(u, v, w)
     nat.set_parf(parf)
     nat.set_temperature(temp)
+    if tuned:
+        nat.set_shape(**tuned["shapes"])
+    if args.shapes:
+        nat.set_shape(**parse_shapes(args.shapes))
     log(f"[bench] model resident on the GPU after {time.perf_counter() - t0:.1f} s")
 
     if args.pmc_probe:
@@ -572,13 +592,14 @@ def main():
     bstep = step_bytes(live, n_cells, n_slots)
     step_roof = {"bytes_per_step": bstep, "achieved": bstep / (ms_step * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
                  "unit": "GB/s", "frac": bstep / (ms_step * 1e-3) / HBM_PEAK}
+    shapes_used = nat.shape()
     del model, nat
     large = None
     if not args.no_large:
-        large = large_leg(pi)
+        large = large_leg(pi, tune=not args.no_tune)
     ens = None
     if not args.no_ensemble:
-        ens = ensemble_leg(pi, members=args.ens_members)
+        ens = ensemble_leg(pi, members=args.ens_members, tune=not args.no_tune)
     serve = None
     if not args.no_serve:
         serve = serving_leg(pi, rate=args.serve_rate, window=args.serve_window)
@@ -607,6 +628,9 @@ def main():
                    "individuals_per_species": args.individuals, "dt": dt},
         "tracer_cell_steps_per_s": cell_steps,
         "live_individuals": [live0, live1],
+        "launch_shapes": shapes_used,
+        "autotune": None if tuned is None else {"trials_ms_per_step": tuned["trials"], "seconds": tuned["seconds"],
+                                                "searched_on": "a second instance of the same workload"},
         "roofline": roof,
         "issue_roofline": issue,
         "step_roofline": step_roof,
@@ -639,7 +663,32 @@ def logical_table(kt, nkt, live, n_cells, n_slots):
     return logical
 
 
-def large_leg(pi, steps=200, warmup=2, probe=False):
+def tune_clone(pi, spec, uvw, parf, temp, dt, warmup=3, steps=10, reps=3):
+    """Launch shapes for the workload, searched on a second instance of the same
+    spec (same initial state and inputs) so the measured model's state does not
+    advance: autotune.tune_model after `warmup` steps.  Returns its report."""
+    t0 = time.perf_counter()
+    clone = pi.PlanktonModel(pi.HIPArch(0), spec=spec)._native
+    try:
+        clone.set_velocity(*uvw)
+        clone.set_parf(parf)
+        clone.set_temperature(temp)
+        clone.run(dt, warmup)
+        clone.sync()
+        res = pi.autotune.tune_model(clone, dt, steps=steps, reps=reps, log=log)
+    finally:
+        clone.close()
+    res["seconds"] = time.perf_counter() - t0
+    log(f"[bench] launch shapes {res['shapes']} (searched in {res['seconds']:.1f} s)")
+    return res
+
+
+def parse_shapes(text):
+    """--shapes: JSON {name: value} of launch shapes, or None."""
+    return json.loads(text) if text else None
+
+
+def large_leg(pi, steps=200, warmup=2, probe=False, tune=True, shapes=None):
     """The large single instance: 512 x 512 x 128 (periodic x/y, bounded z),
     4 species x 8,388,608 individuals (capacity 2 x that), a 128-mode Fourier
     flow, dt = 30 s, 200 steps; initial state drawn on the device.  Steps per
@@ -650,12 +699,19 @@ def large_leg(pi, steps=200, warmup=2, probe=False):
     spec, (u, v, w) = pi.workloads.config_3d(n=512, nz=128, N=N, nsp=nsp, cap=2 * N, seed=3, nmodes=128,
                                              host_state=False)
     g = spec.grid
+    dt = 30.0
+    parf = np.full((g.Nx, g.Ny), 1823.7921666666664, np.float32)
+    temp = pi.default_temperature(g, dt, 1)[..., 12].astype(np.float32)
+    tuned = tune_clone(pi, spec, (u, v, w), parf, temp, dt, steps=4, reps=2) if tune and not probe else None
     model = pi.PlanktonModel(pi.HIPArch(0), spec=spec)
     nat = model._native
-    dt = 30.0
     nat.set_velocity(u, v, w)
-    nat.set_parf(np.full((g.Nx, g.Ny), 1823.7921666666664, np.float32))
-    nat.set_temperature(pi.default_temperature(g, dt, 1)[..., 12].astype(np.float32))
+    nat.set_parf(parf)
+    nat.set_temperature(temp)
+    if tuned:
+        nat.set_shape(**tuned["shapes"])
+    if shapes:
+        nat.set_shape(**shapes)
     log(f"[bench] large instance resident on the GPU after {time.perf_counter() - t0:.1f} s")
     if probe:
         probe_copy(pi)
@@ -689,6 +745,8 @@ def large_leg(pi, steps=200, warmup=2, probe=False):
            "step_roofline": {"bytes_per_step": bstep, "achieved": bstep / (ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
                              "unit": "GB/s", "frac": bstep / (ms * 1e-3) / HBM_PEAK},
            "logical_kernels": logical,
+           "launch_shapes": nat.shape(),
+           "autotune": None if tuned is None else {"trials_ms_per_step": tuned["trials"], "seconds": tuned["seconds"]},
            "kernels": {name: {"ms_per_launch": t / max(cnt, 1), "launches": cnt} for name, (t, cnt) in kt.items()}}
     log(f"[bench] large instance: {steps} steps in {wall:.3f} s ({ms:.3f} ms/step), "
         f"{out['individual_steps_per_s']:.4g} individual-steps/s")

@@ -198,6 +198,24 @@ pi_status pi_individual_steps(pi_model* m, int64_t* n);
 pi_status pi_enable_timing(pi_model* m, int32_t on);
 pi_status pi_kernel_times(pi_model* m, int32_t* n, const char** names, float* ms, int32_t* counts, int32_t cap);
 
+/* Launch shapes: how the work of a step is split over workgroups.  No
+ * counterpart in the reference (KernelAbstractions picks its own workgroup
+ * sizes).  Every accepted value gives the same results bit for bit; only the
+ * speed changes, so a search by timing steps (the package's autotune) may set
+ * them per model.  A model in a batch keeps its own values: run lengths apply
+ * per member; block counts are those of its share of a batched launch. */
+enum {
+    PI_S_XY_RUN = 0,       /* rows of one level per work item of the fused x/y sweep (1..1024; 16) */
+    PI_S_FINAL_RUN,        /* levels per column run of the z sweep + update (1..4096; 16) */
+    PI_S_ADV_BLOCKS,       /* workgroups of the advection launch, at most (1..2^24; 16384) */
+    PI_S_PHYS_BLOCKS,      /* workgroups of the physiology launch, at most (1..2^24; 4096) */
+    PI_S_CELLSUM_PER_CU,   /* workgroups per compute unit of the per-cell Chl/pop sums (1..64; 8) */
+    PI_S_PLK_BLOCKS,       /* workgroups of the per-cell exchange sums, at most (1..2^24; 2^22) */
+    PI_NSHAPE
+};
+pi_status pi_set_launch_shape(pi_model* m, int32_t which, int64_t value);
+pi_status pi_get_launch_shape(const pi_model* m, int32_t which, int64_t* value);
+
 /* Device memory.  pi_footprint: the bytes a model with this descriptor would
  * allocate (a dry run of pi_model_create that touches no device);
  * pi_model_footprint: the bytes a model holds.  No counterpart in the

@@ -19,6 +19,7 @@ from .simulation import (PlanktonSimulation, PlanktonDiagnostics, update, defaul
 from ._native import CapacityError, NativeError, load_library, lib_path
 from .ensemble import Ensemble
 from . import workloads
+from . import autotune
 
 update_ = update          # `update!`
 TimeStep_ = TimeStep      # `TimeStep!`

@@ -34,6 +34,9 @@ GRID_ID = {"par": 0, "par₀": 1, "par0": 1, "Chl": 2, "pop": 3, "temp": 4, "PAR
            "u": 6, "v": 7, "w": 8, "diag_PAR": 9}
 PLK_BASE = 10
 DIAG_ID = {"num": 0, "graz": 1, "mort": 2, "dvid": 3}
+# launch shapes (pi_set_launch_shape): name -> (id, default)
+SHAPES = {"xy_run": (0, 16), "final_run": (1, 16), "adv_blocks": (2, 16384), "phys_blocks": (3, 4096),
+          "cellsum_per_cu": (4, 8), "plk_blocks": (5, 1 << 22)}
 
 fp = C.POINTER(C.c_float)
 i64p = C.POINTER(C.c_int64)

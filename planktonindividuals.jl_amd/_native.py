@@ -63,6 +63,10 @@ def load_library():
     lib.pi_footprint.argtypes = [C.POINTER(_abi.ModelDesc), C.POINTER(C.c_int64)]
     lib.pi_model_footprint.restype = C.c_int
     lib.pi_model_footprint.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    lib.pi_set_launch_shape.restype = C.c_int
+    lib.pi_set_launch_shape.argtypes = [C.c_void_p, C.c_int32, C.c_int64]
+    lib.pi_get_launch_shape.restype = C.c_int
+    lib.pi_get_launch_shape.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]
     for f, a in (("create", [C.POINTER(C.c_void_p)]), ("add", [C.c_void_p, C.c_void_p]),
                  ("remove", [C.c_void_p, C.c_void_p]), ("run", [C.c_void_p, C.c_float, C.c_int64]),
                  ("run_noalloc", [C.c_void_p, C.c_float, C.c_int64]),
@@ -250,6 +254,21 @@ class NativeModel(Handle):
         self._call("model_footprint", C.byref(v))
         return v.value
 
+    def set_shape(self, **values):
+        """pi_set_launch_shape: set launch shapes by name (_abi.SHAPES); the results
+        do not depend on them."""
+        for name, v in values.items():
+            self._call("set_launch_shape", _abi.SHAPES[name][0], C.c_int64(int(v)))
+
+    def shape(self):
+        """{name: value} of every launch shape."""
+        out = {}
+        for name, (i, _) in _abi.SHAPES.items():
+            v = C.c_int64(0)
+            self._call("get_launch_shape", i, C.byref(v))
+            out[name] = v.value
+        return out
+
     def close(self):
         if self.batch is not None:
             self.batch.remove(self)
@@ -291,6 +310,7 @@ class NativeBatch:
         check(self.lib, "pi_", self.lib.pi_batch_create(C.byref(raw)))
         self.raw = raw
         self.members = []
+        self.budget = 16384                   # the library's default block budget
 
     def add(self, model):
         check(self.lib, "pi_", self.lib.pi_batch_add(self.raw, model.raw))
@@ -315,23 +335,14 @@ class NativeBatch:
 
     def set_budget(self, blocks):
         check(self.lib, "pi_", self.lib.pi_batch_set_budget(self.raw, C.c_int64(int(blocks))))
+        self.budget = int(blocks)
 
-    def autotune(self, dt, candidates=(4096, 8192, 16384, 32768, 65536), steps=3):
-        """Block budget of the batched launches chosen by timing `steps` steps
-        under each candidate (the steps are real steps of every member; any
-        budget gives the same results).  Returns {budget: ms per step}."""
-        import time
-        res = {}
-        for b in candidates:
-            self.set_budget(b)
-            self.run(dt, 1)
-            self.sync()
-            t0 = time.perf_counter()
-            self.run(dt, steps)
-            self.sync()
-            res[b] = (time.perf_counter() - t0) * 1e3 / steps
-        self.set_budget(min(res, key=res.get))
-        return res
+    def autotune(self, dt, steps=4, reps=3, shapes=None, log=None):
+        """The block budget and the launch shapes of every member-shape class,
+        chosen by timing real steps of every member (autotune.tune_batch; no
+        choice changes any result).  Returns its report."""
+        from .autotune import tune_batch
+        return tune_batch(self, dt, steps=steps, reps=reps, shapes=shapes, log=log)
 
     def graphs(self, on):
         check(self.lib, "pi_", self.lib.pi_batch_graphs(self.raw, 1 if on else 0))

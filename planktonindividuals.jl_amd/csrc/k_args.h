@@ -84,6 +84,7 @@ struct XYArgs {
     const float* v;                   /* vel₁ v */
     float* out;                       /* slab: temp after the y sweep (z halos of interior lines filled) */
     float dt;
+    int run;                          /* rows of one level per work item (launch shape, any value >= 1) */
 };
 
 struct FillArgs {
@@ -105,6 +106,7 @@ struct FinalArgs {
     float kDOC, kPOC, kDON, kPON, kDOP, kPOP, Nit;
     float lPOC, lmin, ldust, DFeFrac, kdiss, lig, lFe;
     int fill_halo;                    /* also write the halo cells of the new tracers (halo 2, no aliasing) */
+    int run;                          /* levels per column run (launch shape, any value >= 1) */
 };
 
 struct ScanIn {
@@ -197,10 +199,10 @@ void reg_util(KInfo* k);
 /* grid sizes of the kernels whose work is one item per thread */
 long long sweep_x_blocks(const DevGrid& g);
 long long sweep_y_items(const DevGrid& g);
-long long final_items(const DevGrid& g);
+long long final_items(const DevGrid& g, int run);
 bool xy_supported(const DevGrid& g);   /* k_xy handles this grid */
 bool xy_wide(const DevGrid& g);        /* with the wide blocks (k_xyw) */
-long long xy_blocks(const DevGrid& g);
+long long xy_blocks(const DevGrid& g, int run);
 uint32_t scan_tiles(uint32_t n);
 
 /* initial state generated on the device (initialize_plankton!,

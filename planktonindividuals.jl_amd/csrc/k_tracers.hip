@@ -367,7 +367,6 @@ __device__ __forceinline__ int halo_src(int p, int H, int N, int topo)
 }
 
 /* ---------------------------------------------- z sweep + the rest of the update */
-#define RUN_Z 16
 
 /* the tracers that receive particle exchange (plk), tracer_update.jl:22-43 */
 __device__ __forceinline__ constexpr bool has_plk(int q)
@@ -579,8 +578,8 @@ __device__ __forceinline__ void final_run(const FinalArgs& a, const DevGrid& g, 
     const float dt = a.dt;
     const float Az = g.dx * g.dy;
     const float kvAz = a.kv * Az;
-    const int k0 = g.Hz + 1 + r * RUN_Z;
-    const int k1 = min(k0 + RUN_Z, g.Nz + g.Hz + 1);     /* cells k0..k1-1, faces k0..k1 */
+    const int k0 = g.Hz + 1 + r * a.run;
+    const int k1 = min(k0 + a.run, g.Nz + g.Hz + 1);     /* cells k0..k1-1, faces k0..k1 */
     uint32_t c = (uint32_t)gidx(g, i, j, k0);            /* cell below the current face */
     float cu[FG_MAX], cs[3] = {0.0f, 0.0f, 0.0f}, fz[FG_MAX], dz[FG_MAX];
     HaloImg hi;
@@ -649,7 +648,7 @@ __device__ __forceinline__ void k_final_body(const FinalArgs& a, Blk K)
     const DevState* S = a.st;
     if (S->err) return;
     const DevGrid& g = a.g;
-    const int nrun = (g.Nz + RUN_Z - 1) / RUN_Z;
+    const int nrun = (g.Nz + a.run - 1) / a.run;
     const long long nthr = (long long)g.Nx * g.Ny * nrun * FG_NGRP;
     const float* C0 = S->tr[S->tcur][0];
     float* OUT = S->tr[S->tcur ^ 1][0];
@@ -679,7 +678,7 @@ PI_KERNEL2(k_final1, k_final_body<true>, FinalArgs, 256)
  * threads up to Nx = XY_T, of XY_W threads above).
  *
  * A group of roundup64(Nx) threads owns one x-row (thread = cell i) and walks
- * a run of RUN_XY rows of one level in y.  Each row step computes the x sweep
+ * a run of rows (XYArgs::run) of one level in y.  Each row step computes the x sweep
  * of one row: every thread the flux of its west face; the east face comes
  * from the next thread through LDS (one barrier per row, double-buffered); the
  * east face of the last cell is the first face's flux when x is Periodic
@@ -690,7 +689,6 @@ PI_KERNEL2(k_final1, k_final_body<true>, FinalArgs, 256)
  * Rows outside the interior are the reference's halo fill of the x-swept
  * field: the x sweep of the row they copy (periodic image or Bounded edge),
  * recomputed bit for bit.  Tracers are processed in two passes of 7. */
-#define RUN_XY 16
 #define XY_T 256
 #define XY_W 512
 
@@ -731,7 +729,7 @@ struct XYShared {
 };
 
 /* One pass (tracers Q0..Q1-1) over a run of rows of one level.  Row step n
- * (n = 0 .. RUN_XY+3) is the x sweep of row jr = j0 - 2 + n.  Tracer rows are
+ * (n = 0 .. run+3) is the x sweep of row jr = j0 - 2 + n.  Tracer rows are
  * loaded straight into LDS (global_load_lds: no registers) one row step ahead,
  * each cell once: the x stencil reads its neighbours from LDS.  In the interval
  * between two barriers a thread finishes row n-1 (east face from LDS, the y
@@ -744,7 +742,7 @@ __device__ __forceinline__ void xy_run(const XYArgs& a, const DevGrid& g, const 
                                        XYShared<TB>& sh, int t, int gbase, int i, bool xin, bool act, int j0, int k)
 {
     constexpr int NQ = Q1 - Q0;
-    constexpr int NS = RUN_XY + 4;
+    const int NS = a.run + 4;
     const uint32_t sy = (uint32_t)g.Px, nc = (uint32_t)g.ncell;
     const PI_GLOBAL float* U = gp(a.u);
     const PI_GLOBAL float* V = gp(a.v);
@@ -822,7 +820,7 @@ __device__ __forceinline__ void xy_run(const XYArgs& a, const DevGrid& g, const 
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) G[q] = zero ? 0.0f : G[q];
                 const int jo = f - 1;                             /* row updated by faces f-1 and f */
-                if (s >= 4 && act && xin && jo < min(j0 + RUN_XY, g.Ny)) {
+                if (s >= 4 && act && xin && jo < min(j0 + a.run, g.Ny)) {
                     const uint32_t co = cellof(jo), cn = cellof(f);
                     const float dTy = lmask<LAND>(g, co, cn, Fy.T - Tp);
                     /* z halo images of this cell on its (interior) line: Bounded z copies the edge level */
@@ -898,7 +896,7 @@ __device__ __forceinline__ void k_xy_body(const XYArgs& a, Blk K)
     const int x = t - gbase;
     const bool xin = grp < G && x < g.Nx;
     const int i = min(x, g.Nx - 1);
-    const int nrun = (g.Ny + RUN_XY - 1) / RUN_XY;
+    const int nrun = (g.Ny + a.run - 1) / a.run;
     const uint32_t nitem = (uint32_t)nrun * (uint32_t)g.Nz;
     const PI_GLOBAL float* C0 = gp((const float*)S->tr[S->tcur][0]);
     PI_GLOBAL float* OUT = gp(a.out);
@@ -907,8 +905,8 @@ __device__ __forceinline__ void k_xy_body(const XYArgs& a, Blk K)
         const bool act = grp < G && item < nitem;
         const uint32_t it = act ? item : base;
         const int r = (int)(it % (uint32_t)nrun), k = (int)(it / (uint32_t)nrun);
-        xy_run<LAND, 0, 7>(a, g, C0, OUT, sh, t, gbase, i, xin, act, r * RUN_XY, k);
-        xy_run<LAND, 7, 14>(a, g, C0, OUT, sh, t, gbase, i, xin, act, r * RUN_XY, k);
+        xy_run<LAND, 0, 7>(a, g, C0, OUT, sh, t, gbase, i, xin, act, r * a.run, k);
+        xy_run<LAND, 7, 14>(a, g, C0, OUT, sh, t, gbase, i, xin, act, r * a.run, k);
     }
 }
 __device__ __forceinline__ void k_xy0_body(const XYArgs& a, Blk K) { k_xy_body<false, XY_T>(a, K); }
@@ -926,11 +924,11 @@ bool xy_supported(const DevGrid& g)
            !(g.tx == PI_PERIODIC && g.Nx < g.Hx) && !(g.ty == PI_PERIODIC && g.Ny < g.Hy);
 }
 bool xy_wide(const DevGrid& g) { return g.Nx > XY_T; }
-long long xy_blocks(const DevGrid& g)
+long long xy_blocks(const DevGrid& g, int run)
 {
     const int nxr = (g.Nx + 63) & ~63;
     const long long G = (xy_wide(g) ? XY_W : XY_T) / nxr;
-    const long long nitem = (long long)((g.Ny + RUN_XY - 1) / RUN_XY) * g.Nz;
+    const long long nitem = (long long)((g.Ny + run - 1) / run) * g.Nz;
     return (nitem + G - 1) / G;
 }
 
@@ -942,7 +940,7 @@ long long sweep_x_blocks(const DevGrid& g)
     return (nwave + 3) / 4;
 }
 long long sweep_y_items(const DevGrid& g) { return (long long)g.Nx * ((g.Ny + RUN_Y - 1) / RUN_Y) * g.Nz; }
-long long final_items(const DevGrid& g) { return (long long)g.Nx * g.Ny * ((g.Nz + RUN_Z - 1) / RUN_Z) * FG_NGRP; }
+long long final_items(const DevGrid& g, int run) { return (long long)g.Nx * g.Ny * ((g.Nz + run - 1) / run) * FG_NGRP; }
 
 void reg_tracers(KInfo* k)
 {

@@ -134,6 +134,7 @@ struct pi_model {
     hipGraphExec_t gexec = nullptr;
     float gdt = 0.0f;
     size_t tcount = 0;
+    long long shape[PI_NSHAPE] = {16, 16, 16384, 4096, 8, 1 << 22};   /* pi_set_launch_shape */
 };
 
 static hipError_t dalloc(pi_model* m, void** p, size_t bytes)
@@ -481,6 +482,28 @@ pi_status pi_model_destroy(pi_model* m)
     return PI_OK;
 }
 
+pi_status pi_set_launch_shape(pi_model* m, int32_t which, int64_t value)
+{
+    static const long long lo[PI_NSHAPE] = {1, 1, 1, 1, 1, 1};
+    static const long long hi[PI_NSHAPE] = {1024, 4096, 1 << 24, 1 << 24, 64, 1 << 24};
+    if (!m) return fail(PI_ERR_ARG, "null model");
+    if (which < 0 || which >= PI_NSHAPE) return fail(PI_ERR_ARG, "launch shape: unknown parameter");
+    if (value < lo[which] || value > hi[which]) return fail(PI_ERR_ARG, "launch shape: value out of range");
+    if (m->shape[which] == value) return PI_OK;
+    HIPCHECK(hipStreamSynchronize(m->stream));      /* a replayed step may still run */
+    drop_graph(m);
+    m->shape[which] = value;
+    return PI_OK;
+}
+
+pi_status pi_get_launch_shape(const pi_model* m, int32_t which, int64_t* value)
+{
+    if (!m || !value) return fail(PI_ERR_ARG, "null argument");
+    if (which < 0 || which >= PI_NSHAPE) return fail(PI_ERR_ARG, "launch shape: unknown parameter");
+    *value = m->shape[which];
+    return PI_OK;
+}
+
 pi_status pi_set_diag_tracers(pi_model* m, int32_t n, const int32_t* ids)
 {
     if (!m) return fail(PI_ERR_ARG, "null model");
@@ -717,7 +740,8 @@ static void build_step(pi_model* m, float dt, Plan& P)
         {
             XYArgs xa = zeroed<XYArgs>();
             xa.g = g; xa.st = st; xa.dt = dt; xa.u = V1.u; xa.v = V1.v; xa.out = m->T2[0];
-            const long long xyb = fxy ? xy_blocks(g) : 0;
+            xa.run = (int)m->shape[PI_S_XY_RUN];
+            const long long xyb = fxy ? xy_blocks(g, xa.run) : 0;
             const bool wide = xy_wide(g);
             P.add(K_XY0, "sweep_xy", land || wide ? 0 : xyb, xa);
             P.add(K_XY1, "sweep_xy", land && !wide ? xyb : 0, xa);
@@ -758,9 +782,9 @@ static void build_step(pi_model* m, float dt, Plan& P)
         aa.seed = m->seed;
         aa.newc = m->newc; aa.stayf = m->stayf; aa.small = m->small; aa.qcap = m->qcap; aa.cnts = m->cnts;
         aa.dstamp = m->dstamp; aa.dcell = m->dcell; aa.cap = m->cap;
-        /* up to 16384 blocks: with 8.4M individuals (two entries per thread)
-         * ~5% faster than 4096 blocks; 32768 is no faster, 65536 much slower */
-        P.add(K_ADV, "adv", grid_for(2 * m->ctot, 256, 16384), aa);
+        /* up to 16384 blocks by default: with 8.4M individuals (two entries per
+         * thread) ~5% faster than 4096 blocks; 32768 is no faster, 65536 much slower */
+        P.add(K_ADV, "adv", grid_for(2 * m->ctot, 256, (int)m->shape[PI_S_ADV_BLOCKS]), aa);
     }
     {
         ScanIn sp = scan_plain(m->stayf, (uint32_t)(2 * m->ctot));      /* stayer prefix over the sorted part */
@@ -778,8 +802,9 @@ static void build_step(pi_model* m, float dt, Plan& P)
         P.add(K_PLACE_SMALL, "bin", pb, b);
         P.add(K_MIXSORT, "bin", cb, b);
         P.add(K_BIGCELLS, "bin", 256, b);
-        /* one resident round (8 blocks per CU at its register use): ~5 us faster than 8192 blocks */
-        P.add(K_CELLSUM, "bin", std::min<long long>(grid_for(m->ncell, 256, 8192), 8ll * m->ncu), b);
+        /* one resident round by default (8 blocks per CU at its register use): ~5 us
+         * faster than 8192 blocks */
+        P.add(K_CELLSUM, "bin", std::min<long long>(grid_for(m->ncell, 256, 8192), m->shape[PI_S_CELLSUM_PER_CU] * m->ncu), b);
         P.add(K_CELLSUM_LONG, "bin", m->long_blocks, b);
     }
     {
@@ -799,7 +824,7 @@ static void build_step(pi_model* m, float dt, Plan& P)
         pa.newc = m->newc; pa.sflag = m->sflag;
         pa.con = m->con; pa.diag_num = m->diag; pa.dt = dt; pa.seed = m->seed;
         pa.dstamp = m->dstamp; pa.dcell = m->dcell; pa.rstamp = m->rstamp; pa.rcons = m->rcons;
-        P.add(K_PHYS, "phys", pb, pa);
+        P.add(K_PHYS, "phys", grid_for(2 * m->ctot, 256, (int)m->shape[PI_S_PHYS_BLOCKS]), pa);
     }
     {
         PlkArgs ka = zeroed<PlkArgs>();
@@ -808,9 +833,9 @@ static void build_step(pi_model* m, float dt, Plan& P)
         for (int q = 0; q < PI_NTRACERS; ++q) ka.plk[q] = m->plk[q] ? m->plk[q] : m->plk[PI_DIC];
         ka.lng = m->lng; ka.lg = m->skey; ka.lm = m->ssrc;
         ka.segp = m->segp;                 /* k_plk copies seg -> segp (the binning above has read segp) */
-        /* one cell per thread: with a loop over cells each iteration's loads
-         * would wait for the previous iteration's stores (one counter for both) */
-        P.add(K_PLK, "plk", grid_for(m->ncell, 256, 1 << 22), ka);
+        /* one cell per thread by default: with a loop over cells each iteration's
+         * loads wait for the previous iteration's stores (one counter for both) */
+        P.add(K_PLK, "plk", grid_for(m->ncell, 256, (int)m->shape[PI_S_PLK_BLOCKS]), ka);
         P.add(K_PLK_LONG, "plk", m->long_blocks, ka);
     }
     P.mark(K_JOIN);
@@ -829,7 +854,8 @@ static void build_step(pi_model* m, float dt, Plan& P)
         fn.DFeFrac = m->bgc[PI_B_DFeFrac]; fn.kdiss = m->bgc[PI_B_kdiss]; fn.lig = m->bgc[PI_B_ligand];
         fn.lFe = m->bgc[PI_B_lambda_Fe];
         fn.fill_halo = fxy ? 1 : 0;       /* k_final writes the halo images itself */
-        const long long fb = grid_for(final_items(g), 256, 8192);
+        fn.run = (int)m->shape[PI_S_FINAL_RUN];
+        const long long fb = grid_for(final_items(g, fn.run), 256, 8192);
         P.add(K_FINAL0, "final", land ? 0 : fb, fn);
         P.add(K_FINAL1, "final", land ? fb : 0, fn);
         /* fill_halo_tracer! of the new tracers: x, then y, then z over whole planes */

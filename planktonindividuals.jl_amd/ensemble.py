@@ -63,24 +63,30 @@ class Ensemble:
             if self.check_every and self.done[self.sims[0]] % self.check_every == 0:
                 self.check()
 
-    def autotune(self, candidates=(4096, 8192, 16384, 32768, 65536), steps=3):
-        """Picks the block budget of the batched launches by timing `steps`
-        real steps of the ensemble under each candidate (the steps count: the
-        members advance; every candidate gives the same results).  Returns
-        {budget: ms per step} and keeps the fastest."""
-        import time
-        res = {}
-        for b in candidates:
-            self.batch.set_budget(b)
-            self.step(1)                     # tables rebuilt for the new budget
-            self.batch.sync()
-            t0 = time.perf_counter()
-            self.step(steps)
-            self.batch.sync()
-            res[b] = (time.perf_counter() - t0) * 1e3 / steps
-        best = min(res, key=res.get)
-        self.batch.set_budget(best)
-        self.budget = best
+    def autotune(self, steps=4, reps=3, shapes=None, log=None):
+        """Searches the batch's block budget and the launch shapes of every
+        member-shape class by timing real steps of the ensemble (the members
+        advance; no choice changes any result): autotune.tune_batch.  The
+        steps are taken with the inputs staged as `step` stages them."""
+        from .autotune import tune_batch
+
+        class _Runner:
+            members = self.batch.members
+
+            def __init__(r):
+                r.budget = self.batch.budget
+
+            def set_budget(r, b):
+                self.batch.set_budget(b)
+
+            def run(r, dt, k):
+                self.step(k)
+
+            def sync(r):
+                self.batch.sync()
+
+        res = tune_batch(_Runner(), self.ΔT, steps=steps, reps=reps, shapes=shapes, log=log)
+        self.budget = res["budget"]
         return res
 
     def check(self):

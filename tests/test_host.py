@@ -37,7 +37,8 @@ def test_oracle_exports_mirror_abi(orc):
     for s in declared_symbols():
         name = "orc_" + s[3:]
         if s in ("pi_init", "pi_version", "pi_model_create", "pi_run", "pi_sync", "pi_enable_timing",
-                 "pi_kernel_times", "pi_probe_copy", "pi_footprint", "pi_model_footprint", "pi_generate_state"):
+                 "pi_kernel_times", "pi_probe_copy", "pi_footprint", "pi_model_footprint", "pi_generate_state",
+                     "pi_set_launch_shape", "pi_get_launch_shape"):
             continue
         if s.startswith("pi_batch_"):       # batches of GPU models: no CPU counterpart
             continue
