@@ -667,8 +667,9 @@ __device__ __forceinline__ void k_final_body(const FinalArgs& a, Blk K)
         }
     }
 }
-PI_KERNEL2(k_final0, k_final_body<false>, FinalArgs, 256)
-PI_KERNEL2(k_final1, k_final_body<true>, FinalArgs, 256)
+/* at least 5 waves per SIMD: 0.398 -> 0.383 ms on the 256x256x64 run (4 waves at 108 VGPRs) */
+PI_KERNEL3W(k_final0, k_final_body<false>, FinalArgs, 256, xcd_block(), 5)
+PI_KERNEL3W(k_final1, k_final_body<true>, FinalArgs, 256, xcd_block(), 5)
 
 /* ------------------------------------------------- fused x + y sweeps (k_xy)
  * calc_Gcsˣ! + multi_dim_x! + fill_halo_tracer! + calc_Gcsʸ! + multi_dim_y!
@@ -913,10 +914,12 @@ __device__ __forceinline__ void k_xy0_body(const XYArgs& a, Blk K) { k_xy_body<f
 __device__ __forceinline__ void k_xy1_body(const XYArgs& a, Blk K) { k_xy_body<true, XY_T>(a, K); }
 __device__ __forceinline__ void k_xyw0_body(const XYArgs& a, Blk K) { k_xy_body<false, XY_W>(a, K); }
 __device__ __forceinline__ void k_xyw1_body(const XYArgs& a, Blk K) { k_xy_body<true, XY_W>(a, K); }
-PI_KERNEL_XCD(k_xy0, XYArgs, XY_T)
-PI_KERNEL_XCD(k_xy1, XYArgs, XY_T)
-PI_KERNEL_XCD(k_xyw0, XYArgs, XY_W)
-PI_KERNEL_XCD(k_xyw1, XYArgs, XY_W)
+/* at least 4 waves per SIMD (128 VGPRs instead of 130-134, which allowed 3):
+ * 0.460 -> 0.327 ms on the 256x256x64 run */
+PI_KERNEL3W(k_xy0, k_xy0_body, XYArgs, XY_T, xcd_block(), 4)
+PI_KERNEL3W(k_xy1, k_xy1_body, XYArgs, XY_T, xcd_block(), 4)
+PI_KERNEL3W(k_xyw0, k_xyw0_body, XYArgs, XY_W, xcd_block(), 4)
+PI_KERNEL3W(k_xyw1, k_xyw1_body, XYArgs, XY_W, xcd_block(), 4)
 
 bool xy_supported(const DevGrid& g)
 {

@@ -205,6 +205,17 @@ struct Blk { uint32_t b, n; };
         const uint4 e = bt[blockIdx.x];                                                               \
         BODY(A[e.x], Blk{e.y, e.z});                                                                  \
     }
+/* PI_KERNEL3 with at least WAVES waves per SIMD (the register budget the
+ * compiler must fit: 512 / WAVES VGPRs) */
+#define PI_KERNEL3W(NAME, BODY, ARGS, THREADS, BIDX, WAVES)                                              \
+    __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WAVES)))             \
+    NAME(ARGS a) { BODY(a, Blk{BIDX, gridDim.x}); }                                                     \
+    __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WAVES)))             \
+    NAME##_m(const ARGS* __restrict__ A, const uint4* __restrict__ bt)                                  \
+    {                                                                                                   \
+        const uint4 e = bt[blockIdx.x];                                                                 \
+        BODY(A[e.x], Blk{e.y, e.z});                                                                    \
+    }
 #define PI_KERNEL2(NAME, BODY, ARGS, THREADS) PI_KERNEL3(NAME, BODY, ARGS, THREADS, xcd_block())
 #define PI_KERNEL(NAME, ARGS, THREADS) PI_KERNEL3(NAME, NAME##_body, ARGS, THREADS, blockIdx.x)
 #define PI_KERNEL_XCD(NAME, ARGS, THREADS) PI_KERNEL3(NAME, NAME##_body, ARGS, THREADS, xcd_block())
