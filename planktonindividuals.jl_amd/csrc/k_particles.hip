@@ -1091,8 +1091,14 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
         OUT.flags[e] = ofl;
         OUT.cell[e] = c;
         /* a dying individual leaves its byte as it was (GRAZ or MORT set: not
-         * alive for the marks); a daughter may be reborn into that slot */
-        if (ac) a.sflag[(long long)sp * a.cap + slot] = ofl;
+         * alive for the marks); a daughter may be reborn into that slot.  The
+         * byte of a living individual already holds its input flags (every
+         * writer of an entry's flags writes its slot's byte too), so it is
+         * written only when the decisions changed: one scattered byte store
+         * per slot and step cost ~0.14 ms per step at 8.4M individuals (a
+         * partial cache-line write-back each), the changes are a few percent */
+        if (ac && ofl != (uint8_t)(fl & (F_AC | F_GRAZ | F_MORT | F_DVID)))
+            a.sflag[(long long)sp * a.cap + slot] = ofl;
         a.con.v[0][e] = L0; a.con.v[1][e] = L1; a.con.v[2][e] = L2;
         a.con.v[3][e] = L3; a.con.v[4][e] = L4; a.con.v[5][e] = L5;
         a.con.tag[e] = tag;
