@@ -20,12 +20,14 @@ from ._native import CapacityError, NativeError, load_library, lib_path
 from .ensemble import Ensemble
 from . import workloads
 from . import autotune
+from .output import PlanktonOutputWriter, write_output
 
 update_ = update          # `update!`
 TimeStep_ = TimeStep      # `TimeStep!`
+write_output_ = write_output   # `write_output!`
 
 __all__ = ["RectilinearGrid", "Periodic", "Bounded", "QuotaMode", "PlanktonModel", "HIPArch", "GPU",
            "phyto_setup", "PlanktonSimulation", "PlanktonDiagnostics", "update", "TimeStep",
            "bgc_params_default", "phyt_params_default", "update_bgc_params", "update_phyt_params",
            "default_tracer_init", "default_PARF", "default_temperature", "CapacityError",
-           "TRACER_NAMES", "isfunctional", "Ensemble"]
+           "TRACER_NAMES", "isfunctional", "Ensemble", "PlanktonOutputWriter", "write_output"]

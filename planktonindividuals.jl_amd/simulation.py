@@ -7,6 +7,7 @@ import numpy as np
 
 from . import _abi
 from ._abi import BOUNDED
+from .output import write_output
 
 
 def default_PARF(grid, ΔT, iterations):
@@ -57,6 +58,11 @@ class PlanktonDiagnostics:
         self.model = model
         self.iteration_interval = int(iteration_interval)
         self.tracer_names = tuple(t for t in tracer if t != "PAR")
+        # key orders of merge(NamedTuple{tracer}, (PAR,)) and of
+        # merge(NamedTuple{phytoplankton}, (num, graz, mort, dvid, ptc))
+        self._tracer_keys = tuple(tracer) + (() if "PAR" in tracer else ("PAR",))
+        self._phyto_keys = tuple(phytoplankton) + tuple(
+            p for p in ("num", "graz", "mort", "dvid", "ptc") if p not in phytoplankton)
         model._native.set_diag_tracers([_abi.TRACER_ID[t] for t in self.tracer_names])
 
     @property
@@ -77,6 +83,14 @@ class PlanktonDiagnostics:
     def reset(self):
         self.model._native.reset_diags()
 
+    def tracer_keys(self):
+        """keys(diags.tracer), in the reference's order."""
+        return self._tracer_keys
+
+    def phyto_keys(self):
+        """keys(diags.phytos.spN), in the reference's order."""
+        return self._phyto_keys
+
 
 class PlanktonSimulation:
     """PlanktonSimulation(model; ΔT, iterations, PARF=nothing, temp=nothing, diags=nothing,
@@ -85,8 +99,7 @@ class PlanktonSimulation:
 
     def __init__(self, model, *, ΔT, iterations, PARF=None, temp=None, diags=None, vels=None,
                  ΔT_vel=None, ΔT_PAR=3600.0, ΔT_temp=3600.0, output_writer=None):
-        if output_writer is not None:
-            raise NotImplementedError("output writers are outside the MI355X hot path")
+        self.output_writer = output_writer
         self.model = model
         self.ΔT = np.float32(ΔT)
         self.iterations = int(iterations)
@@ -176,12 +189,15 @@ def update(sim, time_offset=None, check_every=1):
     off = time_offsets(time_offset)
     m = sim.model
     nat = m._native
+    writer = getattr(sim, "output_writer", None)
     for i in range(1, sim.iterations + 1):
         stage_inputs(sim, i, off)
         nat.step(np.float32(sim.ΔT))
         advance_clock(sim)
-        if check_every and (i % check_every == 0):
+        if (check_every and (i % check_every == 0)) or writer is not None:
             nat.sync()
+        if writer is not None:
+            write_output(writer, m, sim.diags)               # write_output! (update.jl:30, 45)
     nat.sync()
 
 
