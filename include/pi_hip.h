@@ -62,6 +62,20 @@ enum {
     PI_NPARAMS = 40
 };
 
+/* ---- physiology modes (src/PlanktonIndividuals.jl:65,71: CarbonMode, QuotaMode) */
+enum { PI_MODE_QUOTA = 0, PI_MODE_CARBON = 1 };
+
+/* ---- CarbonMode parameters: 24 floats in the reference's `param_names` order
+ *      (CarbonMode/plankton_generation.jl:14-16), in the first 24 of a species'
+ *      PI_NPARAMS floats (the rest unused). ------------------------------- */
+enum {
+    PI_C_Nsuper = 0, PI_C_Cquota, PI_C_mean, PI_C_var, PI_C_alpha, PI_C_Phi, PI_C_Topt, PI_C_Tmax,
+    PI_C_Ea, PI_C_PCmax, PI_C_Chl2C, PI_C_respir, PI_C_f_T2B, PI_C_grz_P, PI_C_dvid_type, PI_C_dvid_P,
+    PI_C_dvid_reg, PI_C_dvid_reg2, PI_C_mort_P, PI_C_mort_reg, PI_C_grazFracC, PI_C_mortFracC,
+    PI_C_thermal, PI_C_is_bact,
+    PI_NCPARAMS = 24
+};
+
 /* ---- biogeochemical parameters (Parameters/param_default.jl:45-71) ------- */
 enum {
     PI_B_kw = 0, PI_B_kc, PI_B_kDOC, PI_B_Nit, PI_B_kDON, PI_B_kDOP, PI_B_kPOC,
@@ -80,6 +94,9 @@ enum {
     PI_F_ac,                                                         /* uint8   */
     PI_NFIELDS
 };
+/* CarbonMode (CarbonMode/plankton_generation.jl:1-12): the damaged biomass Bd
+ * is the column PI_F_Cq; its Nq and Pq columns are zero. */
+#define PI_F_Bd PI_F_Cq
 
 /* ---- grid-shaped outputs (timestepper + diagnostics) --------------------- */
 enum {
@@ -116,6 +133,7 @@ typedef struct pi_model_desc {
     uint64_t rng_seed;           /* key of the shared uniform stream (pi_math.h) */
     int32_t n_diag_tracers;      /* tracers accumulated into diags.tracer besides PAR */
     const int32_t* diag_tracers; /* tracer ids, may be NULL when n_diag_tracers == 0 */
+    int32_t mode;                /* PI_MODE_QUOTA or PI_MODE_CARBON (PlanktonModel(...; mode)) */
 } pi_model_desc;
 
 /* Host-side view of one species' individuals, all arrays of length capacity.

@@ -1,7 +1,8 @@
 """TEST INFRASTRUCTURE: checks that the oracle's pin tests can fail.
 
 Each entry below plants one plausible mistake in a copy of oracle/pi_oracle.cpp
-(a wrong index, sign, exponent, dropped term or halo plane), builds the copy
+(a wrong index, sign, exponent, dropped term or halo plane, in QuotaMode and
+CarbonMode), builds the copy
 into a temporary library, and runs the CPU pin tests against it
 (PI_ORACLE_LIB).  Every mistake must make at least one pin fail.  The
 committed source is never modified.
@@ -69,6 +70,23 @@ MISTAKES = [
      "p.Chl[i] = p.Chl[i] + dt * p.BS[i] * p.rho[i];"),
     ("find_inds without the +1", "p.xi[i] = (int64_t)M::mask(p.ac[i], p.x[i] + L(1)) + Hx;",
      "p.xi[i] = (int64_t)M::mask(p.ac[i], p.x[i]) + Hx;"),
+    ("carbon: tempFunc_PS shape constant", "FT regT = shape_dec(x, xmax, L(1.0e-1f), 4);",
+     "FT regT = shape_dec(x, xmax, L(4.0e-2f), 4);"),
+    ("carbon: bacteria term dropped", "(light * (one - q[PI_C_is_bact]) + q[PI_C_is_bact])",
+     "(light * (one - q[PI_C_is_bact]))"),
+    ("carbon: thermal damage sign", "FT td = (T - q[PI_C_Topt]) * q[PI_C_f_T2B] * (Bm - Bd);",
+     "FT td = (q[PI_C_Topt] - T) * q[PI_C_f_T2B] * (Bm - Bd);"),
+    ("carbon: repair added to the damage", "p.Cq[i] = p.Cq[i] + (p.TD[i] - p.RP[i]) * dt;",
+     "p.Cq[i] = p.Cq[i] + (p.TD[i] + p.RP[i]) * dt;"),
+    ("carbon: daughter copies Bd", "p.Sz[d] = p.Sz[i]; p.Bm[d] = p.Bm[i]; p.Chl[d] = p.Chl[i];",
+     "p.Sz[d] = p.Sz[i]; p.Bm[d] = p.Bm[i]; p.Chl[d] = p.Chl[i]; p.Cq[d] = p.Cq[i];"),
+    ("carbon: DOC loss fraction swapped", "plk[PI_DOC][c] += M::mask(a, p.Bm[i] * fC) * loss[i];",
+     "plk[PI_DOC][c] += M::mask(a, p.Bm[i] * (L(1) - fC)) * loss[i];"),
+    ("carbon: thermal mortality threshold", "L(0.99f), L(1.0e-6f), 4)", "L(0.9f), L(1.0e-6f), 4)"),
+    ("carbon: respiration with tempFunc_PS", "q[PI_C_respir] * p.Bm[i] * c_tempFunc(tT[i], q)",
+     "q[PI_C_respir] * p.Bm[i] * c_tempFunc_PS(tT[i], q)"),
+    ("carbon: Chl from Cquota", "p.Chl[i] = p.Bm[i] * q[PI_C_Chl2C];", "p.Chl[i] = p.Bm[i] * q[PI_C_Cquota];"),
+    ("carbon: PS on the whole biomass", "M::max(z0, p.Bm[i] - p.Cq[i])", "M::max(z0, p.Bm[i])"),
 ]
 
 
@@ -90,7 +108,8 @@ def main(only):
                             "-fno-finite-math-only", "-fopenmp", "-fPIC", "-shared", "-iquote", HERE, "-o", lib, src], check=True)
             env = dict(os.environ, PI_ORACLE_LIB=lib)
             r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-m", "not gpu", "-p", "no:cacheprovider",
-                                "tests/test_oracle_pins.py", "tests/test_oracle_pins_physics.py"],
+                                "tests/test_oracle_pins.py", "tests/test_oracle_pins_physics.py",
+                                "tests/test_oracle_pins_carbon.py"],
                                cwd=REPO, env=env, capture_output=True, text=True)
             first = next((l for l in r.stdout.splitlines() if l.startswith("FAILED")), "")
             (caught if r.returncode != 0 else missed).append(name)

@@ -88,7 +88,8 @@ class OracleModel:
         pi = _pkg()
         lib = load()
         desc, keep = pi._abi.make_desc(spec.grid, spec.n_species, spec.max_individuals,
-                                       spec.species_params, spec.bgc_vector, spec.seed, diag_tracers)
+                                       spec.species_params, spec.bgc_vector, spec.seed, diag_tracers,
+                                       getattr(spec, "mode_id", pi._abi.MODE_QUOTA))
         raw = C.c_void_p()
         pi._native.check(lib, "orc_", lib.orc_model_create(C.byref(desc), precision, C.byref(raw)))
         del keep

@@ -3,7 +3,7 @@
  *
  * A CPU restatement, kernel by kernel and in the same floating-point operation
  * order, of PlanktonIndividuals.jl v0.7.5's CPU() time step for QuotaMode
- * without abiotic particles, as run by `julia -t 1` (KernelAbstractions' CPU
+ * and CarbonMode without abiotic particles, as run by `julia -t 1` (KernelAbstractions' CPU
  * backend with one thread visits every ndrange index in ascending order, so
  * every `@atomic` accumulation happens in slot order, kernel after kernel).
  *
@@ -96,10 +96,12 @@ struct Oracle : orc_model {
     FT t = 0;
     std::vector<int> diag_tr;
     bool failed = false;
+    int mode = PI_MODE_QUOTA;       /* PlanktonModel(...; mode) */
 
     struct Plank {
-        V x, y, z, Sz, Bm, Cq, Nq, Pq, Chl, gen, age;
+        V x, y, z, Sz, Bm, Cq, Nq, Pq, Chl, gen, age;    /* CarbonMode: Bd in Cq */
         V PS, VDOC, VNH4, VNO3, VPO4, rho, resp, BS, exu, graz, mort, dvid;
+        V TD, RP;                                     /* CarbonMode thermal damage and repair */
         std::vector<int64_t> xi, yi, zi, idx;
         std::vector<uint8_t> ac;
     };
@@ -160,10 +162,13 @@ struct Oracle : orc_model {
         }
         nsp = d->n_species; cap = d->capacity;
         if (nsp < 1 || cap < 1) return fail(PI_ERR_ARG, "n_species and capacity must be >= 1");
+        mode = d->mode;
+        if (mode != PI_MODE_QUOTA && mode != PI_MODE_CARBON) return fail(PI_ERR_UNSUPPORTED, "physiology mode");
         P.resize(nsp); dvid_type.resize(nsp);
         for (int s = 0; s < nsp; ++s) {
             for (int q = 0; q < PI_NPARAMS; ++q) P[s][q] = (FT)d->params[s * PI_NPARAMS + q];
-            dvid_type[s] = (int)d->params[s * PI_NPARAMS + PI_P_dvid_type];
+            const int dt_idx = mode == PI_MODE_CARBON ? (int)PI_C_dvid_type : (int)PI_P_dvid_type;
+            dvid_type[s] = (int)d->params[s * PI_NPARAMS + dt_idx];
             if (dvid_type[s] < 1 || dvid_type[s] > 3)
                 return fail(PI_ERR_ARG, "Wrong cell division type, must be in 1 to 3");
         }
@@ -175,7 +180,7 @@ struct Oracle : orc_model {
         for (auto& p : sp) {
             for (V* f : {&p.x, &p.y, &p.z, &p.Sz, &p.Bm, &p.Cq, &p.Nq, &p.Pq, &p.Chl, &p.gen, &p.age,
                          &p.PS, &p.VDOC, &p.VNH4, &p.VNO3, &p.VPO4, &p.rho, &p.resp, &p.BS, &p.exu,
-                         &p.graz, &p.mort, &p.dvid})
+                         &p.graz, &p.mort, &p.dvid, &p.TD, &p.RP})
                 f->assign(cap, (FT)0);
             p.xi.assign(cap, 0); p.yi.assign(cap, 0); p.zi.assign(cap, 0); p.idx.assign(cap, 0);
             p.ac.assign(cap, 0);
@@ -523,6 +528,7 @@ struct Oracle : orc_model {
 
     /* ---- plankton_update! (Plankton/QuotaMode/plankton_update.jl:1-34) ---- */
     pi_status plankton_update(int s, FT dt) {
+        if (mode == PI_MODE_CARBON) return carbon_update(s, dt);
         Plank& p = sp[s];
         const auto& q = P[s];
         diags_proc(diagSp[s][1], p.graz, p);
@@ -699,6 +705,171 @@ struct Oracle : orc_model {
             plk[PI_NH4][c] += M::mask(a, -p.VNH4[i] * dt);
             plk[PI_NO3][c] += M::mask(a, -p.VNO3[i] * dt);
             plk[PI_PO4][c] += M::mask(a, -p.VPO4[i] * dt);
+        }
+    }
+
+    /* ================================================ CarbonMode ======= */
+    /* tempFunc_PS / tempFunc (CarbonMode/growth_kernels.jl:1-17): regT with k = 0.1 */
+    static FT c_tempFunc_PS(FT T, const std::array<FT, PI_NPARAMS>& q) {
+        FT x = T - q[PI_C_Topt], xmax = q[PI_C_Tmax] - q[PI_C_Topt];
+        FT regT = shape_dec(x, xmax, L(1.0e-1f), 4);
+        FT k = M::exp(-q[PI_C_Ea] / (L(8.3145f) * (T + L(273.15f)))) * regT;
+        k = M::max(L(0), k);
+        FT OGT = M::exp(-q[PI_C_Ea] / (L(8.3145f) * (q[PI_C_Topt] + L(273.15f))));
+        return M::min(L(1), k / OGT);
+    }
+    static FT c_tempFunc(FT T, const std::array<FT, PI_NPARAMS>& q) {
+        FT k = M::exp(-q[PI_C_Ea] / (L(8.3145f) * (T + L(273.15f))));
+        k = M::max(L(0), k);
+        FT OGT = M::exp(-q[PI_C_Ea] / (L(8.3145f) * (q[PI_C_Topt] + L(273.15f))));
+        return M::min(L(1), k / OGT);
+    }
+    /* gamma_alloc (CarbonMode/growth_kernels.jl:22-26) */
+    static FT gamma_alloc(FT Bm, FT Bd, const std::array<FT, PI_NPARAMS>& q) {
+        FT g = M::mask(M::isless(L(0), Bd), Bd / M::max(L(1.0e-30f), Bm));
+        return M::mask(q[PI_C_thermal] == L(1), M::min(L(1), g));
+    }
+    /* calc_loss_kernel! (CarbonMode/consume_loss.jl:11-21): DOC and POC only */
+    void c_calc_loss(const Plank& p, const V& loss, FT fC) {
+        for (int64_t i = 0; i < cap; ++i) {
+            int64_t c = cell_of(p, i);
+            bool a = p.ac[i];
+            plk[PI_DOC][c] += M::mask(a, p.Bm[i] * fC) * loss[i];
+            plk[PI_POC][c] += M::mask(a, p.Bm[i] * (L(1) - fC)) * loss[i];
+        }
+    }
+    /* plankton_update! (CarbonMode/plankton_update.jl:1-32) */
+    pi_status carbon_update(int s, FT dt) {
+        Plank& p = sp[s];
+        const auto& q = P[s];
+        diags_proc(diagSp[s][1], p.graz, p);
+        c_calc_loss(p, p.graz, q[PI_C_grazFracC]);
+        inactivate(p, p.graz);
+        diags_proc(diagSp[s][2], p.mort, p);
+        c_calc_loss(p, p.mort, q[PI_C_mortFracC]);
+        inactivate(p, p.mort);
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) p.dvid[i] = M::mask(p.ac[i], p.dvid[i]);
+        diags_proc(diagSp[s][3], p.dvid, p);
+        FT dvidnum = 0;
+        for (int64_t i = 0; i < cap; ++i) dvidnum += M::mask(p.ac[i], p.dvid[i]);
+        std::vector<int64_t> free_ind;
+        for (int64_t i = 0; i < cap; ++i) if (!p.ac[i]) free_ind.push_back(i + 1);
+        if (dvidnum > (FT)free_ind.size()) {
+            char buf[160];
+            snprintf(buf, sizeof buf, "number of individual exceeds the capacity at timestep %g days",
+                     (double)t / 86400.0);
+            failed = true;
+            return fail(PI_ERR_CAPACITY, buf);
+        }
+        c_divide(p, free_ind);
+        c_growth(s, dt);
+        for (int64_t i = 0; i < cap; ++i)    /* calc_consume_kernel! (consume_loss.jl:2-9): DIC only */
+            plk[PI_DIC][cell_of(p, i)] += M::mask(p.ac[i], (p.resp[i] - p.PS[i]) * dt);
+        for (int64_t i = 0; i < cap; ++i) diagSp[s][0][cell_of(p, i)] += (FT)((p.ac[i] && p.ac[i]) ? 1 : 0);
+        return PI_OK;
+    }
+    /* divide! (CarbonMode/division_death.jl:28-70): the daughter copies x, y, z, Sz,
+     * Bm, Chl, gen, ac, dvid; Bd and age stay those of the slot */
+    void c_divide(Plank& p, const std::vector<int64_t>& free_ind) {
+        FT acc = 0;
+        for (int64_t i = 0; i < cap; ++i) { acc = acc + p.dvid[i]; idc[i] = acc; }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) idc_int[i] = (int64_t)idc[i];
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i)
+            if (p.dvid[i] == L(1)) p.idx[i] = free_ind[(size_t)(idc_int[i] - 1)];
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {
+            if (p.dvid[i] == L(1) && p.idx[i] != 0) {
+                int64_t d = p.idx[i] - 1;
+                p.x[d] = p.x[i]; p.y[d] = p.y[i]; p.z[d] = p.z[i];
+                p.Sz[d] = p.Sz[i]; p.Bm[d] = p.Bm[i]; p.Chl[d] = p.Chl[i];
+                p.gen[d] = p.gen[i]; p.ac[d] = p.ac[i]; p.dvid[d] = p.dvid[i];
+            }
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* divide_to_half_kernel! */
+            FT h = (L(2) - p.dvid[i]) / L(2);
+            p.Sz[i] = p.Sz[i] * h; p.Bm[i] = p.Bm[i] * h; p.Cq[i] = p.Cq[i] * h; p.Chl[i] = p.Chl[i] * h;
+            p.gen[i] = p.gen[i] + p.dvid[i];
+            p.age[i] = p.age[i] * (L(1) - p.dvid[i]);
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) p.idx[i] = 0;
+    }
+    /* plankton_growth! (CarbonMode/plankton_growth.jl:2-29, growth_kernels.jl:28-120,
+     * division_death_probability.jl:176-278) */
+    void c_growth(int s, FT dt) {
+        Plank& p = sp[s];
+        const auto& q = P[s];
+        const FT e30 = L(1.0e-30f), z0 = L(0), one = L(1);
+        const bool thermal_on = q[PI_C_thermal] == one;
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_PS_kernel! with calc_photosynthesis */
+            FT aI = tpar[i] * q[PI_C_alpha] * q[PI_C_Phi];
+            FT T = tT[i];
+            FT PCm = q[PI_C_PCmax] * (c_tempFunc(T, q) * q[PI_C_thermal] + c_tempFunc_PS(T, q) * (one - q[PI_C_thermal]));
+            FT light = one - M::exp(-aI / M::max(e30, PCm) * p.Chl[i] / M::max(e30, p.Bm[i]));
+            FT PS = PCm * M::max(z0, p.Bm[i] - p.Cq[i]) * (light * (one - q[PI_C_is_bact]) + q[PI_C_is_bact]);
+            p.PS[i] = M::mask(p.ac[i], PS);
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i)     /* calc_BS_kernel! */
+            p.BS[i] = p.PS[i] * (one - gamma_alloc(p.Bm[i], p.Cq[i], q));
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i)     /* calc_repair_kernel! */
+            p.RP[i] = p.PS[i] * gamma_alloc(p.Bm[i], p.Cq[i], q);
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_thermal_damage_kernel! */
+            FT T = tT[i], Bm = p.Bm[i], Bd = p.Cq[i];
+            FT td = (T - q[PI_C_Topt]) * q[PI_C_f_T2B] * (Bm - Bd);
+            td = M::mask(M::isless(q[PI_C_Topt], T), td);
+            td = M::mask(M::isless(Bd, Bm), td);
+            td = M::mask(thermal_on, td);
+            p.TD[i] = M::max(z0, M::min(td, (Bm - Bd) / dt));
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i)     /* calc_respir_kernel! */
+            p.resp[i] = M::mask(p.ac[i], q[PI_C_respir] * p.Bm[i] * c_tempFunc(tT[i], q));
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* update_quotas_kernel! */
+            p.Bm[i] = p.Bm[i] + (p.BS[i] - p.resp[i]) * dt;
+            p.Cq[i] = p.Cq[i] + (p.TD[i] - p.RP[i]) * dt;
+            p.age[i] = p.age[i] + M::mask(p.ac[i], dt / L(3600));
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* update_cellsize_kernel! */
+            p.Sz[i] = p.Bm[i] / (q[PI_C_Cquota] * q[PI_C_Nsuper]);
+            p.Chl[i] = p.Bm[i] * q[PI_C_Chl2C];
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i)     /* calc_graz_quadratic_kernel! */
+            p.graz[i] = M::mask(p.ac[i], tpop[i] * q[PI_C_grz_P]);
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_dvid_kernel! */
+            FT P_ = q[PI_C_dvid_P], Sz = p.Sz[i], v;
+            FT hour = M::rem(t, L(86400)) / L(3600);
+            if (dvid_type[s] == 1) v = P_ * shape_inc(Sz, q[PI_C_dvid_reg], L(2.0e-3f), 2);
+            else if (dvid_type[s] == 2) v = P_ * shape_inc(hour, q[PI_C_dvid_reg], L(1.0e-5f), 4);
+            else v = P_ * shape_inc(Sz, q[PI_C_dvid_reg], L(1.0e-3f), 4) * shape_inc(hour, q[PI_C_dvid_reg2], L(1.0e-5f), 4);
+            p.dvid[i] = M::mask(p.ac[i], v);
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_thermal_mort_kernel! / calc_mort_kernel! */
+            FT v = thermal_on
+                ? q[PI_C_mort_P] * shape_inc(p.Cq[i] / M::max(e30, p.Bm[i]), L(0.99f), L(1.0e-6f), 4)
+                : q[PI_C_mort_P] * shape_dec(p.Sz[i], q[PI_C_mort_reg], L(1.0e-5f), 4);
+            p.mort[i] = M::mask(p.ac[i], v);
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* get_probability! */
+            float a, b, c;
+            pi_uniform3(seed, iteration, s, PI_KIND_DECISION, i, &a, &b, &c);
+            bool ac = p.ac[i];
+            p.graz[i] = (M::isless((FT)a, p.graz[i] * dt) && ac) ? L(1) : L(0);
+            p.mort[i] = (M::isless((FT)b, p.mort[i] * dt) && ac) ? L(1) : L(0);
+            p.dvid[i] = (M::isless((FT)c, p.dvid[i] * dt) && ac) ? L(1) : L(0);
         }
     }
 

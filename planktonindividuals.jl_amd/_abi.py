@@ -21,6 +21,11 @@ PARAM_NAMES = ("Nsuper", "Cquota", "mean", "var", "Chl2Cint", "α", "Φ", "Topt"
                "Chl2N", "R_NC", "R_PC", "k_mtb", "respir", "grz_P", "dvid_type", "dvid_P",
                "dvid_reg", "dvid_reg2", "mort_P", "mort_reg", "grazFracC", "grazFracN",
                "grazFracP", "mortFracC", "mortFracN", "mortFracP")
+# CarbonMode parameter order (CarbonMode/plankton_generation.jl:14-16)
+CARBON_PARAM_NAMES = ("Nsuper", "Cquota", "mean", "var", "α", "Φ", "Topt", "Tmax", "Ea", "PCmax", "Chl2C",
+                      "respir", "f_T2B", "grz_P", "dvid_type", "dvid_P", "dvid_reg", "dvid_reg2", "mort_P",
+                      "mort_reg", "grazFracC", "mortFracC", "thermal", "is_bact")
+MODE_QUOTA, MODE_CARBON = 0, 1
 BGC_NAMES = ("kw", "kc", "kDOC", "Nit", "kDON", "kDOP", "kPOC", "kPON", "kPOP", "κh", "κv",
              "κhP", "κvP", "shared_graz", "w_sink_org", "w_sink_inorg", "lambda_POC",
              "lambda_min", "kdiss", "lambda_dust", "lambda_Fe", "ligand", "DFeFrac")
@@ -54,7 +59,8 @@ class GridDesc(C.Structure):
 class ModelDesc(C.Structure):
     _fields_ = [("grid", GridDesc), ("n_species", C.c_int32), ("capacity", C.c_int64),
                 ("params", fp), ("bgc", fp), ("rng_seed", C.c_uint64),
-                ("n_diag_tracers", C.c_int32), ("diag_tracers", C.POINTER(C.c_int32))]
+                ("n_diag_tracers", C.c_int32), ("diag_tracers", C.POINTER(C.c_int32)),
+                ("mode", C.c_int32)]
 
 
 class IndividualsHost(C.Structure):
@@ -69,7 +75,7 @@ def ptr(a, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
 
 
-def make_desc(grid, n_species, capacity, params, bgc, seed, diag_tracers=()):
+def make_desc(grid, n_species, capacity, params, bgc, seed, diag_tracers=(), mode=MODE_QUOTA):
     """Build a ModelDesc; returns (desc, keepalive) — keep the second alive while desc is used."""
     dzC = np.ascontiguousarray(grid.dzC, dtype=np.float32)
     dzF = np.ascontiguousarray(grid.dzF, dtype=np.float32)
@@ -83,7 +89,7 @@ def make_desc(grid, n_species, capacity, params, bgc, seed, diag_tracers=()):
                  ptr(dzC, C.c_float), ptr(dzF, C.c_float),
                  None if lm is None else lm.ctypes.data_as(u8p))
     d = ModelDesc(g, n_species, capacity, ptr(params, C.c_float), ptr(bgc, C.c_float),
-                  C.c_uint64(seed), len(dt), ptr(dt, C.c_int32) if len(dt) else None)
+                  C.c_uint64(seed), len(dt), ptr(dt, C.c_int32) if len(dt) else None, int(mode))
     return d, (dzC, dzF, lm, params, bgc, dt)
 
 

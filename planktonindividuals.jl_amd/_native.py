@@ -215,11 +215,11 @@ class Handle:
         return v.value
 
 
-def footprint(grid, nsp, cap, params, bgc, seed=0, diag_tracers=()):
+def footprint(grid, nsp, cap, params, bgc, seed=0, diag_tracers=(), mode=_abi.MODE_QUOTA):
     """Device bytes a model of this shape allocates (pi_footprint: a dry run of
     pi_model_create, no device needed)."""
     lib = load_library()
-    desc, keep = _abi.make_desc(grid, nsp, cap, params, bgc, seed, diag_tracers)
+    desc, keep = _abi.make_desc(grid, nsp, cap, params, bgc, seed, diag_tracers, mode)
     v = C.c_int64(0)
     check(lib, "pi_", lib.pi_footprint(C.byref(desc), C.byref(v)))
     del keep
@@ -229,10 +229,10 @@ def footprint(grid, nsp, cap, params, bgc, seed=0, diag_tracers=()):
 class NativeModel(Handle):
     """A model living on the GPU (libpi_hip.so)."""
 
-    def __init__(self, grid, nsp, cap, params, bgc, seed, diag_tracers=(), device=0):
+    def __init__(self, grid, nsp, cap, params, bgc, seed, diag_tracers=(), device=0, mode=_abi.MODE_QUOTA):
         lib = load_library()
         check(lib, "pi_", lib.pi_init(device))
-        desc, keep = _abi.make_desc(grid, nsp, cap, params, bgc, seed, diag_tracers)
+        desc, keep = _abi.make_desc(grid, nsp, cap, params, bgc, seed, diag_tracers, mode)
         raw = C.c_void_p()
         check(lib, "pi_", lib.pi_model_create(C.byref(desc), C.byref(raw)))
         del keep

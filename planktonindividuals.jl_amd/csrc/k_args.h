@@ -18,6 +18,7 @@ struct AdvArgs {
     uint32_t* cnts;           /* per cell: queued arrivals */
     int32_t* dstamp;          /* per slot: iteration in which its occupant dies */
     uint32_t* dcell;          /* per slot: that occupant's cell in this step */
+    float* dbd;               /* CarbonMode, per slot: that occupant's Bd (null in QuotaMode) */
     long long cap;
 };
 
@@ -47,6 +48,7 @@ struct PhysArgs {
     const uint32_t* dcell;
     int32_t* rstamp;          /* per slot: iteration in which a daughter was born there */
     float* rcons;             /* per slot x 5: that daughter's consumption */
+    const float* dbd;         /* CarbonMode, per slot: Bd of an occupant dying in this step */
 };
 
 struct PlkArgs {
@@ -178,7 +180,7 @@ enum KId {
     K_VELHALF, K_MARKS, K_SCAN_P, K_SCAN_F, K_CAPCHECK, K_ADV, K_CELLCOUNT, K_PLACE_STAY,
     K_PLACE_SMALL, K_MIXSORT, K_BIGCELLS, K_CELLSUM, K_CELLSUM_LONG, K_PAR, K_PHYS, K_PLK, K_PLK_LONG,
     K_SWEEPX0, K_SWEEPX1, K_XY0, K_XY1, K_SWEEPY0, K_SWEEPY1, K_HALOCOPY, K_FILL, K_FINAL0, K_FINAL1, K_ACCUM,
-    K_FINISH, K_XYW0, K_XYW1, K_NKID
+    K_FINISH, K_XYW0, K_XYW1, K_PHYSC, K_NKID
 };
 /* plan markers (not kernels): fork to / join from the side stream */
 enum { K_FORK = K_NKID, K_JOIN = K_NKID + 1 };
@@ -217,6 +219,7 @@ struct InitArgs {
     int nsp;
     long long N[PI_MAXSP];
     uint64_t seed;
+    int mode;                 /* PI_MODE_QUOTA / PI_MODE_CARBON */
 };
 struct TracerInitArgs {
     DevGrid g;

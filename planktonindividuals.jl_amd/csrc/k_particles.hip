@@ -187,6 +187,7 @@ __device__ __forceinline__ void place_entry(const AdvArgs& a, DevState* st, cons
             const long long r = (long long)sp * a.cap + slot;
             a.dstamp[r] = iteration;
             a.dcell[r] = c;
+            if (a.dbd) a.dbd[r] = S.Cq[e];  /* CarbonMode: the Bd a daughter born there inherits */
         }
     }
     const uint32_t sq = (e >> 8) & (PI_NQ - 1);       /* the same for every lane of a wave */
@@ -880,6 +881,67 @@ __device__ void growth(PState& s, const Trs& t, const float* q, float OGT, bool 
     ddvid = jl_islessf(U2, dvid * dt) && ac;
 }
 
+/* CarbonMode plankton_growth! for one individual (CarbonMode/plankton_growth.jl:2-29,
+ * growth_kernels.jl:1-120, division_death_probability.jl:56-136).  s.Cq holds
+ * the damaged biomass Bd; the rates returned are PS and resp (RS). */
+__device__ __forceinline__ float ogt_carbon(const float* q)
+{
+    return pi_expf(-q[PI_C_Ea] / (8.3145f * (q[PI_C_Topt] + 273.15f)));
+}
+__device__ void growth_carbon(PState& s, const Trs& t, const float* q, float OGT, bool ac, float dt, float tmodel,
+                              int dvid_type, float U0, float U1, float U2, Rates& r,
+                              bool& dgraz, bool& dmort, bool& ddvid)
+{
+    const float e30 = 1.0e-30f;
+    const bool thermal = q[PI_C_thermal] == 1.0f;
+    /* tempFunc / tempFunc_PS (growth_kernels.jl:1-17), regT with k = 0.1 */
+    const float kx = pi_expf(-q[PI_C_Ea] / (8.3145f * (t.T + 273.15f)));
+    const float x = t.T - q[PI_C_Topt], xmax = q[PI_C_Tmax] - q[PI_C_Topt];
+    const float tps = dminf(1.0f, dmaxf(0.0f, kx * shape_dec(x, xmax, 1.0e-1f, 4)) / OGT);
+    const float tf = dminf(1.0f, dmaxf(0.0f, kx) / OGT);
+    /* calc_PS_kernel! (calc_photosynthesis) */
+    const float aI = t.par * q[PI_C_alpha] * q[PI_C_Phi];
+    const float PCm = q[PI_C_PCmax] * (tf * q[PI_C_thermal] + tps * (1.0f - q[PI_C_thermal]));
+    const float light = 1.0f - pi_expf(-aI / dmaxf(e30, PCm) * s.Chl / dmaxf(e30, s.Bm));
+    const float PS = jl_maskf(ac, PCm * dmaxf(0.0f, s.Bm - s.Cq) *
+                                      (light * (1.0f - q[PI_C_is_bact]) + q[PI_C_is_bact]));
+    /* gamma_alloc, calc_BS_kernel!, calc_repair_kernel! */
+    float gam = jl_maskf(jl_islessf(0.0f, s.Cq), s.Cq / dmaxf(e30, s.Bm));
+    gam = jl_maskf(thermal, dminf(1.0f, gam));
+    const float BS = PS * (1.0f - gam);
+    const float RP = PS * gam;
+    /* calc_thermal_damage_kernel! (not masked by ac) */
+    float TD = (t.T - q[PI_C_Topt]) * q[PI_C_f_T2B] * (s.Bm - s.Cq);
+    TD = jl_maskf(jl_islessf(q[PI_C_Topt], t.T), TD);
+    TD = jl_maskf(jl_islessf(s.Cq, s.Bm), TD);
+    TD = jl_maskf(thermal, TD);
+    TD = dmaxf(0.0f, dminf(TD, (s.Bm - s.Cq) / dt));
+    /* calc_respir_kernel! */
+    const float RS = jl_maskf(ac, q[PI_C_respir] * s.Bm * tf);
+    /* update_quotas_kernel!, update_cellsize_kernel! */
+    s.Bm = s.Bm + (BS - RS) * dt;
+    s.Cq = s.Cq + (TD - RP) * dt;
+    s.age = s.age + jl_maskf(ac, dt / 3600.0f);
+    s.Sz = s.Bm / (q[PI_C_Cquota] * q[PI_C_Nsuper]);
+    s.Chl = s.Bm * q[PI_C_Chl2C];
+    r.PS = PS; r.resp = RS;
+    r.VDOC = r.VNH4 = r.VNO3 = r.VPO4 = r.BS = r.exu = r.rho = 0.0f;
+    /* probabilities + get_probability! */
+    const float graz = jl_maskf(ac, t.pop * q[PI_C_grz_P]);
+    float dv;
+    const float hour = fmodf(tmodel, 86400.0f) / 3600.0f;
+    if (dvid_type == 1) dv = q[PI_C_dvid_P] * shape_inc(s.Sz, q[PI_C_dvid_reg], 2.0e-3f, 2);
+    else if (dvid_type == 2) dv = q[PI_C_dvid_P] * shape_inc(hour, q[PI_C_dvid_reg], 1.0e-5f, 4);
+    else dv = q[PI_C_dvid_P] * shape_inc(s.Sz, q[PI_C_dvid_reg], 1.0e-3f, 4) * shape_inc(hour, q[PI_C_dvid_reg2], 1.0e-5f, 4);
+    const float dvid = jl_maskf(ac, dv);
+    const float mort = thermal
+        ? jl_maskf(ac, q[PI_C_mort_P] * shape_inc(s.Cq / dmaxf(e30, s.Bm), 0.99f, 1.0e-6f, 4))
+        : jl_maskf(ac, q[PI_C_mort_P] * shape_dec(s.Sz, q[PI_C_mort_reg], 1.0e-5f, 4));
+    dgraz = jl_islessf(U0, graz * dt) && ac;
+    dmort = jl_islessf(U1, mort * dt) && ac;
+    ddvid = jl_islessf(U2, dvid * dt) && ac;
+}
+
 /* k-th set bit (0-based) of a word */
 __device__ __forceinline__ int select_bit(uint32_t w, uint32_t k)
 {
@@ -905,8 +967,10 @@ __device__ __forceinline__ uint32_t valid_mask(const PhysArgs& a, uint32_t w)
     return rem >= 32 ? 0xFFFFFFFFu : ((1u << rem) - 1u);
 }
 
+template <int MODE>
 __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
 {
+    constexpr bool CARBON = MODE == PI_MODE_CARBON;
     const DevState* st = a.st;
     if (st->err) return;
     const DevGrid& g = a.g;
@@ -930,7 +994,8 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
     for (int i = threadIdx.x; i < a.nsp; i += blockDim.x) s_doff[i] = st->doff[i];
     for (int i = threadIdx.x; i < a.nsp * PI_NPARAMS; i += blockDim.x) s_par[i] = a.params[i];
     __syncthreads();
-    for (int i = threadIdx.x; i < a.nsp; i += blockDim.x) s_ogt[i] = ogt_of(s_par + i * PI_NPARAMS);
+    for (int i = threadIdx.x; i < a.nsp; i += blockDim.x)
+        s_ogt[i] = CARBON ? ogt_carbon(s_par + i * PI_NPARAMS) : ogt_of(s_par + i * PI_NPARAMS);
     __syncthreads();
     const uint32_t hhh = (uint32_t)gidx(g, g.Hx, g.Hy, g.Hz);
     const uint32_t stride = K.n * blockDim.x;
@@ -952,7 +1017,7 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
         const int sp = (int)(key >> PI_SLOT_BITS);
         const uint32_t slot = key & PI_SLOT_MASK;
         const float* q = s_par + sp * PI_NPARAMS;
-        const int dvid_type = (int)q[PI_P_dvid_type];
+        const int dvid_type = (int)q[CARBON ? (int)PI_C_dvid_type : (int)PI_P_dvid_type];
         const float OGT = s_ogt[sp];
         const uint8_t fl = IN.flags[src];
         PState s;
@@ -961,11 +1026,15 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
         s.Pq = IN.Pq[src]; s.Chl = IN.Chl[src]; s.gen = IN.gen[src]; s.age = IN.age[src];
         /* find_NPT_kernel! (Individuals/utils.jl:14-30), ac = true at gather time */
         Trs t;
-        t.NH4 = dmaxf(0.0f, TR_NH4[c]);
-        t.NO3 = dmaxf(0.0f, TR_NO3[c]);
-        t.PO4 = dmaxf(0.0f, TR_PO4[c]);
-        t.DOC = dmaxf(0.0f, TR_DOC[c]);
-        t.DFe = dmaxf(0.0f, TR_DFe[c]);
+        if (CARBON) {                        /* CarbonMode growth reads only par, T and pop */
+            t.NH4 = t.NO3 = t.PO4 = t.DOC = t.DFe = 0.0f;
+        } else {
+            t.NH4 = dmaxf(0.0f, TR_NH4[c]);
+            t.NO3 = dmaxf(0.0f, TR_NO3[c]);
+            t.PO4 = dmaxf(0.0f, TR_PO4[c]);
+            t.DOC = dmaxf(0.0f, TR_DOC[c]);
+            t.DFe = dmaxf(0.0f, TR_DFe[c]);
+        }
         t.par = PAR[c];
         t.dpar = PAR[c] - PAR0[c];
         t.T = a.temp[c];
@@ -975,7 +1044,15 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
         bool ac = true;
         uint8_t tag = 0;
         float L0 = 0.f, L1 = 0.f, L2 = 0.f, L3 = 0.f, L4 = 0.f, L5 = 0.f;
-        if (fl & (F_GRAZ | F_MORT)) {
+        if (CARBON && (fl & (F_GRAZ | F_MORT))) {
+            /* calc_loss! of CarbonMode (CarbonMode/consume_loss.jl:11-21): Bm to DOC and POC */
+            const bool gz = (fl & F_GRAZ) != 0;
+            const float fC = gz ? q[PI_C_grazFracC] : q[PI_C_mortFracC];
+            L0 = s.Bm * fC * 1.0f;
+            L3 = s.Bm * (1.0f - fC) * 1.0f;
+            tag = gz ? T_GRAZ : T_MORT;
+            ac = false;
+        } else if (fl & (F_GRAZ | F_MORT)) {
             const bool gz = (fl & F_GRAZ) != 0;
             const float fC = gz ? q[PI_P_grazFracC] : q[PI_P_mortFracC];
             const float fN = gz ? q[PI_P_grazFracN] : q[PI_P_mortFracN];
@@ -1011,18 +1088,25 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
             const uint32_t dslot = lo * 32u + (uint32_t)bit;
             /* daughter = copy before halving; divide_to_half on both */
             PState d = s;
+            const long long dr = (long long)sp * a.cap + dslot;
+            const bool reborn = a.dstamp[dr] == (int32_t)iteration;
+            if (CARBON) {
+                /* copy_daughter_individuals! of CarbonMode copies x, y, z, Sz, Bm, Chl,
+                 * gen: the daughter keeps the slot's Bd (its occupant's, dying in this
+                 * step, or the last one's) */
+                d.Cq = reborn ? a.dbd[dr] : a.tab.Cq[dr];
+                d.Nq = d.Pq = 0.0f;
+            }
             const float h = (2.0f - 1.0f) / 2.0f;
             s.Sz = s.Sz * h; s.Bm = s.Bm * h; s.Cq = s.Cq * h; s.Nq = s.Nq * h; s.Pq = s.Pq * h; s.Chl = s.Chl * h;
             s.gen = s.gen + 1.0f; s.age = s.age * (1.0f - 1.0f);
             d.Sz = d.Sz * h; d.Bm = d.Bm * h; d.Cq = d.Cq * h; d.Nq = d.Nq * h; d.Pq = d.Pq * h; d.Chl = d.Chl * h;
             d.gen = d.gen + 1.0f;
-            const long long dr = (long long)sp * a.cap + dslot;
             d.age = a.tab.age[dr] * (1.0f - 1.0f);
             /* The daughter's slot kept what find_inds!/find_NPT! wrote into it this
              * step: if its previous occupant was still active then (it dies in this
              * step), the tracers gathered at that occupant's cell and that cell;
              * otherwise zeros and the halo cell (Hx,Hy,Hz). */
-            const bool reborn = a.dstamp[dr] == (int32_t)iteration;
             const uint32_t dc = reborn ? a.dcell[dr] : hhh;
             Trs tz;
             if (reborn) {
@@ -1041,13 +1125,14 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
             pi_uniform3(a.seed, iteration, sp, PI_KIND_DECISION, dslot, &U0, &U1, &U2);
             Rates rd;
             bool gz, mt, dv;
-            growth(d, tz, q, OGT, true, dt, tmodel, dvid_type, U0, U1, U2, rd, gz, mt, dv);
+            if (CARBON) growth_carbon(d, tz, q, OGT, true, dt, tmodel, dvid_type, U0, U1, U2, rd, gz, mt, dv);
+            else growth(d, tz, q, OGT, true, dt, tmodel, dvid_type, U0, U1, U2, rd, gz, mt, dv);
             if (reborn) {   /* its calc_consume! lands in that cell, at the slot's place */
                 a.rcons[dr * 5 + 0] = (rd.resp - rd.PS) * dt;
-                a.rcons[dr * 5 + 1] = (rd.exu - rd.VDOC) * dt;
-                a.rcons[dr * 5 + 2] = -rd.VNH4 * dt;
-                a.rcons[dr * 5 + 3] = -rd.VNO3 * dt;
-                a.rcons[dr * 5 + 4] = -rd.VPO4 * dt;
+                a.rcons[dr * 5 + 1] = CARBON ? 0.0f : (rd.exu - rd.VDOC) * dt;
+                a.rcons[dr * 5 + 2] = CARBON ? 0.0f : -rd.VNH4 * dt;
+                a.rcons[dr * 5 + 3] = CARBON ? 0.0f : -rd.VNO3 * dt;
+                a.rcons[dr * 5 + 4] = CARBON ? 0.0f : -rd.VPO4 * dt;
                 a.rstamp[dr] = (int32_t)iteration;
             }
             const uint32_t o = n + s_doff[sp] + r;
@@ -1064,14 +1149,16 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
         pi_uniform3(a.seed, iteration, sp, PI_KIND_DECISION, slot, &U0, &U1, &U2);
         Rates r;
         bool gz, mt, dv;
-        growth(s, t, q, OGT, ac, dt, tmodel, dvid_type, U0, U1, U2, r, gz, mt, dv);
+        if (CARBON) growth_carbon(s, t, q, OGT, ac, dt, tmodel, dvid_type, U0, U1, U2, r, gz, mt, dv);
+        else growth(s, t, q, OGT, ac, dt, tmodel, dvid_type, U0, U1, U2, r, gz, mt, dv);
         if (ac) {
+            /* calc_consume! (QuotaMode/consume_loss.jl:2-14; CarbonMode: DIC only) */
             tag = T_ALIVE | (divider ? T_DIV : 0);
             L0 = (r.resp - r.PS) * dt;
-            L1 = (r.exu - r.VDOC) * dt;
-            L2 = -r.VNH4 * dt;
-            L3 = -r.VNO3 * dt;
-            L4 = -r.VPO4 * dt;
+            L1 = CARBON ? 0.0f : (r.exu - r.VDOC) * dt;
+            L2 = CARBON ? 0.0f : -r.VNH4 * dt;
+            L3 = CARBON ? 0.0f : -r.VNO3 * dt;
+            L4 = CARBON ? 0.0f : -r.VPO4 * dt;
             L5 = 0.0f;
         } else {
             /* inactive from now on: record its last state in the slot table */
@@ -1104,7 +1191,10 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
         a.con.tag[e] = tag;
     }
 }
-PI_KERNEL_XCD(k_phys, PhysArgs, 256)
+__device__ __forceinline__ void k_physq_body(const PhysArgs& a, Blk K) { k_phys_body<PI_MODE_QUOTA>(a, K); }
+__device__ __forceinline__ void k_physc_body(const PhysArgs& a, Blk K) { k_phys_body<PI_MODE_CARBON>(a, K); }
+PI_KERNEL3(k_phys, k_physq_body, PhysArgs, 256, xcd_block())
+PI_KERNEL3(k_physc, k_physc_body, PhysArgs, 256, xcd_block())
 
 /* Ordered per-cell sums: calc_loss! (graz), calc_loss! (mort), calc_consume! per
  * species in slot order (plankton_update.jl:5-29), and the per-species
@@ -1444,6 +1534,7 @@ void reg_particles(KInfo* k)
     PI_REG(k, K_CELLSUM_LONG, k_cellsum_long, BinArgs, 256, true);
     PI_REG(k, K_PAR, k_par, ParArgs, 256, true);
     PI_REG(k, K_PHYS, k_phys, PhysArgs, 256, true);
+    PI_REG(k, K_PHYSC, k_physc, PhysArgs, 256, true);
     PI_REG(k, K_PLK, k_plk, PlkArgs, 256, true);
     PI_REG(k, K_PLK_LONG, k_plk_long, PlkArgs, 256, true);
 }

@@ -328,11 +328,12 @@ __global__ void __launch_bounds__(256) k_init_individuals(InitArgs a)
         const float y = jl_maskf(ac, uy * (float)g.Ny);
         const float z = jl_maskf(ac, uz * (float)g.Nz);
         const float Sz = jl_maskf(ac, jl_maxf(1.0f, Sz0 * q[PI_P_var] + q[PI_P_mean]));
-        const float Bm = (q[PI_P_Cquota] * Sz) * q[PI_P_Nsuper];
-        const float Cq = (uc * q[PI_P_Cqmax]) * Bm;
-        const float Nq = un * (q[PI_P_Nqmax] * (Bm + Cq) - Bm * q[PI_P_R_NC]);
-        const float Pq = up * (q[PI_P_Pqmax] * (Bm + Cq) - Bm * q[PI_P_R_PC]);
-        const float Chl = Bm * q[PI_P_Chl2Cint];
+        const float Bm = (q[PI_P_Cquota] * Sz) * q[PI_P_Nsuper];     /* PI_C_Cquota, PI_C_Nsuper alike */
+        const bool carbon = a.mode == PI_MODE_CARBON;   /* CarbonMode/plankton_generation.jl:34-56: Bd = 0 */
+        const float Cq = carbon ? 0.0f : (uc * q[PI_P_Cqmax]) * Bm;
+        const float Nq = carbon ? 0.0f : un * (q[PI_P_Nqmax] * (Bm + Cq) - Bm * q[PI_P_R_NC]);
+        const float Pq = carbon ? 0.0f : up * (q[PI_P_Pqmax] * (Bm + Cq) - Bm * q[PI_P_R_PC]);
+        const float Chl = Bm * q[carbon ? (int)PI_C_Chl2C : (int)PI_P_Chl2Cint];
         if (ac && g.has_land) {   /* mask_individuals! (Individuals/utils.jl:89-100) */
             const int xi = (int)truncf(x + 1.0f) + g.Hx, yi = (int)truncf(y + 1.0f) + g.Hy, zi = (int)truncf(z + 1.0f) + g.Hz;
             ac = g.lm[gidx(g, xi, yi, zi)] != 0;

@@ -135,6 +135,8 @@ struct pi_model {
     float gdt = 0.0f;
     size_t tcount = 0;
     long long shape[PI_NSHAPE] = {16, 16, 16384, 4096, 8, 1 << 22};   /* pi_set_launch_shape */
+    int mode = PI_MODE_QUOTA;  /* physiology (pi_model_desc::mode) */
+    float* dbd = nullptr;      /* CarbonMode: per slot, the Bd of an occupant dying in the step */
 };
 
 static hipError_t dalloc(pi_model* m, void** p, size_t bytes)
@@ -371,6 +373,8 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     DALLOC(m->sflag, m->ctot);
     DALLOC(m->dstamp, m->ctot * 4); DALLOC(m->rstamp, m->ctot * 4);
     DALLOC(m->dcell, m->ctot * 4); DALLOC(m->rcons, m->ctot * 5 * 4);
+    m->mode = d->mode;
+    if (m->mode == PI_MODE_CARBON) DALLOC(m->dbd, m->ctot * 4);
     if (!m->dry) {   /* stamps start below any iteration */
         std::vector<int32_t> neg(m->ctot, -1);
         DEVCALL(hipMemcpy(m->dstamp, neg.data(), m->ctot * 4, hipMemcpyHostToDevice));
@@ -413,8 +417,11 @@ static pi_status check_desc(const pi_model_desc* d)
     if (gd.topo_z != PI_BOUNDED) return fail(PI_ERR_ARG, "grid: z topology must be Bounded");
     if (d->n_species < 1 || d->n_species > PI_MAXSP) return fail(PI_ERR_ARG, "n_species out of range");
     if (d->capacity < 1 || d->capacity > (long long)PI_SLOT_MASK) return fail(PI_ERR_ARG, "capacity out of range");
+    if (d->mode != PI_MODE_QUOTA && d->mode != PI_MODE_CARBON)
+        return fail(PI_ERR_UNSUPPORTED, "physiology mode: QuotaMode and CarbonMode are implemented");
+    const int dvt = d->mode == PI_MODE_CARBON ? (int)PI_C_dvid_type : (int)PI_P_dvid_type;
     for (int s = 0; s < d->n_species; ++s) {
-        float t = d->params[s * PI_NPARAMS + PI_P_dvid_type];
+        float t = d->params[s * PI_NPARAMS + dvt];
         if (!(t == 1.0f || t == 2.0f || t == 3.0f)) return fail(PI_ERR_ARG, "Wrong cell division type, must be in 1 to 3");
     }
     return PI_OK;
@@ -592,6 +599,7 @@ pi_status pi_generate_state(pi_model* m, const int64_t* n_individuals, const flo
     a.g = m->g; a.st = m->d_state; a.tab = m->tab; a.params = m->d_params; a.cap = m->cap; a.nsp = m->nsp;
     for (int s = 0; s < m->nsp; ++s) a.N[s] = n_individuals[s];
     a.seed = m->seed;
+    a.mode = m->mode;
     launch_init_individuals(m->stream, a);
     TracerInitArgs t{};
     t.g = m->g; t.slab = m->h_state.tr[m->h_state.tcur][0]; t.seed = m->seed; t.step = 1.0e-4f;
@@ -781,7 +789,7 @@ static void build_step(pi_model* m, float dt, Plan& P)
         aa.sz = std::sqrt(m->bgc[PI_B_kappa_vP] * dt);
         aa.seed = m->seed;
         aa.newc = m->newc; aa.stayf = m->stayf; aa.small = m->small; aa.qcap = m->qcap; aa.cnts = m->cnts;
-        aa.dstamp = m->dstamp; aa.dcell = m->dcell; aa.cap = m->cap;
+        aa.dstamp = m->dstamp; aa.dcell = m->dcell; aa.cap = m->cap; aa.dbd = m->dbd;
         /* up to 16384 blocks by default: with 8.4M individuals (two entries per
          * thread) ~5% faster than 4096 blocks; 32768 is no faster, 65536 much slower */
         P.add(K_ADV, "adv", grid_for(2 * m->ctot, 256, (int)m->shape[PI_S_ADV_BLOCKS]), aa);
@@ -823,8 +831,10 @@ static void build_step(pi_model* m, float dt, Plan& P)
         pa.bmA = m->bm; pa.bmD = m->bm + (size_t)m->nsp * m->nw; pa.G = m->G; pa.nw = m->nw;
         pa.newc = m->newc; pa.sflag = m->sflag;
         pa.con = m->con; pa.diag_num = m->diag; pa.dt = dt; pa.seed = m->seed;
-        pa.dstamp = m->dstamp; pa.dcell = m->dcell; pa.rstamp = m->rstamp; pa.rcons = m->rcons;
-        P.add(K_PHYS, "phys", grid_for(2 * m->ctot, 256, (int)m->shape[PI_S_PHYS_BLOCKS]), pa);
+        pa.dstamp = m->dstamp; pa.dcell = m->dcell; pa.rstamp = m->rstamp; pa.rcons = m->rcons; pa.dbd = m->dbd;
+        const long long phb = grid_for(2 * m->ctot, 256, (int)m->shape[PI_S_PHYS_BLOCKS]);
+        P.add(K_PHYS, "phys", m->mode == PI_MODE_QUOTA ? phb : 0, pa);
+        P.add(K_PHYSC, "phys", m->mode == PI_MODE_CARBON ? phb : 0, pa);
     }
     {
         PlkArgs ka = zeroed<PlkArgs>();

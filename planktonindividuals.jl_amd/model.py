@@ -51,12 +51,15 @@ def _mask(ac, v):
     return np.where(ac, v, np.copysign(np.float32(0), v)).astype(np.float32)
 
 
-def initialize_plankton(p, N, cap, grid, seed, sp):
-    """initialize_plankton! (QuotaMode/plankton_generation.jl:35-72) with the draws of
-    `randn!`/`rand!` taken from the shared stream (the reference's generator is
-    entropy-seeded).  `p` is the species' 40-float parameter row."""
+def initialize_plankton(p, N, cap, grid, seed, sp, mode=_abi.MODE_QUOTA):
+    """initialize_plankton! (QuotaMode/plankton_generation.jl:35-72,
+    CarbonMode/plankton_generation.jl:34-56) with the draws of `randn!`/`rand!`
+    taken from the shared stream (the reference's generator is entropy-seeded).
+    `p` is the species' parameter row.  CarbonMode: Bm and Chl from the size,
+    the damaged biomass Bd (in the Cq column) zero, no N or P quotas."""
     f32 = np.float32
-    q = {n: p[i] for i, n in enumerate(_abi.PARAM_NAMES)}
+    carbon = mode == _abi.MODE_CARBON
+    q = {n: p[i] for i, n in enumerate(_abi.CARBON_PARAM_NAMES if carbon else _abi.PARAM_NAMES)}
     slots = np.arange(cap, dtype=np.int64)
     ux, uy, uz = stream.uniform3(seed, 0, sp, stream.KIND_INIT_POS, slots)
     uc, un, up = stream.uniform3(seed, 0, sp, stream.KIND_INIT_QUOTA, slots)
@@ -74,10 +77,16 @@ def initialize_plankton(p, N, cap, grid, seed, sp):
     z = _mask(ac, uz * f32(grid.Nz))
     Sz = _mask(ac, np.maximum(f32(1.0), Sz * q["var"] + q["mean"]))
     Bm = (q["Cquota"] * Sz) * q["Nsuper"]
-    Cq = (uc * q["Cqmax"]) * Bm
-    Nq = un * (q["Nqmax"] * (Bm + Cq) - Bm * q["R_NC"])
-    Pq = up * (q["Pqmax"] * (Bm + Cq) - Bm * q["R_PC"])
-    Chl = Bm * q["Chl2Cint"]
+    if carbon:
+        Cq = np.zeros(cap, f32)
+        Nq = np.zeros(cap, f32)
+        Pq = np.zeros(cap, f32)
+        Chl = Bm * q["Chl2C"]
+    else:
+        Cq = (uc * q["Cqmax"]) * Bm
+        Nq = un * (q["Nqmax"] * (Bm + Cq) - Bm * q["R_NC"])
+        Pq = up * (q["Pqmax"] * (Bm + Cq) - Bm * q["R_PC"])
+        Chl = Bm * q["Chl2Cint"]
     # mask_individuals! over the first N slots (Individuals/utils.jl:89-100)
     Hx, Hy, Hz = grid.Hx, grid.Hy, grid.Hz
     n = slice(0, N)
@@ -104,6 +113,12 @@ class _SpeciesData:
             raise AttributeError(name)
         if name == "idx":
             return np.zeros(self._model.max_individuals, np.int64)
+        if self._model.spec.mode_id == _abi.MODE_CARBON:
+            # CarbonMode columns (CarbonMode/plankton_generation.jl:2-12): Bd is the Cq column
+            if name in ("Cq", "Nq", "Pq"):
+                raise AttributeError(name)
+            if name == "Bd":
+                name = "Cq"
         return self._model._native.field(self._sp, name)
 
 
@@ -172,8 +187,8 @@ class ModelSpec:
                  bgc_params=None, tracer_initial=None, phyto=None, abiotic=None, t=0.0,
                  max_candidates=25, seed=0, host_state=True):
         mode = mode if mode is not None else P.QuotaMode()
-        if not isinstance(mode, P.QuotaMode):
-            raise NotImplementedError(f"{mode}: only QuotaMode is implemented on the MI355X hot path")
+        if not isinstance(mode, (P.QuotaMode, P.CarbonMode)):
+            raise NotImplementedError(f"{mode}: QuotaMode and CarbonMode are implemented on the MI355X hot path")
         if abiotic is not None:
             raise NotImplementedError("abiotic particles are outside the MI355X hot path")
         if FT is not np.float32:
@@ -206,7 +221,8 @@ class ModelSpec:
         self.phyt_params = pp
         self.n_species = phyto.Nsp
         self.N = list(phyto.N)
-        self.species_params = P.species_param_matrix(pp, phyto.Nsp)
+        self.mode_id = P.mode_id(mode)
+        self.species_params = P.species_param_matrix(pp, phyto.Nsp, mode)
         self.bgc_vector = P.bgc_vector(bgc)
         self.seed = int(seed)
         self.t = np.float32(t)
@@ -218,7 +234,8 @@ class ModelSpec:
         self.tracer_ranges = None
         if self.host_state:
             self.initial_individuals = [initialize_plankton(self.species_params[s], int(phyto.N[s]),
-                                                            self.max_individuals, grid, self.seed, s)
+                                                            self.max_individuals, grid, self.seed, s,
+                                                            self.mode_id)
                                         for s in range(self.n_species)]
             self.initial_tracers = generate_tracers(grid, source, self.seed)
         else:
@@ -249,7 +266,7 @@ class PlanktonModel:
         self.species_params = spec.species_params
         self.t = spec.t
         self._native = NativeModel(self.grid, self.n_species, self.max_individuals, self.species_params,
-                                   spec.bgc_vector, spec.seed, (), device=arch.device)
+                                   spec.bgc_vector, spec.seed, (), device=arch.device, mode=spec.mode_id)
         if spec.host_state:
             for s in range(self.n_species):
                 self._native.upload_individuals(s, spec.initial_individuals[s])

@@ -305,29 +305,36 @@ def _fmt(spec, x):
 
 DYNAMICS_FIELDS = ("gen", "age", "Sz", "Bm", "Cq", "Nq", "Pq", "Chl")
 DYNAMICS_SPECS = ("%3.0f", "%2.2f", "%6.0f", "%1.2f", "%1.2f", "%1.2f", "%.8E", "%.8E", "%.8E", "%.8E", "%.8E")
+# CarbonMode (write_outputs.jl:142-157): day, hour, pop, gen, age, size and Bm averages
+CARBON_DYNAMICS_FIELDS = ("gen", "age", "Sz", "Bm")
+CARBON_DYNAMICS_SPECS = ("%3.0f", "%2.2f", "%6.0f", "%1.2f", "%1.2f", "%1.2f", "%.8E")
 
 
-def dynamics_line(t, columns):
-    """The QuotaMode log line (write_outputs.jl:98-117) from Float32 model time t
-    and the species' columns {field: array, "ac": Bool array}."""
+def dynamics_line(t, columns, carbon=False):
+    """The log line of QuotaMode (write_outputs.jl:98-117) or CarbonMode
+    (:142-157) from Float32 model time t and the species' columns {field:
+    array, "ac": Bool array}."""
+    fields, specs = (CARBON_DYNAMICS_FIELDS, CARBON_DYNAMICS_SPECS) if carbon else (DYNAMICS_FIELDS, DYNAMICS_SPECS)
     t = np.float32(t)
     ac = np.asarray(columns["ac"], bool)
     pop = int(np.count_nonzero(ac))                        # dot(ac, ac)
     with np.errstate(divide="ignore", invalid="ignore"):
-        aves = [dot_ac(columns[f], ac) / np.float32(pop) for f in DYNAMICS_FIELDS]
+        aves = [dot_ac(columns[f], ac) / np.float32(pop) for f in fields]
     day = np.float32(math.trunc(float(t) / 86400.0))       # t ÷ 86400
     hour = np.float32(np.fmod(t, np.float32(86400.0))) / np.float32(3600.0)
     vals = [day, hour, pop] + aves
-    return "  ".join(_fmt(s, v) for s, v in zip(DYNAMICS_SPECS, vals))
+    return "  ".join(_fmt(s, v) for s, v in zip(specs, vals))
 
 
 def write_species_dynamics(t, model, filepath):
-    """write_species_dynamics(t, phytos, filepath, ::QuotaMode): one line per species."""
+    """write_species_dynamics(t, phytos, filepath, mode): one line per species."""
+    carbon = getattr(getattr(model, "spec", None), "mode_id", 0) == 1
+    fields = CARBON_DYNAMICS_FIELDS if carbon else DYNAMICS_FIELDS
     for s in range(model.n_species):
         d = model.individuals.phytos[f"sp{s + 1}"].data
-        cols = {f: getattr(d, f) for f in DYNAMICS_FIELDS + ("ac",)}
+        cols = {f: getattr(d, f) for f in fields + ("ac",)}
         with open(os.path.join(filepath, "dynamic_species" + str(s + 1).zfill(3) + ".txt"), "a") as io:
-            io.write(dynamics_line(t, cols) + "\n")
+            io.write(dynamics_line(t, cols, carbon) + "\n")
 
 
 # ---------------------------------------------------------------- HDF5 files

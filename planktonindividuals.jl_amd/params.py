@@ -6,7 +6,7 @@ species' NamedTuple is built, QuotaMode/plankton_generation.jl:22-31).
 """
 import numpy as np
 
-from ._abi import PARAM_NAMES, BGC_NAMES
+from ._abi import PARAM_NAMES, BGC_NAMES, CARBON_PARAM_NAMES, MODE_QUOTA, MODE_CARBON
 
 
 class QuotaMode:
@@ -17,6 +17,8 @@ class QuotaMode:
 
 
 class CarbonMode:
+    """CarbonMode <: AbstractMode (src/PlanktonIndividuals.jl:65)."""
+
     def __repr__(self):
         return "CarbonMode()"
 
@@ -47,10 +49,31 @@ def bgc_params_default(FT=np.float32):
     return {k: FT(v) for k, v in d.items()}
 
 
+def mode_id(mode):
+    """The ABI's physiology mode of a mode object (QuotaMode when None)."""
+    if mode is None or isinstance(mode, QuotaMode):
+        return MODE_QUOTA
+    if isinstance(mode, CarbonMode):
+        return MODE_CARBON
+    raise NotImplementedError(f"{mode} physiology is not part of the MI355X hot path")
+
+
+def param_names(mode):
+    return CARBON_PARAM_NAMES if mode_id(mode) == MODE_CARBON else PARAM_NAMES
+
+
 def phyt_params_default(N=1, mode=None):
-    """phyt_params_default(N, ::QuotaMode) — param_default.jl:234-284 (Float64 vectors)."""
-    if mode is not None and not isinstance(mode, QuotaMode):
-        raise NotImplementedError(f"{mode} physiology is not part of the MI355X hot path")
+    """phyt_params_default(N, ::QuotaMode) — param_default.jl:234-284, and
+    phyt_params_default(N, ::CarbonMode) — param_default.jl:290-323 (Float64 vectors)."""
+    if mode_id(mode) == MODE_CARBON:
+        params = {
+            "Nsuper": [1], "Cquota": [1.8e-11], "mean": [1.2], "var": [0.3], "Chl2C": [0.10],
+            "α": [2.0e-2], "Φ": [4.0e-5], "Topt": [27.0], "Tmax": [30.0], "Ea": [5.3e4],
+            "PCmax": [4.2e-5], "respir": [1.2e-6], "f_T2B": [2.7e-6], "grz_P": [0.0], "dvid_P": [5e-5],
+            "dvid_type": [1], "dvid_reg": [1.9], "dvid_reg2": [12.0], "mort_P": [5e-5], "mort_reg": [0.5],
+            "grazFracC": [0.7], "mortFracC": [0.5], "thermal": [1.0], "is_bact": [0.0],
+        }
+        return params if N == 1 else generate_n_species_params(N, params)
     params = {
         "Nsuper": [1], "Cquota": [1.8e-11], "mean": [1.2], "var": [0.3], "Chl2Cint": [0.10],
         "α": [2.0e-2], "Φ": [4.0e-5], "Topt": [27.0], "Tmax": [30.0], "Ea": [5.3e4],
@@ -94,16 +117,18 @@ def update_phyt_params(tmp, FT=np.float32, *, N=1, mode=None):
     return parameters
 
 
-def species_param_matrix(phyt_params, nsp):
-    """(nsp, 40) float32 in param_names order: `p = NamedTuple{param_names}(FT.(tmp))`."""
+def species_param_matrix(phyt_params, nsp, mode=None):
+    """(nsp, 40) float32 in the mode's param_names order (QuotaMode: 40 names;
+    CarbonMode: 24, the rest zero): `p = NamedTuple{param_names}(FT.(tmp))`."""
+    names = param_names(mode)
     out = np.zeros((nsp, len(PARAM_NAMES)), dtype=np.float32)
     for s in range(nsp):
-        for q, name in enumerate(PARAM_NAMES):
+        for q, name in enumerate(names):
             if name not in phyt_params:
                 raise ValueError(f"PARAM: parameter not found {name}")
             vals = phyt_params[name]
             out[s, q] = np.float32(np.float64(vals[s]))
-    dt = out[:, PARAM_NAMES.index("dvid_type")]
+    dt = out[:, names.index("dvid_type")]
     if np.any((dt < 1) | (dt > 3)):
         raise ValueError("Wrong cell division type, must be in 1 to 3")
     return out

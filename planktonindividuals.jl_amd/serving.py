@@ -50,7 +50,7 @@ def simulation_for(spec, vel, steps, dt=60.0, device=0):
 
 def member_footprint(spec):
     return _native.footprint(spec.grid, spec.n_species, spec.max_individuals, spec.species_params,
-                             spec.bgc_vector, spec.seed)
+                             spec.bgc_vector, spec.seed, mode=spec.mode_id)
 
 
 class EnsembleServer:

@@ -80,6 +80,10 @@ def test_dynamics_line(out):
     assert want[3] == "3.50" and want[6] == "1.40000000E+01"
     empty = dict(cols, ac=np.zeros(n, bool))
     assert out.dynamics_line(t, empty) == "  ".join(["  1", "14.50", "     0"] + ["NaN"] * 8)
+    # CarbonMode: gen, age and size with %1.2f, Bm with %.8E and nothing else
+    cline = out.dynamics_line(t, cols, carbon=True)
+    assert cline == "  ".join(want[:6] + ["%.8E" % means[3]])
+    assert cline.endswith("1.40000000E+01") and len(cline.split()) == 7
 
 
 # ---------------------------------------------------------------- HDF5 container
