@@ -40,7 +40,8 @@ MISTAKES = [
     ("mortality with pow 2", "shape_dec(p.Sz[i], q[PI_P_mort_reg], L(1.0e-5f), 4)",
      "shape_dec(p.Sz[i], q[PI_P_mort_reg], L(1.0e-5f), 2)"),
     ("Sizer with pow 4", "shape_inc(Sz, q[PI_P_dvid_reg], L(2.0e-3f), 2)", "shape_inc(Sz, q[PI_P_dvid_reg], L(2.0e-3f), 4)"),
-    ("Timer hour in minutes", "FT hour = M::rem(t, L(86400)) / L(3600);", "FT hour = M::rem(t, L(86400)) / L(60);"),
+    ("Timer hour in minutes", "FT P_ = q[PI_P_dvid_P], Sz = p.Sz[i], v;\n            FT hour = M::rem(t, L(86400)) / L(3600);",
+     "FT P_ = q[PI_P_dvid_P], Sz = p.Sz[i], v;\n            FT hour = M::rem(t, L(86400)) / L(60);"),
     ("all species' pop when shared_graz != 1",
      "                std::fill(pop.begin(), pop.end(), (FT)0);\n                acc_counts(sp[s]);",
      "                std::fill(pop.begin(), pop.end(), (FT)0);\n                for (int s2 = 0; s2 < nsp; ++s2) acc_counts(sp[s2]);"),
@@ -54,10 +55,11 @@ MISTAKES = [
      "return L(0.5f) * (T + std::fabs(T)) * (c[I(i, j, k - 1)] + pm * d) +"),
     ("x flux from the wrong side", "return L(0.5f) * (T + std::fabs(T)) * (c[I(i - 1, j, k)] + pp * d) +",
      "return L(0.5f) * (T + std::fabs(T)) * (c[I(i, j, k)] + pp * d) +"),
-    ("division: daughter takes the first free slot", "p.idx[i] = free_ind[(size_t)(idc_int[i] - 1)];",
-     "p.idx[i] = free_ind[0];"),
-    ("PS without the Chl:C ratio", "M::exp(-aI / M::max(e30, PCm) * p.Chl[i] / M::max(e30, p.Bm[i]))",
-     "M::exp(-aI / M::max(e30, PCm) / M::max(e30, p.Bm[i]))"),
+    ("division: daughter takes the first free slot",
+     "/* get_tind_kernel! */\n            if (p.dvid[i] == L(1)) p.idx[i] = free_ind[(size_t)(idc_int[i] - 1)];",
+     "/* get_tind_kernel! */\n            if (p.dvid[i] == L(1)) p.idx[i] = free_ind[0];"),
+    ("PS without the Chl:C ratio", "FT PS = PCm * (one - M::exp(-aI / M::max(e30, PCm) * p.Chl[i] / M::max(e30, p.Bm[i])))",
+     "FT PS = PCm * (one - M::exp(-aI / M::max(e30, PCm) / M::max(e30, p.Bm[i])))"),
     ("NO3 uptake with the NH4 half-saturation", "tNO3[i] / M::max(e30, tNO3[i] + q[PI_P_KsatNO3])",
      "tNO3[i] / M::max(e30, tNO3[i] + q[PI_P_KsatNH4])"),
     ("rhoChl light gate at 1", "r = M::mask(M::isless(L(1.0e-1f), tpar[i]), r);", "r = M::mask(M::isless(L(1.0f), tpar[i]), r);"),
