@@ -200,6 +200,24 @@ pi_status pi_reset_diags(pi_model* m);
  * resets those accumulators. */
 pi_status pi_set_diag_tracers(pi_model* m, int32_t n, const int32_t* ids);
 
+/* Flux boundary conditions of the tracers: set_bc!(model; tracer, pos, bc_value)
+ * (HaloRegions/boundary_conditions.jl:12-29), applied to the tracer tendency
+ * by apply_bcs! (HaloRegions/apply_bcs.jl:1-99) after the forcing and before
+ * apply_tendency!: per tracer, in the order west, east, south, north, bottom,
+ * top, Gc += bc·ΔT·A/V at the first interior cell of west/south/bottom and
+ * Gc −= bc·ΔT·A/V at the last of east/north/top (A = Ax, Ay, Az of that cell,
+ * V its volume).  `kind` follows getbc's dispatch (boundary_conditions.jl:
+ * 31-40): PI_BCK_NONE removes it (`nothing`), PI_BCK_CONST takes values[0] (a
+ * Number), PI_BCK_FIELD a face array (a 2-D array: west/east (Ny,Nz),
+ * south/north (Nx,Nz), bottom/top (Nx,Ny), column-major), PI_BCK_SERIES nt
+ * face arrays one after the other (a 3-D array (face..., nΔT) read at the
+ * model's iteration: a step whose iteration exceeds nt is refused with
+ * PI_ERR_ARG, where the reference reads out of bounds).  Values are copied. */
+enum { PI_BC_WEST = 0, PI_BC_EAST, PI_BC_SOUTH, PI_BC_NORTH, PI_BC_BOTTOM, PI_BC_TOP, PI_NBC };
+enum { PI_BCK_NONE = 0, PI_BCK_CONST, PI_BCK_FIELD, PI_BCK_SERIES };
+pi_status pi_set_tracer_bc(pi_model* m, int32_t tracer, int32_t face, int32_t kind, const float* values,
+                           int64_t nt);
+
 /* Scalars. */
 pi_status pi_get_iteration(pi_model* m, int64_t* iteration);
 pi_status pi_count_active(pi_model* m, int32_t sp, int64_t* n);   /* dot(ac, ac) (models.jl:157-164) */

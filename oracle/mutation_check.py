@@ -19,6 +19,20 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "pi_oracle.cpp")
 
+BC_BOTTOM = """            if (bck[q][PI_BC_BOTTOM])
+                for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
+                    const int ii = i + Hx, jj = j + Hy, kk = Nz + Hz;
+                    G[I(ii, jj, kk)] += getbc(q, PI_BC_BOTTOM, i, j, iter) * dt * Az() / vol(kk);
+                }
+"""
+BC_TOP = """            if (bck[q][PI_BC_TOP])
+                for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
+                    const int ii = i + Hx, jj = j + Hy, kk = 1 + Hz;
+                    G[I(ii, jj, kk)] -= getbc(q, PI_BC_TOP, i, j, iter) * dt * Az() / vol(kk);
+                }
+"""
+BOTTOM_TOP, TOP_BOTTOM = BC_BOTTOM + BC_TOP, BC_TOP + BC_BOTTOM
+
 MISTAKES = [
     ("sinking dzC offset", "/ dzC_at(k) * dt;", "/ dzC_at(k + Hz) * dt;"),
     ("bottom flux_sink not zeroed at the last level", "planes_zero(c, 2, Nz + Hz, Hz + 1); }",
@@ -89,6 +103,19 @@ MISTAKES = [
      "q[PI_C_respir] * p.Bm[i] * c_tempFunc_PS(tT[i], q)"),
     ("carbon: Chl from Cquota", "p.Chl[i] = p.Bm[i] * q[PI_C_Chl2C];", "p.Chl[i] = p.Bm[i] * q[PI_C_Cquota];"),
     ("carbon: PS on the whole biomass", "M::max(z0, p.Bm[i] - p.Cq[i])", "M::max(z0, p.Bm[i])"),
+    ("flux bc: east adds", "G[I(ii, jj, kk)] -= getbc(q, PI_BC_EAST, j, k, iter)",
+     "G[I(ii, jj, kk)] += getbc(q, PI_BC_EAST, j, k, iter)"),
+    ("flux bc: south with Ax", "getbc(q, PI_BC_SOUTH, i, k, iter) * dt * Ay(kk)",
+     "getbc(q, PI_BC_SOUTH, i, k, iter) * dt * Ax(kk)"),
+    ("flux bc: west on the halo plane", "const int ii = 1 + Hx, jj = j + Hy, kk = k + Hz;",
+     "const int ii = Hx, jj = j + Hy, kk = k + Hz;"),
+    ("flux bc: bottom on the top level", "const int ii = i + Hx, jj = j + Hy, kk = Nz + Hz;",
+     "const int ii = i + Hx, jj = j + Hy, kk = 1 + Hz;"),
+    ("flux bc: face array transposed", "return bca[q][f][(int64_t)(a - 1) + na * ((int64_t)(b - 1) + nb * tt)];",
+     "return bca[q][f][(int64_t)(b - 1) + nb * ((int64_t)(a - 1) + na * tt)];"),
+    ("flux bc: series read at the first iteration",
+     "const int64_t tt = bck[q][f] == PI_BCK_SERIES ? (int64_t)(t - 1) : 0;", "const int64_t tt = 0;"),
+    ("flux bc: top before bottom", BOTTOM_TOP, TOP_BOTTOM),
 ]
 
 
@@ -111,7 +138,7 @@ def main(only):
             env = dict(os.environ, PI_ORACLE_LIB=lib)
             r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-m", "not gpu", "-p", "no:cacheprovider",
                                 "tests/test_oracle_pins.py", "tests/test_oracle_pins_physics.py",
-                                "tests/test_oracle_pins_carbon.py"],
+                                "tests/test_oracle_pins_carbon.py", "tests/test_oracle_pins_bcs.py"],
                                cwd=REPO, env=env, capture_output=True, text=True)
             first = next((l for l in r.stdout.splitlines() if l.startswith("FAILED")), "")
             (caught if r.returncode != 0 else missed).append(name)

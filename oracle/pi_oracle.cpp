@@ -95,6 +95,12 @@ struct Oracle : orc_model {
     int64_t ind_steps = 0;          /* Σ over steps of the individuals stepped (alive at the gather) */
     FT t = 0;
     std::vector<int> diag_tr;
+    /* flux boundary conditions per tracer and face (set_bc!, boundary_conditions.jl:12-29):
+     * getbc's kind, the Number, the face array(s) (column-major, face size x nt) */
+    int bck[PI_NTRACERS][PI_NBC] = {};
+    FT bcv[PI_NTRACERS][PI_NBC] = {};
+    V bca[PI_NTRACERS][PI_NBC];
+    int64_t bcnt[PI_NTRACERS][PI_NBC] = {};
     bool failed = false;
     int mode = PI_MODE_QUOTA;       /* PlanktonModel(...; mode) */
 
@@ -1039,6 +1045,50 @@ struct Oracle : orc_model {
                                excess * DFe * DFeFrac * lFe * dt;
         }
     }
+    /* getbc (HaloRegions/boundary_conditions.jl:31-40): bc, bc[a, b] or bc[a, b, t] */
+    FT getbc(int q, int f, int a, int b, int t) const {
+        if (bck[q][f] == PI_BCK_CONST) return bcv[q][f];
+        const int64_t na = f <= PI_BC_EAST ? Ny : Nx, nb = f <= PI_BC_NORTH ? Nz : Ny;
+        const int64_t tt = bck[q][f] == PI_BCK_SERIES ? (int64_t)(t - 1) : 0;
+        return bca[q][f][(int64_t)(a - 1) + na * ((int64_t)(b - 1) + nb * tt)];
+    }
+    /* apply_bcs! (HaloRegions/apply_bcs.jl:5-99): per tracer the west, east, south,
+     * north, bottom and top kernels in that order, each a no-op without a condition */
+    void apply_bcs(int iter, FT dt) {
+        for (int q = 0; q < PI_NTRACERS; ++q) {
+            V& G = Gcs[q];
+            if (bck[q][PI_BC_WEST])
+                for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) {
+                    const int ii = 1 + Hx, jj = j + Hy, kk = k + Hz;
+                    G[I(ii, jj, kk)] += getbc(q, PI_BC_WEST, j, k, iter) * dt * Ax(kk) / vol(kk);
+                }
+            if (bck[q][PI_BC_EAST])
+                for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) {
+                    const int ii = Nx + Hx, jj = j + Hy, kk = k + Hz;
+                    G[I(ii, jj, kk)] -= getbc(q, PI_BC_EAST, j, k, iter) * dt * Ax(kk) / vol(kk);
+                }
+            if (bck[q][PI_BC_SOUTH])
+                for (int k = 1; k <= Nz; ++k) for (int i = 1; i <= Nx; ++i) {
+                    const int ii = i + Hx, jj = 1 + Hy, kk = k + Hz;
+                    G[I(ii, jj, kk)] += getbc(q, PI_BC_SOUTH, i, k, iter) * dt * Ay(kk) / vol(kk);
+                }
+            if (bck[q][PI_BC_NORTH])
+                for (int k = 1; k <= Nz; ++k) for (int i = 1; i <= Nx; ++i) {
+                    const int ii = i + Hx, jj = Ny + Hy, kk = k + Hz;
+                    G[I(ii, jj, kk)] -= getbc(q, PI_BC_NORTH, i, k, iter) * dt * Ay(kk) / vol(kk);
+                }
+            if (bck[q][PI_BC_BOTTOM])
+                for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
+                    const int ii = i + Hx, jj = j + Hy, kk = Nz + Hz;
+                    G[I(ii, jj, kk)] += getbc(q, PI_BC_BOTTOM, i, j, iter) * dt * Az() / vol(kk);
+                }
+            if (bck[q][PI_BC_TOP])
+                for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
+                    const int ii = i + Hx, jj = j + Hy, kk = 1 + Hz;
+                    G[I(ii, jj, kk)] -= getbc(q, PI_BC_TOP, i, j, iter) * dt * Az() / vol(kk);
+                }
+        }
+    }
     /* tracer_update! (Biogeochemistry/tracer_update.jl:22-43) */
     void tracer_update(FT dt) {
         tracer_advection(dt);
@@ -1046,7 +1096,7 @@ struct Oracle : orc_model {
         tracer_sinking(dt);
         for (int q = 0; q < PI_NTRACERS; ++q) std::fill(ttemp[q].begin(), ttemp[q].end(), (FT)0);
         tracer_forcing(dt);
-        /* apply_bcs! is a no-op without flux boundary conditions (apply_bcs.jl:101-106) */
+        apply_bcs((int)iteration, dt);
         for (int q = 0; q < PI_NTRACERS; ++q)       /* apply_tendency_kernel! (:6-20) */
             OMP_FOR
             for (int k = Hz + 1; k <= Hz + Nz; ++k) for (int j = Hy + 1; j <= Hy + Ny; ++j)
@@ -1061,6 +1111,10 @@ struct Oracle : orc_model {
      * Model/time_step.jl:12-180 */
     pi_status step(float dt_in) {
         if (failed) return fail(PI_ERR_CAPACITY, "model is in an error state from an earlier step");
+        for (int q = 0; q < PI_NTRACERS; ++q)      /* bc[:, :, iteration] must exist */
+            for (int f = 0; f < PI_NBC; ++f)
+                if (bcnt[q][f] && iteration + 1 > bcnt[q][f])
+                    return fail(PI_ERR_ARG, "BC: time-dependent flux boundary condition shorter than the run");
         const FT dt = (FT)dt_in;
         for (const auto& p : sp) for (int64_t i = 0; i < cap; ++i) ind_steps += p.ac[i] ? 1 : 0;
         iteration += 1;
@@ -1182,6 +1236,25 @@ pi_status orc_set_temperature(orc_model* m, const float* tt) {
     DISPATCH(m, o->set_temperature(tt); return PI_OK);
 }
 pi_status orc_step(orc_model* m, float dt) { DISPATCH(m, return o->step(dt)); }
+template <typename FT>
+static pi_status set_bc_impl(Oracle<FT>* o, int32_t q, int32_t f, int32_t kind, const float* v, int64_t nt) {
+    if (q < 0 || q >= PI_NTRACERS) return fail(PI_ERR_ARG, "tracer id");
+    if (f < 0 || f >= PI_NBC) return fail(PI_ERR_ARG, "boundary position");
+    if (kind < PI_BCK_NONE || kind > PI_BCK_SERIES) return fail(PI_ERR_ARG, "boundary condition kind");
+    if (kind != PI_BCK_NONE && !v) return fail(PI_ERR_ARG, "null boundary condition values");
+    if (kind == PI_BCK_SERIES && nt < 1) return fail(PI_ERR_ARG, "BC: nt >= 1 required");
+    const int64_t n1 = f <= PI_BC_EAST ? o->Ny : o->Nx, n2 = f <= PI_BC_NORTH ? o->Nz : o->Ny;
+    const int64_t n = kind == PI_BCK_FIELD ? n1 * n2 : kind == PI_BCK_SERIES ? n1 * n2 * nt : 0;
+    o->bck[q][f] = kind;
+    o->bcv[q][f] = kind == PI_BCK_CONST ? (FT)v[0] : (FT)0;     /* FT.(bc_value) */
+    o->bca[q][f].assign(n, (FT)0);
+    for (int64_t a = 0; a < n; ++a) o->bca[q][f][a] = (FT)v[a];
+    o->bcnt[q][f] = kind == PI_BCK_SERIES ? nt : 0;
+    return PI_OK;
+}
+pi_status orc_set_tracer_bc(orc_model* m, int32_t q, int32_t f, int32_t kind, const float* v, int64_t nt) {
+    DISPATCH(m, return set_bc_impl(o, q, f, kind, v, nt));
+}
 
 template <typename FT, typename OUT>
 static pi_status field_impl(Oracle<FT>* o, int32_t s, int32_t f, OUT* out, int64_t n, bool ints_ok) {

@@ -13,7 +13,7 @@ from .params import (QuotaMode, CarbonMode, MacroMolecularMode, IronEnergyMode,
                      bgc_params_default, phyt_params_default, update_bgc_params, update_phyt_params)
 from .fields import default_tracer_init, generate_tracers, fill_halo_tracer
 from .model import (Architecture, HIPArch, GPU, isfunctional, phyto_setup, PlanktonModel, ModelSpec,
-                    TimeStep, initialize_plankton)
+                    TimeStep, initialize_plankton, set_bc, validate_bcs, BoundaryConditions)
 from .simulation import (PlanktonSimulation, PlanktonDiagnostics, update, default_PARF,
                          default_temperature)
 from ._native import CapacityError, NativeError, load_library, lib_path
@@ -25,9 +25,10 @@ from .output import PlanktonOutputWriter, write_output
 update_ = update          # `update!`
 TimeStep_ = TimeStep      # `TimeStep!`
 write_output_ = write_output   # `write_output!`
+set_bc_ = set_bc               # `set_bc!`
 
 __all__ = ["RectilinearGrid", "Periodic", "Bounded", "QuotaMode", "PlanktonModel", "HIPArch", "GPU",
            "phyto_setup", "PlanktonSimulation", "PlanktonDiagnostics", "update", "TimeStep",
            "bgc_params_default", "phyt_params_default", "update_bgc_params", "update_phyt_params",
            "default_tracer_init", "default_PARF", "default_temperature", "CapacityError",
-           "TRACER_NAMES", "isfunctional", "Ensemble", "PlanktonOutputWriter", "write_output"]
+           "TRACER_NAMES", "isfunctional", "Ensemble", "PlanktonOutputWriter", "write_output", "set_bc"]

@@ -39,6 +39,9 @@ GRID_ID = {"par": 0, "par₀": 1, "par0": 1, "Chl": 2, "pop": 3, "temp": 4, "PAR
            "u": 6, "v": 7, "w": 8, "diag_PAR": 9}
 PLK_BASE = 10
 DIAG_ID = {"num": 0, "graz": 1, "mort": 2, "dvid": 3}
+# flux boundary conditions (pi_set_tracer_bc): positions and getbc's kinds
+BC_POS = {"west": 0, "east": 1, "south": 2, "north": 3, "bottom": 4, "top": 5}
+BCK_NONE, BCK_CONST, BCK_FIELD, BCK_SERIES = range(4)
 # launch shapes (pi_set_launch_shape): name -> (id, default)
 SHAPES = {"xy_run": (0, 16), "final_run": (1, 16), "adv_blocks": (2, 16384), "phys_blocks": (3, 4096),
           "cellsum_per_cu": (4, 8), "plk_blocks": (5, 1 << 22)}
@@ -128,6 +131,7 @@ def declare(lib, prefix, model_t=C.c_void_p):
         "get_iteration": (C.c_int, [M, i64p]),
         "count_active": (C.c_int, [M, C.c_int32, i64p]),
         "individual_steps": (C.c_int, [M, i64p]),
+        "set_tracer_bc": (C.c_int, [M, C.c_int32, C.c_int32, C.c_int32, fp, C.c_int64]),
     }
     for name, (res, args) in proto.items():
         f = getattr(lib, prefix + name)

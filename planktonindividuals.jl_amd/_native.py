@@ -155,6 +155,28 @@ class Handle:
     def step(self, dt):
         self._call("step", C.c_float(dt))
 
+    def set_tracer_bc(self, tid, pos, bc_value):
+        """Flux boundary condition of tracer `tid` on face `pos` (set_bc!,
+        boundary_conditions.jl:12-29): None, a number, a face array (2-D) or a
+        face array per iteration (3-D), converted to Float32 like FT.(bc_value)."""
+        f = _abi.BC_POS[pos]
+        if bc_value is None:
+            self._call("set_tracer_bc", tid, f, _abi.BCK_NONE, None, C.c_int64(0))
+            return
+        a = np.asarray(bc_value, dtype=np.float32)
+        if a.ndim == 0:
+            v = np.ascontiguousarray(a.reshape(1))
+            self._call("set_tracer_bc", tid, f, _abi.BCK_CONST, v.ctypes.data_as(_abi.fp), C.c_int64(0))
+            return
+        g = self.grid
+        face = (g.Ny, g.Nz) if pos in ("west", "east") else (g.Nx, g.Nz) if pos in ("south", "north") else (g.Nx, g.Ny)
+        if a.ndim not in (2, 3) or a.shape[:2] != face:
+            raise ValueError(f"BC: grid mismatch, size(bc) must equal to {face} for a constant flux boundary "
+                             f"condition or {face + ('nΔT',)} for a time-dependent one.")
+        v = np.asfortranarray(a)
+        kind, nt = (_abi.BCK_FIELD, 0) if a.ndim == 2 else (_abi.BCK_SERIES, a.shape[2])
+        self._call("set_tracer_bc", tid, f, kind, v.ctypes.data_as(_abi.fp), C.c_int64(nt))
+
     # ---- downloads
     def field(self, sp, name):
         fid = _abi.FIELD_ID[name]

@@ -109,6 +109,17 @@ struct FinalArgs {
     float lPOC, lmin, ldust, DFeFrac, kdiss, lig, lFe;
     int fill_halo;                    /* also write the halo cells of the new tracers (halo 2, no aliasing) */
     int run;                          /* levels per column run (launch shape, any value >= 1) */
+    const struct BcTab* bc;           /* flux boundary conditions (the k_final2/3 instantiations) */
+    uint8_t bcm[PI_NTRACERS];         /* per tracer, bit f set = a boundary condition on face f */
+};
+
+/* Flux boundary conditions of the tracers (set_bc!, boundary_conditions.jl:12-29),
+ * in device memory: per tracer and face (PI_BC_WEST..PI_BC_TOP) getbc's kind,
+ * the Number, and the face array(s) (column-major, face size x nt) */
+struct BcTab {
+    int32_t kind[PI_NTRACERS][PI_NBC];
+    float val[PI_NTRACERS][PI_NBC];
+    const float* arr[PI_NTRACERS][PI_NBC];
 };
 
 struct ScanIn {
@@ -180,7 +191,7 @@ enum KId {
     K_VELHALF, K_MARKS, K_SCAN_P, K_SCAN_F, K_CAPCHECK, K_ADV, K_CELLCOUNT, K_PLACE_STAY,
     K_PLACE_SMALL, K_MIXSORT, K_BIGCELLS, K_CELLSUM, K_CELLSUM_LONG, K_PAR, K_PHYS, K_PLK, K_PLK_LONG,
     K_SWEEPX0, K_SWEEPX1, K_XY0, K_XY1, K_SWEEPY0, K_SWEEPY1, K_HALOCOPY, K_FILL, K_FINAL0, K_FINAL1, K_ACCUM,
-    K_FINISH, K_XYW0, K_XYW1, K_PHYSC, K_NKID
+    K_FINISH, K_XYW0, K_XYW1, K_PHYSC, K_FINAL2, K_FINAL3, K_NKID
 };
 /* plan markers (not kernels): fork to / join from the side stream */
 enum { K_FORK = K_NKID, K_JOIN = K_NKID + 1 };

@@ -375,12 +375,49 @@ __device__ __forceinline__ constexpr bool has_plk(int q)
            q == PI_DOP || q == PI_POC || q == PI_PON || q == PI_POP;
 }
 
-struct ZCell {      /* cell cc (level k) above face c */
+struct ZCell {      /* cell cc (level k; column i, j; all 1-based padded) above face c */
     uint32_t cc, c;
-    int k;
+    int k, i, j;
+    int it;         /* the model's iteration during this step (flux boundary conditions) */
     float dT, coef, khAx, khAy, inv_vol, dzf, vol;
     double ydzf, yvol;
 };
+
+/* getbc (HaloRegions/boundary_conditions.jl:31-40) of tracer q on face f at
+ * face position (p, r) (1-based interior, face array size np x nr) */
+__device__ __forceinline__ float bc_at(const BcTab* T, int q, int f, int p, int r, int np, int nr, int it)
+{
+    const int kind = T->kind[q][f];
+    if (kind == PI_BCK_CONST) return T->val[q][f];
+    const long long t = kind == PI_BCK_SERIES ? (long long)(it - 1) : 0ll;
+    return T->arr[q][f][(long long)(p - 1) + (long long)np * ((long long)(r - 1) + (long long)nr * t)];
+}
+
+/* apply_bcs! (HaloRegions/apply_bcs.jl:5-47, 87-99) for tracer q at the cell of
+ * z: in the order west, east, south, north, bottom, top, Gc += or -=
+ * ((bc·ΔT)·A)/V at the interior cells next to the face; only faces with a
+ * boundary condition (bit f of a.bcm[q]) are looked at */
+__device__ __forceinline__ float flux_bcs(const FinalArgs& a, const DevGrid& g, int q, const ZCell& z, float G)
+{
+    const BcTab* T = a.bc;
+    const uint32_t m = a.bcm[q];
+    const int i = z.i - g.Hx, j = z.j - g.Hy, k = z.k - g.Hz;
+    const float dt = a.dt;
+    const float Ax = g.dy * dzF_at(g, z.k), Ay = g.dx * dzF_at(g, z.k), Az = g.dx * g.dy;
+    if ((m & (1u << PI_BC_WEST)) && i == 1)
+        G = G + divc((bc_at(T, q, PI_BC_WEST, j, k, g.Ny, g.Nz, z.it) * dt) * Ax, z.vol, z.yvol);
+    if ((m & (1u << PI_BC_EAST)) && i == g.Nx)
+        G = G - divc((bc_at(T, q, PI_BC_EAST, j, k, g.Ny, g.Nz, z.it) * dt) * Ax, z.vol, z.yvol);
+    if ((m & (1u << PI_BC_SOUTH)) && j == 1)
+        G = G + divc((bc_at(T, q, PI_BC_SOUTH, i, k, g.Nx, g.Nz, z.it) * dt) * Ay, z.vol, z.yvol);
+    if ((m & (1u << PI_BC_NORTH)) && j == g.Ny)
+        G = G - divc((bc_at(T, q, PI_BC_NORTH, i, k, g.Nx, g.Nz, z.it) * dt) * Ay, z.vol, z.yvol);
+    if ((m & (1u << PI_BC_BOTTOM)) && k == g.Nz)
+        G = G + divc((bc_at(T, q, PI_BC_BOTTOM, i, j, g.Nx, g.Ny, z.it) * dt) * Az, z.vol, z.yvol);
+    if ((m & (1u << PI_BC_TOP)) && k == 1)
+        G = G - divc((bc_at(T, q, PI_BC_TOP, i, j, g.Nx, g.Ny, z.it) * dt) * Az, z.vol, z.yvol);
+    return G;
+}
 
 /* Gcs, diffusion, sinking, forcing and apply_tendency (and the diffusive flux
  * of the face below) of tracer q at the cell above face z.c, from its operands:
@@ -388,7 +425,7 @@ struct ZCell {      /* cell cc (level k) above face c */
  * pv = the z-sweep input at the cell, fzq/fq = advective fluxes of its top and
  * bottom faces, dzq = diffusive flux of its top face, csq = c above the cell
  * (sinking tracers), pk = plk. */
-template <bool LAND>
+template <bool LAND, bool BCS>
 __device__ __forceinline__ void cell_update(const FinalArgs& a, const DevGrid& g, const ZCell& z, int q, const float* tt,
                                             float cq, float cn, float xm, float xp, float ym, float yp, float pv,
                                             float fzq, float fq, float dzq, float csq, float pk,
@@ -453,6 +490,13 @@ __device__ __forceinline__ void cell_update(const FinalArgs& a, const DevGrid& g
         }
         default: break;
     }
+    /* apply_bcs! (tracer_update.jl:37-38): boundary cells of tracers with a flux
+     * boundary condition */
+    if (BCS) {
+        const bool edge = z.i == g.Hx + 1 || z.i == g.Hx + g.Nx || z.j == g.Hy + 1 || z.j == g.Hy + g.Ny ||
+                          z.k == g.Hz + 1 || z.k == g.Hz + g.Nz;
+        if (edge && a.bcm[q]) G = flux_bcs(a, g, q, z, G);
+    }
     /* apply_tendency_kernel! (tracer_update.jl:6-20); the tracers without
      * particle exchange add a zero plk */
     out = cq + (G + divc(pk, z.vol, z.yvol));
@@ -515,7 +559,7 @@ __device__ __forceinline__ void face_fluxes_grp(const DevGrid& g, const Face& F,
 /* the tracers of group G at one cell: the fluxes of the face below it, the
  * updates, the stores and the carried state (cu, cs, fz, dz; indexed by the
  * position in the group, cs by sink_index) */
-template <bool LAND, int G>
+template <bool LAND, bool BCS, int G>
 __device__ __forceinline__ void final_grp(const FinalArgs& a, const DevGrid& g, const float* C0, float* OUT,
                                           const Face& Fb, bool zero, const ZCell& z, const float* tt, float* cu,
                                           float* fz, float* dz, float* cs, float (*pw)[4], const HaloImg& hi)
@@ -533,7 +577,7 @@ __device__ __forceinline__ void final_grp(const FinalArgs& a, const DevGrid& g, 
         const float pk = has_plk(q) ? ld(fld(a.plk, g, q), cc) : 0.0f;
         const int si = sink_index(q);
         /* pw[i][1] = the z-sweep input at cc */
-        cell_update<LAND>(a, g, z, q, tt, cu[i], cnv[i], ld(co, cc - 1), ld(co, cc + 1), ld(co, cc - sy),
+        cell_update<LAND, BCS>(a, g, z, q, tt, cu[i], cnv[i], ld(co, cc - 1), ld(co, cc + 1), ld(co, cc - sy),
                           ld(co, cc + sy), pw[i][1], fz[i], f[i], dz[i], si >= 0 ? cs[si] : 0.0f,
                           pk, outv[i], dv[i]);
     }
@@ -570,9 +614,9 @@ __device__ __forceinline__ void final_grp(const FinalArgs& a, const DevGrid& g, 
 }
 
 /* one column run (cells k0..k1-1 of column i, j) for the tracers of group G */
-template <bool LAND, int G>
+template <bool LAND, bool BCS, int G>
 __device__ __forceinline__ void final_run(const FinalArgs& a, const DevGrid& g, const float* C0, float* OUT, int i,
-                                          int j, int r)
+                                          int j, int r, int it)
 {
     const uint32_t sz = (uint32_t)g.Px * (uint32_t)g.Py;
     const float dt = a.dt;
@@ -621,7 +665,7 @@ __device__ __forceinline__ void final_run(const FinalArgs& a, const DevGrid& g, 
             pw[x][3] = ld(fld(a.prev, g, fgrp(G, x)), c + sz);
         }
         ZCell z;
-        z.c = c; z.cc = c - sz; z.k = kk - 1;
+        z.c = c; z.cc = c - sz; z.k = kk - 1; z.i = i; z.j = j; z.it = it;
         const Face Fb = face(g, ld(a.w, c), dt, dzC_at(g, kk), g.ydzC[kk - 1], Az);
         const bool zero = kk == g.Nz + g.Hz + 1 && g.tz == PI_BOUNDED;
         z.dT = lmask<LAND>(g, z.cc, c, Fa.T - Fb.T);
@@ -637,16 +681,17 @@ __device__ __forceinline__ void final_run(const FinalArgs& a, const DevGrid& g, 
 #pragma unroll
         for (int x = 0; x < fg_n(G); ++x) tt[fgrp(G, x)] = dmaxf(0.0f, cu[x]);
         if (G == 3) tt[PI_POC] = dmaxf(0.0f, ld(fld(C0, g, PI_POC), z.cc));
-        final_grp<LAND, G>(a, g, C0, OUT, Fb, zero, z, tt, cu, fz, dz, cs, pw, hi);
+        final_grp<LAND, BCS, G>(a, g, C0, OUT, Fb, zero, z, tt, cu, fz, dz, cs, pw, hi);
         Fa = Fb;
     }
 }
 
-template <bool LAND>
+template <bool LAND, bool BCS>
 __device__ __forceinline__ void k_final_body(const FinalArgs& a, Blk K)
 {
     const DevState* S = a.st;
     if (S->err) return;
+    const int it = BCS ? (int)(S->iteration + 1) : 0;   /* model.iteration during TimeStep! */
     const DevGrid& g = a.g;
     const int nrun = (g.Nz + a.run - 1) / a.run;
     const long long nthr = (long long)g.Nx * g.Ny * nrun * FG_NGRP;
@@ -660,16 +705,23 @@ __device__ __forceinline__ void k_final_body(const FinalArgs& a, Blk K)
         const int j = (int)((u >> 2) % g.Ny) + 1 + g.Hy;
         const int r = (int)((u >> 2) / g.Ny);
         switch (G) {
-            case 0: final_run<LAND, 0>(a, g, C0, OUT, i, j, r); break;
-            case 1: final_run<LAND, 1>(a, g, C0, OUT, i, j, r); break;
-            case 2: final_run<LAND, 2>(a, g, C0, OUT, i, j, r); break;
-            default: final_run<LAND, 3>(a, g, C0, OUT, i, j, r); break;
+            case 0: final_run<LAND, BCS, 0>(a, g, C0, OUT, i, j, r, it); break;
+            case 1: final_run<LAND, BCS, 1>(a, g, C0, OUT, i, j, r, it); break;
+            case 2: final_run<LAND, BCS, 2>(a, g, C0, OUT, i, j, r, it); break;
+            default: final_run<LAND, BCS, 3>(a, g, C0, OUT, i, j, r, it); break;
         }
     }
 }
 /* at least 5 waves per SIMD: 0.398 -> 0.383 ms on the 256x256x64 run (4 waves at 108 VGPRs) */
-PI_KERNEL3W(k_final0, k_final_body<false>, FinalArgs, 256, xcd_block(), 5)
-PI_KERNEL3W(k_final1, k_final_body<true>, FinalArgs, 256, xcd_block(), 5)
+__device__ __forceinline__ void k_final0_body(const FinalArgs& a, Blk K) { k_final_body<false, false>(a, K); }
+__device__ __forceinline__ void k_final1_body(const FinalArgs& a, Blk K) { k_final_body<true, false>(a, K); }
+/* the same with flux boundary conditions (apply_bcs!) */
+__device__ __forceinline__ void k_final2_body(const FinalArgs& a, Blk K) { k_final_body<false, true>(a, K); }
+__device__ __forceinline__ void k_final3_body(const FinalArgs& a, Blk K) { k_final_body<true, true>(a, K); }
+PI_KERNEL3W(k_final0, k_final0_body, FinalArgs, 256, xcd_block(), 5)
+PI_KERNEL3W(k_final1, k_final1_body, FinalArgs, 256, xcd_block(), 5)
+PI_KERNEL3W(k_final2, k_final2_body, FinalArgs, 256, xcd_block(), 5)
+PI_KERNEL3W(k_final3, k_final3_body, FinalArgs, 256, xcd_block(), 5)
 
 /* ------------------------------------------------- fused x + y sweeps (k_xy)
  * calc_Gcsˣ! + multi_dim_x! + fill_halo_tracer! + calc_Gcsʸ! + multi_dim_y!
@@ -955,6 +1007,8 @@ void reg_tracers(KInfo* k)
     PI_REG(k, K_HALOCOPY, k_halo_copy, FillArgs, 256, true);
     PI_REG(k, K_FINAL0, k_final0, FinalArgs, 256, true);
     PI_REG(k, K_FINAL1, k_final1, FinalArgs, 256, true);
+    PI_REG(k, K_FINAL2, k_final2, FinalArgs, 256, true);
+    PI_REG(k, K_FINAL3, k_final3, FinalArgs, 256, true);
     PI_REG(k, K_XY0, k_xy0, XYArgs, XY_T, true);
     PI_REG(k, K_XY1, k_xy1, XYArgs, XY_T, true);
     PI_REG(k, K_XYW0, k_xyw0, XYArgs, XY_W, true);

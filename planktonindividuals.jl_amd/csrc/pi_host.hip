@@ -137,6 +137,13 @@ struct pi_model {
     long long shape[PI_NSHAPE] = {16, 16, 16384, 4096, 8, 1 << 22};   /* pi_set_launch_shape */
     int mode = PI_MODE_QUOTA;  /* physiology (pi_model_desc::mode) */
     float* dbd = nullptr;      /* CarbonMode: per slot, the Bd of an occupant dying in the step */
+    /* flux boundary conditions of the tracers (pi_set_tracer_bc): the table on
+     * the host and on the device, each face array's own buffer, the nt of each
+     * time-dependent one */
+    BcTab h_bc{};
+    BcTab* d_bc = nullptr;
+    float* bc_buf[PI_NTRACERS][PI_NBC]{};
+    long long bc_nt[PI_NTRACERS][PI_NBC]{};
 };
 
 static hipError_t dalloc(pi_model* m, void** p, size_t bytes)
@@ -308,6 +315,7 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     }
     DALLOC(m->d_params, m->h_params.size() * 4);
     DEVCALL(hipMemcpy(m->d_params, m->h_params.data(), m->h_params.size() * 4, hipMemcpyHostToDevice));
+    DALLOC(m->d_bc, sizeof(BcTab));    /* zeroed: no flux boundary conditions */
     /* store x2 */
     DevState& hs = m->h_state;
     memset(&hs, 0, sizeof hs);
@@ -481,6 +489,7 @@ pi_status pi_model_destroy(pi_model* m)
     drop_graph(m);
     for (auto& t : m->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (void* p : m->allocs) (void)hipFree(p);
+    for (auto& row : m->bc_buf) for (float* p : row) if (p) (void)hipFree(p);
     (void)hipStreamDestroy(m->stream);
     (void)hipStreamDestroy(m->side);
     (void)hipEventDestroy(m->ev_fork);
@@ -682,6 +691,51 @@ pi_status pi_set_temperature(pi_model* m, const float* t)
     return PI_OK;
 }
 
+pi_status pi_set_tracer_bc(pi_model* m, int32_t q, int32_t f, int32_t kind, const float* values, int64_t nt)
+{
+    if (!m) return fail(PI_ERR_ARG, "null model");
+    if (q < 0 || q >= PI_NTRACERS) return fail(PI_ERR_ARG, "tracer id");
+    if (f < 0 || f >= PI_NBC) return fail(PI_ERR_ARG, "boundary position");
+    if (kind < PI_BCK_NONE || kind > PI_BCK_SERIES) return fail(PI_ERR_ARG, "boundary condition kind");
+    if (kind != PI_BCK_NONE && !values) return fail(PI_ERR_ARG, "null boundary condition values");
+    if (kind == PI_BCK_SERIES && nt < 1)
+        return fail(PI_ERR_ARG, "BC: a time-dependent flux boundary condition needs nt >= 1");
+    const DevGrid& g = m->g;
+    const long long n1 = f <= PI_BC_EAST ? g.Ny : g.Nx, n2 = f <= PI_BC_NORTH ? g.Nz : g.Ny;
+    const long long n = kind == PI_BCK_FIELD ? n1 * n2 : kind == PI_BCK_SERIES ? n1 * n2 * nt : 0;
+    HIPCHECK(hipStreamSynchronize(m->stream));      /* a step may still read the old values */
+    if (m->bc_buf[q][f]) {
+        HIPCHECK(hipFree(m->bc_buf[q][f]));
+        m->bc_buf[q][f] = nullptr;
+    }
+    m->h_bc.kind[q][f] = kind;
+    m->h_bc.val[q][f] = kind == PI_BCK_CONST ? values[0] : 0.0f;
+    m->h_bc.arr[q][f] = nullptr;
+    m->bc_nt[q][f] = kind == PI_BCK_SERIES ? nt : 0;
+    if (n) {
+        float* d = nullptr;
+        HIPCHECK(hipMalloc((void**)&d, (size_t)n * 4));
+        m->bc_buf[q][f] = d;
+        HIPCHECK(hipMemcpy(d, values, (size_t)n * 4, hipMemcpyHostToDevice));
+        m->h_bc.arr[q][f] = d;
+    }
+    HIPCHECK(hipMemcpy(m->d_bc, &m->h_bc, sizeof(BcTab), hipMemcpyHostToDevice));
+    drop_graph(m);
+    return PI_OK;
+}
+
+/* A step reads a time-dependent flux boundary condition at the model's
+ * iteration: refuse `n` more steps that would pass the end of one. */
+static pi_status bc_steps_ok(const pi_model* m, int64_t n)
+{
+    for (int q = 0; q < PI_NTRACERS; ++q)
+        for (int f = 0; f < PI_NBC; ++f)
+            if (m->bc_nt[q][f] && m->host_iter + n > m->bc_nt[q][f])
+                return fail(PI_ERR_ARG, "BC: the time-dependent flux boundary condition of a tracer has no value "
+                                        "for this iteration (size(bc, 3) < iteration)");
+    return PI_OK;
+}
+
 /* ------------------------------------------------------------------ step */
 
 static void plan_scan(Plan& P, const char* grp, const ScanIn& in, uint32_t* partials, uint32_t* out)
@@ -865,9 +919,17 @@ static void build_step(pi_model* m, float dt, Plan& P)
         fn.lFe = m->bgc[PI_B_lambda_Fe];
         fn.fill_halo = fxy ? 1 : 0;       /* k_final writes the halo images itself */
         fn.run = (int)m->shape[PI_S_FINAL_RUN];
+        fn.bc = m->d_bc;
+        bool bcs = false;
+        for (int q = 0; q < PI_NTRACERS; ++q) {
+            for (int f = 0; f < PI_NBC; ++f) fn.bcm[q] |= (uint8_t)((m->h_bc.kind[q][f] != PI_BCK_NONE) << f);
+            bcs |= fn.bcm[q] != 0;
+        }
         const long long fb = grid_for(final_items(g, fn.run), 256, 8192);
-        P.add(K_FINAL0, "final", land ? 0 : fb, fn);
-        P.add(K_FINAL1, "final", land ? fb : 0, fn);
+        P.add(K_FINAL0, "final", !land && !bcs ? fb : 0, fn);
+        P.add(K_FINAL1, "final", land && !bcs ? fb : 0, fn);
+        P.add(K_FINAL2, "final", !land && bcs ? fb : 0, fn);
+        P.add(K_FINAL3, "final", land && bcs ? fb : 0, fn);
         /* fill_halo_tracer! of the new tracers: x, then y, then z over whole planes */
         fa.f = nullptr;
         fa.lines_full = 1;
@@ -935,6 +997,8 @@ pi_status pi_step(pi_model* m, float dt)
     pi_status s = rebuild_if_dirty(m);
     if (s != PI_OK) return s;
     if (m->h_state.err) return fail(PI_ERR_CAPACITY, "model is in an error state from an earlier step");
+    s = bc_steps_ok(m, 1);
+    if (s != PI_OK) return s;
     return enqueue_step(m, dt);
 }
 
@@ -957,6 +1021,8 @@ pi_status pi_run(pi_model* m, float dt, int64_t n)
     pi_status s = rebuild_if_dirty(m);
     if (s != PI_OK) return s;
     if (m->h_state.err) return fail(PI_ERR_CAPACITY, "model is in an error state from an earlier step");
+    s = bc_steps_ok(m, n);
+    if (s != PI_OK) return s;
     for (int64_t i = 0; i < n; ++i) {
         if (m->timing || !m->vel_same || !graphs_allowed()) {
             s = enqueue_step(m, dt);
@@ -1443,6 +1509,10 @@ pi_status pi_batch_run(pi_batch* b, float dt, int64_t n)
     if (!b) return fail(PI_ERR_ARG, "null batch");
     const size_t nm = b->mem.size();
     if (nm == 0) return PI_OK;
+    for (pi_model* md : b->mem) {
+        pi_status s = bc_steps_ok(md, n);
+        if (s != PI_OK) return s;
+    }
     b->next.resize(nm);
     for (int64_t i = 0; i < n; ++i) {
         bool same = b->tables;
@@ -1492,6 +1562,10 @@ pi_status pi_batch_run_noalloc(pi_batch* b, float dt, int64_t n)
     if (!b) return fail(PI_ERR_ARG, "null batch");
     const size_t nm = b->mem.size();
     if (nm == 0 || n <= 0) return PI_OK;
+    for (pi_model* md : b->mem) {
+        pi_status s = bc_steps_ok(md, n);
+        if (s != PI_OK) return s;
+    }
     int cur = 0;
     HIPCHECK(hipGetDevice(&cur));
     if (cur != b->dev) HIPCHECK(hipSetDevice(b->dev));

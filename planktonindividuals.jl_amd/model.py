@@ -142,9 +142,18 @@ class _Individuals:
         self.abiotics = {}
 
 
+class BoundaryConditions:
+    """BoundaryConditions (src/PlanktonIndividuals.jl:27-34): the flux boundary
+    condition on each face, `None` (nothing), a number or an array."""
+
+    def __init__(self):
+        self.west = self.east = self.south = self.north = self.bottom = self.top = None
+
+
 class _Field:
     def __init__(self, model, tid):
         self._m, self._t = model, tid
+        self.bc = BoundaryConditions()
 
     @property
     def data(self):
@@ -294,6 +303,44 @@ class PlanktonModel:
                 f"├── phytoplankton: {self.n_species} species with {N} individuals for each species\n"
                 "├── No abiotic particles available\n"
                 f"└── maximum number of individuals: {self.max_individuals} per species")
+
+
+def set_bc(model, *, tracer, pos, bc_value):
+    """set_bc!(model; tracer, pos, bc_value) — HaloRegions/boundary_conditions.jl:12-29:
+    the flux boundary condition of `tracer` on `pos` (:west, :east, :south,
+    :north, :bottom, :top) becomes FT.(bc_value): a number, a face array (2-D:
+    (Ny, Nz) west/east, (Nx, Nz) south/north, (Nx, Ny) bottom/top) or one face
+    array per iteration (3-D, read at the model's iteration).  The value is
+    applied to the tracer's tendency every step by apply_bcs! (apply_bcs.jl)."""
+    tracer, pos = str(tracer).lstrip(":"), str(pos).lstrip(":")
+    if tracer not in _abi.TRACER_ID:
+        raise AssertionError("tracer in tracer_names")
+    if pos not in _abi.BC_POS:
+        raise ValueError(f"type BoundaryConditions has no field {pos}")
+    if bc_value is None:                       # back to `nothing`
+        v = None
+    else:
+        v = np.float32(bc_value) if np.ndim(bc_value) == 0 else np.asarray(bc_value, dtype=np.float32)
+    model._native.set_tracer_bc(_abi.TRACER_ID[tracer], pos, v)
+    setattr(model.tracers[tracer].bc, pos, v)
+
+
+def validate_bcs(tracers, grid, nΔT):
+    """validate_bcs (boundary_conditions.jl:62-83): every array boundary condition
+    has the face's size, and a time-dependent one nΔT entries."""
+    sizes = {"west": (grid.Ny, grid.Nz), "east": (grid.Ny, grid.Nz), "south": (grid.Nx, grid.Nz),
+             "north": (grid.Nx, grid.Nz), "bottom": (grid.Nx, grid.Ny), "top": (grid.Nx, grid.Ny)}
+    for name in tracers:
+        for pos, size in sizes.items():
+            bc = getattr(tracers[name].bc, pos)
+            if bc is None or np.ndim(bc) == 0:
+                continue
+            if np.ndim(bc) == 2 and np.shape(bc) != size:
+                raise ValueError(f"BC: grid mismatch, size(bc) must equal to {size} for a constant flux "
+                                 "boundary condition.")
+            if np.ndim(bc) == 3 and np.shape(bc) != size + (nΔT,):
+                raise ValueError(f"BC: grid mismatch, size(bc) must equal to {size + (nΔT,)} for a "
+                                 "time-dependent flux boundary condition.")
 
 
 def TimeStep(model, ΔT, diags=None):

@@ -8,6 +8,7 @@ import numpy as np
 from . import _abi
 from ._abi import BOUNDED
 from .output import write_output
+from .model import validate_bcs
 
 
 def default_PARF(grid, ΔT, iterations):
@@ -121,6 +122,7 @@ class PlanktonSimulation:
         self.validate_velocity()
         self.validate_PARF()
         self.validate_temp()
+        validate_bcs(model.tracers, g, self.iterations)       # simulations.jl:81
 
     # ---- validators (Simulation/utils.jl:36-127)
     def _check_step(self, ΔT_in, what):

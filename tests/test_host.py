@@ -109,6 +109,7 @@ def test_simulation_validators(pi):
     m = Fake()
     m.grid = g
     m.n_species = 1
+    m.tracers = {}
 
     class N:
         def set_diag_tracers(self, ids):
