@@ -124,6 +124,16 @@ typedef struct pi_grid_desc {
     const uint8_t* landmask;     /* padded Bool landmask (Grids/utils.jl:92-129); NULL = no land */
 } pi_grid_desc;
 
+/* LatLonGrid (Grids/lat_lon_grid.jl:1-37, built by LatLonGrid(...) at :70-200
+ * or LoadLatLonGrid(...) at :230-327): spacings, areas and volumes as arrays
+ * with halos, column-major.  Replace Δx, Δy of pi_grid_desc in every formula
+ * of Grids/areas_volumes_spacings.jl:24-38 (ΔxC(i,j,k) = dxC[i,j], Ax(i,j,k) =
+ * Ax[i,j,k], volume(i,j,k) = Vol[i,j,k], ...); dzC, dzF stay in pi_grid_desc. */
+typedef struct pi_metrics {
+    const float *dxC, *dyC, *dxF, *dyF, *Az;   /* (Nx+2Hx, Ny+2Hy) */
+    const float *Ax, *Ay, *Vol;                /* (Nx+2Hx, Ny+2Hy, Nz+2Hz) */
+} pi_metrics;
+
 typedef struct pi_model_desc {
     pi_grid_desc grid;
     int32_t n_species;
@@ -134,6 +144,8 @@ typedef struct pi_model_desc {
     int32_t n_diag_tracers;      /* tracers accumulated into diags.tracer besides PAR */
     const int32_t* diag_tracers; /* tracer ids, may be NULL when n_diag_tracers == 0 */
     int32_t mode;                /* PI_MODE_QUOTA or PI_MODE_CARBON (PlanktonModel(...; mode)) */
+    const pi_metrics* metrics;   /* LatLonGrid metrics; NULL = RectilinearGrid (dx, dy of grid).
+                                    The last member: callers must set it (NULL for rectilinear). */
 } pi_model_desc;
 
 /* Host-side view of one species' individuals, all arrays of length capacity.

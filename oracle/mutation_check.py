@@ -116,6 +116,23 @@ MISTAKES = [
     ("flux bc: series read at the first iteration",
      "const int64_t tt = bck[q][f] == PI_BCK_SERIES ? (int64_t)(t - 1) : 0;", "const int64_t tt = 0;"),
     ("flux bc: top before bottom", BOTTOM_TOP, TOP_BOTTOM),
+    ("latlon: corner divisor of the row below", "F(xi, yi + 1, zi) / D(xi, yi + 1, zi);",
+     "F(xi, yi + 1, zi) / D(xi, yi, zi);"),
+    ("latlon: walk without the second halo offset", "const int ci = clampi(p.xi[i] + Hx, Px), cj = clampi(p.yi[i] + Hy, Py);",
+     "const int ci = clampi(p.xi[i], Px), cj = clampi(p.yi[i], Py);"),
+    ("latlon: x transport with Ay", "FT T = Ax(i, j, k) * u[I(i, j, k)];", "FT T = Ay(i, j, k) * u[I(i, j, k)];"),
+    ("latlon: y transport difference with one area",
+     "Ay(i, j + 1, k) * w[I(i, j + 1, k)] - Ay(i, j, k) * w[I(i, j, k)]",
+     "Ay(i, j, k) * w[I(i, j + 1, k)] - Ay(i, j, k) * w[I(i, j, k)]"),
+    ("latlon: CFL with dxF", "FT C = std::fabs(u[I(i, j, k)] * dt / DxC(i, j));",
+     "FT C = std::fabs(u[I(i, j, k)] * dt / DxF(i, j));"),
+    ("latlon: meridional diffusion over dyC", "(dy0(c, i, j, k) / DyF(i, j))", "(dy0(c, i, j, k) / DyC(i, j))"),
+    ("latlon: sweep over the volume of the cell north",
+     "tt = tt - dt / vol(i, j, k) * (dG - c[I(i, j, k)] * dT);", "tt = tt - dt / vol(i, j + 1, k) * (dG - c[I(i, j, k)] * dT);"),
+    ("latlon: PAR over the column area", "FT atten = (Chl[I(ii, jj, kk)] / vol(ii, jj, kk) * kc + kw) * dzF_at(kk);",
+     "FT atten = (Chl[I(ii, jj, kk)] / (Az(ii, jj) * dzF_at(kk)) * kc + kw) * dzF_at(kk);"),
+    ("latlon: exchange over the volume below", "tr[q][n] = tr[q][n] + (Gcs[q][n] + plk[q][n] / vol(i, j, k));",
+     "tr[q][n] = tr[q][n] + (Gcs[q][n] + plk[q][n] / vol(i, j, k + 1));"),
 ]
 
 
@@ -138,7 +155,8 @@ def main(only):
             env = dict(os.environ, PI_ORACLE_LIB=lib)
             r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-m", "not gpu", "-p", "no:cacheprovider",
                                 "tests/test_oracle_pins.py", "tests/test_oracle_pins_physics.py",
-                                "tests/test_oracle_pins_carbon.py", "tests/test_oracle_pins_bcs.py"],
+                                "tests/test_oracle_pins_carbon.py", "tests/test_oracle_pins_bcs.py",
+                                "tests/test_oracle_pins_latlon.py"],
                                cwd=REPO, env=env, capture_output=True, text=True)
             first = next((l for l in r.stdout.splitlines() if l.startswith("FAILED")), "")
             (caught if r.returncode != 0 else missed).append(name)

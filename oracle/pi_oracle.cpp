@@ -136,10 +136,22 @@ struct Oracle : orc_model {
         return dzC[k - 1];
     }
     inline FT dzF_at(int k) const { return dzF[k - 1]; }
-    inline FT Ax(int k) const { return dy * dzF_at(k); }
-    inline FT Ay(int k) const { return dx * dzF_at(k); }
-    inline FT Az() const { return dx * dy; }
-    inline FT vol(int k) const { return dx * dy * dzF_at(k); }
+    /* LatLonGrid (Grids/lat_lon_grid.jl:1-37): the spacings, areas and volumes
+     * are arrays with halos, (Px, Py) or (Px, Py, Pz); RectilinearGrid: Δx, Δy
+     * and dzC, dzF (Grids/areas_volumes_spacings.jl:9-38 for both) */
+    bool gm = false;
+    V mdxC, mdyC, mdxF, mdyF, mAz, mAx, mAy, mVol;
+    inline int64_t I2(int i, int j) const { return (int64_t)(i - 1) + (int64_t)Px * (j - 1); }
+    inline FT DxC(int i, int j) const { return gm ? mdxC[I2(i, j)] : dx; }
+    inline FT DyC(int i, int j) const { return gm ? mdyC[I2(i, j)] : dy; }
+    inline FT DxF(int i, int j) const { return gm ? mdxF[I2(i, j)] : dx; }
+    inline FT DyF(int i, int j) const { return gm ? mdyF[I2(i, j)] : dy; }
+    inline FT Ax(int i, int j, int k) const { return gm ? mAx[I(i, j, k)] : dy * dzF_at(k); }
+    inline FT Ay(int i, int j, int k) const { return gm ? mAy[I(i, j, k)] : dx * dzF_at(k); }
+    inline FT Az(int i, int j) const { return gm ? mAz[I2(i, j)] : dx * dy; }
+    inline FT vol(int i, int j, int k) const { return gm ? mVol[I(i, j, k)] : dx * dy * dzF_at(k); }
+    /* a padded index kept inside the array (the random walk's stale index + H) */
+    static int clampi(int64_t i, int P) { return (int)(i < 1 ? 1 : i > P ? P : i); }
 
     /* =============================================================== setup */
     pi_status init(const pi_model_desc* d) {
@@ -152,6 +164,16 @@ struct Oracle : orc_model {
         Px = Nx + 2 * Hx; Py = Ny + 2 * Hy; Pz = Nz + 2 * Hz;
         ncell = (int64_t)Px * Py * Pz;
         dx = (FT)g.dx; dy = (FT)g.dy;
+        if (d->metrics) {
+            const pi_metrics& mt = *d->metrics;
+            if (!mt.dxC || !mt.dyC || !mt.dxF || !mt.dyF || !mt.Az || !mt.Ax || !mt.Ay || !mt.Vol)
+                return fail(PI_ERR_ARG, "grid metrics: every array is required");
+            gm = true;
+            const int64_t n2 = (int64_t)Px * Py;
+            auto cp = [&](V& dst, const float* src, int64_t n) { dst.resize(n); for (int64_t a = 0; a < n; ++a) dst[a] = (FT)src[a]; };
+            cp(mdxC, mt.dxC, n2); cp(mdyC, mt.dyC, n2); cp(mdxF, mt.dxF, n2); cp(mdyF, mt.dyF, n2); cp(mAz, mt.Az, n2);
+            cp(mAx, mt.Ax, ncell); cp(mAy, mt.Ay, ncell); cp(mVol, mt.Vol, ncell);
+        }
         dzC.resize(Pz); dzF.resize(Pz);
         for (int k = 0; k < Pz; ++k) { dzC[k] = (FT)g.dzC[k]; dzF[k] = (FT)g.dzF[k]; }
         lm.assign(ncell, 0);
@@ -331,16 +353,19 @@ struct Oracle : orc_model {
         FT xd = M::mod1(xf), yd = M::mod1(yf), zd = M::mod1(zf);
         int64_t xi = (int64_t)xf + Hx, yi = (int64_t)yf + Hy, zi = (int64_t)zf + Hz;
         const FT one = L(1);
-        auto D = [&](int64_t k) -> FT { return comp == 0 ? dx : comp == 1 ? dy : dzC_at(k); };
+        /* ΔxC / ΔyC / ΔzC of the corner (normalized_trilinear_interpolation_u/v/w) */
+        auto D = [&](int64_t i, int64_t j, int64_t k) -> FT {
+            return comp == 0 ? DxC((int)i, (int)j) : comp == 1 ? DyC((int)i, (int)j) : dzC_at(k);
+        };
         auto F = [&](int64_t i, int64_t j, int64_t k) -> FT { return f[I((int)i, (int)j, (int)k)]; };
-        FT s = (one - xd) * (one - yd) * (one - zd) * F(xi, yi, zi) / D(zi);
-        s = s + (one - xd) * (one - yd) * zd * F(xi, yi, zi + 1) / D(zi + 1);
-        s = s + (one - xd) * yd * (one - zd) * F(xi, yi + 1, zi) / D(zi);
-        s = s + (one - xd) * yd * zd * F(xi, yi + 1, zi + 1) / D(zi + 1);
-        s = s + xd * (one - yd) * (one - zd) * F(xi + 1, yi, zi) / D(zi);
-        s = s + xd * (one - yd) * zd * F(xi + 1, yi, zi + 1) / D(zi + 1);
-        s = s + xd * yd * (one - zd) * F(xi + 1, yi + 1, zi) / D(zi);
-        s = s + xd * yd * zd * F(xi + 1, yi + 1, zi + 1) / D(zi + 1);
+        FT s = (one - xd) * (one - yd) * (one - zd) * F(xi, yi, zi) / D(xi, yi, zi);
+        s = s + (one - xd) * (one - yd) * zd * F(xi, yi, zi + 1) / D(xi, yi, zi + 1);
+        s = s + (one - xd) * yd * (one - zd) * F(xi, yi + 1, zi) / D(xi, yi + 1, zi);
+        s = s + (one - xd) * yd * zd * F(xi, yi + 1, zi + 1) / D(xi, yi + 1, zi + 1);
+        s = s + xd * (one - yd) * (one - zd) * F(xi + 1, yi, zi) / D(xi + 1, yi, zi);
+        s = s + xd * (one - yd) * zd * F(xi + 1, yi, zi + 1) / D(xi + 1, yi, zi + 1);
+        s = s + xd * yd * (one - zd) * F(xi + 1, yi + 1, zi) / D(xi + 1, yi + 1, zi);
+        s = s + xd * yd * zd * F(xi + 1, yi + 1, zi + 1) / D(xi + 1, yi + 1, zi + 1);
         return s;
     }
     /* vel_interpolate_kernel! (particle_advection_kernels.jl:26-37) */
@@ -426,8 +451,9 @@ struct Oracle : orc_model {
         const FT sx = M::sqrt(kx * dt), sy = M::sqrt(ky * dt), sz = M::sqrt(kz * dt);
         OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {   /* calc_diffusion_kernel! :2-8, stale xi/yi/zi (+H twice) */
-            rx[i] = rx[i] * sx / dx;
-            ry[i] = ry[i] * sy / dy;
+            const int ci = clampi(p.xi[i] + Hx, Px), cj = clampi(p.yi[i] + Hy, Py);
+            rx[i] = rx[i] * sx / DxC(ci, cj);
+            ry[i] = ry[i] * sy / DyC(ci, cj);
             rz[i] = rz[i] * sz / dzC_at(p.zi[i] + Hz);
         }
         OMP_FOR
@@ -461,7 +487,7 @@ struct Oracle : orc_model {
         OMP_FOR
         for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
             int ii = i + Hx, jj = j + Hy, kk = ki + Hz;
-            FT atten = (Chl[I(ii, jj, kk)] / vol(kk) * kc + kw) * dzF_at(kk);
+            FT atten = (Chl[I(ii, jj, kk)] / vol(ii, jj, kk) * kc + kw) * dzF_at(kk);
             int64_t f = (int64_t)(i - 1) + (int64_t)Nx * (j - 1);
             par[I(ii, jj, kk)] = PARF[f] * (L(1) - M::exp(-atten)) / atten;
             PARF[f] = PARF[f] * M::exp(-atten);
@@ -902,8 +928,8 @@ struct Oracle : orc_model {
         return M::max(L(0), M::min(M::min(L(1), d0(C) + d1(C) * th), th * (L(1) - C) / (C + L(1.0e-20f))));
     }
     FT flux_x(const FT* c, const FT* u, int i, int j, int k, FT dt) const {
-        FT C = std::fabs(u[I(i, j, k)] * dt / dx);
-        FT T = Ax(k) * u[I(i, j, k)];
+        FT C = std::fabs(u[I(i, j, k)] * dt / DxC(i, j));
+        FT T = Ax(i, j, k) * u[I(i, j, k)];
         FT d = dx0(c, i, j, k);
         FT pp = Psi(C, theta(dxm(c, i, j, k), d));
         FT pm = Psi(C, theta(dxp(c, i, j, k), d));
@@ -911,8 +937,8 @@ struct Oracle : orc_model {
                L(0.5f) * (T - std::fabs(T)) * (c[I(i, j, k)] - pm * d);
     }
     FT flux_y(const FT* c, const FT* v, int i, int j, int k, FT dt) const {
-        FT C = std::fabs(v[I(i, j, k)] * dt / dy);
-        FT T = Ay(k) * v[I(i, j, k)];
+        FT C = std::fabs(v[I(i, j, k)] * dt / DyC(i, j));
+        FT T = Ay(i, j, k) * v[I(i, j, k)];
         FT d = dy0(c, i, j, k);
         FT pp = Psi(C, theta(dym(c, i, j, k), d));
         FT pm = Psi(C, theta(dyp(c, i, j, k), d));
@@ -921,7 +947,7 @@ struct Oracle : orc_model {
     }
     FT flux_z(const FT* c, const FT* w, int i, int j, int k, FT dt) const {
         FT C = std::fabs(w[I(i, j, k)] * dt / dzC_at(k));
-        FT T = Az() * w[I(i, j, k)];
+        FT T = Az(i, j) * w[I(i, j, k)];
         FT d = dz0(c, i, j, k);
         FT pp = Psi(C, theta(dzm(c, i, j, k), d));
         FT pm = Psi(C, theta(dzp(c, i, j, k), d));
@@ -951,16 +977,16 @@ struct Oracle : orc_model {
                         FT dG, dT;
                         if (dim == 0) {
                             dG = dxp(G, i, j, k);
-                            dT = M::mask(LM(i, j, k), M::mask(LM(i + 1, j, k), Ax(k) * w[I(i + 1, j, k)] - Ax(k) * w[I(i, j, k)]));
+                            dT = M::mask(LM(i, j, k), M::mask(LM(i + 1, j, k), Ax(i + 1, j, k) * w[I(i + 1, j, k)] - Ax(i, j, k) * w[I(i, j, k)]));
                         } else if (dim == 1) {
                             dG = dyp(G, i, j, k);
-                            dT = M::mask(LM(i, j, k), M::mask(LM(i, j + 1, k), Ay(k) * w[I(i, j + 1, k)] - Ay(k) * w[I(i, j, k)]));
+                            dT = M::mask(LM(i, j, k), M::mask(LM(i, j + 1, k), Ay(i, j + 1, k) * w[I(i, j + 1, k)] - Ay(i, j, k) * w[I(i, j, k)]));
                         } else {
                             dG = dzp(G, i, j, k);
-                            dT = M::mask(LM(i, j, k), M::mask(LM(i, j, k + 1), Az() * w[I(i, j, k)] - Az() * w[I(i, j, k + 1)]));
+                            dT = M::mask(LM(i, j, k), M::mask(LM(i, j, k + 1), Az(i, j) * w[I(i, j, k)] - Az(i, j) * w[I(i, j, k + 1)]));
                         }
                         FT& tt = ttemp[q][I(i, j, k)];
-                        tt = tt - dt / vol(k) * (dG - c[I(i, j, k)] * dT);
+                        tt = tt - dt / vol(i, j, k) * (dG - c[I(i, j, k)] * dT);
                     }
             }
             fill_halo_tracer(ttemp);
@@ -974,16 +1000,16 @@ struct Oracle : orc_model {
         const FT kx = B[PI_B_kappa_h], ky = B[PI_B_kappa_h], kz = B[PI_B_kappa_v];
         for (int q = 0; q < PI_NTRACERS; ++q) {
             const FT* c = tr[q].data();
-            auto fx = [&](int i, int j, int k) { return kx * Ax(k) * (dx0(c, i, j, k) / dx); };
-            auto fy = [&](int i, int j, int k) { return ky * Ay(k) * (dy0(c, i, j, k) / dy); };
-            auto fz = [&](int i, int j, int k) { return kz * Az() * (dz0(c, i, j, k) / dzF_at(k)); };
+            auto fx = [&](int i, int j, int k) { return kx * Ax(i, j, k) * (dx0(c, i, j, k) / DxF(i, j)); };
+            auto fy = [&](int i, int j, int k) { return ky * Ay(i, j, k) * (dy0(c, i, j, k) / DyF(i, j)); };
+            auto fz = [&](int i, int j, int k) { return kz * Az(i, j) * (dz0(c, i, j, k) / dzF_at(k)); };
             OMP_FOR
             for (int k = Hz + 1; k <= Hz + Nz; ++k) for (int j = Hy + 1; j <= Hy + Ny; ++j)
                 for (int i = Hx + 1; i <= Hx + Nx; ++i) {
                     FT ddx = M::mask(LM(i, j, k), M::mask(LM(i + 1, j, k), fx(i + 1, j, k) - fx(i, j, k)));
                     FT ddy = M::mask(LM(i, j, k), M::mask(LM(i, j + 1, k), fy(i, j + 1, k) - fy(i, j, k)));
                     FT ddz = M::mask(LM(i, j, k), M::mask(LM(i, j, k + 1), fz(i, j, k) - fz(i, j, k + 1)));
-                    FT lap = L(1) / vol(k) * (ddx + ddy + ddz);
+                    FT lap = L(1) / vol(i, j, k) * (ddx + ddy + ddz);
                     FT& G = Gcs[q][I(i, j, k)];
                     G = G + lap * dt;
                 }
@@ -1060,32 +1086,32 @@ struct Oracle : orc_model {
             if (bck[q][PI_BC_WEST])
                 for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) {
                     const int ii = 1 + Hx, jj = j + Hy, kk = k + Hz;
-                    G[I(ii, jj, kk)] += getbc(q, PI_BC_WEST, j, k, iter) * dt * Ax(kk) / vol(kk);
+                    G[I(ii, jj, kk)] += getbc(q, PI_BC_WEST, j, k, iter) * dt * Ax(ii, jj, kk) / vol(ii, jj, kk);
                 }
             if (bck[q][PI_BC_EAST])
                 for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j) {
                     const int ii = Nx + Hx, jj = j + Hy, kk = k + Hz;
-                    G[I(ii, jj, kk)] -= getbc(q, PI_BC_EAST, j, k, iter) * dt * Ax(kk) / vol(kk);
+                    G[I(ii, jj, kk)] -= getbc(q, PI_BC_EAST, j, k, iter) * dt * Ax(ii, jj, kk) / vol(ii, jj, kk);
                 }
             if (bck[q][PI_BC_SOUTH])
                 for (int k = 1; k <= Nz; ++k) for (int i = 1; i <= Nx; ++i) {
                     const int ii = i + Hx, jj = 1 + Hy, kk = k + Hz;
-                    G[I(ii, jj, kk)] += getbc(q, PI_BC_SOUTH, i, k, iter) * dt * Ay(kk) / vol(kk);
+                    G[I(ii, jj, kk)] += getbc(q, PI_BC_SOUTH, i, k, iter) * dt * Ay(ii, jj, kk) / vol(ii, jj, kk);
                 }
             if (bck[q][PI_BC_NORTH])
                 for (int k = 1; k <= Nz; ++k) for (int i = 1; i <= Nx; ++i) {
                     const int ii = i + Hx, jj = Ny + Hy, kk = k + Hz;
-                    G[I(ii, jj, kk)] -= getbc(q, PI_BC_NORTH, i, k, iter) * dt * Ay(kk) / vol(kk);
+                    G[I(ii, jj, kk)] -= getbc(q, PI_BC_NORTH, i, k, iter) * dt * Ay(ii, jj, kk) / vol(ii, jj, kk);
                 }
             if (bck[q][PI_BC_BOTTOM])
                 for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
                     const int ii = i + Hx, jj = j + Hy, kk = Nz + Hz;
-                    G[I(ii, jj, kk)] += getbc(q, PI_BC_BOTTOM, i, j, iter) * dt * Az() / vol(kk);
+                    G[I(ii, jj, kk)] += getbc(q, PI_BC_BOTTOM, i, j, iter) * dt * Az(ii, jj) / vol(ii, jj, kk);
                 }
             if (bck[q][PI_BC_TOP])
                 for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
                     const int ii = i + Hx, jj = j + Hy, kk = 1 + Hz;
-                    G[I(ii, jj, kk)] -= getbc(q, PI_BC_TOP, i, j, iter) * dt * Az() / vol(kk);
+                    G[I(ii, jj, kk)] -= getbc(q, PI_BC_TOP, i, j, iter) * dt * Az(ii, jj) / vol(ii, jj, kk);
                 }
         }
     }
@@ -1102,7 +1128,7 @@ struct Oracle : orc_model {
             for (int k = Hz + 1; k <= Hz + Nz; ++k) for (int j = Hy + 1; j <= Hy + Ny; ++j)
                 for (int i = Hx + 1; i <= Hx + Nx; ++i) {
                     int64_t n = I(i, j, k);
-                    tr[q][n] = tr[q][n] + (Gcs[q][n] + plk[q][n] / vol(k));
+                    tr[q][n] = tr[q][n] + (Gcs[q][n] + plk[q][n] / vol(i, j, k));
                 }
         fill_halo_tracer(tr);
     }

@@ -8,7 +8,7 @@ gfx950).  Import this directory under any module name (its folder name is not a
 Python identifier), e.g. with `importlib`.
 """
 from ._abi import TRACER_NAMES
-from .grids import RectilinearGrid, Periodic, Bounded
+from .grids import RectilinearGrid, LatLonGrid, LoadLatLonGrid, Periodic, Bounded
 from .params import (QuotaMode, CarbonMode, MacroMolecularMode, IronEnergyMode,
                      bgc_params_default, phyt_params_default, update_bgc_params, update_phyt_params)
 from .fields import default_tracer_init, generate_tracers, fill_halo_tracer
@@ -27,7 +27,7 @@ TimeStep_ = TimeStep      # `TimeStep!`
 write_output_ = write_output   # `write_output!`
 set_bc_ = set_bc               # `set_bc!`
 
-__all__ = ["RectilinearGrid", "Periodic", "Bounded", "QuotaMode", "PlanktonModel", "HIPArch", "GPU",
+__all__ = ["RectilinearGrid", "LatLonGrid", "LoadLatLonGrid", "Periodic", "Bounded", "QuotaMode", "PlanktonModel", "HIPArch", "GPU",
            "phyto_setup", "PlanktonSimulation", "PlanktonDiagnostics", "update", "TimeStep",
            "bgc_params_default", "phyt_params_default", "update_bgc_params", "update_phyt_params",
            "default_tracer_init", "default_PARF", "default_temperature", "CapacityError",

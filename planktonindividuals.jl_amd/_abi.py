@@ -59,11 +59,16 @@ class GridDesc(C.Structure):
                 ("dzC", fp), ("dzF", fp), ("landmask", u8p)]
 
 
+class Metrics(C.Structure):
+    """pi_metrics: LatLonGrid spacings, areas and volumes (padded, column-major)."""
+    _fields_ = [(n, fp) for n in ("dxC", "dyC", "dxF", "dyF", "Az", "Ax", "Ay", "Vol")]
+
+
 class ModelDesc(C.Structure):
     _fields_ = [("grid", GridDesc), ("n_species", C.c_int32), ("capacity", C.c_int64),
                 ("params", fp), ("bgc", fp), ("rng_seed", C.c_uint64),
                 ("n_diag_tracers", C.c_int32), ("diag_tracers", C.POINTER(C.c_int32)),
-                ("mode", C.c_int32)]
+                ("mode", C.c_int32), ("metrics", C.POINTER(Metrics))]
 
 
 class IndividualsHost(C.Structure):
@@ -91,9 +96,14 @@ def make_desc(grid, n_species, capacity, params, bgc, seed, diag_tracers=(), mod
                  np.float32(grid.Δx), np.float32(grid.Δy),
                  ptr(dzC, C.c_float), ptr(dzF, C.c_float),
                  None if lm is None else lm.ctypes.data_as(u8p))
+    marrs, mt = (), None
+    if hasattr(grid, "metric_arrays"):                 # LatLonGrid
+        marrs = grid.metric_arrays()
+        mt = Metrics(*(ptr(a, C.c_float) for a in marrs))
     d = ModelDesc(g, n_species, capacity, ptr(params, C.c_float), ptr(bgc, C.c_float),
-                  C.c_uint64(seed), len(dt), ptr(dt, C.c_int32) if len(dt) else None, int(mode))
-    return d, (dzC, dzF, lm, params, bgc, dt)
+                  C.c_uint64(seed), len(dt), ptr(dt, C.c_int32) if len(dt) else None, int(mode),
+                  C.pointer(mt) if mt is not None else None)
+    return d, (dzC, dzF, lm, params, bgc, dt, marrs, mt)
 
 
 def individuals_view(arrays):

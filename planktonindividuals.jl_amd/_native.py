@@ -139,6 +139,12 @@ class Handle:
 
     def set_velocity(self, u, v, w):
         u, v, w = (np.asfortranarray(a, dtype=np.float32) for a in (u, v, w))
+        g = self.grid
+        b = [t == _abi.BOUNDED for t in g.topology_ids]
+        want = ((g.Nx + b[0], g.Ny, g.Nz), (g.Nx, g.Ny + b[1], g.Nz), (g.Nx, g.Ny, g.Nz + b[2]))
+        for a, s, n in zip((u, v, w), want, "uvw"):
+            if a.shape != s:       # the library reads exactly this many values (vel_copy!)
+                raise ValueError(f"Dimension mismatch: the size of {n} must be {s}, got {a.shape}.")
         self._call("set_velocity", u.ctypes.data_as(_abi.fp), v.ctypes.data_as(_abi.fp),
                    w.ctypes.data_as(_abi.fp))
 

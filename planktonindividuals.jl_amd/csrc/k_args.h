@@ -191,7 +191,8 @@ enum KId {
     K_VELHALF, K_MARKS, K_SCAN_P, K_SCAN_F, K_CAPCHECK, K_ADV, K_CELLCOUNT, K_PLACE_STAY,
     K_PLACE_SMALL, K_MIXSORT, K_BIGCELLS, K_CELLSUM, K_CELLSUM_LONG, K_PAR, K_PHYS, K_PLK, K_PLK_LONG,
     K_SWEEPX0, K_SWEEPX1, K_XY0, K_XY1, K_SWEEPY0, K_SWEEPY1, K_HALOCOPY, K_FILL, K_FINAL0, K_FINAL1, K_ACCUM,
-    K_FINISH, K_XYW0, K_XYW1, K_PHYSC, K_FINAL2, K_FINAL3, K_NKID
+    K_FINISH, K_XYW0, K_XYW1, K_PHYSC, K_FINAL2, K_FINAL3,
+    K_ADV_GM, K_SWEEPX2, K_SWEEPX3, K_SWEEPY2, K_SWEEPY3, K_FINAL4, K_FINAL5, K_FINAL6, K_FINAL7, K_NKID
 };
 /* plan markers (not kernels): fork to / join from the side stream */
 enum { K_FORK = K_NKID, K_JOIN = K_NKID + 1 };

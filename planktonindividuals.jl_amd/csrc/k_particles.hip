@@ -45,6 +45,7 @@
  * instructions: slower), re-dealing lanes by corner rows (fewer cache lines per
  * gather: no change), staging each block's velocity neighbourhood in LDS (the
  * texture path relieved, 20% slower). */
+template <bool GM>
 __device__ __forceinline__ float itpl(const DevGrid& g, const float* __restrict__ f,
                                       float x, float y, float z, int comp)
 {
@@ -69,6 +70,21 @@ __device__ __forceinline__ float itpl(const DevGrid& g, const float* __restrict_
     const float f100 = p00.y, f101 = p01.y, f110 = p10.y, f111 = p11.y;
     (void)sx;
     const float ox = 1.0f - xd, oy = 1.0f - yd, oz = 1.0f - zd;
+    if (GM && comp < 2) {
+        /* LatLonGrid: ΔxC / ΔyC of each corner column (the z corners share it) */
+        const float* M = comp == 0 ? g.mdxC : g.mdyC;
+        const long long m = col2(g, xi, yi);
+        const float E00 = M[m], E01 = M[m + g.Px], E10 = M[m + 1], E11 = M[m + 1 + g.Px];
+        float s = (ox * oy * oz * f000) / E00;
+        s = s + (ox * oy * zd * f001) / E00;
+        s = s + (ox * yd * oz * f010) / E01;
+        s = s + (ox * yd * zd * f011) / E01;
+        s = s + (xd * oy * oz * f100) / E10;
+        s = s + (xd * oy * zd * f101) / E10;
+        s = s + (xd * yd * oz * f110) / E11;
+        s = s + (xd * yd * zd * f111) / E11;
+        return s;
+    }
     float s = divc(ox * oy * oz * f000, D0, R0);
     s = s + divc(ox * oy * zd * f001, D1, R1);
     s = s + divc(ox * yd * oz * f010, D0, R0);
@@ -111,35 +127,37 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
 
 /* RK4 advection (particle_advection.jl:2-56) and the random walk
  * (particle_diffusion.jl:2-33) of one individual, up to its new position. */
+template <bool GM>
 __device__ __forceinline__ void move(const AdvArgs& a, const DevGrid& g, float x0, float y0, float z0,
-                                     float r0, float r1, float r2, long long zprev, float& x, float& y, float& z)
+                                     float r0, float r1, float r2, long long xprev, long long yprev, long long zprev,
+                                     float& x, float& y, float& z)
 {
     const float dt = a.dt;
     /* ---- RK4, particle_advection.jl:2-56 */
-    float u1 = itpl(g, a.v0.u, x0, y0, z0, 0);
-    float v1 = itpl(g, a.v0.v, x0, y0, z0, 1);
-    float w1 = itpl(g, a.v0.w, x0, y0, z0, 2);
+    float u1 = itpl<GM>(g, a.v0.u, x0, y0, z0, 0);
+    float v1 = itpl<GM>(g, a.v0.v, x0, y0, z0, 1);
+    float w1 = itpl<GM>(g, a.v0.w, x0, y0, z0, 2);
     float u2 = u1, v2 = v1, w2 = w1;
     float px = pbc(x0 + 0.5f * u1 * dt, g.Nx, g.tx);
     float py = pbc(y0 + 0.5f * v1 * dt, g.Ny, g.ty);
     float pz = pbc(z0 - 0.5f * w1 * dt, g.Nz, g.tz);
-    u1 = itpl(g, a.vh.u, px, py, pz, 0);
-    v1 = itpl(g, a.vh.v, px, py, pz, 1);
-    w1 = itpl(g, a.vh.w, px, py, pz, 2);
+    u1 = itpl<GM>(g, a.vh.u, px, py, pz, 0);
+    v1 = itpl<GM>(g, a.vh.v, px, py, pz, 1);
+    w1 = itpl<GM>(g, a.vh.w, px, py, pz, 2);
     u2 = u2 + u1 * 2.0f; v2 = v2 + v1 * 2.0f; w2 = w2 + w1 * 2.0f;
     px = pbc(x0 + 0.5f * u1 * dt, g.Nx, g.tx);
     py = pbc(y0 + 0.5f * v1 * dt, g.Ny, g.ty);
     pz = pbc(z0 - 0.5f * w1 * dt, g.Nz, g.tz);
-    u1 = itpl(g, a.vh.u, px, py, pz, 0);
-    v1 = itpl(g, a.vh.v, px, py, pz, 1);
-    w1 = itpl(g, a.vh.w, px, py, pz, 2);
+    u1 = itpl<GM>(g, a.vh.u, px, py, pz, 0);
+    v1 = itpl<GM>(g, a.vh.v, px, py, pz, 1);
+    w1 = itpl<GM>(g, a.vh.w, px, py, pz, 2);
     u2 = u2 + u1 * 2.0f; v2 = v2 + v1 * 2.0f; w2 = w2 + w1 * 2.0f;
     px = pbc(x0 + 1.0f * u1 * dt, g.Nx, g.tx);
     py = pbc(y0 + 1.0f * v1 * dt, g.Ny, g.ty);
     pz = pbc(z0 - 1.0f * w1 * dt, g.Nz, g.tz);
-    u1 = itpl(g, a.v1.u, px, py, pz, 0);
-    v1 = itpl(g, a.v1.v, px, py, pz, 1);
-    w1 = itpl(g, a.v1.w, px, py, pz, 2);
+    u1 = itpl<GM>(g, a.v1.u, px, py, pz, 0);
+    v1 = itpl<GM>(g, a.v1.v, px, py, pz, 1);
+    w1 = itpl<GM>(g, a.v1.w, px, py, pz, 2);
     u2 = u2 + u1; v2 = v2 + v1; w2 = w2 + w1;
     u2 = divc(u2, 6.0f, g.y6); v2 = divc(v2, 6.0f, g.y6); w2 = divc(w2, 6.0f, g.y6);
     x = pbc(x0 + 1.0f * u2 * dt, g.Nx, g.tx);
@@ -147,8 +165,17 @@ __device__ __forceinline__ void move(const AdvArgs& a, const DevGrid& g, float x
     z = pbc(z0 - 1.0f * w2 * dt, g.Nz, g.tz);
     /* ---- random walk, particle_diffusion.jl:2-33 (stale zi from the previous find_inds) */
     r0 = (r0 - 0.5f) * 2.0f; r1 = (r1 - 0.5f) * 2.0f; r2 = (r2 - 0.5f) * 2.0f;
-    r0 = divc(r0 * a.sx, g.dx, g.ydx);
-    r1 = divc(r1 * a.sy, g.dy, g.ydy);
+    if (GM) {           /* ΔxC, ΔyC at the stale (xi + Hx, yi + Hy), kept inside the array */
+        long long kx = xprev + g.Hx, ky = yprev + g.Hy;
+        kx = kx < 1 ? 1 : kx > g.Px ? g.Px : kx;
+        ky = ky < 1 ? 1 : ky > g.Py ? g.Py : ky;
+        const long long m = col2(g, (int)kx, (int)ky);
+        r0 = (r0 * a.sx) / g.mdxC[m];
+        r1 = (r1 * a.sy) / g.mdyC[m];
+    } else {
+        r0 = divc(r0 * a.sx, g.dx, g.ydx);
+        r1 = divc(r1 * a.sy, g.dy, g.ydy);
+    }
     {
         long long kz = zprev + g.Hz;
         kz = kz < 1 ? 1 : kz > g.Pz ? g.Pz : kz;
@@ -202,6 +229,7 @@ __device__ __forceinline__ void place_entry(const AdvArgs& a, DevState* st, cons
  * the device state are read before the loops that call these: a load that may
  * alias the loop's stores would be a vector load waiting for every outstanding
  * store.) */
+template <bool GM>
 __device__ __forceinline__ void move_entry(const AdvArgs& a, const GStore& S, uint32_t e, long long iteration, uint8_t fl,
                                            float& x, float& y, float& z)
 {
@@ -212,11 +240,15 @@ __device__ __forceinline__ void move_entry(const AdvArgs& a, const GStore& S, ui
     float r0, r1, r2;
     pi_uniform3(a.seed, iteration, sp, PI_KIND_DIFFUSION, slot, &r0, &r1, &r2);
     const uint32_t cprev = S.cell[e];
+    /* the previous step's find_inds (1-based padded), (0, 0, 0) before the first */
     const long long zprev = (fl & F_ZERO) ? 0 : (long long)(cprev / (uint32_t)(g.Px * g.Py)) + 1;
-    move(a, g, S.x[e], S.y[e], S.z[e], r0, r1, r2, zprev, x, y, z);
+    const long long xprev = (fl & F_ZERO) ? 0 : (long long)(cprev % (uint32_t)g.Px) + 1;
+    const long long yprev = (fl & F_ZERO) ? 0 : (long long)((cprev / (uint32_t)g.Px) % (uint32_t)g.Py) + 1;
+    move<GM>(a, g, S.x[e], S.y[e], S.z[e], r0, r1, r2, xprev, yprev, zprev, x, y, z);
 }
 
-__device__ __forceinline__ void k_adv_body(const AdvArgs& a, Blk K)
+template <bool GM>
+__device__ __forceinline__ void k_adv_impl(const AdvArgs& a, Blk K)
 {
     DevState* st = a.st;
     if (st->err) return;
@@ -228,14 +260,18 @@ __device__ __forceinline__ void k_adv_body(const AdvArgs& a, Blk K)
     for (uint32_t e = K.b * blockDim.x + threadIdx.x; e < nloop; e += K.n * blockDim.x) {
         const uint8_t fl = e < n ? S.flags[e] : (uint8_t)0;
         float x = 0.0f, y = 0.0f, z = 0.0f;
-        if (fl & F_AC) move_entry(a, S, e, iteration, fl, x, y, z);
+        if (fl & F_AC) move_entry<GM>(a, S, e, iteration, fl, x, y, z);
         place_entry(a, st, S, e, nA, (int32_t)iteration, (fl & F_AC) != 0, fl, x, y, z);
         if (!(fl & F_AC) && e < nA) a.stayf[e] = 0u;
     }
 }
 /* at least 5 waves per SIMD (88 VGPRs, no spill): the gathers of the corner
  * values wait less; 0.467 -> 0.436 ms on the 256x256x64 run (4 waves at 105) */
+__device__ __forceinline__ void k_adv_body(const AdvArgs& a, Blk K) { k_adv_impl<false>(a, K); }
+__device__ __forceinline__ void k_adv_gm_body(const AdvArgs& a, Blk K) { k_adv_impl<true>(a, K); }
 PI_KERNEL3W(k_adv, k_adv_body, AdvArgs, 256, xcd_block(), 5)
+/* LatLonGrid: the spacings of each corner column from the metric arrays */
+PI_KERNEL3W(k_adv_gm, k_adv_gm_body, AdvArgs, 256, xcd_block(), 5)
 
 /* Survivors of this step's losses and the dividers, as per-species slot bitmaps
  * (plankton_update.jl:5-24: ac after grazing!/mortality!, dvid .*= ac).  They
@@ -703,7 +739,7 @@ __device__ __forceinline__ void k_par_impl(Blk K, DevState* st, DevGrid g, const
             for (int k = 1; k <= g.Nz; ++k) {
                 const int kk = k + g.Hz;
                 const long long c = gidx(g, ii, jj, kk);
-                const float vol = g.dx * g.dy * dzF_at(g, kk);
+                const float vol = cell_vol(g, c, kk);
                 const float at = (Chl[c] / vol * kc + kw) * dzF_at(g, kk);
                 const float ex = pi_expf(-at);
                 const float p = P * (1.0f - ex) / at;
@@ -732,7 +768,7 @@ __device__ __forceinline__ void k_par_impl(Blk K, DevState* st, DevGrid g, const
             dgv[r] = 0.0f;
             if (it >= nitem || col >= ncol) continue;
             const long long c = gidx(g, col % g.Nx + 1 + g.Hx, col / g.Nx + 1 + g.Hy, kk);
-            const float vol = g.dx * g.dy * dzF_at(g, kk);
+            const float vol = cell_vol(g, c, kk);
             const float at = (Chl[c] / vol * kc + kw) * dzF_at(g, kk);
             dgv[r] = diag_par[c];
             s_at[it] = at;
@@ -1524,6 +1560,7 @@ PI_KERNEL(k_plk_long, PlkArgs, 256)
 void reg_particles(KInfo* k)
 {
     PI_REG(k, K_ADV, k_adv, AdvArgs, 256, true);
+    PI_REG(k, K_ADV_GM, k_adv_gm, AdvArgs, 256, true);
     PI_REG(k, K_MARKS, k_marks, MarksArgs, 256, true);
     PI_REG(k, K_CELLCOUNT, k_cellcount, BinArgs, 256, true);
     PI_REG(k, K_PLACE_STAY, k_place_stay, BinArgs, 256, true);

@@ -77,6 +77,22 @@ __device__ __forceinline__ Face face(const DevGrid& g, float u, float dt, float 
     return f;
 }
 
+/* the same with the Courant number's spacing D from a metric array (LatLonGrid):
+ * an IEEE division */
+__device__ __forceinline__ Face face_gm(const DevGrid& g, float u, float dt, float D, float A)
+{
+    Face f;
+    f.C = fabsf((u * dt) / D);
+    f.d0 = divc((2.0f - f.C) * (1.0f - f.C), 6.0f, g.y6);
+    f.d1 = divc(1.0f - f.C * f.C, 6.0f, g.y6);
+    f.T = A * u;
+    f.omc = 1.0f - f.C;
+    f.Ce = f.C + 1.0e-20f;
+    f.yCe = 1.0 / (double)f.Ce;
+    f.fast = f.C < 0x1p60f && ((f.T > 0.0f && f.T < 0x1p126f) || (f.T < 0.0f && f.T > -0x1p126f));
+    return f;
+}
+
 /* θ and Ψ (DST3FL.jl:23-57) */
 __device__ __forceinline__ float theta(float dn, float dd)
 {
@@ -182,7 +198,7 @@ __device__ __forceinline__ void sweep_x_half(const SweepArgs& a, const DevGrid& 
     }
 }
 
-template <bool LAND>
+template <bool LAND, bool GM>
 __device__ __forceinline__ void k_sweep_x_body(const SweepArgs& a, Blk K)
 {
     const DevState* S = a.st;
@@ -208,18 +224,26 @@ __device__ __forceinline__ void k_sweep_x_body(const SweepArgs& a, Blk K)
     const uint32_t cf = wrap ? c - (uint32_t)g.Nx : c;
     const float* C0 = S->tr[S->tcur][0];
     const float dt = a.dt;
-    const float A = g.dy * dzF_at(g, k);
-    const Face Fc = face(g, ld(a.vel, cf), dt, g.dx, g.ydx, A);
+    /* Trans_x = Ax·u, CFLx = |u·ΔT/ΔxC| at the face (DST3FL.jl:4,13) */
+    const Face Fc = GM ? face_gm(g, ld(a.vel, cf), dt, g.mdxC[col2(g, wrap ? i - g.Nx : i, j)], g.mAx[cf])
+                       : face(g, ld(a.vel, cf), dt, g.dx, g.ydx, g.dy * dzF_at(g, k));
     const bool zero = fi == g.Nx && g.tx == PI_BOUNDED;
     const bool upd = valid && lane < 63 && fi < g.Nx;
     const float T1 = __shfl_down(Fc.T, 1, 64);
     const float dT = lmask<LAND>(g, c, c + 1, T1 - Fc.T);
-    const float coef = divc(dt, g.vol[k - 1], g.yvol[k - 1]);
+    const float coef = GM ? dt / g.mVol[c] : divc(dt, g.vol[k - 1], g.yvol[k - 1]);
     sweep_x_half<LAND, 0, 7>(a, g, Fc, zero, upd, c, cf, coef, dT, C0);
     sweep_x_half<LAND, 7, 14>(a, g, Fc, zero, upd, c, cf, coef, dT, C0);
 }
-PI_KERNEL2(k_sweep_x0, k_sweep_x_body<false>, SweepArgs, 256)
-PI_KERNEL2(k_sweep_x1, k_sweep_x_body<true>, SweepArgs, 256)
+__device__ __forceinline__ void k_sweep_x0_body(const SweepArgs& a, Blk K) { k_sweep_x_body<false, false>(a, K); }
+__device__ __forceinline__ void k_sweep_x1_body(const SweepArgs& a, Blk K) { k_sweep_x_body<true, false>(a, K); }
+__device__ __forceinline__ void k_sweep_x2_body(const SweepArgs& a, Blk K) { k_sweep_x_body<false, true>(a, K); }
+__device__ __forceinline__ void k_sweep_x3_body(const SweepArgs& a, Blk K) { k_sweep_x_body<true, true>(a, K); }
+PI_KERNEL2(k_sweep_x0, k_sweep_x0_body, SweepArgs, 256)
+PI_KERNEL2(k_sweep_x1, k_sweep_x1_body, SweepArgs, 256)
+/* LatLonGrid: areas, spacings and volumes from the metric arrays */
+PI_KERNEL2(k_sweep_x2, k_sweep_x2_body, SweepArgs, 256)
+PI_KERNEL2(k_sweep_x3, k_sweep_x3_body, SweepArgs, 256)
 
 /* ----------------------------------------------------------------- y sweep */
 #define RUN_Y 16
@@ -241,7 +265,7 @@ __device__ __forceinline__ void sweep_y_half(const SweepArgs& a, const DevGrid& 
     }
 }
 
-template <bool LAND>
+template <bool LAND, bool GM>
 __device__ __forceinline__ void k_sweep_y_body(const SweepArgs& a, Blk K)
 {
     const DevState* S = a.st;
@@ -261,26 +285,37 @@ __device__ __forceinline__ void k_sweep_y_body(const SweepArgs& a, Blk K)
         const int j0 = g.Hy + 1 + r * RUN_Y;
         const int j1 = min(j0 + RUN_Y, g.Ny + g.Hy + 1);     /* cells j0..j1-1, faces j0..j1 */
         const float A = g.dx * dzF_at(g, k);
-        const float coef = divc(dt, g.vol[k - 1], g.yvol[k - 1]);
+        const float coef = GM ? 0.0f : divc(dt, g.vol[k - 1], g.yvol[k - 1]);
         uint32_t c = (uint32_t)gidx(g, i, j0, k);            /* cell north of the current face */
         float fl[PI_NTRACERS];                               /* fluxes of the previous face */
-        Face Fa = face(g, ld(a.vel, c), dt, g.dy, g.ydy, A);
+        /* Trans_y = Ay·v, CFLy = |v·ΔT/ΔyC| at the face (DST3FL.jl:5,14) */
+        Face Fa = GM ? face_gm(g, ld(a.vel, c), dt, g.mdyC[col2(g, i, j0)], g.mAy[c])
+                     : face(g, ld(a.vel, c), dt, g.dy, g.ydy, A);
         face_fluxes<LAND, false, 0, 7>(g, Fa, false, c, sy, a.prev, fl);
         face_fluxes<LAND, false, 7, 14>(g, Fa, false, c, sy, a.prev, fl + 7);
 #pragma unroll 1
         for (int jj = j0 + 1; jj <= j1; ++jj) {
             c += sy;
-            const uint32_t cf = wrapy && jj == g.Ny + g.Hy + 1 ? c - back : c;
-            const Face Fb = face(g, ld(a.vel, cf), dt, g.dy, g.ydy, A);
+            const bool wj = wrapy && jj == g.Ny + g.Hy + 1;
+            const uint32_t cf = wj ? c - back : c;
+            const Face Fb = GM ? face_gm(g, ld(a.vel, cf), dt, g.mdyC[col2(g, i, wj ? jj - g.Ny : jj)], g.mAy[cf])
+                               : face(g, ld(a.vel, cf), dt, g.dy, g.ydy, A);
             const bool zero = jj == g.Ny + g.Hy + 1 && g.ty == PI_BOUNDED;
-            sweep_y_half<LAND, 0, 7>(a, g, Fb, Fa, zero, c, cf, sy, coef, C0, fl);
-            sweep_y_half<LAND, 7, 14>(a, g, Fb, Fa, zero, c, cf, sy, coef, C0, fl);
+            const float cf_coef = GM ? dt / g.mVol[c - sy] : coef;     /* the cell south of the face */
+            sweep_y_half<LAND, 0, 7>(a, g, Fb, Fa, zero, c, cf, sy, cf_coef, C0, fl);
+            sweep_y_half<LAND, 7, 14>(a, g, Fb, Fa, zero, c, cf, sy, cf_coef, C0, fl);
             Fa = Fb;
         }
     }
 }
-PI_KERNEL2(k_sweep_y0, k_sweep_y_body<false>, SweepArgs, 256)
-PI_KERNEL2(k_sweep_y1, k_sweep_y_body<true>, SweepArgs, 256)
+__device__ __forceinline__ void k_sweep_y0_body(const SweepArgs& a, Blk K) { k_sweep_y_body<false, false>(a, K); }
+__device__ __forceinline__ void k_sweep_y1_body(const SweepArgs& a, Blk K) { k_sweep_y_body<true, false>(a, K); }
+__device__ __forceinline__ void k_sweep_y2_body(const SweepArgs& a, Blk K) { k_sweep_y_body<false, true>(a, K); }
+__device__ __forceinline__ void k_sweep_y3_body(const SweepArgs& a, Blk K) { k_sweep_y_body<true, true>(a, K); }
+PI_KERNEL2(k_sweep_y0, k_sweep_y0_body, SweepArgs, 256)
+PI_KERNEL2(k_sweep_y1, k_sweep_y1_body, SweepArgs, 256)
+PI_KERNEL2(k_sweep_y2, k_sweep_y2_body, SweepArgs, 256)
+PI_KERNEL2(k_sweep_y3, k_sweep_y3_body, SweepArgs, 256)
 
 /* fill_halo_{west,east} / {south,north} / {top,bottom} of one dimension, on the
  * lines given by (o1, o2) ranges.  lines_full = 1 covers every padded line (the
@@ -381,7 +416,15 @@ struct ZCell {      /* cell cc (level k; column i, j; all 1-based padded) above 
     int it;         /* the model's iteration during this step (flux boundary conditions) */
     float dT, coef, khAx, khAy, inv_vol, dzf, vol;
     double ydzf, yvol;
+    /* LatLonGrid: κv·Az of the column, κh·Ax / κh·Ay of the east and north
+     * faces (khAx, khAy: west and south), ΔxF / ΔyF of the four faces */
+    float kvAz, khAxE, khAyN, dxFW, dxFE, dyFS, dyFN;
 };
+
+/* x / volume(i, j, k) of the cell of z: by its reciprocal (divc) on a
+ * RectilinearGrid, an IEEE division on a LatLonGrid */
+template <bool GM>
+__device__ __forceinline__ float per_vol(float x, const ZCell& z) { return GM ? x / z.vol : divc(x, z.vol, z.yvol); }
 
 /* getbc (HaloRegions/boundary_conditions.jl:31-40) of tracer q on face f at
  * face position (p, r) (1-based interior, face array size np x nr) */
@@ -397,25 +440,27 @@ __device__ __forceinline__ float bc_at(const BcTab* T, int q, int f, int p, int 
  * z: in the order west, east, south, north, bottom, top, Gc += or -=
  * ((bc·ΔT)·A)/V at the interior cells next to the face; only faces with a
  * boundary condition (bit f of a.bcm[q]) are looked at */
+template <bool GM>
 __device__ __forceinline__ float flux_bcs(const FinalArgs& a, const DevGrid& g, int q, const ZCell& z, float G)
 {
     const BcTab* T = a.bc;
     const uint32_t m = a.bcm[q];
     const int i = z.i - g.Hx, j = z.j - g.Hy, k = z.k - g.Hz;
     const float dt = a.dt;
-    const float Ax = g.dy * dzF_at(g, z.k), Ay = g.dx * dzF_at(g, z.k), Az = g.dx * g.dy;
+    const float Ax = GM ? g.mAx[z.cc] : g.dy * dzF_at(g, z.k), Ay = GM ? g.mAy[z.cc] : g.dx * dzF_at(g, z.k);
+    const float Az = GM ? g.mAz[col2(g, z.i, z.j)] : g.dx * g.dy;
     if ((m & (1u << PI_BC_WEST)) && i == 1)
-        G = G + divc((bc_at(T, q, PI_BC_WEST, j, k, g.Ny, g.Nz, z.it) * dt) * Ax, z.vol, z.yvol);
+        G = G + per_vol<GM>((bc_at(T, q, PI_BC_WEST, j, k, g.Ny, g.Nz, z.it) * dt) * Ax, z);
     if ((m & (1u << PI_BC_EAST)) && i == g.Nx)
-        G = G - divc((bc_at(T, q, PI_BC_EAST, j, k, g.Ny, g.Nz, z.it) * dt) * Ax, z.vol, z.yvol);
+        G = G - per_vol<GM>((bc_at(T, q, PI_BC_EAST, j, k, g.Ny, g.Nz, z.it) * dt) * Ax, z);
     if ((m & (1u << PI_BC_SOUTH)) && j == 1)
-        G = G + divc((bc_at(T, q, PI_BC_SOUTH, i, k, g.Nx, g.Nz, z.it) * dt) * Ay, z.vol, z.yvol);
+        G = G + per_vol<GM>((bc_at(T, q, PI_BC_SOUTH, i, k, g.Nx, g.Nz, z.it) * dt) * Ay, z);
     if ((m & (1u << PI_BC_NORTH)) && j == g.Ny)
-        G = G - divc((bc_at(T, q, PI_BC_NORTH, i, k, g.Nx, g.Nz, z.it) * dt) * Ay, z.vol, z.yvol);
+        G = G - per_vol<GM>((bc_at(T, q, PI_BC_NORTH, i, k, g.Nx, g.Nz, z.it) * dt) * Ay, z);
     if ((m & (1u << PI_BC_BOTTOM)) && k == g.Nz)
-        G = G + divc((bc_at(T, q, PI_BC_BOTTOM, i, j, g.Nx, g.Ny, z.it) * dt) * Az, z.vol, z.yvol);
+        G = G + per_vol<GM>((bc_at(T, q, PI_BC_BOTTOM, i, j, g.Nx, g.Ny, z.it) * dt) * Az, z);
     if ((m & (1u << PI_BC_TOP)) && k == 1)
-        G = G - divc((bc_at(T, q, PI_BC_TOP, i, j, g.Nx, g.Ny, z.it) * dt) * Az, z.vol, z.yvol);
+        G = G - per_vol<GM>((bc_at(T, q, PI_BC_TOP, i, j, g.Nx, g.Ny, z.it) * dt) * Az, z);
     return G;
 }
 
@@ -425,7 +470,7 @@ __device__ __forceinline__ float flux_bcs(const FinalArgs& a, const DevGrid& g, 
  * pv = the z-sweep input at the cell, fzq/fq = advective fluxes of its top and
  * bottom faces, dzq = diffusive flux of its top face, csq = c above the cell
  * (sinking tracers), pk = plk. */
-template <bool LAND, bool BCS>
+template <bool LAND, bool BCS, bool GM>
 __device__ __forceinline__ void cell_update(const FinalArgs& a, const DevGrid& g, const ZCell& z, int q, const float* tt,
                                             float cq, float cn, float xm, float xp, float ym, float yp, float pv,
                                             float fzq, float fq, float dzq, float csq, float pk,
@@ -434,16 +479,24 @@ __device__ __forceinline__ void cell_update(const FinalArgs& a, const DevGrid& g
     const uint32_t cc = z.cc, c = z.c, sy = (uint32_t)g.Px;
     const int k = z.k;
     const float dt = a.dt;
-    const float kvAz = a.kv * (g.dx * g.dy);
+    const float kvAz = GM ? z.kvAz : a.kv * (g.dx * g.dy);
     /* κv·Az·(δz c / dzF) at face c (Advection/tracer_diffusion.jl:1-30) */
     d = kvAz * divc(lmask<LAND>(g, cc, c, cq - cn), z.dzf, z.ydzf);
     const float temp3 = pv - z.coef * (lmask<LAND>(g, cc, c, fzq - fq) - cq * z.dT);
     /* Gcs = tracer_temp - tracer (calc_tracer_tendency!) */
     float G = temp3 - cq;
-    const float fxa = z.khAx * divc(lmask<LAND>(g, cc - 1, cc, cq - xm), g.dx, g.ydx);
-    const float fxb = z.khAx * divc(lmask<LAND>(g, cc, cc + 1, xp - cq), g.dx, g.ydx);
-    const float fya = z.khAy * divc(lmask<LAND>(g, cc - sy, cc, cq - ym), g.dy, g.ydy);
-    const float fyb = z.khAy * divc(lmask<LAND>(g, cc, cc + sy, yp - cq), g.dy, g.ydy);
+    float fxa, fxb, fya, fyb;
+    if (GM) {       /* κ·A(i)·(δ⁰c(i) / ΔF(i)) at the west/east and south/north faces */
+        fxa = z.khAx * (lmask<LAND>(g, cc - 1, cc, cq - xm) / z.dxFW);
+        fxb = z.khAxE * (lmask<LAND>(g, cc, cc + 1, xp - cq) / z.dxFE);
+        fya = z.khAy * (lmask<LAND>(g, cc - sy, cc, cq - ym) / z.dyFS);
+        fyb = z.khAyN * (lmask<LAND>(g, cc, cc + sy, yp - cq) / z.dyFN);
+    } else {
+        fxa = z.khAx * divc(lmask<LAND>(g, cc - 1, cc, cq - xm), g.dx, g.ydx);
+        fxb = z.khAx * divc(lmask<LAND>(g, cc, cc + 1, xp - cq), g.dx, g.ydx);
+        fya = z.khAy * divc(lmask<LAND>(g, cc - sy, cc, cq - ym), g.dy, g.ydy);
+        fyb = z.khAy * divc(lmask<LAND>(g, cc, cc + sy, yp - cq), g.dy, g.ydy);
+    }
     const float ddx = lmask<LAND>(g, cc, cc + 1, fxb - fxa);
     const float ddy = lmask<LAND>(g, cc, cc + sy, fyb - fya);
     const float ddz = lmask<LAND>(g, cc, c, dzq - d);
@@ -495,11 +548,11 @@ __device__ __forceinline__ void cell_update(const FinalArgs& a, const DevGrid& g
     if (BCS) {
         const bool edge = z.i == g.Hx + 1 || z.i == g.Hx + g.Nx || z.j == g.Hy + 1 || z.j == g.Hy + g.Ny ||
                           z.k == g.Hz + 1 || z.k == g.Hz + g.Nz;
-        if (edge && a.bcm[q]) G = flux_bcs(a, g, q, z, G);
+        if (edge && a.bcm[q]) G = flux_bcs<GM>(a, g, q, z, G);
     }
     /* apply_tendency_kernel! (tracer_update.jl:6-20); the tracers without
      * particle exchange add a zero plk */
-    out = cq + (G + divc(pk, z.vol, z.yvol));
+    out = cq + (G + per_vol<GM>(pk, z));
 }
 
 __device__ __forceinline__ int sink_index(int q) { return q == PI_PFe_inorg ? 0 : q == PI_Dust ? 1 : q == PI_PFe_bio ? 2 : -1; }
@@ -559,7 +612,7 @@ __device__ __forceinline__ void face_fluxes_grp(const DevGrid& g, const Face& F,
 /* the tracers of group G at one cell: the fluxes of the face below it, the
  * updates, the stores and the carried state (cu, cs, fz, dz; indexed by the
  * position in the group, cs by sink_index) */
-template <bool LAND, bool BCS, int G>
+template <bool LAND, bool BCS, bool GM, int G>
 __device__ __forceinline__ void final_grp(const FinalArgs& a, const DevGrid& g, const float* C0, float* OUT,
                                           const Face& Fb, bool zero, const ZCell& z, const float* tt, float* cu,
                                           float* fz, float* dz, float* cs, float (*pw)[4], const HaloImg& hi)
@@ -577,7 +630,7 @@ __device__ __forceinline__ void final_grp(const FinalArgs& a, const DevGrid& g, 
         const float pk = has_plk(q) ? ld(fld(a.plk, g, q), cc) : 0.0f;
         const int si = sink_index(q);
         /* pw[i][1] = the z-sweep input at cc */
-        cell_update<LAND, BCS>(a, g, z, q, tt, cu[i], cnv[i], ld(co, cc - 1), ld(co, cc + 1), ld(co, cc - sy),
+        cell_update<LAND, BCS, GM>(a, g, z, q, tt, cu[i], cnv[i], ld(co, cc - 1), ld(co, cc + 1), ld(co, cc - sy),
                           ld(co, cc + sy), pw[i][1], fz[i], f[i], dz[i], si >= 0 ? cs[si] : 0.0f,
                           pk, outv[i], dv[i]);
     }
@@ -614,13 +667,14 @@ __device__ __forceinline__ void final_grp(const FinalArgs& a, const DevGrid& g, 
 }
 
 /* one column run (cells k0..k1-1 of column i, j) for the tracers of group G */
-template <bool LAND, bool BCS, int G>
+template <bool LAND, bool BCS, bool GM, int G>
 __device__ __forceinline__ void final_run(const FinalArgs& a, const DevGrid& g, const float* C0, float* OUT, int i,
                                           int j, int r, int it)
 {
     const uint32_t sz = (uint32_t)g.Px * (uint32_t)g.Py;
     const float dt = a.dt;
-    const float Az = g.dx * g.dy;
+    const long long m2 = col2(g, i, j);
+    const float Az = GM ? g.mAz[m2] : g.dx * g.dy;
     const float kvAz = a.kv * Az;
     const int k0 = g.Hz + 1 + r * a.run;
     const int k1 = min(k0 + a.run, g.Nz + g.Hz + 1);     /* cells k0..k1-1, faces k0..k1 */
@@ -669,24 +723,37 @@ __device__ __forceinline__ void final_run(const FinalArgs& a, const DevGrid& g, 
         const Face Fb = face(g, ld(a.w, c), dt, dzC_at(g, kk), g.ydzC[kk - 1], Az);
         const bool zero = kk == g.Nz + g.Hz + 1 && g.tz == PI_BOUNDED;
         z.dT = lmask<LAND>(g, z.cc, c, Fa.T - Fb.T);
-        z.vol = g.vol[z.k - 1];
-        z.yvol = g.yvol[z.k - 1];
-        z.coef = divc(dt, z.vol, z.yvol);
-        z.khAx = a.kh * (g.dy * dzF_at(g, z.k));
-        z.khAy = a.kh * (g.dx * dzF_at(g, z.k));
-        z.inv_vol = divc(1.0f, z.vol, z.yvol);
+        if (GM) {
+            const uint32_t cc = z.cc;
+            z.vol = g.mVol[cc];
+            z.yvol = 0.0;
+            z.coef = dt / z.vol;
+            z.khAx = a.kh * g.mAx[cc]; z.khAxE = a.kh * g.mAx[cc + 1];
+            z.khAy = a.kh * g.mAy[cc]; z.khAyN = a.kh * g.mAy[cc + (uint32_t)g.Px];
+            z.dxFW = g.mdxF[m2]; z.dxFE = g.mdxF[m2 + 1];
+            z.dyFS = g.mdyF[m2]; z.dyFN = g.mdyF[m2 + g.Px];
+            z.kvAz = kvAz;
+            z.inv_vol = 1.0f / z.vol;
+        } else {
+            z.vol = g.vol[z.k - 1];
+            z.yvol = g.yvol[z.k - 1];
+            z.coef = divc(dt, z.vol, z.yvol);
+            z.khAx = a.kh * (g.dy * dzF_at(g, z.k));
+            z.khAy = a.kh * (g.dx * dzF_at(g, z.k));
+            z.inv_vol = divc(1.0f, z.vol, z.yvol);
+        }
         z.dzf = dzF_at(g, kk);
         z.ydzf = g.ydzF[kk - 1];
         float tt[PI_NTRACERS] = {};    /* tracer_temp = max(0, c) (tracer_forcings.jl:3-5) */
 #pragma unroll
         for (int x = 0; x < fg_n(G); ++x) tt[fgrp(G, x)] = dmaxf(0.0f, cu[x]);
         if (G == 3) tt[PI_POC] = dmaxf(0.0f, ld(fld(C0, g, PI_POC), z.cc));
-        final_grp<LAND, BCS, G>(a, g, C0, OUT, Fb, zero, z, tt, cu, fz, dz, cs, pw, hi);
+        final_grp<LAND, BCS, GM, G>(a, g, C0, OUT, Fb, zero, z, tt, cu, fz, dz, cs, pw, hi);
         Fa = Fb;
     }
 }
 
-template <bool LAND, bool BCS>
+template <bool LAND, bool BCS, bool GM>
 __device__ __forceinline__ void k_final_body(const FinalArgs& a, Blk K)
 {
     const DevState* S = a.st;
@@ -705,23 +772,27 @@ __device__ __forceinline__ void k_final_body(const FinalArgs& a, Blk K)
         const int j = (int)((u >> 2) % g.Ny) + 1 + g.Hy;
         const int r = (int)((u >> 2) / g.Ny);
         switch (G) {
-            case 0: final_run<LAND, BCS, 0>(a, g, C0, OUT, i, j, r, it); break;
-            case 1: final_run<LAND, BCS, 1>(a, g, C0, OUT, i, j, r, it); break;
-            case 2: final_run<LAND, BCS, 2>(a, g, C0, OUT, i, j, r, it); break;
-            default: final_run<LAND, BCS, 3>(a, g, C0, OUT, i, j, r, it); break;
+            case 0: final_run<LAND, BCS, GM, 0>(a, g, C0, OUT, i, j, r, it); break;
+            case 1: final_run<LAND, BCS, GM, 1>(a, g, C0, OUT, i, j, r, it); break;
+            case 2: final_run<LAND, BCS, GM, 2>(a, g, C0, OUT, i, j, r, it); break;
+            default: final_run<LAND, BCS, GM, 3>(a, g, C0, OUT, i, j, r, it); break;
         }
     }
 }
 /* at least 5 waves per SIMD: 0.398 -> 0.383 ms on the 256x256x64 run (4 waves at 108 VGPRs) */
-__device__ __forceinline__ void k_final0_body(const FinalArgs& a, Blk K) { k_final_body<false, false>(a, K); }
-__device__ __forceinline__ void k_final1_body(const FinalArgs& a, Blk K) { k_final_body<true, false>(a, K); }
-/* the same with flux boundary conditions (apply_bcs!) */
-__device__ __forceinline__ void k_final2_body(const FinalArgs& a, Blk K) { k_final_body<false, true>(a, K); }
-__device__ __forceinline__ void k_final3_body(const FinalArgs& a, Blk K) { k_final_body<true, true>(a, K); }
-PI_KERNEL3W(k_final0, k_final0_body, FinalArgs, 256, xcd_block(), 5)
-PI_KERNEL3W(k_final1, k_final1_body, FinalArgs, 256, xcd_block(), 5)
-PI_KERNEL3W(k_final2, k_final2_body, FinalArgs, 256, xcd_block(), 5)
-PI_KERNEL3W(k_final3, k_final3_body, FinalArgs, 256, xcd_block(), 5)
+/* k_final<n>: bit 0 landmask, bit 1 flux boundary conditions (apply_bcs!),
+ * bit 2 LatLonGrid metrics */
+#define PI_FINAL(N, LAND, BCS, GM)                                                                    \
+    __device__ __forceinline__ void k_final##N##_body(const FinalArgs& a, Blk K) { k_final_body<LAND, BCS, GM>(a, K); } \
+    PI_KERNEL3W(k_final##N, k_final##N##_body, FinalArgs, 256, xcd_block(), 5)
+PI_FINAL(0, false, false, false)
+PI_FINAL(1, true, false, false)
+PI_FINAL(2, false, true, false)
+PI_FINAL(3, true, true, false)
+PI_FINAL(4, false, false, true)
+PI_FINAL(5, true, false, true)
+PI_FINAL(6, false, true, true)
+PI_FINAL(7, true, true, true)
 
 /* ------------------------------------------------- fused x + y sweeps (k_xy)
  * calc_Gcsˣ! + multi_dim_x! + fill_halo_tracer! + calc_Gcsʸ! + multi_dim_y!
@@ -975,7 +1046,7 @@ PI_KERNEL3W(k_xyw1, k_xyw1_body, XYArgs, XY_W, xcd_block(), 4)
 
 bool xy_supported(const DevGrid& g)
 {
-    return g.Hx == 2 && g.Hy == 2 && g.Hz == 2 && g.Nx <= XY_W && g.tz == PI_BOUNDED &&
+    return !g.gm && g.Hx == 2 && g.Hy == 2 && g.Hz == 2 && g.Nx <= XY_W && g.tz == PI_BOUNDED &&
            !(g.tx == PI_PERIODIC && g.Nx < g.Hx) && !(g.ty == PI_PERIODIC && g.Ny < g.Hy);
 }
 bool xy_wide(const DevGrid& g) { return g.Nx > XY_T; }
@@ -1001,14 +1072,22 @@ void reg_tracers(KInfo* k)
 {
     PI_REG(k, K_SWEEPX0, k_sweep_x0, SweepArgs, 256, false);
     PI_REG(k, K_SWEEPX1, k_sweep_x1, SweepArgs, 256, false);
+    PI_REG(k, K_SWEEPX2, k_sweep_x2, SweepArgs, 256, false);
+    PI_REG(k, K_SWEEPX3, k_sweep_x3, SweepArgs, 256, false);
     PI_REG(k, K_SWEEPY0, k_sweep_y0, SweepArgs, 256, true);
     PI_REG(k, K_SWEEPY1, k_sweep_y1, SweepArgs, 256, true);
+    PI_REG(k, K_SWEEPY2, k_sweep_y2, SweepArgs, 256, true);
+    PI_REG(k, K_SWEEPY3, k_sweep_y3, SweepArgs, 256, true);
     PI_REG(k, K_FILL, k_fill_line, FillArgs, 256, true);
     PI_REG(k, K_HALOCOPY, k_halo_copy, FillArgs, 256, true);
     PI_REG(k, K_FINAL0, k_final0, FinalArgs, 256, true);
     PI_REG(k, K_FINAL1, k_final1, FinalArgs, 256, true);
     PI_REG(k, K_FINAL2, k_final2, FinalArgs, 256, true);
     PI_REG(k, K_FINAL3, k_final3, FinalArgs, 256, true);
+    PI_REG(k, K_FINAL4, k_final4, FinalArgs, 256, true);
+    PI_REG(k, K_FINAL5, k_final5, FinalArgs, 256, true);
+    PI_REG(k, K_FINAL6, k_final6, FinalArgs, 256, true);
+    PI_REG(k, K_FINAL7, k_final7, FinalArgs, 256, true);
     PI_REG(k, K_XY0, k_xy0, XYArgs, XY_T, true);
     PI_REG(k, K_XY1, k_xy1, XYArgs, XY_T, true);
     PI_REG(k, K_XYW0, k_xyw0, XYArgs, XY_W, true);

@@ -50,6 +50,11 @@ struct DevGrid {
     const double* ydzF;  /* Pz */
     const double* yvol;  /* Pz */
     const float* vol;    /* Pz: (Δx·Δy)·dzF[k], the cell volume as the reference forms it */
+    /* LatLonGrid (gm = 1): the spacings, areas and volumes per cell
+     * (Grids/lat_lon_grid.jl:1-37), padded (Px, Py) and (Px, Py, Pz) arrays;
+     * every quotient by them is an IEEE division */
+    int gm;
+    const float *mdxC, *mdyC, *mdxF, *mdyF, *mAz, *mAx, *mAy, *mVol;
 };
 
 /* a / b correctly rounded, without branches, given yd = RN_double(1/b).
@@ -144,6 +149,13 @@ __device__ __forceinline__ float dzC_at(const DevGrid& g, long long k) {   /* cl
     return g.dzC[k - 1];
 }
 __device__ __forceinline__ float dzF_at(const DevGrid& g, int k) { return g.dzF[k - 1]; }
+/* volume(i, j, k) of the cell c at level k (Grids/areas_volumes_spacings.jl:22,38) */
+__device__ __forceinline__ float cell_vol(const DevGrid& g, long long c, int k)
+{
+    return g.gm ? g.mVol[c] : g.dx * g.dy * dzF_at(g, k);
+}
+/* column (i, j) of the padded 2-D metric arrays, 1-based */
+__device__ __forceinline__ long long col2(const DevGrid& g, int i, int j) { return (long long)(i - 1) + (long long)g.Px * (j - 1); }
 /* a / b in 3 operations for the hot paths, given yd = RN_double(1/b): the
  * product rounded once to double then to float, which is RN(a/b) except when
  * a/b is exactly a rounding boundary of the subnormal range (an odd multiple

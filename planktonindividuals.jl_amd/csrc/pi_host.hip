@@ -302,6 +302,20 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
         DEVCALL(hipMemcpy(dvol, vol.data(), 4ll * Pz, hipMemcpyHostToDevice));
         g.ydzC = tabs; g.ydzF = tabs + Pz; g.yvol = tabs + 2 * Pz; g.vol = dvol;
     }
+    g.gm = 0;
+    if (d->metrics) {      /* LatLonGrid: the metric arrays, with halos */
+        const pi_metrics& mt = *d->metrics;
+        const float* src[8] = {mt.dxC, mt.dyC, mt.dxF, mt.dyF, mt.Az, mt.Ax, mt.Ay, mt.Vol};
+        const float** dst[8] = {&g.mdxC, &g.mdyC, &g.mdxF, &g.mdyF, &g.mAz, &g.mAx, &g.mAy, &g.mVol};
+        for (int a = 0; a < 8; ++a) {
+            const long long n = a < 5 ? (long long)g.Px * g.Py : g.ncell;
+            float* p;
+            DALLOC(p, n * 4);
+            DEVCALL(hipMemcpy(p, src[a], n * 4, hipMemcpyHostToDevice));
+            *dst[a] = p;
+        }
+        g.gm = 1;
+    }
     g.has_land = 0; g.lm = nullptr;
     if (gd.landmask) {
         bool any0 = false;
@@ -427,6 +441,11 @@ static pi_status check_desc(const pi_model_desc* d)
     if (d->capacity < 1 || d->capacity > (long long)PI_SLOT_MASK) return fail(PI_ERR_ARG, "capacity out of range");
     if (d->mode != PI_MODE_QUOTA && d->mode != PI_MODE_CARBON)
         return fail(PI_ERR_UNSUPPORTED, "physiology mode: QuotaMode and CarbonMode are implemented");
+    if (d->metrics) {
+        const pi_metrics& mt = *d->metrics;
+        if (!mt.dxC || !mt.dyC || !mt.dxF || !mt.dyF || !mt.Az || !mt.Ax || !mt.Ay || !mt.Vol)
+            return fail(PI_ERR_ARG, "grid metrics: every array is required");
+    }
     const int dvt = d->mode == PI_MODE_CARBON ? (int)PI_C_dvid_type : (int)PI_P_dvid_type;
     for (int s = 0; s < d->n_species; ++s) {
         float t = d->params[s * PI_NPARAMS + dvt];
@@ -814,8 +833,11 @@ static void build_step(pi_model* m, float dt, Plan& P)
             sa.vel = V1.u; sa.dim = 0;
             sa.out = m->T1[0]; sa.prev = nullptr;
             const long long xb = fxy ? 0 : sweep_x_blocks(g);
-            P.add(K_SWEEPX0, "sweep_x", land ? 0 : xb, sa);
-            P.add(K_SWEEPX1, "sweep_x", land ? xb : 0, sa);
+            const bool gm = g.gm != 0;
+            P.add(K_SWEEPX0, "sweep_x", !gm && !land ? xb : 0, sa);
+            P.add(K_SWEEPX1, "sweep_x", !gm && land ? xb : 0, sa);
+            P.add(K_SWEEPX2, "sweep_x", gm && !land ? xb : 0, sa);
+            P.add(K_SWEEPX3, "sweep_x", gm && land ? xb : 0, sa);
             FillArgs fa = zeroed<FillArgs>();
             fa.g = g; fa.st = st; fa.nf = PI_NTRACERS; fa.dim = 1; fa.lines_full = 0;
             fa.f = m->T1[0];
@@ -824,8 +846,10 @@ static void build_step(pi_model* m, float dt, Plan& P)
             sa.vel = V1.v; sa.dim = 1;
             sa.out = m->T2[0]; sa.prev = m->T1[0];
             const long long yb = fxy ? 0 : grid_for(sweep_y_items(g), 256, 8192);
-            P.add(K_SWEEPY0, "sweep_y", land ? 0 : yb, sa);
-            P.add(K_SWEEPY1, "sweep_y", land ? yb : 0, sa);
+            P.add(K_SWEEPY0, "sweep_y", !gm && !land ? yb : 0, sa);
+            P.add(K_SWEEPY1, "sweep_y", !gm && land ? yb : 0, sa);
+            P.add(K_SWEEPY2, "sweep_y", gm && !land ? yb : 0, sa);
+            P.add(K_SWEEPY3, "sweep_y", gm && land ? yb : 0, sa);
             fa.dim = 2;
             fa.f = m->T2[0];
             P.add(K_FILL, "fill", fxy ? 0 : blocks_for((long long)g.Nx * g.Ny * PI_NTRACERS), fa);
@@ -846,7 +870,9 @@ static void build_step(pi_model* m, float dt, Plan& P)
         aa.dstamp = m->dstamp; aa.dcell = m->dcell; aa.cap = m->cap; aa.dbd = m->dbd;
         /* up to 16384 blocks by default: with 8.4M individuals (two entries per
          * thread) ~5% faster than 4096 blocks; 32768 is no faster, 65536 much slower */
-        P.add(K_ADV, "adv", grid_for(2 * m->ctot, 256, (int)m->shape[PI_S_ADV_BLOCKS]), aa);
+        const long long ab = grid_for(2 * m->ctot, 256, (int)m->shape[PI_S_ADV_BLOCKS]);
+        P.add(K_ADV, "adv", g.gm ? 0 : ab, aa);
+        P.add(K_ADV_GM, "adv", g.gm ? ab : 0, aa);
     }
     {
         ScanIn sp = scan_plain(m->stayf, (uint32_t)(2 * m->ctot));      /* stayer prefix over the sorted part */
@@ -926,10 +952,10 @@ static void build_step(pi_model* m, float dt, Plan& P)
             bcs |= fn.bcm[q] != 0;
         }
         const long long fb = grid_for(final_items(g, fn.run), 256, 8192);
-        P.add(K_FINAL0, "final", !land && !bcs ? fb : 0, fn);
-        P.add(K_FINAL1, "final", land && !bcs ? fb : 0, fn);
-        P.add(K_FINAL2, "final", !land && bcs ? fb : 0, fn);
-        P.add(K_FINAL3, "final", land && bcs ? fb : 0, fn);
+        /* k_final<n>: bit 0 land, bit 1 flux boundary conditions, bit 2 LatLonGrid */
+        const int fv = (land ? 1 : 0) | (bcs ? 2 : 0) | (g.gm ? 4 : 0);
+        static const KId fk[8] = {K_FINAL0, K_FINAL1, K_FINAL2, K_FINAL3, K_FINAL4, K_FINAL5, K_FINAL6, K_FINAL7};
+        for (int v = 0; v < 8; ++v) P.add(fk[v], "final", v == fv ? fb : 0, fn);
         /* fill_halo_tracer! of the new tracers: x, then y, then z over whole planes */
         fa.f = nullptr;
         fa.lines_full = 1;
