@@ -76,6 +76,11 @@ enum {
     PI_NCPARAMS = 24
 };
 
+/* ---- abiotic particle parameters (Abiotic/particle_generation.jl:8,
+ *      Parameters/param_default.jl:329-343) ----------------------------------- */
+enum { PI_A_Nsuper = 0, PI_A_Rd, PI_A_release_P, PI_A_sz_min, PI_A_Ktr, PI_NAPARAMS = 5 };
+#define PI_MAXSA 4          /* abiotic species per model */
+
 /* ---- biogeochemical parameters (Parameters/param_default.jl:45-71) ------- */
 enum {
     PI_B_kw = 0, PI_B_kc, PI_B_kDOC, PI_B_Nit, PI_B_kDON, PI_B_kDOP, PI_B_kPOC,
@@ -144,8 +149,19 @@ typedef struct pi_model_desc {
     int32_t n_diag_tracers;      /* tracers accumulated into diags.tracer besides PAR */
     const int32_t* diag_tracers; /* tracer ids, may be NULL when n_diag_tracers == 0 */
     int32_t mode;                /* PI_MODE_QUOTA or PI_MODE_CARBON (PlanktonModel(...; mode)) */
-    const pi_metrics* metrics;   /* LatLonGrid metrics; NULL = RectilinearGrid (dx, dy of grid).
-                                    The last member: callers must set it (NULL for rectilinear). */
+    const pi_metrics* metrics;   /* LatLonGrid metrics; NULL = RectilinearGrid (dx, dy of grid) */
+    /* abiotic particles (abiotic_setup, src/PlanktonIndividuals.jl:124-129;
+     * Individuals/Abiotic/particle_generation.jl:1-35): n_abiotic species of
+     * `capacity` slots each, PI_NAPARAMS floats per species.  They are carried
+     * by the flow and the random walk, created at the faces by their flux
+     * boundary conditions and counted per cell; n_interactions (the pairs of
+     * palat.intac) must be 0: phytoplankton-particle interaction needs the
+     * max_ptc parameter of IronEnergyMode (particle_interaction.jl:60), which
+     * this library does not implement.  The release pairs (palat.release) change
+     * nothing without interaction (ptc stays 0) and are not passed. */
+    int32_t n_abiotic;
+    const float* abiotic_params;
+    int32_t n_interactions;
 } pi_model_desc;
 
 /* Host-side view of one species' individuals, all arrays of length capacity.
@@ -229,6 +245,24 @@ enum { PI_BC_WEST = 0, PI_BC_EAST, PI_BC_SOUTH, PI_BC_NORTH, PI_BC_BOTTOM, PI_BC
 enum { PI_BCK_NONE = 0, PI_BCK_CONST, PI_BCK_FIELD, PI_BCK_SERIES };
 pi_status pi_set_tracer_bc(pi_model* m, int32_t tracer, int32_t face, int32_t kind, const float* values,
                            int64_t nt);
+
+/* Abiotic particles (Abiotic/particle_generation.jl): the columns of species
+ * `sa` (x, y, z: float, xi, yi, zi: int64, ac: uint8; merg and idx are always
+ * 0 here), each of length capacity.  Upload: x, y, z, ac (initialize_abiotic_particle!,
+ * :24-35, drawn on the host); xi, yi, zi start at 0. */
+enum { PI_AB_x = 0, PI_AB_y, PI_AB_z, PI_AB_xi, PI_AB_yi, PI_AB_zi, PI_AB_ac };
+pi_status pi_upload_abiotic(pi_model* m, int32_t sa, const float* x, const float* y, const float* z,
+                            const uint8_t* ac);
+pi_status pi_download_abiotic(pi_model* m, int32_t sa, int32_t field, void* host, int64_t n);
+/* diags.abiotics.saN.num (padded): particles per cell, accumulated per step
+ * (time_step.jl:152-159); zeroed by pi_reset_diags. */
+pi_status pi_download_abiotic_diag(pi_model* m, int32_t sa, float* padded);
+/* set_bc_particle!(model; sa, pos, bc_value) (boundary_conditions.jl:31-39):
+ * a flux of particles through a face (same positions, kinds and nt rule as
+ * pi_set_tracer_bc); particles_from_bcs! (particle_interaction.jl:249-267)
+ * creates one particle at the cell next to the face when a uniform draw times
+ * sz_min·Nsuper is below bc·ΔT·A. */
+pi_status pi_set_particle_bc(pi_model* m, int32_t sa, int32_t face, int32_t kind, const float* values, int64_t nt);
 
 /* Scalars. */
 pi_status pi_get_iteration(pi_model* m, int64_t* iteration);

@@ -47,6 +47,10 @@ enum { PI_KIND_DIFFUSION = 0, PI_KIND_DECISION = 1 };
 /* initial-condition sub-streams (iteration field 0): positions, quotas, the two
  * uniforms of the size normal, and per tracer (slot = padded cell index) */
 enum { PI_KIND_INIT_POS = 16, PI_KIND_INIT_QUOTA = 17, PI_KIND_INIT_SIZE = 18, PI_KIND_TRACER = 32 };
+/* abiotic particles: species field PI_ABIO_SPECIES + sa; the random walk uses
+ * PI_KIND_DIFFUSION, particles_from_bcs! one draw per padded cell (slot = the
+ * cell's index), the initial positions PI_KIND_INIT_POS */
+enum { PI_ABIO_SPECIES = 64, PI_KIND_ABIO_BC = 2 };
 
 PI_HD void pi_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4])
 {

@@ -22,13 +22,13 @@ SRC = os.path.join(HERE, "pi_oracle.cpp")
 BC_BOTTOM = """            if (bck[q][PI_BC_BOTTOM])
                 for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
                     const int ii = i + Hx, jj = j + Hy, kk = Nz + Hz;
-                    G[I(ii, jj, kk)] += getbc(q, PI_BC_BOTTOM, i, j, iter) * dt * Az() / vol(kk);
+                    G[I(ii, jj, kk)] += getbc(q, PI_BC_BOTTOM, i, j, iter) * dt * Az(ii, jj) / vol(ii, jj, kk);
                 }
 """
 BC_TOP = """            if (bck[q][PI_BC_TOP])
                 for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i) {
                     const int ii = i + Hx, jj = j + Hy, kk = 1 + Hz;
-                    G[I(ii, jj, kk)] -= getbc(q, PI_BC_TOP, i, j, iter) * dt * Az() / vol(kk);
+                    G[I(ii, jj, kk)] -= getbc(q, PI_BC_TOP, i, j, iter) * dt * Az(ii, jj) / vol(ii, jj, kk);
                 }
 """
 BOTTOM_TOP, TOP_BOTTOM = BC_BOTTOM + BC_TOP, BC_TOP + BC_BOTTOM
@@ -39,8 +39,8 @@ MISTAKES = [
      "planes_zero(c, 2, Nz + Hz + 1, Hz); }"),
     ("z sign", "oz[i] = pz[i] - M::mask(ac[i], wt * w[i] * dt);", "oz[i] = pz[i] + M::mask(ac[i], wt * w[i] * dt);"),
     ("w face offset", "FT zs = comp == 2 ? z + L(1) : z + L(0.5f);", "FT zs = z + L(0.5f);"),
-    ("PAR with dzC for dzF", "FT atten = (Chl[I(ii, jj, kk)] / vol(kk) * kc + kw) * dzF_at(kk);",
-     "FT atten = (Chl[I(ii, jj, kk)] / vol(kk) * kc + kw) * dzC_at(kk);"),
+    ("PAR with dzC for dzF", "FT atten = (Chl[I(ii, jj, kk)] / vol(ii, jj, kk) * kc + kw) * dzF_at(kk);",
+     "FT atten = (Chl[I(ii, jj, kk)] / vol(ii, jj, kk) * kc + kw) * dzC_at(kk);"),
     ("nitrification source", "*F[PI_NO3] = *F[PI_NO3] + T(PI_NH4, n) * Nit * dt;",
      "*F[PI_NO3] = *F[PI_NO3] + T(PI_NO3, n) * Nit * dt;"),
     ("dust scavenging dropped in PFe_inorg", "(lmin + ldust * T(PI_Dust, n)) * DFe * DFeFrac * dt +\n                               excess",
@@ -105,8 +105,8 @@ MISTAKES = [
     ("carbon: PS on the whole biomass", "M::max(z0, p.Bm[i] - p.Cq[i])", "M::max(z0, p.Bm[i])"),
     ("flux bc: east adds", "G[I(ii, jj, kk)] -= getbc(q, PI_BC_EAST, j, k, iter)",
      "G[I(ii, jj, kk)] += getbc(q, PI_BC_EAST, j, k, iter)"),
-    ("flux bc: south with Ax", "getbc(q, PI_BC_SOUTH, i, k, iter) * dt * Ay(kk)",
-     "getbc(q, PI_BC_SOUTH, i, k, iter) * dt * Ax(kk)"),
+    ("flux bc: south with Ax", "getbc(q, PI_BC_SOUTH, i, k, iter) * dt * Ay(ii, jj, kk)",
+     "getbc(q, PI_BC_SOUTH, i, k, iter) * dt * Ax(ii, jj, kk)"),
     ("flux bc: west on the halo plane", "const int ii = 1 + Hx, jj = j + Hy, kk = k + Hz;",
      "const int ii = Hx, jj = j + Hy, kk = k + Hz;"),
     ("flux bc: bottom on the top level", "const int ii = i + Hx, jj = j + Hy, kk = Nz + Hz;",
@@ -133,6 +133,23 @@ MISTAKES = [
      "FT atten = (Chl[I(ii, jj, kk)] / (Az(ii, jj) * dzF_at(kk)) * kc + kw) * dzF_at(kk);"),
     ("latlon: exchange over the volume below", "tr[q][n] = tr[q][n] + (Gcs[q][n] + plk[q][n] / vol(i, j, k));",
      "tr[q][n] = tr[q][n] + (Gcs[q][n] + plk[q][n] / vol(i, j, k + 1));"),
+    ("abiotic: created at the cell centre", "p.x[r] = (FT)(float)(i - Hx) - L(0.4f);",
+     "p.x[r] = (FT)(float)(i - Hx) - L(0.5f);"),
+    ("abiotic: draw of the next cell", "pi_uniform3(seed, iteration, PI_ABIO_SPECIES + a, PI_KIND_ABIO_BC, c, &u0, &u1, &u2);",
+     "pi_uniform3(seed, iteration, PI_ABIO_SPECIES + a, PI_KIND_ABIO_BC, c + 1, &u0, &u1, &u2);"),
+    ("abiotic: top flux through Ax", "put(i + Hx, j + Hy, 1 + Hz, bcv(PI_BC_TOP, i, j), Az(i + Hx, j + Hy));",
+     "put(i + Hx, j + Hy, 1 + Hz, bcv(PI_BC_TOP, i, j), Ax(i + Hx, j + Hy, 1 + Hz));"),
+    ("abiotic: free slots from the end", "for (int64_t i = 0; i < cap; ++i) if (!p.ac[i]) freei.push_back(i);",
+     "for (int64_t i = cap - 1; i >= 0; --i) if (!p.ac[i]) freei.push_back(i);"),
+    ("abiotic: walk on the phytoplankton sub-stream", "particle_diffusion(ab[a], PI_ABIO_SPECIES + a, dt);",
+     "particle_diffusion(ab[a], a, dt);"),
+    ("abiotic: num counts every slot", "abNum[a][cell_of(ab[a], i)] += (FT)(ab[a].ac[i] ? 1 : 0);",
+     "abNum[a][cell_of(ab[a], i)] += (FT)1;"),
+    ("abiotic: series one iteration late", "const int64_t tt = abck[a][f] == PI_BCK_SERIES ? iteration - 1 : 0;",
+     "const int64_t tt = abck[a][f] == PI_BCK_SERIES ? iteration : 0;"),
+    ("abiotic: the first face decides at an edge",
+     "flag[c] = M::isless(rnd(c) * szN * Ns, b * dt * A) ? (FT)1 : (FT)0;",
+     "if (flag[c] == (FT)0) flag[c] = M::isless(rnd(c) * szN * Ns, b * dt * A) ? (FT)1 : (FT)0;"),
 ]
 
 
@@ -156,7 +173,7 @@ def main(only):
             r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-m", "not gpu", "-p", "no:cacheprovider",
                                 "tests/test_oracle_pins.py", "tests/test_oracle_pins_physics.py",
                                 "tests/test_oracle_pins_carbon.py", "tests/test_oracle_pins_bcs.py",
-                                "tests/test_oracle_pins_latlon.py"],
+                                "tests/test_oracle_pins_latlon.py", "tests/test_oracle_pins_abiotic.py"],
                                cwd=REPO, env=env, capture_output=True, text=True)
             first = next((l for l in r.stdout.splitlines() if l.startswith("FAILED")), "")
             (caught if r.returncode != 0 else missed).append(name)

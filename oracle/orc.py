@@ -89,7 +89,8 @@ class OracleModel:
         lib = load()
         desc, keep = pi._abi.make_desc(spec.grid, spec.n_species, spec.max_individuals,
                                        spec.species_params, spec.bgc_vector, spec.seed, diag_tracers,
-                                       getattr(spec, "mode_id", pi._abi.MODE_QUOTA))
+                                       getattr(spec, "mode_id", pi._abi.MODE_QUOTA),
+                                       getattr(spec, "abiotic_matrix", None))
         raw = C.c_void_p()
         pi._native.check(lib, "orc_", lib.orc_model_create(C.byref(desc), precision, C.byref(raw)))
         del keep
@@ -101,6 +102,8 @@ class OracleModel:
             self.h.upload_individuals(s, spec.initial_individuals[s])
         for tid, c in enumerate(spec.initial_tracers):
             self.h.upload_tracer(tid, c)
+        for a, arrays in enumerate(getattr(spec, "initial_abiotics", [])):
+            self.h.upload_abiotic(a, arrays)
 
     def __getattr__(self, k):
         return getattr(self.h, k)

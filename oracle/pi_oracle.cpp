@@ -119,6 +119,17 @@ struct Oracle : orc_model {
     V diagPAR;
     std::vector<V> diagTr;
     std::vector<std::array<V, 5>> diagSp;   /* num, graz, mort, dvid, ptc */
+    /* abiotic particles (Individuals/Abiotic): per species the x, y, z, ac and
+     * xi, yi, zi columns (in a Plank), parameters, diags.abiotics.saN.num and
+     * the particle flux boundary conditions */
+    int nsa = 0;
+    std::vector<std::array<FT, PI_NAPARAMS>> AP;
+    std::vector<Plank> ab;
+    std::vector<V> abNum;
+    int abck[PI_MAXSA][PI_NBC] = {};
+    FT abcv[PI_MAXSA][PI_NBC] = {};
+    V abca[PI_MAXSA][PI_NBC];
+    int64_t abcnt[PI_MAXSA][PI_NBC] = {};
     /* timestepper scratch (timestepper.jl:40-55) */
     V rx, ry, rz, vx, vy, vz, u1, v1, w1, u2, v2, w2;
     V tNH4, tNO3, tPO4, tDOC, tDFe, tpar, tT, tpop, tdpar, idc;
@@ -222,6 +233,17 @@ struct Oracle : orc_model {
         diagTr.assign(diag_tr.size(), V(ncell, 0));
         diagSp.resize(nsp);
         for (auto& a : diagSp) for (auto& v : a) v.assign(ncell, 0);
+        nsa = d->n_abiotic;
+        if (nsa < 0 || nsa > PI_MAXSA) return fail(PI_ERR_ARG, "n_abiotic out of range");
+        if (d->n_interactions != 0) return fail(PI_ERR_UNSUPPORTED, "phytoplankton-particle interaction needs IronEnergyMode");
+        AP.resize(nsa); ab.resize(nsa); abNum.assign(nsa, V(ncell, 0));
+        for (int a = 0; a < nsa; ++a) {
+            for (int q = 0; q < PI_NAPARAMS; ++q) AP[a][q] = (FT)d->abiotic_params[a * PI_NAPARAMS + q];
+            Plank& p = ab[a];
+            for (V* f : {&p.x, &p.y, &p.z}) f->assign(cap, (FT)0);
+            p.xi.assign(cap, 0); p.yi.assign(cap, 0); p.zi.assign(cap, 0);
+            p.ac.assign(cap, 0);
+        }
         for (V* f : {&rx, &ry, &rz, &vx, &vy, &vz, &u1, &v1, &w1, &u2, &v2, &w2, &tNH4, &tNO3, &tPO4,
                      &tDOC, &tDFe, &tpar, &tT, &tpop, &tdpar, &idc})
             f->assign(cap, 0);
@@ -1133,6 +1155,67 @@ struct Oracle : orc_model {
         fill_halo_tracer(tr);
     }
 
+    /* particles_from_bcs! (Individuals/Abiotic/particle_interaction.jl:158-267):
+     * tr_temp = 0; calc_particle_bc_{top,bottom,east,west,north,south}! write
+     * isless(rnd·sz_min·Nsuper, bc·ΔT·A) at the cells next to each face with a
+     * condition (later faces overwrite earlier ones); the k-th flagged cell in
+     * column-major order fills the k-th free slot at (i, j, k) - H - 0.4. */
+    pi_status particles_from_bcs(int a, FT dt) {
+        Plank& p = ab[a];
+        const FT szN = AP[a][PI_A_sz_min], Ns = AP[a][PI_A_Nsuper];
+        V flag(ncell, (FT)0);
+        auto rnd = [&](int64_t c) {
+            float u0, u1, u2;
+            pi_uniform3(seed, iteration, PI_ABIO_SPECIES + a, PI_KIND_ABIO_BC, c, &u0, &u1, &u2);
+            return (FT)u0;
+        };
+        auto bcv = [&](int f, int i, int j) -> FT {
+            if (abck[a][f] == PI_BCK_CONST) return abcv[a][f];
+            const int64_t na = f <= PI_BC_EAST ? Ny : Nx, nb = f <= PI_BC_NORTH ? Nz : Ny;
+            const int64_t tt = abck[a][f] == PI_BCK_SERIES ? iteration - 1 : 0;
+            return abca[a][f][(int64_t)(i - 1) + na * ((int64_t)(j - 1) + nb * tt)];
+        };
+        auto put = [&](int ii, int jj, int kk, FT b, FT A) {
+            const int64_t c = I(ii, jj, kk);
+            flag[c] = M::isless(rnd(c) * szN * Ns, b * dt * A) ? (FT)1 : (FT)0;
+        };
+        if (abck[a][PI_BC_TOP])
+            for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i)
+                put(i + Hx, j + Hy, 1 + Hz, bcv(PI_BC_TOP, i, j), Az(i + Hx, j + Hy));
+        if (abck[a][PI_BC_BOTTOM])
+            for (int j = 1; j <= Ny; ++j) for (int i = 1; i <= Nx; ++i)
+                put(i + Hx, j + Hy, Nz + Hz, bcv(PI_BC_BOTTOM, i, j), Az(i + Hx, j + Hy));
+        if (abck[a][PI_BC_EAST])
+            for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j)
+                put(Nx + Hx, j + Hy, k + Hz, bcv(PI_BC_EAST, j, k), Ax(Nx + Hx, j + Hy, k + Hz));
+        if (abck[a][PI_BC_WEST])
+            for (int k = 1; k <= Nz; ++k) for (int j = 1; j <= Ny; ++j)
+                put(1 + Hx, j + Hy, k + Hz, bcv(PI_BC_WEST, j, k), Ax(1 + Hx, j + Hy, k + Hz));
+        if (abck[a][PI_BC_NORTH])
+            for (int k = 1; k <= Nz; ++k) for (int i = 1; i <= Nx; ++i)
+                put(i + Hx, Ny + Hy, k + Hz, bcv(PI_BC_NORTH, i, k), Ay(i + Hx, Ny + Hy, k + Hz));
+        if (abck[a][PI_BC_SOUTH])
+            for (int k = 1; k <= Nz; ++k) for (int i = 1; i <= Nx; ++i)
+                put(i + Hx, 1 + Hy, k + Hz, bcv(PI_BC_SOUTH, i, k), Ay(i + Hx, 1 + Hy, k + Hz));
+        std::vector<int64_t> inds, freei;
+        for (int64_t c = 0; c < ncell; ++c) if (flag[c] == (FT)1) inds.push_back(c);    /* findall, column-major */
+        if (inds.empty()) return PI_OK;
+        for (int64_t i = 0; i < cap; ++i) if (!p.ac[i]) freei.push_back(i);
+        if (inds.size() > freei.size()) {
+            failed = true;
+            return fail(PI_ERR_CAPACITY, "number of abiotic particles exceeds the capacity");
+        }
+        for (size_t k = 0; k < inds.size(); ++k) {       /* copy_abiotic_particle_from_field_kernel! */
+            const int64_t c = inds[k], r = freei[k];
+            const int i = (int)(c % Px) + 1, j = (int)((c / Px) % Py) + 1, kk = (int)(c / ((int64_t)Px * Py)) + 1;
+            p.ac[r] = 1;
+            p.x[r] = (FT)(float)(i - Hx) - L(0.4f);
+            p.y[r] = (FT)(float)(j - Hy) - L(0.4f);
+            p.z[r] = (FT)(float)(kk - Hz) - L(0.4f);
+        }
+        return PI_OK;
+    }
+
     /* ================================================== TimeStep! =======
      * Model/time_step.jl:12-180 */
     pi_status step(float dt_in) {
@@ -1141,6 +1224,10 @@ struct Oracle : orc_model {
             for (int f = 0; f < PI_NBC; ++f)
                 if (bcnt[q][f] && iteration + 1 > bcnt[q][f])
                     return fail(PI_ERR_ARG, "BC: time-dependent flux boundary condition shorter than the run");
+        for (int a = 0; a < nsa; ++a)
+            for (int f = 0; f < PI_NBC; ++f)
+                if (abcnt[a][f] && iteration + 1 > abcnt[a][f])
+                    return fail(PI_ERR_ARG, "BC: time-dependent particle boundary condition shorter than the run");
         const FT dt = (FT)dt_in;
         for (const auto& p : sp) for (int64_t i = 0; i < cap; ++i) ind_steps += p.ac[i] ? 1 : 0;
         iteration += 1;
@@ -1150,6 +1237,13 @@ struct Oracle : orc_model {
             for (int64_t n = 0; n < ncell; ++n) velh[q][n] = (vel0[q][n] + vel1[q][n]) * L(0.5f);
         for (int q = 0; q < PI_NTRACERS; ++q) std::fill(plk[q].begin(), plk[q].end(), (FT)0);
         std::fill(Chl.begin(), Chl.end(), (FT)0);
+        for (int a = 0; a < nsa; ++a) {                 /* abiotic particles (time_step.jl:21-38) */
+            pi_status st = particles_from_bcs(a, dt);
+            if (st != PI_OK) return st;
+            particle_advection(ab[a], dt);
+            particle_diffusion(ab[a], PI_ABIO_SPECIES + a, dt);
+            find_inds(ab[a]);
+        }
         if (B[PI_B_shared_graz] == L(1)) {
             std::fill(pop.begin(), pop.end(), (FT)0);
             for (int s = 0; s < nsp; ++s) {
@@ -1181,6 +1275,8 @@ struct Oracle : orc_model {
                 if (st != PI_OK) return st;
             }
         }
+        for (int a = 0; a < nsa; ++a)                   /* diags_proc!(num, ac, ac, xi, yi, zi) (:152-159) */
+            for (int64_t i = 0; i < cap; ++i) abNum[a][cell_of(ab[a], i)] += (FT)(ab[a].ac[i] ? 1 : 0);
         tracer_update(dt);
         OMP_FOR
         for (int64_t n = 0; n < ncell; ++n) diagPAR[n] += par[n];
@@ -1281,6 +1377,67 @@ static pi_status set_bc_impl(Oracle<FT>* o, int32_t q, int32_t f, int32_t kind, 
 pi_status orc_set_tracer_bc(orc_model* m, int32_t q, int32_t f, int32_t kind, const float* v, int64_t nt) {
     DISPATCH(m, return set_bc_impl(o, q, f, kind, v, nt));
 }
+template <typename FT>
+static pi_status set_pbc_impl(Oracle<FT>* o, int32_t a, int32_t f, int32_t kind, const float* v, int64_t nt) {
+    if (a < 0 || a >= o->nsa) return fail(PI_ERR_ARG, "abiotic species out of range");
+    if (f < 0 || f >= PI_NBC) return fail(PI_ERR_ARG, "boundary position");
+    if (kind < PI_BCK_NONE || kind > PI_BCK_SERIES) return fail(PI_ERR_ARG, "boundary condition kind");
+    if (kind != PI_BCK_NONE && !v) return fail(PI_ERR_ARG, "null boundary condition values");
+    if (kind == PI_BCK_SERIES && nt < 1) return fail(PI_ERR_ARG, "BC: nt >= 1 required");
+    const int64_t n1 = f <= PI_BC_EAST ? o->Ny : o->Nx, n2 = f <= PI_BC_NORTH ? o->Nz : o->Ny;
+    const int64_t n = kind == PI_BCK_FIELD ? n1 * n2 : kind == PI_BCK_SERIES ? n1 * n2 * nt : 0;
+    o->abck[a][f] = kind;
+    o->abcv[a][f] = kind == PI_BCK_CONST ? (FT)v[0] : (FT)0;
+    o->abca[a][f].assign(n, (FT)0);
+    for (int64_t i = 0; i < n; ++i) o->abca[a][f][i] = (FT)v[i];
+    o->abcnt[a][f] = kind == PI_BCK_SERIES ? nt : 0;
+    return PI_OK;
+}
+pi_status orc_set_particle_bc(orc_model* m, int32_t a, int32_t f, int32_t kind, const float* v, int64_t nt) {
+    DISPATCH(m, return set_pbc_impl(o, a, f, kind, v, nt));
+}
+template <typename FT>
+static pi_status up_abio_impl(Oracle<FT>* o, int32_t a, const float* x, const float* y, const float* z, const uint8_t* ac) {
+    if (a < 0 || a >= o->nsa) return fail(PI_ERR_ARG, "abiotic species out of range");
+    auto& p = o->ab[a];
+    for (int64_t i = 0; i < o->cap; ++i) {
+        p.x[i] = (FT)x[i]; p.y[i] = (FT)y[i]; p.z[i] = (FT)z[i]; p.ac[i] = ac[i] ? 1 : 0;
+        p.xi[i] = p.yi[i] = p.zi[i] = 0;
+    }
+    return PI_OK;
+}
+pi_status orc_upload_abiotic(orc_model* m, int32_t a, const float* x, const float* y, const float* z, const uint8_t* ac) {
+    if (!x || !y || !z || !ac) return fail(PI_ERR_ARG, "null argument");
+    DISPATCH(m, return up_abio_impl(o, a, x, y, z, ac));
+}
+template <typename FT>
+static pi_status down_abio_impl(Oracle<FT>* o, int32_t a, int32_t f, void* h, int64_t n) {
+    if (a < 0 || a >= o->nsa) return fail(PI_ERR_ARG, "abiotic species out of range");
+    if (n != o->cap) return fail(PI_ERR_ARG, "n must equal capacity");
+    const auto& p = o->ab[a];
+    for (int64_t i = 0; i < n; ++i) {
+        switch (f) {
+            case PI_AB_x: ((float*)h)[i] = (float)p.x[i]; break;
+            case PI_AB_y: ((float*)h)[i] = (float)p.y[i]; break;
+            case PI_AB_z: ((float*)h)[i] = (float)p.z[i]; break;
+            case PI_AB_xi: ((int64_t*)h)[i] = p.xi[i]; break;
+            case PI_AB_yi: ((int64_t*)h)[i] = p.yi[i]; break;
+            case PI_AB_zi: ((int64_t*)h)[i] = p.zi[i]; break;
+            case PI_AB_ac: ((uint8_t*)h)[i] = p.ac[i]; break;
+            default: return fail(PI_ERR_ARG, "abiotic field id");
+        }
+    }
+    return PI_OK;
+}
+pi_status orc_download_abiotic(orc_model* m, int32_t a, int32_t f, void* h, int64_t n) {
+    if (!h) return fail(PI_ERR_ARG, "null argument");
+    DISPATCH(m, return down_abio_impl(o, a, f, h, n));
+}
+pi_status orc_download_abiotic_diag(orc_model* m, int32_t a, float* h) {
+    if (!h) return fail(PI_ERR_ARG, "null argument");
+    DISPATCH(m, if (a < 0 || a >= o->nsa) return fail(PI_ERR_ARG, "abiotic species out of range");
+             for (int64_t n = 0; n < o->ncell; ++n) h[n] = (float)o->abNum[a][n]; return PI_OK);
+}
 
 template <typename FT, typename OUT>
 static pi_status field_impl(Oracle<FT>* o, int32_t s, int32_t f, OUT* out, int64_t n, bool ints_ok) {
@@ -1359,6 +1516,7 @@ pi_status orc_reset_diags(orc_model* m) {
     DISPATCH(m, std::fill(o->diagPAR.begin(), o->diagPAR.end(), 0);
              for (auto& v : o->diagTr) std::fill(v.begin(), v.end(), 0);
              for (auto& a : o->diagSp) for (auto& v : a) std::fill(v.begin(), v.end(), 0);
+             for (auto& v : o->abNum) std::fill(v.begin(), v.end(), 0);
              return PI_OK);
 }
 pi_status orc_set_diag_tracers(orc_model* m, int32_t n, const int32_t* ids) {

@@ -2,7 +2,7 @@
  * pi_oracle.h - TEST INFRASTRUCTURE ONLY.
  *
  * CPU restatement of the reference PlanktonIndividuals.jl CPU() time step
- * (QuotaMode and CarbonMode, tracer flux boundary conditions, no abiotic particles), used as the checker for libpi_hip.so.
+ * (QuotaMode and CarbonMode, tracer flux boundary conditions, abiotic particles without interaction), used as the checker for libpi_hip.so.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it; the product never does.
  *
@@ -31,6 +31,11 @@ pi_status orc_set_temperature(orc_model* m, const float* temp);
 pi_status orc_step(orc_model* m, float dt);
 pi_status orc_set_tracer_bc(orc_model* m, int32_t tracer, int32_t face, int32_t kind, const float* values,
                             int64_t nt);
+pi_status orc_set_particle_bc(orc_model* m, int32_t sa, int32_t face, int32_t kind, const float* values, int64_t nt);
+pi_status orc_upload_abiotic(orc_model* m, int32_t sa, const float* x, const float* y, const float* z,
+                             const uint8_t* ac);
+pi_status orc_download_abiotic(orc_model* m, int32_t sa, int32_t field, void* host, int64_t n);
+pi_status orc_download_abiotic_diag(orc_model* m, int32_t sa, float* padded);
 /* Downloads convert to float32 in FP64 mode; *_f64 variants give binary64. */
 pi_status orc_download_field(orc_model* m, int32_t sp, int32_t field, void* host, int64_t n);
 pi_status orc_download_field_f64(orc_model* m, int32_t sp, int32_t field, double* host, int64_t n);

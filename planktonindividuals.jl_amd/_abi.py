@@ -26,6 +26,10 @@ CARBON_PARAM_NAMES = ("Nsuper", "Cquota", "mean", "var", "α", "Φ", "Topt", "Tm
                       "respir", "f_T2B", "grz_P", "dvid_type", "dvid_P", "dvid_reg", "dvid_reg2", "mort_P",
                       "mort_reg", "grazFracC", "mortFracC", "thermal", "is_bact")
 MODE_QUOTA, MODE_CARBON = 0, 1
+# abiotic particle parameter order (Abiotic/particle_generation.jl:8)
+ABIOTIC_PARAM_NAMES = ("Nsuper", "Rd", "release_P", "sz_min", "Ktr")
+MAX_ABIOTIC = 4                                  # PI_MAXSA
+ABIO_FIELD_ID = {"x": 0, "y": 1, "z": 2, "xi": 3, "yi": 4, "zi": 5, "ac": 6}
 BGC_NAMES = ("kw", "kc", "kDOC", "Nit", "kDON", "kDOP", "kPOC", "kPON", "kPOP", "κh", "κv",
              "κhP", "κvP", "shared_graz", "w_sink_org", "w_sink_inorg", "lambda_POC",
              "lambda_min", "kdiss", "lambda_dust", "lambda_Fe", "ligand", "DFeFrac")
@@ -68,7 +72,8 @@ class ModelDesc(C.Structure):
     _fields_ = [("grid", GridDesc), ("n_species", C.c_int32), ("capacity", C.c_int64),
                 ("params", fp), ("bgc", fp), ("rng_seed", C.c_uint64),
                 ("n_diag_tracers", C.c_int32), ("diag_tracers", C.POINTER(C.c_int32)),
-                ("mode", C.c_int32), ("metrics", C.POINTER(Metrics))]
+                ("mode", C.c_int32), ("metrics", C.POINTER(Metrics)),
+                ("n_abiotic", C.c_int32), ("abiotic_params", fp), ("n_interactions", C.c_int32)]
 
 
 class IndividualsHost(C.Structure):
@@ -83,7 +88,8 @@ def ptr(a, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
 
 
-def make_desc(grid, n_species, capacity, params, bgc, seed, diag_tracers=(), mode=MODE_QUOTA):
+def make_desc(grid, n_species, capacity, params, bgc, seed, diag_tracers=(), mode=MODE_QUOTA,
+              abiotic_params=None, n_interactions=0):
     """Build a ModelDesc; returns (desc, keepalive) — keep the second alive while desc is used."""
     dzC = np.ascontiguousarray(grid.dzC, dtype=np.float32)
     dzF = np.ascontiguousarray(grid.dzF, dtype=np.float32)
@@ -100,10 +106,14 @@ def make_desc(grid, n_species, capacity, params, bgc, seed, diag_tracers=(), mod
     if hasattr(grid, "metric_arrays"):                 # LatLonGrid
         marrs = grid.metric_arrays()
         mt = Metrics(*(ptr(a, C.c_float) for a in marrs))
+    ap = None
+    if abiotic_params is not None and len(abiotic_params):
+        ap = np.ascontiguousarray(abiotic_params, dtype=np.float32).reshape(-1)
     d = ModelDesc(g, n_species, capacity, ptr(params, C.c_float), ptr(bgc, C.c_float),
                   C.c_uint64(seed), len(dt), ptr(dt, C.c_int32) if len(dt) else None, int(mode),
-                  C.pointer(mt) if mt is not None else None)
-    return d, (dzC, dzF, lm, params, bgc, dt, marrs, mt)
+                  C.pointer(mt) if mt is not None else None,
+                  0 if ap is None else len(ap) // len(ABIOTIC_PARAM_NAMES), ptr(ap, C.c_float), int(n_interactions))
+    return d, (dzC, dzF, lm, params, bgc, dt, marrs, mt, ap)
 
 
 def individuals_view(arrays):
@@ -142,6 +152,10 @@ def declare(lib, prefix, model_t=C.c_void_p):
         "count_active": (C.c_int, [M, C.c_int32, i64p]),
         "individual_steps": (C.c_int, [M, i64p]),
         "set_tracer_bc": (C.c_int, [M, C.c_int32, C.c_int32, C.c_int32, fp, C.c_int64]),
+        "set_particle_bc": (C.c_int, [M, C.c_int32, C.c_int32, C.c_int32, fp, C.c_int64]),
+        "upload_abiotic": (C.c_int, [M, C.c_int32, fp, fp, fp, u8p]),
+        "download_abiotic": (C.c_int, [M, C.c_int32, C.c_int32, C.c_void_p, C.c_int64]),
+        "download_abiotic_diag": (C.c_int, [M, C.c_int32, fp]),
     }
     for name, (res, args) in proto.items():
         f = getattr(lib, prefix + name)

@@ -165,14 +165,17 @@ class Handle:
         """Flux boundary condition of tracer `tid` on face `pos` (set_bc!,
         boundary_conditions.jl:12-29): None, a number, a face array (2-D) or a
         face array per iteration (3-D), converted to Float32 like FT.(bc_value)."""
+        self._set_bc("set_tracer_bc", tid, pos, bc_value)
+
+    def _set_bc(self, fn, who, pos, bc_value):
         f = _abi.BC_POS[pos]
         if bc_value is None:
-            self._call("set_tracer_bc", tid, f, _abi.BCK_NONE, None, C.c_int64(0))
+            self._call(fn, who, f, _abi.BCK_NONE, None, C.c_int64(0))
             return
         a = np.asarray(bc_value, dtype=np.float32)
         if a.ndim == 0:
             v = np.ascontiguousarray(a.reshape(1))
-            self._call("set_tracer_bc", tid, f, _abi.BCK_CONST, v.ctypes.data_as(_abi.fp), C.c_int64(0))
+            self._call(fn, who, f, _abi.BCK_CONST, v.ctypes.data_as(_abi.fp), C.c_int64(0))
             return
         g = self.grid
         face = (g.Ny, g.Nz) if pos in ("west", "east") else (g.Nx, g.Nz) if pos in ("south", "north") else (g.Nx, g.Ny)
@@ -181,7 +184,31 @@ class Handle:
                              f"condition or {face + ('nΔT',)} for a time-dependent one.")
         v = np.asfortranarray(a)
         kind, nt = (_abi.BCK_FIELD, 0) if a.ndim == 2 else (_abi.BCK_SERIES, a.shape[2])
-        self._call("set_tracer_bc", tid, f, kind, v.ctypes.data_as(_abi.fp), C.c_int64(nt))
+        self._call(fn, who, f, kind, v.ctypes.data_as(_abi.fp), C.c_int64(nt))
+
+    # ---- abiotic particles
+    def upload_abiotic(self, sa, arrays):
+        x, y, z = (np.ascontiguousarray(arrays[n], dtype=np.float32) for n in ("x", "y", "z"))
+        ac = np.ascontiguousarray(arrays["ac"], dtype=np.uint8)
+        assert x.shape == y.shape == z.shape == ac.shape == (self.cap,)
+        self._call("upload_abiotic", sa, x.ctypes.data_as(_abi.fp), y.ctypes.data_as(_abi.fp),
+                   z.ctypes.data_as(_abi.fp), ac.ctypes.data_as(_abi.u8p))
+
+    def abiotic_field(self, sa, name):
+        fid = _abi.ABIO_FIELD_ID[name]
+        out = np.empty(self.cap, np.float32 if fid <= 2 else np.int64 if fid <= 5 else np.uint8)
+        self._call("download_abiotic", sa, fid, out.ctypes.data_as(C.c_void_p), self.cap)
+        return out.astype(bool) if name == "ac" else out
+
+    def abiotic_diag(self, sa):
+        out = np.empty(self.grid.padded_shape, np.float32, order="F")
+        self._call("download_abiotic_diag", sa, out.ctypes.data_as(_abi.fp))
+        return out
+
+    def set_particle_bc(self, sa, pos, bc_value):
+        """Particle flux boundary condition of abiotic species `sa` on face
+        `pos` (set_bc_particle!, boundary_conditions.jl:31-39)."""
+        self._set_bc("set_particle_bc", sa, pos, bc_value)
 
     # ---- downloads
     def field(self, sp, name):
@@ -243,11 +270,11 @@ class Handle:
         return v.value
 
 
-def footprint(grid, nsp, cap, params, bgc, seed=0, diag_tracers=(), mode=_abi.MODE_QUOTA):
+def footprint(grid, nsp, cap, params, bgc, seed=0, diag_tracers=(), mode=_abi.MODE_QUOTA, abiotic_params=None):
     """Device bytes a model of this shape allocates (pi_footprint: a dry run of
     pi_model_create, no device needed)."""
     lib = load_library()
-    desc, keep = _abi.make_desc(grid, nsp, cap, params, bgc, seed, diag_tracers, mode)
+    desc, keep = _abi.make_desc(grid, nsp, cap, params, bgc, seed, diag_tracers, mode, abiotic_params)
     v = C.c_int64(0)
     check(lib, "pi_", lib.pi_footprint(C.byref(desc), C.byref(v)))
     del keep
@@ -257,10 +284,11 @@ def footprint(grid, nsp, cap, params, bgc, seed=0, diag_tracers=(), mode=_abi.MO
 class NativeModel(Handle):
     """A model living on the GPU (libpi_hip.so)."""
 
-    def __init__(self, grid, nsp, cap, params, bgc, seed, diag_tracers=(), device=0, mode=_abi.MODE_QUOTA):
+    def __init__(self, grid, nsp, cap, params, bgc, seed, diag_tracers=(), device=0, mode=_abi.MODE_QUOTA,
+                 abiotic_params=None):
         lib = load_library()
         check(lib, "pi_", lib.pi_init(device))
-        desc, keep = _abi.make_desc(grid, nsp, cap, params, bgc, seed, diag_tracers, mode)
+        desc, keep = _abi.make_desc(grid, nsp, cap, params, bgc, seed, diag_tracers, mode, abiotic_params)
         raw = C.c_void_p()
         check(lib, "pi_", lib.pi_model_create(C.byref(desc), C.byref(raw)))
         del keep

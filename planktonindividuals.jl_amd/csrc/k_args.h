@@ -122,6 +122,34 @@ struct BcTab {
     const float* arr[PI_NTRACERS][PI_NBC];
 };
 
+/* Abiotic particles (Individuals/Abiotic): per species sa, `cap` slots at
+ * sa·cap.  The particle flux boundary conditions (set_bc_particle!) per species
+ * and face, in device memory like BcTab. */
+struct AbioBc {
+    int32_t kind[PI_MAXSA][PI_NBC];
+    float val[PI_MAXSA][PI_NBC];
+    const float* arr[PI_MAXSA][PI_NBC];
+};
+/* error species code of a failed abiotic capacity check: PI_ERR_SP_ABIO + sa */
+#define PI_ERR_SP_ABIO 1000u
+struct AbioArgs {
+    AdvArgs adv;                      /* grid, state, velocities, Δt, walk scales, stream seed */
+    int nsa;
+    long long cap, ncell;
+    float *x, *y, *z;                 /* [nsa][cap] */
+    int32_t *xi, *yi, *zi;            /* find_inds! of the previous step (0 before the first) */
+    uint8_t* ac;
+    uint32_t* cflag;                  /* [nsa][ncell]: 1 = a particle is created at this cell */
+    const uint32_t* cpre;             /* its exclusive prefix, total at [nsa·ncell] */
+    uint32_t* fflag;                  /* [nsa][cap]: 1 = free slot */
+    const uint32_t* fpre;
+    uint32_t* flist;                  /* [nsa][cap]: the free slots of each species in ascending order */
+    float* num;                       /* [nsa][ncell]: diags.abiotics.saN.num */
+    const AbioBc* bc;
+    uint32_t bcm;                     /* bit 6·sa + f: species sa has a condition on face f */
+    float p[PI_MAXSA][PI_NAPARAMS];
+};
+
 struct ScanIn {
     const uint32_t* a;
     uint32_t n;               /* length (upper bound when dn is set) */
@@ -192,7 +220,8 @@ enum KId {
     K_PLACE_SMALL, K_MIXSORT, K_BIGCELLS, K_CELLSUM, K_CELLSUM_LONG, K_PAR, K_PHYS, K_PLK, K_PLK_LONG,
     K_SWEEPX0, K_SWEEPX1, K_XY0, K_XY1, K_SWEEPY0, K_SWEEPY1, K_HALOCOPY, K_FILL, K_FINAL0, K_FINAL1, K_ACCUM,
     K_FINISH, K_XYW0, K_XYW1, K_PHYSC, K_FINAL2, K_FINAL3,
-    K_ADV_GM, K_SWEEPX2, K_SWEEPX3, K_SWEEPY2, K_SWEEPY3, K_FINAL4, K_FINAL5, K_FINAL6, K_FINAL7, K_NKID
+    K_ADV_GM, K_SWEEPX2, K_SWEEPX3, K_SWEEPY2, K_SWEEPY3, K_FINAL4, K_FINAL5, K_FINAL6, K_FINAL7,
+    K_ABIO_MARK, K_ABIO_FREE, K_ABIO_LIST, K_ABIO_CHECK, K_ABIO_SPAWN, K_ABIO_ADV, K_ABIO_ADV_GM, K_NKID
 };
 /* plan markers (not kernels): fork to / join from the side stream */
 enum { K_FORK = K_NKID, K_JOIN = K_NKID + 1 };

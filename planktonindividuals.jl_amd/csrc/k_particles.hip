@@ -1556,9 +1556,172 @@ __device__ __forceinline__ void k_plk_long_body(const PlkArgs& a, Blk K)
 }
 PI_KERNEL(k_plk_long, PlkArgs, 256)
 
+/* ------------------------------------------------------ abiotic particles
+ * Individuals/Abiotic/particle_interaction.jl:158-267 (particles_from_bcs!)
+ * and the abiotic part of TimeStep! (Model/time_step.jl:21-38, 152-159). */
+
+/* the area of face f of cell (i, j, k) (1-based padded): Az top/bottom, Ax
+ * east/west, Ay north/south (Grids/areas_volumes_spacings.jl) */
+__device__ __forceinline__ float face_area(const DevGrid& g, int f, int i, int j, int k, long long c)
+{
+    if (f == PI_BC_TOP || f == PI_BC_BOTTOM) return g.gm ? g.mAz[col2(g, i, j)] : g.dx * g.dy;
+    if (f == PI_BC_EAST || f == PI_BC_WEST) return g.gm ? g.mAx[c] : g.dy * dzF_at(g, k);
+    return g.gm ? g.mAy[c] : g.dx * dzF_at(g, k);
+}
+
+/* calc_particle_bcs! (:158-236): the cells next to each face with a
+ * condition get isless(u·sz_min·Nsuper, bc·ΔT·A); the faces are evaluated in
+ * the order top, bottom, east, west, north, south, so at an edge or a corner
+ * the last of them decides.  u is the cell's draw of rand!(rnd_3d). */
+__device__ __forceinline__ void k_abio_mark_body(const AbioArgs& a, Blk K)
+{
+    const DevState* st = a.adv.st;
+    if (st->err) return;
+    const DevGrid& g = a.adv.g;
+    const long long n = (long long)a.nsa * a.ncell;
+    const int it = (int)(st->iteration + 1);
+    const uint32_t sxy = (uint32_t)g.Px * (uint32_t)g.Py;
+    for (long long t = K.b * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)K.n * blockDim.x) {
+        const int sa = (int)(t / a.ncell);
+        const long long c = t - (long long)sa * a.ncell;
+        const uint32_t m = (a.bcm >> (6 * sa)) & 63u;
+        uint32_t flag = 0;
+        if (m) {
+            const int i = (int)(c % g.Px) + 1, j = (int)((c / g.Px) % g.Py) + 1, k = (int)(c / sxy) + 1;
+            const int ii = i - g.Hx, jj = j - g.Hy, kk = k - g.Hz;          /* interior, 1-based */
+            if (ii >= 1 && ii <= g.Nx && jj >= 1 && jj <= g.Ny && kk >= 1 && kk <= g.Nz) {
+                static constexpr int order[6] = {PI_BC_TOP, PI_BC_BOTTOM, PI_BC_EAST, PI_BC_WEST, PI_BC_NORTH, PI_BC_SOUTH};
+                float u, u1, u2;
+                pi_uniform3(a.adv.seed, it, PI_ABIO_SPECIES + sa, PI_KIND_ABIO_BC, c, &u, &u1, &u2);
+                const float lhs = (u * a.p[sa][PI_A_sz_min]) * a.p[sa][PI_A_Nsuper];
+#pragma unroll
+                for (int o = 0; o < 6; ++o) {
+                    const int f = order[o];
+                    if (!((m >> f) & 1u)) continue;
+                    const bool on = f == PI_BC_TOP ? kk == 1 : f == PI_BC_BOTTOM ? kk == g.Nz : f == PI_BC_EAST ? ii == g.Nx
+                                  : f == PI_BC_WEST ? ii == 1 : f == PI_BC_NORTH ? jj == g.Ny : jj == 1;
+                    if (!on) continue;
+                    /* getbc at the face position: (i, j) top/bottom, (j, k) east/west, (i, k) north/south */
+                    const int p0 = f <= PI_BC_EAST ? jj : ii, p1 = f <= PI_BC_NORTH ? kk : jj;
+                    const int n0 = f <= PI_BC_EAST ? g.Ny : g.Nx, n1 = f <= PI_BC_NORTH ? g.Nz : g.Ny;
+                    const int kind = a.bc->kind[sa][f];
+                    float b;
+                    if (kind == PI_BCK_CONST) b = a.bc->val[sa][f];
+                    else {
+                        const long long tt = kind == PI_BCK_SERIES ? (long long)(it - 1) : 0ll;
+                        b = a.bc->arr[sa][f][(long long)(p0 - 1) + (long long)n0 * ((long long)(p1 - 1) + (long long)n1 * tt)];
+                    }
+                    flag = jl_islessf(lhs, (b * a.adv.dt) * face_area(g, f, i, j, k, c)) ? 1u : 0u;
+                }
+            }
+        }
+        a.cflag[t] = flag;
+    }
+}
+PI_KERNEL(k_abio_mark, AbioArgs, 256)
+
+/* findall(isequal(false), abiotic.ac): the free slots, flagged */
+__device__ __forceinline__ void k_abio_free_body(const AbioArgs& a, Blk K)
+{
+    if (a.adv.st->err) return;
+    const long long n = (long long)a.nsa * a.cap;
+    for (long long t = K.b * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)K.n * blockDim.x)
+        a.fflag[t] = a.ac[t] ? 0u : 1u;
+}
+PI_KERNEL(k_abio_free, AbioArgs, 256)
+
+/* the free slots of each species in ascending order (deactive_ind) */
+__device__ __forceinline__ void k_abio_list_body(const AbioArgs& a, Blk K)
+{
+    if (a.adv.st->err) return;
+    const long long n = (long long)a.nsa * a.cap;
+    for (long long t = K.b * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)K.n * blockDim.x) {
+        if (a.ac[t]) continue;
+        const long long sa = t / a.cap;
+        a.flist[sa * a.cap + (a.fpre[t] - a.fpre[sa * a.cap])] = (uint32_t)(t - sa * a.cap);
+    }
+}
+PI_KERNEL(k_abio_list, AbioArgs, 256)
+
+/* length(inds) > length(deactive_ind) throws (:256-259): checked before any
+ * buffer of the step changes */
+__device__ __forceinline__ void k_abio_check_body(const AbioArgs& a, Blk K)
+{
+    DevState* st = a.adv.st;
+    if (threadIdx.x != 0 || st->err) return;
+    for (int sa = 0; sa < a.nsa; ++sa) {
+        const uint32_t make = a.cpre[(long long)(sa + 1) * a.ncell] - a.cpre[(long long)sa * a.ncell];
+        const uint32_t free = a.fpre[(long long)(sa + 1) * a.cap] - a.fpre[(long long)sa * a.cap];
+        if (make > free) { st->err = PI_ERR_CAPACITY; st->err_sp = PI_ERR_SP_ABIO + (uint32_t)sa; return; }
+    }
+}
+PI_KERNEL(k_abio_check, AbioArgs, 64)
+
+/* copy_abiotic_particle_from_field! (:238-247): the k-th flagged cell in
+ * column-major order fills the k-th free slot, at the cell's corner + 0.6 */
+__device__ __forceinline__ void k_abio_spawn_body(const AbioArgs& a, Blk K)
+{
+    if (a.adv.st->err) return;
+    const DevGrid& g = a.adv.g;
+    const long long n = (long long)a.nsa * a.ncell;
+    const uint32_t sxy = (uint32_t)g.Px * (uint32_t)g.Py;
+    for (long long t = K.b * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)K.n * blockDim.x) {
+        if (!a.cflag[t]) continue;
+        const long long sa = t / a.ncell, c = t - sa * a.ncell;
+        const uint32_t kth = a.cpre[t] - a.cpre[sa * a.ncell];
+        const long long r = sa * a.cap + a.flist[sa * a.cap + kth];
+        const int i = (int)(c % g.Px) + 1, j = (int)((c / g.Px) % g.Py) + 1, k = (int)(c / sxy) + 1;
+        a.ac[r] = 1;
+        a.x[r] = (float)(i - g.Hx) - 0.4f;
+        a.y[r] = (float)(j - g.Hy) - 0.4f;
+        a.z[r] = (float)(k - g.Hz) - 0.4f;
+    }
+}
+PI_KERNEL(k_abio_spawn, AbioArgs, 256)
+
+/* particle_advection!, particle_diffusion! and find_inds! of every abiotic
+ * slot (time_step.jl:27-36), then diags_proc! of num (:152-159); inactive
+ * slots keep ±0 positions and the halo cell (Hx, Hy, Hz) */
+template <bool GM>
+__device__ __forceinline__ void k_abio_adv_impl(const AbioArgs& a, Blk K)
+{
+    const DevState* st = a.adv.st;
+    if (st->err) return;
+    const DevGrid& g = a.adv.g;
+    const long long n = (long long)a.nsa * a.cap;
+    const long long it = st->iteration + 1;
+    for (long long t = K.b * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)K.n * blockDim.x) {
+        const long long sa = t / a.cap, slot = t - sa * a.cap;
+        if (!a.ac[t]) {
+            a.x[t] = copysignf(0.0f, a.x[t]); a.y[t] = copysignf(0.0f, a.y[t]); a.z[t] = copysignf(0.0f, a.z[t]);
+            a.xi[t] = g.Hx; a.yi[t] = g.Hy; a.zi[t] = g.Hz;
+            continue;
+        }
+        float r0, r1, r2, x, y, z;
+        pi_uniform3(a.adv.seed, it, PI_ABIO_SPECIES + (int)sa, PI_KIND_DIFFUSION, slot, &r0, &r1, &r2);
+        move<GM>(a.adv, g, a.x[t], a.y[t], a.z[t], r0, r1, r2, a.xi[t], a.yi[t], a.zi[t], x, y, z);
+        int xi = (int)(x + 1.0f) + g.Hx, yi = (int)(y + 1.0f) + g.Hy, zi = (int)(z + 1.0f) + g.Hz;
+        a.x[t] = x; a.y[t] = y; a.z[t] = z;
+        a.xi[t] = xi; a.yi[t] = yi; a.zi[t] = zi;
+        xi = min(max(xi, 1), g.Px); yi = min(max(yi, 1), g.Py); zi = min(max(zi, 1), g.Pz);
+        atomicAdd(&a.num[sa * a.ncell + gidx(g, xi, yi, zi)], 1.0f);
+    }
+}
+__device__ __forceinline__ void k_abio_adv_body(const AbioArgs& a, Blk K) { k_abio_adv_impl<false>(a, K); }
+__device__ __forceinline__ void k_abio_adv_gm_body(const AbioArgs& a, Blk K) { k_abio_adv_impl<true>(a, K); }
+PI_KERNEL(k_abio_adv, AbioArgs, 256)
+PI_KERNEL(k_abio_adv_gm, AbioArgs, 256)
+
 /* ------------------------------------------------------------- registry */
 void reg_particles(KInfo* k)
 {
+    PI_REG(k, K_ABIO_MARK, k_abio_mark, AbioArgs, 256, true);
+    PI_REG(k, K_ABIO_FREE, k_abio_free, AbioArgs, 256, true);
+    PI_REG(k, K_ABIO_LIST, k_abio_list, AbioArgs, 256, true);
+    PI_REG(k, K_ABIO_CHECK, k_abio_check, AbioArgs, 64, false);
+    PI_REG(k, K_ABIO_SPAWN, k_abio_spawn, AbioArgs, 256, true);
+    PI_REG(k, K_ABIO_ADV, k_abio_adv, AbioArgs, 256, true);
+    PI_REG(k, K_ABIO_ADV_GM, k_abio_adv_gm, AbioArgs, 256, true);
     PI_REG(k, K_ADV, k_adv, AdvArgs, 256, true);
     PI_REG(k, K_ADV_GM, k_adv_gm, AdvArgs, 256, true);
     PI_REG(k, K_MARKS, k_marks, MarksArgs, 256, true);

@@ -144,6 +144,20 @@ struct pi_model {
     BcTab* d_bc = nullptr;
     float* bc_buf[PI_NTRACERS][PI_NBC]{};
     long long bc_nt[PI_NTRACERS][PI_NBC]{};
+    /* abiotic particles: nsa species of cap slots, their state, the scratch of
+     * particles_from_bcs! (cell flags, free slots, prefixes), num, and their
+     * flux boundary conditions */
+    int nsa = 0;
+    float aparams[PI_MAXSA][PI_NAPARAMS]{};
+    float *ax = nullptr, *ay = nullptr, *az = nullptr, *anum = nullptr;
+    int32_t *axi = nullptr, *ayi = nullptr, *azi = nullptr;
+    uint8_t* aac = nullptr;
+    uint32_t *acflag = nullptr, *acpre = nullptr, *afflag = nullptr, *afpre = nullptr, *aflist = nullptr,
+             *apart = nullptr;
+    AbioBc h_abc{};
+    AbioBc* d_abc = nullptr;
+    float* abc_buf[PI_MAXSA][PI_NBC]{};
+    long long abc_nt[PI_MAXSA][PI_NBC]{};
 };
 
 static hipError_t dalloc(pi_model* m, void** p, size_t bytes)
@@ -196,9 +210,13 @@ static pi_status sync_state(pi_model* m)
     if (m->h_state.err == PI_ERR_HIP) return fail(PI_ERR_HIP, "device-side consistency check failed in division");
     if (m->h_state.err == PI_ERR_CAPACITY) {
         char buf[200];
-        snprintf(buf, sizeof buf,
-                 "number of individual exceeds the capacity at timestep %g days (species %u)",
-                 (double)(m->h_state.iteration + 1) * (double)m->last_dt / 86400.0, m->h_state.err_sp + 1);
+        const double day = (double)(m->h_state.iteration + 1) * (double)m->last_dt / 86400.0;
+        if (m->h_state.err_sp >= PI_ERR_SP_ABIO)      /* particles_from_bcs! (particle_interaction.jl:256-259) */
+            snprintf(buf, sizeof buf, "number of abiotic particles exceeds the capacity at timestep %g days "
+                     "(abiotic species %u)", day, m->h_state.err_sp - PI_ERR_SP_ABIO + 1);
+        else
+            snprintf(buf, sizeof buf, "number of individual exceeds the capacity at timestep %g days (species %u)",
+                     day, m->h_state.err_sp + 1);
         return fail(PI_ERR_CAPACITY, buf);
     }
     return PI_OK;
@@ -330,6 +348,20 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     DALLOC(m->d_params, m->h_params.size() * 4);
     DEVCALL(hipMemcpy(m->d_params, m->h_params.data(), m->h_params.size() * 4, hipMemcpyHostToDevice));
     DALLOC(m->d_bc, sizeof(BcTab));    /* zeroed: no flux boundary conditions */
+    m->nsa = d->n_abiotic;
+    if (m->nsa > 0) {                  /* abiotic particles: state and scratch, all zeroed */
+        const long long ns = (long long)m->nsa * m->cap, nc = (long long)m->nsa * g.ncell;
+        for (int sa = 0; sa < m->nsa; ++sa)
+            for (int q = 0; q < PI_NAPARAMS; ++q) m->aparams[sa][q] = d->abiotic_params[sa * PI_NAPARAMS + q];
+        DALLOC(m->ax, ns * 4); DALLOC(m->ay, ns * 4); DALLOC(m->az, ns * 4);
+        DALLOC(m->axi, ns * 4); DALLOC(m->ayi, ns * 4); DALLOC(m->azi, ns * 4);
+        DALLOC(m->aac, ns);
+        DALLOC(m->acflag, nc * 4); DALLOC(m->acpre, (nc + 1) * 4);
+        DALLOC(m->afflag, ns * 4); DALLOC(m->afpre, (ns + 1) * 4); DALLOC(m->aflist, ns * 4);
+        DALLOC(m->apart, (long long)scan_tiles((uint32_t)std::max(nc, ns)) * 4 + 64);
+        DALLOC(m->anum, nc * 4);
+        DALLOC(m->d_abc, sizeof(AbioBc));
+    }
     /* store x2 */
     DevState& hs = m->h_state;
     memset(&hs, 0, sizeof hs);
@@ -446,6 +478,16 @@ static pi_status check_desc(const pi_model_desc* d)
         if (!mt.dxC || !mt.dyC || !mt.dxF || !mt.dyF || !mt.Az || !mt.Ax || !mt.Ay || !mt.Vol)
             return fail(PI_ERR_ARG, "grid metrics: every array is required");
     }
+    if (d->n_abiotic < 0 || d->n_abiotic > PI_MAXSA) return fail(PI_ERR_ARG, "n_abiotic out of range [0, 4]");
+    if (d->n_abiotic > 0 && !d->abiotic_params) return fail(PI_ERR_ARG, "abiotic particle parameters missing");
+    if (d->n_interactions != 0)
+        return fail(PI_ERR_UNSUPPORTED, "phytoplankton-particle interaction (palat.intac) needs IronEnergyMode's "
+                                        "max_ptc; IronEnergyMode is not implemented");
+    {
+        const long long nc = (long long)(gd.Nx + 2 * gd.Hx) * (gd.Ny + 2 * gd.Hy) * (gd.Nz + 2 * gd.Hz);
+        if ((long long)d->n_abiotic * nc >= (1ll << 31) || (long long)d->n_abiotic * d->capacity >= (1ll << 31))
+            return fail(PI_ERR_ARG, "abiotic particles: species x cells and species x capacity must stay below 2^31");
+    }
     const int dvt = d->mode == PI_MODE_CARBON ? (int)PI_C_dvid_type : (int)PI_P_dvid_type;
     for (int s = 0; s < d->n_species; ++s) {
         float t = d->params[s * PI_NPARAMS + dvt];
@@ -509,6 +551,7 @@ pi_status pi_model_destroy(pi_model* m)
     for (auto& t : m->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (void* p : m->allocs) (void)hipFree(p);
     for (auto& row : m->bc_buf) for (float* p : row) if (p) (void)hipFree(p);
+    for (auto& row : m->abc_buf) for (float* p : row) if (p) (void)hipFree(p);
     (void)hipStreamDestroy(m->stream);
     (void)hipStreamDestroy(m->side);
     (void)hipEventDestroy(m->ev_fork);
@@ -752,6 +795,95 @@ static pi_status bc_steps_ok(const pi_model* m, int64_t n)
             if (m->bc_nt[q][f] && m->host_iter + n > m->bc_nt[q][f])
                 return fail(PI_ERR_ARG, "BC: the time-dependent flux boundary condition of a tracer has no value "
                                         "for this iteration (size(bc, 3) < iteration)");
+    for (int sa = 0; sa < m->nsa; ++sa)
+        for (int f = 0; f < PI_NBC; ++f)
+            if (m->abc_nt[sa][f] && m->host_iter + n > m->abc_nt[sa][f])
+                return fail(PI_ERR_ARG, "BC: the time-dependent flux boundary condition of an abiotic particle "
+                                        "has no value for this iteration (size(bc, 3) < iteration)");
+    return PI_OK;
+}
+
+pi_status pi_set_particle_bc(pi_model* m, int32_t sa, int32_t f, int32_t kind, const float* values, int64_t nt)
+{
+    if (!m) return fail(PI_ERR_ARG, "null model");
+    if (sa < 0 || sa >= m->nsa) return fail(PI_ERR_ARG, "abiotic species out of range");
+    if (f < 0 || f >= PI_NBC) return fail(PI_ERR_ARG, "boundary position");
+    if (kind < PI_BCK_NONE || kind > PI_BCK_SERIES) return fail(PI_ERR_ARG, "boundary condition kind");
+    if (kind != PI_BCK_NONE && !values) return fail(PI_ERR_ARG, "null boundary condition values");
+    if (kind == PI_BCK_SERIES && nt < 1)
+        return fail(PI_ERR_ARG, "BC: a time-dependent flux boundary condition needs nt >= 1");
+    const DevGrid& g = m->g;
+    const long long n1 = f <= PI_BC_EAST ? g.Ny : g.Nx, n2 = f <= PI_BC_NORTH ? g.Nz : g.Ny;
+    const long long n = kind == PI_BCK_FIELD ? n1 * n2 : kind == PI_BCK_SERIES ? n1 * n2 * nt : 0;
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    if (m->abc_buf[sa][f]) {
+        HIPCHECK(hipFree(m->abc_buf[sa][f]));
+        m->abc_buf[sa][f] = nullptr;
+    }
+    m->h_abc.kind[sa][f] = kind;
+    m->h_abc.val[sa][f] = kind == PI_BCK_CONST ? values[0] : 0.0f;
+    m->h_abc.arr[sa][f] = nullptr;
+    m->abc_nt[sa][f] = kind == PI_BCK_SERIES ? nt : 0;
+    if (n) {
+        float* d = nullptr;
+        HIPCHECK(hipMalloc((void**)&d, (size_t)n * 4));
+        m->abc_buf[sa][f] = d;
+        HIPCHECK(hipMemcpy(d, values, (size_t)n * 4, hipMemcpyHostToDevice));
+        m->h_abc.arr[sa][f] = d;
+    }
+    HIPCHECK(hipMemcpy(m->d_abc, &m->h_abc, sizeof(AbioBc), hipMemcpyHostToDevice));
+    drop_graph(m);
+    return PI_OK;
+}
+
+pi_status pi_upload_abiotic(pi_model* m, int32_t sa, const float* x, const float* y, const float* z, const uint8_t* ac)
+{
+    if (!m || !x || !y || !z || !ac) return fail(PI_ERR_ARG, "null argument");
+    if (sa < 0 || sa >= m->nsa) return fail(PI_ERR_ARG, "abiotic species out of range");
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    const size_t o = (size_t)sa * m->cap, n = (size_t)m->cap;
+    HIPCHECK(hipMemcpy(m->ax + o, x, n * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(m->ay + o, y, n * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(m->az + o, z, n * 4, hipMemcpyHostToDevice));
+    std::vector<uint8_t> a(n);
+    for (size_t i = 0; i < n; ++i) a[i] = ac[i] ? 1 : 0;
+    HIPCHECK(hipMemcpy(m->aac + o, a.data(), n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemset(m->axi + o, 0, n * 4));       /* xi, yi, zi start at 0 */
+    HIPCHECK(hipMemset(m->ayi + o, 0, n * 4));
+    HIPCHECK(hipMemset(m->azi + o, 0, n * 4));
+    return PI_OK;
+}
+
+pi_status pi_download_abiotic(pi_model* m, int32_t sa, int32_t field, void* host, int64_t n)
+{
+    if (!m || !host) return fail(PI_ERR_ARG, "null argument");
+    if (sa < 0 || sa >= m->nsa) return fail(PI_ERR_ARG, "abiotic species out of range");
+    if (field < PI_AB_x || field > PI_AB_ac) return fail(PI_ERR_ARG, "abiotic field id");
+    if (n != m->cap) return fail(PI_ERR_ARG, "n must equal capacity");
+    pi_status s = sync_state(m);
+    if (s != PI_OK && s != PI_ERR_CAPACITY) return s;
+    const size_t o = (size_t)sa * m->cap;
+    if (field <= PI_AB_z) {
+        const float* src = field == PI_AB_x ? m->ax : field == PI_AB_y ? m->ay : m->az;
+        HIPCHECK(hipMemcpy(host, src + o, (size_t)n * 4, hipMemcpyDeviceToHost));
+    } else if (field <= PI_AB_zi) {
+        const int32_t* src = field == PI_AB_xi ? m->axi : field == PI_AB_yi ? m->ayi : m->azi;
+        std::vector<int32_t> t((size_t)n);
+        HIPCHECK(hipMemcpy(t.data(), src + o, (size_t)n * 4, hipMemcpyDeviceToHost));
+        int64_t* out = (int64_t*)host;
+        for (int64_t i = 0; i < n; ++i) out[i] = t[(size_t)i];
+    } else {
+        HIPCHECK(hipMemcpy(host, m->aac + o, (size_t)n, hipMemcpyDeviceToHost));
+    }
+    return PI_OK;
+}
+
+pi_status pi_download_abiotic_diag(pi_model* m, int32_t sa, float* h)
+{
+    if (!m || !h) return fail(PI_ERR_ARG, "null argument");
+    if (sa < 0 || sa >= m->nsa) return fail(PI_ERR_ARG, "abiotic species out of range");
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    HIPCHECK(hipMemcpy(h, m->anum + (size_t)sa * m->ncell, (size_t)m->ncell * 4, hipMemcpyDeviceToHost));
     return PI_OK;
 }
 
@@ -763,6 +895,26 @@ static void plan_scan(Plan& P, const char* grp, const ScanIn& in, uint32_t* part
     a.in = in; a.partials = partials; a.out = out; a.nb = scan_tiles(in.n);
     P.add(K_SCAN_P, grp, a.nb, a);
     P.add(K_SCAN_F, grp, a.nb, a);
+}
+
+static AbioArgs abio_args(const pi_model* m, float dt, const Vel& V0, const Vel& VH, const Vel& V1)
+{
+    AbioArgs a = zeroed<AbioArgs>();
+    a.adv.g = m->g; a.adv.st = m->d_state; a.adv.v0 = V0; a.adv.vh = VH; a.adv.v1 = V1; a.adv.dt = dt;
+    a.adv.sx = std::sqrt(m->bgc[PI_B_kappa_hP] * dt);
+    a.adv.sy = std::sqrt(m->bgc[PI_B_kappa_hP] * dt);
+    a.adv.sz = std::sqrt(m->bgc[PI_B_kappa_vP] * dt);
+    a.adv.seed = m->seed;
+    a.nsa = m->nsa; a.cap = m->cap; a.ncell = m->ncell;
+    a.x = m->ax; a.y = m->ay; a.z = m->az; a.xi = m->axi; a.yi = m->ayi; a.zi = m->azi; a.ac = m->aac;
+    a.cflag = m->acflag; a.cpre = m->acpre; a.fflag = m->afflag; a.fpre = m->afpre; a.flist = m->aflist;
+    a.num = m->anum; a.bc = m->d_abc;
+    for (int sa = 0; sa < m->nsa; ++sa) {
+        for (int f = 0; f < PI_NBC; ++f)
+            if (m->h_abc.kind[sa][f] != PI_BCK_NONE) a.bcm |= 1u << (6 * sa + f);
+        for (int q = 0; q < PI_NAPARAMS; ++q) a.p[sa][q] = m->aparams[sa][q];
+    }
+    return a;
 }
 
 /* Builds the plan of the model's next step and advances the host-side
@@ -798,9 +950,25 @@ static void build_step(pi_model* m, float dt, Plan& P)
         a.nsp = m->nsp; a.nw = m->nw; a.cap = m->cap;
         P.add(K_MARKS, "marks+scan", grid_for((long long)m->nsp * m->nw, 256, 8192), a);
         plan_scan(P, "marks+scan", scan_bits(m->bm + (size_t)m->nsp * m->nw, m->bm, m->nsp, m->nw, m->cap), m->partials, m->G);
+        /* abiotic particles_from_bcs!: the cells that create a particle, the free
+         * slots, and their capacity check, before anything changes */
+        AbioArgs ab = abio_args(m, dt, V0, VH, V1);
+        const long long nc = (long long)m->nsa * m->ncell, ns = (long long)m->nsa * m->cap;
+        const bool pbc = ab.bcm != 0;
+        P.add(K_ABIO_MARK, "abiotic", pbc ? grid_for(nc, 256, 8192) : 0, ab);
+        P.add(K_ABIO_FREE, "abiotic", pbc ? grid_for(ns, 256, 8192) : 0, ab);
+        plan_scan(P, "abiotic", scan_plain(m->acflag, (uint32_t)(pbc ? nc : 0)), m->apart, m->acpre);
+        plan_scan(P, "abiotic", scan_plain(m->afflag, (uint32_t)(pbc ? ns : 0)), m->apart, m->afpre);
+        P.add(K_ABIO_LIST, "abiotic", pbc ? grid_for(ns, 256, 8192) : 0, ab);
+        P.add(K_ABIO_CHECK, "abiotic", pbc ? 1 : 0, ab);
         CapArgs c = zeroed<CapArgs>();
         c.st = st; c.G = m->G; c.nsp = m->nsp; c.nw = m->nw;
         P.add(K_CAPCHECK, "marks+scan", 1, c);
+        /* abiotic particles: creation at the faces, then advection, the random
+         * walk, find_inds! and num (time_step.jl:21-38, 152-159) */
+        P.add(K_ABIO_SPAWN, "abiotic", pbc ? grid_for(nc, 256, 8192) : 0, ab);
+        P.add(K_ABIO_ADV, "abiotic", m->nsa && !g.gm ? grid_for(ns, 256, 8192) : 0, ab);
+        P.add(K_ABIO_ADV_GM, "abiotic", m->nsa && g.gm ? grid_for(ns, 256, 8192) : 0, ab);
     }
     const bool land = g.has_land != 0;
     /* Periodic with N < H: the fills read halo planes of the slab they fill */
@@ -1172,6 +1340,7 @@ pi_status pi_reset_diags(pi_model* m)
     HIPCHECK(hipMemset(m->diagPAR, 0, m->ncell * 4));
     HIPCHECK(hipMemset(m->diag, 0, (size_t)m->nsp * 4 * m->ncell * 4));
     for (float* p : m->diag_tr) HIPCHECK(hipMemset(p, 0, m->ncell * 4));
+    if (m->nsa) HIPCHECK(hipMemset(m->anum, 0, (size_t)m->nsa * m->ncell * 4));
     return PI_OK;
 }
 

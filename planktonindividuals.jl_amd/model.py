@@ -45,6 +45,45 @@ class phyto_setup:
         self.Nsp = Nsp
 
 
+class Palat:
+    """Palat(intac, release) (src/PlanktonIndividuals.jl:118-121): (phytoplankton,
+    abiotic) species pairs that interact, and pairs that release particles."""
+
+    def __init__(self, intac=(), release=()):
+        self.intac = list(intac)
+        self.release = list(release)
+
+
+class abiotic_setup:
+    """abiotic_setup(params, N, Nsa, palat) (src/PlanktonIndividuals.jl:122-127)."""
+
+    def __init__(self, params=None, N=(2 ** 10,), Nsa=1, palat=None):
+        self.params = params
+        self.N = list(N)
+        self.Nsa = Nsa
+        self.palat = palat if palat is not None else Palat()
+
+
+def initialize_abiotic_particle(N, cap, grid, seed, sa):
+    """initialize_abiotic_particle! (Abiotic/particle_generation.jl:24-35): the
+    first N slots active, positions U·N per axis (the draws from the shared
+    stream, species PI_ABIO_SPECIES + sa), then mask_individuals! (land)."""
+    f32 = np.float32
+    slots = np.arange(cap, dtype=np.int64)
+    ux, uy, uz = stream.uniform3(seed, 0, stream.ABIO_SPECIES + sa, stream.KIND_INIT_POS, slots)
+    ac = np.zeros(cap, dtype=bool)
+    ac[:N] = True
+    x = _mask(ac, ux * f32(grid.Nx))
+    y = _mask(ac, uy * f32(grid.Ny))
+    z = _mask(ac, uz * f32(grid.Nz))
+    n = slice(0, N)
+    xi = np.trunc(_mask(ac[n], x[n] + f32(1))).astype(np.int64) + grid.Hx
+    yi = np.trunc(_mask(ac[n], y[n] + f32(1))).astype(np.int64) + grid.Hy
+    zi = np.trunc(_mask(ac[n], z[n] + f32(1))).astype(np.int64) + grid.Hz
+    ac[n] = grid.landmask[xi - 1, yi - 1, zi - 1] & ac[n]
+    return dict(x=x, y=y, z=z, ac=ac)
+
+
 # ------------------------------------------------------ initial individuals
 def _mask(ac, v):
     """Bool * x  ==  ifelse(b, x, copysign(0, x))."""
@@ -136,10 +175,31 @@ class _Phytos(dict):
             raise AttributeError(k)
 
 
+class _AbioticData:
+    """`model.individuals.abiotics.saN.data`: x, y, z, xi, yi, zi, ac (merg and idx stay 0)."""
+
+    def __init__(self, model, sa):
+        self._model, self._sa = model, sa
+
+    def __getattr__(self, name):
+        if name.startswith("_"):
+            raise AttributeError(name)
+        if name in ("merg", "idx"):
+            return np.zeros(self._model.max_individuals, np.int64)
+        return self._model._native.abiotic_field(self._sa, name)
+
+
+class _Abiotic:
+    def __init__(self, model, sa):
+        self.data = _AbioticData(model, sa)
+        self.p = dict(zip(_abi.ABIOTIC_PARAM_NAMES, model.spec.abiotic_matrix[sa]))
+        self.bc = BoundaryConditions()
+
+
 class _Individuals:
     def __init__(self, model):
         self.phytos = _Phytos({f"sp{s + 1}": _Species(model, s) for s in range(model.n_species)})
-        self.abiotics = {}
+        self.abiotics = _Phytos({f"sa{a + 1}": _Abiotic(model, a) for a in range(model.spec.n_abiotic)})
 
 
 class BoundaryConditions:
@@ -198,8 +258,37 @@ class ModelSpec:
         mode = mode if mode is not None else P.QuotaMode()
         if not isinstance(mode, (P.QuotaMode, P.CarbonMode)):
             raise NotImplementedError(f"{mode}: QuotaMode and CarbonMode are implemented on the MI355X hot path")
-        if abiotic is not None:
-            raise NotImplementedError("abiotic particles are outside the MI355X hot path")
+        self.n_abiotic, self.abiotic_params, self.abiotic_matrix, self.abiotic_N = 0, None, None, []
+        if abiotic is not None:                       # models.jl:99-117
+            if not isinstance(abiotic, abiotic_setup):
+                raise ValueError("PlanktonModel:`abiotic` must be either Nothing or `abiotic_setup`!")
+            if max(abiotic.N) > max_individuals:
+                raise AssertionError("maximum(abiotic.N) ≤ max_individuals")
+            if len(abiotic.N) != abiotic.Nsa:
+                raise ValueError(f"PlanktonModel: `abiotic`: The length of `N` must be {abiotic.Nsa}, the same as "
+                                 "`Nsa`, each species has its own initial condition")
+            if abiotic.Nsa > _abi.MAX_ABIOTIC:
+                raise NotImplementedError(f"at most {_abi.MAX_ABIOTIC} abiotic species per model on the MI355X path")
+            if abiotic.params is None:
+                ap = P.update_abiotic_params(P.abiotic_params_default(abiotic.Nsa), FT, N=abiotic.Nsa)
+            elif isinstance(abiotic.params, dict):
+                ap = P.update_abiotic_params(abiotic.params, FT, N=abiotic.Nsa)
+            else:
+                raise ValueError("Abiotic particle parameters must be either Nothing or Dict!")
+            nphy = phyto.Nsp if phyto is not None else 1
+            for pair in list(abiotic.palat.intac) + list(abiotic.palat.release):     # models.jl:121-142
+                sp, sa = (str(v).lstrip(":") for v in pair)
+                if not (sp.startswith("sp") and sp[2:].isdigit() and 1 <= int(sp[2:]) <= nphy):
+                    raise ValueError(f"Abiotic: {sp} is not generated")
+                if not (sa.startswith("sa") and sa[2:].isdigit() and 1 <= int(sa[2:]) <= abiotic.Nsa):
+                    raise ValueError(f"Abiotic: {sa} is not generated")
+            if abiotic.palat.intac:
+                raise NotImplementedError("phytoplankton-particle interaction needs IronEnergyMode's max_ptc "
+                                          "(particle_interaction.jl:60); IronEnergyMode is not implemented")
+            self.n_abiotic = abiotic.Nsa
+            self.abiotic_params = ap
+            self.abiotic_matrix = P.abiotic_param_matrix(ap, abiotic.Nsa)
+            self.abiotic_N = list(abiotic.N)
         if FT is not np.float32:
             raise ValueError("FT must be Float32")
         if bgc_params is None:
@@ -240,6 +329,7 @@ class ModelSpec:
         # (pi_generate_state); only the tracer value ranges are kept
         self.host_state = bool(host_state)
         self.initial_individuals = self.initial_tracers = None
+        self.initial_abiotics = []
         self.tracer_ranges = None
         if self.host_state:
             self.initial_individuals = [initialize_plankton(self.species_params[s], int(phyto.N[s]),
@@ -247,6 +337,8 @@ class ModelSpec:
                                                             self.mode_id)
                                         for s in range(self.n_species)]
             self.initial_tracers = generate_tracers(grid, source, self.seed)
+            self.initial_abiotics = [initialize_abiotic_particle(int(self.abiotic_N[a]), self.max_individuals, grid,
+                                                                 self.seed, a) for a in range(self.n_abiotic)]
         else:
             self.tracer_ranges = tracer_ranges(source)
 
@@ -275,7 +367,8 @@ class PlanktonModel:
         self.species_params = spec.species_params
         self.t = spec.t
         self._native = NativeModel(self.grid, self.n_species, self.max_individuals, self.species_params,
-                                   spec.bgc_vector, spec.seed, (), device=arch.device, mode=spec.mode_id)
+                                   spec.bgc_vector, spec.seed, (), device=arch.device, mode=spec.mode_id,
+                                   abiotic_params=spec.abiotic_matrix)
         if spec.host_state:
             for s in range(self.n_species):
                 self._native.upload_individuals(s, spec.initial_individuals[s])
@@ -283,6 +376,11 @@ class PlanktonModel:
                 self._native.upload_tracer(tid, c)
         else:
             self._native.generate_state(spec.N, *spec.tracer_ranges)
+            for a in range(spec.n_abiotic):       # abiotic positions are drawn on the host
+                self._native.upload_abiotic(a, initialize_abiotic_particle(
+                    int(spec.abiotic_N[a]), spec.max_individuals, spec.grid, spec.seed, a))
+        for a, arrays in enumerate(spec.initial_abiotics):
+            self._native.upload_abiotic(a, arrays)
         self.individuals = _Individuals(self)
         self.tracers = _Tracers({n: _Field(self, i) for i, n in enumerate(_abi.TRACER_NAMES)})
         self.timestepper = _Timestepper(self)
@@ -301,7 +399,10 @@ class PlanktonModel:
                 f"├── grid: {self.grid!r}\n"
                 f"├── {self.mode} is selected for phytoplankton physiology\n"
                 f"├── phytoplankton: {self.n_species} species with {N} individuals for each species\n"
-                "├── No abiotic particles available\n"
+                + ("├── No abiotic particles available\n" if not self.spec.n_abiotic else
+                   f"├── abiotic particles: {self.spec.n_abiotic} species with "
+                   f"{[int(self._native.abiotic_field(a, 'ac').sum()) for a in range(self.spec.n_abiotic)]} "
+                   "particles for each species\n") +
                 f"└── maximum number of individuals: {self.max_individuals} per species")
 
 
@@ -323,6 +424,24 @@ def set_bc(model, *, tracer, pos, bc_value):
         v = np.float32(bc_value) if np.ndim(bc_value) == 0 else np.asarray(bc_value, dtype=np.float32)
     model._native.set_tracer_bc(_abi.TRACER_ID[tracer], pos, v)
     setattr(model.tracers[tracer].bc, pos, v)
+
+
+def set_bc_particle(model, *, sa, pos, bc_value):
+    """set_bc_particle!(model; sa, pos, bc_value) — boundary_conditions.jl:31-39:
+    the flux of abiotic particles of species `sa` through face `pos` (a number,
+    a face array or one per iteration), turned into particles at the cells
+    next to the face by particles_from_bcs! every step."""
+    sa, pos = str(sa).lstrip(":"), str(pos).lstrip(":")
+    if sa not in model.individuals.abiotics:
+        raise KeyError(sa)
+    if pos not in _abi.BC_POS:
+        raise ValueError(f"type BoundaryConditions has no field {pos}")
+    if bc_value is None:
+        v = None
+    else:
+        v = np.float32(bc_value) if np.ndim(bc_value) == 0 else np.asarray(bc_value, dtype=np.float32)
+    model._native.set_particle_bc(int(sa[2:]) - 1, pos, v)
+    setattr(model.individuals.abiotics[sa].bc, pos, v)
 
 
 def validate_bcs(tracers, grid, nΔT):

@@ -6,7 +6,7 @@ species' NamedTuple is built, QuotaMode/plankton_generation.jl:22-31).
 """
 import numpy as np
 
-from ._abi import PARAM_NAMES, BGC_NAMES, CARBON_PARAM_NAMES, MODE_QUOTA, MODE_CARBON
+from ._abi import PARAM_NAMES, BGC_NAMES, CARBON_PARAM_NAMES, ABIOTIC_PARAM_NAMES, MODE_QUOTA, MODE_CARBON
 
 
 class QuotaMode:
@@ -136,3 +136,34 @@ def species_param_matrix(phyt_params, nsp, mode=None):
 
 def bgc_vector(bgc):
     return np.array([bgc[k] for k in BGC_NAMES], dtype=np.float32)
+
+
+def abiotic_params_default(N=1):
+    """abiotic_params_default(N) — param_default.jl:325-343."""
+    params = {"Nsuper": [1], "Rd": [1.0e-4], "release_P": [1.0e-6], "sz_min": [1.0e-3], "Ktr": [0.0e-5]}
+    if N == 1:
+        return params
+    return generate_n_species_params(N, params)
+
+
+def update_abiotic_params(tmp, FT=np.float32, *, N=1):
+    """update_abiotic_params(tmp::Dict, FT; N) — param_update.jl:48-69."""
+    parameters = abiotic_params_default(N)
+    for key, val in tmp.items():
+        if key not in parameters:
+            raise ValueError(f"PARAM: abiotic parameter not found {key}")
+        v = list(np.atleast_1d(np.asarray(val, dtype=np.float64)))
+        parameters[key] = [FT(x) for x in v]
+    return parameters
+
+
+def abiotic_param_matrix(params, nsa):
+    """(nsa, 5) float32 in param_names order (Abiotic/particle_generation.jl:8-20):
+    `p = NamedTuple{param_names}(FT.(tmp))`, tmp[i] = params[name][sa]."""
+    out = np.zeros((nsa, len(ABIOTIC_PARAM_NAMES)), dtype=np.float32)
+    for s in range(nsa):
+        for q, name in enumerate(ABIOTIC_PARAM_NAMES):
+            if name not in params:
+                raise ValueError(f"PARAM: parameter not found {name}")
+            out[s, q] = np.float32(np.float64(params[name][s]))
+    return out

@@ -81,6 +81,12 @@ class PlanktonDiagnostics:
             out[f"sp{s + 1}"] = d
         return out
 
+    @property
+    def abiotics(self):
+        """diags.abiotics.saN.num: abiotic particles per cell, accumulated per step."""
+        n = getattr(getattr(self.model, "spec", None), "n_abiotic", 0)
+        return {f"sa{a + 1}": {"num": self.model._native.abiotic_diag(a)} for a in range(n)}
+
     def reset(self):
         self.model._native.reset_diags()
 
@@ -123,6 +129,8 @@ class PlanktonSimulation:
         self.validate_PARF()
         self.validate_temp()
         validate_bcs(model.tracers, g, self.iterations)       # simulations.jl:81
+        abiotics = getattr(getattr(model, "individuals", None), "abiotics", {})
+        validate_bcs(abiotics, g, self.iterations)             # simulations.jl:82
 
     # ---- validators (Simulation/utils.jl:36-127)
     def _check_step(self, ΔT_in, what):

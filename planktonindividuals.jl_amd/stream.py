@@ -12,6 +12,10 @@ KIND_INIT_POS = 16      # x, y, z
 KIND_INIT_QUOTA = 17    # Cq, Nq, Pq
 KIND_INIT_SIZE = 18     # two uniforms for the Box-Muller normal of Sz
 KIND_TRACER = 32        # + tracer id; slot = padded cell index
+# abiotic particles: species field ABIO_SPECIES + sa; particles_from_bcs! draws
+# KIND_ABIO_BC per padded cell (slot = the cell's index)
+ABIO_SPECIES = 64
+KIND_ABIO_BC = 2
 
 _M0 = np.uint64(0xD2511F53)
 _M1 = np.uint64(0xCD9E8D57)

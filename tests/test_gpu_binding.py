@@ -38,7 +38,8 @@ class ModelDesc(C.Structure):         # pi_model_desc
     _fields_ = [("grid", GridDesc), ("n_species", C.c_int32), ("capacity", C.c_int64),
                 ("params", fp), ("bgc", fp), ("seed", C.c_uint64),
                 ("n_diag", C.c_int32), ("diag", C.POINTER(C.c_int32)), ("mode", C.c_int32),
-                ("metrics", C.c_void_p)]   # NULL: RectilinearGrid
+                ("metrics", C.c_void_p),   # NULL: RectilinearGrid
+                ("n_abiotic", C.c_int32), ("abiotic_params", C.c_void_p), ("n_interactions", C.c_int32)]
 
 
 FLOAT_COLS = ("x", "y", "z", "Sz", "Bm", "Cq", "Nq", "Pq", "Chl", "gen", "age", "graz", "mort", "dvid")
@@ -92,7 +93,8 @@ class Binding:
         dzC, dzF, lm, prm, bg, dtr = self.keep
         gd = GridDesc(*g["N"], *g["H"], *g["topo"], g["dx"], g["dy"], _p(dzC), _p(dzF), _p(lm, C.c_uint8))
         d = ModelDesc(gd, self.nsp, cap, _p(prm), _p(bg), seed, len(dtr),
-                      _p(dtr, C.c_int32) if len(dtr) else None, 0, None)    # PI_MODE_QUOTA, no metrics
+                      _p(dtr, C.c_int32) if len(dtr) else None, 0, None,    # PI_MODE_QUOTA, no metrics
+                      0, None, 0)                                            # no abiotic particles
         self.h = VP()
         _ok(lib, lib.pi_model_create(C.byref(d), C.byref(self.h)))
         for s, col in enumerate(columns):
