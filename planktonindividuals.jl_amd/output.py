@@ -18,12 +18,13 @@ src/Output/write_outputs.jl).
   it; Bool columns are an HDF5 enum over int8 (FALSE = 0, TRUE = 1).  The
   diagnostics are zeroed after they are written (on the device).
 * File rotation (`max_filesize`): as start_next_diags_file /
-  start_next_phytoplankton_file (write_outputs.jl:42-75): when the current file
+  start_next_phytoplankton_file / start_next_abiotic_particle_file
+  (write_outputs.jl:42-75): when the current file
   has reached the limit, the first file is renamed `<prefix>_part1.jld2` and
   writing continues in `<prefix>_part2.jld2`, `_part3`, ...
 
-The reference's per-individual rate diagnostics and abiotic particles are not
-on this path (their outputs are absent; `ptc` is written as zeros).
+The reference's per-individual rate diagnostics are not on this path (`ptc`
+is written as zeros).
 """
 import ctypes as C
 import ctypes.util
@@ -203,7 +204,9 @@ class PlanktonOutputWriter:
     """PlanktonOutputWriter(; dir="./results", diags_prefix="diags",
     phytoplankton_prefix="phytoplankton", write_log=false, save_diags=false,
     save_phytoplankton=false, phytoplankton_include=(:x, :y, :z),
-    phytoplankton_iteration_interval=1, max_filesize=Inf) — output_writers.jl:19-80.
+    phytoplankton_iteration_interval=1, abiotic_particle_prefix="abiotic_particle",
+    save_abiotic_particle=false, abiotic_particle_include=(:x, :y, :z),
+    abiotic_particle_iteration_interval=1, max_filesize=Inf) — output_writers.jl:19-80.
     Like the reference, an existing `dir` is removed and created again."""
 
     def __init__(self, *, dir="./results", diags_prefix="diags", phytoplankton_prefix="phytoplankton",
@@ -211,8 +214,6 @@ class PlanktonOutputWriter:
                  save_phytoplankton=False, save_abiotic_particle=False, phytoplankton_include=("x", "y", "z"),
                  abiotic_particle_include=("x", "y", "z"), phytoplankton_iteration_interval=1,
                  abiotic_particle_iteration_interval=1, max_filesize=math.inf):
-        if save_abiotic_particle:
-            raise NotImplementedError("abiotic particles are not on the MI355X hot path")
         if os.path.isdir(dir):
             shutil.rmtree(dir)
         os.mkdir(dir)
@@ -220,12 +221,13 @@ class PlanktonOutputWriter:
         self.write_log = bool(write_log)
         self.save_diags = bool(save_diags)
         self.save_phytoplankton = bool(save_phytoplankton)
-        self.save_abiotic_particle = False
+        self.save_abiotic_particle = bool(save_abiotic_particle)
         self.diags_file = os.path.join(dir, diags_prefix + ".jld2") if save_diags else ""
         self.phytoplankton_file = os.path.join(dir, phytoplankton_prefix + ".jld2") if save_phytoplankton else ""
         self.phytoplankton_include = tuple(phytoplankton_include)
         self.phytoplankton_iteration_interval = int(phytoplankton_iteration_interval)
-        self.abiotic_particle_file = ""
+        self.abiotic_particle_file = (os.path.join(dir, abiotic_particle_prefix + ".jld2")
+                                      if save_abiotic_particle else "")
         self.abiotic_particle_include = tuple(abiotic_particle_include)
         self.abiotic_particle_iteration_interval = int(abiotic_particle_iteration_interval)
         self.max_filesize = max_filesize
@@ -234,12 +236,41 @@ class PlanktonOutputWriter:
         self.part_abiotic_particle = 1
 
     def __repr__(self):
+        """show(io, writer) — output_writers.jl:93-128."""
         d = f"diagnostics are saved as {self.diags_file}" if self.save_diags else "diagnostics are not saved"
-        p = (f"phytoplankton is saved as {self.phytoplankton_file}" if self.save_phytoplankton
-             else "phytoplankton are not saved")
+        p = (f"phytoplankton is saved as {self.phytoplankton_file}\n"
+             f"│   ├── every {self.phytoplankton_iteration_interval} seconds\n"
+             f"│   └── including: {_julia_tuple(self.phytoplankton_include)}"
+             if self.save_phytoplankton else "phytoplankton are not saved")
+        a = (f"abiotic particle is saved as {self.abiotic_particle_file}\n"
+             f"│   ├── every {self.abiotic_particle_iteration_interval} seconds\n"
+             f"│   └── including: {_julia_tuple(self.abiotic_particle_include)}"
+             if self.save_abiotic_particle else "abiotic particle are not saved")
         return (f"PlanktonOutputWriter:\n├── files are saved at {self.filepath}\n├── {d}\n├── {p}\n"
-                f"├── abiotic particles are not saved\n├── write log: {str(self.write_log).lower()}\n"
-                f"└── Maximum file size: {self.max_filesize}")
+                f"├── {a}\n├── write log: {str(self.write_log).lower()}\n"
+                f"└── Maximum file size: {humanize_filesize(self.max_filesize)}")
+
+
+def _julia_tuple(names):
+    """How Julia prints a tuple of Symbols: (:x, :y, :z), (:x,) for one."""
+    body = ", ".join(":" + n for n in names)
+    return "(" + body + ("," if len(names) == 1 else "") + ")"
+
+
+def humanize_filesize(s):
+    """humanize_filesize (output_writers.jl:129-142): the first power of 1024
+    above the size names the unit, printed as %3.1f of 1024·s/unit.  The unit
+    is Julia's Int 1024^power, which wraps to 0 from 1024^7 on, so sizes of
+    1024^6 B and more (and Inf) end in YiB with unit 0: "Inf YiB"."""
+    suffix = ("B", "KiB", "MiB", "GiB", "TiB", "PiB", "EiB", "ZiB", "YiB")
+    big, unit = suffix[0], 1024
+    for power in range(1, len(suffix) + 1):
+        unit = 1024 ** power if power <= 6 else 0
+        big = suffix[power - 1]
+        if s < unit:
+            break
+    v = math.inf if unit == 0 else 1024 * s / unit
+    return _fmt("%3.1f", v) + " " + big
 
 
 def _filesize(path):
@@ -277,7 +308,14 @@ def write_output(writer, model, diags):
         if _filesize(writer.phytoplankton_file) >= writer.max_filesize:
             writer.phytoplankton_file, writer.part_phytoplankton = _next_part(writer.phytoplankton_file,
                                                                               writer.part_phytoplankton)
-        write_individuals(model, writer.phytoplankton_file, model.t, it, writer.phytoplankton_include)
+        write_individuals(model.individuals.phytos, writer.phytoplankton_file, model.t, it,
+                          writer.phytoplankton_include)
+    if writer.save_abiotic_particle and it % writer.abiotic_particle_iteration_interval == 0:
+        if _filesize(writer.abiotic_particle_file) >= writer.max_filesize:
+            writer.abiotic_particle_file, writer.part_abiotic_particle = _next_part(
+                writer.abiotic_particle_file, writer.part_abiotic_particle)
+        write_individuals(model.individuals.abiotics, writer.abiotic_particle_file, model.t, it,
+                          writer.abiotic_particle_include)
 
 
 # ---------------------------------------------------------------- text logs
@@ -342,17 +380,16 @@ def _interior(a, grid):
     return a[grid.Hx:grid.Hx + grid.Nx, grid.Hy:grid.Hy + grid.Ny, grid.Hz:grid.Hz + grid.Nz]
 
 
-def write_individuals(model, filepath, t, it, atts):
-    """write_individuals_to_jld2 (write_outputs.jl:159-169)."""
+def write_individuals(particles, filepath, t, it, atts):
+    """write_individuals_to_jld2 (write_outputs.jl:159-169): `particles` is
+    model.individuals.phytos or model.individuals.abiotics (name → .data)."""
     h = hdf5()
     fid = h.open(filepath)
     try:
         h.write(fid, f"timeseries/t/{it}", np.float32(t))
-        for s in range(model.n_species):
-            sp = f"sp{s + 1}"
-            d = model.individuals.phytos[sp].data
+        for sp, x in particles.items():
             for att in atts:
-                h.write(fid, f"timeseries/{sp}/{att}/{it}", getattr(d, att))
+                h.write(fid, f"timeseries/{sp}/{att}/{it}", getattr(x.data, att))
     finally:
         h.close(fid)
 
@@ -371,6 +408,9 @@ def write_diags(diags, filepath, t, it, ncounts, grid):
         for sp, procs in diags.phytos.items():
             for proc in diags.phyto_keys():
                 h.write(fid, f"timeseries/phyto/{sp}/{proc}/{it}", (_interior(procs[proc], grid) / n).astype(np.float32))
+        for sa, procs in getattr(diags, "abiotics", {}).items():
+            for proc in procs:
+                h.write(fid, f"timeseries/abiotic/{sa}/{proc}/{it}", (_interior(procs[proc], grid) / n).astype(np.float32))
     finally:
         h.close(fid)
     diags.reset()

@@ -75,3 +75,60 @@ def test_update_with_output_writer_matches_oracle(pi, orc, tmp_path):
         lines = open(os.path.join(d, f"dynamic_species00{s + 1}.txt")).read().splitlines()
         assert lines == logs[s], s
     assert not out.has_dataset(os.path.join(d, "diags.jld2"), "timeseries/t/4")
+
+
+def test_abiotic_particle_output_matches_oracle(pi, orc, tmp_path):
+    """save_abiotic_particle and the abiotic num diagnostic through update! on the
+    GPU: every saved column and timeseries/abiotic/saN/num equal the oracle's
+    state at the same steps (write_outputs.jl:31-38, 159-169, 182-186)."""
+    from importlib import import_module
+    out = import_module("pi_amd.output")
+    try:
+        out.hdf5()
+    except RuntimeError as e:
+        pytest.skip(str(e))
+    g = pi.RectilinearGrid(size=(12, 10, 3), x=(0, 12e3), y=(0, 10e3), z=(0, -30))
+    spec = pi.ModelSpec(g, max_individuals=3000, seed=88, phyto=pi.phyto_setup({"dvid_P": [1e-3]}, [800], 1),
+                        abiotic=pi.abiotic_setup({"sz_min": [1e-3, 2e-3]}, [300, 200], 2),
+                        bgc_params={"κhP": 2.0, "κh": 10.0})
+    u, v, w = pi.workloads.cellular_flow(g, U=1.0, L=4e3)
+    dt, n = 60.0, 6
+    PARF = np.full((g.Nx, g.Ny, n + 1), 600.0, np.float32)
+    temp = pi.default_temperature(g, dt, n).astype(np.float32)
+    d = str(tmp_path / "res")
+    writer = pi.PlanktonOutputWriter(dir=d, save_diags=True, save_abiotic_particle=True,
+                                     abiotic_particle_include=("x", "y", "z", "xi", "ac"),
+                                     abiotic_particle_iteration_interval=2)
+    gm = pi.PlanktonModel(pi.HIPArch(), spec=spec)
+    om = orc.OracleModel(spec)
+    flux = np.float32(0.3 * spec.abiotic_matrix[0, 3] * spec.abiotic_matrix[0, 0] / (dt * g.Δy * 10.0))
+    pi.set_bc_particle(gm, sa="sa1", pos="west", bc_value=flux)
+    om.set_particle_bc(0, "west", flux)
+    diags = pi.PlanktonDiagnostics(gm, iteration_interval=3)
+    sim = pi.PlanktonSimulation(gm, ΔT=dt, iterations=n, PARF=PARF, temp=temp, ΔT_PAR=dt, diags=diags,
+                                vels=dict(u=u[..., None], v=v[..., None], w=w[..., None]), ΔT_vel=dt * 1e6,
+                                output_writer=writer)
+    pi.update(sim)
+    H = (g.Hx, g.Hy, g.Hz)
+    inner = lambda a: a[H[0]:H[0] + g.Nx, H[1]:H[1] + g.Ny, H[2]:H[2] + g.Nz]
+    om.set_velocity(u, v, w)
+    t = np.float32(0.0)
+    for it in range(1, n + 1):
+        om.set_parf(PARF[:, :, it - 1])
+        om.set_temperature(temp[:, :, :, int(np.floor(t / np.float32(3600.0)))])
+        om.step(np.float32(dt))
+        t = np.float32(np.float32(it) * np.float32(dt))
+        if it % 2 == 0:
+            f = os.path.join(d, "abiotic_particle.jld2")
+            assert out.read_dataset(f, f"timeseries/t/{it}") == t
+            for a in range(2):
+                for att in ("x", "y", "z", "xi", "ac"):
+                    got = out.read_dataset(f, f"timeseries/sa{a + 1}/{att}/{it}")
+                    assert np.array_equal(got, om.abiotic_field(a, att)), (it, a, att)
+        if it % 3 == 0:
+            f = os.path.join(d, "diags.jld2")
+            for a in range(2):
+                got = out.read_dataset(f, f"timeseries/abiotic/sa{a + 1}/num/{it}")
+                assert np.array_equal(got, inner(om.abiotic_diag(a)) / np.float32(3.0)), (it, a)
+            om.reset_diags()
+    assert om.abiotic_field(0, "ac").sum() > 300

@@ -222,3 +222,73 @@ def test_writer_without_limits_keeps_one_file(out, tmp_path):
     assert os.listdir(d) == ["diags.jld2"]
     for it in (1, 2, 3):
         assert out.read_dataset(os.path.join(d, "diags.jld2"), f"timeseries/t/{it}") == np.float32(it * 30.0)
+
+
+class _AbioticModel(_Model):
+    """The stand-in with abiotic particles (model.individuals.abiotics.saN.data)."""
+
+    def __init__(self, n, nsa=2):
+        super().__init__(n, nsp=1)
+        rng = np.random.default_rng(n + 1)
+        ab = {}
+        for a in range(nsa):
+            c = {f: rng.random(n).astype(np.float32) for f in ("x", "y", "z")}
+            c["ac"] = rng.random(n) < 0.5
+            c["zi"] = rng.integers(0, 5, n).astype(np.int64)
+            ab[f"sa{a + 1}"] = _Sp(c)
+        self.individuals.abiotics = ab
+
+
+class _AbioticDiags(_Diags):
+    @property
+    def abiotics(self):
+        return {f"sa{a + 1}": {"num": self.acc * (10 + a)} for a in range(2)}
+
+
+def test_abiotic_particles_and_their_diagnostics(out, tmp_path):
+    """save_abiotic_particle: every `abiotic_particle_iteration_interval` steps the
+    included columns of every abiotic species go to timeseries/saN/<att>/<iter>
+    of abiotic_particle.jld2, rotated like the other files (write_outputs.jl:31-38,
+    65-74, 159-169); the diagnostics file carries timeseries/abiotic/saN/num/<iter>
+    as interior ./ ncounts (write_outputs.jl:182-186)."""
+    _h5(out)
+    d = str(tmp_path / "res")
+    w = out.PlanktonOutputWriter(dir=d, save_diags=True, save_abiotic_particle=True,
+                                 abiotic_particle_prefix="ptc", abiotic_particle_include=("x", "ac", "zi"),
+                                 abiotic_particle_iteration_interval=3, max_filesize=1)
+    m = _AbioticModel(30)
+    diags = _AbioticDiags(m, 2)
+    for it in range(1, 7):
+        m.iteration, m.t = it, np.float32(it * 10.0)
+        out.write_output(w, m, diags)
+    files = sorted(os.listdir(d))
+    assert files == ["diags_part1.jld2", "diags_part2.jld2", "diags_part3.jld2", "ptc_part1.jld2", "ptc_part2.jld2"]
+    assert w.abiotic_particle_file == os.path.join(d, "ptc_part2.jld2") and w.part_abiotic_particle == 2
+    p1, p2 = os.path.join(d, "ptc_part1.jld2"), os.path.join(d, "ptc_part2.jld2")
+    assert out.read_dataset(p1, "timeseries/t/3") == np.float32(30.0)
+    assert out.has_dataset(p2, "timeseries/t/6") and not out.has_dataset(p2, "timeseries/t/3")
+    for sa in ("sa1", "sa2"):
+        data = m.individuals.abiotics[sa].data
+        for att in ("x", "ac", "zi"):
+            got = out.read_dataset(p2, f"timeseries/{sa}/{att}/6")
+            assert got.dtype == getattr(data, att).dtype and np.array_equal(got, getattr(data, att)), (sa, att)
+        assert not out.has_dataset(p2, f"timeseries/{sa}/y/6")
+    num = out.read_dataset(os.path.join(d, "diags_part2.jld2"), "timeseries/abiotic/sa2/num/4")
+    assert num.shape == (3, 2, 2) and np.all(num == np.float32(11.0) / np.float32(2.0))
+
+
+def test_writer_show_and_humanize_filesize(out, tmp_path):
+    """show(io, writer) and humanize_filesize (output_writers.jl:93-142): the
+    included columns as a Julia tuple of Symbols; sizes in the first power of
+    1024 above them, with Julia's Int 1024^power wrapping to 0 from 1024^7 on
+    (so Inf prints as "Inf YiB")."""
+    assert [out.humanize_filesize(v) for v in (0, 1023, 1024, 1536, 5e6, 3 * 1024 ** 3, math.inf)] == \
+        ["0.0 B", "1023.0 B", "1.0 KiB", "1.5 KiB", "4.8 MiB", "3.0 GiB", "Inf YiB"]
+    d = str(tmp_path / "s")
+    w = out.PlanktonOutputWriter(dir=d, save_abiotic_particle=True, abiotic_particle_include=("z",),
+                                 abiotic_particle_iteration_interval=5, max_filesize=2048)
+    assert str(w).splitlines() == [
+        "PlanktonOutputWriter:", f"├── files are saved at {d}", "├── diagnostics are not saved",
+        "├── phytoplankton are not saved", f"├── abiotic particle is saved as {d}/abiotic_particle.jld2",
+        "│   ├── every 5 seconds", "│   └── including: (:z,)", "├── write log: false",
+        "└── Maximum file size: 2.0 KiB"]
