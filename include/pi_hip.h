@@ -49,8 +49,9 @@ enum {
     PI_NTRACERS = 14
 };
 
-/* ---- QuotaMode parameters: 40 floats in the reference's `param_names` order
- *      (QuotaMode/plankton_generation.jl:16-20). ---------------------------- */
+/* ---- QuotaMode parameters: the first 40 of a species' PI_NPARAMS floats, in
+ *      the reference's `param_names` order (QuotaMode/plankton_generation.jl:16-20);
+ *      every mode's row is PI_NPARAMS floats, the unused tail zero. ------- */
 enum {
     PI_P_Nsuper = 0, PI_P_Cquota, PI_P_mean, PI_P_var, PI_P_Chl2Cint, PI_P_alpha,
     PI_P_Phi, PI_P_Topt, PI_P_Tmax, PI_P_Ea, PI_P_PCmax, PI_P_VDOCmax, PI_P_VNO3max,
@@ -59,11 +60,13 @@ enum {
     PI_P_respir, PI_P_grz_P, PI_P_dvid_type, PI_P_dvid_P, PI_P_dvid_reg, PI_P_dvid_reg2,
     PI_P_mort_P, PI_P_mort_reg, PI_P_grazFracC, PI_P_grazFracN, PI_P_grazFracP,
     PI_P_mortFracC, PI_P_mortFracN, PI_P_mortFracP,
-    PI_NPARAMS = 40
+    PI_NQPARAMS = 40,
+    PI_NPARAMS = 64
 };
 
-/* ---- physiology modes (src/PlanktonIndividuals.jl:65,71: CarbonMode, QuotaMode) */
-enum { PI_MODE_QUOTA = 0, PI_MODE_CARBON = 1 };
+/* ---- physiology modes (src/PlanktonIndividuals.jl:65-77: CarbonMode, QuotaMode,
+ *      MacroMolecularMode) */
+enum { PI_MODE_QUOTA = 0, PI_MODE_CARBON = 1, PI_MODE_MACROMOLECULAR = 2 };
 
 /* ---- CarbonMode parameters: 24 floats in the reference's `param_names` order
  *      (CarbonMode/plankton_generation.jl:14-16), in the first 24 of a species'
@@ -74,6 +77,20 @@ enum {
     PI_C_dvid_reg, PI_C_dvid_reg2, PI_C_mort_P, PI_C_mort_reg, PI_C_grazFracC, PI_C_mortFracC,
     PI_C_thermal, PI_C_is_bact,
     PI_NCPARAMS = 24
+};
+
+/* ---- MacroMolecularMode parameters: 47 floats in the reference's `param_names`
+ *      order (MacroMolecularMode/plankton_generation.jl:15-21). ----------- */
+enum {
+    PI_M_Nsuper = 0, PI_M_C_DNA, PI_M_var, PI_M_PRO2DNA, PI_M_RNA2DNA, PI_M_CH2DNA, PI_M_Chl2DNA,
+    PI_M_alpha, PI_M_Phi, PI_M_Topt, PI_M_Tmax, PI_M_Ea,
+    PI_M_PCmax, PI_M_VDOCmax, PI_M_VNO3max, PI_M_VNH4max, PI_M_VPO4max, PI_M_KsatDOC, PI_M_KsatNH4,
+    PI_M_KsatNO3, PI_M_KsatPO4, PI_M_CHmax, PI_M_NSTmax, PI_M_PSTmax,
+    PI_M_Chl2N, PI_M_R_NC_PRO, PI_M_R_NC_DNA, PI_M_R_NC_RNA, PI_M_R_PC_DNA, PI_M_R_PC_RNA, PI_M_respir,
+    PI_M_k_pro, PI_M_k_sat_pro, PI_M_k_rna, PI_M_k_sat_rna, PI_M_k_dna, PI_M_k_sat_dna,
+    PI_M_dvid_P, PI_M_grz_P, PI_M_mort_P, PI_M_mort_reg, PI_M_grazFracC, PI_M_grazFracN, PI_M_grazFracP,
+    PI_M_mortFracC, PI_M_mortFracN, PI_M_mortFracP,
+    PI_NMPARAMS = 47
 };
 
 /* ---- abiotic particle parameters (Abiotic/particle_generation.jl:8,
@@ -91,17 +108,29 @@ enum {
     PI_NBGC = 23
 };
 
-/* ---- individual fields (QuotaMode StructArray columns) ------------------- */
+/* ---- individual fields (QuotaMode StructArray columns, then PI_NEX extra
+ *      float columns that only the modes with more state use) --------------- */
+#define PI_NEX 8
 enum {
     PI_F_x = 0, PI_F_y, PI_F_z, PI_F_Sz, PI_F_Bm, PI_F_Cq, PI_F_Nq, PI_F_Pq,
     PI_F_Chl, PI_F_gen, PI_F_age, PI_F_graz, PI_F_mort, PI_F_dvid,   /* float32 */
     PI_F_xi, PI_F_yi, PI_F_zi,                                       /* int64   */
     PI_F_ac,                                                         /* uint8   */
-    PI_NFIELDS
+    PI_F_ex0,                                                        /* float32: PI_F_ex0 + k, k < PI_NEX */
+    PI_NFIELDS = PI_F_ex0 + PI_NEX
 };
 /* CarbonMode (CarbonMode/plankton_generation.jl:1-12): the damaged biomass Bd
  * is the column PI_F_Cq; its Nq and Pq columns are zero. */
 #define PI_F_Bd PI_F_Cq
+/* MacroMolecularMode (MacroMolecularMode/plankton_generation.jl:1-12): protein,
+ * DNA, carbohydrate/lipid and the N and P stores in the QuotaMode columns, RNA
+ * in the first extra column. */
+#define PI_F_PRO PI_F_Bm
+#define PI_F_DNA PI_F_Sz
+#define PI_F_CH  PI_F_Cq
+#define PI_F_NST PI_F_Nq
+#define PI_F_PST PI_F_Pq
+#define PI_F_RNA PI_F_ex0
 
 /* ---- grid-shaped outputs (timestepper + diagnostics) --------------------- */
 enum {
@@ -148,7 +177,7 @@ typedef struct pi_model_desc {
     uint64_t rng_seed;           /* key of the shared uniform stream (pi_math.h) */
     int32_t n_diag_tracers;      /* tracers accumulated into diags.tracer besides PAR */
     const int32_t* diag_tracers; /* tracer ids, may be NULL when n_diag_tracers == 0 */
-    int32_t mode;                /* PI_MODE_QUOTA or PI_MODE_CARBON (PlanktonModel(...; mode)) */
+    int32_t mode;                /* PI_MODE_QUOTA, PI_MODE_CARBON or PI_MODE_MACROMOLECULAR (PlanktonModel(...; mode)) */
     const pi_metrics* metrics;   /* LatLonGrid metrics; NULL = RectilinearGrid (dx, dy of grid) */
     /* abiotic particles (abiotic_setup, src/PlanktonIndividuals.jl:124-129;
      * Individuals/Abiotic/particle_generation.jl:1-35): n_abiotic species of
@@ -170,6 +199,7 @@ typedef struct pi_individuals_host {
     float *x, *y, *z, *Sz, *Bm, *Cq, *Nq, *Pq, *Chl, *gen, *age, *graz, *mort, *dvid;
     int64_t *xi, *yi, *zi;
     uint8_t *ac;
+    float *ex[PI_NEX];           /* the extra columns (PI_F_ex0 + k); NULL when the mode has none */
 } pi_individuals_host;
 
 typedef struct pi_model pi_model;
@@ -186,10 +216,12 @@ pi_status pi_model_create(const pi_model_desc* desc, pi_model** out);
 pi_status pi_model_destroy(pi_model* m);
 
 /* Initial state (generate_individuals / generate_tracers, Individuals.jl:42-72,
- * tracer_fields.jl:93-134): full-capacity SoA for species `sp`. */
+ * tracer_fields.jl:93-134): full-capacity SoA for species `sp` (the mode's
+ * columns: MacroMolecularMode fills PI_F_PRO ... and ex[0] = RNA). */
 pi_status pi_upload_individuals(pi_model* m, int32_t sp, const pi_individuals_host* h);
 /* The same initial state generated on the device (no host arrays): every
- * species' slots as initialize_plankton! (QuotaMode/plankton_generation.jl:35-72)
+ * species' slots as initialize_plankton! (QuotaMode/plankton_generation.jl:35-72,
+ * CarbonMode/plankton_generation.jl:34-56, MacroMolecularMode/plankton_generation.jl:31-78)
  * with n_individuals[sp] active, and the tracers as generate_tracers with the
  * (initial_condition, rand_noise) source (tracer_fields.jl:93-134): value
  * c0 * (lower + k * 1f-4), k < n drawn per padded cell, then fill_halo_tracer!. */
@@ -215,7 +247,8 @@ pi_status pi_step(pi_model* m, float dt);
 pi_status pi_run(pi_model* m, float dt, int64_t n);
 pi_status pi_sync(pi_model* m);
 
-/* Outputs (Output/write_outputs.jl:159-202 read these). */
+/* Outputs (Output/write_outputs.jl:159-202 read these).  pi_download_field
+ * refuses an extra column (PI_F_ex0 + k) that the model's mode does not have. */
 pi_status pi_download_field(pi_model* m, int32_t sp, int32_t field, void* host, int64_t n);
 pi_status pi_download_tracer(pi_model* m, int32_t tracer, float* padded);
 pi_status pi_download_grid(pi_model* m, int32_t which, float* host, int64_t n);

@@ -1,8 +1,8 @@
 """TEST INFRASTRUCTURE: checks that the oracle's pin tests can fail.
 
 Each entry below plants one plausible mistake in a copy of oracle/pi_oracle.cpp
-(a wrong index, sign, exponent, dropped term or halo plane, in QuotaMode and
-CarbonMode), builds the copy
+(a wrong index, sign, exponent, dropped term or halo plane, in QuotaMode,
+CarbonMode and MacroMolecularMode), builds the copy
 into a temporary library, and runs the CPU pin tests against it
 (PI_ORACLE_LIB).  Every mistake must make at least one pin fail.  The
 committed source is never modified.
@@ -150,6 +150,24 @@ MISTAKES = [
     ("abiotic: the first face decides at an edge",
      "flag[c] = M::isless(rnd(c) * szN * Ns, b * dt * A) ? (FT)1 : (FT)0;",
      "if (flag[c] == (FT)0) flag[c] = M::isless(rnd(c) * szN * Ns, b * dt * A) ? (FT)1 : (FT)0;"),
+    ("mm: protein synthesis from protein", "p.S_PRO[i] = q[PI_M_k_pro] * RNA[i] * limit_PRO * tf;",
+     "p.S_PRO[i] = q[PI_M_k_pro] * PRO[i] * limit_PRO * tf;"),
+    ("mm: synthesis masked by ac", "p.S_PRO[i] = q[PI_M_k_pro] * RNA[i] * limit_PRO * tf;",
+     "p.S_PRO[i] = M::mask(p.ac[i], q[PI_M_k_pro] * RNA[i] * limit_PRO * tf);"),
+    ("mm: protein respired with the DNA N:C", "NST[i] = NST[i] + M::max(z0, (z0 - CH[i])) * q[PI_M_R_NC_PRO];",
+     "NST[i] = NST[i] + M::max(z0, (z0 - CH[i])) * q[PI_M_R_NC_DNA];"),
+    ("mm: exudation taken times dt", "CH[i] = CH[i] - p.exu[i];", "CH[i] = CH[i] - p.exu[i] * dt;"),
+    ("mm: DNA synthesis not gated",
+     "p.S_DNA[i] = M::mask(M::isless(DNA[i] / (q[PI_M_C_DNA] * Ns), L(2)), q[PI_M_k_dna] * PRO[i] * limit_DNA * tf);",
+     "p.S_DNA[i] = q[PI_M_k_dna] * PRO[i] * limit_DNA * tf;"),
+    ("mm: mortality falling with age", "shape_inc(p.age[i], q[PI_M_mort_reg], L(1.0e-4f), 4)",
+     "shape_dec(p.age[i], q[PI_M_mort_reg], L(1.0e-4f), 4)"),
+    ("mm: division below two DNA units", "M::isless(L(2), DNA[i] / (q[PI_M_C_DNA] * q[PI_M_Nsuper]))",
+     "M::isless(DNA[i] / (q[PI_M_C_DNA] * q[PI_M_Nsuper]), L(2))"),
+    ("mm: total N without chlorophyll", "RNA * q[PI_M_R_NC_RNA] + NST + Chl / L(893.49f) * L(4.0f);",
+     "RNA * q[PI_M_R_NC_RNA] + NST;"),
+    ("mm: daughter without RNA",
+     "PRO[d] = PRO[i]; DNA[d] = DNA[i]; RNA[d] = RNA[i]; CH[d] = CH[i];", "PRO[d] = PRO[i]; DNA[d] = DNA[i]; CH[d] = CH[i];"),
 ]
 
 
@@ -173,7 +191,8 @@ def main(only):
             r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-m", "not gpu", "-p", "no:cacheprovider",
                                 "tests/test_oracle_pins.py", "tests/test_oracle_pins_physics.py",
                                 "tests/test_oracle_pins_carbon.py", "tests/test_oracle_pins_bcs.py",
-                                "tests/test_oracle_pins_latlon.py", "tests/test_oracle_pins_abiotic.py"],
+                                "tests/test_oracle_pins_latlon.py", "tests/test_oracle_pins_abiotic.py",
+                                "tests/test_oracle_pins_mm.py"],
                                cwd=REPO, env=env, capture_output=True, text=True)
             first = next((l for l in r.stdout.splitlines() if l.startswith("FAILED")), "")
             (caught if r.returncode != 0 else missed).append(name)

@@ -2,8 +2,9 @@
  * pi_oracle.cpp - TEST INFRASTRUCTURE ONLY (the checker, never the product).
  *
  * A CPU restatement, kernel by kernel and in the same floating-point operation
- * order, of PlanktonIndividuals.jl v0.7.5's CPU() time step for QuotaMode
- * and CarbonMode without abiotic particles, as run by `julia -t 1` (KernelAbstractions' CPU
+ * order, of PlanktonIndividuals.jl v0.7.5's CPU() time step for QuotaMode,
+ * CarbonMode and MacroMolecularMode, with abiotic particles (without their
+ * interaction with phytoplankton), as run by `julia -t 1` (KernelAbstractions' CPU
  * backend with one thread visits every ndrange index in ascending order, so
  * every `@atomic` accumulation happens in slot order, kernel after kernel).
  *
@@ -108,6 +109,9 @@ struct Oracle : orc_model {
         V x, y, z, Sz, Bm, Cq, Nq, Pq, Chl, gen, age;    /* CarbonMode: Bd in Cq */
         V PS, VDOC, VNH4, VNO3, VPO4, rho, resp, BS, exu, graz, mort, dvid;
         V TD, RP;                                     /* CarbonMode thermal damage and repair */
+        /* MacroMolecularMode: RNA, and the synthesis rates; PRO, DNA, CH, NST,
+         * PST are the columns Bm, Sz, Cq, Nq, Pq */
+        V RNA, S_PRO, S_DNA, S_RNA;
         std::vector<int64_t> xi, yi, zi, idx;
         std::vector<uint8_t> ac;
     };
@@ -202,12 +206,13 @@ struct Oracle : orc_model {
         nsp = d->n_species; cap = d->capacity;
         if (nsp < 1 || cap < 1) return fail(PI_ERR_ARG, "n_species and capacity must be >= 1");
         mode = d->mode;
-        if (mode != PI_MODE_QUOTA && mode != PI_MODE_CARBON) return fail(PI_ERR_UNSUPPORTED, "physiology mode");
+        if (mode != PI_MODE_QUOTA && mode != PI_MODE_CARBON && mode != PI_MODE_MACROMOLECULAR)
+            return fail(PI_ERR_UNSUPPORTED, "physiology mode");
         P.resize(nsp); dvid_type.resize(nsp);
         for (int s = 0; s < nsp; ++s) {
             for (int q = 0; q < PI_NPARAMS; ++q) P[s][q] = (FT)d->params[s * PI_NPARAMS + q];
             const int dt_idx = mode == PI_MODE_CARBON ? (int)PI_C_dvid_type : (int)PI_P_dvid_type;
-            dvid_type[s] = (int)d->params[s * PI_NPARAMS + dt_idx];
+            dvid_type[s] = mode == PI_MODE_MACROMOLECULAR ? 1 : (int)d->params[s * PI_NPARAMS + dt_idx];
             if (dvid_type[s] < 1 || dvid_type[s] > 3)
                 return fail(PI_ERR_ARG, "Wrong cell division type, must be in 1 to 3");
         }
@@ -219,7 +224,7 @@ struct Oracle : orc_model {
         for (auto& p : sp) {
             for (V* f : {&p.x, &p.y, &p.z, &p.Sz, &p.Bm, &p.Cq, &p.Nq, &p.Pq, &p.Chl, &p.gen, &p.age,
                          &p.PS, &p.VDOC, &p.VNH4, &p.VNO3, &p.VPO4, &p.rho, &p.resp, &p.BS, &p.exu,
-                         &p.graz, &p.mort, &p.dvid, &p.TD, &p.RP})
+                         &p.graz, &p.mort, &p.dvid, &p.TD, &p.RP, &p.RNA, &p.S_PRO, &p.S_DNA, &p.S_RNA})
                 f->assign(cap, (FT)0);
             p.xi.assign(cap, 0); p.yi.assign(cap, 0); p.zi.assign(cap, 0); p.idx.assign(cap, 0);
             p.ac.assign(cap, 0);
@@ -583,6 +588,7 @@ struct Oracle : orc_model {
     /* ---- plankton_update! (Plankton/QuotaMode/plankton_update.jl:1-34) ---- */
     pi_status plankton_update(int s, FT dt) {
         if (mode == PI_MODE_CARBON) return carbon_update(s, dt);
+        if (mode == PI_MODE_MACROMOLECULAR) return mm_update(s, dt);
         Plank& p = sp[s];
         const auto& q = P[s];
         diags_proc(diagSp[s][1], p.graz, p);
@@ -916,6 +922,228 @@ struct Oracle : orc_model {
                 : q[PI_C_mort_P] * shape_dec(p.Sz[i], q[PI_C_mort_reg], L(1.0e-5f), 4);
             p.mort[i] = M::mask(p.ac[i], v);
         }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* get_probability! */
+            float a, b, c;
+            pi_uniform3(seed, iteration, s, PI_KIND_DECISION, i, &a, &b, &c);
+            bool ac = p.ac[i];
+            p.graz[i] = (M::isless((FT)a, p.graz[i] * dt) && ac) ? L(1) : L(0);
+            p.mort[i] = (M::isless((FT)b, p.mort[i] * dt) && ac) ? L(1) : L(0);
+            p.dvid[i] = (M::isless((FT)c, p.dvid[i] * dt) && ac) ? L(1) : L(0);
+        }
+    }
+
+    /* ========================================== MacroMolecularMode ===== */
+    using Q = std::array<FT, PI_NPARAMS>;
+    /* total_C_biomass / total_N_biomass / total_P_biomass
+     * (MacroMolecularMode/plankton_generation.jl:84-95) */
+    static FT m_totC(FT PRO, FT DNA, FT RNA, FT CH, FT Chl) {
+        return PRO + DNA + RNA + CH + Chl / L(893.49f) * L(55.0f);
+    }
+    static FT m_totN(FT PRO, FT DNA, FT RNA, FT NST, FT Chl, const Q& q) {
+        return PRO * q[PI_M_R_NC_PRO] + DNA * q[PI_M_R_NC_DNA] + RNA * q[PI_M_R_NC_RNA] + NST + Chl / L(893.49f) * L(4.0f);
+    }
+    static FT m_totP(FT DNA, FT RNA, FT PST, const Q& q) {
+        return DNA * q[PI_M_R_PC_DNA] + RNA * q[PI_M_R_PC_RNA] + PST;
+    }
+    /* tempFunc_PS / tempFunc (MacroMolecularMode/growth_kernels.jl:1-17) */
+    static FT m_tempFunc_PS(FT T, const Q& q) {
+        FT x = T - q[PI_M_Topt], xmax = q[PI_M_Tmax] - q[PI_M_Topt];
+        FT regT = shape_dec(x, xmax, L(4.0e-2f), 4);
+        FT k = M::exp(-q[PI_M_Ea] / (L(8.3145f) * (T + L(273.15f)))) * regT;
+        k = M::max(L(0), k);
+        FT OGT = M::exp(-q[PI_M_Ea] / (L(8.3145f) * (q[PI_M_Topt] + L(273.15f))));
+        return M::min(L(1), k / OGT);
+    }
+    static FT m_tempFunc(FT T, const Q& q) {
+        FT k = M::exp(-q[PI_M_Ea] / (L(8.3145f) * (T + L(273.15f))));
+        k = M::max(L(0), k);
+        FT OGT = M::exp(-q[PI_M_Ea] / (L(8.3145f) * (q[PI_M_Topt] + L(273.15f))));
+        return M::min(L(1), k / OGT);
+    }
+    /* calc_loss_kernel! (MacroMolecularMode/consume_loss.jl:17-26) */
+    void m_calc_loss(const Plank& p, const V& loss, FT fC, FT fN, FT fP, const Q& q) {
+        const V &PRO = p.Bm, &DNA = p.Sz, &RNA = p.RNA, &CH = p.Cq, &NST = p.Nq, &PST = p.Pq;
+        for (int64_t i = 0; i < cap; ++i) {
+            int64_t c = cell_of(p, i);
+            bool a = p.ac[i];
+            FT C = m_totC(PRO[i], DNA[i], RNA[i], CH[i], p.Chl[i]);
+            FT N = m_totN(PRO[i], DNA[i], RNA[i], NST[i], p.Chl[i], q);
+            FT P_ = m_totP(DNA[i], RNA[i], PST[i], q);
+            plk[PI_DOC][c] += M::mask(a, C * fC) * loss[i];
+            plk[PI_DON][c] += M::mask(a, N * fN) * loss[i];
+            plk[PI_DOP][c] += M::mask(a, P_ * fP) * loss[i];
+            plk[PI_POC][c] += M::mask(a, C * (L(1) - fC)) * loss[i];
+            plk[PI_PON][c] += M::mask(a, N * (L(1) - fN)) * loss[i];
+            plk[PI_POP][c] += M::mask(a, P_ * (L(1) - fP)) * loss[i];
+        }
+    }
+    /* plankton_update! (MacroMolecularMode/plankton_update.jl:1-33) */
+    pi_status mm_update(int s, FT dt) {
+        Plank& p = sp[s];
+        const auto& q = P[s];
+        diags_proc(diagSp[s][1], p.graz, p);
+        m_calc_loss(p, p.graz, q[PI_M_grazFracC], q[PI_M_grazFracN], q[PI_M_grazFracP], q);
+        inactivate(p, p.graz);
+        diags_proc(diagSp[s][2], p.mort, p);
+        m_calc_loss(p, p.mort, q[PI_M_mortFracC], q[PI_M_mortFracN], q[PI_M_mortFracP], q);
+        inactivate(p, p.mort);
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) p.dvid[i] = M::mask(p.ac[i], p.dvid[i]);
+        diags_proc(diagSp[s][3], p.dvid, p);
+        FT dvidnum = 0;
+        for (int64_t i = 0; i < cap; ++i) dvidnum += M::mask(p.ac[i], p.dvid[i]);
+        std::vector<int64_t> free_ind;
+        for (int64_t i = 0; i < cap; ++i) if (!p.ac[i]) free_ind.push_back(i + 1);
+        if (dvidnum > (FT)free_ind.size()) {
+            char buf[160];
+            snprintf(buf, sizeof buf, "number of individual exceeds the capacity at timestep %g days",
+                     (double)t / 86400.0);
+            failed = true;
+            return fail(PI_ERR_CAPACITY, buf);
+        }
+        m_divide(p, free_ind);
+        m_growth(s, dt);
+        calc_consume(p, dt);   /* MacroMolecularMode/consume_loss.jl:2-14: as QuotaMode's */
+        for (int64_t i = 0; i < cap; ++i) diagSp[s][0][cell_of(p, i)] += (FT)((p.ac[i] && p.ac[i]) ? 1 : 0);
+        return PI_OK;
+    }
+    /* divide! (MacroMolecularMode/division_death.jl:28-78): the daughter copies x,
+     * y, z, PRO, DNA, RNA, CH, NST, PST, Chl, gen, ac, dvid; age stays the slot's */
+    void m_divide(Plank& p, const std::vector<int64_t>& free_ind) {
+        V &PRO = p.Bm, &DNA = p.Sz, &RNA = p.RNA, &CH = p.Cq, &NST = p.Nq, &PST = p.Pq;
+        FT acc = 0;
+        for (int64_t i = 0; i < cap; ++i) { acc = acc + p.dvid[i]; idc[i] = acc; }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) idc_int[i] = (int64_t)idc[i];
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i)
+            if (p.dvid[i] == L(1)) p.idx[i] = free_ind[(size_t)(idc_int[i] - 1)];
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {                  /* copy_daughter_individuals_kernel! */
+            if (p.dvid[i] == L(1) && p.idx[i] != 0) {
+                int64_t d = p.idx[i] - 1;
+                p.x[d] = p.x[i]; p.y[d] = p.y[i]; p.z[d] = p.z[i];
+                PRO[d] = PRO[i]; DNA[d] = DNA[i]; RNA[d] = RNA[i]; CH[d] = CH[i]; NST[d] = NST[i]; PST[d] = PST[i];
+                p.Chl[d] = p.Chl[i]; p.gen[d] = p.gen[i]; p.ac[d] = p.ac[i]; p.dvid[d] = p.dvid[i];
+            }
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {                  /* divide_to_half_kernel! */
+            FT h = (L(2) - p.dvid[i]) / L(2);
+            PRO[i] = PRO[i] * h; DNA[i] = DNA[i] * h; RNA[i] = RNA[i] * h; CH[i] = CH[i] * h;
+            NST[i] = NST[i] * h; PST[i] = PST[i] * h; p.Chl[i] = p.Chl[i] * h;
+            p.gen[i] = p.gen[i] + p.dvid[i];
+            p.age[i] = p.age[i] * (L(1) - p.dvid[i]);
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) p.idx[i] = 0;
+    }
+    /* plankton_growth! (MacroMolecularMode/plankton_growth.jl:2-28, growth_kernels.jl:19-200,
+     * division_death_probability.jl:39-44,96-101,258-278), every kernel over every slot */
+    void m_growth(int s, FT dt) {
+        Plank& p = sp[s];
+        const auto& q = P[s];
+        V &PRO = p.Bm, &DNA = p.Sz, &RNA = p.RNA, &CH = p.Cq, &NST = p.Nq, &PST = p.Pq;
+        const FT e30 = L(1.0e-30f), z0 = L(0), one = L(1);
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_inorganic_uptake_kernel! */
+            FT aI = tpar[i] * q[PI_M_alpha] * q[PI_M_Phi];                            /* calc_PS */
+            FT PCm = q[PI_M_PCmax] * m_tempFunc_PS(tT[i], q);
+            FT PS = PCm * (one - M::exp(-aI / M::max(e30, PCm) * p.Chl[i] / M::max(e30, PRO[i]))) * PRO[i];
+            p.PS[i] = M::mask(p.ac[i], PS);
+            FT N_tot = m_totN(PRO[i], DNA[i], RNA[i], NST[i], p.Chl[i], q);           /* calc_NP_uptake */
+            FT P_tot = m_totP(DNA[i], RNA[i], PST[i], q);
+            FT R_NST = NST[i] / M::max(e30, N_tot);
+            FT R_PST = PST[i] / M::max(e30, P_tot);
+            FT regQN = shape_dec(R_NST, q[PI_M_NSTmax], L(1.0e-4f), 4);
+            FT regQP = shape_dec(R_PST, q[PI_M_PSTmax], L(1.0e-4f), 4);
+            FT T = tT[i];
+            bool a = p.ac[i];
+            FT VNH4 = M::mask(a, q[PI_M_VNH4max] * regQN * tNH4[i] / M::max(e30, tNH4[i] + q[PI_M_KsatNH4]) * m_tempFunc(T, q) * PRO[i]);
+            FT VNO3 = M::mask(a, q[PI_M_VNO3max] * regQN * tNO3[i] / M::max(e30, tNO3[i] + q[PI_M_KsatNO3]) * m_tempFunc(T, q) * PRO[i]);
+            FT VPO4 = M::mask(a, q[PI_M_VPO4max] * regQP * tPO4[i] / M::max(e30, tPO4[i] + q[PI_M_KsatPO4]) * m_tempFunc(T, q) * PRO[i]);
+            p.VNH4[i] = M::min(VNH4, tNH4[i] / dt / M::max(one, tpop[i]));
+            p.VNO3[i] = M::min(VNO3, tNO3[i] / dt / M::max(one, tpop[i]));
+            p.VPO4[i] = M::min(VPO4, tPO4[i] / dt / M::max(one, tpop[i]));
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* update_quotas_1_kernel! */
+            CH[i] = CH[i] + p.PS[i] * dt;
+            PST[i] = PST[i] + p.VPO4[i] * dt;
+            NST[i] = NST[i] + (p.VNH4[i] + p.VNO3[i]) * dt;
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_organic_uptake_kernel! (calc_DOC_uptake) */
+            FT C_tot = m_totC(PRO[i], DNA[i], RNA[i], CH[i], p.Chl[i]);
+            FT R_CH = CH[i] / M::max(e30, C_tot);
+            FT regQ = shape_dec(R_CH, q[PI_M_CHmax], L(1.0e-4f), 4);
+            FT VN = q[PI_M_VDOCmax] * regQ * tDOC[i] / M::max(e30, tDOC[i] + q[PI_M_KsatDOC]) * m_tempFunc(tT[i], q) * PRO[i];
+            p.VDOC[i] = M::mask(p.ac[i], M::min(VN, tDOC[i] / dt / M::max(one, tpop[i])));
+            p.VDOC[i] = M::mask(M::isless(L(0.05f), p.PS[i] / (p.VDOC[i] + p.PS[i])), p.VDOC[i]);
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_ρChl_kernel! */
+            FT r = p.PS[i] / M::max(e30, PRO[i]) /
+                   M::max(e30, tpar[i] * q[PI_M_alpha] * q[PI_M_Phi] * p.Chl[i] / M::max(e30, PRO[i]));
+            r = M::mask(M::isless(L(0.1f), tpar[i]), r);
+            p.rho[i] = M::mask(p.ac[i], r);
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i)     /* calc_respir_kernel! */
+            p.resp[i] = M::mask(p.ac[i], q[PI_M_respir] * PRO[i] * m_tempFunc(tT[i], q));
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* update_quotas_2_kernel! */
+            CH[i] = CH[i] + (p.VDOC[i] - p.resp[i]) * dt;
+            PRO[i] = PRO[i] - M::max(z0, (z0 - CH[i]));
+            NST[i] = NST[i] + M::max(z0, (z0 - CH[i])) * q[PI_M_R_NC_PRO];
+            CH[i] = CH[i] + M::max(z0, (z0 - CH[i]));
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_BS_kernel! */
+            const FT Ns = q[PI_M_Nsuper];
+            FT limit_PRO = M::min(CH[i] / (CH[i] + q[PI_M_k_sat_pro] * Ns),
+                                  NST[i] / (NST[i] + q[PI_M_k_sat_pro] * Ns * q[PI_M_R_NC_PRO]));
+            FT limit_DNA = M::min(M::min(CH[i] / (CH[i] + q[PI_M_k_sat_dna] * Ns),
+                                         NST[i] / (NST[i] + q[PI_M_k_sat_dna] * Ns * q[PI_M_R_NC_DNA])),
+                                  PST[i] / (PST[i] + q[PI_M_k_sat_dna] * Ns * q[PI_M_R_PC_DNA]));
+            FT limit_RNA = M::min(M::min(CH[i] / (CH[i] + q[PI_M_k_sat_rna] * Ns),
+                                         NST[i] / (NST[i] + q[PI_M_k_sat_rna] * Ns * q[PI_M_R_NC_RNA])),
+                                  PST[i] / (PST[i] + q[PI_M_k_sat_rna] * Ns * q[PI_M_R_PC_RNA]));
+            FT tf = m_tempFunc(tT[i], q);
+            p.S_PRO[i] = q[PI_M_k_pro] * RNA[i] * limit_PRO * tf;
+            p.S_DNA[i] = M::mask(M::isless(DNA[i] / (q[PI_M_C_DNA] * Ns), L(2)), q[PI_M_k_dna] * PRO[i] * limit_DNA * tf);
+            p.S_RNA[i] = q[PI_M_k_rna] * PRO[i] * limit_RNA * tf;
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* update_biomass_kernel! */
+            PRO[i] = PRO[i] + dt * p.S_PRO[i];
+            DNA[i] = DNA[i] + dt * p.S_DNA[i];
+            RNA[i] = RNA[i] + dt * p.S_RNA[i];
+            CH[i] = CH[i] - dt * (p.S_PRO[i] + p.S_DNA[i] + p.S_RNA[i] + p.S_PRO[i] * p.rho[i]);
+            NST[i] = NST[i] - dt * (p.S_PRO[i] * q[PI_M_R_NC_PRO] + p.S_DNA[i] * q[PI_M_R_NC_DNA] +
+                                    p.S_RNA[i] * q[PI_M_R_NC_RNA] + p.S_PRO[i] * p.rho[i] * L(4.0f) / L(55.0f));
+            PST[i] = PST[i] - dt * (p.S_DNA[i] * q[PI_M_R_PC_DNA] + p.S_RNA[i] * q[PI_M_R_PC_RNA]);
+            p.Chl[i] = p.Chl[i] + dt * p.S_PRO[i] * p.rho[i] * L(893.49f) / L(55.0f);
+            p.age[i] = p.age[i] + M::mask(p.ac[i], dt / L(3600));
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_exudation_kernel! */
+            FT tot_C = m_totC(PRO[i], DNA[i], RNA[i], CH[i], p.Chl[i]);
+            p.exu[i] = M::max(z0, CH[i] - q[PI_M_CHmax] * tot_C);
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i)     /* update_CH_kernel! */
+            CH[i] = CH[i] - p.exu[i];
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i)     /* calc_graz_quadratic_kernel! */
+            p.graz[i] = M::mask(p.ac[i], tpop[i] * q[PI_M_grz_P]);
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i)     /* calc_MM_mort_kernel! */
+            p.mort[i] = M::mask(p.ac[i], q[PI_M_mort_P] * shape_inc(p.age[i], q[PI_M_mort_reg], L(1.0e-4f), 4));
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i)     /* calc_MM_dvid_kernel! */
+            p.dvid[i] = M::mask(M::isless(L(2), DNA[i] / (q[PI_M_C_DNA] * q[PI_M_Nsuper])), q[PI_M_dvid_P]);
         OMP_FOR
         for (int64_t i = 0; i < cap; ++i) {   /* get_probability! */
             float a, b, c;
@@ -1331,6 +1559,7 @@ static pi_status upload_impl(Oracle<FT>* o, int32_t s, const pi_individuals_host
     cp(p.x, h->x); cp(p.y, h->y); cp(p.z, h->z); cp(p.Sz, h->Sz); cp(p.Bm, h->Bm); cp(p.Cq, h->Cq);
     cp(p.Nq, h->Nq); cp(p.Pq, h->Pq); cp(p.Chl, h->Chl); cp(p.gen, h->gen); cp(p.age, h->age);
     cp(p.graz, h->graz); cp(p.mort, h->mort); cp(p.dvid, h->dvid);
+    if (o->mode == PI_MODE_MACROMOLECULAR) cp(p.RNA, h->ex[0]);
     for (int64_t i = 0; i < o->cap; ++i) {
         p.xi[i] = h->xi ? h->xi[i] : 0; p.yi[i] = h->yi ? h->yi[i] : 0; p.zi[i] = h->zi ? h->zi[i] : 0;
         p.ac[i] = h->ac ? (h->ac[i] != 0) : 0;
@@ -1451,6 +1680,7 @@ static pi_status field_impl(Oracle<FT>* o, int32_t s, int32_t f, OUT* out, int64
         case PI_F_Nq: src = &p.Nq; break; case PI_F_Pq: src = &p.Pq; break; case PI_F_Chl: src = &p.Chl; break;
         case PI_F_gen: src = &p.gen; break; case PI_F_age: src = &p.age; break;
         case PI_F_graz: src = &p.graz; break; case PI_F_mort: src = &p.mort; break; case PI_F_dvid: src = &p.dvid; break;
+        case PI_F_ex0: if (o->mode == PI_MODE_MACROMOLECULAR) src = &p.RNA; break;   /* PI_F_RNA */
         default: break;
     }
     if (src) { for (int64_t i = 0; i < n; ++i) out[i] = (OUT)(*src)[i]; return PI_OK; }

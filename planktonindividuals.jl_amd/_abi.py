@@ -25,7 +25,17 @@ PARAM_NAMES = ("Nsuper", "Cquota", "mean", "var", "Chl2Cint", "α", "Φ", "Topt"
 CARBON_PARAM_NAMES = ("Nsuper", "Cquota", "mean", "var", "α", "Φ", "Topt", "Tmax", "Ea", "PCmax", "Chl2C",
                       "respir", "f_T2B", "grz_P", "dvid_type", "dvid_P", "dvid_reg", "dvid_reg2", "mort_P",
                       "mort_reg", "grazFracC", "mortFracC", "thermal", "is_bact")
-MODE_QUOTA, MODE_CARBON = 0, 1
+# MacroMolecularMode parameter order (MacroMolecularMode/plankton_generation.jl:15-21)
+MM_PARAM_NAMES = ("Nsuper", "C_DNA", "var", "PRO2DNA", "RNA2DNA", "CH2DNA", "Chl2DNA",
+                  "α", "Φ", "Topt", "Tmax", "Ea",
+                  "PCmax", "VDOCmax", "VNO3max", "VNH4max", "VPO4max", "KsatDOC", "KsatNH4", "KsatNO3",
+                  "KsatPO4", "CHmax", "NSTmax", "PSTmax",
+                  "Chl2N", "R_NC_PRO", "R_NC_DNA", "R_NC_RNA", "R_PC_DNA", "R_PC_RNA", "respir",
+                  "k_pro", "k_sat_pro", "k_rna", "k_sat_rna", "k_dna", "k_sat_dna",
+                  "dvid_P", "grz_P", "mort_P", "mort_reg", "grazFracC", "grazFracN", "grazFracP",
+                  "mortFracC", "mortFracN", "mortFracP")
+MODE_QUOTA, MODE_CARBON, MODE_MACROMOLECULAR = 0, 1, 2
+NPARAMS = 64                                     # floats per species row (PI_NPARAMS)
 # abiotic particle parameter order (Abiotic/particle_generation.jl:8)
 ABIOTIC_PARAM_NAMES = ("Nsuper", "Rd", "release_P", "sz_min", "Ktr")
 MAX_ABIOTIC = 4                                  # PI_MAXSA
@@ -37,7 +47,15 @@ BGC_NAMES = ("kw", "kc", "kDOC", "Nit", "kDON", "kDOP", "kPOC", "kPON", "kPOP", 
 FLOAT_FIELDS = ("x", "y", "z", "Sz", "Bm", "Cq", "Nq", "Pq", "Chl", "gen", "age",
                 "graz", "mort", "dvid")
 INT_FIELDS = ("xi", "yi", "zi")
-FIELD_ID = {n: i for i, n in enumerate(FLOAT_FIELDS + INT_FIELDS + ("ac",))}
+NEX = 8                                          # extra float columns (PI_NEX)
+EX_FIELDS = tuple(f"ex{k}" for k in range(NEX))
+FIELD_ID = {n: i for i, n in enumerate(FLOAT_FIELDS + INT_FIELDS + ("ac",) + EX_FIELDS)}
+# the columns of the other modes in the ABI's columns (PI_F_Bd, PI_F_PRO, ... of pi_hip.h)
+FIELD_ALIAS = {"Bd": "Cq", "PRO": "Bm", "DNA": "Sz", "RNA": "ex0", "CH": "Cq", "NST": "Nq", "PST": "Pq"}
+# the state columns of each mode's StructArray, in the ABI's names
+MODE_COLUMNS = {MODE_QUOTA: ("Sz", "Bm", "Cq", "Nq", "Pq", "Chl"),
+                MODE_CARBON: ("Sz", "Bm", "Cq", "Chl"),
+                MODE_MACROMOLECULAR: ("Sz", "Bm", "Cq", "Nq", "Pq", "Chl", "ex0")}
 
 GRID_ID = {"par": 0, "par₀": 1, "par0": 1, "Chl": 2, "pop": 3, "temp": 4, "PARF": 5,
            "u": 6, "v": 7, "w": 8, "diag_PAR": 9}
@@ -77,7 +95,7 @@ class ModelDesc(C.Structure):
 
 
 class IndividualsHost(C.Structure):
-    _fields_ = [(n, fp) for n in FLOAT_FIELDS] + [(n, i64p) for n in INT_FIELDS] + [("ac", u8p)]
+    _fields_ = [(n, fp) for n in FLOAT_FIELDS] + [(n, i64p) for n in INT_FIELDS] + [("ac", u8p), ("ex", fp * NEX)]
 
 
 def ptr(a, ctype):
@@ -127,6 +145,9 @@ def individuals_view(arrays):
         setattr(h, n, ptr(a, C.c_int64) if a is not None else None)
     a = arrays.get("ac")
     h.ac = ptr(a, C.c_uint8) if a is not None else None
+    for k, n in enumerate(EX_FIELDS):
+        a = arrays.get(n)
+        h.ex[k] = ptr(a, C.c_float) if a is not None else None
     return h
 
 

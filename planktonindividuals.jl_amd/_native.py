@@ -119,8 +119,9 @@ class Handle:
 
     # ---- uploads
     def upload_individuals(self, sp, arrays):
+        arrays = {_abi.FIELD_ALIAS.get(k, k): v for k, v in arrays.items()}
         keep = {}
-        for n in _abi.FLOAT_FIELDS:
+        for n in _abi.FLOAT_FIELDS + _abi.EX_FIELDS:
             if arrays.get(n) is not None:
                 keep[n] = np.ascontiguousarray(arrays[n], dtype=np.float32)
                 assert keep[n].shape == (self.cap,)
@@ -212,6 +213,7 @@ class Handle:
 
     # ---- downloads
     def field(self, sp, name):
+        name = _abi.FIELD_ALIAS.get(name, name)
         fid = _abi.FIELD_ID[name]
         if name in _abi.INT_FIELDS:
             out = np.empty(self.cap, np.int64)

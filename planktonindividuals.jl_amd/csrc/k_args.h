@@ -221,7 +221,8 @@ enum KId {
     K_SWEEPX0, K_SWEEPX1, K_XY0, K_XY1, K_SWEEPY0, K_SWEEPY1, K_HALOCOPY, K_FILL, K_FINAL0, K_FINAL1, K_ACCUM,
     K_FINISH, K_XYW0, K_XYW1, K_PHYSC, K_FINAL2, K_FINAL3,
     K_ADV_GM, K_SWEEPX2, K_SWEEPX3, K_SWEEPY2, K_SWEEPY3, K_FINAL4, K_FINAL5, K_FINAL6, K_FINAL7,
-    K_ABIO_MARK, K_ABIO_FREE, K_ABIO_LIST, K_ABIO_CHECK, K_ABIO_SPAWN, K_ABIO_ADV, K_ABIO_ADV_GM, K_NKID
+    K_ABIO_MARK, K_ABIO_FREE, K_ABIO_LIST, K_ABIO_CHECK, K_ABIO_SPAWN, K_ABIO_ADV, K_ABIO_ADV_GM,
+    K_PHYSM, K_IDLEM, K_NKID
 };
 /* plan markers (not kernels): fork to / join from the side stream */
 enum { K_FORK = K_NKID, K_JOIN = K_NKID + 1 };
@@ -260,7 +261,17 @@ struct InitArgs {
     int nsp;
     long long N[PI_MAXSP];
     uint64_t seed;
-    int mode;                 /* PI_MODE_QUOTA / PI_MODE_CARBON */
+    int mode;                 /* PI_MODE_QUOTA / PI_MODE_CARBON / PI_MODE_MACROMOLECULAR */
+};
+/* MacroMolecularMode: the growth of the slots that are inactive through a
+ * whole step, whose state keeps changing (biosynthesis is not masked by ac) */
+struct IdleArgs {
+    const DevState* st;
+    SlotTable tab;
+    const float* params;
+    long long cap;
+    int nsp;
+    float dt;
 };
 struct TracerInitArgs {
     DevGrid g;
@@ -280,7 +291,7 @@ void launch_store_to_table(hipStream_t s, DevState* st, const SlotTable& tab, lo
 void launch_rebuild(hipStream_t s, DevState* st, const SlotTable& tab, long long cap, int nsp, uint8_t* sflag);
 void launch_probe_copy(hipStream_t s, void* dst, const void* src, long long n16);
 void launch_table_field(hipStream_t s, const DevState* st, const SlotTable& tab, int sp, long long cap, int field,
-                        const float* params, const DevGrid& g, void* out);
+                        const float* params, const DevGrid& g, void* out, int mode);
 void launch_count(hipStream_t s, const DevState* st, int sp, unsigned long long* out, long long maxn);
 ScanIn scan_plain(const uint32_t* a, uint32_t n);
 ScanIn scan_bits(const uint32_t* D, const uint32_t* A, int nsp, int nw, long long cap);

@@ -802,7 +802,7 @@ __device__ __forceinline__ void k_par_body(const ParArgs& x, Blk K) { k_par_impl
 PI_KERNEL(k_par, ParArgs, 256)
 
 /* ------------------------------------------------------------- physiology */
-struct PState { float x, y, z, Sz, Bm, Cq, Nq, Pq, Chl, gen, age; };
+struct PState { float x, y, z, Sz, Bm, Cq, Nq, Pq, Chl, gen, age; float ex[PI_NEX]; };
 struct Trs { float NH4, NO3, PO4, DOC, DFe, par, dpar, T, pop; };
 struct Rates { float PS, VDOC, VNH4, VNO3, VPO4, resp, BS, exu, rho; };
 
@@ -978,6 +978,111 @@ __device__ void growth_carbon(PState& s, const Trs& t, const float* q, float OGT
     ddvid = jl_islessf(U2, dvid * dt) && ac;
 }
 
+/* MacroMolecularMode plankton_growth! for one individual
+ * (MacroMolecularMode/plankton_growth.jl:2-28, growth_kernels.jl:1-200,
+ * division_death_probability.jl:39-44,96-101,258-278).  Columns: Bm = PRO,
+ * Sz = DNA, ex[0] = RNA, Cq = CH, Nq = NST, Pq = PST.  The rates returned are
+ * PS, VDOC, VNH4, VNO3, VPO4, resp and exu (calc_consume!). */
+__device__ __forceinline__ float ogt_mm(const float* q)
+{
+    return pi_expf(-q[PI_M_Ea] / (8.3145f * (q[PI_M_Topt] + 273.15f)));
+}
+/* total_C_biomass / total_N_biomass / total_P_biomass (plankton_generation.jl:84-95) */
+__device__ __forceinline__ float mm_totC(const PState& s)
+{
+    return s.Bm + s.Sz + s.ex[0] + s.Cq + s.Chl / 893.49f * 55.0f;
+}
+__device__ __forceinline__ float mm_totN(const PState& s, const float* q)
+{
+    return s.Bm * q[PI_M_R_NC_PRO] + s.Sz * q[PI_M_R_NC_DNA] + s.ex[0] * q[PI_M_R_NC_RNA] + s.Nq + s.Chl / 893.49f * 4.0f;
+}
+__device__ __forceinline__ float mm_totP(const PState& s, const float* q)
+{
+    return s.Sz * q[PI_M_R_PC_DNA] + s.ex[0] * q[PI_M_R_PC_RNA] + s.Pq;
+}
+__device__ void growth_mm(PState& s, const Trs& t, const float* q, float OGT, bool ac, float dt,
+                          float U0, float U1, float U2, Rates& r, bool& dgraz, bool& dmort, bool& ddvid)
+{
+    const float e30 = 1.0e-30f;
+    /* tempFunc_PS / tempFunc (growth_kernels.jl:1-17) */
+    const float kx = pi_expf(-q[PI_M_Ea] / (8.3145f * (t.T + 273.15f)));
+    const float x = t.T - q[PI_M_Topt], xmax = q[PI_M_Tmax] - q[PI_M_Topt];
+    const float tps = dminf(1.0f, dmaxf(0.0f, kx * shape_dec(x, xmax, 4.0e-2f, 4)) / OGT);
+    const float tf = dminf(1.0f, dmaxf(0.0f, kx) / OGT);
+    {   /* calc_inorganic_uptake_kernel! (calc_PS, calc_NP_uptake) */
+        const float aI = t.par * q[PI_M_alpha] * q[PI_M_Phi];
+        const float PCm = q[PI_M_PCmax] * tps;
+        r.PS = jl_maskf(ac, PCm * (1.0f - pi_expf(-aI / dmaxf(e30, PCm) * s.Chl / dmaxf(e30, s.Bm))) * s.Bm);
+        const float Rn = s.Nq / dmaxf(e30, mm_totN(s, q));
+        const float Rp = s.Pq / dmaxf(e30, mm_totP(s, q));
+        const float regQN = shape_dec(Rn, q[PI_M_NSTmax], 1.0e-4f, 4);
+        const float regQP = shape_dec(Rp, q[PI_M_PSTmax], 1.0e-4f, 4);
+        const float VNH4 = jl_maskf(ac, q[PI_M_VNH4max] * regQN * t.NH4 / dmaxf(e30, t.NH4 + q[PI_M_KsatNH4]) * tf * s.Bm);
+        const float VNO3 = jl_maskf(ac, q[PI_M_VNO3max] * regQN * t.NO3 / dmaxf(e30, t.NO3 + q[PI_M_KsatNO3]) * tf * s.Bm);
+        const float VPO4 = jl_maskf(ac, q[PI_M_VPO4max] * regQP * t.PO4 / dmaxf(e30, t.PO4 + q[PI_M_KsatPO4]) * tf * s.Bm);
+        const float pp = dmaxf(1.0f, t.pop);
+        r.VNH4 = dminf(VNH4, t.NH4 / dt / pp);
+        r.VNO3 = dminf(VNO3, t.NO3 / dt / pp);
+        r.VPO4 = dminf(VPO4, t.PO4 / dt / pp);
+    }
+    /* update_quotas_1_kernel! */
+    s.Cq = s.Cq + r.PS * dt;
+    s.Pq = s.Pq + r.VPO4 * dt;
+    s.Nq = s.Nq + (r.VNH4 + r.VNO3) * dt;
+    {   /* calc_organic_uptake_kernel! (calc_DOC_uptake) */
+        const float regQ = shape_dec(s.Cq / dmaxf(e30, mm_totC(s)), q[PI_M_CHmax], 1.0e-4f, 4);
+        const float VN = q[PI_M_VDOCmax] * regQ * t.DOC / dmaxf(e30, t.DOC + q[PI_M_KsatDOC]) * tf * s.Bm;
+        r.VDOC = jl_maskf(ac, dminf(VN, t.DOC / dt / dmaxf(1.0f, t.pop)));
+        r.VDOC = jl_maskf(jl_islessf(0.05f, r.PS / (r.VDOC + r.PS)), r.VDOC);
+    }
+    {   /* calc_ρChl_kernel! */
+        float rr = r.PS / dmaxf(e30, s.Bm) / dmaxf(e30, t.par * q[PI_M_alpha] * q[PI_M_Phi] * s.Chl / dmaxf(e30, s.Bm));
+        rr = jl_maskf(jl_islessf(0.1f, t.par), rr);
+        r.rho = jl_maskf(ac, rr);
+    }
+    /* calc_respir_kernel! */
+    r.resp = jl_maskf(ac, q[PI_M_respir] * s.Bm * tf);
+    /* update_quotas_2_kernel!: respiration beyond CH is taken from protein */
+    s.Cq = s.Cq + (r.VDOC - r.resp) * dt;
+    s.Bm = s.Bm - dmaxf(0.0f, (0.0f - s.Cq));
+    s.Nq = s.Nq + dmaxf(0.0f, (0.0f - s.Cq)) * q[PI_M_R_NC_PRO];
+    s.Cq = s.Cq + dmaxf(0.0f, (0.0f - s.Cq));
+    /* calc_BS_kernel! (not masked by ac) */
+    const float kN = q[PI_M_Nsuper];
+    const float lPRO = dminf(s.Cq / (s.Cq + q[PI_M_k_sat_pro] * kN),
+                             s.Nq / (s.Nq + q[PI_M_k_sat_pro] * kN * q[PI_M_R_NC_PRO]));
+    const float lDNA = dminf(dminf(s.Cq / (s.Cq + q[PI_M_k_sat_dna] * kN),
+                                   s.Nq / (s.Nq + q[PI_M_k_sat_dna] * kN * q[PI_M_R_NC_DNA])),
+                             s.Pq / (s.Pq + q[PI_M_k_sat_dna] * kN * q[PI_M_R_PC_DNA]));
+    const float lRNA = dminf(dminf(s.Cq / (s.Cq + q[PI_M_k_sat_rna] * kN),
+                                   s.Nq / (s.Nq + q[PI_M_k_sat_rna] * kN * q[PI_M_R_NC_RNA])),
+                             s.Pq / (s.Pq + q[PI_M_k_sat_rna] * kN * q[PI_M_R_PC_RNA]));
+    const float SPRO = q[PI_M_k_pro] * s.ex[0] * lPRO * tf;
+    const float SDNA = jl_maskf(jl_islessf(s.Sz / (q[PI_M_C_DNA] * kN), 2.0f), q[PI_M_k_dna] * s.Bm * lDNA * tf);
+    const float SRNA = q[PI_M_k_rna] * s.Bm * lRNA * tf;
+    /* update_biomass_kernel! */
+    s.Bm = s.Bm + dt * SPRO;
+    s.Sz = s.Sz + dt * SDNA;
+    s.ex[0] = s.ex[0] + dt * SRNA;
+    s.Cq = s.Cq - dt * (SPRO + SDNA + SRNA + SPRO * r.rho);
+    s.Nq = s.Nq - dt * (SPRO * q[PI_M_R_NC_PRO] + SDNA * q[PI_M_R_NC_DNA] + SRNA * q[PI_M_R_NC_RNA] +
+                        SPRO * r.rho * 4.0f / 55.0f);
+    s.Pq = s.Pq - dt * (SDNA * q[PI_M_R_PC_DNA] + SRNA * q[PI_M_R_PC_RNA]);
+    s.Chl = s.Chl + dt * SPRO * r.rho * 893.49f / 55.0f;
+    s.age = s.age + jl_maskf(ac, dt / 3600.0f);
+    /* calc_exudation_kernel!, update_CH_kernel! (not masked by ac) */
+    r.exu = dmaxf(0.0f, s.Cq - q[PI_M_CHmax] * mm_totC(s));
+    s.Cq = s.Cq - r.exu;
+    r.BS = 0.0f;
+    /* calc_graz_quadratic!, calc_MM_mort!, calc_MM_dvid!, get_probability! */
+    const float graz = jl_maskf(ac, t.pop * q[PI_M_grz_P]);
+    const float mort = jl_maskf(ac, q[PI_M_mort_P] * shape_inc(s.age, q[PI_M_mort_reg], 1.0e-4f, 4));
+    const float dvid = jl_maskf(jl_islessf(2.0f, s.Sz / (q[PI_M_C_DNA] * q[PI_M_Nsuper])), q[PI_M_dvid_P]);
+    dgraz = jl_islessf(U0, graz * dt) && ac;
+    dmort = jl_islessf(U1, mort * dt) && ac;
+    ddvid = jl_islessf(U2, dvid * dt) && ac;
+}
+
 /* k-th set bit (0-based) of a word */
 __device__ __forceinline__ int select_bit(uint32_t w, uint32_t k)
 {
@@ -1007,6 +1112,8 @@ template <int MODE>
 __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
 {
     constexpr bool CARBON = MODE == PI_MODE_CARBON;
+    constexpr bool MM = MODE == PI_MODE_MACROMOLECULAR;
+    constexpr int NEX = mode_nex(MODE);
     const DevState* st = a.st;
     if (st->err) return;
     const DevGrid& g = a.g;
@@ -1031,7 +1138,8 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
     for (int i = threadIdx.x; i < a.nsp * PI_NPARAMS; i += blockDim.x) s_par[i] = a.params[i];
     __syncthreads();
     for (int i = threadIdx.x; i < a.nsp; i += blockDim.x)
-        s_ogt[i] = CARBON ? ogt_carbon(s_par + i * PI_NPARAMS) : ogt_of(s_par + i * PI_NPARAMS);
+        s_ogt[i] = CARBON ? ogt_carbon(s_par + i * PI_NPARAMS) : MM ? ogt_mm(s_par + i * PI_NPARAMS)
+                                                                     : ogt_of(s_par + i * PI_NPARAMS);
     __syncthreads();
     const uint32_t hhh = (uint32_t)gidx(g, g.Hx, g.Hy, g.Hz);
     const uint32_t stride = K.n * blockDim.x;
@@ -1053,13 +1161,14 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
         const int sp = (int)(key >> PI_SLOT_BITS);
         const uint32_t slot = key & PI_SLOT_MASK;
         const float* q = s_par + sp * PI_NPARAMS;
-        const int dvid_type = (int)q[CARBON ? (int)PI_C_dvid_type : (int)PI_P_dvid_type];
+        const int dvid_type = MM ? 1 : (int)q[CARBON ? (int)PI_C_dvid_type : (int)PI_P_dvid_type];
         const float OGT = s_ogt[sp];
         const uint8_t fl = IN.flags[src];
         PState s;
         s.x = IN.x[src]; s.y = IN.y[src]; s.z = IN.z[src];
         s.Sz = IN.Sz[src]; s.Bm = IN.Bm[src]; s.Cq = IN.Cq[src]; s.Nq = IN.Nq[src];
         s.Pq = IN.Pq[src]; s.Chl = IN.Chl[src]; s.gen = IN.gen[src]; s.age = IN.age[src];
+        for (int k = 0; k < NEX; ++k) s.ex[k] = IN.ex[k][src];
         /* find_NPT_kernel! (Individuals/utils.jl:14-30), ac = true at gather time */
         Trs t;
         if (CARBON) {                        /* CarbonMode growth reads only par, T and pop */
@@ -1086,6 +1195,22 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
             const float fC = gz ? q[PI_C_grazFracC] : q[PI_C_mortFracC];
             L0 = s.Bm * fC * 1.0f;
             L3 = s.Bm * (1.0f - fC) * 1.0f;
+            tag = gz ? T_GRAZ : T_MORT;
+            ac = false;
+        } else if (MM && (fl & (F_GRAZ | F_MORT))) {
+            /* calc_loss! of MacroMolecularMode (MacroMolecularMode/consume_loss.jl:17-26):
+             * the C, N and P of all the individual's pools */
+            const bool gz = (fl & F_GRAZ) != 0;
+            const float fC = gz ? q[PI_M_grazFracC] : q[PI_M_mortFracC];
+            const float fN = gz ? q[PI_M_grazFracN] : q[PI_M_mortFracN];
+            const float fP = gz ? q[PI_M_grazFracP] : q[PI_M_mortFracP];
+            const float tC = mm_totC(s), tN = mm_totN(s, q), tP = mm_totP(s, q);
+            L0 = tC * fC * 1.0f;
+            L1 = tN * fN * 1.0f;
+            L2 = tP * fP * 1.0f;
+            L3 = tC * (1.0f - fC) * 1.0f;
+            L4 = tN * (1.0f - fN) * 1.0f;
+            L5 = tP * (1.0f - fP) * 1.0f;
             tag = gz ? T_GRAZ : T_MORT;
             ac = false;
         } else if (fl & (F_GRAZ | F_MORT)) {
@@ -1137,6 +1262,7 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
             s.Sz = s.Sz * h; s.Bm = s.Bm * h; s.Cq = s.Cq * h; s.Nq = s.Nq * h; s.Pq = s.Pq * h; s.Chl = s.Chl * h;
             s.gen = s.gen + 1.0f; s.age = s.age * (1.0f - 1.0f);
             d.Sz = d.Sz * h; d.Bm = d.Bm * h; d.Cq = d.Cq * h; d.Nq = d.Nq * h; d.Pq = d.Pq * h; d.Chl = d.Chl * h;
+            for (int k = 0; k < NEX; ++k) { s.ex[k] = s.ex[k] * h; d.ex[k] = d.ex[k] * h; }
             d.gen = d.gen + 1.0f;
             d.age = a.tab.age[dr] * (1.0f - 1.0f);
             /* The daughter's slot kept what find_inds!/find_NPT! wrote into it this
@@ -1162,6 +1288,7 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
             Rates rd;
             bool gz, mt, dv;
             if (CARBON) growth_carbon(d, tz, q, OGT, true, dt, tmodel, dvid_type, U0, U1, U2, rd, gz, mt, dv);
+            else if (MM) growth_mm(d, tz, q, OGT, true, dt, U0, U1, U2, rd, gz, mt, dv);
             else growth(d, tz, q, OGT, true, dt, tmodel, dvid_type, U0, U1, U2, rd, gz, mt, dv);
             if (reborn) {   /* its calc_consume! lands in that cell, at the slot's place */
                 a.rcons[dr * 5 + 0] = (rd.resp - rd.PS) * dt;
@@ -1176,6 +1303,7 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
             OUT.x[o] = d.x; OUT.y[o] = d.y; OUT.z[o] = d.z;
             OUT.Sz[o] = d.Sz; OUT.Bm[o] = d.Bm; OUT.Cq[o] = d.Cq; OUT.Nq[o] = d.Nq;
             OUT.Pq[o] = d.Pq; OUT.Chl[o] = d.Chl; OUT.gen[o] = d.gen; OUT.age[o] = d.age;
+            for (int k = 0; k < NEX; ++k) OUT.ex[k][o] = d.ex[k];
             const uint8_t dfl = (uint8_t)(F_AC | (gz ? F_GRAZ : 0) | (mt ? F_MORT : 0) | (dv ? F_DVID : 0));
             OUT.flags[o] = dfl;
             OUT.cell[o] = dc;
@@ -1186,9 +1314,11 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
         Rates r;
         bool gz, mt, dv;
         if (CARBON) growth_carbon(s, t, q, OGT, ac, dt, tmodel, dvid_type, U0, U1, U2, r, gz, mt, dv);
+        else if (MM) growth_mm(s, t, q, OGT, ac, dt, U0, U1, U2, r, gz, mt, dv);
         else growth(s, t, q, OGT, ac, dt, tmodel, dvid_type, U0, U1, U2, r, gz, mt, dv);
         if (ac) {
-            /* calc_consume! (QuotaMode/consume_loss.jl:2-14; CarbonMode: DIC only) */
+            /* calc_consume! (QuotaMode/consume_loss.jl:2-14, MacroMolecularMode/consume_loss.jl:2-9;
+             * CarbonMode: DIC only) */
             tag = T_ALIVE | (divider ? T_DIV : 0);
             L0 = (r.resp - r.PS) * dt;
             L1 = CARBON ? 0.0f : (r.exu - r.VDOC) * dt;
@@ -1202,6 +1332,7 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
             a.tab.x[ts] = s.x; a.tab.y[ts] = s.y; a.tab.z[ts] = s.z;
             a.tab.Sz[ts] = s.Sz; a.tab.Bm[ts] = s.Bm; a.tab.Cq[ts] = s.Cq; a.tab.Nq[ts] = s.Nq;
             a.tab.Pq[ts] = s.Pq; a.tab.Chl[ts] = s.Chl; a.tab.gen[ts] = s.gen; a.tab.age[ts] = s.age;
+            for (int k = 0; k < NEX; ++k) a.tab.ex[k][ts] = s.ex[k];
             a.tab.flags[ts] = 0;
             a.tab.cell[ts] = c;
             a.tab.death[ts] = (int32_t)iteration;
@@ -1210,6 +1341,7 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
         OUT.x[e] = s.x; OUT.y[e] = s.y; OUT.z[e] = s.z;
         OUT.Sz[e] = s.Sz; OUT.Bm[e] = s.Bm; OUT.Cq[e] = s.Cq; OUT.Nq[e] = s.Nq;
         OUT.Pq[e] = s.Pq; OUT.Chl[e] = s.Chl; OUT.gen[e] = s.gen; OUT.age[e] = s.age;
+        for (int k = 0; k < NEX; ++k) OUT.ex[k][e] = s.ex[k];
         const uint8_t ofl = ac ? (uint8_t)(F_AC | (gz ? F_GRAZ : 0) | (mt ? F_MORT : 0) | (dv ? F_DVID : 0)) : (uint8_t)0;
         OUT.flags[e] = ofl;
         OUT.cell[e] = c;
@@ -1229,8 +1361,42 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
 }
 __device__ __forceinline__ void k_physq_body(const PhysArgs& a, Blk K) { k_phys_body<PI_MODE_QUOTA>(a, K); }
 __device__ __forceinline__ void k_physc_body(const PhysArgs& a, Blk K) { k_phys_body<PI_MODE_CARBON>(a, K); }
+__device__ __forceinline__ void k_physm_body(const PhysArgs& a, Blk K) { k_phys_body<PI_MODE_MACROMOLECULAR>(a, K); }
 PI_KERNEL3(k_phys, k_physq_body, PhysArgs, 256, xcd_block())
 PI_KERNEL3(k_physc, k_physc_body, PhysArgs, 256, xcd_block())
+PI_KERNEL3(k_physm, k_physm_body, PhysArgs, 256, xcd_block())
+
+/* MacroMolecularMode: the slots inactive through the whole step.  Their
+ * plankton_growth! runs with ac = false and the zeros that find_NPT! gathered
+ * for them, and changes them: protein, DNA and RNA synthesis
+ * (calc_BS_kernel!), update_quotas_2 and the exudation are not masked by ac.
+ * Rows written in this step (an individual dying now, death = this
+ * iteration) already hold their step; live individuals' rows are stale copies
+ * (the store holds them) and evolve to no effect. */
+__device__ __forceinline__ void k_idlem_body(const IdleArgs& a, Blk K)
+{
+    const DevState* st = a.st;
+    if (st->err) return;
+    const int32_t it = (int32_t)(st->iteration + 1);
+    const long long total = a.cap * a.nsp;
+    const SlotTable& T = a.tab;
+    Trs t;
+    t.NH4 = t.NO3 = t.PO4 = t.DOC = t.DFe = t.par = t.dpar = t.T = t.pop = 0.0f;
+    for (long long r = K.b * (long long)blockDim.x + threadIdx.x; r < total; r += (long long)K.n * blockDim.x) {
+        if ((T.flags[r] & F_AC) || T.death[r] >= it) continue;
+        const float* q = a.params + (r / a.cap) * PI_NPARAMS;
+        PState s;
+        s.x = s.y = s.z = s.gen = 0.0f;
+        s.Sz = T.Sz[r]; s.Bm = T.Bm[r]; s.Cq = T.Cq[r]; s.Nq = T.Nq[r]; s.Pq = T.Pq[r];
+        s.Chl = T.Chl[r]; s.age = T.age[r]; s.ex[0] = T.ex[0][r];
+        Rates rr;
+        bool gz, mt, dv;
+        growth_mm(s, t, q, ogt_mm(q), false, a.dt, 1.0f, 1.0f, 1.0f, rr, gz, mt, dv);
+        T.Sz[r] = s.Sz; T.Bm[r] = s.Bm; T.Cq[r] = s.Cq; T.Nq[r] = s.Nq; T.Pq[r] = s.Pq;
+        T.Chl[r] = s.Chl; T.age[r] = s.age; T.ex[0][r] = s.ex[0];
+    }
+}
+PI_KERNEL(k_idlem, IdleArgs, 256)
 
 /* Ordered per-cell sums: calc_loss! (graz), calc_loss! (mort), calc_consume! per
  * species in slot order (plankton_update.jl:5-29), and the per-species
@@ -1735,6 +1901,8 @@ void reg_particles(KInfo* k)
     PI_REG(k, K_PAR, k_par, ParArgs, 256, true);
     PI_REG(k, K_PHYS, k_phys, PhysArgs, 256, true);
     PI_REG(k, K_PHYSC, k_physc, PhysArgs, 256, true);
+    PI_REG(k, K_PHYSM, k_physm, PhysArgs, 256, true);
+    PI_REG(k, K_IDLEM, k_idlem, IdleArgs, 256, true);
     PI_REG(k, K_PLK, k_plk, PlkArgs, 256, true);
     PI_REG(k, K_PLK_LONG, k_plk_long, PlkArgs, 256, true);
 }

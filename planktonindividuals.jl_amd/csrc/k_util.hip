@@ -219,6 +219,8 @@ __global__ void k_store_to_table(DevState* st, SlotTable tab, long long cap)
         tab.x[r] = S.x[e]; tab.y[r] = S.y[e]; tab.z[r] = S.z[e];
         tab.Sz[r] = S.Sz[e]; tab.Bm[r] = S.Bm[e]; tab.Cq[r] = S.Cq[e]; tab.Nq[r] = S.Nq[e];
         tab.Pq[r] = S.Pq[e]; tab.Chl[r] = S.Chl[e]; tab.gen[r] = S.gen[e]; tab.age[r] = S.age[e];
+        for (int k = 0; k < PI_NEX; ++k)
+            if (tab.ex[k]) tab.ex[k][r] = S.ex[k][e];
         tab.flags[r] = fl;
         tab.cell[r] = S.cell[e];
         tab.death[r] = (int32_t)st->iteration;
@@ -240,6 +242,8 @@ __global__ void k_rebuild(DevState* st, SlotTable tab, long long cap, int nsp, u
         S.x[e] = tab.x[r]; S.y[e] = tab.y[r]; S.z[e] = tab.z[r];
         S.Sz[e] = tab.Sz[r]; S.Bm[e] = tab.Bm[r]; S.Cq[e] = tab.Cq[r]; S.Nq[e] = tab.Nq[r];
         S.Pq[e] = tab.Pq[r]; S.Chl[e] = tab.Chl[r]; S.gen[e] = tab.gen[r]; S.age[e] = tab.age[r];
+        for (int k = 0; k < PI_NEX; ++k)
+            if (tab.ex[k]) S.ex[k][e] = tab.ex[k][r];
         S.flags[e] = fl;
         S.cell[e] = tab.cell[r];
     }
@@ -249,11 +253,13 @@ __global__ void k_rebuild(DevState* st, SlotTable tab, long long cap, int nsp, u
  * Rows of inactive slots written before the current iteration have gone
  * through at least one step while inactive, whose fixed point is: x = y = z = 0
  * (boundaries * ac), xi = yi = zi = H (find_inds), Sz = Bm / (Cquota * Nsuper)
- * (update_cellsize! runs on every slot) and zero decisions; every other field
- * is unchanged by a step with ac = false. */
+ * (update_cellsize! runs on every slot; QuotaMode and CarbonMode) and zero
+ * decisions; every other field is unchanged by a step with ac = false, except
+ * in MacroMolecularMode, whose inactive rows k_idlem steps every step. */
 __global__ void k_table_field(const DevState* st, SlotTable tab, int sp, long long cap, int field,
-                              const float* params, DevGrid g, void* out)
+                              const float* params, DevGrid g, void* out, int mode)
 {
+    const bool mm = mode == PI_MODE_MACROMOLECULAR;
     const long long it = st->iteration;
     const float* q = params + sp * PI_NPARAMS;
     for (long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x; s < cap; s += (long long)gridDim.x * blockDim.x) {
@@ -266,7 +272,7 @@ __global__ void k_table_field(const DevState* st, SlotTable tab, int sp, long lo
             case PI_F_x: v = settled ? 0.0f : tab.x[r]; break;
             case PI_F_y: v = settled ? 0.0f : tab.y[r]; break;
             case PI_F_z: v = settled ? 0.0f : tab.z[r]; break;
-            case PI_F_Sz: v = settled ? tab.Bm[r] / (q[PI_P_Cquota] * q[PI_P_Nsuper]) : tab.Sz[r]; break;
+            case PI_F_Sz: v = (settled && !mm) ? tab.Bm[r] / (q[PI_P_Cquota] * q[PI_P_Nsuper]) : tab.Sz[r]; break;
             case PI_F_Bm: v = tab.Bm[r]; break;
             case PI_F_Cq: v = tab.Cq[r]; break;
             case PI_F_Nq: v = tab.Nq[r]; break;
@@ -277,9 +283,11 @@ __global__ void k_table_field(const DevState* st, SlotTable tab, int sp, long lo
             case PI_F_graz: v = (!settled && (fl & F_GRAZ)) ? 1.0f : 0.0f; break;
             case PI_F_mort: v = (!settled && (fl & F_MORT)) ? 1.0f : 0.0f; break;
             case PI_F_dvid: v = (!settled && (fl & F_DVID)) ? 1.0f : 0.0f; break;
-            default: break;
+            default:
+                if (field >= PI_F_ex0 && tab.ex[field - PI_F_ex0]) v = tab.ex[field - PI_F_ex0][r];
+                break;
         }
-        if (field <= PI_F_dvid) { ((float*)out)[s] = v; continue; }
+        if (field <= PI_F_dvid || field >= PI_F_ex0) { ((float*)out)[s] = v; continue; }
         if (field == PI_F_ac) { ((uint8_t*)out)[s] = act ? 1 : 0; continue; }
         long long xi, yi, zi;
         if (settled) { xi = g.Hx; yi = g.Hy; zi = g.Hz; }
@@ -305,7 +313,8 @@ __global__ void k_count(const DevState* st, int sp, unsigned long long* out)
 }
 
 /* -------------------------------------------------- initial state on device
- * initialize_plankton! (QuotaMode/plankton_generation.jl:35-72) for every slot
+ * initialize_plankton! (QuotaMode/plankton_generation.jl:35-72; CarbonMode and
+ * MacroMolecularMode alike, plankton_generation.jl of each) for every slot
  * of every species, written into the slot table like an upload of the host
  * arrays (ModelSpec in model.py computes the same values on the host with the
  * same stream and the same binary64 log/cos). */
@@ -322,18 +331,34 @@ __global__ void __launch_bounds__(256) k_init_individuals(InitArgs a)
         pi_uniform3(a.seed, 0, sp, PI_KIND_INIT_POS, (uint64_t)i, &ux, &uy, &uz);
         pi_uniform3(a.seed, 0, sp, PI_KIND_INIT_QUOTA, (uint64_t)i, &uc, &un, &up);
         pi_uniform3(a.seed, 0, sp, PI_KIND_INIT_SIZE, (uint64_t)i, &ua, &ub, &u3);
-        const float Sz0 = pi_init_normal(ua, ub);
         bool ac = i < a.N[sp];
         const float x = jl_maskf(ac, ux * (float)g.Nx);
         const float y = jl_maskf(ac, uy * (float)g.Ny);
         const float z = jl_maskf(ac, uz * (float)g.Nz);
-        const float Sz = jl_maskf(ac, jl_maxf(1.0f, Sz0 * q[PI_P_var] + q[PI_P_mean]));
-        const float Bm = (q[PI_P_Cquota] * Sz) * q[PI_P_Nsuper];     /* PI_C_Cquota, PI_C_Nsuper alike */
-        const bool carbon = a.mode == PI_MODE_CARBON;   /* CarbonMode/plankton_generation.jl:34-56: Bd = 0 */
-        const float Cq = carbon ? 0.0f : (uc * q[PI_P_Cqmax]) * Bm;
-        const float Nq = carbon ? 0.0f : un * (q[PI_P_Nqmax] * (Bm + Cq) - Bm * q[PI_P_R_NC]);
-        const float Pq = carbon ? 0.0f : up * (q[PI_P_Pqmax] * (Bm + Cq) - Bm * q[PI_P_R_PC]);
-        const float Chl = Bm * q[carbon ? (int)PI_C_Chl2C : (int)PI_P_Chl2Cint];
+        float Sz, Bm, Cq, Nq, Pq, Chl, RNA = 0.0f;
+        if (a.mode == PI_MODE_MACROMOLECULAR) {
+            /* MacroMolecularMode/plankton_generation.jl:31-78: DNA, RNA and protein from
+             * one draw triple, CH, NST and PST from another; Sz = DNA, Bm = PRO,
+             * Cq = CH, Nq = NST, Pq = PST */
+            const float var = q[PI_M_var], cN = q[PI_M_C_DNA], Ns = q[PI_M_Nsuper];
+            const float DNA = jl_maskf(ac, (ua * var + 1.0f) * cN * Ns);
+            RNA = jl_maskf(ac, (ub * var * 2.0f + 1.0f - var) * cN * Ns) * q[PI_M_RNA2DNA];
+            Bm = jl_maskf(ac, (u3 * var * 2.0f + 1.0f - var) * cN * Ns) * q[PI_M_PRO2DNA];
+            Cq = (uc * var * 2.0f + 1.0f - var) * DNA * q[PI_M_CH2DNA];
+            Nq = (un * var * 2.0f + 1.0f - var) * Cq * 16.0f / 106.0f;
+            Pq = (up * var * 2.0f + 1.0f - var) * Cq / 106.0f;
+            Chl = DNA * q[PI_M_Chl2DNA] * 893.49f / 55.0f;
+            Sz = DNA;
+        } else {
+            const float Sz0 = pi_init_normal(ua, ub);
+            Sz = jl_maskf(ac, jl_maxf(1.0f, Sz0 * q[PI_P_var] + q[PI_P_mean]));
+            Bm = (q[PI_P_Cquota] * Sz) * q[PI_P_Nsuper];     /* PI_C_Cquota, PI_C_Nsuper alike */
+            const bool carbon = a.mode == PI_MODE_CARBON;   /* CarbonMode/plankton_generation.jl:34-56: Bd = 0 */
+            Cq = carbon ? 0.0f : (uc * q[PI_P_Cqmax]) * Bm;
+            Nq = carbon ? 0.0f : un * (q[PI_P_Nqmax] * (Bm + Cq) - Bm * q[PI_P_R_NC]);
+            Pq = carbon ? 0.0f : up * (q[PI_P_Pqmax] * (Bm + Cq) - Bm * q[PI_P_R_PC]);
+            Chl = Bm * q[carbon ? (int)PI_C_Chl2C : (int)PI_P_Chl2Cint];
+        }
         if (ac && g.has_land) {   /* mask_individuals! (Individuals/utils.jl:89-100) */
             const int xi = (int)truncf(x + 1.0f) + g.Hx, yi = (int)truncf(y + 1.0f) + g.Hy, zi = (int)truncf(z + 1.0f) + g.Hz;
             ac = g.lm[gidx(g, xi, yi, zi)] != 0;
@@ -341,6 +366,7 @@ __global__ void __launch_bounds__(256) k_init_individuals(InitArgs a)
         a.tab.x[r] = x; a.tab.y[r] = y; a.tab.z[r] = z;
         a.tab.Sz[r] = Sz; a.tab.Bm[r] = Bm; a.tab.Cq[r] = Cq; a.tab.Nq[r] = Nq; a.tab.Pq[r] = Pq;
         a.tab.Chl[r] = Chl;
+        if (a.tab.ex[0]) a.tab.ex[0][r] = RNA;
         a.tab.gen[r] = i < a.N[sp] ? 1.0f : 0.0f;
         a.tab.age[r] = 0.0f;
         a.tab.flags[r] = (uint8_t)((ac ? F_AC : 0) | F_ZERO);
@@ -417,9 +443,10 @@ void launch_probe_copy(hipStream_t s, void* dst, const void* src, long long n16)
     hipLaunchKernelGGL(k_probe_copy, dim3(grid_for(n16, 256, 8192)), dim3(256), 0, s, (uint4*)dst, (const uint4*)src, n16);
 }
 void launch_table_field(hipStream_t s, const DevState* st, const SlotTable& tab, int sp, long long cap, int field,
-                        const float* params, const DevGrid& g, void* out)
+                        const float* params, const DevGrid& g, void* out, int mode)
 {
-    hipLaunchKernelGGL(k_table_field, dim3(grid_for(cap, 256, 8192)), dim3(256), 0, s, st, tab, sp, cap, field, params, g, out);
+    hipLaunchKernelGGL(k_table_field, dim3(grid_for(cap, 256, 8192)), dim3(256), 0, s, st, tab, sp, cap, field, params, g,
+                       out, mode);
 }
 void launch_count(hipStream_t s, const DevState* st, int sp, unsigned long long* out, long long maxn)
 {

@@ -80,7 +80,10 @@ struct Store {
     float *x, *y, *z, *Sz, *Bm, *Cq, *Nq, *Pq, *Chl, *gen, *age;
     uint8_t* flags;
     uint32_t* cell;
+    float* ex[PI_NEX];           /* extra state columns of the modes that have them (else null) */
 };
+/* extra state columns per individual of a physiology mode */
+__host__ __device__ constexpr int mode_nex(int mode) { return mode == PI_MODE_MACROMOLECULAR ? 1 : 0; }
 
 /* Pointers read from device memory (the DevState) are generic pointers, and
  * accesses through them compile to FLAT instructions (64-bit addresses, both
@@ -114,6 +117,7 @@ struct GStore {
     PI_GLOBAL float *x, *y, *z, *Sz, *Bm, *Cq, *Nq, *Pq, *Chl, *gen, *age;
     PI_GLOBAL uint8_t* flags;
     PI_GLOBAL uint32_t* cell;
+    PI_GLOBAL float* ex[PI_NEX];
 };
 __device__ __forceinline__ GStore gstore(const Store& s)
 {
@@ -121,6 +125,7 @@ __device__ __forceinline__ GStore gstore(const Store& s)
     g.key = gp(s.key); g.x = gp(s.x); g.y = gp(s.y); g.z = gp(s.z); g.Sz = gp(s.Sz); g.Bm = gp(s.Bm);
     g.Cq = gp(s.Cq); g.Nq = gp(s.Nq); g.Pq = gp(s.Pq); g.Chl = gp(s.Chl); g.gen = gp(s.gen); g.age = gp(s.age);
     g.flags = gp(s.flags); g.cell = gp(s.cell);
+    for (int k = 0; k < PI_NEX; ++k) g.ex[k] = gp(s.ex[k]);
     return g;
 }
 
@@ -129,6 +134,7 @@ struct SlotTable {               /* per species, stride cap */
     uint8_t* flags;
     uint32_t* cell;
     int32_t* death;              /* iteration at which the row was last written */
+    float* ex[PI_NEX];
 };
 
 struct Contrib {

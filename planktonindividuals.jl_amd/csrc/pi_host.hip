@@ -368,11 +368,13 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     /* a step's output holds the live individuals of its input (those dying
      * this step stay as inactive entries) plus the daughters: <= 2 x capacity */
     const long long scap = 2 * m->ctot;
+    const int nex = mode_nex(d->mode);   /* extra state columns of the mode */
     for (int b = 0; b < 2; ++b) {
         Store& S = hs.store[b];
         DALLOC(S.key, scap * 4);
         for (float** f : {&S.x, &S.y, &S.z, &S.Sz, &S.Bm, &S.Cq, &S.Nq, &S.Pq, &S.Chl, &S.gen, &S.age})
             DALLOC(*f, scap * 4);
+        for (int k = 0; k < nex; ++k) DALLOC(S.ex[k], scap * 4);
         DALLOC(S.flags, scap);
         DALLOC(S.cell, scap * 4);
     }
@@ -380,6 +382,7 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     SlotTable& T = m->tab;
     for (float** f : {&T.x, &T.y, &T.z, &T.Sz, &T.Bm, &T.Cq, &T.Nq, &T.Pq, &T.Chl, &T.gen, &T.age})
         DALLOC(*f, m->ctot * 4);
+    for (int k = 0; k < nex; ++k) DALLOC(T.ex[k], m->ctot * 4);
     DALLOC(T.flags, m->ctot);
     DALLOC(T.cell, m->ctot * 4);
     DALLOC(T.death, m->ctot * 4);
@@ -471,8 +474,8 @@ static pi_status check_desc(const pi_model_desc* d)
     if (gd.topo_z != PI_BOUNDED) return fail(PI_ERR_ARG, "grid: z topology must be Bounded");
     if (d->n_species < 1 || d->n_species > PI_MAXSP) return fail(PI_ERR_ARG, "n_species out of range");
     if (d->capacity < 1 || d->capacity > (long long)PI_SLOT_MASK) return fail(PI_ERR_ARG, "capacity out of range");
-    if (d->mode != PI_MODE_QUOTA && d->mode != PI_MODE_CARBON)
-        return fail(PI_ERR_UNSUPPORTED, "physiology mode: QuotaMode and CarbonMode are implemented");
+    if (d->mode != PI_MODE_QUOTA && d->mode != PI_MODE_CARBON && d->mode != PI_MODE_MACROMOLECULAR)
+        return fail(PI_ERR_UNSUPPORTED, "physiology mode: QuotaMode, CarbonMode and MacroMolecularMode are implemented");
     if (d->metrics) {
         const pi_metrics& mt = *d->metrics;
         if (!mt.dxC || !mt.dyC || !mt.dxF || !mt.dyF || !mt.Az || !mt.Ax || !mt.Ay || !mt.Vol)
@@ -489,7 +492,7 @@ static pi_status check_desc(const pi_model_desc* d)
             return fail(PI_ERR_ARG, "abiotic particles: species x cells and species x capacity must stay below 2^31");
     }
     const int dvt = d->mode == PI_MODE_CARBON ? (int)PI_C_dvid_type : (int)PI_P_dvid_type;
-    for (int s = 0; s < d->n_species; ++s) {
+    for (int s = 0; s < d->n_species && d->mode != PI_MODE_MACROMOLECULAR; ++s) {   /* MM: no dvid_type */
         float t = d->params[s * PI_NPARAMS + dvt];
         if (!(t == 1.0f || t == 2.0f || t == 3.0f)) return fail(PI_ERR_ARG, "Wrong cell division type, must be in 1 to 3");
     }
@@ -636,9 +639,11 @@ pi_status pi_upload_individuals(pi_model* m, int32_t sp, const pi_individuals_ho
         flags[i] = f;
     }
     const SlotTable& T = m->tab;
-    const float* src[11] = {h->x, h->y, h->z, h->Sz, h->Bm, h->Cq, h->Nq, h->Pq, h->Chl, h->gen, h->age};
-    float* dst[11] = {T.x, T.y, T.z, T.Sz, T.Bm, T.Cq, T.Nq, T.Pq, T.Chl, T.gen, T.age};
-    for (int k = 0; k < 11; ++k) {
+    const float* src[11 + PI_NEX] = {h->x, h->y, h->z, h->Sz, h->Bm, h->Cq, h->Nq, h->Pq, h->Chl, h->gen, h->age};
+    float* dst[11 + PI_NEX] = {T.x, T.y, T.z, T.Sz, T.Bm, T.Cq, T.Nq, T.Pq, T.Chl, T.gen, T.age};
+    const int nex = mode_nex(m->mode);
+    for (int k = 0; k < nex; ++k) { src[11 + k] = h->ex[k]; dst[11 + k] = T.ex[k]; }
+    for (int k = 0; k < 11 + nex; ++k) {
         if (src[k]) HIPCHECK(hipMemcpy(dst[k] + off, src[k], cap * 4, hipMemcpyHostToDevice));
         else HIPCHECK(hipMemset(dst[k] + off, 0, cap * 4));
     }
@@ -1083,6 +1088,10 @@ static void build_step(pi_model* m, float dt, Plan& P)
         const long long phb = grid_for(2 * m->ctot, 256, (int)m->shape[PI_S_PHYS_BLOCKS]);
         P.add(K_PHYS, "phys", m->mode == PI_MODE_QUOTA ? phb : 0, pa);
         P.add(K_PHYSC, "phys", m->mode == PI_MODE_CARBON ? phb : 0, pa);
+        P.add(K_PHYSM, "phys", m->mode == PI_MODE_MACROMOLECULAR ? phb : 0, pa);
+        IdleArgs ia = zeroed<IdleArgs>();
+        ia.st = st; ia.tab = m->tab; ia.params = m->d_params; ia.cap = m->cap; ia.nsp = m->nsp; ia.dt = dt;
+        P.add(K_IDLEM, "phys", m->mode == PI_MODE_MACROMOLECULAR ? grid_for(m->ctot, 256, 8192) : 0, ia);
     }
     {
         PlkArgs ka = zeroed<PlkArgs>();
@@ -1257,6 +1266,7 @@ pi_status pi_download_field(pi_model* m, int32_t sp, int32_t field, void* host, 
 {
     if (!m || !host) return fail(PI_ERR_ARG, "null argument");
     if (sp < 0 || sp >= m->nsp || field < 0 || field >= PI_NFIELDS) return fail(PI_ERR_ARG, "field id");
+    if (field >= PI_F_ex0 && field - PI_F_ex0 >= mode_nex(m->mode)) return fail(PI_ERR_ARG, "field id: not a column of this mode");
     if (n != m->cap) return fail(PI_ERR_ARG, "n must equal capacity");
     pi_status s = rebuild_if_dirty(m);
     if (s != PI_OK) return s;
@@ -1265,7 +1275,7 @@ pi_status pi_download_field(pi_model* m, int32_t sp, int32_t field, void* host, 
     launch_store_to_table(m->stream, m->d_state, m->tab, m->cap, 2 * m->ctot);
     const size_t es = (field >= PI_F_xi && field <= PI_F_zi) ? 8 : field == PI_F_ac ? 1 : 4;
     void* tmp = (void*)m->skey;   /* scratch, 8*ctot bytes >= 8*cap */
-    launch_table_field(m->stream, m->d_state, m->tab, sp, m->cap, field, m->d_params, m->g, tmp);
+    launch_table_field(m->stream, m->d_state, m->tab, sp, m->cap, field, m->d_params, m->g, tmp, m->mode);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(host, tmp, es * m->cap, hipMemcpyDeviceToHost, m->stream));
     HIPCHECK(hipStreamSynchronize(m->stream));

@@ -95,14 +95,20 @@ def initialize_plankton(p, N, cap, grid, seed, sp, mode=_abi.MODE_QUOTA):
     CarbonMode/plankton_generation.jl:34-56) with the draws of `randn!`/`rand!`
     taken from the shared stream (the reference's generator is entropy-seeded).
     `p` is the species' parameter row.  CarbonMode: Bm and Chl from the size,
-    the damaged biomass Bd (in the Cq column) zero, no N or P quotas."""
+    the damaged biomass Bd (in the Cq column) zero, no N or P quotas.
+    MacroMolecularMode (MacroMolecularMode/plankton_generation.jl:31-78): DNA,
+    RNA and protein from the third draw triple, CH, NST and PST from the second;
+    returned in the ABI's columns (Sz = DNA, Bm = PRO, Cq = CH, Nq = NST,
+    Pq = PST, ex0 = RNA)."""
     f32 = np.float32
     carbon = mode == _abi.MODE_CARBON
-    q = {n: p[i] for i, n in enumerate(_abi.CARBON_PARAM_NAMES if carbon else _abi.PARAM_NAMES)}
+    mm = mode == _abi.MODE_MACROMOLECULAR
+    names = _abi.CARBON_PARAM_NAMES if carbon else _abi.MM_PARAM_NAMES if mm else _abi.PARAM_NAMES
+    q = {n: p[i] for i, n in enumerate(names)}
     slots = np.arange(cap, dtype=np.int64)
     ux, uy, uz = stream.uniform3(seed, 0, sp, stream.KIND_INIT_POS, slots)
     uc, un, up = stream.uniform3(seed, 0, sp, stream.KIND_INIT_QUOTA, slots)
-    a, b, _ = stream.uniform3(seed, 0, sp, stream.KIND_INIT_SIZE, slots)
+    a, b, u3 = stream.uniform3(seed, 0, sp, stream.KIND_INIT_SIZE, slots)
     # randn via Box-Muller on (0,1] x [0,1), with the binary64 log and cos of
     # include/pi_math.h (the device generates the same values)
     Sz = stream.init_normal(a, b)
@@ -114,18 +120,30 @@ def initialize_plankton(p, N, cap, grid, seed, sp, mode=_abi.MODE_QUOTA):
     x = _mask(ac, ux * f32(grid.Nx))
     y = _mask(ac, uy * f32(grid.Ny))
     z = _mask(ac, uz * f32(grid.Nz))
-    Sz = _mask(ac, np.maximum(f32(1.0), Sz * q["var"] + q["mean"]))
-    Bm = (q["Cquota"] * Sz) * q["Nsuper"]
-    if carbon:
-        Cq = np.zeros(cap, f32)
-        Nq = np.zeros(cap, f32)
-        Pq = np.zeros(cap, f32)
-        Chl = Bm * q["Chl2C"]
+    extra = {}
+    if mm:
+        var, cN, Ns = q["var"], q["C_DNA"], q["Nsuper"]
+        spread = lambda u: u * var * f32(2) + f32(1) - var      # rand * var * 2 + 1 - var
+        Sz = _mask(ac, (a * var + f32(1)) * cN * Ns)             # DNA
+        extra["ex0"] = (_mask(ac, spread(b) * cN * Ns) * q["RNA2DNA"]).astype(f32)
+        Bm = _mask(ac, spread(u3) * cN * Ns) * q["PRO2DNA"]     # PRO
+        Cq = spread(uc) * Sz * q["CH2DNA"]                      # CH
+        Nq = spread(un) * Cq * f32(16) / f32(106)               # NST
+        Pq = spread(up) * Cq / f32(106)                         # PST
+        Chl = Sz * q["Chl2DNA"] * f32(893.49) / f32(55)
     else:
-        Cq = (uc * q["Cqmax"]) * Bm
-        Nq = un * (q["Nqmax"] * (Bm + Cq) - Bm * q["R_NC"])
-        Pq = up * (q["Pqmax"] * (Bm + Cq) - Bm * q["R_PC"])
-        Chl = Bm * q["Chl2Cint"]
+        Sz = _mask(ac, np.maximum(f32(1.0), Sz * q["var"] + q["mean"]))
+        Bm = (q["Cquota"] * Sz) * q["Nsuper"]
+        if carbon:
+            Cq = np.zeros(cap, f32)
+            Nq = np.zeros(cap, f32)
+            Pq = np.zeros(cap, f32)
+            Chl = Bm * q["Chl2C"]
+        else:
+            Cq = (uc * q["Cqmax"]) * Bm
+            Nq = un * (q["Nqmax"] * (Bm + Cq) - Bm * q["R_NC"])
+            Pq = up * (q["Pqmax"] * (Bm + Cq) - Bm * q["R_PC"])
+            Chl = Bm * q["Chl2Cint"]
     # mask_individuals! over the first N slots (Individuals/utils.jl:89-100)
     Hx, Hy, Hz = grid.Hx, grid.Hy, grid.Hz
     n = slice(0, N)
@@ -134,10 +152,10 @@ def initialize_plankton(p, N, cap, grid, seed, sp, mode=_abi.MODE_QUOTA):
     zi = np.trunc(_mask(ac[n], z[n] + f32(1))).astype(np.int64) + Hz
     ac[n] = grid.landmask[xi - 1, yi - 1, zi - 1] & ac[n]
     zeros = np.zeros(cap, f32)
-    return dict(x=x, y=y, z=z, Sz=Sz, Bm=Bm.astype(f32), Cq=Cq.astype(f32), Nq=Nq.astype(f32),
+    return dict(x=x, y=y, z=z, Sz=Sz.astype(f32), Bm=Bm.astype(f32), Cq=Cq.astype(f32), Nq=Nq.astype(f32),
                 Pq=Pq.astype(f32), Chl=Chl.astype(f32), gen=gen, age=age, graz=zeros.copy(),
                 mort=zeros.copy(), dvid=zeros.copy(), xi=np.zeros(cap, np.int64),
-                yi=np.zeros(cap, np.int64), zi=np.zeros(cap, np.int64), ac=ac)
+                yi=np.zeros(cap, np.int64), zi=np.zeros(cap, np.int64), ac=ac, **extra)
 
 
 # ------------------------------------------------------------------- views
@@ -152,19 +170,26 @@ class _SpeciesData:
             raise AttributeError(name)
         if name == "idx":
             return np.zeros(self._model.max_individuals, np.int64)
-        if self._model.spec.mode_id == _abi.MODE_CARBON:
+        mode = self._model.spec.mode_id
+        if mode == _abi.MODE_CARBON:
             # CarbonMode columns (CarbonMode/plankton_generation.jl:2-12): Bd is the Cq column
             if name in ("Cq", "Nq", "Pq"):
                 raise AttributeError(name)
             if name == "Bd":
                 name = "Cq"
+        elif mode == _abi.MODE_MACROMOLECULAR:
+            # MacroMolecularMode columns (MacroMolecularMode/plankton_generation.jl:2-12)
+            if name in ("Sz", "Bm", "Cq", "Nq", "Pq", "Bd") or name in _abi.EX_FIELDS:
+                raise AttributeError(name)
+        elif name in _abi.FIELD_ALIAS or name in _abi.EX_FIELDS:
+            raise AttributeError(name)
         return self._model._native.field(self._sp, name)
 
 
 class _Species:
     def __init__(self, model, sp):
         self.data = _SpeciesData(model, sp)
-        self.p = dict(zip(_abi.PARAM_NAMES, model.species_params[sp]))
+        self.p = dict(zip(P.param_names(model.mode), model.species_params[sp]))
 
 
 class _Phytos(dict):
@@ -256,8 +281,9 @@ class ModelSpec:
                  bgc_params=None, tracer_initial=None, phyto=None, abiotic=None, t=0.0,
                  max_candidates=25, seed=0, host_state=True):
         mode = mode if mode is not None else P.QuotaMode()
-        if not isinstance(mode, (P.QuotaMode, P.CarbonMode)):
-            raise NotImplementedError(f"{mode}: QuotaMode and CarbonMode are implemented on the MI355X hot path")
+        if not isinstance(mode, (P.QuotaMode, P.CarbonMode, P.MacroMolecularMode)):
+            raise NotImplementedError(f"{mode}: QuotaMode, CarbonMode and MacroMolecularMode are implemented "
+                                      "on the MI355X hot path")
         self.n_abiotic, self.abiotic_params, self.abiotic_matrix, self.abiotic_N = 0, None, None, []
         if abiotic is not None:                       # models.jl:99-117
             if not isinstance(abiotic, abiotic_setup):

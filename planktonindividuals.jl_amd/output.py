@@ -346,13 +346,20 @@ DYNAMICS_SPECS = ("%3.0f", "%2.2f", "%6.0f", "%1.2f", "%1.2f", "%1.2f", "%.8E", 
 # CarbonMode (write_outputs.jl:142-157): day, hour, pop, gen, age, size and Bm averages
 CARBON_DYNAMICS_FIELDS = ("gen", "age", "Sz", "Bm")
 CARBON_DYNAMICS_SPECS = ("%3.0f", "%2.2f", "%6.0f", "%1.2f", "%1.2f", "%1.2f", "%.8E")
+# MacroMolecularMode (write_outputs.jl:77-96): day, hour, pop, gen, age, then the
+# protein, DNA, RNA, CH, NST, PST and Chl averages
+MM_DYNAMICS_FIELDS = ("gen", "age", "PRO", "DNA", "RNA", "CH", "NST", "PST", "Chl")
+MM_DYNAMICS_SPECS = ("%3.0f", "%2.2f", "%6.0f", "%1.2f", "%1.2f") + ("%.8E",) * 7
+_DYNAMICS = {0: (DYNAMICS_FIELDS, DYNAMICS_SPECS), 1: (CARBON_DYNAMICS_FIELDS, CARBON_DYNAMICS_SPECS),
+             2: (MM_DYNAMICS_FIELDS, MM_DYNAMICS_SPECS)}
 
 
-def dynamics_line(t, columns, carbon=False):
-    """The log line of QuotaMode (write_outputs.jl:98-117) or CarbonMode
-    (:142-157) from Float32 model time t and the species' columns {field:
+def dynamics_line(t, columns, mode=0):
+    """The log line of QuotaMode (write_outputs.jl:98-117), CarbonMode
+    (:142-157) or MacroMolecularMode (:77-96; `mode` is the ABI's mode id, True
+    for CarbonMode) from Float32 model time t and the species' columns {field:
     array, "ac": Bool array}."""
-    fields, specs = (CARBON_DYNAMICS_FIELDS, CARBON_DYNAMICS_SPECS) if carbon else (DYNAMICS_FIELDS, DYNAMICS_SPECS)
+    fields, specs = _DYNAMICS[int(mode)]
     t = np.float32(t)
     ac = np.asarray(columns["ac"], bool)
     pop = int(np.count_nonzero(ac))                        # dot(ac, ac)
@@ -366,13 +373,13 @@ def dynamics_line(t, columns, carbon=False):
 
 def write_species_dynamics(t, model, filepath):
     """write_species_dynamics(t, phytos, filepath, mode): one line per species."""
-    carbon = getattr(getattr(model, "spec", None), "mode_id", 0) == 1
-    fields = CARBON_DYNAMICS_FIELDS if carbon else DYNAMICS_FIELDS
+    mode = getattr(getattr(model, "spec", None), "mode_id", 0)
+    fields = _DYNAMICS[mode][0]
     for s in range(model.n_species):
         d = model.individuals.phytos[f"sp{s + 1}"].data
         cols = {f: getattr(d, f) for f in fields + ("ac",)}
         with open(os.path.join(filepath, "dynamic_species" + str(s + 1).zfill(3) + ".txt"), "a") as io:
-            io.write(dynamics_line(t, cols, carbon) + "\n")
+            io.write(dynamics_line(t, cols, mode) + "\n")
 
 
 # ---------------------------------------------------------------- HDF5 files

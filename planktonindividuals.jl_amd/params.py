@@ -6,7 +6,8 @@ species' NamedTuple is built, QuotaMode/plankton_generation.jl:22-31).
 """
 import numpy as np
 
-from ._abi import PARAM_NAMES, BGC_NAMES, CARBON_PARAM_NAMES, ABIOTIC_PARAM_NAMES, MODE_QUOTA, MODE_CARBON
+from ._abi import (PARAM_NAMES, BGC_NAMES, CARBON_PARAM_NAMES, MM_PARAM_NAMES, ABIOTIC_PARAM_NAMES, MODE_QUOTA,
+                   MODE_CARBON, MODE_MACROMOLECULAR, NPARAMS)
 
 
 class QuotaMode:
@@ -24,6 +25,8 @@ class CarbonMode:
 
 
 class MacroMolecularMode:
+    """MacroMolecularMode <: AbstractMode (src/PlanktonIndividuals.jl:77)."""
+
     def __repr__(self):
         return "MacroMolecularMode()"
 
@@ -55,16 +58,35 @@ def mode_id(mode):
         return MODE_QUOTA
     if isinstance(mode, CarbonMode):
         return MODE_CARBON
+    if isinstance(mode, MacroMolecularMode):
+        return MODE_MACROMOLECULAR
     raise NotImplementedError(f"{mode} physiology is not part of the MI355X hot path")
 
 
 def param_names(mode):
-    return CARBON_PARAM_NAMES if mode_id(mode) == MODE_CARBON else PARAM_NAMES
+    return {MODE_CARBON: CARBON_PARAM_NAMES, MODE_MACROMOLECULAR: MM_PARAM_NAMES}.get(mode_id(mode), PARAM_NAMES)
 
 
 def phyt_params_default(N=1, mode=None):
-    """phyt_params_default(N, ::QuotaMode) — param_default.jl:234-284, and
-    phyt_params_default(N, ::CarbonMode) — param_default.jl:290-323 (Float64 vectors)."""
+    """phyt_params_default(N, ::QuotaMode) — param_default.jl:234-284,
+    phyt_params_default(N, ::CarbonMode) — param_default.jl:290-323 and
+    phyt_params_default(N, ::MacroMolecularMode) — param_default.jl:94-146
+    (Float64 vectors)."""
+    if mode_id(mode) == MODE_MACROMOLECULAR:
+        params = {
+            "Nsuper": [1], "C_DNA": [1.8e-13], "var": [0.3], "RNA2DNA": [1.73], "PRO2DNA": [34.83],
+            "CH2DNA": [23.3], "Chl2DNA": [3.45], "α": [2.0e-2], "Φ": [4.0e-5], "Topt": [27.0],
+            "Tmax": [30.0], "Ea": [5.3e4], "PCmax": [6.2e-5], "VDOCmax": [0.0], "VNH4max": [6.9e-6],
+            "VNO3max": [6.9e-6], "VPO4max": [1.2e-6], "KsatNH4": [0.005], "KsatNO3": [0.010],
+            "KsatPO4": [0.003], "KsatDOC": [0.0], "NSTmax": [0.12], "PSTmax": [0.70], "CHmax": [0.4],
+            "respir": [1.2e-6], "k_pro": [6.0e-5], "k_sat_pro": [4.5e-13], "k_dna": [1.6e-7],
+            "k_sat_dna": [1.0e-15], "k_rna": [3.0e-7], "k_sat_rna": [1.0e-12], "Chl2N": [3.0],
+            "R_NC_PRO": [1 / 4.5], "R_NC_DNA": [1 / 2.9], "R_PC_DNA": [1 / 11.1], "R_NC_RNA": [1 / 2.8],
+            "R_PC_RNA": [1 / 10.7], "dvid_P": [1.0e-5], "grz_P": [0.0], "mort_P": [5e-5], "mort_reg": [0.5],
+            "grazFracC": [0.7], "grazFracN": [0.7], "grazFracP": [0.7],
+            "mortFracC": [0.5], "mortFracN": [0.5], "mortFracP": [0.5],
+        }
+        return params if N == 1 else generate_n_species_params(N, params)
     if mode_id(mode) == MODE_CARBON:
         params = {
             "Nsuper": [1], "Cquota": [1.8e-11], "mean": [1.2], "var": [0.3], "Chl2C": [0.10],
@@ -118,19 +140,21 @@ def update_phyt_params(tmp, FT=np.float32, *, N=1, mode=None):
 
 
 def species_param_matrix(phyt_params, nsp, mode=None):
-    """(nsp, 40) float32 in the mode's param_names order (QuotaMode: 40 names;
-    CarbonMode: 24, the rest zero): `p = NamedTuple{param_names}(FT.(tmp))`."""
+    """(nsp, 64) float32 in the mode's param_names order (QuotaMode: 40 names;
+    CarbonMode: 24; MacroMolecularMode: 47; the rest zero):
+    `p = NamedTuple{param_names}(FT.(tmp))`."""
     names = param_names(mode)
-    out = np.zeros((nsp, len(PARAM_NAMES)), dtype=np.float32)
+    out = np.zeros((nsp, NPARAMS), dtype=np.float32)
     for s in range(nsp):
         for q, name in enumerate(names):
             if name not in phyt_params:
                 raise ValueError(f"PARAM: parameter not found {name}")
             vals = phyt_params[name]
             out[s, q] = np.float32(np.float64(vals[s]))
-    dt = out[:, names.index("dvid_type")]
-    if np.any((dt < 1) | (dt > 3)):
-        raise ValueError("Wrong cell division type, must be in 1 to 3")
+    if "dvid_type" in names:                      # MacroMolecularMode divides by DNA content
+        dt = out[:, names.index("dvid_type")]
+        if np.any((dt < 1) | (dt > 3)):
+            raise ValueError("Wrong cell division type, must be in 1 to 3")
     return out
 
 

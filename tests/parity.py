@@ -28,12 +28,13 @@ def first_diff(a, b):
 
 
 def compare_models(gpu, ref, nsp, tracer_ids=range(14), grid_fields=("par", "Chl", "pop", "diag_PAR", "PARF"),
-                   diags=True, diag_tracers=()):
+                   diags=True, diag_tracers=(), extra_fields=()):
     """Returns a list of mismatch descriptions (empty = identical).  Besides the
     individuals and tracers: par, Chl, pop, the accumulated PAR diagnostic
     (diags.tracer.PAR, time_step.jl:167), PARF after the column scan, the
     per-species diagnostics, and the tracer diagnostics of `diag_tracers`
-    (time_step.jl:168-172)."""
+    (time_step.jl:168-172); `extra_fields` adds the extra state columns of a
+    mode (ex0 = RNA in MacroMolecularMode)."""
     bad = []
     if gpu.individual_steps() != ref.individual_steps():
         bad.append(f"individual steps: {gpu.individual_steps()} vs {ref.individual_steps()}")
@@ -42,7 +43,7 @@ def compare_models(gpu, ref, nsp, tracer_ids=range(14), grid_fields=("par", "Chl
         if not same(a, b):
             bad.append(f"diag tracer {q}: {first_diff(a, b)}")
     for s in range(nsp):
-        for f in FLOAT_FIELDS + INT_FIELDS:
+        for f in FLOAT_FIELDS + INT_FIELDS + tuple(extra_fields):
             a, b = gpu.field(s, f), ref.field(s, f)
             if not same(a, b):
                 bad.append(f"sp{s + 1}.{f}: {first_diff(a, b)}")

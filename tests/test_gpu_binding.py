@@ -45,8 +45,9 @@ class ModelDesc(C.Structure):         # pi_model_desc
 FLOAT_COLS = ("x", "y", "z", "Sz", "Bm", "Cq", "Nq", "Pq", "Chl", "gen", "age", "graz", "mort", "dvid")
 
 
-class IndividualsHost(C.Structure):   # pi_individuals_host
-    _fields_ = [(n, fp) for n in FLOAT_COLS] + [("xi", i64p), ("yi", i64p), ("zi", i64p), ("ac", u8p)]
+class IndividualsHost(C.Structure):   # pi_individuals_host (no extra columns in QuotaMode)
+    _fields_ = ([(n, fp) for n in FLOAT_COLS] + [("xi", i64p), ("yi", i64p), ("zi", i64p), ("ac", u8p)] +
+                [("ex", fp * 8)])
 
 
 def _bind(path):

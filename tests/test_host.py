@@ -63,7 +63,8 @@ def test_param_defaults_and_update(pi):
     with pytest.raises(ValueError, match="phyt parameter not found"):
         pi.update_phyt_params({"nope": [1]})
     m = pi.params.species_param_matrix(pi.phyt_params_default(2), 2)
-    assert m.shape == (2, 40) and m[0, pi._abi.PARAM_NAMES.index("R_NC")] == np.float32(16 / 106)
+    assert m.shape == (2, 64) and m[0, pi._abi.PARAM_NAMES.index("R_NC")] == np.float32(16 / 106)
+    assert not m[:, 40:].any()
     with pytest.raises(ValueError, match="division type"):
         pi.params.species_param_matrix(pi.update_phyt_params({"dvid_type": [7]}), 1)
 
@@ -75,7 +76,7 @@ def test_modelspec_validation(pi):
     with pytest.raises(ValueError, match="length of `N`"):
         pi.ModelSpec(g, max_individuals=100, phyto=pi.phyto_setup(None, [20, 20], 1))
     with pytest.raises(NotImplementedError):
-        pi.ModelSpec(g, mode=pi.MacroMolecularMode())
+        pi.ModelSpec(g, mode=pi.IronEnergyMode())
     with pytest.raises(ValueError, match="parameter not found"):
         pi.ModelSpec(g, mode=pi.CarbonMode(), phyto=pi.phyto_setup({"Cqmax": [0.4]}, [10], 1))
     s = pi.ModelSpec(g, max_individuals=64, phyto=pi.phyto_setup(None, [40], 1), seed=3)

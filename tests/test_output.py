@@ -81,9 +81,17 @@ def test_dynamics_line(out):
     empty = dict(cols, ac=np.zeros(n, bool))
     assert out.dynamics_line(t, empty) == "  ".join(["  1", "14.50", "     0"] + ["NaN"] * 8)
     # CarbonMode: gen, age and size with %1.2f, Bm with %.8E and nothing else
-    cline = out.dynamics_line(t, cols, carbon=True)
+    cline = out.dynamics_line(t, cols, 1)
     assert cline == "  ".join(want[:6] + ["%.8E" % means[3]])
     assert cline.endswith("1.40000000E+01") and len(cline.split()) == 7
+    # MacroMolecularMode: gen and age with %1.2f, then PRO, DNA, RNA, CH, NST, PST, Chl with %.8E
+    mcols = {f: np.arange(1, n + 1, dtype=np.float32) * np.float32(k + 1)
+             for k, f in enumerate(out.MM_DYNAMICS_FIELDS)}
+    mcols["ac"] = ac
+    mm = [np.float32(np.float32((k + 1) * 14) / np.float32(4)) for k in range(9)]
+    mline = out.dynamics_line(t, mcols, 2)
+    assert mline == "  ".join(["  1", "14.50", "     4", "%1.2f" % mm[0], "%1.2f" % mm[1]] + ["%.8E" % m for m in mm[2:]])
+    assert len(mline.split()) == 12 and mline.split()[5] == "1.05000000E+01"
 
 
 # ---------------------------------------------------------------- HDF5 container
