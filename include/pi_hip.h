@@ -64,9 +64,9 @@ enum {
     PI_NPARAMS = 64
 };
 
-/* ---- physiology modes (src/PlanktonIndividuals.jl:65-77: CarbonMode, QuotaMode,
- *      MacroMolecularMode) */
-enum { PI_MODE_QUOTA = 0, PI_MODE_CARBON = 1, PI_MODE_MACROMOLECULAR = 2 };
+/* ---- physiology modes (src/PlanktonIndividuals.jl:65-81: CarbonMode, QuotaMode,
+ *      MacroMolecularMode, IronEnergyMode) */
+enum { PI_MODE_QUOTA = 0, PI_MODE_CARBON = 1, PI_MODE_MACROMOLECULAR = 2, PI_MODE_IRONENERGY = 3 };
 
 /* ---- CarbonMode parameters: 24 floats in the reference's `param_names` order
  *      (CarbonMode/plankton_generation.jl:14-16), in the first 24 of a species'
@@ -91,6 +91,24 @@ enum {
     PI_M_dvid_P, PI_M_grz_P, PI_M_mort_P, PI_M_mort_reg, PI_M_grazFracC, PI_M_grazFracN, PI_M_grazFracP,
     PI_M_mortFracC, PI_M_mortFracN, PI_M_mortFracP,
     PI_NMPARAMS = 47
+};
+
+/* ---- IronEnergyMode parameters: 64 floats in the reference's `param_names`
+ *      order (IronEnergyMode/plankton_generation.jl:28-40). ---------------- */
+enum {
+    PI_I_Nsuper = 0, PI_I_Cquota, PI_I_SA, PI_I_mean, PI_I_var, PI_I_Chl2Cint,
+    PI_I_alpha, PI_I_Topt, PI_I_Tmax, PI_I_Ea, PI_I_is_nr, PI_I_is_croc, PI_I_is_tric,
+    PI_I_PCmax, PI_I_VNO3max, PI_I_VNH4max, PI_I_VPO4max,
+    PI_I_k_cf, PI_I_k_rs, PI_I_k_nr, PI_I_k_nf, PI_I_k_mtb,
+    PI_I_e_cf, PI_I_e_rs, PI_I_e_nr, PI_I_e_nf,
+    PI_I_k_Fe_ST2PS, PI_I_k_Fe_PS2ST, PI_I_k_Fe_ST2NR, PI_I_k_Fe_NR2ST, PI_I_k_Fe_ST2NF, PI_I_k_Fe_NF2ST,
+    PI_I_KfePS, PI_I_KfeNR, PI_I_KfeNF, PI_I_KsatNH4, PI_I_KsatNO3, PI_I_KsatPO4, PI_I_KSAFe,
+    PI_I_CHmax, PI_I_qNH4max, PI_I_qNO3max, PI_I_qPmax, PI_I_qFemax,
+    PI_I_Chl2N, PI_I_R_NC, PI_I_R_PC, PI_I_NF_clock,
+    PI_I_grz_P, PI_I_dvid_type, PI_I_dvid_P, PI_I_dvid_reg, PI_I_dvid_reg2, PI_I_mort_P, PI_I_mort_reg,
+    PI_I_grazFracC, PI_I_grazFracN, PI_I_grazFracP, PI_I_grazFracFe,
+    PI_I_mortFracC, PI_I_mortFracN, PI_I_mortFracP, PI_I_mortFracFe, PI_I_max_ptc,
+    PI_NIPARAMS = 64
 };
 
 /* ---- abiotic particle parameters (Abiotic/particle_generation.jl:8,
@@ -131,6 +149,19 @@ enum {
 #define PI_F_NST PI_F_Nq
 #define PI_F_PST PI_F_Pq
 #define PI_F_RNA PI_F_ex0
+/* IronEnergyMode (IronEnergyMode/plankton_generation.jl:1-26): Sz, Bm, Chl as
+ * in QuotaMode, the carbon, NH4 and P reserves in the Cq, Nq, Pq columns, the
+ * rest in the extra columns; ptc (abiotic particles attached) stays 0 here. */
+#define PI_F_qNH4  PI_F_Nq
+#define PI_F_qP    PI_F_Pq
+#define PI_F_qNO3  (PI_F_ex0 + 0)
+#define PI_F_qFe   (PI_F_ex0 + 1)
+#define PI_F_qFePS (PI_F_ex0 + 2)
+#define PI_F_qFeNR (PI_F_ex0 + 3)
+#define PI_F_qFeNF (PI_F_ex0 + 4)
+#define PI_F_tdark (PI_F_ex0 + 5)
+#define PI_F_exEn  (PI_F_ex0 + 6)
+#define PI_F_ptc   (PI_F_ex0 + 7)
 
 /* ---- grid-shaped outputs (timestepper + diagnostics) --------------------- */
 enum {
@@ -177,7 +208,7 @@ typedef struct pi_model_desc {
     uint64_t rng_seed;           /* key of the shared uniform stream (pi_math.h) */
     int32_t n_diag_tracers;      /* tracers accumulated into diags.tracer besides PAR */
     const int32_t* diag_tracers; /* tracer ids, may be NULL when n_diag_tracers == 0 */
-    int32_t mode;                /* PI_MODE_QUOTA, PI_MODE_CARBON or PI_MODE_MACROMOLECULAR (PlanktonModel(...; mode)) */
+    int32_t mode;                /* PI_MODE_QUOTA, _CARBON, _MACROMOLECULAR or _IRONENERGY (PlanktonModel(...; mode)) */
     const pi_metrics* metrics;   /* LatLonGrid metrics; NULL = RectilinearGrid (dx, dy of grid) */
     /* abiotic particles (abiotic_setup, src/PlanktonIndividuals.jl:124-129;
      * Individuals/Abiotic/particle_generation.jl:1-35): n_abiotic species of

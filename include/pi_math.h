@@ -46,7 +46,8 @@
 enum { PI_KIND_DIFFUSION = 0, PI_KIND_DECISION = 1 };
 /* initial-condition sub-streams (iteration field 0): positions, quotas, the two
  * uniforms of the size normal, and per tracer (slot = padded cell index) */
-enum { PI_KIND_INIT_POS = 16, PI_KIND_INIT_QUOTA = 17, PI_KIND_INIT_SIZE = 18, PI_KIND_TRACER = 32 };
+enum { PI_KIND_INIT_POS = 16, PI_KIND_INIT_QUOTA = 17, PI_KIND_INIT_SIZE = 18, PI_KIND_INIT_FE = 19,
+       PI_KIND_TRACER = 32 };
 /* abiotic particles: species field PI_ABIO_SPECIES + sa; the random walk uses
  * PI_KIND_DIFFUSION, particles_from_bcs! one draw per padded cell (slot = the
  * cell's index), the initial positions PI_KIND_INIT_POS */
@@ -201,6 +202,18 @@ PI_HD float pi_pow4f(float x) { double d = (double)x; double s = d * d; return (
 PI_HD double pi_pow2d(double x) { return x * x; }
 PI_HD double pi_pow4d(double x) { double s = x * x; return s * s; }
 
+/* Float32 x ^ y for a non-integer y (the cell's surface area Sz^(2/3) of
+ * IronEnergyMode): exp(y log x) in binary64 with the functions above, rounded
+ * once to Float32; 0 for x = 0, +Inf for x = +Inf, NaN for x < 0 or NaN
+ * (Julia throws a DomainError for a negative base). */
+PI_HD float pi_powf(float x, float y)
+{
+    if (x != x || x < 0.0f) return NAN;
+    if (x == 0.0f) return 0.0f;
+    if (x == INFINITY) return INFINITY;
+    return (float)pi_exp_f64((double)y * pi_log_f64((double)x));
+}
+
 /* ------------------------------------------------- Julia IEEE semantics -- */
 PI_HD int pi_signbitf(float x) { union { float f; uint32_t i; } b; b.f = x; return (int)(b.i >> 31); }
 PI_HD int pi_signbitd(double x) { union { double f; uint64_t i; } b; b.f = x; return (int)(b.i >> 63); }
@@ -240,6 +253,15 @@ PI_HD int jl_islessd(double x, double y)
 {
     int xn = (x != x), yn = (y != y);
     return (!xn && (yn || (pi_signbitd(x) && !pi_signbitd(y)))) || (x < y);
+}
+/* Base.isequal for IEEE floats: -0 and +0 differ, NaN equals NaN. */
+PI_HD int jl_isequalf(float x, float y)
+{
+    return (x != x && y != y) || (x == y && pi_signbitf(x) == pi_signbitf(y));
+}
+PI_HD int jl_isequald(double x, double y)
+{
+    return (x != x && y != y) || (x == y && pi_signbitd(x) == pi_signbitd(y));
 }
 /* Bool * x  ==  ifelse(b, x, copysign(0, x)) */
 PI_HD float jl_maskf(int b, float x) { return b ? x : copysignf(0.0f, x); }

@@ -2,7 +2,7 @@
 
 Each entry below plants one plausible mistake in a copy of oracle/pi_oracle.cpp
 (a wrong index, sign, exponent, dropped term or halo plane, in QuotaMode,
-CarbonMode and MacroMolecularMode), builds the copy
+CarbonMode, MacroMolecularMode and IronEnergyMode), builds the copy
 into a temporary library, and runs the CPU pin tests against it
 (PI_ORACLE_LIB).  Every mistake must make at least one pin fail.  The
 committed source is never modified.
@@ -168,6 +168,23 @@ MISTAKES = [
      "RNA * q[PI_M_R_NC_RNA] + NST;"),
     ("mm: daughter without RNA",
      "PRO[d] = PRO[i]; DNA[d] = DNA[i]; RNA[d] = RNA[i]; CH[d] = CH[i];", "PRO[d] = PRO[i]; DNA[d] = DNA[i]; CH[d] = CH[i];"),
+    ("iron: a single return at sunset", "FT f_PS2ST = q[PI_I_k_Fe_PS2ST] * qFePS * (FT)(falling + dark);",
+     "FT f_PS2ST = q[PI_I_k_Fe_PS2ST] * qFePS * (FT)(falling || dark);"),
+    ("iron: DIC charged with the carbon fixed", "plk[PI_DIC][c] += M::mask(a, (p.RS[i] - p.PS[i]) * dt);",
+     "plk[PI_DIC][c] += M::mask(a, (p.RS[i] - p.CF[i]) * dt);"),
+    ("iron: surface area as Sz^(1/3)", "M::pow(p.Sz[i], L(2.0f) / L(3.0f))", "M::pow(p.Sz[i], L(1.0f) / L(3.0f))"),
+    ("iron: the NH4 reserve not returned", "plk[PI_NH4][c] += M::mask(a, p.Nq[i]) * loss[i];", "(void)0;"),
+    ("iron: lost iron without the NR pool", "FT Fe = p.qFe[i] + p.qFePS[i] + p.qFeNF[i] + p.qFeNR[i];",
+     "FT Fe = p.qFe[i] + p.qFePS[i] + p.qFeNF[i];"),
+    ("iron: nitrate reduction adds to qNO3", "p.qNO3[i] = p.qNO3[i] + -p.NR[i] * dt;",
+     "p.qNO3[i] = p.qNO3[i] + p.NR[i] * dt;"),
+    ("iron: light does not reset tdark",
+     "p.tdark[i] = p.tdark[i] * (one - (FT)M::isless(L(0), tpar[i])) + M::mask(M::isequal(L(0), tpar[i]), dt);",
+     "p.tdark[i] = p.tdark[i] + M::mask(M::isequal(L(0), tpar[i]), dt);"),
+    ("iron: dead slots keep their iron", "FT f_PS2ST = q[PI_I_k_Fe_PS2ST] * qFePS * (FT)(falling + dark);",
+     "FT f_PS2ST = M::mask(p.ac[i], q[PI_I_k_Fe_PS2ST] * qFePS * (FT)(falling + dark));"),
+    ("iron: storage to photosynthesis on falling light", "FT f_ST2PS = M::mask(rising, q[PI_I_k_Fe_ST2PS] * qFe);",
+     "FT f_ST2PS = M::mask(falling, q[PI_I_k_Fe_ST2PS] * qFe);"),
 ]
 
 
@@ -192,7 +209,7 @@ def main(only):
                                 "tests/test_oracle_pins.py", "tests/test_oracle_pins_physics.py",
                                 "tests/test_oracle_pins_carbon.py", "tests/test_oracle_pins_bcs.py",
                                 "tests/test_oracle_pins_latlon.py", "tests/test_oracle_pins_abiotic.py",
-                                "tests/test_oracle_pins_mm.py"],
+                                "tests/test_oracle_pins_mm.py", "tests/test_oracle_pins_iron.py"],
                                cwd=REPO, env=env, capture_output=True, text=True)
             first = next((l for l in r.stdout.splitlines() if l.startswith("FAILED")), "")
             (caught if r.returncode != 0 else missed).append(name)

@@ -3,8 +3,8 @@
  *
  * A CPU restatement, kernel by kernel and in the same floating-point operation
  * order, of PlanktonIndividuals.jl v0.7.5's CPU() time step for QuotaMode,
- * CarbonMode and MacroMolecularMode, with abiotic particles (without their
- * interaction with phytoplankton), as run by `julia -t 1` (KernelAbstractions' CPU
+ * CarbonMode, MacroMolecularMode and IronEnergyMode, with abiotic particles
+ * (without their interaction with phytoplankton), as run by `julia -t 1` (KernelAbstractions' CPU
  * backend with one thread visits every ndrange index in ascending order, so
  * every `@atomic` accumulation happens in slot order, kernel after kernel).
  *
@@ -55,6 +55,8 @@ template <> struct N_<float> {
     static float pow4(float x) { return pi_pow4f(x); }
     static float sqrt(float x) { return std::sqrt(x); }
     static float rem(float x, float y) { return std::fmod(x, y); }
+    static int isequal(float a, float b) { return jl_isequalf(a, b); }
+    static float pow(float x, float y) { return pi_powf(x, y); }
 };
 template <> struct N_<double> {
     static double max(double a, double b) { return jl_maxd(a, b); }
@@ -67,6 +69,12 @@ template <> struct N_<double> {
     static double pow4(double x) { return pi_pow4d(x); }
     static double sqrt(double x) { return std::sqrt(x); }
     static double rem(double x, double y) { return std::fmod(x, y); }
+    static int isequal(double a, double b) { return jl_isequald(a, b); }
+    static double pow(double x, double y) {
+        if (x != x || x < 0.0) return NAN;
+        if (x == 0.0) return 0.0;
+        return pi_exp_f64(y * pi_log_f64(x));
+    }
 };
 
 /* Float32-rounded literals (Julia `0.5f0` etc.), widened in FP64 mode. */
@@ -112,6 +120,10 @@ struct Oracle : orc_model {
         /* MacroMolecularMode: RNA, and the synthesis rates; PRO, DNA, CH, NST,
          * PST are the columns Bm, Sz, Cq, Nq, Pq */
         V RNA, S_PRO, S_DNA, S_RNA;
+        /* IronEnergyMode: CH, qNH4, qP are the columns Cq, Nq, Pq; the other
+         * pools and rates (allocated for IronEnergyMode models only) */
+        V qNO3, qFe, qFePS, qFeNR, qFeNF, tdark, exEn, ptc;
+        V VFe, RS, ERS, CF, ECF, NR, ENR, NF, ENF, ST2PS, PS2ST, ST2NR, NR2ST, ST2NF, NF2ST;
         std::vector<int64_t> xi, yi, zi, idx;
         std::vector<uint8_t> ac;
     };
@@ -206,12 +218,12 @@ struct Oracle : orc_model {
         nsp = d->n_species; cap = d->capacity;
         if (nsp < 1 || cap < 1) return fail(PI_ERR_ARG, "n_species and capacity must be >= 1");
         mode = d->mode;
-        if (mode != PI_MODE_QUOTA && mode != PI_MODE_CARBON && mode != PI_MODE_MACROMOLECULAR)
-            return fail(PI_ERR_UNSUPPORTED, "physiology mode");
+        if (mode < PI_MODE_QUOTA || mode > PI_MODE_IRONENERGY) return fail(PI_ERR_UNSUPPORTED, "physiology mode");
         P.resize(nsp); dvid_type.resize(nsp);
         for (int s = 0; s < nsp; ++s) {
             for (int q = 0; q < PI_NPARAMS; ++q) P[s][q] = (FT)d->params[s * PI_NPARAMS + q];
-            const int dt_idx = mode == PI_MODE_CARBON ? (int)PI_C_dvid_type : (int)PI_P_dvid_type;
+            const int dt_idx = mode == PI_MODE_CARBON ? (int)PI_C_dvid_type
+                             : mode == PI_MODE_IRONENERGY ? (int)PI_I_dvid_type : (int)PI_P_dvid_type;
             dvid_type[s] = mode == PI_MODE_MACROMOLECULAR ? 1 : (int)d->params[s * PI_NPARAMS + dt_idx];
             if (dvid_type[s] < 1 || dvid_type[s] > 3)
                 return fail(PI_ERR_ARG, "Wrong cell division type, must be in 1 to 3");
@@ -228,6 +240,11 @@ struct Oracle : orc_model {
                 f->assign(cap, (FT)0);
             p.xi.assign(cap, 0); p.yi.assign(cap, 0); p.zi.assign(cap, 0); p.idx.assign(cap, 0);
             p.ac.assign(cap, 0);
+            if (mode == PI_MODE_IRONENERGY)
+                for (V* f : {&p.qNO3, &p.qFe, &p.qFePS, &p.qFeNR, &p.qFeNF, &p.tdark, &p.exEn, &p.ptc, &p.VFe, &p.RS,
+                             &p.ERS, &p.CF, &p.ECF, &p.NR, &p.ENR, &p.NF, &p.ENF, &p.ST2PS, &p.PS2ST, &p.ST2NR,
+                             &p.NR2ST, &p.ST2NF, &p.NF2ST})
+                    f->assign(cap, (FT)0);
         }
         for (int q = 0; q < PI_NTRACERS; ++q) {
             tr[q].assign(ncell, 0); Gcs[q].assign(ncell, 0); ttemp[q].assign(ncell, 0); plk[q].assign(ncell, 0);
@@ -589,6 +606,7 @@ struct Oracle : orc_model {
     pi_status plankton_update(int s, FT dt) {
         if (mode == PI_MODE_CARBON) return carbon_update(s, dt);
         if (mode == PI_MODE_MACROMOLECULAR) return mm_update(s, dt);
+        if (mode == PI_MODE_IRONENERGY) return iron_update(s, dt);
         Plank& p = sp[s];
         const auto& q = P[s];
         diags_proc(diagSp[s][1], p.graz, p);
@@ -1155,6 +1173,303 @@ struct Oracle : orc_model {
         }
     }
 
+    /* ============================================ IronEnergyMode ======= */
+    /* tempFunc_CF / tempFunc (IronEnergyMode/growth_kernels.jl:1-17) */
+    static FT i_tempFunc_CF(FT T, const Q& q) {
+        FT x = T - q[PI_I_Topt], xmax = q[PI_I_Tmax] - q[PI_I_Topt];
+        FT regT = shape_dec(x, xmax, L(4.0e-2f), 4);
+        FT k = M::exp(-q[PI_I_Ea] / (L(8.3145f) * (T + L(273.15f)))) * regT;
+        k = M::max(L(0), k);
+        FT OGT = M::exp(-q[PI_I_Ea] / (L(8.3145f) * (q[PI_I_Topt] + L(273.15f))));
+        return M::min(L(1), k / OGT);
+    }
+    static FT i_tempFunc(FT T, const Q& q) {
+        FT k = M::exp(-q[PI_I_Ea] / (L(8.3145f) * (T + L(273.15f))));
+        k = M::max(L(0), k);
+        FT OGT = M::exp(-q[PI_I_Ea] / (L(8.3145f) * (q[PI_I_Topt] + L(273.15f))));
+        return M::min(L(1), k / OGT);
+    }
+    /* calc_loss_kernel! (IronEnergyMode/consume_loss.jl:18-33) */
+    void i_calc_loss(const Plank& p, const V& loss, FT fC, FT fN, FT fP, FT fFe, FT R_NC, FT R_PC) {
+        for (int64_t i = 0; i < cap; ++i) {
+            int64_t c = cell_of(p, i);
+            bool a = p.ac[i];
+            FT Fe = p.qFe[i] + p.qFePS[i] + p.qFeNF[i] + p.qFeNR[i];
+            plk[PI_NH4][c] += M::mask(a, p.Nq[i]) * loss[i];
+            plk[PI_NO3][c] += M::mask(a, p.qNO3[i]) * loss[i];
+            plk[PI_DOC][c] += M::mask(a, (p.Bm[i] + p.Cq[i]) * fC) * loss[i];
+            plk[PI_DON][c] += M::mask(a, p.Bm[i] * R_NC * fN) * loss[i];
+            plk[PI_DOP][c] += M::mask(a, (p.Bm[i] * R_PC + p.Pq[i]) * fP) * loss[i];
+            plk[PI_DFe][c] += M::mask(a, Fe * fFe) * loss[i];
+            plk[PI_POC][c] += M::mask(a, (p.Bm[i] + p.Cq[i]) * (L(1) - fC)) * loss[i];
+            plk[PI_PON][c] += M::mask(a, p.Bm[i] * R_NC * (L(1) - fN)) * loss[i];
+            plk[PI_POP][c] += M::mask(a, (p.Bm[i] * R_PC + p.Pq[i]) * (L(1) - fP)) * loss[i];
+            plk[PI_PFe_bio][c] += M::mask(a, Fe * (L(1) - fFe)) * loss[i];
+        }
+    }
+    /* plankton_update! (IronEnergyMode/plankton_update.jl:1-33) */
+    pi_status iron_update(int s, FT dt) {
+        Plank& p = sp[s];
+        const auto& q = P[s];
+        diags_proc(diagSp[s][1], p.graz, p);
+        i_calc_loss(p, p.graz, q[PI_I_grazFracC], q[PI_I_grazFracN], q[PI_I_grazFracP], q[PI_I_grazFracFe],
+                    q[PI_I_R_NC], q[PI_I_R_PC]);
+        inactivate(p, p.graz);
+        diags_proc(diagSp[s][2], p.mort, p);
+        i_calc_loss(p, p.mort, q[PI_I_mortFracC], q[PI_I_mortFracN], q[PI_I_mortFracP], q[PI_I_mortFracFe],
+                    q[PI_I_R_NC], q[PI_I_R_PC]);
+        inactivate(p, p.mort);
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) p.dvid[i] = M::mask(p.ac[i], p.dvid[i]);
+        diags_proc(diagSp[s][3], p.dvid, p);
+        FT dvidnum = 0;
+        for (int64_t i = 0; i < cap; ++i) dvidnum += M::mask(p.ac[i], p.dvid[i]);
+        std::vector<int64_t> free_ind;
+        for (int64_t i = 0; i < cap; ++i) if (!p.ac[i]) free_ind.push_back(i + 1);
+        if (dvidnum > (FT)free_ind.size()) {
+            char buf[160];
+            snprintf(buf, sizeof buf, "number of individual exceeds the capacity at timestep %g days",
+                     (double)t / 86400.0);
+            failed = true;
+            return fail(PI_ERR_CAPACITY, buf);
+        }
+        i_divide(p, free_ind);
+        i_growth(s, dt);
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_consume_kernel! (consume_loss.jl:2-16) */
+            int64_t c = cell_of(p, i);
+            bool a = p.ac[i];
+            plk[PI_DIC][c] += M::mask(a, (p.RS[i] - p.PS[i]) * dt);
+            plk[PI_NH4][c] += M::mask(a, -p.VNH4[i] * dt);
+            plk[PI_NO3][c] += M::mask(a, -p.VNO3[i] * dt);
+            plk[PI_PO4][c] += M::mask(a, -p.VPO4[i] * dt);
+            plk[PI_DFe][c] += M::mask(a, -p.VFe[i] * dt);
+        }
+        for (int64_t i = 0; i < cap; ++i) diagSp[s][0][cell_of(p, i)] += (FT)((p.ac[i] && p.ac[i]) ? 1 : 0);
+        return PI_OK;
+    }
+    /* divide! (IronEnergyMode/division_death.jl:38-110): the daughter copies every
+     * pool, exEn, Chl, gen, ac, dvid and tdark; age and ptc stay the slot's; both
+     * halves get the halved pools and exEn (not tdark) */
+    void i_divide(Plank& p, const std::vector<int64_t>& free_ind) {
+        FT acc = 0;
+        for (int64_t i = 0; i < cap; ++i) { acc = acc + p.dvid[i]; idc[i] = acc; }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) idc_int[i] = (int64_t)idc[i];
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i)
+            if (p.dvid[i] == L(1)) p.idx[i] = free_ind[(size_t)(idc_int[i] - 1)];
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {                  /* copy_daughter_individuals_kernel! */
+            if (p.dvid[i] == L(1) && p.idx[i] != 0) {
+                int64_t d = p.idx[i] - 1;
+                p.x[d] = p.x[i]; p.y[d] = p.y[i]; p.z[d] = p.z[i];
+                p.Sz[d] = p.Sz[i]; p.Bm[d] = p.Bm[i]; p.Cq[d] = p.Cq[i];
+                p.qNO3[d] = p.qNO3[i]; p.Nq[d] = p.Nq[i]; p.Pq[d] = p.Pq[i];
+                p.qFe[d] = p.qFe[i]; p.qFePS[d] = p.qFePS[i]; p.qFeNR[d] = p.qFeNR[i]; p.qFeNF[d] = p.qFeNF[i];
+                p.exEn[d] = p.exEn[i]; p.Chl[d] = p.Chl[i]; p.gen[d] = p.gen[i]; p.ac[d] = p.ac[i];
+                p.dvid[d] = p.dvid[i]; p.tdark[d] = p.tdark[i];
+            }
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {                  /* divide_to_half_kernel! */
+            FT h = (L(2) - p.dvid[i]) / L(2);
+            p.Sz[i] = p.Sz[i] * h; p.Bm[i] = p.Bm[i] * h; p.Cq[i] = p.Cq[i] * h;
+            p.qNO3[i] = p.qNO3[i] * h; p.Nq[i] = p.Nq[i] * h; p.Pq[i] = p.Pq[i] * h;
+            p.qFe[i] = p.qFe[i] * h; p.qFePS[i] = p.qFePS[i] * h; p.qFeNR[i] = p.qFeNR[i] * h;
+            p.qFeNF[i] = p.qFeNF[i] * h; p.exEn[i] = p.exEn[i] * h; p.Chl[i] = p.Chl[i] * h;
+            p.gen[i] = p.gen[i] + p.dvid[i];
+            p.age[i] = p.age[i] * (L(1) - p.dvid[i]);
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) p.idx[i] = 0;
+    }
+    /* plankton_growth! (IronEnergyMode/plankton_growth.jl:2-30, growth_kernels.jl:19-366,
+     * division_death_probability.jl:26-95,266-278), every kernel over every slot.
+     * CH = Cq, qNH4 = Nq, qP = Pq. */
+    void i_growth(int s, FT dt) {
+        Plank& p = sp[s];
+        const auto& q = P[s];
+        V &CH = p.Cq, &qNH4 = p.Nq, &qP = p.Pq;
+        const FT e30 = L(1.0e-30f), z0 = L(0), one = L(1);
+        const FT is_nr = q[PI_I_is_nr], is_nf = q[PI_I_is_croc] + q[PI_I_is_tric];
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_trs_uptake_kernel! */
+            bool a = p.ac[i];
+            FT T = tT[i], Bm = p.Bm[i], pp = M::max(one, tpop[i]);
+            FT Qn = qNH4[i] / M::max(e30, Bm + CH[i]);                               /* calc_NH4_uptake */
+            FT regQN = shape_dec(Qn, q[PI_I_qNH4max], L(1.0e-4f), 2);
+            FT V1 = M::mask(a, q[PI_I_VNH4max] * regQN * tNH4[i] / M::max(e30, tNH4[i] + q[PI_I_KsatNH4]) * i_tempFunc(T, q) * Bm);
+            p.VNH4[i] = M::min(V1, tNH4[i] / dt / pp) * is_nr;
+            FT Qn3 = p.qNO3[i] / M::max(e30, Bm + CH[i]);                            /* calc_NO3_uptake */
+            FT regQN3 = shape_dec(Qn3, q[PI_I_qNO3max], L(1.0e-4f), 2);
+            FT V2 = M::mask(a, q[PI_I_VNO3max] * regQN3 * tNO3[i] / M::max(e30, tNO3[i] + q[PI_I_KsatNO3]) * i_tempFunc(T, q) * Bm);
+            p.VNO3[i] = M::min(V2, tNO3[i] / dt / pp) * is_nr;
+            FT Qp = (qP[i] + Bm * q[PI_I_R_PC]) / M::max(e30, Bm + CH[i]);            /* calc_P_uptake */
+            FT regQP = shape_dec(Qp, q[PI_I_qPmax], L(1.0e-4f), 2);
+            FT V3 = M::mask(a, q[PI_I_VPO4max] * regQP * tPO4[i] / M::max(e30, tPO4[i] + q[PI_I_KsatPO4]) * i_tempFunc(T, q) * Bm);
+            p.VPO4[i] = M::min(V3, tPO4[i] / dt / pp);
+            FT Qfe = (p.qFe[i] + p.qFePS[i] + p.qFeNF[i] + p.qFeNR[i]) / M::max(e30, Bm + CH[i]);   /* calc_Fe_uptake */
+            FT regQFe = shape_dec(Qfe, q[PI_I_qFemax], L(1.0e-4f), 2);
+            FT SA = q[PI_I_SA] * M::pow(p.Sz[i], L(2.0f) / L(3.0f));
+            FT V4 = M::mask(a, q[PI_I_KSAFe] * SA * tDFe[i] * regQFe * q[PI_I_Nsuper] * i_tempFunc(T, q));
+            p.VFe[i] = M::min(V4, tDFe[i] / dt / pp);
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* update_state_1_kernel! */
+            qP[i] = qP[i] + p.VPO4[i] * dt;
+            p.qNO3[i] = p.qNO3[i] + p.VNO3[i] * dt;
+            qNH4[i] = qNH4[i] + p.VNH4[i] * dt;
+            p.qFe[i] = p.qFe[i] + p.VFe[i] * dt;
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_PS_kernel! (calc_PS) */
+            FT aI = tpar[i] * q[PI_I_alpha] * p.Chl[i] / M::max(e30, p.Bm[i]);
+            FT Qfe_ps = p.qFePS[i] / M::max(e30, p.Bm[i] + CH[i]);
+            FT Ksat = Qfe_ps / M::max(e30, Qfe_ps + q[PI_I_KfePS]);
+            p.PS[i] = q[PI_I_PCmax] * (one - M::exp(-aI)) * p.Bm[i] * Ksat;
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_respiration_kernel! (calc_respir) */
+            bool a = p.ac[i];
+            FT RS = M::mask(a, CH[i] * q[PI_I_k_rs] * i_tempFunc(tT[i], q));
+            RS = M::min(RS, CH[i] / dt);
+            p.RS[i] = RS;
+            p.ERS[i] = M::mask(a, RS * q[PI_I_e_rs]);
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_carbon_fixation_kernel! (calc_CF) */
+            bool a = p.ac[i];
+            FT Qc = CH[i] / M::max(e30, p.Bm[i] + CH[i]);
+            FT regQC = shape_dec(Qc, q[PI_I_CHmax], L(1.0e-4f), 2);
+            p.CF[i] = M::mask(a, q[PI_I_k_cf] * regQC * i_tempFunc_CF(tT[i], q) * p.Bm[i]);
+            p.ECF[i] = M::mask(a, p.CF[i] * q[PI_I_e_cf]);
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_NO3_reduction_kernel! (calc_NR) */
+            bool a = p.ac[i];
+            FT Qn = qNH4[i] / M::max(e30, p.Bm[i] + CH[i]);
+            FT reg = shape_dec(Qn, q[PI_I_qNH4max], L(1.0e-4f), 2);
+            FT Qfe_NR = p.qFeNR[i] / M::max(e30, p.Bm[i] + CH[i]);
+            FT Ksat = Qfe_NR / M::max(e30, Qfe_NR + q[PI_I_KfeNR]);
+            FT NR = M::mask(a, q[PI_I_k_nr] * reg * p.qNO3[i] * Ksat * i_tempFunc(tT[i], q));
+            NR = M::min(NR, p.qNO3[i] / dt);
+            FT ENR = M::mask(a, NR * q[PI_I_e_nr]);
+            p.NR[i] = NR * is_nr;
+            p.ENR[i] = ENR * is_nr;
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_nitrogen_fixation_kernel! (calc_NF) */
+            bool a = p.ac[i];
+            FT Qn = qNH4[i] / M::max(e30, p.Bm[i] + CH[i]);
+            FT reg = shape_dec(Qn, q[PI_I_qNH4max], L(1.0e-4f), 2);
+            FT Qfe_NF = p.qFeNF[i] / M::max(e30, p.Bm[i] + CH[i]);
+            FT Ksat = Qfe_NF / M::max(e30, Qfe_NF + q[PI_I_KfeNF]);
+            FT NF = M::mask(a, q[PI_I_k_nf] * reg * Ksat * i_tempFunc(tT[i], q) * p.Bm[i]);
+            FT ENF = M::mask(a, NF * q[PI_I_e_nf]);
+            p.NF[i] = NF * is_nf;
+            p.ENF[i] = ENF * is_nf;
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* energy_allocation_kernel! (energy_alloc) */
+            FT PS = p.PS[i], ERS = p.ERS[i];
+            FT ECFt = M::min(PS, p.ECF[i]);
+            FT ENFt = M::min(p.ENF[i], PS + ERS - ECFt) * is_nf;
+            FT ENRt = M::min(p.ENR[i], PS + ERS - ECFt) * is_nr;
+            FT ERSt = M::min(ERS, M::max(z0, ECFt + ENFt + ENRt - PS));
+            p.exEn[i] = M::max(z0, PS - ECFt - ENFt - ENRt);
+            p.ERS[i] = ERSt; p.ECF[i] = ECFt; p.ENF[i] = ENFt; p.ENR[i] = ENRt;
+            p.RS[i] = p.ERS[i] / q[PI_I_e_rs];
+            p.CF[i] = p.ECF[i] / q[PI_I_e_cf];
+            p.NF[i] = p.ENF[i] / q[PI_I_e_nf];
+            p.NR[i] = p.ENR[i] / q[PI_I_e_nr];
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* update_state_2_kernel! */
+            CH[i] = CH[i] + (p.CF[i] - p.RS[i]) * dt;
+            p.qNO3[i] = p.qNO3[i] + -p.NR[i] * dt;
+            qNH4[i] = qNH4[i] + p.NR[i] * dt;
+            qNH4[i] = qNH4[i] + p.NF[i] * dt;
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_ρChl_kernel! */
+            FT r = q[PI_I_Chl2N] / M::max(e30, tpar[i] * q[PI_I_alpha] * p.Chl[i] / p.Bm[i]);
+            r = M::mask(M::isless(L(1.0e-1f), tpar[i]), r);
+            p.rho[i] = M::mask(p.ac[i], r);
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i)     /* calc_BS_kernel! */
+            p.BS[i] = M::mask(p.ac[i], M::min(M::min(CH[i], qNH4[i] / q[PI_I_R_NC]), qP[i] / q[PI_I_R_PC]) * q[PI_I_k_mtb]);
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_iron_fluxes_kernel! (iron_alloc); Bool sums are Int */
+            FT par = tpar[i], dpar = tdpar[i], qFe = p.qFe[i], qFePS = p.qFePS[i];
+            FT qFeNR = p.qFeNR[i], qFeNF = p.qFeNF[i], tdark = p.tdark[i];
+            int dark = M::isequal(L(0), par), light = M::isless(L(0), par);
+            int rising = M::isless(L(0), dpar), falling = M::isless(dpar, L(0));
+            int early = M::isless(tdark, q[PI_I_NF_clock]), late = M::isless(q[PI_I_NF_clock], tdark);
+            FT f_ST2PS = M::mask(rising, q[PI_I_k_Fe_ST2PS] * qFe);
+            FT f_PS2ST = q[PI_I_k_Fe_PS2ST] * qFePS * (FT)(falling + dark);
+            FT f_ST2NR = M::mask(early, M::mask(dark, q[PI_I_k_Fe_ST2NR] * qFe)) * q[PI_I_is_nr];
+            FT f_NR2ST = q[PI_I_k_Fe_NR2ST] * qFeNR * (FT)((dark && late) + light) * q[PI_I_is_nr];
+            FT f_ST2NF_cr = M::mask(early, M::mask(dark, q[PI_I_k_Fe_ST2NF] * qFe));
+            FT f_NF2ST_cr = q[PI_I_k_Fe_NF2ST] * qFeNF * (FT)((dark && late) + light);
+            FT f_ST2NF_tr = M::mask(rising, q[PI_I_k_Fe_ST2NF] * qFe);
+            FT f_NF2ST_tr = q[PI_I_k_Fe_NF2ST] * qFeNF * (FT)(falling + dark);
+            FT f_ST2NF = f_ST2NF_cr * q[PI_I_is_croc] + f_ST2NF_tr * q[PI_I_is_tric];
+            FT f_NF2ST = f_NF2ST_cr * q[PI_I_is_croc] + f_NF2ST_tr * q[PI_I_is_tric];
+            f_ST2PS = M::min(f_ST2PS, qFe / dt);
+            f_PS2ST = M::min(f_PS2ST, qFePS / dt);
+            f_ST2NR = M::min(f_ST2NR, qFe / dt - f_ST2PS);
+            f_NR2ST = M::min(f_NR2ST, qFeNR / dt);
+            f_ST2NF = M::min(f_ST2NF, qFe / dt - f_ST2PS);
+            f_NF2ST = M::min(f_NF2ST, qFeNF / dt);
+            p.ST2PS[i] = f_ST2PS; p.PS2ST[i] = f_PS2ST; p.ST2NR[i] = f_ST2NR;
+            p.NR2ST[i] = f_NR2ST; p.ST2NF[i] = f_ST2NF; p.NF2ST[i] = f_NF2ST;
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* update_biomass_kernel! */
+            p.Bm[i] = p.Bm[i] + dt * p.BS[i];
+            CH[i] = CH[i] - dt * p.BS[i];
+            qNH4[i] = qNH4[i] - dt * p.BS[i] * q[PI_I_R_NC];
+            qP[i] = qP[i] - dt * p.BS[i] * q[PI_I_R_PC];
+            p.Chl[i] = p.Chl[i] + dt * p.BS[i] * q[PI_I_R_NC] * p.rho[i];
+            p.age[i] = p.age[i] + M::mask(p.ac[i], dt / L(3600));
+            p.qFe[i] = p.qFe[i] + dt * (p.PS2ST[i] - p.ST2PS[i] + p.NR2ST[i] - p.ST2NR[i] + p.NF2ST[i] - p.ST2NF[i]);
+            p.qFePS[i] = p.qFePS[i] + dt * (p.ST2PS[i] - p.PS2ST[i]);
+            p.qFeNR[i] = p.qFeNR[i] + dt * (p.ST2NR[i] - p.NR2ST[i]);
+            p.qFeNF[i] = p.qFeNF[i] + dt * (p.ST2NF[i] - p.NF2ST[i]);
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i)     /* update_cellsize_kernel! */
+            p.Sz[i] = p.Bm[i] / (q[PI_I_Cquota] * q[PI_I_Nsuper]);
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i)     /* update_tdark_kernel! */
+            p.tdark[i] = p.tdark[i] * (one - (FT)M::isless(L(0), tpar[i])) + M::mask(M::isequal(L(0), tpar[i]), dt);
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i)     /* calc_graz_quadratic_kernel! */
+            p.graz[i] = M::mask(p.ac[i], tpop[i] * q[PI_I_grz_P]);
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i)     /* calc_mort_kernel! */
+            p.mort[i] = M::mask(p.ac[i], q[PI_I_mort_P] * shape_dec(p.Sz[i], q[PI_I_mort_reg], L(1.0e-5f), 4));
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* calc_dvid_kernel! */
+            FT P_ = q[PI_I_dvid_P], Sz = p.Sz[i], v;
+            FT hour = M::rem(t, L(86400)) / L(3600);
+            if (dvid_type[s] == 1) v = P_ * shape_inc(Sz, q[PI_I_dvid_reg], L(2.0e-3f), 2);
+            else if (dvid_type[s] == 2) v = P_ * shape_inc(hour, q[PI_I_dvid_reg], L(1.0e-5f), 4);
+            else v = P_ * shape_inc(Sz, q[PI_I_dvid_reg], L(1.0e-3f), 4) * shape_inc(hour, q[PI_I_dvid_reg2], L(1.0e-5f), 4);
+            p.dvid[i] = M::mask(p.ac[i], v);
+        }
+        OMP_FOR
+        for (int64_t i = 0; i < cap; ++i) {   /* get_probability! */
+            float a, b, c;
+            pi_uniform3(seed, iteration, s, PI_KIND_DECISION, i, &a, &b, &c);
+            bool ac = p.ac[i];
+            p.graz[i] = (M::isless((FT)a, p.graz[i] * dt) && ac) ? L(1) : L(0);
+            p.mort[i] = (M::isless((FT)b, p.mort[i] * dt) && ac) ? L(1) : L(0);
+            p.dvid[i] = (M::isless((FT)c, p.dvid[i] * dt) && ac) ? L(1) : L(0);
+        }
+    }
+
     /* ================================================== tracers ========= */
     /* Difference operators (Biogeochemistry/Advection/operaters.jl:1-27) */
     inline FT dxp(const FT* c, int i, int j, int k) const { return M::mask(LM(i, j, k), M::mask(LM(i + 1, j, k), c[I(i + 1, j, k)] - c[I(i, j, k)])); }
@@ -1560,6 +1875,10 @@ static pi_status upload_impl(Oracle<FT>* o, int32_t s, const pi_individuals_host
     cp(p.Nq, h->Nq); cp(p.Pq, h->Pq); cp(p.Chl, h->Chl); cp(p.gen, h->gen); cp(p.age, h->age);
     cp(p.graz, h->graz); cp(p.mort, h->mort); cp(p.dvid, h->dvid);
     if (o->mode == PI_MODE_MACROMOLECULAR) cp(p.RNA, h->ex[0]);
+    if (o->mode == PI_MODE_IRONENERGY) {
+        std::vector<FT>* dst[8] = {&p.qNO3, &p.qFe, &p.qFePS, &p.qFeNR, &p.qFeNF, &p.tdark, &p.exEn, &p.ptc};
+        for (int k = 0; k < 8; ++k) cp(*dst[k], h->ex[k]);
+    }
     for (int64_t i = 0; i < o->cap; ++i) {
         p.xi[i] = h->xi ? h->xi[i] : 0; p.yi[i] = h->yi ? h->yi[i] : 0; p.zi[i] = h->zi ? h->zi[i] : 0;
         p.ac[i] = h->ac ? (h->ac[i] != 0) : 0;
@@ -1682,6 +2001,10 @@ static pi_status field_impl(Oracle<FT>* o, int32_t s, int32_t f, OUT* out, int64
         case PI_F_graz: src = &p.graz; break; case PI_F_mort: src = &p.mort; break; case PI_F_dvid: src = &p.dvid; break;
         case PI_F_ex0: if (o->mode == PI_MODE_MACROMOLECULAR) src = &p.RNA; break;   /* PI_F_RNA */
         default: break;
+    }
+    if (o->mode == PI_MODE_IRONENERGY && f >= PI_F_ex0 && f < PI_F_ex0 + 8) {
+        const std::vector<FT>* ie[8] = {&p.qNO3, &p.qFe, &p.qFePS, &p.qFeNR, &p.qFeNF, &p.tdark, &p.exEn, &p.ptc};
+        src = ie[f - PI_F_ex0];
     }
     if (src) { for (int64_t i = 0; i < n; ++i) out[i] = (OUT)(*src)[i]; return PI_OK; }
     if (!ints_ok) return fail(PI_ERR_ARG, "field is not floating point");

@@ -34,7 +34,20 @@ MM_PARAM_NAMES = ("Nsuper", "C_DNA", "var", "PRO2DNA", "RNA2DNA", "CH2DNA", "Chl
                   "k_pro", "k_sat_pro", "k_rna", "k_sat_rna", "k_dna", "k_sat_dna",
                   "dvid_P", "grz_P", "mort_P", "mort_reg", "grazFracC", "grazFracN", "grazFracP",
                   "mortFracC", "mortFracN", "mortFracP")
-MODE_QUOTA, MODE_CARBON, MODE_MACROMOLECULAR = 0, 1, 2
+# IronEnergyMode parameter order (IronEnergyMode/plankton_generation.jl:28-40)
+IRON_PARAM_NAMES = ("Nsuper", "Cquota", "SA", "mean", "var", "Chl2Cint",
+                    "α", "Topt", "Tmax", "Ea", "is_nr", "is_croc", "is_tric",
+                    "PCmax", "VNO3max", "VNH4max", "VPO4max",
+                    "k_cf", "k_rs", "k_nr", "k_nf", "k_mtb",
+                    "e_cf", "e_rs", "e_nr", "e_nf",
+                    "k_Fe_ST2PS", "k_Fe_PS2ST", "k_Fe_ST2NR", "k_Fe_NR2ST", "k_Fe_ST2NF", "k_Fe_NF2ST",
+                    "KfePS", "KfeNR", "KfeNF", "KsatNH4", "KsatNO3", "KsatPO4", "KSAFe",
+                    "CHmax", "qNH4max", "qNO3max", "qPmax", "qFemax",
+                    "Chl2N", "R_NC", "R_PC", "NF_clock",
+                    "grz_P", "dvid_type", "dvid_P", "dvid_reg", "dvid_reg2", "mort_P", "mort_reg",
+                    "grazFracC", "grazFracN", "grazFracP", "grazFracFe",
+                    "mortFracC", "mortFracN", "mortFracP", "mortFracFe", "max_ptc")
+MODE_QUOTA, MODE_CARBON, MODE_MACROMOLECULAR, MODE_IRONENERGY = 0, 1, 2, 3
 NPARAMS = 64                                     # floats per species row (PI_NPARAMS)
 # abiotic particle parameter order (Abiotic/particle_generation.jl:8)
 ABIOTIC_PARAM_NAMES = ("Nsuper", "Rd", "release_P", "sz_min", "Ktr")
@@ -51,11 +64,19 @@ NEX = 8                                          # extra float columns (PI_NEX)
 EX_FIELDS = tuple(f"ex{k}" for k in range(NEX))
 FIELD_ID = {n: i for i, n in enumerate(FLOAT_FIELDS + INT_FIELDS + ("ac",) + EX_FIELDS)}
 # the columns of the other modes in the ABI's columns (PI_F_Bd, PI_F_PRO, ... of pi_hip.h)
-FIELD_ALIAS = {"Bd": "Cq", "PRO": "Bm", "DNA": "Sz", "RNA": "ex0", "CH": "Cq", "NST": "Nq", "PST": "Pq"}
+FIELD_ALIAS = {"Bd": "Cq", "PRO": "Bm", "DNA": "Sz", "RNA": "ex0", "CH": "Cq", "NST": "Nq", "PST": "Pq",
+               "qNH4": "Nq", "qP": "Pq", "qNO3": "ex0", "qFe": "ex1", "qFePS": "ex2", "qFeNR": "ex3",
+               "qFeNF": "ex4", "tdark": "ex5", "exEn": "ex6", "ptc": "ex7"}
 # the state columns of each mode's StructArray, in the ABI's names
 MODE_COLUMNS = {MODE_QUOTA: ("Sz", "Bm", "Cq", "Nq", "Pq", "Chl"),
                 MODE_CARBON: ("Sz", "Bm", "Cq", "Chl"),
-                MODE_MACROMOLECULAR: ("Sz", "Bm", "Cq", "Nq", "Pq", "Chl", "ex0")}
+                MODE_MACROMOLECULAR: ("Sz", "Bm", "Cq", "Nq", "Pq", "Chl", "ex0"),
+                MODE_IRONENERGY: ("Sz", "Bm", "Cq", "Nq", "Pq", "Chl") + EX_FIELDS}
+# the names of each mode's StructArray columns (the ABI's column names where they differ)
+MODE_NAMES = {MODE_QUOTA: ("Sz", "Bm", "Cq", "Nq", "Pq", "Chl"), MODE_CARBON: ("Sz", "Bm", "Bd", "Chl"),
+              MODE_MACROMOLECULAR: ("DNA", "PRO", "CH", "NST", "PST", "Chl", "RNA"),
+              MODE_IRONENERGY: ("Sz", "Bm", "CH", "qNH4", "qP", "Chl", "qNO3", "qFe", "qFePS", "qFeNR", "qFeNF",
+                                "tdark", "exEn", "ptc")}
 
 GRID_ID = {"par": 0, "par₀": 1, "par0": 1, "Chl": 2, "pop": 3, "temp": 4, "PARF": 5,
            "u": 6, "v": 7, "w": 8, "diag_PAR": 9}

@@ -111,6 +111,7 @@ struct FinalArgs {
     int run;                          /* levels per column run (launch shape, any value >= 1) */
     const struct BcTab* bc;           /* flux boundary conditions (the k_final2/3 instantiations) */
     uint8_t bcm[PI_NTRACERS];         /* per tracer, bit f set = a boundary condition on face f */
+    int plk_fe;                       /* IronEnergyMode: DFe and PFe_bio receive particle exchange too */
 };
 
 /* Flux boundary conditions of the tracers (set_bc!, boundary_conditions.jl:12-29),
@@ -222,7 +223,7 @@ enum KId {
     K_FINISH, K_XYW0, K_XYW1, K_PHYSC, K_FINAL2, K_FINAL3,
     K_ADV_GM, K_SWEEPX2, K_SWEEPX3, K_SWEEPY2, K_SWEEPY3, K_FINAL4, K_FINAL5, K_FINAL6, K_FINAL7,
     K_ABIO_MARK, K_ABIO_FREE, K_ABIO_LIST, K_ABIO_CHECK, K_ABIO_SPAWN, K_ABIO_ADV, K_ABIO_ADV_GM,
-    K_PHYSM, K_IDLEM, K_NKID
+    K_PHYSM, K_IDLEM, K_PHYSI, K_IDLEI, K_PLKI, K_PLK_LONGI, K_NKID
 };
 /* plan markers (not kernels): fork to / join from the side stream */
 enum { K_FORK = K_NKID, K_JOIN = K_NKID + 1 };

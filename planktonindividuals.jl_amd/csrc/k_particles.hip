@@ -1083,6 +1083,165 @@ __device__ void growth_mm(PState& s, const Trs& t, const float* q, float OGT, bo
     ddvid = jl_islessf(U2, dvid * dt) && ac;
 }
 
+/* IronEnergyMode plankton_growth! for one individual (IronEnergyMode/plankton_growth.jl:2-30,
+ * growth_kernels.jl:1-366, division_death_probability.jl).  Columns: Sz, Bm, Chl;
+ * Cq = CH, Nq = qNH4, Pq = qP; ex = qNO3, qFe, qFePS, qFeNR, qFeNF, tdark, exEn,
+ * ptc.  The rates returned for calc_consume! are PS, resp (= RS after the energy
+ * allocation), VNH4, VNO3, VPO4 and VFe (in VDOC's place). */
+__device__ __forceinline__ float ogt_iron(const float* q)
+{
+    return pi_expf(-q[PI_I_Ea] / (8.3145f * (q[PI_I_Topt] + 273.15f)));
+}
+__device__ void growth_iron(PState& s, const Trs& t, const float* q, float OGT, bool ac, float dt, float tmodel,
+                            int dvid_type, float U0, float U1, float U2, Rates& r,
+                            bool& dgraz, bool& dmort, bool& ddvid)
+{
+    const float e30 = 1.0e-30f;
+    float &CH = s.Cq, &qNH4 = s.Nq, &qP = s.Pq;
+    float &qNO3 = s.ex[0], &qFe = s.ex[1], &qFePS = s.ex[2], &qFeNR = s.ex[3], &qFeNF = s.ex[4];
+    float &tdark = s.ex[5], &exEn = s.ex[6];
+    const float is_nr = q[PI_I_is_nr], is_nf = q[PI_I_is_croc] + q[PI_I_is_tric];
+    /* tempFunc_CF / tempFunc (growth_kernels.jl:1-17) */
+    const float kx = pi_expf(-q[PI_I_Ea] / (8.3145f * (t.T + 273.15f)));
+    const float xT = t.T - q[PI_I_Topt], xmax = q[PI_I_Tmax] - q[PI_I_Topt];
+    const float tcf = dminf(1.0f, dmaxf(0.0f, kx * shape_dec(xT, xmax, 4.0e-2f, 4)) / OGT);
+    const float tf = dminf(1.0f, dmaxf(0.0f, kx) / OGT);
+    const float pp = dmaxf(1.0f, t.pop);
+    float VFe;
+    {   /* calc_trs_uptake_kernel! (calc_NH4_uptake, calc_NO3_uptake, calc_P_uptake, calc_Fe_uptake) */
+        const float BC = dmaxf(e30, s.Bm + CH);
+        const float rNH4 = shape_dec(qNH4 / BC, q[PI_I_qNH4max], 1.0e-4f, 2);
+        const float VNH4 = jl_maskf(ac, q[PI_I_VNH4max] * rNH4 * t.NH4 / dmaxf(e30, t.NH4 + q[PI_I_KsatNH4]) * tf * s.Bm);
+        r.VNH4 = dminf(VNH4, t.NH4 / dt / pp) * is_nr;
+        const float rNO3 = shape_dec(qNO3 / BC, q[PI_I_qNO3max], 1.0e-4f, 2);
+        const float VNO3 = jl_maskf(ac, q[PI_I_VNO3max] * rNO3 * t.NO3 / dmaxf(e30, t.NO3 + q[PI_I_KsatNO3]) * tf * s.Bm);
+        r.VNO3 = dminf(VNO3, t.NO3 / dt / pp) * is_nr;
+        const float rP = shape_dec((qP + s.Bm * q[PI_I_R_PC]) / BC, q[PI_I_qPmax], 1.0e-4f, 2);
+        const float VPO4 = jl_maskf(ac, q[PI_I_VPO4max] * rP * t.PO4 / dmaxf(e30, t.PO4 + q[PI_I_KsatPO4]) * tf * s.Bm);
+        r.VPO4 = dminf(VPO4, t.PO4 / dt / pp);
+        const float rFe = shape_dec((qFe + qFePS + qFeNF + qFeNR) / BC, q[PI_I_qFemax], 1.0e-4f, 2);
+        const float SA = q[PI_I_SA] * pi_powf(s.Sz, 2.0f / 3.0f);
+        const float V = jl_maskf(ac, q[PI_I_KSAFe] * SA * t.DFe * rFe * q[PI_I_Nsuper] * tf);
+        VFe = dminf(V, t.DFe / dt / pp);
+    }
+    /* update_state_1_kernel! */
+    qP = qP + r.VPO4 * dt;
+    qNO3 = qNO3 + r.VNO3 * dt;
+    qNH4 = qNH4 + r.VNH4 * dt;
+    qFe = qFe + VFe * dt;
+    /* calc_PS_kernel! (not masked by ac) */
+    float PS;
+    {
+        const float aI = t.par * q[PI_I_alpha] * s.Chl / dmaxf(e30, s.Bm);
+        const float Qps = qFePS / dmaxf(e30, s.Bm + CH);
+        const float Ksat = Qps / dmaxf(e30, Qps + q[PI_I_KfePS]);
+        PS = q[PI_I_PCmax] * (1.0f - pi_expf(-aI)) * s.Bm * Ksat;
+    }
+    /* calc_respiration_kernel! */
+    float RS = jl_maskf(ac, CH * q[PI_I_k_rs] * tf);
+    RS = dminf(RS, CH / dt);
+    float ERS = jl_maskf(ac, RS * q[PI_I_e_rs]);
+    /* calc_carbon_fixation_kernel! */
+    const float regQC = shape_dec(CH / dmaxf(e30, s.Bm + CH), q[PI_I_CHmax], 1.0e-4f, 2);
+    float CF = jl_maskf(ac, q[PI_I_k_cf] * regQC * tcf * s.Bm);
+    float ECF = jl_maskf(ac, CF * q[PI_I_e_cf]);
+    /* calc_NO3_reduction_kernel! */
+    const float regN = shape_dec(qNH4 / dmaxf(e30, s.Bm + CH), q[PI_I_qNH4max], 1.0e-4f, 2);
+    float NR, ENR;
+    {
+        const float Qnr = qFeNR / dmaxf(e30, s.Bm + CH);
+        const float Ksat = Qnr / dmaxf(e30, Qnr + q[PI_I_KfeNR]);
+        float n = jl_maskf(ac, q[PI_I_k_nr] * regN * qNO3 * Ksat * tf);
+        n = dminf(n, qNO3 / dt);
+        const float en = jl_maskf(ac, n * q[PI_I_e_nr]);
+        NR = n * is_nr;
+        ENR = en * is_nr;
+    }
+    /* calc_nitrogen_fixation_kernel! */
+    float NF, ENF;
+    {
+        const float Qnf = qFeNF / dmaxf(e30, s.Bm + CH);
+        const float Ksat = Qnf / dmaxf(e30, Qnf + q[PI_I_KfeNF]);
+        const float n = jl_maskf(ac, q[PI_I_k_nf] * regN * Ksat * tf * s.Bm);
+        const float en = jl_maskf(ac, n * q[PI_I_e_nf]);
+        NF = n * is_nf;
+        ENF = en * is_nf;
+    }
+    /* energy_allocation_kernel! (energy_alloc) */
+    {
+        const float ECFt = dminf(PS, ECF);
+        const float ENFt = dminf(ENF, PS + ERS - ECFt) * is_nf;
+        const float ENRt = dminf(ENR, PS + ERS - ECFt) * is_nr;
+        const float ERSt = dminf(ERS, dmaxf(0.0f, ECFt + ENFt + ENRt - PS));
+        exEn = dmaxf(0.0f, PS - ECFt - ENFt - ENRt);
+        ERS = ERSt; ECF = ECFt;
+        RS = ERS / q[PI_I_e_rs];
+        CF = ECF / q[PI_I_e_cf];
+        NF = ENFt / q[PI_I_e_nf];
+        NR = ENRt / q[PI_I_e_nr];
+    }
+    /* update_state_2_kernel! */
+    CH = CH + (CF - RS) * dt;
+    qNO3 = qNO3 + -NR * dt;
+    qNH4 = qNH4 + NR * dt;
+    qNH4 = qNH4 + NF * dt;
+    /* calc_ρChl_kernel! */
+    float rho = q[PI_I_Chl2N] / dmaxf(e30, t.par * q[PI_I_alpha] * s.Chl / s.Bm);
+    rho = jl_maskf(ac, jl_maskf(jl_islessf(1.0e-1f, t.par), rho));
+    /* calc_BS_kernel! */
+    const float BS = jl_maskf(ac, dminf(dminf(CH, qNH4 / q[PI_I_R_NC]), qP / q[PI_I_R_PC]) * q[PI_I_k_mtb]);
+    /* calc_iron_fluxes_kernel! (iron_alloc, not masked by ac); Bool sums are Int */
+    float ST2PS, PS2ST, ST2NR, NR2ST, ST2NF, NF2ST;
+    {
+        const bool dark = jl_isequalf(0.0f, t.par), light = jl_islessf(0.0f, t.par);
+        const bool rising = jl_islessf(0.0f, t.dpar), falling = jl_islessf(t.dpar, 0.0f);
+        const bool early = jl_islessf(tdark, q[PI_I_NF_clock]), late = jl_islessf(q[PI_I_NF_clock], tdark);
+        ST2PS = jl_maskf(rising, q[PI_I_k_Fe_ST2PS] * qFe);
+        PS2ST = q[PI_I_k_Fe_PS2ST] * qFePS * (float)((int)falling + (int)dark);
+        ST2NR = jl_maskf(early, jl_maskf(dark, q[PI_I_k_Fe_ST2NR] * qFe)) * q[PI_I_is_nr];
+        NR2ST = q[PI_I_k_Fe_NR2ST] * qFeNR * (float)((int)(dark && late) + (int)light) * q[PI_I_is_nr];
+        const float ST2NF_cr = jl_maskf(early, jl_maskf(dark, q[PI_I_k_Fe_ST2NF] * qFe));
+        const float NF2ST_cr = q[PI_I_k_Fe_NF2ST] * qFeNF * (float)((int)(dark && late) + (int)light);
+        const float ST2NF_tr = jl_maskf(rising, q[PI_I_k_Fe_ST2NF] * qFe);
+        const float NF2ST_tr = q[PI_I_k_Fe_NF2ST] * qFeNF * (float)((int)falling + (int)dark);
+        ST2NF = ST2NF_cr * q[PI_I_is_croc] + ST2NF_tr * q[PI_I_is_tric];
+        NF2ST = NF2ST_cr * q[PI_I_is_croc] + NF2ST_tr * q[PI_I_is_tric];
+        ST2PS = dminf(ST2PS, qFe / dt);
+        PS2ST = dminf(PS2ST, qFePS / dt);
+        ST2NR = dminf(ST2NR, qFe / dt - ST2PS);
+        NR2ST = dminf(NR2ST, qFeNR / dt);
+        ST2NF = dminf(ST2NF, qFe / dt - ST2PS);
+        NF2ST = dminf(NF2ST, qFeNF / dt);
+    }
+    /* update_biomass_kernel! */
+    s.Bm = s.Bm + dt * BS;
+    CH = CH - dt * BS;
+    qNH4 = qNH4 - dt * BS * q[PI_I_R_NC];
+    qP = qP - dt * BS * q[PI_I_R_PC];
+    s.Chl = s.Chl + dt * BS * q[PI_I_R_NC] * rho;
+    s.age = s.age + jl_maskf(ac, dt / 3600.0f);
+    qFe = qFe + dt * (PS2ST - ST2PS + NR2ST - ST2NR + NF2ST - ST2NF);
+    qFePS = qFePS + dt * (ST2PS - PS2ST);
+    qFeNR = qFeNR + dt * (ST2NR - NR2ST);
+    qFeNF = qFeNF + dt * (ST2NF - NF2ST);
+    /* update_cellsize_kernel!, update_tdark_kernel! */
+    s.Sz = s.Bm / (q[PI_I_Cquota] * q[PI_I_Nsuper]);
+    tdark = tdark * (1.0f - (float)jl_islessf(0.0f, t.par)) + jl_maskf(jl_isequalf(0.0f, t.par), dt);
+    r.PS = PS; r.resp = RS; r.VDOC = VFe; r.BS = BS; r.exu = 0.0f; r.rho = rho;
+    /* calc_graz_quadratic!, calc_mort!, calc_dvid!, get_probability! */
+    const float graz = jl_maskf(ac, t.pop * q[PI_I_grz_P]);
+    const float mort = jl_maskf(ac, q[PI_I_mort_P] * shape_dec(s.Sz, q[PI_I_mort_reg], 1.0e-5f, 4));
+    float dv;
+    const float hour = fmodf(tmodel, 86400.0f) / 3600.0f;
+    if (dvid_type == 1) dv = q[PI_I_dvid_P] * shape_inc(s.Sz, q[PI_I_dvid_reg], 2.0e-3f, 2);
+    else if (dvid_type == 2) dv = q[PI_I_dvid_P] * shape_inc(hour, q[PI_I_dvid_reg], 1.0e-5f, 4);
+    else dv = q[PI_I_dvid_P] * shape_inc(s.Sz, q[PI_I_dvid_reg], 1.0e-3f, 4) * shape_inc(hour, q[PI_I_dvid_reg2], 1.0e-5f, 4);
+    const float dvid = jl_maskf(ac, dv);
+    dgraz = jl_islessf(U0, graz * dt) && ac;
+    dmort = jl_islessf(U1, mort * dt) && ac;
+    ddvid = jl_islessf(U2, dvid * dt) && ac;
+}
+
 /* k-th set bit (0-based) of a word */
 __device__ __forceinline__ int select_bit(uint32_t w, uint32_t k)
 {
@@ -1113,7 +1272,11 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
 {
     constexpr bool CARBON = MODE == PI_MODE_CARBON;
     constexpr bool MM = MODE == PI_MODE_MACROMOLECULAR;
+    constexpr bool IRON = MODE == PI_MODE_IRONENERGY;
     constexpr int NEX = mode_nex(MODE);
+    /* the extra columns divide_to_half! halves: RNA; the iron pools, qNO3 and
+     * exEn (not tdark or ptc) */
+    constexpr uint32_t HALF = MM ? 0x1u : IRON ? 0x5Fu : 0u;
     const DevState* st = a.st;
     if (st->err) return;
     const DevGrid& g = a.g;
@@ -1139,7 +1302,7 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
     __syncthreads();
     for (int i = threadIdx.x; i < a.nsp; i += blockDim.x)
         s_ogt[i] = CARBON ? ogt_carbon(s_par + i * PI_NPARAMS) : MM ? ogt_mm(s_par + i * PI_NPARAMS)
-                                                                     : ogt_of(s_par + i * PI_NPARAMS);
+                 : IRON ? ogt_iron(s_par + i * PI_NPARAMS) : ogt_of(s_par + i * PI_NPARAMS);
     __syncthreads();
     const uint32_t hhh = (uint32_t)gidx(g, g.Hx, g.Hy, g.Hz);
     const uint32_t stride = K.n * blockDim.x;
@@ -1161,7 +1324,8 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
         const int sp = (int)(key >> PI_SLOT_BITS);
         const uint32_t slot = key & PI_SLOT_MASK;
         const float* q = s_par + sp * PI_NPARAMS;
-        const int dvid_type = MM ? 1 : (int)q[CARBON ? (int)PI_C_dvid_type : (int)PI_P_dvid_type];
+        const int dvid_type = MM ? 1 : (int)q[CARBON ? (int)PI_C_dvid_type : IRON ? (int)PI_I_dvid_type
+                                                                                     : (int)PI_P_dvid_type];
         const float OGT = s_ogt[sp];
         const uint8_t fl = IN.flags[src];
         PState s;
@@ -1188,13 +1352,34 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
         /* grazing! then mortality! (QuotaMode/division_death.jl:2-25, consume_loss.jl:17-33) */
         bool ac = true;
         uint8_t tag = 0;
-        float L0 = 0.f, L1 = 0.f, L2 = 0.f, L3 = 0.f, L4 = 0.f, L5 = 0.f;
+        float L0 = 0.f, L1 = 0.f, L2 = 0.f, L3 = 0.f, L4 = 0.f, L5 = 0.f, L6 = 0.f, L7 = 0.f, L8 = 0.f, L9 = 0.f;
         if (CARBON && (fl & (F_GRAZ | F_MORT))) {
             /* calc_loss! of CarbonMode (CarbonMode/consume_loss.jl:11-21): Bm to DOC and POC */
             const bool gz = (fl & F_GRAZ) != 0;
             const float fC = gz ? q[PI_C_grazFracC] : q[PI_C_mortFracC];
             L0 = s.Bm * fC * 1.0f;
             L3 = s.Bm * (1.0f - fC) * 1.0f;
+            tag = gz ? T_GRAZ : T_MORT;
+            ac = false;
+        } else if (IRON && (fl & (F_GRAZ | F_MORT))) {
+            /* calc_loss! of IronEnergyMode (IronEnergyMode/consume_loss.jl:18-33): the NH4
+             * and NO3 reserves as they are, C, N, P and Fe split by the fractions */
+            const bool gz = (fl & F_GRAZ) != 0;
+            const float fC = gz ? q[PI_I_grazFracC] : q[PI_I_mortFracC];
+            const float fN = gz ? q[PI_I_grazFracN] : q[PI_I_mortFracN];
+            const float fP = gz ? q[PI_I_grazFracP] : q[PI_I_mortFracP];
+            const float fFe = gz ? q[PI_I_grazFracFe] : q[PI_I_mortFracFe];
+            const float Fe = s.ex[1] + s.ex[2] + s.ex[4] + s.ex[3];     /* qFe + qFePS + qFeNF + qFeNR */
+            L6 = s.Nq * 1.0f;
+            L7 = s.ex[0] * 1.0f;
+            L0 = (s.Bm + s.Cq) * fC * 1.0f;
+            L1 = s.Bm * q[PI_I_R_NC] * fN * 1.0f;
+            L2 = (s.Bm * q[PI_I_R_PC] + s.Pq) * fP * 1.0f;
+            L8 = Fe * fFe * 1.0f;
+            L3 = (s.Bm + s.Cq) * (1.0f - fC) * 1.0f;
+            L4 = s.Bm * q[PI_I_R_NC] * (1.0f - fN) * 1.0f;
+            L5 = (s.Bm * q[PI_I_R_PC] + s.Pq) * (1.0f - fP) * 1.0f;
+            L9 = Fe * (1.0f - fFe) * 1.0f;
             tag = gz ? T_GRAZ : T_MORT;
             ac = false;
         } else if (MM && (fl & (F_GRAZ | F_MORT))) {
@@ -1262,7 +1447,8 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
             s.Sz = s.Sz * h; s.Bm = s.Bm * h; s.Cq = s.Cq * h; s.Nq = s.Nq * h; s.Pq = s.Pq * h; s.Chl = s.Chl * h;
             s.gen = s.gen + 1.0f; s.age = s.age * (1.0f - 1.0f);
             d.Sz = d.Sz * h; d.Bm = d.Bm * h; d.Cq = d.Cq * h; d.Nq = d.Nq * h; d.Pq = d.Pq * h; d.Chl = d.Chl * h;
-            for (int k = 0; k < NEX; ++k) { s.ex[k] = s.ex[k] * h; d.ex[k] = d.ex[k] * h; }
+            for (int k = 0; k < NEX; ++k)
+                if ((HALF >> k) & 1u) { s.ex[k] = s.ex[k] * h; d.ex[k] = d.ex[k] * h; }
             d.gen = d.gen + 1.0f;
             d.age = a.tab.age[dr] * (1.0f - 1.0f);
             /* The daughter's slot kept what find_inds!/find_NPT! wrote into it this
@@ -1289,13 +1475,15 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
             bool gz, mt, dv;
             if (CARBON) growth_carbon(d, tz, q, OGT, true, dt, tmodel, dvid_type, U0, U1, U2, rd, gz, mt, dv);
             else if (MM) growth_mm(d, tz, q, OGT, true, dt, U0, U1, U2, rd, gz, mt, dv);
+            else if (IRON) growth_iron(d, tz, q, OGT, true, dt, tmodel, dvid_type, U0, U1, U2, rd, gz, mt, dv);
             else growth(d, tz, q, OGT, true, dt, tmodel, dvid_type, U0, U1, U2, rd, gz, mt, dv);
             if (reborn) {   /* its calc_consume! lands in that cell, at the slot's place */
-                a.rcons[dr * 5 + 0] = (rd.resp - rd.PS) * dt;
-                a.rcons[dr * 5 + 1] = CARBON ? 0.0f : (rd.exu - rd.VDOC) * dt;
-                a.rcons[dr * 5 + 2] = CARBON ? 0.0f : -rd.VNH4 * dt;
-                a.rcons[dr * 5 + 3] = CARBON ? 0.0f : -rd.VNO3 * dt;
-                a.rcons[dr * 5 + 4] = CARBON ? 0.0f : -rd.VPO4 * dt;
+                a.rcons[dr * PI_RCONS + 0] = (rd.resp - rd.PS) * dt;
+                a.rcons[dr * PI_RCONS + 1] = (CARBON || IRON) ? 0.0f : (rd.exu - rd.VDOC) * dt;
+                a.rcons[dr * PI_RCONS + 2] = CARBON ? 0.0f : -rd.VNH4 * dt;
+                a.rcons[dr * PI_RCONS + 3] = CARBON ? 0.0f : -rd.VNO3 * dt;
+                a.rcons[dr * PI_RCONS + 4] = CARBON ? 0.0f : -rd.VPO4 * dt;
+                a.rcons[dr * PI_RCONS + 5] = IRON ? -rd.VDOC * dt : 0.0f;   /* DFe (IronEnergyMode: VFe) */
                 a.rstamp[dr] = (int32_t)iteration;
             }
             const uint32_t o = n + s_doff[sp] + r;
@@ -1315,17 +1503,18 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
         bool gz, mt, dv;
         if (CARBON) growth_carbon(s, t, q, OGT, ac, dt, tmodel, dvid_type, U0, U1, U2, r, gz, mt, dv);
         else if (MM) growth_mm(s, t, q, OGT, ac, dt, U0, U1, U2, r, gz, mt, dv);
+        else if (IRON) growth_iron(s, t, q, OGT, ac, dt, tmodel, dvid_type, U0, U1, U2, r, gz, mt, dv);
         else growth(s, t, q, OGT, ac, dt, tmodel, dvid_type, U0, U1, U2, r, gz, mt, dv);
         if (ac) {
             /* calc_consume! (QuotaMode/consume_loss.jl:2-14, MacroMolecularMode/consume_loss.jl:2-9;
              * CarbonMode: DIC only) */
             tag = T_ALIVE | (divider ? T_DIV : 0);
             L0 = (r.resp - r.PS) * dt;
-            L1 = CARBON ? 0.0f : (r.exu - r.VDOC) * dt;
+            L1 = (CARBON || IRON) ? 0.0f : (r.exu - r.VDOC) * dt;
             L2 = CARBON ? 0.0f : -r.VNH4 * dt;
             L3 = CARBON ? 0.0f : -r.VNO3 * dt;
             L4 = CARBON ? 0.0f : -r.VPO4 * dt;
-            L5 = 0.0f;
+            L5 = IRON ? -r.VDOC * dt : 0.0f;     /* DFe (IronEnergyMode: VFe) */
         } else {
             /* inactive from now on: record its last state in the slot table */
             const long long ts = (long long)sp * a.cap + slot;
@@ -1356,25 +1545,32 @@ __device__ __forceinline__ void k_phys_body(const PhysArgs& a, Blk K)
             a.sflag[(long long)sp * a.cap + slot] = ofl;
         a.con.v[0][e] = L0; a.con.v[1][e] = L1; a.con.v[2][e] = L2;
         a.con.v[3][e] = L3; a.con.v[4][e] = L4; a.con.v[5][e] = L5;
+        if (IRON) { a.con.v[6][e] = L6; a.con.v[7][e] = L7; a.con.v[8][e] = L8; a.con.v[9][e] = L9; }
         a.con.tag[e] = tag;
     }
 }
 __device__ __forceinline__ void k_physq_body(const PhysArgs& a, Blk K) { k_phys_body<PI_MODE_QUOTA>(a, K); }
 __device__ __forceinline__ void k_physc_body(const PhysArgs& a, Blk K) { k_phys_body<PI_MODE_CARBON>(a, K); }
 __device__ __forceinline__ void k_physm_body(const PhysArgs& a, Blk K) { k_phys_body<PI_MODE_MACROMOLECULAR>(a, K); }
+__device__ __forceinline__ void k_physi_body(const PhysArgs& a, Blk K) { k_phys_body<PI_MODE_IRONENERGY>(a, K); }
 PI_KERNEL3(k_phys, k_physq_body, PhysArgs, 256, xcd_block())
 PI_KERNEL3(k_physc, k_physc_body, PhysArgs, 256, xcd_block())
 PI_KERNEL3(k_physm, k_physm_body, PhysArgs, 256, xcd_block())
+PI_KERNEL3(k_physi, k_physi_body, PhysArgs, 256, xcd_block())
 
-/* MacroMolecularMode: the slots inactive through the whole step.  Their
- * plankton_growth! runs with ac = false and the zeros that find_NPT! gathered
- * for them, and changes them: protein, DNA and RNA synthesis
- * (calc_BS_kernel!), update_quotas_2 and the exudation are not masked by ac.
- * Rows written in this step (an individual dying now, death = this
- * iteration) already hold their step; live individuals' rows are stale copies
- * (the store holds them) and evolve to no effect. */
-__device__ __forceinline__ void k_idlem_body(const IdleArgs& a, Blk K)
+/* MacroMolecularMode and IronEnergyMode: the slots inactive through the whole
+ * step.  Their plankton_growth! runs with ac = false and the zeros that
+ * find_NPT! gathered for them, and changes them: in MacroMolecularMode protein,
+ * DNA and RNA synthesis (calc_BS_kernel!), update_quotas_2 and the exudation, in
+ * IronEnergyMode the iron allocation (iron_alloc with par = dpar = 0), the dark
+ * clock tdark and exEn are not masked by ac.  Rows written in this step (an
+ * individual dying now, death = this iteration) already hold their step; live
+ * individuals' rows are stale copies (the store holds them) and evolve to no
+ * effect. */
+template <int MODE>
+__device__ __forceinline__ void k_idle_body(const IdleArgs& a, Blk K)
 {
+    constexpr int NEX = mode_nex(MODE);
     const DevState* st = a.st;
     if (st->err) return;
     const int32_t it = (int32_t)(st->iteration + 1);
@@ -1388,15 +1584,21 @@ __device__ __forceinline__ void k_idlem_body(const IdleArgs& a, Blk K)
         PState s;
         s.x = s.y = s.z = s.gen = 0.0f;
         s.Sz = T.Sz[r]; s.Bm = T.Bm[r]; s.Cq = T.Cq[r]; s.Nq = T.Nq[r]; s.Pq = T.Pq[r];
-        s.Chl = T.Chl[r]; s.age = T.age[r]; s.ex[0] = T.ex[0][r];
+        s.Chl = T.Chl[r]; s.age = T.age[r];
+        for (int k = 0; k < NEX; ++k) s.ex[k] = T.ex[k][r];
         Rates rr;
         bool gz, mt, dv;
-        growth_mm(s, t, q, ogt_mm(q), false, a.dt, 1.0f, 1.0f, 1.0f, rr, gz, mt, dv);
+        if (MODE == PI_MODE_MACROMOLECULAR) growth_mm(s, t, q, ogt_mm(q), false, a.dt, 1.0f, 1.0f, 1.0f, rr, gz, mt, dv);
+        else growth_iron(s, t, q, ogt_iron(q), false, a.dt, 0.0f, 1, 1.0f, 1.0f, 1.0f, rr, gz, mt, dv);
         T.Sz[r] = s.Sz; T.Bm[r] = s.Bm; T.Cq[r] = s.Cq; T.Nq[r] = s.Nq; T.Pq[r] = s.Pq;
-        T.Chl[r] = s.Chl; T.age[r] = s.age; T.ex[0][r] = s.ex[0];
+        T.Chl[r] = s.Chl; T.age[r] = s.age;
+        for (int k = 0; k < NEX; ++k) T.ex[k][r] = s.ex[k];
     }
 }
+__device__ __forceinline__ void k_idlem_body(const IdleArgs& a, Blk K) { k_idle_body<PI_MODE_MACROMOLECULAR>(a, K); }
+__device__ __forceinline__ void k_idlei_body(const IdleArgs& a, Blk K) { k_idle_body<PI_MODE_IRONENERGY>(a, K); }
 PI_KERNEL(k_idlem, IdleArgs, 256)
+PI_KERNEL(k_idlei, IdleArgs, 256)
 
 /* Ordered per-cell sums: calc_loss! (graz), calc_loss! (mort), calc_consume! per
  * species in slot order (plankton_update.jl:5-29), and the per-species
@@ -1406,17 +1608,24 @@ PI_KERNEL(k_idlem, IdleArgs, 256)
  * 0 = grazing loss, 1 = mortality loss, 2 = consumption (alive, or a daughter
  * reborn into the slot of this dying entry) */
 struct PlkSums {
-    float DIC, NH4, NO3, PO4, DOC, DON, DOP, POC, PON, POP;
+    float DIC, NH4, NO3, PO4, DOC, DON, DOP, POC, PON, POP, DFe, PFe;
     uint32_t ng, nm, nd, na;
 };
+/* a loss: v0..v5 to DOC DON DOP POC PON POP; IronEnergyMode also v6..v9 to NH4
+ * NO3 DFe PFe_bio (IronEnergyMode/consume_loss.jl:18-33) */
+template <bool IRON>
 __device__ __forceinline__ void plk_loss(PlkSums& S, const float* v)
 {
     S.DOC = S.DOC + v[0]; S.DON = S.DON + v[1]; S.DOP = S.DOP + v[2];
     S.POC = S.POC + v[3]; S.PON = S.PON + v[4]; S.POP = S.POP + v[5];
+    if (IRON) { S.NH4 = S.NH4 + v[6]; S.NO3 = S.NO3 + v[7]; S.DFe = S.DFe + v[8]; S.PFe = S.PFe + v[9]; }
 }
-__device__ __forceinline__ void plk_cons(PlkSums& S, float c0, float c1, float c2, float c3, float c4)
+/* a consumption: DIC DOC NH4 NO3 PO4 (+ DFe in IronEnergyMode) */
+template <bool IRON>
+__device__ __forceinline__ void plk_cons(PlkSums& S, float c0, float c1, float c2, float c3, float c4, float c5)
 {
     S.DIC = S.DIC + c0; S.DOC = S.DOC + c1; S.NH4 = S.NH4 + c2; S.NO3 = S.NO3 + c3; S.PO4 = S.PO4 + c4;
+    if (IRON) S.DFe = S.DFe + c5;
 }
 /* the per-species diagnostics of a cell as atomic additions without return:
  * no load for a later store to wait on, and the counts are integers, so the
@@ -1433,8 +1642,10 @@ __device__ __forceinline__ void plk_diag(const PlkArgs& a, long long c, uint32_t
 
 #define PLK_FAST 6   /* segments up to this length are loaded into registers at once */
 
+template <bool IRON>
 __device__ __forceinline__ void k_plk_body(const PlkArgs& a, Blk K)
 {
+    constexpr int NV = IRON ? 10 : 6;        /* loss values per entry */
     if (a.st->err) return;
     const int32_t it = (int32_t)(a.st->iteration + 1);
     const DevGrid& g = a.g;
@@ -1455,14 +1666,14 @@ __device__ __forceinline__ void k_plk_body(const PlkArgs& a, Blk K)
              * species run on registers (predicated, in slot order) */
             uint32_t key[PLK_FAST];
             uint8_t tag[PLK_FAST];
-            float v[PLK_FAST][6];
+            float v[PLK_FAST][NV];
 #pragma unroll
             for (int x = 0; x < PLK_FAST; ++x) {
                 const bool in = (uint32_t)x < n;
                 key[x] = in ? a.pkey[s0 + x] : 0xFFFFFFFFu;
                 tag[x] = in ? a.con.tag[s0 + x] : (uint8_t)0;
 #pragma unroll
-                for (int j = 0; j < 6; ++j) v[x][j] = in ? a.con.v[j][s0 + x] : 0.0f;
+                for (int j = 0; j < NV; ++j) v[x][j] = in ? a.con.v[j][s0 + x] : 0.0f;
             }
             uint32_t e = 0;
             while (e < n) {
@@ -1475,23 +1686,23 @@ __device__ __forceinline__ void k_plk_body(const PlkArgs& a, Blk K)
                     if ((uint32_t)x > e && (uint32_t)x < n && (key[x] >> PI_SLOT_BITS) != sp) e1 = (uint32_t)x;
 #pragma unroll
                 for (int x = 0; x < PLK_FAST; ++x)
-                    if ((uint32_t)x >= e && (uint32_t)x < e1 && (tag[x] & T_GRAZ)) { plk_loss(S, v[x]); ++S.ng; }
+                    if ((uint32_t)x >= e && (uint32_t)x < e1 && (tag[x] & T_GRAZ)) { plk_loss<IRON>(S, v[x]); ++S.ng; }
 #pragma unroll
                 for (int x = 0; x < PLK_FAST; ++x)
-                    if ((uint32_t)x >= e && (uint32_t)x < e1 && (tag[x] & T_MORT)) { plk_loss(S, v[x]); ++S.nm; }
+                    if ((uint32_t)x >= e && (uint32_t)x < e1 && (tag[x] & T_MORT)) { plk_loss<IRON>(S, v[x]); ++S.nm; }
 #pragma unroll
                 for (int x = 0; x < PLK_FAST; ++x) {
                     if (!((uint32_t)x >= e && (uint32_t)x < e1)) continue;
                     if (tag[x] & T_ALIVE) {
-                        plk_cons(S, v[x][0], v[x][1], v[x][2], v[x][3], v[x][4]);
+                        plk_cons<IRON>(S, v[x][0], v[x][1], v[x][2], v[x][3], v[x][4], v[x][5]);
                         ++S.na;
                         if (tag[x] & T_DIV) ++S.nd;
                     } else {
                         /* died here this step; a daughter born into its slot consumes here */
                         const long long r = (long long)sp * a.cap + (key[x] & PI_SLOT_MASK);
                         if (a.rstamp[r] == it) {
-                            const float* rc = a.rcons + r * 5;
-                            plk_cons(S, rc[0], rc[1], rc[2], rc[3], rc[4]);
+                            const float* rc = a.rcons + r * PI_RCONS;
+                            plk_cons<IRON>(S, rc[0], rc[1], rc[2], rc[3], rc[4], rc[5]);
                             ++S.na;
                         }
                     }
@@ -1505,30 +1716,31 @@ __device__ __forceinline__ void k_plk_body(const PlkArgs& a, Blk K)
                 const uint32_t sp = a.pkey[e] >> PI_SLOT_BITS;
                 uint32_t e1 = e + 1;
                 while (e1 < s1 && (a.pkey[e1] >> PI_SLOT_BITS) == sp) ++e1;
-                float v[6];
+                float v[NV];
                 for (uint32_t x = e; x < e1; ++x) {
                     if (a.con.tag[x] & T_GRAZ) {
-                        for (int j = 0; j < 6; ++j) v[j] = a.con.v[j][x];
-                        plk_loss(S, v); ++S.ng;
+                        for (int j = 0; j < NV; ++j) v[j] = a.con.v[j][x];
+                        plk_loss<IRON>(S, v); ++S.ng;
                     }
                 }
                 for (uint32_t x = e; x < e1; ++x) {
                     if (a.con.tag[x] & T_MORT) {
-                        for (int j = 0; j < 6; ++j) v[j] = a.con.v[j][x];
-                        plk_loss(S, v); ++S.nm;
+                        for (int j = 0; j < NV; ++j) v[j] = a.con.v[j][x];
+                        plk_loss<IRON>(S, v); ++S.nm;
                     }
                 }
                 for (uint32_t x = e; x < e1; ++x) {
                     const uint8_t t = a.con.tag[x];
                     if (t & T_ALIVE) {
-                        plk_cons(S, a.con.v[0][x], a.con.v[1][x], a.con.v[2][x], a.con.v[3][x], a.con.v[4][x]);
+                        plk_cons<IRON>(S, a.con.v[0][x], a.con.v[1][x], a.con.v[2][x], a.con.v[3][x], a.con.v[4][x],
+                                       IRON ? a.con.v[5][x] : 0.0f);
                         ++S.na;
                         if (t & T_DIV) ++S.nd;
                     } else {
                         const long long r = (long long)sp * a.cap + (a.pkey[x] & PI_SLOT_MASK);
                         if (a.rstamp[r] == it) {
-                            const float* rc = a.rcons + r * 5;
-                            plk_cons(S, rc[0], rc[1], rc[2], rc[3], rc[4]);
+                            const float* rc = a.rcons + r * PI_RCONS;
+                            plk_cons<IRON>(S, rc[0], rc[1], rc[2], rc[3], rc[4], rc[5]);
                             ++S.na;
                         }
                     }
@@ -1540,9 +1752,13 @@ __device__ __forceinline__ void k_plk_body(const PlkArgs& a, Blk K)
         a.plk[PI_DIC][c] = S.DIC; a.plk[PI_NH4][c] = S.NH4; a.plk[PI_NO3][c] = S.NO3; a.plk[PI_PO4][c] = S.PO4;
         a.plk[PI_DOC][c] = S.DOC; a.plk[PI_DON][c] = S.DON; a.plk[PI_DOP][c] = S.DOP;
         a.plk[PI_POC][c] = S.POC; a.plk[PI_PON][c] = S.PON; a.plk[PI_POP][c] = S.POP;
+        if (IRON) { a.plk[PI_DFe][c] = S.DFe; a.plk[PI_PFe_bio][c] = S.PFe; }
     }
 }
-PI_KERNEL_XCD(k_plk, PlkArgs, 256)
+__device__ __forceinline__ void k_plkq_body(const PlkArgs& a, Blk K) { k_plk_body<false>(a, K); }
+__device__ __forceinline__ void k_plki_body(const PlkArgs& a, Blk K) { k_plk_body<true>(a, K); }
+PI_KERNEL3(k_plk, k_plkq_body, PlkArgs, 256, xcd_block())
+PI_KERNEL3(k_plki, k_plki_body, PlkArgs, 256, xcd_block())
 
 
 /* The ordered sums of k_plk for the listed long cells, one block per cell.
@@ -1554,12 +1770,12 @@ PI_KERNEL_XCD(k_plk, PlkArgs, 256)
  * one.  An entry that contributes nothing to a tracer is staged as +0.0: a
  * sum that starts at +0.0 never becomes −0.0, so adding +0.0 leaves it
  * unchanged, as skipping the entry does. */
-template <int U>
+template <int U, int NV>
 __device__ __forceinline__ void stage_loss(const PlkArgs& a, const uint32_t* L, uint32_t from, uint32_t nL,
                                            uint32_t t0, uint32_t step, float (*dst)[LPAD])
 {
     uint32_t x[U];
-    float v[U][6];
+    float v[U][NV];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const uint32_t i = t0 + u * step;
@@ -1568,23 +1784,23 @@ __device__ __forceinline__ void stage_loss(const PlkArgs& a, const uint32_t* L, 
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
-        for (int r = 0; r < 6; ++r) v[u][r] = x[u] != 0xFFFFFFFFu ? a.con.v[r][x[u]] : 0.0f;
+        for (int r = 0; r < NV; ++r) v[u][r] = x[u] != 0xFFFFFFFFu ? a.con.v[r][x[u]] : 0.0f;
 #pragma unroll
     for (int u = 0; u < U; ++u)
         if (x[u] != 0xFFFFFFFFu) {
 #pragma unroll
-            for (int r = 0; r < 6; ++r) dst[r][t0 + u * step] = v[u][r];
+            for (int r = 0; r < NV; ++r) dst[r][t0 + u * step] = v[u][r];
         }
 }
 /* returns (consuming entries) | (dividing entries) << 16 of the staged ones */
-template <int U>
+template <int U, int NC>
 __device__ __forceinline__ uint32_t stage_cons(const PlkArgs& a, uint32_t sp, int32_t it, uint32_t from, uint32_t e1,
                                                uint32_t t0, uint32_t step, float (*dst)[LPAD])
 {
     uint32_t na = 0, nd = 0;
     bool ok[U];
     uint8_t tg[U];
-    float v[U][5];
+    float v[U][NC];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const uint32_t i = t0 + u * step;
@@ -1594,7 +1810,7 @@ __device__ __forceinline__ uint32_t stage_cons(const PlkArgs& a, uint32_t sp, in
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
-        for (int r = 0; r < 5; ++r) v[u][r] = ok[u] ? a.con.v[r][from + t0 + u * step] : 0.0f;
+        for (int r = 0; r < NC; ++r) v[u][r] = ok[u] ? a.con.v[r][from + t0 + u * step] : 0.0f;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         if (!ok[u]) continue;
@@ -1607,19 +1823,21 @@ __device__ __forceinline__ uint32_t stage_cons(const PlkArgs& a, uint32_t sp, in
             const long long r = (long long)sp * a.cap + (a.pkey[x] & PI_SLOT_MASK);
             const bool rb = a.rstamp[r] == it;
 #pragma unroll
-            for (int q = 0; q < 5; ++q) v[u][q] = rb ? a.rcons[r * 5 + q] : 0.0f;
+            for (int q = 0; q < NC; ++q) v[u][q] = rb ? a.rcons[r * PI_RCONS + q] : 0.0f;
             na += rb ? 1u : 0u;
         }
 #pragma unroll
-        for (int q = 0; q < 5; ++q) dst[q][t0 + u * step] = v[u][q];
+        for (int q = 0; q < NC; ++q) dst[q][t0 + u * step] = v[u][q];
     }
     return na | (nd << 16);
 }
 
+template <bool IRON>
 __device__ __forceinline__ void k_plk_long_body(const PlkArgs& a, Blk K)
 {
+    constexpr int NV = IRON ? 10 : 6, NC = IRON ? 6 : 5;   /* staged loss / consumption rows */
     if (a.st->err) return;
-    __shared__ __align__(16) float sv[2][6][LPAD];
+    __shared__ __align__(16) float sv[2][NV][LPAD];
     __shared__ uint32_t ws[4];
     __shared__ uint32_t ncnt[2];
     const int32_t it = (int32_t)(a.st->iteration + 1);
@@ -1627,9 +1845,12 @@ __device__ __forceinline__ void k_plk_long_body(const PlkArgs& a, Blk K)
     const uint32_t t = threadIdx.x;
     const long long ncell = a.g.ncell;
     /* lane -> staged row: the loss passes (lanes 4-9: DOC DON DOP POC PON POP
-     * <- v0..v5) and the consumption pass (DIC DOC NH4 NO3 PO4 <- c0..c4) */
-    const int lrow = t >= 4 && t < 10 ? (int)t - 4 : -1;
-    const int crow = t == 0 ? 0 : t == 1 ? 2 : t == 2 ? 3 : t == 3 ? 4 : t == 4 ? 1 : -1;
+     * <- v0..v5; IronEnergyMode also lanes 1, 2, 10, 11: NH4 NO3 DFe PFe_bio
+     * <- v6..v9) and the consumption pass (DIC DOC NH4 NO3 PO4 <- c0..c4;
+     * IronEnergyMode also DFe <- c5) */
+    const int lrow = t >= 4 && t < 10 ? (int)t - 4
+                   : !IRON ? -1 : t == 1 ? 6 : t == 2 ? 7 : t == 10 ? 8 : t == 11 ? 9 : -1;
+    const int crow = t == 0 ? 0 : t == 1 ? 2 : t == 2 ? 3 : t == 3 ? 4 : t == 4 ? 1 : (IRON && t == 10) ? 5 : -1;
     for (uint32_t b = K.b; b < nl; b += K.n) {
         const long long c = a.lng[b];
         const uint32_t s0 = a.seg[c], s1 = a.seg[c + 1];
@@ -1674,25 +1895,25 @@ __device__ __forceinline__ void k_plk_long_body(const PlkArgs& a, Blk K)
                 const uint32_t* L = (pass == 0 ? a.lg : a.lm) + e;
                 const uint32_t nL = pass == 0 ? nG : nM;
                 if (nL == 0) continue;
-                stage_loss<4>(a, L, 0, nL, t, 256, sv[0]);
+                stage_loss<4, NV>(a, L, 0, nL, t, 256, sv[0]);
                 __syncthreads();
                 uint32_t k = 0;
                 for (uint32_t base = 0; base < nL; base += LCH, ++k) {
-                    if (t >= 64) { if (base + LCH < nL) stage_loss<6>(a, L, base + LCH, nL, t - 64, 192, sv[(k + 1) & 1]); }
+                    if (t >= 64) { if (base + LCH < nL) stage_loss<6, NV>(a, L, base + LCH, nL, t - 64, 192, sv[(k + 1) & 1]); }
                     else if (lrow >= 0) S = chain_sum(S, sv[k & 1][lrow], min((uint32_t)LCH, nL - base));
                     __syncthreads();
                 }
             }
             /* calc_consume! over every entry of the range */
             uint32_t na = 0, nd = 0;
-            uint32_t pk = stage_cons<4>(a, sp, it, e, e1, t, 256, sv[0]);
+            uint32_t pk = stage_cons<4, NC>(a, sp, it, e, e1, t, 256, sv[0]);
             na += pk & 0xFFFFu; nd += pk >> 16;
             __syncthreads();
             uint32_t k = 0;
             for (uint32_t base = e; base < e1; base += LCH, ++k) {
                 if (t >= 64) {
                     if (base + LCH < e1) {
-                        pk = stage_cons<6>(a, sp, it, base + LCH, e1, t - 64, 192, sv[(k + 1) & 1]);
+                        pk = stage_cons<6, NC>(a, sp, it, base + LCH, e1, t - 64, 192, sv[(k + 1) & 1]);
                         na += pk & 0xFFFFu; nd += pk >> 16;
                     }
                 } else if (crow >= 0) {
@@ -1714,13 +1935,17 @@ __device__ __forceinline__ void k_plk_long_body(const PlkArgs& a, Blk K)
             __syncthreads();
             e = e1;
         }
-        constexpr int QOF[10] = {PI_DIC, PI_NH4, PI_NO3, PI_PO4, PI_DOC, PI_DON, PI_DOP, PI_POC, PI_PON, PI_POP};
+        constexpr int QOF[12] = {PI_DIC, PI_NH4, PI_NO3, PI_PO4, PI_DOC, PI_DON, PI_DOP, PI_POC, PI_PON, PI_POP,
+                                 PI_DFe, PI_PFe_bio};
 #pragma unroll
-        for (int l = 0; l < 10; ++l)
+        for (int l = 0; l < (IRON ? 12 : 10); ++l)
             if (t == (uint32_t)l) a.plk[QOF[l]][c] = S;
     }
 }
-PI_KERNEL(k_plk_long, PlkArgs, 256)
+__device__ __forceinline__ void k_plk_longq_body(const PlkArgs& a, Blk K) { k_plk_long_body<false>(a, K); }
+__device__ __forceinline__ void k_plk_longi_body(const PlkArgs& a, Blk K) { k_plk_long_body<true>(a, K); }
+PI_KERNEL3(k_plk_long, k_plk_longq_body, PlkArgs, 256, blockIdx.x)
+PI_KERNEL3(k_plk_longi, k_plk_longi_body, PlkArgs, 256, blockIdx.x)
 
 /* ------------------------------------------------------ abiotic particles
  * Individuals/Abiotic/particle_interaction.jl:158-267 (particles_from_bcs!)
@@ -1903,6 +2128,10 @@ void reg_particles(KInfo* k)
     PI_REG(k, K_PHYSC, k_physc, PhysArgs, 256, true);
     PI_REG(k, K_PHYSM, k_physm, PhysArgs, 256, true);
     PI_REG(k, K_IDLEM, k_idlem, IdleArgs, 256, true);
+    PI_REG(k, K_PHYSI, k_physi, PhysArgs, 256, true);
+    PI_REG(k, K_IDLEI, k_idlei, IdleArgs, 256, true);
     PI_REG(k, K_PLK, k_plk, PlkArgs, 256, true);
     PI_REG(k, K_PLK_LONG, k_plk_long, PlkArgs, 256, true);
+    PI_REG(k, K_PLKI, k_plki, PlkArgs, 256, true);
+    PI_REG(k, K_PLK_LONGI, k_plk_longi, PlkArgs, 256, true);
 }

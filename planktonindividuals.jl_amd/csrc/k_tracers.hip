@@ -403,7 +403,8 @@ __device__ __forceinline__ int halo_src(int p, int H, int N, int topo)
 
 /* ---------------------------------------------- z sweep + the rest of the update */
 
-/* the tracers that receive particle exchange (plk), tracer_update.jl:22-43 */
+/* the tracers that receive particle exchange (plk), tracer_update.jl:22-43;
+ * DFe and PFe_bio also in IronEnergyMode models (FinalArgs::plk_fe) */
 __device__ __forceinline__ constexpr bool has_plk(int q)
 {
     return q == PI_DIC || q == PI_NH4 || q == PI_NO3 || q == PI_PO4 || q == PI_DOC || q == PI_DON ||
@@ -627,7 +628,7 @@ __device__ __forceinline__ void final_grp(const FinalArgs& a, const DevGrid& g, 
         const int q = fgrp(G, i);
         const float* co = fld(C0, g, q);
         cnv[i] = ld(co, c);
-        const float pk = has_plk(q) ? ld(fld(a.plk, g, q), cc) : 0.0f;
+        const float pk = (has_plk(q) || ((q == PI_DFe || q == PI_PFe_bio) && a.plk_fe)) ? ld(fld(a.plk, g, q), cc) : 0.0f;
         const int si = sink_index(q);
         /* pw[i][1] = the z-sweep input at cc */
         cell_update<LAND, BCS, GM>(a, g, z, q, tt, cu[i], cnv[i], ld(co, cc - 1), ld(co, cc + 1), ld(co, cc - sy),

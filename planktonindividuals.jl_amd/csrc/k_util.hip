@@ -259,7 +259,7 @@ __global__ void k_rebuild(DevState* st, SlotTable tab, long long cap, int nsp, u
 __global__ void k_table_field(const DevState* st, SlotTable tab, int sp, long long cap, int field,
                               const float* params, DevGrid g, void* out, int mode)
 {
-    const bool mm = mode == PI_MODE_MACROMOLECULAR;
+    const bool mm = mode == PI_MODE_MACROMOLECULAR || mode == PI_MODE_IRONENERGY;   /* k_idle* steps their rows */
     const long long it = st->iteration;
     const float* q = params + sp * PI_NPARAMS;
     for (long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x; s < cap; s += (long long)gridDim.x * blockDim.x) {
@@ -336,7 +336,28 @@ __global__ void __launch_bounds__(256) k_init_individuals(InitArgs a)
         const float y = jl_maskf(ac, uy * (float)g.Ny);
         const float z = jl_maskf(ac, uz * (float)g.Nz);
         float Sz, Bm, Cq, Nq, Pq, Chl, RNA = 0.0f;
-        if (a.mode == PI_MODE_MACROMOLECULAR) {
+        float fe[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};   /* IronEnergyMode: qNO3, qFe, qFePS, qFeNR, qFeNF */
+        if (a.mode == PI_MODE_IRONENERGY) {
+            /* IronEnergyMode/plankton_generation.jl:48-94: the size normal as QuotaMode's,
+             * CH, qNO3, qNH4 from the quota triple, qP from the size triple's third
+             * draw, qFe from its own triple; the maxima at a tenth; the iron split
+             * into photosynthesis, nitrate reduction and N fixation pools */
+            float uf, uf1, uf2;
+            pi_uniform3(a.seed, 0, sp, PI_KIND_INIT_FE, (uint64_t)i, &uf, &uf1, &uf2);
+            const float Sz0 = pi_init_normal(ua, ub);
+            Sz = jl_maskf(ac, jl_maxf(1.0f, Sz0 * q[PI_I_var] + q[PI_I_mean]));
+            Bm = q[PI_I_Cquota] * Sz * q[PI_I_Nsuper];
+            Cq = uc * (q[PI_I_CHmax] * 0.1f) * Bm;                            /* CH */
+            fe[0] = un * ((q[PI_I_qNO3max] * 0.1f) * (Bm + Cq));              /* qNO3 */
+            Nq = up * ((q[PI_I_qNH4max] * 0.1f) * (Bm + Cq));                 /* qNH4 */
+            Pq = u3 * ((q[PI_I_qPmax] * 0.1f) * (Bm + Cq));                   /* qP */
+            float qFe = uf * ((q[PI_I_qFemax] * 0.1f) * (Bm + Cq));
+            fe[2] = qFe * 0.4f;
+            fe[3] = qFe * 0.3f * q[PI_I_is_nr];
+            fe[4] = qFe * 0.3f * (q[PI_I_is_croc] + q[PI_I_is_tric]);
+            fe[1] = qFe - (fe[2] + fe[3] + fe[4]);
+            Chl = Bm * q[PI_I_Chl2Cint];
+        } else if (a.mode == PI_MODE_MACROMOLECULAR) {
             /* MacroMolecularMode/plankton_generation.jl:31-78: DNA, RNA and protein from
              * one draw triple, CH, NST and PST from another; Sz = DNA, Bm = PRO,
              * Cq = CH, Nq = NST, Pq = PST */
@@ -366,7 +387,11 @@ __global__ void __launch_bounds__(256) k_init_individuals(InitArgs a)
         a.tab.x[r] = x; a.tab.y[r] = y; a.tab.z[r] = z;
         a.tab.Sz[r] = Sz; a.tab.Bm[r] = Bm; a.tab.Cq[r] = Cq; a.tab.Nq[r] = Nq; a.tab.Pq[r] = Pq;
         a.tab.Chl[r] = Chl;
-        if (a.tab.ex[0]) a.tab.ex[0][r] = RNA;
+        if (a.mode == PI_MODE_MACROMOLECULAR) a.tab.ex[0][r] = RNA;
+        if (a.mode == PI_MODE_IRONENERGY) {
+            for (int k = 0; k < 5; ++k) a.tab.ex[k][r] = fe[k];
+            a.tab.ex[5][r] = a.tab.ex[6][r] = a.tab.ex[7][r] = 0.0f;     /* tdark, exEn, ptc */
+        }
         a.tab.gen[r] = i < a.N[sp] ? 1.0f : 0.0f;
         a.tab.age[r] = 0.0f;
         a.tab.flags[r] = (uint8_t)((ac ? F_AC : 0) | F_ZERO);

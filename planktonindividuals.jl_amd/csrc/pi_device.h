@@ -83,7 +83,15 @@ struct Store {
     float* ex[PI_NEX];           /* extra state columns of the modes that have them (else null) */
 };
 /* extra state columns per individual of a physiology mode */
-__host__ __device__ constexpr int mode_nex(int mode) { return mode == PI_MODE_MACROMOLECULAR ? 1 : 0; }
+__host__ __device__ constexpr int mode_nex(int mode)
+{
+    return mode == PI_MODE_MACROMOLECULAR ? 1 : mode == PI_MODE_IRONENERGY ? 8 : 0;
+}
+/* contribution values per store entry: the losses of IronEnergyMode reach ten
+ * tracers (NH4, NO3, DFe and PFe_bio besides the six of the other modes) */
+__host__ __device__ constexpr int mode_ncon(int mode) { return mode == PI_MODE_IRONENERGY ? 10 : 6; }
+/* values per slot of a reborn daughter's consumption: DIC, DOC, NH4, NO3, PO4, DFe */
+#define PI_RCONS 6
 
 /* Pointers read from device memory (the DevState) are generic pointers, and
  * accesses through them compile to FLAT instructions (64-bit addresses, both
@@ -138,7 +146,7 @@ struct SlotTable {               /* per species, stride cap */
 };
 
 struct Contrib {
-    float* v[6];
+    float* v[10];                /* v[6..9] only for IronEnergyMode models (else null) */
     uint8_t* tag;
 };
 

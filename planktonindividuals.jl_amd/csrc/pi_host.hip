@@ -403,6 +403,8 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     const int plk_ids[10] = {PI_DIC, PI_NH4, PI_NO3, PI_PO4, PI_DOC, PI_DON, PI_DOP, PI_POC, PI_PON, PI_POP};
     DALLOC(m->plk_slab, PI_NTRACERS * nc * 4);
     for (int q : plk_ids) m->plk[q] = m->plk_slab + q * nc;
+    if (d->mode == PI_MODE_IRONENERGY)   /* its losses and uptake reach DFe and PFe_bio too */
+        for (int q : {PI_DFe, PI_PFe_bio}) m->plk[q] = m->plk_slab + q * nc;
     DALLOC(m->diagPAR, nc * 4);
     DALLOC(m->diag, (long long)m->nsp * 4 * nc * 4);
     /* sort / scan scratch */
@@ -417,7 +419,7 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     m->long_blocks = (int)std::min<long long>((long long)g.Nx * g.Ny * g.Nz, 512);
     DALLOC(m->bm, 2ll * m->nsp * m->nw * 4);
     DALLOC(m->G, (2ll * m->nsp * m->nw + 1) * 4);
-    for (int q = 0; q < 6; ++q) DALLOC(m->con.v[q], m->ctot * 4);
+    for (int q = 0; q < mode_ncon(d->mode); ++q) DALLOC(m->con.v[q], m->ctot * 4);
     DALLOC(m->con.tag, m->ctot);
     DALLOC(m->d_count, 8);
     DALLOC(m->newc, 2 * m->ctot * 4); DALLOC(m->stayf, 2 * m->ctot * 4); DALLOC(m->P, (2 * m->ctot + 1) * 4);
@@ -429,7 +431,7 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     DALLOC(m->mix, (long long)m->qcap * PI_NQ * 4);   /* shards as the queue's (by queue position) */
     DALLOC(m->sflag, m->ctot);
     DALLOC(m->dstamp, m->ctot * 4); DALLOC(m->rstamp, m->ctot * 4);
-    DALLOC(m->dcell, m->ctot * 4); DALLOC(m->rcons, m->ctot * 5 * 4);
+    DALLOC(m->dcell, m->ctot * 4); DALLOC(m->rcons, m->ctot * PI_RCONS * 4);
     m->mode = d->mode;
     if (m->mode == PI_MODE_CARBON) DALLOC(m->dbd, m->ctot * 4);
     if (!m->dry) {   /* stamps start below any iteration */
@@ -474,8 +476,7 @@ static pi_status check_desc(const pi_model_desc* d)
     if (gd.topo_z != PI_BOUNDED) return fail(PI_ERR_ARG, "grid: z topology must be Bounded");
     if (d->n_species < 1 || d->n_species > PI_MAXSP) return fail(PI_ERR_ARG, "n_species out of range");
     if (d->capacity < 1 || d->capacity > (long long)PI_SLOT_MASK) return fail(PI_ERR_ARG, "capacity out of range");
-    if (d->mode != PI_MODE_QUOTA && d->mode != PI_MODE_CARBON && d->mode != PI_MODE_MACROMOLECULAR)
-        return fail(PI_ERR_UNSUPPORTED, "physiology mode: QuotaMode, CarbonMode and MacroMolecularMode are implemented");
+    if (d->mode < PI_MODE_QUOTA || d->mode > PI_MODE_IRONENERGY) return fail(PI_ERR_UNSUPPORTED, "physiology mode");
     if (d->metrics) {
         const pi_metrics& mt = *d->metrics;
         if (!mt.dxC || !mt.dyC || !mt.dxF || !mt.dyF || !mt.Az || !mt.Ax || !mt.Ay || !mt.Vol)
@@ -491,7 +492,8 @@ static pi_status check_desc(const pi_model_desc* d)
         if ((long long)d->n_abiotic * nc >= (1ll << 31) || (long long)d->n_abiotic * d->capacity >= (1ll << 31))
             return fail(PI_ERR_ARG, "abiotic particles: species x cells and species x capacity must stay below 2^31");
     }
-    const int dvt = d->mode == PI_MODE_CARBON ? (int)PI_C_dvid_type : (int)PI_P_dvid_type;
+    const int dvt = d->mode == PI_MODE_CARBON ? (int)PI_C_dvid_type
+                  : d->mode == PI_MODE_IRONENERGY ? (int)PI_I_dvid_type : (int)PI_P_dvid_type;
     for (int s = 0; s < d->n_species && d->mode != PI_MODE_MACROMOLECULAR; ++s) {   /* MM: no dvid_type */
         float t = d->params[s * PI_NPARAMS + dvt];
         if (!(t == 1.0f || t == 2.0f || t == 3.0f)) return fail(PI_ERR_ARG, "Wrong cell division type, must be in 1 to 3");
@@ -1089,9 +1091,12 @@ static void build_step(pi_model* m, float dt, Plan& P)
         P.add(K_PHYS, "phys", m->mode == PI_MODE_QUOTA ? phb : 0, pa);
         P.add(K_PHYSC, "phys", m->mode == PI_MODE_CARBON ? phb : 0, pa);
         P.add(K_PHYSM, "phys", m->mode == PI_MODE_MACROMOLECULAR ? phb : 0, pa);
+        P.add(K_PHYSI, "phys", m->mode == PI_MODE_IRONENERGY ? phb : 0, pa);
         IdleArgs ia = zeroed<IdleArgs>();
         ia.st = st; ia.tab = m->tab; ia.params = m->d_params; ia.cap = m->cap; ia.nsp = m->nsp; ia.dt = dt;
-        P.add(K_IDLEM, "phys", m->mode == PI_MODE_MACROMOLECULAR ? grid_for(m->ctot, 256, 8192) : 0, ia);
+        const long long ib = grid_for(m->ctot, 256, 8192);
+        P.add(K_IDLEM, "phys", m->mode == PI_MODE_MACROMOLECULAR ? ib : 0, ia);
+        P.add(K_IDLEI, "phys", m->mode == PI_MODE_IRONENERGY ? ib : 0, ia);
     }
     {
         PlkArgs ka = zeroed<PlkArgs>();
@@ -1102,8 +1107,12 @@ static void build_step(pi_model* m, float dt, Plan& P)
         ka.segp = m->segp;                 /* k_plk copies seg -> segp (the binning above has read segp) */
         /* one cell per thread by default: with a loop over cells each iteration's
          * loads wait for the previous iteration's stores (one counter for both) */
-        P.add(K_PLK, "plk", grid_for(m->ncell, 256, (int)m->shape[PI_S_PLK_BLOCKS]), ka);
-        P.add(K_PLK_LONG, "plk", m->long_blocks, ka);
+        const bool iron = m->mode == PI_MODE_IRONENERGY;
+        const long long kb = grid_for(m->ncell, 256, (int)m->shape[PI_S_PLK_BLOCKS]);
+        P.add(K_PLK, "plk", iron ? 0 : kb, ka);
+        P.add(K_PLK_LONG, "plk", iron ? 0 : m->long_blocks, ka);
+        P.add(K_PLKI, "plk", iron ? kb : 0, ka);
+        P.add(K_PLK_LONGI, "plk", iron ? m->long_blocks : 0, ka);
     }
     P.mark(K_JOIN);
     /* tracers: k_final and the halo fill of the new tracers, after the join */
@@ -1112,7 +1121,7 @@ static void build_step(pi_model* m, float dt, Plan& P)
         fa.g = g; fa.st = st; fa.nf = PI_NTRACERS;
         FinalArgs fn = zeroed<FinalArgs>();
         fn.g = g; fn.st = st; fn.w = V1.w; fn.dt = dt;
-        fn.prev = m->T2[0]; fn.plk = m->plk_slab;
+        fn.prev = m->T2[0]; fn.plk = m->plk_slab; fn.plk_fe = m->mode == PI_MODE_IRONENERGY ? 1 : 0;
         fn.kh = m->bgc[PI_B_kappa_h]; fn.kv = m->bgc[PI_B_kappa_v];
         fn.wsink_inorg = m->bgc[PI_B_w_sink_inorg]; fn.wsink_org = m->bgc[PI_B_w_sink_org];
         fn.kDOC = m->bgc[PI_B_kDOC]; fn.kPOC = m->bgc[PI_B_kPOC]; fn.kDON = m->bgc[PI_B_kDON];

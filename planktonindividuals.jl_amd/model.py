@@ -103,7 +103,9 @@ def initialize_plankton(p, N, cap, grid, seed, sp, mode=_abi.MODE_QUOTA):
     f32 = np.float32
     carbon = mode == _abi.MODE_CARBON
     mm = mode == _abi.MODE_MACROMOLECULAR
-    names = _abi.CARBON_PARAM_NAMES if carbon else _abi.MM_PARAM_NAMES if mm else _abi.PARAM_NAMES
+    iron = mode == _abi.MODE_IRONENERGY
+    names = (_abi.CARBON_PARAM_NAMES if carbon else _abi.MM_PARAM_NAMES if mm else
+             _abi.IRON_PARAM_NAMES if iron else _abi.PARAM_NAMES)
     q = {n: p[i] for i, n in enumerate(names)}
     slots = np.arange(cap, dtype=np.int64)
     ux, uy, uz = stream.uniform3(seed, 0, sp, stream.KIND_INIT_POS, slots)
@@ -131,6 +133,25 @@ def initialize_plankton(p, N, cap, grid, seed, sp, mode=_abi.MODE_QUOTA):
         Nq = spread(un) * Cq * f32(16) / f32(106)               # NST
         Pq = spread(up) * Cq / f32(106)                         # PST
         Chl = Sz * q["Chl2DNA"] * f32(893.49) / f32(55)
+    elif iron:
+        # IronEnergyMode/plankton_generation.jl:48-94: the maxima at a tenth, qP from
+        # the size triple's third draw, qFe from its own triple; columns Cq = CH,
+        # Nq = qNH4, Pq = qP, ex0..ex7 = qNO3, qFe, qFePS, qFeNR, qFeNF, tdark, exEn, ptc
+        uf, _, _ = stream.uniform3(seed, 0, sp, stream.KIND_INIT_FE, slots)
+        Sz = _mask(ac, np.maximum(f32(1.0), Sz * q["var"] + q["mean"]))
+        Bm = q["Cquota"] * Sz * q["Nsuper"]
+        Cq = uc * (q["CHmax"] * f32(0.1)) * Bm
+        BC = Bm + Cq
+        extra["ex0"] = (un * ((q["qNO3max"] * f32(0.1)) * BC)).astype(f32)
+        Nq = up * ((q["qNH4max"] * f32(0.1)) * BC)
+        Pq = u3 * ((q["qPmax"] * f32(0.1)) * BC)
+        qFe = uf * ((q["qFemax"] * f32(0.1)) * BC)
+        PSf = qFe * f32(0.4)
+        NRf = qFe * f32(0.3) * q["is_nr"]
+        NFf = qFe * f32(0.3) * (q["is_croc"] + q["is_tric"])
+        extra.update(ex1=(qFe - (PSf + NRf + NFf)).astype(f32), ex2=PSf.astype(f32), ex3=NRf.astype(f32),
+                     ex4=NFf.astype(f32), ex5=np.zeros(cap, f32), ex6=np.zeros(cap, f32), ex7=np.zeros(cap, f32))
+        Chl = Bm * q["Chl2Cint"]
     else:
         Sz = _mask(ac, np.maximum(f32(1.0), Sz * q["var"] + q["mean"]))
         Bm = (q["Cquota"] * Sz) * q["Nsuper"]
@@ -177,9 +198,11 @@ class _SpeciesData:
                 raise AttributeError(name)
             if name == "Bd":
                 name = "Cq"
-        elif mode == _abi.MODE_MACROMOLECULAR:
-            # MacroMolecularMode columns (MacroMolecularMode/plankton_generation.jl:2-12)
-            if name in ("Sz", "Bm", "Cq", "Nq", "Pq", "Bd") or name in _abi.EX_FIELDS:
+        elif mode in (_abi.MODE_MACROMOLECULAR, _abi.MODE_IRONENERGY):
+            # MacroMolecularMode / IronEnergyMode columns (plankton_generation.jl:1-12 of
+            # each): their own names only
+            state = ("Sz", "Bm", "Cq", "Nq", "Pq") + _abi.EX_FIELDS + tuple(_abi.FIELD_ALIAS)
+            if name in state and name not in _abi.MODE_NAMES[mode]:
                 raise AttributeError(name)
         elif name in _abi.FIELD_ALIAS or name in _abi.EX_FIELDS:
             raise AttributeError(name)
@@ -281,9 +304,8 @@ class ModelSpec:
                  bgc_params=None, tracer_initial=None, phyto=None, abiotic=None, t=0.0,
                  max_candidates=25, seed=0, host_state=True):
         mode = mode if mode is not None else P.QuotaMode()
-        if not isinstance(mode, (P.QuotaMode, P.CarbonMode, P.MacroMolecularMode)):
-            raise NotImplementedError(f"{mode}: QuotaMode, CarbonMode and MacroMolecularMode are implemented "
-                                      "on the MI355X hot path")
+        if not isinstance(mode, (P.QuotaMode, P.CarbonMode, P.MacroMolecularMode, P.IronEnergyMode)):
+            raise NotImplementedError(f"{mode}: not a physiology mode")
         self.n_abiotic, self.abiotic_params, self.abiotic_matrix, self.abiotic_N = 0, None, None, []
         if abiotic is not None:                       # models.jl:99-117
             if not isinstance(abiotic, abiotic_setup):

@@ -350,8 +350,13 @@ CARBON_DYNAMICS_SPECS = ("%3.0f", "%2.2f", "%6.0f", "%1.2f", "%1.2f", "%1.2f", "
 # protein, DNA, RNA, CH, NST, PST and Chl averages
 MM_DYNAMICS_FIELDS = ("gen", "age", "PRO", "DNA", "RNA", "CH", "NST", "PST", "Chl")
 MM_DYNAMICS_SPECS = ("%3.0f", "%2.2f", "%6.0f", "%1.2f", "%1.2f") + ("%.8E",) * 7
+# IronEnergyMode (write_outputs.jl:118-140) averages a column `En` that its
+# StructArray does not have (it has exEn), so the reference fails there before
+# writing; reading `En` from the species data raises the same way here
+IRON_DYNAMICS_FIELDS = ("gen", "age", "Sz", "Bm", "En", "CH", "qNO3", "qNH4", "qP", "qFe", "Chl")
+IRON_DYNAMICS_SPECS = ("%3.0f", "%2.2f", "%6.0f", "%1.2f", "%1.2f", "%1.2f") + ("%.8E",) * 8
 _DYNAMICS = {0: (DYNAMICS_FIELDS, DYNAMICS_SPECS), 1: (CARBON_DYNAMICS_FIELDS, CARBON_DYNAMICS_SPECS),
-             2: (MM_DYNAMICS_FIELDS, MM_DYNAMICS_SPECS)}
+             2: (MM_DYNAMICS_FIELDS, MM_DYNAMICS_SPECS), 3: (IRON_DYNAMICS_FIELDS, IRON_DYNAMICS_SPECS)}
 
 
 def dynamics_line(t, columns, mode=0):

@@ -11,6 +11,7 @@ KIND_DECISION = 1
 KIND_INIT_POS = 16      # x, y, z
 KIND_INIT_QUOTA = 17    # Cq, Nq, Pq
 KIND_INIT_SIZE = 18     # two uniforms for the Box-Muller normal of Sz
+KIND_INIT_FE = 19       # IronEnergyMode: the iron quota
 KIND_TRACER = 32        # + tracer id; slot = padded cell index
 # abiotic particles: species field ABIO_SPECIES + sa; particles_from_bcs! draws
 # KIND_ABIO_BC per padded cell (slot = the cell's index)

@@ -76,7 +76,7 @@ def test_modelspec_validation(pi):
     with pytest.raises(ValueError, match="length of `N`"):
         pi.ModelSpec(g, max_individuals=100, phyto=pi.phyto_setup(None, [20, 20], 1))
     with pytest.raises(NotImplementedError):
-        pi.ModelSpec(g, mode=pi.IronEnergyMode())
+        pi.ModelSpec(g, mode=object())
     with pytest.raises(ValueError, match="parameter not found"):
         pi.ModelSpec(g, mode=pi.CarbonMode(), phyto=pi.phyto_setup({"Cqmax": [0.4]}, [10], 1))
     s = pi.ModelSpec(g, max_individuals=64, phyto=pi.phyto_setup(None, [40], 1), seed=3)
