@@ -176,7 +176,10 @@ enum {
     PI_G_plk_DIC      /* timestepper.plk.<tracer> : add the tracer id (padded) */
 };
 enum {                 /* per-species diagnostics (Diagnostics/diagnostics_struct.jl:97-104) */
-    PI_D_num = 0, PI_D_graz, PI_D_mort, PI_D_dvid, PI_NDIAG = 4
+    PI_D_num = 0, PI_D_graz, PI_D_mort, PI_D_dvid,
+    PI_D_ptc,          /* particles attached, accumulated per step when the model has abiotic particles
+                        * (time_step.jl:140-149; nonzero only in IronEnergyMode) */
+    PI_NDIAG = 5
 };
 
 typedef struct pi_grid_desc {
@@ -214,14 +217,21 @@ typedef struct pi_model_desc {
      * Individuals/Abiotic/particle_generation.jl:1-35): n_abiotic species of
      * `capacity` slots each, PI_NAPARAMS floats per species.  They are carried
      * by the flow and the random walk, created at the faces by their flux
-     * boundary conditions and counted per cell; n_interactions (the pairs of
-     * palat.intac) must be 0: phytoplankton-particle interaction needs the
-     * max_ptc parameter of IronEnergyMode (particle_interaction.jl:60), which
-     * this library does not implement.  The release pairs (palat.release) change
-     * nothing without interaction (ptc stays 0) and are not passed. */
+     * boundary conditions and counted per cell. */
     int32_t n_abiotic;
     const float* abiotic_params;
+    /* phytoplankton-particle interaction (Abiotic/particle_interaction.jl:1-145,
+     * time_step.jl:120-150): the pairs of palat.intac and palat.release as
+     * (species, abiotic species) int32 pairs, 0-based, at most 4 of each, run
+     * in this order every step after the physiology.  Interaction needs
+     * IronEnergyMode (its max_ptc and ptc column); release works in every mode
+     * (ptc of the other modes stays 0).  max_candidates (models.jl:60, 25 by
+     * default) in [1, 64]. */
     int32_t n_interactions;
+    const int32_t* interactions;
+    int32_t n_releases;
+    const int32_t* releases;
+    int32_t max_candidates;
 } pi_model_desc;
 
 /* Host-side view of one species' individuals, all arrays of length capacity.

@@ -52,6 +52,9 @@ enum { PI_KIND_INIT_POS = 16, PI_KIND_INIT_QUOTA = 17, PI_KIND_INIT_SIZE = 18, P
  * PI_KIND_DIFFUSION, particles_from_bcs! one draw per padded cell (slot = the
  * cell's index), the initial positions PI_KIND_INIT_POS */
 enum { PI_ABIO_SPECIES = 64, PI_KIND_ABIO_BC = 2 };
+/* particle_release!: the draw of release pair p (species field PI_RELEASE_SPECIES
+ * + p, slot = the individual's slot) */
+enum { PI_RELEASE_SPECIES = 96, PI_KIND_RELEASE = 3 };
 
 PI_HD void pi_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4])
 {

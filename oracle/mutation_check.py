@@ -2,7 +2,8 @@
 
 Each entry below plants one plausible mistake in a copy of oracle/pi_oracle.cpp
 (a wrong index, sign, exponent, dropped term or halo plane, in QuotaMode,
-CarbonMode, MacroMolecularMode and IronEnergyMode), builds the copy
+CarbonMode, MacroMolecularMode, IronEnergyMode and the particle
+interaction), builds the copy
 into a temporary library, and runs the CPU pin tests against it
 (PI_ORACLE_LIB).  Every mistake must make at least one pin fail.  The
 committed source is never modified.
@@ -32,6 +33,20 @@ BC_TOP = """            if (bck[q][PI_BC_TOP])
                 }
 """
 BOTTOM_TOP, TOP_BOTTOM = BC_BOTTOM + BC_TOP, BC_TOP + BC_BOTTOM
+
+INTERACT = """        for (const auto& pr : pint) particle_interaction(pr[0], pr[1]);   /* time_step.jl:120-128 */
+        for (size_t q = 0; q < prel.size(); ++q) {                        /* :130-137 */
+            pi_status st = particle_release(prel[q][0], prel[q][1], (int)q, dt);
+            if (st != PI_OK) return st;
+        }
+        if (nsa > 0 && mode == PI_MODE_IRONENERGY)       /* diags_proc!(ptc, ptc, ac, ...) (:140-149) */
+            for (int s = 0; s < nsp; ++s)
+                for (int64_t i = 0; i < cap; ++i) diagSp[s][4][cell_of(sp[s], i)] += M::mask(sp[s].ac[i], sp[s].ptc[i]);
+"""
+NUM = """        for (int a = 0; a < nsa; ++a)                   /* diags_proc!(num, ac, ac, xi, yi, zi) (:152-159) */
+            for (int64_t i = 0; i < cap; ++i) abNum[a][cell_of(ab[a], i)] += (FT)(ab[a].ac[i] ? 1 : 0);
+"""
+INTERACT_NUM, NUM_INTERACT = INTERACT + NUM, NUM + INTERACT
 
 MISTAKES = [
     ("sinking dzC offset", "/ dzC_at(k) * dt;", "/ dzC_at(k + Hz) * dt;"),
@@ -185,6 +200,28 @@ MISTAKES = [
      "FT f_PS2ST = M::mask(p.ac[i], q[PI_I_k_Fe_PS2ST] * qFePS * (FT)(falling + dark));"),
     ("iron: storage to photosynthesis on falling light", "FT f_ST2PS = M::mask(rising, q[PI_I_k_Fe_ST2PS] * qFe);",
      "FT f_ST2PS = M::mask(falling, q[PI_I_k_Fe_ST2PS] * qFe);"),
+    ("interaction: residue of the 0-based index", "const int64_t slot = ((i + 1) % mc) + 1;",
+     "const int64_t slot = (i % mc) + 1;"),
+    ("interaction: the first individual of a residue kept", "intac[(size_t)(slot - 1) + (size_t)mc * k] = i + 1;",
+     "if (!intac[(size_t)(slot - 1) + (size_t)mc * k]) intac[(size_t)(slot - 1) + (size_t)mc * k] = i + 1;"),
+    ("interaction: across cells", "if (xi1 == xi2 && yi1 == yi2 && zi1 == zi2) {", "if (true) {"),
+    ("interaction: distance in grid units", "FT dx = (x1 - x2) * DxF((int)xi1, (int)yi1);", "FT dx = (x1 - x2);"),
+    ("interaction: max_ptc inclusive", "if (id > 0 && pl.ptc[id - 1] < maxptc) {", "if (id > 0 && pl.ptc[id - 1] <= maxptc) {"),
+    ("interaction: particles in reverse slot order",
+     "for (int64_t k = 0; k < cap; ++k) {\n            if (!pa.ac[k]) continue;\n            for (int j = 1;",
+     "for (int64_t k = cap - 1; k >= 0; --k) {\n            if (!pa.ac[k]) continue;\n            for (int j = 1;"),
+    ("release: x from a fresh draw", "pa.x[k] = pl.x[i] + rx[i];", "pa.x[k] = pl.x[i] + (FT)b0;"),
+    ("release: y from the first species' draws", "pi_uniform3(seed, iteration, nsp - 1, PI_KIND_DECISION, i, &b0, &b1, &b2);",
+     "pi_uniform3(seed, iteration, 0, PI_KIND_DECISION, i, &b0, &b1, &b2);"),
+    ("release: z with the mortality draw", "pa.z[k] = pl.z[i] + (FT)b2;", "pa.z[k] = pl.z[i] + (FT)b1;"),
+    ("release: free slots from the end", "for (int64_t k = 0; k < cap; ++k) if (!pa.ac[k]) free_ind.push_back(k + 1);",
+     "for (int64_t k = cap - 1; k >= 0; --k) if (!pa.ac[k]) free_ind.push_back(k + 1);"),
+    ("release: ptc kept", "                pl.ptc[i] -= L(1);\n", "\n"),
+    ("release: probability without ptc", "R[i] = (M::isless((FT)u0, rP * dt * pl.ptc[i]) && pl.ac[i]) ? L(1) : L(0);",
+     "R[i] = (M::isless((FT)u0, rP * dt) && pl.ac[i]) ? L(1) : L(0);"),
+    ("ptc diagnostic of every slot", "diagSp[s][4][cell_of(sp[s], i)] += M::mask(sp[s].ac[i], sp[s].ptc[i]);",
+     "diagSp[s][4][cell_of(sp[s], i)] += sp[s].ptc[i];"),
+    ("abiotic num before the interaction", INTERACT_NUM, NUM_INTERACT),
 ]
 
 
@@ -209,7 +246,8 @@ def main(only):
                                 "tests/test_oracle_pins.py", "tests/test_oracle_pins_physics.py",
                                 "tests/test_oracle_pins_carbon.py", "tests/test_oracle_pins_bcs.py",
                                 "tests/test_oracle_pins_latlon.py", "tests/test_oracle_pins_abiotic.py",
-                                "tests/test_oracle_pins_mm.py", "tests/test_oracle_pins_iron.py"],
+                                "tests/test_oracle_pins_mm.py", "tests/test_oracle_pins_iron.py",
+                                "tests/test_oracle_pins_interact.py"],
                                cwd=REPO, env=env, capture_output=True, text=True)
             first = next((l for l in r.stdout.splitlines() if l.startswith("FAILED")), "")
             (caught if r.returncode != 0 else missed).append(name)

@@ -90,7 +90,7 @@ class OracleModel:
         desc, keep = pi._abi.make_desc(spec.grid, spec.n_species, spec.max_individuals,
                                        spec.species_params, spec.bgc_vector, spec.seed, diag_tracers,
                                        getattr(spec, "mode_id", pi._abi.MODE_QUOTA),
-                                       getattr(spec, "abiotic_matrix", None))
+                                       getattr(spec, "abiotic_matrix", None), getattr(spec, "palat", None))
         raw = C.c_void_p()
         pi._native.check(lib, "orc_", lib.orc_model_create(C.byref(desc), precision, C.byref(raw)))
         del keep

@@ -139,6 +139,11 @@ struct Oracle : orc_model {
      * xi, yi, zi columns (in a Plank), parameters, diags.abiotics.saN.num and
      * the particle flux boundary conditions */
     int nsa = 0;
+    /* palat.intac / palat.release pairs (species, abiotic species), max_candidates
+     * and timestepper.intac (max_candidates x capacity, models.jl:100-104) */
+    std::vector<std::array<int, 2>> pint, prel;
+    int mc = 25;
+    std::vector<int64_t> intac;
     std::vector<std::array<FT, PI_NAPARAMS>> AP;
     std::vector<Plank> ab;
     std::vector<V> abNum;
@@ -257,7 +262,12 @@ struct Oracle : orc_model {
         for (auto& a : diagSp) for (auto& v : a) v.assign(ncell, 0);
         nsa = d->n_abiotic;
         if (nsa < 0 || nsa > PI_MAXSA) return fail(PI_ERR_ARG, "n_abiotic out of range");
-        if (d->n_interactions != 0) return fail(PI_ERR_UNSUPPORTED, "phytoplankton-particle interaction needs IronEnergyMode");
+        if (d->n_interactions && mode != PI_MODE_IRONENERGY)
+            return fail(PI_ERR_ARG, "phytoplankton-particle interaction needs IronEnergyMode (max_ptc, ptc)");
+        for (int q = 0; q < d->n_interactions; ++q) pint.push_back({d->interactions[2 * q], d->interactions[2 * q + 1]});
+        for (int q = 0; q < d->n_releases; ++q) prel.push_back({d->releases[2 * q], d->releases[2 * q + 1]});
+        mc = d->max_candidates > 0 ? d->max_candidates : 25;
+        if (!pint.empty()) intac.assign((size_t)mc * cap, 0);
         AP.resize(nsa); ab.resize(nsa); abNum.assign(nsa, V(ncell, 0));
         for (int a = 0; a < nsa; ++a) {
             for (int q = 0; q < PI_NAPARAMS; ++q) AP[a][q] = (FT)d->abiotic_params[a * PI_NAPARAMS + q];
@@ -1470,6 +1480,96 @@ struct Oracle : orc_model {
         }
     }
 
+    /* ===================== phytoplankton-particle interaction and release == */
+    /* calc_distance (Abiotic/particle_interaction.jl:3-14) */
+    FT calc_distance(int64_t xi1, int64_t yi1, int64_t zi1, FT x1, FT y1, FT z1,
+                     int64_t xi2, int64_t yi2, int64_t zi2, FT x2, FT y2, FT z2) const {
+        if (xi1 == xi2 && yi1 == yi2 && zi1 == zi2) {
+            FT dx = (x1 - x2) * DxF((int)xi1, (int)yi1);
+            FT dy = (y1 - y2) * DyF((int)xi1, (int)yi1);
+            FT dz = (z1 - z2) * dzF_at((int)zi1);
+            return (FT)std::sqrt(M::pow2(dx) + M::pow2(dy) + M::pow2(dz));
+        }
+        return L(1.0e20f);
+    }
+    /* particle_interaction! (:17-84): reset_interaction!, calc_interaction_kernel!
+     * (every individual against every particle, later individuals overwriting the
+     * candidate of their residue), merge_particle_kernel! (each particle in turn
+     * joins the first candidate below max_ptc) */
+    void particle_interaction(int s, int a) {
+        Plank& pl = sp[s];
+        Plank& pa = ab[a];
+        const FT Rd = AP[a][PI_A_Rd], maxptc = P[s][PI_I_max_ptc];
+        std::fill(intac.begin(), intac.end(), (int64_t)0);
+        for (int64_t i = 0; i < cap; ++i) {
+            if (!pl.ac[i]) continue;
+            for (int64_t k = 0; k < cap; ++k) {
+                if (!pa.ac[k]) continue;
+                FT dist = calc_distance(pl.xi[i], pl.yi[i], pl.zi[i], pl.x[i], pl.y[i], pl.z[i],
+                                        pa.xi[k], pa.yi[k], pa.zi[k], pa.x[k], pa.y[k], pa.z[k]);
+                if (dist < Rd) {
+                    const int64_t slot = ((i + 1) % mc) + 1;
+                    intac[(size_t)(slot - 1) + (size_t)mc * k] = i + 1;
+                }
+            }
+        }
+        for (int64_t k = 0; k < cap; ++k) {
+            if (!pa.ac[k]) continue;
+            for (int j = 1; j <= mc; ++j) {
+                const int64_t id = intac[(size_t)(j - 1) + (size_t)mc * k];
+                if (id > 0 && pl.ptc[id - 1] < maxptc) {
+                    pl.ptc[id - 1] += L(1);
+                    pa.ac[k] = 0;
+                    break;
+                }
+            }
+        }
+    }
+    /* particle_release! (:86-145): get_release_probability! (rnd.x = this call's
+     * draw), the capacity check, get_tind!, release_abiotic_particle_kernel! at
+     * x + rnd.x, y + rnd.y, z + rnd.z with rnd.y, rnd.z still the last species'
+     * draws of get_probability! (mortality and division); ptc of the other modes
+     * is 0, so they release nothing */
+    pi_status particle_release(int s, int a, int q, FT dt) {
+        if (mode != PI_MODE_IRONENERGY) return PI_OK;
+        Plank& pl = sp[s];
+        Plank& pa = ab[a];
+        const FT rP = AP[a][PI_A_release_P];
+        V R(cap, (FT)0), rx(cap, (FT)0);
+        FT releasenum = 0;
+        for (int64_t i = 0; i < cap; ++i) {
+            float u0, u1, u2;
+            pi_uniform3(seed, iteration, PI_RELEASE_SPECIES + q, PI_KIND_RELEASE, i, &u0, &u1, &u2);
+            rx[i] = (FT)u0;
+            R[i] = (M::isless((FT)u0, rP * dt * pl.ptc[i]) && pl.ac[i]) ? L(1) : L(0);
+        }
+        for (int64_t i = 0; i < cap; ++i) releasenum += M::mask(pl.ac[i], R[i]);
+        std::vector<int64_t> free_ind;
+        for (int64_t k = 0; k < cap; ++k) if (!pa.ac[k]) free_ind.push_back(k + 1);
+        if (releasenum > (FT)free_ind.size()) {
+            char buf[160];
+            snprintf(buf, sizeof buf, "number of abiotic particles exceeds the capacity at timestep %g days",
+                     (double)t / 86400.0);
+            failed = true;
+            return fail(PI_ERR_CAPACITY, buf);
+        }
+        FT acc = 0;
+        for (int64_t i = 0; i < cap; ++i) {
+            acc = acc + R[i];
+            if (R[i] == L(1)) {
+                const int64_t k = free_ind[(size_t)((int64_t)acc - 1)] - 1;
+                float b0, b1, b2;
+                pi_uniform3(seed, iteration, nsp - 1, PI_KIND_DECISION, i, &b0, &b1, &b2);
+                pa.ac[k] = 1;
+                pa.x[k] = pl.x[i] + rx[i];
+                pa.y[k] = pl.y[i] + (FT)b1;
+                pa.z[k] = pl.z[i] + (FT)b2;
+                pl.ptc[i] -= L(1);
+            }
+        }
+        return PI_OK;
+    }
+
     /* ================================================== tracers ========= */
     /* Difference operators (Biogeochemistry/Advection/operaters.jl:1-27) */
     inline FT dxp(const FT* c, int i, int j, int k) const { return M::mask(LM(i, j, k), M::mask(LM(i + 1, j, k), c[I(i + 1, j, k)] - c[I(i, j, k)])); }
@@ -1818,6 +1918,14 @@ struct Oracle : orc_model {
                 if (st != PI_OK) return st;
             }
         }
+        for (const auto& pr : pint) particle_interaction(pr[0], pr[1]);   /* time_step.jl:120-128 */
+        for (size_t q = 0; q < prel.size(); ++q) {                        /* :130-137 */
+            pi_status st = particle_release(prel[q][0], prel[q][1], (int)q, dt);
+            if (st != PI_OK) return st;
+        }
+        if (nsa > 0 && mode == PI_MODE_IRONENERGY)       /* diags_proc!(ptc, ptc, ac, ...) (:140-149) */
+            for (int s = 0; s < nsp; ++s)
+                for (int64_t i = 0; i < cap; ++i) diagSp[s][4][cell_of(sp[s], i)] += M::mask(sp[s].ac[i], sp[s].ptc[i]);
         for (int a = 0; a < nsa; ++a)                   /* diags_proc!(num, ac, ac, xi, yi, zi) (:152-159) */
             for (int64_t i = 0; i < cap; ++i) abNum[a][cell_of(ab[a], i)] += (FT)(ab[a].ac[i] ? 1 : 0);
         tracer_update(dt);

@@ -81,7 +81,7 @@ MODE_NAMES = {MODE_QUOTA: ("Sz", "Bm", "Cq", "Nq", "Pq", "Chl"), MODE_CARBON: ("
 GRID_ID = {"par": 0, "par₀": 1, "par0": 1, "Chl": 2, "pop": 3, "temp": 4, "PARF": 5,
            "u": 6, "v": 7, "w": 8, "diag_PAR": 9}
 PLK_BASE = 10
-DIAG_ID = {"num": 0, "graz": 1, "mort": 2, "dvid": 3}
+DIAG_ID = {"num": 0, "graz": 1, "mort": 2, "dvid": 3, "ptc": 4}
 # flux boundary conditions (pi_set_tracer_bc): positions and getbc's kinds
 BC_POS = {"west": 0, "east": 1, "south": 2, "north": 3, "bottom": 4, "top": 5}
 BCK_NONE, BCK_CONST, BCK_FIELD, BCK_SERIES = range(4)
@@ -112,7 +112,9 @@ class ModelDesc(C.Structure):
                 ("params", fp), ("bgc", fp), ("rng_seed", C.c_uint64),
                 ("n_diag_tracers", C.c_int32), ("diag_tracers", C.POINTER(C.c_int32)),
                 ("mode", C.c_int32), ("metrics", C.POINTER(Metrics)),
-                ("n_abiotic", C.c_int32), ("abiotic_params", fp), ("n_interactions", C.c_int32)]
+                ("n_abiotic", C.c_int32), ("abiotic_params", fp), ("n_interactions", C.c_int32),
+                ("interactions", C.POINTER(C.c_int32)), ("n_releases", C.c_int32),
+                ("releases", C.POINTER(C.c_int32)), ("max_candidates", C.c_int32)]
 
 
 class IndividualsHost(C.Structure):
@@ -128,7 +130,9 @@ def ptr(a, ctype):
 
 
 def make_desc(grid, n_species, capacity, params, bgc, seed, diag_tracers=(), mode=MODE_QUOTA,
-              abiotic_params=None, n_interactions=0):
+              abiotic_params=None, palat=None):
+    """`palat` = (interaction pairs, release pairs, max_candidates), the pairs
+    0-based (species, abiotic species)."""
     """Build a ModelDesc; returns (desc, keepalive) — keep the second alive while desc is used."""
     dzC = np.ascontiguousarray(grid.dzC, dtype=np.float32)
     dzF = np.ascontiguousarray(grid.dzF, dtype=np.float32)
@@ -151,8 +155,14 @@ def make_desc(grid, n_species, capacity, params, bgc, seed, diag_tracers=(), mod
     d = ModelDesc(g, n_species, capacity, ptr(params, C.c_float), ptr(bgc, C.c_float),
                   C.c_uint64(seed), len(dt), ptr(dt, C.c_int32) if len(dt) else None, int(mode),
                   C.pointer(mt) if mt is not None else None,
-                  0 if ap is None else len(ap) // len(ABIOTIC_PARAM_NAMES), ptr(ap, C.c_float), int(n_interactions))
-    return d, (dzC, dzF, lm, params, bgc, dt, marrs, mt, ap)
+                  0 if ap is None else len(ap) // len(ABIOTIC_PARAM_NAMES), ptr(ap, C.c_float))
+    intac, release, mc = palat if palat is not None else ((), (), 25)
+    pi_ = np.ascontiguousarray(np.array(intac, dtype=np.int32).reshape(-1))
+    pr = np.ascontiguousarray(np.array(release, dtype=np.int32).reshape(-1))
+    d.n_interactions, d.n_releases, d.max_candidates = len(pi_) // 2, len(pr) // 2, int(mc)
+    d.interactions = ptr(pi_, C.c_int32) if len(pi_) else None
+    d.releases = ptr(pr, C.c_int32) if len(pr) else None
+    return d, (dzC, dzF, lm, params, bgc, dt, marrs, mt, ap, pi_, pr)
 
 
 def individuals_view(arrays):

@@ -272,11 +272,12 @@ class Handle:
         return v.value
 
 
-def footprint(grid, nsp, cap, params, bgc, seed=0, diag_tracers=(), mode=_abi.MODE_QUOTA, abiotic_params=None):
+def footprint(grid, nsp, cap, params, bgc, seed=0, diag_tracers=(), mode=_abi.MODE_QUOTA, abiotic_params=None,
+              palat=None):
     """Device bytes a model of this shape allocates (pi_footprint: a dry run of
     pi_model_create, no device needed)."""
     lib = load_library()
-    desc, keep = _abi.make_desc(grid, nsp, cap, params, bgc, seed, diag_tracers, mode, abiotic_params)
+    desc, keep = _abi.make_desc(grid, nsp, cap, params, bgc, seed, diag_tracers, mode, abiotic_params, palat)
     v = C.c_int64(0)
     check(lib, "pi_", lib.pi_footprint(C.byref(desc), C.byref(v)))
     del keep
@@ -287,10 +288,10 @@ class NativeModel(Handle):
     """A model living on the GPU (libpi_hip.so)."""
 
     def __init__(self, grid, nsp, cap, params, bgc, seed, diag_tracers=(), device=0, mode=_abi.MODE_QUOTA,
-                 abiotic_params=None):
+                 abiotic_params=None, palat=None):
         lib = load_library()
         check(lib, "pi_", lib.pi_init(device))
-        desc, keep = _abi.make_desc(grid, nsp, cap, params, bgc, seed, diag_tracers, mode, abiotic_params)
+        desc, keep = _abi.make_desc(grid, nsp, cap, params, bgc, seed, diag_tracers, mode, abiotic_params, palat)
         raw = C.c_void_p()
         check(lib, "pi_", lib.pi_model_create(C.byref(desc), C.byref(raw)))
         del keep

@@ -20,6 +20,7 @@ struct AdvArgs {
     uint32_t* dcell;          /* per slot: that occupant's cell in this step */
     float* dbd;               /* CarbonMode, per slot: that occupant's Bd (null in QuotaMode) */
     long long cap;
+    int dbd_ptc;              /* IronEnergyMode: stash the occupant's ptc (extra column 7) instead */
 };
 
 struct PhysArgs {
@@ -48,7 +49,10 @@ struct PhysArgs {
     const uint32_t* dcell;
     int32_t* rstamp;          /* per slot: iteration in which a daughter was born there */
     float* rcons;             /* per slot x 5: that daughter's consumption */
-    const float* dbd;         /* CarbonMode, per slot: Bd of an occupant dying in this step */
+    const float* dbd;         /* CarbonMode, per slot: Bd of an occupant dying in this step
+                               * (IronEnergyMode: its ptc) */
+    uint32_t* rdidx;          /* per slot: the next-store index of a daughter reborn there
+                               * (models with phytoplankton-particle interaction; else null) */
 };
 
 struct PlkArgs {
@@ -133,6 +137,7 @@ struct AbioBc {
 };
 /* error species code of a failed abiotic capacity check: PI_ERR_SP_ABIO + sa */
 #define PI_ERR_SP_ABIO 1000u
+#define PI_ERR_SP_RELEASE 2000u   /* + abiotic species: particle_release! found too few free slots */
 struct AbioArgs {
     AdvArgs adv;                      /* grid, state, velocities, Δt, walk scales, stream seed */
     int nsa;
@@ -146,6 +151,7 @@ struct AbioArgs {
     const uint32_t* fpre;
     uint32_t* flist;                  /* [nsa][cap]: the free slots of each species in ascending order */
     float* num;                       /* [nsa][ncell]: diags.abiotics.saN.num */
+    int num_late;                     /* 1: num is counted after the interactions (k_interact) */
     const AbioBc* bc;
     uint32_t bcm;                     /* bit 6·sa + f: species sa has a condition on face f */
     float p[PI_MAXSA][PI_NAPARAMS];
@@ -223,7 +229,7 @@ enum KId {
     K_FINISH, K_XYW0, K_XYW1, K_PHYSC, K_FINAL2, K_FINAL3,
     K_ADV_GM, K_SWEEPX2, K_SWEEPX3, K_SWEEPY2, K_SWEEPY3, K_FINAL4, K_FINAL5, K_FINAL6, K_FINAL7,
     K_ABIO_MARK, K_ABIO_FREE, K_ABIO_LIST, K_ABIO_CHECK, K_ABIO_SPAWN, K_ABIO_ADV, K_ABIO_ADV_GM,
-    K_PHYSM, K_IDLEM, K_PHYSI, K_IDLEI, K_PLKI, K_PLK_LONGI, K_NKID
+    K_PHYSM, K_IDLEM, K_PHYSI, K_IDLEI, K_PLKI, K_PLK_LONGI, K_INTERACT, K_NKID
 };
 /* plan markers (not kernels): fork to / join from the side stream */
 enum { K_FORK = K_NKID, K_JOIN = K_NKID + 1 };
@@ -264,6 +270,34 @@ struct InitArgs {
     uint64_t seed;
     int mode;                 /* PI_MODE_QUOTA / PI_MODE_CARBON / PI_MODE_MACROMOLECULAR */
 };
+/* phytoplankton-particle interaction and release (Abiotic/particle_interaction.jl:1-157,
+ * time_step.jl:120-159): one workgroup per model runs the pairs in order */
+#define PI_MAXPAIRS 4     /* palat.intac and palat.release pairs per model */
+#define PI_MAXCAND 64     /* max_candidates */
+struct InteractArgs {
+    DevGrid g;
+    DevState* st;
+    const uint32_t* seg;              /* this step's cell segments of the sorted input */
+    const int32_t* rstamp;            /* per slot: iteration in which a daughter was reborn there */
+    const uint32_t* rdidx;            /* per slot: that daughter's index in the next store */
+    const float* params;              /* nsp * PI_NPARAMS (max_ptc) */
+    float ap[PI_MAXSA][PI_NAPARAMS];
+    long long cap;
+    int nsp, nsa;
+    float *x, *y, *z;                 /* abiotic [nsa][cap] */
+    int32_t *xi, *yi, *zi;
+    uint8_t* ac;
+    float* num;                       /* [nsa][ncell] */
+    float* diag_ptc;                  /* [nsp][ncell]: diags.phytos.spN.ptc (IronEnergyMode; else null) */
+    uint32_t *cnt, *off, *fill, *list;            /* per cell counts / offsets, the cells' particle lists */
+    uint32_t *rflag, *rrank, *fflag, *fpre, *flist;   /* per slot: releases, free abiotic slots */
+    int nint, nrel, mc;
+    int pint[PI_MAXPAIRS][2], prel[PI_MAXPAIRS][2];   /* (species, abiotic species) */
+    int num_late;                     /* count num here (after the interactions), not in k_abio_adv */
+    float dt;
+    uint64_t seed;
+};
+
 /* MacroMolecularMode: the growth of the slots that are inactive through a
  * whole step, whose state keeps changing (biosynthesis is not masked by ac) */
 struct IdleArgs {

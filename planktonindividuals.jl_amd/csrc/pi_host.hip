@@ -158,6 +158,13 @@ struct pi_model {
     AbioBc* d_abc = nullptr;
     float* abc_buf[PI_MAXSA][PI_NBC]{};
     long long abc_nt[PI_MAXSA][PI_NBC]{};
+    /* phytoplankton-particle interaction and release: the pairs, the scratch of
+     * k_interact, the per-slot index of a reborn daughter, diags.phytos.spN.ptc */
+    int npint = 0, nprel = 0, mc = 25;
+    int pint[PI_MAXPAIRS][2]{}, prel[PI_MAXPAIRS][2]{};
+    uint32_t *icnt = nullptr, *ioff = nullptr, *ifill = nullptr, *ilist = nullptr, *rflag = nullptr,
+             *rrank = nullptr, *ifflag = nullptr, *ifpre = nullptr, *iflist = nullptr, *rdidx = nullptr;
+    float* diag_ptc = nullptr;
 };
 
 static hipError_t dalloc(pi_model* m, void** p, size_t bytes)
@@ -211,7 +218,10 @@ static pi_status sync_state(pi_model* m)
     if (m->h_state.err == PI_ERR_CAPACITY) {
         char buf[200];
         const double day = (double)(m->h_state.iteration + 1) * (double)m->last_dt / 86400.0;
-        if (m->h_state.err_sp >= PI_ERR_SP_ABIO)      /* particles_from_bcs! (particle_interaction.jl:256-259) */
+        if (m->h_state.err_sp >= PI_ERR_SP_RELEASE)   /* particle_release! (particle_interaction.jl:131-134) */
+            snprintf(buf, sizeof buf, "number of abiotic particles exceeds the capacity at timestep %g days "
+                     "(release into abiotic species %u)", day, m->h_state.err_sp - PI_ERR_SP_RELEASE + 1);
+        else if (m->h_state.err_sp >= PI_ERR_SP_ABIO)      /* particles_from_bcs! (particle_interaction.jl:256-259) */
             snprintf(buf, sizeof buf, "number of abiotic particles exceeds the capacity at timestep %g days "
                      "(abiotic species %u)", day, m->h_state.err_sp - PI_ERR_SP_ABIO + 1);
         else
@@ -360,6 +370,17 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
         DALLOC(m->afflag, ns * 4); DALLOC(m->afpre, (ns + 1) * 4); DALLOC(m->aflist, ns * 4);
         DALLOC(m->apart, (long long)scan_tiles((uint32_t)std::max(nc, ns)) * 4 + 64);
         DALLOC(m->anum, nc * 4);
+        /* diags.phytos.spN.ptc (IronEnergyMode: the only mode whose ptc can change) */
+        if (d->mode == PI_MODE_IRONENERGY) DALLOC(m->diag_ptc, (long long)m->nsp * g.ncell * 4);
+        m->npint = d->n_interactions; m->nprel = d->n_releases; m->mc = d->max_candidates;
+        for (int q = 0; q < m->npint; ++q) { m->pint[q][0] = d->interactions[2 * q]; m->pint[q][1] = d->interactions[2 * q + 1]; }
+        for (int q = 0; q < m->nprel; ++q) { m->prel[q][0] = d->releases[2 * q]; m->prel[q][1] = d->releases[2 * q + 1]; }
+        if (m->npint + m->nprel > 0) {
+            DALLOC(m->icnt, g.ncell * 4); DALLOC(m->ioff, g.ncell * 4); DALLOC(m->ifill, g.ncell * 4);
+            DALLOC(m->ilist, m->cap * 4); DALLOC(m->rflag, m->cap * 4); DALLOC(m->rrank, m->cap * 4);
+            DALLOC(m->ifflag, m->cap * 4); DALLOC(m->ifpre, m->cap * 4); DALLOC(m->iflist, m->cap * 4);
+            DALLOC(m->rdidx, m->ctot * 4);
+        }
         DALLOC(m->d_abc, sizeof(AbioBc));
     }
     /* store x2 */
@@ -433,7 +454,7 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
     DALLOC(m->dstamp, m->ctot * 4); DALLOC(m->rstamp, m->ctot * 4);
     DALLOC(m->dcell, m->ctot * 4); DALLOC(m->rcons, m->ctot * PI_RCONS * 4);
     m->mode = d->mode;
-    if (m->mode == PI_MODE_CARBON) DALLOC(m->dbd, m->ctot * 4);
+    if (m->mode == PI_MODE_CARBON || m->mode == PI_MODE_IRONENERGY) DALLOC(m->dbd, m->ctot * 4);
     if (!m->dry) {   /* stamps start below any iteration */
         std::vector<int32_t> neg(m->ctot, -1);
         DEVCALL(hipMemcpy(m->dstamp, neg.data(), m->ctot * 4, hipMemcpyHostToDevice));
@@ -484,9 +505,19 @@ static pi_status check_desc(const pi_model_desc* d)
     }
     if (d->n_abiotic < 0 || d->n_abiotic > PI_MAXSA) return fail(PI_ERR_ARG, "n_abiotic out of range [0, 4]");
     if (d->n_abiotic > 0 && !d->abiotic_params) return fail(PI_ERR_ARG, "abiotic particle parameters missing");
-    if (d->n_interactions != 0)
-        return fail(PI_ERR_UNSUPPORTED, "phytoplankton-particle interaction (palat.intac) needs IronEnergyMode's "
-                                        "max_ptc; IronEnergyMode is not implemented");
+    if (d->n_interactions < 0 || d->n_interactions > PI_MAXPAIRS || d->n_releases < 0 || d->n_releases > PI_MAXPAIRS)
+        return fail(PI_ERR_UNSUPPORTED, "at most 4 interaction and 4 release pairs");
+    if ((d->n_interactions && !d->interactions) || (d->n_releases && !d->releases))
+        return fail(PI_ERR_ARG, "interaction / release pairs missing");
+    if (d->n_interactions && d->mode != PI_MODE_IRONENERGY)
+        return fail(PI_ERR_ARG, "phytoplankton-particle interaction needs IronEnergyMode (max_ptc, ptc)");
+    if (d->n_interactions + d->n_releases > 0 && (d->max_candidates < 1 || d->max_candidates > PI_MAXCAND))
+        return fail(PI_ERR_UNSUPPORTED, "max_candidates must be in [1, 64]");
+    for (int q = 0; q < d->n_interactions + d->n_releases; ++q) {
+        const int32_t* pr = q < d->n_interactions ? d->interactions + 2 * q : d->releases + 2 * (q - d->n_interactions);
+        if (pr[0] < 0 || pr[0] >= d->n_species || pr[1] < 0 || pr[1] >= d->n_abiotic)
+            return fail(PI_ERR_ARG, "interaction / release pair out of range");
+    }
     {
         const long long nc = (long long)(gd.Nx + 2 * gd.Hx) * (gd.Ny + 2 * gd.Hy) * (gd.Nz + 2 * gd.Hz);
         if ((long long)d->n_abiotic * nc >= (1ll << 31) || (long long)d->n_abiotic * d->capacity >= (1ll << 31))
@@ -916,6 +947,7 @@ static AbioArgs abio_args(const pi_model* m, float dt, const Vel& V0, const Vel&
     a.x = m->ax; a.y = m->ay; a.z = m->az; a.xi = m->axi; a.yi = m->ayi; a.zi = m->azi; a.ac = m->aac;
     a.cflag = m->acflag; a.cpre = m->acpre; a.fflag = m->afflag; a.fpre = m->afpre; a.flist = m->aflist;
     a.num = m->anum; a.bc = m->d_abc;
+    a.num_late = m->npint + m->nprel > 0 ? 1 : 0;
     for (int sa = 0; sa < m->nsa; ++sa) {
         for (int f = 0; f < PI_NBC; ++f)
             if (m->h_abc.kind[sa][f] != PI_BCK_NONE) a.bcm |= 1u << (6 * sa + f);
@@ -1043,6 +1075,7 @@ static void build_step(pi_model* m, float dt, Plan& P)
         aa.seed = m->seed;
         aa.newc = m->newc; aa.stayf = m->stayf; aa.small = m->small; aa.qcap = m->qcap; aa.cnts = m->cnts;
         aa.dstamp = m->dstamp; aa.dcell = m->dcell; aa.cap = m->cap; aa.dbd = m->dbd;
+        aa.dbd_ptc = m->mode == PI_MODE_IRONENERGY ? 1 : 0;
         /* up to 16384 blocks by default: with 8.4M individuals (two entries per
          * thread) ~5% faster than 4096 blocks; 32768 is no faster, 65536 much slower */
         const long long ab = grid_for(2 * m->ctot, 256, (int)m->shape[PI_S_ADV_BLOCKS]);
@@ -1087,6 +1120,7 @@ static void build_step(pi_model* m, float dt, Plan& P)
         pa.newc = m->newc; pa.sflag = m->sflag;
         pa.con = m->con; pa.diag_num = m->diag; pa.dt = dt; pa.seed = m->seed;
         pa.dstamp = m->dstamp; pa.dcell = m->dcell; pa.rstamp = m->rstamp; pa.rcons = m->rcons; pa.dbd = m->dbd;
+        pa.rdidx = m->rdidx;
         const long long phb = grid_for(2 * m->ctot, 256, (int)m->shape[PI_S_PHYS_BLOCKS]);
         P.add(K_PHYS, "phys", m->mode == PI_MODE_QUOTA ? phb : 0, pa);
         P.add(K_PHYSC, "phys", m->mode == PI_MODE_CARBON ? phb : 0, pa);
@@ -1113,6 +1147,22 @@ static void build_step(pi_model* m, float dt, Plan& P)
         P.add(K_PLK_LONG, "plk", iron ? 0 : m->long_blocks, ka);
         P.add(K_PLKI, "plk", iron ? kb : 0, ka);
         P.add(K_PLK_LONGI, "plk", iron ? m->long_blocks : 0, ka);
+    }
+    {   /* phytoplankton-particle interaction, release, ptc and num (time_step.jl:120-159) */
+        InteractArgs ia = zeroed<InteractArgs>();
+        ia.g = g; ia.st = st; ia.seg = m->seg; ia.rstamp = m->rstamp; ia.rdidx = m->rdidx; ia.params = m->d_params;
+        for (int sa = 0; sa < m->nsa; ++sa) for (int q = 0; q < PI_NAPARAMS; ++q) ia.ap[sa][q] = m->aparams[sa][q];
+        ia.cap = m->cap; ia.nsp = m->nsp; ia.nsa = m->nsa;
+        ia.x = m->ax; ia.y = m->ay; ia.z = m->az; ia.xi = m->axi; ia.yi = m->ayi; ia.zi = m->azi; ia.ac = m->aac;
+        ia.num = m->anum; ia.diag_ptc = m->diag_ptc;
+        ia.cnt = m->icnt; ia.off = m->ioff; ia.fill = m->ifill; ia.list = m->ilist;
+        ia.rflag = m->rflag; ia.rrank = m->rrank; ia.fflag = m->ifflag; ia.fpre = m->ifpre; ia.flist = m->iflist;
+        ia.nint = m->npint; ia.nrel = m->nprel; ia.mc = m->mc;
+        for (int q = 0; q < PI_MAXPAIRS; ++q)
+            for (int j = 0; j < 2; ++j) { ia.pint[q][j] = m->pint[q][j]; ia.prel[q][j] = m->prel[q][j]; }
+        ia.num_late = m->npint + m->nprel > 0 ? 1 : 0;
+        ia.dt = dt; ia.seed = m->seed;
+        P.add(K_INTERACT, "interact", (m->npint + m->nprel > 0 || m->diag_ptc) ? 1 : 0, ia);
     }
     P.mark(K_JOIN);
     /* tracers: k_final and the halo fill of the new tracers, after the join */
@@ -1338,6 +1388,11 @@ pi_status pi_download_diag(pi_model* m, int32_t sp, int32_t which, float* h)
     if (!m || !h || sp < 0 || sp >= m->nsp || which < 0 || which >= PI_NDIAG) return fail(PI_ERR_ARG, "diag id");
     pi_status s = sync_state(m);
     if (s != PI_OK && s != PI_ERR_CAPACITY) return s;
+    if (which == PI_D_ptc) {   /* zero unless an IronEnergyMode model with abiotic particles */
+        if (m->diag_ptc) HIPCHECK(hipMemcpy(h, m->diag_ptc + (long long)sp * m->ncell, m->ncell * 4, hipMemcpyDeviceToHost));
+        else memset(h, 0, m->ncell * 4);
+        return s;
+    }
     HIPCHECK(hipMemcpy(h, m->diag + ((long long)sp * 4 + which) * m->ncell, m->ncell * 4, hipMemcpyDeviceToHost));
     return s;
 }
@@ -1360,6 +1415,7 @@ pi_status pi_reset_diags(pi_model* m)
     HIPCHECK(hipMemset(m->diag, 0, (size_t)m->nsp * 4 * m->ncell * 4));
     for (float* p : m->diag_tr) HIPCHECK(hipMemset(p, 0, m->ncell * 4));
     if (m->nsa) HIPCHECK(hipMemset(m->anum, 0, (size_t)m->nsa * m->ncell * 4));
+    if (m->diag_ptc) HIPCHECK(hipMemset(m->diag_ptc, 0, (size_t)m->nsp * m->ncell * 4));
     return PI_OK;
 }
 

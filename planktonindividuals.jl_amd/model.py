@@ -307,6 +307,7 @@ class ModelSpec:
         if not isinstance(mode, (P.QuotaMode, P.CarbonMode, P.MacroMolecularMode, P.IronEnergyMode)):
             raise NotImplementedError(f"{mode}: not a physiology mode")
         self.n_abiotic, self.abiotic_params, self.abiotic_matrix, self.abiotic_N = 0, None, None, []
+        self.palat = ((), (), int(max_candidates))
         if abiotic is not None:                       # models.jl:99-117
             if not isinstance(abiotic, abiotic_setup):
                 raise ValueError("PlanktonModel:`abiotic` must be either Nothing or `abiotic_setup`!")
@@ -324,15 +325,24 @@ class ModelSpec:
             else:
                 raise ValueError("Abiotic particle parameters must be either Nothing or Dict!")
             nphy = phyto.Nsp if phyto is not None else 1
-            for pair in list(abiotic.palat.intac) + list(abiotic.palat.release):     # models.jl:121-142
-                sp, sa = (str(v).lstrip(":") for v in pair)
-                if not (sp.startswith("sp") and sp[2:].isdigit() and 1 <= int(sp[2:]) <= nphy):
-                    raise ValueError(f"Abiotic: {sp} is not generated")
-                if not (sa.startswith("sa") and sa[2:].isdigit() and 1 <= int(sa[2:]) <= abiotic.Nsa):
-                    raise ValueError(f"Abiotic: {sa} is not generated")
-            if abiotic.palat.intac:
-                raise NotImplementedError("phytoplankton-particle interaction needs IronEnergyMode's max_ptc "
-                                          "(particle_interaction.jl:60); IronEnergyMode is not implemented")
+            pairs = {"intac": [], "release": []}
+            for kind in ("intac", "release"):                                       # models.jl:121-142
+                for pair in getattr(abiotic.palat, kind):
+                    sp, sa = (str(v).lstrip(":") for v in pair)
+                    if not (sp.startswith("sp") and sp[2:].isdigit() and 1 <= int(sp[2:]) <= nphy):
+                        raise ValueError(f"Abiotic: {sp} is not generated")
+                    if not (sa.startswith("sa") and sa[2:].isdigit() and 1 <= int(sa[2:]) <= abiotic.Nsa):
+                        raise ValueError(f"Abiotic: {sa} is not generated")
+                    pairs[kind].append((int(sp[2:]) - 1, int(sa[2:]) - 1))
+            if pairs["intac"] and not isinstance(mode, P.IronEnergyMode):
+                # particle_interaction! reads plank_p.max_ptc (particle_interaction.jl:60), an
+                # IronEnergyMode parameter: the other modes fail there
+                raise ValueError("phytoplankton-particle interaction needs IronEnergyMode (max_ptc)")
+            if len(pairs["intac"]) > 4 or len(pairs["release"]) > 4:
+                raise NotImplementedError("at most 4 interaction and 4 release pairs per model on the MI355X path")
+            if (pairs["intac"] or pairs["release"]) and not 1 <= int(max_candidates) <= 64:
+                raise NotImplementedError("max_candidates in [1, 64] on the MI355X path")
+            self.palat = (pairs["intac"], pairs["release"], int(max_candidates))
             self.n_abiotic = abiotic.Nsa
             self.abiotic_params = ap
             self.abiotic_matrix = P.abiotic_param_matrix(ap, abiotic.Nsa)
@@ -416,7 +426,7 @@ class PlanktonModel:
         self.t = spec.t
         self._native = NativeModel(self.grid, self.n_species, self.max_individuals, self.species_params,
                                    spec.bgc_vector, spec.seed, (), device=arch.device, mode=spec.mode_id,
-                                   abiotic_params=spec.abiotic_matrix)
+                                   abiotic_params=spec.abiotic_matrix, palat=spec.palat)
         if spec.host_state:
             for s in range(self.n_species):
                 self._native.upload_individuals(s, spec.initial_individuals[s])

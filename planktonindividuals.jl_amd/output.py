@@ -23,8 +23,9 @@ src/Output/write_outputs.jl).
   has reached the limit, the first file is renamed `<prefix>_part1.jld2` and
   writing continues in `<prefix>_part2.jld2`, `_part3`, ...
 
-The reference's per-individual rate diagnostics are not on this path (`ptc`
-is written as zeros).
+The reference's per-individual rate diagnostics are not on this path; the
+`ptc` diagnostic is the device's (zero unless an IronEnergyMode model has
+abiotic particles).
 """
 import ctypes as C
 import ctypes.util

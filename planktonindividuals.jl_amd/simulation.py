@@ -77,7 +77,7 @@ class PlanktonDiagnostics:
         out = {}
         for s in range(self.model.n_species):
             d = {k: self.model._native.diag(s, k) for k in ("num", "graz", "mort", "dvid")}
-            d["ptc"] = np.zeros(self.model.grid.padded_shape, np.float32, order="F")
+            d["ptc"] = self.model._native.diag(s, "ptc")     # zero unless IronEnergyMode with abiotics
             out[f"sp{s + 1}"] = d
         return out
 

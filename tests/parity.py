@@ -32,7 +32,7 @@ def compare_models(gpu, ref, nsp, tracer_ids=range(14), grid_fields=("par", "Chl
     """Returns a list of mismatch descriptions (empty = identical).  Besides the
     individuals and tracers: par, Chl, pop, the accumulated PAR diagnostic
     (diags.tracer.PAR, time_step.jl:167), PARF after the column scan, the
-    per-species diagnostics, and the tracer diagnostics of `diag_tracers`
+    per-species diagnostics (ptc included), and the tracer diagnostics of `diag_tracers`
     (time_step.jl:168-172); `extra_fields` adds the extra state columns of a
     mode (ex0 = RNA in MacroMolecularMode)."""
     bad = []
@@ -57,7 +57,7 @@ def compare_models(gpu, ref, nsp, tracer_ids=range(14), grid_fields=("par", "Chl
             bad.append(f"{g}: {first_diff(a, b)}")
     if diags:
         for s in range(nsp):
-            for d in ("num", "graz", "mort", "dvid"):
+            for d in ("num", "graz", "mort", "dvid", "ptc"):
                 a, b = gpu.diag(s, d), ref.diag(s, d)
                 if not same(a, b):
                     bad.append(f"diag sp{s + 1}.{d}: {first_diff(a, b)}")

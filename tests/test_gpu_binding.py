@@ -39,7 +39,9 @@ class ModelDesc(C.Structure):         # pi_model_desc
                 ("params", fp), ("bgc", fp), ("seed", C.c_uint64),
                 ("n_diag", C.c_int32), ("diag", C.POINTER(C.c_int32)), ("mode", C.c_int32),
                 ("metrics", C.c_void_p),   # NULL: RectilinearGrid
-                ("n_abiotic", C.c_int32), ("abiotic_params", C.c_void_p), ("n_interactions", C.c_int32)]
+                ("n_abiotic", C.c_int32), ("abiotic_params", C.c_void_p), ("n_interactions", C.c_int32),
+                ("interactions", C.c_void_p), ("n_releases", C.c_int32), ("releases", C.c_void_p),
+                ("max_candidates", C.c_int32)]
 
 
 FLOAT_COLS = ("x", "y", "z", "Sz", "Bm", "Cq", "Nq", "Pq", "Chl", "gen", "age", "graz", "mort", "dvid")
