@@ -151,7 +151,7 @@ struct AbioArgs {
     const uint32_t* fpre;
     uint32_t* flist;                  /* [nsa][cap]: the free slots of each species in ascending order */
     float* num;                       /* [nsa][ncell]: diags.abiotics.saN.num */
-    int num_late;                     /* 1: num is counted after the interactions (k_interact) */
+    int num_late;                     /* 1: num is counted after the interactions (k_int_num) */
     const AbioBc* bc;
     uint32_t bcm;                     /* bit 6·sa + f: species sa has a condition on face f */
     float p[PI_MAXSA][PI_NAPARAMS];
@@ -229,7 +229,9 @@ enum KId {
     K_FINISH, K_XYW0, K_XYW1, K_PHYSC, K_FINAL2, K_FINAL3,
     K_ADV_GM, K_SWEEPX2, K_SWEEPX3, K_SWEEPY2, K_SWEEPY3, K_FINAL4, K_FINAL5, K_FINAL6, K_FINAL7,
     K_ABIO_MARK, K_ABIO_FREE, K_ABIO_LIST, K_ABIO_CHECK, K_ABIO_SPAWN, K_ABIO_ADV, K_ABIO_ADV_GM,
-    K_PHYSM, K_IDLEM, K_PHYSI, K_IDLEI, K_PLKI, K_PLK_LONGI, K_INTERACT, K_NKID
+    K_PHYSM, K_IDLEM, K_PHYSI, K_IDLEI, K_PLKI, K_PLK_LONGI,
+    K_INT_ZERO, K_INT_COUNT, K_INT_LIST, K_INT_CELLS, K_REL_ZERO, K_REL_FLAG, K_REL_CHECK, K_REL_LIST,
+    K_REL_WRITE, K_INT_DIAG, K_INT_NUM, K_NKID
 };
 /* plan markers (not kernels): fork to / join from the side stream */
 enum { K_FORK = K_NKID, K_JOIN = K_NKID + 1 };
@@ -289,11 +291,11 @@ struct InteractArgs {
     uint8_t* ac;
     float* num;                       /* [nsa][ncell] */
     float* diag_ptc;                  /* [nsp][ncell]: diags.phytos.spN.ptc (IronEnergyMode; else null) */
-    uint32_t *cnt, *off, *fill, *list;            /* per cell counts / offsets, the cells' particle lists */
-    uint32_t *rflag, *rrank, *fflag, *fpre, *flist;   /* per slot: releases, free abiotic slots */
-    int nint, nrel, mc;
+    uint32_t *cnt, *off, *fill, *list;            /* per cell counts / offsets (+ total), the cells' particle lists */
+    uint32_t *rflag, *rrank, *fflag, *fpre, *flist;   /* per slot: releases, free abiotic slots (ranks + totals) */
+    int q;                            /* the pair this launch handles */
+    int mc;
     int pint[PI_MAXPAIRS][2], prel[PI_MAXPAIRS][2];   /* (species, abiotic species) */
-    int num_late;                     /* count num here (after the interactions), not in k_abio_adv */
     float dt;
     uint64_t seed;
 };

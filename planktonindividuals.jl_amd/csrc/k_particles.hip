@@ -2108,192 +2108,222 @@ PI_KERNEL(k_abio_adv, AbioArgs, 256)
 PI_KERNEL(k_abio_adv_gm, AbioArgs, 256)
 
 /* ------------------------------------ phytoplankton-particle interaction
- * Abiotic/particle_interaction.jl:1-157 and time_step.jl:120-159, run after the
+ * Abiotic/particle_interaction.jl:1-145 and time_step.jl:120-159, run after the
  * physiology of every species on the next step's store (k_phys has written it):
  * the living individuals at their cells, a daughter reborn into a dying
  * individual's slot at that slot's cell, other daughters at the halo cell
- * (Hx, Hy, Hz) where no particle is.  One workgroup of 1024 threads per model
- * runs the pairs in the reference's order; its work per pair is linear in the
- * slots and cells, plus, per cell, particles x individuals of the cell (the
- * reference's calc_interaction_kernel! pairs every individual with every
- * particle and keeps those in the same cell). */
+ * (Hx, Hy, Hz) where no particle is.  The reference pairs every individual with
+ * every particle (calc_interaction_kernel!) and keeps those of the same cell;
+ * here the pair's particles are listed per cell (count, scan, fill) and one
+ * thread per cell walks its particles in slot order against the species'
+ * individuals of the cell, so the work is linear in the slots and cells plus,
+ * per cell, particles x individuals of the cell.  The reference's order matters
+ * only within a cell (a particle can join an individual of its own cell only),
+ * so cells run in parallel.  Releases are ranked by device-wide scans.  Every
+ * kernel below handles the pair a.q of a.pint / a.prel. */
 
-/* exclusive prefix sums of v[0, n) into out by one workgroup; returns the total */
-__device__ uint32_t block_scan_excl(const uint32_t* v, uint32_t* out, long long n, uint32_t* wsum)
-{
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6, nw = (int)(blockDim.x >> 6);
-    uint32_t carry = 0;
-    for (long long base = 0; base < n; base += blockDim.x) {
-        const long long i = base + t;
-        const uint32_t x = i < n ? v[i] : 0u;
-        uint32_t sx = x;
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t y = __shfl_up(sx, d, 64);
-            if (lane >= d) sx += y;
-        }
-        if (lane == 63) wsum[w] = sx;
-        __syncthreads();
-        if (w == 0) {
-            const uint32_t ws = lane < nw ? wsum[lane] : 0u;
-            uint32_t s2 = ws;
-            for (int d = 1; d < 64; d <<= 1) {
-                const uint32_t y = __shfl_up(s2, d, 64);
-                if (lane >= d) s2 += y;
-            }
-            if (lane < nw) wsum[lane] = s2 - ws;
-            if (lane == nw - 1) wsum[32] = s2;
-        }
-        __syncthreads();
-        if (i < n) out[i] = carry + wsum[w] + sx - x;
-        carry += wsum[32];
-        __syncthreads();
-    }
-    return carry;
+__device__ __forceinline__ uint32_t int_acell(const InteractArgs& a, long long r)
+{   /* the particle's cell (find_inds! of this step; stale for inactive slots) */
+    const DevGrid& g = a.g;
+    const int xi = min(max((int)a.xi[r], 1), g.Px), yi = min(max((int)a.yi[r], 1), g.Py), zi = min(max((int)a.zi[r], 1), g.Pz);
+    return (uint32_t)gidx(g, xi, yi, zi);
+}
+__device__ __forceinline__ uint32_t int_nout(const InteractArgs& a, const DevState* st)
+{   /* rows of the next store: the cell segments and the daughters after them */
+    uint32_t n = a.seg[a.g.ncell];
+    for (int s = 0; s < a.nsp; ++s) n += st->dtot[s];
+    return n;
 }
 
-__device__ __forceinline__ void k_interact_body(const InteractArgs& a, Blk K)
+/* reset_interaction!: the per-cell counts and fill positions */
+__device__ __forceinline__ void k_int_zero_body(const InteractArgs& a, Blk K)
+{
+    if (a.st->err) return;
+    for (long long c = K.b * (long long)blockDim.x + threadIdx.x; c < a.g.ncell; c += (long long)K.n * blockDim.x) {
+        a.cnt[c] = 0u;
+        a.fill[c] = 0u;
+    }
+}
+__device__ __forceinline__ void k_int_count_body(const InteractArgs& a, Blk K)
+{
+    if (a.st->err) return;
+    const long long r0 = (long long)a.pint[a.q][1] * a.cap;
+    for (long long k = K.b * (long long)blockDim.x + threadIdx.x; k < a.cap; k += (long long)K.n * blockDim.x)
+        if (a.ac[r0 + k]) atomicAdd(&a.cnt[int_acell(a, r0 + k)], 1u);
+}
+__device__ __forceinline__ void k_int_list_body(const InteractArgs& a, Blk K)
+{
+    if (a.st->err) return;
+    const long long r0 = (long long)a.pint[a.q][1] * a.cap;
+    for (long long k = K.b * (long long)blockDim.x + threadIdx.x; k < a.cap; k += (long long)K.n * blockDim.x)
+        if (a.ac[r0 + k]) {
+            const uint32_t c = int_acell(a, r0 + k);
+            a.list[a.off[c] + atomicAdd(&a.fill[c], 1u)] = (uint32_t)k;
+        }
+}
+/* calc_interaction_kernel! and merge_particle_kernel! (:22-72), one cell per thread */
+__device__ __forceinline__ void k_int_cells_body(const InteractArgs& a, Blk K)
+{
+    DevState* st = a.st;
+    if (st->err) return;
+    const DevGrid& g = a.g;
+    const int sp = a.pint[a.q][0], sa = a.pint[a.q][1];
+    const long long r0 = (long long)sa * a.cap;
+    const float Rd = a.ap[sa][PI_A_Rd];
+    const float maxptc = a.params[sp * PI_NPARAMS + PI_I_max_ptc];
+    const int32_t it = (int32_t)(st->iteration + 1);
+    const GStore O = gstore(st->store[1 - st->scur]);
+    const uint32_t sxy = (uint32_t)g.Px * (uint32_t)g.Py;
+    for (long long c = K.b * (long long)blockDim.x + threadIdx.x; c < g.ncell; c += (long long)K.n * blockDim.x) {
+        const uint32_t m = a.cnt[c];
+        if (!m) continue;
+        uint32_t* L = a.list + a.off[c];
+        for (uint32_t q = 1; q < m; ++q) {          /* the cell's particles in slot order */
+            const uint32_t v = L[q];
+            uint32_t j = q;
+            while (j > 0 && L[j - 1] > v) { L[j] = L[j - 1]; --j; }
+            L[j] = v;
+        }
+        /* this species' individuals of the cell (the segment is ordered by species, slot) */
+        uint32_t e0 = a.seg[c], e1 = a.seg[c + 1];
+        while (e0 < e1 && (O.key[e0] >> PI_SLOT_BITS) < (uint32_t)sp) ++e0;
+        uint32_t e2 = e0;
+        while (e2 < e1 && (O.key[e2] >> PI_SLOT_BITS) == (uint32_t)sp) ++e2;
+        if (e0 == e2) continue;
+        const int ci = (int)(c % g.Px) + 1, cj = (int)((c / g.Px) % g.Py) + 1, ck = (int)(c / sxy) + 1;
+        const float DX = g.gm ? g.mdxF[col2(g, ci, cj)] : g.dx;       /* ΔxF, ΔyF, ΔzF of the cell */
+        const float DY = g.gm ? g.mdyF[col2(g, ci, cj)] : g.dy;
+        const float DZ = dzF_at(g, ck);
+        for (uint32_t q = 0; q < m; ++q) {
+            const long long r = r0 + L[q];
+            uint32_t cidx[PI_MAXCAND];                    /* intac[:, k]: the candidate's row, ~0u = none */
+            for (int u = 0; u < a.mc; ++u) cidx[u] = ~0u;
+            const float px = a.x[r], py = a.y[r], pz = a.z[r];
+            for (uint32_t e = e0; e < e2; ++e) {
+                const uint32_t slot = O.key[e] & PI_SLOT_MASK;
+                uint32_t o;
+                if (O.flags[e] & F_AC) o = e;
+                else if (a.rstamp[(long long)sp * a.cap + slot] == it) o = a.rdidx[(long long)sp * a.cap + slot];
+                else continue;
+                const float dx = (O.x[o] - px) * DX, dy = (O.y[o] - py) * DY, dz = (O.z[o] - pz) * DZ;
+                const float dist = sqrtf(pi_pow2f(dx) + pi_pow2f(dy) + pi_pow2f(dz));
+                if (dist < Rd) cidx[(slot + 1u) % (uint32_t)a.mc] = o;   /* slot = (i % max_candidates) + 1 */
+            }
+            for (int u = 0; u < a.mc; ++u) {              /* merge_particle_kernel! */
+                const uint32_t o = cidx[u];
+                if (o != ~0u && O.ex[7][o] < maxptc) {
+                    O.ex[7][o] = O.ex[7][o] + 1.0f;
+                    a.ac[r] = 0;
+                    break;
+                }
+            }
+        }
+    }
+}
+
+/* particle_release! (:86-138): flags of the releasers by slot and of the free
+ * particle slots, two scans, the capacity check, the free-slot list, then the
+ * k-th releaser in slot order fills the k-th free slot; rnd.x is this call's
+ * draw, rnd.y and rnd.z still hold the last species' mortality and division
+ * draws of get_probability! */
+__device__ __forceinline__ void k_rel_zero_body(const InteractArgs& a, Blk K)
+{
+    if (a.st->err) return;
+    const long long r0 = (long long)a.prel[a.q][1] * a.cap;
+    for (long long s = K.b * (long long)blockDim.x + threadIdx.x; s < a.cap; s += (long long)K.n * blockDim.x) {
+        a.rflag[s] = 0u;
+        a.fflag[s] = a.ac[r0 + s] ? 0u : 1u;
+    }
+}
+__device__ __forceinline__ void k_rel_flag_body(const InteractArgs& a, Blk K)
+{
+    const DevState* st = a.st;
+    if (st->err) return;
+    const int sp = a.prel[a.q][0];
+    const float rP = a.ap[a.prel[a.q][1]][PI_A_release_P];
+    const int32_t it = (int32_t)(st->iteration + 1);
+    const GStore O = gstore(st->store[1 - st->scur]);
+    const uint32_t nout = int_nout(a, st);
+    for (long long e = K.b * (long long)blockDim.x + threadIdx.x; e < nout; e += (long long)K.n * blockDim.x) {
+        const uint32_t key = O.key[e];
+        if ((key >> PI_SLOT_BITS) != (uint32_t)sp || !(O.flags[e] & F_AC)) continue;
+        const uint32_t slot = key & PI_SLOT_MASK;
+        float u0, u1, u2;
+        pi_uniform3(a.seed, it, PI_RELEASE_SPECIES + a.q, PI_KIND_RELEASE, slot, &u0, &u1, &u2);
+        if (jl_islessf(u0, rP * a.dt * O.ex[7][e])) a.rflag[slot] = 1u;
+    }
+}
+__device__ __forceinline__ void k_rel_check_body(const InteractArgs& a, Blk K)
 {
     (void)K;
     DevState* st = a.st;
+    if (threadIdx.x == 0 && !st->err && a.rrank[a.cap] > a.fpre[a.cap]) {
+        st->err = PI_ERR_CAPACITY;
+        st->err_sp = PI_ERR_SP_RELEASE + (uint32_t)a.prel[a.q][1];
+    }
+}
+__device__ __forceinline__ void k_rel_list_body(const InteractArgs& a, Blk K)
+{
+    if (a.st->err) return;
+    for (long long s = K.b * (long long)blockDim.x + threadIdx.x; s < a.cap; s += (long long)K.n * blockDim.x)
+        if (a.fflag[s]) a.flist[a.fpre[s]] = (uint32_t)s;
+}
+__device__ __forceinline__ void k_rel_write_body(const InteractArgs& a, Blk K)
+{
+    const DevState* st = a.st;
     if (st->err) return;
-    __shared__ uint32_t wsum[33];
-    const DevGrid& g = a.g;
-    const long long ncell = g.ncell, cap = a.cap;
+    const int sp = a.prel[a.q][0];
+    const long long r0 = (long long)a.prel[a.q][1] * a.cap;
     const int32_t it = (int32_t)(st->iteration + 1);
     const GStore O = gstore(st->store[1 - st->scur]);
-    uint32_t nout = a.seg[ncell];
-    for (int s = 0; s < a.nsp; ++s) nout += st->dtot[s];
-    const long long t = threadIdx.x, nt = blockDim.x;
-    const uint32_t sxy = (uint32_t)g.Px * (uint32_t)g.Py;
-    auto acell = [&](long long r) {   /* the particle's cell (find_inds! of this step) */
-        const int xi = min(max((int)a.xi[r], 1), g.Px), yi = min(max((int)a.yi[r], 1), g.Py), zi = min(max((int)a.zi[r], 1), g.Pz);
-        return (uint32_t)gidx(g, xi, yi, zi);
-    };
-    /* particle_interaction! (:17-84), pair by pair */
-    for (int p = 0; p < a.nint; ++p) {
-        const int sp = a.pint[p][0], sa = a.pint[p][1];
-        const long long r0 = (long long)sa * cap;
-        const float Rd = a.ap[sa][PI_A_Rd];
-        const float maxptc = a.params[sp * PI_NPARAMS + PI_I_max_ptc];
-        for (long long c = t; c < ncell; c += nt) { a.cnt[c] = 0u; a.fill[c] = 0u; }
-        __syncthreads();
-        for (long long k = t; k < cap; k += nt)
-            if (a.ac[r0 + k]) atomicAdd(&a.cnt[acell(r0 + k)], 1u);
-        __syncthreads();
-        block_scan_excl(a.cnt, a.off, ncell, wsum);
-        for (long long k = t; k < cap; k += nt)
-            if (a.ac[r0 + k]) {
-                const uint32_t c = acell(r0 + k);
-                a.list[a.off[c] + atomicAdd(&a.fill[c], 1u)] = (uint32_t)k;
-            }
-        __syncthreads();
-        for (long long c = t; c < ncell; c += nt) {
-            const uint32_t m = a.cnt[c];
-            if (!m) continue;
-            uint32_t* L = a.list + a.off[c];
-            for (uint32_t q = 1; q < m; ++q) {          /* the cell's particles in slot order */
-                const uint32_t v = L[q];
-                uint32_t j = q;
-                while (j > 0 && L[j - 1] > v) { L[j] = L[j - 1]; --j; }
-                L[j] = v;
-            }
-            /* this species' individuals of the cell (the segment is ordered by species, slot) */
-            uint32_t e0 = a.seg[c], e1 = a.seg[c + 1];
-            while (e0 < e1 && (O.key[e0] >> PI_SLOT_BITS) < (uint32_t)sp) ++e0;
-            uint32_t e2 = e0;
-            while (e2 < e1 && (O.key[e2] >> PI_SLOT_BITS) == (uint32_t)sp) ++e2;
-            if (e0 == e2) continue;
-            const int ci = (int)(c % g.Px) + 1, cj = (int)((c / g.Px) % g.Py) + 1, ck = (int)(c / sxy) + 1;
-            const float DX = g.gm ? g.mdxF[col2(g, ci, cj)] : g.dx;       /* ΔxF, ΔyF, ΔzF of the cell */
-            const float DY = g.gm ? g.mdyF[col2(g, ci, cj)] : g.dy;
-            const float DZ = dzF_at(g, ck);
-            for (uint32_t q = 0; q < m; ++q) {
-                const long long r = r0 + L[q];
-                uint32_t cand[PI_MAXCAND], cidx[PI_MAXCAND];   /* calc_interaction_kernel!: intac[:, k] */
-                for (int u = 0; u < a.mc; ++u) cand[u] = 0u;
-                const float px = a.x[r], py = a.y[r], pz = a.z[r];
-                for (uint32_t e = e0; e < e2; ++e) {
-                    const uint32_t slot = O.key[e] & PI_SLOT_MASK;
-                    uint32_t o;
-                    if (O.flags[e] & F_AC) o = e;
-                    else if (a.rstamp[(long long)sp * cap + slot] == it) o = a.rdidx[(long long)sp * cap + slot];
-                    else continue;
-                    const float dx = (O.x[o] - px) * DX, dy = (O.y[o] - py) * DY, dz = (O.z[o] - pz) * DZ;
-                    const float dist = sqrtf(pi_pow2f(dx) + pi_pow2f(dy) + pi_pow2f(dz));
-                    if (dist < Rd) {                            /* slot = (i % max_candidates) + 1 */
-                        const uint32_t i1 = slot + 1u, u = i1 % (uint32_t)a.mc;
-                        cand[u] = i1; cidx[u] = o;
-                    }
-                }
-                for (int u = 0; u < a.mc; ++u) {               /* merge_particle_kernel! */
-                    if (cand[u] > 0u && O.ex[7][cidx[u]] < maxptc) {
-                        O.ex[7][cidx[u]] = O.ex[7][cidx[u]] + 1.0f;
-                        a.ac[r] = 0;
-                        break;
-                    }
-                }
-            }
-        }
-        __syncthreads();
+    const uint32_t nout = int_nout(a, st);
+    for (long long e = K.b * (long long)blockDim.x + threadIdx.x; e < nout; e += (long long)K.n * blockDim.x) {
+        const uint32_t key = O.key[e];
+        if ((key >> PI_SLOT_BITS) != (uint32_t)sp || !(O.flags[e] & F_AC)) continue;
+        const uint32_t slot = key & PI_SLOT_MASK;
+        if (!a.rflag[slot]) continue;
+        float u0, u1, u2, v0, v1, v2;
+        pi_uniform3(a.seed, it, PI_RELEASE_SPECIES + a.q, PI_KIND_RELEASE, slot, &u0, &u1, &u2);
+        pi_uniform3(a.seed, it, a.nsp - 1, PI_KIND_DECISION, slot, &v0, &v1, &v2);
+        const long long r = r0 + a.flist[a.rrank[slot]];
+        a.ac[r] = 1;
+        a.x[r] = O.x[e] + u0;
+        a.y[r] = O.y[e] + v1;
+        a.z[r] = O.z[e] + v2;
+        O.ex[7][e] = O.ex[7][e] - 1.0f;
     }
-    /* particle_release! (:86-145), pair by pair: the k-th releasing individual in
-     * slot order fills the k-th free slot of the particle species; rnd.x is this
-     * call's draw, rnd.y and rnd.z still hold the last species' mortality and
-     * division draws of get_probability! */
-    for (int p = 0; p < a.nrel; ++p) {
-        const int sp = a.prel[p][0], sa = a.prel[p][1];
-        const long long r0 = (long long)sa * cap;
-        const float rP = a.ap[sa][PI_A_release_P];
-        for (long long s = t; s < cap; s += nt) { a.rflag[s] = 0u; a.fflag[s] = a.ac[r0 + s] ? 0u : 1u; }
-        __syncthreads();
-        for (long long e = t; e < nout; e += nt) {
-            const uint32_t key = O.key[e];
-            if ((key >> PI_SLOT_BITS) != (uint32_t)sp || !(O.flags[e] & F_AC)) continue;
-            const uint32_t slot = key & PI_SLOT_MASK;
-            float u0, u1, u2;
-            pi_uniform3(a.seed, it, PI_RELEASE_SPECIES + p, PI_KIND_RELEASE, slot, &u0, &u1, &u2);
-            if (jl_islessf(u0, rP * a.dt * O.ex[7][e])) a.rflag[slot] = 1u;
-        }
-        __syncthreads();
-        const uint32_t nR = block_scan_excl(a.rflag, a.rrank, cap, wsum);
-        const uint32_t nF = block_scan_excl(a.fflag, a.fpre, cap, wsum);
-        if (nR > nF) {
-            if (t == 0) { st->err = PI_ERR_CAPACITY; st->err_sp = PI_ERR_SP_RELEASE + (uint32_t)sa; }
-            return;
-        }
-        for (long long s = t; s < cap; s += nt)
-            if (a.fflag[s]) a.flist[a.fpre[s]] = (uint32_t)s;
-        __syncthreads();
-        for (long long e = t; e < nout; e += nt) {
-            const uint32_t key = O.key[e];
-            if ((key >> PI_SLOT_BITS) != (uint32_t)sp || !(O.flags[e] & F_AC)) continue;
-            const uint32_t slot = key & PI_SLOT_MASK;
-            if (!a.rflag[slot]) continue;
-            float u0, u1, u2, v0, v1, v2;
-            pi_uniform3(a.seed, it, PI_RELEASE_SPECIES + p, PI_KIND_RELEASE, slot, &u0, &u1, &u2);
-            pi_uniform3(a.seed, it, a.nsp - 1, PI_KIND_DECISION, slot, &v0, &v1, &v2);
-            const long long r = r0 + a.flist[a.rrank[slot]];
-            a.ac[r] = 1;
-            a.x[r] = O.x[e] + u0;
-            a.y[r] = O.y[e] + v1;
-            a.z[r] = O.z[e] + v2;
-            O.ex[7][e] = O.ex[7][e] - 1.0f;
-        }
-        __syncthreads();
-    }
-    /* diags_proc! of ptc for every species (:140-149) */
-    if (a.diag_ptc)
-        for (long long e = t; e < nout; e += nt) {
-            if (!(O.flags[e] & F_AC)) continue;
-            const float v = O.ex[7][e];
-            if (v != 0.0f) atomicAdd(&a.diag_ptc[(long long)(O.key[e] >> PI_SLOT_BITS) * ncell + O.cell[e]], v);
-        }
-    /* diags_proc! of num for every abiotic species (:152-159) */
-    if (a.num_late)
-        for (long long r = t; r < (long long)a.nsa * cap; r += nt)
-            if (a.ac[r]) atomicAdd(&a.num[(r / cap) * ncell + acell(r)], 1.0f);
 }
-PI_KERNEL(k_interact, InteractArgs, 1024)
+/* diags_proc! of ptc for every species (time_step.jl:140-149) */
+__device__ __forceinline__ void k_int_diag_body(const InteractArgs& a, Blk K)
+{
+    const DevState* st = a.st;
+    if (st->err) return;
+    const GStore O = gstore(st->store[1 - st->scur]);
+    const uint32_t nout = int_nout(a, st);
+    for (long long e = K.b * (long long)blockDim.x + threadIdx.x; e < nout; e += (long long)K.n * blockDim.x) {
+        if (!(O.flags[e] & F_AC)) continue;
+        const float v = O.ex[7][e];
+        if (v != 0.0f) atomicAdd(&a.diag_ptc[(long long)(O.key[e] >> PI_SLOT_BITS) * a.g.ncell + O.cell[e]], v);
+    }
+}
+/* diags_proc! of num for every abiotic species (:152-159), after the pairs */
+__device__ __forceinline__ void k_int_num_body(const InteractArgs& a, Blk K)
+{
+    if (a.st->err) return;
+    for (long long r = K.b * (long long)blockDim.x + threadIdx.x; r < (long long)a.nsa * a.cap; r += (long long)K.n * blockDim.x)
+        if (a.ac[r]) atomicAdd(&a.num[(r / a.cap) * a.g.ncell + int_acell(a, r)], 1.0f);
+}
+PI_KERNEL(k_int_zero, InteractArgs, 256)
+PI_KERNEL(k_int_count, InteractArgs, 256)
+PI_KERNEL(k_int_list, InteractArgs, 256)
+PI_KERNEL(k_int_cells, InteractArgs, 256)
+PI_KERNEL(k_rel_zero, InteractArgs, 256)
+PI_KERNEL(k_rel_flag, InteractArgs, 256)
+PI_KERNEL(k_rel_check, InteractArgs, 64)
+PI_KERNEL(k_rel_list, InteractArgs, 256)
+PI_KERNEL(k_rel_write, InteractArgs, 256)
+PI_KERNEL(k_int_diag, InteractArgs, 256)
+PI_KERNEL(k_int_num, InteractArgs, 256)
 
 /* ------------------------------------------------------------- registry */
 void reg_particles(KInfo* k)
@@ -2326,5 +2356,15 @@ void reg_particles(KInfo* k)
     PI_REG(k, K_PLK_LONG, k_plk_long, PlkArgs, 256, true);
     PI_REG(k, K_PLKI, k_plki, PlkArgs, 256, true);
     PI_REG(k, K_PLK_LONGI, k_plk_longi, PlkArgs, 256, true);
-    PI_REG(k, K_INTERACT, k_interact, InteractArgs, 1024, false);
+    PI_REG(k, K_INT_ZERO, k_int_zero, InteractArgs, 256, true);
+    PI_REG(k, K_INT_COUNT, k_int_count, InteractArgs, 256, true);
+    PI_REG(k, K_INT_LIST, k_int_list, InteractArgs, 256, true);
+    PI_REG(k, K_INT_CELLS, k_int_cells, InteractArgs, 256, true);
+    PI_REG(k, K_REL_ZERO, k_rel_zero, InteractArgs, 256, true);
+    PI_REG(k, K_REL_FLAG, k_rel_flag, InteractArgs, 256, true);
+    PI_REG(k, K_REL_CHECK, k_rel_check, InteractArgs, 64, false);
+    PI_REG(k, K_REL_LIST, k_rel_list, InteractArgs, 256, true);
+    PI_REG(k, K_REL_WRITE, k_rel_write, InteractArgs, 256, true);
+    PI_REG(k, K_INT_DIAG, k_int_diag, InteractArgs, 256, true);
+    PI_REG(k, K_INT_NUM, k_int_num, InteractArgs, 256, true);
 }

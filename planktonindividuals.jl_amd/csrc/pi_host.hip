@@ -159,7 +159,7 @@ struct pi_model {
     float* abc_buf[PI_MAXSA][PI_NBC]{};
     long long abc_nt[PI_MAXSA][PI_NBC]{};
     /* phytoplankton-particle interaction and release: the pairs, the scratch of
-     * k_interact, the per-slot index of a reborn daughter, diags.phytos.spN.ptc */
+     * the interaction kernels, the per-slot index of a reborn daughter, diags.phytos.spN.ptc */
     int npint = 0, nprel = 0, mc = 25;
     int pint[PI_MAXPAIRS][2]{}, prel[PI_MAXPAIRS][2]{};
     uint32_t *icnt = nullptr, *ioff = nullptr, *ifill = nullptr, *ilist = nullptr, *rflag = nullptr,
@@ -376,9 +376,10 @@ static pi_status create_impl(const pi_model_desc* d, pi_model* m)
         for (int q = 0; q < m->npint; ++q) { m->pint[q][0] = d->interactions[2 * q]; m->pint[q][1] = d->interactions[2 * q + 1]; }
         for (int q = 0; q < m->nprel; ++q) { m->prel[q][0] = d->releases[2 * q]; m->prel[q][1] = d->releases[2 * q + 1]; }
         if (m->npint + m->nprel > 0) {
-            DALLOC(m->icnt, g.ncell * 4); DALLOC(m->ioff, g.ncell * 4); DALLOC(m->ifill, g.ncell * 4);
-            DALLOC(m->ilist, m->cap * 4); DALLOC(m->rflag, m->cap * 4); DALLOC(m->rrank, m->cap * 4);
-            DALLOC(m->ifflag, m->cap * 4); DALLOC(m->ifpre, m->cap * 4); DALLOC(m->iflist, m->cap * 4);
+            /* scan outputs hold the total after the last entry */
+            DALLOC(m->icnt, g.ncell * 4); DALLOC(m->ioff, (g.ncell + 1) * 4); DALLOC(m->ifill, g.ncell * 4);
+            DALLOC(m->ilist, m->cap * 4); DALLOC(m->rflag, m->cap * 4); DALLOC(m->rrank, (m->cap + 1) * 4);
+            DALLOC(m->ifflag, m->cap * 4); DALLOC(m->ifpre, (m->cap + 1) * 4); DALLOC(m->iflist, m->cap * 4);
             DALLOC(m->rdidx, m->ctot * 4);
         }
         DALLOC(m->d_abc, sizeof(AbioBc));
@@ -1157,12 +1158,34 @@ static void build_step(pi_model* m, float dt, Plan& P)
         ia.num = m->anum; ia.diag_ptc = m->diag_ptc;
         ia.cnt = m->icnt; ia.off = m->ioff; ia.fill = m->ifill; ia.list = m->ilist;
         ia.rflag = m->rflag; ia.rrank = m->rrank; ia.fflag = m->ifflag; ia.fpre = m->ifpre; ia.flist = m->iflist;
-        ia.nint = m->npint; ia.nrel = m->nprel; ia.mc = m->mc;
+        ia.mc = m->mc;
         for (int q = 0; q < PI_MAXPAIRS; ++q)
             for (int j = 0; j < 2; ++j) { ia.pint[q][j] = m->pint[q][j]; ia.prel[q][j] = m->prel[q][j]; }
-        ia.num_late = m->npint + m->nprel > 0 ? 1 : 0;
         ia.dt = dt; ia.seed = m->seed;
-        P.add(K_INTERACT, "interact", (m->npint + m->nprel > 0 || m->diag_ptc) ? 1 : 0, ia);
+        /* every model lists the same records (PI_MAXPAIRS of each kind); absent pairs launch nothing */
+        const long long nc = m->ncell, cap = m->cap;
+        for (int q = 0; q < PI_MAXPAIRS; ++q) {
+            const bool on = q < m->npint;
+            ia.q = on ? q : 0;
+            P.add(K_INT_ZERO, "interact", on ? grid_for(nc, 256, 8192) : 0, ia);
+            P.add(K_INT_COUNT, "interact", on ? grid_for(cap, 256, 8192) : 0, ia);
+            plan_scan(P, "interact", scan_plain(m->icnt, (uint32_t)(on ? nc : 0)), m->apart, m->ioff);
+            P.add(K_INT_LIST, "interact", on ? grid_for(cap, 256, 8192) : 0, ia);
+            P.add(K_INT_CELLS, "interact", on ? grid_for(nc, 256, 8192) : 0, ia);
+        }
+        for (int q = 0; q < PI_MAXPAIRS; ++q) {
+            const bool on = q < m->nprel;
+            ia.q = on ? q : 0;
+            P.add(K_REL_ZERO, "interact", on ? grid_for(cap, 256, 8192) : 0, ia);
+            P.add(K_REL_FLAG, "interact", on ? grid_for(m->ctot, 256, 8192) : 0, ia);
+            plan_scan(P, "interact", scan_plain(m->rflag, (uint32_t)(on ? cap : 0)), m->apart, m->rrank);
+            plan_scan(P, "interact", scan_plain(m->ifflag, (uint32_t)(on ? cap : 0)), m->apart, m->ifpre);
+            P.add(K_REL_CHECK, "interact", on ? 1 : 0, ia);
+            P.add(K_REL_LIST, "interact", on ? grid_for(cap, 256, 8192) : 0, ia);
+            P.add(K_REL_WRITE, "interact", on ? grid_for(m->ctot, 256, 8192) : 0, ia);
+        }
+        P.add(K_INT_DIAG, "interact", m->diag_ptc ? grid_for(m->ctot, 256, 8192) : 0, ia);
+        P.add(K_INT_NUM, "interact", m->npint + m->nprel > 0 ? grid_for((long long)m->nsa * cap, 256, 8192) : 0, ia);
     }
     P.mark(K_JOIN);
     /* tracers: k_final and the halo fill of the new tracers, after the join */
