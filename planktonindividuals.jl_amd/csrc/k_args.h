@@ -115,7 +115,6 @@ struct FinalArgs {
     int run;                          /* levels per column run (launch shape, any value >= 1) */
     const struct BcTab* bc;           /* flux boundary conditions (the k_final2/3 instantiations) */
     uint8_t bcm[PI_NTRACERS];         /* per tracer, bit f set = a boundary condition on face f */
-    int plk_fe;                       /* IronEnergyMode: DFe and PFe_bio receive particle exchange too */
 };
 
 /* Flux boundary conditions of the tracers (set_bc!, boundary_conditions.jl:12-29),
@@ -231,7 +230,8 @@ enum KId {
     K_ABIO_MARK, K_ABIO_FREE, K_ABIO_LIST, K_ABIO_CHECK, K_ABIO_SPAWN, K_ABIO_ADV, K_ABIO_ADV_GM,
     K_PHYSM, K_IDLEM, K_PHYSI, K_IDLEI, K_PLKI, K_PLK_LONGI,
     K_INT_ZERO, K_INT_COUNT, K_INT_LIST, K_INT_CELLS, K_REL_ZERO, K_REL_FLAG, K_REL_CHECK, K_REL_LIST,
-    K_REL_WRITE, K_INT_DIAG, K_INT_NUM, K_NKID
+    K_REL_WRITE, K_INT_DIAG, K_INT_NUM, K_FINAL8, K_FINAL9, K_FINAL10, K_FINAL11, K_FINAL12, K_FINAL13,
+    K_FINAL14, K_FINAL15, K_NKID
 };
 /* plan markers (not kernels): fork to / join from the side stream */
 enum { K_FORK = K_NKID, K_JOIN = K_NKID + 1 };

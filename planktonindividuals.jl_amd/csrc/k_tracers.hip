@@ -404,7 +404,7 @@ __device__ __forceinline__ int halo_src(int p, int H, int N, int topo)
 /* ---------------------------------------------- z sweep + the rest of the update */
 
 /* the tracers that receive particle exchange (plk), tracer_update.jl:22-43;
- * DFe and PFe_bio also in IronEnergyMode models (FinalArgs::plk_fe) */
+ * DFe and PFe_bio also in IronEnergyMode models (the FE variants of k_final) */
 __device__ __forceinline__ constexpr bool has_plk(int q)
 {
     return q == PI_DIC || q == PI_NH4 || q == PI_NO3 || q == PI_PO4 || q == PI_DOC || q == PI_DON ||
@@ -613,7 +613,7 @@ __device__ __forceinline__ void face_fluxes_grp(const DevGrid& g, const Face& F,
 /* the tracers of group G at one cell: the fluxes of the face below it, the
  * updates, the stores and the carried state (cu, cs, fz, dz; indexed by the
  * position in the group, cs by sink_index) */
-template <bool LAND, bool BCS, bool GM, int G>
+template <bool LAND, bool BCS, bool GM, bool FE, int G>
 __device__ __forceinline__ void final_grp(const FinalArgs& a, const DevGrid& g, const float* C0, float* OUT,
                                           const Face& Fb, bool zero, const ZCell& z, const float* tt, float* cu,
                                           float* fz, float* dz, float* cs, float (*pw)[4], const HaloImg& hi)
@@ -628,7 +628,7 @@ __device__ __forceinline__ void final_grp(const FinalArgs& a, const DevGrid& g, 
         const int q = fgrp(G, i);
         const float* co = fld(C0, g, q);
         cnv[i] = ld(co, c);
-        const float pk = (has_plk(q) || ((q == PI_DFe || q == PI_PFe_bio) && a.plk_fe)) ? ld(fld(a.plk, g, q), cc) : 0.0f;
+        const float pk = (has_plk(q) || (FE && (q == PI_DFe || q == PI_PFe_bio))) ? ld(fld(a.plk, g, q), cc) : 0.0f;
         const int si = sink_index(q);
         /* pw[i][1] = the z-sweep input at cc */
         cell_update<LAND, BCS, GM>(a, g, z, q, tt, cu[i], cnv[i], ld(co, cc - 1), ld(co, cc + 1), ld(co, cc - sy),
@@ -668,7 +668,7 @@ __device__ __forceinline__ void final_grp(const FinalArgs& a, const DevGrid& g, 
 }
 
 /* one column run (cells k0..k1-1 of column i, j) for the tracers of group G */
-template <bool LAND, bool BCS, bool GM, int G>
+template <bool LAND, bool BCS, bool GM, bool FE, int G>
 __device__ __forceinline__ void final_run(const FinalArgs& a, const DevGrid& g, const float* C0, float* OUT, int i,
                                           int j, int r, int it)
 {
@@ -749,12 +749,12 @@ __device__ __forceinline__ void final_run(const FinalArgs& a, const DevGrid& g, 
 #pragma unroll
         for (int x = 0; x < fg_n(G); ++x) tt[fgrp(G, x)] = dmaxf(0.0f, cu[x]);
         if (G == 3) tt[PI_POC] = dmaxf(0.0f, ld(fld(C0, g, PI_POC), z.cc));
-        final_grp<LAND, BCS, GM, G>(a, g, C0, OUT, Fb, zero, z, tt, cu, fz, dz, cs, pw, hi);
+        final_grp<LAND, BCS, GM, FE, G>(a, g, C0, OUT, Fb, zero, z, tt, cu, fz, dz, cs, pw, hi);
         Fa = Fb;
     }
 }
 
-template <bool LAND, bool BCS, bool GM>
+template <bool LAND, bool BCS, bool GM, bool FE>
 __device__ __forceinline__ void k_final_body(const FinalArgs& a, Blk K)
 {
     const DevState* S = a.st;
@@ -773,27 +773,37 @@ __device__ __forceinline__ void k_final_body(const FinalArgs& a, Blk K)
         const int j = (int)((u >> 2) % g.Ny) + 1 + g.Hy;
         const int r = (int)((u >> 2) / g.Ny);
         switch (G) {
-            case 0: final_run<LAND, BCS, GM, 0>(a, g, C0, OUT, i, j, r, it); break;
-            case 1: final_run<LAND, BCS, GM, 1>(a, g, C0, OUT, i, j, r, it); break;
-            case 2: final_run<LAND, BCS, GM, 2>(a, g, C0, OUT, i, j, r, it); break;
-            default: final_run<LAND, BCS, GM, 3>(a, g, C0, OUT, i, j, r, it); break;
+            case 0: final_run<LAND, BCS, GM, FE, 0>(a, g, C0, OUT, i, j, r, it); break;
+            case 1: final_run<LAND, BCS, GM, FE, 1>(a, g, C0, OUT, i, j, r, it); break;
+            case 2: final_run<LAND, BCS, GM, FE, 2>(a, g, C0, OUT, i, j, r, it); break;
+            default: final_run<LAND, BCS, GM, FE, 3>(a, g, C0, OUT, i, j, r, it); break;
         }
     }
 }
 /* at least 5 waves per SIMD: 0.398 -> 0.383 ms on the 256x256x64 run (4 waves at 108 VGPRs) */
 /* k_final<n>: bit 0 landmask, bit 1 flux boundary conditions (apply_bcs!),
- * bit 2 LatLonGrid metrics */
-#define PI_FINAL(N, LAND, BCS, GM)                                                                    \
-    __device__ __forceinline__ void k_final##N##_body(const FinalArgs& a, Blk K) { k_final_body<LAND, BCS, GM>(a, K); } \
+ * bit 2 LatLonGrid metrics, bit 3 IronEnergyMode (DFe and PFe_bio exchange:
+ * a compile-time switch, since the extra loads spill registers of the other
+ * variants, 0.39 -> 0.77 ms on the 256x256x64 run when read from a flag) */
+#define PI_FINAL(N, LAND, BCS, GM, FE)                                                                \
+    __device__ __forceinline__ void k_final##N##_body(const FinalArgs& a, Blk K) { k_final_body<LAND, BCS, GM, FE>(a, K); } \
     PI_KERNEL3W(k_final##N, k_final##N##_body, FinalArgs, 256, xcd_block(), 5)
-PI_FINAL(0, false, false, false)
-PI_FINAL(1, true, false, false)
-PI_FINAL(2, false, true, false)
-PI_FINAL(3, true, true, false)
-PI_FINAL(4, false, false, true)
-PI_FINAL(5, true, false, true)
-PI_FINAL(6, false, true, true)
-PI_FINAL(7, true, true, true)
+PI_FINAL(0, false, false, false, false)
+PI_FINAL(1, true, false, false, false)
+PI_FINAL(2, false, true, false, false)
+PI_FINAL(3, true, true, false, false)
+PI_FINAL(4, false, false, true, false)
+PI_FINAL(5, true, false, true, false)
+PI_FINAL(6, false, true, true, false)
+PI_FINAL(7, true, true, true, false)
+PI_FINAL(8, false, false, false, true)
+PI_FINAL(9, true, false, false, true)
+PI_FINAL(10, false, true, false, true)
+PI_FINAL(11, true, true, false, true)
+PI_FINAL(12, false, false, true, true)
+PI_FINAL(13, true, false, true, true)
+PI_FINAL(14, false, true, true, true)
+PI_FINAL(15, true, true, true, true)
 
 /* ------------------------------------------------- fused x + y sweeps (k_xy)
  * calc_Gcsˣ! + multi_dim_x! + fill_halo_tracer! + calc_Gcsʸ! + multi_dim_y!
@@ -1089,6 +1099,14 @@ void reg_tracers(KInfo* k)
     PI_REG(k, K_FINAL5, k_final5, FinalArgs, 256, true);
     PI_REG(k, K_FINAL6, k_final6, FinalArgs, 256, true);
     PI_REG(k, K_FINAL7, k_final7, FinalArgs, 256, true);
+    PI_REG(k, K_FINAL8, k_final8, FinalArgs, 256, true);
+    PI_REG(k, K_FINAL9, k_final9, FinalArgs, 256, true);
+    PI_REG(k, K_FINAL10, k_final10, FinalArgs, 256, true);
+    PI_REG(k, K_FINAL11, k_final11, FinalArgs, 256, true);
+    PI_REG(k, K_FINAL12, k_final12, FinalArgs, 256, true);
+    PI_REG(k, K_FINAL13, k_final13, FinalArgs, 256, true);
+    PI_REG(k, K_FINAL14, k_final14, FinalArgs, 256, true);
+    PI_REG(k, K_FINAL15, k_final15, FinalArgs, 256, true);
     PI_REG(k, K_XY0, k_xy0, XYArgs, XY_T, true);
     PI_REG(k, K_XY1, k_xy1, XYArgs, XY_T, true);
     PI_REG(k, K_XYW0, k_xyw0, XYArgs, XY_W, true);

@@ -1194,7 +1194,7 @@ static void build_step(pi_model* m, float dt, Plan& P)
         fa.g = g; fa.st = st; fa.nf = PI_NTRACERS;
         FinalArgs fn = zeroed<FinalArgs>();
         fn.g = g; fn.st = st; fn.w = V1.w; fn.dt = dt;
-        fn.prev = m->T2[0]; fn.plk = m->plk_slab; fn.plk_fe = m->mode == PI_MODE_IRONENERGY ? 1 : 0;
+        fn.prev = m->T2[0]; fn.plk = m->plk_slab;
         fn.kh = m->bgc[PI_B_kappa_h]; fn.kv = m->bgc[PI_B_kappa_v];
         fn.wsink_inorg = m->bgc[PI_B_w_sink_inorg]; fn.wsink_org = m->bgc[PI_B_w_sink_org];
         fn.kDOC = m->bgc[PI_B_kDOC]; fn.kPOC = m->bgc[PI_B_kPOC]; fn.kDON = m->bgc[PI_B_kDON];
@@ -1211,10 +1211,12 @@ static void build_step(pi_model* m, float dt, Plan& P)
             bcs |= fn.bcm[q] != 0;
         }
         const long long fb = grid_for(final_items(g, fn.run), 256, 8192);
-        /* k_final<n>: bit 0 land, bit 1 flux boundary conditions, bit 2 LatLonGrid */
-        const int fv = (land ? 1 : 0) | (bcs ? 2 : 0) | (g.gm ? 4 : 0);
-        static const KId fk[8] = {K_FINAL0, K_FINAL1, K_FINAL2, K_FINAL3, K_FINAL4, K_FINAL5, K_FINAL6, K_FINAL7};
-        for (int v = 0; v < 8; ++v) P.add(fk[v], "final", v == fv ? fb : 0, fn);
+        /* k_final<n>: bit 0 land, bit 1 flux boundary conditions, bit 2 LatLonGrid,
+         * bit 3 IronEnergyMode (DFe and PFe_bio exchange) */
+        const int fv = (land ? 1 : 0) | (bcs ? 2 : 0) | (g.gm ? 4 : 0) | (m->mode == PI_MODE_IRONENERGY ? 8 : 0);
+        static const KId fk[16] = {K_FINAL0, K_FINAL1, K_FINAL2, K_FINAL3, K_FINAL4, K_FINAL5, K_FINAL6, K_FINAL7,
+                                   K_FINAL8, K_FINAL9, K_FINAL10, K_FINAL11, K_FINAL12, K_FINAL13, K_FINAL14, K_FINAL15};
+        for (int v = 0; v < 16; ++v) P.add(fk[v], "final", v == fv ? fb : 0, fn);
         /* fill_halo_tracer! of the new tracers: x, then y, then z over whole planes */
         fa.f = nullptr;
         fa.lines_full = 1;
