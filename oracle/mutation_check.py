@@ -76,8 +76,11 @@ MISTAKES = [
      "                std::fill(pop.begin(), pop.end(), (FT)0);\n                for (int s2 = 0; s2 < nsp; ++s2) acc_counts(sp[s2]);"),
     ("bounded velocity halo like a tracer halo", "else planes_bcast(c, dim, N + H + 2, H - 1, N + H + 1);",
      "else planes_bcast(c, dim, N + H + 1, H, N + H);"),
-    ("grazing loss without Cq", "plk[PI_POC][c] += M::mask(a, (p.Bm[i] + p.Cq[i]) * (L(1) - fC)) * loss[i];",
-     "plk[PI_POC][c] += M::mask(a, (p.Bm[i]) * (L(1) - fC)) * loss[i];"),
+    ("grazing loss without Cq",
+     "plk[PI_POC][c] += M::mask(a, (p.Bm[i] + p.Cq[i]) * (L(1) - fC)) * loss[i];\n"
+     "            plk[PI_PON][c] += M::mask(a, (p.Bm[i] * R_NC + p.Nq[i]) * (L(1) - fN)) * loss[i];",
+     "plk[PI_POC][c] += M::mask(a, (p.Bm[i]) * (L(1) - fC)) * loss[i];\n"
+     "            plk[PI_PON][c] += M::mask(a, (p.Bm[i] * R_NC + p.Nq[i]) * (L(1) - fN)) * loss[i];"),
     ("z diffusion sign", "FT ddz = M::mask(LM(i, j, k), M::mask(LM(i, j, k + 1), fz(i, j, k) - fz(i, j, k + 1)));",
      "FT ddz = M::mask(LM(i, j, k), M::mask(LM(i, j, k + 1), fz(i, j, k + 1) - fz(i, j, k)));"),
     ("z flux from the wrong side", "return L(0.5f) * (T + std::fabs(T)) * (c[I(i, j, k)] + pm * d) +",
@@ -91,10 +94,16 @@ MISTAKES = [
      "FT PS = PCm * (one - M::exp(-aI / M::max(e30, PCm) / M::max(e30, p.Bm[i])))"),
     ("NO3 uptake with the NH4 half-saturation", "tNO3[i] / M::max(e30, tNO3[i] + q[PI_P_KsatNO3])",
      "tNO3[i] / M::max(e30, tNO3[i] + q[PI_P_KsatNH4])"),
-    ("rhoChl light gate at 1", "r = M::mask(M::isless(L(1.0e-1f), tpar[i]), r);", "r = M::mask(M::isless(L(1.0f), tpar[i]), r);"),
+    ("rhoChl light gate at 1", "M::max(e30, tpar[i] * q[PI_P_alpha] * q[PI_P_Phi] * p.Chl[i] / p.Bm[i]);\n"
+     "            r = M::mask(M::isless(L(1.0e-1f), tpar[i]), r);",
+     "M::max(e30, tpar[i] * q[PI_P_alpha] * q[PI_P_Phi] * p.Chl[i] / p.Bm[i]);\n"
+     "            r = M::mask(M::isless(L(1.0f), tpar[i]), r);"),
     ("DOC uptake gate at 10 %", "M::isless(L(1.0e-2f), p.PS[i] / (p.VDOC[i] + p.PS[i]))",
      "M::isless(L(1.0e-1f), p.PS[i] / (p.VDOC[i] + p.PS[i]))"),
-    ("tempFunc_PS shape constant", "FT regT = shape_dec(x, xmax, L(4.0e-2f), 4);", "FT regT = shape_dec(x, xmax, L(4.0e-3f), 4);"),
+    ("tempFunc_PS shape constant", "FT x = T - q[PI_P_Topt], xmax = q[PI_P_Tmax] - q[PI_P_Topt];\n"
+     "        FT regT = shape_dec(x, xmax, L(4.0e-2f), 4);",
+     "FT x = T - q[PI_P_Topt], xmax = q[PI_P_Tmax] - q[PI_P_Topt];\n"
+     "        FT regT = shape_dec(x, xmax, L(4.0e-3f), 4);"),
     ("gather NO3 from NH4", "tNO3[i] = M::mask(a, M::max(z0, tr[PI_NO3][c]));", "tNO3[i] = M::mask(a, M::max(z0, tr[PI_NH4][c]));"),
     ("uptake cap without pop", "FT pp = M::max(one, tpop[i]);", "FT pp = one;"),
     ("Chl synthesis without R_NC", "p.Chl[i] = p.Chl[i] + dt * p.BS[i] * q[PI_P_R_NC] * p.rho[i];",
