@@ -233,3 +233,33 @@ def test_interaction_member_in_a_batch(pi, orc):
         bad = compare_models(h, om, s.n_species, extra_fields=EX) + _compare_abiotic(h, om, s.n_abiotic)
         assert not bad, "\n".join(bad[:20])
     batch.close()
+
+
+def test_interaction_budget_at_size(pi):
+    """1,048,576 individuals and 524,288 particles on a 128x128x32 grid with
+    divisions and deaths, 20 steps: ptc summed over every slot (dead slots keep
+    theirs, a daughter takes its slot's) plus the active particles stays the
+    initial particle count; ptc integral in [0, max_ptc]; the ptc diagnostic is
+    the living individuals' ptc at the last step."""
+    g = pi.RectilinearGrid(size=(128, 128, 32), x=(0, 128e3), y=(0, 128e3), z=(0, -320))
+    spec = pi.ModelSpec(g, mode=pi.IronEnergyMode(), max_individuals=2 ** 21, seed=131,
+                        phyto=pi.phyto_setup({"max_ptc": [3.0], "dvid_P": [2e-3], "dvid_reg": [1.3], "mort_P": [1e-3],
+                                              "mort_reg": [1.4], "grz_P": [1e-6]}, [2 ** 20], 1),
+                        abiotic=pi.abiotic_setup({"Rd": [300.0], "release_P": [1e-4]}, [2 ** 19], 1,
+                                                 palat=pi.Palat(intac=[("sp1", "sa1")], release=[("sp1", "sa1")])),
+                        bgc_params={"κhP": 1.0, "κh": 1.0})
+    m = pi.PlanktonModel(pi.HIPArch(), spec=spec)._native
+    m.set_velocity(*pi.workloads.cellular_flow(g, U=0.5, L=32e3))
+    m.set_parf(np.full((128, 128), 500.0, np.float32))
+    m.set_temperature(np.full((128, 128, 32), 20.0, np.float32))
+    m.run(60.0, 19)
+    m.reset_diags()
+    m.run(60.0, 1)
+    m.sync()
+    ptc = m.field(0, "ptc").astype(np.float64)
+    ac = m.field(0, "ac")
+    npart = int(m.abiotic_field(0, "ac").sum())
+    assert np.all(ptc == np.round(ptc)) and ptc.min() >= 0 and ptc.max() <= 3
+    assert ptc.sum() + npart == 2 ** 19 and ptc.sum() > 1000
+    assert m.diag(0, "ptc").sum() == ptc[ac].sum()
+    assert m.diag(0, "dvid").sum() > 0 and m.diag(0, "mort").sum() > 0
