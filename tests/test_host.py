@@ -164,3 +164,27 @@ def test_footprint_is_a_dry_run(pi):
     assert abs((f[2] - f[1]) - 2 * (f[1] - f[0])) < 64 * 256
     g2 = pi.RectilinearGrid(size=(32, 32, 4), x=(0, 32e3), y=(0, 32e3), z=(0, -40))
     assert nat.footprint(g2, 1, 10000, prm, bgc) > f[0]
+
+
+def test_ctypes_mirror_matches_header(pi, tmp_path):
+    """The Python mirror of the C ABI's structs (size and every field's offset)
+    against include/pi_hip.h compiled by gcc."""
+    A = pi._abi
+    structs = {"pi_grid_desc": A.GridDesc, "pi_metrics": A.Metrics, "pi_model_desc": A.ModelDesc,
+               "pi_individuals_host": A.IndividualsHost}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pi_hip.h"', "int main(void) {"]
+    for cname, st in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f in st._fields_:
+            lines.append(f'printf("{cname}.{f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                        check=True).stdout.splitlines())
+    for cname, st in structs.items():
+        assert int(got[cname]) == C.sizeof(st), cname
+        for f in st._fields_:
+            assert int(got[f"{cname}.{f[0]}"]) == getattr(st, f[0]).offset, f"{cname}.{f[0]}"
