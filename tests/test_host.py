@@ -188,3 +188,14 @@ def test_ctypes_mirror_matches_header(pi, tmp_path):
         assert int(got[cname]) == C.sizeof(st), cname
         for f in st._fields_:
             assert int(got[f"{cname}.{f[0]}"]) == getattr(st, f[0]).offset, f"{cname}.{f[0]}"
+
+
+def test_binding_test_struct_matches_abi(pi):
+    """The reference-layout mirror used by the binding test (written apart from
+    the package, as a Julia binding would be) has the ABI's size and offsets."""
+    import test_gpu_binding as B
+    for mine, ref in ((B.ModelDesc, pi._abi.ModelDesc), (B.GridDesc, pi._abi.GridDesc),
+                      (B.IndividualsHost, pi._abi.IndividualsHost)):
+        assert C.sizeof(mine) == C.sizeof(ref), mine.__name__
+        assert [getattr(mine, f[0]).offset for f in mine._fields_] == \
+               [getattr(ref, f[0]).offset for f in ref._fields_], mine.__name__
