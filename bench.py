@@ -453,6 +453,7 @@ def main():
                     help="skip the plain 3-D run (for profiling the other legs); prints their results only")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-ensemble leg")
+    ap.add_argument("--no-interaction", action="store_true", help="skip the particle-interaction leg")
     ap.add_argument("--ens-members", type=int, default=256)
     ap.add_argument("--ens-batched-only", action="store_true",
                     help="ensemble leg without the one-by-one pass (profiling the batched launches)")
@@ -603,6 +604,9 @@ def main():
     serve = None
     if not args.no_serve:
         serve = serving_leg(pi, rate=args.serve_rate, window=args.serve_window)
+    inter = None
+    if not args.no_interaction:
+        inter = interaction_leg(pi)
     base = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle) ...")
@@ -638,6 +642,7 @@ def main():
         "cpu_baseline": base,
         "large_instance": large,
         "ensemble": ens,
+        "interaction": inter,
         "serving": serve,
         "logical_kernels": logical,
         "kernels": kernels,
@@ -751,6 +756,46 @@ def large_leg(pi, steps=200, warmup=2, probe=False, tune=True, shapes=None):
     log(f"[bench] large instance: {steps} steps in {wall:.3f} s ({ms:.3f} ms/step), "
         f"{out['individual_steps_per_s']:.4g} individual-steps/s")
     del model, nat
+    return out
+
+
+def interaction_leg(pi, steps=20):
+    """Phytoplankton-particle interaction and release (particle_interaction.jl,
+    time_step.jl:119-159) on an IronEnergyMode model: 128x128x32, 1,048,576
+    individuals and 524,288 abiotic particles in a cellular flow, with one
+    interaction and one release pair against the same model without pairs;
+    the `interact` launch group timed with HIP events."""
+    g = pi.RectilinearGrid(size=(128, 128, 32), x=(0, 128e3), y=(0, 128e3), z=(0, -320))
+    out = {"workload": "IronEnergyMode 128x128x32, 1048576 individuals (capacity 2097152), 524288 abiotic particles "
+                       "(Rd 300 m, release_P 1e-4), cellular flow, dt 60 s", "steps": steps}
+    for pairs in (False, True):
+        pal = pi.Palat(intac=[("sp1", "sa1")], release=[("sp1", "sa1")]) if pairs else pi.Palat()
+        spec = pi.ModelSpec(g, mode=pi.IronEnergyMode(), max_individuals=2 ** 21, seed=7,
+                            phyto=pi.phyto_setup({"max_ptc": [3.0]}, [2 ** 20], 1),
+                            abiotic=pi.abiotic_setup({"Rd": [300.0], "release_P": [1e-4]}, [2 ** 19], 1, palat=pal),
+                            bgc_params={"κhP": 1.0, "κh": 1.0})
+        nat = pi.PlanktonModel(pi.HIPArch(), spec=spec)._native
+        nat.set_velocity(*pi.workloads.cellular_flow(g, U=0.5, L=32e3))
+        nat.set_parf(np.full((128, 128), 500.0, np.float32))
+        nat.set_temperature(np.full((128, 128, 32), 20.0, np.float32))
+        nat.run(60.0, 3)
+        nat.sync()
+        t0 = time.perf_counter()
+        nat.run(60.0, steps)
+        nat.sync()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        nat.enable_timing(True)
+        nat.run(60.0, 5)
+        nat.sync()
+        kt = nat.kernel_times()
+        nat.enable_timing(False)
+        key = "with_pairs" if pairs else "without_pairs"
+        out[key] = {"ms_per_step": ms, "interact_ms_per_step": kt.get("interact", (0.0, 0))[0] / 5,
+                    "particles_active": int(nat.abiotic_field(0, "ac").sum()),
+                    "ptc_held": float(nat.field(0, "ptc").sum())}
+        log(f"[bench] interaction ({key}): {ms:.3f} ms/step, interact group "
+            f"{out[key]['interact_ms_per_step']:.3f} ms/step")
+        del nat
     return out
 
 
